@@ -1,0 +1,159 @@
+/*
+ * pcf_dns.h — C ABI of the MI355X plane-Couette KMM SBDF3 step library
+ * (libpcfdns.so).  Plain C types only; every entry point returns 0 on success
+ * and a negative PCF_E_* code on failure, with a message from pcf_last_error().
+ *
+ * Layout convention at the boundary: every spectral field is exchanged in the
+ * reference's full-complex layout, C order (n, Ny, Nz) of interleaved
+ * complex128 (re, im) doubles — axis 0 is the wall-normal Shen/Chebyshev index,
+ * axis 1 the streamwise Fourier index, axis 2 the spanwise Fourier index, both
+ * in FFT order.  Mean-mode vectors (v00, w00 and their nonlinear rows) are real
+ * float64 of length n.  Internally the library keeps the Hermitian half
+ * spectrum (spanwise kz < Nz/2; the kz = Nz/2 Nyquist plane is identically
+ * zero in every field) and the two halves are converted exactly at the
+ * boundary.
+ *
+ * Threading: one handle per stream; calls on one handle are not thread-safe.
+ */
+#ifndef PCF_DNS_H
+#define PCF_DNS_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PCF_ABI_VERSION 1
+
+/* error codes */
+#define PCF_OK 0
+#define PCF_E_INVALID (-1)   /* bad argument or configuration        */
+#define PCF_E_HIP (-2)       /* HIP runtime failure                  */
+#define PCF_E_UNSUPPORTED (-3) /* option not built                   */
+#define PCF_E_FIXED_DT (-4)  /* SBDF3 refuses a changed dt           */
+
+/* precision modes */
+#define PCF_PREC_FP64 0   /* every stage in FP64                              */
+#define PCF_PREC_MIXED 1  /* transforms + nonlinear FP32; solves, mean mode,
+                             history and constraint rows FP64                */
+
+/* time integrators (reference: examples/channelflow_kmm.py:149-159) */
+#define PCF_TI_SBDF3 3
+
+/* nonlinear forms (reference: examples/channelflow_kmm.py:140-142) */
+#define PCF_NL_ROTATIONAL 1
+
+/*
+ * Problem specification.  Mirrors the constructor of the reference's
+ * PlaneCouetteFluctuationJax (examples/pcf_fluctuations_jax.py:35-75) with the
+ * benchmark knobs of examples/channelflow_kmm.py:111-193:
+ *   N = (n0, ny, nz), domain = ((-1,1), (0,Ly), (0,Lz)), nu = U_wall/Re,
+ *   padding = (1, 1.5, 1.5), time_integrator = SBDF3,
+ *   nonlinear_form = rotational, coefficient_path = optimized.
+ */
+typedef struct pcf_config {
+  int32_t abi_version;     /* = PCF_ABI_VERSION                              */
+  int32_t n0, ny, nz;      /* Chebyshev points; Fourier modes (even)         */
+  double lx[2];            /* wall-normal domain, must be (-1, 1)            */
+  double ly, lz;           /* periodic lengths                               */
+  double nu;               /* kinematic viscosity = U_wall / Re              */
+  double u_wall;           /* laminar base flow U = u_wall * x               */
+  double dt;               /* fixed time step                                */
+  double padding_y, padding_z; /* dealiasing factors (1.5 = 3/2 rule)        */
+  int32_t precision;       /* PCF_PREC_*                                     */
+  int32_t time_integrator; /* PCF_TI_SBDF3                                   */
+  int32_t nonlinear_form;  /* PCF_NL_ROTATIONAL                              */
+  int32_t device;          /* HIP device ordinal                             */
+} pcf_config;
+
+typedef struct pcf_handle pcf_handle;
+
+/* Field identifiers for pcf_set_field / pcf_get_field.  Names follow the
+ * reference KMMState (examples/channelflow_kmm.py:50-101):
+ *   u = (u0 in TB, u1 in TD, u2 in TD), g in TD,
+ *   nonlinear_old/older = (Nu in TB, Ng in TD, Nv00, Nw00),
+ *   solution_old/older  = (u0 in TB, g in TD, v00, w00).                    */
+enum pcf_field {
+  PCF_F_U0 = 0, PCF_F_U1, PCF_F_U2, PCF_F_G,
+  PCF_F_NL_OLD_U, PCF_F_NL_OLD_G, PCF_F_NL_OLD_V, PCF_F_NL_OLD_W,
+  PCF_F_NL_OLDER_U, PCF_F_NL_OLDER_G, PCF_F_NL_OLDER_V, PCF_F_NL_OLDER_W,
+  PCF_F_SOL_OLD_U, PCF_F_SOL_OLD_G, PCF_F_SOL_OLD_V, PCF_F_SOL_OLD_W,
+  PCF_F_SOL_OLDER_U, PCF_F_SOL_OLDER_G, PCF_F_SOL_OLDER_V, PCF_F_SOL_OLDER_W,
+  PCF_F_COUNT
+};
+
+/* Library identity / errors ------------------------------------------------ */
+int32_t pcf_abi_version(void);
+const char *pcf_last_error(void);
+
+/* Size (elements) of a field in the boundary layout: complex elements for
+ * spectral fields (n*ny*nz), real elements for mean-mode vectors (n).
+ * Host-only; needs no GPU. */
+int64_t pcf_field_size(const pcf_config *cfg, int32_t field);
+
+/* Host-only operator export for verification (no GPU needed).  Writes the
+ * closed-form setup matrix `which` (row-major, rows x cols doubles) to `out`
+ * when out != NULL; always reports its shape.  Matrices:
+ *   0 C0_B=(psi_i,psi_j)  1 C2_B=(psi_i,psi_j'')  2 C4_B=(psi_i,psi_j'''')
+ *   3 C0_D=(phi_i,phi_j)  4 C2_D=(phi_i,phi_j'')   (Chebyshev-weighted)
+ *   5 VB  = clamped-basis values at the Gauss nodes   (n0 x n0-4)
+ *   6 VD  = Dirichlet-basis values                    (n0 x n0-2)
+ *   7 VD1 = Dirichlet-basis x-derivative values        (n0 x n0-2)
+ *   8 A1, 9 A0: Nu rows from nodal values (n0-4 x n0)
+ *   10 AG: Ng rows from nodal values (n0-2 x n0)
+ *   11 R: f = P_TD(d/dx u0) reconstruction (n0-2 x n0-4)
+ *   12 VB1 = clamped-basis x-derivative values        (n0 x n0-4)
+ *   13 FB, 14 FD: Galerkin projection of nodal values onto the clamped /
+ *      Dirichlet basis (n0-4 | n0-2 x n0) — composite.py:361-365
+ * Replaces the SymPy inner() assembly of examples/channelflow_kmm.py:261-409. */
+int32_t pcf_setup_matrix(const pcf_config *cfg, int32_t which, double *out,
+                         int32_t *rows, int32_t *cols);
+
+/* Lifecycle ----------------------------------------------------------------- */
+/* Replaces PlaneCouetteFluctuationJax.__init__ (examples/pcf_fluctuations_jax.py:35-75,
+ * examples/channelflow_kmm.py:111-246): builds operators, allocates device state. */
+int32_t pcf_create(const pcf_config *cfg, pcf_handle **out);
+int32_t pcf_destroy(pcf_handle *h);
+
+/* Stream: all work of the handle is enqueued on this hipStream_t (NULL = the
+ * library's own stream).  Passed as void* to keep HIP types out of the ABI. */
+int32_t pcf_set_stream(pcf_handle *h, void *hip_stream);
+int32_t pcf_synchronize(pcf_handle *h);
+
+/* State exchange (host memory, boundary layout).  Together these carry the
+ * whole KMMState pytree; history scalars travel through pcf_set_history. */
+int32_t pcf_set_field(pcf_handle *h, int32_t field, const double *host);
+int32_t pcf_get_field(pcf_handle *h, int32_t field, double *host);
+int32_t pcf_set_history(pcf_handle *h, double history_steps, double have_old,
+                        double previous_dt);
+int32_t pcf_get_history(pcf_handle *h, double *history_steps, double *have_old,
+                        double *previous_dt);
+
+/* Advance nsteps steps: IMEXRK3 startup while history_steps < 2, then the
+ * steady SBDF3 step.  Replaces KMM.solve / KMM.step
+ * (examples/channelflow_kmm.py:937-1064,1123-1137,1231-1247). Asynchronous. */
+int32_t pcf_step(pcf_handle *h, int32_t nsteps);
+
+/* Steady SBDF3 steps only (requires history_steps >= 2); the unit the
+ * benchmark times.  Replaces _step_sbdf3_steady (channelflow_kmm.py:1016-1064). */
+int32_t pcf_step_steady(pcf_handle *h, int32_t nsteps);
+
+/* Capture nsteps steady steps into a HIP graph owned by the handle and replay
+ * it `replays` times (pcf_step_steady semantics). */
+int32_t pcf_step_steady_graph(pcf_handle *h, int32_t nsteps, int32_t replays);
+
+/* Diagnostics of the current state (synchronous).  out[8] =
+ * {Epert, Etot, divL2, u_top, u_bot, wall_shear_top, wall_shear_bot,
+ *  mean_shear}; replaces PlaneCouetteFluctuationJax.diagnostics
+ * (examples/pcf_fluctuations_jax.py:131-161). */
+int32_t pcf_diagnostics(pcf_handle *h, double *out8);
+
+/* Per-stage timing support: number of kernels launched by one steady step. */
+int32_t pcf_kernels_per_step(pcf_handle *h);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PCF_DNS_H */

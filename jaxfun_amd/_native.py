@@ -1,0 +1,139 @@
+"""ctypes binding of libpcfdns.so (C ABI declared in include/pcf_dns.h).
+
+The library is built in-tree by ``jaxfun_amd.build`` and loaded from the
+package directory.  There is no fallback: if the shared object is missing or
+fails to load, every entry point raises.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().parent / "libpcfdns.so"
+
+ABI_VERSION = 1
+PREC_FP64 = 0
+PREC_MIXED = 1
+TI_SBDF3 = 3
+NL_ROTATIONAL = 1
+
+# enum pcf_field
+FIELDS = (
+    "U0", "U1", "U2", "G",
+    "NL_OLD_U", "NL_OLD_G", "NL_OLD_V", "NL_OLD_W",
+    "NL_OLDER_U", "NL_OLDER_G", "NL_OLDER_V", "NL_OLDER_W",
+    "SOL_OLD_U", "SOL_OLD_G", "SOL_OLD_V", "SOL_OLD_W",
+    "SOL_OLDER_U", "SOL_OLDER_G", "SOL_OLDER_V", "SOL_OLDER_W",
+)
+FIELD_ID = {name: i for i, name in enumerate(FIELDS)}
+
+# pcf_setup_matrix indices
+MATRIX_ID = {
+    "C0B": 0, "C2B": 1, "C4B": 2, "C0D": 3, "C2D": 4, "VB": 5, "VD": 6, "VD1": 7,
+    "A1": 8, "A0": 9, "AG": 10, "R": 11, "VB1": 12, "FB": 13, "FD": 14,
+}
+
+# Every symbol include/pcf_dns.h declares.
+EXPORTS = (
+    "pcf_abi_version", "pcf_last_error", "pcf_field_size", "pcf_setup_matrix",
+    "pcf_create", "pcf_destroy", "pcf_set_stream", "pcf_synchronize",
+    "pcf_set_field", "pcf_get_field", "pcf_set_history", "pcf_get_history",
+    "pcf_step", "pcf_step_steady", "pcf_step_steady_graph", "pcf_diagnostics",
+    "pcf_kernels_per_step",
+)
+
+
+class PcfConfig(C.Structure):
+    _fields_ = [
+        ("abi_version", C.c_int32),
+        ("n0", C.c_int32),
+        ("ny", C.c_int32),
+        ("nz", C.c_int32),
+        ("lx", C.c_double * 2),
+        ("ly", C.c_double),
+        ("lz", C.c_double),
+        ("nu", C.c_double),
+        ("u_wall", C.c_double),
+        ("dt", C.c_double),
+        ("padding_y", C.c_double),
+        ("padding_z", C.c_double),
+        ("precision", C.c_int32),
+        ("time_integrator", C.c_int32),
+        ("nonlinear_form", C.c_int32),
+        ("device", C.c_int32),
+    ]
+
+
+class PcfError(RuntimeError):
+    """Raised when a library call returns a negative status."""
+
+
+_lib = None
+
+
+def load() -> C.CDLL:
+    """Load the shared library (once); raise if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise PcfError(
+            f"{LIB_PATH} is not built; run `python -m jaxfun_amd.build` (hipcc, gfx950)"
+        )
+    lib = C.CDLL(str(LIB_PATH))
+    P = C.POINTER
+    dp = P(C.c_double)
+    cfgp = P(PcfConfig)
+    h = C.c_void_p
+    sig = {
+        "pcf_abi_version": ([], C.c_int32),
+        "pcf_last_error": ([], C.c_char_p),
+        "pcf_field_size": ([cfgp, C.c_int32], C.c_int64),
+        "pcf_setup_matrix": ([cfgp, C.c_int32, dp, P(C.c_int32), P(C.c_int32)], C.c_int32),
+        "pcf_create": ([cfgp, P(C.c_void_p)], C.c_int32),
+        "pcf_destroy": ([h], C.c_int32),
+        "pcf_set_stream": ([h, C.c_void_p], C.c_int32),
+        "pcf_synchronize": ([h], C.c_int32),
+        "pcf_set_field": ([h, C.c_int32, dp], C.c_int32),
+        "pcf_get_field": ([h, C.c_int32, dp], C.c_int32),
+        "pcf_set_history": ([h, C.c_double, C.c_double, C.c_double], C.c_int32),
+        "pcf_get_history": ([h, dp, dp, dp], C.c_int32),
+        "pcf_step": ([h, C.c_int32], C.c_int32),
+        "pcf_step_steady": ([h, C.c_int32], C.c_int32),
+        "pcf_step_steady_graph": ([h, C.c_int32, C.c_int32], C.c_int32),
+        "pcf_diagnostics": ([h, dp], C.c_int32),
+        "pcf_kernels_per_step": ([h], C.c_int32),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = res
+    if lib.pcf_abi_version() != ABI_VERSION:
+        raise PcfError("libpcfdns ABI version mismatch; rebuild the library")
+    _lib = lib
+    return lib
+
+
+def check(status: int) -> int:
+    if status < 0:
+        msg = load().pcf_last_error().decode(errors="replace")
+        if status == -3:
+            raise NotImplementedError(msg)
+        if status == -1:
+            raise ValueError(msg)
+        raise PcfError(f"libpcfdns error {status}: {msg}")
+    return status
+
+
+def setup_matrix(cfg: PcfConfig, name: str) -> np.ndarray:
+    """Fetch a closed-form setup matrix (host only, no GPU)."""
+    lib = load()
+    r, c = C.c_int32(), C.c_int32()
+    check(lib.pcf_setup_matrix(C.byref(cfg), MATRIX_ID[name], None, C.byref(r), C.byref(c)))
+    out = np.empty((r.value, c.value), dtype=np.float64)
+    check(lib.pcf_setup_matrix(C.byref(cfg), MATRIX_ID[name],
+                               out.ctypes.data_as(C.POINTER(C.c_double)), None, None))
+    return out

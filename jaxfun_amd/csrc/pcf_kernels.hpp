@@ -1,0 +1,124 @@
+// Kernel interfaces for the plane-Couette KMM step (gfx950).
+//
+// Device data layout (all complex values interleaved re/im):
+//   spectral field   [n][m]          m = iy * KZ + kz, kz < KZ = nz/2
+//                                    (Hermitian half spectrum; the spanwise
+//                                    Nyquist plane is identically zero and not
+//                                    stored; the streamwise Nyquist row iy=ny/2
+//                                    is stored and kept zero)
+//   nodal pencils    [f][x][m]       after the wall-normal transform
+//   padded y-lines   [f][x][y][kz]   y < MY = floor(1.5 ny)
+// Mode index is fastest everywhere, so the Chebyshev contractions read and
+// write whole 64-wide rows and the per-mode solve reads one 16-byte element
+// per coefficient row.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+
+namespace pcf {
+
+struct Geo {
+  int n0, nb, nd;  // Chebyshev points / clamped / Dirichlet dofs
+  int ny, nz, kzs; // Fourier modes; kzs = nz/2 stored spanwise modes
+  int my, mz;      // padded physical grid
+  int m;           // modes per wall-normal index = ny * kzs
+  double df1, df2; // 2 pi / L (scaled wavenumber factors)
+  double uwall;    // base flow U = uwall * x
+  double W;        // Fourier mass weight (2pi/df1)(2pi/df2) of every Galerkin row
+  double nu;
+  int npmax;       // largest parity block of the per-mode systems (<= 64)
+};
+
+// Radix plan of one FFT length (radices 2, 3, 4).
+struct FFTPlan {
+  int n;
+  int nst;
+  int radix[12];
+};
+
+// Contraction matrices in the precision T, transposed so that a chunk of
+// outputs is contiguous (wave-uniform scalar loads).
+template <typename T>
+struct NodalMats {
+  const T *VBt;   // [nb][ldx]  VB^T  (zero padded to ldx)
+  const T *VDt;   // [nd][ldx]
+  const T *VD1t;  // [nd][ldx]
+  int ldx;        // >= n0, multiple of the x-chunk
+  const T *A1t;   // [n0][ldi]  A1^T
+  const T *A0t;   // [n0][ldi]
+  const T *AGt;   // [n0][ldi]
+  int ldi;        // >= nd, multiple of the row chunk
+  const T *xnode; // [n0]
+  const T *twy;   // [my] exp(-2 pi i t / my) as (cos, sin) pairs
+  const T *twz;   // [mz]
+};
+
+// Parity-split Galerkin blocks for the per-mode solves.  For basis s (0 = clamped,
+// 1 = Dirichlet), parity p, block t (0: C0, 1: C2, 2: C4): a (np x np) row-major
+// copy at row[s][p][t] and a column-major copy at col[s][p][t].
+struct SolveMats {
+  const double *row[2][2][3];
+  const double *col[2][2][3];
+  int np[2][2];
+  const double *Rt;  // [nb][nd] column-major reconstruction matrix R
+};
+
+// One implicit update: rhs = M z + L zL + sum_l e_l N_l with z = sum c_l x_l,
+// zL = sum d_l x_l; solve (M - gamma L) x = rhs, then rebuild u1/u2.
+struct SolveArgs {
+  const double2 *u0[3];
+  const double2 *g[3];
+  const double *v[3];
+  const double *w[3];
+  double c[3], d[3];
+  int nx;
+  const double2 *Nu[3];
+  const double2 *Ng[3];
+  const double *Nv[3];
+  const double *Nw[3];
+  double e[3];
+  int nn;
+  double gamma;
+  double2 *u0_out, *g_out;
+  double *v_out, *w_out;
+  double2 *u1, *u2;
+};
+
+struct Workspace {
+  void *X;   // [6][n0][m]   nodal u, omega
+  void *Y;   // [6][n0][my][kzs]
+  void *Hz;  // [3][n0][my][kzs]
+  void *Hx;  // [3][n0][m]
+};
+
+// Launchers (return hipError_t of the launch).
+template <typename T>
+hipError_t launch_nodal_backward(const Geo &g, const NodalMats<T> &nm, const double2 *u0,
+                                 const double2 *u1, const double2 *u2, Workspace &ws,
+                                 const FFTPlan &py, hipStream_t s);
+template <typename T>
+hipError_t launch_rotational(const Geo &g, const NodalMats<T> &nm, Workspace &ws,
+                             const FFTPlan &pz, hipStream_t s);
+template <typename T>
+hipError_t launch_forward_rhs(const Geo &g, const NodalMats<T> &nm, Workspace &ws,
+                              const FFTPlan &py, double2 *Nu, double2 *Ng, double *Nv,
+                              double *Nw, hipStream_t s);
+hipError_t launch_solve(const Geo &g, const SolveMats &sm, const SolveArgs &a, hipStream_t s);
+
+// Diagnostics partial sums: out[0] = sum wF |u|^2 (perturbation), out[1] = sum
+// wF |div u|^2, both over the full spectrum (Hermitian weights applied), and
+// out[2] = sum_x wF Re u1_00(x) x, out[3] = sum_x wF Re du1_00/dx(x).
+hipError_t launch_diag(const Geo &g, const double *VB, const double *VD, const double *VB1,
+                       const double *VD1, const double *fejer, const double *xnode,
+                       const double2 *u0, const double2 *u1, const double2 *u2, double *out,
+                       hipStream_t s);
+
+int kernels_per_stage_rhs();
+size_t solve_lds_bytes(const Geo &g);
+// Largest butterfly count per thread any FFT launch needs (host-side check
+// against the kernels' register budgets).
+bool fft_fits(const Geo &g, const FFTPlan &py, const FFTPlan &pz, int elem_bytes);
+
+}  // namespace pcf

@@ -1,0 +1,55 @@
+// Host-side operator construction for the plane-Couette KMM step.
+//
+// Everything the kernels need is a dense 1-D matrix along the wall-normal
+// Chebyshev axis, built once here in long double from closed forms and then
+// rounded to double (and float for the mixed-precision stages):
+//
+//  * Shen stencils (reference src/jaxfun/galerkin/composite.py:816-826)
+//      Dirichlet  phi_i = T_i - T_{i+2}
+//      clamped    psi_i = T_i + 2(-i-2)/(i+3) T_{i+2} + (i+1)/(i+3) T_{i+4}
+//  * Chebyshev derivative matrix D (integer valued: T_b' = sum_a D[a][b] T_a),
+//    so D^2 and D^4 are exact in double.
+//  * Galerkin blocks C_m = S diag(h) D^m S^T with h = (pi, pi/2, ...) — the
+//    Chebyshev-weighted (psi_i, psi_j^{(m)}) the reference assembles with
+//    SymPy inner() (examples/channelflow_kmm.py:261-409).
+//  * Transform matrices: basis values / x-derivatives at the Gauss nodes
+//    x_j = cos(pi + (2j+1) pi / (2 n0)) (Chebyshev.py:159-181), and the
+//    Galerkin projection of nodal values onto the Dirichlet basis,
+//    F_D = M_D^{-1} S_D Wq with Wq[n][j] = (pi/n0) T_n(x_j)
+//    (Chebyshev.py:303-318, composite.py:361-365).
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace pcf {
+
+struct Mat {
+  int rows = 0, cols = 0;
+  std::vector<double> a;  // row-major
+  double &operator()(int i, int j) { return a[(size_t)i * cols + j]; }
+  double operator()(int i, int j) const { return a[(size_t)i * cols + j]; }
+};
+
+struct Operators {
+  int n0 = 0, nb = 0, nd = 0;
+  // Galerkin blocks
+  Mat C0B, C2B, C4B, C0D, C2D;
+  // nodal transforms
+  Mat VB, VD, VD1, VB1;       // (n0 x nb|nd): basis values / x-derivatives at nodes
+  Mat A1, A0, AG;             // (nb|nd x n0): nonlinear rows from nodal values
+  Mat R;                      // (nd x nb): f = P_D(d/dx u0)
+  Mat FB, FD;                 // (nb|nd x n0): Galerkin projection of nodal values
+  std::vector<double> x;      // Gauss nodes (ascending)
+  std::vector<double> fejer;  // Fejer-I weights on the nodes
+  std::vector<double> dphi_top, dphi_bot;  // phi_i'(+1), phi_i'(-1)
+};
+
+// Returns an empty string on success, else an error message.
+std::string build_operators(int n0, Operators &ops);
+
+// Export one matrix by index (see pcf_setup_matrix in include/pcf_dns.h).
+const Mat *operator_by_index(const Operators &ops, int which);
+
+}  // namespace pcf

@@ -1,0 +1,381 @@
+"""Plane-Couette fluctuation DNS on the MI355X: drop-in for the reference solver.
+
+``PlaneCouetteFluctuationHIP`` mirrors ``PlaneCouetteFluctuationJax``
+(reference ``examples/pcf_fluctuations_jax.py:29-161`` over
+``examples/channelflow_kmm.py:104-1308``): the same constructor keywords, the
+same ``KMMState`` fields in the same full-complex layout, and the same
+``initial_state / step / solve / set_dt / diagnostics`` methods, so a runner
+that builds the reference solver from a problem spec
+(``production/oracles.py:1442-1480``) and drives ``solver.solve(state, steps)``
+(``production/oracles.py:3110-3123``) can construct this class instead.
+
+Every step runs in libpcfdns.so (hand-written HIP kernels); this module only
+moves state across the boundary and builds initial conditions from the
+library's closed-form setup matrices.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass, replace
+
+import numpy as np
+
+from . import _native as nat
+
+_PRECISIONS = {"fp64": nat.PREC_FP64, "float64": nat.PREC_FP64, "mixed": nat.PREC_MIXED}
+
+
+@dataclass(frozen=True)
+class KMMState:
+    """Coefficient-space KMM state (reference ``channelflow_kmm.py:50-101``).
+
+    ``u = (u0, u1, u2)`` with u0 in the clamped space TB, u1/u2 and g in the
+    Dirichlet space TD, all complex128 of shape (n, Ny, Nz) in FFT order.
+    History tuples are ``(Nu, Ng, Nv00, Nw00)`` and ``(u0, g, v00, w00)``.
+    """
+
+    u: tuple
+    g: np.ndarray
+    nonlinear_old: tuple | None = None
+    nonlinear_older: tuple | None = None
+    solution_old: tuple | None = None
+    solution_older: tuple | None = None
+    history_steps: float = 0.0
+    have_old: float = 0.0
+    previous_dt: float = 0.0
+
+
+class PlaneCouetteFluctuationHIP:
+    """KMM SBDF3 plane-Couette solver whose step runs in HIP on one GPU.
+
+    Constructor follows ``PlaneCouetteFluctuationJax.__init__``
+    (``examples/pcf_fluctuations_jax.py:35-75``).  Built options:
+    Chebyshev family, SBDF3 with IMEXRK3 startup, rotational nonlinear form,
+    optimized coefficient path.  Other valid reference values raise
+    ``NotImplementedError``; invalid values raise ``ValueError`` as in
+    ``examples/channelflow_kmm.py:140-187``.
+    """
+
+    def __init__(
+        self,
+        N=(17, 16, 16),
+        domain=((-1.0, 1.0), (0.0, 4.0 * math.pi), (0.0, 2.0 * math.pi)),
+        Re: float = 600.0,
+        U_wall: float = 1.0,
+        dt: float = 0.01,
+        family: str = "C",
+        padding_factor=(1.0, 1.5, 1.5),
+        perturbation_amplitude: float = 0.05,
+        timestepper=None,
+        time_integrator: str | None = "SBDF3",
+        nonlinear_form: str = "rotational",
+        coefficient_path: str = "optimized",
+        solve_batching: str = "batched",
+        precision: str = "fp64",
+        device: int = 0,
+    ) -> None:
+        nonlinear_form = str(nonlinear_form).lower()
+        if nonlinear_form not in {"gradient", "rotational"}:
+            raise ValueError("nonlinear_form must be 'gradient' or 'rotational'")
+        coefficient_path = str(coefficient_path).lower()
+        if coefficient_path not in {"optimized", "transform"}:
+            raise ValueError("coefficient_path must be 'optimized' or 'transform'")
+        solve_batching = str(solve_batching).lower()
+        if solve_batching not in {"batched", "separate"}:
+            raise ValueError("solve_batching must be 'batched' or 'separate'")
+        if time_integrator is not None and time_integrator not in {"IMEXRK222", "IMEXRK3", "CNAB2", "SBDF3"}:
+            raise ValueError("time_integrator must be one of {'IMEXRK222', 'IMEXRK3', 'CNAB2', 'SBDF3'}")
+        fam = str(family).upper()
+        if not (fam.startswith("C") or fam.startswith("L")):
+            raise ValueError("family must be 'L' or 'C'")
+        if fam.startswith("L"):
+            raise NotImplementedError("only the Chebyshev family is built")
+        if time_integrator != "SBDF3":
+            raise NotImplementedError("only SBDF3 (IMEXRK3 startup) is built")
+        if nonlinear_form != "rotational":
+            raise NotImplementedError("only the rotational nonlinear form is built")
+        if coefficient_path != "optimized":
+            raise NotImplementedError("only the optimized coefficient path is built")
+        if str(precision).lower() not in _PRECISIONS:
+            raise ValueError("precision must be 'fp64' or 'mixed'")
+        if tuple(float(v) for v in domain[0]) != (-1.0, 1.0):
+            raise ValueError("wall-normal domain must be (-1, 1)")
+
+        self.N = tuple(int(n) for n in N)
+        self.domain = domain
+        self.Re = float(Re)
+        self.U_wall = float(U_wall)
+        self.nu = self.U_wall / self.Re
+        self.dt = float(dt)
+        self.family = "C"
+        self.padding_factor = tuple(float(p) for p in padding_factor)
+        self.perturbation_amplitude = float(perturbation_amplitude)
+        self.time_integrator = "SBDF3"
+        self.nonlinear_form = nonlinear_form
+        self.coefficient_path = coefficient_path
+        self.solve_batching = solve_batching
+        self.precision = str(precision).lower()
+        self.Ly = float(domain[1][1] - domain[1][0])
+        self.Lz = float(domain[2][1] - domain[2][0])
+
+        cfg = nat.PcfConfig()
+        cfg.abi_version = nat.ABI_VERSION
+        cfg.n0, cfg.ny, cfg.nz = self.N
+        cfg.lx[0], cfg.lx[1] = -1.0, 1.0
+        cfg.ly, cfg.lz = self.Ly, self.Lz
+        cfg.nu = self.nu
+        cfg.u_wall = self.U_wall
+        cfg.dt = self.dt
+        cfg.padding_y, cfg.padding_z = self.padding_factor[1], self.padding_factor[2]
+        cfg.precision = _PRECISIONS[self.precision]
+        cfg.time_integrator = nat.TI_SBDF3
+        cfg.nonlinear_form = nat.NL_ROTATIONAL
+        cfg.device = int(device)
+        self._cfg = cfg
+        self._lib = nat.load()
+        handle = C.c_void_p()
+        nat.check(self._lib.pcf_create(C.byref(cfg), C.byref(handle)))
+        self._h = handle
+        self._loaded = None  # id of the state object resident on the device
+
+    # ------------------------------------------------------------ lifecycle
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._lib.pcf_destroy(self._h)
+            self._h = None
+
+    def __del__(self):  # pragma: no cover - interpreter shutdown order
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ------------------------------------------------------------ geometry
+    @property
+    def nb(self) -> int:
+        return self.N[0] - 4
+
+    @property
+    def nd(self) -> int:
+        return self.N[0] - 2
+
+    def matrix(self, name: str) -> np.ndarray:
+        return nat.setup_matrix(self._cfg, name)
+
+    def wavenumbers(self):
+        """Scaled (ky, kz) grids in FFT order (``tensorproductspace.py:253-271``)."""
+        def k(n, L):
+            idx = np.arange(n)
+            kk = np.where(idx < (n + 1) // 2, idx, idx - n).astype(float)
+            df = 2 * math.pi / L if abs(L - 2 * math.pi) > 1e-12 else 1.0
+            return kk * df
+        return k(self.N[1], self.Ly)[:, None], k(self.N[2], self.Lz)[None, :]
+
+    def mask_nyquist(self, c: np.ndarray) -> np.ndarray:
+        c = np.array(c, copy=True)
+        c[..., self.N[1] // 2, :] = 0
+        c[..., :, self.N[2] // 2] = 0
+        return c
+
+    # ------------------------------------------------------------ states
+    def _with_zero_history(self, u, g) -> KMMState:
+        """``_reset_flow_history`` (``channelflow_kmm.py:637-652``)."""
+        zu = np.zeros_like(u[0])
+        zg = np.zeros_like(g)
+        zv = np.zeros(self.nd)
+        z = (zu, zg, zv, zv)
+        return KMMState(u=tuple(u), g=g, nonlinear_old=z, nonlinear_older=z, solution_old=z,
+                        solution_older=z, history_steps=0.0, have_old=0.0, previous_dt=0.0)
+
+    def zero_state(self) -> KMMState:
+        n1, n2 = self.N[1], self.N[2]
+        u = (np.zeros((self.nb, n1, n2), complex), np.zeros((self.nd, n1, n2), complex),
+             np.zeros((self.nd, n1, n2), complex))
+        return self._with_zero_history(u, np.zeros((self.nd, n1, n2), complex))
+
+    def state_from_separable(self, profiles, fourier):
+        """Initial state from separable physical fields (host setup, closed form).
+
+        ``profiles[i]`` are wall-normal nodal values and ``fourier[i]`` the
+        (Ny, Nz) full-complex Fourier coefficients of component i; the result
+        equals ``state_from_physical`` (``channelflow_kmm.py:680-686``) of the
+        product fields, because the Galerkin projection acts on the wall-normal
+        factor only.
+        """
+        FB, FD = self.matrix("FB"), self.matrix("FD")
+        u0 = self.mask_nyquist(np.einsum("i,jk->ijk", FB @ profiles[0], fourier[0]))
+        u1 = self.mask_nyquist(np.einsum("i,jk->ijk", FD @ profiles[1], fourier[1]))
+        u2 = self.mask_nyquist(np.einsum("i,jk->ijk", FD @ profiles[2], fourier[2]))
+        ky, kz = self.wavenumbers()
+        g = self.mask_nyquist(1j * ky[None] * u2 - 1j * kz[None] * u1)
+        return self._with_zero_history((u0, u1, u2), g)
+
+    def initial_state(self) -> KMMState:
+        """Analytic fluctuation IC (``pcf_fluctuations_jax.py:77-97``).
+
+        u0 = a (1-x^2) sin(2 pi y/Ly) cos(2 pi z/Lz), u1 = a (1-x^2) cos(.) sin(.),
+        u2 = a (1-x^2) sin(4 pi y/Ly) cos(4 pi z/Lz).  The Fourier factors have
+        exact two-term expansions; on the reference's discrete grid
+        (fft(norm='forward')) they are the coefficients below.
+        """
+        a = self.perturbation_amplitude
+        x = self.matrix("VB")  # only for the node count; nodes come from the library
+        n0, n1, n2 = self.N
+        xn = np.cos(np.pi + (2 * np.arange(n0) + 1) * np.pi / (2 * n0))
+        prof = a * (1.0 - xn**2)
+
+        def coeffs(kind_y, my, kind_z, mz):
+            # sin(2 pi m t/L) -> (e^{i m} - e^{-i m}) / 2i ; cos -> (e^{i m} + e^{-i m}) / 2
+            cy = np.zeros(n1, complex)
+            cz = np.zeros(n2, complex)
+            if kind_y == "sin":
+                cy[my], cy[-my] = 1 / 2j, -1 / 2j
+            else:
+                cy[my], cy[-my] = 0.5, 0.5
+            if kind_z == "sin":
+                cz[mz], cz[-mz] = 1 / 2j, -1 / 2j
+            else:
+                cz[mz], cz[-mz] = 0.5, 0.5
+            return np.outer(cy, cz)
+
+        del x
+        f0 = coeffs("sin", 1, "cos", 1)
+        f1 = coeffs("cos", 1, "sin", 1)
+        f2 = coeffs("sin", 2, "cos", 2)
+        return self.state_from_separable((prof, prof, prof), (f0, f1, f2))
+
+    # ------------------------------------------------------------ device I/O
+    def _set(self, name, arr):
+        arr = np.ascontiguousarray(arr)
+        if np.iscomplexobj(arr):
+            arr = arr.astype(np.complex128, copy=False)
+        else:
+            arr = arr.astype(np.float64, copy=False)
+        nat.check(self._lib.pcf_set_field(self._h, nat.FIELD_ID[name],
+                                          arr.ctypes.data_as(C.POINTER(C.c_double))))
+
+    def _get(self, name, rows, vector=False):
+        if vector:
+            out = np.empty(rows, np.float64)
+        else:
+            out = np.empty((rows, self.N[1], self.N[2]), np.complex128)
+        nat.check(self._lib.pcf_get_field(self._h, nat.FIELD_ID[name],
+                                          out.ctypes.data_as(C.POINTER(C.c_double))))
+        return out
+
+    def load(self, state: KMMState) -> None:
+        """Copy a state (with its history) to the device."""
+        if state.nonlinear_old is None or state.solution_old is None:
+            state = self._with_zero_history(state.u, state.g)
+        self._set("U0", state.u[0])
+        self._set("U1", state.u[1])
+        self._set("U2", state.u[2])
+        self._set("G", state.g)
+        for tag, grp in (("NL_OLD", state.nonlinear_old), ("NL_OLDER", state.nonlinear_older),
+                         ("SOL_OLD", state.solution_old), ("SOL_OLDER", state.solution_older)):
+            for comp, arr in zip("UGVW", grp):
+                self._set(f"{tag}_{comp}", np.real(arr) if comp in "VW" else arr)
+        nat.check(self._lib.pcf_set_history(self._h, float(state.history_steps),
+                                            float(state.have_old), float(state.previous_dt)))
+        self._loaded = None
+
+    def fetch(self) -> KMMState:
+        """Copy the device state back (full-complex reference layout)."""
+        nb, nd = self.nb, self.nd
+        u = (self._get("U0", nb), self._get("U1", nd), self._get("U2", nd))
+        g = self._get("G", nd)
+        groups = []
+        for tag in ("NL_OLD", "NL_OLDER", "SOL_OLD", "SOL_OLDER"):
+            groups.append((self._get(f"{tag}_U", nb), self._get(f"{tag}_G", nd),
+                           self._get(f"{tag}_V", nd, True), self._get(f"{tag}_W", nd, True)))
+        hs, ho, pd = C.c_double(), C.c_double(), C.c_double()
+        nat.check(self._lib.pcf_get_history(self._h, C.byref(hs), C.byref(ho), C.byref(pd)))
+        st = KMMState(u=u, g=g, nonlinear_old=groups[0], nonlinear_older=groups[1],
+                      solution_old=groups[2], solution_older=groups[3],
+                      history_steps=hs.value, have_old=ho.value, previous_dt=pd.value)
+        self._loaded = id(st)
+        return st
+
+    def _ensure_loaded(self, state: KMMState) -> None:
+        if self._loaded != id(state):
+            self.load(state)
+
+    def advance(self, steps: int) -> None:
+        """Advance the device-resident state (no host transfer)."""
+        nat.check(self._lib.pcf_step(self._h, int(steps)))
+
+    def advance_steady(self, steps: int, graph_block: int = 0) -> None:
+        """Steady SBDF3 steps only; with graph_block (multiple of 12) replay a HIP graph."""
+        if graph_block:
+            if steps % graph_block:
+                raise ValueError("steps must be a multiple of graph_block")
+            nat.check(self._lib.pcf_step_steady_graph(self._h, int(graph_block), steps // graph_block))
+        else:
+            nat.check(self._lib.pcf_step_steady(self._h, int(steps)))
+
+    def synchronize(self) -> None:
+        nat.check(self._lib.pcf_synchronize(self._h))
+
+    # ------------------------------------------------------------ reference API
+    def step(self, state: KMMState, dt=None, factor_args=None) -> KMMState:
+        """One step (``channelflow_kmm.py:1123-1137``); state in, new state out."""
+        if dt is not None and not math.isclose(float(dt), self.dt, rel_tol=8 * np.finfo(float).eps):
+            raise NotImplementedError(
+                "SBDF3 currently supports fixed dt only; start a fresh solver/history "
+                "for a different timestep")
+        self._ensure_loaded(state)
+        self.advance(1)
+        return self.fetch()
+
+    def solve(self, state: KMMState, steps: int) -> KMMState:
+        """``KMM.solve`` (``channelflow_kmm.py:1231-1247``)."""
+        self._ensure_loaded(state)
+        self.advance(int(steps))
+        return self.fetch()
+
+    def set_dt(self, dt: float) -> None:
+        """``KMM.set_dt`` (``channelflow_kmm.py:1207-1222``): SBDF3 is fixed-step."""
+        if not math.isclose(float(dt), self.dt, rel_tol=8.0 * np.finfo(float).eps, abs_tol=0.0):
+            raise NotImplementedError(
+                "SBDF3 currently supports fixed dt only; start a fresh solver/history "
+                "for a different timestep")
+
+    def diagnostics(self, state: KMMState | None = None) -> dict:
+        """``PlaneCouetteFluctuationJax.diagnostics`` (``pcf_fluctuations_jax.py:131-161``)."""
+        if state is not None:
+            self._ensure_loaded(state)
+        out = np.zeros(8)
+        nat.check(self._lib.pcf_diagnostics(self._h, out.ctypes.data_as(C.POINTER(C.c_double))))
+        keys = ("Epert", "Etot", "divL2", "u_top", "u_bot", "wall_shear_top", "wall_shear_bot",
+                "mean_shear")
+        return {k: float(v) for k, v in zip(keys, out)}
+
+    @staticmethod
+    def benchmark_state_fields(state: KMMState):
+        return (*state.u, state.g)
+
+
+def state_leaves(state: KMMState) -> list:
+    """Leaves in the reference pytree order (``channelflow_kmm.py:65-76``)."""
+    out = list(state.u) + [state.g]
+    for grp in (state.nonlinear_old, state.nonlinear_older, state.solution_old, state.solution_older):
+        out.extend(grp)
+    out.extend([np.asarray(state.history_steps), np.asarray(state.have_old), np.asarray(state.previous_dt)])
+    return out
+
+
+def state_checksum(state: KMMState) -> dict:
+    """Full-state checksum as ``production/benchmark.py:234-260`` computes it."""
+    real_sum = imag_sum = norm2 = 0.0
+    count = 0
+    leaves = state_leaves(state)
+    for v in leaves:
+        v = np.asarray(v)
+        real_sum += float(np.sum(np.real(v), dtype=np.float64))
+        imag_sum += float(np.sum(np.imag(v), dtype=np.float64))
+        norm2 += float(np.sum(np.abs(v) ** 2, dtype=np.float64))
+        count += int(v.size)
+    return {"leaf_count": len(leaves), "element_count": count, "real_sum": real_sum,
+            "imag_sum": imag_sum, "l2_norm": math.sqrt(norm2)}

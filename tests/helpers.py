@@ -1,0 +1,58 @@
+"""Shared helpers for the parity tests (oracle <-> product state conversion)."""
+
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from jaxfun_amd.pcf import KMMState
+from oracle.kmm import OracleState, PCFOracle
+
+
+def to_oracle(s: KMMState) -> OracleState:
+    return OracleState(u=tuple(np.array(a) for a in s.u), g=np.array(s.g),
+                       nonlinear_old=tuple(np.array(a) for a in s.nonlinear_old),
+                       nonlinear_older=tuple(np.array(a) for a in s.nonlinear_older),
+                       solution_old=tuple(np.array(a) for a in s.solution_old),
+                       solution_older=tuple(np.array(a) for a in s.solution_older),
+                       history_steps=float(s.history_steps), have_old=float(s.have_old),
+                       previous_dt=float(s.previous_dt))
+
+
+def leaves(s) -> list:
+    out = list(s.u) + [s.g]
+    for grp in (s.nonlinear_old, s.nonlinear_older, s.solution_old, s.solution_older):
+        out.extend(grp)
+    return out
+
+
+LEAF_NAMES = ["u0", "u1", "u2", "g"] + [f"{t}.{c}" for t in ("nl_old", "nl_older", "sol_old", "sol_older")
+                                        for c in ("u", "g", "v", "w")]
+
+
+def max_rel_errors(a, b) -> dict:
+    """Per-leaf max |a-b| / scale, b the reference.
+
+    scale = max(max |b_leaf|, 1e-3 * max |b| over the leaf's group), the groups
+    being the state (u0, u1, u2, g) and each history tuple.  The floor keeps a
+    leaf that is pure round-off in both (e.g. the mean-mode nonlinear rows of a
+    symmetric initial field) from reading as an O(1) relative error.
+    """
+    la, lb = leaves(a), leaves(b)
+    groups = [range(0, 4)] + [range(4 + 4 * k, 8 + 4 * k) for k in range(4)]
+    out = {}
+    for grp in groups:
+        gmax = max((float(np.max(np.abs(np.asarray(lb[i])))) for i in grp), default=0.0)
+        for i in grp:
+            x, y = np.asarray(la[i]), np.asarray(lb[i])
+            scale = max(float(np.max(np.abs(y))), 1e-3 * gmax)
+            err = float(np.max(np.abs(x - y)))
+            out[LEAF_NAMES[i]] = err / scale if scale > 0 else err
+    return out
+
+
+def oracle_for(solver) -> PCFOracle:
+    return PCFOracle(N=solver.N, domain=solver.domain, Re=solver.Re, U_wall=solver.U_wall, dt=solver.dt,
+                     padding_factor=solver.padding_factor,
+                     perturbation_amplitude=solver.perturbation_amplitude)
