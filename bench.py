@@ -1,0 +1,220 @@
+#!/usr/bin/env python3
+"""Throughput of the plane-Couette KMM SBDF3 step on one MI355X.
+
+Workload (one "step" = one steady SBDF3 rotational step of the whole field):
+65x64x64 (65 Chebyshev points, 64x64 Fourier modes, 3/2 dealiasing), Re=1000,
+dt=0.005, seeded divergence-free perturbation with energy 1e-3 in the lowest
+12 modes, FP64 throughout.  The two IMEXRK3 startup steps, operator setup and
+the warm-up are excluded; the timed steps replay 12-step HIP graphs with the
+state resident in HBM.
+
+Also reported on the same line:
+  * the mixed-precision step (FP32 transforms and nonlinear term, FP64 solves)
+    and its throughput ratio to FP64;
+  * per-kernel mean durations from HIP events on the library's stream, and the
+    roofline of the dominant kernel;
+  * the CPU FP64 oracle timed on this host's cores on a bounded sample.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "DNS steps/s at 65×64×64: mixed vs FP64 on GPU; adjoint/forward cost ratio"
+WORKLOAD = dict(N=(65, 64, 64), Re=1000.0, dt=0.005, energy=1e-3, K=12, seed=0)
+PUBLISHED_STEPS_PER_S = 1000.0 / 27.552  # reference, RTX 5090 Laptop FP64
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+FP32_PEAK_TFLOPS = 157.3
+GRAPH_BLOCK = 12
+
+
+def stage_model(solver, precision: str) -> dict:
+    """Algorithmic HBM bytes and flops per launch of each stage kernel.
+
+    Each stage reads its inputs once and writes its outputs once; spectral
+    state and history are complex FP64 (16 B); nodal/physical intermediates
+    are complex in the compute precision (16 B FP64, 8 B FP32).
+    """
+    n0, ny, nz = solver.N
+    nb, nd = n0 - 4, n0 - 2
+    kzs = nz // 2
+    m = ny * kzs
+    my = int(1.5 * ny)
+    V = 16 if precision == "fp64" else 8
+    c16 = 16
+    nodal_b = (nb + 2 * nd) * m * c16 + 6 * n0 * m * V
+    nodal_f = m * n0 * (4 * nb + 16 * nd)
+    ffti_b = 6 * n0 * m * V + 6 * n0 * my * kzs * V
+    mz = int(1.5 * nz)
+    fft_flops = lambda n, count: 5.0 * n * __import__("math").log2(n) * count  # noqa: E731
+    ffti_f = fft_flops(my, 6 * n0 * kzs)
+    rot_b = 6 * n0 * my * kzs * V + 3 * n0 * my * kzs * V
+    rot_f = fft_flops(mz, 3 * n0 * my // 2) * 2 + fft_flops(mz, 3 * n0 * my // 2) + 30 * n0 * my * mz
+    fftf_b = 3 * n0 * my * kzs * V + 3 * n0 * m * V
+    fftf_f = fft_flops(my, 3 * n0 * kzs)
+    rhs_b = 3 * n0 * m * V + (nb + nd) * m * c16
+    rhs_f = m * n0 * 16 * nd
+    solve_b = (3 * (nb + nd) * m * c16) * 2 + (nb + nd) * m * c16 + 2 * nd * m * c16
+    solve_f = m * (2 * (nb * nb + nd * nd) * 4 + 4 * nb * nd)
+    return {
+        "nodal_backward": (nodal_b, nodal_f),
+        "fft_y_inverse": (ffti_b, ffti_f),
+        "rotational": (rot_b, rot_f),
+        "fft_y_forward": (fftf_b, fftf_f),
+        "forward_rhs": (rhs_b, rhs_f),
+        "solve": (solve_b, solve_f),
+    }
+
+
+def profile_stages(solver, nsteps: int) -> dict:
+    lib = solver._lib
+    n = lib.pcf_stage_count()
+    out = (C.c_double * n)()
+    from jaxfun_amd import _native as nat
+
+    nat.check(lib.pcf_profile_steady(solver._h, int(nsteps), out))
+    return {lib.pcf_stage_name(i).decode(): float(out[i]) for i in range(n)}
+
+
+def run_gpu(precision: str, steps: int, warmup: int):
+    from jaxfun_amd import PlaneCouetteFluctuationHIP
+    from jaxfun_amd.ic import seeded_state
+
+    w = WORKLOAD
+    solver = PlaneCouetteFluctuationHIP(N=w["N"], Re=w["Re"], dt=w["dt"], precision=precision)
+    state = seeded_state(solver, seed=w["seed"], energy=w["energy"], K=w["K"])
+    solver.load(state)
+    solver.advance(2)  # IMEXRK3 startup, excluded
+    solver.advance_steady(warmup, graph_block=GRAPH_BLOCK)
+    solver.synchronize()
+    t0 = time.perf_counter()
+    solver.advance_steady(steps, graph_block=GRAPH_BLOCK)
+    solver.synchronize()
+    t1 = time.perf_counter()
+    stages = profile_stages(solver, 48)
+    diag = solver.diagnostics()
+    return solver, state, (t1 - t0), stages, diag
+
+
+def roofline(solver, precision: str, stages: dict) -> dict:
+    model = stage_model(solver, precision)
+    name = max(stages, key=lambda k: stages[k])
+    nbytes, flops = model[name]
+    sec = stages[name] * 1e-3
+    peak_f = (FP64_PEAK_TFLOPS if precision == "fp64" else FP32_PEAK_TFLOPS) * 1e12
+    hbm_t = nbytes / (HBM_PEAK_GBS * 1e9)
+    cmp_t = flops / peak_f
+    if cmp_t > hbm_t:
+        ach = flops / sec / 1e12
+        return {"kernel": name, "bound": "mfma", "achieved": ach, "peak": peak_f / 1e12,
+                "unit": "TFLOP/s", "frac": ach / (peak_f / 1e12), "traffic": None,
+                "algorithmic_bytes": nbytes, "flops": flops, "avg_ms": stages[name]}
+    ach = nbytes / sec / 1e9
+    return {"kernel": name, "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": ach / HBM_PEAK_GBS, "traffic": None, "algorithmic_bytes": nbytes,
+            "flops": flops, "avg_ms": stages[name]}
+
+
+def cpu_baseline(state, budget_s: float) -> dict:
+    """FP64 oracle (numpy/scipy port of the reference algorithm) on host cores."""
+    import numpy as np
+
+    from oracle.kmm import OracleState, PCFOracle, set_workers
+
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:  # pragma: no cover
+        cores = os.cpu_count() or 1
+    cores = min(cores, int(os.environ.get("OMP_NUM_THREADS", cores)))
+    set_workers(cores)
+    w = WORKLOAD
+    o = PCFOracle(N=w["N"], Re=w["Re"], dt=w["dt"])
+    s = OracleState(u=tuple(np.array(a) for a in state.u), g=np.array(state.g))
+    s = o.reset_history(s)
+    s = o.solve(s, 3)  # two startup steps + one warm steady step, excluded
+    n = 0
+    t0 = time.perf_counter()
+    while True:
+        s = o.step(s)
+        n += 1
+        el = time.perf_counter() - t0
+        if n >= 3 and el >= budget_s:
+            break
+    return {"value": n / el, "unit": "steps/s", "cores": cores, "kind": "port",
+            "sample": f"{n} steady SBDF3 steps of the 65x64x64 workload after 2 startup + 1 warm-up "
+                      f"step (scipy.fft workers={cores}, FP64 oracle)"}
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2400)
+    ap.add_argument("--warmup", type=int, default=240)
+    ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    args = ap.parse_args(argv)
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
+              "(this build has no multi-GPU path)", file=sys.stderr)
+        return 2
+    steps = max(GRAPH_BLOCK, (args.steps // GRAPH_BLOCK) * GRAPH_BLOCK)
+    warmup = max(GRAPH_BLOCK, (args.warmup // GRAPH_BLOCK) * GRAPH_BLOCK)
+
+    solver, state, el, stages, diag = run_gpu("fp64", steps, warmup)
+    value = steps / el
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": warmup,
+        "ms_per_step": 1e3 * el / steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": value / PUBLISHED_STEPS_PER_S,
+        "dtype": "f64",
+        "data": "synthetic (seeded divergence-free perturbation, seed 0)",
+        "config": {"workload": "plane-Couette KMM SBDF3 rotational step, 65x64x64, Re=1000, "
+                               "dt=0.005, E=1e-3 over the lowest 12 modes, FP64",
+                   "N": list(WORKLOAD["N"]), "Re": WORKLOAD["Re"], "dt": WORKLOAD["dt"],
+                   "padding": [1.0, 1.5, 1.5], "graph_block_steps": GRAPH_BLOCK},
+        "stages_ms": stages,
+        "roofline": roofline(solver, "fp64", stages),
+        "final_diagnostics": diag,
+        "adjoint_forward_ratio": None,
+    }
+    print(f"# fp64: {value:.1f} steps/s ({1e3 * el / steps:.4f} ms/step)", file=sys.stderr, flush=True)
+    solver.close()
+    if not args.no_mixed:
+        msolver, _, mel, mstages, mdiag = run_gpu("mixed", steps, warmup)
+        mval = steps / mel
+        line["mixed"] = {"value": mval, "unit": "steps/s", "ms_per_step": 1e3 * mel / steps,
+                         "ratio_vs_fp64": mval / value, "stages_ms": mstages,
+                         "roofline": roofline(msolver, "mixed", mstages)}
+        print(f"# mixed: {mval:.1f} steps/s", file=sys.stderr, flush=True)
+        msolver.close()
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(state, args.cpu_seconds)
+        print(f"# cpu: {line['cpu_baseline']['value']:.3f} steps/s", file=sys.stderr, flush=True)
+    else:
+        line["cpu_baseline"] = None
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

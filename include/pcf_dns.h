@@ -149,8 +149,18 @@ int32_t pcf_step_steady_graph(pcf_handle *h, int32_t nsteps, int32_t replays);
  * (examples/pcf_fluctuations_jax.py:131-161). */
 int32_t pcf_diagnostics(pcf_handle *h, double *out8);
 
-/* Per-stage timing support: number of kernels launched by one steady step. */
+/* Per-stage timing.  One steady step launches pcf_kernels_per_step() kernels,
+ * one per stage, in the order pcf_stage_name(0..pcf_stage_count()-1):
+ * nodal_backward, fft_y_inverse, rotational, fft_y_forward, forward_rhs, solve.
+ * pcf_profile_steady runs nsteps steady steps eagerly with HIP events around
+ * every kernel on the handle's stream and writes the mean duration of each
+ * stage (milliseconds) to stage_ms[pcf_stage_count()].  No reference
+ * counterpart (the reference reports only whole-step wall time,
+ * production/benchmark.py:282-354). */
 int32_t pcf_kernels_per_step(pcf_handle *h);
+int32_t pcf_stage_count(void);
+const char *pcf_stage_name(int32_t i);
+int32_t pcf_profile_steady(pcf_handle *h, int32_t nsteps, double *stage_ms);
 
 #ifdef __cplusplus
 }

@@ -42,7 +42,7 @@ EXPORTS = (
     "pcf_create", "pcf_destroy", "pcf_set_stream", "pcf_synchronize",
     "pcf_set_field", "pcf_get_field", "pcf_set_history", "pcf_get_history",
     "pcf_step", "pcf_step_steady", "pcf_step_steady_graph", "pcf_diagnostics",
-    "pcf_kernels_per_step",
+    "pcf_kernels_per_step", "pcf_stage_count", "pcf_stage_name", "pcf_profile_steady",
 )
 
 
@@ -106,6 +106,9 @@ def load() -> C.CDLL:
         "pcf_step_steady_graph": ([h, C.c_int32, C.c_int32], C.c_int32),
         "pcf_diagnostics": ([h, dp], C.c_int32),
         "pcf_kernels_per_step": ([h], C.c_int32),
+        "pcf_stage_count": ([], C.c_int32),
+        "pcf_stage_name": ([C.c_int32], C.c_char_p),
+        "pcf_profile_steady": ([h, C.c_int32, dp], C.c_int32),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)

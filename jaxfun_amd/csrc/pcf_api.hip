@@ -71,6 +71,10 @@ double dfactor(double L) {
   return std::fabs(L - tp) > 1e-12 ? tp / L : 1.0;
 }
 
+struct StageTimer {
+  hipEvent_t ev[6][2];
+};
+
 template <typename T>
 struct DevMats {
   T *VBt = nullptr, *VDt = nullptr, *VD1t = nullptr, *A1t = nullptr, *A0t = nullptr,
@@ -105,6 +109,7 @@ struct pcf_handle {
   double *dVB = nullptr, *dVD = nullptr, *dVB1 = nullptr, *dVD1 = nullptr, *dfejer = nullptr,
          *dxn = nullptr, *ddiag = nullptr;
   std::map<int, hipGraphExec_t> graphs;
+  StageTimer *timer = nullptr;  // non-null while pcf_profile_steady runs
 
   template <typename T>
   int32_t alloc(T **p, size_t count) {
@@ -208,24 +213,47 @@ int32_t build_solve_mats(pcf_handle *h) {
   return PCF_OK;
 }
 
+// Stage names, in launch order, for per-kernel timing.
+const char *const STAGE_NAMES[6] = {"nodal_backward", "fft_y_inverse", "rotational",
+                                    "fft_y_forward", "forward_rhs", "solve"};
+
+// Record a timing event at a stage edge; accumulate the previous step's
+// interval when a stage starts again.
+inline hipError_t mark(pcf_handle *h, int stage, int edge) {
+  StageTimer *t = h->timer;
+  if (!t) return hipSuccess;
+  return hipEventRecord(t->ev[stage][edge], h->stream);
+}
+
+template <typename T>
+hipError_t nonlinear_stages(pcf_handle *h, const pcf::NodalMats<T> &nm, const double2 *u0, int slot) {
+  const pcf::Geo &g = h->geo;
+  hipError_t e;
+  if ((e = mark(h, 0, 0))) return e;
+  if ((e = pcf::launch_nodal<T>(g, nm, u0, h->u1, h->u2, h->ws, h->stream))) return e;
+  if ((e = mark(h, 0, 1))) return e;
+  if ((e = mark(h, 1, 0))) return e;
+  if ((e = pcf::launch_fft_y<T>(g, nm, h->ws, h->py, true, h->stream))) return e;
+  if ((e = mark(h, 1, 1))) return e;
+  if ((e = mark(h, 2, 0))) return e;
+  if ((e = pcf::launch_rotational<T>(g, nm, h->ws, h->pz, h->stream))) return e;
+  if ((e = mark(h, 2, 1))) return e;
+  if ((e = mark(h, 3, 0))) return e;
+  if ((e = pcf::launch_fft_y<T>(g, nm, h->ws, h->py, false, h->stream))) return e;
+  if ((e = mark(h, 3, 1))) return e;
+  if ((e = mark(h, 4, 0))) return e;
+  if ((e = pcf::launch_rhs<T>(g, nm, h->ws, h->Nu[slot], h->Ng[slot], h->Nv[slot], h->Nw[slot],
+                              h->stream)))
+    return e;
+  return mark(h, 4, 1);
+}
+
 // Nonlinear rows of the state whose wall-normal velocity is u0 (with the
 // handle's u1/u2) into nonlinear slot `slot`.
 int32_t nonlinear_into(pcf_handle *h, const double2 *u0, int slot) {
-  const pcf::Geo &g = h->geo;
-  hipError_t e;
-  if (h->cfg.precision == PCF_PREC_FP64) {
-    e = pcf::launch_nodal_backward<double>(g, h->md.view, u0, h->u1, h->u2, h->ws, h->py, h->stream);
-    if (e == hipSuccess) e = pcf::launch_rotational<double>(g, h->md.view, h->ws, h->pz, h->stream);
-    if (e == hipSuccess)
-      e = pcf::launch_forward_rhs<double>(g, h->md.view, h->ws, h->py, h->Nu[slot], h->Ng[slot],
-                                          h->Nv[slot], h->Nw[slot], h->stream);
-  } else {
-    e = pcf::launch_nodal_backward<float>(g, h->mf.view, u0, h->u1, h->u2, h->ws, h->py, h->stream);
-    if (e == hipSuccess) e = pcf::launch_rotational<float>(g, h->mf.view, h->ws, h->pz, h->stream);
-    if (e == hipSuccess)
-      e = pcf::launch_forward_rhs<float>(g, h->mf.view, h->ws, h->py, h->Nu[slot], h->Ng[slot],
-                                         h->Nv[slot], h->Nw[slot], h->stream);
-  }
+  hipError_t e = (h->cfg.precision == PCF_PREC_FP64)
+                     ? nonlinear_stages<double>(h, h->md.view, u0, slot)
+                     : nonlinear_stages<float>(h, h->mf.view, u0, slot);
   if (e != hipSuccess) return fail(PCF_E_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
   return PCF_OK;
 }
@@ -286,7 +314,9 @@ int32_t steady_step(pcf_handle *h) {
   set_n(a, 2, h, h->no[2], dt * 6.0 / 11.0);
   a.nn = 3;
   a.gamma = dt * 6.0 / 11.0;
-  hipError_t e = pcf::launch_solve(h->geo, h->sm, a, h->stream);
+  hipError_t e = mark(h, 5, 0);
+  if (e == hipSuccess) e = pcf::launch_solve(h->geo, h->sm, a, h->stream);
+  if (e == hipSuccess) e = mark(h, 5, 1);
   if (e != hipSuccess) return fail(PCF_E_HIP, std::string("solve launch: ") + hipGetErrorString(e));
   rotate(h);
   h->history_steps = 2.0;
@@ -705,9 +735,48 @@ int32_t pcf_diagnostics(pcf_handle *h, double *out) {
   return PCF_OK;
 }
 
+int32_t pcf_stage_count(void) { return 6; }
+
+const char *pcf_stage_name(int32_t i) { return (i >= 0 && i < 6) ? STAGE_NAMES[i] : ""; }
+
+int32_t pcf_profile_steady(pcf_handle *h, int32_t nsteps, double *stage_ms) {
+  if (!h || !stage_ms) return fail(PCF_E_INVALID, "null argument");
+  if (h->history_steps < 2.0) return fail(PCF_E_INVALID, "history not bootstrapped (history_steps < 2)");
+  if (nsteps <= 0) return fail(PCF_E_INVALID, "nsteps must be positive");
+  PCF_HIP(hipSetDevice(h->device));
+  StageTimer t{};
+  for (int i = 0; i < 6; ++i)
+    for (int j = 0; j < 2; ++j) PCF_HIP(hipEventCreate(&t.ev[i][j]));
+  double acc[6] = {0, 0, 0, 0, 0, 0};
+  int32_t r = PCF_OK;
+  h->timer = &t;
+  for (int s = 0; s < nsteps && !r; ++s) {
+    r = steady_step(h);
+    if (r) break;
+    if (hipEventSynchronize(t.ev[5][1]) != hipSuccess) {
+      r = fail(PCF_E_HIP, "event synchronize failed");
+      break;
+    }
+    for (int i = 0; i < 6; ++i) {
+      float ms = 0.f;
+      if (hipEventElapsedTime(&ms, t.ev[i][0], t.ev[i][1]) != hipSuccess) {
+        r = fail(PCF_E_HIP, "event elapsed time failed");
+        break;
+      }
+      acc[i] += ms;
+    }
+  }
+  h->timer = nullptr;
+  for (int i = 0; i < 6; ++i)
+    for (int j = 0; j < 2; ++j) (void)hipEventDestroy(t.ev[i][j]);
+  if (r) return r;
+  for (int i = 0; i < 6; ++i) stage_ms[i] = acc[i] / nsteps;
+  return PCF_OK;
+}
+
 int32_t pcf_kernels_per_step(pcf_handle *h) {
   (void)h;
-  return pcf::kernels_per_stage_rhs() + 1;
+  return 6;
 }
 
 }  // extern "C"

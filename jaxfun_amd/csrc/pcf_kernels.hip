@@ -755,21 +755,30 @@ int pick_kc(int kzs) {
 }  // namespace
 
 template <typename T>
-hipError_t launch_nodal_backward(const Geo &g, const NodalMats<T> &nm, const double2 *u0,
-                                 const double2 *u1, const double2 *u2, Workspace &ws,
-                                 const FFTPlan &py, hipStream_t s) {
+hipError_t launch_nodal(const Geo &g, const NodalMats<T> &nm, const double2 *u0, const double2 *u1,
+                        const double2 *u2, Workspace &ws, hipStream_t s) {
   using V = typename VT<T>::V;
   dim3 grid((g.m + 255) / 256, (g.n0 + XC_NODAL - 1) / XC_NODAL);
   hipLaunchKernelGGL((k_nodal_backward<T, XC_NODAL>), grid, dim3(256), 0, s, g, nm, u0, u1, u2,
                      reinterpret_cast<V *>(ws.X));
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return e;
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_fft_y(const Geo &g, const NodalMats<T> &nm, Workspace &ws, const FFTPlan &py,
+                        bool inverse, hipStream_t s) {
+  using V = typename VT<T>::V;
   const int kc = pick_kc(g.kzs);
-  dim3 gy(g.kzs / kc, g.n0, 6);
   size_t lds = (size_t)g.my * kc * sizeof(V);
-  hipLaunchKernelGGL((k_fft_y<T, true>), gy, dim3(256), lds, s, g,
-                     reinterpret_cast<const V *>(nm.twy), py, kc,
-                     reinterpret_cast<const V *>(ws.X), reinterpret_cast<V *>(ws.Y));
+  if (inverse) {
+    hipLaunchKernelGGL((k_fft_y<T, true>), dim3(g.kzs / kc, g.n0, 6), dim3(256), lds, s, g,
+                       reinterpret_cast<const V *>(nm.twy), py, kc,
+                       reinterpret_cast<const V *>(ws.X), reinterpret_cast<V *>(ws.Y));
+  } else {
+    hipLaunchKernelGGL((k_fft_y<T, false>), dim3(g.kzs / kc, g.n0, 3), dim3(256), lds, s, g,
+                       reinterpret_cast<const V *>(nm.twy), py, kc,
+                       reinterpret_cast<const V *>(ws.Hz), reinterpret_cast<V *>(ws.Hx));
+  }
   return hipGetLastError();
 }
 
@@ -795,18 +804,9 @@ hipError_t launch_rotational(const Geo &g, const NodalMats<T> &nm, Workspace &ws
 }
 
 template <typename T>
-hipError_t launch_forward_rhs(const Geo &g, const NodalMats<T> &nm, Workspace &ws,
-                              const FFTPlan &py, double2 *Nu, double2 *Ng, double *Nv,
-                              double *Nw, hipStream_t s) {
+hipError_t launch_rhs(const Geo &g, const NodalMats<T> &nm, Workspace &ws, double2 *Nu,
+                      double2 *Ng, double *Nv, double *Nw, hipStream_t s) {
   using V = typename VT<T>::V;
-  const int kc = pick_kc(g.kzs);
-  dim3 gy(g.kzs / kc, g.n0, 3);
-  size_t lds = (size_t)g.my * kc * sizeof(V);
-  hipLaunchKernelGGL((k_fft_y<T, false>), gy, dim3(256), lds, s, g,
-                     reinterpret_cast<const V *>(nm.twy), py, kc,
-                     reinterpret_cast<const V *>(ws.Hz), reinterpret_cast<V *>(ws.Hx));
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return e;
   dim3 grid((g.m + 255) / 256, (g.nd + IC_RHS - 1) / IC_RHS);
   hipLaunchKernelGGL((k_forward_rhs<T, IC_RHS>), grid, dim3(256), 0, s, g, nm,
                      reinterpret_cast<const V *>(ws.Hx), Nu, Ng, Nv, Nw);
@@ -833,7 +833,6 @@ hipError_t launch_diag(const Geo &g, const double *VB, const double *VD, const d
   return hipGetLastError();
 }
 
-int kernels_per_stage_rhs() { return 5; }
 
 bool fft_fits(const Geo &g, const FFTPlan &py, const FFTPlan &pz, int elem_bytes) {
   const int kc = pick_kc(g.kzs);
@@ -851,23 +850,17 @@ bool fft_fits(const Geo &g, const FFTPlan &py, const FFTPlan &pz, int elem_bytes
   return true;
 }
 
-template hipError_t launch_nodal_backward<double>(const Geo &, const NodalMats<double> &,
-                                                  const double2 *, const double2 *,
-                                                  const double2 *, Workspace &, const FFTPlan &,
-                                                  hipStream_t);
-template hipError_t launch_nodal_backward<float>(const Geo &, const NodalMats<float> &,
-                                                 const double2 *, const double2 *,
-                                                 const double2 *, Workspace &, const FFTPlan &,
-                                                 hipStream_t);
-template hipError_t launch_rotational<double>(const Geo &, const NodalMats<double> &, Workspace &,
-                                              const FFTPlan &, hipStream_t);
-template hipError_t launch_rotational<float>(const Geo &, const NodalMats<float> &, Workspace &,
-                                             const FFTPlan &, hipStream_t);
-template hipError_t launch_forward_rhs<double>(const Geo &, const NodalMats<double> &,
-                                               Workspace &, const FFTPlan &, double2 *,
-                                               double2 *, double *, double *, hipStream_t);
-template hipError_t launch_forward_rhs<float>(const Geo &, const NodalMats<float> &, Workspace &,
-                                              const FFTPlan &, double2 *, double2 *, double *,
-                                              double *, hipStream_t);
+#define PCF_INSTANTIATE(T)                                                                     \
+  template hipError_t launch_nodal<T>(const Geo &, const NodalMats<T> &, const double2 *,        \
+                                      const double2 *, const double2 *, Workspace &, hipStream_t); \
+  template hipError_t launch_fft_y<T>(const Geo &, const NodalMats<T> &, Workspace &,            \
+                                      const FFTPlan &, bool, hipStream_t);                        \
+  template hipError_t launch_rotational<T>(const Geo &, const NodalMats<T> &, Workspace &,       \
+                                           const FFTPlan &, hipStream_t);                        \
+  template hipError_t launch_rhs<T>(const Geo &, const NodalMats<T> &, Workspace &, double2 *,   \
+                                    double2 *, double *, double *, hipStream_t);
+PCF_INSTANTIATE(double)
+PCF_INSTANTIATE(float)
+#undef PCF_INSTANTIATE
 
 }  // namespace pcf

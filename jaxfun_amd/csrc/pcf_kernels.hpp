@@ -93,18 +93,19 @@ struct Workspace {
   void *Hx;  // [3][n0][m]
 };
 
-// Launchers (return hipError_t of the launch).
+// Launchers (return hipError_t of the launch).  One kernel each, in step order.
 template <typename T>
-hipError_t launch_nodal_backward(const Geo &g, const NodalMats<T> &nm, const double2 *u0,
-                                 const double2 *u1, const double2 *u2, Workspace &ws,
-                                 const FFTPlan &py, hipStream_t s);
+hipError_t launch_nodal(const Geo &g, const NodalMats<T> &nm, const double2 *u0, const double2 *u1,
+                        const double2 *u2, Workspace &ws, hipStream_t s);
+template <typename T>
+hipError_t launch_fft_y(const Geo &g, const NodalMats<T> &nm, Workspace &ws, const FFTPlan &py,
+                        bool inverse, hipStream_t s);
 template <typename T>
 hipError_t launch_rotational(const Geo &g, const NodalMats<T> &nm, Workspace &ws,
                              const FFTPlan &pz, hipStream_t s);
 template <typename T>
-hipError_t launch_forward_rhs(const Geo &g, const NodalMats<T> &nm, Workspace &ws,
-                              const FFTPlan &py, double2 *Nu, double2 *Ng, double *Nv,
-                              double *Nw, hipStream_t s);
+hipError_t launch_rhs(const Geo &g, const NodalMats<T> &nm, Workspace &ws, double2 *Nu,
+                      double2 *Ng, double *Nv, double *Nw, hipStream_t s);
 hipError_t launch_solve(const Geo &g, const SolveMats &sm, const SolveArgs &a, hipStream_t s);
 
 // Diagnostics partial sums: out[0] = sum wF |u|^2 (perturbation), out[1] = sum
@@ -115,7 +116,6 @@ hipError_t launch_diag(const Geo &g, const double *VB, const double *VD, const d
                        const double2 *u0, const double2 *u1, const double2 *u2, double *out,
                        hipStream_t s);
 
-int kernels_per_stage_rhs();
 size_t solve_lds_bytes(const Geo &g);
 // Largest butterfly count per thread any FFT launch needs (host-side check
 // against the kernels' register budgets).

@@ -22,12 +22,13 @@ pytestmark = pytest.mark.gpu
 GOLDEN = json.loads((Path(__file__).parent / "golden" / "reference_sbdf3_published.json").read_text())
 
 FP64_TOL = 1e-12
-# Nonlinear history rows (Nu, Ng, Nv00, Nw00) are wall-normal derivatives of a
-# projected product: d/dx amplifies rounding of the high Chebyshev modes by
-# O(n0^2), so two FP64 evaluation orders of the same rows differ by ~1e-12
-# relative (tests/test_oracle.py::test_nonlinear_rows_conditioning measures the
-# spread inside the oracle itself).  They are held to 5e-12.
-HIST_TOL = 5e-12
+# Nonlinear history rows (Nu, Ng, Nv00, Nw00) are held to HIST_TOL instead:
+# Nu is a wall-normal derivative of the dealiased product, and inside the
+# oracle alone a 1e-16 relative perturbation of the state moves Nu by ~2e-12
+# (33^3) to ~1.7e-11 (65^3) relative (tests/test_oracle.py::
+# test_nonlinear_rows_rounding_sensitivity), so two FP64 evaluation orders
+# cannot agree more closely than that.  The state itself agrees to ~1e-15.
+HIST_TOL = 5e-11
 
 
 def assert_parity(err, where):
@@ -135,3 +136,19 @@ def test_published_golden(case):
     assert cs["leaf_count"] == gc["leaf_count"] and cs["element_count"] == gc["element_count"]
     assert math.isclose(cs["l2_norm"], gc["l2_norm"], rel_tol=1e-11)
     assert math.isclose(cs["real_sum"], gc["real_sum"], rel_tol=1e-9)
+
+
+def test_mixed_precision_close_to_fp64():
+    """FP32 transforms/nonlinear term, FP64 solves: velocity stays within a
+    stated tolerance of the FP64 step (2e-6 relative after 50 steps here)."""
+    N = (33, 32, 32)
+    a = make(N, Re=1000.0, dt=5e-3)
+    b = make(N, Re=1000.0, dt=5e-3, precision="mixed")
+    s0 = seeded_state(a, seed=0, energy=1e-3, K=8)
+    sa = a.solve(s0, 50)
+    sb = b.solve(s0, 50)
+    for x, y in zip(sb.u, sa.u):
+        rel = float(np.max(np.abs(x - y)) / np.max(np.abs(y)))
+        print("mixed vs fp64", rel)
+        assert rel < 2e-6
+    assert float(np.max(np.abs(sb.u[0] - sa.u[0]))) > 1e-14  # really a different path
