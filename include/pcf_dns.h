@@ -151,7 +151,8 @@ int32_t pcf_diagnostics(pcf_handle *h, double *out8);
 
 /* Per-stage timing.  One steady step launches pcf_kernels_per_step() kernels,
  * one per stage, in the order pcf_stage_name(0..pcf_stage_count()-1):
- * nodal_backward, fft_y_inverse, rotational, fft_y_forward, forward_rhs, solve.
+ * nodal_backward, fft_y_inverse, rotational, fft_y_forward, forward_rhs, solve,
+ * reconstruct.
  * pcf_profile_steady runs nsteps steady steps eagerly with HIP events around
  * every kernel on the handle's stream and writes the mean duration of each
  * stage (milliseconds) to stage_ms[pcf_stage_count()].  No reference

@@ -72,7 +72,7 @@ double dfactor(double L) {
 }
 
 struct StageTimer {
-  hipEvent_t ev[6][2];
+  hipEvent_t ev[7][2];
 };
 
 template <typename T>
@@ -105,7 +105,10 @@ struct pcf_handle {
   pcf::Workspace ws{};
   DevMats<double> md;
   DevMats<float> mf;
-  pcf::SolveMats sm{};
+  pcf::StructMats sm{};
+  double *Rt = nullptr;       // parity-compressed reconstruction rows, [nd][ldr]
+  int ldr = 0;
+  double *scratch = nullptr;  // per-row data of the structured solves
   double *dVB = nullptr, *dVD = nullptr, *dVB1 = nullptr, *dVD1 = nullptr, *dfejer = nullptr,
          *dxn = nullptr, *ddiag = nullptr;
   std::map<int, hipGraphExec_t> graphs;
@@ -172,50 +175,45 @@ int32_t build_nodal(pcf_handle *h, DevMats<T> &dm, int xc, int ic) {
   return PCF_OK;
 }
 
-int32_t build_solve_mats(pcf_handle *h) {
+int32_t build_struct_mats(pcf_handle *h) {
   const pcf::Operators &o = h->ops;
-  const pcf::Mat *blocks[2][3] = {{&o.C0B, &o.C2B, &o.C4B}, {&o.C0D, &o.C2D, nullptr}};
   int32_t r;
-  for (int s = 0; s < 2; ++s)
+  int npmax = 0;
+  for (int b = 0; b < 2; ++b)
     for (int p = 0; p < 2; ++p) {
-      const pcf::Mat &ref = *blocks[s][0];
-      const int n = (ref.rows - p + 1) / 2;
-      h->sm.np[s][p] = n;
-      for (int t = 0; t < 3; ++t) {
-        if (!blocks[s][t]) {
-          h->sm.row[s][p][t] = nullptr;
-          h->sm.col[s][p][t] = nullptr;
-          continue;
-        }
-        const pcf::Mat &M = *blocks[s][t];
-        std::vector<double> rm((size_t)n * n), cm((size_t)n * n);
-        for (int i = 0; i < n; ++i)
-          for (int j = 0; j < n; ++j) {
-            double v = M(2 * i + p, 2 * j + p);
-            rm[(size_t)i * n + j] = v;
-            cm[(size_t)j * n + i] = v;
-          }
-        double *dr = nullptr, *dc = nullptr;
-        if ((r = h->upload(&dr, rm))) return r;
-        if ((r = h->upload(&dc, cm))) return r;
-        h->sm.row[s][p][t] = dr;
-        h->sm.col[s][p][t] = dc;
-      }
+      const pcf::Operators::Struct &st = o.st[b][p];
+      pcf::StructSys &ss = h->sm.sys[b][p];
+      ss.n = st.n;
+      npmax = std::max(npmax, st.n);
+      double *d = nullptr;
+      if ((r = h->upload(&d, st.b0))) return r;
+      ss.b0 = d;
+      if ((r = h->upload(&d, st.b2))) return r;
+      ss.b2 = d;
+      if ((r = h->upload(&d, st.b4))) return r;
+      ss.b4 = d;
+      if ((r = h->upload(&d, st.U))) return r;
+      ss.U = d;
+      if ((r = h->upload(&d, st.V))) return r;
+      ss.V = d;
     }
-  std::vector<double> Rt((size_t)o.nb * o.nd);
-  for (int i = 0; i < o.nd; ++i)
-    for (int j = 0; j < o.nb; ++j) Rt[(size_t)j * o.nd + i] = o.R(i, j);
-  double *dR = nullptr;
-  if ((r = h->upload(&dR, Rt))) return r;
-  h->sm.Rt = dR;
-  h->geo.npmax = std::max(std::max(h->sm.np[0][0], h->sm.np[0][1]),
-                          std::max(h->sm.np[1][0], h->sm.np[1][1]));
+  h->geo.npmax = npmax;
+  // parity-compressed reconstruction rows: Rc[i][k] = R[i][2k + 1 - (i & 1)]
+  h->ldr = (o.nb + 2) / 2;
+  std::vector<double> Rc((size_t)o.nd * h->ldr, 0.0);
+  for (int i = 0; i < o.nd; ++i) {
+    const int j0 = 1 - (i & 1);
+    for (int k = 0; j0 + 2 * k < o.nb; ++k) Rc[(size_t)i * h->ldr + k] = o.R(i, j0 + 2 * k);
+  }
+  if ((r = h->upload(&h->Rt, Rc))) return r;
+  if ((r = h->alloc(&h->scratch, pcf::solve_scratch_doubles(h->geo)))) return r;
   return PCF_OK;
 }
 
 // Stage names, in launch order, for per-kernel timing.
-const char *const STAGE_NAMES[6] = {"nodal_backward", "fft_y_inverse", "rotational",
-                                    "fft_y_forward", "forward_rhs", "solve"};
+constexpr int NSTAGE = 7;
+const char *const STAGE_NAMES[NSTAGE] = {"nodal_backward", "fft_y_inverse", "rotational",
+                                         "fft_y_forward", "forward_rhs", "solve", "reconstruct"};
 
 // Record a timing event at a stage edge; accumulate the previous step's
 // interval when a stage starts again.
@@ -256,6 +254,17 @@ int32_t nonlinear_into(pcf_handle *h, const double2 *u0, int slot) {
                      : nonlinear_stages<float>(h, h->mf.view, u0, slot);
   if (e != hipSuccess) return fail(PCF_E_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
   return PCF_OK;
+}
+
+// Structured solve + reconstruction of one implicit update.
+hipError_t implicit_update(pcf_handle *h, const pcf::SolveArgs &a, bool steady) {
+  hipError_t e;
+  if ((e = mark(h, 5, 0))) return e;
+  if ((e = pcf::launch_solve(h->geo, h->sm, a, h->scratch, steady, h->stream))) return e;
+  if ((e = mark(h, 5, 1))) return e;
+  if ((e = mark(h, 6, 0))) return e;
+  if ((e = pcf::launch_reconstruct(h->geo, h->Rt, h->ldr, a, h->stream))) return e;
+  return mark(h, 6, 1);
 }
 
 pcf::SolveArgs base_args(pcf_handle *h, int out_slot) {
@@ -314,9 +323,7 @@ int32_t steady_step(pcf_handle *h) {
   set_n(a, 2, h, h->no[2], dt * 6.0 / 11.0);
   a.nn = 3;
   a.gamma = dt * 6.0 / 11.0;
-  hipError_t e = mark(h, 5, 0);
-  if (e == hipSuccess) e = pcf::launch_solve(h->geo, h->sm, a, h->stream);
-  if (e == hipSuccess) e = mark(h, 5, 1);
+  hipError_t e = implicit_update(h, a, true);
   if (e != hipSuccess) return fail(PCF_E_HIP, std::string("solve launch: ") + hipGetErrorString(e));
   rotate(h);
   h->history_steps = 2.0;
@@ -352,7 +359,7 @@ int32_t startup_step(pcf_handle *h) {
       a.nn = 2;
     }
     a.gamma = gamma;
-    hipError_t e = pcf::launch_solve(h->geo, h->sm, a, h->stream);
+    hipError_t e = implicit_update(h, a, false);
     if (e != hipSuccess) return fail(PCF_E_HIP, std::string("solve launch: ") + hipGetErrorString(e));
   }
   rotate(h);
@@ -525,13 +532,7 @@ int32_t pcf_create(const pcf_config *cfg, pcf_handle **out) {
   } else {
     if ((r = build_nodal<float>(h, h->mf, 8, 8))) return bail(r);
   }
-  if ((r = build_solve_mats(h))) return bail(r);
-  int max_lds = 0;
-  if (hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, cfg->device) !=
-      hipSuccess)
-    max_lds = 64 * 1024;
-  if (pcf::solve_lds_bytes(g) > (size_t)max_lds)
-    return bail(fail(PCF_E_UNSUPPORTED, "per-mode solve exceeds the device's LDS at this n0"));
+  if ((r = build_struct_mats(h))) return bail(r);
   if ((r = h->upload(&h->dVB, h->ops.VB.a))) return bail(r);
   if ((r = h->upload(&h->dVD, h->ops.VD.a))) return bail(r);
   if ((r = h->upload(&h->dVB1, h->ops.VB1.a))) return bail(r);
@@ -735,9 +736,9 @@ int32_t pcf_diagnostics(pcf_handle *h, double *out) {
   return PCF_OK;
 }
 
-int32_t pcf_stage_count(void) { return 6; }
+int32_t pcf_stage_count(void) { return NSTAGE; }
 
-const char *pcf_stage_name(int32_t i) { return (i >= 0 && i < 6) ? STAGE_NAMES[i] : ""; }
+const char *pcf_stage_name(int32_t i) { return (i >= 0 && i < NSTAGE) ? STAGE_NAMES[i] : ""; }
 
 int32_t pcf_profile_steady(pcf_handle *h, int32_t nsteps, double *stage_ms) {
   if (!h || !stage_ms) return fail(PCF_E_INVALID, "null argument");
@@ -745,19 +746,19 @@ int32_t pcf_profile_steady(pcf_handle *h, int32_t nsteps, double *stage_ms) {
   if (nsteps <= 0) return fail(PCF_E_INVALID, "nsteps must be positive");
   PCF_HIP(hipSetDevice(h->device));
   StageTimer t{};
-  for (int i = 0; i < 6; ++i)
+  for (int i = 0; i < NSTAGE; ++i)
     for (int j = 0; j < 2; ++j) PCF_HIP(hipEventCreate(&t.ev[i][j]));
-  double acc[6] = {0, 0, 0, 0, 0, 0};
+  double acc[NSTAGE] = {0, 0, 0, 0, 0, 0, 0};
   int32_t r = PCF_OK;
   h->timer = &t;
   for (int s = 0; s < nsteps && !r; ++s) {
     r = steady_step(h);
     if (r) break;
-    if (hipEventSynchronize(t.ev[5][1]) != hipSuccess) {
+    if (hipEventSynchronize(t.ev[NSTAGE - 1][1]) != hipSuccess) {
       r = fail(PCF_E_HIP, "event synchronize failed");
       break;
     }
-    for (int i = 0; i < 6; ++i) {
+    for (int i = 0; i < NSTAGE; ++i) {
       float ms = 0.f;
       if (hipEventElapsedTime(&ms, t.ev[i][0], t.ev[i][1]) != hipSuccess) {
         r = fail(PCF_E_HIP, "event elapsed time failed");
@@ -767,16 +768,16 @@ int32_t pcf_profile_steady(pcf_handle *h, int32_t nsteps, double *stage_ms) {
     }
   }
   h->timer = nullptr;
-  for (int i = 0; i < 6; ++i)
+  for (int i = 0; i < NSTAGE; ++i)
     for (int j = 0; j < 2; ++j) (void)hipEventDestroy(t.ev[i][j]);
   if (r) return r;
-  for (int i = 0; i < 6; ++i) stage_ms[i] = acc[i] / nsteps;
+  for (int i = 0; i < NSTAGE; ++i) stage_ms[i] = acc[i] / nsteps;
   return PCF_OK;
 }
 
 int32_t pcf_kernels_per_step(pcf_handle *h) {
   (void)h;
-  return 6;
+  return NSTAGE;
 }
 
 }  // extern "C"

@@ -18,6 +18,8 @@
 //                      biharmonic solves, mean-mode solve, velocity
 //                      reconstruction (sbdf3.py:44-62, tpmatrix.py:686-799,
 //                      channelflow_kmm.py:855-870)
+#include <algorithm>
+
 #include "pcf_kernels.hpp"
 
 namespace pcf {
@@ -231,7 +233,7 @@ __global__ void __launch_bounds__(256) k_nodal_backward(Geo g, NodalMats<T> nm,
 // One block per (kz chunk, node x, field): the streamwise lines of KC spanwise
 // modes.  INV: spectral [iy][kz] -> zero-padded [y][kz] lines, inverse FFT.
 // !INV: [y][kz] lines -> forward FFT -> truncated [iy][kz], Nyquist row zeroed.
-template <typename T, bool INV>
+template <typename T, bool INV, int MAXB>
 __global__ void __launch_bounds__(256) k_fft_y(Geo g, const typename VT<T>::V *__restrict__ tw,
                                                FFTPlan plan, int KC,
                                                const typename VT<T>::V *__restrict__ src,
@@ -252,7 +254,7 @@ __global__ void __launch_bounds__(256) k_fft_y(Geo g, const typename VT<T>::V *_
       buf[y * KC + c] = s[(size_t)iy * KZ + kc0 + c];
     }
     __syncthreads();
-    fft_lds<V, T, true, true, 8>(buf, tw, plan, KC, 1, KC);
+    fft_lds<V, T, true, true, MAXB>(buf, tw, plan, KC, 1, KC);
     V *d = dst + ((size_t)f * g.n0 + x) * (size_t)MY * KZ;
     for (int idx = tid; idx < MY * KC; idx += nthr) {
       int y = idx / KC, c = idx - y * KC;
@@ -265,7 +267,7 @@ __global__ void __launch_bounds__(256) k_fft_y(Geo g, const typename VT<T>::V *_
       buf[idx] = s[(size_t)y * KZ + kc0 + c];
     }
     __syncthreads();
-    fft_lds<V, T, false, true, 8>(buf, tw, plan, KC, 1, KC);
+    fft_lds<V, T, false, true, MAXB>(buf, tw, plan, KC, 1, KC);
     V *d = dst + ((size_t)f * g.n0 + x) * g.m;
     for (int idx = tid; idx < NY * KC; idx += nthr) {
       int iy = idx / KC, c = idx - iy * KC;
@@ -281,7 +283,7 @@ __global__ void __launch_bounds__(256) k_fft_y(Geo g, const typename VT<T>::V *_
 // nodal fields.  The two real lines of a field ride in one complex FFT
 // (line A as the real part, line B as the imaginary part), both for the
 // c2r synthesis and the r2c analysis.
-template <typename T>
+template <typename T, int MAXB>
 __global__ void __launch_bounds__(256) k_rotational(Geo g, NodalMats<T> nm, FFTPlan plan, int UPB,
                                                     const typename VT<T>::V *__restrict__ Y,
                                                     typename VT<T>::V *__restrict__ Hz) {
@@ -318,7 +320,7 @@ __global__ void __launch_bounds__(256) k_rotational(Geo g, NodalMats<T> nm, FFTP
     buf[idx] = z;
   }
   __syncthreads();
-  fft_lds<V, T, true, false, 12>(buf, tw, plan, UPB * 6, MZ, 1);
+  fft_lds<V, T, true, false, MAXB>(buf, tw, plan, UPB * 6, MZ, 1);
 
   const T U = (T)g.uwall;
   for (int idx = tid; idx < UPB * MZ; idx += nthr) {
@@ -347,7 +349,7 @@ __global__ void __launch_bounds__(256) k_rotational(Geo g, NodalMats<T> nm, FFTP
     o[2 * MZ] = h2;
   }
   __syncthreads();
-  fft_lds<V, T, false, false, 12>(out, tw, plan, UPB * 3, MZ, 1);
+  fft_lds<V, T, false, false, MAXB>(out, tw, plan, UPB * 3, MZ, 1);
 
   const T scale = (T)(1.0 / ((double)MY * (double)MZ));
   for (int idx = tid; idx < UPB * 3 * KZ; idx += nthr) {
@@ -428,254 +430,591 @@ __global__ void __launch_bounds__(256) k_forward_rhs(Geo g, NodalMats<T> nm,
 }
 
 // ------------------------------------------------------------ k_solve ----
-__device__ __forceinline__ double readlane_d(double v, int lane) {
-  unsigned long long b = __double_as_longlong(v);
-  int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffull), lane);
-  int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
-  return __longlong_as_double(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
-}
+// Per-mode implicit update with O(n) structured solves.
+//
+// Parity-split (rows of one parity of the Shen index), each per-mode matrix
+//   A = s0 C0 + s2 C2 + s4 C4
+// is banded (lower bandwidth P, near-upper width Q) plus a far-upper part of
+// rank R for columns j >= i + Q + 1:
+//   A[i][j] = sf * sum_r U_r(i) V_r(j).
+// From the Chebyshev closed forms (T_a, T_b'') = (pi/2) b (b^2 - a^2) and the
+// fourth-derivative analogue, with the boundary sums of the Shen stencils
+// (sum S = sum S a^2 = 0 for the clamped basis, sum S = 0 for the Dirichlet
+// basis): the clamped system has P = Q = 2 and R = 2 (from C4 only), the
+// Dirichlet system P = Q = 1 and R = 1 (C2_D[i][j] = -4 pi (I+1) beyond the
+// band).  One thread solves one (mode, parity) system in three sweeps:
+//   A: bottom-up right-hand side rhs = C0 delta + C2 alpha + C4 beta + N, the
+//      far part through running suffix sums;
+//   B: top-down LU without pivoting (the reference's no-pivot banded LU,
+//      diamatrix.py:1937-1973), tracking each row's far coefficients;
+//   C: bottom-up back substitution with running suffix sums.
+// Row data between sweeps lives in a global scratch array laid out
+// [system][row][value][mode], so consecutive lanes (consecutive modes) touch
+// consecutive addresses.
+template <int P, int Q, int R, bool HAS4>
+__device__ void structured_solve(const StructSys &ss, int n, int par, int m, int M, double s0,
+                                 double s2, double s4, double sf, double a0z, double a0l,
+                                 double a2z, double a2l, double a4l, const double2 *const *xl,
+                                 const double *cl, const double *dl, int nx,
+                                 const double2 *const *nl, const double *el, int nn,
+                                 const double *vx[3], const double *wx[3],
+                                 const double *vn[3], const double *wn[3], bool mean,
+                                 double *scr, double2 *xout, double *vout, double *wout) {
+  constexpr int F = Q + 1;
+  constexpr int BW = P + Q + 1;
+  constexpr int NV = 1 + Q + R + 2;  // dinv, e[1..Q], w[0..R-1], y (re, im)
+  const size_t rowstride = (size_t)NV * M;
+  auto scr_at = [&](int i, int v) -> double & { return scr[(size_t)i * rowstride + (size_t)v * M + m]; };
 
-__device__ __forceinline__ void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-// One parity block of one per-mode system, solved by one wavefront.
-//   A = s0 C0 + s2 C2 + s4 C4                 (n x n, lower bandwidth pbw)
-//   rhs_i = sum_j C0_ij delta_J + C2_ij alpha_J + C4_ij beta_J + N_I
-// (J = 2 j + par is the full Shen index).  LU without pivoting, as the
-// reference's banded kernel (diamatrix.py:1937-1973): rows are reduced in
-// order with lanes holding columns; the L multipliers are wave-uniform so the
-// forward substitution is a scalar recurrence; the back substitution runs
-// column by column with lanes holding rows.
-__device__ void wave_solve(int n, int par, int pbw, const double *r0, const double *r2,
-                           const double *r4, const double *c0, const double *c2,
-                           const double *c4, double s0, double s2, double s4,
-                           const double2 *delta, const double2 *alpha, const double2 *beta,
-                           const double2 *Nc, double *U, int ldu, double2 *y, double *ipiv,
-                           double2 *xout) {
-  const int lane = threadIdx.x & 63;
-  // right-hand side, lanes = rows
-  if (lane < n) {
-    double2 r = Nc[2 * lane + par];
-    for (int j = 0; j < n; ++j) {
-      const int J = 2 * j + par;
-      const double a0 = c0[(size_t)j * n + lane], a2 = c2[(size_t)j * n + lane];
-      double2 dl = delta[J], al = alpha[J];
-      r.x += a0 * dl.x + a2 * al.x;
-      r.y += a0 * dl.y + a2 * al.y;
-      if (c4) {
-        const double a4 = c4[(size_t)j * n + lane];
-        double2 be = beta[J];
-        r.x += a4 * be.x;
-        r.y += a4 * be.y;
+  // z, zL and the nonlinear combination of parity row j (full index 2j+par)
+  auto load_row = [&](int j, double2 &z, double2 &zl, double2 &nc) {
+    z = double2{0, 0};
+    zl = double2{0, 0};
+    nc = double2{0, 0};
+    if (j < 0 || j >= n) return;
+    const int J = 2 * j + par;
+    if (!mean) {
+      const size_t o = (size_t)J * M + m;
+      for (int l = 0; l < nx; ++l) {
+        double2 v = xl[l][o];
+        z.x += cl[l] * v.x;
+        z.y += cl[l] * v.y;
+        zl.x += dl[l] * v.x;
+        zl.y += dl[l] * v.y;
+      }
+      for (int l = 0; l < nn; ++l) {
+        double2 v = nl[l][o];
+        nc.x += el[l] * v.x;
+        nc.y += el[l] * v.y;
+      }
+    } else {
+      for (int l = 0; l < nx; ++l) {
+        z.x += cl[l] * vx[l][J];
+        z.y += cl[l] * wx[l][J];
+        zl.x += dl[l] * vx[l][J];
+        zl.y += dl[l] * wx[l][J];
+      }
+      for (int l = 0; l < nn; ++l) {
+        nc.x += el[l] * vn[l][J];
+        nc.y += el[l] * wn[l][J];
       }
     }
-    y[lane] = r;
+  };
+
+  // ---- sweep A: right-hand side, bottom-up
+  {
+    constexpr int WN = P + F + 1;  // window rows i-P .. i+F
+    double2 dl_[WN], al_[WN], bl_[WN], nc_[WN];
+    auto fill = [&](int k, int j) {
+      double2 z, zl;
+      load_row(j, z, zl, nc_[k]);
+      dl_[k] = double2{a0z * z.x + a0l * zl.x, a0z * z.y + a0l * zl.y};
+      al_[k] = double2{a2z * z.x + a2l * zl.x, a2z * z.y + a2l * zl.y};
+      bl_[k] = double2{a4l * zl.x, a4l * zl.y};
+    };
+#pragma unroll
+    for (int k = 0; k < WN; ++k) fill(k, n - 1 - P + k);
+    double2 sig[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) sig[r] = double2{0, 0};
+    for (int i = n - 1; i >= 0; --i) {
+      if (i + F < n) {
+        const double2 ph = HAS4 ? bl_[P + F] : al_[P + F];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const double v = ss.V[(size_t)(i + F) * R + r];
+          sig[r].x += v * ph.x;
+          sig[r].y += v * ph.y;
+        }
+      }
+      double2 rh = nc_[P];
+      const double *b0 = ss.b0 + (size_t)i * BW, *b2 = ss.b2 + (size_t)i * BW;
+#pragma unroll
+      for (int o = 0; o < BW; ++o) {
+        const double c0 = b0[o], c2 = b2[o];
+        rh.x += c0 * dl_[o].x + c2 * al_[o].x;
+        rh.y += c0 * dl_[o].y + c2 * al_[o].y;
+        if (HAS4) {
+          const double c4 = ss.b4[(size_t)i * BW + o];
+          rh.x += c4 * bl_[o].x;
+          rh.y += c4 * bl_[o].y;
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const double u = ss.U[(size_t)i * R + r];
+        rh.x += u * sig[r].x;
+        rh.y += u * sig[r].y;
+      }
+      scr_at(i, NV - 2) = rh.x;
+      scr_at(i, NV - 1) = rh.y;
+#pragma unroll
+      for (int k = WN - 1; k > 0; --k) {
+        dl_[k] = dl_[k - 1];
+        al_[k] = al_[k - 1];
+        bl_[k] = bl_[k - 1];
+        nc_[k] = nc_[k - 1];
+      }
+      fill(0, i - 1 - P);
+    }
   }
-  wave_sync();
-  // forward elimination, lanes = columns
-  double prev1 = 0.0, prev2 = 0.0, ip1 = 0.0, ip2 = 0.0;
-  double2 y1{0, 0}, y2{0, 0};
-  for (int i = 0; i < n; ++i) {
-    double cur = 0.0;
-    if (lane < n) {
-      const size_t o = (size_t)i * n + lane;
-      cur = s0 * r0[o] + s2 * r2[o];
-      if (r4) cur += s4 * r4[o];
+
+  // ---- sweep B: LU without pivoting, top-down
+  {
+    double pd[P + 1], pe[P + 1][Q + 1], pw[P + 1][R];
+    double2 py[P + 1];
+#pragma unroll
+    for (int t = 0; t <= P; ++t) {
+      pd[t] = 0;
+      py[t] = double2{0, 0};
+#pragma unroll
+      for (int q = 0; q <= Q; ++q) pe[t][q] = 0;
+#pragma unroll
+      for (int r = 0; r < R; ++r) pw[t][r] = 0;
     }
-    double2 yi = y[i];
-    if (pbw >= 2 && i >= 2) {
-      const double l = readlane_d(cur, i - 2) * ip2;
-      cur -= l * prev2;
-      yi.x -= l * y2.x;
-      yi.y -= l * y2.y;
+    for (int i = 0; i < n; ++i) {
+      double a[BW];
+      const double *b0 = ss.b0 + (size_t)i * BW, *b2 = ss.b2 + (size_t)i * BW;
+#pragma unroll
+      for (int o = 0; o < BW; ++o) {
+        double v = s0 * b0[o] + s2 * b2[o];
+        if (HAS4) v += s4 * ss.b4[(size_t)i * BW + o];
+        a[o] = v;
+      }
+      double wi[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) wi[r] = sf * ss.U[(size_t)i * R + r];
+      double2 yi{scr_at(i, NV - 2), scr_at(i, NV - 1)};
+      // eliminate columns i-P .. i-1 with the stored rows t = P .. 1 above
+#pragma unroll
+      for (int t = P; t >= 1; --t) {
+        if (i - t < 0) continue;
+        const double l = a[P - t] * pd[t];  // pd holds 1/diag
+#pragma unroll
+        for (int o = -t + 1; o <= Q; ++o) {
+          if (i + o >= n) continue;
+          const int rel = o + t;  // column offset within row i-t
+          double u;
+          if (rel <= Q) {
+            u = pe[t][rel];
+          } else {
+            u = 0.0;
+#pragma unroll
+            for (int r = 0; r < R; ++r) u += pw[t][r] * ss.V[(size_t)(i + o) * R + r];
+          }
+          a[P + o] -= l * u;
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) wi[r] -= l * pw[t][r];
+        yi.x -= l * py[t].x;
+        yi.y -= l * py[t].y;
+      }
+      const double dinv = 1.0 / a[P];
+      scr_at(i, 0) = dinv;
+#pragma unroll
+      for (int q = 1; q <= Q; ++q) scr_at(i, q) = a[P + q];
+#pragma unroll
+      for (int r = 0; r < R; ++r) scr_at(i, 1 + Q + r) = wi[r];
+      scr_at(i, NV - 2) = yi.x;
+      scr_at(i, NV - 1) = yi.y;
+#pragma unroll
+      for (int t = P; t >= 2; --t) {
+        pd[t] = pd[t - 1];
+        py[t] = py[t - 1];
+#pragma unroll
+        for (int q = 0; q <= Q; ++q) pe[t][q] = pe[t - 1][q];
+#pragma unroll
+        for (int r = 0; r < R; ++r) pw[t][r] = pw[t - 1][r];
+      }
+      pd[1] = dinv;
+      py[1] = yi;
+#pragma unroll
+      for (int q = 1; q <= Q; ++q) pe[1][q] = a[P + q];
+#pragma unroll
+      for (int r = 0; r < R; ++r) pw[1][r] = wi[r];
     }
-    if (i >= 1) {
-      const double l = readlane_d(cur, i - 1) * ip1;
-      cur -= l * prev1;
-      yi.x -= l * y1.x;
-      yi.y -= l * y1.y;
-    }
-    const double ip = 1.0 / readlane_d(cur, i);
-    if (lane < n) U[(size_t)i * ldu + lane] = cur;
-    if (lane == 0) {
-      y[i] = yi;
-      ipiv[i] = ip;
-    }
-    prev2 = prev1;
-    prev1 = cur;
-    ip2 = ip1;
-    ip1 = ip;
-    y2 = y1;
-    y1 = yi;
   }
-  wave_sync();
-  // back substitution, lanes = rows
-  double2 yv = (lane < n) ? y[lane] : double2{0, 0};
-  double2 xv{0, 0};
-  for (int j = n - 1; j >= 0; --j) {
-    const double ipj = ipiv[j];
-    double2 xj{readlane_d(yv.x, j) * ipj, readlane_d(yv.y, j) * ipj};
-    if (lane == j) xv = xj;
-    if (lane < j) {
-      const double u = U[(size_t)lane * ldu + j];
-      yv.x -= u * xj.x;
-      yv.y -= u * xj.y;
+
+  // ---- sweep C: back substitution, bottom-up
+  {
+    double2 xw[F + 1];
+#pragma unroll
+    for (int k = 0; k <= F; ++k) xw[k] = double2{0, 0};
+    double2 tau[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) tau[r] = double2{0, 0};
+    for (int i = n - 1; i >= 0; --i) {
+      if (i + F < n) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const double v = ss.V[(size_t)(i + F) * R + r];
+          tau[r].x += v * xw[F].x;
+          tau[r].y += v * xw[F].y;
+        }
+      }
+      double2 acc{scr_at(i, NV - 2), scr_at(i, NV - 1)};
+#pragma unroll
+      for (int q = 1; q <= Q; ++q) {
+        const double e = scr_at(i, q);
+        acc.x -= e * xw[q].x;
+        acc.y -= e * xw[q].y;
+      }
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const double w = scr_at(i, 1 + Q + r);
+        acc.x -= w * tau[r].x;
+        acc.y -= w * tau[r].y;
+      }
+      const double dinv = scr_at(i, 0);
+      const double2 xi{acc.x * dinv, acc.y * dinv};
+      const int J = 2 * i + par;
+      if (mean) {
+        vout[J] = xi.x;
+        wout[J] = xi.y;
+      } else {
+        xout[(size_t)J * M + m] = xi;
+      }
+#pragma unroll
+      for (int k = F; k >= 2; --k) xw[k] = xw[k - 1];
+      xw[1] = xi;
     }
   }
-  if (lane < n) xout[2 * lane + par] = xv;
 }
 
-// Per-mode implicit update.  One block of four wavefronts per mode:
-// wave 0/1 = clamped (u0) system even/odd rows, wave 2/3 = Dirichlet (g)
-// system even/odd rows.  The mean-mode block additionally solves the 1-D
-// Dirichlet system for (v00, w00) packed as one complex right-hand side.
-__global__ void __launch_bounds__(256) k_solve(Geo g, SolveMats sm, SolveArgs a) {
-  extern __shared__ __align__(16) unsigned char smem[];
-  const int nb = g.nb, nd = g.nd;
-  const int m = blockIdx.x;
-  const int tid = threadIdx.x, wave = tid >> 6;
-  double2 *alpha_u = reinterpret_cast<double2 *>(smem);
-  double2 *beta_u = alpha_u + nb;
-  double2 *delta_u = beta_u + nb;
-  double2 *N_u = delta_u + nb;
-  double2 *alpha_g = N_u + nb;
-  double2 *delta_g = alpha_g + nd;
-  double2 *N_g = delta_g + nd;
-  double2 *x_u = N_g + nd;
-  double2 *x_g = x_u + nb;
-  double2 *x_m = x_g + nd;         // mean-mode (v00 + i w00) solution
-  const int npm = g.npmax, ldu = g.npmax + 1;
-  double2 *ybuf = x_m + nd;        // [4][npm]
-  double *ipiv = reinterpret_cast<double *>(ybuf + 4 * npm);  // [4][npm]
-  double *Ubase = ipiv + 4 * npm;  // [4][npm * ldu]
+// Threads [0, 4 Mp): (system, parity) = (t / Mp) with system 0 = clamped u0,
+// 1 = Dirichlet g; lanes over modes.  Threads [4 Mp, 4 Mp + 2): the 1-D mean
+// system S00 = C0_D - gamma nu C2_D for (v00 + i w00), parity = lane.
+__global__ void __launch_bounds__(256) k_solve(Geo g, StructMats sm, SolveArgs a, double *scratch) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int M = g.m, Mp = (M + 63) / 64 * 64;
+  int sp = t / Mp, m = t - sp * Mp;
+  bool mean = false;
+  if (sp >= 4) {
+    if (sp > 4 || m >= 2) return;
+    mean = true;
+    sp = 2 + m;
+    m = 0;
+  } else if (m >= M) {
+    return;
+  }
+  const int sys = sp >> 1, par = sp & 1;
+  const StructSys &ss = sm.sys[sys][par];
+  const int n = ss.n;
+  double kyd, kzd, kyo;
+  mode_k(g, m, kyd, kzd, kyo);
+  const double k2 = mean ? 0.0 : kyd * kyd + kzd * kzd;
+  const double W = mean ? 1.0 : g.W, nu = g.nu, gm = a.gamma, wn = W * nu;
+  // scratch rows: the four mode systems, then the two mean systems
+  double *scr = mean ? scratch + (size_t)4 * g.npmax * 7 * M + (size_t)(sp - 2) * g.npmax * 7
+                     : scratch + (size_t)sp * g.npmax * 7 * M;
+  const int Ms = mean ? 1 : M;
+  if (sys == 0) {
+    // Su = W[(1 + 2 gm nu k2) C2 - gm nu C4 - (k2 + gm nu k2^2) C0]
+    // M z + L zL: delta = -k2 W z + k2^2 W nu zL, alpha = W z - 2 k2 W nu zL, beta = W nu zL
+    const double s2 = W * (1 + 2 * gm * nu * k2), s4 = -W * gm * nu, s0 = -W * (k2 + gm * nu * k2 * k2);
+    structured_solve<2, 2, 2, true>(ss, n, par, m, Ms, s0, s2, s4, s4, -k2 * W, k2 * k2 * wn, W,
+                                    -2 * k2 * wn, wn, a.u0, a.c, a.d, a.nx, a.Nu, a.e, a.nn, a.v,
+                                    a.w, a.Nv, a.Nw, false, scr, a.u0_out, nullptr, nullptr);
+  } else {
+    // Sg = W[(1 + gm nu k2) C0 - gm nu C2]; M = W C0, L = W nu (C2 - k2 C0)
+    const double s0 = W * (1 + gm * nu * k2), s2 = -W * gm * nu;
+    structured_solve<1, 1, 1, false>(ss, n, par, mean ? 0 : m, Ms, s0, s2, 0.0, s2, W, -k2 * wn,
+                                     0.0, wn, 0.0, a.g, a.c, a.d, a.nx, a.Ng, a.e, a.nn, a.v, a.w,
+                                     a.Nv, a.Nw, mean, scr, a.g_out, a.v_out, a.w_out);
+  }
+}
 
+// ------------------------------------------------------ k_solve_steady ----
+// Steady SBDF3 update: rhs = M z + N with z = sum_l c_l x_l over the three
+// solution levels and N = sum_l e_l N_l over the three nonlinear levels.  The
+// mass operators M are banded (W (C2_B - k^2 C0_B) and W C0_D have no far
+// part), so the right-hand side of row i only needs rows i-P .. i+Q and is
+// formed inside the top-down elimination sweep: two sweeps instead of three.
+// Global loads run two rows ahead of their use and the back substitution
+// fetches the next row's factors while it solves the current one, so each row
+// costs arithmetic latency, not a memory round trip.
+struct RowRaw {
+  double2 x[3];
+  double2 n[3];
+};
+
+template <int P, int Q, int R>
+__device__ void steady_solve(const StructSys &ss, int n, int par, int m, int M, double s0,
+                             double s2, double s4, double sf, double mb0, double mb2,
+                             const SolveArgs &a, bool isu, bool mean, double *scr,
+                             double2 *xout) {
+  constexpr int F = Q + 1;
+  constexpr int BW = P + Q + 1;
+  constexpr int NV = 1 + Q + R + 2;
+  const size_t rowstride = (size_t)NV * M;
+  auto scr_at = [&](int i, int v) -> double & { return scr[(size_t)i * rowstride + (size_t)v * M + m]; };
+
+  auto issue = [&](int j) {
+    RowRaw r;
+#pragma unroll
+    for (int l = 0; l < 3; ++l) r.x[l] = r.n[l] = double2{0, 0};
+    if (j < 0 || j >= n) return r;
+    const int J = 2 * j + par;
+    if (!mean) {
+      const size_t o = (size_t)J * M + m;
+#pragma unroll
+      for (int l = 0; l < 3; ++l) {
+        r.x[l] = isu ? a.u0[l][o] : a.g[l][o];
+        r.n[l] = isu ? a.Nu[l][o] : a.Ng[l][o];
+      }
+    } else {
+#pragma unroll
+      for (int l = 0; l < 3; ++l) {
+        r.x[l] = double2{a.v[l][J], a.w[l][J]};
+        r.n[l] = double2{a.Nv[l][J], a.Nw[l][J]};
+      }
+    }
+    return r;
+  };
+  auto zof = [&](const RowRaw &r) {
+    double2 z{0, 0};
+#pragma unroll
+    for (int l = 0; l < 3; ++l) {
+      z.x += a.c[l] * r.x[l].x;
+      z.y += a.c[l] * r.x[l].y;
+    }
+    return z;
+  };
+  auto nof = [&](const RowRaw &r) {
+    double2 z{0, 0};
+#pragma unroll
+    for (int l = 0; l < 3; ++l) {
+      z.x += a.e[l] * r.n[l].x;
+      z.y += a.e[l] * r.n[l].y;
+    }
+    return z;
+  };
+
+  // ---- sweep B: rhs + LU without pivoting, top-down
+  {
+    double2 zw[BW], nw[Q + 1];
+#pragma unroll
+    for (int o = 0; o < BW; ++o) zw[o] = double2{0, 0};
+#pragma unroll
+    for (int q = 0; q <= Q; ++q) {
+      RowRaw r = issue(q);
+      zw[P + q] = zof(r);
+      nw[q] = nof(r);
+    }
+    RowRaw pf1 = issue(Q + 1), pf2 = issue(Q + 2);
+    double pd[P + 1], pe[P + 1][Q + 1], pw[P + 1][R];
+    double2 py[P + 1];
+#pragma unroll
+    for (int t = 0; t <= P; ++t) {
+      pd[t] = 0;
+      py[t] = double2{0, 0};
+#pragma unroll
+      for (int q = 0; q <= Q; ++q) pe[t][q] = 0;
+#pragma unroll
+      for (int r = 0; r < R; ++r) pw[t][r] = 0;
+    }
+    for (int i = 0; i < n; ++i) {
+      const double *b0 = ss.b0 + (size_t)i * BW, *b2 = ss.b2 + (size_t)i * BW;
+      double a_[BW];
+      double2 yi = nw[0];
+#pragma unroll
+      for (int o = 0; o < BW; ++o) {
+        const double c0 = b0[o], c2 = b2[o];
+        const double mo = mb0 * c0 + mb2 * c2;
+        yi.x += mo * zw[o].x;
+        yi.y += mo * zw[o].y;
+        double v = s0 * c0 + s2 * c2;
+        if (R == 2) v += s4 * ss.b4[(size_t)i * BW + o];
+        a_[o] = v;
+      }
+      double wi[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) wi[r] = sf * ss.U[(size_t)i * R + r];
+#pragma unroll
+      for (int t = P; t >= 1; --t) {
+        if (i - t < 0) continue;
+        const double l = a_[P - t] * pd[t];
+#pragma unroll
+        for (int o = -t + 1; o <= Q; ++o) {
+          if (i + o >= n) continue;
+          const int rel = o + t;
+          double u;
+          if (rel <= Q) {
+            u = pe[t][rel];
+          } else {
+            u = 0.0;
+#pragma unroll
+            for (int r = 0; r < R; ++r) u += pw[t][r] * ss.V[(size_t)(i + o) * R + r];
+          }
+          a_[P + o] -= l * u;
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) wi[r] -= l * pw[t][r];
+        yi.x -= l * py[t].x;
+        yi.y -= l * py[t].y;
+      }
+      const double dinv = 1.0 / a_[P];
+      scr_at(i, 0) = dinv;
+#pragma unroll
+      for (int q = 1; q <= Q; ++q) scr_at(i, q) = a_[P + q];
+#pragma unroll
+      for (int r = 0; r < R; ++r) scr_at(i, 1 + Q + r) = wi[r];
+      scr_at(i, NV - 2) = yi.x;
+      scr_at(i, NV - 1) = yi.y;
+#pragma unroll
+      for (int t = P; t >= 2; --t) {
+        pd[t] = pd[t - 1];
+        py[t] = py[t - 1];
+#pragma unroll
+        for (int q = 0; q <= Q; ++q) pe[t][q] = pe[t - 1][q];
+#pragma unroll
+        for (int r = 0; r < R; ++r) pw[t][r] = pw[t - 1][r];
+      }
+      pd[1] = dinv;
+      py[1] = yi;
+#pragma unroll
+      for (int q = 1; q <= Q; ++q) pe[1][q] = a_[P + q];
+#pragma unroll
+      for (int r = 0; r < R; ++r) pw[1][r] = wi[r];
+      // advance windows: z rows i+1-P .. i+1+Q, N rows i+1 .. i+1+Q
+#pragma unroll
+      for (int o = 0; o < BW - 1; ++o) zw[o] = zw[o + 1];
+#pragma unroll
+      for (int q = 0; q < Q; ++q) nw[q] = nw[q + 1];
+      zw[BW - 1] = zof(pf1);
+      nw[Q] = nof(pf1);
+      pf1 = pf2;
+      pf2 = issue(i + Q + 3);
+    }
+  }
+
+  // ---- sweep C: back substitution, bottom-up, one row of factors in flight
+  {
+    double2 xw[F + 1];
+#pragma unroll
+    for (int k = 0; k <= F; ++k) xw[k] = double2{0, 0};
+    double2 tau[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) tau[r] = double2{0, 0};
+    double cur[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) cur[v] = scr_at(n - 1, v);
+    for (int i = n - 1; i >= 0; --i) {
+      double nxt[NV];
+      const int ip = (i > 0) ? i - 1 : 0;
+#pragma unroll
+      for (int v = 0; v < NV; ++v) nxt[v] = scr_at(ip, v);
+      if (i + F < n) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const double vv = ss.V[(size_t)(i + F) * R + r];
+          tau[r].x += vv * xw[F].x;
+          tau[r].y += vv * xw[F].y;
+        }
+      }
+      double2 acc{cur[NV - 2], cur[NV - 1]};
+#pragma unroll
+      for (int q = 1; q <= Q; ++q) {
+        acc.x -= cur[q] * xw[q].x;
+        acc.y -= cur[q] * xw[q].y;
+      }
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        acc.x -= cur[1 + Q + r] * tau[r].x;
+        acc.y -= cur[1 + Q + r] * tau[r].y;
+      }
+      const double2 xi{acc.x * cur[0], acc.y * cur[0]};
+      const int J = 2 * i + par;
+      if (mean) {
+        a.v_out[J] = xi.x;
+        a.w_out[J] = xi.y;
+      } else {
+        xout[(size_t)J * M + m] = xi;
+      }
+#pragma unroll
+      for (int k = F; k >= 2; --k) xw[k] = xw[k - 1];
+      xw[1] = xi;
+#pragma unroll
+      for (int v = 0; v < NV; ++v) cur[v] = nxt[v];
+    }
+  }
+}
+
+__global__ void __launch_bounds__(256) k_solve_steady(Geo g, StructMats sm, SolveArgs a,
+                                                      double *scratch) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int M = g.m, Mp = (M + 63) / 64 * 64;
+  int sp = t / Mp, m = t - sp * Mp;
+  bool mean = false;
+  if (sp >= 4) {
+    if (sp > 4 || m >= 2) return;
+    mean = true;
+    sp = 2 + m;
+    m = 0;
+  } else if (m >= M) {
+    return;
+  }
+  const int sys = sp >> 1, par = sp & 1;
+  const StructSys &ss = sm.sys[sys][par];
+  double kyd, kzd, kyo;
+  mode_k(g, m, kyd, kzd, kyo);
+  const double k2 = mean ? 0.0 : kyd * kyd + kzd * kzd;
+  const double W = mean ? 1.0 : g.W, nu = g.nu, gm = a.gamma;
+  double *scr = mean ? scratch + (size_t)4 * g.npmax * 7 * M + (size_t)(sp - 2) * g.npmax * 7
+                     : scratch + (size_t)sp * g.npmax * 7 * M;
+  const int Ms = mean ? 1 : M;
+  if (sys == 0) {
+    const double s2 = W * (1 + 2 * gm * nu * k2), s4 = -W * gm * nu, s0 = -W * (k2 + gm * nu * k2 * k2);
+    steady_solve<2, 2, 2>(ss, ss.n, par, m, Ms, s0, s2, s4, s4, -W * k2, W, a, true, false, scr,
+                          a.u0_out);
+  } else {
+    const double s0 = W * (1 + gm * nu * k2), s2 = -W * gm * nu;
+    steady_solve<1, 1, 1>(ss, ss.n, par, m, Ms, s0, s2, 0.0, s2, W, 0.0, a, false, mean, scr,
+                          a.g_out);
+  }
+}
+
+// Velocity reconstruction (channelflow_kmm.py:855-870): f = R u0 with
+// R = P_D d/dx on the clamped basis, u1 = i (ky f + kz g) / k^2,
+// u2 = i (kz f - ky g) / k^2, mean mode (0,0) <- (v00, w00).  R couples
+// only opposite parities (d/dx flips parity), so row i reads u0 rows
+// j = 1 - (i & 1), 3 - (i & 1), ...; Rc holds those entries, [i][k] for
+// j = 2k + 1 - (i & 1).  One thread per (row, mode), lanes over modes.
+__global__ void __launch_bounds__(256) k_reconstruct(Geo g, const double *__restrict__ Rc, int ldk,
+                                                     const double2 *__restrict__ u0,
+                                                     const double2 *__restrict__ gg,
+                                                     const double *__restrict__ v,
+                                                     const double *__restrict__ w,
+                                                     double2 *__restrict__ u1,
+                                                     double2 *__restrict__ u2) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = blockIdx.y;
+  if (m >= g.m) return;
+  const size_t o = (size_t)i * g.m + m;
+  if (m == 0) {
+    u1[o] = double2{v[i], 0.0};
+    u2[o] = double2{w[i], 0.0};
+    return;
+  }
+  const int j0 = 1 - (i & 1);
+  const int nk = (g.nb - j0 + 1) / 2;
+  const double *r = Rc + (size_t)i * ldk;
+  double2 f{0, 0};
+  for (int k = 0; k < nk; ++k) {
+    const double2 c = u0[(size_t)(j0 + 2 * k) * g.m + m];
+    f.x += r[k] * c.x;
+    f.y += r[k] * c.y;
+  }
   double kyd, kzd, kyo;
   mode_k(g, m, kyd, kzd, kyo);
   const double k2 = kyd * kyd + kzd * kzd;
-  const double W = g.W, nu = g.nu, gm = a.gamma;
-
-  // z / zL / N combinations into the operator-coefficient arrays.
-  for (int idx = tid; idx < nb + nd; idx += blockDim.x) {
-    const bool isu = idx < nb;
-    const int i = isu ? idx : idx - nb;
-    const size_t o = (size_t)i * g.m + m;
-    double2 z{0, 0}, zl{0, 0}, nn{0, 0};
-    for (int l = 0; l < a.nx; ++l) {
-      double2 v = isu ? a.u0[l][o] : a.g[l][o];
-      z.x += a.c[l] * v.x;
-      z.y += a.c[l] * v.y;
-      zl.x += a.d[l] * v.x;
-      zl.y += a.d[l] * v.y;
-    }
-    for (int l = 0; l < a.nn; ++l) {
-      double2 v = isu ? a.Nu[l][o] : a.Ng[l][o];
-      nn.x += a.e[l] * v.x;
-      nn.y += a.e[l] * v.y;
-    }
-    if (isu) {
-      // M = W (C2 - k2 C0), L = W nu (C4 - 2 k2 C2 + k2^2 C0)
-      const double wn = W * nu;
-      alpha_u[i] = double2{W * z.x - 2 * k2 * wn * zl.x, W * z.y - 2 * k2 * wn * zl.y};
-      beta_u[i] = double2{wn * zl.x, wn * zl.y};
-      delta_u[i] = double2{-k2 * W * z.x + k2 * k2 * wn * zl.x, -k2 * W * z.y + k2 * k2 * wn * zl.y};
-      N_u[i] = nn;
-    } else {
-      // M = W C0, L = W nu (C2 - k2 C0)
-      const double wn = W * nu;
-      alpha_g[i] = double2{wn * zl.x, wn * zl.y};
-      delta_g[i] = double2{W * z.x - k2 * wn * zl.x, W * z.y - k2 * wn * zl.y};
-      N_g[i] = nn;
-    }
-  }
-  __syncthreads();
-
-  {
-    const int s = wave >> 1, par = wave & 1;
-    const int n = sm.np[s][par];
-    double *U = Ubase + (size_t)wave * npm * ldu;
-    double2 *y = ybuf + wave * npm;
-    double *ip = ipiv + wave * npm;
-    if (s == 0) {
-      // Su = M - gamma L = W[(1 + 2 gamma nu k2) C2 - gamma nu C4 - (k2 + gamma nu k2^2) C0]
-      const double s2 = W * (1 + 2 * gm * nu * k2), s4 = -W * gm * nu, s0 = -W * (k2 + gm * nu * k2 * k2);
-      wave_solve(n, par, 2, sm.row[0][par][0], sm.row[0][par][1], sm.row[0][par][2],
-                 sm.col[0][par][0], sm.col[0][par][1], sm.col[0][par][2], s0, s2, s4, delta_u,
-                 alpha_u, beta_u, N_u, U, ldu, y, ip, x_u);
-    } else {
-      // Sg = M - gamma L = W[(1 + gamma nu k2) C0 - gamma nu C2]
-      const double s0 = W * (1 + gm * nu * k2), s2 = -W * gm * nu;
-      wave_solve(n, par, 1, sm.row[1][par][0], sm.row[1][par][1], nullptr, sm.col[1][par][0],
-                 sm.col[1][par][1], nullptr, s0, s2, 0.0, delta_g, alpha_g, nullptr, N_g, U, ldu,
-                 y, ip, x_g);
-    }
-  }
-  __syncthreads();
-
-  // Mean mode: S00 (v + i w) = M00 z + L00 zL + N with M00 = C0_D, L00 = nu C2_D,
-  // S00 = C0_D - gamma nu C2_D (no Fourier weight: a 1-D system).  The g-system
-  // coefficient arrays are free again and hold its right-hand-side terms.
-  const bool mean = (m == 0);
-  if (mean) {
-    for (int i = tid; i < nd; i += blockDim.x) {
-      double2 z{0, 0}, zl{0, 0}, nn{0, 0};
-      for (int l = 0; l < a.nx; ++l) {
-        z.x += a.c[l] * a.v[l][i];
-        z.y += a.c[l] * a.w[l][i];
-        zl.x += a.d[l] * a.v[l][i];
-        zl.y += a.d[l] * a.w[l][i];
-      }
-      for (int l = 0; l < a.nn; ++l) {
-        nn.x += a.e[l] * a.Nv[l][i];
-        nn.y += a.e[l] * a.Nw[l][i];
-      }
-      delta_g[i] = z;                              // coefficient of C0
-      alpha_g[i] = double2{nu * zl.x, nu * zl.y};  // coefficient of C2
-      N_g[i] = nn;
-    }
-    __syncthreads();
-    if (wave >= 2) {
-      const int par = wave & 1;
-      const int n = sm.np[1][par];
-      wave_solve(n, par, 1, sm.row[1][par][0], sm.row[1][par][1], nullptr, sm.col[1][par][0],
-                 sm.col[1][par][1], nullptr, 1.0, -gm * nu, 0.0, delta_g, alpha_g, nullptr, N_g,
-                 Ubase + (size_t)wave * npm * ldu, ldu, ybuf + wave * npm, ipiv + wave * npm, x_m);
-    }
-    __syncthreads();
-    for (int i = tid; i < nd; i += blockDim.x) {
-      a.v_out[i] = x_m[i].x;
-      a.w_out[i] = x_m[i].y;
-    }
-  }
-  for (int i = tid; i < nb; i += blockDim.x) a.u0_out[(size_t)i * g.m + m] = x_u[i];
-
-  // reconstruction: f = R u0_new; u1 = i (ky f + kz g)/k2, u2 = i (kz f - ky g)/k2
   const double den = (k2 == 0.0) ? 1.0 : k2;
-  const double kyo2 = kyd / den, kzo2 = kzd / den;
-  const double2 *xu = x_u;
-  for (int i = tid; i < nd; i += blockDim.x) {
-    const size_t o = (size_t)i * g.m + m;
-    const double2 gi = x_g[i];
-    a.g_out[o] = gi;
-    if (mean) {
-      a.u1[o] = double2{x_m[i].x, 0.0};
-      a.u2[o] = double2{x_m[i].y, 0.0};
-    } else {
-      double2 f{0, 0};
-      for (int j = 0; j < nb; ++j) {
-        const double r = sm.Rt[(size_t)j * nd + i];
-        f.x += r * xu[j].x;
-        f.y += r * xu[j].y;
-      }
-      // i (a) = (-a.y, a.x)
-      double2 s1{kyo2 * f.x + kzo2 * gi.x, kyo2 * f.y + kzo2 * gi.y};
-      double2 s2{kzo2 * f.x - kyo2 * gi.x, kzo2 * f.y - kyo2 * gi.y};
-      a.u1[o] = double2{-s1.y, s1.x};
-      a.u2[o] = double2{-s2.y, s2.x};
-    }
-  }
+  const double ky = kyd / den, kz = kzd / den;
+  const double2 gi = gg[o];
+  const double2 s1{ky * f.x + kz * gi.x, ky * f.y + kz * gi.y};
+  const double2 s2{kz * f.x - ky * gi.x, kz * f.y - ky * gi.y};
+  u1[o] = double2{-s1.y, s1.x};
+  u2[o] = double2{-s2.y, s2.x};
 }
 
 // -------------------------------------------------------------- k_diag ----
@@ -764,29 +1103,44 @@ hipError_t launch_nodal(const Geo &g, const NodalMats<T> &nm, const double2 *u0,
   return hipGetLastError();
 }
 
+// Butterflies per thread of the largest pass of `plan` over `nbatch` sequences.
+int max_butterflies(const FFTPlan &plan, int nbatch) {
+  int mb = 0;
+  for (int s = 0; s < plan.nst; ++s) mb = std::max(mb, (plan.n / plan.radix[s] * nbatch + 255) / 256);
+  return mb;
+}
+
 template <typename T>
 hipError_t launch_fft_y(const Geo &g, const NodalMats<T> &nm, Workspace &ws, const FFTPlan &py,
                         bool inverse, hipStream_t s) {
   using V = typename VT<T>::V;
   const int kc = pick_kc(g.kzs);
-  size_t lds = (size_t)g.my * kc * sizeof(V);
+  const size_t lds = (size_t)g.my * kc * sizeof(V);
+  const int mb = max_butterflies(py, kc);
+  const dim3 grid(g.kzs / kc, g.n0, inverse ? 6 : 3);
+  const V *tw = reinterpret_cast<const V *>(nm.twy);
+  const V *src = reinterpret_cast<const V *>(inverse ? ws.X : ws.Hz);
+  V *dst = reinterpret_cast<V *>(inverse ? ws.Y : ws.Hx);
+#define PCF_FFTY(INV, MB) hipLaunchKernelGGL((k_fft_y<T, INV, MB>), grid, dim3(256), lds, s, g, tw, py, kc, src, dst)
   if (inverse) {
-    hipLaunchKernelGGL((k_fft_y<T, true>), dim3(g.kzs / kc, g.n0, 6), dim3(256), lds, s, g,
-                       reinterpret_cast<const V *>(nm.twy), py, kc,
-                       reinterpret_cast<const V *>(ws.X), reinterpret_cast<V *>(ws.Y));
+    if (mb <= 2) PCF_FFTY(true, 2);
+    else if (mb <= 4) PCF_FFTY(true, 4);
+    else PCF_FFTY(true, 8);
   } else {
-    hipLaunchKernelGGL((k_fft_y<T, false>), dim3(g.kzs / kc, g.n0, 3), dim3(256), lds, s, g,
-                       reinterpret_cast<const V *>(nm.twy), py, kc,
-                       reinterpret_cast<const V *>(ws.Hz), reinterpret_cast<V *>(ws.Hx));
+    if (mb <= 2) PCF_FFTY(false, 2);
+    else if (mb <= 4) PCF_FFTY(false, 4);
+    else PCF_FFTY(false, 8);
   }
+#undef PCF_FFTY
   return hipGetLastError();
 }
 
 template <typename T>
 int rot_units(const Geo &g) {
   using V = typename VT<T>::V;
-  // keep LDS (9 lines of MZ per unit) under 64 KB
-  int upb = 4;
+  // two y-line pairs per block keep the butterfly registers small; LDS holds
+  // 9 lines of MZ per unit
+  int upb = 2;
   while (upb > 1 && (size_t)upb * 9 * g.mz * sizeof(V) > 65536) upb >>= 1;
   return upb;
 }
@@ -797,9 +1151,16 @@ hipError_t launch_rotational(const Geo &g, const NodalMats<T> &nm, Workspace &ws
   using V = typename VT<T>::V;
   const int upb = rot_units<T>(g);
   const int nunits = g.n0 * (g.my / 2);
-  size_t lds = (size_t)upb * 9 * g.mz * sizeof(V);
-  hipLaunchKernelGGL((k_rotational<T>), dim3((nunits + upb - 1) / upb), dim3(256), lds, s, g, nm,
-                     pz, upb, reinterpret_cast<const V *>(ws.Y), reinterpret_cast<V *>(ws.Hz));
+  const size_t lds = (size_t)upb * 9 * g.mz * sizeof(V);
+  const int mb = max_butterflies(pz, upb * 6);
+  const dim3 grid((nunits + upb - 1) / upb);
+  const V *Y = reinterpret_cast<const V *>(ws.Y);
+  V *Hz = reinterpret_cast<V *>(ws.Hz);
+#define PCF_ROT(MB) hipLaunchKernelGGL((k_rotational<T, MB>), grid, dim3(256), lds, s, g, nm, pz, upb, Y, Hz)
+  if (mb <= 3) PCF_ROT(3);
+  else if (mb <= 6) PCF_ROT(6);
+  else PCF_ROT(12);
+#undef PCF_ROT
   return hipGetLastError();
 }
 
@@ -813,14 +1174,27 @@ hipError_t launch_rhs(const Geo &g, const NodalMats<T> &nm, Workspace &ws, doubl
   return hipGetLastError();
 }
 
-size_t solve_lds_bytes(const Geo &g) {
-  return (size_t)(4 * g.nb + 3 * g.nd + g.nb + 2 * g.nd + 4 * g.npmax) * sizeof(double2) +
-         (size_t)4 * g.npmax * sizeof(double) +
-         (size_t)4 * g.npmax * (g.npmax + 1) * sizeof(double);
+size_t solve_scratch_doubles(const Geo &g) {
+  return (size_t)4 * g.npmax * 7 * g.m + (size_t)2 * g.npmax * 7;
 }
 
-hipError_t launch_solve(const Geo &g, const SolveMats &sm, const SolveArgs &a, hipStream_t s) {
-  hipLaunchKernelGGL(k_solve, dim3(g.m), dim3(256), solve_lds_bytes(g), s, g, sm, a);
+hipError_t launch_solve(const Geo &g, const StructMats &sm, const SolveArgs &a, double *scratch,
+                        bool steady, hipStream_t s) {
+  const int Mp = (g.m + 63) / 64 * 64;
+  const int threads = 4 * Mp + 64;
+  if (steady)
+    hipLaunchKernelGGL(k_solve_steady, dim3((threads + 255) / 256), dim3(256), 0, s, g, sm, a,
+                       scratch);
+  else
+    hipLaunchKernelGGL(k_solve, dim3((threads + 255) / 256), dim3(256), 0, s, g, sm, a, scratch);
+  return hipGetLastError();
+}
+
+hipError_t launch_reconstruct(const Geo &g, const double *Rc, int ldk, const SolveArgs &a,
+                              hipStream_t s) {
+  dim3 grid((g.m + 255) / 256, g.nd);
+  hipLaunchKernelGGL(k_reconstruct, grid, dim3(256), 0, s, g, Rc, ldk, a.u0_out, a.g_out, a.v_out,
+                     a.w_out, a.u1, a.u2);
   return hipGetLastError();
 }
 
@@ -837,17 +1211,10 @@ hipError_t launch_diag(const Geo &g, const double *VB, const double *VD, const d
 bool fft_fits(const Geo &g, const FFTPlan &py, const FFTPlan &pz, int elem_bytes) {
   const int kc = pick_kc(g.kzs);
   if ((size_t)g.my * kc * elem_bytes > 65536) return false;
-  for (int s = 0; s < py.nst; ++s) {
-    const int total = py.n / py.radix[s] * kc;
-    if ((total + 255) / 256 > 8) return false;
-  }
+  if (max_butterflies(py, kc) > 8) return false;
   const int upb = (elem_bytes == 16) ? rot_units<double>(g) : rot_units<float>(g);
   if ((size_t)upb * 9 * g.mz * elem_bytes > 65536) return false;
-  for (int s = 0; s < pz.nst; ++s) {
-    const int total = pz.n / pz.radix[s] * upb * 6;
-    if ((total + 255) / 256 > 12) return false;
-  }
-  return true;
+  return max_butterflies(pz, upb * 6) <= 12;
 }
 
 #define PCF_INSTANTIATE(T)                                                                     \

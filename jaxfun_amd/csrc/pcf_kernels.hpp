@@ -55,14 +55,19 @@ struct NodalMats {
   const T *twz;   // [mz]
 };
 
-// Parity-split Galerkin blocks for the per-mode solves.  For basis s (0 = clamped,
-// 1 = Dirichlet), parity p, block t (0: C0, 1: C2, 2: C4): a (np x np) row-major
-// copy at row[s][p][t] and a column-major copy at col[s][p][t].
-struct SolveMats {
-  const double *row[2][2][3];
-  const double *col[2][2][3];
-  int np[2][2];
-  const double *Rt;  // [nb][nd] column-major reconstruction matrix R
+// Structured per-mode operator data of one (basis, parity) block: the band
+// of each Galerkin block (row-major [n][P+Q+1], offsets -P..Q) and the far
+// factors U [n][R], V [n][R] with C[i][j] = sum_r U_r(i) V_r(j) for j > i+Q
+// (C4 for the clamped basis, C2 for the Dirichlet basis; the others have no
+// far part).
+struct StructSys {
+  int n;
+  const double *b0, *b2, *b4;
+  const double *U, *V;
+};
+
+struct StructMats {
+  StructSys sys[2][2];  // [basis: 0 clamped, 1 Dirichlet][parity]
 };
 
 // One implicit update: rhs = M z + L zL + sum_l e_l N_l with z = sum c_l x_l,
@@ -106,7 +111,13 @@ hipError_t launch_rotational(const Geo &g, const NodalMats<T> &nm, Workspace &ws
 template <typename T>
 hipError_t launch_rhs(const Geo &g, const NodalMats<T> &nm, Workspace &ws, double2 *Nu,
                       double2 *Ng, double *Nv, double *Nw, hipStream_t s);
-hipError_t launch_solve(const Geo &g, const SolveMats &sm, const SolveArgs &a, hipStream_t s);
+// steady = true: SBDF3 update (three solution and nonlinear levels, d = 0);
+// false: general update (IMEXRK3 stages).
+hipError_t launch_solve(const Geo &g, const StructMats &sm, const SolveArgs &a, double *scratch,
+                        bool steady, hipStream_t s);
+hipError_t launch_reconstruct(const Geo &g, const double *Rc, int ldk, const SolveArgs &a,
+                              hipStream_t s);
+size_t solve_scratch_doubles(const Geo &g);
 
 // Diagnostics partial sums: out[0] = sum wF |u|^2 (perturbation), out[1] = sum
 // wF |div u|^2, both over the full spectrum (Hermitian weights applied), and
@@ -116,7 +127,6 @@ hipError_t launch_diag(const Geo &g, const double *VB, const double *VD, const d
                        const double2 *u0, const double2 *u1, const double2 *u2, double *out,
                        hipStream_t s);
 
-size_t solve_lds_bytes(const Geo &g);
 // Largest butterfly count per thread any FFT launch needs (host-side check
 // against the kernels' register budgets).
 bool fft_fits(const Geo &g, const FFTPlan &py, const FFTPlan &pz, int elem_bytes);

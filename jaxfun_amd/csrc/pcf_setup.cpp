@@ -1,6 +1,7 @@
 // Closed-form operator construction (see pcf_setup.hpp for the formulas).
 #include "pcf_setup.hpp"
 
+#include <algorithm>
 #include <cmath>
 
 namespace pcf {
@@ -166,6 +167,98 @@ std::string build_operators(int n0, Operators &ops) {
   ops.R = to_double(R);
   ops.FB = to_double(FB);
   ops.FD = to_double(FD);
+
+  // Structured form of the parity blocks.  Far-upper factors from
+  //   (T_a, T_b^(2)) = (pi/2)  b (b^2 - a^2)
+  //   (T_a, T_b^(4)) = (pi/48) [ (b^7 - 8b^5 + 16b^3) - (3b^5 + 16b) a^2
+  //                              + (3b^3 + 8b) a^4 - b a^6 ]      (b >= a)
+  // and the stencil boundary sums: for the clamped basis sum S = sum S a^2 = 0,
+  // so C2_B has no far part and C4_B's far part is rank 2 (the a^4, a^6 terms);
+  // for the Dirichlet basis sum S = 0 and sum S a^2 = -4(I+1), so C2_D's far
+  // part is rank 1: -4 pi (I+1).
+  {
+    const LMat *full[2][3] = {{&C0B, &C2B, &C4B}, {&C0D, &C2D, nullptr}};
+    const LMat *stc[2] = {&SB, &SD};
+    for (int b = 0; b < 2; ++b)
+      for (int p = 0; p < 2; ++p) {
+        Operators::Struct &st = ops.st[b][p];
+        const int nfull = full[b][0]->rows;
+        const int n = (nfull - p + 1) / 2;
+        st.n = n;
+        st.P = (b == 0) ? 2 : 1;
+        st.Q = (b == 0) ? 2 : 1;
+        st.R = (b == 0) ? 2 : 1;
+        const int BW = st.P + st.Q + 1;
+        st.b0.assign((size_t)n * BW, 0.0);
+        st.b2.assign((size_t)n * BW, 0.0);
+        st.b4.assign((size_t)n * BW, 0.0);
+        st.U.assign((size_t)n * st.R, 0.0);
+        st.V.assign((size_t)n * st.R, 0.0);
+        std::vector<double> *bands[3] = {&st.b0, &st.b2, &st.b4};
+        for (int i = 0; i < n; ++i)
+          for (int o = 0; o < BW; ++o) {
+            const int j = i + o - st.P;
+            if (j < 0 || j >= n) continue;
+            for (int t = 0; t < 3; ++t)
+              if (full[b][t]) (*bands[t])[(size_t)i * BW + o] = (double)(*full[b][t])(2 * i + p, 2 * j + p);
+          }
+        const LMat &S = *stc[b];
+        for (int i = 0; i < n; ++i) {
+          const int I = 2 * i + p;
+          ld m2 = 0, m4 = 0, m6 = 0, b1 = 0, b3 = 0;
+          for (int a = 0; a < N; ++a) {
+            const ld sv = S(I, a);
+            if (sv == 0) continue;
+            const ld aa = (ld)a * a;
+            m2 += sv * aa;
+            m4 += sv * aa * aa;
+            m6 += sv * aa * aa * aa;
+            b1 += sv * a;
+            b3 += sv * a * aa;
+          }
+          if (b == 0) {
+            st.U[(size_t)i * 2 + 0] = (double)(PI_L / 48 * m4);
+            st.U[(size_t)i * 2 + 1] = (double)(PI_L / 48 * m6);
+            st.V[(size_t)i * 2 + 0] = (double)(3 * b3 + 8 * b1);
+            st.V[(size_t)i * 2 + 1] = (double)(-b1);
+          } else {
+            // (pi/2) (sum_a S a^2) (-sum_b S b), and -sum_b S b = 2 for every row
+            st.U[i] = (double)(PI_L / 2 * m2 * 2);
+            st.V[i] = 1.0;
+          }
+        }
+        // verify: band + far reproduces the block that has a far part (C4_B,
+        // C2_D); every other block vanishes off the band
+        const LMat *farm = full[b][b == 0 ? 2 : 1];
+        ld worst = 0, scale = 0, bscale[3] = {0, 0, 0};
+        for (int t = 0; t < 3; ++t)
+          if (full[b][t])
+            for (int i = 0; i < n; ++i)
+              for (int j = 0; j < n; ++j)
+                bscale[t] = std::max(bscale[t], fabsl((*full[b][t])(2 * i + p, 2 * j + p)));
+        for (int i = 0; i < n; ++i)
+          for (int j = 0; j < n; ++j) {
+            const ld ref = (*farm)(2 * i + p, 2 * j + p);
+            scale = std::max(scale, fabsl(ref));
+            ld rec = 0;
+            if (j - i > st.Q) {
+              for (int r = 0; r < st.R; ++r)
+                rec += (ld)st.U[(size_t)i * st.R + r] * st.V[(size_t)j * st.R + r];
+            } else if (j - i >= -st.P) {
+              rec = (b == 0 ? st.b4 : st.b2)[(size_t)i * BW + (j - i + st.P)];
+            }
+            worst = std::max(worst, fabsl(rec - ref));
+            for (int t = 0; t < 3; ++t) {
+              if (!full[b][t] || full[b][t] == farm) continue;
+              // exact zeros up to the long-double residue of the rational stencils
+              if ((j - i > st.Q || j - i < -st.P) &&
+                  fabsl((*full[b][t])(2 * i + p, 2 * j + p)) > 1e-14L * bscale[t])
+                return "structured form: unexpected entry outside the band";
+            }
+          }
+        if (worst > 1e-11L * scale) return "structured form does not reproduce the Galerkin block";
+      }
+  }
 
   // Fejer-I weights (Chebyshev.py:183-209).
   ops.fejer.resize(N);

@@ -44,6 +44,13 @@ struct Operators {
   std::vector<double> x;      // Gauss nodes (ascending)
   std::vector<double> fejer;  // Fejer-I weights on the nodes
   std::vector<double> dphi_top, dphi_bot;  // phi_i'(+1), phi_i'(-1)
+  // Band + far low-rank form of the parity blocks (see build_operators):
+  // [basis 0 clamped / 1 Dirichlet][parity]
+  struct Struct {
+    int n = 0, P = 0, Q = 0, R = 0;
+    std::vector<double> b0, b2, b4;  // [n][P+Q+1], offsets -P..Q
+    std::vector<double> U, V;        // [n][R]
+  } st[2][2];
 };
 
 // Returns an empty string on success, else an error message.

@@ -118,7 +118,7 @@ def test_fixed_dt_rejected():
     solver.set_dt(solver.dt)
 
 
-@pytest.mark.parametrize("case", [c for c in GOLDEN["cases"] if c["N"] == [65, 64, 64]],
+@pytest.mark.parametrize("case", GOLDEN["cases"],
                          ids=lambda c: "x".join(map(str, c["N"])))
 def test_published_golden(case):
     """Reference-published FP64 results after the benchmark's step count."""
