@@ -533,6 +533,11 @@ int32_t pcf_create(const pcf_config *cfg, pcf_handle **out) {
     if ((r = build_nodal<float>(h, h->mf, 8, 8))) return bail(r);
   }
   if ((r = build_struct_mats(h))) return bail(r);
+  {
+    hipError_t e = pcf::kernels_init(h->geo);
+    if (e != hipSuccess)
+      return bail(fail(PCF_E_UNSUPPORTED, std::string("kernel LDS limits: ") + hipGetErrorString(e)));
+  }
   if ((r = h->upload(&h->dVB, h->ops.VB.a))) return bail(r);
   if ((r = h->upload(&h->dVD, h->ops.VD.a))) return bail(r);
   if ((r = h->upload(&h->dVB1, h->ops.VB1.a))) return bail(r);

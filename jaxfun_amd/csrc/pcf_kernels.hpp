@@ -118,6 +118,8 @@ hipError_t launch_solve(const Geo &g, const StructMats &sm, const SolveArgs &a, 
 hipError_t launch_reconstruct(const Geo &g, const double *Rc, int ldk, const SolveArgs &a,
                               hipStream_t s);
 size_t solve_scratch_doubles(const Geo &g);
+// Per-configuration kernel attributes (dynamic LDS limits); call once per handle.
+hipError_t kernels_init(const Geo &g);
 
 // Diagnostics partial sums: out[0] = sum wF |u|^2 (perturbation), out[1] = sum
 // wF |div u|^2, both over the full spectrum (Hermitian weights applied), and
