@@ -1,0 +1,224 @@
+"""Host-side checks of libpcfdns.so that need no GPU.
+
+* the library loads and exports every entry point include/pcf_dns.h declares;
+* the closed-form setup matrices agree with the oracle's independent
+  brute-force quadrature and transforms;
+* configuration errors surface as the reference's exception types;
+* the seeded initial condition is divergence-free, Hermitian and at the
+  requested energy.
+"""
+
+import ctypes as C
+import math
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from jaxfun_amd import _native as nat
+from oracle.kmm import (
+    PCFOracle,
+    cheb_backward,
+    cheb_derivative,
+    galerkin_matrix,
+    stencil_clamped,
+    stencil_dirichlet,
+)
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+def cfg_for(n0, ny=16, nz=16, **kw):
+    c = nat.PcfConfig()
+    c.abi_version = nat.ABI_VERSION
+    c.n0, c.ny, c.nz = n0, ny, nz
+    c.lx[0], c.lx[1] = -1.0, 1.0
+    c.ly, c.lz = 4 * math.pi, 2 * math.pi
+    c.nu, c.u_wall, c.dt = 1 / 400, 1.0, 1e-3
+    c.padding_y = c.padding_z = 1.5
+    c.precision = nat.PREC_FP64
+    c.time_integrator = nat.TI_SBDF3
+    c.nonlinear_form = nat.NL_ROTATIONAL
+    for k, v in kw.items():
+        setattr(c, k, v)
+    return c
+
+
+def test_library_exports_every_declared_symbol():
+    lib = nat.load()
+    header = (ROOT / "include" / "pcf_dns.h").read_text()
+    declared = set(re.findall(r"^\s*(?:int32_t|int64_t|const char \*)\s*\*?\s*(pcf_\w+)\s*\(", header, re.M))
+    assert declared, "no declarations parsed"
+    assert declared == set(nat.EXPORTS)
+    for name in declared:
+        assert hasattr(lib, name), name
+    assert lib.pcf_abi_version() == nat.ABI_VERSION
+
+
+@pytest.mark.parametrize("n0", [9, 17, 33, 65])
+def test_galerkin_blocks_match_quadrature(n0):
+    cfg = cfg_for(n0)
+    SB, SD = stencil_clamped(n0), stencil_dirichlet(n0)
+    ref = {"C0B": galerkin_matrix(SB, SB, 0), "C2B": galerkin_matrix(SB, SB, 2),
+           "C4B": galerkin_matrix(SB, SB, 4), "C0D": galerkin_matrix(SD, SD, 0),
+           "C2D": galerkin_matrix(SD, SD, 2)}
+    for name, R in ref.items():
+        M = nat.setup_matrix(cfg, name)
+        assert M.shape == R.shape
+        # the quadrature sums entries up to ~1e9 in FP64: ~1e-13 relative rounding
+        assert np.max(np.abs(M - R)) <= 1e-12 * np.max(np.abs(R)), name
+
+
+def test_nodal_transforms_match_oracle():
+    n0 = 17
+    cfg = cfg_for(n0)
+    SB, SD = stencil_clamped(n0), stencil_dirichlet(n0)
+    VB, VD, VD1, VB1 = (nat.setup_matrix(cfg, k) for k in ("VB", "VD", "VD1", "VB1"))
+    rng = np.random.default_rng(0)
+    cb, cd = rng.standard_normal(n0 - 4), rng.standard_normal(n0 - 2)
+    assert np.allclose(VB @ cb, cheb_backward(SB.T @ cb), atol=1e-13)
+    assert np.allclose(VD @ cd, cheb_backward(SD.T @ cd), atol=1e-13)
+    assert np.allclose(VD1 @ cd, cheb_backward(cheb_derivative(SD.T @ cd, 1)), atol=1e-12)
+    assert np.allclose(VB1 @ cb, cheb_backward(cheb_derivative(SB.T @ cb, 1)), atol=1e-12)
+
+
+def test_projection_and_nonlinear_rows_match_oracle():
+    """FD/FB/A1/A0/AG/R against the oracle's transform + mass-solve route."""
+    N = (17, 8, 8)
+    cfg = cfg_for(N[0], N[1], N[2])
+    o = PCFOracle(N=N, Re=400.0, dt=1e-3)
+    rng = np.random.default_rng(3)
+    vals = rng.standard_normal((N[0], N[1], N[2])) + 1j * rng.standard_normal((N[0], N[1], N[2]))
+    # projection of nodal values along x only (Fourier axes untouched)
+    from oracle.kmm import cheb_scalar_product
+
+    L = np.tensordot(o.SD, cheb_scalar_product(vals, N[0]), axes=(1, 0))
+    ref_fd = np.linalg.solve(o.MD1, L.reshape(N[0] - 2, -1)).reshape(L.shape)
+    FD = nat.setup_matrix(cfg, "FD")
+    assert np.allclose(np.tensordot(FD, vals, axes=(1, 0)), ref_fd, atol=1e-13)
+    L = np.tensordot(o.SB, cheb_scalar_product(vals, N[0]), axes=(1, 0))
+    ref_fb = np.linalg.solve(o.MB1, L.reshape(N[0] - 4, -1)).reshape(L.shape)
+    FB = nat.setup_matrix(cfg, "FB")
+    assert np.allclose(np.tensordot(FB, vals, axes=(1, 0)), ref_fb, atol=1e-12)
+    # Nu / Ng rows: H given as nodal values; oracle projects then differentiates
+    H = tuple(rng.standard_normal(vals.shape) + 1j * rng.standard_normal(vals.shape) for _ in range(3))
+    H = tuple(o.mask(h) for h in H)
+    Hd = tuple(np.tensordot(FD, h, axes=(1, 0)) for h in H)
+    Nu, Ng, Nv, Nw = o.nonlinear_rhs(Hd)
+    A1, A0, AG = (nat.setup_matrix(cfg, k) for k in ("A1", "A0", "AG"))
+    ky, kz = o.K1e, o.K2e
+    k2 = o.K1**2 + o.K2**2
+    c1 = 1j * ky * H[1] + 1j * kz * H[2]
+    cg = 1j * kz * H[1] - 1j * ky * H[2]
+    mine_u = o.W * (np.tensordot(A1, c1, axes=(1, 0)) + np.tensordot(A0, k2 * H[0], axes=(1, 0)))
+    mine_g = o.W * np.tensordot(AG, cg, axes=(1, 0))
+    assert np.max(np.abs(mine_u - Nu)) <= 1e-12 * np.max(np.abs(Nu))
+    assert np.max(np.abs(mine_g - Ng)) <= 1e-13 * np.max(np.abs(Ng))
+    assert np.allclose(-AG @ np.real(H[1][:, 0, 0]), Nv, atol=1e-13)
+    # reconstruction f = P_D d/dx u0
+    R = nat.setup_matrix(cfg, "R")
+    u0 = rng.standard_normal((N[0] - 4, 1, 1)) + 0j
+    a = cheb_derivative(o.to_orth(u0, o.SB), 1)
+    w = np.tensordot(o.SD, a * o.h[:, None, None], axes=(1, 0))
+    f_ref = np.linalg.solve(o.MD1, w.reshape(N[0] - 2, -1)).reshape(w.shape)
+    assert np.allclose(np.tensordot(R, u0, axes=(1, 0)), f_ref, atol=1e-12)
+
+
+def test_structured_setup_accepts_all_benchmark_sizes():
+    for n0 in (9, 17, 33, 65, 97, 128, 129):
+        M = nat.setup_matrix(cfg_for(n0), "C4B")
+        assert M.shape == (n0 - 4, n0 - 4)
+
+
+def test_config_errors():
+    lib = nat.load()
+    bad = [cfg_for(8), cfg_for(17, ny=15), cfg_for(17, dt=0.0), cfg_for(17, nu=-1.0),
+           cfg_for(17, precision=7), cfg_for(17, time_integrator=1), cfg_for(17, abi_version=99)]
+    for c in bad:
+        assert lib.pcf_field_size(C.byref(c), 0) < 0
+        assert lib.pcf_last_error()
+    c = cfg_for(17)
+    assert lib.pcf_field_size(C.byref(c), nat.FIELD_ID["U0"]) == 13 * 16 * 16
+    assert lib.pcf_field_size(C.byref(c), nat.FIELD_ID["G"]) == 15 * 16 * 16
+    assert lib.pcf_field_size(C.byref(c), nat.FIELD_ID["SOL_OLD_V"]) == 15
+    with pytest.raises(ValueError):
+        nat.check(-1)
+
+
+def test_python_boundary_errors_match_reference():
+    """Invalid values raise ValueError, valid-but-unbuilt options
+    NotImplementedError (examples/channelflow_kmm.py:140-187)."""
+    from jaxfun_amd import PlaneCouetteFluctuationHIP as S
+
+    with pytest.raises(ValueError):
+        S(nonlinear_form="skew")
+    with pytest.raises(ValueError):
+        S(coefficient_path="fast")
+    with pytest.raises(ValueError):
+        S(time_integrator="RK4")
+    with pytest.raises(ValueError):
+        S(family="X")
+    with pytest.raises(NotImplementedError):
+        S(family="L")
+    with pytest.raises(NotImplementedError):
+        S(nonlinear_form="gradient")
+    with pytest.raises(NotImplementedError):
+        S(time_integrator="CNAB2")
+
+
+class _HostSolver:
+    """Just the host-side geometry of PlaneCouetteFluctuationHIP (no GPU handle)."""
+
+    def __init__(self, N):
+        from jaxfun_amd.pcf import PlaneCouetteFluctuationHIP as S
+
+        self.N = N
+        self.Ly, self.Lz = 4 * math.pi, 2 * math.pi
+        self.domain = ((-1.0, 1.0), (0.0, self.Ly), (0.0, self.Lz))
+        self._cfg = cfg_for(*N)
+        self.matrix = lambda name: nat.setup_matrix(self._cfg, name)
+        self.wavenumbers = S.wavenumbers.__get__(self)
+        self.mask_nyquist = S.mask_nyquist.__get__(self)
+        self._with_zero_history = S._with_zero_history.__get__(self)
+        self.nb, self.nd = N[0] - 4, N[0] - 2
+
+
+def test_seeded_state_properties():
+    from jaxfun_amd.ic import perturbation_energy, seeded_state
+
+    N = (17, 16, 16)
+    hs = _HostSolver(N)
+    s = seeded_state(hs, seed=0, energy=1e-3, K=4)
+    o = PCFOracle(N=N, Re=1000.0, dt=5e-3)
+    # energy: Epert = int |u|^2 = 2 V E
+    V = 2 * hs.Ly * hs.Lz
+    assert math.isclose(perturbation_energy(hs, s.u), 2 * V * 1e-3, rel_tol=1e-12)
+    assert math.isclose(o.diagnostics(_to_oracle(s))["Epert"], 2 * V * 1e-3, rel_tol=1e-12)
+    # divergence-free and Hermitian (real physical fields)
+    assert o.diagnostics(_to_oracle(s))["divL2"] < 1e-14
+    for a in list(s.u) + [s.g]:
+        iy = (-np.arange(N[1])) % N[1]
+        iz = (-np.arange(N[2])) % N[2]
+        mirror = np.conj(a[:, iy][:, :, iz])
+        assert np.max(np.abs(a - mirror)) < 1e-15
+        assert np.all(a[:, N[1] // 2, :] == 0) and np.all(a[:, :, N[2] // 2] == 0)
+    # deterministic in the seed
+    s2 = seeded_state(hs, seed=0, energy=1e-3, K=4)
+    assert all(np.array_equal(a, b) for a, b in zip(s.u, s2.u))
+
+
+def _to_oracle(s):
+    from tests.helpers import to_oracle
+
+    return to_oracle(s)
+
+
+def test_bench_refuses_multi_gpu():
+    import subprocess
+    import sys
+
+    r = subprocess.run([sys.executable, str(ROOT / "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode != 0
+    assert "one GPU" in r.stderr
