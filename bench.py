@@ -66,8 +66,11 @@ def stage_model(solver, precision: str) -> dict:
     fftf_f = fft_flops(my, 3 * n0 * kzs)
     rhs_b = 3 * n0 * m * V + (nb + nd) * m * c16
     rhs_f = m * n0 * 16 * nd
-    solve_b = (3 * (nb + nd) * m * c16) * 2 + (nb + nd) * m * c16 + 2 * nd * m * c16
-    solve_f = m * (2 * (nb * nb + nd * nd) * 4 + 4 * nb * nd)
+    # three solution + three nonlinear levels in, new (u0, g) out
+    solve_b = 7 * (nb + nd) * m * c16
+    solve_f = m * 2 * (nb + nd) * 40  # O(n) structured sweeps, ~40 flops per row
+    recon_b = (nb + nd) * m * c16 + 2 * nd * m * c16
+    recon_f = m * nd * (nb // 2) * 4
     return {
         "nodal_backward": (nodal_b, nodal_f),
         "fft_y_inverse": (ffti_b, ffti_f),
@@ -75,6 +78,7 @@ def stage_model(solver, precision: str) -> dict:
         "fft_y_forward": (fftf_b, fftf_f),
         "forward_rhs": (rhs_b, rhs_f),
         "solve": (solve_b, solve_f),
+        "reconstruct": (recon_b, recon_f),
     }
 
 
@@ -88,7 +92,7 @@ def profile_stages(solver, nsteps: int) -> dict:
     return {lib.pcf_stage_name(i).decode(): float(out[i]) for i in range(n)}
 
 
-def run_gpu(precision: str, steps: int, warmup: int):
+def run_gpu(precision: str, steps: int, warmup: int, eager: bool = False):
     from jaxfun_amd import PlaneCouetteFluctuationHIP
     from jaxfun_amd.ic import seeded_state
 
@@ -97,10 +101,11 @@ def run_gpu(precision: str, steps: int, warmup: int):
     state = seeded_state(solver, seed=w["seed"], energy=w["energy"], K=w["K"])
     solver.load(state)
     solver.advance(2)  # IMEXRK3 startup, excluded
-    solver.advance_steady(warmup, graph_block=GRAPH_BLOCK)
+    block = 0 if eager else GRAPH_BLOCK
+    solver.advance_steady(warmup, graph_block=block)
     solver.synchronize()
     t0 = time.perf_counter()
-    solver.advance_steady(steps, graph_block=GRAPH_BLOCK)
+    solver.advance_steady(steps, graph_block=block)
     solver.synchronize()
     t1 = time.perf_counter()
     stages = profile_stages(solver, 48)
@@ -108,23 +113,65 @@ def run_gpu(precision: str, steps: int, warmup: int):
     return solver, state, (t1 - t0), stages, diag
 
 
+# stage name -> kernel-name fragment in the rocprofv3 traffic summary
+_KERNEL_OF = {"nodal_backward": "k_nodal_backward", "fft_y_inverse": "k_fft_y<double, true",
+              "rotational": "k_rotational", "fft_y_forward": "k_fft_y<double, false",
+              "forward_rhs": "k_forward_rhs", "solve": "k_solve_steady", "reconstruct": "k_reconstruct"}
+
+
+def measured_traffic(stage: str, precision: str):
+    """HBM bytes per launch from the committed counter summary (FETCH_SIZE and
+    WRITE_SIZE runs, copy-calibrated), or None when no summary covers it."""
+    path = ROOT / "profiles" / f"traffic_65x64x64_{precision}.json"
+    if not path.exists():
+        return None
+    data = json.loads(path.read_text())
+    frag = _KERNEL_OF.get(stage, stage)
+    if precision != "fp64":
+        frag = frag.replace("double", "float")
+    for name, rec in data["kernels"].items():
+        if frag in name:
+            return rec["traffic_bytes"]
+    return None
+
+
 def roofline(solver, precision: str, stages: dict) -> dict:
     model = stage_model(solver, precision)
     name = max(stages, key=lambda k: stages[k])
     nbytes, flops = model[name]
     sec = stages[name] * 1e-3
+    traffic = measured_traffic(name, precision)
     peak_f = (FP64_PEAK_TFLOPS if precision == "fp64" else FP32_PEAK_TFLOPS) * 1e12
     hbm_t = nbytes / (HBM_PEAK_GBS * 1e9)
     cmp_t = flops / peak_f
     if cmp_t > hbm_t:
         ach = flops / sec / 1e12
         return {"kernel": name, "bound": "mfma", "achieved": ach, "peak": peak_f / 1e12,
-                "unit": "TFLOP/s", "frac": ach / (peak_f / 1e12), "traffic": None,
+                "unit": "TFLOP/s", "frac": ach / (peak_f / 1e12), "traffic": traffic,
                 "algorithmic_bytes": nbytes, "flops": flops, "avg_ms": stages[name]}
     ach = nbytes / sec / 1e9
     return {"kernel": name, "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": ach / HBM_PEAK_GBS, "traffic": None, "algorithmic_bytes": nbytes,
+            "frac": ach / HBM_PEAK_GBS, "traffic": traffic, "algorithmic_bytes": nbytes,
             "flops": flops, "avg_ms": stages[name]}
+
+
+def copy_calibration(mb: int) -> dict:
+    """One device-to-device copy of known size through the HIP runtime (its blit
+    copy kernel): that kernel's FETCH_SIZE / WRITE_SIZE counters give the scale
+    that turns the step kernels' counters into HBM bytes."""
+    hip = C.CDLL("libamdhip64.so")
+    n = mb * (1 << 20)
+    src, dst = C.c_void_p(), C.c_void_p()
+    for p in (src, dst):
+        if hip.hipMalloc(C.byref(p), C.c_size_t(n)) != 0:
+            raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(src, 1, C.c_size_t(n))
+    hip.hipDeviceSynchronize()
+    hip.hipMemcpy(dst, src, C.c_size_t(n), 3)  # hipMemcpyDeviceToDevice
+    hip.hipDeviceSynchronize()
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    return {"bytes_read": n, "bytes_written": n}
 
 
 def cpu_baseline(state, budget_s: float) -> dict:
@@ -165,6 +212,10 @@ def main(argv=None) -> int:
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--eager", action="store_true",
+                    help="launch steps one by one instead of replaying HIP graphs (for counter runs)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
+                    help="also run a device-to-device copy of MB megabytes (HBM counter calibration)")
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -173,7 +224,7 @@ def main(argv=None) -> int:
     steps = max(GRAPH_BLOCK, (args.steps // GRAPH_BLOCK) * GRAPH_BLOCK)
     warmup = max(GRAPH_BLOCK, (args.warmup // GRAPH_BLOCK) * GRAPH_BLOCK)
 
-    solver, state, el, stages, diag = run_gpu("fp64", steps, warmup)
+    solver, state, el, stages, diag = run_gpu("fp64", steps, warmup, args.eager)
     value = steps / el
     line = {
         "metric": METRIC,
@@ -200,13 +251,15 @@ def main(argv=None) -> int:
     print(f"# fp64: {value:.1f} steps/s ({1e3 * el / steps:.4f} ms/step)", file=sys.stderr, flush=True)
     solver.close()
     if not args.no_mixed:
-        msolver, _, mel, mstages, mdiag = run_gpu("mixed", steps, warmup)
+        msolver, _, mel, mstages, mdiag = run_gpu("mixed", steps, warmup, args.eager)
         mval = steps / mel
         line["mixed"] = {"value": mval, "unit": "steps/s", "ms_per_step": 1e3 * mel / steps,
                          "ratio_vs_fp64": mval / value, "stages_ms": mstages,
                          "roofline": roofline(msolver, "mixed", mstages)}
         print(f"# mixed: {mval:.1f} steps/s", file=sys.stderr, flush=True)
         msolver.close()
+    if args.copy_calibration:
+        line["copy_calibration"] = copy_calibration(args.copy_calibration)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(state, args.cpu_seconds)
         print(f"# cpu: {line['cpu_baseline']['value']:.3f} steps/s", file=sys.stderr, flush=True)
