@@ -204,6 +204,46 @@ def cpu_baseline(state, budget_s: float) -> dict:
                       f"step (scipy.fft workers={cores}, FP64 oracle)"}
 
 
+def mixed_error(state, steps: int) -> dict:
+    """Per-field error of the mixed-precision step against the FP64 step after
+    ``steps`` steps (startup included) from the same seeded state:
+    max |mixed - fp64| / max |fp64| over each field's coefficients."""
+    import numpy as np
+
+    from jaxfun_amd import PlaneCouetteFluctuationHIP
+
+    w = WORKLOAD
+    out = {}
+    for prec in ("fp64", "mixed"):
+        s = PlaneCouetteFluctuationHIP(N=w["N"], Re=w["Re"], dt=w["dt"], precision=prec)
+        out[prec] = s.solve(state, steps)
+        s.close()
+    ref, mix = out["fp64"], out["mixed"]
+    err = {}
+    for name, a, b in (("u0", mix.u[0], ref.u[0]), ("u1", mix.u[1], ref.u[1]),
+                       ("u2", mix.u[2], ref.u[2]), ("g", mix.g, ref.g)):
+        err[name] = float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+    err["velocity"] = max(err["u0"], err["u1"], err["u2"])
+    err["steps"] = steps
+    return err
+
+
+def pcie_inclusive(state, steps: int) -> dict:
+    """``solve(state, steps)`` through the host boundary: state copied in from
+    host memory, stepped, and copied back (full-complex reference layout)."""
+    from jaxfun_amd import PlaneCouetteFluctuationHIP
+
+    w = WORKLOAD
+    s = PlaneCouetteFluctuationHIP(N=w["N"], Re=w["Re"], dt=w["dt"], precision="fp64")
+    s.solve(state, 2)  # warm the handle (first-touch allocations)
+    t0 = time.perf_counter()
+    s.solve(state, steps)
+    el = time.perf_counter() - t0
+    s.close()
+    return {"value": steps / el, "unit": "steps/s", "steps": steps,
+            "note": "host state in, host state out, startup steps included"}
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -212,6 +252,10 @@ def main(argv=None) -> int:
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--error-steps", type=int, default=1000,
+                    help="steps of the mixed-vs-FP64 per-field error check (0: skip)")
+    ap.add_argument("--pcie-steps", type=int, default=240,
+                    help="steps of the host-boundary (PCIe-inclusive) solve timing (0: skip)")
     ap.add_argument("--eager", action="store_true",
                     help="launch steps one by one instead of replaying HIP graphs (for counter runs)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
@@ -258,6 +302,12 @@ def main(argv=None) -> int:
                          "roofline": roofline(msolver, "mixed", mstages)}
         print(f"# mixed: {mval:.1f} steps/s", file=sys.stderr, flush=True)
         msolver.close()
+        if args.error_steps:
+            line["mixed"]["error_vs_fp64"] = mixed_error(state, args.error_steps)
+            print(f"# mixed error after {args.error_steps} steps: "
+                  f"{line['mixed']['error_vs_fp64']['velocity']:.3e}", file=sys.stderr, flush=True)
+    if args.pcie_steps:
+        line["pcie_inclusive"] = pcie_inclusive(state, args.pcie_steps)
     if args.copy_calibration:
         line["copy_calibration"] = copy_calibration(args.copy_calibration)
     if not args.no_cpu:

@@ -138,7 +138,7 @@ class PlaneCouetteFluctuationHIP:
         handle = C.c_void_p()
         nat.check(self._lib.pcf_create(C.byref(cfg), C.byref(handle)))
         self._h = handle
-        self._loaded = None  # id of the state object resident on the device
+        self._loaded = None  # the state object whose values are resident on the device
 
     # ------------------------------------------------------------ lifecycle
     def close(self) -> None:
@@ -295,19 +295,21 @@ class PlaneCouetteFluctuationHIP:
         st = KMMState(u=u, g=g, nonlinear_old=groups[0], nonlinear_older=groups[1],
                       solution_old=groups[2], solution_older=groups[3],
                       history_steps=hs.value, have_old=ho.value, previous_dt=pd.value)
-        self._loaded = id(st)
+        self._loaded = st
         return st
 
     def _ensure_loaded(self, state: KMMState) -> None:
-        if self._loaded != id(state):
+        if self._loaded is not state:
             self.load(state)
 
     def advance(self, steps: int) -> None:
         """Advance the device-resident state (no host transfer)."""
+        self._loaded = None
         nat.check(self._lib.pcf_step(self._h, int(steps)))
 
     def advance_steady(self, steps: int, graph_block: int = 0) -> None:
         """Steady SBDF3 steps only; with graph_block (multiple of 12) replay a HIP graph."""
+        self._loaded = None
         if graph_block:
             if steps % graph_block:
                 raise ValueError("steps must be a multiple of graph_block")

@@ -23,7 +23,7 @@ Only ``nonlinear_form="rotational"`` with ``coefficient_path="optimized"`` (the
 configuration every published SBDF3 number uses) is restated, for the
 Chebyshev family.
 
-Pins (see ``tests/test_oracle_golden.py``): the reference's published FP64
+Pins (``tests/test_oracle.py``): the reference's published FP64
 diagnostics and full-state checksums after 325/50/20 SBDF3 steps at
 65x64x64/97x96x96/128^3 (``benchmark-results/*.json``), closed-form linear
 decay rates, library DCT/FFT identities, brute-force quadrature of every
