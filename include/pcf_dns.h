@@ -163,6 +163,33 @@ int32_t pcf_stage_count(void);
 const char *pcf_stage_name(int32_t i);
 int32_t pcf_profile_steady(pcf_handle *h, int32_t nsteps, double *stage_ms);
 
+/* Reverse mode.  Replaces jax.value_and_grad / jax.vjp over the scanned,
+ * rematerialised step (production/autograd_cost_benchmark.py:228-330,
+ * src/jaxfun/integrators/cnab2.py:55) for the velocity/vorticity state.
+ *
+ * pcf_vjp advances the current state nsteps steps (exactly as pcf_step) and
+ * then computes the gradient, with respect to every leaf of the state it
+ * started from, of either
+ *   PCF_SEED_ENERGY:    the final perturbation energy Epert (diagnostics out[0],
+ *                       channelflow_kmm.py:1302-1308), or
+ *   PCF_SEED_COTANGENT: the linear functional Re sum(c * u_final) given by the
+ *                       cotangents c of the final U0, U1, U2, G set with
+ *                       pcf_set_cotangent (unset fields count as zero).
+ * The forward is recomputed from state snapshots every checkpoint_every steps
+ * (the reference's jax.checkpoint per step, cnab2.py:55, is checkpoint_every = 1).
+ * On return the handle holds the final state.  Synchronous.
+ *
+ * Gradients and cotangents use jax's convention for real functions of complex
+ * leaves (directional derivative Re sum(g d), examples/pcf_minimal_seed_jax.py:78-94)
+ * in the boundary layout of pcf_get_field.  They are exact on the states this
+ * library represents (real velocity fields: Hermitian spectra with the Nyquist
+ * planes zero); components that no such state can move are returned as zero. */
+#define PCF_SEED_ENERGY 0
+#define PCF_SEED_COTANGENT 1
+int32_t pcf_set_cotangent(pcf_handle *h, int32_t field, const double *host);
+int32_t pcf_vjp(pcf_handle *h, int32_t nsteps, int32_t checkpoint_every, int32_t seed);
+int32_t pcf_get_gradient(pcf_handle *h, int32_t field, double *host);
+
 #ifdef __cplusplus
 }
 #endif

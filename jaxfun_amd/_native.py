@@ -43,7 +43,10 @@ EXPORTS = (
     "pcf_set_field", "pcf_get_field", "pcf_set_history", "pcf_get_history",
     "pcf_step", "pcf_step_steady", "pcf_step_steady_graph", "pcf_diagnostics",
     "pcf_kernels_per_step", "pcf_stage_count", "pcf_stage_name", "pcf_profile_steady",
+    "pcf_set_cotangent", "pcf_vjp", "pcf_get_gradient",
 )
+SEED_ENERGY = 0
+SEED_COTANGENT = 1
 
 
 class PcfConfig(C.Structure):
@@ -109,6 +112,9 @@ def load() -> C.CDLL:
         "pcf_stage_count": ([], C.c_int32),
         "pcf_stage_name": ([C.c_int32], C.c_char_p),
         "pcf_profile_steady": ([h, C.c_int32, dp], C.c_int32),
+        "pcf_set_cotangent": ([h, C.c_int32, dp], C.c_int32),
+        "pcf_vjp": ([h, C.c_int32, C.c_int32, C.c_int32], C.c_int32),
+        "pcf_get_gradient": ([h, C.c_int32, dp], C.c_int32),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)

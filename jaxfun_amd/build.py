@@ -9,6 +9,7 @@ import os
 import shutil
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 from pathlib import Path
 
 PKG = Path(__file__).resolve().parent
@@ -38,15 +39,24 @@ def _hipcc() -> str:
 def build(verbose: bool = False) -> Path:
     hipcc = _hipcc()
     BUILD.mkdir(exist_ok=True)
-    objs = []
-    for src in ("pcf_setup.cpp", "pcf_kernels.hip", "pcf_api.hip"):
+    srcs = ("pcf_setup.cpp", "pcf_kernels.hip", "pcf_adjoint.hip", "pcf_api.hip")
+    cmds, objs = [], []
+    for src in srcs:
         obj = BUILD / (Path(src).stem + ".o")
         cmd = [hipcc, "-c", str(CSRC / src), "-o", str(obj)]
         cmd += HIP_FLAGS if src.endswith(".hip") else ["-O3", "-std=c++17", "-fPIC"]
         if verbose:
             print(" ".join(cmd), flush=True)
-        subprocess.run(cmd, check=True)
+        cmds.append(cmd)
         objs.append(str(obj))
+    # independent translation units: compile side by side
+    with ThreadPoolExecutor(max_workers=len(cmds)) as pool:
+        for res in pool.map(lambda c: subprocess.run(c, capture_output=True, text=True), cmds):
+            if res.returncode:
+                sys.stderr.write(res.stdout + res.stderr)
+                raise subprocess.CalledProcessError(res.returncode, res.args)
+            elif verbose and res.stderr:
+                sys.stderr.write(res.stderr)
     tmp = LIB.with_suffix(".so.tmp")
     cmd = [hipcc, "-shared", f"--offload-arch={ARCH}", "-o", str(tmp), *objs]
     if verbose:

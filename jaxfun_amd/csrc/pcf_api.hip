@@ -17,6 +17,7 @@
 #include <vector>
 
 #include "../../include/pcf_dns.h"
+#include "pcf_adjoint.hpp"
 #include "pcf_kernels.hpp"
 #include "pcf_setup.hpp"
 
@@ -113,6 +114,32 @@ struct pcf_handle {
          *dxn = nullptr, *ddiag = nullptr;
   std::map<int, hipGraphExec_t> graphs;
   StageTimer *timer = nullptr;  // non-null while pcf_profile_steady runs
+
+  // Reverse mode, set up by the first pcf_vjp.
+  struct Snap {
+    char *dev = nullptr;
+    int so[3], no[4];
+    double hs, ho, pd;
+  };
+  struct Adj {
+    bool ready = false;
+    pcf::StructMats smt{};        // transposed, index-reversed parity blocks
+    double *RcT = nullptr;        // parity-compressed R^T rows, [nb][ldrT]
+    int ldrT = 0;
+    double *QB = nullptr, *QD = nullptr;  // Fejer-weighted Gram matrices of the bases
+    pcf::AdjMats<double> ad{};
+    pcf::AdjMats<float> af{};
+    pcf::Prim muX{}, rho[3]{}, rs[3]{}, gh[4]{};
+    double2 *mu_u[3]{};
+    double2 *cot_u[3]{}, *cot_g = nullptr;
+    bool cot_set[4] = {false, false, false, false};
+    int rr[3] = {0, 1, 2};        // rho ring: rho_k, rho_k+1, rho_k+2
+    std::vector<void *> tape;     // padded y-line fields of each nonlinear evaluation
+    std::vector<Snap> snaps;      // state every `every` steps, then the final state
+    size_t snap_bytes = 0;
+    bool have_grad = false;
+  } adj;
+  std::vector<void *> adj_allocs;  // freed on destroy or when the tape grows
 
   template <typename T>
   int32_t alloc(T **p, size_t count) {
@@ -247,11 +274,16 @@ hipError_t nonlinear_stages(pcf_handle *h, const pcf::NodalMats<T> &nm, const do
 }
 
 // Nonlinear rows of the state whose wall-normal velocity is u0 (with the
-// handle's u1/u2) into nonlinear slot `slot`.
-int32_t nonlinear_into(pcf_handle *h, const double2 *u0, int slot) {
+// handle's u1/u2) into nonlinear slot `slot`.  With ytape set, the padded
+// y-line fields of u and curl u are written there (and kept) instead of the
+// workspace: the reverse pass reads them back.
+int32_t nonlinear_into(pcf_handle *h, const double2 *u0, int slot, void *ytape = nullptr) {
+  void *saved = h->ws.Y;
+  if (ytape) h->ws.Y = ytape;
   hipError_t e = (h->cfg.precision == PCF_PREC_FP64)
                      ? nonlinear_stages<double>(h, h->md.view, u0, slot)
                      : nonlinear_stages<float>(h, h->mf.view, u0, slot);
+  h->ws.Y = saved;
   if (e != hipSuccess) return fail(PCF_E_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
   return PCF_OK;
 }
@@ -308,10 +340,10 @@ void rotate(pcf_handle *h) {
 }
 
 // One steady SBDF3 step (sbdf3.py:44-62; channelflow_kmm.py:1016-1064).
-int32_t steady_step(pcf_handle *h) {
+int32_t steady_step(pcf_handle *h, void *const *tape = nullptr) {
   const double dt = h->cfg.dt;
   const int A = h->no[0];
-  int32_t r = nonlinear_into(h, h->u0[h->so[0]], A);
+  int32_t r = nonlinear_into(h, h->u0[h->so[0]], A, tape ? tape[0] : nullptr);
   if (r) return r;
   pcf::SolveArgs a = base_args(h, h->so[2]);
   set_x(a, 0, h, h->so[0], 18.0 / 11.0, 0.0);
@@ -335,18 +367,18 @@ int32_t steady_step(pcf_handle *h) {
 // One IMEXRK3 startup step (channelflow_kmm.py:872-935 inside the SBDF3
 // wrapper :937-1014).  The stage-0 nonlinear rows are the step's current
 // nonlinear rows, which the history shift keeps.
-int32_t startup_step(pcf_handle *h) {
+int32_t startup_step(pcf_handle *h, void *const *tape = nullptr) {
   const double dt = h->cfg.dt;
   const int A = h->no[0], B = h->no[3], C = h->no[2];
   const int cur = h->so[0], out = h->so[2];
-  int32_t r = nonlinear_into(h, h->u0[cur], A);
+  int32_t r = nonlinear_into(h, h->u0[cur], A, tape ? tape[0] : nullptr);
   if (r) return r;
   const int nslot[3] = {A, B, C};
   for (int rk = 0; rk < 3; ++rk) {
     const double gamma = (IMEX_A[rk] + IMEX_B[rk]) * dt / 2.0;
     const int xs = (rk == 0) ? cur : out;
     if (rk > 0) {
-      r = nonlinear_into(h, h->u0[out], nslot[rk]);
+      r = nonlinear_into(h, h->u0[out], nslot[rk], tape ? tape[rk] : nullptr);
       if (r) return r;
     }
     pcf::SolveArgs a = base_args(h, out);
@@ -426,6 +458,367 @@ bool field_is_vector(int f) {
   return f == PCF_F_NL_OLD_V || f == PCF_F_NL_OLD_W || f == PCF_F_NL_OLDER_V ||
          f == PCF_F_NL_OLDER_W || f == PCF_F_SOL_OLD_V || f == PCF_F_SOL_OLD_W ||
          f == PCF_F_SOL_OLDER_V || f == PCF_F_SOL_OLDER_W;
+}
+
+
+// ----------------------------------------------------------------------------
+// Reverse mode.  Cotangents mu are gradients for Re sum conj(a) b in the
+// Hermitian half layout; the exported gradient is jax's convention conj(mu)
+// expanded to the full spectrum (examples/pcf_minimal_seed_jax.py:78-94).
+//
+// Levels: x_k = (u0, g, v00, w00) of state k, N_k its nonlinear rows.  A steady
+// step k solves S x_{k+1} = sum_j c_j M x_{k-j} + sum_j e_j N_{k-j}; its
+// transpose gives rho_k = S^{-T} mu_{x_{k+1}}, and then
+//   mu_{N_k} = e_0 rho_k + e_1 rho_{k+1} + e_2 rho_{k+2},
+//   mu_{x_k} = M^T (c_0 rho_k + c_1 rho_{k+1} + c_2 rho_{k+2}) + R^T F'(u_k)^T mu_{N_k},
+// with rho_k = 0 for the IMEXRK3 start-up steps, whose three stages are
+// reversed one by one.  The forward is recomputed segment by segment from
+// state snapshots taken every `every` steps; the recompute keeps the padded
+// y-line fields of every nonlinear evaluation (the tape), so the reverse
+// never repeats the wall-normal transform.
+
+const double C_SBDF[3] = {18.0 / 11.0, -9.0 / 11.0, 2.0 / 11.0};
+
+std::vector<std::pair<void *, size_t>> state_pieces(pcf_handle *h) {
+  const pcf::Geo &g = h->geo;
+  const size_t M = g.m, c = sizeof(double2), r = sizeof(double);
+  std::vector<std::pair<void *, size_t>> p;
+  for (int s = 0; s < 3; ++s) {
+    p.push_back({h->u0[s], g.nb * M * c});
+    p.push_back({h->g[s], g.nd * M * c});
+    p.push_back({h->v[s], g.nd * r});
+    p.push_back({h->w[s], g.nd * r});
+  }
+  for (int s = 0; s < 4; ++s) {
+    p.push_back({h->Nu[s], g.nb * M * c});
+    p.push_back({h->Ng[s], g.nd * M * c});
+    p.push_back({h->Nv[s], g.nd * r});
+    p.push_back({h->Nw[s], g.nd * r});
+  }
+  p.push_back({h->u1, g.nd * M * c});
+  p.push_back({h->u2, g.nd * M * c});
+  return p;
+}
+
+hipError_t snap_save(pcf_handle *h, pcf_handle::Snap &sn) {
+  size_t off = 0;
+  for (auto &pc : state_pieces(h)) {
+    hipError_t e = hipMemcpyAsync(sn.dev + off, pc.first, pc.second, hipMemcpyDeviceToDevice, h->stream);
+    if (e != hipSuccess) return e;
+    off += (pc.second + 255) / 256 * 256;
+  }
+  std::copy(h->so, h->so + 3, sn.so);
+  std::copy(h->no, h->no + 4, sn.no);
+  sn.hs = h->history_steps;
+  sn.ho = h->have_old;
+  sn.pd = h->previous_dt;
+  return hipSuccess;
+}
+
+hipError_t snap_load(pcf_handle *h, const pcf_handle::Snap &sn) {
+  size_t off = 0;
+  for (auto &pc : state_pieces(h)) {
+    hipError_t e = hipMemcpyAsync(pc.first, sn.dev + off, pc.second, hipMemcpyDeviceToDevice, h->stream);
+    if (e != hipSuccess) return e;
+    off += (pc.second + 255) / 256 * 256;
+  }
+  std::copy(sn.so, sn.so + 3, h->so);
+  std::copy(sn.no, sn.no + 4, h->no);
+  h->history_steps = sn.hs;
+  h->have_old = sn.ho;
+  h->previous_dt = sn.pd;
+  return hipSuccess;
+}
+
+template <typename T>
+int32_t adj_alloc(pcf_handle *h, T **p, size_t count) {
+  void *q = nullptr;
+  hipError_t e = hipMalloc(&q, std::max<size_t>(count, 1) * sizeof(T));
+  if (e != hipSuccess) return fail(PCF_E_HIP, std::string("hipMalloc (adjoint): ") + hipGetErrorString(e));
+  h->adj_allocs.push_back(q);
+  *p = static_cast<T *>(q);
+  return PCF_OK;
+}
+
+int32_t alloc_prim(pcf_handle *h, pcf::Prim &p) {
+  const pcf::Geo &g = h->geo;
+  int32_t r;
+  if ((r = h->alloc(&p.u0, (size_t)g.nb * g.m))) return r;
+  if ((r = h->alloc(&p.g, (size_t)g.nd * g.m))) return r;
+  if ((r = h->alloc(&p.v, (size_t)g.nd))) return r;
+  return h->alloc(&p.w, (size_t)g.nd);
+}
+
+pcf::PrimC cprim(const pcf::Prim &p) { return pcf::PrimC{p.u0, p.g, p.v, p.w}; }
+
+hipError_t zero_prim(pcf_handle *h, const pcf::Prim &p) {
+  const pcf::Geo &g = h->geo;
+  hipError_t e;
+  if ((e = hipMemsetAsync(p.u0, 0, (size_t)g.nb * g.m * sizeof(double2), h->stream))) return e;
+  if ((e = hipMemsetAsync(p.g, 0, (size_t)g.nd * g.m * sizeof(double2), h->stream))) return e;
+  if ((e = hipMemsetAsync(p.v, 0, (size_t)g.nd * sizeof(double), h->stream))) return e;
+  return hipMemsetAsync(p.w, 0, (size_t)g.nd * sizeof(double), h->stream);
+}
+
+template <typename T>
+int32_t build_adj_mats(pcf_handle *h, pcf::AdjMats<T> &am) {
+  const pcf::Operators &o = h->ops;
+  const int ldx = (o.n0 + 7) / 8 * 8, ldi = (o.nd + 7) / 8 * 8;
+  auto pad = [](const pcf::Mat &Mx, int ld) {  // row-major [rows][ld]
+    std::vector<T> t((size_t)Mx.rows * ld, (T)0);
+    for (int i = 0; i < Mx.rows; ++i)
+      for (int j = 0; j < Mx.cols; ++j) t[(size_t)i * ld + j] = (T)Mx(i, j);
+    return t;
+  };
+  T *d = nullptr;
+  int32_t r;
+  if ((r = h->upload(&d, pad(o.A1, ldx)))) return r;
+  am.A1 = d;
+  if ((r = h->upload(&d, pad(o.A0, ldx)))) return r;
+  am.A0 = d;
+  if ((r = h->upload(&d, pad(o.AG, ldx)))) return r;
+  am.AG = d;
+  am.ldx = ldx;
+  if ((r = h->upload(&d, pad(o.VB, ldi)))) return r;
+  am.VB = d;
+  if ((r = h->upload(&d, pad(o.VD, ldi)))) return r;
+  am.VD = d;
+  if ((r = h->upload(&d, pad(o.VD1, ldi)))) return r;
+  am.VD1 = d;
+  am.ldi = ldi;
+  return PCF_OK;
+}
+
+// One-time reverse-mode set-up: transposed structures, R^T, energy Gram
+// matrices, contraction matrices and cotangent buffers.
+int32_t adj_setup(pcf_handle *h) {
+  if (h->adj.ready) return PCF_OK;
+  const pcf::Operators &o = h->ops;
+  const pcf::Geo &g = h->geo;
+  auto &A = h->adj;
+  int32_t r;
+  if (!pcf::rot_adj_fits(g, h->pz, h->cfg.precision == PCF_PREC_FP64 ? 16 : 8))
+    return fail(PCF_E_UNSUPPORTED, "padded spanwise grid too large for the cross-product adjoint");
+  // A' = J A^T J per parity block: same band offsets, b'[i][d+P] = b[n-1-i-d][d+P],
+  // far factors U'[i] = V[n-1-i], V'[j] = U[n-1-j].
+  for (int b = 0; b < 2; ++b)
+    for (int p = 0; p < 2; ++p) {
+      const pcf::Operators::Struct &st = o.st[b][p];
+      const int n = st.n, BW = st.P + st.Q + 1, R = st.R;
+      std::vector<double> bt[3];
+      const std::vector<double> *src[3] = {&st.b0, &st.b2, &st.b4};
+      for (int t = 0; t < 3; ++t) {
+        bt[t].assign((size_t)n * BW, 0.0);
+        for (int i = 0; i < n; ++i)
+          for (int d = -st.P; d <= st.Q; ++d) {
+            const int row = n - 1 - i - d;
+            if (row < 0 || row >= n) continue;
+            bt[t][(size_t)i * BW + d + st.P] = (*src[t])[(size_t)row * BW + d + st.P];
+          }
+      }
+      std::vector<double> Ut((size_t)n * R), Vt((size_t)n * R);
+      for (int i = 0; i < n; ++i)
+        for (int q = 0; q < R; ++q) {
+          Ut[(size_t)i * R + q] = st.V[(size_t)(n - 1 - i) * R + q];
+          Vt[(size_t)i * R + q] = st.U[(size_t)(n - 1 - i) * R + q];
+        }
+      pcf::StructSys &ss = A.smt.sys[b][p];
+      ss.n = n;
+      double *d = nullptr;
+      if ((r = h->upload(&d, bt[0]))) return r;
+      ss.b0 = d;
+      if ((r = h->upload(&d, bt[1]))) return r;
+      ss.b2 = d;
+      if ((r = h->upload(&d, bt[2]))) return r;
+      ss.b4 = d;
+      if ((r = h->upload(&d, Ut))) return r;
+      ss.U = d;
+      if ((r = h->upload(&d, Vt))) return r;
+      ss.V = d;
+    }
+  // RcT[j][k] = R[2k + 1 - (j & 1)][j]
+  A.ldrT = (o.nd + 2) / 2;
+  std::vector<double> RcT((size_t)o.nb * A.ldrT, 0.0);
+  for (int j = 0; j < o.nb; ++j) {
+    const int i0 = 1 - (j & 1);
+    for (int k = 0; i0 + 2 * k < o.nd; ++k) RcT[(size_t)j * A.ldrT + k] = o.R(i0 + 2 * k, j);
+  }
+  if ((r = h->upload(&A.RcT, RcT))) return r;
+  // Q = V^T diag(fejer) V for the clamped and Dirichlet bases
+  auto gram = [&](const pcf::Mat &Vm) {
+    std::vector<double> Q((size_t)Vm.cols * Vm.cols, 0.0);
+    for (int i = 0; i < Vm.cols; ++i)
+      for (int j = 0; j < Vm.cols; ++j) {
+        double s = 0;
+        for (int x = 0; x < Vm.rows; ++x) s += Vm(x, i) * o.fejer[x] * Vm(x, j);
+        Q[(size_t)i * Vm.cols + j] = s;
+      }
+    return Q;
+  };
+  if ((r = h->upload(&A.QB, gram(o.VB)))) return r;
+  if ((r = h->upload(&A.QD, gram(o.VD)))) return r;
+  if (h->cfg.precision == PCF_PREC_FP64) {
+    if ((r = build_adj_mats<double>(h, A.ad))) return r;
+  } else {
+    if ((r = build_adj_mats<float>(h, A.af))) return r;
+  }
+  if ((r = alloc_prim(h, A.muX))) return r;
+  for (int i = 0; i < 3; ++i)
+    if ((r = alloc_prim(h, A.rho[i])) || (r = alloc_prim(h, A.rs[i]))) return r;
+  for (int i = 0; i < 4; ++i)
+    if ((r = alloc_prim(h, A.gh[i]))) return r;
+  for (int i = 0; i < 3; ++i)
+    if ((r = h->alloc(&A.mu_u[i], (size_t)(i == 0 ? g.nb : g.nd) * g.m))) return r;
+  for (int i = 0; i < 3; ++i)
+    if ((r = h->alloc(&A.cot_u[i], (size_t)(i == 0 ? g.nb : g.nd) * g.m))) return r;
+  if ((r = h->alloc(&A.cot_g, (size_t)g.nd * g.m))) return r;
+  A.snap_bytes = 0;
+  for (auto &pc : state_pieces(h)) A.snap_bytes += (pc.second + 255) / 256 * 256;
+  A.ready = true;
+  return PCF_OK;
+}
+
+// Tape slots for `every` steps and snapshots for `steps` steps.
+int32_t adj_reserve(pcf_handle *h, int steps, int every) {
+  auto &A = h->adj;
+  const pcf::Geo &g = h->geo;
+  const size_t ve = (h->cfg.precision == PCF_PREC_FP64) ? sizeof(double2) : sizeof(float2);
+  const size_t slot_bytes = 6 * (size_t)g.n0 * g.my * g.kzs * ve;
+  const size_t need_tape = (size_t)std::min(every, steps) + 4;  // start-up steps evaluate 3 times
+  const size_t need_snap = (size_t)((steps + every - 1) / every) + 1;
+  int32_t r;
+  while (A.tape.size() < need_tape) {
+    unsigned char *p = nullptr;
+    if ((r = adj_alloc(h, &p, slot_bytes))) return r;
+    A.tape.push_back(p);
+  }
+  while (A.snaps.size() < need_snap) {
+    pcf_handle::Snap sn;
+    if ((r = adj_alloc(h, &sn.dev, A.snap_bytes))) return r;
+    A.snaps.push_back(sn);
+  }
+  return PCF_OK;
+}
+
+template <typename T>
+hipError_t nonlinear_adjoint(pcf_handle *h, const pcf::NodalMats<T> &nm, const pcf::AdjMats<T> &am,
+                             const pcf::PrimSum &nu, const void *ytape) {
+  const pcf::Geo &g = h->geo;
+  auto &A = h->adj;
+  hipError_t e;
+  if ((e = pcf::launch_rhs_adj<T>(g, am, nu, h->ws, h->stream))) return e;
+  if ((e = pcf::launch_fft_y_fields<T>(g, nm, h->py, true, 3, h->ws.Hx, h->ws.Hz, h->stream))) return e;
+  if ((e = pcf::launch_rot_adj<T>(g, nm, ytape, h->ws, h->pz, h->stream))) return e;
+  if ((e = pcf::launch_fft_y_fields<T>(g, nm, h->py, false, 6, h->ws.Y, h->ws.X, h->stream))) return e;
+  return pcf::launch_nodal_adj<T>(g, am, h->ws, A.mu_u[0], A.mu_u[1], A.mu_u[2], h->stream);
+}
+
+hipError_t nl_adj(pcf_handle *h, const pcf::PrimSum &nu, const void *ytape) {
+  return (h->cfg.precision == PCF_PREC_FP64) ? nonlinear_adjoint<double>(h, h->md.view, h->adj.ad, nu, ytape)
+                                             : nonlinear_adjoint<float>(h, h->mf.view, h->adj.af, nu, ytape);
+}
+
+void ps_add(pcf::PrimSum &s, const pcf::Prim &p, double w) {
+  s.t[s.n] = cprim(p);
+  s.wgt[s.n] = w;
+  ++s.n;
+}
+
+// mu_{x_k} from the nonlinear-adjoint output: R^T mu_u, or zero when state k
+// is the initial state (its velocity leaves keep mu_u).
+hipError_t recon_or_zero(pcf_handle *h, bool leaves) {
+  auto &A = h->adj;
+  if (leaves) return zero_prim(h, A.muX);
+  return pcf::launch_recon_adj(h->geo, A.RcT, A.ldrT, A.mu_u[0], A.mu_u[1], A.mu_u[2], nullptr, A.muX,
+                               h->stream);
+}
+
+// Reverse of step k (state k -> k+1); mu_{x_{k+1}} in muX on entry, mu_{x_k}
+// on exit.  tape[0..2] are the step's nonlinear evaluations.
+hipError_t reverse_step(pcf_handle *h, int k, bool steady, void *const *tape) {
+  auto &A = h->adj;
+  const pcf::Geo &g = h->geo;
+  const double dt = h->cfg.dt;
+  const double e[3] = {dt * 18.0 / 11.0, -dt * 18.0 / 11.0, dt * 6.0 / 11.0};
+  int *rr = A.rr;
+  {
+    const int t = rr[2];
+    rr[2] = rr[1];
+    rr[1] = rr[0];
+    rr[0] = t;  // rr[0] now receives rho_k
+  }
+  hipError_t er;
+  if (steady) {
+    if ((er = pcf::launch_adj_solve(g, A.smt, dt * 6.0 / 11.0, cprim(A.muX), A.rho[rr[0]], h->scratch,
+                                    h->stream)))
+      return er;
+    pcf::PrimSum nu{};
+    for (int j = 0; j < 3; ++j) ps_add(nu, A.rho[rr[j]], e[j]);
+    if ((er = nl_adj(h, nu, tape[0]))) return er;
+    if ((er = recon_or_zero(h, k == 0))) return er;
+    pcf::PrimSum ga{};
+    for (int j = 0; j < 3; ++j) ps_add(ga, A.rho[rr[j]], C_SBDF[j]);
+    return pcf::launch_adj_mass(g, h->sm, 0.0, pcf::PrimC{}, ga, A.muX, h->stream);
+  }
+  if ((er = zero_prim(h, A.rho[rr[0]]))) return er;
+  for (int s = 2; s >= 0; --s) {
+    const double gamma = (IMEX_A[s] + IMEX_B[s]) * dt / 2.0;
+    if ((er = pcf::launch_adj_solve(g, A.smt, gamma, cprim(A.muX), A.rs[s], h->scratch, h->stream)))
+      return er;
+    pcf::PrimSum nu{};
+    ps_add(nu, A.rs[s], dt * IMEX_A[s]);
+    if (s < 2) ps_add(nu, A.rs[s + 1], dt * IMEX_B[s + 1]);
+    if (s == 0) {
+      ps_add(nu, A.rho[rr[1]], e[1]);
+      ps_add(nu, A.rho[rr[2]], e[2]);
+    }
+    if ((er = nl_adj(h, nu, tape[s]))) return er;
+    if ((er = recon_or_zero(h, s == 0 && k == 0))) return er;
+    pcf::PrimSum ga{};
+    if (s == 0) {
+      ps_add(ga, A.rho[rr[1]], C_SBDF[1]);
+      ps_add(ga, A.rho[rr[2]], C_SBDF[2]);
+    }
+    if ((er = pcf::launch_adj_mass(g, h->sm, gamma, cprim(A.rs[s]), ga, A.muX, h->stream))) return er;
+  }
+  return hipSuccess;
+}
+
+// Cotangent of a full-spectrum jax gradient g into the half layout:
+// mu(k) = (conj g(k) + g(-k)) / 2 for every stored k.
+void half_from_cotangent(const pcf::Geo &g, int rows, const double *full, std::vector<double2> &half) {
+  half.assign((size_t)rows * g.m, double2{0, 0});
+  for (int n = 0; n < rows; ++n)
+    for (int iy = 0; iy < g.ny; ++iy)
+      for (int kz = 0; kz < g.kzs; ++kz) {
+        const int my_ = (g.ny - iy) % g.ny, mz_ = (g.nz - kz) % g.nz;
+        const double *a = full + (((size_t)n * g.ny + iy) * g.nz + kz) * 2;
+        const double *b = full + (((size_t)n * g.ny + my_) * g.nz + mz_) * 2;
+        half[((size_t)n * g.ny + iy) * g.kzs + kz] = double2{0.5 * (a[0] + b[0]), 0.5 * (-a[1] + b[1])};
+      }
+}
+
+// Half-layout cotangent -> jax gradient on the full spectrum: conj, then the
+// Hermitian expansion; spanwise Nyquist plane zero.
+void full_from_half_grad(const pcf::Geo &g, int rows, const std::vector<double2> &half, double *full) {
+  for (int n = 0; n < rows; ++n)
+    for (int iy = 0; iy < g.ny; ++iy) {
+      double *row = full + ((size_t)n * g.ny + iy) * g.nz * 2;
+      const double2 *hr = half.data() + ((size_t)n * g.ny + iy) * g.kzs;
+      const double2 *mr = half.data() + ((size_t)n * g.ny + (g.ny - iy) % g.ny) * g.kzs;
+      for (int kz = 0; kz < g.nz; ++kz) {
+        double re = 0, im = 0;
+        if (kz < g.kzs) {
+          re = hr[kz].x;
+          im = -hr[kz].y;
+        } else if (kz > g.kzs) {
+          re = mr[g.nz - kz].x;
+          im = mr[g.nz - kz].y;
+        }
+        row[2 * kz] = re;
+        row[2 * kz + 1] = im;
+      }
+    }
 }
 
 }  // namespace
@@ -551,6 +944,7 @@ int32_t pcf_create(const pcf_config *cfg, pcf_handle **out) {
 
 int32_t pcf_destroy(pcf_handle *h) {
   if (!h) return PCF_OK;
+  for (void *p : h->adj_allocs) (void)hipFree(p);
   for (auto &kv : h->graphs) (void)hipGraphExecDestroy(kv.second);
   for (void *p : h->allocs) (void)hipFree(p);
   if (h->own_stream && h->stream) (void)hipStreamDestroy(h->stream);
@@ -777,6 +1171,174 @@ int32_t pcf_profile_steady(pcf_handle *h, int32_t nsteps, double *stage_ms) {
     for (int j = 0; j < 2; ++j) (void)hipEventDestroy(t.ev[i][j]);
   if (r) return r;
   for (int i = 0; i < NSTAGE; ++i) stage_ms[i] = acc[i] / nsteps;
+  return PCF_OK;
+}
+
+int32_t pcf_set_cotangent(pcf_handle *h, int32_t field, const double *host) {
+  if (!h || !host) return fail(PCF_E_INVALID, "null argument");
+  int idx;
+  switch (field) {
+    case PCF_F_U0: idx = 0; break;
+    case PCF_F_U1: idx = 1; break;
+    case PCF_F_U2: idx = 2; break;
+    case PCF_F_G: idx = 3; break;
+    default: return fail(PCF_E_INVALID, "cotangents are taken for U0, U1, U2 and G only");
+  }
+  PCF_HIP(hipSetDevice(h->device));
+  int32_t r = adj_setup(h);
+  if (r) return r;
+  const pcf::Geo &g = h->geo;
+  const int rows = (idx == 0) ? g.nb : g.nd;
+  std::vector<double2> half;
+  half_from_cotangent(g, rows, host, half);
+  double2 *dst = (idx < 3) ? h->adj.cot_u[idx] : h->adj.cot_g;
+  PCF_HIP(hipStreamSynchronize(h->stream));
+  PCF_HIP(hipMemcpy(dst, half.data(), half.size() * sizeof(double2), hipMemcpyHostToDevice));
+  h->adj.cot_set[idx] = true;
+  return PCF_OK;
+}
+
+int32_t pcf_vjp(pcf_handle *h, int32_t steps, int32_t checkpoint_every, int32_t seed) {
+  if (!h) return fail(PCF_E_INVALID, "null handle");
+  if (steps < 1) return fail(PCF_E_INVALID, "steps must be >= 1");
+  if (checkpoint_every < 1) return fail(PCF_E_INVALID, "checkpoint_every must be >= 1");
+  if (seed != PCF_SEED_ENERGY && seed != PCF_SEED_COTANGENT) return fail(PCF_E_INVALID, "unknown seed");
+  if (seed == PCF_SEED_COTANGENT &&
+      !(h->adj.cot_set[0] || h->adj.cot_set[1] || h->adj.cot_set[2] || h->adj.cot_set[3]))
+    return fail(PCF_E_INVALID, "no cotangent set (pcf_set_cotangent)");
+  PCF_HIP(hipSetDevice(h->device));
+  int32_t r;
+  if ((r = adj_setup(h))) return r;
+  if ((r = adj_reserve(h, steps, checkpoint_every))) return r;
+  auto &A = h->adj;
+  const pcf::Geo &g = h->geo;
+  const int every = checkpoint_every;
+  const int nseg = (steps + every - 1) / every;
+  const int h0 = (int)std::floor(std::min(2.0, std::max(0.0, h->history_steps)));
+  const double dt = h->cfg.dt;
+  A.have_grad = false;
+  auto hip_fail = [&](hipError_t e, const char *what) {
+    return fail(PCF_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
+  };
+  hipError_t e;
+  // forward, keeping a snapshot at the start of every segment
+  for (int k = 0; k < steps; ++k) {
+    if (k % every == 0 && (e = snap_save(h, A.snaps[k / every]))) return hip_fail(e, "snapshot");
+    r = (h->history_steps < 2.0) ? startup_step(h) : steady_step(h);
+    if (r) return r;
+  }
+  pcf_handle::Snap &fin = A.snaps[nseg];
+  if ((e = snap_save(h, fin))) return hip_fail(e, "snapshot");
+  // seed: mu of the final state's (u, g) -> mu_{x_K}
+  if (seed == PCF_SEED_ENERGY) {
+    const double scale = 2.0 * h->cfg.ly * h->cfg.lz;
+    if ((e = pcf::launch_energy_seed(g, A.QB, A.QD, scale, h->u0[h->so[0]], h->u1, h->u2, A.mu_u[0],
+                                     A.mu_u[1], A.mu_u[2], h->stream)))
+      return hip_fail(e, "energy seed");
+    e = pcf::launch_recon_adj(g, A.RcT, A.ldrT, A.mu_u[0], A.mu_u[1], A.mu_u[2], nullptr, A.muX,
+                              h->stream);
+  } else {
+    e = pcf::launch_recon_adj(g, A.RcT, A.ldrT, A.cot_u[0], A.cot_u[1], A.cot_u[2], A.cot_g, A.muX,
+                              h->stream);
+  }
+  if (e) return hip_fail(e, "seed");
+  for (int i = 0; i < 3; ++i) {
+    if ((e = zero_prim(h, A.rho[i]))) return hip_fail(e, "memset");
+    A.rr[i] = i;
+  }
+  // reverse, segment by segment from the last
+  std::vector<int> first;
+  for (int seg = nseg - 1; seg >= 0; --seg) {
+    const int ka = seg * every, kb = std::min(steps, ka + every);
+    if ((e = snap_load(h, A.snaps[seg]))) return hip_fail(e, "restore");
+    first.assign(kb - ka, 0);
+    int slot = 0;
+    for (int k = ka; k < kb; ++k) {
+      const bool st = h->history_steps >= 2.0;
+      first[k - ka] = slot;
+      r = st ? steady_step(h, &A.tape[slot]) : startup_step(h, &A.tape[slot]);
+      if (r) return r;
+      slot += st ? 1 : 3;
+    }
+    for (int k = kb - 1; k >= ka; --k)
+      if ((e = reverse_step(h, k, h0 + k >= 2, &A.tape[first[k - ka]]))) return hip_fail(e, "reverse step");
+  }
+  // history leaves of the initial state: levels -1 and -2
+  {
+    const double e1 = -dt * 18.0 / 11.0, e2 = dt * 6.0 / 11.0;
+    const pcf::Prim &r0 = A.rho[A.rr[0]], &r1 = A.rho[A.rr[1]];
+    pcf::PrimSum s{};
+    for (int i = 0; i < 2; ++i)
+      if ((e = zero_prim(h, A.gh[i]))) return hip_fail(e, "memset");
+    ps_add(s, r0, C_SBDF[1]);
+    ps_add(s, r1, C_SBDF[2]);
+    if ((e = pcf::launch_adj_mass(g, h->sm, 0.0, pcf::PrimC{}, s, A.gh[0], h->stream))) return hip_fail(e, "history");
+    s = pcf::PrimSum{};
+    ps_add(s, r0, C_SBDF[2]);
+    if ((e = pcf::launch_adj_mass(g, h->sm, 0.0, pcf::PrimC{}, s, A.gh[1], h->stream))) return hip_fail(e, "history");
+    s = pcf::PrimSum{};
+    ps_add(s, r0, e1);
+    ps_add(s, r1, e2);
+    if ((e = pcf::launch_prim_sum(g, s, A.gh[2], h->stream))) return hip_fail(e, "history");
+    s = pcf::PrimSum{};
+    ps_add(s, r0, e2);
+    if ((e = pcf::launch_prim_sum(g, s, A.gh[3], h->stream))) return hip_fail(e, "history");
+  }
+  if ((e = snap_load(h, fin))) return hip_fail(e, "restore");
+  PCF_HIP(hipStreamSynchronize(h->stream));
+  A.have_grad = true;
+  return PCF_OK;
+}
+
+int32_t pcf_get_gradient(pcf_handle *h, int32_t field, double *host) {
+  if (!h || !host) return fail(PCF_E_INVALID, "null argument");
+  if (field < 0 || field >= PCF_F_COUNT) return fail(PCF_E_INVALID, "unknown field");
+  auto &A = h->adj;
+  if (!A.have_grad) return fail(PCF_E_INVALID, "no gradient: call pcf_vjp first");
+  const pcf::Geo &g = h->geo;
+  PCF_HIP(hipSetDevice(h->device));
+  PCF_HIP(hipStreamSynchronize(h->stream));
+  const size_t M = g.m;
+  auto fetch = [&](const void *src, size_t count, std::vector<double2> &dst) -> hipError_t {
+    dst.resize(count);
+    return hipMemcpy(dst.data(), src, count * sizeof(double2), hipMemcpyDeviceToHost);
+  };
+  std::vector<double2> half, extra;
+  int rows = g.nd;
+  if (field == PCF_F_U0 || field == PCF_F_U1 || field == PCF_F_U2 || field == PCF_F_G) {
+    if (field == PCF_F_U0) {
+      rows = g.nb;
+      PCF_HIP(fetch(A.mu_u[0], rows * M, half));
+      PCF_HIP(fetch(A.muX.u0, rows * M, extra));
+      for (size_t i = 0; i < half.size(); ++i) {
+        half[i].x += extra[i].x;
+        half[i].y += extra[i].y;
+      }
+    } else if (field == PCF_F_G) {
+      PCF_HIP(fetch(A.muX.g, rows * M, half));
+    } else {
+      const int c = (field == PCF_F_U1) ? 1 : 2;
+      PCF_HIP(fetch(A.mu_u[c], rows * M, half));
+      std::vector<double> vec(rows);
+      PCF_HIP(hipMemcpy(vec.data(), c == 1 ? A.muX.v : A.muX.w, rows * sizeof(double), hipMemcpyDeviceToHost));
+      for (int i = 0; i < rows; ++i) half[(size_t)i * M].x += vec[i];
+    }
+    full_from_half_grad(g, rows, half, host);
+    return PCF_OK;
+  }
+  int grp, comp;
+  if (field >= PCF_F_SOL_OLDER_U) grp = 1, comp = field - PCF_F_SOL_OLDER_U;
+  else if (field >= PCF_F_SOL_OLD_U) grp = 0, comp = field - PCF_F_SOL_OLD_U;
+  else if (field >= PCF_F_NL_OLDER_U) grp = 3, comp = field - PCF_F_NL_OLDER_U;
+  else grp = 2, comp = field - PCF_F_NL_OLD_U;
+  const pcf::Prim &P = A.gh[grp];
+  if (comp >= 2) {
+    PCF_HIP(hipMemcpy(host, comp == 2 ? P.v : P.w, g.nd * sizeof(double), hipMemcpyDeviceToHost));
+    return PCF_OK;
+  }
+  rows = (comp == 0) ? g.nb : g.nd;
+  PCF_HIP(fetch(comp == 0 ? (const void *)P.u0 : (const void *)P.g, rows * M, half));
+  full_from_half_grad(g, rows, half, host);
   return PCF_OK;
 }
 

@@ -20,145 +20,10 @@
 //                      channelflow_kmm.py:855-870)
 #include <algorithm>
 
-#include "pcf_kernels.hpp"
+#include "pcf_adjoint.hpp"
+#include "pcf_device.hpp"
 
 namespace pcf {
-
-template <typename T>
-struct VT;
-template <>
-struct VT<double> {
-  using V = double2;
-};
-template <>
-struct VT<float> {
-  using V = float2;
-};
-
-template <typename V>
-__device__ __forceinline__ V cadd(V a, V b) {
-  return V{a.x + b.x, a.y + b.y};
-}
-template <typename V>
-__device__ __forceinline__ V csub(V a, V b) {
-  return V{a.x - b.x, a.y - b.y};
-}
-template <typename V>
-__device__ __forceinline__ V cmul(V a, V b) {
-  return V{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
-}
-template <typename V>
-__device__ __forceinline__ V cconj(V a) {
-  return V{a.x, -a.y};
-}
-
-// ------------------------------------------------------------------ FFT ----
-// In-place Stockham passes over LDS.  Each thread gathers its butterflies into
-// registers, the block synchronises, then writes the permuted outputs; so one
-// buffer suffices.  Forward = exp(-i...), inverse = exp(+i...) without scaling.
-// Twiddles come from a table tw[t] = (cos, -sin)(2 pi t / n).
-
-template <typename V, typename T, bool INV>
-__device__ __forceinline__ V tw_get(const V *tw, int t) {
-  V w = tw[t];
-  return INV ? V{w.x, -w.y} : w;
-}
-
-template <typename V, typename T, int R, bool INV>
-__device__ __forceinline__ void dft_small(V *a) {
-  if constexpr (R == 2) {
-    V t = a[0];
-    a[0] = cadd(t, a[1]);
-    a[1] = csub(t, a[1]);
-  } else if constexpr (R == 4) {
-    V s02 = cadd(a[0], a[2]), d02 = csub(a[0], a[2]);
-    V s13 = cadd(a[1], a[3]), d13 = csub(a[1], a[3]);
-    // forward: y1 = d02 - i d13, y3 = d02 + i d13
-    V jd13 = INV ? V{-d13.y, d13.x} : V{d13.y, -d13.x};  // (+-i) d13
-    a[0] = cadd(s02, s13);
-    a[2] = csub(s02, s13);
-    a[1] = cadd(d02, jd13);
-    a[3] = csub(d02, jd13);
-  } else if constexpr (R == 3) {
-    const T c = (T)-0.5;
-    const T s = INV ? (T)0.86602540378443864676372317075294 : (T)-0.86602540378443864676372317075294;
-    V t1 = cadd(a[1], a[2]), t2 = csub(a[1], a[2]);
-    V m = V{a[0].x + c * t1.x, a[0].y + c * t1.y};
-    V n = V{-s * t2.y, s * t2.x};  // i s t2
-    a[0] = cadd(a[0], t1);
-    a[1] = cadd(m, n);
-    a[2] = csub(m, n);
-  }
-}
-
-// batch address: base(b) = b * bstride; element e at base + e * estride.
-// BATCH_FAST: consecutive threads take consecutive batches (coalesced for
-// column-interleaved layouts); otherwise consecutive butterflies.
-template <typename V, typename T, int R, bool INV, bool BATCH_FAST, int MAXB>
-__device__ void fft_pass(V *buf, const V *tw, int n, int p, int nbatch, int bstride,
-                         int estride, int tid, int nthr) {
-  const int nbf = n / R;
-  const int total = nbf * nbatch;
-  const int twstep = n / (p * R);
-  V v[MAXB][R];
-#pragma unroll
-  for (int t = 0; t < MAXB; ++t) {
-    int idx = tid + t * nthr;
-    if (idx < total) {
-      int b = BATCH_FAST ? idx % nbatch : idx / nbf;
-      int i = BATCH_FAST ? idx / nbatch : idx % nbf;
-      V *base = buf + (size_t)b * bstride;
-#pragma unroll
-      for (int j = 0; j < R; ++j) v[t][j] = base[(i + j * nbf) * estride];
-    }
-  }
-  __syncthreads();
-#pragma unroll
-  for (int t = 0; t < MAXB; ++t) {
-    int idx = tid + t * nthr;
-    if (idx < total) {
-      int b = BATCH_FAST ? idx % nbatch : idx / nbf;
-      int i = BATCH_FAST ? idx / nbatch : idx % nbf;
-      int k = i % p;
-      if (p > 1) {
-#pragma unroll
-        for (int j = 1; j < R; ++j) v[t][j] = cmul(v[t][j], tw_get<V, T, INV>(tw, j * k * twstep));
-      }
-      dft_small<V, T, R, INV>(v[t]);
-      V *base = buf + (size_t)b * bstride;
-      int o = (i - k) * R + k;
-#pragma unroll
-      for (int j = 0; j < R; ++j) base[(o + j * p) * estride] = v[t][j];
-    }
-  }
-  __syncthreads();
-}
-
-template <typename V, typename T, bool INV, bool BATCH_FAST, int MAXB>
-__device__ void fft_lds(V *buf, const V *tw, const FFTPlan &plan, int nbatch, int bstride,
-                        int estride) {
-  const int tid = threadIdx.x, nthr = blockDim.x;
-  int p = 1;
-  for (int s = 0; s < plan.nst; ++s) {
-    const int r = plan.radix[s];
-    if (r == 4)
-      fft_pass<V, T, 4, INV, BATCH_FAST, MAXB>(buf, tw, plan.n, p, nbatch, bstride, estride, tid, nthr);
-    else if (r == 2)
-      fft_pass<V, T, 2, INV, BATCH_FAST, MAXB>(buf, tw, plan.n, p, nbatch, bstride, estride, tid, nthr);
-    else
-      fft_pass<V, T, 3, INV, BATCH_FAST, MAXB>(buf, tw, plan.n, p, nbatch, bstride, estride, tid, nthr);
-    p *= r;
-  }
-}
-
-// ---------------------------------------------------- wavenumber helpers ----
-__device__ __forceinline__ void mode_k(const Geo &g, int m, double &ky, double &kz, double &ky_odd) {
-  const int iy = m / g.kzs, iz = m - iy * g.kzs;
-  const int kyi = (iy < g.ny / 2) ? iy : iy - g.ny;  // FFT order
-  ky = kyi * g.df1;
-  ky_odd = (iy == g.ny / 2) ? 0.0 : ky;  // odd derivatives drop the Nyquist wavenumber
-  kz = iz * g.df2;
-}
 
 // -------------------------------------------------- k_nodal_backward ----
 // For each mode (lane) and a chunk of XC nodes: U0 = VB u0, U1 = VD u1,
@@ -1440,16 +1305,16 @@ int max_butterflies(const FFTPlan &plan, int nbatch) {
 }
 
 template <typename T>
-hipError_t launch_fft_y(const Geo &g, const NodalMats<T> &nm, Workspace &ws, const FFTPlan &py,
-                        bool inverse, hipStream_t s) {
+hipError_t launch_fft_y_fields(const Geo &g, const NodalMats<T> &nm, const FFTPlan &py, bool inverse,
+                               int nf, const void *src_, void *dst_, hipStream_t s) {
   using V = typename VT<T>::V;
   const int kc = pick_kc(g.kzs);
   const size_t lds = (size_t)g.my * kc * sizeof(V);
   const int mb = max_butterflies(py, kc);
-  const dim3 grid(g.kzs / kc, g.n0, inverse ? 6 : 3);
+  const dim3 grid(g.kzs / kc, g.n0, nf);
   const V *tw = reinterpret_cast<const V *>(nm.twy);
-  const V *src = reinterpret_cast<const V *>(inverse ? ws.X : ws.Hz);
-  V *dst = reinterpret_cast<V *>(inverse ? ws.Y : ws.Hx);
+  const V *src = reinterpret_cast<const V *>(src_);
+  V *dst = reinterpret_cast<V *>(dst_);
 #define PCF_FFTY(INV, MB) hipLaunchKernelGGL((k_fft_y<T, INV, MB>), grid, dim3(256), lds, s, g, tw, py, kc, src, dst)
   if (inverse) {
     if (mb <= 2) PCF_FFTY(true, 2);
@@ -1462,6 +1327,13 @@ hipError_t launch_fft_y(const Geo &g, const NodalMats<T> &nm, Workspace &ws, con
   }
 #undef PCF_FFTY
   return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_fft_y(const Geo &g, const NodalMats<T> &nm, Workspace &ws, const FFTPlan &py,
+                        bool inverse, hipStream_t s) {
+  return inverse ? launch_fft_y_fields<T>(g, nm, py, true, 6, ws.X, ws.Y, s)
+                 : launch_fft_y_fields<T>(g, nm, py, false, 3, ws.Hz, ws.Hx, s);
 }
 
 template <typename T>
@@ -1577,6 +1449,8 @@ bool fft_fits(const Geo &g, const FFTPlan &py, const FFTPlan &pz, int elem_bytes
                                       const double2 *, const double2 *, Workspace &, hipStream_t); \
   template hipError_t launch_fft_y<T>(const Geo &, const NodalMats<T> &, Workspace &,            \
                                       const FFTPlan &, bool, hipStream_t);                        \
+  template hipError_t launch_fft_y_fields<T>(const Geo &, const NodalMats<T> &, const FFTPlan &,   \
+                                             bool, int, const void *, void *, hipStream_t);       \
   template hipError_t launch_rotational<T>(const Geo &, const NodalMats<T> &, Workspace &,       \
                                            const FFTPlan &, hipStream_t);                        \
   template hipError_t launch_rhs<T>(const Geo &, const NodalMats<T> &, Workspace &, double2 *,   \

@@ -354,6 +354,56 @@ class PlaneCouetteFluctuationHIP:
                 "mean_shear")
         return {k: float(v) for k, v in zip(keys, out)}
 
+    # ------------------------------------------------------------ reverse mode
+    def vjp(self, state: KMMState, steps: int, cotangent=None, checkpoint_every: int = 20):
+        """Run ``steps`` steps from ``state`` and pull a cotangent back to it.
+
+        ``cotangent`` is ``(u0, u1, u2, g)`` of the final state in jax's
+        convention for real functions of complex leaves (directional
+        derivative ``Re sum(c * d)``, ``examples/pcf_minimal_seed_jax.py:78-94``);
+        ``None`` differentiates the final perturbation energy ``Epert``
+        (``channelflow_kmm.py:1302-1308``).  The forward is recomputed from
+        snapshots every ``checkpoint_every`` steps.  Returns
+        ``(final_state, gradient)`` with the gradient a ``KMMState`` holding
+        d/d(leaf) of every leaf of ``state`` (history scalars 0.0).
+        """
+        self._ensure_loaded(state)
+        self._loaded = None
+        seed = nat.SEED_ENERGY
+        if cotangent is not None:
+            for name, arr in zip(("U0", "U1", "U2", "G"), cotangent):
+                a = np.ascontiguousarray(np.asarray(arr, dtype=np.complex128))
+                nat.check(self._lib.pcf_set_cotangent(self._h, nat.FIELD_ID[name],
+                                                      a.ctypes.data_as(C.POINTER(C.c_double))))
+            seed = nat.SEED_COTANGENT
+        nat.check(self._lib.pcf_vjp(self._h, int(steps), int(checkpoint_every), seed))
+        grad = self._gradient()
+        return self.fetch(), grad
+
+    def value_and_grad(self, state: KMMState, steps: int, checkpoint_every: int = 20):
+        """Final ``Epert`` and its gradient with respect to the leaves of ``state``
+        (the reference's ``jax.value_and_grad`` of the rolled-out energy,
+        ``production/autograd_cost_benchmark.py:228-330``)."""
+        final, grad = self.vjp(state, steps, None, checkpoint_every)
+        return self.diagnostics()["Epert"], grad, final
+
+    def _gradient(self) -> KMMState:
+        nb, nd = self.nb, self.nd
+
+        def get(name, rows, vector=False):
+            out = np.empty(rows, np.float64) if vector else np.empty((rows, self.N[1], self.N[2]), np.complex128)
+            nat.check(self._lib.pcf_get_gradient(self._h, nat.FIELD_ID[name],
+                                                 out.ctypes.data_as(C.POINTER(C.c_double))))
+            return out
+
+        groups = []
+        for tag in ("NL_OLD", "NL_OLDER", "SOL_OLD", "SOL_OLDER"):
+            groups.append((get(f"{tag}_U", nb), get(f"{tag}_G", nd), get(f"{tag}_V", nd, True),
+                           get(f"{tag}_W", nd, True)))
+        return KMMState(u=(get("U0", nb), get("U1", nd), get("U2", nd)), g=get("G", nd),
+                        nonlinear_old=groups[0], nonlinear_older=groups[1], solution_old=groups[2],
+                        solution_older=groups[3])
+
     @staticmethod
     def benchmark_state_fields(state: KMMState):
         return (*state.u, state.g)

@@ -315,6 +315,17 @@ class OracleAdjoint:
         return out
 
     # ------------------------------------------------------------ objective
+    def vjp(self, s0: OracleState, steps: int, mu_final: OracleState) -> OracleState:
+        """Pull the cotangent of the state after ``steps`` steps back to ``s0``."""
+        o = self.o
+        traj = [s0]
+        for _ in range(int(steps) - 1):
+            traj.append(o.step(traj[-1]))
+        mu = mu_final
+        for k in range(int(steps) - 1, -1, -1):
+            mu = self.step_vjp(traj[k], mu)
+        return mu
+
     def value_and_grad(self, s0: OracleState, steps: int):
         """Epert after ``steps`` steps and its cotangent with respect to every leaf of s0."""
         o = self.o
@@ -327,6 +338,15 @@ class OracleAdjoint:
         for k in range(int(steps) - 1, -1, -1):
             mu = self.step_vjp(traj[k], mu)
         return value, mu
+
+
+def from_jax_convention(g: OracleState) -> OracleState:
+    """Inverse of ``jax_convention`` (conj is an involution)."""
+    return jax_convention(g)
+
+
+def zeros_like_state(s: OracleState) -> OracleState:
+    return _zeros_like_state(s)
 
 
 def _add_tuple(a, b):
