@@ -244,6 +244,80 @@ def pcie_inclusive(state, steps: int) -> dict:
             "note": "host state in, host state out, startup steps included"}
 
 
+# Adjoint workload: gradient of the final perturbation energy after 400 SBDF3
+# steps (IMEXRK3 start-up included) with respect to the seeded initial field,
+# checkpoints every 20 steps.
+ADJOINT = dict(N=(65, 64, 64), Re=1000.0, dt=0.005, energy=1e-4, K=12, seed=0, steps=400, every=20)
+
+
+def adjoint_leg(precision: str, reps: int = 3) -> dict:
+    """Wall time of value + gradient over ADJOINT['steps'] steps (recompute
+    included) against the same forward step sequence, state resident on the GPU."""
+    from jaxfun_amd import PlaneCouetteFluctuationHIP
+    from jaxfun_amd import _native as nat
+    from jaxfun_amd.ic import seeded_state
+
+    a = ADJOINT
+    solver = PlaneCouetteFluctuationHIP(N=a["N"], Re=a["Re"], dt=a["dt"], precision=precision)
+    s0 = seeded_state(solver, seed=a["seed"], energy=a["energy"], K=a["K"])
+    lib, h = solver._lib, solver._h
+
+    def fwd():
+        solver.load(s0)
+        solver.synchronize()
+        t0 = time.perf_counter()
+        solver.advance(a["steps"])
+        solver.synchronize()
+        return time.perf_counter() - t0
+
+    def adj():
+        solver.load(s0)
+        solver.synchronize()
+        t0 = time.perf_counter()
+        nat.check(lib.pcf_vjp(h, a["steps"], a["every"], nat.SEED_ENERGY))
+        return time.perf_counter() - t0
+
+    fwd()
+    adj()  # first call allocates the tape and snapshots
+    tf = sorted(fwd() for _ in range(reps))[reps // 2]
+    ta = sorted(adj() for _ in range(reps))[reps // 2]
+    solver.close()
+    return {"precision": precision, "forward_ms": 1e3 * tf, "value_and_grad_ms": 1e3 * ta, "ratio": ta / tf}
+
+
+def adjoint_fd_check(n_dirs: int = 8, eps: float = 1e-6) -> dict:
+    """FP64 gradient against central differences of the HIP forward along
+    seeded divergence-free directions (same energy as the initial field), and
+    the mixed-precision gradient against the FP64 one."""
+    import numpy as np
+
+    from jaxfun_amd import PlaneCouetteFluctuationHIP
+    from jaxfun_amd.ic import seeded_state
+
+    a = ADJOINT
+    solver = PlaneCouetteFluctuationHIP(N=a["N"], Re=a["Re"], dt=a["dt"], precision="fp64")
+    s0 = seeded_state(solver, seed=a["seed"], energy=a["energy"], K=a["K"])
+    value, grad, _ = solver.value_and_grad(s0, a["steps"], checkpoint_every=a["every"])
+    errs = []
+    for k in range(n_dirs):
+        d = seeded_state(solver, seed=100 + k, energy=a["energy"], K=a["K"])
+
+        def J(sign):
+            s = type(s0)(u=tuple(x + sign * eps * y for x, y in zip(s0.u, d.u)), g=s0.g + sign * eps * d.g)
+            return solver.diagnostics(solver.solve(s, a["steps"]))["Epert"]
+
+        fd = (J(1.0) - J(-1.0)) / (2 * eps)
+        an = float(sum(np.real(np.sum(x * y)) for x, y in zip((*grad.u, grad.g), (*d.u, d.g))))
+        errs.append(abs(an - fd) / abs(fd))
+    solver.close()
+    mixed = PlaneCouetteFluctuationHIP(N=a["N"], Re=a["Re"], dt=a["dt"], precision="mixed")
+    _, gm, _ = mixed.value_and_grad(s0, a["steps"], checkpoint_every=a["every"])
+    mixed.close()
+    gerr = max(float(np.max(np.abs(x - y)) / np.max(np.abs(y))) for x, y in zip((*gm.u, gm.g), (*grad.u, grad.g)))
+    return {"directions": n_dirs, "eps": eps, "max_rel_err": max(errs), "value": value,
+            "mixed_vs_fp64_grad_rel": gerr}
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -254,6 +328,9 @@ def main(argv=None) -> int:
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--error-steps", type=int, default=1000,
                     help="steps of the mixed-vs-FP64 per-field error check (0: skip)")
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--adjoint-only", action="store_true",
+                    help="run only the adjoint timing leg (for kernel profiles)")
     ap.add_argument("--pcie-steps", type=int, default=240,
                     help="steps of the host-boundary (PCIe-inclusive) solve timing (0: skip)")
     ap.add_argument("--eager", action="store_true",
@@ -265,6 +342,9 @@ def main(argv=None) -> int:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
               "(this build has no multi-GPU path)", file=sys.stderr)
         return 2
+    if args.adjoint_only:
+        print(json.dumps({"adjoint": {p: adjoint_leg(p) for p in ("fp64", "mixed")}}), flush=True)
+        return 0
     steps = max(GRAPH_BLOCK, (args.steps // GRAPH_BLOCK) * GRAPH_BLOCK)
     warmup = max(GRAPH_BLOCK, (args.warmup // GRAPH_BLOCK) * GRAPH_BLOCK)
 
@@ -308,6 +388,15 @@ def main(argv=None) -> int:
                   f"{line['mixed']['error_vs_fp64']['velocity']:.3e}", file=sys.stderr, flush=True)
     if args.pcie_steps:
         line["pcie_inclusive"] = pcie_inclusive(state, args.pcie_steps)
+    if not args.no_adjoint:
+        adj = {"workload": f"grad of Epert after {ADJOINT['steps']} steps wrt the seeded initial field, "
+                           f"65x64x64, Re=1000, dt=0.005, E=1e-4, checkpoints every {ADJOINT['every']}",
+               "mixed": adjoint_leg("mixed"), "fp64": adjoint_leg("fp64")}
+        adj["fd_check"] = adjoint_fd_check()
+        line["adjoint"] = adj
+        line["adjoint_forward_ratio"] = adj["mixed"]["ratio"]
+        print(f"# adjoint/forward: mixed {adj['mixed']['ratio']:.3f}, fp64 {adj['fp64']['ratio']:.3f}; "
+              f"FD max rel err {adj['fd_check']['max_rel_err']:.2e}", file=sys.stderr, flush=True)
     if args.copy_calibration:
         line["copy_calibration"] = copy_calibration(args.copy_calibration)
     if not args.no_cpu:

@@ -151,3 +151,41 @@ def test_mixed_precision_gradient_close_to_fp64():
               for a, b in zip(list(g32.u) + [g32.g], list(g64.u) + [g64.g]))
     print("mixed vs fp64 gradient", err, flush=True)
     assert 1e-14 < err < 1e-4
+
+
+def test_torch_autograd_rollout_matches_finite_differences():
+    """torch.autograd over rollout(): a loss mixing several final leaves."""
+    import torch
+
+    from jaxfun_amd.autograd import rollout, rollout_energy
+
+    N, steps = (17, 16, 16), 5
+    solver = make(N)
+    s0 = seeded_state(solver, seed=6, energy=5e-2, K=4)
+    w = seeded_state(solver, seed=7, energy=1.0, K=4).u[1]
+
+    def loss_np(final):
+        return float(np.sum(np.abs(final.u[0]) ** 2) + np.sum(np.real(final.u[1] * np.conj(w)))
+                     + np.sum(np.abs(final.g) ** 2))
+
+    xs = [torch.from_numpy(np.array(a)).requires_grad_() for a in (*s0.u, s0.g)]
+    out = rollout(solver, steps, *xs, checkpoint_every=2)
+    wt = torch.from_numpy(np.array(w))
+    loss = (out[0].abs() ** 2).sum() + (out[1] * wt.conj()).real.sum() + (out[3].abs() ** 2).sum()
+    assert abs(float(loss) - loss_np(make(N).solve(s0, steps))) <= 1e-12 * abs(float(loss))
+    loss.backward()
+    fwd = make(N)
+    d = seeded_state(solver, seed=8, energy=5e-2, K=4)
+    eps = 1e-5
+    fd = (loss_np(fwd.solve(_axpy(s0, d, eps), steps)) - loss_np(fwd.solve(_axpy(s0, d, -eps), steps))) / (2 * eps)
+    # torch's gradient of a real loss is dL/dRe + i dL/dIm: pairs as Re sum(conj(grad) d)
+    an = float(sum(np.real(np.sum(np.conj(x.grad.numpy()) * b)) for x, b in zip(xs, (*d.u, d.g))))
+    print("torch rollout", an, fd, flush=True)
+    assert abs(an - fd) <= 1e-7 * abs(fd)
+
+    ys = [torch.from_numpy(np.array(a)).requires_grad_() for a in (*s0.u, s0.g)]
+    E = rollout_energy(solver, steps, *ys, checkpoint_every=2)
+    (3.0 * E).backward()
+    _, grad, _ = make(N).value_and_grad(s0, steps, checkpoint_every=2)
+    for y, gj in zip(ys, (*grad.u, grad.g)):
+        assert np.array_equal(y.grad.numpy(), 3.0 * np.conj(gj))
