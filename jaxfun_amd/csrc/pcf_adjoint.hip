@@ -8,18 +8,16 @@
 //                    low-rank part becomes far-upper under index reversal)
 //   k_adj_mass       (M + gamma L)^T and the SBDF3 history gather M^T sum c_l rho_l
 //   k_recon_adj      transpose of the velocity reconstruction
-//   k_rhs_adj        transpose of the nonlinear rows (Nu, Ng, Nv00, Nw00)
 //   k_rot_adj        cross-product adjoint between the spanwise FFTs
-//   k_nodal_adj      transpose of the wall-normal transform of u and curl u
 //   k_energy_seed    cotangent of the perturbation energy
+// The transposed wall-normal contractions (nonlinear rows, transform of u and
+// curl u) run on the matrix cores (pcf_mfma.hip).
 #include "pcf_adjoint.hpp"
 #include "pcf_device.hpp"
 
 namespace pcf {
 namespace {
 
-constexpr int IC_ADJ = 8;  // output-row chunk of the contractions
-constexpr int XC_ADJ = 8;  // node chunk of the nonlinear-row adjoint
 
 __device__ __forceinline__ bool nyquist_row(const Geo &g, int m) { return m / g.kzs == g.ny / 2; }
 
@@ -176,8 +174,11 @@ __device__ __forceinline__ SysLane sys_lane(const Geo &g, int t) {
 }
 
 // rho = S^{-T} mu, S = M - gamma L, on the transposed index-reversed blocks.
+// With nu.u0 set, also nu = e0 rho + e1 r1 + e2 r2 (the cotangent of the
+// step's nonlinear rows, r1/r2 the two later rho).
 __global__ void __launch_bounds__(256) k_adj_solve(Geo g, StructMats smt, double gamma, PrimC in,
-                                                   Prim out, double *scratch) {
+                                                   Prim out, double *scratch, Prim nuo, PrimC r1,
+                                                   PrimC r2, double e0, double e1, double e2) {
   const SysLane L = sys_lane(g, blockIdx.x * blockDim.x + threadIdx.x);
   if (!L.valid) return;
   const int M = g.m, m = L.m, par = L.par;
@@ -188,11 +189,20 @@ __global__ void __launch_bounds__(256) k_adj_solve(Geo g, StructMats smt, double
                        : scratch + (size_t)sp * g.npmax * 7 * M;
   const int Ms = L.mean ? 1 : M;
   const int lane = L.mean ? 0 : m;
+  const bool wnu = nuo.u0 != nullptr;
   if (!L.mean && nyquist_row(g, m)) {
     double2 *dst = (L.sys == 0) ? out.u0 : out.g;
-    for (int i = 0; i < n; ++i) dst[(size_t)(2 * i + par) * M + m] = double2{0, 0};
+    double2 *dn = (L.sys == 0) ? nuo.u0 : nuo.g;
+    for (int i = 0; i < n; ++i) {
+      dst[(size_t)(2 * i + par) * M + m] = double2{0, 0};
+      if (wnu) dn[(size_t)(2 * i + par) * M + m] = double2{0, 0};
+    }
     return;
   }
+  auto comb = [&](double2 x, const double2 *a, const double2 *b, size_t o) {
+    const double2 p = a[o], q = b[o];
+    return double2{e0 * x.x + e1 * p.x + e2 * q.x, e0 * x.y + e1 * p.y + e2 * q.y};
+  };
   double kyd, kzd, kyo;
   mode_k(g, m, kyd, kzd, kyo);
   const double k2 = L.mean ? 0.0 : kyd * kyd + kzd * kzd;
@@ -203,7 +213,11 @@ __global__ void __launch_bounds__(256) k_adj_solve(Geo g, StructMats smt, double
     band_lowrank_solve<2, 2, 2, true>(
         ss, n, lane, Ms, s0, s2, s4, s4, scr,
         [&](int ir) { return in.u0[(size_t)row(ir) * M + m]; },
-        [&](int ir, double2 x) { out.u0[(size_t)row(ir) * M + m] = x; });
+        [&](int ir, double2 x) {
+          const size_t o = (size_t)row(ir) * M + m;
+          out.u0[o] = x;
+          if (wnu) nuo.u0[o] = comb(x, r1.u0, r2.u0, o);
+        });
   } else {
     const double s0 = W * (1 + gm * nu * k2), s2 = -W * gm * nu;
     if (L.mean) {
@@ -211,14 +225,23 @@ __global__ void __launch_bounds__(256) k_adj_solve(Geo g, StructMats smt, double
           ss, n, lane, Ms, s0, s2, 0.0, s2, scr,
           [&](int ir) { return double2{in.v[row(ir)], in.w[row(ir)]}; },
           [&](int ir, double2 x) {
-            out.v[row(ir)] = x.x;
-            out.w[row(ir)] = x.y;
+            const int j = row(ir);
+            out.v[j] = x.x;
+            out.w[j] = x.y;
+            if (wnu) {
+              nuo.v[j] = e0 * x.x + e1 * r1.v[j] + e2 * r2.v[j];
+              nuo.w[j] = e0 * x.y + e1 * r1.w[j] + e2 * r2.w[j];
+            }
           });
     } else {
       band_lowrank_solve<1, 1, 1, false>(
           ss, n, lane, Ms, s0, s2, 0.0, s2, scr,
           [&](int ir) { return in.g[(size_t)row(ir) * M + m]; },
-          [&](int ir, double2 x) { out.g[(size_t)row(ir) * M + m] = x; });
+          [&](int ir, double2 x) {
+            const size_t o = (size_t)row(ir) * M + m;
+            out.g[o] = x;
+            if (wnu) nuo.g[o] = comb(x, r1.g, r2.g, o);
+          });
     }
   }
 }
@@ -405,96 +428,99 @@ __global__ void __launch_bounds__(256) k_recon_adj(Geo g, const double *__restri
   }
 }
 
-// mu_h0 = W k^2 A0^T nu_u, mu_h1 = -i W (ky A1^T nu_u + kz AG^T nu_g),
-// mu_h2 = -i W (kz A1^T nu_u - ky AG^T nu_g); mean mode: mu_h1,2 -= AG^T nu_v,w.
-template <typename T, int XC>
-__global__ void __launch_bounds__(256) k_rhs_adj(Geo g, AdjMats<T> am, PrimSum nu,
-                                                 typename VT<T>::V *__restrict__ Hx) {
-  using V = typename VT<T>::V;
+// mu_x = (transpose of the reconstruction)(mu_u) + M^T sum_t c_t rho_t for a
+// steady step (M = W (C2_B - k^2 C0_B), W C0_D, C0_D: banded, no far part),
+// one thread per (row, mode); g_extra (optional) adds to mu_g.
+__global__ void __launch_bounds__(256) k_adj_combine(Geo g, StructMats sm, const double *__restrict__ RcT,
+                                                     int ldk, const double2 *__restrict__ mu0,
+                                                     const double2 *__restrict__ mu1,
+                                                     const double2 *__restrict__ mu2,
+                                                     const double2 *__restrict__ g_extra, PrimSum gat,
+                                                     Prim out) {
   const int m = blockIdx.x * blockDim.x + threadIdx.x;
-  const int x0 = blockIdx.y * XC;
+  const int j = blockIdx.y;
   if (m >= g.m) return;
-  const size_t plane = (size_t)g.n0 * g.m;
-  if (nyquist_row(g, m)) {
-#pragma unroll
-    for (int q = 0; q < XC; ++q) {
-      const int x = x0 + q;
-      if (x < g.n0) {
-        const size_t o = (size_t)x * g.m + m;
-        Hx[o] = Hx[plane + o] = Hx[2 * plane + o] = V{0, 0};
-      }
-    }
-    return;
-  }
-  V su[XC], s0[XC], sg[XC];
-  T sv[XC], sw[XC];
-#pragma unroll
-  for (int q = 0; q < XC; ++q) {
-    su[q] = s0[q] = sg[q] = V{0, 0};
-    sv[q] = sw[q] = 0;
-  }
-  const bool mean = (m == 0);
-  for (int i = 0; i < g.nd; ++i) {
-    const size_t oi = (size_t)i * g.m + m;
-    double2 cu{0, 0}, cg{0, 0};
-    double cv = 0, cw = 0;
-    for (int t = 0; t < nu.n; ++t) {
-      const double wt = nu.wgt[t];
-      if (i < g.nb) {
-        const double2 a = nu.t[t].u0[oi];
-        cu.x += wt * a.x;
-        cu.y += wt * a.y;
-      }
-      const double2 b = nu.t[t].g[oi];
-      cg.x += wt * b.x;
-      cg.y += wt * b.y;
-      if (mean) {
-        cv += wt * nu.t[t].v[i];
-        cw += wt * nu.t[t].w[i];
-      }
-    }
-    const V vu{(T)cu.x, (T)cu.y}, vg{(T)cg.x, (T)cg.y};
-    const T tv = (T)cv, tw = (T)cw;
-    const T *ag = am.AG + (size_t)i * am.ldx + x0;
-    if (i < g.nb) {
-      const T *a1 = am.A1 + (size_t)i * am.ldx + x0;
-      const T *a0 = am.A0 + (size_t)i * am.ldx + x0;
-#pragma unroll
-      for (int q = 0; q < XC; ++q) {
-        su[q].x += a1[q] * vu.x;
-        su[q].y += a1[q] * vu.y;
-        s0[q].x += a0[q] * vu.x;
-        s0[q].y += a0[q] * vu.y;
-      }
-    }
-#pragma unroll
-    for (int q = 0; q < XC; ++q) {
-      sg[q].x += ag[q] * vg.x;
-      sg[q].y += ag[q] * vg.y;
-      sv[q] += ag[q] * tv;
-      sw[q] += ag[q] * tw;
-    }
-  }
+  const int M = g.m;
+  const size_t o = (size_t)j * M + m;
+  const bool nyq = nyquist_row(g, m);
   double kyd, kzd, kyo;
   mode_k(g, m, kyd, kzd, kyo);
-  const T W = (T)g.W, ky = (T)kyo, kz = (T)kzd, k2 = (T)(kyd * kyd + kzd * kzd);
-#pragma unroll
-  for (int q = 0; q < XC; ++q) {
-    const int x = x0 + q;
-    if (x >= g.n0) continue;
-    const size_t o = (size_t)x * g.m + m;
-    V h0{W * k2 * s0[q].x, W * k2 * s0[q].y};
-    // -i W z = W (z.y, -z.x)
-    V z1{ky * su[q].x + kz * sg[q].x, ky * su[q].y + kz * sg[q].y};
-    V z2{kz * su[q].x - ky * sg[q].x, kz * su[q].y - ky * sg[q].y};
-    V h1{W * z1.y, -W * z1.x}, h2{W * z2.y, -W * z2.x};
-    if (mean) {
-      h1.x -= sv[q];
-      h2.x -= sw[q];
+  const double k2 = kyd * kyd + kzd * kzd;
+  const double den = (k2 == 0.0) ? 1.0 : k2;
+  const double a = kyd / den, b = kzd / den;
+  const int par = j & 1, jp = j >> 1;
+  // banded M^T of the gathered rho: sum over rows i = jp - d of the parity block
+  auto gather = [&](const StructSys &ss, int P, int Q, double m0, double m2, int comp) {
+    double2 acc{0, 0};
+    const int BW = P + Q + 1;
+    for (int d = -P; d <= Q; ++d) {
+      const int i = jp - d;
+      if (i < 0 || i >= ss.n) continue;
+      const size_t bo = (size_t)i * BW + (d + P);
+      const double c = m0 * ss.b0[bo] + m2 * ss.b2[bo];
+      const size_t oi = (size_t)(2 * i + par) * M + m;
+      for (int t = 0; t < gat.n; ++t) {
+        const double2 r = (comp == 0) ? gat.t[t].u0[oi] : gat.t[t].g[oi];
+        acc.x += c * gat.wgt[t] * r.x;
+        acc.y += c * gat.wgt[t] * r.y;
+      }
     }
-    Hx[o] = h0;
-    Hx[plane + o] = h1;
-    Hx[2 * plane + o] = h2;
+    return acc;
+  };
+  if (j < g.nb) {
+    double2 acc = nyq ? double2{0, 0} : mu0[o];
+    if (m != 0 && !nyq) {
+      const int i0 = 1 - (j & 1);
+      const int nk = (g.nd - i0 + 1) / 2;
+      const double *r = RcT + (size_t)j * ldk;
+      for (int k = 0; k < nk; ++k) {
+        const size_t oi = (size_t)(i0 + 2 * k) * M + m;
+        const double2 p = mu1[oi], q = mu2[oi];
+        const double2 s{a * p.x + b * q.x, a * p.y + b * q.y};
+        acc.x += r[k] * s.y;
+        acc.y -= r[k] * s.x;
+      }
+    }
+    if (gat.n && !nyq) {
+      const double2 gsum = gather(sm.sys[0][par], 2, 2, -g.W * k2, g.W, 0);
+      acc.x += gsum.x;
+      acc.y += gsum.y;
+    }
+    out.u0[o] = acc;
+  }
+  if (j < g.nd) {
+    double2 gv{0, 0};
+    if (m != 0 && !nyq) {
+      const double2 p = mu1[o], q = mu2[o];
+      const double2 s{b * p.x - a * q.x, b * p.y - a * q.y};
+      gv = double2{s.y, -s.x};
+    }
+    if (g_extra && !nyq) {
+      gv.x += g_extra[o].x;
+      gv.y += g_extra[o].y;
+    }
+    if (gat.n && !nyq) {
+      const double2 gsum = gather(sm.sys[1][par], 1, 1, g.W, 0.0, 1);
+      gv.x += gsum.x;
+      gv.y += gsum.y;
+    }
+    out.g[o] = gv;
+    if (m == 0) {
+      double v = mu1[o].x, w = mu2[o].x;
+      const StructSys &ss = sm.sys[1][par];
+      for (int d = -1; d <= 1; ++d) {
+        const int i = jp - d;
+        if (i < 0 || i >= ss.n) continue;
+        const double c = ss.b0[(size_t)i * 3 + (d + 1)];
+        const int J = 2 * i + par;
+        for (int t = 0; t < gat.n; ++t) {
+          v += c * gat.wgt[t] * gat.t[t].v[J];
+          w += c * gat.wgt[t] * gat.t[t].w[J];
+        }
+      }
+      out.v[j] = v;
+      out.w[j] = w;
+    }
   }
 }
 
@@ -595,63 +621,6 @@ __global__ void __launch_bounds__(256) k_rot_adj(Geo g, NodalMats<T> nm, FFTPlan
   }
 }
 
-// mu_u0 = VB^T (mX0 - i kz mW1 + i ky mW2),
-// mu_u1 = VD^T (mX1 + i kz mW0) + VD1^T mW2,
-// mu_u2 = VD^T (mX2 - i ky mW0) - VD1^T mW1   (ky without the Nyquist wavenumber).
-template <typename T, int IC>
-__global__ void __launch_bounds__(256) k_nodal_adj(Geo g, AdjMats<T> am,
-                                                   const typename VT<T>::V *__restrict__ X,
-                                                   double2 *__restrict__ mu0,
-                                                   double2 *__restrict__ mu1,
-                                                   double2 *__restrict__ mu2) {
-  using V = typename VT<T>::V;
-  const int m = blockIdx.x * blockDim.x + threadIdx.x;
-  const int i0 = blockIdx.y * IC;
-  if (m >= g.m) return;
-  const bool nyq = nyquist_row(g, m);
-  double kyd, kzd, kyo;
-  mode_k(g, m, kyd, kzd, kyo);
-  const T ky = (T)kyo, kz = (T)kzd;
-  V a0[IC], a1[IC], a2[IC];
-#pragma unroll
-  for (int q = 0; q < IC; ++q) a0[q] = a1[q] = a2[q] = V{0, 0};
-  const size_t plane = (size_t)g.n0 * g.m;
-  if (!nyq) {
-    for (int x = 0; x < g.n0; ++x) {
-      const size_t o = (size_t)x * g.m + m;
-      const V X0 = X[o], X1 = X[plane + o], X2 = X[2 * plane + o];
-      const V W0 = X[3 * plane + o], W1 = X[4 * plane + o], W2 = X[5 * plane + o];
-      // i k z = (-k z.y, k z.x)
-      const V t0{X0.x + kz * W1.y - ky * W2.y, X0.y - kz * W1.x + ky * W2.x};
-      const V t1{X1.x - kz * W0.y, X1.y + kz * W0.x};
-      const V t2{X2.x + ky * W0.y, X2.y - ky * W0.x};
-      const T *vb = am.VB + (size_t)x * am.ldi + i0;
-      const T *vd = am.VD + (size_t)x * am.ldi + i0;
-      const T *vd1 = am.VD1 + (size_t)x * am.ldi + i0;
-#pragma unroll
-      for (int q = 0; q < IC; ++q) {
-        const T b = vb[q], d = vd[q], d1 = vd1[q];
-        a0[q].x += b * t0.x;
-        a0[q].y += b * t0.y;
-        a1[q].x += d * t1.x + d1 * W2.x;
-        a1[q].y += d * t1.y + d1 * W2.y;
-        a2[q].x += d * t2.x - d1 * W1.x;
-        a2[q].y += d * t2.y - d1 * W1.y;
-      }
-    }
-  }
-#pragma unroll
-  for (int q = 0; q < IC; ++q) {
-    const int i = i0 + q;
-    const size_t o = (size_t)i * g.m + m;
-    if (i < g.nb) mu0[o] = double2{(double)a0[q].x, (double)a0[q].y};
-    if (i < g.nd) {
-      mu1[o] = double2{(double)a1[q].x, (double)a1[q].y};
-      mu2[o] = double2{(double)a2[q].x, (double)a2[q].y};
-    }
-  }
-}
-
 __global__ void __launch_bounds__(256) k_energy_seed(Geo g, const double *__restrict__ QB,
                                                      const double *__restrict__ QD, double scale,
                                                      const double2 *__restrict__ u0,
@@ -738,9 +707,19 @@ int rot_adj_units(const Geo &g) {
 }  // namespace
 
 hipError_t launch_adj_solve(const Geo &g, const StructMats &smt, double gamma, PrimC in, Prim out,
-                            double *scratch, hipStream_t s) {
+                            double *scratch, hipStream_t s, Prim nu, PrimC r1, PrimC r2, double e0,
+                            double e1, double e2) {
   hipLaunchKernelGGL(k_adj_solve, dim3((sys_threads(g) + 255) / 256), dim3(256), 0, s, g, smt, gamma,
-                     in, out, scratch);
+                     in, out, scratch, nu, r1, r2, e0, e1, e2);
+  return hipGetLastError();
+}
+
+hipError_t launch_adj_combine(const Geo &g, const StructMats &sm, const double *RcT, int ldk,
+                              const double2 *mu0, const double2 *mu1, const double2 *mu2,
+                              const double2 *g_extra, const PrimSum &gather, Prim out, hipStream_t s) {
+  dim3 grid((g.m + 255) / 256, g.nd);
+  hipLaunchKernelGGL(k_adj_combine, grid, dim3(256), 0, s, g, sm, RcT, ldk, mu0, mu1, mu2, g_extra,
+                     gather, out);
   return hipGetLastError();
 }
 
@@ -756,16 +735,6 @@ hipError_t launch_recon_adj(const Geo &g, const double *RcT, int ldk, const doub
                             Prim out, hipStream_t s) {
   dim3 grid((g.m + 255) / 256, g.nd);
   hipLaunchKernelGGL(k_recon_adj, grid, dim3(256), 0, s, g, RcT, ldk, mu0, mu1, mu2, g_extra, out);
-  return hipGetLastError();
-}
-
-template <typename T>
-hipError_t launch_rhs_adj(const Geo &g, const AdjMats<T> &am, const PrimSum &nu, Workspace &ws,
-                          hipStream_t s) {
-  using V = typename VT<T>::V;
-  dim3 grid((g.m + 255) / 256, (g.n0 + XC_ADJ - 1) / XC_ADJ);
-  hipLaunchKernelGGL((k_rhs_adj<T, XC_ADJ>), grid, dim3(256), 0, s, g, am, nu,
-                     reinterpret_cast<V *>(ws.Hx));
   return hipGetLastError();
 }
 
@@ -791,16 +760,6 @@ hipError_t launch_rot_adj(const Geo &g, const NodalMats<T> &nm, const void *Yf, 
   return hipGetLastError();
 }
 
-template <typename T>
-hipError_t launch_nodal_adj(const Geo &g, const AdjMats<T> &am, Workspace &ws, double2 *mu0,
-                            double2 *mu1, double2 *mu2, hipStream_t s) {
-  using V = typename VT<T>::V;
-  dim3 grid((g.m + 255) / 256, (g.nd + IC_ADJ - 1) / IC_ADJ);
-  hipLaunchKernelGGL((k_nodal_adj<T, IC_ADJ>), grid, dim3(256), 0, s, g, am,
-                     reinterpret_cast<const V *>(ws.X), mu0, mu1, mu2);
-  return hipGetLastError();
-}
-
 hipError_t launch_prim_sum(const Geo &g, const PrimSum &in, Prim out, hipStream_t s) {
   dim3 grid((g.m + 255) / 256, g.nd);
   hipLaunchKernelGGL(k_prim_sum, grid, dim3(256), 0, s, g, in, out);
@@ -823,12 +782,8 @@ bool rot_adj_fits(const Geo &g, const FFTPlan &pz, int elem_bytes) {
 }
 
 #define PCF_ADJ_INSTANTIATE(T)                                                                     \
-  template hipError_t launch_rhs_adj<T>(const Geo &, const AdjMats<T> &, const PrimSum &,          \
-                                        Workspace &, hipStream_t);                                 \
   template hipError_t launch_rot_adj<T>(const Geo &, const NodalMats<T> &, const void *,           \
-                                        Workspace &, const FFTPlan &, hipStream_t);                \
-  template hipError_t launch_nodal_adj<T>(const Geo &, const AdjMats<T> &, Workspace &, double2 *, \
-                                          double2 *, double2 *, hipStream_t);
+                                        Workspace &, const FFTPlan &, hipStream_t);
 PCF_ADJ_INSTANTIATE(double)
 PCF_ADJ_INSTANTIATE(float)
 #undef PCF_ADJ_INSTANTIATE

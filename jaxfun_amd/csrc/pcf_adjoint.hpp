@@ -50,8 +50,15 @@ struct PrimSum {
 
 // mu_out = S^{-T} mu_in per mode, S = M - gamma L (startup and steady systems);
 // smt holds the transposed, index-reversed structure of every parity block.
+// With nu.u0 non-null it also writes nu = e0 out + e1 r1 + e2 r2.
 hipError_t launch_adj_solve(const Geo &g, const StructMats &smt, double gamma, PrimC in, Prim out,
-                            double *scratch, hipStream_t s);
+                            double *scratch, hipStream_t s, Prim nu = Prim{}, PrimC r1 = PrimC{},
+                            PrimC r2 = PrimC{}, double e0 = 0, double e1 = 0, double e2 = 0);
+// out = reconstruction adjoint of (mu_u0, mu_u1, mu_u2) [+ g_extra on g]
+//       + M^T sum_t c_t rho_t (the steady SBDF3 gather; banded).
+hipError_t launch_adj_combine(const Geo &g, const StructMats &sm, const double *RcT, int ldk,
+                              const double2 *mu0, const double2 *mu1, const double2 *mu2,
+                              const double2 *g_extra, const PrimSum &gather, Prim out, hipStream_t s);
 // out += (M + gamma L)^T r_stage (if r_stage.u0) + M^T (sum of gather terms).
 hipError_t launch_adj_mass(const Geo &g, const StructMats &sm, double gamma, PrimC r_stage,
                            const PrimSum &gather, Prim out, hipStream_t s);
@@ -60,19 +67,11 @@ hipError_t launch_adj_mass(const Geo &g, const StructMats &sm, double gamma, Pri
 hipError_t launch_recon_adj(const Geo &g, const double *RcT, int ldk, const double2 *mu0,
                             const double2 *mu1, const double2 *mu2, const double2 *g_extra,
                             Prim out, hipStream_t s);
-// Nodal cotangents of H (ws.Hx) from the weighted nonlinear-row cotangents.
-template <typename T>
-hipError_t launch_rhs_adj(const Geo &g, const AdjMats<T> &am, const PrimSum &nu, Workspace &ws,
-                          hipStream_t s);
 // Cross-product adjoint on the padded grid: forward fields Yf (6 padded y-line
 // fields), cotangent of H in ws.Hz (3 fields) -> cotangents of (u, omega) in ws.Y.
 template <typename T>
 hipError_t launch_rot_adj(const Geo &g, const NodalMats<T> &nm, const void *Yf, Workspace &ws,
                           const FFTPlan &pz, hipStream_t s);
-// Wall-normal transform adjoint: ws.X (6 nodal fields) -> (mu_u0, mu_u1, mu_u2).
-template <typename T>
-hipError_t launch_nodal_adj(const Geo &g, const AdjMats<T> &am, Workspace &ws, double2 *mu0,
-                            double2 *mu1, double2 *mu2, hipStream_t s);
 // Streamwise FFT of nf fields between arbitrary buffers (layouts as k_fft_y).
 template <typename T>
 hipError_t launch_fft_y_fields(const Geo &g, const NodalMats<T> &nm, const FFTPlan &py, bool inverse,
@@ -83,6 +82,21 @@ hipError_t launch_energy_seed(const Geo &g, const double *QB, const double *QD, 
                               double2 *mu1, double2 *mu2, hipStream_t s);
 // out = sum_t w_t in_t (every component; overwrites out).
 hipError_t launch_prim_sum(const Geo &g, const PrimSum &in, Prim out, hipStream_t s);
+// Matrix-core (MFMA) wall-normal contractions: forward nodal transform and
+// nonlinear rows, and their adjoints (same outputs as the launchers above).
+template <typename T>
+hipError_t launch_nodal_mfma(const Geo &g, const NodalMats<T> &nm, const double2 *u0, const double2 *u1,
+                             const double2 *u2, Workspace &ws, hipStream_t s);
+template <typename T>
+hipError_t launch_rhs_mfma(const Geo &g, const NodalMats<T> &nm, Workspace &ws, double2 *Nu, double2 *Ng,
+                           double *Nv, double *Nw, hipStream_t s);
+template <typename T>
+hipError_t launch_nodal_adj_mfma(const Geo &g, const AdjMats<T> &am, Workspace &ws, double2 *mu0,
+                                 double2 *mu1, double2 *mu2, hipStream_t s);
+template <typename T>
+hipError_t launch_rhs_adj_mfma(const Geo &g, const AdjMats<T> &am, Prim nu, Workspace &ws, hipStream_t s);
+// Per-configuration attributes of the matrix-core kernels (dynamic LDS above 64 KB).
+hipError_t mfma_init(const Geo &g, bool fp64);
 // LDS bytes of k_rot_adj; false when the padded spanwise grid does not fit.
 bool rot_adj_fits(const Geo &g, const FFTPlan &pz, int elem_bytes);
 

@@ -129,7 +129,7 @@ struct pcf_handle {
     double *QB = nullptr, *QD = nullptr;  // Fejer-weighted Gram matrices of the bases
     pcf::AdjMats<double> ad{};
     pcf::AdjMats<float> af{};
-    pcf::Prim muX{}, rho[3]{}, rs[3]{}, gh[4]{};
+    pcf::Prim muX{}, rho[3]{}, rs[3]{}, gh[4]{}, nu{};
     double2 *mu_u[3]{};
     double2 *cot_u[3]{}, *cot_g = nullptr;
     bool cot_set[4] = {false, false, false, false};
@@ -255,7 +255,7 @@ hipError_t nonlinear_stages(pcf_handle *h, const pcf::NodalMats<T> &nm, const do
   const pcf::Geo &g = h->geo;
   hipError_t e;
   if ((e = mark(h, 0, 0))) return e;
-  if ((e = pcf::launch_nodal<T>(g, nm, u0, h->u1, h->u2, h->ws, h->stream))) return e;
+  if ((e = pcf::launch_nodal_mfma<T>(g, nm, u0, h->u1, h->u2, h->ws, h->stream))) return e;
   if ((e = mark(h, 0, 1))) return e;
   if ((e = mark(h, 1, 0))) return e;
   if ((e = pcf::launch_fft_y<T>(g, nm, h->ws, h->py, true, h->stream))) return e;
@@ -267,7 +267,7 @@ hipError_t nonlinear_stages(pcf_handle *h, const pcf::NodalMats<T> &nm, const do
   if ((e = pcf::launch_fft_y<T>(g, nm, h->ws, h->py, false, h->stream))) return e;
   if ((e = mark(h, 3, 1))) return e;
   if ((e = mark(h, 4, 0))) return e;
-  if ((e = pcf::launch_rhs<T>(g, nm, h->ws, h->Nu[slot], h->Ng[slot], h->Nv[slot], h->Nw[slot],
+  if ((e = pcf::launch_rhs_mfma<T>(g, nm, h->ws, h->Nu[slot], h->Ng[slot], h->Nv[slot], h->Nw[slot],
                               h->stream)))
     return e;
   return mark(h, 4, 1);
@@ -563,7 +563,7 @@ hipError_t zero_prim(pcf_handle *h, const pcf::Prim &p) {
 template <typename T>
 int32_t build_adj_mats(pcf_handle *h, pcf::AdjMats<T> &am) {
   const pcf::Operators &o = h->ops;
-  const int ldx = (o.n0 + 7) / 8 * 8, ldi = (o.nd + 7) / 8 * 8;
+  const int ldx = (o.n0 + 15) / 16 * 16, ldi = (o.nd + 15) / 16 * 16;  // whole 16-row MFMA tiles
   auto pad = [](const pcf::Mat &Mx, int ld) {  // row-major [rows][ld]
     std::vector<T> t((size_t)Mx.rows * ld, (T)0);
     for (int i = 0; i < Mx.rows; ++i)
@@ -662,7 +662,7 @@ int32_t adj_setup(pcf_handle *h) {
   } else {
     if ((r = build_adj_mats<float>(h, A.af))) return r;
   }
-  if ((r = alloc_prim(h, A.muX))) return r;
+  if ((r = alloc_prim(h, A.muX)) || (r = alloc_prim(h, A.nu))) return r;
   for (int i = 0; i < 3; ++i)
     if ((r = alloc_prim(h, A.rho[i])) || (r = alloc_prim(h, A.rs[i]))) return r;
   for (int i = 0; i < 4; ++i)
@@ -702,20 +702,27 @@ int32_t adj_reserve(pcf_handle *h, int steps, int every) {
 
 template <typename T>
 hipError_t nonlinear_adjoint(pcf_handle *h, const pcf::NodalMats<T> &nm, const pcf::AdjMats<T> &am,
-                             const pcf::PrimSum &nu, const void *ytape) {
+                             const pcf::Prim &nu, const void *ytape) {
   const pcf::Geo &g = h->geo;
   auto &A = h->adj;
   hipError_t e;
-  if ((e = pcf::launch_rhs_adj<T>(g, am, nu, h->ws, h->stream))) return e;
+  if ((e = pcf::launch_rhs_adj_mfma<T>(g, am, nu, h->ws, h->stream))) return e;
   if ((e = pcf::launch_fft_y_fields<T>(g, nm, h->py, true, 3, h->ws.Hx, h->ws.Hz, h->stream))) return e;
   if ((e = pcf::launch_rot_adj<T>(g, nm, ytape, h->ws, h->pz, h->stream))) return e;
   if ((e = pcf::launch_fft_y_fields<T>(g, nm, h->py, false, 6, h->ws.Y, h->ws.X, h->stream))) return e;
-  return pcf::launch_nodal_adj<T>(g, am, h->ws, A.mu_u[0], A.mu_u[1], A.mu_u[2], h->stream);
+  return pcf::launch_nodal_adj_mfma<T>(g, am, h->ws, A.mu_u[0], A.mu_u[1], A.mu_u[2], h->stream);
 }
 
+// mu_u = F'(u)^T nu for the nonlinear evaluation whose forward fields are ytape;
+// a weighted sum of cotangents is first folded into adj.nu.
 hipError_t nl_adj(pcf_handle *h, const pcf::PrimSum &nu, const void *ytape) {
-  return (h->cfg.precision == PCF_PREC_FP64) ? nonlinear_adjoint<double>(h, h->md.view, h->adj.ad, nu, ytape)
-                                             : nonlinear_adjoint<float>(h, h->mf.view, h->adj.af, nu, ytape);
+  auto &A = h->adj;
+  if (!(nu.n == 1 && nu.wgt[0] == 1.0 && nu.t[0].u0 == A.nu.u0)) {
+    hipError_t e = pcf::launch_prim_sum(h->geo, nu, A.nu, h->stream);
+    if (e != hipSuccess) return e;
+  }
+  return (h->cfg.precision == PCF_PREC_FP64) ? nonlinear_adjoint<double>(h, h->md.view, A.ad, A.nu, ytape)
+                                             : nonlinear_adjoint<float>(h, h->mf.view, A.af, A.nu, ytape);
 }
 
 void ps_add(pcf::PrimSum &s, const pcf::Prim &p, double w) {
@@ -749,16 +756,22 @@ hipError_t reverse_step(pcf_handle *h, int k, bool steady, void *const *tape) {
   }
   hipError_t er;
   if (steady) {
+    // rho_k and, in the same pass, mu_{N_k} = e0 rho_k + e1 rho_k+1 + e2 rho_k+2
     if ((er = pcf::launch_adj_solve(g, A.smt, dt * 6.0 / 11.0, cprim(A.muX), A.rho[rr[0]], h->scratch,
-                                    h->stream)))
+                                    h->stream, A.nu, cprim(A.rho[rr[1]]), cprim(A.rho[rr[2]]), e[0], e[1],
+                                    e[2])))
       return er;
     pcf::PrimSum nu{};
-    for (int j = 0; j < 3; ++j) ps_add(nu, A.rho[rr[j]], e[j]);
+    ps_add(nu, A.nu, 1.0);
     if ((er = nl_adj(h, nu, tape[0]))) return er;
-    if ((er = recon_or_zero(h, k == 0))) return er;
     pcf::PrimSum ga{};
     for (int j = 0; j < 3; ++j) ps_add(ga, A.rho[rr[j]], C_SBDF[j]);
-    return pcf::launch_adj_mass(g, h->sm, 0.0, pcf::PrimC{}, ga, A.muX, h->stream);
+    if (k == 0) {  // initial state: its velocity leaves keep mu_u
+      if ((er = zero_prim(h, A.muX))) return er;
+      return pcf::launch_adj_mass(g, h->sm, 0.0, pcf::PrimC{}, ga, A.muX, h->stream);
+    }
+    return pcf::launch_adj_combine(g, h->sm, A.RcT, A.ldrT, A.mu_u[0], A.mu_u[1], A.mu_u[2], nullptr, ga,
+                                   A.muX, h->stream);
   }
   if ((er = zero_prim(h, A.rho[rr[0]]))) return er;
   for (int s = 2; s >= 0; --s) {
@@ -921,13 +934,14 @@ int32_t pcf_create(const pcf_config *cfg, pcf_handle **out) {
   h->ws.Hx = wsp + (nX + nY + nHz) * ve;
 
   if (cfg->precision == PCF_PREC_FP64) {
-    if ((r = build_nodal<double>(h, h->md, 8, 8))) return bail(r);
+    if ((r = build_nodal<double>(h, h->md, 16, 16))) return bail(r);
   } else {
-    if ((r = build_nodal<float>(h, h->mf, 8, 8))) return bail(r);
+    if ((r = build_nodal<float>(h, h->mf, 16, 16))) return bail(r);
   }
   if ((r = build_struct_mats(h))) return bail(r);
   {
     hipError_t e = pcf::kernels_init(h->geo);
+    if (e == hipSuccess) e = pcf::mfma_init(h->geo, cfg->precision == PCF_PREC_FP64);
     if (e != hipSuccess)
       return bail(fail(PCF_E_UNSUPPORTED, std::string("kernel LDS limits: ") + hipGetErrorString(e)));
   }
@@ -1221,11 +1235,24 @@ int32_t pcf_vjp(pcf_handle *h, int32_t steps, int32_t checkpoint_every, int32_t 
     return fail(PCF_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
   };
   hipError_t e;
-  // forward, keeping a snapshot at the start of every segment
-  for (int k = 0; k < steps; ++k) {
-    if (k % every == 0 && (e = snap_save(h, A.snaps[k / every]))) return hip_fail(e, "snapshot");
-    r = (h->history_steps < 2.0) ? startup_step(h) : steady_step(h);
-    if (r) return r;
+  // forward, keeping a snapshot at the start of every segment; the last
+  // segment keeps its tape, so it is not recomputed
+  const int last_ka = (nseg - 1) * every;
+  std::vector<int> first_last;
+  {
+    int slot = 0;
+    for (int k = 0; k < steps; ++k) {
+      if (k % every == 0 && (e = snap_save(h, A.snaps[k / every]))) return hip_fail(e, "snapshot");
+      const bool st = h->history_steps >= 2.0;
+      void *const *tp = nullptr;
+      if (k >= last_ka) {
+        first_last.push_back(slot);
+        tp = &A.tape[slot];
+        slot += st ? 1 : 3;
+      }
+      r = st ? steady_step(h, tp) : startup_step(h, tp);
+      if (r) return r;
+    }
   }
   pcf_handle::Snap &fin = A.snaps[nseg];
   if ((e = snap_save(h, fin))) return hip_fail(e, "snapshot");
@@ -1250,15 +1277,19 @@ int32_t pcf_vjp(pcf_handle *h, int32_t steps, int32_t checkpoint_every, int32_t 
   std::vector<int> first;
   for (int seg = nseg - 1; seg >= 0; --seg) {
     const int ka = seg * every, kb = std::min(steps, ka + every);
-    if ((e = snap_load(h, A.snaps[seg]))) return hip_fail(e, "restore");
-    first.assign(kb - ka, 0);
-    int slot = 0;
-    for (int k = ka; k < kb; ++k) {
-      const bool st = h->history_steps >= 2.0;
-      first[k - ka] = slot;
-      r = st ? steady_step(h, &A.tape[slot]) : startup_step(h, &A.tape[slot]);
-      if (r) return r;
-      slot += st ? 1 : 3;
+    if (seg == nseg - 1) {
+      first = first_last;
+    } else {
+      if ((e = snap_load(h, A.snaps[seg]))) return hip_fail(e, "restore");
+      first.assign(kb - ka, 0);
+      int slot = 0;
+      for (int k = ka; k < kb; ++k) {
+        const bool st = h->history_steps >= 2.0;
+        first[k - ka] = slot;
+        r = st ? steady_step(h, &A.tape[slot]) : startup_step(h, &A.tape[slot]);
+        if (r) return r;
+        slot += st ? 1 : 3;
+      }
     }
     for (int k = kb - 1; k >= ka; --k)
       if ((e = reverse_step(h, k, h0 + k >= 2, &A.tape[first[k - ka]]))) return hip_fail(e, "reverse step");

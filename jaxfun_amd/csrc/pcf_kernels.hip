@@ -1,9 +1,10 @@
 // HIP kernels of the plane-Couette KMM SBDF3 step for gfx950 (MI355X).
 //
 // Stage map of one step (reference call stack examples/channelflow_kmm.py:1016-1064):
-//   k_nodal_backward   wall-normal transform of u and curl(u) for every mode
+//   k_nodal_mfma       wall-normal transform of u and curl(u) for every mode
 //                      (TensorProductSpace.backward/backward_primitive along x,
-//                      tensorproductspace.py:459-594 + Chebyshev.py:263-355)
+//                      tensorproductspace.py:459-594 + Chebyshev.py:263-355);
+//                      matrix cores, pcf_mfma.hip
 //   k_fft_y (inverse)  zero-padded streamwise c2c FFT per (field, node, kz)
 //                      (Fourier.py:128-151)
 //   k_rotational       spanwise c2r FFTs of two y-lines per unit, the
@@ -12,9 +13,10 @@
 //                      nonlinear.py:17-29) and spanwise r2c FFTs with truncation
 //   k_fft_y (forward)  streamwise FFT with truncation + Nyquist mask
 //                      (Fourier.py:170-185, tensorproductspace.py:276-293)
-//   k_forward_rhs      Dirichlet projection of H fused with the coefficient-space
-//                      KMM rows Nu, Ng, Nv00, Nw00 (channelflow_kmm.py:810-829)
-//   k_solve            SBDF3/IMEXRK3 right-hand side, FP64 per-mode Helmholtz and
+//   k_rhs_mfma         Dirichlet projection of H fused with the coefficient-space
+//                      KMM rows Nu, Ng, Nv00, Nw00 (channelflow_kmm.py:810-829);
+//                      matrix cores, pcf_mfma.hip
+//   k_solve*           SBDF3/IMEXRK3 right-hand side, FP64 per-mode Helmholtz and
 //                      biharmonic solves, mean-mode solve, velocity
 //                      reconstruction (sbdf3.py:44-62, tpmatrix.py:686-799,
 //                      channelflow_kmm.py:855-870)
@@ -24,77 +26,6 @@
 #include "pcf_device.hpp"
 
 namespace pcf {
-
-// -------------------------------------------------- k_nodal_backward ----
-// For each mode (lane) and a chunk of XC nodes: U0 = VB u0, U1 = VD u1,
-// U2 = VD u2, D1 = VD1 u1, D2 = VD1 u2, then
-// w0 = i ky U2 - i kz U1, w1 = i kz U0 - D2, w2 = D1 - i ky U0.
-template <typename T, int XC>
-__global__ void __launch_bounds__(256) k_nodal_backward(Geo g, NodalMats<T> nm,
-                                                        const double2 *__restrict__ u0,
-                                                        const double2 *__restrict__ u1,
-                                                        const double2 *__restrict__ u2,
-                                                        typename VT<T>::V *__restrict__ X) {
-  using V = typename VT<T>::V;
-  const int m = blockIdx.x * blockDim.x + threadIdx.x;
-  const int x0 = blockIdx.y * XC;
-  if (m >= g.m) return;
-  V aU0[XC], aU1[XC], aU2[XC], aD1[XC], aD2[XC];
-#pragma unroll
-  for (int q = 0; q < XC; ++q) aU0[q] = aU1[q] = aU2[q] = aD1[q] = aD2[q] = V{0, 0};
-#pragma unroll 4
-  for (int i = 0; i < g.nb; ++i) {
-    double2 cd = u0[(size_t)i * g.m + m];
-    V c{(T)cd.x, (T)cd.y};
-    const T *vb = nm.VBt + (size_t)i * nm.ldx + x0;
-#pragma unroll
-    for (int q = 0; q < XC; ++q) {
-      aU0[q].x += vb[q] * c.x;
-      aU0[q].y += vb[q] * c.y;
-    }
-  }
-#pragma unroll 4
-  for (int i = 0; i < g.nd; ++i) {
-    double2 ad = u1[(size_t)i * g.m + m], bd = u2[(size_t)i * g.m + m];
-    V a{(T)ad.x, (T)ad.y}, b{(T)bd.x, (T)bd.y};
-    const T *vd = nm.VDt + (size_t)i * nm.ldx + x0;
-    const T *vd1 = nm.VD1t + (size_t)i * nm.ldx + x0;
-#pragma unroll
-    for (int q = 0; q < XC; ++q) {
-      T s = vd[q], d = vd1[q];
-      aU1[q].x += s * a.x;
-      aU1[q].y += s * a.y;
-      aU2[q].x += s * b.x;
-      aU2[q].y += s * b.y;
-      aD1[q].x += d * a.x;
-      aD1[q].y += d * a.y;
-      aD2[q].x += d * b.x;
-      aD2[q].y += d * b.y;
-    }
-  }
-  double kyd, kzd, kyo;
-  mode_k(g, m, kyd, kzd, kyo);
-  const T ky = (T)kyo, kz = (T)kzd;
-  const size_t plane = (size_t)g.n0 * g.m;
-#pragma unroll
-  for (int q = 0; q < XC; ++q) {
-    const int x = x0 + q;
-    if (x < g.n0) {
-      const size_t o = (size_t)x * g.m + m;
-      V U0 = aU0[q], U1 = aU1[q], U2 = aU2[q];
-      // i k c = (-k c.y, k c.x)
-      V w0{-ky * U2.y + kz * U1.y, ky * U2.x - kz * U1.x};
-      V w1{-kz * U0.y - aD2[q].x, kz * U0.x - aD2[q].y};
-      V w2{aD1[q].x + ky * U0.y, aD1[q].y - ky * U0.x};
-      X[o] = U0;
-      X[plane + o] = U1;
-      X[2 * plane + o] = U2;
-      X[3 * plane + o] = w0;
-      X[4 * plane + o] = w1;
-      X[5 * plane + o] = w2;
-    }
-  }
-}
 
 // ---------------------------------------------------------- k_fft_y ----
 // One block per (kz chunk, node x, field): the streamwise lines of KC spanwise
@@ -232,68 +163,6 @@ __global__ void __launch_bounds__(256) k_rotational(Geo g, NodalMats<T> nm, FFTP
     V *dst = Hz + f * fstride + ((size_t)x * MY + 2 * y2) * KZ;
     dst[k] = ha;
     dst[KZ + k] = hb;
-  }
-}
-
-// ----------------------------------------------------- k_forward_rhs ----
-// For each mode (lane) and a chunk of IC output rows:
-//   Nu = W (A1 (i ky H1 + i kz H2) + A0 (k^2 H0)),  Ng = W AG (i kz H1 - i ky H2),
-// and for the mean mode Nv00 = -AG Re H1, Nw00 = -AG Re H2 (1-D mass rows).
-template <typename T, int IC>
-__global__ void __launch_bounds__(256) k_forward_rhs(Geo g, NodalMats<T> nm,
-                                                     const typename VT<T>::V *__restrict__ Hx,
-                                                     double2 *__restrict__ Nu,
-                                                     double2 *__restrict__ Ng,
-                                                     double *__restrict__ Nv,
-                                                     double *__restrict__ Nw) {
-  using V = typename VT<T>::V;
-  const int m = blockIdx.x * blockDim.x + threadIdx.x;
-  const int i0 = blockIdx.y * IC;
-  if (m >= g.m) return;
-  double kyd, kzd, kyo;
-  mode_k(g, m, kyd, kzd, kyo);
-  const T ky = (T)kyo, kz = (T)kzd, k2 = (T)(kyd * kyd + kzd * kzd);
-  V aU[IC], aG[IC];
-  T aV[IC], aW[IC];
-#pragma unroll
-  for (int q = 0; q < IC; ++q) {
-    aU[q] = aG[q] = V{0, 0};
-    aV[q] = aW[q] = 0;
-  }
-  const size_t plane = (size_t)g.n0 * g.m;
-#pragma unroll 4
-  for (int x = 0; x < g.n0; ++x) {
-    const size_t o = (size_t)x * g.m + m;
-    V h0 = Hx[o], h1 = Hx[plane + o], h2 = Hx[2 * plane + o];
-    // c1 = i ky h1 + i kz h2 ; cg = i kz h1 - i ky h2 ; c0 = k^2 h0
-    V c1{-ky * h1.y - kz * h2.y, ky * h1.x + kz * h2.x};
-    V cg{-kz * h1.y + ky * h2.y, kz * h1.x - ky * h2.x};
-    V c0{k2 * h0.x, k2 * h0.y};
-    const T *a1 = nm.A1t + (size_t)x * nm.ldi + i0;
-    const T *a0 = nm.A0t + (size_t)x * nm.ldi + i0;
-    const T *ag = nm.AGt + (size_t)x * nm.ldi + i0;
-#pragma unroll
-    for (int q = 0; q < IC; ++q) {
-      aU[q].x += a1[q] * c1.x + a0[q] * c0.x;
-      aU[q].y += a1[q] * c1.y + a0[q] * c0.y;
-      aG[q].x += ag[q] * cg.x;
-      aG[q].y += ag[q] * cg.y;
-      aV[q] += ag[q] * h1.x;
-      aW[q] += ag[q] * h2.x;
-    }
-  }
-  const double W = g.W;
-#pragma unroll
-  for (int q = 0; q < IC; ++q) {
-    const int i = i0 + q;
-    if (i < g.nb) Nu[(size_t)i * g.m + m] = double2{W * (double)aU[q].x, W * (double)aU[q].y};
-    if (i < g.nd) {
-      Ng[(size_t)i * g.m + m] = double2{W * (double)aG[q].x, W * (double)aG[q].y};
-      if (m == 0) {
-        Nv[i] = -(double)aV[q];
-        Nw[i] = -(double)aW[q];
-      }
-    }
   }
 }
 
@@ -1277,8 +1146,6 @@ __global__ void __launch_bounds__(256) k_diag(Geo g, const double *__restrict__ 
 
 // ------------------------------------------------------------ launchers ----
 namespace {
-constexpr int XC_NODAL = 8;
-constexpr int IC_RHS = 8;
 
 int pick_kc(int kzs) {
   int kc = 16;
@@ -1286,16 +1153,6 @@ int pick_kc(int kzs) {
   return kc;
 }
 }  // namespace
-
-template <typename T>
-hipError_t launch_nodal(const Geo &g, const NodalMats<T> &nm, const double2 *u0, const double2 *u1,
-                        const double2 *u2, Workspace &ws, hipStream_t s) {
-  using V = typename VT<T>::V;
-  dim3 grid((g.m + 255) / 256, (g.n0 + XC_NODAL - 1) / XC_NODAL);
-  hipLaunchKernelGGL((k_nodal_backward<T, XC_NODAL>), grid, dim3(256), 0, s, g, nm, u0, u1, u2,
-                     reinterpret_cast<V *>(ws.X));
-  return hipGetLastError();
-}
 
 // Butterflies per thread of the largest pass of `plan` over `nbatch` sequences.
 int max_butterflies(const FFTPlan &plan, int nbatch) {
@@ -1362,16 +1219,6 @@ hipError_t launch_rotational(const Geo &g, const NodalMats<T> &nm, Workspace &ws
   else if (mb <= 6) PCF_ROT(6);
   else PCF_ROT(12);
 #undef PCF_ROT
-  return hipGetLastError();
-}
-
-template <typename T>
-hipError_t launch_rhs(const Geo &g, const NodalMats<T> &nm, Workspace &ws, double2 *Nu,
-                      double2 *Ng, double *Nv, double *Nw, hipStream_t s) {
-  using V = typename VT<T>::V;
-  dim3 grid((g.m + 255) / 256, (g.nd + IC_RHS - 1) / IC_RHS);
-  hipLaunchKernelGGL((k_forward_rhs<T, IC_RHS>), grid, dim3(256), 0, s, g, nm,
-                     reinterpret_cast<const V *>(ws.Hx), Nu, Ng, Nv, Nw);
   return hipGetLastError();
 }
 
@@ -1445,16 +1292,13 @@ bool fft_fits(const Geo &g, const FFTPlan &py, const FFTPlan &pz, int elem_bytes
 }
 
 #define PCF_INSTANTIATE(T)                                                                     \
-  template hipError_t launch_nodal<T>(const Geo &, const NodalMats<T> &, const double2 *,        \
-                                      const double2 *, const double2 *, Workspace &, hipStream_t); \
   template hipError_t launch_fft_y<T>(const Geo &, const NodalMats<T> &, Workspace &,            \
                                       const FFTPlan &, bool, hipStream_t);                        \
   template hipError_t launch_fft_y_fields<T>(const Geo &, const NodalMats<T> &, const FFTPlan &,   \
                                              bool, int, const void *, void *, hipStream_t);       \
   template hipError_t launch_rotational<T>(const Geo &, const NodalMats<T> &, Workspace &,       \
                                            const FFTPlan &, hipStream_t);                        \
-  template hipError_t launch_rhs<T>(const Geo &, const NodalMats<T> &, Workspace &, double2 *,   \
-                                    double2 *, double *, double *, hipStream_t);
+
 PCF_INSTANTIATE(double)
 PCF_INSTANTIATE(float)
 #undef PCF_INSTANTIATE

@@ -38,18 +38,18 @@ struct FFTPlan {
   int radix[12];
 };
 
-// Contraction matrices in the precision T, transposed so that a chunk of
-// outputs is contiguous (wave-uniform scalar loads).
+// Contraction matrices in the precision T, transposed so that the 16 output
+// rows of one matrix-core tile are contiguous for each contraction index.
 template <typename T>
 struct NodalMats {
   const T *VBt;   // [nb][ldx]  VB^T  (zero padded to ldx)
   const T *VDt;   // [nd][ldx]
   const T *VD1t;  // [nd][ldx]
-  int ldx;        // >= n0, multiple of the x-chunk
+  int ldx;        // >= n0, multiple of 16
   const T *A1t;   // [n0][ldi]  A1^T
   const T *A0t;   // [n0][ldi]
   const T *AGt;   // [n0][ldi]
-  int ldi;        // >= nd, multiple of the row chunk
+  int ldi;        // >= nd, multiple of 16
   const T *xnode; // [n0]
   const T *twy;   // [my] exp(-2 pi i t / my) as (cos, sin) pairs
   const T *twz;   // [mz]
@@ -100,17 +100,11 @@ struct Workspace {
 
 // Launchers (return hipError_t of the launch).  One kernel each, in step order.
 template <typename T>
-hipError_t launch_nodal(const Geo &g, const NodalMats<T> &nm, const double2 *u0, const double2 *u1,
-                        const double2 *u2, Workspace &ws, hipStream_t s);
-template <typename T>
 hipError_t launch_fft_y(const Geo &g, const NodalMats<T> &nm, Workspace &ws, const FFTPlan &py,
                         bool inverse, hipStream_t s);
 template <typename T>
 hipError_t launch_rotational(const Geo &g, const NodalMats<T> &nm, Workspace &ws,
                              const FFTPlan &pz, hipStream_t s);
-template <typename T>
-hipError_t launch_rhs(const Geo &g, const NodalMats<T> &nm, Workspace &ws, double2 *Nu,
-                      double2 *Ng, double *Nv, double *Nw, hipStream_t s);
 // steady = true: SBDF3 update (three solution and nonlinear levels, d = 0);
 // false: general update (IMEXRK3 stages).
 hipError_t launch_solve(const Geo &g, const StructMats &sm, const SolveArgs &a, double *scratch,

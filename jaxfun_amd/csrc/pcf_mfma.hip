@@ -1,0 +1,538 @@
+// Wall-normal contractions of the step and of its adjoint on the matrix cores.
+//
+// Every one of them is a small real GEMM per field, Out[r][c] = sum_k A[r][k]
+// B[k][c]: A a (<= 65 x 65) basis/projection matrix, B the spectral or nodal
+// field with its complex modes laid out as pairs of real columns (re, im).  The
+// per-mode wavenumber factors (i ky, i kz, k^2) are applied to B as it is read
+// and to the accumulators in the epilogue, where a lane takes its column's
+// partner component (re <-> im) from the neighbouring lane.
+//
+// One wave computes one 16 x 16 tile (16 output rows x 8 modes) with
+// v_mfma_f64_16x16x4f64 (FP64) or v_mfma_f32_16x16x4f32 (mixed precision).
+// Operand layout of both: A[i][k] in lane 16k + i, B[k][j] in lane 16k + j.
+// Result: FP64 lane l, register r holds D[4r + l/16][l%16]; FP32 holds
+// D[4(l/16) + r][l%16].
+#include <algorithm>
+
+#include "pcf_adjoint.hpp"
+#include "pcf_device.hpp"
+
+namespace pcf {
+namespace {
+
+typedef double d4v __attribute__((ext_vector_type(4)));
+typedef float f4v __attribute__((ext_vector_type(4)));
+
+template <typename T>
+struct Mfma;
+template <>
+struct Mfma<double> {
+  using acc = d4v;
+  static __device__ __forceinline__ acc mma(double a, double b, acc c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int l, int r) { return 4 * r + (l >> 4); }
+};
+template <>
+struct Mfma<float> {
+  using acc = f4v;
+  static __device__ __forceinline__ acc mma(float a, float b, acc c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int l, int r) { return 4 * (l >> 4) + r; }
+};
+
+template <typename T>
+__device__ __forceinline__ T partner(T v) {
+  return __shfl_xor(v, 1, 64);
+}
+
+template <typename T>
+__device__ __forceinline__ T comp_of(const double2 &z, int comp) {
+  return (T)(comp ? z.y : z.x);
+}
+
+// Lane geometry of one tile: real column c of the tile, its mode and component.
+struct Lane {
+  int l, c, mcol, comp;
+  bool valid;  // mode inside the spectrum
+};
+__device__ __forceinline__ Lane lane_of(const Geo &g, int ct) {
+  Lane L;
+  L.l = threadIdx.x & 63;
+  L.c = L.l & 15;
+  L.mcol = 8 * ct + (L.c >> 1);
+  L.comp = L.c & 1;
+  L.valid = L.mcol < g.m;
+  return L;
+}
+
+__device__ __forceinline__ bool nyq_mode(const Geo &g, int m) { return m / g.kzs == g.ny / 2; }
+
+// ------------------------------------------------------------ staging ----
+// A block owns one tile of 8 modes (16 real columns) and one wave per 16-row
+// output tile.  Its B panels (NF fields x Kp rows x 16 columns, in T) are first
+// staged into LDS with independent, coalesced loads by all threads, so the
+// MFMA loop only waits on LDS and on the small (L2-resident) A matrices.
+// Staging loads per thread of each kernel; the launchers check that
+// blockDim * STAGE covers the kernel's panel.
+constexpr int STAGE_NODAL = 6, STAGE_RHS = 3, STAGE_NODAL_ADJ = 3, STAGE_RHS_ADJ = 4;
+
+template <typename T>
+struct Panel {
+  T *b;
+  int kp;
+  __device__ T &at(int f, int k, int c) const { return b[((size_t)f * kp + k) * 16 + c]; }
+};
+
+// ------------------------------------------------------------ forward ----
+// U0 = VB u0, U1 = VD u1, U2 = VD u2, D1 = VD1 u1, D2 = VD1 u2 (rows: nodes x),
+// then w0 = i ky U2 - i kz U1, w1 = i kz U0 - D2, w2 = D1 - i ky U0.
+template <typename T>
+__global__ void __launch_bounds__(512) k_nodal_mfma(Geo g, NodalMats<T> nm, const double2 *__restrict__ u0,
+                                                    const double2 *__restrict__ u1,
+                                                    const double2 *__restrict__ u2,
+                                                    typename VT<T>::V *__restrict__ X) {
+  using MM = Mfma<T>;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int ct = blockIdx.x;
+  const int M = g.m, kp = (g.nd + 3) / 4 * 4;
+  const Panel<T> B{reinterpret_cast<T *>(smem), kp};
+  {
+    // every load of the thread in flight before the first LDS store
+    const int total = 3 * kp * 8;
+    double2 z[STAGE_NODAL];
+#pragma unroll
+    for (int t = 0; t < STAGE_NODAL; ++t) {
+      const int idx = threadIdx.x + t * blockDim.x;
+      const int f = idx / (kp * 8), rem = idx - f * kp * 8, k = rem >> 3, j = rem & 7;
+      const int m = 8 * ct + j;
+      z[t] = double2{0, 0};
+      if (idx < total && m < M) {
+        if (f == 0 && k < g.nb) z[t] = u0[(size_t)k * M + m];
+        if (f == 1 && k < g.nd) z[t] = u1[(size_t)k * M + m];
+        if (f == 2 && k < g.nd) z[t] = u2[(size_t)k * M + m];
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < STAGE_NODAL; ++t) {
+      const int idx = threadIdx.x + t * blockDim.x;
+      if (idx >= total) continue;
+      const int f = idx / (kp * 8), rem = idx - f * kp * 8, k = rem >> 3, j = rem & 7;
+      B.at(f, k, 2 * j) = (T)z[t].x;
+      B.at(f, k, 2 * j + 1) = (T)z[t].y;
+    }
+  }
+  __syncthreads();
+  const Lane L = lane_of(g, ct);
+  double kyd, kzd, kyo;
+  mode_k(g, L.valid ? L.mcol : 0, kyd, kzd, kyo);
+  const T ky = (T)kyo, kz = (T)kzd;
+  const T s = L.comp ? (T)1 : (T)-1;  // re lane: i z -> -z.im ; im lane: i z -> +z.re
+  const size_t plane = (size_t)g.n0 * 2 * M;
+  T *Xr = reinterpret_cast<T *>(X);
+  for (int x0 = (threadIdx.x >> 6) * 16; x0 < g.n0; x0 += (blockDim.x >> 6) * 16) {
+  const int xa = x0 + (L.l & 15);
+  typename MM::acc U0{}, U1{}, U2{}, D1{}, D2{};
+#pragma unroll 4
+  for (int kk = 0; kk < kp; kk += 4) {
+    const int k = kk + (L.l >> 4);
+    const T b0 = B.at(0, k, L.c), b1 = B.at(1, k, L.c), b2 = B.at(2, k, L.c);
+    T aB = 0, aD = 0, aD1 = 0;
+    if (k < g.nb) aB = nm.VBt[(size_t)k * nm.ldx + xa];
+    if (k < g.nd) {
+      aD = nm.VDt[(size_t)k * nm.ldx + xa];
+      aD1 = nm.VD1t[(size_t)k * nm.ldx + xa];
+    }
+    U0 = MM::mma(aB, b0, U0);
+    U1 = MM::mma(aD, b1, U1);
+    U2 = MM::mma(aD, b2, U2);
+    D1 = MM::mma(aD1, b1, D1);
+    D2 = MM::mma(aD1, b2, D2);
+  }
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const T u0v = U0[r], u1v = U1[r], u2v = U2[r], d1 = D1[r], d2 = D2[r];
+    const T p0 = partner(u0v), p1 = partner(u1v), p2 = partner(u2v);
+    const T w0 = s * (ky * p2 - kz * p1);
+    const T w1 = s * kz * p0 - d2;
+    const T w2 = d1 - s * ky * p0;
+    const int x = x0 + MM::row(L.l, r);
+    if (L.valid && x < g.n0) {
+      const size_t o = (size_t)x * 2 * M + 2 * L.mcol + L.comp;
+      Xr[o] = u0v;
+      Xr[plane + o] = u1v;
+      Xr[2 * plane + o] = u2v;
+      Xr[3 * plane + o] = w0;
+      Xr[4 * plane + o] = w1;
+      Xr[5 * plane + o] = w2;
+    }
+  }
+  }
+}
+
+// Nu = W (A1 c1 + A0 c0), Ng = W AG cg with c1 = i ky h1 + i kz h2, c0 = k^2 h0,
+// cg = i kz h1 - i ky h2 (rows: Shen index i, K: nodes x); mean mode
+// Nv00 = -AG Re h1, Nw00 = -AG Re h2 by the block that holds mode 0.
+template <typename T>
+__global__ void __launch_bounds__(512) k_rhs_mfma(Geo g, NodalMats<T> nm,
+                                                  const typename VT<T>::V *__restrict__ Hx,
+                                                  double2 *__restrict__ Nu, double2 *__restrict__ Ng,
+                                                  double *__restrict__ Nv, double *__restrict__ Nw) {
+  using MM = Mfma<T>;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int ct = blockIdx.x;
+  const int M = g.m, kp = (g.n0 + 3) / 4 * 4;
+  const size_t plane = (size_t)g.n0 * M;
+  const Panel<T> B{reinterpret_cast<T *>(smem), kp};
+  {
+    using V = typename VT<T>::V;
+    const int total = kp * 8;
+    V hv[STAGE_RHS][3];
+#pragma unroll
+    for (int t = 0; t < STAGE_RHS; ++t) {
+      const int idx = threadIdx.x + t * blockDim.x;
+      const int x = idx >> 3, m = 8 * ct + (idx & 7);
+      hv[t][0] = hv[t][1] = hv[t][2] = V{0, 0};
+      if (idx < total && m < M && x < g.n0) {
+        const size_t o = (size_t)x * M + m;
+        hv[t][0] = Hx[o];
+        hv[t][1] = Hx[plane + o];
+        hv[t][2] = Hx[2 * plane + o];
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < STAGE_RHS; ++t) {
+    const int idx = threadIdx.x + t * blockDim.x;
+    if (idx >= total) continue;
+    const int x = idx >> 3, j = idx & 7;
+    const int m = 8 * ct + j;
+    T c1r = 0, c1i = 0, c0r = 0, c0i = 0, cgr = 0, cgi = 0;
+    if (m < M && x < g.n0) {
+      double kyd, kzd, kyo;
+      mode_k(g, m, kyd, kzd, kyo);
+      const T ky = (T)kyo, kz = (T)kzd, k2 = (T)(kyd * kyd + kzd * kzd);
+      const V h0 = hv[t][0], h1 = hv[t][1], h2 = hv[t][2];
+      c1r = -ky * h1.y - kz * h2.y;
+      c1i = ky * h1.x + kz * h2.x;
+      cgr = -kz * h1.y + ky * h2.y;
+      cgi = kz * h1.x - ky * h2.x;
+      c0r = k2 * h0.x;
+      c0i = k2 * h0.y;
+    }
+    B.at(0, x, 2 * j) = c1r;
+    B.at(0, x, 2 * j + 1) = c1i;
+    B.at(1, x, 2 * j) = c0r;
+    B.at(1, x, 2 * j + 1) = c0i;
+    B.at(2, x, 2 * j) = cgr;
+    B.at(2, x, 2 * j + 1) = cgi;
+    }
+  }
+  __syncthreads();
+  const Lane L = lane_of(g, ct);
+  for (int i0 = (threadIdx.x >> 6) * 16; i0 < g.nd; i0 += (blockDim.x >> 6) * 16) {
+    const int ia = i0 + (L.l & 15);
+    typename MM::acc AU{}, AG{};
+#pragma unroll 4
+    for (int kk = 0; kk < kp; kk += 4) {
+      const int x = kk + (L.l >> 4);
+      T a1 = 0, a0 = 0, ag = 0;
+      if (x < g.n0) {
+        a1 = nm.A1t[(size_t)x * nm.ldi + ia];
+        a0 = nm.A0t[(size_t)x * nm.ldi + ia];
+        ag = nm.AGt[(size_t)x * nm.ldi + ia];
+      }
+      AU = MM::mma(a1, B.at(0, x, L.c), AU);
+      AU = MM::mma(a0, B.at(1, x, L.c), AU);
+      AG = MM::mma(ag, B.at(2, x, L.c), AG);
+    }
+    const double W = g.W;
+    double *Nur = reinterpret_cast<double *>(Nu);
+    double *Ngr = reinterpret_cast<double *>(Ng);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int i = i0 + MM::row(L.l, r);
+      if (!L.valid) continue;
+      const size_t o = (size_t)i * 2 * M + 2 * L.mcol + L.comp;
+      if (i < g.nb) Nur[o] = W * (double)AU[r];
+      if (i < g.nd) Ngr[o] = W * (double)AG[r];
+    }
+  }
+  if (ct == 0) {
+    for (int i = threadIdx.x; i < g.nd; i += blockDim.x) {
+      T sv = 0, sw = 0;
+      for (int x = 0; x < g.n0; ++x) {
+        const T a = nm.AGt[(size_t)x * nm.ldi + i];
+        sv += a * Hx[plane + (size_t)x * M].x;
+        sw += a * Hx[2 * plane + (size_t)x * M].x;
+      }
+      Nv[i] = -(double)sv;
+      Nw[i] = -(double)sw;
+    }
+  }
+}
+
+// ------------------------------------------------------------ adjoint ----
+// mu_u0 = VB^T (X0 - i kz W1 + i ky W2), mu_u1 = VD^T (X1 + i kz W0) + VD1^T W2,
+// mu_u2 = VD^T (X2 - i ky W0) - VD1^T W1 (rows: Shen index i, K: nodes x).
+template <typename T>
+__global__ void __launch_bounds__(512) k_nodal_adj_mfma(Geo g, AdjMats<T> am,
+                                                        const typename VT<T>::V *__restrict__ X,
+                                                        double2 *__restrict__ mu0,
+                                                        double2 *__restrict__ mu1,
+                                                        double2 *__restrict__ mu2) {
+  using MM = Mfma<T>;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int ct = blockIdx.x;
+  const int M = g.m, kp = (g.n0 + 3) / 4 * 4;
+  const size_t plane = (size_t)g.n0 * M;
+  const Panel<T> B{reinterpret_cast<T *>(smem), kp};
+  using V = typename VT<T>::V;
+  const int total = kp * 8;
+  V xv[STAGE_NODAL_ADJ][6];
+#pragma unroll
+  for (int t = 0; t < STAGE_NODAL_ADJ; ++t) {
+    const int idx = threadIdx.x + t * blockDim.x;
+    const int x = idx >> 3, m = 8 * ct + (idx & 7);
+#pragma unroll
+    for (int f = 0; f < 6; ++f) xv[t][f] = V{0, 0};
+    if (idx < total && m < M && x < g.n0 && !nyq_mode(g, m)) {
+      const size_t o = (size_t)x * M + m;
+#pragma unroll
+      for (int f = 0; f < 6; ++f) xv[t][f] = X[f * plane + o];
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < STAGE_NODAL_ADJ; ++t) {
+    const int idx = threadIdx.x + t * blockDim.x;
+    if (idx >= total) continue;
+    const int x = idx >> 3, j = idx & 7;
+    const int m = 8 * ct + j;
+    T v[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    if (m < M && x < g.n0 && !nyq_mode(g, m)) {
+      double kyd, kzd, kyo;
+      mode_k(g, m, kyd, kzd, kyo);
+      const T ky = (T)kyo, kz = (T)kzd;
+      const V X0 = xv[t][0], X1 = xv[t][1], X2 = xv[t][2];
+      const V W0 = xv[t][3], W1 = xv[t][4], W2 = xv[t][5];
+      v[0] = X0.x + kz * W1.y - ky * W2.y;  // t0
+      v[1] = X0.y - kz * W1.x + ky * W2.x;
+      v[2] = X1.x - kz * W0.y;  // t1
+      v[3] = X1.y + kz * W0.x;
+      v[4] = X2.x + ky * W0.y;  // t2
+      v[5] = X2.y - ky * W0.x;
+      v[6] = W2.x;  // w2
+      v[7] = W2.y;
+      v[8] = -W1.x;  // -w1
+      v[9] = -W1.y;
+    }
+#pragma unroll
+    for (int f = 0; f < 5; ++f) {
+      B.at(f, x, 2 * j) = v[2 * f];
+      B.at(f, x, 2 * j + 1) = v[2 * f + 1];
+    }
+  }
+  __syncthreads();
+  const Lane L = lane_of(g, ct);
+  double *o0 = reinterpret_cast<double *>(mu0);
+  double *o1 = reinterpret_cast<double *>(mu1);
+  double *o2 = reinterpret_cast<double *>(mu2);
+  for (int i0 = (threadIdx.x >> 6) * 16; i0 < g.nd; i0 += (blockDim.x >> 6) * 16) {
+  const int ia = i0 + (L.l & 15);
+  typename MM::acc A0{}, A1{}, A2{};
+#pragma unroll 4
+  for (int kk = 0; kk < kp; kk += 4) {
+    const int x = kk + (L.l >> 4);
+    T ab = 0, ad = 0, ad1 = 0;
+    if (x < g.n0) {
+      ab = am.VB[(size_t)x * am.ldi + ia];
+      ad = am.VD[(size_t)x * am.ldi + ia];
+      ad1 = am.VD1[(size_t)x * am.ldi + ia];
+    }
+    A0 = MM::mma(ab, B.at(0, x, L.c), A0);
+    A1 = MM::mma(ad, B.at(1, x, L.c), A1);
+    A1 = MM::mma(ad1, B.at(3, x, L.c), A1);
+    A2 = MM::mma(ad, B.at(2, x, L.c), A2);
+    A2 = MM::mma(ad1, B.at(4, x, L.c), A2);
+  }
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int i = i0 + MM::row(L.l, r);
+    if (!L.valid) continue;
+    const size_t o = (size_t)i * 2 * M + 2 * L.mcol + L.comp;
+    if (i < g.nb) o0[o] = (double)A0[r];
+    if (i < g.nd) {
+      o1[o] = (double)A1[r];
+      o2[o] = (double)A2[r];
+    }
+  }
+  }
+}
+
+// mu_h0 = W k^2 A0^T nu_u, mu_h1 = -i W (ky A1^T nu_u + kz AG^T nu_g),
+// mu_h2 = -i W (kz A1^T nu_u - ky AG^T nu_g) (rows: nodes x, K: Shen index i);
+// mean mode: mu_h1,2 -= AG^T nu_v,w.  Nyquist row: zero.
+template <typename T>
+__global__ void __launch_bounds__(512) k_rhs_adj_mfma(Geo g, AdjMats<T> am, Prim nu,
+                                                      typename VT<T>::V *__restrict__ Hx) {
+  using MM = Mfma<T>;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int ct = blockIdx.x;
+  const int M = g.m, kp = (g.nd + 3) / 4 * 4;
+  const Panel<T> B{reinterpret_cast<T *>(smem), kp};
+  {
+    const int total = 2 * kp * 8;
+    double2 z[STAGE_RHS_ADJ];
+#pragma unroll
+    for (int t = 0; t < STAGE_RHS_ADJ; ++t) {
+      const int idx = threadIdx.x + t * blockDim.x;
+      const int f = idx / (kp * 8), rem = idx - f * kp * 8, i = rem >> 3, j = rem & 7;
+      const int m = 8 * ct + j;
+      z[t] = double2{0, 0};
+      if (idx < total && m < M && !nyq_mode(g, m)) {
+        if (f == 0 && i < g.nb) z[t] = nu.u0[(size_t)i * M + m];
+        if (f == 1 && i < g.nd) z[t] = nu.g[(size_t)i * M + m];
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < STAGE_RHS_ADJ; ++t) {
+      const int idx = threadIdx.x + t * blockDim.x;
+      if (idx >= total) continue;
+      const int f = idx / (kp * 8), rem = idx - f * kp * 8, i = rem >> 3, j = rem & 7;
+      B.at(f, i, 2 * j) = (T)z[t].x;
+      B.at(f, i, 2 * j + 1) = (T)z[t].y;
+    }
+  }
+  __syncthreads();
+  const Lane L = lane_of(g, ct);
+  double kyd, kzd, kyo;
+  mode_k(g, L.valid ? L.mcol : 0, kyd, kzd, kyo);
+  const bool live = L.valid && !nyq_mode(g, L.mcol);
+  const T W = (T)g.W, ky = (T)kyo, kz = (T)kzd, k2 = (T)(kyd * kyd + kzd * kzd);
+  const T s = L.comp ? (T)1 : (T)-1;  // -i z: re lane +z.im, im lane -z.re
+  const size_t plane = (size_t)g.n0 * 2 * M;
+  T *Hr = reinterpret_cast<T *>(Hx);
+  const bool mean_lane = L.valid && L.mcol == 0 && L.comp == 0;
+  for (int x0 = (threadIdx.x >> 6) * 16; x0 < g.n0; x0 += (blockDim.x >> 6) * 16) {
+  const int xa = x0 + (L.l & 15);
+  typename MM::acc P0{}, P1{}, PG{};
+#pragma unroll 4
+  for (int kk = 0; kk < kp; kk += 4) {
+    const int i = kk + (L.l >> 4);
+    T a0 = 0, a1 = 0, ag = 0;
+    if (i < g.nb) {
+      a0 = am.A0[(size_t)i * am.ldx + xa];
+      a1 = am.A1[(size_t)i * am.ldx + xa];
+    }
+    if (i < g.nd) ag = am.AG[(size_t)i * am.ldx + xa];
+    const T bu = B.at(0, i, L.c), bg = B.at(1, i, L.c);
+    P0 = MM::mma(a0, bu, P0);
+    P1 = MM::mma(a1, bu, P1);
+    PG = MM::mma(ag, bg, PG);
+  }
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const T p1 = partner(P1[r]), pg = partner(PG[r]);
+    T h0 = W * k2 * P0[r];
+    T h1 = -s * W * (ky * p1 + kz * pg);
+    T h2 = -s * W * (kz * p1 - ky * pg);
+    const int x = x0 + MM::row(L.l, r);
+    if (mean_lane && x < g.n0) {  // real part of the mean mode: - AG^T nu_v, - AG^T nu_w
+      T sv = 0, sw = 0;
+      for (int i = 0; i < g.nd; ++i) {
+        const T a = am.AG[(size_t)i * am.ldx + x];
+        sv += a * (T)nu.v[i];
+        sw += a * (T)nu.w[i];
+      }
+      h1 -= sv;
+      h2 -= sw;
+    }
+    if (L.valid && x < g.n0) {
+      const size_t o = (size_t)x * 2 * M + 2 * L.mcol + L.comp;
+      Hr[o] = live ? h0 : (T)0;
+      Hr[plane + o] = live ? h1 : (T)0;
+      Hr[2 * plane + o] = live ? h2 : (T)0;
+    }
+  }
+  }
+}
+
+}  // namespace
+
+// One block per tile of 8 modes, one wave per 16 output rows; LDS holds the
+// block's B panels: nf fields x roundup(K, 4) rows x 16 columns of T.
+static dim3 mfma_grid(const Geo &g) { return dim3((g.m + 7) / 8); }
+static dim3 mfma_block(int rows) { return dim3(64 * std::min(8, (rows + 15) / 16)); }
+// true when blockDim * stage loads cover `items` staged items
+static bool stage_covers(int rows, int stage, int items) { return (int)mfma_block(rows).x * stage >= items; }
+template <typename T>
+static size_t panel_bytes(int nf, int K) {
+  return (size_t)nf * ((K + 3) / 4 * 4) * 16 * sizeof(T);
+}
+
+template <typename T>
+hipError_t launch_nodal_mfma(const Geo &g, const NodalMats<T> &nm, const double2 *u0, const double2 *u1,
+                             const double2 *u2, Workspace &ws, hipStream_t s) {
+  using V = typename VT<T>::V;
+  if (!stage_covers(g.n0, STAGE_NODAL, 3 * ((g.nd + 3) / 4 * 4) * 8)) return hipErrorInvalidValue;
+  hipLaunchKernelGGL((k_nodal_mfma<T>), mfma_grid(g), mfma_block(g.n0), panel_bytes<T>(3, g.nd), s, g, nm,
+                     u0, u1, u2, reinterpret_cast<V *>(ws.X));
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_rhs_mfma(const Geo &g, const NodalMats<T> &nm, Workspace &ws, double2 *Nu, double2 *Ng,
+                           double *Nv, double *Nw, hipStream_t s) {
+  using V = typename VT<T>::V;
+  if (!stage_covers(g.nd, STAGE_RHS, ((g.n0 + 3) / 4 * 4) * 8)) return hipErrorInvalidValue;
+  hipLaunchKernelGGL((k_rhs_mfma<T>), mfma_grid(g), mfma_block(g.nd), panel_bytes<T>(3, g.n0), s, g, nm,
+                     reinterpret_cast<const V *>(ws.Hx), Nu, Ng, Nv, Nw);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_nodal_adj_mfma(const Geo &g, const AdjMats<T> &am, Workspace &ws, double2 *mu0,
+                                 double2 *mu1, double2 *mu2, hipStream_t s) {
+  using V = typename VT<T>::V;
+  if (!stage_covers(g.nd, STAGE_NODAL_ADJ, ((g.n0 + 3) / 4 * 4) * 8)) return hipErrorInvalidValue;
+  hipLaunchKernelGGL((k_nodal_adj_mfma<T>), mfma_grid(g), mfma_block(g.nd), panel_bytes<T>(5, g.n0), s, g,
+                     am, reinterpret_cast<const V *>(ws.X), mu0, mu1, mu2);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_rhs_adj_mfma(const Geo &g, const AdjMats<T> &am, Prim nu, Workspace &ws, hipStream_t s) {
+  using V = typename VT<T>::V;
+  if (!stage_covers(g.n0, STAGE_RHS_ADJ, 2 * ((g.nd + 3) / 4 * 4) * 8)) return hipErrorInvalidValue;
+  hipLaunchKernelGGL((k_rhs_adj_mfma<T>), mfma_grid(g), mfma_block(g.n0), panel_bytes<T>(2, g.nd), s, g, am,
+                     nu, reinterpret_cast<V *>(ws.Hx));
+  return hipGetLastError();
+}
+
+hipError_t mfma_init(const Geo &g, bool fp64) {
+  // the largest panel (five fields of the transform adjoint) may exceed the
+  // default 64 KB of dynamic LDS at the largest sizes
+  const size_t bytes = fp64 ? panel_bytes<double>(5, g.n0) : panel_bytes<float>(5, g.n0);
+  if (bytes <= 65536) return hipSuccess;
+  const void *fn = fp64 ? reinterpret_cast<const void *>(&k_nodal_adj_mfma<double>)
+                        : reinterpret_cast<const void *>(&k_nodal_adj_mfma<float>);
+  hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+  if (e != hipSuccess) (void)hipGetLastError();
+  return e;
+}
+
+#define PCF_MFMA_INSTANTIATE(T)                                                                     \
+  template hipError_t launch_nodal_mfma<T>(const Geo &, const NodalMats<T> &, const double2 *,      \
+                                           const double2 *, const double2 *, Workspace &,           \
+                                           hipStream_t);                                            \
+  template hipError_t launch_rhs_mfma<T>(const Geo &, const NodalMats<T> &, Workspace &, double2 *, \
+                                         double2 *, double *, double *, hipStream_t);               \
+  template hipError_t launch_nodal_adj_mfma<T>(const Geo &, const AdjMats<T> &, Workspace &,        \
+                                               double2 *, double2 *, double2 *, hipStream_t);       \
+  template hipError_t launch_rhs_adj_mfma<T>(const Geo &, const AdjMats<T> &, Prim, Workspace &,    \
+                                             hipStream_t);
+PCF_MFMA_INSTANTIATE(double)
+PCF_MFMA_INSTANTIATE(float)
+#undef PCF_MFMA_INSTANTIATE
+
+}  // namespace pcf
