@@ -529,7 +529,8 @@ __global__ void __launch_bounds__(256) k_adj_combine(Geo g, StructMats sm, const
 // are the forward u, omega; 6-8 the cotangent of H.  With a = u + (0, U x, 0),
 // b = omega + (0, 0, U) and c = -mu_H / (MY MZ) on the grid:
 //   mu_u = b x c,  mu_omega = c x a.
-template <typename T, int MAXB>
+// NZC > 0: padded spanwise length fixed at compile time (0: from g).
+template <typename T, int MAXB, int NZC>
 __global__ void __launch_bounds__(256) k_rot_adj(Geo g, NodalMats<T> nm, FFTPlan plan, int UPB,
                                                  const typename VT<T>::V *__restrict__ Yf,
                                                  const typename VT<T>::V *__restrict__ Hz,
@@ -538,11 +539,12 @@ __global__ void __launch_bounds__(256) k_rot_adj(Geo g, NodalMats<T> nm, FFTPlan
   extern __shared__ __align__(16) unsigned char smem[];
   V *buf = reinterpret_cast<V *>(smem);  // [9][UPB][MZ]
   const int tid = threadIdx.x, nthr = blockDim.x;
-  const int MZ = g.mz, KZ = g.kzs, MY = g.my;
+  const int MZ = NZC ? NZC : g.mz, KZ = g.kzs, MY = g.my;
   const int nunits = g.n0 * (MY / 2);
   const int ubase = blockIdx.x * UPB;
   const size_t fstride = (size_t)g.n0 * MY * KZ;
-  const V *tw = reinterpret_cast<const V *>(nm.twz);
+  V *tw = buf + (size_t)9 * UPB * MZ;  // twiddle table in LDS
+  for (int i = tid; i < MZ; i += nthr) tw[i] = reinterpret_cast<const V *>(nm.twz)[i];
 
   for (int idx = tid; idx < 9 * UPB * MZ; idx += nthr) {
     const int f = idx / (UPB * MZ), rem = idx - f * UPB * MZ;
@@ -567,7 +569,10 @@ __global__ void __launch_bounds__(256) k_rot_adj(Geo g, NodalMats<T> nm, FFTPlan
     buf[idx] = z;
   }
   __syncthreads();
-  fft_lds<V, T, true, false, MAXB>(buf, tw, plan, 9 * UPB, MZ, 1);
+  if constexpr (NZC > 0)
+    fft_lds_c<V, T, NZC, true, false, MAXB>(buf, tw, 9 * UPB, MZ, 1);
+  else
+    fft_lds<V, T, true, false, MAXB>(buf, tw, plan, 9 * UPB, MZ, 1);
 
   const T U = (T)g.uwall;
   const T sc = (T)(-1.0 / ((double)MY * (double)MZ));
@@ -603,7 +608,10 @@ __global__ void __launch_bounds__(256) k_rot_adj(Geo g, NodalMats<T> nm, FFTPlan
     for (int f = 0; f < 6; ++f) p[f * fs] = o[f];
   }
   __syncthreads();
-  fft_lds<V, T, false, false, MAXB>(buf, tw, plan, 6 * UPB, MZ, 1);
+  if constexpr (NZC > 0)
+    fft_lds_c<V, T, NZC, false, false, MAXB>(buf, tw, 6 * UPB, MZ, 1);
+  else
+    fft_lds<V, T, false, false, MAXB>(buf, tw, plan, 6 * UPB, MZ, 1);
 
   for (int idx = tid; idx < 6 * UPB * KZ; idx += nthr) {
     const int f = idx / (UPB * KZ), rem = idx - f * UPB * KZ;
@@ -700,7 +708,7 @@ template <typename T>
 int rot_adj_units(const Geo &g) {
   using V = typename VT<T>::V;
   int upb = 2;
-  while (upb > 1 && (size_t)upb * 9 * g.mz * sizeof(V) > 65536) upb >>= 1;
+  while (upb > 1 && ((size_t)upb * 9 * g.mz + g.mz) * sizeof(V) > 65536) upb >>= 1;
   return upb;
 }
 
@@ -744,19 +752,29 @@ hipError_t launch_rot_adj(const Geo &g, const NodalMats<T> &nm, const void *Yf, 
   using V = typename VT<T>::V;
   const int upb = rot_adj_units<T>(g);
   const int nunits = g.n0 * (g.my / 2);
-  const size_t lds = (size_t)upb * 9 * g.mz * sizeof(V);
+  const size_t lds = ((size_t)upb * 9 * g.mz + g.mz) * sizeof(V);
   const int mb = max_bf(pz, upb * 9);
   const dim3 grid((nunits + upb - 1) / upb);
   const V *yf = reinterpret_cast<const V *>(Yf);
   const V *hz = reinterpret_cast<const V *>(ws.Hz);
   V *yo = reinterpret_cast<V *>(ws.Y);
-#define PCF_ROTADJ(MB) \
-  hipLaunchKernelGGL((k_rot_adj<T, MB>), grid, dim3(256), lds, s, g, nm, pz, upb, yf, hz, yo)
-  if (mb <= 3) PCF_ROTADJ(3);
-  else if (mb <= 6) PCF_ROTADJ(6);
-  else if (mb <= 12) PCF_ROTADJ(12);
-  else PCF_ROTADJ(18);
+#define PCF_ROTADJN(MB, NZ) \
+  hipLaunchKernelGGL((k_rot_adj<T, MB, NZ>), grid, dim3(256), lds, s, g, nm, pz, upb, yf, hz, yo)
+#define PCF_ROTADJ(MB)                            \
+  switch (g.mz) {                                 \
+    case 24: PCF_ROTADJN(MB, 24); break;          \
+    case 48: PCF_ROTADJN(MB, 48); break;          \
+    case 96: PCF_ROTADJN(MB, 96); break;          \
+    case 144: PCF_ROTADJN(MB, 144); break;        \
+    case 192: PCF_ROTADJN(MB, 192); break;        \
+    default: PCF_ROTADJN(MB, 0);                  \
+  }
+  if (mb <= 3) PCF_ROTADJ(3)
+  else if (mb <= 6) PCF_ROTADJ(6)
+  else if (mb <= 12) PCF_ROTADJ(12)
+  else PCF_ROTADJ(18)
 #undef PCF_ROTADJ
+#undef PCF_ROTADJN
   return hipGetLastError();
 }
 
@@ -777,7 +795,7 @@ hipError_t launch_energy_seed(const Geo &g, const double *QB, const double *QD, 
 
 bool rot_adj_fits(const Geo &g, const FFTPlan &pz, int elem_bytes) {
   const int upb = (elem_bytes == 16) ? rot_adj_units<double>(g) : rot_adj_units<float>(g);
-  if ((size_t)upb * 9 * g.mz * elem_bytes > 65536) return false;
+  if (((size_t)upb * 9 * g.mz + g.mz) * elem_bytes > 65536) return false;
   return max_bf(pz, upb * 9) <= 18;
 }
 

@@ -133,6 +133,79 @@ __device__ void fft_lds(V *buf, const V *tw, const FFTPlan &plan, int nbatch, in
   }
 }
 
+// Compile-time-length variant: with N, the radix sequence and every stride
+// known to the compiler, the index arithmetic of each pass is shifts and
+// multiplications instead of integer divisions by runtime values.
+struct CPlan {
+  int nst;
+  int r[12];
+};
+__host__ __device__ constexpr CPlan cplan(int n) {
+  CPlan p{0, {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0}};
+  int m = n;
+  while (m % 4 == 0) p.r[p.nst++] = 4, m /= 4;
+  while (m % 2 == 0) p.r[p.nst++] = 2, m /= 2;
+  while (m % 3 == 0) p.r[p.nst++] = 3, m /= 3;
+  return p;
+}
+
+template <typename V, typename T, int N, int R, int P, bool INV, bool BATCH_FAST, int MAXB>
+__device__ __forceinline__ void fft_pass_c(V *buf, const V *tw, int nbatch, int bstride, int estride,
+                                           int tid, int nthr) {
+  constexpr int nbf = N / R;
+  constexpr int twstep = N / (P * R);
+  const int total = nbf * nbatch;
+  V v[MAXB][R];
+#pragma unroll
+  for (int t = 0; t < MAXB; ++t) {
+    const int idx = tid + t * nthr;
+    if (idx < total) {
+      const int b = BATCH_FAST ? idx % nbatch : idx / nbf;
+      const int i = BATCH_FAST ? idx / nbatch : idx % nbf;
+      const V *base = buf + (size_t)b * bstride;
+#pragma unroll
+      for (int j = 0; j < R; ++j) v[t][j] = base[(i + j * nbf) * estride];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int t = 0; t < MAXB; ++t) {
+    const int idx = tid + t * nthr;
+    if (idx < total) {
+      const int b = BATCH_FAST ? idx % nbatch : idx / nbf;
+      const int i = BATCH_FAST ? idx / nbatch : idx % nbf;
+      const int k = i % P;
+      if constexpr (P > 1) {
+#pragma unroll
+        for (int j = 1; j < R; ++j) v[t][j] = cmul(v[t][j], tw_get<V, T, INV>(tw, j * k * twstep));
+      }
+      dft_small<V, T, R, INV>(v[t]);
+      V *base = buf + (size_t)b * bstride;
+      const int o = (i - k) * R + k;
+#pragma unroll
+      for (int j = 0; j < R; ++j) base[(o + j * P) * estride] = v[t][j];
+    }
+  }
+  __syncthreads();
+}
+
+template <typename V, typename T, int N, int S, int P, bool INV, bool BATCH_FAST, int MAXB>
+__device__ __forceinline__ void fft_stages_c(V *buf, const V *tw, int nbatch, int bstride, int estride,
+                                             int tid, int nthr) {
+  constexpr CPlan cp = cplan(N);
+  if constexpr (S < cp.nst) {
+    constexpr int R = cp.r[S];
+    fft_pass_c<V, T, N, R, P, INV, BATCH_FAST, MAXB>(buf, tw, nbatch, bstride, estride, tid, nthr);
+    fft_stages_c<V, T, N, S + 1, P * R, INV, BATCH_FAST, MAXB>(buf, tw, nbatch, bstride, estride, tid, nthr);
+  }
+}
+
+template <typename V, typename T, int N, bool INV, bool BATCH_FAST, int MAXB>
+__device__ void fft_lds_c(V *buf, const V *tw, int nbatch, int bstride, int estride) {
+  fft_stages_c<V, T, N, 0, 1, INV, BATCH_FAST, MAXB>(buf, tw, nbatch, bstride, estride, threadIdx.x,
+                                                      blockDim.x);
+}
+
 // ---------------------------------------------------- wavenumber helpers ----
 __device__ __forceinline__ void mode_k(const Geo &g, int m, double &ky, double &kz, double &ky_odd) {
   const int iy = m / g.kzs, iz = m - iy * g.kzs;

@@ -31,17 +31,22 @@ namespace pcf {
 // One block per (kz chunk, node x, field): the streamwise lines of KC spanwise
 // modes.  INV: spectral [iy][kz] -> zero-padded [y][kz] lines, inverse FFT.
 // !INV: [y][kz] lines -> forward FFT -> truncated [iy][kz], Nyquist row zeroed.
-template <typename T, bool INV, int MAXB>
+// NYC, KCC > 0: padded streamwise length and column batch fixed at compile
+// time (the launcher dispatches the common values); 0 reads them at run time.
+template <typename T, bool INV, int MAXB, int NYC, int KCC>
 __global__ void __launch_bounds__(256) k_fft_y(Geo g, const typename VT<T>::V *__restrict__ tw,
-                                               FFTPlan plan, int KC,
+                                               FFTPlan plan, int KCr,
                                                const typename VT<T>::V *__restrict__ src,
                                                typename VT<T>::V *__restrict__ dst) {
   using V = typename VT<T>::V;
   extern __shared__ __align__(16) unsigned char smem[];
   V *buf = reinterpret_cast<V *>(smem);
+  const int KC = KCC ? KCC : KCr;
   const int kc0 = blockIdx.x * KC, x = blockIdx.y, f = blockIdx.z;
   const int tid = threadIdx.x, nthr = blockDim.x;
-  const int MY = g.my, NY = g.ny, KZ = g.kzs;
+  const int MY = NYC ? NYC : g.my, NY = g.ny, KZ = g.kzs;
+  V *twl = buf + (size_t)MY * KC;  // twiddle table in LDS: the passes never touch global memory
+  for (int i = tid; i < MY; i += nthr) twl[i] = tw[i];
   if (INV) {
     const V *s = src + ((size_t)f * g.n0 + x) * g.m;
     for (int idx = tid; idx < MY * KC; idx += nthr) buf[idx] = V{0, 0};
@@ -52,7 +57,10 @@ __global__ void __launch_bounds__(256) k_fft_y(Geo g, const typename VT<T>::V *_
       buf[y * KC + c] = s[(size_t)iy * KZ + kc0 + c];
     }
     __syncthreads();
-    fft_lds<V, T, true, true, MAXB>(buf, tw, plan, KC, 1, KC);
+    if constexpr (NYC > 0)
+      fft_lds_c<V, T, NYC, true, true, MAXB>(buf, twl, KC, 1, KC);
+    else
+      fft_lds<V, T, true, true, MAXB>(buf, twl, plan, KC, 1, KC);
     V *d = dst + ((size_t)f * g.n0 + x) * (size_t)MY * KZ;
     for (int idx = tid; idx < MY * KC; idx += nthr) {
       int y = idx / KC, c = idx - y * KC;
@@ -65,7 +73,10 @@ __global__ void __launch_bounds__(256) k_fft_y(Geo g, const typename VT<T>::V *_
       buf[idx] = s[(size_t)y * KZ + kc0 + c];
     }
     __syncthreads();
-    fft_lds<V, T, false, true, MAXB>(buf, tw, plan, KC, 1, KC);
+    if constexpr (NYC > 0)
+      fft_lds_c<V, T, NYC, false, true, MAXB>(buf, twl, KC, 1, KC);
+    else
+      fft_lds<V, T, false, true, MAXB>(buf, twl, plan, KC, 1, KC);
     V *d = dst + ((size_t)f * g.n0 + x) * g.m;
     for (int idx = tid; idx < NY * KC; idx += nthr) {
       int iy = idx / KC, c = idx - iy * KC;
@@ -81,20 +92,24 @@ __global__ void __launch_bounds__(256) k_fft_y(Geo g, const typename VT<T>::V *_
 // nodal fields.  The two real lines of a field ride in one complex FFT
 // (line A as the real part, line B as the imaginary part), both for the
 // c2r synthesis and the r2c analysis.
-template <typename T, int MAXB>
+// NZC > 0: the padded spanwise length is the compile-time NZC (the launcher
+// dispatches the common lengths); NZC = 0 reads it from g.
+template <typename T, int MAXB, int NZC>
 __global__ void __launch_bounds__(256) k_rotational(Geo g, NodalMats<T> nm, FFTPlan plan, int UPB,
                                                     const typename VT<T>::V *__restrict__ Y,
                                                     typename VT<T>::V *__restrict__ Hz) {
   using V = typename VT<T>::V;
   extern __shared__ __align__(16) unsigned char smem[];
+  const int MZ = NZC ? NZC : g.mz;
   V *buf = reinterpret_cast<V *>(smem);  // [UPB][6][MZ]
-  V *out = buf + (size_t)UPB * 6 * g.mz; // [UPB][3][MZ]
+  V *out = buf + (size_t)UPB * 6 * MZ;   // [UPB][3][MZ]
   const int tid = threadIdx.x, nthr = blockDim.x;
-  const int MZ = g.mz, KZ = g.kzs, MY = g.my;
+  const int KZ = g.kzs, MY = g.my;
   const int nunits = g.n0 * (MY / 2);
   const int ubase = blockIdx.x * UPB;
   const size_t fstride = (size_t)g.n0 * MY * KZ;  // field stride in Y / Hz
-  const V *tw = reinterpret_cast<const V *>(nm.twz);
+  V *tw = out + (size_t)UPB * 3 * g.mz;            // twiddle table in LDS
+  for (int i = tid; i < MZ; i += nthr) tw[i] = reinterpret_cast<const V *>(nm.twz)[i];
 
   for (int idx = tid; idx < UPB * 6 * MZ; idx += nthr) {
     int lu = idx / (6 * MZ), rem = idx - lu * 6 * MZ;
@@ -118,7 +133,10 @@ __global__ void __launch_bounds__(256) k_rotational(Geo g, NodalMats<T> nm, FFTP
     buf[idx] = z;
   }
   __syncthreads();
-  fft_lds<V, T, true, false, MAXB>(buf, tw, plan, UPB * 6, MZ, 1);
+  if constexpr (NZC > 0)
+    fft_lds_c<V, T, NZC, true, false, MAXB>(buf, tw, UPB * 6, MZ, 1);
+  else
+    fft_lds<V, T, true, false, MAXB>(buf, tw, plan, UPB * 6, MZ, 1);
 
   const T U = (T)g.uwall;
   for (int idx = tid; idx < UPB * MZ; idx += nthr) {
@@ -147,7 +165,10 @@ __global__ void __launch_bounds__(256) k_rotational(Geo g, NodalMats<T> nm, FFTP
     o[2 * MZ] = h2;
   }
   __syncthreads();
-  fft_lds<V, T, false, false, MAXB>(out, tw, plan, UPB * 3, MZ, 1);
+  if constexpr (NZC > 0)
+    fft_lds_c<V, T, NZC, false, false, MAXB>(out, tw, UPB * 3, MZ, 1);
+  else
+    fft_lds<V, T, false, false, MAXB>(out, tw, plan, UPB * 3, MZ, 1);
 
   const T scale = (T)(1.0 / ((double)MY * (double)MZ));
   for (int idx = tid; idx < UPB * 3 * KZ; idx += nthr) {
@@ -1166,23 +1187,39 @@ hipError_t launch_fft_y_fields(const Geo &g, const NodalMats<T> &nm, const FFTPl
                                int nf, const void *src_, void *dst_, hipStream_t s) {
   using V = typename VT<T>::V;
   const int kc = pick_kc(g.kzs);
-  const size_t lds = (size_t)g.my * kc * sizeof(V);
+  const size_t lds = ((size_t)g.my * kc + g.my) * sizeof(V);
   const int mb = max_butterflies(py, kc);
   const dim3 grid(g.kzs / kc, g.n0, nf);
   const V *tw = reinterpret_cast<const V *>(nm.twy);
   const V *src = reinterpret_cast<const V *>(src_);
   V *dst = reinterpret_cast<V *>(dst_);
-#define PCF_FFTY(INV, MB) hipLaunchKernelGGL((k_fft_y<T, INV, MB>), grid, dim3(256), lds, s, g, tw, py, kc, src, dst)
+#define PCF_FFTYN(INV, MB, NY, KC) \
+  hipLaunchKernelGGL((k_fft_y<T, INV, MB, NY, KC>), grid, dim3(256), lds, s, g, tw, py, kc, src, dst)
+#define PCF_FFTYK(INV, MB, NY)                       \
+  if (kc == 16) PCF_FFTYN(INV, MB, NY, 16);          \
+  else if (kc == 8) PCF_FFTYN(INV, MB, NY, 8);       \
+  else PCF_FFTYN(INV, MB, NY, 0);
+#define PCF_FFTY(INV, MB)                            \
+  switch (g.my) {                                    \
+    case 24: { PCF_FFTYK(INV, MB, 24) } break;       \
+    case 48: { PCF_FFTYK(INV, MB, 48) } break;       \
+    case 96: { PCF_FFTYK(INV, MB, 96) } break;       \
+    case 144: { PCF_FFTYK(INV, MB, 144) } break;     \
+    case 192: { PCF_FFTYK(INV, MB, 192) } break;     \
+    default: PCF_FFTYN(INV, MB, 0, 0);               \
+  }
   if (inverse) {
-    if (mb <= 2) PCF_FFTY(true, 2);
-    else if (mb <= 4) PCF_FFTY(true, 4);
-    else PCF_FFTY(true, 8);
+    if (mb <= 2) PCF_FFTY(true, 2)
+    else if (mb <= 4) PCF_FFTY(true, 4)
+    else PCF_FFTY(true, 8)
   } else {
-    if (mb <= 2) PCF_FFTY(false, 2);
-    else if (mb <= 4) PCF_FFTY(false, 4);
-    else PCF_FFTY(false, 8);
+    if (mb <= 2) PCF_FFTY(false, 2)
+    else if (mb <= 4) PCF_FFTY(false, 4)
+    else PCF_FFTY(false, 8)
   }
 #undef PCF_FFTY
+#undef PCF_FFTYK
+#undef PCF_FFTYN
   return hipGetLastError();
 }
 
@@ -1199,7 +1236,7 @@ int rot_units(const Geo &g) {
   // two y-line pairs per block keep the butterfly registers small; LDS holds
   // 9 lines of MZ per unit
   int upb = 2;
-  while (upb > 1 && (size_t)upb * 9 * g.mz * sizeof(V) > 65536) upb >>= 1;
+  while (upb > 1 && ((size_t)upb * 9 * g.mz + g.mz) * sizeof(V) > 65536) upb >>= 1;
   return upb;
 }
 
@@ -1209,16 +1246,27 @@ hipError_t launch_rotational(const Geo &g, const NodalMats<T> &nm, Workspace &ws
   using V = typename VT<T>::V;
   const int upb = rot_units<T>(g);
   const int nunits = g.n0 * (g.my / 2);
-  const size_t lds = (size_t)upb * 9 * g.mz * sizeof(V);
+  const size_t lds = ((size_t)upb * 9 * g.mz + g.mz) * sizeof(V);
   const int mb = max_butterflies(pz, upb * 6);
   const dim3 grid((nunits + upb - 1) / upb);
   const V *Y = reinterpret_cast<const V *>(ws.Y);
   V *Hz = reinterpret_cast<V *>(ws.Hz);
-#define PCF_ROT(MB) hipLaunchKernelGGL((k_rotational<T, MB>), grid, dim3(256), lds, s, g, nm, pz, upb, Y, Hz)
-  if (mb <= 3) PCF_ROT(3);
-  else if (mb <= 6) PCF_ROT(6);
-  else PCF_ROT(12);
+#define PCF_ROTN(MB, NZ) \
+  hipLaunchKernelGGL((k_rotational<T, MB, NZ>), grid, dim3(256), lds, s, g, nm, pz, upb, Y, Hz)
+#define PCF_ROT(MB)                              \
+  switch (g.mz) {                                \
+    case 24: PCF_ROTN(MB, 24); break;            \
+    case 48: PCF_ROTN(MB, 48); break;            \
+    case 96: PCF_ROTN(MB, 96); break;            \
+    case 144: PCF_ROTN(MB, 144); break;          \
+    case 192: PCF_ROTN(MB, 192); break;          \
+    default: PCF_ROTN(MB, 0);                    \
+  }
+  if (mb <= 3) PCF_ROT(3)
+  else if (mb <= 6) PCF_ROT(6)
+  else PCF_ROT(12)
 #undef PCF_ROT
+#undef PCF_ROTN
   return hipGetLastError();
 }
 
@@ -1284,10 +1332,10 @@ hipError_t launch_diag(const Geo &g, const double *VB, const double *VD, const d
 
 bool fft_fits(const Geo &g, const FFTPlan &py, const FFTPlan &pz, int elem_bytes) {
   const int kc = pick_kc(g.kzs);
-  if ((size_t)g.my * kc * elem_bytes > 65536) return false;
+  if (((size_t)g.my * kc + g.my) * elem_bytes > 65536) return false;
   if (max_butterflies(py, kc) > 8) return false;
   const int upb = (elem_bytes == 16) ? rot_units<double>(g) : rot_units<float>(g);
-  if ((size_t)upb * 9 * g.mz * elem_bytes > 65536) return false;
+  if (((size_t)upb * 9 * g.mz + g.mz) * elem_bytes > 65536) return false;
   return max_butterflies(pz, upb * 6) <= 12;
 }
 
