@@ -113,19 +113,19 @@ def run_gpu(precision: str, steps: int, warmup: int, eager: bool = False):
     return solver, state, (t1 - t0), stages, diag
 
 
-# stage name -> kernel-name fragment in the rocprofv3 traffic summary
-_KERNEL_OF = {"nodal_backward": "k_nodal_backward", "fft_y_inverse": "k_fft_y<double, true",
-              "rotational": "k_rotational", "fft_y_forward": "k_fft_y<double, false",
-              "forward_rhs": "k_forward_rhs", "solve": "k_solve_steady", "reconstruct": "k_reconstruct"}
+# stage name -> kernel-name fragment (FP64 instance) in the rocprofv3 traffic summary
+_KERNEL_OF = {"nodal_backward": "k_nodal_mfma<double>", "fft_y_inverse": "k_fft_y<double, true",
+              "rotational": "k_rotational<double", "fft_y_forward": "k_fft_y<double, false",
+              "forward_rhs": "k_rhs_mfma<double>", "solve": "k_solve_steady", "reconstruct": "k_reconstruct"}
+TRAFFIC_FILE = ROOT / "profiles" / "traffic_65x64x64.json"
 
 
 def measured_traffic(stage: str, precision: str):
     """HBM bytes per launch from the committed counter summary (FETCH_SIZE and
     WRITE_SIZE runs, copy-calibrated), or None when no summary covers it."""
-    path = ROOT / "profiles" / f"traffic_65x64x64_{precision}.json"
-    if not path.exists():
+    if not TRAFFIC_FILE.exists():
         return None
-    data = json.loads(path.read_text())
+    data = json.loads(TRAFFIC_FILE.read_text())
     frag = _KERNEL_OF.get(stage, stage)
     if precision != "fp64":
         frag = frag.replace("double", "float")

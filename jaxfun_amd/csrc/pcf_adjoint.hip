@@ -546,28 +546,31 @@ __global__ void __launch_bounds__(256) k_rot_adj(Geo g, NodalMats<T> nm, FFTPlan
   V *tw = buf + (size_t)9 * UPB * MZ;  // twiddle table in LDS
   for (int i = tid; i < MZ; i += nthr) tw[i] = reinterpret_cast<const V *>(nm.twz)[i];
 
-  for (int idx = tid; idx < 9 * UPB * MZ; idx += nthr) {
-    const int f = idx / (UPB * MZ), rem = idx - f * UPB * MZ;
-    const int lu = rem / MZ, k = rem - lu * MZ;
-    const int u = ubase + lu;
-    V z{0, 0};
-    if (u < nunits) {
-      const int x = u / (MY / 2), y2 = u - x * (MY / 2);
-      const V *src = (f < 6) ? Yf + f * fstride : Hz + (f - 6) * fstride;
-      const V *ra = src + ((size_t)x * MY + 2 * y2) * KZ;
-      const V *rb = ra + KZ;
-      if (k == 0) {
-        z = V{ra[0].x, rb[0].x};
-      } else if (k < KZ) {
-        V a = ra[k], b = rb[k];
-        z = V{a.x - b.y, a.y + b.x};
-      } else if (k > MZ - KZ) {
-        V a = ra[MZ - k], b = rb[MZ - k];
-        z = V{a.x + b.y, b.x - a.y};
-      }
-    }
-    buf[idx] = z;
-  }
+  gather_batched<4>(
+      9 * UPB * MZ,
+      [&](int idx) {
+        const int f = idx / (UPB * MZ), rem = idx - f * UPB * MZ;
+        const int lu = rem / MZ, k = rem - lu * MZ;
+        const int u = ubase + lu;
+        V z{0, 0};
+        if (u < nunits) {
+          const int x = u / (MY / 2), y2 = u - x * (MY / 2);
+          const V *src = (f < 6) ? Yf + f * fstride : Hz + (f - 6) * fstride;
+          const V *ra = src + ((size_t)x * MY + 2 * y2) * KZ;
+          const V *rb = ra + KZ;
+          if (k == 0) {
+            z = V{ra[0].x, rb[0].x};
+          } else if (k < KZ) {
+            const V a = ra[k], b = rb[k];
+            z = V{a.x - b.y, a.y + b.x};
+          } else if (k > MZ - KZ) {
+            const V a = ra[MZ - k], b = rb[MZ - k];
+            z = V{a.x + b.y, b.x - a.y};
+          }
+        }
+        return z;
+      },
+      [&](int idx, const V &z) { buf[idx] = z; });
   __syncthreads();
   if constexpr (NZC > 0)
     fft_lds_c<V, T, NZC, true, false, MAXB>(buf, tw, 9 * UPB, MZ, 1);

@@ -59,6 +59,10 @@ hipError_t launch_adj_solve(const Geo &g, const StructMats &smt, double gamma, P
 hipError_t launch_adj_combine(const Geo &g, const StructMats &sm, const double *RcT, int ldk,
                               const double2 *mu0, const double2 *mu1, const double2 *mu2,
                               const double2 *g_extra, const PrimSum &gather, Prim out, hipStream_t s);
+// Steady transposed solve with the factors of the transposed blocks (smt).
+hipError_t launch_adj_solve_factored(const Geo &g, const StructMats &smt, const FactorSet &fs, PrimC in,
+                                     Prim out, Prim nu, PrimC r1, PrimC r2, double e0, double e1, double e2,
+                                     hipStream_t s);
 // out += (M + gamma L)^T r_stage (if r_stage.u0) + M^T (sum of gather terms).
 hipError_t launch_adj_mass(const Geo &g, const StructMats &sm, double gamma, PrimC r_stage,
                            const PrimSum &gather, Prim out, hipStream_t s);

@@ -107,6 +107,7 @@ struct pcf_handle {
   DevMats<double> md;
   DevMats<float> mf;
   pcf::StructMats sm{};
+  pcf::FactorSet fsteady{};   // LU factors of the steady SBDF3 systems
   double *Rt = nullptr;       // parity-compressed reconstruction rows, [nd][ldr]
   int ldr = 0;
   double *scratch = nullptr;  // per-row data of the structured solves
@@ -124,6 +125,7 @@ struct pcf_handle {
   struct Adj {
     bool ready = false;
     pcf::StructMats smt{};        // transposed, index-reversed parity blocks
+    pcf::FactorSet fsteady_t{};   // their steady LU factors
     double *RcT = nullptr;        // parity-compressed R^T rows, [nb][ldrT]
     int ldrT = 0;
     double *QB = nullptr, *QD = nullptr;  // Fejer-weighted Gram matrices of the bases
@@ -237,6 +239,20 @@ int32_t build_struct_mats(pcf_handle *h) {
   return PCF_OK;
 }
 
+// Storage for one FactorSet and its factorisation at gamma.
+int32_t build_factors(pcf_handle *h, const pcf::StructMats &sm, double gamma, pcf::FactorSet &fs) {
+  int32_t r;
+  for (int b = 0; b < 2; ++b)
+    for (int p = 0; p < 2; ++p)
+      if ((r = h->alloc(&fs.blk[b][p], pcf::factor_doubles(h->geo, sm, b, p)))) return r;
+  for (int p = 0; p < 2; ++p)
+    if ((r = h->alloc(&fs.mean[p], pcf::factor_doubles(h->geo, sm, -1, p)))) return r;
+  hipError_t e = pcf::launch_factor(h->geo, sm, gamma, fs, h->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  if (e != hipSuccess) return fail(PCF_E_HIP, std::string("factorisation: ") + hipGetErrorString(e));
+  return PCF_OK;
+}
+
 // Stage names, in launch order, for per-kernel timing.
 constexpr int NSTAGE = 7;
 const char *const STAGE_NAMES[NSTAGE] = {"nodal_backward", "fft_y_inverse", "rotational",
@@ -292,7 +308,9 @@ int32_t nonlinear_into(pcf_handle *h, const double2 *u0, int slot, void *ytape =
 hipError_t implicit_update(pcf_handle *h, const pcf::SolveArgs &a, bool steady) {
   hipError_t e;
   if ((e = mark(h, 5, 0))) return e;
-  if ((e = pcf::launch_solve(h->geo, h->sm, a, h->scratch, steady, h->stream))) return e;
+  if ((e = steady ? pcf::launch_solve_factored(h->geo, h->sm, h->fsteady, a, h->stream)
+                  : pcf::launch_solve(h->geo, h->sm, a, h->scratch, h->stream)))
+    return e;
   if ((e = mark(h, 5, 1))) return e;
   if ((e = mark(h, 6, 0))) return e;
   if ((e = pcf::launch_reconstruct(h->geo, h->Rt, h->ldr, a, h->stream))) return e;
@@ -636,6 +654,7 @@ int32_t adj_setup(pcf_handle *h) {
       if ((r = h->upload(&d, Vt))) return r;
       ss.V = d;
     }
+  if ((r = build_factors(h, A.smt, h->cfg.dt * 6.0 / 11.0, A.fsteady_t))) return r;
   // RcT[j][k] = R[2k + 1 - (j & 1)][j]
   A.ldrT = (o.nd + 2) / 2;
   std::vector<double> RcT((size_t)o.nb * A.ldrT, 0.0);
@@ -757,9 +776,9 @@ hipError_t reverse_step(pcf_handle *h, int k, bool steady, void *const *tape) {
   hipError_t er;
   if (steady) {
     // rho_k and, in the same pass, mu_{N_k} = e0 rho_k + e1 rho_k+1 + e2 rho_k+2
-    if ((er = pcf::launch_adj_solve(g, A.smt, dt * 6.0 / 11.0, cprim(A.muX), A.rho[rr[0]], h->scratch,
-                                    h->stream, A.nu, cprim(A.rho[rr[1]]), cprim(A.rho[rr[2]]), e[0], e[1],
-                                    e[2])))
+    if ((er = pcf::launch_adj_solve_factored(g, A.smt, A.fsteady_t, cprim(A.muX), A.rho[rr[0]], A.nu,
+                                             cprim(A.rho[rr[1]]), cprim(A.rho[rr[2]]), e[0], e[1], e[2],
+                                             h->stream)))
       return er;
     pcf::PrimSum nu{};
     ps_add(nu, A.nu, 1.0);
@@ -940,11 +959,11 @@ int32_t pcf_create(const pcf_config *cfg, pcf_handle **out) {
   }
   if ((r = build_struct_mats(h))) return bail(r);
   {
-    hipError_t e = pcf::kernels_init(h->geo);
-    if (e == hipSuccess) e = pcf::mfma_init(h->geo, cfg->precision == PCF_PREC_FP64);
+    hipError_t e = pcf::mfma_init(h->geo, cfg->precision == PCF_PREC_FP64);
     if (e != hipSuccess)
       return bail(fail(PCF_E_UNSUPPORTED, std::string("kernel LDS limits: ") + hipGetErrorString(e)));
   }
+  if ((r = build_factors(h, h->sm, cfg->dt * 6.0 / 11.0, h->fsteady))) return bail(r);
   if ((r = h->upload(&h->dVB, h->ops.VB.a))) return bail(r);
   if ((r = h->upload(&h->dVD, h->ops.VD.a))) return bail(r);
   if ((r = h->upload(&h->dVB1, h->ops.VB1.a))) return bail(r);

@@ -206,6 +206,28 @@ __device__ void fft_lds_c(V *buf, const V *tw, int nbatch, int bstride, int estr
                                                       blockDim.x);
 }
 
+// Block-cooperative gather: element idx < total of the block's staging set is
+// load(idx) (a global read, possibly several) and goes to store(idx, value)
+// (an LDS write).  B elements per thread are loaded before any is stored, so
+// B global round trips overlap instead of following one another.
+template <int B, class Load, class Store>
+__device__ __forceinline__ void gather_batched(int total, Load load, Store store) {
+  using Val = decltype(load(0));
+  for (int base = threadIdx.x; base < total; base += B * blockDim.x) {
+    Val v[B];
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+      const int idx = base + b * blockDim.x;
+      if (idx < total) v[b] = load(idx);
+    }
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+      const int idx = base + b * blockDim.x;
+      if (idx < total) store(idx, v[b]);
+    }
+  }
+}
+
 // ---------------------------------------------------- wavenumber helpers ----
 __device__ __forceinline__ void mode_k(const Geo &g, int m, double &ky, double &kz, double &ky_odd) {
   const int iy = m / g.kzs, iz = m - iy * g.kzs;

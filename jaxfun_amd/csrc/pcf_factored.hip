@@ -1,0 +1,456 @@
+// Steady SBDF3 solves with per-mode LU factors computed once per handle.
+//
+// The steady systems S = M - gamma L (gamma = 6 dt / 11, sbdf3.py:11-12) depend
+// on the mode only through k^2 and do not change between steps.  k_factor runs
+// the banded + far low-rank elimination of every (basis, parity, mode) block
+// once and keeps, per row, the pivot inverse, the near-upper entries, the
+// far-part coefficients and the multipliers (7 values per row for the clamped
+// blocks, 4 for the Dirichlet ones: 11 MB at 65x64x64).  Each step then only
+// forms the right-hand side (banded, in parallel over rows) and runs the
+// forward and back substitutions, whose factor loads do not depend on the
+// substitution chain.  The reference likewise factors once and substitutes per
+// step (src/jaxfun/la/tpmatrix.py:1034-1150, 686-799).
+#include "pcf_adjoint.hpp"
+#include "pcf_device.hpp"
+
+namespace pcf {
+namespace {
+
+// factor value layout of one row: [0] 1/pivot, [1..Q] near-upper entries,
+// [Q+1..Q+R] far coefficients, [Q+R+1..Q+R+P] multipliers of rows i-1..i-P
+template <int P, int Q, int R, bool HAS4>
+__device__ void factor_block(const StructSys &ss, int n, double s0, double s2, double s4, double sf,
+                             double *F, size_t rs, size_t vs) {
+  constexpr int BW = P + Q + 1;
+  double pd[P + 1], pe[P + 1][Q + 1], pw[P + 1][R];
+#pragma unroll
+  for (int t = 0; t <= P; ++t) {
+    pd[t] = 0;
+#pragma unroll
+    for (int q = 0; q <= Q; ++q) pe[t][q] = 0;
+#pragma unroll
+    for (int r = 0; r < R; ++r) pw[t][r] = 0;
+  }
+  for (int i = 0; i < n; ++i) {
+    double a[BW];
+#pragma unroll
+    for (int o = 0; o < BW; ++o) {
+      double v = s0 * ss.b0[(size_t)i * BW + o] + s2 * ss.b2[(size_t)i * BW + o];
+      if (HAS4) v += s4 * ss.b4[(size_t)i * BW + o];
+      a[o] = v;
+    }
+    double wi[R], lt[P + 1];
+#pragma unroll
+    for (int r = 0; r < R; ++r) wi[r] = sf * ss.U[(size_t)i * R + r];
+#pragma unroll
+    for (int t = 0; t <= P; ++t) lt[t] = 0.0;
+#pragma unroll
+    for (int t = P; t >= 1; --t) {
+      if (i - t < 0) continue;
+      const double l = a[P - t] * pd[t];
+      lt[t] = l;
+#pragma unroll
+      for (int o = -t + 1; o <= Q; ++o) {
+        if (i + o >= n) continue;
+        const int rel = o + t;
+        double u;
+        if (rel <= Q) {
+          u = pe[t][rel];
+        } else {
+          u = 0.0;
+#pragma unroll
+          for (int r = 0; r < R; ++r) u += pw[t][r] * ss.V[(size_t)(i + o) * R + r];
+        }
+        a[P + o] -= l * u;
+      }
+#pragma unroll
+      for (int r = 0; r < R; ++r) wi[r] -= l * pw[t][r];
+    }
+    const double dinv = 1.0 / a[P];
+    double *f = F + (size_t)i * rs;
+    f[0] = dinv;
+#pragma unroll
+    for (int q = 1; q <= Q; ++q) f[q * vs] = a[P + q];
+#pragma unroll
+    for (int r = 0; r < R; ++r) f[(1 + Q + r) * vs] = wi[r];
+#pragma unroll
+    for (int t = 1; t <= P; ++t) f[(Q + R + t) * vs] = lt[t];
+#pragma unroll
+    for (int t = P; t >= 2; --t) {
+      pd[t] = pd[t - 1];
+#pragma unroll
+      for (int q = 0; q <= Q; ++q) pe[t][q] = pe[t - 1][q];
+#pragma unroll
+      for (int r = 0; r < R; ++r) pw[t][r] = pw[t - 1][r];
+    }
+    pd[1] = dinv;
+#pragma unroll
+    for (int q = 1; q <= Q; ++q) pe[1][q] = a[P + q];
+#pragma unroll
+    for (int r = 0; r < R; ++r) pw[1][r] = wi[r];
+  }
+}
+
+// Forward then back substitution with stored factors.  load(i) gives the
+// right-hand side of row i, keep(i, y)/kept(i) park the forward result, and
+// store(i, x) receives the solution (rows visited n-1 .. 0).
+template <int P, int Q, int R, class Load, class Keep, class Kept, class Store>
+__device__ void substitute(const StructSys &ss, int n, const double *F, size_t rs, size_t vs, Load load,
+                           Keep keep, Kept kept, Store store) {
+  constexpr int F1 = Q + 1;
+  {
+    double2 yw[P + 1];
+#pragma unroll
+    for (int t = 0; t <= P; ++t) yw[t] = double2{0, 0};
+    double ln[P + 1];
+#pragma unroll
+    for (int t = 1; t <= P; ++t) ln[t] = F[(Q + R + t) * vs];
+    for (int i = 0; i < n; ++i) {
+      double l[P + 1];
+#pragma unroll
+      for (int t = 1; t <= P; ++t) l[t] = ln[t];
+      if (i + 1 < n) {
+#pragma unroll
+        for (int t = 1; t <= P; ++t) ln[t] = F[(size_t)(i + 1) * rs + (Q + R + t) * vs];
+      }
+      double2 y = load(i);
+#pragma unroll
+      for (int t = 1; t <= P; ++t) {
+        y.x -= l[t] * yw[t].x;
+        y.y -= l[t] * yw[t].y;
+      }
+      keep(i, y);
+#pragma unroll
+      for (int t = P; t >= 2; --t) yw[t] = yw[t - 1];
+      yw[1] = y;
+    }
+  }
+  {
+    double2 xw[F1 + 1];
+#pragma unroll
+    for (int k = 0; k <= F1; ++k) xw[k] = double2{0, 0};
+    double2 tau[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) tau[r] = double2{0, 0};
+    double cur[1 + Q + R], nxt[1 + Q + R];
+#pragma unroll
+    for (int v = 0; v < 1 + Q + R; ++v) cur[v] = F[(size_t)(n - 1) * rs + v * vs];
+    for (int i = n - 1; i >= 0; --i) {
+      if (i > 0) {
+#pragma unroll
+        for (int v = 0; v < 1 + Q + R; ++v) nxt[v] = F[(size_t)(i - 1) * rs + v * vs];
+      }
+      if (i + F1 < n) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const double vv = ss.V[(size_t)(i + F1) * R + r];
+          tau[r].x += vv * xw[F1].x;
+          tau[r].y += vv * xw[F1].y;
+        }
+      }
+      double2 acc = kept(i);
+#pragma unroll
+      for (int q = 1; q <= Q; ++q) {
+        acc.x -= cur[q] * xw[q].x;
+        acc.y -= cur[q] * xw[q].y;
+      }
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        acc.x -= cur[1 + Q + r] * tau[r].x;
+        acc.y -= cur[1 + Q + r] * tau[r].y;
+      }
+      const double2 xi{acc.x * cur[0], acc.y * cur[0]};
+      store(i, xi);
+#pragma unroll
+      for (int k = F1; k >= 2; --k) xw[k] = xw[k - 1];
+      xw[1] = xi;
+#pragma unroll
+      for (int v = 0; v < 1 + Q + R; ++v) cur[v] = nxt[v];
+    }
+  }
+}
+
+constexpr int NF_B = 7, NF_D = 4;  // factor values per row: clamped (2,2,2), Dirichlet (1,1,1)
+
+struct Lane2 {
+  int sys, par, m;
+  bool mean, valid;
+};
+__device__ __forceinline__ Lane2 lane2(const Geo &g, int t) {
+  const int M = g.m, Mp = (M + 63) / 64 * 64;
+  Lane2 s{};
+  const int sp = t / Mp, m = t - sp * Mp;
+  s.valid = true;
+  if (sp >= 4) {
+    s.valid = (sp == 4 && m < 2);
+    s.mean = true;
+    s.sys = 1;
+    s.par = m;
+    s.m = 0;
+    return s;
+  }
+  s.valid = m < M;
+  s.mean = false;
+  s.sys = sp >> 1;
+  s.par = sp & 1;
+  s.m = m;
+  return s;
+}
+
+__global__ void __launch_bounds__(256) k_factor(Geo g, StructMats sm, double gamma, FactorSet fs) {
+  const Lane2 L = lane2(g, blockIdx.x * blockDim.x + threadIdx.x);
+  if (!L.valid) return;
+  const StructSys &ss = sm.sys[L.sys][L.par];
+  double kyd, kzd, kyo;
+  mode_k(g, L.m, kyd, kzd, kyo);
+  const double k2 = L.mean ? 0.0 : kyd * kyd + kzd * kzd;
+  const double W = L.mean ? 1.0 : g.W, nu = g.nu, gm = gamma;
+  const size_t M = g.m;
+  if (L.sys == 0) {
+    const double s2 = W * (1 + 2 * gm * nu * k2), s4 = -W * gm * nu, s0 = -W * (k2 + gm * nu * k2 * k2);
+    factor_block<2, 2, 2, true>(ss, ss.n, s0, s2, s4, s4, fs.blk[0][L.par] + L.m, NF_B * M, M);
+  } else {
+    const double s0 = W * (1 + gm * nu * k2), s2 = -W * gm * nu;
+    if (L.mean)
+      factor_block<1, 1, 1, false>(ss, ss.n, s0, s2, 0.0, s2, fs.mean[L.par], NF_D, 1);
+    else
+      factor_block<1, 1, 1, false>(ss, ss.n, s0, s2, 0.0, s2, fs.blk[1][L.par] + L.m, NF_D * M, M);
+  }
+}
+
+// Steady update: rhs = M z + N, z = sum c_l x_l, N = sum e_l N_l over the three
+// levels, then the factored solve.  Block: SL modes of one (basis, parity)
+// block; the Dirichlet blocks of mode 0 also carry the mean system in lane SL.
+template <int SL>
+__global__ void __launch_bounds__(256) k_solve_factored(Geo g, StructMats sm, FactorSet fs, SolveArgs a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  constexpr int SLP = SL + 1;
+  const int M = g.m;
+  const int bps = (M + SL - 1) / SL;
+  const int spb = blockIdx.x / bps;
+  const int m0 = (blockIdx.x - spb * bps) * SL;
+  const bool has_mean = (spb >= 2) && (m0 == 0);
+  double *zn = reinterpret_cast<double *>(smem);  // [row][4][SLP]: z.x, z.y, rhs.x, rhs.y
+  double *fl = zn + (size_t)g.npmax * 4 * SLP;    // [row][NF][SLP]: the lanes' factors
+  const int sys = spb >> 1, par = spb & 1;
+  const StructSys &ss = sm.sys[sys][par];
+  const int n = ss.n;
+  const int NF = (sys == 0) ? NF_B : NF_D;
+  auto ZN = [&](int j, int v, int lane) -> double & { return zn[((size_t)j * 4 + v) * SLP + lane]; };
+  // factors of every (row, value, lane), consecutive lanes on consecutive modes
+  gather_batched<8>(
+      n * NF * SLP,
+      [&](int idx) {
+        const int j = idx / (NF * SLP), rem = idx - j * NF * SLP, v = rem / SLP, lane = rem - v * SLP;
+        const bool mean = (lane == SL);
+        const int m = m0 + lane;
+        if (mean ? !has_mean : m >= M) return 0.0;
+        return mean ? fs.mean[par][(size_t)j * NF_D + v] : fs.blk[sys][par][((size_t)j * NF + v) * M + m];
+      },
+      [&](int idx, double v) { fl[idx] = v; });
+
+  // phase 0: z and N of every (row, lane)
+  struct ZNv {
+    double2 z, n;
+  };
+  gather_batched<4>(
+      n * SLP,
+      [&](int idx) {
+        const int j = idx / SLP, lane = idx - j * SLP;
+        const bool mean = (lane == SL);
+        const int m = m0 + lane;
+        ZNv r{{0, 0}, {0, 0}};
+        if (mean ? !has_mean : m >= M) return r;
+        const int J = 2 * j + par;
+        if (!mean) {
+          const size_t o = (size_t)J * M + m;
+#pragma unroll
+          for (int l = 0; l < 3; ++l) {
+            const double2 xv = (sys == 0) ? a.u0[l][o] : a.g[l][o];
+            const double2 nv = (sys == 0) ? a.Nu[l][o] : a.Ng[l][o];
+            r.z.x += a.c[l] * xv.x;
+            r.z.y += a.c[l] * xv.y;
+            r.n.x += a.e[l] * nv.x;
+            r.n.y += a.e[l] * nv.y;
+          }
+        } else {
+#pragma unroll
+          for (int l = 0; l < 3; ++l) {
+            r.z.x += a.c[l] * a.v[l][J];
+            r.z.y += a.c[l] * a.w[l][J];
+            r.n.x += a.e[l] * a.Nv[l][J];
+            r.n.y += a.e[l] * a.Nw[l][J];
+          }
+        }
+        return r;
+      },
+      [&](int idx, const ZNv &r) {
+        const int j = idx / SLP, lane = idx - j * SLP;
+        ZN(j, 0, lane) = r.z.x;
+        ZN(j, 1, lane) = r.z.y;
+        ZN(j, 2, lane) = r.n.x;
+        ZN(j, 3, lane) = r.n.y;
+      });
+  __syncthreads();
+  // phase 1: rhs = N + M z with the banded mass operator (no far part)
+  const int P = (sys == 0) ? 2 : 1, Q = P, BW = P + Q + 1;
+  for (int idx = threadIdx.x; idx < n * SLP; idx += blockDim.x) {
+    const int j = idx / SLP, lane = idx - j * SLP;
+    const bool mean = (lane == SL);
+    const int m = m0 + lane;
+    if (mean ? !has_mean : m >= M) continue;
+    double kyd, kzd, kyo;
+    mode_k(g, mean ? 0 : m, kyd, kzd, kyo);
+    const double k2 = mean ? 0.0 : kyd * kyd + kzd * kzd;
+    const double W = mean ? 1.0 : g.W;
+    const double mb0 = (sys == 0) ? -W * k2 : W, mb2 = (sys == 0) ? W : 0.0;
+    double rx = ZN(j, 2, lane), ry = ZN(j, 3, lane);
+    for (int d = -P; d <= Q; ++d) {
+      const int jj = j + d;
+      if (jj < 0 || jj >= n) continue;
+      const size_t bo = (size_t)j * BW + (d + P);
+      const double c = mb0 * ss.b0[bo] + mb2 * ss.b2[bo];
+      rx += c * ZN(jj, 0, lane);
+      ry += c * ZN(jj, 1, lane);
+    }
+    ZN(j, 2, lane) = rx;
+    ZN(j, 3, lane) = ry;
+  }
+  __syncthreads();
+  // phase 2: substitutions, one thread per lane (thread 64, a second wave, takes the mean)
+  int lane;
+  bool mean = false;
+  if (threadIdx.x < SL) {
+    lane = threadIdx.x;
+  } else if (threadIdx.x == 64 && has_mean) {
+    lane = SL;
+    mean = true;
+  } else {
+    return;
+  }
+  const int m = mean ? 0 : m0 + lane;
+  if (!mean && m >= M) return;
+  auto load = [&](int i) { return double2{ZN(i, 2, lane), ZN(i, 3, lane)}; };
+  auto keep = [&](int i, double2 y) {
+    ZN(i, 2, lane) = y.x;
+    ZN(i, 3, lane) = y.y;
+  };
+  const double *F = fl + lane;
+  if (sys == 0) {
+    substitute<2, 2, 2>(ss, n, F, (size_t)NF_B * SLP, (size_t)SLP, load, keep, load,
+                        [&](int i, double2 x) { a.u0_out[(size_t)(2 * i + par) * M + m] = x; });
+  } else if (mean) {
+    substitute<1, 1, 1>(ss, n, F, (size_t)NF_D * SLP, (size_t)SLP, load, keep, load, [&](int i, double2 x) {
+      a.v_out[2 * i + par] = x.x;
+      a.w_out[2 * i + par] = x.y;
+    });
+  } else {
+    substitute<1, 1, 1>(ss, n, F, (size_t)NF_D * SLP, (size_t)SLP, load, keep, load,
+                        [&](int i, double2 x) { a.g_out[(size_t)(2 * i + par) * M + m] = x; });
+  }
+}
+
+// Transposed steady solve of the reverse pass on the index-reversed transposed
+// blocks: rho = S^{-T} mu, and nu = e0 rho + e1 r1 + e2 r2.  The solution rows
+// double as the parking space of the forward substitution.
+__global__ void __launch_bounds__(256) k_adj_solve_factored(Geo g, StructMats smt, FactorSet fs, PrimC in,
+                                                            Prim out, Prim nuo, PrimC r1, PrimC r2,
+                                                            double e0, double e1, double e2) {
+  const Lane2 L = lane2(g, blockIdx.x * blockDim.x + threadIdx.x);
+  if (!L.valid) return;
+  const int M = g.m, m = L.m, par = L.par;
+  const StructSys &ss = smt.sys[L.sys][par];
+  const int n = ss.n;
+  auto row = [&](int ir) { return 2 * (n - 1 - ir) + par; };
+  if (!L.mean && (m / g.kzs == g.ny / 2)) {  // Nyquist row: masked
+    double2 *dst = (L.sys == 0) ? out.u0 : out.g;
+    double2 *dn = (L.sys == 0) ? nuo.u0 : nuo.g;
+    for (int i = 0; i < n; ++i) {
+      dst[(size_t)(2 * i + par) * M + m] = double2{0, 0};
+      dn[(size_t)(2 * i + par) * M + m] = double2{0, 0};
+    }
+    return;
+  }
+  if (L.sys == 0) {
+    double2 *x = out.u0;
+    auto at = [&](int ir) -> double2 & { return x[(size_t)row(ir) * M + m]; };
+    substitute<2, 2, 2>(
+        ss, n, fs.blk[0][par] + m, (size_t)NF_B * M, (size_t)M,
+        [&](int ir) { return in.u0[(size_t)row(ir) * M + m]; }, [&](int ir, double2 y) { at(ir) = y; },
+        [&](int ir) { return at(ir); },
+        [&](int ir, double2 v) {
+          const size_t o = (size_t)row(ir) * M + m;
+          x[o] = v;
+          const double2 p = r1.u0[o], q = r2.u0[o];
+          nuo.u0[o] = double2{e0 * v.x + e1 * p.x + e2 * q.x, e0 * v.y + e1 * p.y + e2 * q.y};
+        });
+  } else if (L.mean) {
+    substitute<1, 1, 1>(
+        ss, n, fs.mean[par], NF_D, 1, [&](int ir) { return double2{in.v[row(ir)], in.w[row(ir)]}; },
+        [&](int ir, double2 y) {
+          out.v[row(ir)] = y.x;
+          out.w[row(ir)] = y.y;
+        },
+        [&](int ir) { return double2{out.v[row(ir)], out.w[row(ir)]}; },
+        [&](int ir, double2 v) {
+          const int j = row(ir);
+          out.v[j] = v.x;
+          out.w[j] = v.y;
+          nuo.v[j] = e0 * v.x + e1 * r1.v[j] + e2 * r2.v[j];
+          nuo.w[j] = e0 * v.y + e1 * r1.w[j] + e2 * r2.w[j];
+        });
+  } else {
+    double2 *x = out.g;
+    auto at = [&](int ir) -> double2 & { return x[(size_t)row(ir) * M + m]; };
+    substitute<1, 1, 1>(
+        ss, n, fs.blk[1][par] + m, (size_t)NF_D * M, (size_t)M,
+        [&](int ir) { return in.g[(size_t)row(ir) * M + m]; }, [&](int ir, double2 y) { at(ir) = y; },
+        [&](int ir) { return at(ir); },
+        [&](int ir, double2 v) {
+          const size_t o = (size_t)row(ir) * M + m;
+          x[o] = v;
+          const double2 p = r1.g[o], q = r2.g[o];
+          nuo.g[o] = double2{e0 * v.x + e1 * p.x + e2 * q.x, e0 * v.y + e1 * p.y + e2 * q.y};
+        });
+  }
+}
+
+// z/rhs (4) and factors (up to 7) per row and lane slot
+size_t factored_lds_bytes(const Geo &g, int sl) { return (size_t)g.npmax * (4 + NF_B) * (sl + 1) * sizeof(double); }
+
+}  // namespace
+
+size_t factor_doubles(const Geo &g, const StructMats &sm, int basis, int par) {
+  if (basis < 0) return (size_t)sm.sys[1][par].n * NF_D;  // mean system
+  return (size_t)sm.sys[basis][par].n * (basis == 0 ? NF_B : NF_D) * g.m;
+}
+
+hipError_t launch_factor(const Geo &g, const StructMats &sm, double gamma, FactorSet fs, hipStream_t s) {
+  const int threads = 4 * ((g.m + 63) / 64 * 64) + 64;
+  hipLaunchKernelGGL(k_factor, dim3((threads + 255) / 256), dim3(256), 0, s, g, sm, gamma, fs);
+  return hipGetLastError();
+}
+
+hipError_t launch_solve_factored(const Geo &g, const StructMats &sm, const FactorSet &fs, const SolveArgs &a,
+                                 hipStream_t s) {
+  // the most lanes per block whose rows and factors fit the default 64 KB of LDS
+#define PCF_FSOLVE(SLV)                                                                                  \
+  hipLaunchKernelGGL(k_solve_factored<SLV>, dim3(4 * ((g.m + SLV - 1) / SLV)), dim3(256),                 \
+                     factored_lds_bytes(g, SLV), s, g, sm, fs, a)
+  if (factored_lds_bytes(g, 16) <= 65536) PCF_FSOLVE(16);
+  else if (factored_lds_bytes(g, 8) <= 65536) PCF_FSOLVE(8);
+  else return hipErrorInvalidValue;
+#undef PCF_FSOLVE
+  return hipGetLastError();
+}
+
+hipError_t launch_adj_solve_factored(const Geo &g, const StructMats &smt, const FactorSet &fs, PrimC in,
+                                     Prim out, Prim nu, PrimC r1, PrimC r2, double e0, double e1, double e2,
+                                     hipStream_t s) {
+  const int threads = 4 * ((g.m + 63) / 64 * 64) + 64;
+  hipLaunchKernelGGL(k_adj_solve_factored, dim3((threads + 255) / 256), dim3(256), 0, s, g, smt, fs, in, out,
+                     nu, r1, r2, e0, e1, e2);
+  return hipGetLastError();
+}
+
+}  // namespace pcf

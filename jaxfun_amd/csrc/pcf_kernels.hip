@@ -51,11 +51,13 @@ __global__ void __launch_bounds__(256) k_fft_y(Geo g, const typename VT<T>::V *_
     const V *s = src + ((size_t)f * g.n0 + x) * g.m;
     for (int idx = tid; idx < MY * KC; idx += nthr) buf[idx] = V{0, 0};
     __syncthreads();
-    for (int idx = tid; idx < NY * KC; idx += nthr) {
-      int iy = idx / KC, c = idx - iy * KC;
-      int y = (iy < NY / 2) ? iy : iy + MY - NY;
-      buf[y * KC + c] = s[(size_t)iy * KZ + kc0 + c];
-    }
+    gather_batched<4>(
+        NY * KC, [&](int idx) { return s[(size_t)(idx / KC) * KZ + kc0 + idx % KC]; },
+        [&](int idx, const V &v) {
+          const int iy = idx / KC, c = idx - iy * KC;
+          const int y = (iy < NY / 2) ? iy : iy + MY - NY;
+          buf[y * KC + c] = v;
+        });
     __syncthreads();
     if constexpr (NYC > 0)
       fft_lds_c<V, T, NYC, true, true, MAXB>(buf, twl, KC, 1, KC);
@@ -68,10 +70,9 @@ __global__ void __launch_bounds__(256) k_fft_y(Geo g, const typename VT<T>::V *_
     }
   } else {
     const V *s = src + ((size_t)f * g.n0 + x) * (size_t)MY * KZ;
-    for (int idx = tid; idx < MY * KC; idx += nthr) {
-      int y = idx / KC, c = idx - y * KC;
-      buf[idx] = s[(size_t)y * KZ + kc0 + c];
-    }
+    gather_batched<4>(
+        MY * KC, [&](int idx) { return s[(size_t)(idx / KC) * KZ + kc0 + idx % KC]; },
+        [&](int idx, const V &v) { buf[idx] = v; });
     __syncthreads();
     if constexpr (NYC > 0)
       fft_lds_c<V, T, NYC, false, true, MAXB>(buf, twl, KC, 1, KC);
@@ -111,27 +112,30 @@ __global__ void __launch_bounds__(256) k_rotational(Geo g, NodalMats<T> nm, FFTP
   V *tw = out + (size_t)UPB * 3 * g.mz;            // twiddle table in LDS
   for (int i = tid; i < MZ; i += nthr) tw[i] = reinterpret_cast<const V *>(nm.twz)[i];
 
-  for (int idx = tid; idx < UPB * 6 * MZ; idx += nthr) {
-    int lu = idx / (6 * MZ), rem = idx - lu * 6 * MZ;
-    int f = rem / MZ, k = rem - f * MZ;
-    int u = ubase + lu;
-    V z{0, 0};
-    if (u < nunits) {
-      int x = u / (MY / 2), y2 = u - x * (MY / 2);
-      const V *ra = Y + f * fstride + ((size_t)x * MY + 2 * y2) * KZ;
-      const V *rb = ra + KZ;
-      if (k == 0) {
-        z = V{ra[0].x, rb[0].x};
-      } else if (k < KZ) {
-        V a = ra[k], b = rb[k];
-        z = V{a.x - b.y, a.y + b.x};
-      } else if (k > MZ - KZ) {
-        V a = ra[MZ - k], b = rb[MZ - k];
-        z = V{a.x + b.y, b.x - a.y};
-      }
-    }
-    buf[idx] = z;
-  }
+  gather_batched<4>(
+      UPB * 6 * MZ,
+      [&](int idx) {
+        const int lu = idx / (6 * MZ), rem = idx - lu * 6 * MZ;
+        const int f = rem / MZ, k = rem - f * MZ;
+        const int u = ubase + lu;
+        V z{0, 0};
+        if (u < nunits) {
+          const int x = u / (MY / 2), y2 = u - x * (MY / 2);
+          const V *ra = Y + f * fstride + ((size_t)x * MY + 2 * y2) * KZ;
+          const V *rb = ra + KZ;
+          if (k == 0) {
+            z = V{ra[0].x, rb[0].x};
+          } else if (k < KZ) {
+            const V a = ra[k], b = rb[k];
+            z = V{a.x - b.y, a.y + b.x};
+          } else if (k > MZ - KZ) {
+            const V a = ra[MZ - k], b = rb[MZ - k];
+            z = V{a.x + b.y, b.x - a.y};
+          }
+        }
+        return z;
+      },
+      [&](int idx, const V &z) { buf[idx] = z; });
   __syncthreads();
   if constexpr (NZC > 0)
     fft_lds_c<V, T, NZC, true, false, MAXB>(buf, tw, UPB * 6, MZ, 1);
@@ -482,581 +486,6 @@ __global__ void __launch_bounds__(256) k_solve(Geo g, StructMats sm, SolveArgs a
   }
 }
 
-// ------------------------------------------------------ k_solve_steady ----
-// Steady SBDF3 update: rhs = M z + N with z = sum_l c_l x_l over the three
-// solution levels and N = sum_l e_l N_l over the three nonlinear levels.  The
-// mass operators M are banded (W (C2_B - k^2 C0_B) and W C0_D have no far
-// part), so the right-hand side of row i only needs rows i-P .. i+Q and is
-// formed inside the top-down elimination sweep: two sweeps instead of three.
-// Global loads run two rows ahead of their use and the back substitution
-// fetches the next row's factors while it solves the current one, so each row
-// costs arithmetic latency, not a memory round trip.
-struct RowRaw {
-  double2 x[3];
-  double2 n[3];
-};
-
-template <int P, int Q, int R>
-__device__ void steady_solve(const StructSys &ss, int n, int par, int m, int M, double s0,
-                             double s2, double s4, double sf, double mb0, double mb2,
-                             const SolveArgs &a, bool isu, bool mean, double *scr,
-                             double2 *xout) {
-  constexpr int F = Q + 1;
-  constexpr int BW = P + Q + 1;
-  constexpr int NV = 1 + Q + R + 2;
-  const size_t rowstride = (size_t)NV * M;
-  auto scr_at = [&](int i, int v) -> double & { return scr[(size_t)i * rowstride + (size_t)v * M + m]; };
-
-  auto issue = [&](int j) {
-    RowRaw r;
-#pragma unroll
-    for (int l = 0; l < 3; ++l) r.x[l] = r.n[l] = double2{0, 0};
-    if (j < 0 || j >= n) return r;
-    const int J = 2 * j + par;
-    if (!mean) {
-      const size_t o = (size_t)J * M + m;
-#pragma unroll
-      for (int l = 0; l < 3; ++l) {
-        r.x[l] = isu ? a.u0[l][o] : a.g[l][o];
-        r.n[l] = isu ? a.Nu[l][o] : a.Ng[l][o];
-      }
-    } else {
-#pragma unroll
-      for (int l = 0; l < 3; ++l) {
-        r.x[l] = double2{a.v[l][J], a.w[l][J]};
-        r.n[l] = double2{a.Nv[l][J], a.Nw[l][J]};
-      }
-    }
-    return r;
-  };
-  auto zof = [&](const RowRaw &r) {
-    double2 z{0, 0};
-#pragma unroll
-    for (int l = 0; l < 3; ++l) {
-      z.x += a.c[l] * r.x[l].x;
-      z.y += a.c[l] * r.x[l].y;
-    }
-    return z;
-  };
-  auto nof = [&](const RowRaw &r) {
-    double2 z{0, 0};
-#pragma unroll
-    for (int l = 0; l < 3; ++l) {
-      z.x += a.e[l] * r.n[l].x;
-      z.y += a.e[l] * r.n[l].y;
-    }
-    return z;
-  };
-
-  // ---- sweep B: rhs + LU without pivoting, top-down
-  {
-    double2 zw[BW], nw[Q + 1];
-#pragma unroll
-    for (int o = 0; o < BW; ++o) zw[o] = double2{0, 0};
-#pragma unroll
-    for (int q = 0; q <= Q; ++q) {
-      RowRaw r = issue(q);
-      zw[P + q] = zof(r);
-      nw[q] = nof(r);
-    }
-    RowRaw pf1 = issue(Q + 1), pf2 = issue(Q + 2);
-    double pd[P + 1], pe[P + 1][Q + 1], pw[P + 1][R];
-    double2 py[P + 1];
-#pragma unroll
-    for (int t = 0; t <= P; ++t) {
-      pd[t] = 0;
-      py[t] = double2{0, 0};
-#pragma unroll
-      for (int q = 0; q <= Q; ++q) pe[t][q] = 0;
-#pragma unroll
-      for (int r = 0; r < R; ++r) pw[t][r] = 0;
-    }
-    // operator rows of the next iteration are fetched one row ahead
-    double c0n[BW], c2n[BW], c4n[BW], un[R], vn[Q + P][R];
-    auto fetch_ops = [&](int i) {
-      const int ii = (i < n) ? i : n - 1;
-#pragma unroll
-      for (int o = 0; o < BW; ++o) {
-        c0n[o] = ss.b0[(size_t)ii * BW + o];
-        c2n[o] = ss.b2[(size_t)ii * BW + o];
-        c4n[o] = (R == 2) ? ss.b4[(size_t)ii * BW + o] : 0.0;
-      }
-#pragma unroll
-      for (int r = 0; r < R; ++r) un[r] = ss.U[(size_t)ii * R + r];
-      // far factors of the columns i+1 .. i+Q (used by eliminations reaching them)
-#pragma unroll
-      for (int o = 1; o <= Q + P; ++o) {
-        const int j = (ii + o < n) ? ii + o : n - 1;
-#pragma unroll
-        for (int r = 0; r < R; ++r) vn[o - 1][r] = ss.V[(size_t)j * R + r];
-      }
-    };
-    fetch_ops(0);
-    for (int i = 0; i < n; ++i) {
-      double c0[BW], c2[BW], c4[BW], ui[R], vcol[Q + P][R];
-#pragma unroll
-      for (int o = 0; o < BW; ++o) {
-        c0[o] = c0n[o];
-        c2[o] = c2n[o];
-        c4[o] = c4n[o];
-      }
-#pragma unroll
-      for (int r = 0; r < R; ++r) ui[r] = un[r];
-#pragma unroll
-      for (int o = 0; o < Q + P; ++o)
-#pragma unroll
-        for (int r = 0; r < R; ++r) vcol[o][r] = vn[o][r];
-      fetch_ops(i + 1);
-      double a_[BW];
-      double2 yi = nw[0];
-#pragma unroll
-      for (int o = 0; o < BW; ++o) {
-        const double mo = mb0 * c0[o] + mb2 * c2[o];
-        yi.x += mo * zw[o].x;
-        yi.y += mo * zw[o].y;
-        double v = s0 * c0[o] + s2 * c2[o];
-        if (R == 2) v += s4 * c4[o];
-        a_[o] = v;
-      }
-      double wi[R];
-#pragma unroll
-      for (int r = 0; r < R; ++r) wi[r] = sf * ui[r];
-#pragma unroll
-      for (int t = P; t >= 1; --t) {
-        if (i - t < 0) continue;
-        const double l = a_[P - t] * pd[t];
-#pragma unroll
-        for (int o = -t + 1; o <= Q; ++o) {
-          if (i + o >= n) continue;
-          const int rel = o + t;
-          double u;
-          if (rel <= Q) {
-            u = pe[t][rel];
-          } else {
-            u = 0.0;
-#pragma unroll
-            for (int r = 0; r < R; ++r) u += pw[t][r] * vcol[o - 1][r];
-          }
-          a_[P + o] -= l * u;
-        }
-#pragma unroll
-        for (int r = 0; r < R; ++r) wi[r] -= l * pw[t][r];
-        yi.x -= l * py[t].x;
-        yi.y -= l * py[t].y;
-      }
-      const double dinv = 1.0 / a_[P];
-      scr_at(i, 0) = dinv;
-#pragma unroll
-      for (int q = 1; q <= Q; ++q) scr_at(i, q) = a_[P + q];
-#pragma unroll
-      for (int r = 0; r < R; ++r) scr_at(i, 1 + Q + r) = wi[r];
-      scr_at(i, NV - 2) = yi.x;
-      scr_at(i, NV - 1) = yi.y;
-#pragma unroll
-      for (int t = P; t >= 2; --t) {
-        pd[t] = pd[t - 1];
-        py[t] = py[t - 1];
-#pragma unroll
-        for (int q = 0; q <= Q; ++q) pe[t][q] = pe[t - 1][q];
-#pragma unroll
-        for (int r = 0; r < R; ++r) pw[t][r] = pw[t - 1][r];
-      }
-      pd[1] = dinv;
-      py[1] = yi;
-#pragma unroll
-      for (int q = 1; q <= Q; ++q) pe[1][q] = a_[P + q];
-#pragma unroll
-      for (int r = 0; r < R; ++r) pw[1][r] = wi[r];
-      // advance windows: z rows i+1-P .. i+1+Q, N rows i+1 .. i+1+Q
-#pragma unroll
-      for (int o = 0; o < BW - 1; ++o) zw[o] = zw[o + 1];
-#pragma unroll
-      for (int q = 0; q < Q; ++q) nw[q] = nw[q + 1];
-      zw[BW - 1] = zof(pf1);
-      nw[Q] = nof(pf1);
-      pf1 = pf2;
-      pf2 = issue(i + Q + 3);
-    }
-  }
-
-  // ---- sweep C: back substitution, bottom-up, one row of factors in flight
-  {
-    double2 xw[F + 1];
-#pragma unroll
-    for (int k = 0; k <= F; ++k) xw[k] = double2{0, 0};
-    double2 tau[R];
-#pragma unroll
-    for (int r = 0; r < R; ++r) tau[r] = double2{0, 0};
-    double cur[NV], vf[R];
-#pragma unroll
-    for (int v = 0; v < NV; ++v) cur[v] = scr_at(n - 1, v);
-#pragma unroll
-    for (int r = 0; r < R; ++r) vf[r] = (n - 1 + F < n) ? ss.V[(size_t)(n - 1 + F) * R + r] : 0.0;
-    for (int i = n - 1; i >= 0; --i) {
-      double nxt[NV], vfn[R];
-      const int ip = (i > 0) ? i - 1 : 0;
-#pragma unroll
-      for (int v = 0; v < NV; ++v) nxt[v] = scr_at(ip, v);
-#pragma unroll
-      for (int r = 0; r < R; ++r) vfn[r] = (ip + F < n) ? ss.V[(size_t)(ip + F) * R + r] : 0.0;
-      if (i + F < n) {
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-          tau[r].x += vf[r] * xw[F].x;
-          tau[r].y += vf[r] * xw[F].y;
-        }
-      }
-      double2 acc{cur[NV - 2], cur[NV - 1]};
-#pragma unroll
-      for (int q = 1; q <= Q; ++q) {
-        acc.x -= cur[q] * xw[q].x;
-        acc.y -= cur[q] * xw[q].y;
-      }
-#pragma unroll
-      for (int r = 0; r < R; ++r) {
-        acc.x -= cur[1 + Q + r] * tau[r].x;
-        acc.y -= cur[1 + Q + r] * tau[r].y;
-      }
-      const double2 xi{acc.x * cur[0], acc.y * cur[0]};
-      const int J = 2 * i + par;
-      if (mean) {
-        a.v_out[J] = xi.x;
-        a.w_out[J] = xi.y;
-      } else {
-        xout[(size_t)J * M + m] = xi;
-      }
-#pragma unroll
-      for (int k = F; k >= 2; --k) xw[k] = xw[k - 1];
-      xw[1] = xi;
-#pragma unroll
-      for (int v = 0; v < NV; ++v) cur[v] = nxt[v];
-#pragma unroll
-      for (int r = 0; r < R; ++r) vf[r] = vfn[r];
-    }
-  }
-}
-
-__global__ void __launch_bounds__(256) k_solve_steady(Geo g, StructMats sm, SolveArgs a,
-                                                      double *scratch) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  const int M = g.m, Mp = (M + 63) / 64 * 64;
-  // Mp is a multiple of 64, so every wavefront works on one (system, parity):
-  // make that explicit so the operator data is read with scalar loads.
-  int sp = __builtin_amdgcn_readfirstlane(t / Mp);
-  int m = t - sp * Mp;
-  bool mean = false;
-  if (sp >= 4) {
-    if (sp > 4 || m >= 2) return;
-    mean = true;
-    sp = 2 + m;
-    m = 0;
-  } else if (m >= M) {
-    return;
-  }
-  const int sys = sp >> 1, par = sp & 1;
-  const StructSys &ss = sm.sys[sys][par];
-  double kyd, kzd, kyo;
-  mode_k(g, m, kyd, kzd, kyo);
-  const double k2 = mean ? 0.0 : kyd * kyd + kzd * kzd;
-  const double W = mean ? 1.0 : g.W, nu = g.nu, gm = a.gamma;
-  double *scr = mean ? scratch + (size_t)4 * g.npmax * 7 * M + (size_t)(sp - 2) * g.npmax * 7
-                     : scratch + (size_t)sp * g.npmax * 7 * M;
-  const int Ms = mean ? 1 : M;
-  if (sys == 0) {
-    const double s2 = W * (1 + 2 * gm * nu * k2), s4 = -W * gm * nu, s0 = -W * (k2 + gm * nu * k2 * k2);
-    steady_solve<2, 2, 2>(ss, ss.n, par, m, Ms, s0, s2, s4, s4, -W * k2, W, a, true, false, scr,
-                          a.u0_out);
-  } else {
-    const double s0 = W * (1 + gm * nu * k2), s2 = -W * gm * nu;
-    steady_solve<1, 1, 1>(ss, ss.n, par, m, Ms, s0, s2, 0.0, s2, W, 0.0, a, false, mean, scr,
-                          a.g_out);
-  }
-}
-
-// ------------------------------------------------- k_solve_steady_lds ----
-// The steady update again, with every per-row quantity of the sequential
-// sweeps in LDS: a block of SL lanes (one lane per mode, all lanes on the same
-// (system, parity)) first loads z and N of all its rows with independent
-// loads, then runs the two sweeps against LDS only.  LDS per block:
-// npmax * (4 + 7) * SL doubles, lane index fastest (bank-conflict free).
-template <int P, int Q, int R, int SL>
-__device__ void steady_solve_lds(const StructSys &ss, int n, int par, int m, int M, bool active,
-                                 double s0, double s2, double s4, double sf, double mb0,
-                                 double mb2, const SolveArgs &a, bool isu, bool mean,
-                                 double *zn, double *rd, double2 *xout, int lane) {
-  constexpr int F = Q + 1;
-  constexpr int BW = P + Q + 1;
-  constexpr int NV = 1 + Q + R + 2;
-  auto ZN = [&](int j, int v) -> double & { return zn[((size_t)j * 4 + v) * SL + lane]; };
-  auto RD = [&](int j, int v) -> double & { return rd[((size_t)j * NV + v) * SL + lane]; };
-
-  if (!active) return;
-  auto zrow = [&](int j) { return (j >= 0 && j < n) ? double2{ZN(j, 0), ZN(j, 1)} : double2{0, 0}; };
-
-  // ---- sweep B: rhs + LU without pivoting, top-down
-  {
-    double2 zw[BW];
-#pragma unroll
-    for (int o = 0; o < BW; ++o) zw[o] = zrow(o - P);
-    double2 znext = zrow(Q + 1);
-    double pd[P + 1], pe[P + 1][Q + 1], pw[P + 1][R];
-    double2 py[P + 1];
-#pragma unroll
-    for (int t = 0; t <= P; ++t) {
-      pd[t] = 0;
-      py[t] = double2{0, 0};
-#pragma unroll
-      for (int q = 0; q <= Q; ++q) pe[t][q] = 0;
-#pragma unroll
-      for (int r = 0; r < R; ++r) pw[t][r] = 0;
-    }
-    double c0n[BW], c2n[BW], c4n[BW], un[R], vn[Q][R];
-    auto fetch_ops = [&](int i) {
-      const int ii = (i < n) ? i : n - 1;
-#pragma unroll
-      for (int o = 0; o < BW; ++o) {
-        c0n[o] = ss.b0[(size_t)ii * BW + o];
-        c2n[o] = ss.b2[(size_t)ii * BW + o];
-        c4n[o] = (R == 2) ? ss.b4[(size_t)ii * BW + o] : 0.0;
-      }
-#pragma unroll
-      for (int r = 0; r < R; ++r) un[r] = ss.U[(size_t)ii * R + r];
-#pragma unroll
-      for (int o = 1; o <= Q; ++o) {
-        const int j = (ii + o < n) ? ii + o : n - 1;
-#pragma unroll
-        for (int r = 0; r < R; ++r) vn[o - 1][r] = ss.V[(size_t)j * R + r];
-      }
-    };
-    fetch_ops(0);
-    double2 ncur{ZN(0, 2), ZN(0, 3)};
-    for (int i = 0; i < n; ++i) {
-      double c0[BW], c2[BW], c4[BW], ui[R], vcol[Q][R];
-#pragma unroll
-      for (int o = 0; o < BW; ++o) {
-        c0[o] = c0n[o];
-        c2[o] = c2n[o];
-        c4[o] = c4n[o];
-      }
-#pragma unroll
-      for (int r = 0; r < R; ++r) ui[r] = un[r];
-#pragma unroll
-      for (int o = 0; o < Q; ++o)
-#pragma unroll
-        for (int r = 0; r < R; ++r) vcol[o][r] = vn[o][r];
-      fetch_ops(i + 1);
-      const double2 nthis = ncur;
-      const int inx = (i + 1 < n) ? i + 1 : i;
-      ncur = double2{ZN(inx, 2), ZN(inx, 3)};
-      double a_[BW];
-      double2 yi = nthis;
-#pragma unroll
-      for (int o = 0; o < BW; ++o) {
-        const double mo = mb0 * c0[o] + mb2 * c2[o];
-        yi.x += mo * zw[o].x;
-        yi.y += mo * zw[o].y;
-        double v = s0 * c0[o] + s2 * c2[o];
-        if (R == 2) v += s4 * c4[o];
-        a_[o] = v;
-      }
-      double wi[R];
-#pragma unroll
-      for (int r = 0; r < R; ++r) wi[r] = sf * ui[r];
-#pragma unroll
-      for (int t = P; t >= 1; --t) {
-        if (i - t < 0) continue;
-        const double l = a_[P - t] * pd[t];
-#pragma unroll
-        for (int o = -t + 1; o <= Q; ++o) {
-          if (i + o >= n) continue;
-          const int rel = o + t;
-          double u;
-          if (rel <= Q) {
-            u = pe[t][rel];
-          } else {
-            u = 0.0;
-#pragma unroll
-            for (int r = 0; r < R; ++r) u += pw[t][r] * vcol[o - 1][r];
-          }
-          a_[P + o] -= l * u;
-        }
-#pragma unroll
-        for (int r = 0; r < R; ++r) wi[r] -= l * pw[t][r];
-        yi.x -= l * py[t].x;
-        yi.y -= l * py[t].y;
-      }
-      const double dinv = 1.0 / a_[P];
-      RD(i, 0) = dinv;
-#pragma unroll
-      for (int q = 1; q <= Q; ++q) RD(i, q) = a_[P + q];
-#pragma unroll
-      for (int r = 0; r < R; ++r) RD(i, 1 + Q + r) = wi[r];
-      RD(i, NV - 2) = yi.x;
-      RD(i, NV - 1) = yi.y;
-#pragma unroll
-      for (int t = P; t >= 2; --t) {
-        pd[t] = pd[t - 1];
-        py[t] = py[t - 1];
-#pragma unroll
-        for (int q = 0; q <= Q; ++q) pe[t][q] = pe[t - 1][q];
-#pragma unroll
-        for (int r = 0; r < R; ++r) pw[t][r] = pw[t - 1][r];
-      }
-      pd[1] = dinv;
-      py[1] = yi;
-#pragma unroll
-      for (int q = 1; q <= Q; ++q) pe[1][q] = a_[P + q];
-#pragma unroll
-      for (int r = 0; r < R; ++r) pw[1][r] = wi[r];
-#pragma unroll
-      for (int o = 0; o < BW - 1; ++o) zw[o] = zw[o + 1];
-      zw[BW - 1] = znext;
-      znext = zrow(i + Q + 2);
-    }
-  }
-
-  // ---- sweep C: back substitution, bottom-up
-  {
-    double2 xw[F + 1];
-#pragma unroll
-    for (int k = 0; k <= F; ++k) xw[k] = double2{0, 0};
-    double2 tau[R];
-#pragma unroll
-    for (int r = 0; r < R; ++r) tau[r] = double2{0, 0};
-    double cur[NV], vf[R];
-#pragma unroll
-    for (int v = 0; v < NV; ++v) cur[v] = RD(n - 1, v);
-#pragma unroll
-    for (int r = 0; r < R; ++r) vf[r] = 0.0;
-    for (int i = n - 1; i >= 0; --i) {
-      double nxt[NV], vfn[R];
-      const int ip = (i > 0) ? i - 1 : 0;
-#pragma unroll
-      for (int v = 0; v < NV; ++v) nxt[v] = RD(ip, v);
-#pragma unroll
-      for (int r = 0; r < R; ++r) vfn[r] = (ip + F < n) ? ss.V[(size_t)(ip + F) * R + r] : 0.0;
-      if (i + F < n) {
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-          tau[r].x += vf[r] * xw[F].x;
-          tau[r].y += vf[r] * xw[F].y;
-        }
-      }
-      double2 acc{cur[NV - 2], cur[NV - 1]};
-#pragma unroll
-      for (int q = 1; q <= Q; ++q) {
-        acc.x -= cur[q] * xw[q].x;
-        acc.y -= cur[q] * xw[q].y;
-      }
-#pragma unroll
-      for (int r = 0; r < R; ++r) {
-        acc.x -= cur[1 + Q + r] * tau[r].x;
-        acc.y -= cur[1 + Q + r] * tau[r].y;
-      }
-      const double2 xi{acc.x * cur[0], acc.y * cur[0]};
-      const int J = 2 * i + par;
-      if (mean) {
-        a.v_out[J] = xi.x;
-        a.w_out[J] = xi.y;
-      } else {
-        xout[(size_t)J * M + m] = xi;
-      }
-#pragma unroll
-      for (int k = F; k >= 2; --k) xw[k] = xw[k - 1];
-      xw[1] = xi;
-#pragma unroll
-      for (int v = 0; v < NV; ++v) cur[v] = nxt[v];
-#pragma unroll
-      for (int r = 0; r < R; ++r) vf[r] = vfn[r];
-    }
-  }
-}
-
-// Block b: (system, parity) = b / bps with bps = ceil(M / SL) blocks per
-// system, modes (b % bps) * SL + lane; the last block solves the two mean
-// parities in lanes 0 and 1.
-template <int SL>
-__global__ void __launch_bounds__(256) k_solve_steady_lds(Geo g, StructMats sm, SolveArgs a) {
-  extern __shared__ __align__(16) unsigned char smem[];
-  constexpr int SLP = SL + 1;  // lane slot SL: the mean system in the blocks that carry it
-  const int M = g.m;
-  const int bps = (M + SL - 1) / SL;
-  const int spb = blockIdx.x / bps;  // block's (system, parity)
-  const int m0 = (blockIdx.x - spb * bps) * SL;
-  const bool has_mean = (spb >= 2) && (m0 == 0);  // Dirichlet blocks of mode 0 also solve S00
-  double *zn = reinterpret_cast<double *>(smem);
-  double *rd = zn + (size_t)g.npmax * 4 * SLP;
-  const int sys = spb >> 1, par = spb & 1;
-  const StructSys &ss = sm.sys[sys][par];
-  const int n = ss.n;
-
-  // phase 0 (all threads): z = sum c_l x_l and N = sum e_l N_l of every row
-  // of every lane, thread -> (row, lane) with lanes fastest
-  for (int idx = threadIdx.x; idx < n * SLP; idx += blockDim.x) {
-    const int j = idx / SLP, lane = idx - j * SLP;
-    const bool mean = (lane == SL);
-    const int m = m0 + lane;
-    if (mean ? !has_mean : m >= M) continue;
-    const int J = 2 * j + par;
-    double2 z{0, 0}, nn{0, 0};
-    if (!mean) {
-      const size_t o = (size_t)J * M + m;
-#pragma unroll
-      for (int l = 0; l < 3; ++l) {
-        const double2 xv = (sys == 0) ? a.u0[l][o] : a.g[l][o];
-        const double2 nv = (sys == 0) ? a.Nu[l][o] : a.Ng[l][o];
-        z.x += a.c[l] * xv.x;
-        z.y += a.c[l] * xv.y;
-        nn.x += a.e[l] * nv.x;
-        nn.y += a.e[l] * nv.y;
-      }
-    } else {
-#pragma unroll
-      for (int l = 0; l < 3; ++l) {
-        z.x += a.c[l] * a.v[l][J];
-        z.y += a.c[l] * a.w[l][J];
-        nn.x += a.e[l] * a.Nv[l][J];
-        nn.y += a.e[l] * a.Nw[l][J];
-      }
-    }
-    double *q = zn + (size_t)j * 4 * SLP + lane;
-    q[0] = z.x;
-    q[SLP] = z.y;
-    q[2 * SLP] = nn.x;
-    q[3 * SLP] = nn.y;
-  }
-  __syncthreads();
-
-  // sweeps: threads 0..SL-1 take the modes, thread 64 (a second wavefront,
-  // running concurrently) takes the mean system when the block carries it
-  int lane;
-  bool mean = false;
-  if (threadIdx.x < SL) {
-    lane = threadIdx.x;
-  } else if (threadIdx.x == 64 && has_mean) {
-    lane = SL;
-    mean = true;
-  } else {
-    return;
-  }
-  const int m = mean ? 0 : m0 + lane;
-  const bool active = mean || m < M;
-  double kyd, kzd, kyo;
-  mode_k(g, active ? m : 0, kyd, kzd, kyo);
-  const double k2 = mean ? 0.0 : kyd * kyd + kzd * kzd;
-  const double W = mean ? 1.0 : g.W, nu = g.nu, gm = a.gamma;
-  const int Ms = mean ? 1 : M;
-  if (sys == 0) {
-    const double s2 = W * (1 + 2 * gm * nu * k2), s4 = -W * gm * nu, s0 = -W * (k2 + gm * nu * k2 * k2);
-    steady_solve_lds<2, 2, 2, SLP>(ss, n, par, m, Ms, active, s0, s2, s4, s4, -W * k2, W, a, true,
-                                   false, zn, rd, a.u0_out, lane);
-  } else {
-    const double s0 = W * (1 + gm * nu * k2), s2 = -W * gm * nu;
-    steady_solve_lds<1, 1, 1, SLP>(ss, n, par, m, Ms, active, s0, s2, 0.0, s2, W, 0.0, a, false,
-                                   mean, zn, rd, a.g_out, lane);
-  }
-}
-
 // Velocity reconstruction (channelflow_kmm.py:855-870): f = R u0 with
 // R = P_D d/dx on the clamped basis, u1 = i (ky f + kz g) / k^2,
 // u2 = i (kz f - ky g) / k^2, mean mode (0,0) <- (v00, w00).  R couples
@@ -1270,46 +699,16 @@ hipError_t launch_rotational(const Geo &g, const NodalMats<T> &nm, Workspace &ws
   return hipGetLastError();
 }
 
-// LDS of the steady solve: z/N (4) and LU row data (7) per row and lane slot.
-size_t steady_lds_bytes(const Geo &g, int sl) {
-  return (size_t)g.npmax * (4 + 7) * (sl + 1) * sizeof(double);
-}
-
 size_t solve_scratch_doubles(const Geo &g) {
   return (size_t)4 * g.npmax * 7 * g.m + (size_t)2 * g.npmax * 7;
 }
 
 hipError_t launch_solve(const Geo &g, const StructMats &sm, const SolveArgs &a, double *scratch,
-                        bool steady, hipStream_t s) {
+                        hipStream_t s) {
   const int Mp = (g.m + 63) / 64 * 64;
   const int threads = 4 * Mp + 64;
-  if (steady) {
-    // 32 lanes per block while the LDS rows fit (n <= 40), else 16
-    if (g.npmax <= 40) {
-      const int bps = (g.m + 31) / 32;
-      hipLaunchKernelGGL(k_solve_steady_lds<32>, dim3(4 * bps), dim3(256), steady_lds_bytes(g, 32), s,
-                         g, sm, a);
-    } else {
-      const int bps = (g.m + 15) / 16;
-      hipLaunchKernelGGL(k_solve_steady_lds<16>, dim3(4 * bps), dim3(256), steady_lds_bytes(g, 16), s,
-                         g, sm, a);
-    }
-    (void)scratch;
-  } else
-    hipLaunchKernelGGL(k_solve, dim3((threads + 255) / 256), dim3(256), 0, s, g, sm, a, scratch);
+  hipLaunchKernelGGL(k_solve, dim3((threads + 255) / 256), dim3(256), 0, s, g, sm, a, scratch);
   return hipGetLastError();
-}
-
-hipError_t kernels_init(const Geo &g) {
-  hipError_t e;
-  if (g.npmax <= 40)
-    e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_solve_steady_lds<32>),
-                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)steady_lds_bytes(g, 32));
-  else
-    e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_solve_steady_lds<16>),
-                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)steady_lds_bytes(g, 16));
-  if (e != hipSuccess) (void)hipGetLastError();  // do not leave the error sticky
-  return e;
 }
 
 hipError_t launch_reconstruct(const Geo &g, const double *Rc, int ldk, const SolveArgs &a,

@@ -91,6 +91,14 @@ struct SolveArgs {
   double2 *u1, *u2;
 };
 
+// Per-mode LU factors of the steady systems at one gamma (computed once per
+// handle): [row][value][mode] per (basis, parity) block, [row][value] for the
+// two parities of the mean system.
+struct FactorSet {
+  double *blk[2][2];
+  double *mean[2];
+};
+
 struct Workspace {
   void *X;   // [6][n0][m]   nodal u, omega
   void *Y;   // [6][n0][my][kzs]
@@ -105,15 +113,18 @@ hipError_t launch_fft_y(const Geo &g, const NodalMats<T> &nm, Workspace &ws, con
 template <typename T>
 hipError_t launch_rotational(const Geo &g, const NodalMats<T> &nm, Workspace &ws,
                              const FFTPlan &pz, hipStream_t s);
-// steady = true: SBDF3 update (three solution and nonlinear levels, d = 0);
-// false: general update (IMEXRK3 stages).
+// General implicit update (IMEXRK3 stages): the factorisation runs with the solve.
 hipError_t launch_solve(const Geo &g, const StructMats &sm, const SolveArgs &a, double *scratch,
-                        bool steady, hipStream_t s);
+                        hipStream_t s);
 hipError_t launch_reconstruct(const Geo &g, const double *Rc, int ldk, const SolveArgs &a,
                               hipStream_t s);
 size_t solve_scratch_doubles(const Geo &g);
-// Per-configuration kernel attributes (dynamic LDS limits); call once per handle.
-hipError_t kernels_init(const Geo &g);
+// Factored steady solves: factor storage (basis -1: the mean system), the
+// one-time factorisation at gamma, and the per-step solve (SBDF3 update).
+size_t factor_doubles(const Geo &g, const StructMats &sm, int basis, int par);
+hipError_t launch_factor(const Geo &g, const StructMats &sm, double gamma, FactorSet fs, hipStream_t s);
+hipError_t launch_solve_factored(const Geo &g, const StructMats &sm, const FactorSet &fs, const SolveArgs &a,
+                                 hipStream_t s);
 
 // Diagnostics partial sums: out[0] = sum wF |u|^2 (perturbation), out[1] = sum
 // wF |div u|^2, both over the full spectrum (Hermitian weights applied), and

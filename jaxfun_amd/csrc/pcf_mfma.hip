@@ -403,6 +403,20 @@ __global__ void __launch_bounds__(512) k_rhs_adj_mfma(Geo g, AdjMats<T> am, Prim
       B.at(f, i, 2 * j + 1) = (T)z[t].y;
     }
   }
+  // mean mode (block 0): AG^T nu_v and AG^T nu_w for every node, one node per thread
+  T *meanv = reinterpret_cast<T *>(smem) + (size_t)2 * kp * 16;
+  if (ct == 0) {
+    for (int x = threadIdx.x; x < g.n0; x += blockDim.x) {
+      T sv = 0, sw = 0;
+      for (int i = 0; i < g.nd; ++i) {
+        const T a = am.AG[(size_t)i * am.ldx + x];
+        sv += a * (T)nu.v[i];
+        sw += a * (T)nu.w[i];
+      }
+      meanv[2 * x] = sv;
+      meanv[2 * x + 1] = sw;
+    }
+  }
   __syncthreads();
   const Lane L = lane_of(g, ct);
   double kyd, kzd, kyo;
@@ -438,14 +452,8 @@ __global__ void __launch_bounds__(512) k_rhs_adj_mfma(Geo g, AdjMats<T> am, Prim
     T h2 = -s * W * (kz * p1 - ky * pg);
     const int x = x0 + MM::row(L.l, r);
     if (mean_lane && x < g.n0) {  // real part of the mean mode: - AG^T nu_v, - AG^T nu_w
-      T sv = 0, sw = 0;
-      for (int i = 0; i < g.nd; ++i) {
-        const T a = am.AG[(size_t)i * am.ldx + x];
-        sv += a * (T)nu.v[i];
-        sw += a * (T)nu.w[i];
-      }
-      h1 -= sv;
-      h2 -= sw;
+      h1 -= meanv[2 * x];
+      h2 -= meanv[2 * x + 1];
     }
     if (L.valid && x < g.n0) {
       const size_t o = (size_t)x * 2 * M + 2 * L.mcol + L.comp;
@@ -504,8 +512,9 @@ template <typename T>
 hipError_t launch_rhs_adj_mfma(const Geo &g, const AdjMats<T> &am, Prim nu, Workspace &ws, hipStream_t s) {
   using V = typename VT<T>::V;
   if (!stage_covers(g.n0, STAGE_RHS_ADJ, 2 * ((g.nd + 3) / 4 * 4) * 8)) return hipErrorInvalidValue;
-  hipLaunchKernelGGL((k_rhs_adj_mfma<T>), mfma_grid(g), mfma_block(g.n0), panel_bytes<T>(2, g.nd), s, g, am,
-                     nu, reinterpret_cast<V *>(ws.Hx));
+  hipLaunchKernelGGL((k_rhs_adj_mfma<T>), mfma_grid(g), mfma_block(g.n0),
+                     panel_bytes<T>(2, g.nd) + 2 * (size_t)g.n0 * sizeof(T), s, g, am, nu,
+                     reinterpret_cast<V *>(ws.Hx));
   return hipGetLastError();
 }
 
