@@ -116,7 +116,7 @@ def run_gpu(precision: str, steps: int, warmup: int, eager: bool = False):
 # stage name -> kernel-name fragment (FP64 instance) in the rocprofv3 traffic summary
 _KERNEL_OF = {"nodal_backward": "k_nodal_mfma<double>", "fft_y_inverse": "k_fft_y<double, true",
               "rotational": "k_rotational<double", "fft_y_forward": "k_fft_y<double, false",
-              "forward_rhs": "k_rhs_mfma<double>", "solve": "k_solve_steady", "reconstruct": "k_reconstruct"}
+              "forward_rhs": "k_rhs_mfma<double>", "solve": "k_solve_factored", "reconstruct": "k_reconstruct"}
 TRAFFIC_FILE = ROOT / "profiles" / "traffic_65x64x64.json"
 
 
