@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define PCF_ABI_VERSION 1
+#define PCF_ABI_VERSION 2
 
 /* error codes */
 #define PCF_OK 0
@@ -65,6 +65,16 @@ typedef struct pcf_config {
   int32_t time_integrator; /* PCF_TI_SBDF3                                   */
   int32_t nonlinear_form;  /* PCF_NL_ROTATIONAL                              */
   int32_t device;          /* HIP device ordinal                             */
+  /* Magnetohydrodynamics (examples/pcf_mhd_jax.py:69-114): mhd = 1 evolves the
+   * magnetic vector potential A in TD^3 (conducting walls) with magnetic
+   * diffusivity eta = U_wall / Rm, coupled to the flow by the Lorentz force
+   * J x B and the induction EMF u_tot x B; b0 is a uniform background field
+   * added to B in physical space (examples/pcf_mhd_mri_shearpy_jax.py:149-151;
+   * zeros for PlaneCouetteMHDJax).  mhd = 0 ignores eta and b0.            */
+  int32_t mhd;
+  int32_t reserved;
+  double eta;
+  double b0[3];
 } pcf_config;
 
 typedef struct pcf_handle pcf_handle;
@@ -80,6 +90,14 @@ enum pcf_field {
   PCF_F_NL_OLDER_U, PCF_F_NL_OLDER_G, PCF_F_NL_OLDER_V, PCF_F_NL_OLDER_W,
   PCF_F_SOL_OLD_U, PCF_F_SOL_OLD_G, PCF_F_SOL_OLD_V, PCF_F_SOL_OLD_W,
   PCF_F_SOL_OLDER_U, PCF_F_SOL_OLDER_G, PCF_F_SOL_OLDER_V, PCF_F_SOL_OLDER_W,
+  /* MHD handles only (MHDState, examples/pcf_mhd_jax.py:34-66): the vector
+   * potential A = (Ax, Ay, Az) in TD, its nonlinear rows MA HA of the two
+   * previous steps and A of the two previous steps. */
+  PCF_F_AX, PCF_F_AY, PCF_F_AZ,
+  PCF_F_NL_A_OLD_X, PCF_F_NL_A_OLD_Y, PCF_F_NL_A_OLD_Z,
+  PCF_F_NL_A_OLDER_X, PCF_F_NL_A_OLDER_Y, PCF_F_NL_A_OLDER_Z,
+  PCF_F_A_OLD_X, PCF_F_A_OLD_Y, PCF_F_A_OLD_Z,
+  PCF_F_A_OLDER_X, PCF_F_A_OLDER_Y, PCF_F_A_OLDER_Z,
   PCF_F_COUNT
 };
 
@@ -106,6 +124,9 @@ int64_t pcf_field_size(const pcf_config *cfg, int32_t field);
  *   12 VB1 = clamped-basis x-derivative values        (n0 x n0-4)
  *   13 FB, 14 FD: Galerkin projection of nodal values onto the clamped /
  *      Dirichlet basis (n0-4 | n0-2 x n0) — composite.py:361-365
+ *   15 VQ1, 16 VQ2: nodal values VD P_D D^k S_D^T (k = 1, 2) of the Galerkin
+ *      projection P_D of x-derivatives of Dirichlet fields (n0 x n0-2), which
+ *      J = curl B needs — composite.py:293-299, pcf_mhd_jax.py:291-311
  * Replaces the SymPy inner() assembly of examples/channelflow_kmm.py:261-409. */
 int32_t pcf_setup_matrix(const pcf_config *cfg, int32_t which, double *out,
                          int32_t *rows, int32_t *cols);
@@ -148,6 +169,12 @@ int32_t pcf_step_steady_graph(pcf_handle *h, int32_t nsteps, int32_t replays);
  *  mean_shear}; replaces PlaneCouetteFluctuationJax.diagnostics
  * (examples/pcf_fluctuations_jax.py:131-161). */
 int32_t pcf_diagnostics(pcf_handle *h, double *out8);
+
+/* Magnetic diagnostics of an MHD handle (synchronous).  out[2] =
+ * {Emag, divB_L2}: the magnetic energy sum_i int |b_i|^2 and the L2 norm of
+ * div B on the unpadded grid; replaces the additions of
+ * PlaneCouetteMHDJax.diagnostics (examples/pcf_mhd_jax.py:760-789). */
+int32_t pcf_mhd_diagnostics(pcf_handle *h, double *out2);
 
 /* Per-stage timing.  One steady step launches pcf_kernels_per_step() kernels,
  * one per stage, in the order pcf_stage_name(0..pcf_stage_count()-1):

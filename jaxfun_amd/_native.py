@@ -14,7 +14,7 @@ import numpy as np
 
 LIB_PATH = Path(__file__).resolve().parent / "libpcfdns.so"
 
-ABI_VERSION = 1
+ABI_VERSION = 2
 PREC_FP64 = 0
 PREC_MIXED = 1
 TI_SBDF3 = 3
@@ -27,13 +27,18 @@ FIELDS = (
     "NL_OLDER_U", "NL_OLDER_G", "NL_OLDER_V", "NL_OLDER_W",
     "SOL_OLD_U", "SOL_OLD_G", "SOL_OLD_V", "SOL_OLD_W",
     "SOL_OLDER_U", "SOL_OLDER_G", "SOL_OLDER_V", "SOL_OLDER_W",
+    "AX", "AY", "AZ",
+    "NL_A_OLD_X", "NL_A_OLD_Y", "NL_A_OLD_Z",
+    "NL_A_OLDER_X", "NL_A_OLDER_Y", "NL_A_OLDER_Z",
+    "A_OLD_X", "A_OLD_Y", "A_OLD_Z",
+    "A_OLDER_X", "A_OLDER_Y", "A_OLDER_Z",
 )
 FIELD_ID = {name: i for i, name in enumerate(FIELDS)}
 
 # pcf_setup_matrix indices
 MATRIX_ID = {
     "C0B": 0, "C2B": 1, "C4B": 2, "C0D": 3, "C2D": 4, "VB": 5, "VD": 6, "VD1": 7,
-    "A1": 8, "A0": 9, "AG": 10, "R": 11, "VB1": 12, "FB": 13, "FD": 14,
+    "A1": 8, "A0": 9, "AG": 10, "R": 11, "VB1": 12, "FB": 13, "FD": 14, "VQ1": 15, "VQ2": 16,
 }
 
 # Every symbol include/pcf_dns.h declares.
@@ -41,7 +46,7 @@ EXPORTS = (
     "pcf_abi_version", "pcf_last_error", "pcf_field_size", "pcf_setup_matrix",
     "pcf_create", "pcf_destroy", "pcf_set_stream", "pcf_synchronize",
     "pcf_set_field", "pcf_get_field", "pcf_set_history", "pcf_get_history",
-    "pcf_step", "pcf_step_steady", "pcf_step_steady_graph", "pcf_diagnostics",
+    "pcf_step", "pcf_step_steady", "pcf_step_steady_graph", "pcf_diagnostics", "pcf_mhd_diagnostics",
     "pcf_kernels_per_step", "pcf_stage_count", "pcf_stage_name", "pcf_profile_steady",
     "pcf_set_cotangent", "pcf_vjp", "pcf_get_gradient",
 )
@@ -67,6 +72,10 @@ class PcfConfig(C.Structure):
         ("time_integrator", C.c_int32),
         ("nonlinear_form", C.c_int32),
         ("device", C.c_int32),
+        ("mhd", C.c_int32),
+        ("reserved", C.c_int32),
+        ("eta", C.c_double),
+        ("b0", C.c_double * 3),
     ]
 
 
@@ -108,6 +117,7 @@ def load() -> C.CDLL:
         "pcf_step_steady": ([h, C.c_int32], C.c_int32),
         "pcf_step_steady_graph": ([h, C.c_int32, C.c_int32], C.c_int32),
         "pcf_diagnostics": ([h, dp], C.c_int32),
+        "pcf_mhd_diagnostics": ([h, dp], C.c_int32),
         "pcf_kernels_per_step": ([h], C.c_int32),
         "pcf_stage_count": ([], C.c_int32),
         "pcf_stage_name": ([C.c_int32], C.c_char_p),

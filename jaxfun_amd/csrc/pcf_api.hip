@@ -52,6 +52,8 @@ std::string validate(const pcf_config *c) {
   if (c->precision != PCF_PREC_FP64 && c->precision != PCF_PREC_MIXED) return "unknown precision";
   if (c->time_integrator != PCF_TI_SBDF3) return "only SBDF3 is built";
   if (c->nonlinear_form != PCF_NL_ROTATIONAL) return "only the rotational nonlinear form is built";
+  if (c->mhd != 0 && c->mhd != 1) return "mhd must be 0 or 1";
+  if (c->mhd && !(c->eta > 0)) return "eta must be positive";
   return "";
 }
 
@@ -79,7 +81,7 @@ struct StageTimer {
 template <typename T>
 struct DevMats {
   T *VBt = nullptr, *VDt = nullptr, *VD1t = nullptr, *A1t = nullptr, *A0t = nullptr,
-    *AGt = nullptr, *xnode = nullptr, *twy = nullptr, *twz = nullptr;
+    *AGt = nullptr, *xnode = nullptr, *twy = nullptr, *twz = nullptr, *VQ1t = nullptr, *VQ2t = nullptr;
   pcf::NodalMats<T> view{};
 };
 
@@ -100,6 +102,16 @@ struct pcf_handle {
   double2 *u1 = nullptr, *u2 = nullptr;
   double2 *Nu[4]{}, *Ng[4]{};
   double *Nv[4]{}, *Nw[4]{};
+  // MHD: vector potential levels (three components [c][row][m], cstride =
+  // nd * m apart) on the solution ring and their nonlinear rows on the
+  // nonlinear ring; they rotate with the flow's rings (one coupled timeline,
+  // examples/pcf_mhd_jax.py:217-248).
+  bool mhd = false;
+  double2 *A[3]{}, *NA[4]{};
+  size_t cstride = 0;
+  pcf::Geo geo_eta{};         // geo with nu = eta: the A systems are the g systems at eta
+  pcf::FactorSet fA{};        // steady LU factors of the A systems (Dirichlet blocks)
+  double *ddiagm = nullptr;
   int so[3] = {0, 1, 2};     // solution ring: cur, old, older
   int no[4] = {0, 1, 2, 3};  // nonlinear ring: free0, old, older, free1
   double history_steps = 0, have_old = 0, previous_dt = 0;
@@ -184,6 +196,10 @@ int32_t build_nodal(pcf_handle *h, DevMats<T> &dm, int xc, int ic) {
   if ((r = h->upload(&dm.A1t, transpose_pad(o.A1, ldi)))) return r;
   if ((r = h->upload(&dm.A0t, transpose_pad(o.A0, ldi)))) return r;
   if ((r = h->upload(&dm.AGt, transpose_pad(o.AG, ldi)))) return r;
+  if (h->mhd) {
+    if ((r = h->upload(&dm.VQ1t, transpose_pad(o.VQ1, ldx)))) return r;
+    if ((r = h->upload(&dm.VQ2t, transpose_pad(o.VQ2, ldx)))) return r;
+  }
   std::vector<T> xn(n0);
   for (int j = 0; j < n0; ++j) xn[j] = (T)o.x[j];
   if ((r = h->upload(&dm.xnode, xn))) return r;
@@ -199,7 +215,7 @@ int32_t build_nodal(pcf_handle *h, DevMats<T> &dm, int xc, int ic) {
   if ((r = h->upload(&dm.twy, twiddles(h->geo.my)))) return r;
   if ((r = h->upload(&dm.twz, twiddles(h->geo.mz)))) return r;
   dm.view = pcf::NodalMats<T>{dm.VBt, dm.VDt, dm.VD1t, ldx, dm.A1t, dm.A0t, dm.AGt, ldi,
-                              dm.xnode, dm.twy, dm.twz};
+                              dm.xnode, dm.twy, dm.twz, dm.VQ1t, dm.VQ2t};
   (void)nb;
   return PCF_OK;
 }
@@ -247,7 +263,19 @@ int32_t build_factors(pcf_handle *h, const pcf::StructMats &sm, double gamma, pc
       if ((r = h->alloc(&fs.blk[b][p], pcf::factor_doubles(h->geo, sm, b, p)))) return r;
   for (int p = 0; p < 2; ++p)
     if ((r = h->alloc(&fs.mean[p], pcf::factor_doubles(h->geo, sm, -1, p)))) return r;
-  hipError_t e = pcf::launch_factor(h->geo, sm, gamma, fs, h->stream);
+  hipError_t e = pcf::launch_factor(h->geo, sm, gamma, fs, false, h->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  if (e != hipSuccess) return fail(PCF_E_HIP, std::string("factorisation: ") + hipGetErrorString(e));
+  return PCF_OK;
+}
+
+// Steady factors of the vector-potential systems S_A = M_A - gamma L_A, the
+// Dirichlet systems at nu = eta (examples/pcf_mhd_jax.py:116-139).
+int32_t build_factors_A(pcf_handle *h, double gamma) {
+  int32_t r;
+  for (int p = 0; p < 2; ++p)
+    if ((r = h->alloc(&h->fA.blk[1][p], pcf::factor_doubles(h->geo, h->sm, 1, p)))) return r;
+  hipError_t e = pcf::launch_factor(h->geo_eta, h->sm, gamma, h->fA, true, h->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
   if (e != hipSuccess) return fail(PCF_E_HIP, std::string("factorisation: ") + hipGetErrorString(e));
   return PCF_OK;
@@ -266,51 +294,78 @@ inline hipError_t mark(pcf_handle *h, int stage, int edge) {
   return hipEventRecord(t->ev[stage][edge], h->stream);
 }
 
+// Under MHD the same stages carry (u, w, b, j) forward and (H, E) back, and
+// the A fields of the state (A, three components) give b and j and receive
+// the induction rows in NA[slot].
 template <typename T>
-hipError_t nonlinear_stages(pcf_handle *h, const pcf::NodalMats<T> &nm, const double2 *u0, int slot) {
+hipError_t nonlinear_stages(pcf_handle *h, const pcf::NodalMats<T> &nm, const double2 *u0, int slot,
+                            const double2 *A) {
   const pcf::Geo &g = h->geo;
+  const bool mhd = h->mhd;
   hipError_t e;
   if ((e = mark(h, 0, 0))) return e;
   if ((e = pcf::launch_nodal_mfma<T>(g, nm, u0, h->u1, h->u2, h->ws, h->stream))) return e;
+  if (mhd && (e = pcf::launch_nodal_mhd_mfma<T>(g, nm, A, h->cstride, h->ws, h->stream))) return e;
   if ((e = mark(h, 0, 1))) return e;
   if ((e = mark(h, 1, 0))) return e;
-  if ((e = pcf::launch_fft_y<T>(g, nm, h->ws, h->py, true, h->stream))) return e;
+  if ((e = pcf::launch_fft_y<T>(g, nm, h->ws, h->py, true, mhd, h->stream))) return e;
   if ((e = mark(h, 1, 1))) return e;
   if ((e = mark(h, 2, 0))) return e;
-  if ((e = pcf::launch_rotational<T>(g, nm, h->ws, h->pz, h->stream))) return e;
+  if ((e = pcf::launch_rotational<T>(g, nm, h->ws, h->pz, mhd, h->stream))) return e;
   if ((e = mark(h, 2, 1))) return e;
   if ((e = mark(h, 3, 0))) return e;
-  if ((e = pcf::launch_fft_y<T>(g, nm, h->ws, h->py, false, h->stream))) return e;
+  if ((e = pcf::launch_fft_y<T>(g, nm, h->ws, h->py, false, mhd, h->stream))) return e;
   if ((e = mark(h, 3, 1))) return e;
   if ((e = mark(h, 4, 0))) return e;
   if ((e = pcf::launch_rhs_mfma<T>(g, nm, h->ws, h->Nu[slot], h->Ng[slot], h->Nv[slot], h->Nw[slot],
                               h->stream)))
     return e;
+  if (mhd && (e = pcf::launch_rhs_a_mfma<T>(g, nm, h->ws, h->NA[slot], h->cstride, h->stream))) return e;
   return mark(h, 4, 1);
 }
 
 // Nonlinear rows of the state whose wall-normal velocity is u0 (with the
-// handle's u1/u2) into nonlinear slot `slot`.  With ytape set, the padded
-// y-line fields of u and curl u are written there (and kept) instead of the
-// workspace: the reverse pass reads them back.
-int32_t nonlinear_into(pcf_handle *h, const double2 *u0, int slot, void *ytape = nullptr) {
+// handle's u1/u2; under MHD with vector potential A) into nonlinear slot
+// `slot`.  With ytape set, the padded y-line fields of u and curl u are
+// written there (and kept) instead of the workspace: the reverse pass reads
+// them back.
+int32_t nonlinear_into(pcf_handle *h, const double2 *u0, int slot, void *ytape = nullptr,
+                       const double2 *A = nullptr) {
   void *saved = h->ws.Y;
   if (ytape) h->ws.Y = ytape;
   hipError_t e = (h->cfg.precision == PCF_PREC_FP64)
-                     ? nonlinear_stages<double>(h, h->md.view, u0, slot)
-                     : nonlinear_stages<float>(h, h->mf.view, u0, slot);
+                     ? nonlinear_stages<double>(h, h->md.view, u0, slot, A)
+                     : nonlinear_stages<float>(h, h->mf.view, u0, slot, A);
   h->ws.Y = saved;
   if (e != hipSuccess) return fail(PCF_E_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
   return PCF_OK;
 }
 
-// Structured solve + reconstruction of one implicit update.
-hipError_t implicit_update(pcf_handle *h, const pcf::SolveArgs &a, bool steady) {
+// Structured solve + reconstruction of one implicit update; under MHD also
+// the vector-potential solve described by aA (g side, component 0).
+hipError_t implicit_update(pcf_handle *h, const pcf::SolveArgs &a, bool steady,
+                           const pcf::SolveArgs *aA = nullptr) {
   hipError_t e;
   if ((e = mark(h, 5, 0))) return e;
-  if ((e = steady ? pcf::launch_solve_factored(h->geo, h->sm, h->fsteady, a, h->stream)
-                  : pcf::launch_solve(h->geo, h->sm, a, h->scratch, h->stream)))
+  if ((e = steady ? pcf::launch_solve_factored(h->geo, h->sm, h->fsteady, a, 0, 0, h->stream)
+                  : pcf::launch_solve(h->geo, h->sm, a, h->scratch, false, h->stream)))
     return e;
+  if (aA) {
+    if (steady) {
+      if ((e = pcf::launch_solve_factored(h->geo_eta, h->sm, h->fA, *aA, 3, h->cstride, h->stream))) return e;
+    } else {
+      for (int c = 0; c < 3; ++c) {
+        pcf::SolveArgs ac = *aA;
+        const size_t off = c * h->cstride;
+        for (int l = 0; l < 3; ++l) {
+          if (ac.g[l]) ac.g[l] += off;
+          if (ac.Ng[l]) ac.Ng[l] += off;
+        }
+        ac.g_out += off;
+        if ((e = pcf::launch_solve(h->geo_eta, h->sm, ac, h->scratch, true, h->stream))) return e;
+      }
+    }
+  }
   if ((e = mark(h, 5, 1))) return e;
   if ((e = mark(h, 6, 0))) return e;
   if ((e = pcf::launch_reconstruct(h->geo, h->Rt, h->ldr, a, h->stream))) return e;
@@ -357,11 +412,33 @@ void rotate(pcf_handle *h) {
   h->no[3] = n2;
 }
 
-// One steady SBDF3 step (sbdf3.py:44-62; channelflow_kmm.py:1016-1064).
+// The vector-potential part of an implicit update: the g side of SolveArgs
+// carries component 0 of A (the solver steps the other two cstride apart).
+pcf::SolveArgs a_args_from(pcf_handle *h, const pcf::SolveArgs &a, const int xs[3], int nx, const int ns[3],
+                           int nn, int out_slot) {
+  pcf::SolveArgs b{};
+  for (int l = 0; l < nx; ++l) {
+    b.g[l] = h->A[xs[l]];
+    b.c[l] = a.c[l];
+    b.d[l] = a.d[l];
+  }
+  b.nx = nx;
+  for (int l = 0; l < nn; ++l) {
+    b.Ng[l] = h->NA[ns[l]];
+    b.e[l] = a.e[l];
+  }
+  b.nn = nn;
+  b.gamma = a.gamma;
+  b.g_out = h->A[out_slot];
+  return b;
+}
+
+// One steady SBDF3 step (sbdf3.py:44-62; channelflow_kmm.py:1016-1064; under
+// MHD with the vector-potential update, examples/pcf_mhd_jax.py:643-715).
 int32_t steady_step(pcf_handle *h, void *const *tape = nullptr) {
   const double dt = h->cfg.dt;
   const int A = h->no[0];
-  int32_t r = nonlinear_into(h, h->u0[h->so[0]], A, tape ? tape[0] : nullptr);
+  int32_t r = nonlinear_into(h, h->u0[h->so[0]], A, tape ? tape[0] : nullptr, h->A[h->so[0]]);
   if (r) return r;
   pcf::SolveArgs a = base_args(h, h->so[2]);
   set_x(a, 0, h, h->so[0], 18.0 / 11.0, 0.0);
@@ -373,7 +450,12 @@ int32_t steady_step(pcf_handle *h, void *const *tape = nullptr) {
   set_n(a, 2, h, h->no[2], dt * 6.0 / 11.0);
   a.nn = 3;
   a.gamma = dt * 6.0 / 11.0;
-  hipError_t e = implicit_update(h, a, true);
+  pcf::SolveArgs aA{};
+  if (h->mhd) {
+    const int xs[3] = {h->so[0], h->so[1], h->so[2]}, ns[3] = {A, h->no[1], h->no[2]};
+    aA = a_args_from(h, a, xs, 3, ns, 3, h->so[2]);
+  }
+  hipError_t e = implicit_update(h, a, true, h->mhd ? &aA : nullptr);
   if (e != hipSuccess) return fail(PCF_E_HIP, std::string("solve launch: ") + hipGetErrorString(e));
   rotate(h);
   h->history_steps = 2.0;
@@ -383,20 +465,21 @@ int32_t steady_step(pcf_handle *h, void *const *tape = nullptr) {
 }
 
 // One IMEXRK3 startup step (channelflow_kmm.py:872-935 inside the SBDF3
-// wrapper :937-1014).  The stage-0 nonlinear rows are the step's current
-// nonlinear rows, which the history shift keeps.
+// wrapper :937-1014; under MHD with the vector-potential stages,
+// examples/pcf_mhd_jax.py:488-631).  The stage-0 nonlinear rows are the
+// step's current nonlinear rows, which the history shift keeps.
 int32_t startup_step(pcf_handle *h, void *const *tape = nullptr) {
   const double dt = h->cfg.dt;
   const int A = h->no[0], B = h->no[3], C = h->no[2];
   const int cur = h->so[0], out = h->so[2];
-  int32_t r = nonlinear_into(h, h->u0[cur], A, tape ? tape[0] : nullptr);
+  int32_t r = nonlinear_into(h, h->u0[cur], A, tape ? tape[0] : nullptr, h->A[cur]);
   if (r) return r;
   const int nslot[3] = {A, B, C};
   for (int rk = 0; rk < 3; ++rk) {
     const double gamma = (IMEX_A[rk] + IMEX_B[rk]) * dt / 2.0;
     const int xs = (rk == 0) ? cur : out;
     if (rk > 0) {
-      r = nonlinear_into(h, h->u0[out], nslot[rk], tape ? tape[rk] : nullptr);
+      r = nonlinear_into(h, h->u0[out], nslot[rk], tape ? tape[rk] : nullptr, h->A[out]);
       if (r) return r;
     }
     pcf::SolveArgs a = base_args(h, out);
@@ -409,7 +492,12 @@ int32_t startup_step(pcf_handle *h, void *const *tape = nullptr) {
       a.nn = 2;
     }
     a.gamma = gamma;
-    hipError_t e = implicit_update(h, a, false);
+    pcf::SolveArgs aA{};
+    if (h->mhd) {
+      const int xl[3] = {xs, 0, 0}, ns[3] = {nslot[rk], rk > 0 ? nslot[rk - 1] : 0, 0};
+      aA = a_args_from(h, a, xl, 1, ns, a.nn, out);
+    }
+    hipError_t e = implicit_update(h, a, false, h->mhd ? &aA : nullptr);
     if (e != hipSuccess) return fail(PCF_E_HIP, std::string("solve launch: ") + hipGetErrorString(e));
   }
   rotate(h);
@@ -446,6 +534,18 @@ bool field_ref(pcf_handle *h, int f, FieldRef &fr) {
       default: return {h->Nw[s], false, nd};
     }
   };
+  if (f >= PCF_F_AX && f < PCF_F_COUNT) {
+    if (!h->mhd) return false;
+    const int k = f - PCF_F_AX, grp = k / 3, comp = k % 3;
+    // groups: A, NL_A_OLD, NL_A_OLDER, A_OLD, A_OLDER
+    double2 *base = (grp == 0)   ? h->A[h->so[0]]
+                    : (grp == 1) ? h->NA[h->no[1]]
+                    : (grp == 2) ? h->NA[h->no[2]]
+                    : (grp == 3) ? h->A[h->so[1]]
+                                 : h->A[h->so[2]];
+    fr = {base + comp * h->cstride, true, nd};
+    return true;
+  }
   switch (f) {
     case PCF_F_U0: fr = sol(0, 0); return true;
     case PCF_F_G: fr = sol(0, 1); return true;
@@ -913,11 +1013,13 @@ int32_t pcf_create(const pcf_config *cfg, pcf_handle **out) {
   g.uwall = cfg->u_wall;
   g.W = (2.0 * M_PI / g.df1) * (2.0 * M_PI / g.df2);
   g.nu = cfg->nu;
+  h->mhd = cfg->mhd != 0;
+  for (int c = 0; c < 3; ++c) g.b0[c] = h->mhd ? cfg->b0[c] : 0.0;
   if (g.my % 2) return bail(fail(PCF_E_INVALID, "padded streamwise grid must be even"));
   if (g.mz < 2 * g.kzs) return bail(fail(PCF_E_INVALID, "spanwise padding must not truncate"));
   if (!make_plan(g.my, h->py) || !make_plan(g.mz, h->pz))
     return bail(fail(PCF_E_INVALID, "padded grid sizes must factor into 2, 3, 4"));
-  if (!pcf::fft_fits(g, h->py, h->pz, cfg->precision == PCF_PREC_FP64 ? 16 : 8))
+  if (!pcf::fft_fits(g, h->py, h->pz, cfg->precision == PCF_PREC_FP64 ? 16 : 8, h->mhd))
     return bail(fail(PCF_E_INVALID, "FFT sizes exceed the kernels' register budget"));
 
   h->device = cfg->device;
@@ -942,10 +1044,18 @@ int32_t pcf_create(const pcf_config *cfg, pcf_handle **out) {
   }
   if ((r = h->alloc(&h->u1, (size_t)g.nd * M))) return bail(r);
   if ((r = h->alloc(&h->u2, (size_t)g.nd * M))) return bail(r);
+  if (h->mhd) {
+    h->cstride = (size_t)g.nd * M;
+    for (int s = 0; s < 3; ++s)
+      if ((r = h->alloc(&h->A[s], 3 * h->cstride))) return bail(r);
+    for (int s = 0; s < 4; ++s)
+      if ((r = h->alloc(&h->NA[s], 3 * h->cstride))) return bail(r);
+  }
   const size_t ve = (cfg->precision == PCF_PREC_FP64) ? sizeof(double2) : sizeof(float2);
   unsigned char *wsp = nullptr;
-  const size_t nX = 6 * (size_t)g.n0 * M, nY = 6 * (size_t)g.n0 * g.my * g.kzs;
-  const size_t nHz = 3 * (size_t)g.n0 * g.my * g.kzs, nHx = 3 * (size_t)g.n0 * M;
+  const size_t fin = h->mhd ? 12 : 6, fout = h->mhd ? 6 : 3;
+  const size_t nX = fin * g.n0 * M, nY = fin * g.n0 * g.my * g.kzs;
+  const size_t nHz = fout * g.n0 * g.my * g.kzs, nHx = fout * g.n0 * M;
   if ((r = h->alloc(&wsp, (nX + nY + nHz + nHx) * ve))) return bail(r);
   h->ws.X = wsp;
   h->ws.Y = wsp + nX * ve;
@@ -964,6 +1074,12 @@ int32_t pcf_create(const pcf_config *cfg, pcf_handle **out) {
       return bail(fail(PCF_E_UNSUPPORTED, std::string("kernel LDS limits: ") + hipGetErrorString(e)));
   }
   if ((r = build_factors(h, h->sm, cfg->dt * 6.0 / 11.0, h->fsteady))) return bail(r);
+  if (h->mhd) {
+    h->geo_eta = h->geo;
+    h->geo_eta.nu = cfg->eta;
+    if ((r = build_factors_A(h, cfg->dt * 6.0 / 11.0))) return bail(r);
+    if ((r = h->alloc(&h->ddiagm, 2))) return bail(r);
+  }
   if ((r = h->upload(&h->dVB, h->ops.VB.a))) return bail(r);
   if ((r = h->upload(&h->dVD, h->ops.VD.a))) return bail(r);
   if ((r = h->upload(&h->dVB1, h->ops.VB1.a))) return bail(r);
@@ -1168,6 +1284,24 @@ int32_t pcf_diagnostics(pcf_handle *h, double *out) {
   return PCF_OK;
 }
 
+int32_t pcf_mhd_diagnostics(pcf_handle *h, double *out) {
+  if (!h || !out) return fail(PCF_E_INVALID, "null argument");
+  if (!h->mhd) return fail(PCF_E_INVALID, "not an MHD handle");
+  const pcf::Geo &g = h->geo;
+  PCF_HIP(hipSetDevice(h->device));
+  PCF_HIP(hipMemsetAsync(h->ddiagm, 0, 2 * sizeof(double), h->stream));
+  hipError_t e = pcf::launch_diag_mhd(g, h->dVD, h->dVD1, h->dfejer, h->A[h->so[0]], h->cstride, h->ddiagm,
+                                      h->stream);
+  if (e != hipSuccess) return fail(PCF_E_HIP, std::string("diag launch: ") + hipGetErrorString(e));
+  double s[2];
+  PCF_HIP(hipMemcpyAsync(s, h->ddiagm, sizeof(s), hipMemcpyDeviceToHost, h->stream));
+  PCF_HIP(hipStreamSynchronize(h->stream));
+  const double vol = h->cfg.ly * h->cfg.lz;
+  out[0] = vol * s[0];
+  out[1] = std::sqrt(vol * s[1]);
+  return PCF_OK;
+}
+
 int32_t pcf_stage_count(void) { return NSTAGE; }
 
 const char *pcf_stage_name(int32_t i) { return (i >= 0 && i < NSTAGE) ? STAGE_NAMES[i] : ""; }
@@ -1233,6 +1367,7 @@ int32_t pcf_set_cotangent(pcf_handle *h, int32_t field, const double *host) {
 
 int32_t pcf_vjp(pcf_handle *h, int32_t steps, int32_t checkpoint_every, int32_t seed) {
   if (!h) return fail(PCF_E_INVALID, "null handle");
+  if (h->mhd) return fail(PCF_E_UNSUPPORTED, "the reverse mode covers the hydrodynamic step only");
   if (steps < 1) return fail(PCF_E_INVALID, "steps must be >= 1");
   if (checkpoint_every < 1) return fail(PCF_E_INVALID, "checkpoint_every must be >= 1");
   if (seed != PCF_SEED_ENERGY && seed != PCF_SEED_COTANGENT) return fail(PCF_E_INVALID, "unknown seed");
@@ -1393,8 +1528,9 @@ int32_t pcf_get_gradient(pcf_handle *h, int32_t field, double *host) {
 }
 
 int32_t pcf_kernels_per_step(pcf_handle *h) {
-  (void)h;
-  return NSTAGE;
+  if (!h) return fail(PCF_E_INVALID, "null handle");
+  // MHD adds the b/j transform, the induction rows and the A solve
+  return h->mhd ? NSTAGE + 3 : NSTAGE;
 }
 
 }  // extern "C"

@@ -197,8 +197,15 @@ __device__ __forceinline__ Lane2 lane2(const Geo &g, int t) {
   return s;
 }
 
-__global__ void __launch_bounds__(256) k_factor(Geo g, StructMats sm, double gamma, FactorSet fs) {
-  const Lane2 L = lane2(g, blockIdx.x * blockDim.x + threadIdx.x);
+// dir_only: the Dirichlet blocks alone (the vector-potential systems), no mean.
+__global__ void __launch_bounds__(256) k_factor(Geo g, StructMats sm, double gamma, FactorSet fs, bool dir_only) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (dir_only) {
+    const int Mp = (g.m + 63) / 64 * 64;
+    t += 2 * Mp;
+    if (t >= 4 * Mp) return;
+  }
+  const Lane2 L = lane2(g, t);
   if (!L.valid) return;
   const StructSys &ss = sm.sys[L.sys][L.par];
   double kyd, kzd, kyo;
@@ -221,15 +228,27 @@ __global__ void __launch_bounds__(256) k_factor(Geo g, StructMats sm, double gam
 // Steady update: rhs = M z + N, z = sum c_l x_l, N = sum e_l N_l over the three
 // levels, then the factored solve.  Block: SL modes of one (basis, parity)
 // block; the Dirichlet blocks of mode 0 also carry the mean system in lane SL.
+// spb0 = 2: the Dirichlet blocks only, without the mean system, for
+// gridDim.y fields whose g-side pointers advance by cstride per field (the
+// three vector-potential components).
 template <int SL>
-__global__ void __launch_bounds__(256) k_solve_factored(Geo g, StructMats sm, FactorSet fs, SolveArgs a) {
+__global__ void __launch_bounds__(256) k_solve_factored(Geo g, StructMats sm, FactorSet fs, SolveArgs a,
+                                                        int spb0, size_t cstride) {
   extern __shared__ __align__(16) unsigned char smem[];
   constexpr int SLP = SL + 1;
   const int M = g.m;
   const int bps = (M + SL - 1) / SL;
-  const int spb = blockIdx.x / bps;
-  const int m0 = (blockIdx.x - spb * bps) * SL;
-  const bool has_mean = (spb >= 2) && (m0 == 0);
+  const int spb = blockIdx.x / bps + spb0;
+  const int m0 = (blockIdx.x - (spb - spb0) * bps) * SL;
+  const bool has_mean = (spb0 == 0) && (spb >= 2) && (m0 == 0);
+  if (blockIdx.y) {
+    const size_t off = blockIdx.y * cstride;
+    for (int l = 0; l < 3; ++l) {
+      if (a.g[l]) a.g[l] += off;
+      if (a.Ng[l]) a.Ng[l] += off;
+    }
+    a.g_out += off;
+  }
   double *zn = reinterpret_cast<double *>(smem);  // [row][4][SLP]: z.x, z.y, rhs.x, rhs.y
   double *fl = zn + (size_t)g.npmax * 4 * SLP;    // [row][NF][SLP]: the lanes' factors
   const int sys = spb >> 1, par = spb & 1;
@@ -425,18 +444,22 @@ size_t factor_doubles(const Geo &g, const StructMats &sm, int basis, int par) {
   return (size_t)sm.sys[basis][par].n * (basis == 0 ? NF_B : NF_D) * g.m;
 }
 
-hipError_t launch_factor(const Geo &g, const StructMats &sm, double gamma, FactorSet fs, hipStream_t s) {
-  const int threads = 4 * ((g.m + 63) / 64 * 64) + 64;
-  hipLaunchKernelGGL(k_factor, dim3((threads + 255) / 256), dim3(256), 0, s, g, sm, gamma, fs);
+hipError_t launch_factor(const Geo &g, const StructMats &sm, double gamma, FactorSet fs, bool dir_only,
+                         hipStream_t s) {
+  const int Mp = (g.m + 63) / 64 * 64;
+  const int threads = dir_only ? 2 * Mp : 4 * Mp + 64;
+  hipLaunchKernelGGL(k_factor, dim3((threads + 255) / 256), dim3(256), 0, s, g, sm, gamma, fs, dir_only);
   return hipGetLastError();
 }
 
 hipError_t launch_solve_factored(const Geo &g, const StructMats &sm, const FactorSet &fs, const SolveArgs &a,
-                                 hipStream_t s) {
+                                 int ncomp, size_t cstride, hipStream_t s) {
+  // ncomp > 0: the Dirichlet blocks only, for ncomp fields cstride apart
+  const int spb0 = ncomp > 0 ? 2 : 0, nf = ncomp > 0 ? ncomp : 1;
   // the most lanes per block whose rows and factors fit the default 64 KB of LDS
 #define PCF_FSOLVE(SLV)                                                                                  \
-  hipLaunchKernelGGL(k_solve_factored<SLV>, dim3(4 * ((g.m + SLV - 1) / SLV)), dim3(256),                 \
-                     factored_lds_bytes(g, SLV), s, g, sm, fs, a)
+  hipLaunchKernelGGL(k_solve_factored<SLV>, dim3((4 - spb0) * ((g.m + SLV - 1) / SLV), nf), dim3(256),    \
+                     factored_lds_bytes(g, SLV), s, g, sm, fs, a, spb0, cstride)
   if (factored_lds_bytes(g, 16) <= 65536) PCF_FSOLVE(16);
   else if (factored_lds_bytes(g, 8) <= 65536) PCF_FSOLVE(8);
   else return hipErrorInvalidValue;

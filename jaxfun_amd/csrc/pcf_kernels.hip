@@ -95,27 +95,32 @@ __global__ void __launch_bounds__(256) k_fft_y(Geo g, const typename VT<T>::V *_
 // c2r synthesis and the r2c analysis.
 // NZC > 0: the padded spanwise length is the compile-time NZC (the launcher
 // dispatches the common lengths); NZC = 0 reads it from g.
-template <typename T, int MAXB, int NZC>
+// MHD: twelve input fields (u, w, b, j) and six outputs, H = -(u_tot x w_tot)
+// - j x b_tot and E = u_tot x b_tot with b_tot = b + B0
+// (examples/pcf_mhd_jax.py:339-357; the background B0 of
+// examples/pcf_mhd_mri_shearpy_jax.py:149-151).
+template <typename T, int MAXB, int NZC, bool MHD>
 __global__ void __launch_bounds__(256) k_rotational(Geo g, NodalMats<T> nm, FFTPlan plan, int UPB,
                                                     const typename VT<T>::V *__restrict__ Y,
                                                     typename VT<T>::V *__restrict__ Hz) {
   using V = typename VT<T>::V;
+  constexpr int NIN = MHD ? 12 : 6, NOUT = MHD ? 6 : 3;
   extern __shared__ __align__(16) unsigned char smem[];
   const int MZ = NZC ? NZC : g.mz;
-  V *buf = reinterpret_cast<V *>(smem);  // [UPB][6][MZ]
-  V *out = buf + (size_t)UPB * 6 * MZ;   // [UPB][3][MZ]
+  V *buf = reinterpret_cast<V *>(smem);   // [UPB][NIN][MZ]
+  V *out = buf + (size_t)UPB * NIN * MZ;  // [UPB][NOUT][MZ]
   const int tid = threadIdx.x, nthr = blockDim.x;
   const int KZ = g.kzs, MY = g.my;
   const int nunits = g.n0 * (MY / 2);
   const int ubase = blockIdx.x * UPB;
   const size_t fstride = (size_t)g.n0 * MY * KZ;  // field stride in Y / Hz
-  V *tw = out + (size_t)UPB * 3 * g.mz;            // twiddle table in LDS
+  V *tw = out + (size_t)UPB * NOUT * g.mz;         // twiddle table in LDS
   for (int i = tid; i < MZ; i += nthr) tw[i] = reinterpret_cast<const V *>(nm.twz)[i];
 
   gather_batched<4>(
-      UPB * 6 * MZ,
+      UPB * NIN * MZ,
       [&](int idx) {
-        const int lu = idx / (6 * MZ), rem = idx - lu * 6 * MZ;
+        const int lu = idx / (NIN * MZ), rem = idx - lu * NIN * MZ;
         const int f = rem / MZ, k = rem - f * MZ;
         const int u = ubase + lu;
         V z{0, 0};
@@ -138,50 +143,59 @@ __global__ void __launch_bounds__(256) k_rotational(Geo g, NodalMats<T> nm, FFTP
       [&](int idx, const V &z) { buf[idx] = z; });
   __syncthreads();
   if constexpr (NZC > 0)
-    fft_lds_c<V, T, NZC, true, false, MAXB>(buf, tw, UPB * 6, MZ, 1);
+    fft_lds_c<V, T, NZC, true, false, MAXB>(buf, tw, UPB * NIN, MZ, 1);
   else
-    fft_lds<V, T, true, false, MAXB>(buf, tw, plan, UPB * 6, MZ, 1);
+    fft_lds<V, T, true, false, MAXB>(buf, tw, plan, UPB * NIN, MZ, 1);
 
   const T U = (T)g.uwall;
+  const T B0x = (T)g.b0[0], B0y = (T)g.b0[1], B0z = (T)g.b0[2];
   for (int idx = tid; idx < UPB * MZ; idx += nthr) {
     int lu = idx / MZ, n = idx - lu * MZ;
     int u = ubase + lu;
     int x = (u < nunits) ? u / (MY / 2) : 0;
     const T xv = nm.xnode[x];
-    const V *b = buf + (size_t)lu * 6 * MZ + n;
-    V vu0 = b[0], vu1 = b[MZ], vu2 = b[2 * MZ], vw0 = b[3 * MZ], vw1 = b[4 * MZ], vw2 = b[5 * MZ];
-    V h0, h1, h2;
-    {
-      T a0 = vu0.x, a1 = vu1.x + U * xv, a2 = vu2.x, o0 = vw0.x, o1 = vw1.x, o2 = vw2.x + U;
-      h0.x = -(a1 * o2 - a2 * o1);
-      h1.x = -(a2 * o0 - a0 * o2);
-      h2.x = -(a0 * o1 - a1 * o0);
+    const V *b = buf + (size_t)lu * NIN * MZ + n;
+    V *o = out + (size_t)lu * NOUT * MZ + n;
+    // the two real lines ride in the real and imaginary parts
+#pragma unroll
+    for (int part = 0; part < 2; ++part) {
+      auto P = [&](int f) { return part ? b[f * MZ].y : b[f * MZ].x; };
+      const T a0 = P(0), a1 = P(1) + U * xv, a2 = P(2), o0 = P(3), o1 = P(4), o2 = P(5) + U;
+      T h0 = -(a1 * o2 - a2 * o1), h1 = -(a2 * o0 - a0 * o2), h2 = -(a0 * o1 - a1 * o0);
+      T e0 = 0, e1 = 0, e2 = 0;
+      if constexpr (MHD) {
+        const T c0 = P(6) + B0x, c1 = P(7) + B0y, c2 = P(8) + B0z, j0 = P(9), j1 = P(10), j2 = P(11);
+        h0 -= j1 * c2 - j2 * c1;
+        h1 -= j2 * c0 - j0 * c2;
+        h2 -= j0 * c1 - j1 * c0;
+        e0 = a1 * c2 - a2 * c1;
+        e1 = a2 * c0 - a0 * c2;
+        e2 = a0 * c1 - a1 * c0;
+      }
+      const T r[6] = {h0, h1, h2, e0, e1, e2};
+#pragma unroll
+      for (int f = 0; f < NOUT; ++f) {
+        if (part)
+          o[f * MZ].y = r[f];
+        else
+          o[f * MZ].x = r[f];
+      }
     }
-    {
-      T a0 = vu0.y, a1 = vu1.y + U * xv, a2 = vu2.y, o0 = vw0.y, o1 = vw1.y, o2 = vw2.y + U;
-      h0.y = -(a1 * o2 - a2 * o1);
-      h1.y = -(a2 * o0 - a0 * o2);
-      h2.y = -(a0 * o1 - a1 * o0);
-    }
-    V *o = out + (size_t)lu * 3 * MZ + n;
-    o[0] = h0;
-    o[MZ] = h1;
-    o[2 * MZ] = h2;
   }
   __syncthreads();
   if constexpr (NZC > 0)
-    fft_lds_c<V, T, NZC, false, false, MAXB>(out, tw, UPB * 3, MZ, 1);
+    fft_lds_c<V, T, NZC, false, false, MAXB>(out, tw, UPB * NOUT, MZ, 1);
   else
-    fft_lds<V, T, false, false, MAXB>(out, tw, plan, UPB * 3, MZ, 1);
+    fft_lds<V, T, false, false, MAXB>(out, tw, plan, UPB * NOUT, MZ, 1);
 
   const T scale = (T)(1.0 / ((double)MY * (double)MZ));
-  for (int idx = tid; idx < UPB * 3 * KZ; idx += nthr) {
-    int lu = idx / (3 * KZ), rem = idx - lu * 3 * KZ;
+  for (int idx = tid; idx < UPB * NOUT * KZ; idx += nthr) {
+    int lu = idx / (NOUT * KZ), rem = idx - lu * NOUT * KZ;
     int f = rem / KZ, k = rem - f * KZ;
     int u = ubase + lu;
     if (u >= nunits) continue;
     int x = u / (MY / 2), y2 = u - x * (MY / 2);
-    const V *P = out + ((size_t)lu * 3 + f) * MZ;
+    const V *P = out + ((size_t)lu * NOUT + f) * MZ;
     V p = P[k], q = cconj(P[(MZ - k) % MZ]);
     V ha{(p.x + q.x) * (T)0.5 * scale, (p.y + q.y) * (T)0.5 * scale};
     V hb{(p.y - q.y) * (T)0.5 * scale, -(p.x - q.x) * (T)0.5 * scale};
@@ -446,9 +460,15 @@ __device__ void structured_solve(const StructSys &ss, int n, int par, int m, int
 // Threads [0, 4 Mp): (system, parity) = (t / Mp) with system 0 = clamped u0,
 // 1 = Dirichlet g; lanes over modes.  Threads [4 Mp, 4 Mp + 2): the 1-D mean
 // system S00 = C0_D - gamma nu C2_D for (v00 + i w00), parity = lane.
-__global__ void __launch_bounds__(256) k_solve(Geo g, StructMats sm, SolveArgs a, double *scratch) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+// dir_only: threads [0, 2 Mp) take the Dirichlet systems alone (a vector-
+// potential component), without the mean system.
+__global__ void __launch_bounds__(256) k_solve(Geo g, StructMats sm, SolveArgs a, double *scratch, bool dir_only) {
   const int M = g.m, Mp = (M + 63) / 64 * 64;
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (dir_only) {
+    t += 2 * Mp;
+    if (t >= 4 * Mp) return;
+  }
   int sp = t / Mp, m = t - sp * Mp;
   bool mean = false;
   if (sp >= 4) {
@@ -594,6 +614,65 @@ __global__ void __launch_bounds__(256) k_diag(Geo g, const double *__restrict__ 
     for (int q = 0; q < 4; ++q) atomicAdd(&out[q], red[q][0]);
 }
 
+// Magnetic diagnostics of the vector potential A (three Dirichlet components
+// cstride apart): out[0] += sum wF |b|^2 and out[1] += sum wF |div b|^2 over
+// the full spectrum (Hermitian weights), with b = curl A at the nodes
+// (bx = VD (i ky Az - i kz Ay), by = i kz VD Ax - VD1 Az, bz = VD1 Ay - i ky VD Ax;
+// examples/pcf_mhd_jax.py:760-789).
+__global__ void __launch_bounds__(256) k_diag_mhd(Geo g, const double *__restrict__ VD,
+                                                  const double *__restrict__ VD1,
+                                                  const double *__restrict__ fejer,
+                                                  const double2 *__restrict__ A, size_t cstride,
+                                                  double *__restrict__ out) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  double eb = 0.0, dv = 0.0;
+  if (m < g.m) {
+    double kyd, kzd, kyo;
+    mode_k(g, m, kyd, kzd, kyo);
+    const double ky = kyo, kz = kzd;
+    const double wt = ((m % g.kzs) == 0) ? 1.0 : 2.0;
+    for (int x = 0; x < g.n0; ++x) {
+      double2 DX{0, 0}, DY{0, 0}, DZ{0, 0}, EY{0, 0}, EZ{0, 0};
+      for (int i = 0; i < g.nd; ++i) {
+        const size_t o = (size_t)i * g.m + m;
+        const double2 ax = A[o], ay = A[cstride + o], az = A[2 * cstride + o];
+        const double vd = VD[(size_t)x * g.nd + i], vd1 = VD1[(size_t)x * g.nd + i];
+        DX.x += vd * ax.x;
+        DX.y += vd * ax.y;
+        DY.x += vd * ay.x;
+        DY.y += vd * ay.y;
+        DZ.x += vd * az.x;
+        DZ.y += vd * az.y;
+        EY.x += vd1 * ay.x;
+        EY.y += vd1 * ay.y;
+        EZ.x += vd1 * az.x;
+        EZ.y += vd1 * az.y;
+      }
+      // i z = (-z.y, z.x)
+      const double2 bx{-(ky * DZ.y - kz * DY.y), ky * DZ.x - kz * DY.x};
+      const double2 by{-kz * DX.y - EZ.x, kz * DX.x - EZ.y};
+      const double2 bz{EY.x + ky * DX.y, EY.y - ky * DX.x};
+      // div b = d/dx bx + i ky by + i kz bz, d/dx bx = VD1 (i ky Az - i kz Ay)
+      const double2 dbx{-(ky * EZ.y - kz * EY.y), ky * EZ.x - kz * EY.x};
+      const double2 d{dbx.x - ky * by.y - kz * bz.y, dbx.y + ky * by.x + kz * bz.x};
+      const double w = fejer[x] * wt;
+      eb += w * (bx.x * bx.x + bx.y * bx.y + by.x * by.x + by.y * by.y + bz.x * bz.x + bz.y * bz.y);
+      dv += w * (d.x * d.x + d.y * d.y);
+    }
+  }
+  __shared__ double red[2][256];
+  red[0][threadIdx.x] = eb;
+  red[1][threadIdx.x] = dv;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s)
+      for (int q = 0; q < 2; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    for (int q = 0; q < 2; ++q) atomicAdd(&out[q], red[q][0]);
+}
+
 // ------------------------------------------------------------ launchers ----
 namespace {
 
@@ -654,34 +733,38 @@ hipError_t launch_fft_y_fields(const Geo &g, const NodalMats<T> &nm, const FFTPl
 
 template <typename T>
 hipError_t launch_fft_y(const Geo &g, const NodalMats<T> &nm, Workspace &ws, const FFTPlan &py,
-                        bool inverse, hipStream_t s) {
-  return inverse ? launch_fft_y_fields<T>(g, nm, py, true, 6, ws.X, ws.Y, s)
-                 : launch_fft_y_fields<T>(g, nm, py, false, 3, ws.Hz, ws.Hx, s);
+                        bool inverse, bool mhd, hipStream_t s) {
+  return inverse ? launch_fft_y_fields<T>(g, nm, py, true, mhd ? 12 : 6, ws.X, ws.Y, s)
+                 : launch_fft_y_fields<T>(g, nm, py, false, mhd ? 6 : 3, ws.Hz, ws.Hx, s);
 }
 
+// y-line pairs per block: two keep the butterfly registers small; LDS holds
+// (inputs + outputs) lines of MZ per unit plus the twiddle table
 template <typename T>
-int rot_units(const Geo &g) {
+int rot_units(const Geo &g, bool mhd) {
   using V = typename VT<T>::V;
-  // two y-line pairs per block keep the butterfly registers small; LDS holds
-  // 9 lines of MZ per unit
+  const int lines = mhd ? 18 : 9;
   int upb = 2;
-  while (upb > 1 && ((size_t)upb * 9 * g.mz + g.mz) * sizeof(V) > 65536) upb >>= 1;
+  while (upb > 1 && ((size_t)upb * lines * g.mz + g.mz) * sizeof(V) > 65536) upb >>= 1;
   return upb;
 }
 
 template <typename T>
 hipError_t launch_rotational(const Geo &g, const NodalMats<T> &nm, Workspace &ws,
-                             const FFTPlan &pz, hipStream_t s) {
+                             const FFTPlan &pz, bool mhd, hipStream_t s) {
   using V = typename VT<T>::V;
-  const int upb = rot_units<T>(g);
+  const int upb = rot_units<T>(g, mhd);
   const int nunits = g.n0 * (g.my / 2);
-  const size_t lds = ((size_t)upb * 9 * g.mz + g.mz) * sizeof(V);
-  const int mb = max_butterflies(pz, upb * 6);
+  const size_t lds = ((size_t)upb * (mhd ? 18 : 9) * g.mz + g.mz) * sizeof(V);
+  const int mb = max_butterflies(pz, upb * (mhd ? 12 : 6));
   const dim3 grid((nunits + upb - 1) / upb);
   const V *Y = reinterpret_cast<const V *>(ws.Y);
   V *Hz = reinterpret_cast<V *>(ws.Hz);
-#define PCF_ROTN(MB, NZ) \
-  hipLaunchKernelGGL((k_rotational<T, MB, NZ>), grid, dim3(256), lds, s, g, nm, pz, upb, Y, Hz)
+#define PCF_ROTN(MB, NZ)                                                                                 \
+  if (mhd)                                                                                               \
+    hipLaunchKernelGGL((k_rotational<T, MB, NZ, true>), grid, dim3(256), lds, s, g, nm, pz, upb, Y, Hz); \
+  else                                                                                                   \
+    hipLaunchKernelGGL((k_rotational<T, MB, NZ, false>), grid, dim3(256), lds, s, g, nm, pz, upb, Y, Hz)
 #define PCF_ROT(MB)                              \
   switch (g.mz) {                                \
     case 24: PCF_ROTN(MB, 24); break;            \
@@ -703,11 +786,11 @@ size_t solve_scratch_doubles(const Geo &g) {
   return (size_t)4 * g.npmax * 7 * g.m + (size_t)2 * g.npmax * 7;
 }
 
-hipError_t launch_solve(const Geo &g, const StructMats &sm, const SolveArgs &a, double *scratch,
+hipError_t launch_solve(const Geo &g, const StructMats &sm, const SolveArgs &a, double *scratch, bool dir_only,
                         hipStream_t s) {
   const int Mp = (g.m + 63) / 64 * 64;
-  const int threads = 4 * Mp + 64;
-  hipLaunchKernelGGL(k_solve, dim3((threads + 255) / 256), dim3(256), 0, s, g, sm, a, scratch);
+  const int threads = dir_only ? 2 * Mp : 4 * Mp + 64;
+  hipLaunchKernelGGL(k_solve, dim3((threads + 255) / 256), dim3(256), 0, s, g, sm, a, scratch, dir_only);
   return hipGetLastError();
 }
 
@@ -728,23 +811,28 @@ hipError_t launch_diag(const Geo &g, const double *VB, const double *VD, const d
   return hipGetLastError();
 }
 
+hipError_t launch_diag_mhd(const Geo &g, const double *VD, const double *VD1, const double *fejer,
+                           const double2 *A, size_t cstride, double *out, hipStream_t s) {
+  hipLaunchKernelGGL(k_diag_mhd, dim3((g.m + 255) / 256), dim3(256), 0, s, g, VD, VD1, fejer, A, cstride, out);
+  return hipGetLastError();
+}
 
-bool fft_fits(const Geo &g, const FFTPlan &py, const FFTPlan &pz, int elem_bytes) {
+bool fft_fits(const Geo &g, const FFTPlan &py, const FFTPlan &pz, int elem_bytes, bool mhd) {
   const int kc = pick_kc(g.kzs);
   if (((size_t)g.my * kc + g.my) * elem_bytes > 65536) return false;
   if (max_butterflies(py, kc) > 8) return false;
-  const int upb = (elem_bytes == 16) ? rot_units<double>(g) : rot_units<float>(g);
-  if (((size_t)upb * 9 * g.mz + g.mz) * elem_bytes > 65536) return false;
-  return max_butterflies(pz, upb * 6) <= 12;
+  const int upb = (elem_bytes == 16) ? rot_units<double>(g, mhd) : rot_units<float>(g, mhd);
+  if (((size_t)upb * (mhd ? 18 : 9) * g.mz + g.mz) * elem_bytes > 65536) return false;
+  return max_butterflies(pz, upb * (mhd ? 12 : 6)) <= 12;
 }
 
 #define PCF_INSTANTIATE(T)                                                                     \
   template hipError_t launch_fft_y<T>(const Geo &, const NodalMats<T> &, Workspace &,            \
-                                      const FFTPlan &, bool, hipStream_t);                        \
+                                      const FFTPlan &, bool, bool, hipStream_t);                  \
   template hipError_t launch_fft_y_fields<T>(const Geo &, const NodalMats<T> &, const FFTPlan &,   \
                                              bool, int, const void *, void *, hipStream_t);       \
   template hipError_t launch_rotational<T>(const Geo &, const NodalMats<T> &, Workspace &,       \
-                                           const FFTPlan &, hipStream_t);                        \
+                                           const FFTPlan &, bool, hipStream_t);                  \
 
 PCF_INSTANTIATE(double)
 PCF_INSTANTIATE(float)

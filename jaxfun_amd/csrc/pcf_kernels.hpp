@@ -29,6 +29,7 @@ struct Geo {
   double W;        // Fourier mass weight (2pi/df1)(2pi/df2) of every Galerkin row
   double nu;
   int npmax;       // largest parity block of the per-mode systems (<= 64)
+  double b0[3];    // uniform background magnetic field (MHD)
 };
 
 // Radix plan of one FFT length (radices 2, 3, 4).
@@ -53,6 +54,8 @@ struct NodalMats {
   const T *xnode; // [n0]
   const T *twy;   // [my] exp(-2 pi i t / my) as (cos, sin) pairs
   const T *twz;   // [mz]
+  const T *VQ1t;  // [nd][ldx] (VD P_D D S_D^T)^T, MHD only (else null)
+  const T *VQ2t;  // [nd][ldx] (VD P_D D^2 S_D^T)^T
 };
 
 // Structured per-mode operator data of one (basis, parity) block: the band
@@ -99,22 +102,32 @@ struct FactorSet {
   double *mean[2];
 };
 
+// Field counts double under MHD: X/Y hold (u, omega, b, j), Hz/Hx (H, E).
 struct Workspace {
-  void *X;   // [6][n0][m]   nodal u, omega
-  void *Y;   // [6][n0][my][kzs]
-  void *Hz;  // [3][n0][my][kzs]
-  void *Hx;  // [3][n0][m]
+  void *X;   // [6|12][n0][m]   nodal u, omega (, b, j)
+  void *Y;   // [6|12][n0][my][kzs]
+  void *Hz;  // [3|6][n0][my][kzs]
+  void *Hx;  // [3|6][n0][m]
 };
 
 // Launchers (return hipError_t of the launch).  One kernel each, in step order.
 template <typename T>
 hipError_t launch_fft_y(const Geo &g, const NodalMats<T> &nm, Workspace &ws, const FFTPlan &py,
-                        bool inverse, hipStream_t s);
+                        bool inverse, bool mhd, hipStream_t s);
 template <typename T>
 hipError_t launch_rotational(const Geo &g, const NodalMats<T> &nm, Workspace &ws,
-                             const FFTPlan &pz, hipStream_t s);
+                             const FFTPlan &pz, bool mhd, hipStream_t s);
+// MHD: b = curl A and j = curl b at the nodes (X fields 6..11) from the three
+// vector-potential components cstride apart; induction rows NA = W AG E.
+template <typename T>
+hipError_t launch_nodal_mhd_mfma(const Geo &g, const NodalMats<T> &nm, const double2 *A, size_t cstride,
+                                 Workspace &ws, hipStream_t s);
+template <typename T>
+hipError_t launch_rhs_a_mfma(const Geo &g, const NodalMats<T> &nm, Workspace &ws, double2 *NA, size_t cstride,
+                             hipStream_t s);
 // General implicit update (IMEXRK3 stages): the factorisation runs with the solve.
-hipError_t launch_solve(const Geo &g, const StructMats &sm, const SolveArgs &a, double *scratch,
+// dir_only: the Dirichlet systems alone (g side of a), for a vector-potential component.
+hipError_t launch_solve(const Geo &g, const StructMats &sm, const SolveArgs &a, double *scratch, bool dir_only,
                         hipStream_t s);
 hipError_t launch_reconstruct(const Geo &g, const double *Rc, int ldk, const SolveArgs &a,
                               hipStream_t s);
@@ -122,9 +135,12 @@ size_t solve_scratch_doubles(const Geo &g);
 // Factored steady solves: factor storage (basis -1: the mean system), the
 // one-time factorisation at gamma, and the per-step solve (SBDF3 update).
 size_t factor_doubles(const Geo &g, const StructMats &sm, int basis, int par);
-hipError_t launch_factor(const Geo &g, const StructMats &sm, double gamma, FactorSet fs, hipStream_t s);
+// dir_only / ncomp > 0: Dirichlet blocks alone (no mean), for ncomp fields
+// whose g-side pointers are cstride apart.
+hipError_t launch_factor(const Geo &g, const StructMats &sm, double gamma, FactorSet fs, bool dir_only,
+                         hipStream_t s);
 hipError_t launch_solve_factored(const Geo &g, const StructMats &sm, const FactorSet &fs, const SolveArgs &a,
-                                 hipStream_t s);
+                                 int ncomp, size_t cstride, hipStream_t s);
 
 // Diagnostics partial sums: out[0] = sum wF |u|^2 (perturbation), out[1] = sum
 // wF |div u|^2, both over the full spectrum (Hermitian weights applied), and
@@ -134,8 +150,12 @@ hipError_t launch_diag(const Geo &g, const double *VB, const double *VD, const d
                        const double2 *u0, const double2 *u1, const double2 *u2, double *out,
                        hipStream_t s);
 
+// MHD diagnostics partial sums: out[0] = sum wF |curl A|^2, out[1] = sum wF |div curl A|^2.
+hipError_t launch_diag_mhd(const Geo &g, const double *VD, const double *VD1, const double *fejer,
+                           const double2 *A, size_t cstride, double *out, hipStream_t s);
+
 // Largest butterfly count per thread any FFT launch needs (host-side check
 // against the kernels' register budgets).
-bool fft_fits(const Geo &g, const FFTPlan &py, const FFTPlan &pz, int elem_bytes);
+bool fft_fits(const Geo &g, const FFTPlan &py, const FFTPlan &pz, int elem_bytes, bool mhd);
 
 }  // namespace pcf

@@ -171,6 +171,171 @@ __global__ void __launch_bounds__(512) k_nodal_mfma(Geo g, NodalMats<T> nm, cons
   }
 }
 
+// Magnetic field and current of the vector potential A (all three components
+// in the Dirichlet basis), at the nodes: with DX = VD Ax, DY = VD Ay,
+// DZ = VD Az, EY = VD1 Ay, EZ = VD1 Az, QX = VQ1 Ax, RY = VQ2 Ay, RZ = VQ2 Az
+//   bx = i ky DZ - i kz DY,  by = i kz DX - EZ,  bz = EY - i ky DX,
+//   jx = k^2 DX + i ky EY + i kz EZ,
+//   jy = kz^2 DY - ky kz DZ - RY + i ky QX,
+//   jz = ky^2 DZ - ky kz DY - RZ + i kz QX,
+// where VQ1 = VD P_D D S_D^T and VQ2 = VD P_D D^2 S_D^T carry the Galerkin
+// projections of d/dx B onto the Dirichlet basis that J = curl B needs
+// (update_B_from_A / update_J_from_B, examples/pcf_mhd_jax.py:269-311).
+// Fields 6..11 of X receive (bx, by, bz, jx, jy, jz).
+template <typename T>
+__global__ void __launch_bounds__(512) k_nodal_mhd_mfma(Geo g, NodalMats<T> nm, const double2 *__restrict__ A,
+                                                        size_t cstride, typename VT<T>::V *__restrict__ X) {
+  using MM = Mfma<T>;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int ct = blockIdx.x;
+  const int M = g.m, kp = (g.nd + 3) / 4 * 4;
+  const Panel<T> B{reinterpret_cast<T *>(smem), kp};
+  {
+    const int total = 3 * kp * 8;
+    double2 z[STAGE_NODAL];
+#pragma unroll
+    for (int t = 0; t < STAGE_NODAL; ++t) {
+      const int idx = threadIdx.x + t * blockDim.x;
+      const int f = idx / (kp * 8), rem = idx - f * kp * 8, k = rem >> 3, j = rem & 7;
+      const int m = 8 * ct + j;
+      z[t] = double2{0, 0};
+      if (idx < total && m < M && k < g.nd) z[t] = A[f * cstride + (size_t)k * M + m];
+    }
+#pragma unroll
+    for (int t = 0; t < STAGE_NODAL; ++t) {
+      const int idx = threadIdx.x + t * blockDim.x;
+      if (idx >= total) continue;
+      const int f = idx / (kp * 8), rem = idx - f * kp * 8, k = rem >> 3, j = rem & 7;
+      B.at(f, k, 2 * j) = (T)z[t].x;
+      B.at(f, k, 2 * j + 1) = (T)z[t].y;
+    }
+  }
+  __syncthreads();
+  const Lane L = lane_of(g, ct);
+  double kyd, kzd, kyo;
+  mode_k(g, L.valid ? L.mcol : 0, kyd, kzd, kyo);
+  const T ky = (T)kyo, kz = (T)kzd, k2 = (T)(kyo * kyo + kzd * kzd);
+  const T kyy = ky * ky, kzz = kz * kz, kyz = ky * kz;
+  const T s = L.comp ? (T)1 : (T)-1;
+  const size_t plane = (size_t)g.n0 * 2 * M;
+  T *Xr = reinterpret_cast<T *>(X) + 6 * plane;
+  for (int x0 = (threadIdx.x >> 6) * 16; x0 < g.n0; x0 += (blockDim.x >> 6) * 16) {
+    const int xa = x0 + (L.l & 15);
+    typename MM::acc DX{}, DY{}, DZ{}, EY{}, EZ{}, QX{}, RY{}, RZ{};
+#pragma unroll 2
+    for (int kk = 0; kk < kp; kk += 4) {
+      const int k = kk + (L.l >> 4);
+      const T bx = B.at(0, k, L.c), by = B.at(1, k, L.c), bz = B.at(2, k, L.c);
+      T aD = 0, aD1 = 0, aQ1 = 0, aQ2 = 0;
+      if (k < g.nd) {
+        const size_t o = (size_t)k * nm.ldx + xa;
+        aD = nm.VDt[o];
+        aD1 = nm.VD1t[o];
+        aQ1 = nm.VQ1t[o];
+        aQ2 = nm.VQ2t[o];
+      }
+      DX = MM::mma(aD, bx, DX);
+      DY = MM::mma(aD, by, DY);
+      DZ = MM::mma(aD, bz, DZ);
+      EY = MM::mma(aD1, by, EY);
+      EZ = MM::mma(aD1, bz, EZ);
+      QX = MM::mma(aQ1, bx, QX);
+      RY = MM::mma(aQ2, by, RY);
+      RZ = MM::mma(aQ2, bz, RZ);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const T dx = DX[r], dy = DY[r], dz = DZ[r], ey = EY[r], ez = EZ[r], ry = RY[r], rz = RZ[r];
+      const T pdx = partner(dx), pdy = partner(dy), pdz = partner(dz), pey = partner(ey), pez = partner(ez),
+              pqx = partner(QX[r]);
+      const T bxv = s * (ky * pdz - kz * pdy);
+      const T byv = s * kz * pdx - ez;
+      const T bzv = ey - s * ky * pdx;
+      const T jxv = k2 * dx + s * (ky * pey + kz * pez);
+      const T jyv = kzz * dy - kyz * dz - ry + s * ky * pqx;
+      const T jzv = kyy * dz - kyz * dy - rz + s * kz * pqx;
+      const int x = x0 + MM::row(L.l, r);
+      if (L.valid && x < g.n0) {
+        const size_t o = (size_t)x * 2 * M + 2 * L.mcol + L.comp;
+        Xr[o] = bxv;
+        Xr[plane + o] = byv;
+        Xr[2 * plane + o] = bzv;
+        Xr[3 * plane + o] = jxv;
+        Xr[4 * plane + o] = jyv;
+        Xr[5 * plane + o] = jzv;
+      }
+    }
+  }
+}
+
+// Induction rows NA_c = W AG E_c (c = 0, 1, 2) of the nodal EMF E = u_tot x B
+// (Hx fields 3..5): the Dirichlet mass rows of forward_TD(E)
+// (_A_forward_emf + _A_mass_rhs, examples/pcf_mhd_jax.py:157-170).
+template <typename T>
+__global__ void __launch_bounds__(512) k_rhs_a_mfma(Geo g, NodalMats<T> nm,
+                                                    const typename VT<T>::V *__restrict__ Hx,
+                                                    double2 *__restrict__ NA, size_t cstride) {
+  using MM = Mfma<T>;
+  using V = typename VT<T>::V;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int ct = blockIdx.x;
+  const int M = g.m, kp = (g.n0 + 3) / 4 * 4;
+  const size_t plane = (size_t)g.n0 * M;
+  const Panel<T> B{reinterpret_cast<T *>(smem), kp};
+  {
+    const int total = kp * 8;
+    V hv[STAGE_RHS][3];
+#pragma unroll
+    for (int t = 0; t < STAGE_RHS; ++t) {
+      const int idx = threadIdx.x + t * blockDim.x;
+      const int x = idx >> 3, m = 8 * ct + (idx & 7);
+      hv[t][0] = hv[t][1] = hv[t][2] = V{0, 0};
+      if (idx < total && m < M && x < g.n0) {
+        const size_t o = (size_t)x * M + m;
+        hv[t][0] = Hx[3 * plane + o];
+        hv[t][1] = Hx[4 * plane + o];
+        hv[t][2] = Hx[5 * plane + o];
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < STAGE_RHS; ++t) {
+      const int idx = threadIdx.x + t * blockDim.x;
+      if (idx >= total) continue;
+      const int x = idx >> 3, j = idx & 7;
+#pragma unroll
+      for (int f = 0; f < 3; ++f) {
+        B.at(f, x, 2 * j) = hv[t][f].x;
+        B.at(f, x, 2 * j + 1) = hv[t][f].y;
+      }
+    }
+  }
+  __syncthreads();
+  const Lane L = lane_of(g, ct);
+  double *NAr = reinterpret_cast<double *>(NA);
+  for (int i0 = (threadIdx.x >> 6) * 16; i0 < g.nd; i0 += (blockDim.x >> 6) * 16) {
+    const int ia = i0 + (L.l & 15);
+    typename MM::acc C0{}, C1{}, C2{};
+#pragma unroll 4
+    for (int kk = 0; kk < kp; kk += 4) {
+      const int x = kk + (L.l >> 4);
+      const T ag = (x < g.n0) ? nm.AGt[(size_t)x * nm.ldi + ia] : (T)0;
+      C0 = MM::mma(ag, B.at(0, x, L.c), C0);
+      C1 = MM::mma(ag, B.at(1, x, L.c), C1);
+      C2 = MM::mma(ag, B.at(2, x, L.c), C2);
+    }
+    const double W = g.W;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int i = i0 + MM::row(L.l, r);
+      if (!L.valid || i >= g.nd) continue;
+      const size_t o = (size_t)i * 2 * M + 2 * L.mcol + L.comp;
+      NAr[o] = W * (double)C0[r];
+      NAr[2 * cstride + o] = W * (double)C1[r];
+      NAr[4 * cstride + o] = W * (double)C2[r];
+    }
+  }
+}
+
 // Nu = W (A1 c1 + A0 c0), Ng = W AG cg with c1 = i ky h1 + i kz h2, c0 = k^2 h0,
 // cg = i kz h1 - i ky h2 (rows: Shen index i, K: nodes x); mean mode
 // Nv00 = -AG Re h1, Nw00 = -AG Re h2 by the block that holds mode 0.
@@ -499,6 +664,27 @@ hipError_t launch_rhs_mfma(const Geo &g, const NodalMats<T> &nm, Workspace &ws, 
 }
 
 template <typename T>
+hipError_t launch_nodal_mhd_mfma(const Geo &g, const NodalMats<T> &nm, const double2 *A, size_t cstride,
+                                 Workspace &ws, hipStream_t s) {
+  using V = typename VT<T>::V;
+  if (!nm.VQ1t || !nm.VQ2t) return hipErrorInvalidValue;
+  if (!stage_covers(g.n0, STAGE_NODAL, 3 * ((g.nd + 3) / 4 * 4) * 8)) return hipErrorInvalidValue;
+  hipLaunchKernelGGL((k_nodal_mhd_mfma<T>), mfma_grid(g), mfma_block(g.n0), panel_bytes<T>(3, g.nd), s, g,
+                     nm, A, cstride, reinterpret_cast<V *>(ws.X));
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_rhs_a_mfma(const Geo &g, const NodalMats<T> &nm, Workspace &ws, double2 *NA, size_t cstride,
+                             hipStream_t s) {
+  using V = typename VT<T>::V;
+  if (!stage_covers(g.nd, STAGE_RHS, ((g.n0 + 3) / 4 * 4) * 8)) return hipErrorInvalidValue;
+  hipLaunchKernelGGL((k_rhs_a_mfma<T>), mfma_grid(g), mfma_block(g.nd), panel_bytes<T>(3, g.n0), s, g, nm,
+                     reinterpret_cast<const V *>(ws.Hx), NA, cstride);
+  return hipGetLastError();
+}
+
+template <typename T>
 hipError_t launch_nodal_adj_mfma(const Geo &g, const AdjMats<T> &am, Workspace &ws, double2 *mu0,
                                  double2 *mu1, double2 *mu2, hipStream_t s) {
   using V = typename VT<T>::V;
@@ -539,7 +725,11 @@ hipError_t mfma_init(const Geo &g, bool fp64) {
   template hipError_t launch_nodal_adj_mfma<T>(const Geo &, const AdjMats<T> &, Workspace &,        \
                                                double2 *, double2 *, double2 *, hipStream_t);       \
   template hipError_t launch_rhs_adj_mfma<T>(const Geo &, const AdjMats<T> &, Prim, Workspace &,    \
-                                             hipStream_t);
+                                             hipStream_t);                                          \
+  template hipError_t launch_nodal_mhd_mfma<T>(const Geo &, const NodalMats<T> &, const double2 *,  \
+                                               size_t, Workspace &, hipStream_t);                   \
+  template hipError_t launch_rhs_a_mfma<T>(const Geo &, const NodalMats<T> &, Workspace &, double2 *, \
+                                           size_t, hipStream_t);
 PCF_MFMA_INSTANTIATE(double)
 PCF_MFMA_INSTANTIATE(float)
 #undef PCF_MFMA_INSTANTIATE

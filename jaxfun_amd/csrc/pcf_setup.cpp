@@ -151,6 +151,11 @@ std::string build_operators(int n0, Operators &ops) {
   // Projection onto the clamped basis (state_from_physical of the initial data).
   LMat FB;
   if (!solve(C0B, matmul(SB, Wq), FB)) return "singular clamped mass matrix";
+  // MHD: Q_k = P_D D^k S_D^T, nodal values VD Q_k.
+  LMat Q1, Q2;
+  if (!solve(C0D, matmul(HSD, matmul(D, SDt)), Q1)) return "singular Dirichlet mass matrix";
+  if (!solve(C0D, matmul(HSD, matmul(D2, SDt)), Q2)) return "singular Dirichlet mass matrix";
+  LMat VQ1 = matmul(VD, Q1), VQ2 = matmul(VD, Q2);
 
   ops.C0B = to_double(C0B);
   ops.C2B = to_double(C2B);
@@ -167,6 +172,8 @@ std::string build_operators(int n0, Operators &ops) {
   ops.R = to_double(R);
   ops.FB = to_double(FB);
   ops.FD = to_double(FD);
+  ops.VQ1 = to_double(VQ1);
+  ops.VQ2 = to_double(VQ2);
 
   // Structured form of the parity blocks.  Far-upper factors from
   //   (T_a, T_b^(2)) = (pi/2)  b (b^2 - a^2)
@@ -296,6 +303,8 @@ const Mat *operator_by_index(const Operators &ops, int which) {
     case 12: return &ops.VB1;
     case 13: return &ops.FB;
     case 14: return &ops.FD;
+    case 15: return &ops.VQ1;
+    case 16: return &ops.VQ2;
     default: return nullptr;
   }
 }

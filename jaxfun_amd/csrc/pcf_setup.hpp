@@ -17,6 +17,9 @@
 //    Galerkin projection of nodal values onto the Dirichlet basis,
 //    F_D = M_D^{-1} S_D Wq with Wq[n][j] = (pi/n0) T_n(x_j)
 //    (Chebyshev.py:303-318, composite.py:361-365).
+//  * For MHD, the nodal values VD P_D D^k S_D^T (k = 1, 2) of the Galerkin
+//    projection P_D = M_D^{-1} S_D diag(h) of x-derivatives of Dirichlet
+//    fields (composite.py:293-299), which J = curl curl A needs.
 #pragma once
 
 #include <cstdint>
@@ -41,6 +44,7 @@ struct Operators {
   Mat A1, A0, AG;             // (nb|nd x n0): nonlinear rows from nodal values
   Mat R;                      // (nd x nb): f = P_D(d/dx u0)
   Mat FB, FD;                 // (nb|nd x n0): Galerkin projection of nodal values
+  Mat VQ1, VQ2;               // (n0 x nd): VD P_D D^k S_D^T, k = 1, 2 (MHD current)
   std::vector<double> x;      // Gauss nodes (ascending)
   std::vector<double> fejer;  // Fejer-I weights on the nodes
   std::vector<double> dphi_top, dphi_bot;  // phi_i'(+1), phi_i'(-1)

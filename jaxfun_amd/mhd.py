@@ -1,0 +1,176 @@
+"""Plane-Couette MHD on the MI355X: drop-in for the reference's conducting-wall solver.
+
+``PlaneCouetteMHDHIP`` mirrors ``PlaneCouetteMHDJax``
+(reference ``examples/pcf_mhd_jax.py:69-789``): the KMM flow of
+``PlaneCouetteFluctuationHIP`` coupled to a magnetic vector potential
+A = (Ax, Ay, Az) in the Dirichlet space TD^3 (conducting walls), with
+B = curl A, J = curl B, the Lorentz force J x B in the flow forcing and the
+induction EMF u_tot x B driving A, advanced on one SBDF3 timeline with an
+IMEXRK3 start-up.  The state is ``MHDState`` with the reference's fields in
+its full-complex layout; every step runs in libpcfdns.so.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _native as nat
+from .pcf import KMMState, PlaneCouetteFluctuationHIP, state_leaves
+
+_A_GROUPS = (("A", ""), ("nonlinear_A_old", "NL_A_OLD_"), ("nonlinear_A_older", "NL_A_OLDER_"),
+             ("A_old", "A_OLD_"), ("A_older", "A_OLDER_"))
+
+
+@dataclass(frozen=True)
+class MHDState:
+    """Coefficient-space MHD state (reference ``examples/pcf_mhd_jax.py:34-66``).
+
+    ``flow`` is the ``KMMState``; ``A`` and its histories are triples of
+    complex128 (n0-2, Ny, Nz) arrays in TD.
+    """
+
+    flow: KMMState
+    A: tuple
+    nonlinear_A_old: tuple | None = None
+    nonlinear_A_older: tuple | None = None
+    A_old: tuple | None = None
+    A_older: tuple | None = None
+
+
+class PlaneCouetteMHDHIP(PlaneCouetteFluctuationHIP):
+    """Conducting-wall plane-Couette MHD whose step runs in HIP on one GPU.
+
+    Constructor follows ``PlaneCouetteMHDJax.__init__``
+    (``examples/pcf_mhd_jax.py:76-114``): ``Rm`` defaults to ``Re`` and the
+    magnetic diffusivity is ``eta = U_wall / Rm``.  ``background_b`` is the
+    uniform field B0 of the reference's rotating-shear subclass
+    (``examples/pcf_mhd_mri_shearpy_jax.py:149-151``); the conducting-wall
+    class has none.  Built options are those of ``PlaneCouetteFluctuationHIP``.
+    """
+
+    def __init__(self, N=(17, 16, 16), domain=((-1.0, 1.0), (0.0, 4.0 * math.pi), (0.0, 2.0 * math.pi)),
+                 Re: float = 400.0, Rm: float | None = None, U_wall: float = 1.0, dt: float = 0.01,
+                 family: str = "C", padding_factor=(1.0, 1.5, 1.5), perturbation_amplitude: float = 0.1,
+                 magnetic_amplitude: float = 0.05, time_integrator: str | None = "SBDF3",
+                 nonlinear_form: str = "rotational", coefficient_path: str = "optimized",
+                 solve_batching: str = "batched", background_b=(0.0, 0.0, 0.0), precision: str = "fp64",
+                 device: int = 0) -> None:
+        self.Rm = float(Re if Rm is None else Rm)
+        if not self.Rm > 0:
+            raise ValueError("Rm must be positive")
+        self.eta = float(U_wall) / self.Rm
+        self.magnetic_amplitude = float(magnetic_amplitude)
+        self.background_b = tuple(float(v) for v in background_b)
+        super().__init__(N=N, domain=domain, Re=Re, U_wall=U_wall, dt=dt, family=family,
+                         padding_factor=padding_factor, perturbation_amplitude=perturbation_amplitude,
+                         time_integrator=time_integrator, nonlinear_form=nonlinear_form,
+                         coefficient_path=coefficient_path, solve_batching=solve_batching, precision=precision,
+                         device=device)
+
+    def _configure(self, cfg) -> None:
+        cfg.mhd = 1
+        cfg.eta = self.eta
+        for i in range(3):
+            cfg.b0[i] = self.background_b[i]
+
+    # ------------------------------------------------------------ states
+    def _zero_A(self):
+        return tuple(np.zeros((self.nd, self.N[1], self.N[2]), complex) for _ in range(3))
+
+    def _new_state(self, flow: KMMState, A) -> MHDState:
+        """``_new_mhd_state`` / ``_ensure_mhd_history`` (``pcf_mhd_jax.py:217-251``):
+        fresh flow and A histories on one timeline."""
+        flow = self._with_zero_history(flow.u, flow.g)
+        z = self._zero_A()
+        return MHDState(flow=flow, A=tuple(np.asarray(a, complex) for a in A), nonlinear_A_old=z,
+                        nonlinear_A_older=z, A_old=z, A_older=z)
+
+    def zero_state(self) -> MHDState:
+        return self._new_state(super().zero_state(), self._zero_A())
+
+    def initial_state(self) -> MHDState:
+        """``initial_state`` (``pcf_mhd_jax.py:253-267``): the flow seed plus
+        Ax = a (1-x^2) sin(ky y) sin(kz z) / kz with ky, kz the fundamental
+        wavenumbers (exact two-term Fourier expansions, projected onto TD)."""
+        flow = super().initial_state()
+        n0, n1, n2 = self.N
+        A = list(self._zero_A())
+        amp = self.magnetic_amplitude
+        if amp != 0.0:
+            kz = 2.0 * math.pi / self.Lz
+            xn = np.cos(np.pi + (2 * np.arange(n0) + 1) * np.pi / (2 * n0))
+            prof = amp * (1.0 - xn**2) / kz
+            cy = np.zeros(n1, complex)
+            cz = np.zeros(n2, complex)
+            cy[1], cy[-1] = 1 / 2j, -1 / 2j
+            cz[1], cz[-1] = 1 / 2j, -1 / 2j
+            A[0] = self.mask_nyquist(np.einsum("i,jk->ijk", self.matrix("FD") @ prof, np.outer(cy, cz)))
+        return self._new_state(flow, A)
+
+    # ------------------------------------------------------------ device I/O
+    def load(self, state: MHDState) -> None:
+        if any(getattr(state, name) is None for name, _ in _A_GROUPS[1:]) or state.flow.nonlinear_old is None:
+            state = self._new_state(state.flow, state.A)
+        super().load(state.flow)
+        for name, tag in _A_GROUPS:
+            for comp, arr in zip("XYZ", getattr(state, name)):
+                self._set(f"A{comp}" if not tag else f"{tag}{comp}", arr)
+        self._loaded = None
+
+    def fetch(self) -> MHDState:
+        flow = super().fetch()
+        groups = {}
+        for name, tag in _A_GROUPS:
+            groups[name] = tuple(self._get(f"A{c}" if not tag else f"{tag}{c}", self.nd) for c in "XYZ")
+        st = MHDState(flow=flow, **groups)
+        self._loaded = st
+        return st
+
+    # ------------------------------------------------------------ reference API
+    def diagnostics(self, state: MHDState | None = None) -> dict:
+        """``diagnostics`` (``pcf_mhd_jax.py:776-789``): the flow diagnostics plus
+        ``divB_L2`` and ``Emag``."""
+        out = super().diagnostics(state)
+        m = np.zeros(2)
+        nat.check(self._lib.pcf_mhd_diagnostics(self._h, m.ctypes.data_as(C.POINTER(C.c_double))))
+        out["divB_L2"] = float(m[1])
+        out["Emag"] = float(m[0])
+        return out
+
+    def vjp(self, *args, **kwargs):
+        raise NotImplementedError("the reverse mode covers the hydrodynamic step only")
+
+    def value_and_grad(self, *args, **kwargs):
+        raise NotImplementedError("the reverse mode covers the hydrodynamic step only")
+
+    @staticmethod
+    def benchmark_state_fields(state: MHDState):
+        """``benchmark_state_fields`` (``pcf_mhd_jax.py:754-758``)."""
+        return (*state.flow.u, state.flow.g, *state.A)
+
+
+def mhd_state_leaves(state: MHDState) -> list:
+    """Leaves in the reference pytree order (``pcf_mhd_jax.py:46-55``)."""
+    out = state_leaves(state.flow)
+    for name, _ in _A_GROUPS:
+        out.extend(getattr(state, name))
+    return out
+
+
+def mhd_state_checksum(state: MHDState) -> dict:
+    """Full-state checksum as ``production/benchmark.py:234-260`` computes it."""
+    real_sum = imag_sum = norm2 = 0.0
+    count = 0
+    leaves = mhd_state_leaves(state)
+    for v in leaves:
+        v = np.asarray(v)
+        real_sum += float(np.sum(np.real(v), dtype=np.float64))
+        imag_sum += float(np.sum(np.imag(v), dtype=np.float64))
+        norm2 += float(np.sum(np.abs(v) ** 2, dtype=np.float64))
+        count += int(v.size)
+    return {"leaf_count": len(leaves), "element_count": count, "real_sum": real_sum,
+            "imag_sum": imag_sum, "l2_norm": math.sqrt(norm2)}

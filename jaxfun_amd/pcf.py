@@ -133,12 +133,16 @@ class PlaneCouetteFluctuationHIP:
         cfg.time_integrator = nat.TI_SBDF3
         cfg.nonlinear_form = nat.NL_ROTATIONAL
         cfg.device = int(device)
+        self._configure(cfg)
         self._cfg = cfg
         self._lib = nat.load()
         handle = C.c_void_p()
         nat.check(self._lib.pcf_create(C.byref(cfg), C.byref(handle)))
         self._h = handle
         self._loaded = None  # the state object whose values are resident on the device
+
+    def _configure(self, cfg) -> None:
+        """Hook for subclasses to fill the rest of the library configuration."""
 
     # ------------------------------------------------------------ lifecycle
     def close(self) -> None:

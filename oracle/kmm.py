@@ -496,8 +496,8 @@ class PCFOracle:
                        history_steps=0.0, have_old=0.0, previous_dt=0.0)
 
     # ---------------------------------------------------------------- physics
-    def convection(self, s: OracleState):
-        """Rotational convection H = -(u_tot x w_tot) (``channelflow_kmm.py:754-782``)."""
+    def convection_physical(self, s: OracleState):
+        """Physical -(u_tot x w_tot) and u on the padded grid (``channelflow_kmm.py:754-762``)."""
         u0, u1, u2 = s.u
         SB, SD = self.SB, self.SD
         up = (self.tp_backward(u0, SB), self.tp_backward(u1, SD), self.tp_backward(u2, SD))
@@ -507,7 +507,12 @@ class PCFOracle:
         a = (up[0], up[1] + self.U_wall * self.xb, up[2])
         b = (w0, w1, w2 + self.U_wall)
         cross = (a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0])
-        return tuple(self.mask(self.tp_forward_D(-c)) for c in cross)
+        return tuple(-c for c in cross), up
+
+    def convection(self, s: OracleState):
+        """Rotational convection H = -(u_tot x w_tot) (``channelflow_kmm.py:772-782``)."""
+        n, _ = self.convection_physical(s)
+        return tuple(self.mask(self.tp_forward_D(c)) for c in n)
 
     def _dcoef(self, c, kx, k1, k2):
         """differentiate_orthogonal_coeffs (``tensorproductspace.py:638-682``)."""

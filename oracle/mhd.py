@@ -1,0 +1,254 @@
+"""CPU FP64 oracle of the conducting-wall plane-Couette MHD step.
+
+TEST INFRASTRUCTURE ONLY (see ``oracle/__init__.py``).  A numpy restatement,
+in the reference's full-complex layout, of ``PlaneCouetteMHDJax``
+(``examples/pcf_mhd_jax.py:69-789``) for SBDF3 with the IMEXRK3 start-up, the
+rotational flow nonlinearity and the optimized coefficient path:
+
+* the magnetic vector potential A = (Ax, Ay, Az), each component in the
+  Dirichlet Shen space TD (``:143-155``);
+* B = curl A in (TD, TC, TC) and J = curl B in (TC, TD, TD), formed on
+  Chebyshev coefficients with Galerkin projections onto TD
+  (``:269-311``; ``tensorproductspace.py:706-727``, ``composite.py:293-299``);
+* the flow forcing H = forward_TD(-(u_tot x w_tot) - J x B) and the induction
+  forcing HA = forward_TD(u_tot x B) on the 3/2-padded grid (``:339-357``);
+* per-mode Helmholtz solves S_A = M_A - gamma L_A with L_A the eta-weighted
+  Laplacian, eta = U_wall / Rm (``:116-139``);
+* the coupled SBDF3 / IMEXRK3 timeline of flow and A (``:488-715``);
+* diagnostics Emag and divB_L2 (``:760-789``).
+
+Pins (``tests/test_mhd_oracle.py``): the reference's published FP64
+diagnostics and 38-leaf state checksums after 325 steps at 65x64x64 and 50
+steps at 97x96x96 (``benchmark-results/mhd-sbdf3-rotational-*.json``);
+curl identities against physical derivatives of the backward transforms; the
+TD projection against quadrature forward(backward(c)); the hydro oracle
+recovered exactly at zero magnetic amplitude; and the A subsystem's linear
+decay equal to the hydro g subsystem's at Rm = Re.
+"""
+
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, replace
+
+import numpy as np
+
+from .kmm import IMEXRK3_A, IMEXRK3_B, SBDF3_GAMMA, OracleState, PCFOracle, _lu_solve, _nopivot_lu
+
+
+@dataclass
+class MHDOracleState:
+    """Mirror of ``MHDState`` (``examples/pcf_mhd_jax.py:34-66``)."""
+
+    flow: OracleState
+    A: tuple
+    nonlinear_A_old: tuple | None = None
+    nonlinear_A_older: tuple | None = None
+    A_old: tuple | None = None
+    A_older: tuple | None = None
+
+    def leaves(self) -> list:
+        out = self.flow.leaves()
+        for group in (self.A, self.nonlinear_A_old, self.nonlinear_A_older, self.A_old, self.A_older):
+            out.extend(group)
+        return out
+
+
+class MHDOracle(PCFOracle):
+    """``PlaneCouetteMHDJax`` (``examples/pcf_mhd_jax.py:69-114``) for SBDF3/rotational."""
+
+    def __init__(self, N=(17, 16, 16), domain=((-1.0, 1.0), (0.0, 4.0 * math.pi), (0.0, 2.0 * math.pi)),
+                 Re: float = 400.0, Rm: float | None = None, U_wall: float = 1.0, dt: float = 0.01,
+                 padding_factor=(1.0, 1.5, 1.5), perturbation_amplitude: float = 0.1,
+                 magnetic_amplitude: float = 0.05, background_b=(0.0, 0.0, 0.0)) -> None:
+        self.Rm = float(Re if Rm is None else Rm)
+        # uniform background field added to B in physical space
+        # (``_total_B_physical``, examples/pcf_mhd_mri_shearpy_jax.py:149-151)
+        self.background_b = tuple(float(v) for v in background_b)
+        self.eta = float(U_wall) / self.Rm
+        self.magnetic_amplitude = float(magnetic_amplitude)
+        super().__init__(N=N, domain=domain, Re=Re, U_wall=U_wall, dt=dt, padding_factor=padding_factor,
+                         perturbation_amplitude=perturbation_amplitude)
+        self._build_A_operators()
+
+    # ------------------------------------------------------------ A operators
+    def _build_A_operators(self):
+        """M_A = (h, a), L_A = (h, eta lap a) on TD^3 (``pcf_mhd_jax.py:116-139``)."""
+        k2 = self.k2flat
+        self.MA = self._mass_g(k2)
+        self.LA = self.W * self.eta * (self.D_C2[None] - k2[:, None, None] * self.D_C0[None])
+        self._A_factors = {}
+
+        def fac(gamma):
+            key = round(gamma, 17)
+            if key not in self._A_factors:
+                self._A_factors[key] = _nopivot_lu(self.MA - gamma * self.LA)
+            return self._A_factors[key]
+
+        self.SA = fac(self.dt * SBDF3_GAMMA)
+        self.SA_startup = tuple(fac((IMEXRK3_A[rk] + IMEXRK3_B[rk]) * self.dt / 2.0) for rk in range(3))
+
+    def MA_apply(self, c):
+        return self.Mg_apply(c)
+
+    def LA_apply(self, c):
+        k2 = self.K1**2 + self.K2**2
+        e = self.eta * self.W
+        return self._terms(c, [(self.D_C2, e), (self.D_C0, -e * k2)])
+
+    def A_solve(self, fac, rhs):
+        """``_A_solve`` (``pcf_mhd_jax.py:172-178``): per-mode solve, Nyquist mask."""
+        return tuple(self.mask(self._unmodes(_lu_solve(fac, self._modes(r)))) for r in rhs)
+
+    # ------------------------------------------------------------ projections
+    def project_D(self, c):
+        """TD ``project_orthogonal_coeffs`` along x (``composite.py:293-299``):
+        M_D^{-1} S_D diag(h) c."""
+        w = self._apply0(self.SD, c * self.h[:, None, None])
+        shp = w.shape
+        return np.linalg.solve(self.MD1, w.reshape(shp[0], -1)).reshape(shp)
+
+    def B_from_A(self, A):
+        """``update_B_from_A`` (``pcf_mhd_jax.py:269-289``): B in (TD, TC, TC)."""
+        AO = tuple(self.to_orth(a, self.SD) for a in A)
+        bx = self._dcoef(AO[2], 0, 1, 0) - self._dcoef(AO[1], 0, 0, 1)
+        by = self._dcoef(AO[0], 0, 0, 1) - self._dcoef(AO[2], 1, 0, 0)
+        bz = self._dcoef(AO[1], 1, 0, 0) - self._dcoef(AO[0], 0, 1, 0)
+        return (self.mask(self.project_D(bx)), self.mask(by), self.mask(bz))
+
+    def J_from_B(self, B):
+        """``update_J_from_B`` (``pcf_mhd_jax.py:291-311``): J in (TC, TD, TD)."""
+        BO = (self.to_orth(B[0], self.SD), B[1], B[2])
+        jx = self._dcoef(BO[2], 0, 1, 0) - self._dcoef(BO[1], 0, 0, 1)
+        jy = self._dcoef(BO[0], 0, 0, 1) - self._dcoef(BO[2], 1, 0, 0)
+        jz = self._dcoef(BO[1], 1, 0, 0) - self._dcoef(BO[0], 0, 1, 0)
+        return (self.mask(jx), self.mask(self.project_D(jy)), self.mask(self.project_D(jz)))
+
+    def backward_B(self, B, padded=True):
+        """``_backward_B`` (``pcf_mhd_jax.py:314-333``): spaces (TD, TC, TC)."""
+        return (self.tp_backward(B[0], self.SD, padded), self.tp_backward(B[1], None, padded),
+                self.tp_backward(B[2], None, padded))
+
+    def backward_J(self, J, padded=True):
+        """``_backward_J`` (``pcf_mhd_jax.py:335-337``): spaces (TC, TD, TD)."""
+        return (self.tp_backward(J[0], None, padded), self.tp_backward(J[1], self.SD, padded),
+                self.tp_backward(J[2], self.SD, padded))
+
+    # ------------------------------------------------------------ state
+    def zero_A(self):
+        return tuple(np.zeros((self.ND, self.N[1], self.N[2]), dtype=complex) for _ in range(3))
+
+    def reset_mhd_history(self, s: MHDOracleState) -> MHDOracleState:
+        """``_ensure_mhd_history`` restart (``pcf_mhd_jax.py:217-248``)."""
+        z = self.zero_A()
+        return MHDOracleState(flow=self.reset_history(s.flow), A=s.A, nonlinear_A_old=z, nonlinear_A_older=z,
+                              A_old=z, A_older=z)
+
+    def initial_state(self) -> MHDOracleState:
+        """``initial_state`` (``pcf_mhd_jax.py:253-267``)."""
+        flow = super().initial_state()
+        x, y, z = self.mesh()
+        shape = (self.N0, self.N[1], self.N[2])
+        ax = np.zeros(shape)
+        amp = self.magnetic_amplitude
+        if amp != 0.0:
+            ky, kz = 2.0 * np.pi / self.L1, 2.0 * np.pi / self.L2
+            ax = np.broadcast_to(amp * (1.0 - x**2) * (1.0 / kz) * np.sin(ky * y) * np.sin(kz * z), shape)
+        A = tuple(self.mask(self.tp_forward(v.astype(complex), self.SD, self.MD1)) for v in (ax, np.zeros(shape),
+                                                                                              np.zeros(shape)))
+        return self.reset_mhd_history(MHDOracleState(flow=flow, A=A))
+
+    # ------------------------------------------------------------ physics
+    def mhd_convection(self, s: MHDOracleState):
+        """``_mhd_convection`` (``pcf_mhd_jax.py:339-357``): (H, HA)."""
+        n, up = self.convection_physical(s.flow)
+        B = self.B_from_A(s.A)
+        J = self.J_from_B(B)
+        bp = tuple(b + b0 for b, b0 in zip(self.backward_B(B), self.background_b))
+        jp = self.backward_J(J)
+        lor = (jp[1] * bp[2] - jp[2] * bp[1], jp[2] * bp[0] - jp[0] * bp[2], jp[0] * bp[1] - jp[1] * bp[0])
+        H = tuple(self.mask(self.tp_forward_D(a - b)) for a, b in zip(n, lor))
+        ut = (up[0], up[1] + self.U_wall * self.xb, up[2])
+        emf = (ut[1] * bp[2] - ut[2] * bp[1], ut[2] * bp[0] - ut[0] * bp[2], ut[0] * bp[1] - ut[1] * bp[0])
+        HA = tuple(self.mask(self.tp_forward_D(e)) for e in emf)
+        return H, HA
+
+    def nonlinear_A(self, HA):
+        """``_A_mass_rhs`` of HA (``pcf_mhd_jax.py:157-159``)."""
+        return tuple(self.MA_apply(h) for h in HA)
+
+    def step_imexrk3_mhd(self, s: MHDOracleState) -> MHDOracleState:
+        """``_step_imexrk3_mhd`` (``pcf_mhd_jax.py:488-561``) with the start-up factors."""
+        dt = self.dt
+        u_stage, g_stage, A_stage = s.flow.u, s.flow.g, s.A
+        prev = (np.zeros_like(s.flow.u[0]), np.zeros_like(s.flow.g), np.zeros(self.ND), np.zeros(self.ND))
+        prevA = self.zero_A()
+        for rk in range(3):
+            H, HA = self.mhd_convection(MHDOracleState(flow=OracleState(u=u_stage, g=g_stage), A=A_stage))
+            cur = self.nonlinear_rhs(H)
+            curA = self.nonlinear_A(HA)
+            a, b = IMEXRK3_A[rk], IMEXRK3_B[rk]
+            gamma = (a + b) * dt / 2.0
+            v00 = np.real(u_stage[1][:, 0, 0])
+            w00 = np.real(u_stage[2][:, 0, 0])
+            rhs_u = self.Mu_apply(u_stage[0]) + gamma * self.Lu_apply(u_stage[0]) + dt * (a * cur[0] + b * prev[0])
+            rhs_g = self.Mg_apply(g_stage) + gamma * self.Lg_apply(g_stage) + dt * (a * cur[1] + b * prev[1])
+            rhs_v = self.M00 @ v00 + gamma * (self.L00 @ v00) + dt * (a * cur[2] + b * prev[2])
+            rhs_w = self.M00 @ w00 + gamma * (self.L00 @ w00) + dt * (a * cur[3] + b * prev[3])
+            rhs_A = [self.MA_apply(A_stage[i]) + gamma * self.LA_apply(A_stage[i])
+                     + dt * (a * curA[i] + b * prevA[i]) for i in range(3)]
+            u0n, gn, vn, wn = self._solve(self.startup[rk], rhs_u, rhs_g, rhs_v, rhs_w)
+            u_stage = self.reconstruct(u0n, gn, vn, wn)
+            g_stage = gn
+            A_stage = self.A_solve(self.SA_startup[rk], rhs_A)
+            prev, prevA = cur, curA
+        return replace(s, flow=replace(s.flow, u=u_stage, g=g_stage), A=A_stage)
+
+    def step(self, s: MHDOracleState) -> MHDOracleState:
+        """``_step_sbdf3_mhd`` / ``_step_sbdf3_steady`` (``pcf_mhd_jax.py:563-715``)."""
+        f = s.flow
+        cur_p = self.primary(f)
+        H, HA = self.mhd_convection(s)
+        cur_n = self.nonlinear_rhs(H)
+        cur_nA = self.nonlinear_A(HA)
+        if f.history_steps < 2:
+            adv = self.step_imexrk3_mhd(s)
+            hist = min(2.0, f.history_steps + 1.0)
+        else:
+            m0, m1, m2 = self.mass_rows(cur_p), self.mass_rows(f.solution_old), self.mass_rows(f.solution_older)
+            rhs = tuple(
+                (18.0 * a - 9.0 * b + 2.0 * c) / 11.0 + self.dt * ((18.0 * n0 - 18.0 * n1 + 6.0 * n2) / 11.0)
+                for a, b, c, n0, n1, n2 in zip(m0, m1, m2, cur_n, f.nonlinear_old, f.nonlinear_older))
+            rhs_A = [
+                (18.0 * self.MA_apply(s.A[i]) - 9.0 * self.MA_apply(s.A_old[i]) + 2.0 * self.MA_apply(s.A_older[i]))
+                / 11.0 + self.dt * ((18.0 * cur_nA[i] - 18.0 * s.nonlinear_A_old[i] + 6.0 * s.nonlinear_A_older[i])
+                                    / 11.0)
+                for i in range(3)]
+            u0n, gn, vn, wn = self._solve(self.steady, *rhs)
+            adv = replace(s, flow=replace(f, u=self.reconstruct(u0n, gn, vn, wn), g=gn),
+                          A=self.A_solve(self.SA, rhs_A))
+            hist = 2.0
+        flow = replace(adv.flow, nonlinear_old=cur_n, nonlinear_older=f.nonlinear_old, solution_old=cur_p,
+                       solution_older=f.solution_old, history_steps=hist, have_old=1.0, previous_dt=self.dt)
+        return MHDOracleState(flow=flow, A=adv.A, nonlinear_A_old=cur_nA, nonlinear_A_older=s.nonlinear_A_old,
+                              A_old=s.A, A_older=s.A_old)
+
+    def solve(self, s: MHDOracleState, steps: int) -> MHDOracleState:
+        for _ in range(int(steps)):
+            s = self.step(s)
+        return s
+
+    # ------------------------------------------------------------ diagnostics
+    def diagnostics(self, s: MHDOracleState) -> dict:
+        """``diagnostics`` + ``magnetic_divergence_l2`` (``pcf_mhd_jax.py:760-789``)."""
+        out = super().diagnostics(s.flow)
+        B = self.B_from_A(s.A)
+        bp = self.backward_B(B, padded=False)
+        out["Emag"] = float(sum(np.real(self.integrate(np.conj(b) * b)) for b in bp))
+        divb = (self.tp_backward(B[0], self.SD, padded=False, kx=1) + self.tp_backward(B[1], None, padded=False, k1=1)
+                + self.tp_backward(B[2], None, padded=False, k2=1))
+        out["divB_L2"] = float(np.sqrt(np.real(self.integrate(np.conj(divb) * divb))))
+        return out
+
+
+__all__ = ["MHDOracle", "MHDOracleState"]

@@ -56,3 +56,47 @@ def oracle_for(solver) -> PCFOracle:
     return PCFOracle(N=solver.N, domain=solver.domain, Re=solver.Re, U_wall=solver.U_wall, dt=solver.dt,
                      padding_factor=solver.padding_factor,
                      perturbation_amplitude=solver.perturbation_amplitude)
+
+
+# ------------------------------------------------------------------ MHD
+MHD_A_GROUPS = ("A", "nonlinear_A_old", "nonlinear_A_older", "A_old", "A_older")
+MHD_LEAF_NAMES = LEAF_NAMES + [f"{g}.{c}" for g in MHD_A_GROUPS for c in "xyz"]
+
+
+def to_mhd_oracle(s):
+    from oracle.mhd import MHDOracleState
+
+    return MHDOracleState(flow=to_oracle(s.flow), **{g: tuple(np.array(a) for a in getattr(s, g))
+                                                      for g in MHD_A_GROUPS})
+
+
+def mhd_leaves(s) -> list:
+    out = leaves(s.flow)
+    for g in MHD_A_GROUPS:
+        out.extend(getattr(s, g))
+    return out
+
+
+def mhd_max_rel_errors(a, b) -> dict:
+    """max_rel_errors over the flow leaves plus each A group (scale floored at
+    1e-3 of the group's largest entry)."""
+    out = max_rel_errors(a.flow, b.flow)
+    la, lb = mhd_leaves(a), mhd_leaves(b)
+    for k in range(len(MHD_A_GROUPS)):
+        grp = range(20 + 3 * k, 23 + 3 * k)
+        gmax = max(float(np.max(np.abs(np.asarray(lb[i])))) for i in grp)
+        for i in grp:
+            x, y = np.asarray(la[i]), np.asarray(lb[i])
+            scale = max(float(np.max(np.abs(y))), 1e-3 * gmax)
+            err = float(np.max(np.abs(x - y)))
+            out[MHD_LEAF_NAMES[i]] = err / scale if scale > 0 else err
+    return out
+
+
+def mhd_oracle_for(solver):
+    from oracle.mhd import MHDOracle
+
+    return MHDOracle(N=solver.N, domain=solver.domain, Re=solver.Re, Rm=solver.Rm, U_wall=solver.U_wall,
+                     dt=solver.dt, padding_factor=solver.padding_factor,
+                     perturbation_amplitude=solver.perturbation_amplitude,
+                     magnetic_amplitude=solver.magnetic_amplitude, background_b=solver.background_b)

@@ -1,0 +1,112 @@
+"""GPU parity of the conducting-wall MHD step: HIP against the MHD oracle and
+the reference's published MHD goldens.
+
+Bars as for the hydrodynamic step: state leaves (flow and vector potential)
+within 1e-12 relative per leaf group of the FP64 oracle, nonlinear history rows
+within 5e-11 (their rounding sensitivity, see test_gpu_parity.py), published
+energies within 1e-10 relative and checksums within 1e-11 (l2) / 1e-9 (sum).
+"""
+
+import json
+import math
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from jaxfun_amd import PlaneCouetteMHDHIP, mhd_state_checksum
+
+from tests.helpers import mhd_max_rel_errors, mhd_oracle_for, to_mhd_oracle
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = json.loads((Path(__file__).parent / "golden" / "reference_mhd_published.json").read_text())
+FP64_TOL = 1e-12
+HIST_TOL = 5e-11
+
+
+def make(N, precision="fp64", **kw):
+    c = GOLDEN["config"]
+    args = dict(Re=c["Re"], Rm=c["Rm"], dt=c["dt"], perturbation_amplitude=c["velocity_amplitude"],
+                magnetic_amplitude=c["magnetic_amplitude"])
+    args.update(kw)
+    return PlaneCouetteMHDHIP(N=N, precision=precision, **args)
+
+
+def assert_parity(err, where):
+    hist = {k: v for k, v in err.items() if k.startswith(("nl_", "nonlinear_"))}
+    rest = {k: v for k, v in err.items() if k not in hist}
+    assert max(rest.values()) < FP64_TOL, (where, rest)
+    assert max(hist.values()) < HIST_TOL, (where, hist)
+
+
+@pytest.mark.parametrize("N,kw", [((17, 16, 16), {}), ((24, 16, 32), {}),
+                                  ((17, 16, 16), dict(magnetic_amplitude=0.2, Rm=50.0)),
+                                  ((17, 16, 16), dict(magnetic_amplitude=0.1, background_b=(0.0, 0.0, 0.1)))],
+                         ids=["17", "24x16x32", "strong-field", "background-B0"])
+def test_fp64_mhd_steps_match_oracle(N, kw):
+    solver = make(N, **kw)
+    oracle = mhd_oracle_for(solver)
+    s = solver.initial_state()
+    so = to_mhd_oracle(s)
+    done = 0
+    for k in (1, 2, 3, 7):  # two IMEXRK3 start-up steps, then steady SBDF3
+        s = solver.solve(s, k - done)
+        so = oracle.solve(so, k - done)
+        done = k
+        err = mhd_max_rel_errors(s, so)
+        print(N, kw, k, max(err.values()), flush=True)
+        assert_parity(err, (N, k))
+    d, do = solver.diagnostics(s), oracle.diagnostics(so)
+    for key in ("Epert", "Etot", "Emag"):
+        assert math.isclose(d[key], do[key], rel_tol=1e-12), (key, d[key], do[key])
+    assert d["divB_L2"] < 1e-13 and do["divB_L2"] < 1e-13
+
+
+def test_mhd_graph_replay_matches_eager():
+    solver = make((17, 16, 16), magnetic_amplitude=0.1)
+    s = solver.solve(solver.initial_state(), 3)
+    solver.load(s)
+    solver.advance_steady(24)
+    eager = solver.fetch()
+    solver.load(s)
+    solver.advance_steady(24, graph_block=12)
+    graph = solver.fetch()
+    assert max(mhd_max_rel_errors(graph, eager).values()) == 0.0
+
+
+def test_mhd_mixed_precision_close_to_fp64():
+    N = (33, 32, 32)
+    a, b = make(N, magnetic_amplitude=0.05), make(N, precision="mixed", magnetic_amplitude=0.05)
+    s0 = a.initial_state()
+    sa, sb = a.solve(s0, 40), b.solve(s0, 40)
+    for x, y in zip(list(sb.flow.u) + list(sb.A), list(sa.flow.u) + list(sa.A)):
+        rel = float(np.max(np.abs(x - y)) / np.max(np.abs(y)))
+        print("mixed vs fp64", rel, flush=True)
+        assert rel < 2e-6
+
+
+@pytest.mark.parametrize("case", GOLDEN["cases"], ids=lambda c: "x".join(map(str, c["N"])))
+def test_published_mhd_golden(case):
+    """Reference-published FP64 MHD results after the benchmark's step count."""
+    solver = make(tuple(case["N"]))
+    s = solver.solve(solver.initial_state(), case["steps"])
+    d = solver.diagnostics(s)
+    g = case["diagnostics"]
+    print(case["N"], {k: (d[k], g[k]) for k in ("Epert", "Emag")}, flush=True)
+    for k in ("Epert", "Etot", "Emag"):
+        assert math.isclose(d[k], g[k], rel_tol=1e-10), (k, d[k], g[k])
+    for k in ("wall_shear_top", "wall_shear_bot", "u_top", "u_bot", "mean_shear"):
+        assert abs(d[k] - g[k]) < 1e-13, (k, d[k], g[k])
+    assert d["divL2"] < 1e-14 and d["divB_L2"] < 1e-15
+    cs = mhd_state_checksum(s)
+    gc = case["checksum"]
+    assert cs["leaf_count"] == gc["leaf_count"] and cs["element_count"] == gc["element_count"]
+    assert math.isclose(cs["l2_norm"], gc["l2_norm"], rel_tol=1e-11)
+    assert math.isclose(cs["real_sum"], gc["real_sum"], rel_tol=1e-9)
+
+
+def test_mhd_rejects_reverse_mode():
+    solver = make((17, 16, 16))
+    with pytest.raises(NotImplementedError):
+        solver.value_and_grad(solver.initial_state(), 2)

@@ -1,0 +1,149 @@
+"""Pins of the MHD oracle (oracle/mhd.py) against the reference's published
+FP64 results and against identities the mathematics fixes.
+
+* Published goldens (slow): diagnostics and 38-leaf state checksums of
+  ``PlaneCouetteMHDJax`` after 325 SBDF3 steps at 65x64x64 and 50 at 97x96x96
+  (``benchmark-results/mhd-sbdf3-rotational-*.json``; fixture
+  ``tests/golden/reference_mhd_published.json``).
+* B = curl A against derivatives of the backward transforms of A at the nodes.
+* The TD projection inside J = curl B against forward(backward(.)) by
+  Gauss quadrature (the identity ``project_from_orthogonal`` states,
+  ``tensorproductspace.py:706-713``).
+* A = 0 reproduces the hydrodynamic oracle exactly.
+* A decaying Ax mode whose EMF vanishes identically (ky = 0) follows the
+  scalar IMEXRK3/SBDF3 recurrence of its discrete eigenvalue of
+  eta M_A^{-1} L_A, which pins the A operators, the stage and history
+  coefficients and the level order of the A history.
+"""
+
+from __future__ import annotations
+
+import json
+import math
+from pathlib import Path
+
+import numpy as np
+import pytest
+import scipy.linalg as sla
+
+from oracle.kmm import IMEXRK3_A, IMEXRK3_B, PCFOracle
+from oracle.mhd import MHDOracle, MHDOracleState
+
+GOLDEN = json.loads((Path(__file__).parent / "golden" / "reference_mhd_published.json").read_text())
+
+
+def _rand_field(o, rows, seed, K=4):
+    """Hermitian-symmetric, Nyquist-masked random coefficients (real physical field)."""
+    rng = np.random.default_rng(seed)
+    shape = (o.N0, o.N[1], o.N[2])
+    y = np.linspace(0, o.L1, o.N[1], endpoint=False)[None, :, None]
+    z = np.linspace(0, o.L2, o.N[2], endpoint=False)[None, None, :]
+    x = o.xb
+    f = np.zeros(shape)
+    for _ in range(6):
+        a, b = rng.integers(0, K, 2)
+        ph1, ph2 = rng.uniform(0, 2 * np.pi, 2)
+        f = f + rng.normal() * (1 - x**2) * np.cos(2 * np.pi * a * y / o.L1 + ph1) * np.cos(
+            2 * np.pi * b * z / o.L2 + ph2) * (1 + 0.3 * x)
+    c = o.mask(o.tp_forward(f.astype(complex), o.SD, o.MD1))
+    assert c.shape[0] == rows
+    return c
+
+
+def test_curl_matches_nodal_derivatives():
+    o = MHDOracle(N=(17, 16, 16), Re=400.0, dt=1e-3)
+    A = tuple(_rand_field(o, o.ND, s) for s in (1, 2, 3))
+    B = o.B_from_A(A)
+    bp = o.backward_B(B, padded=False)
+    d = lambda c, **k: o.tp_backward(c, o.SD, padded=False, **k)  # noqa: E731
+    ref = (d(A[2], k1=1) - d(A[1], k2=1), d(A[0], k2=1) - d(A[2], kx=1), d(A[1], kx=1) - d(A[0], k1=1))
+    for got, want in zip(bp, ref):
+        assert np.max(np.abs(got - want)) <= 1e-12 * np.max(np.abs(want))
+    # div B = 0 to rounding
+    assert o.diagnostics(MHDOracleState(flow=o.initial_state().flow, A=A))["divB_L2"] < 1e-12
+
+
+def test_current_projection_is_quadrature_forward():
+    """J_y, J_z in TD equal TD.forward of the nodal curl of B (exact quadrature:
+    the products have degree <= 2 n0 - 3)."""
+    o = MHDOracle(N=(17, 16, 16), Re=400.0, dt=1e-3)
+    A = tuple(_rand_field(o, o.ND, s) for s in (4, 5, 6))
+    B = o.B_from_A(A)
+    J = o.J_from_B(B)
+    bx = o.tp_backward(B[0], o.SD, padded=False)
+    bT = lambda c, **k: o.tp_backward(c, None, padded=False, **k)  # noqa: E731
+    dbx = lambda **k: o.tp_backward(B[0], o.SD, padded=False, **k)  # noqa: E731
+    jy = dbx(k2=1) - bT(B[2], kx=1)
+    jz = bT(B[1], kx=1) - dbx(k1=1)
+    jx = bT(B[2], k1=1) - bT(B[1], k2=1)
+    del bx
+    for got, nodal in ((J[1], jy), (J[2], jz)):
+        want = o.mask(o.tp_forward(nodal, o.SD, o.MD1))
+        assert np.max(np.abs(got - want)) <= 1e-12 * np.max(np.abs(want))
+    jxn = o.tp_backward(J[0], None, padded=False)
+    assert np.max(np.abs(jxn - jx)) <= 1e-12 * np.max(np.abs(jx))
+
+
+def test_zero_field_reproduces_hydro_oracle():
+    kw = dict(N=(17, 16, 16), Re=400.0, dt=2e-3, perturbation_amplitude=0.05)
+    m = MHDOracle(magnetic_amplitude=0.0, **kw)
+    h = PCFOracle(**kw)
+    sm, sh = m.initial_state(), h.initial_state()
+    sm, sh = m.solve(sm, 5), h.solve(sh, 5)
+    for a, b in zip(sm.flow.leaves(), sh.leaves()):
+        assert np.array_equal(np.asarray(a), np.asarray(b))
+    assert all(np.max(np.abs(a)) == 0 for a in sm.A)
+
+
+def test_A_decay_follows_scalar_recurrence():
+    """Ax = eps f(x) cos(z), f the least-damped eigenvector of eta M^{-1}(C2 - kz^2 C0):
+    b = (0, dAx/dz, 0) and u_tot x b = 0 exactly, so the A update is linear and
+    every step multiplies the mode by the integrator's scalar amplification."""
+    dt, Rm = 0.01, 50.0
+    o = MHDOracle(N=(17, 8, 8), Re=400.0, Rm=Rm, dt=dt, perturbation_amplitude=0.0, magnetic_amplitude=0.0)
+    kz = o.K2[0, 0, 1]
+    lam, vec = sla.eig(o.eta * (o.D_C2 - kz**2 * o.D_C0), o.D_C0)
+    i = int(np.argmax(lam.real))
+    lam, f = float(lam[i].real), np.real(vec[:, i])
+    eps = 1e-7
+    Ax = np.zeros((o.ND, o.N[1], o.N[2]), complex)
+    Ax[:, 0, 1] = Ax[:, 0, -1] = 0.5 * eps * f
+    s0 = o.initial_state()  # zero flow perturbation and zero A at these amplitudes
+    s = o.reset_mhd_history(MHDOracleState(flow=s0.flow, A=(Ax, 0 * Ax, 0 * Ax)))
+    # scalar reference: IMEXRK3 stages (1 - g l) y' = (1 + g l) y, then SBDF3
+    ys = [1.0]
+    y = 1.0
+    for _ in range(2):
+        for rk in range(3):
+            g = (IMEXRK3_A[rk] + IMEXRK3_B[rk]) * dt / 2.0
+            y = (1 + g * lam) * y / (1 - g * lam)
+        ys.append(y)
+    for _ in range(6):
+        y = (18 * ys[-1] - 9 * ys[-2] + 2 * ys[-3]) / 11 / (1 - 6 * dt / 11 * lam)
+        ys.append(y)
+    for k in range(1, len(ys)):
+        s = o.step(s)
+        want = ys[k] * 0.5 * eps * f
+        err = np.max(np.abs(s.A[0][:, 0, 1] - want)) / np.max(np.abs(0.5 * eps * f))
+        assert err < 1e-11, (k, err, ys[k])
+        assert np.max(np.abs(s.A[1])) < 1e-15 * eps and np.max(np.abs(s.A[2])) < 1e-15 * eps
+
+
+@pytest.mark.slow
+@pytest.mark.parametrize("case", GOLDEN["cases"], ids=lambda c: "x".join(map(str, c["N"])))
+def test_published_mhd_golden(case):
+    cfg = GOLDEN["config"]
+    o = MHDOracle(N=tuple(case["N"]), Re=cfg["Re"], Rm=cfg["Rm"], dt=cfg["dt"],
+                  perturbation_amplitude=cfg["velocity_amplitude"], magnetic_amplitude=cfg["magnetic_amplitude"])
+    s = o.solve(o.initial_state(), case["steps"])
+    d = o.diagnostics(s)
+    for k in ("Epert", "Etot", "Emag"):
+        assert math.isclose(d[k], case["diagnostics"][k], rel_tol=1e-10), (k, d[k], case["diagnostics"][k])
+    for k in ("wall_shear_top", "wall_shear_bot", "mean_shear"):
+        assert abs(d[k] - case["diagnostics"][k]) < 1e-12, k
+    assert d["divB_L2"] < 1e-15 and d["divL2"] < 1e-14
+    c = o.checksum(s)
+    ref = case["checksum"]
+    assert c["leaf_count"] == ref["leaf_count"] and c["element_count"] == ref["element_count"]
+    assert math.isclose(c["l2_norm"], ref["l2_norm"], rel_tol=1e-11)
+    assert math.isclose(c["real_sum"], ref["real_sum"], rel_tol=1e-9)

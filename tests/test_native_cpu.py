@@ -125,6 +125,24 @@ def test_projection_and_nonlinear_rows_match_oracle():
     assert np.allclose(np.tensordot(R, u0, axes=(1, 0)), f_ref, atol=1e-12)
 
 
+def test_mhd_current_matrices_match_oracle_projection():
+    """VQ1 / VQ2 = VD P_D D^k S_D^T against the oracle's Galerkin projection
+    (composite.py:293-299) of the x-derivatives of Dirichlet fields."""
+    from oracle.mhd import MHDOracle
+
+    N = (17, 8, 8)
+    cfg = cfg_for(N[0], N[1], N[2])
+    o = MHDOracle(N=N, Re=400.0, dt=1e-3)
+    rng = np.random.default_rng(5)
+    a = rng.standard_normal((N[0] - 2, 1, 1)) + 0j
+    VD = nat.setup_matrix(cfg, "VD")
+    for k, name in ((1, "VQ1"), (2, "VQ2")):
+        Q = nat.setup_matrix(cfg, name)
+        ref = np.tensordot(VD, o.project_D(cheb_derivative(o.to_orth(a, o.SD), k)), axes=(1, 0))
+        got = np.tensordot(Q, a, axes=(1, 0))
+        assert np.max(np.abs(got - ref)) <= 1e-12 * np.max(np.abs(ref)), name
+
+
 def test_structured_setup_accepts_all_benchmark_sizes():
     for n0 in (9, 17, 33, 65, 97, 128, 129):
         M = nat.setup_matrix(cfg_for(n0), "C4B")
@@ -144,6 +162,12 @@ def test_config_errors():
     assert lib.pcf_field_size(C.byref(c), nat.FIELD_ID["SOL_OLD_V"]) == 15
     with pytest.raises(ValueError):
         nat.check(-1)
+    # MHD: eta must be positive, mhd a flag; A leaves are Dirichlet fields
+    for c in (cfg_for(17, mhd=1, eta=0.0), cfg_for(17, mhd=2, eta=1e-3)):
+        assert lib.pcf_field_size(C.byref(c), 0) < 0
+    c = cfg_for(17, mhd=1, eta=1 / 400)
+    assert lib.pcf_field_size(C.byref(c), nat.FIELD_ID["AX"]) == 15 * 16 * 16
+    assert lib.pcf_field_size(C.byref(c), nat.FIELD_ID["A_OLDER_Z"]) == 15 * 16 * 16
 
 
 def test_python_boundary_errors_match_reference():
