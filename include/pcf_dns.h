@@ -65,17 +65,26 @@ typedef struct pcf_config {
   int32_t time_integrator; /* PCF_TI_SBDF3                                   */
   int32_t nonlinear_form;  /* PCF_NL_ROTATIONAL                              */
   int32_t device;          /* HIP device ordinal                             */
-  /* Magnetohydrodynamics (examples/pcf_mhd_jax.py:69-114): mhd = 1 evolves the
-   * magnetic vector potential A in TD^3 (conducting walls) with magnetic
-   * diffusivity eta = U_wall / Rm, coupled to the flow by the Lorentz force
-   * J x B and the induction EMF u_tot x B; b0 is a uniform background field
-   * added to B in physical space (examples/pcf_mhd_mri_shearpy_jax.py:149-151;
-   * zeros for PlaneCouetteMHDJax).  mhd = 0 ignores eta and b0.            */
+  /* Magnetohydrodynamics: mhd = 1 evolves a magnetic vector potential A with
+   * magnetic diffusivity eta, coupled to the flow by the Lorentz force J x B
+   * and the induction EMF u_tot x B (examples/pcf_mhd_jax.py:69-114).
+   *   walls = PCF_WALLS_CONDUCTING: A in TD^3 (PlaneCouetteMHDJax);
+   *   walls = PCF_WALLS_INSULATING: A_x in TD, A_y, A_z in TC with per-mode
+   *     vacuum-matching wall rows (PlaneCouetteMRIShearpyInsulatingJax,
+   *     examples/pcf_mhd_mri_shearpy_jax.py:516-712).
+   * b0 is a uniform background field added to B in physical space and omega
+   * the rotation rate of the Coriolis force (-2 omega u1, 2 omega u0, 0)
+   * (examples/pcf_mhd_mri_shearpy_jax.py:149-176); a shear rate S of that
+   * class is u_wall = -S.  mhd = 0 ignores walls, eta, b0 and omega.       */
   int32_t mhd;
-  int32_t reserved;
+  int32_t walls;
   double eta;
   double b0[3];
+  double omega;
 } pcf_config;
+
+#define PCF_WALLS_CONDUCTING 0
+#define PCF_WALLS_INSULATING 1
 
 typedef struct pcf_handle pcf_handle;
 
@@ -91,8 +100,9 @@ enum pcf_field {
   PCF_F_SOL_OLD_U, PCF_F_SOL_OLD_G, PCF_F_SOL_OLD_V, PCF_F_SOL_OLD_W,
   PCF_F_SOL_OLDER_U, PCF_F_SOL_OLDER_G, PCF_F_SOL_OLDER_V, PCF_F_SOL_OLDER_W,
   /* MHD handles only (MHDState, examples/pcf_mhd_jax.py:34-66): the vector
-   * potential A = (Ax, Ay, Az) in TD, its nonlinear rows MA HA of the two
-   * previous steps and A of the two previous steps. */
+   * potential A = (Ax, Ay, Az), its nonlinear rows M_A HA of the two previous
+   * steps and A of the two previous steps.  Ax rows: n0-2 (TD); Ay, Az rows:
+   * n0-2 (TD, conducting walls) or n0 (TC, insulating walls). */
   PCF_F_AX, PCF_F_AY, PCF_F_AZ,
   PCF_F_NL_A_OLD_X, PCF_F_NL_A_OLD_Y, PCF_F_NL_A_OLD_Z,
   PCF_F_NL_A_OLDER_X, PCF_F_NL_A_OLDER_Y, PCF_F_NL_A_OLDER_Z,
@@ -127,6 +137,9 @@ int64_t pcf_field_size(const pcf_config *cfg, int32_t field);
  *   15 VQ1, 16 VQ2: nodal values VD P_D D^k S_D^T (k = 1, 2) of the Galerkin
  *      projection P_D of x-derivatives of Dirichlet fields (n0 x n0-2), which
  *      J = curl B needs — composite.py:293-299, pcf_mhd_jax.py:291-311
+ *   17 VC, 18 VC1, 19 VC2: Chebyshev values T_j and derivatives T_j', T_j''
+ *      at the nodes (n0 x n0); 20 WQ: Chebyshev scalar products of nodal
+ *      values (T_k, f) (n0 x n0) — the TC transforms of insulating walls
  * Replaces the SymPy inner() assembly of examples/channelflow_kmm.py:261-409. */
 int32_t pcf_setup_matrix(const pcf_config *cfg, int32_t which, double *out,
                          int32_t *rows, int32_t *cols);
@@ -170,11 +183,14 @@ int32_t pcf_step_steady_graph(pcf_handle *h, int32_t nsteps, int32_t replays);
  * (examples/pcf_fluctuations_jax.py:131-161). */
 int32_t pcf_diagnostics(pcf_handle *h, double *out8);
 
-/* Magnetic diagnostics of an MHD handle (synchronous).  out[2] =
- * {Emag, divB_L2}: the magnetic energy sum_i int |b_i|^2 and the L2 norm of
- * div B on the unpadded grid; replaces the additions of
- * PlaneCouetteMHDJax.diagnostics (examples/pcf_mhd_jax.py:760-789). */
-int32_t pcf_mhd_diagnostics(pcf_handle *h, double *out2);
+/* Magnetic diagnostics of an MHD handle (synchronous).  out[4] =
+ * {Emag, divB_L2, Emag_total, insulating_bc_residual}: the energy
+ * sum_i int |b_i|^2 of the fluctuation field b = curl A, the L2 norm of div b
+ * on the unpadded grid, the energy of b + b0, and (insulating walls; 0
+ * otherwise) the largest wall residual of the vacuum-matching conditions.
+ * Replaces PlaneCouetteMHDJax.diagnostics (examples/pcf_mhd_jax.py:760-789)
+ * and the corresponding entries of examples/pcf_mhd_mri_shearpy_jax.py:431-513,802-836. */
+int32_t pcf_mhd_diagnostics(pcf_handle *h, double *out4);
 
 /* Per-stage timing.  One steady step launches pcf_kernels_per_step() kernels,
  * one per stage, in the order pcf_stage_name(0..pcf_stage_count()-1):

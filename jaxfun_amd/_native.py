@@ -15,6 +15,8 @@ import numpy as np
 LIB_PATH = Path(__file__).resolve().parent / "libpcfdns.so"
 
 ABI_VERSION = 2
+WALLS_CONDUCTING = 0
+WALLS_INSULATING = 1
 PREC_FP64 = 0
 PREC_MIXED = 1
 TI_SBDF3 = 3
@@ -39,6 +41,7 @@ FIELD_ID = {name: i for i, name in enumerate(FIELDS)}
 MATRIX_ID = {
     "C0B": 0, "C2B": 1, "C4B": 2, "C0D": 3, "C2D": 4, "VB": 5, "VD": 6, "VD1": 7,
     "A1": 8, "A0": 9, "AG": 10, "R": 11, "VB1": 12, "FB": 13, "FD": 14, "VQ1": 15, "VQ2": 16,
+    "VC": 17, "VC1": 18, "VC2": 19, "WQ": 20,
 }
 
 # Every symbol include/pcf_dns.h declares.
@@ -73,9 +76,10 @@ class PcfConfig(C.Structure):
         ("nonlinear_form", C.c_int32),
         ("device", C.c_int32),
         ("mhd", C.c_int32),
-        ("reserved", C.c_int32),
+        ("walls", C.c_int32),
         ("eta", C.c_double),
         ("b0", C.c_double * 3),
+        ("omega", C.c_double),
     ]
 
 

@@ -54,6 +54,8 @@ std::string validate(const pcf_config *c) {
   if (c->nonlinear_form != PCF_NL_ROTATIONAL) return "only the rotational nonlinear form is built";
   if (c->mhd != 0 && c->mhd != 1) return "mhd must be 0 or 1";
   if (c->mhd && !(c->eta > 0)) return "eta must be positive";
+  if (c->mhd && c->walls != PCF_WALLS_CONDUCTING && c->walls != PCF_WALLS_INSULATING) return "unknown wall type";
+  if (c->mhd && !std::isfinite(c->omega)) return "omega must be finite";
   return "";
 }
 
@@ -81,8 +83,10 @@ struct StageTimer {
 template <typename T>
 struct DevMats {
   T *VBt = nullptr, *VDt = nullptr, *VD1t = nullptr, *A1t = nullptr, *A0t = nullptr,
-    *AGt = nullptr, *xnode = nullptr, *twy = nullptr, *twz = nullptr, *VQ1t = nullptr, *VQ2t = nullptr;
+    *AGt = nullptr, *xnode = nullptr, *twy = nullptr, *twz = nullptr;
+  T *VQ1t = nullptr, *VQ2t = nullptr, *VCt = nullptr, *VC1t = nullptr, *VC2t = nullptr, *WQt = nullptr;
   pcf::NodalMats<T> view{};
+  pcf::MhdMats<T> mview{};  // MHD contractions (conducting or insulating walls)
 };
 
 }  // namespace
@@ -107,8 +111,11 @@ struct pcf_handle {
   // nonlinear ring; they rotate with the flow's rings (one coupled timeline,
   // examples/pcf_mhd_jax.py:217-248).
   bool mhd = false;
+  bool insul = false;         // insulating walls: Ay, Az in TC (n0 rows)
+  int arows = 0;              // rows of Ay, Az (nd or n0); Ax uses the first nd
   double2 *A[3]{}, *NA[4]{};
-  size_t cstride = 0;
+  size_t cstride = 0;         // arows * m
+  double *dVC = nullptr, *dVC1 = nullptr, *dresid = nullptr;
   pcf::Geo geo_eta{};         // geo with nu = eta: the A systems are the g systems at eta
   pcf::FactorSet fA{};        // steady LU factors of the A systems (Dirichlet blocks)
   double *ddiagm = nullptr;
@@ -163,6 +170,10 @@ struct pcf_handle {
     allocs.push_back(q);
     *p = static_cast<T *>(q);
     e = hipMemset(q, 0, std::max<size_t>(count, 1) * sizeof(T));
+    // hipMemset runs on the null stream and may still be in flight when it
+    // returns; the handle's stream does not wait for the null stream, so wait
+    // here before any kernel on it writes this memory (the setup factors).
+    if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
     if (e != hipSuccess) return fail(PCF_E_HIP, std::string("hipMemset: ") + hipGetErrorString(e));
     return PCF_OK;
   }
@@ -196,9 +207,16 @@ int32_t build_nodal(pcf_handle *h, DevMats<T> &dm, int xc, int ic) {
   if ((r = h->upload(&dm.A1t, transpose_pad(o.A1, ldi)))) return r;
   if ((r = h->upload(&dm.A0t, transpose_pad(o.A0, ldi)))) return r;
   if ((r = h->upload(&dm.AGt, transpose_pad(o.AG, ldi)))) return r;
-  if (h->mhd) {
+  if (h->mhd && !h->insul) {
     if ((r = h->upload(&dm.VQ1t, transpose_pad(o.VQ1, ldx)))) return r;
     if ((r = h->upload(&dm.VQ2t, transpose_pad(o.VQ2, ldx)))) return r;
+  }
+  const int ldc = (n0 + ic - 1) / ic * ic;
+  if (h->mhd && h->insul) {
+    if ((r = h->upload(&dm.VCt, transpose_pad(o.VC, ldx)))) return r;
+    if ((r = h->upload(&dm.VC1t, transpose_pad(o.VC1, ldx)))) return r;
+    if ((r = h->upload(&dm.VC2t, transpose_pad(o.VC2, ldx)))) return r;
+    if ((r = h->upload(&dm.WQt, transpose_pad(o.WQ, ldc)))) return r;
   }
   std::vector<T> xn(n0);
   for (int j = 0; j < n0; ++j) xn[j] = (T)o.x[j];
@@ -215,7 +233,17 @@ int32_t build_nodal(pcf_handle *h, DevMats<T> &dm, int xc, int ic) {
   if ((r = h->upload(&dm.twy, twiddles(h->geo.my)))) return r;
   if ((r = h->upload(&dm.twz, twiddles(h->geo.mz)))) return r;
   dm.view = pcf::NodalMats<T>{dm.VBt, dm.VDt, dm.VD1t, ldx, dm.A1t, dm.A0t, dm.AGt, ldi,
-                              dm.xnode, dm.twy, dm.twz, dm.VQ1t, dm.VQ2t};
+                              dm.xnode, dm.twy, dm.twz};
+  if (h->mhd) {
+    pcf::MhdMats<T> &mm = dm.mview;
+    const double W = h->geo.W;
+    if (h->insul)
+      mm = pcf::MhdMats<T>{dm.VDt, dm.VD1t, dm.VCt, dm.VC1t, dm.VC2t, ldx, nd, n0,
+                           dm.AGt, dm.WQt, ldi, ldc, nd, n0, W, 1.0};
+    else
+      mm = pcf::MhdMats<T>{dm.VDt, dm.VQ1t, dm.VDt, dm.VD1t, dm.VQ2t, ldx, nd, nd,
+                           dm.AGt, dm.AGt, ldi, ldi, nd, nd, W, W};
+  }
   (void)nb;
   return PCF_OK;
 }
@@ -298,14 +326,15 @@ inline hipError_t mark(pcf_handle *h, int stage, int edge) {
 // the A fields of the state (A, three components) give b and j and receive
 // the induction rows in NA[slot].
 template <typename T>
-hipError_t nonlinear_stages(pcf_handle *h, const pcf::NodalMats<T> &nm, const double2 *u0, int slot,
+hipError_t nonlinear_stages(pcf_handle *h, const DevMats<T> &dm, const double2 *u0, int slot,
                             const double2 *A) {
   const pcf::Geo &g = h->geo;
+  const pcf::NodalMats<T> &nm = dm.view;
   const bool mhd = h->mhd;
   hipError_t e;
   if ((e = mark(h, 0, 0))) return e;
   if ((e = pcf::launch_nodal_mfma<T>(g, nm, u0, h->u1, h->u2, h->ws, h->stream))) return e;
-  if (mhd && (e = pcf::launch_nodal_mhd_mfma<T>(g, nm, A, h->cstride, h->ws, h->stream))) return e;
+  if (mhd && (e = pcf::launch_nodal_mhd_mfma<T>(g, dm.mview, A, h->cstride, h->ws, h->stream))) return e;
   if ((e = mark(h, 0, 1))) return e;
   if ((e = mark(h, 1, 0))) return e;
   if ((e = pcf::launch_fft_y<T>(g, nm, h->ws, h->py, true, mhd, h->stream))) return e;
@@ -320,7 +349,7 @@ hipError_t nonlinear_stages(pcf_handle *h, const pcf::NodalMats<T> &nm, const do
   if ((e = pcf::launch_rhs_mfma<T>(g, nm, h->ws, h->Nu[slot], h->Ng[slot], h->Nv[slot], h->Nw[slot],
                               h->stream)))
     return e;
-  if (mhd && (e = pcf::launch_rhs_a_mfma<T>(g, nm, h->ws, h->NA[slot], h->cstride, h->stream))) return e;
+  if (mhd && (e = pcf::launch_rhs_a_mfma<T>(g, dm.mview, h->ws, h->NA[slot], h->cstride, h->stream))) return e;
   return mark(h, 4, 1);
 }
 
@@ -334,8 +363,8 @@ int32_t nonlinear_into(pcf_handle *h, const double2 *u0, int slot, void *ytape =
   void *saved = h->ws.Y;
   if (ytape) h->ws.Y = ytape;
   hipError_t e = (h->cfg.precision == PCF_PREC_FP64)
-                     ? nonlinear_stages<double>(h, h->md.view, u0, slot, A)
-                     : nonlinear_stages<float>(h, h->mf.view, u0, slot, A);
+                     ? nonlinear_stages<double>(h, h->md, u0, slot, A)
+                     : nonlinear_stages<float>(h, h->mf, u0, slot, A);
   h->ws.Y = saved;
   if (e != hipSuccess) return fail(PCF_E_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
   return PCF_OK;
@@ -351,10 +380,13 @@ hipError_t implicit_update(pcf_handle *h, const pcf::SolveArgs &a, bool steady,
                   : pcf::launch_solve(h->geo, h->sm, a, h->scratch, false, h->stream)))
     return e;
   if (aA) {
+    // components solved with the Dirichlet (TD) systems: all three for
+    // conducting walls, Ax alone for insulating walls
+    const int ntd = h->insul ? 1 : 3;
     if (steady) {
-      if ((e = pcf::launch_solve_factored(h->geo_eta, h->sm, h->fA, *aA, 3, h->cstride, h->stream))) return e;
+      if ((e = pcf::launch_solve_factored(h->geo_eta, h->sm, h->fA, *aA, ntd, h->cstride, h->stream))) return e;
     } else {
-      for (int c = 0; c < 3; ++c) {
+      for (int c = 0; c < ntd; ++c) {
         pcf::SolveArgs ac = *aA;
         const size_t off = c * h->cstride;
         for (int l = 0; l < 3; ++l) {
@@ -364,6 +396,24 @@ hipError_t implicit_update(pcf_handle *h, const pcf::SolveArgs &a, bool steady,
         ac.g_out += off;
         if ((e = pcf::launch_solve(h->geo_eta, h->sm, ac, h->scratch, true, h->stream))) return e;
       }
+    }
+    if (h->insul) {
+      pcf::BorderedArgs b{};
+      for (int l = 0; l < aA->nx; ++l) {
+        b.x[l] = aA->g[l];
+        b.c[l] = aA->c[l];
+        b.d[l] = aA->d[l];
+      }
+      b.nx = aA->nx;
+      for (int l = 0; l < aA->nn; ++l) {
+        b.n[l] = aA->Ng[l];
+        b.e[l] = aA->e[l];
+      }
+      b.nn = aA->nn;
+      b.gamma = aA->gamma;
+      b.out = aA->g_out;
+      b.cstride = h->cstride;
+      if ((e = pcf::launch_solve_bordered(h->geo_eta, b, h->stream))) return e;
     }
   }
   if ((e = mark(h, 5, 1))) return e;
@@ -543,7 +593,7 @@ bool field_ref(pcf_handle *h, int f, FieldRef &fr) {
                     : (grp == 2) ? h->NA[h->no[2]]
                     : (grp == 3) ? h->A[h->so[1]]
                                  : h->A[h->so[2]];
-    fr = {base + comp * h->cstride, true, nd};
+    fr = {base + comp * h->cstride, true, comp ? h->arows : nd};
     return true;
   }
   switch (f) {
@@ -562,6 +612,10 @@ bool field_ref(pcf_handle *h, int f, FieldRef &fr) {
 
 int field_rows(const pcf_config *c, int f) {
   const int nb = c->n0 - 4, nd = c->n0 - 2;
+  if (f >= PCF_F_AX && f < PCF_F_COUNT) {
+    const bool tc = c->mhd && c->walls == PCF_WALLS_INSULATING && (f - PCF_F_AX) % 3 != 0;
+    return tc ? c->n0 : nd;
+  }
   switch (f) {
     case PCF_F_U0:
     case PCF_F_NL_OLD_U:
@@ -1014,7 +1068,10 @@ int32_t pcf_create(const pcf_config *cfg, pcf_handle **out) {
   g.W = (2.0 * M_PI / g.df1) * (2.0 * M_PI / g.df2);
   g.nu = cfg->nu;
   h->mhd = cfg->mhd != 0;
+  h->insul = h->mhd && cfg->walls == PCF_WALLS_INSULATING;
+  h->arows = h->insul ? g.n0 : g.nd;
   for (int c = 0; c < 3; ++c) g.b0[c] = h->mhd ? cfg->b0[c] : 0.0;
+  g.omega2 = h->mhd ? 2.0 * cfg->omega : 0.0;
   if (g.my % 2) return bail(fail(PCF_E_INVALID, "padded streamwise grid must be even"));
   if (g.mz < 2 * g.kzs) return bail(fail(PCF_E_INVALID, "spanwise padding must not truncate"));
   if (!make_plan(g.my, h->py) || !make_plan(g.mz, h->pz))
@@ -1045,7 +1102,7 @@ int32_t pcf_create(const pcf_config *cfg, pcf_handle **out) {
   if ((r = h->alloc(&h->u1, (size_t)g.nd * M))) return bail(r);
   if ((r = h->alloc(&h->u2, (size_t)g.nd * M))) return bail(r);
   if (h->mhd) {
-    h->cstride = (size_t)g.nd * M;
+    h->cstride = (size_t)h->arows * M;
     for (int s = 0; s < 3; ++s)
       if ((r = h->alloc(&h->A[s], 3 * h->cstride))) return bail(r);
     for (int s = 0; s < 4; ++s)
@@ -1078,7 +1135,12 @@ int32_t pcf_create(const pcf_config *cfg, pcf_handle **out) {
     h->geo_eta = h->geo;
     h->geo_eta.nu = cfg->eta;
     if ((r = build_factors_A(h, cfg->dt * 6.0 / 11.0))) return bail(r);
-    if ((r = h->alloc(&h->ddiagm, 2))) return bail(r);
+    if ((r = h->alloc(&h->ddiagm, 5))) return bail(r);
+    if (h->insul) {
+      if ((r = h->upload(&h->dVC, h->ops.VC.a))) return bail(r);
+      if ((r = h->upload(&h->dVC1, h->ops.VC1.a))) return bail(r);
+      if ((r = h->alloc(&h->dresid, (size_t)pcf::insul_residual_blocks(g)))) return bail(r);
+    }
   }
   if ((r = h->upload(&h->dVB, h->ops.VB.a))) return bail(r);
   if ((r = h->upload(&h->dVD, h->ops.VD.a))) return bail(r);
@@ -1289,16 +1351,30 @@ int32_t pcf_mhd_diagnostics(pcf_handle *h, double *out) {
   if (!h->mhd) return fail(PCF_E_INVALID, "not an MHD handle");
   const pcf::Geo &g = h->geo;
   PCF_HIP(hipSetDevice(h->device));
-  PCF_HIP(hipMemsetAsync(h->ddiagm, 0, 2 * sizeof(double), h->stream));
-  hipError_t e = pcf::launch_diag_mhd(g, h->dVD, h->dVD1, h->dfejer, h->A[h->so[0]], h->cstride, h->ddiagm,
-                                      h->stream);
+  PCF_HIP(hipMemsetAsync(h->ddiagm, 0, 5 * sizeof(double), h->stream));
+  const double2 *A = h->A[h->so[0]];
+  hipError_t e = h->insul ? pcf::launch_diag_mhd(g, h->dVD, g.nd, h->dVC, h->dVC1, g.n0, h->dfejer, A, h->cstride,
+                                                 h->ddiagm, h->stream)
+                          : pcf::launch_diag_mhd(g, h->dVD, g.nd, h->dVD, h->dVD1, g.nd, h->dfejer, A, h->cstride,
+                                                 h->ddiagm, h->stream);
+  if (e == hipSuccess && h->insul) e = pcf::launch_insul_residual(g, A, h->cstride, h->dresid, h->stream);
   if (e != hipSuccess) return fail(PCF_E_HIP, std::string("diag launch: ") + hipGetErrorString(e));
-  double s[2];
+  double s[5];
+  std::vector<double> bm(h->insul ? pcf::insul_residual_blocks(g) : 0);
   PCF_HIP(hipMemcpyAsync(s, h->ddiagm, sizeof(s), hipMemcpyDeviceToHost, h->stream));
+  if (h->insul) PCF_HIP(hipMemcpyAsync(bm.data(), h->dresid, bm.size() * sizeof(double), hipMemcpyDeviceToHost,
+                                       h->stream));
   PCF_HIP(hipStreamSynchronize(h->stream));
-  const double vol = h->cfg.ly * h->cfg.lz;
-  out[0] = vol * s[0];
-  out[1] = std::sqrt(vol * s[1]);
+  const double area = h->cfg.ly * h->cfg.lz;
+  const double *b0 = g.b0;
+  out[0] = area * s[0];
+  out[1] = std::sqrt(area * s[1]);
+  // int |b + b0|^2 = int |b|^2 + 2 b0 . int b + |b0|^2 V, V = 2 Ly Lz
+  out[2] = out[0] + 2.0 * area * (b0[0] * s[2] + b0[1] * s[3] + b0[2] * s[4]) +
+           (b0[0] * b0[0] + b0[1] * b0[1] + b0[2] * b0[2]) * 2.0 * area;
+  double r = 0.0;
+  for (double v : bm) r = std::max(r, v);
+  out[3] = r;
   return PCF_OK;
 }
 

@@ -95,10 +95,10 @@ __global__ void __launch_bounds__(256) k_fft_y(Geo g, const typename VT<T>::V *_
 // c2r synthesis and the r2c analysis.
 // NZC > 0: the padded spanwise length is the compile-time NZC (the launcher
 // dispatches the common lengths); NZC = 0 reads it from g.
-// MHD: twelve input fields (u, w, b, j) and six outputs, H = -(u_tot x w_tot)
-// - j x b_tot and E = u_tot x b_tot with b_tot = b + B0
-// (examples/pcf_mhd_jax.py:339-357; the background B0 of
-// examples/pcf_mhd_mri_shearpy_jax.py:149-151).
+// MHD: twelve input fields (u, w, b, j) and six outputs,
+// H = -(u_tot x w_tot) + (-2 Omega u1, 2 Omega u0, 0) - j x b_tot and
+// E = u_tot x b_tot with b_tot = b + B0 (examples/pcf_mhd_jax.py:339-357;
+// Coriolis force and background B0 of examples/pcf_mhd_mri_shearpy_jax.py:149-176).
 template <typename T, int MAXB, int NZC, bool MHD>
 __global__ void __launch_bounds__(256) k_rotational(Geo g, NodalMats<T> nm, FFTPlan plan, int UPB,
                                                     const typename VT<T>::V *__restrict__ Y,
@@ -148,7 +148,7 @@ __global__ void __launch_bounds__(256) k_rotational(Geo g, NodalMats<T> nm, FFTP
     fft_lds<V, T, true, false, MAXB>(buf, tw, plan, UPB * NIN, MZ, 1);
 
   const T U = (T)g.uwall;
-  const T B0x = (T)g.b0[0], B0y = (T)g.b0[1], B0z = (T)g.b0[2];
+  const T B0x = (T)g.b0[0], B0y = (T)g.b0[1], B0z = (T)g.b0[2], O2 = (T)g.omega2;
   for (int idx = tid; idx < UPB * MZ; idx += nthr) {
     int lu = idx / MZ, n = idx - lu * MZ;
     int u = ubase + lu;
@@ -164,6 +164,9 @@ __global__ void __launch_bounds__(256) k_rotational(Geo g, NodalMats<T> nm, FFTP
       T h0 = -(a1 * o2 - a2 * o1), h1 = -(a2 * o0 - a0 * o2), h2 = -(a0 * o1 - a1 * o0);
       T e0 = 0, e1 = 0, e2 = 0;
       if constexpr (MHD) {
+        // Coriolis force (-2 Omega u1, 2 Omega u0, 0) on the fluctuation velocity
+        h0 -= O2 * P(1);
+        h1 += O2 * P(0);
         const T c0 = P(6) + B0x, c1 = P(7) + B0y, c2 = P(8) + B0z, j0 = P(9), j1 = P(10), j2 = P(11);
         h0 -= j1 * c2 - j2 * c1;
         h1 -= j2 * c0 - j0 * c2;
@@ -506,6 +509,138 @@ __global__ void __launch_bounds__(256) k_solve(Geo g, StructMats sm, SolveArgs a
   }
 }
 
+// Insulating walls: the A_y / A_z update of one (mode, parity) per thread
+// (examples/pcf_mhd_mri_shearpy_jax.py:559-712).  On the orthogonal Chebyshev
+// basis the radial operator h - gamma eta ((T_i, T_j'') - k^2 h) is diagonal
+// plus a strictly upper part that is rank 2 within each parity:
+// (T_i, T_j'') = (pi/2) j (j^2 - i^2) for j > i, j - i even (both i = 0 and
+// i > 0, since h_0 = pi carries the 1/c_0 = 1/2 of the derivative recurrence).
+// The two wall rows replacing the last two Galerkin rows separate by parity
+// into sum_{j in parity} beta_j x_j = 0 with beta_j = j^2 (P: P'(+-1) = 0) or
+// j^2 + k (Q: Q'(-1) - k Q(-1) = Q'(1) + k Q(1) = 0), because
+// T_j'(+-1) = (+-1)^{j+1} j^2 and T_j(+-1) = (+-1)^j.  So each parity block is
+// upper triangular with its last row replaced by beta: with t = x_last, the
+// back substitution gives x = a + b t, and beta . x = 0 fixes t.  Two top-down
+// passes, O(n) each; the first parks a in the output rows.  The right-hand
+// side (h z + eta[(T_i,T_j'') zL - k^2 h zL] + N, z = sum c x, zL = sum d x)
+// is formed in the first pass with running suffix sums.
+__global__ void __launch_bounds__(256) k_solve_bordered(Geo g, BorderedArgs a) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int M = g.m;
+  if (t >= 2 * M) return;
+  const int m = t >> 1, par = t & 1;
+  const int n = g.n0;
+  double kyd, kzd, kyo;
+  mode_k(g, m, kyd, kzd, kyo);
+  const double k2 = kyd * kyd + kzd * kzd, k = sqrt(k2);
+  const double cy = (k > 0.0) ? kyd / k : 1.0, cz = (k > 0.0) ? kzd / k : 0.0;
+  const bool nyq = (m / g.kzs == g.ny / 2);
+  const double eta = g.nu, ge = a.gamma * eta, C = ge * 0.5 * M_PI;
+  const size_t cs = a.cstride;
+  double2 *oy = a.out + cs, *oz = a.out + 2 * cs;
+  const int last = (n - 1 - par) / 2 * 2 + par;  // highest row of this parity: the wall row
+  // pass 1
+  double2 s3y{0, 0}, s1y{0, 0}, s3z{0, 0}, s1z{0, 0};      // suffix sums of zL (y, z)
+  double2 pa3{0, 0}, pa1{0, 0}, qa3{0, 0}, qa1{0, 0};      // suffix sums of a (P, Q)
+  double pb3 = 0, pb1 = 0, qb3 = 0, qb1 = 0;               // suffix sums of b (P, Q)
+  double2 pba{0, 0}, qba{0, 0};                            // beta . a
+  double pbb = 0, qbb = 0;                                 // beta . b
+  for (int I = last; I >= 0; I -= 2) {
+    const size_t o = (size_t)I * M + m;
+    const double hI = (I == 0) ? M_PI : 0.5 * M_PI, I2 = (double)I * I, I3 = I2 * I;
+    double2 zy{0, 0}, zly{0, 0}, ny{0, 0}, zz{0, 0}, zlz{0, 0}, nz{0, 0};
+    for (int l = 0; l < a.nx; ++l) {
+      const double2 vy = a.x[l][cs + o], vz = a.x[l][2 * cs + o];
+      zy.x += a.c[l] * vy.x;
+      zy.y += a.c[l] * vy.y;
+      zly.x += a.d[l] * vy.x;
+      zly.y += a.d[l] * vy.y;
+      zz.x += a.c[l] * vz.x;
+      zz.y += a.c[l] * vz.y;
+      zlz.x += a.d[l] * vz.x;
+      zlz.y += a.d[l] * vz.y;
+    }
+    for (int l = 0; l < a.nn; ++l) {
+      const double2 vy = a.n[l][cs + o], vz = a.n[l][2 * cs + o];
+      ny.x += a.e[l] * vy.x;
+      ny.y += a.e[l] * vy.y;
+      nz.x += a.e[l] * vz.x;
+      nz.y += a.e[l] * vz.y;
+    }
+    // rhs = h z + eta[(pi/2)(S3 - I^2 S1) - k^2 h zL] + N
+    const double2 ry{hI * zy.x + eta * (0.5 * M_PI * (s3y.x - I2 * s1y.x) - k2 * hI * zly.x) + ny.x,
+                     hI * zy.y + eta * (0.5 * M_PI * (s3y.y - I2 * s1y.y) - k2 * hI * zly.y) + ny.y};
+    const double2 rz{hI * zz.x + eta * (0.5 * M_PI * (s3z.x - I2 * s1z.x) - k2 * hI * zlz.x) + nz.x,
+                     hI * zz.y + eta * (0.5 * M_PI * (s3z.y - I2 * s1z.y) - k2 * hI * zlz.y) + nz.y};
+    s3y.x += I3 * zly.x;
+    s3y.y += I3 * zly.y;
+    s1y.x += I * zly.x;
+    s1y.y += I * zly.y;
+    s3z.x += I3 * zlz.x;
+    s3z.y += I3 * zlz.y;
+    s1z.x += I * zlz.x;
+    s1z.y += I * zlz.y;
+    double2 ap, aq;
+    double bp, bq;
+    if (I == last) {
+      ap = aq = double2{0, 0};
+      bp = bq = 1.0;
+    } else {
+      const double rd = 1.0 / (hI * (1.0 + ge * k2));
+      const double2 rp{cy * ry.x + cz * rz.x, cy * ry.y + cz * rz.y};
+      const double2 rq{cz * ry.x - cy * rz.x, cz * ry.y - cy * rz.y};
+      ap = double2{(rp.x + C * (pa3.x - I2 * pa1.x)) * rd, (rp.y + C * (pa3.y - I2 * pa1.y)) * rd};
+      aq = double2{(rq.x + C * (qa3.x - I2 * qa1.x)) * rd, (rq.y + C * (qa3.y - I2 * qa1.y)) * rd};
+      bp = C * (pb3 - I2 * pb1) * rd;
+      bq = C * (qb3 - I2 * qb1) * rd;
+    }
+    pa3.x += I3 * ap.x;
+    pa3.y += I3 * ap.y;
+    pa1.x += I * ap.x;
+    pa1.y += I * ap.y;
+    qa3.x += I3 * aq.x;
+    qa3.y += I3 * aq.y;
+    qa1.x += I * aq.x;
+    qa1.y += I * aq.y;
+    pb3 += I3 * bp;
+    pb1 += I * bp;
+    qb3 += I3 * bq;
+    qb1 += I * bq;
+    const double betp = I2, betq = I2 + k;
+    pba.x += betp * ap.x;
+    pba.y += betp * ap.y;
+    qba.x += betq * aq.x;
+    qba.y += betq * aq.y;
+    pbb += betp * bp;
+    qbb += betq * bq;
+    oy[o] = ap;  // parking
+    oz[o] = aq;
+  }
+  const double2 tp{-pba.x / pbb, -pba.y / pbb}, tq{-qba.x / qbb, -qba.y / qbb};
+  // pass 2: b again (it depends on k and the row only), x = a + b t, rotate back
+  pb3 = pb1 = qb3 = qb1 = 0;
+  for (int I = last; I >= 0; I -= 2) {
+    const size_t o = (size_t)I * M + m;
+    const double hI = (I == 0) ? M_PI : 0.5 * M_PI, I2 = (double)I * I, I3 = I2 * I;
+    double bp, bq;
+    if (I == last) {
+      bp = bq = 1.0;
+    } else {
+      const double rd = 1.0 / (hI * (1.0 + ge * k2));
+      bp = C * (pb3 - I2 * pb1) * rd;
+      bq = C * (qb3 - I2 * qb1) * rd;
+    }
+    pb3 += I3 * bp;
+    pb1 += I * bp;
+    qb3 += I3 * bq;
+    qb1 += I * bq;
+    const double2 ap = oy[o], aq = oz[o];
+    const double2 p{ap.x + bp * tp.x, ap.y + bp * tp.y}, q{aq.x + bq * tq.x, aq.y + bq * tq.y};
+    oy[o] = nyq ? double2{0, 0} : double2{cy * p.x + cz * q.x, cy * p.y + cz * q.y};
+    oz[o] = nyq ? double2{0, 0} : double2{cz * p.x - cy * q.x, cz * p.y - cy * q.y};
+  }
+}
+
 // Velocity reconstruction (channelflow_kmm.py:855-870): f = R u0 with
 // R = P_D d/dx on the clamped basis, u1 = i (ky f + kz g) / k^2,
 // u2 = i (kz f - ky g) / k^2, mean mode (0,0) <- (v00, w00).  R couples
@@ -614,18 +749,20 @@ __global__ void __launch_bounds__(256) k_diag(Geo g, const double *__restrict__ 
     for (int q = 0; q < 4; ++q) atomicAdd(&out[q], red[q][0]);
 }
 
-// Magnetic diagnostics of the vector potential A (three Dirichlet components
-// cstride apart): out[0] += sum wF |b|^2 and out[1] += sum wF |div b|^2 over
-// the full spectrum (Hermitian weights), with b = curl A at the nodes
-// (bx = VD (i ky Az - i kz Ay), by = i kz VD Ax - VD1 Az, bz = VD1 Ay - i ky VD Ax;
-// examples/pcf_mhd_jax.py:760-789).
-__global__ void __launch_bounds__(256) k_diag_mhd(Geo g, const double *__restrict__ VD,
-                                                  const double *__restrict__ VD1,
+// Magnetic diagnostics of the vector potential A (three components cstride
+// apart): out[0] += sum wF |b|^2, out[1] += sum wF |div b|^2 over the full
+// spectrum (Hermitian weights), out[2..4] += sum_x wF b_c(x) of the mean mode,
+// with b = curl A at the nodes: DX = VDX Ax, DY/DZ = VDY A{y,z}, EY/EZ = VEY A{y,z},
+// bx = i ky DZ - i kz DY, by = i kz DX - EZ, bz = EY - i ky DX, and
+// d/dx bx = i ky EZ - i kz EY (examples/pcf_mhd_jax.py:760-789).
+__global__ void __launch_bounds__(256) k_diag_mhd(Geo g, const double *__restrict__ VDX, int kx,
+                                                  const double *__restrict__ VDY,
+                                                  const double *__restrict__ VEY, int kyz,
                                                   const double *__restrict__ fejer,
                                                   const double2 *__restrict__ A, size_t cstride,
                                                   double *__restrict__ out) {
   const int m = blockIdx.x * blockDim.x + threadIdx.x;
-  double eb = 0.0, dv = 0.0;
+  double eb = 0.0, dv = 0.0, mb[3] = {0.0, 0.0, 0.0};
   if (m < g.m) {
     double kyd, kzd, kyo;
     mode_k(g, m, kyd, kzd, kyo);
@@ -633,44 +770,109 @@ __global__ void __launch_bounds__(256) k_diag_mhd(Geo g, const double *__restric
     const double wt = ((m % g.kzs) == 0) ? 1.0 : 2.0;
     for (int x = 0; x < g.n0; ++x) {
       double2 DX{0, 0}, DY{0, 0}, DZ{0, 0}, EY{0, 0}, EZ{0, 0};
-      for (int i = 0; i < g.nd; ++i) {
-        const size_t o = (size_t)i * g.m + m;
-        const double2 ax = A[o], ay = A[cstride + o], az = A[2 * cstride + o];
-        const double vd = VD[(size_t)x * g.nd + i], vd1 = VD1[(size_t)x * g.nd + i];
+      for (int i = 0; i < kx; ++i) {
+        const double2 ax = A[(size_t)i * g.m + m];
+        const double vd = VDX[(size_t)x * kx + i];
         DX.x += vd * ax.x;
         DX.y += vd * ax.y;
+      }
+      for (int i = 0; i < kyz; ++i) {
+        const size_t o = (size_t)i * g.m + m;
+        const double2 ay = A[cstride + o], az = A[2 * cstride + o];
+        const double vd = VDY[(size_t)x * kyz + i], ve = VEY[(size_t)x * kyz + i];
         DY.x += vd * ay.x;
         DY.y += vd * ay.y;
         DZ.x += vd * az.x;
         DZ.y += vd * az.y;
-        EY.x += vd1 * ay.x;
-        EY.y += vd1 * ay.y;
-        EZ.x += vd1 * az.x;
-        EZ.y += vd1 * az.y;
+        EY.x += ve * ay.x;
+        EY.y += ve * ay.y;
+        EZ.x += ve * az.x;
+        EZ.y += ve * az.y;
       }
       // i z = (-z.y, z.x)
       const double2 bx{-(ky * DZ.y - kz * DY.y), ky * DZ.x - kz * DY.x};
       const double2 by{-kz * DX.y - EZ.x, kz * DX.x - EZ.y};
       const double2 bz{EY.x + ky * DX.y, EY.y - ky * DX.x};
-      // div b = d/dx bx + i ky by + i kz bz, d/dx bx = VD1 (i ky Az - i kz Ay)
       const double2 dbx{-(ky * EZ.y - kz * EY.y), ky * EZ.x - kz * EY.x};
       const double2 d{dbx.x - ky * by.y - kz * bz.y, dbx.y + ky * by.x + kz * bz.x};
       const double w = fejer[x] * wt;
       eb += w * (bx.x * bx.x + bx.y * bx.y + by.x * by.x + by.y * by.y + bz.x * bz.x + bz.y * bz.y);
       dv += w * (d.x * d.x + d.y * d.y);
+      if (m == 0) {
+        mb[0] += fejer[x] * bx.x;
+        mb[1] += fejer[x] * by.x;
+        mb[2] += fejer[x] * bz.x;
+      }
     }
   }
-  __shared__ double red[2][256];
+  __shared__ double red[5][256];
   red[0][threadIdx.x] = eb;
   red[1][threadIdx.x] = dv;
+  for (int q = 0; q < 3; ++q) red[2 + q][threadIdx.x] = mb[q];
   __syncthreads();
   for (int s = blockDim.x / 2; s > 0; s >>= 1) {
     if ((int)threadIdx.x < s)
-      for (int q = 0; q < 2; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + s];
+      for (int q = 0; q < 5; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + s];
     __syncthreads();
   }
   if (threadIdx.x == 0)
-    for (int q = 0; q < 2; ++q) atomicAdd(&out[q], red[q][0]);
+    for (int q = 0; q < 5; ++q) atomicAdd(&out[q], red[q][0]);
+}
+
+// Vacuum-matching residual of insulating walls (A_y, A_z in TC):
+// b = curl A at x = -1, +1 from T_j(+-1) = (+-1)^j, T_j'(+-1) = (+-1)^{j+1} j^2
+// (the TD component Ax vanishes there): bx = i ky Az - i kz Ay,
+// bx' = d/dx of it, by = -Az', bz = Ay'; residuals bx' + s k bx,
+// by + s (i ky / k) bx, bz + s (i kz / k) bx with s = -1, +1 at the lower and
+// upper walls (insulating_bc_residual, examples/pcf_mhd_mri_shearpy_jax.py:802-830).
+// One thread per mode; blockmax[block] = the block's largest |residual|.
+__global__ void __launch_bounds__(256) k_insul_residual(Geo g, const double2 *__restrict__ A, size_t cstride,
+                                                        double *__restrict__ blockmax) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  double r = 0.0;
+  if (m < g.m && m / g.kzs != g.ny / 2) {
+    double kyd, kzd, kyo;
+    mode_k(g, m, kyd, kzd, kyo);
+    const double ky = kyo, kz = kzd, k = sqrt(kyd * kyd + kzd * kzd);
+    const double kn = (k > 0.0) ? k : 1.0;
+    // wall sums: v[w] = sum T_j(w) c_j, d[w] = sum T_j'(w) c_j for c = Ay, Az
+    double2 vy[2] = {{0, 0}, {0, 0}}, vz[2] = {{0, 0}, {0, 0}}, dy[2] = {{0, 0}, {0, 0}}, dz[2] = {{0, 0}, {0, 0}};
+    for (int j = 0; j < g.n0; ++j) {
+      const size_t o = (size_t)j * g.m + m;
+      const double2 ay = A[cstride + o], az = A[2 * cstride + o];
+      const double sg = (j & 1) ? -1.0 : 1.0, jj = (double)j * j;
+      const double t0[2] = {sg, 1.0}, t1[2] = {-sg * jj, jj};
+      for (int w = 0; w < 2; ++w) {
+        vy[w].x += t0[w] * ay.x;
+        vy[w].y += t0[w] * ay.y;
+        vz[w].x += t0[w] * az.x;
+        vz[w].y += t0[w] * az.y;
+        dy[w].x += t1[w] * ay.x;
+        dy[w].y += t1[w] * ay.y;
+        dz[w].x += t1[w] * az.x;
+        dz[w].y += t1[w] * az.y;
+      }
+    }
+    for (int w = 0; w < 2; ++w) {
+      const double s = w ? 1.0 : -1.0;
+      const double2 bx{-(ky * vz[w].y - kz * vy[w].y), ky * vz[w].x - kz * vy[w].x};
+      const double2 bxp{-(ky * dz[w].y - kz * dy[w].y), ky * dz[w].x - kz * dy[w].x};
+      const double2 by{-dz[w].x, -dz[w].y}, bz = dy[w];
+      const double2 rx{bxp.x + s * k * bx.x, bxp.y + s * k * bx.y};
+      // s (i ky / k) bx = s ky / k (-bx.y, bx.x)
+      const double2 ry{by.x - s * ky / kn * bx.y, by.y + s * ky / kn * bx.x};
+      const double2 rz{bz.x - s * kz / kn * bx.y, bz.y + s * kz / kn * bx.x};
+      r = fmax(r, fmax(hypot(rx.x, rx.y), fmax(hypot(ry.x, ry.y), hypot(rz.x, rz.y))));
+    }
+  }
+  __shared__ double red[256];
+  red[threadIdx.x] = r;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) blockmax[blockIdx.x] = red[0];
 }
 
 // ------------------------------------------------------------ launchers ----
@@ -794,6 +996,11 @@ hipError_t launch_solve(const Geo &g, const StructMats &sm, const SolveArgs &a, 
   return hipGetLastError();
 }
 
+hipError_t launch_solve_bordered(const Geo &g, const BorderedArgs &a, hipStream_t s) {
+  hipLaunchKernelGGL(k_solve_bordered, dim3((2 * g.m + 255) / 256), dim3(256), 0, s, g, a);
+  return hipGetLastError();
+}
+
 hipError_t launch_reconstruct(const Geo &g, const double *Rc, int ldk, const SolveArgs &a,
                               hipStream_t s) {
   dim3 grid((g.m + 255) / 256, g.nd);
@@ -811,9 +1018,17 @@ hipError_t launch_diag(const Geo &g, const double *VB, const double *VD, const d
   return hipGetLastError();
 }
 
-hipError_t launch_diag_mhd(const Geo &g, const double *VD, const double *VD1, const double *fejer,
-                           const double2 *A, size_t cstride, double *out, hipStream_t s) {
-  hipLaunchKernelGGL(k_diag_mhd, dim3((g.m + 255) / 256), dim3(256), 0, s, g, VD, VD1, fejer, A, cstride, out);
+hipError_t launch_diag_mhd(const Geo &g, const double *VDX, int kx, const double *VDY, const double *VEY, int kyz,
+                           const double *fejer, const double2 *A, size_t cstride, double *out, hipStream_t s) {
+  hipLaunchKernelGGL(k_diag_mhd, dim3((g.m + 255) / 256), dim3(256), 0, s, g, VDX, kx, VDY, VEY, kyz, fejer, A,
+                     cstride, out);
+  return hipGetLastError();
+}
+
+int insul_residual_blocks(const Geo &g) { return (g.m + 255) / 256; }
+
+hipError_t launch_insul_residual(const Geo &g, const double2 *A, size_t cstride, double *blockmax, hipStream_t s) {
+  hipLaunchKernelGGL(k_insul_residual, dim3(insul_residual_blocks(g)), dim3(256), 0, s, g, A, cstride, blockmax);
   return hipGetLastError();
 }
 

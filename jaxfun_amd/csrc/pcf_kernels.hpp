@@ -30,6 +30,7 @@ struct Geo {
   double nu;
   int npmax;       // largest parity block of the per-mode systems (<= 64)
   double b0[3];    // uniform background magnetic field (MHD)
+  double omega2;   // twice the rotation rate of the Coriolis force (MHD)
 };
 
 // Radix plan of one FFT length (radices 2, 3, 4).
@@ -54,8 +55,24 @@ struct NodalMats {
   const T *xnode; // [n0]
   const T *twy;   // [my] exp(-2 pi i t / my) as (cos, sin) pairs
   const T *twz;   // [mz]
-  const T *VQ1t;  // [nd][ldx] (VD P_D D S_D^T)^T, MHD only (else null)
-  const T *VQ2t;  // [nd][ldx] (VD P_D D^2 S_D^T)^T
+};
+
+// Wall-normal matrices of the MHD contractions, transposed ([K][ld]) like
+// NodalMats.  Nodal b and j take DX = mDX Ax, QX = mQX Ax (K = kx rows of Ax)
+// and DY/DZ = mDY A{y,z}, EY/EZ = mEY A{y,z}, RY/RZ = mRY A{y,z} (K = kyz rows);
+// the induction rows are NA_x = s0 mA0 E_x (n0r rows) and
+// NA_{y,z} = s12 mA12 E_{y,z} (n12r rows), K = nodes.
+//   conducting walls: mDX = mDY = VD, mQX = VQ1, mEY = VD1, mRY = VQ2,
+//                     mA0 = mA12 = AG, s0 = s12 = W;
+//   insulating walls: mDX = VD, mQX = VD1, mDY = VC, mEY = VC1, mRY = VC2,
+//                     mA0 = AG (s0 = W), mA12 = WQ (s12 = 1).
+template <typename T>
+struct MhdMats {
+  const T *mDX, *mQX, *mDY, *mEY, *mRY;
+  int ldx, kx, kyz;
+  const T *mA0, *mA12;
+  int ld0, ld12, n0r, n12r;
+  double s0, s12;
 };
 
 // Structured per-mode operator data of one (basis, parity) block: the band
@@ -120,11 +137,28 @@ hipError_t launch_rotational(const Geo &g, const NodalMats<T> &nm, Workspace &ws
 // MHD: b = curl A and j = curl b at the nodes (X fields 6..11) from the three
 // vector-potential components cstride apart; induction rows NA = W AG E.
 template <typename T>
-hipError_t launch_nodal_mhd_mfma(const Geo &g, const NodalMats<T> &nm, const double2 *A, size_t cstride,
+hipError_t launch_nodal_mhd_mfma(const Geo &g, const MhdMats<T> &mm, const double2 *A, size_t cstride,
                                  Workspace &ws, hipStream_t s);
 template <typename T>
-hipError_t launch_rhs_a_mfma(const Geo &g, const NodalMats<T> &nm, Workspace &ws, double2 *NA, size_t cstride,
+hipError_t launch_rhs_a_mfma(const Geo &g, const MhdMats<T> &mm, Workspace &ws, double2 *NA, size_t cstride,
                              hipStream_t s);
+// Insulating walls: the A_y/A_z update.  rhs_c = h z_c + eta [(T_i, T_j'') zL_c
+// - k^2 h zL_c] + sum_l e_l N_l,c with z = sum c_l x_l, zL = sum d_l x_l, rotated
+// to P = cy Ay + cz Az, Q = cz Ay - cy Az and solved with the bordered radial
+// systems (last row of each parity replaced by the vacuum-matching row), then
+// rotated back.  Components y, z sit at 1 and 2 cstrides from each pointer.
+struct BorderedArgs {
+  const double2 *x[3];
+  double c[3], d[3];
+  int nx;
+  const double2 *n[3];
+  double e[3];
+  int nn;
+  double gamma;
+  double2 *out;
+  size_t cstride;
+};
+hipError_t launch_solve_bordered(const Geo &g, const BorderedArgs &a, hipStream_t s);
 // General implicit update (IMEXRK3 stages): the factorisation runs with the solve.
 // dir_only: the Dirichlet systems alone (g side of a), for a vector-potential component.
 hipError_t launch_solve(const Geo &g, const StructMats &sm, const SolveArgs &a, double *scratch, bool dir_only,
@@ -150,9 +184,16 @@ hipError_t launch_diag(const Geo &g, const double *VB, const double *VD, const d
                        const double2 *u0, const double2 *u1, const double2 *u2, double *out,
                        hipStream_t s);
 
-// MHD diagnostics partial sums: out[0] = sum wF |curl A|^2, out[1] = sum wF |div curl A|^2.
-hipError_t launch_diag_mhd(const Geo &g, const double *VD, const double *VD1, const double *fejer,
-                           const double2 *A, size_t cstride, double *out, hipStream_t s);
+// MHD diagnostics partial sums: out[0] = sum wF |b|^2, out[1] = sum wF |div b|^2
+// (b = curl A), out[2..4] = sum_x wF b_c(x) of the mean mode.  VDX [n0][kx],
+// VDY and VEY [n0][kyz] are row-major nodal matrices (values, x-derivatives)
+// of the Ax and Ay/Az bases.
+hipError_t launch_diag_mhd(const Geo &g, const double *VDX, int kx, const double *VDY, const double *VEY, int kyz,
+                           const double *fejer, const double2 *A, size_t cstride, double *out, hipStream_t s);
+// Insulating walls: per-block maxima of the vacuum-matching wall residuals of
+// b = curl A (blockmax holds insul_residual_blocks(g) doubles).
+int insul_residual_blocks(const Geo &g);
+hipError_t launch_insul_residual(const Geo &g, const double2 *A, size_t cstride, double *blockmax, hipStream_t s);
 
 // Largest butterfly count per thread any FFT launch needs (host-side check
 // against the kernels' register budgets).

@@ -171,24 +171,26 @@ __global__ void __launch_bounds__(512) k_nodal_mfma(Geo g, NodalMats<T> nm, cons
   }
 }
 
-// Magnetic field and current of the vector potential A (all three components
-// in the Dirichlet basis), at the nodes: with DX = VD Ax, DY = VD Ay,
-// DZ = VD Az, EY = VD1 Ay, EZ = VD1 Az, QX = VQ1 Ax, RY = VQ2 Ay, RZ = VQ2 Az
+// Magnetic field and current of the vector potential A at the nodes, with
+// DX = mDX Ax, QX = mQX Ax, DY = mDY Ay, DZ = mDY Az, EY = mEY Ay, EZ = mEY Az,
+// RY = mRY Ay, RZ = mRY Az (MhdMats):
 //   bx = i ky DZ - i kz DY,  by = i kz DX - EZ,  bz = EY - i ky DX,
 //   jx = k^2 DX + i ky EY + i kz EZ,
 //   jy = kz^2 DY - ky kz DZ - RY + i ky QX,
-//   jz = ky^2 DZ - ky kz DY - RZ + i kz QX,
-// where VQ1 = VD P_D D S_D^T and VQ2 = VD P_D D^2 S_D^T carry the Galerkin
-// projections of d/dx B onto the Dirichlet basis that J = curl B needs
+//   jz = ky^2 DZ - ky kz DY - RZ + i kz QX.
+// Conducting walls (A in TD^3): mQX = VD P_D D S_D^T and mRY = VD P_D D^2 S_D^T
+// carry the Galerkin projections of d/dx B onto TD that J = curl B needs
 // (update_B_from_A / update_J_from_B, examples/pcf_mhd_jax.py:269-311).
+// Insulating walls (Ay, Az in TC, B and J in TC^3): mQX = VD1, mDY/mEY/mRY the
+// Chebyshev values and derivatives (examples/pcf_mhd_mri_shearpy_jax.py:548-557).
 // Fields 6..11 of X receive (bx, by, bz, jx, jy, jz).
 template <typename T>
-__global__ void __launch_bounds__(512) k_nodal_mhd_mfma(Geo g, NodalMats<T> nm, const double2 *__restrict__ A,
+__global__ void __launch_bounds__(512) k_nodal_mhd_mfma(Geo g, MhdMats<T> mm, const double2 *__restrict__ A,
                                                         size_t cstride, typename VT<T>::V *__restrict__ X) {
   using MM = Mfma<T>;
   extern __shared__ __align__(16) unsigned char smem[];
   const int ct = blockIdx.x;
-  const int M = g.m, kp = (g.nd + 3) / 4 * 4;
+  const int M = g.m, kp = (mm.kyz + 3) / 4 * 4;
   const Panel<T> B{reinterpret_cast<T *>(smem), kp};
   {
     const int total = 3 * kp * 8;
@@ -199,7 +201,7 @@ __global__ void __launch_bounds__(512) k_nodal_mhd_mfma(Geo g, NodalMats<T> nm, 
       const int f = idx / (kp * 8), rem = idx - f * kp * 8, k = rem >> 3, j = rem & 7;
       const int m = 8 * ct + j;
       z[t] = double2{0, 0};
-      if (idx < total && m < M && k < g.nd) z[t] = A[f * cstride + (size_t)k * M + m];
+      if (idx < total && m < M && k < (f ? mm.kyz : mm.kx)) z[t] = A[f * cstride + (size_t)k * M + m];
     }
 #pragma unroll
     for (int t = 0; t < STAGE_NODAL; ++t) {
@@ -226,22 +228,25 @@ __global__ void __launch_bounds__(512) k_nodal_mhd_mfma(Geo g, NodalMats<T> nm, 
     for (int kk = 0; kk < kp; kk += 4) {
       const int k = kk + (L.l >> 4);
       const T bx = B.at(0, k, L.c), by = B.at(1, k, L.c), bz = B.at(2, k, L.c);
-      T aD = 0, aD1 = 0, aQ1 = 0, aQ2 = 0;
-      if (k < g.nd) {
-        const size_t o = (size_t)k * nm.ldx + xa;
-        aD = nm.VDt[o];
-        aD1 = nm.VD1t[o];
-        aQ1 = nm.VQ1t[o];
-        aQ2 = nm.VQ2t[o];
+      T aD = 0, aQ = 0, aY = 0, aE = 0, aR = 0;
+      const size_t o = (size_t)k * mm.ldx + xa;
+      if (k < mm.kx) {
+        aD = mm.mDX[o];
+        aQ = mm.mQX[o];
+      }
+      if (k < mm.kyz) {
+        aY = mm.mDY[o];
+        aE = mm.mEY[o];
+        aR = mm.mRY[o];
       }
       DX = MM::mma(aD, bx, DX);
-      DY = MM::mma(aD, by, DY);
-      DZ = MM::mma(aD, bz, DZ);
-      EY = MM::mma(aD1, by, EY);
-      EZ = MM::mma(aD1, bz, EZ);
-      QX = MM::mma(aQ1, bx, QX);
-      RY = MM::mma(aQ2, by, RY);
-      RZ = MM::mma(aQ2, bz, RZ);
+      QX = MM::mma(aQ, bx, QX);
+      DY = MM::mma(aY, by, DY);
+      DZ = MM::mma(aY, bz, DZ);
+      EY = MM::mma(aE, by, EY);
+      EZ = MM::mma(aE, bz, EZ);
+      RY = MM::mma(aR, by, RY);
+      RZ = MM::mma(aR, bz, RZ);
     }
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
@@ -256,23 +261,24 @@ __global__ void __launch_bounds__(512) k_nodal_mhd_mfma(Geo g, NodalMats<T> nm, 
       const T jzv = kyy * dz - kyz * dy - rz + s * kz * pqx;
       const int x = x0 + MM::row(L.l, r);
       if (L.valid && x < g.n0) {
-        const size_t o = (size_t)x * 2 * M + 2 * L.mcol + L.comp;
-        Xr[o] = bxv;
-        Xr[plane + o] = byv;
-        Xr[2 * plane + o] = bzv;
-        Xr[3 * plane + o] = jxv;
-        Xr[4 * plane + o] = jyv;
-        Xr[5 * plane + o] = jzv;
+        const size_t oo = (size_t)x * 2 * M + 2 * L.mcol + L.comp;
+        Xr[oo] = bxv;
+        Xr[plane + oo] = byv;
+        Xr[2 * plane + oo] = bzv;
+        Xr[3 * plane + oo] = jxv;
+        Xr[4 * plane + oo] = jyv;
+        Xr[5 * plane + oo] = jzv;
       }
     }
   }
 }
 
-// Induction rows NA_c = W AG E_c (c = 0, 1, 2) of the nodal EMF E = u_tot x B
-// (Hx fields 3..5): the Dirichlet mass rows of forward_TD(E)
-// (_A_forward_emf + _A_mass_rhs, examples/pcf_mhd_jax.py:157-170).
+// Induction rows of the nodal EMF E = u_tot x B (Hx fields 3..5):
+// NA_x = s0 mA0 E_x and NA_{y,z} = s12 mA12 E_{y,z} — the mass rows of the
+// forward transforms of E (_A_forward_emf + _A_mass_rhs,
+// examples/pcf_mhd_jax.py:157-170; examples/pcf_mhd_mri_shearpy_jax.py:633-659).
 template <typename T>
-__global__ void __launch_bounds__(512) k_rhs_a_mfma(Geo g, NodalMats<T> nm,
+__global__ void __launch_bounds__(512) k_rhs_a_mfma(Geo g, MhdMats<T> mm,
                                                     const typename VT<T>::V *__restrict__ Hx,
                                                     double2 *__restrict__ NA, size_t cstride) {
   using MM = Mfma<T>;
@@ -312,26 +318,32 @@ __global__ void __launch_bounds__(512) k_rhs_a_mfma(Geo g, NodalMats<T> nm,
   __syncthreads();
   const Lane L = lane_of(g, ct);
   double *NAr = reinterpret_cast<double *>(NA);
-  for (int i0 = (threadIdx.x >> 6) * 16; i0 < g.nd; i0 += (blockDim.x >> 6) * 16) {
+  const int rows = mm.n0r > mm.n12r ? mm.n0r : mm.n12r;
+  for (int i0 = (threadIdx.x >> 6) * 16; i0 < rows; i0 += (blockDim.x >> 6) * 16) {
     const int ia = i0 + (L.l & 15);
     typename MM::acc C0{}, C1{}, C2{};
 #pragma unroll 4
     for (int kk = 0; kk < kp; kk += 4) {
       const int x = kk + (L.l >> 4);
-      const T ag = (x < g.n0) ? nm.AGt[(size_t)x * nm.ldi + ia] : (T)0;
-      C0 = MM::mma(ag, B.at(0, x, L.c), C0);
-      C1 = MM::mma(ag, B.at(1, x, L.c), C1);
-      C2 = MM::mma(ag, B.at(2, x, L.c), C2);
+      T a0 = 0, a12 = 0;
+      if (x < g.n0) {
+        if (ia < mm.ld0) a0 = mm.mA0[(size_t)x * mm.ld0 + ia];
+        if (ia < mm.ld12) a12 = mm.mA12[(size_t)x * mm.ld12 + ia];
+      }
+      C0 = MM::mma(a0, B.at(0, x, L.c), C0);
+      C1 = MM::mma(a12, B.at(1, x, L.c), C1);
+      C2 = MM::mma(a12, B.at(2, x, L.c), C2);
     }
-    const double W = g.W;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const int i = i0 + MM::row(L.l, r);
-      if (!L.valid || i >= g.nd) continue;
+      if (!L.valid) continue;
       const size_t o = (size_t)i * 2 * M + 2 * L.mcol + L.comp;
-      NAr[o] = W * (double)C0[r];
-      NAr[2 * cstride + o] = W * (double)C1[r];
-      NAr[4 * cstride + o] = W * (double)C2[r];
+      if (i < mm.n0r) NAr[o] = mm.s0 * (double)C0[r];
+      if (i < mm.n12r) {
+        NAr[2 * cstride + o] = mm.s12 * (double)C1[r];
+        NAr[4 * cstride + o] = mm.s12 * (double)C2[r];
+      }
     }
   }
 }
@@ -664,22 +676,26 @@ hipError_t launch_rhs_mfma(const Geo &g, const NodalMats<T> &nm, Workspace &ws, 
 }
 
 template <typename T>
-hipError_t launch_nodal_mhd_mfma(const Geo &g, const NodalMats<T> &nm, const double2 *A, size_t cstride,
+hipError_t launch_nodal_mhd_mfma(const Geo &g, const MhdMats<T> &mm, const double2 *A, size_t cstride,
                                  Workspace &ws, hipStream_t s) {
   using V = typename VT<T>::V;
-  if (!nm.VQ1t || !nm.VQ2t) return hipErrorInvalidValue;
-  if (!stage_covers(g.n0, STAGE_NODAL, 3 * ((g.nd + 3) / 4 * 4) * 8)) return hipErrorInvalidValue;
-  hipLaunchKernelGGL((k_nodal_mhd_mfma<T>), mfma_grid(g), mfma_block(g.n0), panel_bytes<T>(3, g.nd), s, g,
-                     nm, A, cstride, reinterpret_cast<V *>(ws.X));
+  if (!mm.mDX || !mm.mQX || !mm.mDY || !mm.mEY || !mm.mRY) return hipErrorInvalidValue;
+  if (mm.kx > mm.kyz || mm.kyz > g.n0 || mm.ldx < g.n0 || mm.ldx % 16) return hipErrorInvalidValue;
+  if (!stage_covers(g.n0, STAGE_NODAL, 3 * ((mm.kyz + 3) / 4 * 4) * 8)) return hipErrorInvalidValue;
+  hipLaunchKernelGGL((k_nodal_mhd_mfma<T>), mfma_grid(g), mfma_block(g.n0), panel_bytes<T>(3, mm.kyz), s, g,
+                     mm, A, cstride, reinterpret_cast<V *>(ws.X));
   return hipGetLastError();
 }
 
 template <typename T>
-hipError_t launch_rhs_a_mfma(const Geo &g, const NodalMats<T> &nm, Workspace &ws, double2 *NA, size_t cstride,
+hipError_t launch_rhs_a_mfma(const Geo &g, const MhdMats<T> &mm, Workspace &ws, double2 *NA, size_t cstride,
                              hipStream_t s) {
   using V = typename VT<T>::V;
-  if (!stage_covers(g.nd, STAGE_RHS, ((g.n0 + 3) / 4 * 4) * 8)) return hipErrorInvalidValue;
-  hipLaunchKernelGGL((k_rhs_a_mfma<T>), mfma_grid(g), mfma_block(g.nd), panel_bytes<T>(3, g.n0), s, g, nm,
+  const int rows = std::max(mm.n0r, mm.n12r);
+  if (!mm.mA0 || !mm.mA12 || mm.n0r > mm.ld0 || mm.n12r > mm.ld12) return hipErrorInvalidValue;
+  if ((size_t)rows * g.m > cstride) return hipErrorInvalidValue;
+  if (!stage_covers(rows, STAGE_RHS, ((g.n0 + 3) / 4 * 4) * 8)) return hipErrorInvalidValue;
+  hipLaunchKernelGGL((k_rhs_a_mfma<T>), mfma_grid(g), mfma_block(rows), panel_bytes<T>(3, g.n0), s, g, mm,
                      reinterpret_cast<const V *>(ws.Hx), NA, cstride);
   return hipGetLastError();
 }
@@ -726,9 +742,9 @@ hipError_t mfma_init(const Geo &g, bool fp64) {
                                                double2 *, double2 *, double2 *, hipStream_t);       \
   template hipError_t launch_rhs_adj_mfma<T>(const Geo &, const AdjMats<T> &, Prim, Workspace &,    \
                                              hipStream_t);                                          \
-  template hipError_t launch_nodal_mhd_mfma<T>(const Geo &, const NodalMats<T> &, const double2 *,  \
+  template hipError_t launch_nodal_mhd_mfma<T>(const Geo &, const MhdMats<T> &, const double2 *,    \
                                                size_t, Workspace &, hipStream_t);                   \
-  template hipError_t launch_rhs_a_mfma<T>(const Geo &, const NodalMats<T> &, Workspace &, double2 *, \
+  template hipError_t launch_rhs_a_mfma<T>(const Geo &, const MhdMats<T> &, Workspace &, double2 *,   \
                                            size_t, hipStream_t);
 PCF_MFMA_INSTANTIATE(double)
 PCF_MFMA_INSTANTIATE(float)

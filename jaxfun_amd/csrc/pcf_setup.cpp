@@ -156,6 +156,7 @@ std::string build_operators(int n0, Operators &ops) {
   if (!solve(C0D, matmul(HSD, matmul(D, SDt)), Q1)) return "singular Dirichlet mass matrix";
   if (!solve(C0D, matmul(HSD, matmul(D2, SDt)), Q2)) return "singular Dirichlet mass matrix";
   LMat VQ1 = matmul(VD, Q1), VQ2 = matmul(VD, Q2);
+  LMat VC1 = matmul(T, D), VC2 = matmul(T, D2);
 
   ops.C0B = to_double(C0B);
   ops.C2B = to_double(C2B);
@@ -174,6 +175,10 @@ std::string build_operators(int n0, Operators &ops) {
   ops.FD = to_double(FD);
   ops.VQ1 = to_double(VQ1);
   ops.VQ2 = to_double(VQ2);
+  ops.VC = to_double(T);
+  ops.VC1 = to_double(VC1);
+  ops.VC2 = to_double(VC2);
+  ops.WQ = to_double(Wq);
 
   // Structured form of the parity blocks.  Far-upper factors from
   //   (T_a, T_b^(2)) = (pi/2)  b (b^2 - a^2)
@@ -305,6 +310,10 @@ const Mat *operator_by_index(const Operators &ops, int which) {
     case 14: return &ops.FD;
     case 15: return &ops.VQ1;
     case 16: return &ops.VQ2;
+    case 17: return &ops.VC;
+    case 18: return &ops.VC1;
+    case 19: return &ops.VC2;
+    case 20: return &ops.WQ;
     default: return nullptr;
   }
 }

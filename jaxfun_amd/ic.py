@@ -113,3 +113,69 @@ def seeded_state(solver, seed: int, energy: float, K: int):
     s = math.sqrt(target / e_raw)
     u0, u1, u2, g = u0 * s, u1 * s, u2 * s, g * s
     return solver._with_zero_history((u0, u1, u2), g)
+
+
+def magnetic_energy(solver, A) -> float:
+    """int |b|^2 dV of b = curl A from spectral coefficients (host setup)."""
+    VD, VD1 = solver.matrix("VD"), solver.matrix("VD1")
+    if A[1].shape[0] == solver.N[0]:  # Ay, Az in TC
+        VY, VY1 = solver.matrix("VC"), solver.matrix("VC1")
+    else:
+        VY, VY1 = VD, VD1
+    ky, kz = solver.wavenumbers()
+    ev = lambda V, c: np.tensordot(V, c, axes=(1, 0))  # noqa: E731
+    DX, DY, DZ, EY, EZ = ev(VD, A[0]), ev(VY, A[1]), ev(VY, A[2]), ev(VY1, A[1]), ev(VY1, A[2])
+    b = (1j * ky * DZ - 1j * kz * DY, 1j * kz * DX - EZ, EY - 1j * ky * DX)
+    w = fejer_weights(solver.N[0])
+    return sum(solver.Ly * solver.Lz * float(np.sum(w[:, None, None] * np.abs(c) ** 2)) for c in b)
+
+
+def seeded_mhd_state(solver, seed: int, energy_u: float, energy_b: float, K: int):
+    """Seeded MHD state: the velocity of ``seeded_state(solver, seed, energy_u, K)``
+    and a vector potential drawn from ``numpy.random.default_rng([seed, 1])``
+    (standard-normal real and imaginary parts, components x, y, z in turn, for
+    wall-normal index < K at |ky| < K, 0 <= kz < K; kz = 0 plane Hermitian,
+    Nyquist planes and the (0, 0) mode zero).  For insulating walls the
+    tangential potentials are made to satisfy the vacuum-matching rows: per mode
+    and parity, the highest drawn coefficient of P = cy Ay + cz Az and
+    Q = cz Ay - cy Az is set so that sum_j beta_j x_j = 0 (beta_j = j^2 for P,
+    j^2 + k for Q).  B = curl A is solenoidal to round-off; A is scaled so that
+    (1/V) int |b|^2 / 2 dV equals ``energy_b``."""
+    flow = seeded_state(solver, seed, energy_u, K)
+    n0, ny, nz = solver.N
+    rows = [solver._a_rows(i) for i in range(3)]
+    if K > min(ny, nz) // 2 or K > min(rows):
+        raise ValueError("K exceeds the resolved modes")
+    rng = np.random.default_rng([seed, 1])
+    shp = (K, 2 * K - 1, K)
+    kzs = nz // 2
+    kys = np.arange(-(K - 1), K)
+    A = []
+    for c in range(3):
+        re, im = rng.standard_normal(shp), rng.standard_normal(shp)
+        h = np.zeros((rows[c], ny, kzs), complex)
+        for j, kyv in enumerate(kys):
+            h[:K, kyv % ny, :K] = re[:, j, :] + 1j * im[:, j, :]
+        h[:, 0, 0] = 0
+        A.append(h)
+    if rows[1] == n0:  # insulating walls
+        ky, kz = solver.wavenumbers()
+        kyh, kzh = np.broadcast_to(ky, (ny, nz))[:, :kzs], np.broadcast_to(kz, (ny, nz))[:, :kzs]
+        k = np.sqrt(kyh**2 + kzh**2)
+        kn = np.where(k > 0, k, 1.0)
+        cy, cz = np.where(k > 0, kyh / kn, 1.0), np.where(k > 0, kzh / kn, 0.0)
+        P = cy * A[1] + cz * A[2]
+        Q = cz * A[1] - cy * A[2]
+        j = np.arange(n0)[:, None, None]
+        for X, beta in ((P, j * j + 0.0 * k), (Q, j * j + k)):
+            for par in (0, 1):
+                top = (K - 1 - par) // 2 * 2 + par  # highest drawn row of this parity
+                sel = np.arange(par, top, 2)
+                X[top] = -np.sum(beta[sel] * X[sel], axis=0) / beta[top]
+        A[1] = cy * P + cz * Q
+        A[2] = cz * P - cy * Q
+    A = [solver.mask_nyquist(_full_from_half(_hermitian_kz0(h), nz)) for h in A]
+    e_raw = magnetic_energy(solver, A)
+    volume = 2.0 * solver.Ly * solver.Lz
+    s = math.sqrt(2.0 * volume * energy_b / e_raw)
+    return solver._new_state(flow, [a * s for a in A])

@@ -79,7 +79,7 @@ class PlaneCouetteMHDHIP(PlaneCouetteFluctuationHIP):
 
     # ------------------------------------------------------------ states
     def _zero_A(self):
-        return tuple(np.zeros((self.nd, self.N[1], self.N[2]), complex) for _ in range(3))
+        return tuple(np.zeros((self._a_rows(i), self.N[1], self.N[2]), complex) for i in range(3))
 
     def _new_state(self, flow: KMMState, A) -> MHDState:
         """``_new_mhd_state`` / ``_ensure_mhd_history`` (``pcf_mhd_jax.py:217-251``):
@@ -121,11 +121,16 @@ class PlaneCouetteMHDHIP(PlaneCouetteFluctuationHIP):
                 self._set(f"A{comp}" if not tag else f"{tag}{comp}", arr)
         self._loaded = None
 
+    def _a_rows(self, comp: int) -> int:
+        """Rows of A component comp (Ax in TD; Ay, Az in TD for conducting walls)."""
+        return self.nd
+
     def fetch(self) -> MHDState:
         flow = super().fetch()
         groups = {}
         for name, tag in _A_GROUPS:
-            groups[name] = tuple(self._get(f"A{c}" if not tag else f"{tag}{c}", self.nd) for c in "XYZ")
+            groups[name] = tuple(self._get(f"A{c}" if not tag else f"{tag}{c}", self._a_rows(i))
+                                 for i, c in enumerate("XYZ"))
         st = MHDState(flow=flow, **groups)
         self._loaded = st
         return st
@@ -135,10 +140,11 @@ class PlaneCouetteMHDHIP(PlaneCouetteFluctuationHIP):
         """``diagnostics`` (``pcf_mhd_jax.py:776-789``): the flow diagnostics plus
         ``divB_L2`` and ``Emag``."""
         out = super().diagnostics(state)
-        m = np.zeros(2)
+        m = np.zeros(4)
         nat.check(self._lib.pcf_mhd_diagnostics(self._h, m.ctypes.data_as(C.POINTER(C.c_double))))
         out["divB_L2"] = float(m[1])
         out["Emag"] = float(m[0])
+        out["Emag_total"] = float(m[2])
         return out
 
     def vjp(self, *args, **kwargs):
@@ -151,6 +157,126 @@ class PlaneCouetteMHDHIP(PlaneCouetteFluctuationHIP):
     def benchmark_state_fields(state: MHDState):
         """``benchmark_state_fields`` (``pcf_mhd_jax.py:754-758``)."""
         return (*state.flow.u, state.flow.g, *state.A)
+
+
+class PlaneCouetteMRIShearpyInsulatingHIP(PlaneCouetteMHDHIP):
+    """Rotating-shear plane-Couette MHD with insulating (vacuum-matched) walls.
+
+    Mirrors ``PlaneCouetteMRIShearpyInsulatingJax``
+    (``examples/pcf_mhd_mri_shearpy_jax.py:30-99,516-836``): base flow
+    U_b = -S x, Coriolis force of rotation rate ``omega``, uniform background
+    field ``background_b``, A_x in TD and A_y, A_z in the orthogonal Chebyshev
+    space TC with per-mode vacuum-matching wall rows.  Viscosity 1/Re and
+    magnetic diffusivity 1/Rm (U_wall = 1 as in the reference).
+    """
+
+    def __init__(self, N=(17, 16, 16), domain=((-1.0, 1.0), (0.0, 4.0 * math.pi), (0.0, 2.0 * math.pi)),
+                 Re: float = 400.0, Rm: float | None = None, omega: float = 1.0, shear_rate: float = 1.0,
+                 background_b=(0.0, 0.0, 0.025), dt: float = 0.01, family: str = "C",
+                 padding_factor=(1.0, 1.5, 1.5), perturbation_amplitude: float = 0.05,
+                 magnetic_amplitude: float = 0.0, magnetic_seed: str = "ax_yz",
+                 solenoidal_velocity_seed: bool = False, time_integrator: str | None = "SBDF3",
+                 nonlinear_form: str = "rotational", coefficient_path: str = "optimized",
+                 solve_batching: str = "batched", precision: str = "fp64", device: int = 0) -> None:
+        self.omega = float(omega)
+        self.shear_rate = float(shear_rate)
+        self.magnetic_seed = str(magnetic_seed)
+        self.solenoidal_velocity_seed = bool(solenoidal_velocity_seed)
+        if self.magnetic_seed not in {"ax_yz", "sinusoidal_bz_x"}:
+            raise ValueError("magnetic_seed must be one of {ax_yz, sinusoidal_bz_x}")
+        if tuple(float(v) for v in domain[0]) != (-1.0, 1.0):
+            raise ValueError("wall-normal domain must be (-1, 1)")
+        super().__init__(N=N, domain=domain, Re=Re, Rm=Rm, U_wall=1.0, dt=dt, family=family,
+                         padding_factor=padding_factor, perturbation_amplitude=perturbation_amplitude,
+                         magnetic_amplitude=magnetic_amplitude, time_integrator=time_integrator,
+                         nonlinear_form=nonlinear_form, coefficient_path=coefficient_path,
+                         solve_batching=solve_batching, background_b=background_b, precision=precision,
+                         device=device)
+        self.q_shear = self.shear_rate / self.omega if self.omega != 0 else math.inf
+        self.kappa2 = 2.0 * self.omega * (2.0 * self.omega - self.shear_rate)
+
+    def _configure(self, cfg) -> None:
+        super()._configure(cfg)
+        cfg.walls = nat.WALLS_INSULATING
+        cfg.omega = self.omega
+        cfg.u_wall = -self.shear_rate  # base flow U_b = -S x (shearpy :89-92)
+
+    def _a_rows(self, comp: int) -> int:
+        return self.nd if comp == 0 else self.N[0]
+
+    def initial_state(self) -> MHDState:
+        """``initial_state`` (``examples/pcf_mhd_mri_shearpy_jax.py:99-147``): the
+        channel-mode velocity seed and the ``ax_yz`` or ``sinusoidal_bz_x`` field,
+        projected with the library's Galerkin matrices (every term separable)."""
+        n0, n1, n2 = self.N
+        xn = np.cos(np.pi + (2 * np.arange(n0) + 1) * np.pi / (2 * n0))
+        wall = 1.0 - xn**2
+        FB, FD, WQ = self.matrix("FB"), self.matrix("FD"), self.matrix("WQ")
+        h = np.full(n0, np.pi / 2)
+        h[0] = np.pi
+        amp = self.perturbation_amplitude
+
+        def four(kind_y, my, kind_z, mz):
+            cy = np.zeros(n1, complex)
+            cz = np.zeros(n2, complex)
+            for c, kind, mm in ((cy, kind_y, my), (cz, kind_z, mz)):
+                if kind == "one":
+                    c[0] = 1.0
+                elif kind == "sin":
+                    c[mm], c[-mm] = 1 / 2j, -1 / 2j
+                else:
+                    c[mm], c[-mm] = 0.5, 0.5
+            return np.outer(cy, cz)
+
+        def proj(F, terms):
+            out = 0
+            for prof, fc in terms:
+                out = out + np.einsum("i,jk->ijk", F @ prof, fc)
+            return self.mask_nyquist(out)
+
+        t0, t1, t2 = [], [], []
+        kz1 = 2.0 * math.pi / self.Lz
+        if amp != 0.0:
+            if self.solenoidal_velocity_seed:
+                wall_gradient = -4.0 * xn * (1.0 - xn**2)
+                for hm in (1, 2, 3):
+                    t0.append((amp * wall**2, four("one", 0, "cos", hm)))
+                    t1.append((amp * wall**2, four("one", 0, "sin", hm)))
+                    t2.append((-(amp * wall_gradient / (hm * kz1)), four("one", 0, "sin", hm)))
+            else:
+                for hm in (1, 2, 3):
+                    t0.append((amp * wall, four("one", 0, "cos", hm)))
+                    t1.append((amp * wall, four("one", 0, "sin", hm)))
+                t0.append((0.1 * amp * wall, four("sin", 1, "cos", 1)))
+                t1.append((0.1 * amp * wall, four("cos", 1, "sin", 1)))
+                t2.append((0.1 * amp * wall, four("sin", 2, "cos", 2)))
+        z = np.zeros((n0, n1, n2), complex)
+        u0 = proj(FB, t0) if t0 else z[:-4].copy()
+        u1 = proj(FD, t1) if t1 else z[:-2].copy()
+        u2 = proj(FD, t2) if t2 else z[:-2].copy()
+        ky, kz = self.wavenumbers()
+        g = self.mask_nyquist(1j * ky[None] * u2 - 1j * kz[None] * u1)
+        flow = KMMState(u=(u0, u1, u2), g=g)
+        A = list(self._zero_A())
+        mag = self.magnetic_amplitude
+        if mag != 0.0 and self.magnetic_seed == "ax_yz":
+            A[0] = proj(FD, [(mag * wall / kz1, four("sin", 1, "sin", 1))])
+        elif mag != 0.0:
+            # Ay = -(mag h / pi)(cos(pi x) + 1) on the TC basis (h = 1)
+            prof = -(mag / np.pi) * (np.cos(np.pi * xn) + 1.0)
+            A[1] = proj(np.diag(1.0 / h) @ WQ, [(prof, four("one", 0, "one", 0))])
+        return self._new_state(flow, A)
+
+    def diagnostics(self, state: MHDState | None = None) -> dict:
+        """Flow and magnetic diagnostics plus ``insulating_bc_residual``
+        (``examples/pcf_mhd_mri_shearpy_jax.py:802-836``)."""
+        out = super().diagnostics(state)
+        m = np.zeros(4)
+        nat.check(self._lib.pcf_mhd_diagnostics(self._h, m.ctypes.data_as(C.POINTER(C.c_double))))
+        out["insulating_bc_residual"] = float(m[3])
+        out["q_shear"] = self.q_shear
+        out["kappa2"] = self.kappa2
+        return out
 
 
 def mhd_state_leaves(state: MHDState) -> list:

@@ -154,8 +154,8 @@ class OracleAdjoint:
         w0 = o.tp_backward(u2, SD, k1=1) - o.tp_backward(u1, SD, k2=1)
         w1 = o.tp_backward(u0, SB, k2=1) - o.tp_backward(u2, SD, kx=1)
         w2 = o.tp_backward(u1, SD, kx=1) - o.tp_backward(u0, SB, k1=1)
-        a = (up[0], up[1] + o.U_wall * o.xb, up[2])
-        b = (w0, w1, w2 + o.U_wall)
+        a = (up[0], up[1] + o.dUb * o.xb, up[2])
+        b = (w0, w1, w2 + o.dUb)
         mc = tuple(-self.tp_forward_D_adj(o.mask(m)) for m in mu_h)
         ma = _cross(tuple(np.conj(v) for v in b), mc)
         mb = _cross(mc, tuple(np.conj(v) for v in a))

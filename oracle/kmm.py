@@ -311,6 +311,9 @@ class PCFOracle:
         self.Re = float(Re)
         self.U_wall = float(U_wall)
         self.nu = self.U_wall / self.Re
+        # slope of the laminar base flow U_b = dUb * x (pcf_fluctuations_jax.py:73-75;
+        # -shear_rate in examples/pcf_mhd_mri_shearpy_jax.py:89-92)
+        self.dUb = self.U_wall
         self.dt = float(dt)
         self.perturbation_amplitude = float(perturbation_amplitude)
         N0, N1, N2 = self.N
@@ -504,8 +507,8 @@ class PCFOracle:
         w0 = self.tp_backward(u2, SD, k1=1) - self.tp_backward(u1, SD, k2=1)
         w1 = self.tp_backward(u0, SB, k2=1) - self.tp_backward(u2, SD, kx=1)
         w2 = self.tp_backward(u1, SD, kx=1) - self.tp_backward(u0, SB, k1=1)
-        a = (up[0], up[1] + self.U_wall * self.xb, up[2])
-        b = (w0, w1, w2 + self.U_wall)
+        a = (up[0], up[1] + self.dUb * self.xb, up[2])
+        b = (w0, w1, w2 + self.dUb)
         cross = (a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0])
         return tuple(-c for c in cross), up
 
@@ -665,7 +668,7 @@ class PCFOracle:
         u0, u1, u2 = s.u
         up = (self.tp_backward(u0, self.SB, padded=False), self.tp_backward(u1, self.SD, padded=False),
               self.tp_backward(u2, self.SD, padded=False))
-        ut = (up[0], up[1] + self.U_wall * self.xb, up[2])
+        ut = (up[0], up[1] + self.dUb * self.xb, up[2])
         epert = sum(np.real(self.integrate(np.conj(v) * v)) for v in up)
         etot = sum(np.real(self.integrate(np.conj(v) * v)) for v in ut)
         divu = (self.tp_backward(u0, self.SB, padded=False, kx=1) + self.tp_backward(u1, self.SD, padded=False, k1=1)
@@ -677,17 +680,17 @@ class PCFOracle:
         i = np.arange(self.ND)
         d_top = i**2 - (i + 2) ** 2
         d_bot = ((-1.0) ** (i + 1)) * i**2 - ((-1.0) ** (i + 3)) * (i + 2) ** 2
-        top = np.real(np.dot(d_top, u1[:, 0, 0])) + self.U_wall
-        bot = np.real(np.dot(d_bot, u1[:, 0, 0])) + self.U_wall
+        top = np.real(np.dot(d_top, u1[:, 0, 0])) + self.dUb
+        bot = np.real(np.dot(d_bot, u1[:, 0, 0])) + self.dUb
         return {
             "Epert": float(epert),
             "Etot": float(etot),
             "divL2": float(divl2),
-            "u_top": self.U_wall * 1.0,
-            "u_bot": -self.U_wall * 1.0,
+            "u_top": self.dUb * 1.0,
+            "u_bot": -self.dUb * 1.0,
             "wall_shear_top": float(top),
             "wall_shear_bot": float(bot),
-            "mean_shear": float(np.real(self.integrate(dvdx + self.U_wall)) / volume),
+            "mean_shear": float(np.real(self.integrate(dvdx + self.dUb)) / volume),
         }
 
     @staticmethod

@@ -85,6 +85,7 @@ class MHDOracle(PCFOracle):
                 self._A_factors[key] = _nopivot_lu(self.MA - gamma * self.LA)
             return self._A_factors[key]
 
+        self._A_fac = fac
         self.SA = fac(self.dt * SBDF3_GAMMA)
         self.SA_startup = tuple(fac((IMEXRK3_A[rk] + IMEXRK3_B[rk]) * self.dt / 2.0) for rk in range(3))
 
@@ -99,6 +100,27 @@ class MHDOracle(PCFOracle):
     def A_solve(self, fac, rhs):
         """``_A_solve`` (``pcf_mhd_jax.py:172-178``): per-mode solve, Nyquist mask."""
         return tuple(self.mask(self._unmodes(_lu_solve(fac, self._modes(r)))) for r in rhs)
+
+    # hooks of the A subsystem (``pcf_mhd_jax.py:143-178``), overridden for other walls
+    def A_mass(self, A):
+        """``_A_mass_rhs``: mass rows of each component."""
+        return tuple(self.MA_apply(a) for a in A)
+
+    def A_lap(self, A):
+        """``_A_eta_lap``: eta-Laplacian rows of each component."""
+        return tuple(self.LA_apply(a) for a in A)
+
+    def A_solve_gamma(self, gamma, rhs):
+        """Solve (M_A - gamma L_A) A = rhs."""
+        return self.A_solve(self._A_fac(gamma), rhs)
+
+    def forward_emf(self, emf):
+        """``_A_forward_emf``: TD forward transform of each EMF component."""
+        return tuple(self.mask(self.tp_forward_D(e)) for e in emf)
+
+    def flow_forcing(self, n, up):
+        """Extra physical flow forcing before the Lorentz force (none here)."""
+        return n
 
     # ------------------------------------------------------------ projections
     def project_D(self, c):
@@ -162,20 +184,20 @@ class MHDOracle(PCFOracle):
     def mhd_convection(self, s: MHDOracleState):
         """``_mhd_convection`` (``pcf_mhd_jax.py:339-357``): (H, HA)."""
         n, up = self.convection_physical(s.flow)
+        n = self.flow_forcing(n, up)
         B = self.B_from_A(s.A)
         J = self.J_from_B(B)
         bp = tuple(b + b0 for b, b0 in zip(self.backward_B(B), self.background_b))
         jp = self.backward_J(J)
         lor = (jp[1] * bp[2] - jp[2] * bp[1], jp[2] * bp[0] - jp[0] * bp[2], jp[0] * bp[1] - jp[1] * bp[0])
         H = tuple(self.mask(self.tp_forward_D(a - b)) for a, b in zip(n, lor))
-        ut = (up[0], up[1] + self.U_wall * self.xb, up[2])
+        ut = (up[0], up[1] + self.dUb * self.xb, up[2])
         emf = (ut[1] * bp[2] - ut[2] * bp[1], ut[2] * bp[0] - ut[0] * bp[2], ut[0] * bp[1] - ut[1] * bp[0])
-        HA = tuple(self.mask(self.tp_forward_D(e)) for e in emf)
-        return H, HA
+        return H, self.forward_emf(emf)
 
     def nonlinear_A(self, HA):
         """``_A_mass_rhs`` of HA (``pcf_mhd_jax.py:157-159``)."""
-        return tuple(self.MA_apply(h) for h in HA)
+        return self.A_mass(HA)
 
     def step_imexrk3_mhd(self, s: MHDOracleState) -> MHDOracleState:
         """``_step_imexrk3_mhd`` (``pcf_mhd_jax.py:488-561``) with the start-up factors."""
@@ -195,12 +217,12 @@ class MHDOracle(PCFOracle):
             rhs_g = self.Mg_apply(g_stage) + gamma * self.Lg_apply(g_stage) + dt * (a * cur[1] + b * prev[1])
             rhs_v = self.M00 @ v00 + gamma * (self.L00 @ v00) + dt * (a * cur[2] + b * prev[2])
             rhs_w = self.M00 @ w00 + gamma * (self.L00 @ w00) + dt * (a * cur[3] + b * prev[3])
-            rhs_A = [self.MA_apply(A_stage[i]) + gamma * self.LA_apply(A_stage[i])
-                     + dt * (a * curA[i] + b * prevA[i]) for i in range(3)]
+            mA, lA = self.A_mass(A_stage), self.A_lap(A_stage)
+            rhs_A = [mA[i] + gamma * lA[i] + dt * (a * curA[i] + b * prevA[i]) for i in range(3)]
             u0n, gn, vn, wn = self._solve(self.startup[rk], rhs_u, rhs_g, rhs_v, rhs_w)
             u_stage = self.reconstruct(u0n, gn, vn, wn)
             g_stage = gn
-            A_stage = self.A_solve(self.SA_startup[rk], rhs_A)
+            A_stage = self.A_solve_gamma(gamma, rhs_A)
             prev, prevA = cur, curA
         return replace(s, flow=replace(s.flow, u=u_stage, g=g_stage), A=A_stage)
 
@@ -219,14 +241,14 @@ class MHDOracle(PCFOracle):
             rhs = tuple(
                 (18.0 * a - 9.0 * b + 2.0 * c) / 11.0 + self.dt * ((18.0 * n0 - 18.0 * n1 + 6.0 * n2) / 11.0)
                 for a, b, c, n0, n1, n2 in zip(m0, m1, m2, cur_n, f.nonlinear_old, f.nonlinear_older))
+            a0, a1, a2 = self.A_mass(s.A), self.A_mass(s.A_old), self.A_mass(s.A_older)
             rhs_A = [
-                (18.0 * self.MA_apply(s.A[i]) - 9.0 * self.MA_apply(s.A_old[i]) + 2.0 * self.MA_apply(s.A_older[i]))
-                / 11.0 + self.dt * ((18.0 * cur_nA[i] - 18.0 * s.nonlinear_A_old[i] + 6.0 * s.nonlinear_A_older[i])
-                                    / 11.0)
+                (18.0 * a0[i] - 9.0 * a1[i] + 2.0 * a2[i]) / 11.0
+                + self.dt * ((18.0 * cur_nA[i] - 18.0 * s.nonlinear_A_old[i] + 6.0 * s.nonlinear_A_older[i]) / 11.0)
                 for i in range(3)]
             u0n, gn, vn, wn = self._solve(self.steady, *rhs)
             adv = replace(s, flow=replace(f, u=self.reconstruct(u0n, gn, vn, wn), g=gn),
-                          A=self.A_solve(self.SA, rhs_A))
+                          A=self.A_solve_gamma(self.dt * SBDF3_GAMMA, rhs_A))
             hist = 2.0
         flow = replace(adv.flow, nonlinear_old=cur_n, nonlinear_older=f.nonlinear_old, solution_old=cur_p,
                        solution_older=f.solution_old, history_steps=hist, have_old=1.0, previous_dt=self.dt)
@@ -245,10 +267,14 @@ class MHDOracle(PCFOracle):
         B = self.B_from_A(s.A)
         bp = self.backward_B(B, padded=False)
         out["Emag"] = float(sum(np.real(self.integrate(np.conj(b) * b)) for b in bp))
-        divb = (self.tp_backward(B[0], self.SD, padded=False, kx=1) + self.tp_backward(B[1], None, padded=False, k1=1)
-                + self.tp_backward(B[2], None, padded=False, k2=1))
+        divb = self.divB(B)
         out["divB_L2"] = float(np.sqrt(np.real(self.integrate(np.conj(divb) * divb))))
         return out
+
+    def divB(self, B):
+        """div B on the unpadded grid, B in (TD, TC, TC) (``pcf_mhd_jax.py:760-768``)."""
+        return (self.tp_backward(B[0], self.SD, padded=False, kx=1) + self.tp_backward(B[1], None, padded=False, k1=1)
+                + self.tp_backward(B[2], None, padded=False, k2=1))
 
 
 __all__ = ["MHDOracle", "MHDOracleState"]

@@ -100,3 +100,13 @@ def mhd_oracle_for(solver):
                      dt=solver.dt, padding_factor=solver.padding_factor,
                      perturbation_amplitude=solver.perturbation_amplitude,
                      magnetic_amplitude=solver.magnetic_amplitude, background_b=solver.background_b)
+
+
+def insulating_oracle_for(solver):
+    from oracle.mhd_insulating import InsulatingMHDOracle
+
+    return InsulatingMHDOracle(N=solver.N, domain=solver.domain, Re=solver.Re, Rm=solver.Rm, omega=solver.omega,
+                               shear_rate=solver.shear_rate, background_b=solver.background_b, dt=solver.dt,
+                               padding_factor=solver.padding_factor,
+                               perturbation_amplitude=solver.perturbation_amplitude,
+                               magnetic_amplitude=solver.magnetic_amplitude)

@@ -110,3 +110,72 @@ def test_mhd_rejects_reverse_mode():
     solver = make((17, 16, 16))
     with pytest.raises(NotImplementedError):
         solver.value_and_grad(solver.initial_state(), 2)
+
+
+# ---------------------------------------------------------------- insulating walls
+def make_insulating(N, precision="fp64", **kw):
+    from jaxfun_amd import PlaneCouetteMRIShearpyInsulatingHIP
+
+    args = dict(Re=400.0, Rm=300.0, omega=0.4, shear_rate=1.5, background_b=(0.0, 0.02, 0.1), dt=2e-3,
+                perturbation_amplitude=0.05, magnetic_amplitude=0.3)
+    args.update(kw)
+    return PlaneCouetteMRIShearpyInsulatingHIP(N=N, precision=precision, **args)
+
+
+def test_insulating_initial_state_matches_oracle():
+    from tests.helpers import insulating_oracle_for
+
+    solver = make_insulating((17, 16, 16))
+    err = mhd_max_rel_errors(solver.initial_state(), insulating_oracle_for(solver).initial_state())
+    assert max(err.values()) < 1e-13, err
+
+
+@pytest.mark.parametrize("N,kw", [((17, 16, 16), {}), ((24, 16, 32), dict(omega=0.0, shear_rate=-1.0)),
+                                  ((33, 32, 32), dict(Re=1000.0, Rm=1000.0, omega=0.0, shear_rate=-1.0,
+                                                      background_b=(0.0, 0.0, 0.1), dt=5e-3))],
+                         ids=["rot-shear", "couette", "config3-like"])
+def test_insulating_steps_match_oracle(N, kw):
+    from jaxfun_amd.ic import seeded_mhd_state
+    from tests.helpers import insulating_oracle_for
+
+    solver = make_insulating(N, **kw)
+    oracle = insulating_oracle_for(solver)
+    s = seeded_mhd_state(solver, seed=3, energy_u=1e-3, energy_b=1e-3, K=4)
+    so = to_mhd_oracle(s)
+    assert oracle.insulating_bc_residual(so) < 1e-13  # the seed meets the wall rows
+    done = 0
+    for k in (1, 2, 3, 6):
+        s = solver.solve(s, k - done)
+        so = oracle.solve(so, k - done)
+        done = k
+        err = mhd_max_rel_errors(s, so)
+        print(N, kw, k, max(err.values()), flush=True)
+        assert_parity(err, (N, k))
+    d, do = solver.diagnostics(s), oracle.diagnostics(so)
+    for key in ("Epert", "Etot", "Emag", "Emag_total"):
+        assert math.isclose(d[key], do[key], rel_tol=1e-12), (key, d[key], do[key])
+    assert d["insulating_bc_residual"] < 1e-10 and do["insulating_bc_residual"] < 1e-10
+    assert d["divB_L2"] < 1e-12
+
+
+def test_insulating_graph_replay_and_mixed_precision():
+    from jaxfun_amd.ic import seeded_mhd_state
+
+    N = (33, 32, 32)
+    kw = dict(Re=1000.0, Rm=1000.0, omega=0.0, shear_rate=-1.0, background_b=(0.0, 0.0, 0.1), dt=5e-3)
+    a = make_insulating(N, **kw)
+    s0 = seeded_mhd_state(a, seed=0, energy_u=1e-3, energy_b=1e-3, K=8)
+    s = a.solve(s0, 3)
+    a.load(s)
+    a.advance_steady(24)
+    eager = a.fetch()
+    a.load(s)
+    a.advance_steady(24, graph_block=12)
+    assert max(mhd_max_rel_errors(a.fetch(), eager).values()) == 0.0
+    b = make_insulating(N, precision="mixed", **kw)
+    sa, sb = a.solve(s0, 30), b.solve(s0, 30)
+    for x, y in zip(list(sb.flow.u) + list(sb.A), list(sa.flow.u) + list(sa.A)):
+        rel = float(np.max(np.abs(x - y)) / np.max(np.abs(y)))
+        print("insulating mixed vs fp64", rel, flush=True)
+        assert rel < 2e-6
+    assert b.diagnostics(sb)["insulating_bc_residual"] < 1e-10
