@@ -13,7 +13,9 @@ Also reported on the same line:
     and its throughput ratio to FP64;
   * per-kernel mean durations from HIP events on the library's stream, and the
     roofline of the dominant kernel;
-  * the CPU FP64 oracle timed on this host's cores on a bounded sample.
+  * the CPU FP64 oracle timed on this host's cores on a bounded sample;
+  * the adjoint/forward cost ratio with its finite-difference check;
+  * the insulating-wall MHD step (FP64 and mixed) at the same resolution.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -318,6 +320,56 @@ def adjoint_fd_check(n_dirs: int = 8, eps: float = 1e-6) -> dict:
             "mixed_vs_fp64_grad_rel": gerr}
 
 
+MHD = dict(N=(65, 64, 64), Re=1000.0, Rm=1000.0, omega=0.0, shear_rate=-1.0, background_b=(0.0, 0.0, 0.1),
+           dt=0.005, energy_u=1e-3, energy_b=1e-3, K=12, seed=0, steps=500)
+
+
+def mhd_leg(steps: int, warmup: int) -> dict:
+    """Insulating-wall MHD (B0 = (0, 0, 0.1), Omega = 0, S = -1, Re = Rm = 1000,
+    dt = 0.005) at 65x64x64 from a seeded state: steady-step throughput in FP64
+    and mixed precision, and the mixed run's per-field deviation from FP64 with
+    the magnetic diagnostics after MHD['steps'] steps."""
+    from jaxfun_amd import PlaneCouetteMRIShearpyInsulatingHIP
+    from jaxfun_amd.ic import seeded_mhd_state
+
+    w = MHD
+    out = {"workload": "insulating-wall rotating-shear MHD, 65x64x64, Re=Rm=1000, B0=(0,0,0.1), Omega=0, S=-1, "
+                       "dt=0.005, E_u=E_b=1e-3 over the lowest 12 modes"}
+    finals = {}
+    for prec in ("fp64", "mixed"):
+        solver = PlaneCouetteMRIShearpyInsulatingHIP(N=w["N"], Re=w["Re"], Rm=w["Rm"], omega=w["omega"],
+                                                     shear_rate=w["shear_rate"], background_b=w["background_b"],
+                                                     dt=w["dt"], precision=prec)
+        s0 = seeded_mhd_state(solver, seed=w["seed"], energy_u=w["energy_u"], energy_b=w["energy_b"], K=w["K"])
+        solver.load(s0)
+        solver.advance(2)
+        solver.advance_steady(warmup, graph_block=GRAPH_BLOCK)
+        solver.synchronize()
+        t0 = time.perf_counter()
+        solver.advance_steady(steps, graph_block=GRAPH_BLOCK)
+        solver.synchronize()
+        el = time.perf_counter() - t0
+        stages = profile_stages(solver, 24)
+        out[prec] = {"value": steps / el, "unit": "steps/s", "ms_per_step": 1e3 * el / steps, "stages_ms": stages}
+        final = solver.solve(s0, w["steps"])
+        d = solver.diagnostics(final)
+        out[prec]["diagnostics_after"] = {k: d[k] for k in ("Epert", "Emag", "Emag_total", "divB_L2",
+                                                            "insulating_bc_residual")}
+        finals[prec] = final
+        solver.close()
+    import numpy as np
+
+    err = {}
+    a, b = finals["fp64"], finals["mixed"]
+    for name, x, y in zip(("u0", "u1", "u2", "g", "Ax", "Ay", "Az"), list(b.flow.u) + [b.flow.g] + list(b.A),
+                          list(a.flow.u) + [a.flow.g] + list(a.A)):
+        err[name] = float(np.max(np.abs(x - y)) / np.max(np.abs(y)))
+    err["steps"] = w["steps"]
+    out["mixed_error_vs_fp64"] = err
+    out["ratio_mixed_vs_fp64"] = out["mixed"]["value"] / out["fp64"]["value"]
+    return out
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -329,6 +381,7 @@ def main(argv=None) -> int:
     ap.add_argument("--error-steps", type=int, default=1000,
                     help="steps of the mixed-vs-FP64 per-field error check (0: skip)")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-mhd", action="store_true")
     ap.add_argument("--adjoint-only", action="store_true",
                     help="run only the adjoint timing leg (for kernel profiles)")
     ap.add_argument("--pcie-steps", type=int, default=240,
@@ -397,6 +450,12 @@ def main(argv=None) -> int:
         line["adjoint_forward_ratio"] = adj["mixed"]["ratio"]
         print(f"# adjoint/forward: mixed {adj['mixed']['ratio']:.3f}, fp64 {adj['fp64']['ratio']:.3f}; "
               f"FD max rel err {adj['fd_check']['max_rel_err']:.2e}", file=sys.stderr, flush=True)
+    if not args.no_mhd:
+        line["mhd"] = mhd_leg(min(steps, 1200), warmup)
+        print(f"# mhd (insulating): fp64 {line['mhd']['fp64']['value']:.1f}, mixed "
+              f"{line['mhd']['mixed']['value']:.1f} steps/s; mixed error "
+              f"{max(v for k, v in line['mhd']['mixed_error_vs_fp64'].items() if k != 'steps'):.2e}",
+              file=sys.stderr, flush=True)
     if args.copy_calibration:
         line["copy_calibration"] = copy_calibration(args.copy_calibration)
     if not args.no_cpu:

@@ -77,7 +77,7 @@ __device__ __forceinline__ void dft_small(V *a) {
 // BATCH_FAST: consecutive threads take consecutive batches (coalesced for
 // column-interleaved layouts); otherwise consecutive butterflies.
 template <typename V, typename T, int R, bool INV, bool BATCH_FAST, int MAXB>
-__device__ void fft_pass(V *buf, const V *tw, int n, int p, int nbatch, int bstride,
+__device__ __forceinline__ void fft_pass(V *buf, const V *tw, int n, int p, int nbatch, int bstride,
                          int estride, int tid, int nthr) {
   const int nbf = n / R;
   const int total = nbf * nbatch;
@@ -117,7 +117,7 @@ __device__ void fft_pass(V *buf, const V *tw, int n, int p, int nbatch, int bstr
 }
 
 template <typename V, typename T, bool INV, bool BATCH_FAST, int MAXB>
-__device__ void fft_lds(V *buf, const V *tw, const FFTPlan &plan, int nbatch, int bstride,
+__device__ __forceinline__ void fft_lds(V *buf, const V *tw, const FFTPlan &plan, int nbatch, int bstride,
                         int estride) {
   const int tid = threadIdx.x, nthr = blockDim.x;
   int p = 1;
@@ -201,7 +201,7 @@ __device__ __forceinline__ void fft_stages_c(V *buf, const V *tw, int nbatch, in
 }
 
 template <typename V, typename T, int N, bool INV, bool BATCH_FAST, int MAXB>
-__device__ void fft_lds_c(V *buf, const V *tw, int nbatch, int bstride, int estride) {
+__device__ __forceinline__ void fft_lds_c(V *buf, const V *tw, int nbatch, int bstride, int estride) {
   fft_stages_c<V, T, N, 0, 1, INV, BATCH_FAST, MAXB>(buf, tw, nbatch, bstride, estride, threadIdx.x,
                                                       blockDim.x);
 }

@@ -95,16 +95,113 @@ __global__ void __launch_bounds__(256) k_fft_y(Geo g, const typename VT<T>::V *_
 // c2r synthesis and the r2c analysis.
 // NZC > 0: the padded spanwise length is the compile-time NZC (the launcher
 // dispatches the common lengths); NZC = 0 reads it from g.
-// MHD: twelve input fields (u, w, b, j) and six outputs,
-// H = -(u_tot x w_tot) + (-2 Omega u1, 2 Omega u0, 0) - j x b_tot and
-// E = u_tot x b_tot with b_tot = b + B0 (examples/pcf_mhd_jax.py:339-357;
-// Coriolis force and background B0 of examples/pcf_mhd_mri_shearpy_jax.py:149-176).
-template <typename T, int MAXB, int NZC, bool MHD>
+template <typename T, int MAXB, int NZC>
 __global__ void __launch_bounds__(256) k_rotational(Geo g, NodalMats<T> nm, FFTPlan plan, int UPB,
                                                     const typename VT<T>::V *__restrict__ Y,
                                                     typename VT<T>::V *__restrict__ Hz) {
   using V = typename VT<T>::V;
-  constexpr int NIN = MHD ? 12 : 6, NOUT = MHD ? 6 : 3;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int MZ = NZC ? NZC : g.mz;
+  V *buf = reinterpret_cast<V *>(smem);  // [UPB][6][MZ]
+  V *out = buf + (size_t)UPB * 6 * MZ;   // [UPB][3][MZ]
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  const int KZ = g.kzs, MY = g.my;
+  const int nunits = g.n0 * (MY / 2);
+  const int ubase = blockIdx.x * UPB;
+  const size_t fstride = (size_t)g.n0 * MY * KZ;  // field stride in Y / Hz
+  V *tw = out + (size_t)UPB * 3 * g.mz;            // twiddle table in LDS
+  for (int i = tid; i < MZ; i += nthr) tw[i] = reinterpret_cast<const V *>(nm.twz)[i];
+
+  gather_batched<4>(
+      UPB * 6 * MZ,
+      [&](int idx) {
+        const int lu = idx / (6 * MZ), rem = idx - lu * 6 * MZ;
+        const int f = rem / MZ, k = rem - f * MZ;
+        const int u = ubase + lu;
+        V z{0, 0};
+        if (u < nunits) {
+          const int x = u / (MY / 2), y2 = u - x * (MY / 2);
+          const V *ra = Y + f * fstride + ((size_t)x * MY + 2 * y2) * KZ;
+          const V *rb = ra + KZ;
+          if (k == 0) {
+            z = V{ra[0].x, rb[0].x};
+          } else if (k < KZ) {
+            const V a = ra[k], b = rb[k];
+            z = V{a.x - b.y, a.y + b.x};
+          } else if (k > MZ - KZ) {
+            const V a = ra[MZ - k], b = rb[MZ - k];
+            z = V{a.x + b.y, b.x - a.y};
+          }
+        }
+        return z;
+      },
+      [&](int idx, const V &z) { buf[idx] = z; });
+  __syncthreads();
+  if constexpr (NZC > 0)
+    fft_lds_c<V, T, NZC, true, false, MAXB>(buf, tw, UPB * 6, MZ, 1);
+  else
+    fft_lds<V, T, true, false, MAXB>(buf, tw, plan, UPB * 6, MZ, 1);
+
+  const T U = (T)g.uwall;
+  for (int idx = tid; idx < UPB * MZ; idx += nthr) {
+    int lu = idx / MZ, n = idx - lu * MZ;
+    int u = ubase + lu;
+    int x = (u < nunits) ? u / (MY / 2) : 0;
+    const T xv = nm.xnode[x];
+    const V *b = buf + (size_t)lu * 6 * MZ + n;
+    V vu0 = b[0], vu1 = b[MZ], vu2 = b[2 * MZ], vw0 = b[3 * MZ], vw1 = b[4 * MZ], vw2 = b[5 * MZ];
+    V h0, h1, h2;
+    {
+      T a0 = vu0.x, a1 = vu1.x + U * xv, a2 = vu2.x, o0 = vw0.x, o1 = vw1.x, o2 = vw2.x + U;
+      h0.x = -(a1 * o2 - a2 * o1);
+      h1.x = -(a2 * o0 - a0 * o2);
+      h2.x = -(a0 * o1 - a1 * o0);
+    }
+    {
+      T a0 = vu0.y, a1 = vu1.y + U * xv, a2 = vu2.y, o0 = vw0.y, o1 = vw1.y, o2 = vw2.y + U;
+      h0.y = -(a1 * o2 - a2 * o1);
+      h1.y = -(a2 * o0 - a0 * o2);
+      h2.y = -(a0 * o1 - a1 * o0);
+    }
+    V *o = out + (size_t)lu * 3 * MZ + n;
+    o[0] = h0;
+    o[MZ] = h1;
+    o[2 * MZ] = h2;
+  }
+  __syncthreads();
+  if constexpr (NZC > 0)
+    fft_lds_c<V, T, NZC, false, false, MAXB>(out, tw, UPB * 3, MZ, 1);
+  else
+    fft_lds<V, T, false, false, MAXB>(out, tw, plan, UPB * 3, MZ, 1);
+
+  const T scale = (T)(1.0 / ((double)MY * (double)MZ));
+  for (int idx = tid; idx < UPB * 3 * KZ; idx += nthr) {
+    int lu = idx / (3 * KZ), rem = idx - lu * 3 * KZ;
+    int f = rem / KZ, k = rem - f * KZ;
+    int u = ubase + lu;
+    if (u >= nunits) continue;
+    int x = u / (MY / 2), y2 = u - x * (MY / 2);
+    const V *P = out + ((size_t)lu * 3 + f) * MZ;
+    V p = P[k], q = cconj(P[(MZ - k) % MZ]);
+    V ha{(p.x + q.x) * (T)0.5 * scale, (p.y + q.y) * (T)0.5 * scale};
+    V hb{(p.y - q.y) * (T)0.5 * scale, -(p.x - q.x) * (T)0.5 * scale};
+    V *dst = Hz + f * fstride + ((size_t)x * MY + 2 * y2) * KZ;
+    dst[k] = ha;
+    dst[KZ + k] = hb;
+  }
+}
+
+// MHD counterpart of k_rotational: twelve input fields (u, w, b, j) per
+// y-line and six outputs, H = -(u_tot x w_tot) + (-2 Omega u1, 2 Omega u0, 0)
+// - j x b_tot and E = u_tot x b_tot with b_tot = b + B0
+// (examples/pcf_mhd_jax.py:339-357; Coriolis force and background B0 of
+// examples/pcf_mhd_mri_shearpy_jax.py:149-176).
+template <typename T, int MAXB, int NZC>
+__global__ void __launch_bounds__(256) k_rotational_mhd(Geo g, NodalMats<T> nm, FFTPlan plan, int UPB,
+                                                        const typename VT<T>::V *__restrict__ Y,
+                                                    typename VT<T>::V *__restrict__ Hz) {
+  using V = typename VT<T>::V;
+  constexpr int NIN = 12, NOUT = 6;
   extern __shared__ __align__(16) unsigned char smem[];
   const int MZ = NZC ? NZC : g.mz;
   V *buf = reinterpret_cast<V *>(smem);   // [UPB][NIN][MZ]
@@ -148,7 +245,6 @@ __global__ void __launch_bounds__(256) k_rotational(Geo g, NodalMats<T> nm, FFTP
     fft_lds<V, T, true, false, MAXB>(buf, tw, plan, UPB * NIN, MZ, 1);
 
   const T U = (T)g.uwall;
-  const T B0x = (T)g.b0[0], B0y = (T)g.b0[1], B0z = (T)g.b0[2], O2 = (T)g.omega2;
   for (int idx = tid; idx < UPB * MZ; idx += nthr) {
     int lu = idx / MZ, n = idx - lu * MZ;
     int u = ubase + lu;
@@ -156,14 +252,18 @@ __global__ void __launch_bounds__(256) k_rotational(Geo g, NodalMats<T> nm, FFTP
     const T xv = nm.xnode[x];
     const V *b = buf + (size_t)lu * NIN * MZ + n;
     V *o = out + (size_t)lu * NOUT * MZ + n;
+    V in[NIN], res[NOUT];
+#pragma unroll
+    for (int f = 0; f < NIN; ++f) in[f] = b[f * MZ];
     // the two real lines ride in the real and imaginary parts
 #pragma unroll
     for (int part = 0; part < 2; ++part) {
-      auto P = [&](int f) { return part ? b[f * MZ].y : b[f * MZ].x; };
+      auto P = [&](int f) { return part ? in[f].y : in[f].x; };
       const T a0 = P(0), a1 = P(1) + U * xv, a2 = P(2), o0 = P(3), o1 = P(4), o2 = P(5) + U;
       T h0 = -(a1 * o2 - a2 * o1), h1 = -(a2 * o0 - a0 * o2), h2 = -(a0 * o1 - a1 * o0);
       T e0 = 0, e1 = 0, e2 = 0;
-      if constexpr (MHD) {
+      {
+        const T B0x = (T)g.b0[0], B0y = (T)g.b0[1], B0z = (T)g.b0[2], O2 = (T)g.omega2;
         // Coriolis force (-2 Omega u1, 2 Omega u0, 0) on the fluctuation velocity
         h0 -= O2 * P(1);
         h1 += O2 * P(0);
@@ -179,11 +279,13 @@ __global__ void __launch_bounds__(256) k_rotational(Geo g, NodalMats<T> nm, FFTP
 #pragma unroll
       for (int f = 0; f < NOUT; ++f) {
         if (part)
-          o[f * MZ].y = r[f];
+          res[f].y = r[f];
         else
-          o[f * MZ].x = r[f];
+          res[f].x = r[f];
       }
     }
+#pragma unroll
+    for (int f = 0; f < NOUT; ++f) o[f * MZ] = res[f];
   }
   __syncthreads();
   if constexpr (NZC > 0)
@@ -509,133 +611,150 @@ __global__ void __launch_bounds__(256) k_solve(Geo g, StructMats sm, SolveArgs a
   }
 }
 
-// Insulating walls: the A_y / A_z update of one (mode, parity) per thread
-// (examples/pcf_mhd_mri_shearpy_jax.py:559-712).  On the orthogonal Chebyshev
-// basis the radial operator h - gamma eta ((T_i, T_j'') - k^2 h) is diagonal
-// plus a strictly upper part that is rank 2 within each parity:
-// (T_i, T_j'') = (pi/2) j (j^2 - i^2) for j > i, j - i even (both i = 0 and
-// i > 0, since h_0 = pi carries the 1/c_0 = 1/2 of the derivative recurrence).
-// The two wall rows replacing the last two Galerkin rows separate by parity
-// into sum_{j in parity} beta_j x_j = 0 with beta_j = j^2 (P: P'(+-1) = 0) or
-// j^2 + k (Q: Q'(-1) - k Q(-1) = Q'(1) + k Q(1) = 0), because
-// T_j'(+-1) = (+-1)^{j+1} j^2 and T_j(+-1) = (+-1)^j.  So each parity block is
-// upper triangular with its last row replaced by beta: with t = x_last, the
-// back substitution gives x = a + b t, and beta . x = 0 fixes t.  Two top-down
-// passes, O(n) each; the first parks a in the output rows.  The right-hand
-// side (h z + eta[(T_i,T_j'') zL - k^2 h zL] + N, z = sum c x, zL = sum d x)
-// is formed in the first pass with running suffix sums.
-__global__ void __launch_bounds__(256) k_solve_bordered(Geo g, BorderedArgs a) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  const int M = g.m;
-  if (t >= 2 * M) return;
-  const int m = t >> 1, par = t & 1;
-  const int n = g.n0;
-  double kyd, kzd, kyo;
-  mode_k(g, m, kyd, kzd, kyo);
-  const double k2 = kyd * kyd + kzd * kzd, k = sqrt(k2);
-  const double cy = (k > 0.0) ? kyd / k : 1.0, cz = (k > 0.0) ? kzd / k : 0.0;
-  const bool nyq = (m / g.kzs == g.ny / 2);
+// Insulating walls: the A_y / A_z update (examples/pcf_mhd_mri_shearpy_jax.py:559-712).
+// On the orthogonal Chebyshev basis the radial operator
+// h - gamma eta ((T_i, T_j'') - k^2 h) is diagonal plus a strictly upper part
+// that is rank 2 within each parity: (T_i, T_j'') = (pi/2) j (j^2 - i^2) for
+// j > i, j - i even (both i = 0 and i > 0, since h_0 = pi carries the
+// 1/c_0 = 1/2 of the derivative recurrence).  The two wall rows replacing the
+// last two Galerkin rows separate by parity into sum_{j in parity} beta_j x_j = 0
+// with beta_j = j^2 (P: P'(+-1) = 0) or j^2 + k (Q: Q'(-1) - k Q(-1) =
+// Q'(1) + k Q(1) = 0), because T_j'(+-1) = (+-1)^{j+1} j^2 and
+// T_j(+-1) = (+-1)^j.  So each parity block is upper triangular with its last
+// row replaced by beta: with t = x_last, the back substitution gives
+// x = a + b t, and beta . x = 0 fixes t — two top-down passes, O(n) each.
+// Block: BS modes.  Phase 0 (all threads, coalesced over modes): the local part
+// of the right-hand side h z - eta k^2 h zL + N and zL of both components,
+// rotated to P = cy Ay + cz Az, Q = cz Ay - cy Az, into LDS.  Phase 1 (one
+// thread per mode, parity and P/Q): the suffix-sum term
+// eta (pi/2)(S3 - i^2 S1) of zL and the bordered recurrence, out of LDS.
+// Phase 2 (all threads): rotate back, mask the Nyquist row, store.
+template <int BS>
+__global__ void __launch_bounds__(128) k_solve_bordered(Geo g, BorderedArgs a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int n = g.n0, M = g.m;
+  const int m0 = blockIdx.x * BS;
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  double2 *Lx = reinterpret_cast<double2 *>(smem);  // [row][P/Q][lane]: local rhs, then a, then x
+  double2 *Zl = Lx + (size_t)n * 2 * BS;             // [row][P/Q][lane]: zL
   const double eta = g.nu, ge = a.gamma * eta, C = ge * 0.5 * M_PI;
   const size_t cs = a.cstride;
-  double2 *oy = a.out + cs, *oz = a.out + 2 * cs;
-  const int last = (n - 1 - par) / 2 * 2 + par;  // highest row of this parity: the wall row
-  // pass 1
-  double2 s3y{0, 0}, s1y{0, 0}, s3z{0, 0}, s1z{0, 0};      // suffix sums of zL (y, z)
-  double2 pa3{0, 0}, pa1{0, 0}, qa3{0, 0}, qa1{0, 0};      // suffix sums of a (P, Q)
-  double pb3 = 0, pb1 = 0, qb3 = 0, qb1 = 0;               // suffix sums of b (P, Q)
-  double2 pba{0, 0}, qba{0, 0};                            // beta . a
-  double pbb = 0, qbb = 0;                                 // beta . b
-  for (int I = last; I >= 0; I -= 2) {
-    const size_t o = (size_t)I * M + m;
-    const double hI = (I == 0) ? M_PI : 0.5 * M_PI, I2 = (double)I * I, I3 = I2 * I;
-    double2 zy{0, 0}, zly{0, 0}, ny{0, 0}, zz{0, 0}, zlz{0, 0}, nz{0, 0};
-    for (int l = 0; l < a.nx; ++l) {
-      const double2 vy = a.x[l][cs + o], vz = a.x[l][2 * cs + o];
-      zy.x += a.c[l] * vy.x;
-      zy.y += a.c[l] * vy.y;
-      zly.x += a.d[l] * vy.x;
-      zly.y += a.d[l] * vy.y;
-      zz.x += a.c[l] * vz.x;
-      zz.y += a.c[l] * vz.y;
-      zlz.x += a.d[l] * vz.x;
-      zlz.y += a.d[l] * vz.y;
+  auto rot = [&](int m, double &cy, double &cz, double &k2, double &k) {
+    double kyd, kzd, kyo;
+    mode_k(g, m, kyd, kzd, kyo);
+    k2 = kyd * kyd + kzd * kzd;
+    k = sqrt(k2);
+    cy = (k > 0.0) ? kyd / k : 1.0;
+    cz = (k > 0.0) ? kzd / k : 0.0;
+  };
+  // phase 0
+  for (int idx = tid; idx < n * BS; idx += nthr) {
+    const int I = idx / BS, lane = idx - I * BS, m = m0 + lane;
+    double2 lp{0, 0}, lq{0, 0}, zp{0, 0}, zq{0, 0};
+    if (m < M) {
+      double cy, cz, k2, k;
+      rot(m, cy, cz, k2, k);
+      const double hI = (I == 0) ? M_PI : 0.5 * M_PI;
+      const size_t o = (size_t)I * M + m;
+      double2 zy{0, 0}, zly{0, 0}, ny{0, 0}, zz{0, 0}, zlz{0, 0}, nz{0, 0};
+      for (int l = 0; l < a.nx; ++l) {
+        const double2 vy = a.x[l][cs + o], vz = a.x[l][2 * cs + o];
+        zy.x += a.c[l] * vy.x;
+        zy.y += a.c[l] * vy.y;
+        zly.x += a.d[l] * vy.x;
+        zly.y += a.d[l] * vy.y;
+        zz.x += a.c[l] * vz.x;
+        zz.y += a.c[l] * vz.y;
+        zlz.x += a.d[l] * vz.x;
+        zlz.y += a.d[l] * vz.y;
+      }
+      for (int l = 0; l < a.nn; ++l) {
+        const double2 vy = a.n[l][cs + o], vz = a.n[l][2 * cs + o];
+        ny.x += a.e[l] * vy.x;
+        ny.y += a.e[l] * vy.y;
+        nz.x += a.e[l] * vz.x;
+        nz.y += a.e[l] * vz.y;
+      }
+      const double f = eta * k2 * hI;
+      const double2 ly{hI * zy.x - f * zly.x + ny.x, hI * zy.y - f * zly.y + ny.y};
+      const double2 lz{hI * zz.x - f * zlz.x + nz.x, hI * zz.y - f * zlz.y + nz.y};
+      lp = double2{cy * ly.x + cz * lz.x, cy * ly.y + cz * lz.y};
+      lq = double2{cz * ly.x - cy * lz.x, cz * ly.y - cy * lz.y};
+      zp = double2{cy * zly.x + cz * zlz.x, cy * zly.y + cz * zlz.y};
+      zq = double2{cz * zly.x - cy * zlz.x, cz * zly.y - cy * zlz.y};
     }
-    for (int l = 0; l < a.nn; ++l) {
-      const double2 vy = a.n[l][cs + o], vz = a.n[l][2 * cs + o];
-      ny.x += a.e[l] * vy.x;
-      ny.y += a.e[l] * vy.y;
-      nz.x += a.e[l] * vz.x;
-      nz.y += a.e[l] * vz.y;
-    }
-    // rhs = h z + eta[(pi/2)(S3 - I^2 S1) - k^2 h zL] + N
-    const double2 ry{hI * zy.x + eta * (0.5 * M_PI * (s3y.x - I2 * s1y.x) - k2 * hI * zly.x) + ny.x,
-                     hI * zy.y + eta * (0.5 * M_PI * (s3y.y - I2 * s1y.y) - k2 * hI * zly.y) + ny.y};
-    const double2 rz{hI * zz.x + eta * (0.5 * M_PI * (s3z.x - I2 * s1z.x) - k2 * hI * zlz.x) + nz.x,
-                     hI * zz.y + eta * (0.5 * M_PI * (s3z.y - I2 * s1z.y) - k2 * hI * zlz.y) + nz.y};
-    s3y.x += I3 * zly.x;
-    s3y.y += I3 * zly.y;
-    s1y.x += I * zly.x;
-    s1y.y += I * zly.y;
-    s3z.x += I3 * zlz.x;
-    s3z.y += I3 * zlz.y;
-    s1z.x += I * zlz.x;
-    s1z.y += I * zlz.y;
-    double2 ap, aq;
-    double bp, bq;
-    if (I == last) {
-      ap = aq = double2{0, 0};
-      bp = bq = 1.0;
-    } else {
-      const double rd = 1.0 / (hI * (1.0 + ge * k2));
-      const double2 rp{cy * ry.x + cz * rz.x, cy * ry.y + cz * rz.y};
-      const double2 rq{cz * ry.x - cy * rz.x, cz * ry.y - cy * rz.y};
-      ap = double2{(rp.x + C * (pa3.x - I2 * pa1.x)) * rd, (rp.y + C * (pa3.y - I2 * pa1.y)) * rd};
-      aq = double2{(rq.x + C * (qa3.x - I2 * qa1.x)) * rd, (rq.y + C * (qa3.y - I2 * qa1.y)) * rd};
-      bp = C * (pb3 - I2 * pb1) * rd;
-      bq = C * (qb3 - I2 * qb1) * rd;
-    }
-    pa3.x += I3 * ap.x;
-    pa3.y += I3 * ap.y;
-    pa1.x += I * ap.x;
-    pa1.y += I * ap.y;
-    qa3.x += I3 * aq.x;
-    qa3.y += I3 * aq.y;
-    qa1.x += I * aq.x;
-    qa1.y += I * aq.y;
-    pb3 += I3 * bp;
-    pb1 += I * bp;
-    qb3 += I3 * bq;
-    qb1 += I * bq;
-    const double betp = I2, betq = I2 + k;
-    pba.x += betp * ap.x;
-    pba.y += betp * ap.y;
-    qba.x += betq * aq.x;
-    qba.y += betq * aq.y;
-    pbb += betp * bp;
-    qbb += betq * bq;
-    oy[o] = ap;  // parking
-    oz[o] = aq;
+    Lx[((size_t)I * 2 + 0) * BS + lane] = lp;
+    Lx[((size_t)I * 2 + 1) * BS + lane] = lq;
+    Zl[((size_t)I * 2 + 0) * BS + lane] = zp;
+    Zl[((size_t)I * 2 + 1) * BS + lane] = zq;
   }
-  const double2 tp{-pba.x / pbb, -pba.y / pbb}, tq{-qba.x / qbb, -qba.y / qbb};
-  // pass 2: b again (it depends on k and the row only), x = a + b t, rotate back
-  pb3 = pb1 = qb3 = qb1 = 0;
-  for (int I = last; I >= 0; I -= 2) {
-    const size_t o = (size_t)I * M + m;
-    const double hI = (I == 0) ? M_PI : 0.5 * M_PI, I2 = (double)I * I, I3 = I2 * I;
-    double bp, bq;
-    if (I == last) {
-      bp = bq = 1.0;
-    } else {
-      const double rd = 1.0 / (hI * (1.0 + ge * k2));
-      bp = C * (pb3 - I2 * pb1) * rd;
-      bq = C * (qb3 - I2 * qb1) * rd;
+  __syncthreads();
+  // phase 1: thread = (lane, parity, P/Q)
+  if (tid < 4 * BS) {
+    const int lane = tid >> 2, par = (tid >> 1) & 1, pq = tid & 1, m = m0 + lane;
+    if (m < M) {
+      double cy, cz, k2, k;
+      rot(m, cy, cz, k2, k);
+      auto X = [&](int I) -> double2 & { return Lx[((size_t)I * 2 + pq) * BS + lane]; };
+      auto Z = [&](int I) -> const double2 & { return Zl[((size_t)I * 2 + pq) * BS + lane]; };
+      const int last = (n - 1 - par) / 2 * 2 + par;  // highest row of this parity: the wall row
+      const double kb = pq ? k : 0.0;                // beta = i^2 (P) or i^2 + k (Q)
+      double2 s3{0, 0}, s1{0, 0}, a3{0, 0}, a1{0, 0}, ba{0, 0};
+      double b3 = 0, b1 = 0, bb = 0;
+      for (int I = last; I >= 0; I -= 2) {
+        const double hI = (I == 0) ? M_PI : 0.5 * M_PI, I2 = (double)I * I, I3 = I2 * I;
+        const double2 zl = Z(I);
+        double2 av;
+        double bv;
+        if (I == last) {
+          av = double2{0, 0};
+          bv = 1.0;
+        } else {
+          const double2 lr = X(I);
+          const double2 r{lr.x + eta * 0.5 * M_PI * (s3.x - I2 * s1.x), lr.y + eta * 0.5 * M_PI * (s3.y - I2 * s1.y)};
+          const double rd = 1.0 / (hI * (1.0 + ge * k2));
+          av = double2{(r.x + C * (a3.x - I2 * a1.x)) * rd, (r.y + C * (a3.y - I2 * a1.y)) * rd};
+          bv = C * (b3 - I2 * b1) * rd;
+        }
+        s3.x += I3 * zl.x;
+        s3.y += I3 * zl.y;
+        s1.x += I * zl.x;
+        s1.y += I * zl.y;
+        a3.x += I3 * av.x;
+        a3.y += I3 * av.y;
+        a1.x += I * av.x;
+        a1.y += I * av.y;
+        b3 += I3 * bv;
+        b1 += I * bv;
+        const double beta = I2 + kb;
+        ba.x += beta * av.x;
+        ba.y += beta * av.y;
+        bb += beta * bv;
+        X(I) = av;
+      }
+      const double2 t{-ba.x / bb, -ba.y / bb};
+      b3 = b1 = 0;
+      for (int I = last; I >= 0; I -= 2) {
+        const double hI = (I == 0) ? M_PI : 0.5 * M_PI, I2 = (double)I * I, I3 = I2 * I;
+        const double bv = (I == last) ? 1.0 : C * (b3 - I2 * b1) / (hI * (1.0 + ge * k2));
+        b3 += I3 * bv;
+        b1 += I * bv;
+        const double2 av = X(I);
+        X(I) = double2{av.x + bv * t.x, av.y + bv * t.y};
+      }
     }
-    pb3 += I3 * bp;
-    pb1 += I * bp;
-    qb3 += I3 * bq;
-    qb1 += I * bq;
-    const double2 ap = oy[o], aq = oz[o];
-    const double2 p{ap.x + bp * tp.x, ap.y + bp * tp.y}, q{aq.x + bq * tq.x, aq.y + bq * tq.y};
+  }
+  __syncthreads();
+  // phase 2
+  double2 *oy = a.out + cs, *oz = a.out + 2 * cs;
+  for (int idx = tid; idx < n * BS; idx += nthr) {
+    const int I = idx / BS, lane = idx - I * BS, m = m0 + lane;
+    if (m >= M) continue;
+    double cy, cz, k2, k;
+    rot(m, cy, cz, k2, k);
+    const bool nyq = (m / g.kzs == g.ny / 2);
+    const double2 p = Lx[((size_t)I * 2 + 0) * BS + lane], q = Lx[((size_t)I * 2 + 1) * BS + lane];
+    const size_t o = (size_t)I * M + m;
     oy[o] = nyq ? double2{0, 0} : double2{cy * p.x + cz * q.x, cy * p.y + cz * q.y};
     oz[o] = nyq ? double2{0, 0} : double2{cz * p.x - cy * q.x, cz * p.y - cy * q.y};
   }
@@ -946,7 +1065,7 @@ template <typename T>
 int rot_units(const Geo &g, bool mhd) {
   using V = typename VT<T>::V;
   const int lines = mhd ? 18 : 9;
-  int upb = 2;
+  int upb = mhd ? 1 : 2;  // MHD: twelve lines per unit already fill the block
   while (upb > 1 && ((size_t)upb * lines * g.mz + g.mz) * sizeof(V) > 65536) upb >>= 1;
   return upb;
 }
@@ -962,11 +1081,11 @@ hipError_t launch_rotational(const Geo &g, const NodalMats<T> &nm, Workspace &ws
   const dim3 grid((nunits + upb - 1) / upb);
   const V *Y = reinterpret_cast<const V *>(ws.Y);
   V *Hz = reinterpret_cast<V *>(ws.Hz);
-#define PCF_ROTN(MB, NZ)                                                                                 \
-  if (mhd)                                                                                               \
-    hipLaunchKernelGGL((k_rotational<T, MB, NZ, true>), grid, dim3(256), lds, s, g, nm, pz, upb, Y, Hz); \
-  else                                                                                                   \
-    hipLaunchKernelGGL((k_rotational<T, MB, NZ, false>), grid, dim3(256), lds, s, g, nm, pz, upb, Y, Hz)
+#define PCF_ROTN(MB, NZ)                                                                                \
+  if (mhd)                                                                                              \
+    hipLaunchKernelGGL((k_rotational_mhd<T, MB, NZ>), grid, dim3(256), lds, s, g, nm, pz, upb, Y, Hz); \
+  else                                                                                                  \
+    hipLaunchKernelGGL((k_rotational<T, MB, NZ>), grid, dim3(256), lds, s, g, nm, pz, upb, Y, Hz)
 #define PCF_ROT(MB)                              \
   switch (g.mz) {                                \
     case 24: PCF_ROTN(MB, 24); break;            \
@@ -997,7 +1116,14 @@ hipError_t launch_solve(const Geo &g, const StructMats &sm, const SolveArgs &a, 
 }
 
 hipError_t launch_solve_bordered(const Geo &g, const BorderedArgs &a, hipStream_t s) {
-  hipLaunchKernelGGL(k_solve_bordered, dim3((2 * g.m + 255) / 256), dim3(256), 0, s, g, a);
+  // modes per block: the rows of both LDS arrays within the default 64 KB
+  const size_t per_mode = (size_t)g.n0 * 2 * 2 * sizeof(double2);
+  if (8 * per_mode <= 65536)
+    hipLaunchKernelGGL((k_solve_bordered<8>), dim3((g.m + 7) / 8), dim3(128), 8 * per_mode, s, g, a);
+  else if (4 * per_mode <= 65536)
+    hipLaunchKernelGGL((k_solve_bordered<4>), dim3((g.m + 3) / 4), dim3(128), 4 * per_mode, s, g, a);
+  else
+    return hipErrorInvalidValue;
   return hipGetLastError();
 }
 
