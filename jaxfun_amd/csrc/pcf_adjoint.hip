@@ -710,7 +710,7 @@ int max_bf(const FFTPlan &plan, int nbatch) {
 template <typename T>
 int rot_adj_units(const Geo &g) {
   using V = typename VT<T>::V;
-  int upb = 2;
+  int upb = 1;  // nine lines per unit already fill the block (fewer butterflies per thread)
   while (upb > 1 && ((size_t)upb * 9 * g.mz + g.mz) * sizeof(V) > 65536) upb >>= 1;
   return upb;
 }

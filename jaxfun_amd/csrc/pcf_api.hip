@@ -104,6 +104,8 @@ struct pcf_handle {
   double2 *u0[3]{}, *g[3]{};
   double *v[3]{}, *w[3]{};
   double2 *u1 = nullptr, *u2 = nullptr;
+  unsigned char *state_block = nullptr;  // all of the above, contiguous
+  size_t state_bytes = 0;
   double2 *Nu[4]{}, *Ng[4]{};
   double *Nv[4]{}, *Nw[4]{};
   // MHD: vector potential levels (three components [c][row][m], cstride =
@@ -673,12 +675,9 @@ std::vector<std::pair<void *, size_t>> state_pieces(pcf_handle *h) {
 }
 
 hipError_t snap_save(pcf_handle *h, pcf_handle::Snap &sn) {
-  size_t off = 0;
-  for (auto &pc : state_pieces(h)) {
-    hipError_t e = hipMemcpyAsync(sn.dev + off, pc.first, pc.second, hipMemcpyDeviceToDevice, h->stream);
-    if (e != hipSuccess) return e;
-    off += (pc.second + 255) / 256 * 256;
-  }
+  // the state block holds the pieces at the snapshot's offsets: one copy
+  hipError_t e = hipMemcpyAsync(sn.dev, h->state_block, h->state_bytes, hipMemcpyDeviceToDevice, h->stream);
+  if (e != hipSuccess) return e;
   std::copy(h->so, h->so + 3, sn.so);
   std::copy(h->no, h->no + 4, sn.no);
   sn.hs = h->history_steps;
@@ -688,12 +687,8 @@ hipError_t snap_save(pcf_handle *h, pcf_handle::Snap &sn) {
 }
 
 hipError_t snap_load(pcf_handle *h, const pcf_handle::Snap &sn) {
-  size_t off = 0;
-  for (auto &pc : state_pieces(h)) {
-    hipError_t e = hipMemcpyAsync(pc.first, sn.dev + off, pc.second, hipMemcpyDeviceToDevice, h->stream);
-    if (e != hipSuccess) return e;
-    off += (pc.second + 255) / 256 * 256;
-  }
+  hipError_t e = hipMemcpyAsync(h->state_block, sn.dev, h->state_bytes, hipMemcpyDeviceToDevice, h->stream);
+  if (e != hipSuccess) return e;
   std::copy(sn.so, sn.so + 3, h->so);
   std::copy(sn.no, sn.no + 4, h->no);
   h->history_steps = sn.hs;
@@ -1087,20 +1082,37 @@ int32_t pcf_create(const pcf_config *cfg, pcf_handle **out) {
 
   int32_t r;
   const size_t M = g.m;
-  for (int s = 0; s < 3; ++s) {
-    if ((r = h->alloc(&h->u0[s], (size_t)g.nb * M))) return bail(r);
-    if ((r = h->alloc(&h->g[s], (size_t)g.nd * M))) return bail(r);
-    if ((r = h->alloc(&h->v[s], (size_t)g.nd))) return bail(r);
-    if ((r = h->alloc(&h->w[s], (size_t)g.nd))) return bail(r);
+  {
+    // the whole state in one block, laid out as state_pieces() lists it (each
+    // piece 256-byte aligned), so that a snapshot is a single copy
+    const size_t c = sizeof(double2), rb = sizeof(double);
+    auto al = [](size_t b) { return (b + 255) / 256 * 256; };
+    const size_t bu0 = al(g.nb * M * c), bg = al(g.nd * M * c), bv = al(g.nd * rb);
+    h->state_bytes = 3 * (bu0 + bg + 2 * bv) + 4 * (bu0 + bg + 2 * bv) + 2 * bg;
+    unsigned char *blk = nullptr;
+    if ((r = h->alloc(&blk, h->state_bytes))) return bail(r);
+    h->state_block = blk;
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+      unsigned char *q = blk + off;
+      off += bytes;
+      return q;
+    };
+    for (int s = 0; s < 3; ++s) {
+      h->u0[s] = reinterpret_cast<double2 *>(take(bu0));
+      h->g[s] = reinterpret_cast<double2 *>(take(bg));
+      h->v[s] = reinterpret_cast<double *>(take(bv));
+      h->w[s] = reinterpret_cast<double *>(take(bv));
+    }
+    for (int s = 0; s < 4; ++s) {
+      h->Nu[s] = reinterpret_cast<double2 *>(take(bu0));
+      h->Ng[s] = reinterpret_cast<double2 *>(take(bg));
+      h->Nv[s] = reinterpret_cast<double *>(take(bv));
+      h->Nw[s] = reinterpret_cast<double *>(take(bv));
+    }
+    h->u1 = reinterpret_cast<double2 *>(take(bg));
+    h->u2 = reinterpret_cast<double2 *>(take(bg));
   }
-  for (int s = 0; s < 4; ++s) {
-    if ((r = h->alloc(&h->Nu[s], (size_t)g.nb * M))) return bail(r);
-    if ((r = h->alloc(&h->Ng[s], (size_t)g.nd * M))) return bail(r);
-    if ((r = h->alloc(&h->Nv[s], (size_t)g.nd))) return bail(r);
-    if ((r = h->alloc(&h->Nw[s], (size_t)g.nd))) return bail(r);
-  }
-  if ((r = h->alloc(&h->u1, (size_t)g.nd * M))) return bail(r);
-  if ((r = h->alloc(&h->u2, (size_t)g.nd * M))) return bail(r);
   if (h->mhd) {
     h->cstride = (size_t)h->arows * M;
     for (int s = 0; s < 3; ++s)

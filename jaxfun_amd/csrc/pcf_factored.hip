@@ -370,67 +370,96 @@ __global__ void __launch_bounds__(256) k_solve_factored(Geo g, StructMats sm, Fa
 }
 
 // Transposed steady solve of the reverse pass on the index-reversed transposed
-// blocks: rho = S^{-T} mu, and nu = e0 rho + e1 r1 + e2 r2.  The solution rows
-// double as the parking space of the forward substitution.
+// blocks: rho = S^{-T} mu, and nu = e0 rho + e1 r1 + e2 r2.  Block: SL modes of
+// one (basis, parity) block (the Dirichlet blocks of mode 0 also carry the
+// mean system in lane SL), as k_solve_factored: phase 0 stages the factors and
+// the right-hand side rows (reversed) into LDS, phase 1 runs one substitution
+// per lane out of LDS, phase 2 writes rho and nu with coalesced stores.
+template <int SL>
 __global__ void __launch_bounds__(256) k_adj_solve_factored(Geo g, StructMats smt, FactorSet fs, PrimC in,
                                                             Prim out, Prim nuo, PrimC r1, PrimC r2,
                                                             double e0, double e1, double e2) {
-  const Lane2 L = lane2(g, blockIdx.x * blockDim.x + threadIdx.x);
-  if (!L.valid) return;
-  const int M = g.m, m = L.m, par = L.par;
-  const StructSys &ss = smt.sys[L.sys][par];
+  extern __shared__ __align__(16) unsigned char smem[];
+  constexpr int SLP = SL + 1;
+  const int M = g.m;
+  const int bps = (M + SL - 1) / SL;
+  const int spb = blockIdx.x / bps;
+  const int m0 = (blockIdx.x - spb * bps) * SL;
+  const bool has_mean = (spb >= 2) && (m0 == 0);
+  const int sys = spb >> 1, par = spb & 1;
+  const StructSys &ss = smt.sys[sys][par];
   const int n = ss.n;
+  const int NF = (sys == 0) ? NF_B : NF_D;
+  double *zn = reinterpret_cast<double *>(smem);  // [ir][2][SLP]: rhs, then y, then x
+  double *fl = zn + (size_t)g.npmax * 2 * SLP;    // [ir][NF][SLP]
   auto row = [&](int ir) { return 2 * (n - 1 - ir) + par; };
-  if (!L.mean && (m / g.kzs == g.ny / 2)) {  // Nyquist row: masked
-    double2 *dst = (L.sys == 0) ? out.u0 : out.g;
-    double2 *dn = (L.sys == 0) ? nuo.u0 : nuo.g;
-    for (int i = 0; i < n; ++i) {
-      dst[(size_t)(2 * i + par) * M + m] = double2{0, 0};
-      dn[(size_t)(2 * i + par) * M + m] = double2{0, 0};
-    }
-    return;
+  auto ZN = [&](int ir, int v, int lane) -> double & { return zn[((size_t)ir * 2 + v) * SLP + lane]; };
+  auto active = [&](int lane) { return (lane == SL) ? has_mean : (m0 + lane < M); };
+  gather_batched<8>(
+      n * NF * SLP,
+      [&](int idx) {
+        const int j = idx / (NF * SLP), rem = idx - j * NF * SLP, v = rem / SLP, lane = rem - v * SLP;
+        if (!active(lane)) return 0.0;
+        return (lane == SL) ? fs.mean[par][(size_t)j * NF_D + v]
+                            : fs.blk[sys][par][((size_t)j * NF + v) * M + m0 + lane];
+      },
+      [&](int idx, double v) { fl[idx] = v; });
+  gather_batched<4>(
+      n * SLP,
+      [&](int idx) {
+        const int ir = idx / SLP, lane = idx - ir * SLP;
+        if (!active(lane)) return double2{0, 0};
+        const int J = row(ir);
+        if (lane == SL) return double2{in.v[J], in.w[J]};
+        return (sys == 0 ? in.u0 : in.g)[(size_t)J * M + m0 + lane];
+      },
+      [&](int idx, const double2 &v) {
+        const int ir = idx / SLP, lane = idx - ir * SLP;
+        ZN(ir, 0, lane) = v.x;
+        ZN(ir, 1, lane) = v.y;
+      });
+  __syncthreads();
+  int lane = -1;
+  if (threadIdx.x < SL)
+    lane = threadIdx.x;
+  else if (threadIdx.x == 64 && has_mean)
+    lane = SL;
+  if (lane >= 0 && active(lane)) {
+    auto rw = [&](int i) { return double2{ZN(i, 0, lane), ZN(i, 1, lane)}; };
+    auto wr = [&](int i, double2 v) {
+      ZN(i, 0, lane) = v.x;
+      ZN(i, 1, lane) = v.y;
+    };
+    const double *F = fl + lane;
+    if (sys == 0)
+      substitute<2, 2, 2>(ss, n, F, (size_t)NF_B * SLP, (size_t)SLP, rw, wr, rw, wr);
+    else
+      substitute<1, 1, 1>(ss, n, F, (size_t)NF_D * SLP, (size_t)SLP, rw, wr, rw, wr);
   }
-  if (L.sys == 0) {
-    double2 *x = out.u0;
-    auto at = [&](int ir) -> double2 & { return x[(size_t)row(ir) * M + m]; };
-    substitute<2, 2, 2>(
-        ss, n, fs.blk[0][par] + m, (size_t)NF_B * M, (size_t)M,
-        [&](int ir) { return in.u0[(size_t)row(ir) * M + m]; }, [&](int ir, double2 y) { at(ir) = y; },
-        [&](int ir) { return at(ir); },
-        [&](int ir, double2 v) {
-          const size_t o = (size_t)row(ir) * M + m;
-          x[o] = v;
-          const double2 p = r1.u0[o], q = r2.u0[o];
-          nuo.u0[o] = double2{e0 * v.x + e1 * p.x + e2 * q.x, e0 * v.y + e1 * p.y + e2 * q.y};
-        });
-  } else if (L.mean) {
-    substitute<1, 1, 1>(
-        ss, n, fs.mean[par], NF_D, 1, [&](int ir) { return double2{in.v[row(ir)], in.w[row(ir)]}; },
-        [&](int ir, double2 y) {
-          out.v[row(ir)] = y.x;
-          out.w[row(ir)] = y.y;
-        },
-        [&](int ir) { return double2{out.v[row(ir)], out.w[row(ir)]}; },
-        [&](int ir, double2 v) {
-          const int j = row(ir);
-          out.v[j] = v.x;
-          out.w[j] = v.y;
-          nuo.v[j] = e0 * v.x + e1 * r1.v[j] + e2 * r2.v[j];
-          nuo.w[j] = e0 * v.y + e1 * r1.w[j] + e2 * r2.w[j];
-        });
-  } else {
-    double2 *x = out.g;
-    auto at = [&](int ir) -> double2 & { return x[(size_t)row(ir) * M + m]; };
-    substitute<1, 1, 1>(
-        ss, n, fs.blk[1][par] + m, (size_t)NF_D * M, (size_t)M,
-        [&](int ir) { return in.g[(size_t)row(ir) * M + m]; }, [&](int ir, double2 y) { at(ir) = y; },
-        [&](int ir) { return at(ir); },
-        [&](int ir, double2 v) {
-          const size_t o = (size_t)row(ir) * M + m;
-          x[o] = v;
-          const double2 p = r1.g[o], q = r2.g[o];
-          nuo.g[o] = double2{e0 * v.x + e1 * p.x + e2 * q.x, e0 * v.y + e1 * p.y + e2 * q.y};
-        });
+  __syncthreads();
+  for (int idx = threadIdx.x; idx < n * SLP; idx += blockDim.x) {
+    const int ir = idx / SLP, ln = idx - ir * SLP;
+    if (!active(ln)) continue;
+    const int J = row(ir);
+    double2 x{ZN(ir, 0, ln), ZN(ir, 1, ln)};
+    if (ln == SL) {
+      out.v[J] = x.x;
+      out.w[J] = x.y;
+      nuo.v[J] = e0 * x.x + e1 * r1.v[J] + e2 * r2.v[J];
+      nuo.w[J] = e0 * x.y + e1 * r1.w[J] + e2 * r2.w[J];
+      continue;
+    }
+    const int m = m0 + ln;
+    const size_t o = (size_t)J * M + m;
+    double2 v{0, 0};
+    if (m / g.kzs != g.ny / 2) {  // the Nyquist row stays masked
+      const double2 p = (sys == 0 ? r1.u0 : r1.g)[o], q = (sys == 0 ? r2.u0 : r2.g)[o];
+      v = double2{e0 * x.x + e1 * p.x + e2 * q.x, e0 * x.y + e1 * p.y + e2 * q.y};
+    } else {
+      x = double2{0, 0};
+    }
+    (sys == 0 ? out.u0 : out.g)[o] = x;
+    (sys == 0 ? nuo.u0 : nuo.g)[o] = v;
   }
 }
 
@@ -470,9 +499,14 @@ hipError_t launch_solve_factored(const Geo &g, const StructMats &sm, const Facto
 hipError_t launch_adj_solve_factored(const Geo &g, const StructMats &smt, const FactorSet &fs, PrimC in,
                                      Prim out, Prim nu, PrimC r1, PrimC r2, double e0, double e1, double e2,
                                      hipStream_t s) {
-  const int threads = 4 * ((g.m + 63) / 64 * 64) + 64;
-  hipLaunchKernelGGL(k_adj_solve_factored, dim3((threads + 255) / 256), dim3(256), 0, s, g, smt, fs, in, out,
-                     nu, r1, r2, e0, e1, e2);
+  auto lds = [&](int sl) { return (size_t)g.npmax * (2 + NF_B) * (sl + 1) * sizeof(double); };
+#define PCF_ASOLVE(SLV)                                                                                    \
+  hipLaunchKernelGGL(k_adj_solve_factored<SLV>, dim3(4 * ((g.m + SLV - 1) / SLV)), dim3(256), lds(SLV), s, \
+                     g, smt, fs, in, out, nu, r1, r2, e0, e1, e2)
+  if (lds(16) <= 65536) PCF_ASOLVE(16);
+  else if (lds(8) <= 65536) PCF_ASOLVE(8);
+  else return hipErrorInvalidValue;
+#undef PCF_ASOLVE
   return hipGetLastError();
 }
 
