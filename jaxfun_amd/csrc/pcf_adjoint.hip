@@ -428,102 +428,6 @@ __global__ void __launch_bounds__(256) k_recon_adj(Geo g, const double *__restri
   }
 }
 
-// mu_x = (transpose of the reconstruction)(mu_u) + M^T sum_t c_t rho_t for a
-// steady step (M = W (C2_B - k^2 C0_B), W C0_D, C0_D: banded, no far part),
-// one thread per (row, mode); g_extra (optional) adds to mu_g.
-__global__ void __launch_bounds__(256) k_adj_combine(Geo g, StructMats sm, const double *__restrict__ RcT,
-                                                     int ldk, const double2 *__restrict__ mu0,
-                                                     const double2 *__restrict__ mu1,
-                                                     const double2 *__restrict__ mu2,
-                                                     const double2 *__restrict__ g_extra, PrimSum gat,
-                                                     Prim out) {
-  const int m = blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = blockIdx.y;
-  if (m >= g.m) return;
-  const int M = g.m;
-  const size_t o = (size_t)j * M + m;
-  const bool nyq = nyquist_row(g, m);
-  double kyd, kzd, kyo;
-  mode_k(g, m, kyd, kzd, kyo);
-  const double k2 = kyd * kyd + kzd * kzd;
-  const double den = (k2 == 0.0) ? 1.0 : k2;
-  const double a = kyd / den, b = kzd / den;
-  const int par = j & 1, jp = j >> 1;
-  // banded M^T of the gathered rho: sum over rows i = jp - d of the parity block
-  auto gather = [&](const StructSys &ss, int P, int Q, double m0, double m2, int comp) {
-    double2 acc{0, 0};
-    const int BW = P + Q + 1;
-    for (int d = -P; d <= Q; ++d) {
-      const int i = jp - d;
-      if (i < 0 || i >= ss.n) continue;
-      const size_t bo = (size_t)i * BW + (d + P);
-      const double c = m0 * ss.b0[bo] + m2 * ss.b2[bo];
-      const size_t oi = (size_t)(2 * i + par) * M + m;
-      for (int t = 0; t < gat.n; ++t) {
-        const double2 r = (comp == 0) ? gat.t[t].u0[oi] : gat.t[t].g[oi];
-        acc.x += c * gat.wgt[t] * r.x;
-        acc.y += c * gat.wgt[t] * r.y;
-      }
-    }
-    return acc;
-  };
-  if (j < g.nb) {
-    double2 acc = nyq ? double2{0, 0} : mu0[o];
-    if (m != 0 && !nyq) {
-      const int i0 = 1 - (j & 1);
-      const int nk = (g.nd - i0 + 1) / 2;
-      const double *r = RcT + (size_t)j * ldk;
-      for (int k = 0; k < nk; ++k) {
-        const size_t oi = (size_t)(i0 + 2 * k) * M + m;
-        const double2 p = mu1[oi], q = mu2[oi];
-        const double2 s{a * p.x + b * q.x, a * p.y + b * q.y};
-        acc.x += r[k] * s.y;
-        acc.y -= r[k] * s.x;
-      }
-    }
-    if (gat.n && !nyq) {
-      const double2 gsum = gather(sm.sys[0][par], 2, 2, -g.W * k2, g.W, 0);
-      acc.x += gsum.x;
-      acc.y += gsum.y;
-    }
-    out.u0[o] = acc;
-  }
-  if (j < g.nd) {
-    double2 gv{0, 0};
-    if (m != 0 && !nyq) {
-      const double2 p = mu1[o], q = mu2[o];
-      const double2 s{b * p.x - a * q.x, b * p.y - a * q.y};
-      gv = double2{s.y, -s.x};
-    }
-    if (g_extra && !nyq) {
-      gv.x += g_extra[o].x;
-      gv.y += g_extra[o].y;
-    }
-    if (gat.n && !nyq) {
-      const double2 gsum = gather(sm.sys[1][par], 1, 1, g.W, 0.0, 1);
-      gv.x += gsum.x;
-      gv.y += gsum.y;
-    }
-    out.g[o] = gv;
-    if (m == 0) {
-      double v = mu1[o].x, w = mu2[o].x;
-      const StructSys &ss = sm.sys[1][par];
-      for (int d = -1; d <= 1; ++d) {
-        const int i = jp - d;
-        if (i < 0 || i >= ss.n) continue;
-        const double c = ss.b0[(size_t)i * 3 + (d + 1)];
-        const int J = 2 * i + par;
-        for (int t = 0; t < gat.n; ++t) {
-          v += c * gat.wgt[t] * gat.t[t].v[J];
-          w += c * gat.wgt[t] * gat.t[t].w[J];
-        }
-      }
-      out.v[j] = v;
-      out.w[j] = w;
-    }
-  }
-}
-
 // Cross-product adjoint.  LDS layout [field][unit][MZ] (field-major, so the
 // six outputs of all units form one uniformly strided FFT batch).  Fields 0-5
 // are the forward u, omega; 6-8 the cotangent of H.  With a = u + (0, U x, 0),
@@ -722,15 +626,6 @@ hipError_t launch_adj_solve(const Geo &g, const StructMats &smt, double gamma, P
                             double e1, double e2) {
   hipLaunchKernelGGL(k_adj_solve, dim3((sys_threads(g) + 255) / 256), dim3(256), 0, s, g, smt, gamma,
                      in, out, scratch, nu, r1, r2, e0, e1, e2);
-  return hipGetLastError();
-}
-
-hipError_t launch_adj_combine(const Geo &g, const StructMats &sm, const double *RcT, int ldk,
-                              const double2 *mu0, const double2 *mu1, const double2 *mu2,
-                              const double2 *g_extra, const PrimSum &gather, Prim out, hipStream_t s) {
-  dim3 grid((g.m + 255) / 256, g.nd);
-  hipLaunchKernelGGL(k_adj_combine, grid, dim3(256), 0, s, g, sm, RcT, ldk, mu0, mu1, mu2, g_extra,
-                     gather, out);
   return hipGetLastError();
 }
 

@@ -24,6 +24,8 @@ struct AdjMats {
   const T *VB;   // [n0][ldi]  basis values at the nodes (VB[x][i])
   const T *VD;   // [n0][ldi]
   const T *VD1;  // [n0][ldi]
+  const T *VDR;   // [n0][ldi]  VD R (columns: clamped index < nb), R the reconstruction f = R u0
+  const T *VD1R;  // [n0][ldi]  VD1 R
   int ldi;       // >= nd, multiple of the row chunk
 };
 
@@ -54,15 +56,13 @@ struct PrimSum {
 hipError_t launch_adj_solve(const Geo &g, const StructMats &smt, double gamma, PrimC in, Prim out,
                             double *scratch, hipStream_t s, Prim nu = Prim{}, PrimC r1 = PrimC{},
                             PrimC r2 = PrimC{}, double e0 = 0, double e1 = 0, double e2 = 0);
-// out = reconstruction adjoint of (mu_u0, mu_u1, mu_u2) [+ g_extra on g]
-//       + M^T sum_t c_t rho_t (the steady SBDF3 gather; banded).
-hipError_t launch_adj_combine(const Geo &g, const StructMats &sm, const double *RcT, int ldk,
-                              const double2 *mu0, const double2 *mu1, const double2 *mu2,
-                              const double2 *g_extra, const PrimSum &gather, Prim out, hipStream_t s);
-// Steady transposed solve with the factors of the transposed blocks (smt).
+// Steady transposed solve with the factors of the transposed blocks (smt);
+// also nu = e0 out + e1 r1 + e2 r2 and, with sig.u0 set, sig = c0 out + c1 r1
+// + c2 r2.
 hipError_t launch_adj_solve_factored(const Geo &g, const StructMats &smt, const FactorSet &fs, PrimC in,
                                      Prim out, Prim nu, PrimC r1, PrimC r2, double e0, double e1, double e2,
-                                     hipStream_t s);
+                                     hipStream_t s, Prim sig = Prim{}, double c0 = 0, double c1 = 0,
+                                     double c2 = 0);
 // out += (M + gamma L)^T r_stage (if r_stage.u0) + M^T (sum of gather terms).
 hipError_t launch_adj_mass(const Geo &g, const StructMats &sm, double gamma, PrimC r_stage,
                            const PrimSum &gather, Prim out, hipStream_t s);
@@ -99,6 +99,12 @@ hipError_t launch_nodal_adj_mfma(const Geo &g, const AdjMats<T> &am, Workspace &
                                  double2 *mu1, double2 *mu2, hipStream_t s);
 template <typename T>
 hipError_t launch_rhs_adj_mfma(const Geo &g, const AdjMats<T> &am, Prim nu, Workspace &ws, hipStream_t s);
+// mu_x of a steady reverse step in one pass: the transform adjoint above, the
+// reconstruction adjoint (launch_recon_adj) and M^T sig, sig = sum_j C_j rho_{k+j}
+// (the banded steady SBDF3 mass gather).
+template <typename T>
+hipError_t launch_nodal_adj_x_mfma(const Geo &g, const AdjMats<T> &am, const StructMats &sm, PrimC sig,
+                                   Workspace &ws, Prim out, hipStream_t s);
 // Per-configuration attributes of the matrix-core kernels (dynamic LDS above 64 KB).
 hipError_t mfma_init(const Geo &g, bool fp64);
 // LDS bytes of k_rot_adj; false when the padded spanwise grid does not fit.

@@ -135,6 +135,7 @@ struct pcf_handle {
   double *dVB = nullptr, *dVD = nullptr, *dVB1 = nullptr, *dVD1 = nullptr, *dfejer = nullptr,
          *dxn = nullptr, *ddiag = nullptr;
   std::map<int, hipGraphExec_t> graphs;
+  std::map<uint64_t, hipGraphExec_t> seg_graphs;  // steady runs of the reverse pass's forward sweeps
   StageTimer *timer = nullptr;  // non-null while pcf_profile_steady runs
 
   // Reverse mode, set up by the first pcf_vjp.
@@ -153,10 +154,19 @@ struct pcf_handle {
     pcf::AdjMats<double> ad{};
     pcf::AdjMats<float> af{};
     pcf::Prim muX{}, rho[3]{}, rs[3]{}, gh[4]{}, nu{};
+    pcf::Prim sig{};              // sum_j C_j rho_{k+j} of the current steady reverse step
     double2 *mu_u[3]{};
     double2 *cot_u[3]{}, *cot_g = nullptr;
     bool cot_set[4] = {false, false, false, false};
     int rr[3] = {0, 1, 2};        // rho ring: rho_k, rho_k+1, rho_k+2
+    // The reverse sweep runs on its own stream with its own workspace and
+    // solve scratch, so that the recompute of segment s-1 (handle stream,
+    // tape set (s-1) & 1) overlaps the reverse of segment s (tape set s & 1).
+    hipStream_t stream = nullptr;
+    hipEvent_t ev_seed = nullptr, ev_rc[2] = {nullptr, nullptr}, ev_rev[2] = {nullptr, nullptr};
+    pcf::Workspace ws{};
+    double *scratch = nullptr;
+    int per_set = 0;              // tape slots per set
     std::vector<void *> tape;     // padded y-line fields of each nonlinear evaluation
     std::vector<Snap> snaps;      // state every `every` steps, then the final state
     size_t snap_bytes = 0;
@@ -649,7 +659,11 @@ bool field_is_vector(int f) {
 // reversed one by one.  The forward is recomputed segment by segment from
 // state snapshots taken every `every` steps; the recompute keeps the padded
 // y-line fields of every nonlinear evaluation (the tape), so the reverse
-// never repeats the wall-normal transform.
+// never repeats the wall-normal transform.  The recompute of segment s - 1
+// (handle stream, graph-replayed steady runs) runs concurrently with the
+// reverse sweep of segment s (reverse stream, own workspace), with two tape
+// sets ordered by events: at this size every kernel leaves most of the
+// device idle, so the two sweeps share it.
 
 const double C_SBDF[3] = {18.0 / 11.0, -9.0 / 11.0, 2.0 / 11.0};
 
@@ -672,6 +686,41 @@ std::vector<std::pair<void *, size_t>> state_pieces(pcf_handle *h) {
   p.push_back({h->u1, g.nd * M * c});
   p.push_back({h->u2, g.nd * M * c});
   return p;
+}
+
+// n steady steps, each keeping its nonlinear evaluation in tape slot
+// tape_base + i (no tape when tape_base < 0), replayed from a graph cached per
+// (ring phase, n, tape_base): the captured launches and pointers are the same
+// whenever the rings stand at the same phase, so one graph serves every
+// segment of the sweeps that starts there.
+int32_t steady_run(pcf_handle *h, int n, int tape_base) {
+  uint64_t key = 0;
+  for (int i = 0; i < 3; ++i) key = key * 4 + (uint64_t)h->so[i];
+  for (int i = 0; i < 4; ++i) key = key * 4 + (uint64_t)h->no[i];
+  key |= (uint64_t)n << 16 | (uint64_t)(tape_base + 1) << 40;
+  auto it = h->seg_graphs.find(key);
+  if (it == h->seg_graphs.end()) {
+    hipGraph_t graph = nullptr;
+    PCF_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    int32_t r = PCF_OK;
+    for (int i = 0; i < n && !r; ++i) r = steady_step(h, tape_base >= 0 ? &h->adj.tape[tape_base + i] : nullptr);
+    hipError_t e = hipStreamEndCapture(h->stream, &graph);
+    if (r) return r;
+    if (e != hipSuccess) return fail(PCF_E_HIP, std::string("capture: ") + hipGetErrorString(e));
+    hipGraphExec_t exec = nullptr;
+    e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) return fail(PCF_E_HIP, std::string("instantiate: ") + hipGetErrorString(e));
+    it = h->seg_graphs.emplace(key, exec).first;
+    // the capture advanced the rings and the history state n steps
+  } else {
+    for (int i = 0; i < n; ++i) rotate(h);
+    h->history_steps = 2.0;
+    h->have_old = 1.0;
+    h->previous_dt = h->cfg.dt;
+  }
+  PCF_HIP(hipGraphLaunch(it->second, h->stream));
+  return PCF_OK;
 }
 
 hipError_t snap_save(pcf_handle *h, pcf_handle::Snap &sn) {
@@ -721,10 +770,10 @@ pcf::PrimC cprim(const pcf::Prim &p) { return pcf::PrimC{p.u0, p.g, p.v, p.w}; }
 hipError_t zero_prim(pcf_handle *h, const pcf::Prim &p) {
   const pcf::Geo &g = h->geo;
   hipError_t e;
-  if ((e = hipMemsetAsync(p.u0, 0, (size_t)g.nb * g.m * sizeof(double2), h->stream))) return e;
-  if ((e = hipMemsetAsync(p.g, 0, (size_t)g.nd * g.m * sizeof(double2), h->stream))) return e;
-  if ((e = hipMemsetAsync(p.v, 0, (size_t)g.nd * sizeof(double), h->stream))) return e;
-  return hipMemsetAsync(p.w, 0, (size_t)g.nd * sizeof(double), h->stream);
+  if ((e = hipMemsetAsync(p.u0, 0, (size_t)g.nb * g.m * sizeof(double2), h->adj.stream))) return e;
+  if ((e = hipMemsetAsync(p.g, 0, (size_t)g.nd * g.m * sizeof(double2), h->adj.stream))) return e;
+  if ((e = hipMemsetAsync(p.v, 0, (size_t)g.nd * sizeof(double), h->adj.stream))) return e;
+  return hipMemsetAsync(p.w, 0, (size_t)g.nd * sizeof(double), h->adj.stream);
 }
 
 template <typename T>
@@ -752,6 +801,21 @@ int32_t build_adj_mats(pcf_handle *h, pcf::AdjMats<T> &am) {
   am.VD = d;
   if ((r = h->upload(&d, pad(o.VD1, ldi)))) return r;
   am.VD1 = d;
+  // the reconstruction f = R u0 composed into the nodal values of f
+  auto times_R = [&](const pcf::Mat &V) {
+    pcf::Mat P;
+    P.rows = V.rows;
+    P.cols = o.nb;
+    P.a.assign((size_t)P.rows * P.cols, 0.0);
+    for (int x = 0; x < V.rows; ++x)
+      for (int i = 0; i < o.nd; ++i)
+        for (int j = 0; j < o.nb; ++j) P(x, j) += V(x, i) * o.R(i, j);
+    return P;
+  };
+  if ((r = h->upload(&d, pad(times_R(o.VD), ldi)))) return r;
+  am.VDR = d;
+  if ((r = h->upload(&d, pad(times_R(o.VD1), ldi)))) return r;
+  am.VD1R = d;
   am.ldi = ldi;
   return PCF_OK;
 }
@@ -830,7 +894,22 @@ int32_t adj_setup(pcf_handle *h) {
   } else {
     if ((r = build_adj_mats<float>(h, A.af))) return r;
   }
-  if ((r = alloc_prim(h, A.muX)) || (r = alloc_prim(h, A.nu))) return r;
+  if ((r = alloc_prim(h, A.muX)) || (r = alloc_prim(h, A.nu)) || (r = alloc_prim(h, A.sig))) return r;
+  {
+    const size_t ve = (h->cfg.precision == PCF_PREC_FP64) ? sizeof(double2) : sizeof(float2);
+    const size_t nX = 6 * (size_t)g.n0 * g.m, nY = 6 * (size_t)g.n0 * g.my * g.kzs;
+    const size_t nHz = 3 * (size_t)g.n0 * g.my * g.kzs, nHx = 3 * (size_t)g.n0 * g.m;
+    unsigned char *wsp = nullptr;
+    if ((r = h->alloc(&wsp, (nX + nY + nHz + nHx) * ve))) return r;
+    A.ws.X = wsp;
+    A.ws.Y = wsp + nX * ve;
+    A.ws.Hz = wsp + (nX + nY) * ve;
+    A.ws.Hx = wsp + (nX + nY + nHz) * ve;
+    if ((r = h->alloc(&A.scratch, pcf::solve_scratch_doubles(g)))) return r;
+    PCF_HIP(hipStreamCreateWithFlags(&A.stream, hipStreamNonBlocking));
+    for (hipEvent_t *ev : {&A.ev_seed, &A.ev_rc[0], &A.ev_rc[1], &A.ev_rev[0], &A.ev_rev[1]})
+      PCF_HIP(hipEventCreateWithFlags(ev, hipEventDisableTiming));
+  }
   for (int i = 0; i < 3; ++i)
     if ((r = alloc_prim(h, A.rho[i])) || (r = alloc_prim(h, A.rs[i]))) return r;
   for (int i = 0; i < 4; ++i)
@@ -852,10 +931,13 @@ int32_t adj_reserve(pcf_handle *h, int steps, int every) {
   const pcf::Geo &g = h->geo;
   const size_t ve = (h->cfg.precision == PCF_PREC_FP64) ? sizeof(double2) : sizeof(float2);
   const size_t slot_bytes = 6 * (size_t)g.n0 * g.my * g.kzs * ve;
-  const size_t need_tape = (size_t)std::min(every, steps) + 4;  // start-up steps evaluate 3 times
+  const int per_set = std::min(every, steps) + 4;  // start-up steps evaluate 3 times
   const size_t need_snap = (size_t)((steps + every - 1) / every) + 1;
   int32_t r;
-  while (A.tape.size() < need_tape) {
+  // two sets (double buffering between the recompute and the reverse sweep);
+  // slot pointers never change, so cached graphs stay valid
+  A.per_set = per_set;
+  while (A.tape.size() < 2 * (size_t)per_set) {
     unsigned char *p = nullptr;
     if ((r = adj_alloc(h, &p, slot_bytes))) return r;
     A.tape.push_back(p);
@@ -874,11 +956,11 @@ hipError_t nonlinear_adjoint(pcf_handle *h, const pcf::NodalMats<T> &nm, const p
   const pcf::Geo &g = h->geo;
   auto &A = h->adj;
   hipError_t e;
-  if ((e = pcf::launch_rhs_adj_mfma<T>(g, am, nu, h->ws, h->stream))) return e;
-  if ((e = pcf::launch_fft_y_fields<T>(g, nm, h->py, true, 3, h->ws.Hx, h->ws.Hz, h->stream))) return e;
-  if ((e = pcf::launch_rot_adj<T>(g, nm, ytape, h->ws, h->pz, h->stream))) return e;
-  if ((e = pcf::launch_fft_y_fields<T>(g, nm, h->py, false, 6, h->ws.Y, h->ws.X, h->stream))) return e;
-  return pcf::launch_nodal_adj_mfma<T>(g, am, h->ws, A.mu_u[0], A.mu_u[1], A.mu_u[2], h->stream);
+  if ((e = pcf::launch_rhs_adj_mfma<T>(g, am, nu, A.ws, A.stream))) return e;
+  if ((e = pcf::launch_fft_y_fields<T>(g, nm, h->py, true, 3, A.ws.Hx, A.ws.Hz, A.stream))) return e;
+  if ((e = pcf::launch_rot_adj<T>(g, nm, ytape, A.ws, h->pz, A.stream))) return e;
+  if ((e = pcf::launch_fft_y_fields<T>(g, nm, h->py, false, 6, A.ws.Y, A.ws.X, A.stream))) return e;
+  return pcf::launch_nodal_adj_mfma<T>(g, am, A.ws, A.mu_u[0], A.mu_u[1], A.mu_u[2], A.stream);
 }
 
 // mu_u = F'(u)^T nu for the nonlinear evaluation whose forward fields are ytape;
@@ -886,11 +968,26 @@ hipError_t nonlinear_adjoint(pcf_handle *h, const pcf::NodalMats<T> &nm, const p
 hipError_t nl_adj(pcf_handle *h, const pcf::PrimSum &nu, const void *ytape) {
   auto &A = h->adj;
   if (!(nu.n == 1 && nu.wgt[0] == 1.0 && nu.t[0].u0 == A.nu.u0)) {
-    hipError_t e = pcf::launch_prim_sum(h->geo, nu, A.nu, h->stream);
+    hipError_t e = pcf::launch_prim_sum(h->geo, nu, A.nu, A.stream);
     if (e != hipSuccess) return e;
   }
   return (h->cfg.precision == PCF_PREC_FP64) ? nonlinear_adjoint<double>(h, h->md.view, A.ad, A.nu, ytape)
                                              : nonlinear_adjoint<float>(h, h->mf.view, A.af, A.nu, ytape);
+}
+
+// Steady reverse step with k > 0: the nonlinear adjoint of adj.nu with the
+// reconstruction adjoint and the mass gather of adj.sig folded into its last
+// contraction, writing mu_{x_k} to adj.muX.
+template <typename T>
+hipError_t nl_adj_x(pcf_handle *h, const pcf::NodalMats<T> &nm, const pcf::AdjMats<T> &am, const void *ytape) {
+  const pcf::Geo &g = h->geo;
+  auto &A = h->adj;
+  hipError_t e;
+  if ((e = pcf::launch_rhs_adj_mfma<T>(g, am, A.nu, A.ws, A.stream))) return e;
+  if ((e = pcf::launch_fft_y_fields<T>(g, nm, h->py, true, 3, A.ws.Hx, A.ws.Hz, A.stream))) return e;
+  if ((e = pcf::launch_rot_adj<T>(g, nm, ytape, A.ws, h->pz, A.stream))) return e;
+  if ((e = pcf::launch_fft_y_fields<T>(g, nm, h->py, false, 6, A.ws.Y, A.ws.X, A.stream))) return e;
+  return pcf::launch_nodal_adj_x_mfma<T>(g, am, h->sm, cprim(A.sig), A.ws, A.muX, A.stream);
 }
 
 void ps_add(pcf::PrimSum &s, const pcf::Prim &p, double w) {
@@ -905,7 +1002,7 @@ hipError_t recon_or_zero(pcf_handle *h, bool leaves) {
   auto &A = h->adj;
   if (leaves) return zero_prim(h, A.muX);
   return pcf::launch_recon_adj(h->geo, A.RcT, A.ldrT, A.mu_u[0], A.mu_u[1], A.mu_u[2], nullptr, A.muX,
-                               h->stream);
+                               A.stream);
 }
 
 // Reverse of step k (state k -> k+1); mu_{x_{k+1}} in muX on entry, mu_{x_k}
@@ -925,26 +1022,28 @@ hipError_t reverse_step(pcf_handle *h, int k, bool steady, void *const *tape) {
   hipError_t er;
   if (steady) {
     // rho_k and, in the same pass, mu_{N_k} = e0 rho_k + e1 rho_k+1 + e2 rho_k+2
+    // and sig = sum_j C_j rho_{k+j}
     if ((er = pcf::launch_adj_solve_factored(g, A.smt, A.fsteady_t, cprim(A.muX), A.rho[rr[0]], A.nu,
                                              cprim(A.rho[rr[1]]), cprim(A.rho[rr[2]]), e[0], e[1], e[2],
-                                             h->stream)))
+                                             A.stream, A.sig, C_SBDF[0], C_SBDF[1], C_SBDF[2])))
       return er;
-    pcf::PrimSum nu{};
-    ps_add(nu, A.nu, 1.0);
-    if ((er = nl_adj(h, nu, tape[0]))) return er;
-    pcf::PrimSum ga{};
-    for (int j = 0; j < 3; ++j) ps_add(ga, A.rho[rr[j]], C_SBDF[j]);
     if (k == 0) {  // initial state: its velocity leaves keep mu_u
+      pcf::PrimSum nu{};
+      ps_add(nu, A.nu, 1.0);
+      if ((er = nl_adj(h, nu, tape[0]))) return er;
+      pcf::PrimSum ga{};
+      for (int j = 0; j < 3; ++j) ps_add(ga, A.rho[rr[j]], C_SBDF[j]);
       if ((er = zero_prim(h, A.muX))) return er;
-      return pcf::launch_adj_mass(g, h->sm, 0.0, pcf::PrimC{}, ga, A.muX, h->stream);
+      return pcf::launch_adj_mass(g, h->sm, 0.0, pcf::PrimC{}, ga, A.muX, A.stream);
     }
-    return pcf::launch_adj_combine(g, h->sm, A.RcT, A.ldrT, A.mu_u[0], A.mu_u[1], A.mu_u[2], nullptr, ga,
-                                   A.muX, h->stream);
+    // mu_{x_k} = R^T F'(u_k)^T nu + M^T sig in one pass after the transforms
+    return (h->cfg.precision == PCF_PREC_FP64) ? nl_adj_x<double>(h, h->md.view, A.ad, tape[0])
+                                               : nl_adj_x<float>(h, h->mf.view, A.af, tape[0]);
   }
   if ((er = zero_prim(h, A.rho[rr[0]]))) return er;
   for (int s = 2; s >= 0; --s) {
     const double gamma = (IMEX_A[s] + IMEX_B[s]) * dt / 2.0;
-    if ((er = pcf::launch_adj_solve(g, A.smt, gamma, cprim(A.muX), A.rs[s], h->scratch, h->stream)))
+    if ((er = pcf::launch_adj_solve(g, A.smt, gamma, cprim(A.muX), A.rs[s], A.scratch, A.stream)))
       return er;
     pcf::PrimSum nu{};
     ps_add(nu, A.rs[s], dt * IMEX_A[s]);
@@ -960,7 +1059,7 @@ hipError_t reverse_step(pcf_handle *h, int k, bool steady, void *const *tape) {
       ps_add(ga, A.rho[rr[1]], C_SBDF[1]);
       ps_add(ga, A.rho[rr[2]], C_SBDF[2]);
     }
-    if ((er = pcf::launch_adj_mass(g, h->sm, gamma, cprim(A.rs[s]), ga, A.muX, h->stream))) return er;
+    if ((er = pcf::launch_adj_mass(g, h->sm, gamma, cprim(A.rs[s]), ga, A.muX, A.stream))) return er;
   }
   return hipSuccess;
 }
@@ -1167,8 +1266,13 @@ int32_t pcf_create(const pcf_config *cfg, pcf_handle **out) {
 
 int32_t pcf_destroy(pcf_handle *h) {
   if (!h) return PCF_OK;
+  if (h->adj.stream) (void)hipStreamSynchronize(h->adj.stream);
+  for (hipEvent_t ev : {h->adj.ev_seed, h->adj.ev_rc[0], h->adj.ev_rc[1], h->adj.ev_rev[0], h->adj.ev_rev[1]})
+    if (ev) (void)hipEventDestroy(ev);
+  if (h->adj.stream) (void)hipStreamDestroy(h->adj.stream);
   for (void *p : h->adj_allocs) (void)hipFree(p);
   for (auto &kv : h->graphs) (void)hipGraphExecDestroy(kv.second);
+  for (auto &kv : h->seg_graphs) (void)hipGraphExecDestroy(kv.second);
   for (void *p : h->allocs) (void)hipFree(p);
   if (h->own_stream && h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
@@ -1189,6 +1293,8 @@ int32_t pcf_set_stream(pcf_handle *h, void *stream) {
   }
   for (auto &kv : h->graphs) (void)hipGraphExecDestroy(kv.second);
   h->graphs.clear();
+  for (auto &kv : h->seg_graphs) (void)hipGraphExecDestroy(kv.second);
+  h->seg_graphs.clear();
   return PCF_OK;
 }
 
@@ -1477,64 +1583,79 @@ int32_t pcf_vjp(pcf_handle *h, int32_t steps, int32_t checkpoint_every, int32_t 
     return fail(PCF_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
   };
   hipError_t e;
+  // Runs segment seg's steps on the handle stream from the current state,
+  // keeping the tape in set seg & 1 when keep is set; first[i] receives the
+  // tape slot of step ka + i.
+  auto sweep = [&](int seg, bool keep, std::vector<int> &first) -> int32_t {
+    const int ka = seg * every, kb = std::min(steps, ka + every);
+    const int base = (seg & 1) * A.per_set;
+    first.assign(kb - ka, 0);
+    int slot = base;
+    for (int k = ka; k < kb;) {
+      if (h->history_steps >= 2.0) {  // steady to the end of the segment
+        for (int i = 0; i < kb - k; ++i) first[k - ka + i] = slot + i;
+        return steady_run(h, kb - k, keep ? slot : -1);
+      }
+      first[k - ka] = slot;
+      int32_t r = startup_step(h, keep ? &A.tape[slot] : nullptr);
+      if (r) return r;
+      slot += 3;
+      ++k;
+    }
+    return PCF_OK;
+  };
   // forward, keeping a snapshot at the start of every segment; the last
   // segment keeps its tape, so it is not recomputed
-  const int last_ka = (nseg - 1) * every;
-  std::vector<int> first_last;
-  {
-    int slot = 0;
-    for (int k = 0; k < steps; ++k) {
-      if (k % every == 0 && (e = snap_save(h, A.snaps[k / every]))) return hip_fail(e, "snapshot");
-      const bool st = h->history_steps >= 2.0;
-      void *const *tp = nullptr;
-      if (k >= last_ka) {
-        first_last.push_back(slot);
-        tp = &A.tape[slot];
-        slot += st ? 1 : 3;
-      }
-      r = st ? steady_step(h, tp) : startup_step(h, tp);
-      if (r) return r;
-    }
+  std::vector<int> first_rev, first_next;
+  for (int seg = 0; seg < nseg; ++seg) {
+    if ((e = snap_save(h, A.snaps[seg]))) return hip_fail(e, "snapshot");
+    if ((r = sweep(seg, seg == nseg - 1, first_rev))) return r;
   }
   pcf_handle::Snap &fin = A.snaps[nseg];
   if ((e = snap_save(h, fin))) return hip_fail(e, "snapshot");
-  // seed: mu of the final state's (u, g) -> mu_{x_K}
+  // seed (on the reverse stream, after the forward): mu of the final state's
+  // (u, g) -> mu_{x_K}
+  PCF_HIP(hipEventRecord(A.ev_seed, h->stream));
+  PCF_HIP(hipStreamWaitEvent(A.stream, A.ev_seed, 0));
   if (seed == PCF_SEED_ENERGY) {
     const double scale = 2.0 * h->cfg.ly * h->cfg.lz;
     if ((e = pcf::launch_energy_seed(g, A.QB, A.QD, scale, h->u0[h->so[0]], h->u1, h->u2, A.mu_u[0],
-                                     A.mu_u[1], A.mu_u[2], h->stream)))
+                                     A.mu_u[1], A.mu_u[2], A.stream)))
       return hip_fail(e, "energy seed");
-    e = pcf::launch_recon_adj(g, A.RcT, A.ldrT, A.mu_u[0], A.mu_u[1], A.mu_u[2], nullptr, A.muX,
-                              h->stream);
+    e = pcf::launch_recon_adj(g, A.RcT, A.ldrT, A.mu_u[0], A.mu_u[1], A.mu_u[2], nullptr, A.muX, A.stream);
   } else {
-    e = pcf::launch_recon_adj(g, A.RcT, A.ldrT, A.cot_u[0], A.cot_u[1], A.cot_u[2], A.cot_g, A.muX,
-                              h->stream);
+    e = pcf::launch_recon_adj(g, A.RcT, A.ldrT, A.cot_u[0], A.cot_u[1], A.cot_u[2], A.cot_g, A.muX, A.stream);
   }
   if (e) return hip_fail(e, "seed");
   for (int i = 0; i < 3; ++i) {
     if ((e = zero_prim(h, A.rho[i]))) return hip_fail(e, "memset");
     A.rr[i] = i;
   }
-  // reverse, segment by segment from the last
-  std::vector<int> first;
+  // the recompute overwrites the state the seed reads
+  PCF_HIP(hipEventRecord(A.ev_seed, A.stream));
+  PCF_HIP(hipStreamWaitEvent(h->stream, A.ev_seed, 0));
+  // reverse, segment by segment from the last; the recompute of segment
+  // seg - 1 (handle stream) overlaps the reverse of segment seg (reverse
+  // stream).  Tape set b is rewritten only after the reverse that read it
+  // (ev_rev[b]) and read only after the recompute that wrote it (ev_rc[b]).
+  bool rev_pending[2] = {false, false};
   for (int seg = nseg - 1; seg >= 0; --seg) {
     const int ka = seg * every, kb = std::min(steps, ka + every);
-    if (seg == nseg - 1) {
-      first = first_last;
-    } else {
-      if ((e = snap_load(h, A.snaps[seg]))) return hip_fail(e, "restore");
-      first.assign(kb - ka, 0);
-      int slot = 0;
-      for (int k = ka; k < kb; ++k) {
-        const bool st = h->history_steps >= 2.0;
-        first[k - ka] = slot;
-        r = st ? steady_step(h, &A.tape[slot]) : startup_step(h, &A.tape[slot]);
-        if (r) return r;
-        slot += st ? 1 : 3;
-      }
+    if (seg < nseg - 1) {
+      first_rev.swap(first_next);
+      PCF_HIP(hipStreamWaitEvent(A.stream, A.ev_rc[seg & 1], 0));
+    }
+    if (seg > 0) {
+      const int b = (seg - 1) & 1;
+      if (rev_pending[b]) PCF_HIP(hipStreamWaitEvent(h->stream, A.ev_rev[b], 0));
+      if ((e = snap_load(h, A.snaps[seg - 1]))) return hip_fail(e, "restore");
+      if ((r = sweep(seg - 1, true, first_next))) return r;
+      PCF_HIP(hipEventRecord(A.ev_rc[b], h->stream));
     }
     for (int k = kb - 1; k >= ka; --k)
-      if ((e = reverse_step(h, k, h0 + k >= 2, &A.tape[first[k - ka]]))) return hip_fail(e, "reverse step");
+      if ((e = reverse_step(h, k, h0 + k >= 2, &A.tape[first_rev[k - ka]]))) return hip_fail(e, "reverse step");
+    PCF_HIP(hipEventRecord(A.ev_rev[seg & 1], A.stream));
+    rev_pending[seg & 1] = true;
   }
   // history leaves of the initial state: levels -1 and -2
   {
@@ -1545,18 +1666,19 @@ int32_t pcf_vjp(pcf_handle *h, int32_t steps, int32_t checkpoint_every, int32_t 
       if ((e = zero_prim(h, A.gh[i]))) return hip_fail(e, "memset");
     ps_add(s, r0, C_SBDF[1]);
     ps_add(s, r1, C_SBDF[2]);
-    if ((e = pcf::launch_adj_mass(g, h->sm, 0.0, pcf::PrimC{}, s, A.gh[0], h->stream))) return hip_fail(e, "history");
+    if ((e = pcf::launch_adj_mass(g, h->sm, 0.0, pcf::PrimC{}, s, A.gh[0], A.stream))) return hip_fail(e, "history");
     s = pcf::PrimSum{};
     ps_add(s, r0, C_SBDF[2]);
-    if ((e = pcf::launch_adj_mass(g, h->sm, 0.0, pcf::PrimC{}, s, A.gh[1], h->stream))) return hip_fail(e, "history");
+    if ((e = pcf::launch_adj_mass(g, h->sm, 0.0, pcf::PrimC{}, s, A.gh[1], A.stream))) return hip_fail(e, "history");
     s = pcf::PrimSum{};
     ps_add(s, r0, e1);
     ps_add(s, r1, e2);
-    if ((e = pcf::launch_prim_sum(g, s, A.gh[2], h->stream))) return hip_fail(e, "history");
+    if ((e = pcf::launch_prim_sum(g, s, A.gh[2], A.stream))) return hip_fail(e, "history");
     s = pcf::PrimSum{};
     ps_add(s, r0, e2);
-    if ((e = pcf::launch_prim_sum(g, s, A.gh[3], h->stream))) return hip_fail(e, "history");
+    if ((e = pcf::launch_prim_sum(g, s, A.gh[3], A.stream))) return hip_fail(e, "history");
   }
+  PCF_HIP(hipStreamSynchronize(A.stream));
   if ((e = snap_load(h, fin))) return hip_fail(e, "restore");
   PCF_HIP(hipStreamSynchronize(h->stream));
   A.have_grad = true;

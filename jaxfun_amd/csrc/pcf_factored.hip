@@ -378,7 +378,8 @@ __global__ void __launch_bounds__(256) k_solve_factored(Geo g, StructMats sm, Fa
 template <int SL>
 __global__ void __launch_bounds__(256) k_adj_solve_factored(Geo g, StructMats smt, FactorSet fs, PrimC in,
                                                             Prim out, Prim nuo, PrimC r1, PrimC r2,
-                                                            double e0, double e1, double e2) {
+                                                            double e0, double e1, double e2, Prim sg,
+                                                            double c0, double c1, double c2) {
   extern __shared__ __align__(16) unsigned char smem[];
   constexpr int SLP = SL + 1;
   const int M = g.m;
@@ -445,21 +446,28 @@ __global__ void __launch_bounds__(256) k_adj_solve_factored(Geo g, StructMats sm
     if (ln == SL) {
       out.v[J] = x.x;
       out.w[J] = x.y;
-      nuo.v[J] = e0 * x.x + e1 * r1.v[J] + e2 * r2.v[J];
-      nuo.w[J] = e0 * x.y + e1 * r1.w[J] + e2 * r2.w[J];
+      const double2 p{r1.v[J], r1.w[J]}, q{r2.v[J], r2.w[J]};
+      nuo.v[J] = e0 * x.x + e1 * p.x + e2 * q.x;
+      nuo.w[J] = e0 * x.y + e1 * p.y + e2 * q.y;
+      if (sg.u0) {
+        sg.v[J] = c0 * x.x + c1 * p.x + c2 * q.x;
+        sg.w[J] = c0 * x.y + c1 * p.y + c2 * q.y;
+      }
       continue;
     }
     const int m = m0 + ln;
     const size_t o = (size_t)J * M + m;
-    double2 v{0, 0};
+    double2 v{0, 0}, sv{0, 0};
     if (m / g.kzs != g.ny / 2) {  // the Nyquist row stays masked
       const double2 p = (sys == 0 ? r1.u0 : r1.g)[o], q = (sys == 0 ? r2.u0 : r2.g)[o];
       v = double2{e0 * x.x + e1 * p.x + e2 * q.x, e0 * x.y + e1 * p.y + e2 * q.y};
+      sv = double2{c0 * x.x + c1 * p.x + c2 * q.x, c0 * x.y + c1 * p.y + c2 * q.y};
     } else {
       x = double2{0, 0};
     }
     (sys == 0 ? out.u0 : out.g)[o] = x;
     (sys == 0 ? nuo.u0 : nuo.g)[o] = v;
+    if (sg.u0) (sys == 0 ? sg.u0 : sg.g)[o] = sv;
   }
 }
 
@@ -498,11 +506,11 @@ hipError_t launch_solve_factored(const Geo &g, const StructMats &sm, const Facto
 
 hipError_t launch_adj_solve_factored(const Geo &g, const StructMats &smt, const FactorSet &fs, PrimC in,
                                      Prim out, Prim nu, PrimC r1, PrimC r2, double e0, double e1, double e2,
-                                     hipStream_t s) {
+                                     hipStream_t s, Prim sig, double c0, double c1, double c2) {
   auto lds = [&](int sl) { return (size_t)g.npmax * (2 + NF_B) * (sl + 1) * sizeof(double); };
 #define PCF_ASOLVE(SLV)                                                                                    \
   hipLaunchKernelGGL(k_adj_solve_factored<SLV>, dim3(4 * ((g.m + SLV - 1) / SLV)), dim3(256), lds(SLV), s, \
-                     g, smt, fs, in, out, nu, r1, r2, e0, e1, e2)
+                     g, smt, fs, in, out, nu, r1, r2, e0, e1, e2, sig, c0, c1, c2)
   if (lds(16) <= 65536) PCF_ASOLVE(16);
   else if (lds(8) <= 65536) PCF_ASOLVE(8);
   else return hipErrorInvalidValue;

@@ -546,6 +546,176 @@ __global__ void __launch_bounds__(512) k_nodal_adj_mfma(Geo g, AdjMats<T> am,
   }
 }
 
+// The same contraction with the reconstruction adjoint and the steady SBDF3
+// mass gather folded in, producing mu_x of a steady reverse step directly:
+//   mu_x.u0 = VB^T t0 + (VD R)^T c1 + (VD1 R)^T c2 + M0^T sig.u0,
+//   mu_x.g  = VD^T d1 + VD1^T d2 + M1^T sig.g,
+// with t0..t2, w1, w2 the staged fields above, a = ky / k^2, b = kz / k^2 and
+//   c1 = -i (a t1 + b t2), c2 = -i (a w2 - b w1),
+//   d1 = -i (b t1 - a t2), d2 = -i (b w2 + a w1),
+// since mu_u1 = VD^T t1 + VD1^T w2, mu_u2 = VD^T t2 - VD1^T w1 and the
+// reconstruction u1 = i (a f + b g), u2 = i (b f - a g), f = R u0
+// (examples/channelflow_kmm.py:855-870).  Mode 0 has no reconstruction: its
+// u0 rows are VB^T t0 and its (real) mean cotangents ride in the g rows as
+// d1 = t1 + i t2, d2 = w2 - i w1, so the g accumulator holds mu_v + i mu_w.
+// sig = sum_j C_j rho_{k+j} (written by the transposed solve); M0 = W (C2_B -
+// k^2 C0_B) and M1 = W C0_D are banded per parity block, the mean rows use
+// C0_D without W.  Nyquist row: zero.
+constexpr int STAGE_SIG = 4;
+template <typename T>
+__global__ void __launch_bounds__(512) k_nodal_adj_x_mfma(Geo g, AdjMats<T> am, StructMats sm,
+                                                          const typename VT<T>::V *__restrict__ X,
+                                                          PrimC sig, Prim out) {
+  using MM = Mfma<T>;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int ct = blockIdx.x;
+  const int M = g.m, kp = (g.n0 + 3) / 4 * 4, nb = g.nb, nd = g.nd;
+  const size_t plane = (size_t)g.n0 * M;
+  const Panel<T> B{reinterpret_cast<T *>(smem), kp};
+  double2 *sg = reinterpret_cast<double2 *>(smem + (size_t)5 * kp * 16 * sizeof(T));  // [nb + nd][8]
+  using V = typename VT<T>::V;
+  const int total = kp * 8, stot = (nb + nd) * 8;
+  V xv[STAGE_NODAL_ADJ][6];
+  double2 sv[STAGE_SIG];
+#pragma unroll
+  for (int t = 0; t < STAGE_NODAL_ADJ; ++t) {
+    const int idx = threadIdx.x + t * blockDim.x;
+    const int x = idx >> 3, m = 8 * ct + (idx & 7);
+#pragma unroll
+    for (int f = 0; f < 6; ++f) xv[t][f] = V{0, 0};
+    if (idx < total && m < M && x < g.n0 && !nyq_mode(g, m)) {
+      const size_t o = (size_t)x * M + m;
+#pragma unroll
+      for (int f = 0; f < 6; ++f) xv[t][f] = X[f * plane + o];
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < STAGE_SIG; ++t) {
+    const int idx = threadIdx.x + t * blockDim.x;
+    const int r = idx >> 3, m = 8 * ct + (idx & 7);
+    sv[t] = double2{0, 0};
+    if (idx < stot && m < M) sv[t] = (r < nb) ? sig.u0[(size_t)r * M + m] : sig.g[(size_t)(r - nb) * M + m];
+  }
+#pragma unroll
+  for (int t = 0; t < STAGE_NODAL_ADJ; ++t) {
+    const int idx = threadIdx.x + t * blockDim.x;
+    if (idx >= total) continue;
+    const int x = idx >> 3, j = idx & 7;
+    const int m = 8 * ct + j;
+    T v[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    if (m < M && x < g.n0 && !nyq_mode(g, m)) {
+      double kyd, kzd, kyo;
+      mode_k(g, m, kyd, kzd, kyo);
+      const T ky = (T)kyo, kz = (T)kzd;
+      const V X0 = xv[t][0], X1 = xv[t][1], X2 = xv[t][2];
+      const V W0 = xv[t][3], W1 = xv[t][4], W2 = xv[t][5];
+      const T t0r = X0.x + kz * W1.y - ky * W2.y, t0i = X0.y - kz * W1.x + ky * W2.x;
+      const T t1r = X1.x - kz * W0.y, t1i = X1.y + kz * W0.x;
+      const T t2r = X2.x + ky * W0.y, t2i = X2.y - ky * W0.x;
+      v[0] = t0r;
+      v[1] = t0i;
+      if (m == 0) {
+        v[6] = t1r;  // d1 = t1 + i t2 (real parts)
+        v[7] = t2r;
+        v[8] = W2.x;  // d2 = w2 - i w1
+        v[9] = -W1.x;
+      } else {
+        const double k2 = kyd * kyd + kzd * kzd;
+        const T a = (T)(kyd / k2), b = (T)(kzd / k2);
+        // -i z = (z.y, -z.x)
+        const T c1r = a * t1r + b * t2r, c1i = a * t1i + b * t2i;
+        const T c2r = a * W2.x - b * W1.x, c2i = a * W2.y - b * W1.y;
+        const T d1r = b * t1r - a * t2r, d1i = b * t1i - a * t2i;
+        const T d2r = b * W2.x + a * W1.x, d2i = b * W2.y + a * W1.y;
+        v[2] = c1i;
+        v[3] = -c1r;
+        v[4] = c2i;
+        v[5] = -c2r;
+        v[6] = d1i;
+        v[7] = -d1r;
+        v[8] = d2i;
+        v[9] = -d2r;
+      }
+    }
+#pragma unroll
+    for (int f = 0; f < 5; ++f) {
+      B.at(f, x, 2 * j) = v[2 * f];
+      B.at(f, x, 2 * j + 1) = v[2 * f + 1];
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < STAGE_SIG; ++t) {
+    const int idx = threadIdx.x + t * blockDim.x;
+    if (idx < stot) sg[idx] = sv[t];
+  }
+  __syncthreads();
+  const Lane L = lane_of(g, ct);
+  const int jl = L.c >> 1;
+  double kyd, kzd, kyo;
+  mode_k(g, L.valid ? L.mcol : 0, kyd, kzd, kyo);
+  const double k2 = kyd * kyd + kzd * kzd;
+  const bool live = L.valid && !nyq_mode(g, L.mcol);
+  auto sgc = [&](int r) { return L.comp ? sg[r * 8 + jl].y : sg[r * 8 + jl].x; };
+  double *ou = reinterpret_cast<double *>(out.u0);
+  double *og = reinterpret_cast<double *>(out.g);
+  for (int i0 = (threadIdx.x >> 6) * 16; i0 < nd; i0 += (blockDim.x >> 6) * 16) {
+    const int ia = i0 + (L.l & 15);
+    typename MM::acc AU{}, AG{};
+#pragma unroll 4
+    for (int kk = 0; kk < kp; kk += 4) {
+      const int x = kk + (L.l >> 4);
+      T ab = 0, ar = 0, ar1 = 0, ad = 0, ad1 = 0;
+      if (x < g.n0) {
+        const size_t o = (size_t)x * am.ldi + ia;
+        ab = am.VB[o];
+        ar = am.VDR[o];
+        ar1 = am.VD1R[o];
+        ad = am.VD[o];
+        ad1 = am.VD1[o];
+      }
+      AU = MM::mma(ab, B.at(0, x, L.c), AU);
+      AU = MM::mma(ar, B.at(1, x, L.c), AU);
+      AU = MM::mma(ar1, B.at(2, x, L.c), AU);
+      AG = MM::mma(ad, B.at(3, x, L.c), AG);
+      AG = MM::mma(ad1, B.at(4, x, L.c), AG);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int i = i0 + MM::row(L.l, r);
+      if (!L.valid || i >= nd) continue;
+      const int par = i & 1, jp = i >> 1;
+      const size_t o = (size_t)i * 2 * M + 2 * L.mcol + L.comp;
+      if (i < nb) {
+        double u = 0.0;
+        if (live) {
+          const StructSys &s0 = sm.sys[0][par];
+          u = (double)AU[r];
+          for (int d = -2; d <= 2; ++d) {
+            const int ii = jp - d;
+            if (ii < 0 || ii >= s0.n) continue;
+            const size_t bo = (size_t)ii * 5 + (d + 2);
+            u += g.W * (s0.b2[bo] - k2 * s0.b0[bo]) * sgc(2 * ii + par);
+          }
+        }
+        ou[o] = u;
+      }
+      double gv = 0.0, mv = (double)AG[r];
+      const StructSys &s1 = sm.sys[1][par];
+      if (live) {
+        for (int d = -1; d <= 1; ++d) {
+          const int ii = jp - d;
+          if (ii < 0 || ii >= s1.n) continue;
+          const double c = s1.b0[(size_t)ii * 3 + (d + 1)];
+          gv += g.W * c * sgc(nb + 2 * ii + par);
+          if (L.mcol == 0) mv += c * (L.comp ? sig.w : sig.v)[2 * ii + par];
+        }
+      }
+      og[o] = (L.mcol == 0) ? gv : (live ? gv + (double)AG[r] : 0.0);
+      if (L.mcol == 0) (L.comp ? out.w : out.v)[i] = mv;
+    }
+  }
+}
+
 // mu_h0 = W k^2 A0^T nu_u, mu_h1 = -i W (ky A1^T nu_u + kz AG^T nu_g),
 // mu_h2 = -i W (kz A1^T nu_u - ky AG^T nu_g) (rows: nodes x, K: Shen index i);
 // mean mode: mu_h1,2 -= AG^T nu_v,w.  Nyquist row: zero.
@@ -711,6 +881,23 @@ hipError_t launch_nodal_adj_mfma(const Geo &g, const AdjMats<T> &am, Workspace &
 }
 
 template <typename T>
+static size_t nodal_adj_x_bytes(const Geo &g) {
+  return panel_bytes<T>(5, g.n0) + (size_t)(g.nb + g.nd) * 8 * sizeof(double2);
+}
+
+template <typename T>
+hipError_t launch_nodal_adj_x_mfma(const Geo &g, const AdjMats<T> &am, const StructMats &sm, PrimC sig,
+                                   Workspace &ws, Prim out, hipStream_t s) {
+  using V = typename VT<T>::V;
+  if (!am.VDR || !am.VD1R || !sig.u0 || !sig.g || !sig.v || !sig.w) return hipErrorInvalidValue;
+  if (!stage_covers(g.nd, STAGE_NODAL_ADJ, ((g.n0 + 3) / 4 * 4) * 8)) return hipErrorInvalidValue;
+  if (!stage_covers(g.nd, STAGE_SIG, (g.nb + g.nd) * 8)) return hipErrorInvalidValue;
+  hipLaunchKernelGGL((k_nodal_adj_x_mfma<T>), mfma_grid(g), mfma_block(g.nd), nodal_adj_x_bytes<T>(g), s, g,
+                     am, sm, reinterpret_cast<const V *>(ws.X), sig, out);
+  return hipGetLastError();
+}
+
+template <typename T>
 hipError_t launch_rhs_adj_mfma(const Geo &g, const AdjMats<T> &am, Prim nu, Workspace &ws, hipStream_t s) {
   using V = typename VT<T>::V;
   if (!stage_covers(g.n0, STAGE_RHS_ADJ, 2 * ((g.nd + 3) / 4 * 4) * 8)) return hipErrorInvalidValue;
@@ -721,15 +908,23 @@ hipError_t launch_rhs_adj_mfma(const Geo &g, const AdjMats<T> &am, Prim nu, Work
 }
 
 hipError_t mfma_init(const Geo &g, bool fp64) {
-  // the largest panel (five fields of the transform adjoint) may exceed the
-  // default 64 KB of dynamic LDS at the largest sizes
-  const size_t bytes = fp64 ? panel_bytes<double>(5, g.n0) : panel_bytes<float>(5, g.n0);
-  if (bytes <= 65536) return hipSuccess;
-  const void *fn = fp64 ? reinterpret_cast<const void *>(&k_nodal_adj_mfma<double>)
-                        : reinterpret_cast<const void *>(&k_nodal_adj_mfma<float>);
-  hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
-  if (e != hipSuccess) (void)hipGetLastError();
-  return e;
+  // the largest panels (five fields of the transform adjoint, plus the gathered
+  // rows of the fused variant) may exceed the default 64 KB of dynamic LDS
+  auto raise = [](const void *fn, size_t bytes) {
+    if (bytes <= 65536) return hipSuccess;
+    hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    if (e != hipSuccess) (void)hipGetLastError();
+    return e;
+  };
+  hipError_t e;
+  if (fp64) {
+    if ((e = raise(reinterpret_cast<const void *>(&k_nodal_adj_mfma<double>), panel_bytes<double>(5, g.n0))))
+      return e;
+    return raise(reinterpret_cast<const void *>(&k_nodal_adj_x_mfma<double>), nodal_adj_x_bytes<double>(g));
+  }
+  if ((e = raise(reinterpret_cast<const void *>(&k_nodal_adj_mfma<float>), panel_bytes<float>(5, g.n0))))
+    return e;
+  return raise(reinterpret_cast<const void *>(&k_nodal_adj_x_mfma<float>), nodal_adj_x_bytes<float>(g));
 }
 
 #define PCF_MFMA_INSTANTIATE(T)                                                                     \
@@ -742,6 +937,8 @@ hipError_t mfma_init(const Geo &g, bool fp64) {
                                                double2 *, double2 *, double2 *, hipStream_t);       \
   template hipError_t launch_rhs_adj_mfma<T>(const Geo &, const AdjMats<T> &, Prim, Workspace &,    \
                                              hipStream_t);                                          \
+  template hipError_t launch_nodal_adj_x_mfma<T>(const Geo &, const AdjMats<T> &, const StructMats &, \
+                                                 PrimC, Workspace &, Prim, hipStream_t);            \
   template hipError_t launch_nodal_mhd_mfma<T>(const Geo &, const MhdMats<T> &, const double2 *,    \
                                                size_t, Workspace &, hipStream_t);                   \
   template hipError_t launch_rhs_a_mfma<T>(const Geo &, const MhdMats<T> &, Workspace &, double2 *,   \

@@ -133,10 +133,14 @@ def test_gradient_matches_central_differences():
         assert abs(an - fd) <= 1e-6 * abs(fd)
 
 
-def test_checkpoint_interval_does_not_change_gradient():
-    solver = make((17, 16, 16))
+@pytest.mark.parametrize("N,steps,intervals", [((17, 16, 16), 7, (1, 3, 7)), ((33, 32, 32), 24, (24, 2, 5, 5))])
+def test_checkpoint_interval_does_not_change_gradient(N, steps, intervals):
+    """Bit-identical gradients for every checkpoint interval: the recompute of
+    one segment runs concurrently with the reverse sweep of the next on a second
+    stream, with two tape sets; a repeated interval replays cached graphs."""
+    solver = make(N)
     s0 = seeded_state(solver, seed=5, energy=5e-2, K=4)
-    grads = [solver.value_and_grad(s0, 7, checkpoint_every=k)[1] for k in (1, 3, 7)]
+    grads = [solver.value_and_grad(s0, steps, checkpoint_every=k)[1] for k in intervals]
     for g in grads[1:]:
         assert all(np.array_equal(a, b) for a, b in zip(leaves(g), leaves(grads[0])))
 
