@@ -384,6 +384,8 @@ def main(argv=None) -> int:
     ap.add_argument("--no-mhd", action="store_true")
     ap.add_argument("--adjoint-only", action="store_true",
                     help="run only the adjoint timing leg (for kernel profiles)")
+    ap.add_argument("--mhd-only", action="store_true",
+                    help="run only the insulating-wall MHD leg (for kernel profiles)")
     ap.add_argument("--pcie-steps", type=int, default=240,
                     help="steps of the host-boundary (PCIe-inclusive) solve timing (0: skip)")
     ap.add_argument("--eager", action="store_true",
@@ -400,6 +402,9 @@ def main(argv=None) -> int:
         return 0
     steps = max(GRAPH_BLOCK, (args.steps // GRAPH_BLOCK) * GRAPH_BLOCK)
     warmup = max(GRAPH_BLOCK, (args.warmup // GRAPH_BLOCK) * GRAPH_BLOCK)
+    if args.mhd_only:
+        print(json.dumps({"mhd": mhd_leg(min(steps, 1200), warmup)}), flush=True)
+        return 0
 
     solver, state, el, stages, diag = run_gpu("fp64", steps, warmup, args.eager)
     value = steps / el

@@ -93,36 +93,50 @@ __device__ void factor_block(const StructSys &ss, int n, double s0, double s2, d
 
 // Forward then back substitution with stored factors.  load(i) gives the
 // right-hand side of row i, keep(i, y)/kept(i) park the forward result, and
-// store(i, x) receives the solution (rows visited n-1 .. 0).
+// store(i, x) receives the solution (rows visited n-1 .. 0).  Rows go in
+// chunks of SC: the chunk's right-hand sides, factors and far coefficients are
+// read into registers first (independent loads, one wait), then the serial
+// chain runs on registers only, so a row costs its few dependent FMAs rather
+// than a load latency.
 template <int P, int Q, int R, class Load, class Keep, class Kept, class Store>
-__device__ void substitute(const StructSys &ss, int n, const double *F, size_t rs, size_t vs, Load load,
-                           Keep keep, Kept kept, Store store) {
-  constexpr int F1 = Q + 1;
+__device__ __forceinline__ void substitute(const StructSys &ss, int n, const double *F, size_t rs, size_t vs,
+                                           Load load, Keep keep, Kept kept, Store store) {
+  constexpr int F1 = Q + 1, SC = 8;
   {
     double2 yw[P + 1];
 #pragma unroll
     for (int t = 0; t <= P; ++t) yw[t] = double2{0, 0};
-    double ln[P + 1];
+    for (int i0 = 0; i0 < n; i0 += SC) {
+      double2 r[SC];
+      double l[SC][P + 1];
 #pragma unroll
-    for (int t = 1; t <= P; ++t) ln[t] = F[(Q + R + t) * vs];
-    for (int i = 0; i < n; ++i) {
-      double l[P + 1];
+      for (int c = 0; c < SC; ++c) {
+        const int i = i0 + c;
+        r[c] = double2{0, 0};
 #pragma unroll
-      for (int t = 1; t <= P; ++t) l[t] = ln[t];
-      if (i + 1 < n) {
+        for (int t = 1; t <= P; ++t) l[c][t] = 0.0;
+        if (i < n) {
+          r[c] = load(i);
 #pragma unroll
-        for (int t = 1; t <= P; ++t) ln[t] = F[(size_t)(i + 1) * rs + (Q + R + t) * vs];
+          for (int t = 1; t <= P; ++t) l[c][t] = F[(size_t)i * rs + (Q + R + t) * vs];
+        }
       }
-      double2 y = load(i);
 #pragma unroll
-      for (int t = 1; t <= P; ++t) {
-        y.x -= l[t] * yw[t].x;
-        y.y -= l[t] * yw[t].y;
+      for (int c = 0; c < SC; ++c) {
+        const int i = i0 + c;
+        if (i < n) {
+          double2 y = r[c];
+#pragma unroll
+          for (int t = 1; t <= P; ++t) {
+            y.x -= l[c][t] * yw[t].x;
+            y.y -= l[c][t] * yw[t].y;
+          }
+          keep(i, y);
+#pragma unroll
+          for (int t = P; t >= 2; --t) yw[t] = yw[t - 1];
+          yw[1] = y;
+        }
       }
-      keep(i, y);
-#pragma unroll
-      for (int t = P; t >= 2; --t) yw[t] = yw[t - 1];
-      yw[1] = y;
     }
   }
   {
@@ -132,40 +146,55 @@ __device__ void substitute(const StructSys &ss, int n, const double *F, size_t r
     double2 tau[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) tau[r] = double2{0, 0};
-    double cur[1 + Q + R], nxt[1 + Q + R];
+    for (int i1 = n - 1; i1 >= 0; i1 -= SC) {
+      double2 y[SC];
+      double f[SC][1 + Q + R], v[SC][R];
 #pragma unroll
-    for (int v = 0; v < 1 + Q + R; ++v) cur[v] = F[(size_t)(n - 1) * rs + v * vs];
-    for (int i = n - 1; i >= 0; --i) {
-      if (i > 0) {
+      for (int c = 0; c < SC; ++c) {
+        const int i = i1 - c;
+        y[c] = double2{0, 0};
 #pragma unroll
-        for (int v = 0; v < 1 + Q + R; ++v) nxt[v] = F[(size_t)(i - 1) * rs + v * vs];
-      }
-      if (i + F1 < n) {
+        for (int w = 0; w < 1 + Q + R; ++w) f[c][w] = 0.0;
 #pragma unroll
-        for (int r = 0; r < R; ++r) {
-          const double vv = ss.V[(size_t)(i + F1) * R + r];
-          tau[r].x += vv * xw[F1].x;
-          tau[r].y += vv * xw[F1].y;
+        for (int r = 0; r < R; ++r) v[c][r] = 0.0;
+        if (i >= 0) {
+          y[c] = kept(i);
+#pragma unroll
+          for (int w = 0; w < 1 + Q + R; ++w) f[c][w] = F[(size_t)i * rs + w * vs];
+          if (i + F1 < n) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[c][r] = ss.V[(size_t)(i + F1) * R + r];
+          }
         }
       }
-      double2 acc = kept(i);
 #pragma unroll
-      for (int q = 1; q <= Q; ++q) {
-        acc.x -= cur[q] * xw[q].x;
-        acc.y -= cur[q] * xw[q].y;
+      for (int c = 0; c < SC; ++c) {
+        const int i = i1 - c;
+        if (i >= 0) {
+          // tau_r = sum_{j > i + Q} V[j][r] x_j; x_{i+F1} joins it at row i
+#pragma unroll
+          for (int r = 0; r < R; ++r) {
+            tau[r].x += v[c][r] * xw[F1].x;
+            tau[r].y += v[c][r] * xw[F1].y;
+          }
+          double2 acc = y[c];
+#pragma unroll
+          for (int q = 1; q <= Q; ++q) {
+            acc.x -= f[c][q] * xw[q].x;
+            acc.y -= f[c][q] * xw[q].y;
+          }
+#pragma unroll
+          for (int r = 0; r < R; ++r) {
+            acc.x -= f[c][1 + Q + r] * tau[r].x;
+            acc.y -= f[c][1 + Q + r] * tau[r].y;
+          }
+          const double2 xi{acc.x * f[c][0], acc.y * f[c][0]};
+          store(i, xi);
+#pragma unroll
+          for (int k = F1; k >= 2; --k) xw[k] = xw[k - 1];
+          xw[1] = xi;
+        }
       }
-#pragma unroll
-      for (int r = 0; r < R; ++r) {
-        acc.x -= cur[1 + Q + r] * tau[r].x;
-        acc.y -= cur[1 + Q + r] * tau[r].y;
-      }
-      const double2 xi{acc.x * cur[0], acc.y * cur[0]};
-      store(i, xi);
-#pragma unroll
-      for (int k = F1; k >= 2; --k) xw[k] = xw[k - 1];
-      xw[1] = xi;
-#pragma unroll
-      for (int v = 0; v < 1 + Q + R; ++v) cur[v] = nxt[v];
     }
   }
 }
