@@ -15,7 +15,9 @@ Also reported on the same line:
     roofline of the dominant kernel;
   * the CPU FP64 oracle timed on this host's cores on a bounded sample;
   * the adjoint/forward cost ratio with its finite-difference check;
-  * the insulating-wall MHD step (FP64 and mixed) at the same resolution.
+  * the insulating-wall MHD step (FP64 and mixed) at the same resolution;
+  * FP64 steps/s on the reference's other published configurations (hydro
+    97^3 and 128^3, conducting-wall MHD 65^3 and 97^3) beside its figures.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -370,6 +372,42 @@ def mhd_leg(steps: int, warmup: int) -> dict:
     return out
 
 
+# The reference's other published FP64 throughputs (RTX 5090 Laptop, Re = Rm =
+# 400, dt = 1e-3, analytic initial field of amplitude 0.02 / magnetic 0.005,
+# benchmark-results/FINAL_LOCAL_CHECKPOINT.md:58-59,79-81): ms per step.
+PUBLISHED = [("hydro", (97, 96, 96), 92.94), ("hydro", (128, 128, 128), 219.72),
+             ("mhd_conducting", (65, 64, 64), 45.790), ("mhd_conducting", (97, 96, 96), 163.28)]
+
+
+def published_leg(steps: int = 120, warmup: int = 24) -> list:
+    """Steady FP64 steps/s of this build on each published configuration (same
+    solver class, N, Re, Rm, dt and initial field), beside the published rate."""
+    from jaxfun_amd import PlaneCouetteFluctuationHIP, PlaneCouetteMHDHIP
+
+    out = []
+    for kind, N, ms in PUBLISHED:
+        if kind == "hydro":
+            solver = PlaneCouetteFluctuationHIP(N=N, Re=400.0, dt=1e-3, perturbation_amplitude=0.02)
+        else:
+            solver = PlaneCouetteMHDHIP(N=N, Re=400.0, Rm=400.0, dt=1e-3, perturbation_amplitude=0.02,
+                                        magnetic_amplitude=0.005)
+        solver.load(solver.initial_state())
+        solver.advance(2)
+        solver.advance_steady(warmup, graph_block=GRAPH_BLOCK)
+        solver.synchronize()
+        t0 = time.perf_counter()
+        solver.advance_steady(steps, graph_block=GRAPH_BLOCK)
+        solver.synchronize()
+        el = time.perf_counter() - t0
+        solver.close()
+        value = steps / el
+        out.append({"solver": kind, "N": list(N), "dtype": "f64", "value": value, "unit": "steps/s",
+                    "published_steps_per_s": 1e3 / ms, "vs_published": value * ms / 1e3})
+        print(f"# published config {kind} {N}: {value:.1f} steps/s ({value * ms / 1e3:.0f}x)", file=sys.stderr,
+              flush=True)
+    return out
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -384,6 +422,8 @@ def main(argv=None) -> int:
     ap.add_argument("--no-mhd", action="store_true")
     ap.add_argument("--adjoint-only", action="store_true",
                     help="run only the adjoint timing leg (for kernel profiles)")
+    ap.add_argument("--no-published", action="store_true",
+                    help="skip the FP64 runs on the reference's other published configurations")
     ap.add_argument("--mhd-only", action="store_true",
                     help="run only the insulating-wall MHD leg (for kernel profiles)")
     ap.add_argument("--pcie-steps", type=int, default=240,
@@ -461,6 +501,8 @@ def main(argv=None) -> int:
               f"{line['mhd']['mixed']['value']:.1f} steps/s; mixed error "
               f"{max(v for k, v in line['mhd']['mixed_error_vs_fp64'].items() if k != 'steps'):.2e}",
               file=sys.stderr, flush=True)
+    if not args.no_published:
+        line["published_configs"] = published_leg()
     if args.copy_calibration:
         line["copy_calibration"] = copy_calibration(args.copy_calibration)
     if not args.no_cpu:

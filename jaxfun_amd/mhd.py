@@ -1,4 +1,4 @@
-"""Plane-Couette MHD on the MI355X: drop-in for the reference's conducting-wall solver.
+"""Plane-Couette MHD on the MI355X: drop-ins for the reference's vector-potential solvers.
 
 ``PlaneCouetteMHDHIP`` mirrors ``PlaneCouetteMHDJax``
 (reference ``examples/pcf_mhd_jax.py:69-789``): the KMM flow of
@@ -6,8 +6,12 @@
 A = (Ax, Ay, Az) in the Dirichlet space TD^3 (conducting walls), with
 B = curl A, J = curl B, the Lorentz force J x B in the flow forcing and the
 induction EMF u_tot x B driving A, advanced on one SBDF3 timeline with an
-IMEXRK3 start-up.  The state is ``MHDState`` with the reference's fields in
-its full-complex layout; every step runs in libpcfdns.so.
+IMEXRK3 start-up.  ``PlaneCouetteMRIShearpyHIP`` and
+``PlaneCouetteMRIShearpyInsulatingHIP`` mirror the rotating-shear classes of
+``examples/pcf_mhd_mri_shearpy_jax.py`` (conducting and insulating walls), the
+two solvers the reference's vector-potential runner constructs.  The state is
+``MHDState`` with the reference's fields in its full-complex layout; every
+step runs in libpcfdns.so.
 """
 
 from __future__ import annotations
@@ -142,6 +146,7 @@ class PlaneCouetteMHDHIP(PlaneCouetteFluctuationHIP):
         out = super().diagnostics(state)
         m = np.zeros(4)
         nat.check(self._lib.pcf_mhd_diagnostics(self._h, m.ctypes.data_as(C.POINTER(C.c_double))))
+        self._mhd_diag = m
         out["divB_L2"] = float(m[1])
         out["Emag"] = float(m[0])
         out["Emag_total"] = float(m[2])
@@ -159,15 +164,14 @@ class PlaneCouetteMHDHIP(PlaneCouetteFluctuationHIP):
         return (*state.flow.u, state.flow.g, *state.A)
 
 
-class PlaneCouetteMRIShearpyInsulatingHIP(PlaneCouetteMHDHIP):
-    """Rotating-shear plane-Couette MHD with insulating (vacuum-matched) walls.
+class PlaneCouetteMRIShearpyHIP(PlaneCouetteMHDHIP):
+    """Rotating-shear plane-Couette MHD with conducting walls.
 
-    Mirrors ``PlaneCouetteMRIShearpyInsulatingJax``
-    (``examples/pcf_mhd_mri_shearpy_jax.py:30-99,516-836``): base flow
-    U_b = -S x, Coriolis force of rotation rate ``omega``, uniform background
-    field ``background_b``, A_x in TD and A_y, A_z in the orthogonal Chebyshev
-    space TC with per-mode vacuum-matching wall rows.  Viscosity 1/Re and
-    magnetic diffusivity 1/Rm (U_wall = 1 as in the reference).
+    Mirrors ``PlaneCouetteMRIShearpyJax``
+    (``examples/pcf_mhd_mri_shearpy_jax.py:30-514``): base flow U_b = -S x,
+    Coriolis force of rotation rate ``omega``, uniform background field
+    ``background_b``, A in TD^3 as in ``PlaneCouetteMHDHIP``.  Viscosity 1/Re
+    and magnetic diffusivity 1/Rm (U_wall = 1 as in the reference).
     """
 
     def __init__(self, N=(17, 16, 16), domain=((-1.0, 1.0), (0.0, 4.0 * math.pi), (0.0, 2.0 * math.pi)),
@@ -197,12 +201,12 @@ class PlaneCouetteMRIShearpyInsulatingHIP(PlaneCouetteMHDHIP):
 
     def _configure(self, cfg) -> None:
         super()._configure(cfg)
-        cfg.walls = nat.WALLS_INSULATING
         cfg.omega = self.omega
         cfg.u_wall = -self.shear_rate  # base flow U_b = -S x (shearpy :89-92)
 
-    def _a_rows(self, comp: int) -> int:
-        return self.nd if comp == 0 else self.N[0]
+    def _a_projection(self, comp: int) -> np.ndarray:
+        """Nodal values -> coefficients of A component comp (Galerkin projection onto TD)."""
+        return self.matrix("FD")
 
     def initial_state(self) -> MHDState:
         """``initial_state`` (``examples/pcf_mhd_mri_shearpy_jax.py:99-147``): the
@@ -211,9 +215,7 @@ class PlaneCouetteMRIShearpyInsulatingHIP(PlaneCouetteMHDHIP):
         n0, n1, n2 = self.N
         xn = np.cos(np.pi + (2 * np.arange(n0) + 1) * np.pi / (2 * n0))
         wall = 1.0 - xn**2
-        FB, FD, WQ = self.matrix("FB"), self.matrix("FD"), self.matrix("WQ")
-        h = np.full(n0, np.pi / 2)
-        h[0] = np.pi
+        FB, FD = self.matrix("FB"), self.matrix("FD")
         amp = self.perturbation_amplitude
 
         def four(kind_y, my, kind_z, mz):
@@ -260,22 +262,51 @@ class PlaneCouetteMRIShearpyInsulatingHIP(PlaneCouetteMHDHIP):
         A = list(self._zero_A())
         mag = self.magnetic_amplitude
         if mag != 0.0 and self.magnetic_seed == "ax_yz":
-            A[0] = proj(FD, [(mag * wall / kz1, four("sin", 1, "sin", 1))])
+            A[0] = proj(self._a_projection(0), [(mag * wall / kz1, four("sin", 1, "sin", 1))])
         elif mag != 0.0:
-            # Ay = -(mag h / pi)(cos(pi x) + 1) on the TC basis (h = 1)
+            # zero-net-flux vertical field: Ay = -(mag / pi)(cos(pi x) + 1), zero at both walls
             prof = -(mag / np.pi) * (np.cos(np.pi * xn) + 1.0)
-            A[1] = proj(np.diag(1.0 / h) @ WQ, [(prof, four("one", 0, "one", 0))])
+            A[1] = proj(self._a_projection(1), [(prof, four("one", 0, "one", 0))])
         return self._new_state(flow, A)
 
     def diagnostics(self, state: MHDState | None = None) -> dict:
-        """Flow and magnetic diagnostics plus ``insulating_bc_residual``
-        (``examples/pcf_mhd_mri_shearpy_jax.py:802-836``)."""
+        """Flow and magnetic diagnostics plus the rotation numbers
+        (``examples/pcf_mhd_mri_shearpy_jax.py:94-97``)."""
         out = super().diagnostics(state)
-        m = np.zeros(4)
-        nat.check(self._lib.pcf_mhd_diagnostics(self._h, m.ctypes.data_as(C.POINTER(C.c_double))))
-        out["insulating_bc_residual"] = float(m[3])
         out["q_shear"] = self.q_shear
         out["kappa2"] = self.kappa2
+        return out
+
+
+class PlaneCouetteMRIShearpyInsulatingHIP(PlaneCouetteMRIShearpyHIP):
+    """Rotating-shear plane-Couette MHD with insulating (vacuum-matched) walls.
+
+    Mirrors ``PlaneCouetteMRIShearpyInsulatingJax``
+    (``examples/pcf_mhd_mri_shearpy_jax.py:516-836``): as
+    ``PlaneCouetteMRIShearpyHIP``, with A_x in TD and A_y, A_z in the orthogonal
+    Chebyshev space TC with per-mode vacuum-matching wall rows.
+    """
+
+    def _configure(self, cfg) -> None:
+        super()._configure(cfg)
+        cfg.walls = nat.WALLS_INSULATING
+
+    def _a_rows(self, comp: int) -> int:
+        return self.nd if comp == 0 else self.N[0]
+
+    def _a_projection(self, comp: int) -> np.ndarray:
+        """TD projection for A_x; TC coefficients (scalar products / h) for A_y, A_z."""
+        if comp == 0:
+            return self.matrix("FD")
+        h = np.full(self.N[0], np.pi / 2)
+        h[0] = np.pi
+        return np.diag(1.0 / h) @ self.matrix("WQ")
+
+    def diagnostics(self, state: MHDState | None = None) -> dict:
+        """Adds ``insulating_bc_residual``
+        (``examples/pcf_mhd_mri_shearpy_jax.py:802-836``)."""
+        out = super().diagnostics(state)
+        out["insulating_bc_residual"] = float(self._mhd_diag[3])
         return out
 
 

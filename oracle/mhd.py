@@ -278,3 +278,68 @@ class MHDOracle(PCFOracle):
 
 
 __all__ = ["MHDOracle", "MHDOracleState"]
+
+
+class RotatingShearMHDOracle(MHDOracle):
+    """``PlaneCouetteMRIShearpyJax`` (``examples/pcf_mhd_mri_shearpy_jax.py:30-514``)
+    with conducting walls, SBDF3/rotational: base flow U_b = -S x (``:86-92``),
+    Coriolis forcing (``:153-160``) and a uniform background field B0 added to B
+    (``:149-151``); A in TD^3 as in ``MHDOracle``."""
+
+    def __init__(self, N=(17, 16, 16), domain=((-1.0, 1.0), (0.0, 4.0 * math.pi), (0.0, 2.0 * math.pi)),
+                 Re: float = 400.0, Rm: float | None = None, omega: float = 1.0, shear_rate: float = 1.0,
+                 background_b=(0.0, 0.0, 0.025), dt: float = 0.01, padding_factor=(1.0, 1.5, 1.5),
+                 perturbation_amplitude: float = 0.05, magnetic_amplitude: float = 0.0) -> None:
+        self.omega = float(omega)
+        self.shear_rate = float(shear_rate)
+        super().__init__(N=N, domain=domain, Re=Re, Rm=Rm, U_wall=1.0, dt=dt, padding_factor=padding_factor,
+                         perturbation_amplitude=perturbation_amplitude, magnetic_amplitude=magnetic_amplitude,
+                         background_b=background_b)
+        # U_b = -S x, dU_b/dx = -S (shearpy :89-92); nu, eta keep U_wall = 1
+        self.dUb = -self.shear_rate
+
+    def flow_forcing(self, n, up):
+        """Coriolis forcing (shearpy :153-160)."""
+        return (n[0] - 2.0 * self.omega * up[1], n[1] + 2.0 * self.omega * up[0], n[2])
+
+    def shearpy_seed(self):
+        """Physical fields of ``initial_state`` (shearpy :99-147) with the default
+        channel-mode velocity seed and the ``ax_yz`` magnetic seed: the flow
+        state and the nodal A_x."""
+        from .kmm import OracleState
+
+        x, y, z = self.mesh()
+        shape = (self.N0, self.N[1], self.N[2])
+        wall = 1.0 - x**2
+        amp = self.perturbation_amplitude
+        ky, kz = 2.0 * np.pi / self.L1, 2.0 * np.pi / self.L2
+        u0 = np.zeros(shape)
+        u1 = np.zeros(shape)
+        u2 = np.zeros(shape)
+        if amp != 0.0:
+            for hm in (1, 2, 3):
+                u0 = u0 + amp * wall * np.cos(hm * kz * z)
+                u1 = u1 + amp * wall * np.sin(hm * kz * z)
+            u0 = u0 + 0.1 * amp * wall * np.sin(ky * y) * np.cos(kz * z)
+            u1 = u1 + 0.1 * amp * wall * np.cos(ky * y) * np.sin(kz * z)
+            u2 = u2 + 0.1 * amp * wall * np.sin(2.0 * ky * y) * np.cos(2.0 * kz * z)
+        flow = self.state_from_physical(tuple(np.broadcast_to(v, shape).astype(complex) for v in (u0, u1, u2)))
+        ax = np.zeros(shape)
+        if self.magnetic_amplitude != 0.0:
+            ax = np.broadcast_to(self.magnetic_amplitude * wall * (1.0 / kz) * np.sin(ky * y) * np.sin(kz * z), shape)
+        return OracleState(u=flow.u, g=flow.g), ax
+
+    def diagnostics(self, s) -> dict:
+        """Adds Emag_total, the energy of the total field b + B0 (shearpy :431-513)."""
+        out = super().diagnostics(s)
+        bp = self.backward_B(self.B_from_A(s.A), padded=False)
+        out["Emag_total"] = float(sum(np.real(self.integrate(np.conj(b + b0) * (b + b0)))
+                                      for b, b0 in zip(bp, self.background_b)))
+        return out
+
+    def initial_state(self):
+        """``initial_state`` (shearpy :99-147), A projected onto TD^3."""
+        flow, ax = self.shearpy_seed()
+        z = self.zero_A()
+        A = (self.mask(self.tp_forward(ax.astype(complex), self.SD, self.MD1)), z[1], z[2])
+        return self.reset_mhd_history(MHDOracleState(flow=flow, A=A))

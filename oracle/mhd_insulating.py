@@ -35,7 +35,7 @@ import math
 import numpy as np
 
 from .kmm import cheb_norm_squared, fourier_forward
-from .mhd import MHDOracle
+from .mhd import MHDOracleState, RotatingShearMHDOracle
 
 
 def _cheb_D(n: int) -> np.ndarray:
@@ -47,20 +47,18 @@ def _cheb_D(n: int) -> np.ndarray:
     return D
 
 
-class InsulatingMHDOracle(MHDOracle):
-    """``PlaneCouetteMRIShearpyInsulatingJax`` for SBDF3 / rotational / optimized."""
+class InsulatingMHDOracle(RotatingShearMHDOracle):
+    """``PlaneCouetteMRIShearpyInsulatingJax`` for SBDF3 / rotational / optimized:
+    the rotating-shear oracle (base flow, Coriolis, B0) with the insulating A
+    spaces and wall rows."""
 
     def __init__(self, N=(17, 16, 16), domain=((-1.0, 1.0), (0.0, 4.0 * math.pi), (0.0, 2.0 * math.pi)),
                  Re: float = 400.0, Rm: float | None = None, omega: float = 1.0, shear_rate: float = 1.0,
                  background_b=(0.0, 0.0, 0.025), dt: float = 0.01, padding_factor=(1.0, 1.5, 1.5),
                  perturbation_amplitude: float = 0.05, magnetic_amplitude: float = 0.0) -> None:
-        self.omega = float(omega)
-        self.shear_rate = float(shear_rate)
-        super().__init__(N=N, domain=domain, Re=Re, Rm=Rm, U_wall=1.0, dt=dt, padding_factor=padding_factor,
-                         perturbation_amplitude=perturbation_amplitude, magnetic_amplitude=magnetic_amplitude,
-                         background_b=background_b)
-        # U_b = -S x, dU_b/dx = -S (shearpy :89-92); nu, eta keep U_wall = 1
-        self.dUb = -self.shear_rate
+        super().__init__(N=N, domain=domain, Re=Re, Rm=Rm, omega=omega, shear_rate=shear_rate,
+                         background_b=background_b, dt=dt, padding_factor=padding_factor,
+                         perturbation_amplitude=perturbation_amplitude, magnetic_amplitude=magnetic_amplitude)
         n = self.N0
         D = _cheb_D(n)
         self.CD = D
@@ -81,31 +79,10 @@ class InsulatingMHDOracle(MHDOracle):
     def initial_state(self):
         """``initial_state`` (shearpy :99-147), the default channel-mode seed with
         the ``ax_yz`` magnetic seed; A projected onto (TD, TC, TC)."""
-        from .kmm import OracleState
-        from .mhd import MHDOracleState
-
-        x, y, z = self.mesh()
-        shape = (self.N0, self.N[1], self.N[2])
-        wall = 1.0 - x**2
-        amp = self.perturbation_amplitude
-        ky, kz = 2.0 * np.pi / self.L1, 2.0 * np.pi / self.L2
-        u0 = np.zeros(shape)
-        u1 = np.zeros(shape)
-        u2 = np.zeros(shape)
-        if amp != 0.0:
-            for hm in (1, 2, 3):
-                u0 = u0 + amp * wall * np.cos(hm * kz * z)
-                u1 = u1 + amp * wall * np.sin(hm * kz * z)
-            u0 = u0 + 0.1 * amp * wall * np.sin(ky * y) * np.cos(kz * z)
-            u1 = u1 + 0.1 * amp * wall * np.cos(ky * y) * np.sin(kz * z)
-            u2 = u2 + 0.1 * amp * wall * np.sin(2.0 * ky * y) * np.cos(2.0 * kz * z)
-        flow = self.state_from_physical(tuple(np.broadcast_to(v, shape).astype(complex) for v in (u0, u1, u2)))
-        ax = np.zeros(shape)
-        if self.magnetic_amplitude != 0.0:
-            ax = np.broadcast_to(self.magnetic_amplitude * wall * (1.0 / kz) * np.sin(ky * y) * np.sin(kz * z), shape)
-        A = (self.mask(self.tp_forward(ax.astype(complex), self.SD, self.MD1)),
-             np.zeros((self.N0, self.N[1], self.N[2]), complex), np.zeros((self.N0, self.N[1], self.N[2]), complex))
-        return self.reset_mhd_history(MHDOracleState(flow=OracleState(u=flow.u, g=flow.g), A=A))
+        flow, ax = self.shearpy_seed()
+        z = self.zero_A()
+        A = (self.mask(self.tp_forward(ax.astype(complex), self.SD, self.MD1)), z[1], z[2])
+        return self.reset_mhd_history(MHDOracleState(flow=flow, A=A))
 
     # ------------------------------------------------------------ spaces
     def zero_A(self):
@@ -143,10 +120,6 @@ class InsulatingMHDOracle(MHDOracle):
         return fourier_forward(z, 1, self.N[1])
 
     # ------------------------------------------------------------ physics hooks
-    def flow_forcing(self, n, up):
-        """Coriolis forcing (shearpy :153-160)."""
-        return (n[0] - 2.0 * self.omega * up[1], n[1] + 2.0 * self.omega * up[0], n[2])
-
     def forward_emf(self, emf):
         """``_A_forward_emf`` (shearpy :654-659): TD for x, TC for y and z."""
         return (self.mask(self.tp_forward_D(emf[0])), self.mask(self.tp_forward_C(emf[1])),
@@ -218,10 +191,6 @@ class InsulatingMHDOracle(MHDOracle):
 
     def diagnostics(self, s) -> dict:
         out = super().diagnostics(s)
-        B = self.B_from_A(s.A)
-        bp = self.backward_B(B, padded=False)
-        out["Emag_total"] = float(sum(np.real(self.integrate(np.conj(b + b0) * (b + b0)))
-                                      for b, b0 in zip(bp, self.background_b)))
         out["insulating_bc_residual"] = self.insulating_bc_residual(s)
         return out
 

@@ -102,6 +102,16 @@ def mhd_oracle_for(solver):
                      magnetic_amplitude=solver.magnetic_amplitude, background_b=solver.background_b)
 
 
+def shear_oracle_for(solver):
+    from oracle.mhd import RotatingShearMHDOracle
+
+    return RotatingShearMHDOracle(N=solver.N, domain=solver.domain, Re=solver.Re, Rm=solver.Rm, omega=solver.omega,
+                                  shear_rate=solver.shear_rate, background_b=solver.background_b, dt=solver.dt,
+                                  padding_factor=solver.padding_factor,
+                                  perturbation_amplitude=solver.perturbation_amplitude,
+                                  magnetic_amplitude=solver.magnetic_amplitude)
+
+
 def insulating_oracle_for(solver):
     from oracle.mhd_insulating import InsulatingMHDOracle
 

@@ -1,5 +1,6 @@
-"""GPU parity of the conducting-wall MHD step: HIP against the MHD oracle and
-the reference's published MHD goldens.
+"""GPU parity of the MHD steps (conducting walls, rotating shear with conducting
+and insulating walls): HIP against the MHD oracles and the reference's
+published MHD goldens.
 
 Bars as for the hydrodynamic step: state leaves (flow and vector potential)
 within 1e-12 relative per leaf group of the FP64 oracle, nonlinear history rows
@@ -179,3 +180,71 @@ def test_insulating_graph_replay_and_mixed_precision():
         print("insulating mixed vs fp64", rel, flush=True)
         assert rel < 2e-6
     assert b.diagnostics(sb)["insulating_bc_residual"] < 1e-10
+
+
+# ---------------------------------------------------------------- rotating shear, conducting walls
+def make_shear(N, precision="fp64", **kw):
+    from jaxfun_amd import PlaneCouetteMRIShearpyHIP
+
+    args = dict(Re=400.0, Rm=300.0, omega=0.4, shear_rate=1.5, background_b=(0.0, 0.02, 0.1), dt=2e-3,
+                perturbation_amplitude=0.05, magnetic_amplitude=0.3)
+    args.update(kw)
+    return PlaneCouetteMRIShearpyHIP(N=N, precision=precision, **args)
+
+
+def test_shear_initial_states_match_oracle():
+    from tests.helpers import shear_oracle_for
+
+    solver = make_shear((17, 16, 16))
+    o = shear_oracle_for(solver)
+    err = mhd_max_rel_errors(solver.initial_state(), o.initial_state())
+    assert max(err.values()) < 1e-13, err
+    # sinusoidal_bz_x: Ay = -(a / pi)(cos(pi x) + 1), uniform in y and z, projected onto TD
+    s = make_shear((17, 16, 16), magnetic_seed="sinusoidal_bz_x").initial_state()
+    x = o.mesh()[0]
+    ay = np.broadcast_to(-(0.3 / np.pi) * (np.cos(np.pi * x) + 1.0), (o.N0, o.N[1], o.N[2])).astype(complex)
+    want = o.mask(o.tp_forward(ay, o.SD, o.MD1))
+    assert np.max(np.abs(s.A[1] - want)) <= 1e-13 * np.max(np.abs(want))
+    assert np.max(np.abs(s.A[0])) == 0.0 and np.max(np.abs(s.A[2])) == 0.0
+
+
+@pytest.mark.parametrize("N,seeded", [((17, 16, 16), False), ((24, 16, 32), True)], ids=["17-initial", "24x16x32-seeded"])
+def test_shear_steps_match_oracle(N, seeded):
+    from jaxfun_amd.ic import seeded_mhd_state
+    from tests.helpers import shear_oracle_for
+
+    solver = make_shear(N)
+    oracle = shear_oracle_for(solver)
+    s = seeded_mhd_state(solver, seed=3, energy_u=1e-3, energy_b=1e-3, K=4) if seeded else solver.initial_state()
+    so = to_mhd_oracle(s)
+    done = 0
+    for k in (1, 2, 3, 6):
+        s = solver.solve(s, k - done)
+        so = oracle.solve(so, k - done)
+        done = k
+        err = mhd_max_rel_errors(s, so)
+        print(N, seeded, k, max(err.values()), flush=True)
+        assert_parity(err, (N, k))
+    d, do = solver.diagnostics(s), oracle.diagnostics(so)
+    for key in ("Epert", "Etot", "Emag", "Emag_total"):
+        assert math.isclose(d[key], do[key], rel_tol=1e-12), (key, d[key], do[key])
+    assert d["divB_L2"] < 1e-12 and d["q_shear"] == 1.5 / 0.4
+
+
+def test_shear_graph_replay_and_mixed_precision():
+    N = (33, 32, 32)
+    a = make_shear(N)
+    s0 = a.initial_state()
+    s = a.solve(s0, 3)
+    a.load(s)
+    a.advance_steady(24)
+    eager = a.fetch()
+    a.load(s)
+    a.advance_steady(24, graph_block=12)
+    assert max(mhd_max_rel_errors(a.fetch(), eager).values()) == 0.0
+    b = make_shear(N, precision="mixed")
+    sa, sb = a.solve(s0, 30), b.solve(s0, 30)
+    for x, y in zip(list(sb.flow.u) + list(sb.A), list(sa.flow.u) + list(sa.A)):
+        rel = float(np.max(np.abs(x - y)) / np.max(np.abs(y)))
+        print("shear mixed vs fp64", rel, flush=True)
+        assert rel < 2e-6

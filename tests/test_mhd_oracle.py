@@ -27,7 +27,7 @@ import pytest
 import scipy.linalg as sla
 
 from oracle.kmm import IMEXRK3_A, IMEXRK3_B, PCFOracle
-from oracle.mhd import MHDOracle, MHDOracleState
+from oracle.mhd import MHDOracle, MHDOracleState, RotatingShearMHDOracle
 
 GOLDEN = json.loads((Path(__file__).parent / "golden" / "reference_mhd_published.json").read_text())
 
@@ -93,6 +93,38 @@ def test_zero_field_reproduces_hydro_oracle():
     for a, b in zip(sm.flow.leaves(), sh.leaves()):
         assert np.array_equal(np.asarray(a), np.asarray(b))
     assert all(np.max(np.abs(a)) == 0 for a in sm.A)
+
+
+def test_rotating_shear_reduces_to_conducting_mhd():
+    """With Omega = 0, S = -1 (U_b = x) and the same B0 the rotating-shear
+    oracle is the conducting-wall oracle bit for bit: the base-flow slope and
+    the forcing hook are the only differences."""
+    kw = dict(N=(17, 16, 16), Re=400.0, Rm=300.0, dt=2e-3, perturbation_amplitude=0.05, magnetic_amplitude=0.2,
+              background_b=(0.0, 0.02, 0.1))
+    r = RotatingShearMHDOracle(omega=0.0, shear_rate=-1.0, **kw)
+    m = MHDOracle(U_wall=1.0, **kw)
+    s0 = m.initial_state()
+    sr, sm = r.solve(s0, 5), m.solve(s0, 5)
+    for a, b in zip(sr.flow.leaves() + list(sr.A), sm.flow.leaves() + list(sm.A)):
+        assert np.array_equal(np.asarray(a), np.asarray(b))
+    # the two classes' ax_yz seeds are the same field (written independently)
+    for a, b in zip(r.initial_state().A, m.initial_state().A):
+        assert np.max(np.abs(a - b)) <= 1e-15 * max(1.0, np.max(np.abs(b)))
+
+
+def test_coriolis_does_no_work():
+    """u . F_Coriolis = 0 pointwise for any velocity, and the forcing is linear
+    in Omega with the sign pattern (-u1, +u0, 0)."""
+    rng = np.random.default_rng(4)
+    o = RotatingShearMHDOracle(N=(9, 8, 8), omega=0.7, shear_rate=1.5)
+    up = tuple(rng.standard_normal((9, 12, 12)) for _ in range(3))
+    zero = tuple(np.zeros_like(u) for u in up)
+    f = o.flow_forcing(zero, up)
+    assert np.max(np.abs(sum(a * b for a, b in zip(f, up)))) < 1e-14
+    assert np.max(np.abs(f[2])) == 0.0
+    o2 = RotatingShearMHDOracle(N=(9, 8, 8), omega=1.4, shear_rate=1.5)
+    f2 = o2.flow_forcing(zero, up)
+    assert all(np.allclose(b, 2 * a, rtol=0, atol=1e-14) for a, b in zip(f, f2))
 
 
 def test_A_decay_follows_scalar_recurrence():
