@@ -206,8 +206,12 @@ int32_t build_nodal(pcf_handle *h, DevMats<T> &dm, int xc, int ic) {
   const int n0 = o.n0, nb = o.nb, nd = o.nd;
   const int ldx = (n0 + xc - 1) / xc * xc;
   const int ldi = (nd + ic - 1) / ic * ic;
-  auto transpose_pad = [&](const pcf::Mat &M, int ld) {  // M (rows x cols) -> [cols][ld]
-    std::vector<T> t((size_t)M.cols * ld, (T)0);
+  // M (rows x cols) -> [cols][ld], zero rows appended up to a multiple of 4 >= n0:
+  // every matrix-core K loop (4 K rows per step, K <= n0) stays inside the
+  // allocation without bounds checks
+  const int kr = (n0 + 3) / 4 * 4;
+  auto transpose_pad = [&](const pcf::Mat &M, int ld) {
+    std::vector<T> t((size_t)std::max(M.cols, kr) * ld, (T)0);
     for (int i = 0; i < M.rows; ++i)
       for (int j = 0; j < M.cols; ++j) t[(size_t)j * ld + i] = (T)M(i, j);
     return t;
@@ -780,8 +784,9 @@ template <typename T>
 int32_t build_adj_mats(pcf_handle *h, pcf::AdjMats<T> &am) {
   const pcf::Operators &o = h->ops;
   const int ldx = (o.n0 + 15) / 16 * 16, ldi = (o.nd + 15) / 16 * 16;  // whole 16-row MFMA tiles
-  auto pad = [](const pcf::Mat &Mx, int ld) {  // row-major [rows][ld]
-    std::vector<T> t((size_t)Mx.rows * ld, (T)0);
+  const int kr = (o.n0 + 3) / 4 * 4;  // zero rows up to a multiple of 4 >= n0 (K loops)
+  auto pad = [&](const pcf::Mat &Mx, int ld) {  // row-major [rows][ld]
+    std::vector<T> t((size_t)std::max(Mx.rows, kr) * ld, (T)0);
     for (int i = 0; i < Mx.rows; ++i)
       for (int j = 0; j < Mx.cols; ++j) t[(size_t)i * ld + j] = (T)Mx(i, j);
     return t;

@@ -78,6 +78,39 @@ __device__ __forceinline__ bool nyq_mode(const Geo &g, int m) { return m / g.kzs
 // blockDim * STAGE covers the kernel's panel.
 constexpr int STAGE_NODAL = 6, STAGE_RHS = 3, STAGE_NODAL_ADJ = 3, STAGE_RHS_ADJ = 4;
 
+// K loop of a wave's tile: nit steps of 4 K rows.  The A operands (NA values
+// per step, from small L2-resident matrices) are read D steps ahead into a
+// register ring, so a step waits on loads issued D steps earlier instead of
+// one L2 round trip per step.  load(it, a) must be branch-free (clamped
+// address, then select), which lets the ring unroll.
+template <typename T, int NA, int D, class Load, class Step>
+__device__ __forceinline__ void k_loop(int nit, Load load, Step step) {
+  T ring[D][NA];
+  // every load is issued unconditionally (step index clamped to nit - 1), so
+  // the ring carries no branches and the waits stay partial
+#pragma unroll
+  for (int c = 0; c < D; ++c) load(c < nit ? c : nit - 1, ring[c]);
+  int it0 = 0;
+  for (; it0 + D <= nit; it0 += D) {
+#pragma unroll
+    for (int c = 0; c < D; ++c) {
+      step(it0 + c, ring[c]);
+      const int nx = it0 + c + D;
+      load(nx < nit ? nx : nit - 1, ring[c]);
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < D; ++c)
+    if (it0 + c < nit) step(it0 + c, ring[c]);
+}
+
+// Element k of column xa of a [K][ld] matrix.  The host pads every matrix with
+// zero rows up to a multiple of 4 that covers n0, so k < kp needs no check.
+template <typename T>
+__device__ __forceinline__ T mat_at(const T *m, int ld, int k, int xa) {
+  return m[(size_t)k * ld + xa];
+}
+
 template <typename T>
 struct Panel {
   T *b;
@@ -134,22 +167,24 @@ __global__ void __launch_bounds__(512) k_nodal_mfma(Geo g, NodalMats<T> nm, cons
   for (int x0 = (threadIdx.x >> 6) * 16; x0 < g.n0; x0 += (blockDim.x >> 6) * 16) {
   const int xa = x0 + (L.l & 15);
   typename MM::acc U0{}, U1{}, U2{}, D1{}, D2{};
-#pragma unroll 4
-  for (int kk = 0; kk < kp; kk += 4) {
-    const int k = kk + (L.l >> 4);
-    const T b0 = B.at(0, k, L.c), b1 = B.at(1, k, L.c), b2 = B.at(2, k, L.c);
-    T aB = 0, aD = 0, aD1 = 0;
-    if (k < g.nb) aB = nm.VBt[(size_t)k * nm.ldx + xa];
-    if (k < g.nd) {
-      aD = nm.VDt[(size_t)k * nm.ldx + xa];
-      aD1 = nm.VD1t[(size_t)k * nm.ldx + xa];
-    }
-    U0 = MM::mma(aB, b0, U0);
-    U1 = MM::mma(aD, b1, U1);
-    U2 = MM::mma(aD, b2, U2);
-    D1 = MM::mma(aD1, b1, D1);
-    D2 = MM::mma(aD1, b2, D2);
-  }
+  const int kq = L.l >> 4;
+  k_loop<T, 3, 8>(
+      kp / 4,
+      [&](int it, T(&a)[3]) {
+        const int k = 4 * it + kq;
+        a[0] = mat_at(nm.VBt, nm.ldx, k, xa);
+        a[1] = mat_at(nm.VDt, nm.ldx, k, xa);
+        a[2] = mat_at(nm.VD1t, nm.ldx, k, xa);
+      },
+      [&](int it, const T(&a)[3]) {
+        const int k = 4 * it + kq;
+        const T b0 = B.at(0, k, L.c), b1 = B.at(1, k, L.c), b2 = B.at(2, k, L.c);
+        U0 = MM::mma(a[0], b0, U0);
+        U1 = MM::mma(a[1], b1, U1);
+        U2 = MM::mma(a[1], b2, U2);
+        D1 = MM::mma(a[2], b1, D1);
+        D2 = MM::mma(a[2], b2, D2);
+      });
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
     const T u0v = U0[r], u1v = U1[r], u2v = U2[r], d1 = D1[r], d2 = D2[r];
@@ -224,30 +259,30 @@ __global__ void __launch_bounds__(512) k_nodal_mhd_mfma(Geo g, MhdMats<T> mm, co
   for (int x0 = (threadIdx.x >> 6) * 16; x0 < g.n0; x0 += (blockDim.x >> 6) * 16) {
     const int xa = x0 + (L.l & 15);
     typename MM::acc DX{}, DY{}, DZ{}, EY{}, EZ{}, QX{}, RY{}, RZ{};
-#pragma unroll 2
-    for (int kk = 0; kk < kp; kk += 4) {
-      const int k = kk + (L.l >> 4);
-      const T bx = B.at(0, k, L.c), by = B.at(1, k, L.c), bz = B.at(2, k, L.c);
-      T aD = 0, aQ = 0, aY = 0, aE = 0, aR = 0;
-      const size_t o = (size_t)k * mm.ldx + xa;
-      if (k < mm.kx) {
-        aD = mm.mDX[o];
-        aQ = mm.mQX[o];
-      }
-      if (k < mm.kyz) {
-        aY = mm.mDY[o];
-        aE = mm.mEY[o];
-        aR = mm.mRY[o];
-      }
-      DX = MM::mma(aD, bx, DX);
-      QX = MM::mma(aQ, bx, QX);
-      DY = MM::mma(aY, by, DY);
-      DZ = MM::mma(aY, bz, DZ);
-      EY = MM::mma(aE, by, EY);
-      EZ = MM::mma(aE, bz, EZ);
-      RY = MM::mma(aR, by, RY);
-      RZ = MM::mma(aR, bz, RZ);
-    }
+    const int kq = L.l >> 4;
+    // rows k >= kx of mDX, mQX and k >= kyz of the others are zero padding
+    k_loop<T, 5, 6>(
+        kp / 4,
+        [&](int it, T(&a)[5]) {
+          const size_t o = (size_t)(4 * it + kq) * mm.ldx + xa;
+          a[0] = mm.mDX[o];
+          a[1] = mm.mQX[o];
+          a[2] = mm.mDY[o];
+          a[3] = mm.mEY[o];
+          a[4] = mm.mRY[o];
+        },
+        [&](int it, const T(&a)[5]) {
+          const int k = 4 * it + kq;
+          const T bx = B.at(0, k, L.c), by = B.at(1, k, L.c), bz = B.at(2, k, L.c);
+          DX = MM::mma(a[0], bx, DX);
+          QX = MM::mma(a[1], bx, QX);
+          DY = MM::mma(a[2], by, DY);
+          DZ = MM::mma(a[2], bz, DZ);
+          EY = MM::mma(a[3], by, EY);
+          EZ = MM::mma(a[3], bz, EZ);
+          RY = MM::mma(a[4], by, RY);
+          RZ = MM::mma(a[4], bz, RZ);
+        });
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const T dx = DX[r], dy = DY[r], dz = DZ[r], ey = EY[r], ez = EZ[r], ry = RY[r], rz = RZ[r];
@@ -322,18 +357,22 @@ __global__ void __launch_bounds__(512) k_rhs_a_mfma(Geo g, MhdMats<T> mm,
   for (int i0 = (threadIdx.x >> 6) * 16; i0 < rows; i0 += (blockDim.x >> 6) * 16) {
     const int ia = i0 + (L.l & 15);
     typename MM::acc C0{}, C1{}, C2{};
-#pragma unroll 4
-    for (int kk = 0; kk < kp; kk += 4) {
-      const int x = kk + (L.l >> 4);
-      T a0 = 0, a12 = 0;
-      if (x < g.n0) {
-        if (ia < mm.ld0) a0 = mm.mA0[(size_t)x * mm.ld0 + ia];
-        if (ia < mm.ld12) a12 = mm.mA12[(size_t)x * mm.ld12 + ia];
-      }
-      C0 = MM::mma(a0, B.at(0, x, L.c), C0);
-      C1 = MM::mma(a12, B.at(1, x, L.c), C1);
-      C2 = MM::mma(a12, B.at(2, x, L.c), C2);
-    }
+    const int kq = L.l >> 4;
+    // rows past a matrix's width are never stored: clamp their column
+    const int ia0 = ia < mm.ld0 ? ia : mm.ld0 - 1, ia12 = ia < mm.ld12 ? ia : mm.ld12 - 1;
+    k_loop<T, 2, 8>(
+        kp / 4,
+        [&](int it, T(&a)[2]) {
+          const int x = 4 * it + kq;
+          a[0] = mat_at(mm.mA0, mm.ld0, x, ia0);
+          a[1] = mat_at(mm.mA12, mm.ld12, x, ia12);
+        },
+        [&](int it, const T(&a)[2]) {
+          const int x = 4 * it + kq;
+          C0 = MM::mma(a[0], B.at(0, x, L.c), C0);
+          C1 = MM::mma(a[1], B.at(1, x, L.c), C1);
+          C2 = MM::mma(a[1], B.at(2, x, L.c), C2);
+        });
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const int i = i0 + MM::row(L.l, r);
@@ -396,6 +435,10 @@ __global__ void __launch_bounds__(512) k_rhs_mfma(Geo g, NodalMats<T> nm,
       cgi = kz * h1.x - ky * h2.x;
       c0r = k2 * h0.x;
       c0i = k2 * h0.y;
+      if (m == 0) {  // mode 0 has cg = 0: its columns carry (Re h1, Re h2) for the mean rows
+        cgr = h1.x;
+        cgi = h2.x;
+      }
     }
     B.at(0, x, 2 * j) = c1r;
     B.at(0, x, 2 * j + 1) = c1i;
@@ -410,19 +453,21 @@ __global__ void __launch_bounds__(512) k_rhs_mfma(Geo g, NodalMats<T> nm,
   for (int i0 = (threadIdx.x >> 6) * 16; i0 < g.nd; i0 += (blockDim.x >> 6) * 16) {
     const int ia = i0 + (L.l & 15);
     typename MM::acc AU{}, AG{};
-#pragma unroll 4
-    for (int kk = 0; kk < kp; kk += 4) {
-      const int x = kk + (L.l >> 4);
-      T a1 = 0, a0 = 0, ag = 0;
-      if (x < g.n0) {
-        a1 = nm.A1t[(size_t)x * nm.ldi + ia];
-        a0 = nm.A0t[(size_t)x * nm.ldi + ia];
-        ag = nm.AGt[(size_t)x * nm.ldi + ia];
-      }
-      AU = MM::mma(a1, B.at(0, x, L.c), AU);
-      AU = MM::mma(a0, B.at(1, x, L.c), AU);
-      AG = MM::mma(ag, B.at(2, x, L.c), AG);
-    }
+    const int kq = L.l >> 4;
+    k_loop<T, 3, 8>(
+        kp / 4,
+        [&](int it, T(&a)[3]) {
+          const int x = 4 * it + kq;
+          a[0] = mat_at(nm.A1t, nm.ldi, x, ia);
+          a[1] = mat_at(nm.A0t, nm.ldi, x, ia);
+          a[2] = mat_at(nm.AGt, nm.ldi, x, ia);
+        },
+        [&](int it, const T(&a)[3]) {
+          const int x = 4 * it + kq;
+          AU = MM::mma(a[0], B.at(0, x, L.c), AU);
+          AU = MM::mma(a[1], B.at(1, x, L.c), AU);
+          AG = MM::mma(a[2], B.at(2, x, L.c), AG);
+        });
     const double W = g.W;
     double *Nur = reinterpret_cast<double *>(Nu);
     double *Ngr = reinterpret_cast<double *>(Ng);
@@ -432,19 +477,14 @@ __global__ void __launch_bounds__(512) k_rhs_mfma(Geo g, NodalMats<T> nm,
       if (!L.valid) continue;
       const size_t o = (size_t)i * 2 * M + 2 * L.mcol + L.comp;
       if (i < g.nb) Nur[o] = W * (double)AU[r];
-      if (i < g.nd) Ngr[o] = W * (double)AG[r];
-    }
-  }
-  if (ct == 0) {
-    for (int i = threadIdx.x; i < g.nd; i += blockDim.x) {
-      T sv = 0, sw = 0;
-      for (int x = 0; x < g.n0; ++x) {
-        const T a = nm.AGt[(size_t)x * nm.ldi + i];
-        sv += a * Hx[plane + (size_t)x * M].x;
-        sw += a * Hx[2 * plane + (size_t)x * M].x;
+      if (i < g.nd) {
+        if (L.mcol == 0) {
+          Ngr[o] = 0.0;
+          (L.comp ? Nw : Nv)[i] = -(double)AG[r];
+        } else {
+          Ngr[o] = W * (double)AG[r];
+        }
       }
-      Nv[i] = -(double)sv;
-      Nw[i] = -(double)sw;
     }
   }
 }
@@ -517,21 +557,23 @@ __global__ void __launch_bounds__(512) k_nodal_adj_mfma(Geo g, AdjMats<T> am,
   for (int i0 = (threadIdx.x >> 6) * 16; i0 < g.nd; i0 += (blockDim.x >> 6) * 16) {
   const int ia = i0 + (L.l & 15);
   typename MM::acc A0{}, A1{}, A2{};
-#pragma unroll 4
-  for (int kk = 0; kk < kp; kk += 4) {
-    const int x = kk + (L.l >> 4);
-    T ab = 0, ad = 0, ad1 = 0;
-    if (x < g.n0) {
-      ab = am.VB[(size_t)x * am.ldi + ia];
-      ad = am.VD[(size_t)x * am.ldi + ia];
-      ad1 = am.VD1[(size_t)x * am.ldi + ia];
-    }
-    A0 = MM::mma(ab, B.at(0, x, L.c), A0);
-    A1 = MM::mma(ad, B.at(1, x, L.c), A1);
-    A1 = MM::mma(ad1, B.at(3, x, L.c), A1);
-    A2 = MM::mma(ad, B.at(2, x, L.c), A2);
-    A2 = MM::mma(ad1, B.at(4, x, L.c), A2);
-  }
+  const int kq = L.l >> 4;
+  k_loop<T, 3, 8>(
+      kp / 4,
+      [&](int it, T(&a)[3]) {
+        const int x = 4 * it + kq;
+        a[0] = mat_at(am.VB, am.ldi, x, ia);
+        a[1] = mat_at(am.VD, am.ldi, x, ia);
+        a[2] = mat_at(am.VD1, am.ldi, x, ia);
+      },
+      [&](int it, const T(&a)[3]) {
+        const int x = 4 * it + kq;
+        A0 = MM::mma(a[0], B.at(0, x, L.c), A0);
+        A1 = MM::mma(a[1], B.at(1, x, L.c), A1);
+        A1 = MM::mma(a[2], B.at(3, x, L.c), A1);
+        A2 = MM::mma(a[1], B.at(2, x, L.c), A2);
+        A2 = MM::mma(a[2], B.at(4, x, L.c), A2);
+      });
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
     const int i = i0 + MM::row(L.l, r);
@@ -661,24 +703,25 @@ __global__ void __launch_bounds__(512) k_nodal_adj_x_mfma(Geo g, AdjMats<T> am, 
   for (int i0 = (threadIdx.x >> 6) * 16; i0 < nd; i0 += (blockDim.x >> 6) * 16) {
     const int ia = i0 + (L.l & 15);
     typename MM::acc AU{}, AG{};
-#pragma unroll 4
-    for (int kk = 0; kk < kp; kk += 4) {
-      const int x = kk + (L.l >> 4);
-      T ab = 0, ar = 0, ar1 = 0, ad = 0, ad1 = 0;
-      if (x < g.n0) {
-        const size_t o = (size_t)x * am.ldi + ia;
-        ab = am.VB[o];
-        ar = am.VDR[o];
-        ar1 = am.VD1R[o];
-        ad = am.VD[o];
-        ad1 = am.VD1[o];
-      }
-      AU = MM::mma(ab, B.at(0, x, L.c), AU);
-      AU = MM::mma(ar, B.at(1, x, L.c), AU);
-      AU = MM::mma(ar1, B.at(2, x, L.c), AU);
-      AG = MM::mma(ad, B.at(3, x, L.c), AG);
-      AG = MM::mma(ad1, B.at(4, x, L.c), AG);
-    }
+    const int kq = L.l >> 4;
+    k_loop<T, 5, 6>(
+        kp / 4,
+        [&](int it, T(&a)[5]) {
+          const size_t o = (size_t)(4 * it + kq) * am.ldi + ia;
+          a[0] = am.VB[o];
+          a[1] = am.VDR[o];
+          a[2] = am.VD1R[o];
+          a[3] = am.VD[o];
+          a[4] = am.VD1[o];
+        },
+        [&](int it, const T(&a)[5]) {
+          const int x = 4 * it + kq;
+          AU = MM::mma(a[0], B.at(0, x, L.c), AU);
+          AU = MM::mma(a[1], B.at(1, x, L.c), AU);
+          AU = MM::mma(a[2], B.at(2, x, L.c), AU);
+          AG = MM::mma(a[3], B.at(3, x, L.c), AG);
+          AG = MM::mma(a[4], B.at(4, x, L.c), AG);
+        });
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const int i = i0 + MM::row(L.l, r);
@@ -738,7 +781,9 @@ __global__ void __launch_bounds__(512) k_rhs_adj_mfma(Geo g, AdjMats<T> am, Prim
       z[t] = double2{0, 0};
       if (idx < total && m < M && !nyq_mode(g, m)) {
         if (f == 0 && i < g.nb) z[t] = nu.u0[(size_t)i * M + m];
-        if (f == 1 && i < g.nd) z[t] = nu.g[(size_t)i * M + m];
+        // mode 0 contributes nothing through its wavenumbers: its g columns
+        // carry the mean cotangents (nu_v, nu_w) through the AG product
+        if (f == 1 && i < g.nd) z[t] = (m == 0) ? double2{nu.v[i], nu.w[i]} : nu.g[(size_t)i * M + m];
       }
     }
 #pragma unroll
@@ -750,20 +795,6 @@ __global__ void __launch_bounds__(512) k_rhs_adj_mfma(Geo g, AdjMats<T> am, Prim
       B.at(f, i, 2 * j + 1) = (T)z[t].y;
     }
   }
-  // mean mode (block 0): AG^T nu_v and AG^T nu_w for every node, one node per thread
-  T *meanv = reinterpret_cast<T *>(smem) + (size_t)2 * kp * 16;
-  if (ct == 0) {
-    for (int x = threadIdx.x; x < g.n0; x += blockDim.x) {
-      T sv = 0, sw = 0;
-      for (int i = 0; i < g.nd; ++i) {
-        const T a = am.AG[(size_t)i * am.ldx + x];
-        sv += a * (T)nu.v[i];
-        sw += a * (T)nu.w[i];
-      }
-      meanv[2 * x] = sv;
-      meanv[2 * x + 1] = sw;
-    }
-  }
   __syncthreads();
   const Lane L = lane_of(g, ct);
   double kyd, kzd, kyo;
@@ -773,24 +804,26 @@ __global__ void __launch_bounds__(512) k_rhs_adj_mfma(Geo g, AdjMats<T> am, Prim
   const T s = L.comp ? (T)1 : (T)-1;  // -i z: re lane +z.im, im lane -z.re
   const size_t plane = (size_t)g.n0 * 2 * M;
   T *Hr = reinterpret_cast<T *>(Hx);
-  const bool mean_lane = L.valid && L.mcol == 0 && L.comp == 0;
+  const bool mean_col = L.valid && L.mcol == 0;
   for (int x0 = (threadIdx.x >> 6) * 16; x0 < g.n0; x0 += (blockDim.x >> 6) * 16) {
   const int xa = x0 + (L.l & 15);
   typename MM::acc P0{}, P1{}, PG{};
-#pragma unroll 4
-  for (int kk = 0; kk < kp; kk += 4) {
-    const int i = kk + (L.l >> 4);
-    T a0 = 0, a1 = 0, ag = 0;
-    if (i < g.nb) {
-      a0 = am.A0[(size_t)i * am.ldx + xa];
-      a1 = am.A1[(size_t)i * am.ldx + xa];
-    }
-    if (i < g.nd) ag = am.AG[(size_t)i * am.ldx + xa];
-    const T bu = B.at(0, i, L.c), bg = B.at(1, i, L.c);
-    P0 = MM::mma(a0, bu, P0);
-    P1 = MM::mma(a1, bu, P1);
-    PG = MM::mma(ag, bg, PG);
-  }
+  const int kq = L.l >> 4;
+  k_loop<T, 3, 8>(
+      kp / 4,
+      [&](int it, T(&a)[3]) {
+        const int i = 4 * it + kq;
+        a[0] = mat_at(am.A0, am.ldx, i, xa);
+        a[1] = mat_at(am.A1, am.ldx, i, xa);
+        a[2] = mat_at(am.AG, am.ldx, i, xa);
+      },
+      [&](int it, const T(&a)[3]) {
+        const int i = 4 * it + kq;
+        const T bu = B.at(0, i, L.c), bg = B.at(1, i, L.c);
+        P0 = MM::mma(a[0], bu, P0);
+        P1 = MM::mma(a[1], bu, P1);
+        PG = MM::mma(a[2], bg, PG);
+      });
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
     const T p1 = partner(P1[r]), pg = partner(PG[r]);
@@ -798,9 +831,10 @@ __global__ void __launch_bounds__(512) k_rhs_adj_mfma(Geo g, AdjMats<T> am, Prim
     T h1 = -s * W * (ky * p1 + kz * pg);
     T h2 = -s * W * (kz * p1 - ky * pg);
     const int x = x0 + MM::row(L.l, r);
-    if (mean_lane && x < g.n0) {  // real part of the mean mode: - AG^T nu_v, - AG^T nu_w
-      h1 -= meanv[2 * x];
-      h2 -= meanv[2 * x + 1];
+    if (mean_col) {  // the mean mode is real: - AG^T nu_v, - AG^T nu_w in its real lane
+      h0 = 0;
+      h1 = L.comp ? (T)0 : -PG[r];
+      h2 = L.comp ? (T)0 : -pg;
     }
     if (L.valid && x < g.n0) {
       const size_t o = (size_t)x * 2 * M + 2 * L.mcol + L.comp;
@@ -902,7 +936,7 @@ hipError_t launch_rhs_adj_mfma(const Geo &g, const AdjMats<T> &am, Prim nu, Work
   using V = typename VT<T>::V;
   if (!stage_covers(g.n0, STAGE_RHS_ADJ, 2 * ((g.nd + 3) / 4 * 4) * 8)) return hipErrorInvalidValue;
   hipLaunchKernelGGL((k_rhs_adj_mfma<T>), mfma_grid(g), mfma_block(g.n0),
-                     panel_bytes<T>(2, g.nd) + 2 * (size_t)g.n0 * sizeof(T), s, g, am, nu,
+                     panel_bytes<T>(2, g.nd), s, g, am, nu,
                      reinterpret_cast<V *>(ws.Hx));
   return hipGetLastError();
 }
