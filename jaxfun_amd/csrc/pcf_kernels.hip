@@ -816,12 +816,13 @@ __global__ void __launch_bounds__(256) k_diag(Geo g, const double *__restrict__ 
                                               const double2 *__restrict__ u2,
                                               double *__restrict__ out) {
   const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  const int x = blockIdx.y;  // one node per block row: the matrix rows are uniform
   double e = 0.0, dv = 0.0, cross = 0.0, msh = 0.0;
   if (m < g.m) {
     double kyd, kzd, kyo;
     mode_k(g, m, kyd, kzd, kyo);
     const double wt = ((m % g.kzs) == 0) ? 1.0 : 2.0;
-    for (int x = 0; x < g.n0; ++x) {
+    {
       double2 U0{0, 0}, U1{0, 0}, U2{0, 0}, D0{0, 0}, D1{0, 0};
       for (int i = 0; i < g.nb; ++i) {
         double2 c = u0[(size_t)i * g.m + m];
@@ -881,13 +882,14 @@ __global__ void __launch_bounds__(256) k_diag_mhd(Geo g, const double *__restric
                                                   const double2 *__restrict__ A, size_t cstride,
                                                   double *__restrict__ out) {
   const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  const int x = blockIdx.y;  // one node per block row
   double eb = 0.0, dv = 0.0, mb[3] = {0.0, 0.0, 0.0};
   if (m < g.m) {
     double kyd, kzd, kyo;
     mode_k(g, m, kyd, kzd, kyo);
     const double ky = kyo, kz = kzd;
     const double wt = ((m % g.kzs) == 0) ? 1.0 : 2.0;
-    for (int x = 0; x < g.n0; ++x) {
+    {
       double2 DX{0, 0}, DY{0, 0}, DZ{0, 0}, EY{0, 0}, EZ{0, 0};
       for (int i = 0; i < kx; ++i) {
         const double2 ax = A[(size_t)i * g.m + m];
@@ -1139,14 +1141,14 @@ hipError_t launch_diag(const Geo &g, const double *VB, const double *VD, const d
                        const double *VD1, const double *fejer, const double *xnode,
                        const double2 *u0, const double2 *u1, const double2 *u2, double *out,
                        hipStream_t s) {
-  hipLaunchKernelGGL(k_diag, dim3((g.m + 255) / 256), dim3(256), 0, s, g, VB, VD, VB1, VD1, fejer,
+  hipLaunchKernelGGL(k_diag, dim3((g.m + 255) / 256, g.n0), dim3(256), 0, s, g, VB, VD, VB1, VD1, fejer,
                      xnode, u0, u1, u2, out);
   return hipGetLastError();
 }
 
 hipError_t launch_diag_mhd(const Geo &g, const double *VDX, int kx, const double *VDY, const double *VEY, int kyz,
                            const double *fejer, const double2 *A, size_t cstride, double *out, hipStream_t s) {
-  hipLaunchKernelGGL(k_diag_mhd, dim3((g.m + 255) / 256), dim3(256), 0, s, g, VDX, kx, VDY, VEY, kyz, fejer, A,
+  hipLaunchKernelGGL(k_diag_mhd, dim3((g.m + 255) / 256, g.n0), dim3(256), 0, s, g, VDX, kx, VDY, VEY, kyz, fejer, A,
                      cstride, out);
   return hipGetLastError();
 }

@@ -42,6 +42,7 @@ struct FFTPlan {
 
 // Contraction matrices in the precision T, transposed so that the 16 output
 // rows of one matrix-core tile are contiguous for each contraction index.
+// Every matrix carries zero rows up to a multiple of 4 >= n0 (whole K steps).
 template <typename T>
 struct NodalMats {
   const T *VBt;   // [nb][ldx]  VB^T  (zero padded to ldx)
