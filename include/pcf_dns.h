@@ -38,7 +38,17 @@ extern "C" {
 #define PCF_PREC_MIXED 1  /* transforms + nonlinear FP32; solves, mean mode,
                              history and constraint rows FP64                */
 
-/* time integrators (reference: examples/channelflow_kmm.py:149-159) */
+/* time integrators (reference: examples/channelflow_kmm.py:149-159).
+ * SBDF3: two IMEXRK3 start-up steps, then steady SBDF3 (channelflow_kmm.py:937-1064).
+ * IMEXRK3: every step a low-storage IMEXRK3 step (:872-935); the history fields
+ *   are not touched.
+ * IMEXRK222: every step an ARS(2,2,2) step (:1123-1201 with
+ *   src/jaxfun/integrators/imex_rk.py:175-181).  The device history fields are
+ *   scratch under IMEXRK3 and IMEXRK222 (the caller keeps the state's history).
+ * Hydrodynamic step only for IMEXRK3 and IMEXRK222; the reverse mode and the
+ * steady / graph entry points are SBDF3 only. */
+#define PCF_TI_IMEXRK3 1
+#define PCF_TI_IMEXRK222 2
 #define PCF_TI_SBDF3 3
 
 /* nonlinear forms (reference: examples/channelflow_kmm.py:140-142) */
@@ -62,7 +72,7 @@ typedef struct pcf_config {
   double dt;               /* fixed time step                                */
   double padding_y, padding_z; /* dealiasing factors (1.5 = 3/2 rule)        */
   int32_t precision;       /* PCF_PREC_*                                     */
-  int32_t time_integrator; /* PCF_TI_SBDF3                                   */
+  int32_t time_integrator; /* PCF_TI_*                                       */
   int32_t nonlinear_form;  /* PCF_NL_ROTATIONAL                              */
   int32_t device;          /* HIP device ordinal                             */
   /* Magnetohydrodynamics: mhd = 1 evolves a magnetic vector potential A with

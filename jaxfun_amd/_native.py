@@ -19,7 +19,10 @@ WALLS_CONDUCTING = 0
 WALLS_INSULATING = 1
 PREC_FP64 = 0
 PREC_MIXED = 1
+TI_IMEXRK3 = 1
+TI_IMEXRK222 = 2
 TI_SBDF3 = 3
+TIME_INTEGRATORS = {"IMEXRK3": TI_IMEXRK3, "IMEXRK222": TI_IMEXRK222, "SBDF3": TI_SBDF3}
 NL_ROTATIONAL = 1
 
 # enum pcf_field

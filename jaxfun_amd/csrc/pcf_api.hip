@@ -39,6 +39,7 @@ int32_t fail(int32_t code, const std::string &msg) {
 
 const double IMEX_A[3] = {8.0 / 15.0, 5.0 / 12.0, 3.0 / 4.0};   // imex_rk.py:103
 const double IMEX_B[3] = {0.0, -17.0 / 60.0, -5.0 / 12.0};      // imex_rk.py:104
+const double ARS_G = (2.0 - std::sqrt(2.0)) / 2.0;                // imex_rk.py:176
 
 std::string validate(const pcf_config *c) {
   if (!c) return "config is NULL";
@@ -50,7 +51,10 @@ std::string validate(const pcf_config *c) {
   if (!(c->dt > 0)) return "dt must be positive";
   if (!(c->nu > 0)) return "nu must be positive";
   if (c->precision != PCF_PREC_FP64 && c->precision != PCF_PREC_MIXED) return "unknown precision";
-  if (c->time_integrator != PCF_TI_SBDF3) return "only SBDF3 is built";
+  if (c->time_integrator != PCF_TI_SBDF3 && c->time_integrator != PCF_TI_IMEXRK3 &&
+      c->time_integrator != PCF_TI_IMEXRK222)
+    return "unknown time integrator";
+  if (c->mhd && c->time_integrator != PCF_TI_SBDF3) return "the MHD step is built for SBDF3 only";
   if (c->nonlinear_form != PCF_NL_ROTATIONAL) return "only the rotational nonlinear form is built";
   if (c->mhd != 0 && c->mhd != 1) return "mhd must be 0 or 1";
   if (c->mhd && !(c->eta > 0)) return "eta must be positive";
@@ -129,6 +133,7 @@ struct pcf_handle {
   DevMats<float> mf;
   pcf::StructMats sm{};
   pcf::FactorSet fsteady{};   // LU factors of the steady SBDF3 systems
+  pcf::FactorSet fars{};      // LU factors of M - g dt L for the ARS(2,2,2) stages
   double *Rt = nullptr;       // parity-compressed reconstruction rows, [nd][ldr]
   int ldr = 0;
   double *scratch = nullptr;  // per-row data of the structured solves
@@ -570,6 +575,52 @@ int32_t startup_step(pcf_handle *h, void *const *tape = nullptr) {
   h->history_steps = std::min(2.0, h->history_steps + 1.0);
   h->have_old = 1.0;
   h->previous_dt = dt;
+  return PCF_OK;
+}
+
+// One ARS(2,2,2) step, the reference's IMEXRK222 KMM step
+// (channelflow_kmm.py:1123-1201; tableau src/jaxfun/integrators/imex_rk.py:175-181):
+// both stages solve (M - g dt L) x = rhs with
+//   rhs_1 = M x_n + g dt N_0,
+//   rhs_2 = M x_n + dt (d N_0 + (1 - d) N_1) + (1 - g) dt L x_1.
+// Since (M - g dt L) x_1 = rhs_1, the stage-2 linear term is
+// (1 - g) dt L x_1 = c1 (M x_1 - rhs_1) with c1 = (1 - g) / g, so
+//   rhs_2 = M ((1 - c1) x_n + c1 x_1) + dt (d - 1 + g) N_0 + dt (1 - d) N_1,
+// the M z + sum e N form of the factored steady solve (one factor set, fars).
+// The solution ring's other slots and two nonlinear slots are scratch.
+int32_t ars222_step(pcf_handle *h) {
+  const double dt = h->cfg.dt, gm = ARS_G, dl = 1.0 - 1.0 / (2.0 * ARS_G), c1 = (1.0 - gm) / gm;
+  const int cur = h->so[0], x1 = h->so[1], x2 = h->so[2];
+  const int NA = h->no[0], NB = h->no[3];
+  int32_t r = nonlinear_into(h, h->u0[cur], NA);
+  if (r) return r;
+  hipError_t e;
+  {
+    pcf::SolveArgs a = base_args(h, x1);
+    set_x(a, 0, h, cur, 1.0, 0.0);
+    a.nx = 1;
+    set_n(a, 0, h, NA, dt * gm);
+    a.nn = 1;
+    a.gamma = dt * gm;
+    if ((e = pcf::launch_solve_factored(h->geo, h->sm, h->fars, a, 0, 0, h->stream)) ||
+        (e = pcf::launch_reconstruct(h->geo, h->Rt, h->ldr, a, h->stream)))
+      return fail(PCF_E_HIP, std::string("solve launch: ") + hipGetErrorString(e));
+  }
+  if ((r = nonlinear_into(h, h->u0[x1], NB))) return r;
+  {
+    pcf::SolveArgs a = base_args(h, x2);
+    set_x(a, 0, h, cur, 1.0 - c1, 0.0);
+    set_x(a, 1, h, x1, c1, 0.0);
+    a.nx = 2;
+    set_n(a, 0, h, NA, dt * (dl - 1.0 + gm));
+    set_n(a, 1, h, NB, dt * (1.0 - dl));
+    a.nn = 2;
+    a.gamma = dt * gm;
+    if ((e = pcf::launch_solve_factored(h->geo, h->sm, h->fars, a, 0, 0, h->stream)) ||
+        (e = pcf::launch_reconstruct(h->geo, h->Rt, h->ldr, a, h->stream)))
+      return fail(PCF_E_HIP, std::string("solve launch: ") + hipGetErrorString(e));
+  }
+  rotate(h);  // x2 becomes the current slot
   return PCF_OK;
 }
 
@@ -1247,6 +1298,8 @@ int32_t pcf_create(const pcf_config *cfg, pcf_handle **out) {
       return bail(fail(PCF_E_UNSUPPORTED, std::string("kernel LDS limits: ") + hipGetErrorString(e)));
   }
   if ((r = build_factors(h, h->sm, cfg->dt * 6.0 / 11.0, h->fsteady))) return bail(r);
+  if (cfg->time_integrator == PCF_TI_IMEXRK222 && (r = build_factors(h, h->sm, cfg->dt * ARS_G, h->fars)))
+    return bail(r);
   if (h->mhd) {
     h->geo_eta = h->geo;
     h->geo_eta.nu = cfg->eta;
@@ -1388,8 +1441,21 @@ int32_t pcf_step(pcf_handle *h, int32_t nsteps) {
   if (!h) return fail(PCF_E_INVALID, "null handle");
   if (nsteps < 0) return fail(PCF_E_INVALID, "nsteps must be >= 0");
   PCF_HIP(hipSetDevice(h->device));
+  const int ti = h->cfg.time_integrator;
   for (int s = 0; s < nsteps; ++s) {
-    int32_t r = (h->history_steps < 2.0) ? startup_step(h) : steady_step(h);
+    int32_t r;
+    if (ti == PCF_TI_IMEXRK222) {
+      r = ars222_step(h);
+    } else if (ti == PCF_TI_IMEXRK3) {
+      // the IMEXRK3 stages of the start-up step; the history it writes is scratch
+      const double hs = h->history_steps, ho = h->have_old, pd = h->previous_dt;
+      r = startup_step(h);
+      h->history_steps = hs;
+      h->have_old = ho;
+      h->previous_dt = pd;
+    } else {
+      r = (h->history_steps < 2.0) ? startup_step(h) : steady_step(h);
+    }
     if (r) return r;
   }
   return PCF_OK;
@@ -1397,6 +1463,7 @@ int32_t pcf_step(pcf_handle *h, int32_t nsteps) {
 
 int32_t pcf_step_steady(pcf_handle *h, int32_t nsteps) {
   if (!h) return fail(PCF_E_INVALID, "null handle");
+  if (h->cfg.time_integrator != PCF_TI_SBDF3) return fail(PCF_E_UNSUPPORTED, "steady steps are SBDF3 steps");
   if (h->history_steps < 2.0) return fail(PCF_E_INVALID, "history not bootstrapped (history_steps < 2)");
   PCF_HIP(hipSetDevice(h->device));
   for (int s = 0; s < nsteps; ++s) {
@@ -1408,6 +1475,7 @@ int32_t pcf_step_steady(pcf_handle *h, int32_t nsteps) {
 
 int32_t pcf_step_steady_graph(pcf_handle *h, int32_t nsteps, int32_t replays) {
   if (!h) return fail(PCF_E_INVALID, "null handle");
+  if (h->cfg.time_integrator != PCF_TI_SBDF3) return fail(PCF_E_UNSUPPORTED, "steady steps are SBDF3 steps");
   if (h->history_steps < 2.0) return fail(PCF_E_INVALID, "history not bootstrapped (history_steps < 2)");
   if (nsteps <= 0 || nsteps % 12) return fail(PCF_E_INVALID, "graph length must be a positive multiple of 12 steps");
   PCF_HIP(hipSetDevice(h->device));
@@ -1507,6 +1575,7 @@ const char *pcf_stage_name(int32_t i) { return (i >= 0 && i < NSTAGE) ? STAGE_NA
 
 int32_t pcf_profile_steady(pcf_handle *h, int32_t nsteps, double *stage_ms) {
   if (!h || !stage_ms) return fail(PCF_E_INVALID, "null argument");
+  if (h->cfg.time_integrator != PCF_TI_SBDF3) return fail(PCF_E_UNSUPPORTED, "steady steps are SBDF3 steps");
   if (h->history_steps < 2.0) return fail(PCF_E_INVALID, "history not bootstrapped (history_steps < 2)");
   if (nsteps <= 0) return fail(PCF_E_INVALID, "nsteps must be positive");
   PCF_HIP(hipSetDevice(h->device));
@@ -1567,6 +1636,7 @@ int32_t pcf_set_cotangent(pcf_handle *h, int32_t field, const double *host) {
 int32_t pcf_vjp(pcf_handle *h, int32_t steps, int32_t checkpoint_every, int32_t seed) {
   if (!h) return fail(PCF_E_INVALID, "null handle");
   if (h->mhd) return fail(PCF_E_UNSUPPORTED, "the reverse mode covers the hydrodynamic step only");
+  if (h->cfg.time_integrator != PCF_TI_SBDF3) return fail(PCF_E_UNSUPPORTED, "the reverse mode covers SBDF3 only");
   if (steps < 1) return fail(PCF_E_INVALID, "steps must be >= 1");
   if (checkpoint_every < 1) return fail(PCF_E_INVALID, "checkpoint_every must be >= 1");
   if (seed != PCF_SEED_ENERGY && seed != PCF_SEED_COTANGENT) return fail(PCF_E_INVALID, "unknown seed");

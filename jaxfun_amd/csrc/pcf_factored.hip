@@ -312,22 +312,31 @@ __global__ void __launch_bounds__(256) k_solve_factored(Geo g, StructMats sm, Fa
         const int J = 2 * j + par;
         if (!mean) {
           const size_t o = (size_t)J * M + m;
+          // only the nx solution and nn nonlinear levels that are set
 #pragma unroll
           for (int l = 0; l < 3; ++l) {
-            const double2 xv = (sys == 0) ? a.u0[l][o] : a.g[l][o];
-            const double2 nv = (sys == 0) ? a.Nu[l][o] : a.Ng[l][o];
-            r.z.x += a.c[l] * xv.x;
-            r.z.y += a.c[l] * xv.y;
-            r.n.x += a.e[l] * nv.x;
-            r.n.y += a.e[l] * nv.y;
+            if (l < a.nx) {
+              const double2 xv = (sys == 0) ? a.u0[l][o] : a.g[l][o];
+              r.z.x += a.c[l] * xv.x;
+              r.z.y += a.c[l] * xv.y;
+            }
+            if (l < a.nn) {
+              const double2 nv = (sys == 0) ? a.Nu[l][o] : a.Ng[l][o];
+              r.n.x += a.e[l] * nv.x;
+              r.n.y += a.e[l] * nv.y;
+            }
           }
         } else {
 #pragma unroll
           for (int l = 0; l < 3; ++l) {
-            r.z.x += a.c[l] * a.v[l][J];
-            r.z.y += a.c[l] * a.w[l][J];
-            r.n.x += a.e[l] * a.Nv[l][J];
-            r.n.y += a.e[l] * a.Nw[l][J];
+            if (l < a.nx) {
+              r.z.x += a.c[l] * a.v[l][J];
+              r.z.y += a.c[l] * a.w[l][J];
+            }
+            if (l < a.nn) {
+              r.n.x += a.e[l] * a.Nv[l][J];
+              r.n.y += a.e[l] * a.Nw[l][J];
+            }
           }
         }
         return r;
@@ -522,6 +531,13 @@ hipError_t launch_solve_factored(const Geo &g, const StructMats &sm, const Facto
                                  int ncomp, size_t cstride, hipStream_t s) {
   // ncomp > 0: the Dirichlet blocks only, for ncomp fields cstride apart
   const int spb0 = ncomp > 0 ? 2 : 0, nf = ncomp > 0 ? ncomp : 1;
+  // every level the kernel reads must be set
+  if (a.nx < 1 || a.nx > 3 || a.nn < 0 || a.nn > 3 || !a.g_out) return hipErrorInvalidValue;
+  for (int l = 0; l < a.nx; ++l)
+    if (!a.g[l] || (!ncomp && (!a.u0[l] || !a.v[l] || !a.w[l]))) return hipErrorInvalidValue;
+  for (int l = 0; l < a.nn; ++l)
+    if (!a.Ng[l] || (!ncomp && (!a.Nu[l] || !a.Nv[l] || !a.Nw[l]))) return hipErrorInvalidValue;
+  if (!ncomp && (!a.u0_out || !a.v_out || !a.w_out)) return hipErrorInvalidValue;
   // the most lanes per block whose rows and factors fit the default 64 KB of LDS
 #define PCF_FSOLVE(SLV)                                                                                  \
   hipLaunchKernelGGL(k_solve_factored<SLV>, dim3((4 - spb0) * ((g.m + SLV - 1) / SLV), nf), dim3(256),    \

@@ -66,6 +66,10 @@ class PlaneCouetteMHDHIP(PlaneCouetteFluctuationHIP):
         self.Rm = float(Re if Rm is None else Rm)
         if not self.Rm > 0:
             raise ValueError("Rm must be positive")
+        if time_integrator not in {"SBDF3", "IMEXRK3", "IMEXRK222", "CNAB2", None}:
+            raise ValueError("time_integrator must be one of {'IMEXRK222', 'IMEXRK3', 'CNAB2', 'SBDF3'}")
+        if time_integrator != "SBDF3":
+            raise NotImplementedError("the MHD step is built for SBDF3 only")
         self.eta = float(U_wall) / self.Rm
         self.magnetic_amplitude = float(magnetic_amplitude)
         self.background_b = tuple(float(v) for v in background_b)

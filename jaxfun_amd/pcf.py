@@ -52,8 +52,9 @@ class PlaneCouetteFluctuationHIP:
 
     Constructor follows ``PlaneCouetteFluctuationJax.__init__``
     (``examples/pcf_fluctuations_jax.py:35-75``).  Built options:
-    Chebyshev family, SBDF3 with IMEXRK3 startup, rotational nonlinear form,
-    optimized coefficient path.  Other valid reference values raise
+    Chebyshev family; SBDF3 with IMEXRK3 startup, IMEXRK3 alone or IMEXRK222
+    (``time_integrator=None`` selects IMEXRK222 as in the reference); rotational
+    nonlinear form; optimized coefficient path.  Other valid reference values raise
     ``NotImplementedError``; invalid values raise ``ValueError`` as in
     ``examples/channelflow_kmm.py:140-187``.
     """
@@ -92,8 +93,10 @@ class PlaneCouetteFluctuationHIP:
             raise ValueError("family must be 'L' or 'C'")
         if fam.startswith("L"):
             raise NotImplementedError("only the Chebyshev family is built")
-        if time_integrator != "SBDF3":
-            raise NotImplementedError("only SBDF3 (IMEXRK3 startup) is built")
+        if time_integrator is None:  # the reference's default ARS stepper (channelflow_kmm.py:160-172)
+            time_integrator = "IMEXRK222"
+        if time_integrator == "CNAB2":
+            raise NotImplementedError("CNAB2 is not built (SBDF3, IMEXRK3 and IMEXRK222 are)")
         if nonlinear_form != "rotational":
             raise NotImplementedError("only the rotational nonlinear form is built")
         if coefficient_path != "optimized":
@@ -112,7 +115,7 @@ class PlaneCouetteFluctuationHIP:
         self.family = "C"
         self.padding_factor = tuple(float(p) for p in padding_factor)
         self.perturbation_amplitude = float(perturbation_amplitude)
-        self.time_integrator = "SBDF3"
+        self.time_integrator = time_integrator
         self.nonlinear_form = nonlinear_form
         self.coefficient_path = coefficient_path
         self.solve_batching = solve_batching
@@ -130,7 +133,7 @@ class PlaneCouetteFluctuationHIP:
         cfg.dt = self.dt
         cfg.padding_y, cfg.padding_z = self.padding_factor[1], self.padding_factor[2]
         cfg.precision = _PRECISIONS[self.precision]
-        cfg.time_integrator = nat.TI_SBDF3
+        cfg.time_integrator = nat.TIME_INTEGRATORS[self.time_integrator]
         cfg.nonlinear_form = nat.NL_ROTATIONAL
         cfg.device = int(device)
         self._configure(cfg)
@@ -329,23 +332,38 @@ class PlaneCouetteFluctuationHIP:
         """One step (``channelflow_kmm.py:1123-1137``); state in, new state out."""
         if dt is not None and not math.isclose(float(dt), self.dt, rel_tol=8 * np.finfo(float).eps):
             raise NotImplementedError(
-                "SBDF3 currently supports fixed dt only; start a fresh solver/history "
+                "this build runs a fixed dt per solver; start a fresh solver/history "
                 "for a different timestep")
         self._ensure_loaded(state)
         self.advance(1)
-        return self.fetch()
+        return self._result(state)
 
     def solve(self, state: KMMState, steps: int) -> KMMState:
         """``KMM.solve`` (``channelflow_kmm.py:1231-1247``)."""
         self._ensure_loaded(state)
         self.advance(int(steps))
-        return self.fetch()
+        return self._result(state)
+
+    def _result(self, state: KMMState) -> KMMState:
+        """The device state after steps from ``state``, with the history fields
+        each integrator leaves: SBDF3 its own rings; IMEXRK3 those of ``state``
+        unchanged (``replace``, ``channelflow_kmm.py:935``); IMEXRK222 only
+        ``nonlinear_old``, ``have_old`` and ``previous_dt`` of ``state``
+        (``:1194-1200``), the device's history slots being scratch there."""
+        st = self.fetch()
+        if self.time_integrator == "IMEXRK3":
+            st = replace(state, u=st.u, g=st.g)
+        elif self.time_integrator == "IMEXRK222":
+            st = KMMState(u=st.u, g=st.g, nonlinear_old=state.nonlinear_old, have_old=state.have_old,
+                          previous_dt=state.previous_dt)
+        self._loaded = st
+        return st
 
     def set_dt(self, dt: float) -> None:
         """``KMM.set_dt`` (``channelflow_kmm.py:1207-1222``): SBDF3 is fixed-step."""
         if not math.isclose(float(dt), self.dt, rel_tol=8.0 * np.finfo(float).eps, abs_tol=0.0):
             raise NotImplementedError(
-                "SBDF3 currently supports fixed dt only; start a fresh solver/history "
+                "this build runs a fixed dt per solver; start a fresh solver/history "
                 "for a different timestep")
 
     def diagnostics(self, state: KMMState | None = None) -> dict:
@@ -371,6 +389,8 @@ class PlaneCouetteFluctuationHIP:
         ``(final_state, gradient)`` with the gradient a ``KMMState`` holding
         d/d(leaf) of every leaf of ``state`` (history scalars 0.0).
         """
+        if self.time_integrator != "SBDF3":
+            raise NotImplementedError("the reverse mode covers the SBDF3 step only")
         self._ensure_loaded(state)
         self._loaded = None
         seed = nat.SEED_ENERGY
@@ -417,7 +437,8 @@ def state_leaves(state: KMMState) -> list:
     """Leaves in the reference pytree order (``channelflow_kmm.py:65-76``)."""
     out = list(state.u) + [state.g]
     for grp in (state.nonlinear_old, state.nonlinear_older, state.solution_old, state.solution_older):
-        out.extend(grp)
+        if grp is not None:  # None is an empty subtree, as in jax
+            out.extend(grp)
     out.extend([np.asarray(state.history_steps), np.asarray(state.have_old), np.asarray(state.previous_dt)])
     return out
 

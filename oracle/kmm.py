@@ -214,6 +214,11 @@ def galerkin_matrix(S_test: np.ndarray, S_trial: np.ndarray, deriv: int) -> np.n
 IMEXRK3_A = (8.0 / 15.0, 5.0 / 12.0, 3.0 / 4.0)
 IMEXRK3_B = (0.0, -17.0 / 60.0, -5.0 / 12.0)
 SBDF3_GAMMA = 6.0 / 11.0  # sbdf3.py:11-12 IMPLICIT_SCALE
+# ARS(2,2,2) tableau (src/jaxfun/integrators/imex_rk.py:175-181): a[1][1] = a[2][2] = g,
+# a[2][1] = 1 - g, b[1][0] = g, b[2][0] = d, b[2][1] = 1 - d
+ARS222_G = (2.0 - 2.0**0.5) / 2.0
+ARS222_D = 1.0 - 1.0 / (2.0 * ARS222_G)
+INTEGRATORS = ("SBDF3", "IMEXRK3", "IMEXRK222")
 
 
 @dataclass
@@ -233,7 +238,8 @@ class OracleState:
     def leaves(self) -> list:
         out = list(self.u) + [self.g]
         for group in (self.nonlinear_old, self.nonlinear_older, self.solution_old, self.solution_older):
-            out.extend(group)
+            if group is not None:  # None is an empty subtree, as in jax
+                out.extend(group)
         out.extend([np.asarray(self.history_steps), np.asarray(self.have_old), np.asarray(self.previous_dt)])
         return out
 
@@ -305,7 +311,11 @@ class PCFOracle:
         dt: float = 0.01,
         padding_factor=(1.0, 1.5, 1.5),
         perturbation_amplitude: float = 0.05,
+        time_integrator: str = "SBDF3",
     ) -> None:
+        if time_integrator not in INTEGRATORS:
+            raise ValueError(f"time_integrator must be one of {INTEGRATORS}")
+        self.time_integrator = time_integrator
         self.N = tuple(int(n) for n in N)
         self.domain = domain
         self.Re = float(Re)
@@ -399,6 +409,7 @@ class PCFOracle:
         self._fac = fac
         self.steady = fac(self.dt * SBDF3_GAMMA)
         self.startup = tuple(fac((IMEXRK3_A[rk] + IMEXRK3_B[rk]) * self.dt / 2.0) for rk in range(3))
+        self.ars = fac(self.dt * ARS222_G) if self.time_integrator == "IMEXRK222" else None
 
     # ---------------------------------------------------------------- helpers
     def mask(self, c):
@@ -619,8 +630,45 @@ class PCFOracle:
             prev = cur
         return replace(s, u=u_stage, g=g_stage)
 
+    def linear_rows(self, p):
+        """L applied to the primary variables (u0, g, v00, w00)."""
+        return (self.Lu_apply(p[0]), self.Lg_apply(p[1]), self.L00 @ p[2], self.L00 @ p[3])
+
+    def step_ars222(self, s: OracleState) -> OracleState:
+        """The ARS IMEX-RK step with the IMEXRK222 tableau (``channelflow_kmm.py:1123-1201``,
+        ``ars_stage_rhs`` of ``src/jaxfun/integrators/coupled.py:13-44``): one
+        implicit operator M - g dt L, stage k's right-hand side
+        M x_n + dt sum_j b[k+1][j] N_j + dt sum_j a[k+1][j+1] L x_j.  The history
+        fields other than ``nonlinear_old`` are dropped, as the reference's
+        ``KMMState`` constructor there leaves them at their defaults."""
+        dt = self.dt
+        a = ((0.0, 0.0, 0.0), (0.0, ARS222_G, 0.0), (0.0, 1.0 - ARS222_G, ARS222_G))
+        b = ((0.0, 0.0, 0.0), (ARS222_G, 0.0, 0.0), (ARS222_D, 1.0 - ARS222_D, 0.0))
+        base = self.mass_rows(self.primary(s))
+        stage = OracleState(u=s.u, g=s.g)
+        nl, lin = [], []
+        for rk in range(2):
+            nl.append(self.nonlinear_rhs(self.convection(stage)))
+            if rk > 0:
+                lin.append(self.linear_rows(self.primary(stage)))
+            rhs = list(base)
+            for j in range(rk + 1):
+                rhs = [r + dt * b[rk + 1][j] * n for r, n in zip(rhs, nl[j])]
+            for j in range(rk):
+                rhs = [r + dt * a[rk + 1][j + 1] * ell for r, ell in zip(rhs, lin[j])]
+            u0n, gn, vn, wn = self._solve(self.ars, *rhs)
+            stage = OracleState(u=self.reconstruct(u0n, gn, vn, wn), g=gn)
+        return OracleState(u=stage.u, g=stage.g, nonlinear_old=s.nonlinear_old, have_old=s.have_old,
+                           previous_dt=s.previous_dt)
+
     def step(self, s: OracleState) -> OracleState:
-        """``_step_sbdf3`` (``channelflow_kmm.py:937-1064``): startup then steady."""
+        """One step of the configured integrator: ``_step_sbdf3``
+        (``channelflow_kmm.py:937-1064``, startup then steady), ``_step_imexrk3``
+        (``:872-935``, history passed through) or the IMEXRK222 ARS step."""
+        if self.time_integrator == "IMEXRK3":
+            return self.step_imexrk3(s)
+        if self.time_integrator == "IMEXRK222":
+            return self.step_ars222(s)
         cur_p = self.primary(s)
         cur_n = self.nonlinear_rhs(self.convection(s))
         if s.history_steps < 2:

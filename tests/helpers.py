@@ -55,7 +55,8 @@ def max_rel_errors(a, b) -> dict:
 def oracle_for(solver) -> PCFOracle:
     return PCFOracle(N=solver.N, domain=solver.domain, Re=solver.Re, U_wall=solver.U_wall, dt=solver.dt,
                      padding_factor=solver.padding_factor,
-                     perturbation_amplitude=solver.perturbation_amplitude)
+                     perturbation_amplitude=solver.perturbation_amplitude,
+                     time_integrator=getattr(solver, "time_integrator", "SBDF3"))
 
 
 # ------------------------------------------------------------------ MHD

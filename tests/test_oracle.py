@@ -202,6 +202,57 @@ def test_sbdf3_agrees_with_small_dt_imexrk3():
     assert float(np.linalg.norm(va - vb)) < 2e-10
 
 
+def _self_convergence_orders(integrator, T=0.04, counts=(8, 16, 32, 64)):
+    sols = []
+    for steps in counts:
+        o = PCFOracle(N=(9, 4, 4), Re=200.0, dt=T / steps, padding_factor=(1.0, 1.0, 1.0),
+                      perturbation_amplitude=0.01, time_integrator=integrator)
+        s = o.solve(_consistent(o, o.initial_state()), steps)
+        sols.append(np.concatenate([a.ravel() for a in list(s.u) + [s.g]]))
+    errs = [float(np.linalg.norm(sols[i] - sols[i + 1])) for i in range(len(sols) - 1)]
+    return errs, [math.log2(errs[i] / errs[i + 1]) for i in range(len(errs) - 1)]
+
+
+def test_imexrk222_is_second_order():
+    """ARS(2,2,2) self-convergence at the SBDF3 test's setting: orders in [1.8, 2.2]."""
+    errs, orders = _self_convergence_orders("IMEXRK222")
+    assert all(1.8 <= q <= 2.2 for q in orders), (errs, orders)
+
+
+def test_imexrk3_alone_is_second_order():
+    """The low-storage IMEXRK3 integrator run alone: third order in its explicit
+    terms, but the implicit stages (Crank-Nicolson-like, gamma = (a + b) dt / 2)
+    make the coupled scheme second order: orders in [1.8, 2.2]."""
+    errs, orders = _self_convergence_orders("IMEXRK3")
+    assert all(1.8 <= q <= 2.2 for q in orders), (errs, orders)
+
+
+def test_imexrk222_converges_to_sbdf3_solution():
+    """Two independent integrators agree as dt -> 0: ARS(2,2,2) at dt = T/640
+    against SBDF3 at dt = T/80 (errors ~ dt^2 and dt^3 respectively)."""
+    T = 0.02
+    runs = []
+    for integ, steps in (("IMEXRK222", 640), ("SBDF3", 80)):
+        o = PCFOracle(N=(9, 4, 4), Re=200.0, dt=T / steps, padding_factor=(1.0, 1.0, 1.0),
+                      perturbation_amplitude=0.01, time_integrator=integ)
+        s = o.solve(_consistent(o, o.initial_state()), steps)
+        runs.append(np.concatenate([x.ravel() for x in list(s.u) + [s.g]]))
+    assert float(np.linalg.norm(runs[0] - runs[1])) < 1e-9 * float(np.linalg.norm(runs[1]))
+
+
+def test_imexrk222_history_semantics():
+    """After an IMEXRK222 step only ``nonlinear_old``, ``have_old`` and
+    ``previous_dt`` survive (channelflow_kmm.py:1194-1200); IMEXRK3 passes every
+    history field through (``replace`` at :935)."""
+    o = PCFOracle(N=(9, 8, 8), Re=200.0, dt=1e-3, perturbation_amplitude=0.01)
+    s = o.solve(o.initial_state(), 3)  # SBDF3 history filled
+    a = PCFOracle(N=(9, 8, 8), Re=200.0, dt=1e-3, perturbation_amplitude=0.01, time_integrator="IMEXRK222").step(s)
+    assert a.nonlinear_old is s.nonlinear_old and a.nonlinear_older is None and a.solution_old is None
+    assert a.history_steps == 0.0 and a.have_old == s.have_old
+    b = PCFOracle(N=(9, 8, 8), Re=200.0, dt=1e-3, perturbation_amplitude=0.01, time_integrator="IMEXRK3").step(s)
+    assert b.solution_older is s.solution_older and b.history_steps == s.history_steps
+
+
 def test_nonlinear_rows_rounding_sensitivity():
     """Nu is a wall-normal derivative of the dealiased product: a 1e-16 relative
     perturbation of the state moves it by ~1e-12 relative already at 33^3.  This
