@@ -17,7 +17,8 @@ Also reported on the same line:
   * the adjoint/forward cost ratio with its finite-difference check;
   * the insulating-wall MHD step (FP64 and mixed) at the same resolution;
   * FP64 steps/s on the reference's other published configurations (hydro
-    97^3 and 128^3, conducting-wall MHD 65^3 and 97^3) beside its figures.
+    97^3 and 128^3, conducting-wall MHD 65^3 and 97^3, the IMEXRK3 integrator
+    at 65^3) beside its figures.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -374,9 +375,10 @@ def mhd_leg(steps: int, warmup: int) -> dict:
 
 # The reference's other published FP64 throughputs (RTX 5090 Laptop, Re = Rm =
 # 400, dt = 1e-3, analytic initial field of amplitude 0.02 / magnetic 0.005,
-# benchmark-results/FINAL_LOCAL_CHECKPOINT.md:58-59,79-81): ms per step.
+# benchmark-results/FINAL_LOCAL_CHECKPOINT.md:58-59,64,79-81): ms per step.
 PUBLISHED = [("hydro", (97, 96, 96), 92.94), ("hydro", (128, 128, 128), 219.72),
-             ("mhd_conducting", (65, 64, 64), 45.790), ("mhd_conducting", (97, 96, 96), 163.28)]
+             ("mhd_conducting", (65, 64, 64), 45.790), ("mhd_conducting", (97, 96, 96), 163.28),
+             ("hydro_imexrk3", (65, 64, 64), 84.740)]  # :64, the rotational IMEXRK3 integrator alone
 
 
 def published_leg(steps: int = 120, warmup: int = 24) -> list:
@@ -386,17 +388,25 @@ def published_leg(steps: int = 120, warmup: int = 24) -> list:
 
     out = []
     for kind, N, ms in PUBLISHED:
-        if kind == "hydro":
-            solver = PlaneCouetteFluctuationHIP(N=N, Re=400.0, dt=1e-3, perturbation_amplitude=0.02)
+        if kind.startswith("hydro"):
+            integ = "IMEXRK3" if kind == "hydro_imexrk3" else "SBDF3"
+            solver = PlaneCouetteFluctuationHIP(N=N, Re=400.0, dt=1e-3, perturbation_amplitude=0.02,
+                                                time_integrator=integ)
         else:
             solver = PlaneCouetteMHDHIP(N=N, Re=400.0, Rm=400.0, dt=1e-3, perturbation_amplitude=0.02,
                                         magnetic_amplitude=0.005)
         solver.load(solver.initial_state())
-        solver.advance(2)
-        solver.advance_steady(warmup, graph_block=GRAPH_BLOCK)
-        solver.synchronize()
-        t0 = time.perf_counter()
-        solver.advance_steady(steps, graph_block=GRAPH_BLOCK)
+        if kind == "hydro_imexrk3":  # every step three IMEXRK3 stages, launched step by step
+            solver.advance(warmup)
+            solver.synchronize()
+            t0 = time.perf_counter()
+            solver.advance(steps)
+        else:
+            solver.advance(2)
+            solver.advance_steady(warmup, graph_block=GRAPH_BLOCK)
+            solver.synchronize()
+            t0 = time.perf_counter()
+            solver.advance_steady(steps, graph_block=GRAPH_BLOCK)
         solver.synchronize()
         el = time.perf_counter() - t0
         solver.close()

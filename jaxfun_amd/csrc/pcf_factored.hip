@@ -312,31 +312,24 @@ __global__ void __launch_bounds__(256) k_solve_factored(Geo g, StructMats sm, Fa
         const int J = 2 * j + par;
         if (!mean) {
           const size_t o = (size_t)J * M + m;
-          // only the nx solution and nn nonlinear levels that are set
+          // all three levels of each kind are valid (the launcher points unset
+          // ones at a set level with a zero coefficient): branch-free loads
 #pragma unroll
           for (int l = 0; l < 3; ++l) {
-            if (l < a.nx) {
-              const double2 xv = (sys == 0) ? a.u0[l][o] : a.g[l][o];
-              r.z.x += a.c[l] * xv.x;
-              r.z.y += a.c[l] * xv.y;
-            }
-            if (l < a.nn) {
-              const double2 nv = (sys == 0) ? a.Nu[l][o] : a.Ng[l][o];
-              r.n.x += a.e[l] * nv.x;
-              r.n.y += a.e[l] * nv.y;
-            }
+            const double2 xv = (sys == 0) ? a.u0[l][o] : a.g[l][o];
+            const double2 nv = (sys == 0) ? a.Nu[l][o] : a.Ng[l][o];
+            r.z.x += a.c[l] * xv.x;
+            r.z.y += a.c[l] * xv.y;
+            r.n.x += a.e[l] * nv.x;
+            r.n.y += a.e[l] * nv.y;
           }
         } else {
 #pragma unroll
           for (int l = 0; l < 3; ++l) {
-            if (l < a.nx) {
-              r.z.x += a.c[l] * a.v[l][J];
-              r.z.y += a.c[l] * a.w[l][J];
-            }
-            if (l < a.nn) {
-              r.n.x += a.e[l] * a.Nv[l][J];
-              r.n.y += a.e[l] * a.Nw[l][J];
-            }
+            r.z.x += a.c[l] * a.v[l][J];
+            r.z.y += a.c[l] * a.w[l][J];
+            r.n.x += a.e[l] * a.Nv[l][J];
+            r.n.y += a.e[l] * a.Nw[l][J];
           }
         }
         return r;
@@ -527,8 +520,9 @@ hipError_t launch_factor(const Geo &g, const StructMats &sm, double gamma, Facto
   return hipGetLastError();
 }
 
-hipError_t launch_solve_factored(const Geo &g, const StructMats &sm, const FactorSet &fs, const SolveArgs &a,
+hipError_t launch_solve_factored(const Geo &g, const StructMats &sm, const FactorSet &fs, const SolveArgs &args,
                                  int ncomp, size_t cstride, hipStream_t s) {
+  SolveArgs a = args;
   // ncomp > 0: the Dirichlet blocks only, for ncomp fields cstride apart
   const int spb0 = ncomp > 0 ? 2 : 0, nf = ncomp > 0 ? ncomp : 1;
   // every level the kernel reads must be set
@@ -538,6 +532,22 @@ hipError_t launch_solve_factored(const Geo &g, const StructMats &sm, const Facto
   for (int l = 0; l < a.nn; ++l)
     if (!a.Ng[l] || (!ncomp && (!a.Nu[l] || !a.Nv[l] || !a.Nw[l]))) return hipErrorInvalidValue;
   if (!ncomp && (!a.u0_out || !a.v_out || !a.w_out)) return hipErrorInvalidValue;
+  // the kernel reads three levels of each kind: unset ones repeat level 0 of
+  // the solutions with a zero coefficient
+  for (int l = a.nx; l < 3; ++l) {
+    a.u0[l] = a.u0[0];
+    a.g[l] = a.g[0];
+    a.v[l] = a.v[0];
+    a.w[l] = a.w[0];
+    a.c[l] = 0.0;
+  }
+  for (int l = a.nn; l < 3; ++l) {
+    a.Nu[l] = a.u0[0];
+    a.Ng[l] = a.g[0];
+    a.Nv[l] = a.v[0];
+    a.Nw[l] = a.w[0];
+    a.e[l] = 0.0;
+  }
   // the most lanes per block whose rows and factors fit the default 64 KB of LDS
 #define PCF_FSOLVE(SLV)                                                                                  \
   hipLaunchKernelGGL(k_solve_factored<SLV>, dim3((4 - spb0) * ((g.m + SLV - 1) / SLV), nf), dim3(256),    \
