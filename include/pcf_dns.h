@@ -45,8 +45,9 @@ extern "C" {
  * IMEXRK222: every step an ARS(2,2,2) step (:1123-1201 with
  *   src/jaxfun/integrators/imex_rk.py:175-181).  The device history fields are
  *   scratch under IMEXRK3 and IMEXRK222 (the caller keeps the state's history).
- * Hydrodynamic step only for IMEXRK3 and IMEXRK222; the reverse mode and the
- * steady / graph entry points are SBDF3 only. */
+ * The MHD step runs SBDF3 and IMEXRK222 (examples/pcf_mhd_jax.py:389-486, every
+ * history field passed through); the reverse mode and the steady / graph entry
+ * points are SBDF3 only. */
 #define PCF_TI_IMEXRK3 1
 #define PCF_TI_IMEXRK222 2
 #define PCF_TI_SBDF3 3

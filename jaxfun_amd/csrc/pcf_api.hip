@@ -54,7 +54,7 @@ std::string validate(const pcf_config *c) {
   if (c->time_integrator != PCF_TI_SBDF3 && c->time_integrator != PCF_TI_IMEXRK3 &&
       c->time_integrator != PCF_TI_IMEXRK222)
     return "unknown time integrator";
-  if (c->mhd && c->time_integrator != PCF_TI_SBDF3) return "the MHD step is built for SBDF3 only";
+  if (c->mhd && c->time_integrator == PCF_TI_IMEXRK3) return "the MHD step is built for SBDF3 and IMEXRK222";
   if (c->nonlinear_form != PCF_NL_ROTATIONAL) return "only the rotational nonlinear form is built";
   if (c->mhd != 0 && c->mhd != 1) return "mhd must be 0 or 1";
   if (c->mhd && !(c->eta > 0)) return "eta must be positive";
@@ -124,6 +124,7 @@ struct pcf_handle {
   double *dVC = nullptr, *dVC1 = nullptr, *dresid = nullptr;
   pcf::Geo geo_eta{};         // geo with nu = eta: the A systems are the g systems at eta
   pcf::FactorSet fA{};        // steady LU factors of the A systems (Dirichlet blocks)
+  pcf::FactorSet fA_ars{};    // ARS(2,2,2) LU factors of the A systems
   double *ddiagm = nullptr;
   int so[3] = {0, 1, 2};     // solution ring: cur, old, older
   int no[4] = {0, 1, 2, 3};  // nonlinear ring: free0, old, older, free1
@@ -320,11 +321,11 @@ int32_t build_factors(pcf_handle *h, const pcf::StructMats &sm, double gamma, pc
 
 // Steady factors of the vector-potential systems S_A = M_A - gamma L_A, the
 // Dirichlet systems at nu = eta (examples/pcf_mhd_jax.py:116-139).
-int32_t build_factors_A(pcf_handle *h, double gamma) {
+int32_t build_factors_A(pcf_handle *h, double gamma, pcf::FactorSet &fs) {
   int32_t r;
   for (int p = 0; p < 2; ++p)
-    if ((r = h->alloc(&h->fA.blk[1][p], pcf::factor_doubles(h->geo, h->sm, 1, p)))) return r;
-  hipError_t e = pcf::launch_factor(h->geo_eta, h->sm, gamma, h->fA, true, h->stream);
+    if ((r = h->alloc(&fs.blk[1][p], pcf::factor_doubles(h->geo, h->sm, 1, p)))) return r;
+  hipError_t e = pcf::launch_factor(h->geo_eta, h->sm, gamma, fs, true, h->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
   if (e != hipSuccess) return fail(PCF_E_HIP, std::string("factorisation: ") + hipGetErrorString(e));
   return PCF_OK;
@@ -393,11 +394,17 @@ int32_t nonlinear_into(pcf_handle *h, const double2 *u0, int slot, void *ytape =
 
 // Structured solve + reconstruction of one implicit update; under MHD also
 // the vector-potential solve described by aA (g side, component 0).
-hipError_t implicit_update(pcf_handle *h, const pcf::SolveArgs &a, bool steady,
-                           const pcf::SolveArgs *aA = nullptr) {
+// factored: the stored factors fs (flow) and fsA (A), default the steady SBDF3 ones;
+// otherwise factorised on the fly at a.gamma.
+hipError_t implicit_update(pcf_handle *h, const pcf::SolveArgs &a, bool factored,
+                           const pcf::SolveArgs *aA = nullptr, const pcf::FactorSet *fs = nullptr,
+                           const pcf::FactorSet *fsA = nullptr) {
   hipError_t e;
+  const bool steady = factored;
+  if (!fs) fs = &h->fsteady;
+  if (!fsA) fsA = &h->fA;
   if ((e = mark(h, 5, 0))) return e;
-  if ((e = steady ? pcf::launch_solve_factored(h->geo, h->sm, h->fsteady, a, 0, 0, h->stream)
+  if ((e = steady ? pcf::launch_solve_factored(h->geo, h->sm, *fs, a, 0, 0, h->stream)
                   : pcf::launch_solve(h->geo, h->sm, a, h->scratch, false, h->stream)))
     return e;
   if (aA) {
@@ -405,7 +412,7 @@ hipError_t implicit_update(pcf_handle *h, const pcf::SolveArgs &a, bool steady,
     // conducting walls, Ax alone for insulating walls
     const int ntd = h->insul ? 1 : 3;
     if (steady) {
-      if ((e = pcf::launch_solve_factored(h->geo_eta, h->sm, h->fA, *aA, ntd, h->cstride, h->stream))) return e;
+      if ((e = pcf::launch_solve_factored(h->geo_eta, h->sm, *fsA, *aA, ntd, h->cstride, h->stream))) return e;
     } else {
       for (int c = 0; c < ntd; ++c) {
         pcf::SolveArgs ac = *aA;
@@ -587,12 +594,16 @@ int32_t startup_step(pcf_handle *h, void *const *tape = nullptr) {
 // (1 - g) dt L x_1 = c1 (M x_1 - rhs_1) with c1 = (1 - g) / g, so
 //   rhs_2 = M ((1 - c1) x_n + c1 x_1) + dt (d - 1 + g) N_0 + dt (1 - d) N_1,
 // the M z + sum e N form of the factored steady solve (one factor set, fars).
+// Under MHD the vector potential takes the same two stages (its Dirichlet
+// systems with fA_ars; for insulating walls A_y, A_z by the bordered solve,
+// whose replaced wall rows ignore the right-hand side, so the identity holds
+// on every row it uses; examples/pcf_mhd_jax.py:389-486).
 // The solution ring's other slots and two nonlinear slots are scratch.
 int32_t ars222_step(pcf_handle *h) {
   const double dt = h->cfg.dt, gm = ARS_G, dl = 1.0 - 1.0 / (2.0 * ARS_G), c1 = (1.0 - gm) / gm;
   const int cur = h->so[0], x1 = h->so[1], x2 = h->so[2];
   const int NA = h->no[0], NB = h->no[3];
-  int32_t r = nonlinear_into(h, h->u0[cur], NA);
+  int32_t r = nonlinear_into(h, h->u0[cur], NA, nullptr, h->A[cur]);
   if (r) return r;
   hipError_t e;
   {
@@ -602,11 +613,15 @@ int32_t ars222_step(pcf_handle *h) {
     set_n(a, 0, h, NA, dt * gm);
     a.nn = 1;
     a.gamma = dt * gm;
-    if ((e = pcf::launch_solve_factored(h->geo, h->sm, h->fars, a, 0, 0, h->stream)) ||
-        (e = pcf::launch_reconstruct(h->geo, h->Rt, h->ldr, a, h->stream)))
+    pcf::SolveArgs aA{};
+    if (h->mhd) {
+      const int xs[3] = {cur, 0, 0}, ns[3] = {NA, 0, 0};
+      aA = a_args_from(h, a, xs, 1, ns, 1, x1);
+    }
+    if ((e = implicit_update(h, a, true, h->mhd ? &aA : nullptr, &h->fars, &h->fA_ars)))
       return fail(PCF_E_HIP, std::string("solve launch: ") + hipGetErrorString(e));
   }
-  if ((r = nonlinear_into(h, h->u0[x1], NB))) return r;
+  if ((r = nonlinear_into(h, h->u0[x1], NB, nullptr, h->A[x1]))) return r;
   {
     pcf::SolveArgs a = base_args(h, x2);
     set_x(a, 0, h, cur, 1.0 - c1, 0.0);
@@ -616,8 +631,12 @@ int32_t ars222_step(pcf_handle *h) {
     set_n(a, 1, h, NB, dt * (1.0 - dl));
     a.nn = 2;
     a.gamma = dt * gm;
-    if ((e = pcf::launch_solve_factored(h->geo, h->sm, h->fars, a, 0, 0, h->stream)) ||
-        (e = pcf::launch_reconstruct(h->geo, h->Rt, h->ldr, a, h->stream)))
+    pcf::SolveArgs aA{};
+    if (h->mhd) {
+      const int xs[3] = {cur, x1, 0}, ns[3] = {NA, NB, 0};
+      aA = a_args_from(h, a, xs, 2, ns, 2, x2);
+    }
+    if ((e = implicit_update(h, a, true, h->mhd ? &aA : nullptr, &h->fars, &h->fA_ars)))
       return fail(PCF_E_HIP, std::string("solve launch: ") + hipGetErrorString(e));
   }
   rotate(h);  // x2 becomes the current slot
@@ -1303,7 +1322,9 @@ int32_t pcf_create(const pcf_config *cfg, pcf_handle **out) {
   if (h->mhd) {
     h->geo_eta = h->geo;
     h->geo_eta.nu = cfg->eta;
-    if ((r = build_factors_A(h, cfg->dt * 6.0 / 11.0))) return bail(r);
+    if ((r = build_factors_A(h, cfg->dt * 6.0 / 11.0, h->fA))) return bail(r);
+    if (cfg->time_integrator == PCF_TI_IMEXRK222 && (r = build_factors_A(h, cfg->dt * ARS_G, h->fA_ars)))
+      return bail(r);
     if ((r = h->alloc(&h->ddiagm, 5))) return bail(r);
     if (h->insul) {
       if ((r = h->upload(&h->dVC, h->ops.VC.a))) return bail(r);

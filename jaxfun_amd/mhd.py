@@ -18,7 +18,7 @@ from __future__ import annotations
 
 import ctypes as C
 import math
-from dataclasses import dataclass
+from dataclasses import dataclass, replace
 
 import numpy as np
 
@@ -68,8 +68,10 @@ class PlaneCouetteMHDHIP(PlaneCouetteFluctuationHIP):
             raise ValueError("Rm must be positive")
         if time_integrator not in {"SBDF3", "IMEXRK3", "IMEXRK222", "CNAB2", None}:
             raise ValueError("time_integrator must be one of {'IMEXRK222', 'IMEXRK3', 'CNAB2', 'SBDF3'}")
-        if time_integrator != "SBDF3":
-            raise NotImplementedError("the MHD step is built for SBDF3 only")
+        if time_integrator is None:  # the reference's default ARS stepper
+            time_integrator = "IMEXRK222"
+        if time_integrator not in {"SBDF3", "IMEXRK222"}:
+            raise NotImplementedError("the MHD step is built for SBDF3 and IMEXRK222")
         self.eta = float(U_wall) / self.Rm
         self.magnetic_amplitude = float(magnetic_amplitude)
         self.background_b = tuple(float(v) for v in background_b)
@@ -128,6 +130,17 @@ class PlaneCouetteMHDHIP(PlaneCouetteFluctuationHIP):
             for comp, arr in zip("XYZ", getattr(state, name)):
                 self._set(f"A{comp}" if not tag else f"{tag}{comp}", arr)
         self._loaded = None
+
+    def _result(self, state: MHDState) -> MHDState:
+        """The device state after steps from ``state``: SBDF3 with its own
+        rings; IMEXRK222 with every history field of ``state`` passed through
+        (``_step_imex_mhd`` returns ``replace(state, ...)``,
+        ``examples/pcf_mhd_jax.py:482-486``)."""
+        st = self.fetch()
+        if self.time_integrator == "IMEXRK222":
+            st = replace(state, flow=replace(state.flow, u=st.flow.u, g=st.flow.g), A=st.A)
+        self._loaded = st
+        return st
 
     def _a_rows(self, comp: int) -> int:
         """Rows of A component comp (Ax in TD; Ay, Az in TD for conducting walls)."""

@@ -60,7 +60,10 @@ class MHDOracle(PCFOracle):
     def __init__(self, N=(17, 16, 16), domain=((-1.0, 1.0), (0.0, 4.0 * math.pi), (0.0, 2.0 * math.pi)),
                  Re: float = 400.0, Rm: float | None = None, U_wall: float = 1.0, dt: float = 0.01,
                  padding_factor=(1.0, 1.5, 1.5), perturbation_amplitude: float = 0.1,
-                 magnetic_amplitude: float = 0.05, background_b=(0.0, 0.0, 0.0)) -> None:
+                 magnetic_amplitude: float = 0.05, background_b=(0.0, 0.0, 0.0),
+                 time_integrator: str = "SBDF3") -> None:
+        if time_integrator not in ("SBDF3", "IMEXRK222"):
+            raise ValueError("the MHD oracle steps SBDF3 and IMEXRK222")
         self.Rm = float(Re if Rm is None else Rm)
         # uniform background field added to B in physical space
         # (``_total_B_physical``, examples/pcf_mhd_mri_shearpy_jax.py:149-151)
@@ -68,7 +71,7 @@ class MHDOracle(PCFOracle):
         self.eta = float(U_wall) / self.Rm
         self.magnetic_amplitude = float(magnetic_amplitude)
         super().__init__(N=N, domain=domain, Re=Re, U_wall=U_wall, dt=dt, padding_factor=padding_factor,
-                         perturbation_amplitude=perturbation_amplitude)
+                         perturbation_amplitude=perturbation_amplitude, time_integrator=time_integrator)
         self._build_A_operators()
 
     # ------------------------------------------------------------ A operators
@@ -226,8 +229,44 @@ class MHDOracle(PCFOracle):
             prev, prevA = cur, curA
         return replace(s, flow=replace(s.flow, u=u_stage, g=g_stage), A=A_stage)
 
+    def step_ars222_mhd(self, s: MHDOracleState) -> MHDOracleState:
+        """``_step_imex_mhd`` (``pcf_mhd_jax.py:389-486``) with the IMEXRK222 tableau:
+        flow and A advance by the same two ARS stages, explicit linear terms
+        L x_1 and eta-Laplacian(A_1) in the second; every history field passes
+        through (``replace`` at :482-486)."""
+        from .kmm import ARS222_D, ARS222_G
+
+        dt = self.dt
+        a = ((0.0, 0.0, 0.0), (0.0, ARS222_G, 0.0), (0.0, 1.0 - ARS222_G, ARS222_G))
+        b = ((0.0, 0.0, 0.0), (ARS222_G, 0.0, 0.0), (ARS222_D, 1.0 - ARS222_D, 0.0))
+        base = self.mass_rows(self.primary(s.flow))
+        baseA = self.A_mass(s.A)
+        stage = MHDOracleState(flow=OracleState(u=s.flow.u, g=s.flow.g), A=s.A)
+        nl, nlA, lin, linA = [], [], [], []
+        for rk in range(2):
+            H, HA = self.mhd_convection(stage)
+            nl.append(self.nonlinear_rhs(H))
+            nlA.append(self.nonlinear_A(HA))
+            if rk > 0:
+                lin.append(self.linear_rows(self.primary(stage.flow)))
+                linA.append(self.A_lap(stage.A))
+            rhs, rhsA = list(base), list(baseA)
+            for j in range(rk + 1):
+                rhs = [r + dt * b[rk + 1][j] * n for r, n in zip(rhs, nl[j])]
+                rhsA = [r + dt * b[rk + 1][j] * n for r, n in zip(rhsA, nlA[j])]
+            for j in range(rk):
+                rhs = [r + dt * a[rk + 1][j + 1] * ell for r, ell in zip(rhs, lin[j])]
+                rhsA = [r + dt * a[rk + 1][j + 1] * ell for r, ell in zip(rhsA, linA[j])]
+            u0n, gn, vn, wn = self._solve(self.ars, *rhs)
+            stage = MHDOracleState(flow=OracleState(u=self.reconstruct(u0n, gn, vn, wn), g=gn),
+                                   A=self.A_solve_gamma(dt * ARS222_G, rhsA))
+        return replace(s, flow=replace(s.flow, u=stage.flow.u, g=stage.flow.g), A=stage.A)
+
     def step(self, s: MHDOracleState) -> MHDOracleState:
-        """``_step_sbdf3_mhd`` / ``_step_sbdf3_steady`` (``pcf_mhd_jax.py:563-715``)."""
+        """``_step_sbdf3_mhd`` / ``_step_sbdf3_steady`` (``pcf_mhd_jax.py:563-715``),
+        or the IMEXRK222 step."""
+        if self.time_integrator == "IMEXRK222":
+            return self.step_ars222_mhd(s)
         f = s.flow
         cur_p = self.primary(f)
         H, HA = self.mhd_convection(s)
@@ -289,12 +328,13 @@ class RotatingShearMHDOracle(MHDOracle):
     def __init__(self, N=(17, 16, 16), domain=((-1.0, 1.0), (0.0, 4.0 * math.pi), (0.0, 2.0 * math.pi)),
                  Re: float = 400.0, Rm: float | None = None, omega: float = 1.0, shear_rate: float = 1.0,
                  background_b=(0.0, 0.0, 0.025), dt: float = 0.01, padding_factor=(1.0, 1.5, 1.5),
-                 perturbation_amplitude: float = 0.05, magnetic_amplitude: float = 0.0) -> None:
+                 perturbation_amplitude: float = 0.05, magnetic_amplitude: float = 0.0,
+                 time_integrator: str = "SBDF3") -> None:
         self.omega = float(omega)
         self.shear_rate = float(shear_rate)
         super().__init__(N=N, domain=domain, Re=Re, Rm=Rm, U_wall=1.0, dt=dt, padding_factor=padding_factor,
                          perturbation_amplitude=perturbation_amplitude, magnetic_amplitude=magnetic_amplitude,
-                         background_b=background_b)
+                         background_b=background_b, time_integrator=time_integrator)
         # U_b = -S x, dU_b/dx = -S (shearpy :89-92); nu, eta keep U_wall = 1
         self.dUb = -self.shear_rate
 

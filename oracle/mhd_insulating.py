@@ -55,10 +55,12 @@ class InsulatingMHDOracle(RotatingShearMHDOracle):
     def __init__(self, N=(17, 16, 16), domain=((-1.0, 1.0), (0.0, 4.0 * math.pi), (0.0, 2.0 * math.pi)),
                  Re: float = 400.0, Rm: float | None = None, omega: float = 1.0, shear_rate: float = 1.0,
                  background_b=(0.0, 0.0, 0.025), dt: float = 0.01, padding_factor=(1.0, 1.5, 1.5),
-                 perturbation_amplitude: float = 0.05, magnetic_amplitude: float = 0.0) -> None:
+                 perturbation_amplitude: float = 0.05, magnetic_amplitude: float = 0.0,
+                 time_integrator: str = "SBDF3") -> None:
         super().__init__(N=N, domain=domain, Re=Re, Rm=Rm, omega=omega, shear_rate=shear_rate,
                          background_b=background_b, dt=dt, padding_factor=padding_factor,
-                         perturbation_amplitude=perturbation_amplitude, magnetic_amplitude=magnetic_amplitude)
+                         perturbation_amplitude=perturbation_amplitude, magnetic_amplitude=magnetic_amplitude,
+                         time_integrator=time_integrator)
         n = self.N0
         D = _cheb_D(n)
         self.CD = D

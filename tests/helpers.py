@@ -100,7 +100,8 @@ def mhd_oracle_for(solver):
     return MHDOracle(N=solver.N, domain=solver.domain, Re=solver.Re, Rm=solver.Rm, U_wall=solver.U_wall,
                      dt=solver.dt, padding_factor=solver.padding_factor,
                      perturbation_amplitude=solver.perturbation_amplitude,
-                     magnetic_amplitude=solver.magnetic_amplitude, background_b=solver.background_b)
+                     magnetic_amplitude=solver.magnetic_amplitude, background_b=solver.background_b,
+                     time_integrator=solver.time_integrator)
 
 
 def shear_oracle_for(solver):
@@ -110,7 +111,8 @@ def shear_oracle_for(solver):
                                   shear_rate=solver.shear_rate, background_b=solver.background_b, dt=solver.dt,
                                   padding_factor=solver.padding_factor,
                                   perturbation_amplitude=solver.perturbation_amplitude,
-                                  magnetic_amplitude=solver.magnetic_amplitude)
+                                  magnetic_amplitude=solver.magnetic_amplitude,
+                                  time_integrator=solver.time_integrator)
 
 
 def insulating_oracle_for(solver):
@@ -120,4 +122,5 @@ def insulating_oracle_for(solver):
                                shear_rate=solver.shear_rate, background_b=solver.background_b, dt=solver.dt,
                                padding_factor=solver.padding_factor,
                                perturbation_amplitude=solver.perturbation_amplitude,
-                               magnetic_amplitude=solver.magnetic_amplitude)
+                               magnetic_amplitude=solver.magnetic_amplitude,
+                               time_integrator=solver.time_integrator)

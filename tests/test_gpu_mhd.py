@@ -248,3 +248,46 @@ def test_shear_graph_replay_and_mixed_precision():
         rel = float(np.max(np.abs(x - y)) / np.max(np.abs(y)))
         print("shear mixed vs fp64", rel, flush=True)
         assert rel < 2e-6
+
+
+# ---------------------------------------------------------------- IMEXRK222 (ARS 2,2,2)
+@pytest.mark.parametrize("kind", ["conducting", "shear", "insulating", "insulating-config3"])
+def test_imexrk222_mhd_steps_match_oracle(kind):
+    """The ARS step of the three MHD classes against the oracles, which write
+    the reference's second stage literally (explicit L x_1 and eta-Laplacian
+    of A_1); histories pass through on both sides."""
+    from jaxfun_amd.ic import seeded_mhd_state
+    from tests.helpers import insulating_oracle_for, shear_oracle_for
+
+    if kind == "conducting":
+        solver = make((17, 16, 16), magnetic_amplitude=0.1, background_b=(0.0, 0.0, 0.1), time_integrator="IMEXRK222")
+        oracle = mhd_oracle_for(solver)
+        s = solver.initial_state()
+    elif kind == "shear":
+        solver = make_shear((17, 16, 16), time_integrator="IMEXRK222")
+        oracle = shear_oracle_for(solver)
+        s = solver.initial_state()
+    elif kind == "insulating":
+        solver = make_insulating((17, 16, 16), time_integrator="IMEXRK222")
+        oracle = insulating_oracle_for(solver)
+        s = seeded_mhd_state(solver, seed=3, energy_u=1e-3, energy_b=1e-3, K=4)
+    else:
+        solver = make_insulating((33, 32, 32), Re=1000.0, Rm=1000.0, omega=0.0, shear_rate=-1.0,
+                                 background_b=(0.0, 0.0, 0.1), dt=5e-3, time_integrator="IMEXRK222")
+        oracle = insulating_oracle_for(solver)
+        s = seeded_mhd_state(solver, seed=3, energy_u=1e-3, energy_b=1e-3, K=4)
+    so = to_mhd_oracle(s)
+    done = 0
+    for k in (1, 2, 5):
+        s = solver.solve(s, k - done)
+        so = oracle.solve(so, k - done)
+        done = k
+        err = mhd_max_rel_errors(s, so)
+        print(kind, k, max(err.values()), flush=True)
+        assert_parity(err, (kind, k))
+    d, do = solver.diagnostics(s), oracle.diagnostics(so)
+    for key in ("Epert", "Emag"):
+        assert math.isclose(d[key], do[key], rel_tol=1e-12), (key, d[key], do[key])
+    assert d["divB_L2"] < 1e-12
+    if kind.startswith("insulating"):
+        assert d["insulating_bc_residual"] < 1e-10

@@ -161,6 +161,47 @@ def test_A_decay_follows_scalar_recurrence():
         assert np.max(np.abs(s.A[1])) < 1e-15 * eps and np.max(np.abs(s.A[2])) < 1e-15 * eps
 
 
+def test_A_decay_follows_ars222_recurrence():
+    """The same resistive eigenmode under IMEXRK222: each step is
+    y1 = y / (1 - g h l), y' = (y + (1 - g) h l y1) / (1 - g h l) with
+    g = (2 - sqrt 2) / 2 (pins the A stages, the explicit eta-Laplacian of the
+    second stage and its coefficient)."""
+    from oracle.kmm import ARS222_G as G
+
+    dt, Rm = 0.01, 50.0
+    o = MHDOracle(N=(17, 8, 8), Re=400.0, Rm=Rm, dt=dt, perturbation_amplitude=0.0, magnetic_amplitude=0.0,
+                  time_integrator="IMEXRK222")
+    kz = o.K2[0, 0, 1]
+    lam, vec = sla.eig(o.eta * (o.D_C2 - kz**2 * o.D_C0), o.D_C0)
+    i = int(np.argmax(lam.real))
+    lam, f = float(lam[i].real), np.real(vec[:, i])
+    eps = 1e-7
+    Ax = np.zeros((o.ND, o.N[1], o.N[2]), complex)
+    Ax[:, 0, 1] = Ax[:, 0, -1] = 0.5 * eps * f
+    s0 = o.initial_state()
+    s = o.reset_mhd_history(MHDOracleState(flow=s0.flow, A=(Ax, 0 * Ax, 0 * Ax)))
+    y, h = 1.0, dt
+    for k in range(6):
+        y1 = y / (1 - G * h * lam)
+        y = (y + (1 - G) * h * lam * y1) / (1 - G * h * lam)
+        s = o.step(s)
+        err = np.max(np.abs(s.A[0][:, 0, 1] - y * 0.5 * eps * f)) / np.max(np.abs(0.5 * eps * f))
+        assert err < 1e-11, (k, err, y)
+
+
+def test_zero_field_ars222_reproduces_hydro_oracle():
+    kw = dict(N=(17, 16, 16), Re=400.0, dt=2e-3, perturbation_amplitude=0.05, time_integrator="IMEXRK222")
+    m = MHDOracle(magnetic_amplitude=0.0, **kw)
+    h = PCFOracle(**kw)
+    sm, sh = m.initial_state(), h.initial_state()
+    sm, sh = m.solve(sm, 4), h.solve(sh, 4)
+    for a, b in zip(list(sm.flow.u) + [sm.flow.g], list(sh.u) + [sh.g]):
+        assert np.array_equal(np.asarray(a), np.asarray(b))
+    assert all(np.max(np.abs(a)) == 0 for a in sm.A)
+    # the MHD ARS step keeps every history field (pcf_mhd_jax.py:482-486)
+    assert sm.flow.solution_old is not None and sm.nonlinear_A_old is not None
+
+
 @pytest.mark.slow
 @pytest.mark.parametrize("case", GOLDEN["cases"], ids=lambda c: "x".join(map(str, c["N"])))
 def test_published_mhd_golden(case):

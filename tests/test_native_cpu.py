@@ -153,7 +153,7 @@ def test_config_errors():
     lib = nat.load()
     bad = [cfg_for(8), cfg_for(17, ny=15), cfg_for(17, dt=0.0), cfg_for(17, nu=-1.0),
            cfg_for(17, precision=7), cfg_for(17, time_integrator=7), cfg_for(17, abi_version=99),
-           cfg_for(17, mhd=1, eta=1e-3, time_integrator=nat.TI_IMEXRK222)]
+           cfg_for(17, mhd=1, eta=1e-3, time_integrator=nat.TI_IMEXRK3)]
     for c in bad:
         assert lib.pcf_field_size(C.byref(c), 0) < 0
         assert lib.pcf_last_error()
