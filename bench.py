@@ -18,7 +18,7 @@ Also reported on the same line:
   * the insulating-wall MHD step (FP64 and mixed) at the same resolution;
   * FP64 steps/s on the reference's other published configurations (hydro
     97^3 and 128^3, conducting-wall MHD 65^3 and 97^3, the IMEXRK3 integrator
-    at 65^3) beside its figures.
+    and the conducting-wall MHD CNAB2 integrator at 65^3) beside its figures.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -375,10 +375,15 @@ def mhd_leg(steps: int, warmup: int) -> dict:
 
 # The reference's other published FP64 throughputs (RTX 5090 Laptop, Re = Rm =
 # 400, dt = 1e-3, analytic initial field of amplitude 0.02 / magnetic 0.005,
-# benchmark-results/FINAL_LOCAL_CHECKPOINT.md:58-59,64,79-81): ms per step.
+# benchmark-results/FINAL_LOCAL_CHECKPOINT.md:58-59,64,69,79-81 and
+# CHECKPOINT_PHASES_1_3.md:16-18): ms per step.  The two 65^3 integrator figures
+# are earlier checkpoints of the same solver; the MHD CNAB2 one used the
+# coefficient (gradient-form) nonlinear path, run here in the rotational form.
 PUBLISHED = [("hydro", (97, 96, 96), 92.94), ("hydro", (128, 128, 128), 219.72),
              ("mhd_conducting", (65, 64, 64), 45.790), ("mhd_conducting", (97, 96, 96), 163.28),
-             ("hydro_imexrk3", (65, 64, 64), 84.740)]  # :64, the rotational IMEXRK3 integrator alone
+             ("hydro_imexrk3", (65, 64, 64), 84.740),  # the rotational IMEXRK3 integrator alone
+             ("mhd_cnab2", (65, 64, 64), 49.73)]  # conducting MHD, CNAB2, batched solves
+STEPWISE = {"hydro_imexrk3": "IMEXRK3", "mhd_cnab2": "CNAB2"}  # launched step by step (pcf_step)
 
 
 def published_leg(steps: int = 120, warmup: int = 24) -> list:
@@ -388,15 +393,15 @@ def published_leg(steps: int = 120, warmup: int = 24) -> list:
 
     out = []
     for kind, N, ms in PUBLISHED:
+        integ = STEPWISE.get(kind, "SBDF3")
         if kind.startswith("hydro"):
-            integ = "IMEXRK3" if kind == "hydro_imexrk3" else "SBDF3"
             solver = PlaneCouetteFluctuationHIP(N=N, Re=400.0, dt=1e-3, perturbation_amplitude=0.02,
                                                 time_integrator=integ)
         else:
             solver = PlaneCouetteMHDHIP(N=N, Re=400.0, Rm=400.0, dt=1e-3, perturbation_amplitude=0.02,
-                                        magnetic_amplitude=0.005)
+                                        magnetic_amplitude=0.005, time_integrator=integ)
         solver.load(solver.initial_state())
-        if kind == "hydro_imexrk3":  # every step three IMEXRK3 stages, launched step by step
+        if kind in STEPWISE:
             solver.advance(warmup)
             solver.synchronize()
             t0 = time.perf_counter()

@@ -45,12 +45,18 @@ extern "C" {
  * IMEXRK222: every step an ARS(2,2,2) step (:1123-1201 with
  *   src/jaxfun/integrators/imex_rk.py:175-181).  The device history fields are
  *   scratch under IMEXRK3 and IMEXRK222 (the caller keeps the state's history).
- * The MHD step runs SBDF3 and IMEXRK222 (examples/pcf_mhd_jax.py:389-486, every
- * history field passed through); the reverse mode and the steady / graph entry
- * points are SBDF3 only. */
+ * CNAB2: Crank-Nicolson for the linear terms, variable-step Adams-Bashforth-2
+ *   for the nonlinear ones (_cnab2_flow_update, :1071-1121, with
+ *   src/jaxfun/integrators/cnab2.py:211-239): reads NL_OLD when have_old != 0
+ *   (extrapolation ratio dt / previous_dt), then NL_OLD holds the step's
+ *   nonlinear rows, have_old = 1 and previous_dt = dt.
+ * The MHD step runs SBDF3, IMEXRK222 (examples/pcf_mhd_jax.py:389-486, every
+ * history field passed through) and CNAB2 (:359-387, NL_A_OLD beside NL_OLD);
+ * the reverse mode and the steady / graph entry points are SBDF3 only. */
 #define PCF_TI_IMEXRK3 1
 #define PCF_TI_IMEXRK222 2
 #define PCF_TI_SBDF3 3
+#define PCF_TI_CNAB2 4
 
 /* nonlinear forms (reference: examples/channelflow_kmm.py:140-142) */
 #define PCF_NL_ROTATIONAL 1
@@ -175,10 +181,17 @@ int32_t pcf_set_history(pcf_handle *h, double history_steps, double have_old,
 int32_t pcf_get_history(pcf_handle *h, double *history_steps, double *have_old,
                         double *previous_dt);
 
-/* Advance nsteps steps: IMEXRK3 startup while history_steps < 2, then the
- * steady SBDF3 step.  Replaces KMM.solve / KMM.step
- * (examples/channelflow_kmm.py:937-1064,1123-1137,1231-1247). Asynchronous. */
+/* Advance nsteps steps of the configured integrator (SBDF3: IMEXRK3 startup
+ * while history_steps < 2, then the steady step).  Replaces KMM.solve /
+ * KMM.step (examples/channelflow_kmm.py:937-1064,1123-1137,1231-1247).
+ * Asynchronous. */
 int32_t pcf_step(pcf_handle *h, int32_t nsteps);
+
+/* A new step size for the following steps.  Replaces KMM.set_dt
+ * (examples/channelflow_kmm.py:1207-1222): SBDF3 refuses a changed dt
+ * (PCF_E_FIXED_DT); IMEXRK222 refactors its stage systems at the new dt;
+ * IMEXRK3 and CNAB2 factor per step and take it as is.  Synchronous. */
+int32_t pcf_set_dt(pcf_handle *h, double dt);
 
 /* Steady SBDF3 steps only (requires history_steps >= 2); the unit the
  * benchmark times.  Replaces _step_sbdf3_steady (channelflow_kmm.py:1016-1064). */

@@ -22,7 +22,8 @@ PREC_MIXED = 1
 TI_IMEXRK3 = 1
 TI_IMEXRK222 = 2
 TI_SBDF3 = 3
-TIME_INTEGRATORS = {"IMEXRK3": TI_IMEXRK3, "IMEXRK222": TI_IMEXRK222, "SBDF3": TI_SBDF3}
+TI_CNAB2 = 4
+TIME_INTEGRATORS = {"IMEXRK3": TI_IMEXRK3, "IMEXRK222": TI_IMEXRK222, "SBDF3": TI_SBDF3, "CNAB2": TI_CNAB2}
 NL_ROTATIONAL = 1
 
 # enum pcf_field
@@ -52,7 +53,7 @@ EXPORTS = (
     "pcf_abi_version", "pcf_last_error", "pcf_field_size", "pcf_setup_matrix",
     "pcf_create", "pcf_destroy", "pcf_set_stream", "pcf_synchronize",
     "pcf_set_field", "pcf_get_field", "pcf_set_history", "pcf_get_history",
-    "pcf_step", "pcf_step_steady", "pcf_step_steady_graph", "pcf_diagnostics", "pcf_mhd_diagnostics",
+    "pcf_step", "pcf_set_dt", "pcf_step_steady", "pcf_step_steady_graph", "pcf_diagnostics", "pcf_mhd_diagnostics",
     "pcf_kernels_per_step", "pcf_stage_count", "pcf_stage_name", "pcf_profile_steady",
     "pcf_set_cotangent", "pcf_vjp", "pcf_get_gradient",
 )
@@ -121,6 +122,7 @@ def load() -> C.CDLL:
         "pcf_set_history": ([h, C.c_double, C.c_double, C.c_double], C.c_int32),
         "pcf_get_history": ([h, dp, dp, dp], C.c_int32),
         "pcf_step": ([h, C.c_int32], C.c_int32),
+        "pcf_set_dt": ([h, C.c_double], C.c_int32),
         "pcf_step_steady": ([h, C.c_int32], C.c_int32),
         "pcf_step_steady_graph": ([h, C.c_int32, C.c_int32], C.c_int32),
         "pcf_diagnostics": ([h, dp], C.c_int32),
@@ -146,7 +148,7 @@ def load() -> C.CDLL:
 def check(status: int) -> int:
     if status < 0:
         msg = load().pcf_last_error().decode(errors="replace")
-        if status == -3:
+        if status in (-3, -4):  # option not built / SBDF3's fixed dt
             raise NotImplementedError(msg)
         if status == -1:
             raise ValueError(msg)

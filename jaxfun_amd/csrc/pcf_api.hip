@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cfloat>
 #include <cmath>
 #include <complex>
 #include <cstdio>
@@ -52,9 +53,9 @@ std::string validate(const pcf_config *c) {
   if (!(c->nu > 0)) return "nu must be positive";
   if (c->precision != PCF_PREC_FP64 && c->precision != PCF_PREC_MIXED) return "unknown precision";
   if (c->time_integrator != PCF_TI_SBDF3 && c->time_integrator != PCF_TI_IMEXRK3 &&
-      c->time_integrator != PCF_TI_IMEXRK222)
+      c->time_integrator != PCF_TI_IMEXRK222 && c->time_integrator != PCF_TI_CNAB2)
     return "unknown time integrator";
-  if (c->mhd && c->time_integrator == PCF_TI_IMEXRK3) return "the MHD step is built for SBDF3 and IMEXRK222";
+  if (c->mhd && c->time_integrator == PCF_TI_IMEXRK3) return "the MHD step is built for SBDF3, IMEXRK222 and CNAB2";
   if (c->nonlinear_form != PCF_NL_ROTATIONAL) return "only the rotational nonlinear form is built";
   if (c->mhd != 0 && c->mhd != 1) return "mhd must be 0 or 1";
   if (c->mhd && !(c->eta > 0)) return "eta must be positive";
@@ -640,6 +641,47 @@ int32_t ars222_step(pcf_handle *h) {
       return fail(PCF_E_HIP, std::string("solve launch: ") + hipGetErrorString(e));
   }
   rotate(h);  // x2 becomes the current slot
+  return PCF_OK;
+}
+
+// One CNAB2 step (channelflow_kmm.py:1071-1121; under MHD with the
+// vector-potential update, examples/pcf_mhd_jax.py:359-387):
+//   rhs = M x + (dt/2) L x + dt E,  (M - (dt/2) L) x_new = rhs,
+// E = (1 + r/2) N - (r/2) N_old with r = dt / previous_dt when have_old != 0
+// (src/jaxfun/integrators/cnab2.py:211-239), else E = N.  The on-the-fly
+// factorisation of the general solve makes any dt valid.  N lands in the
+// current nonlinear slot, which the ring shift makes NL_OLD.
+int32_t cnab2_step(pcf_handle *h) {
+  const double dt = h->cfg.dt;
+  const int cur = h->so[0], out = h->so[2], NA = h->no[0], NO = h->no[1];
+  const bool have = h->have_old != 0.0;
+  if (have && !(h->previous_dt > 0.0))
+    return fail(PCF_E_INVALID, "CNAB2 history: have_old set with previous_dt <= 0");
+  int32_t r = nonlinear_into(h, h->u0[cur], NA, nullptr, h->A[cur]);
+  if (r) return r;
+  pcf::SolveArgs a = base_args(h, out);
+  set_x(a, 0, h, cur, 1.0, 0.5 * dt);
+  a.nx = 1;
+  if (have) {
+    const double q = dt / h->previous_dt;
+    set_n(a, 0, h, NA, dt * (1.0 + 0.5 * q));
+    set_n(a, 1, h, NO, -dt * 0.5 * q);
+    a.nn = 2;
+  } else {
+    set_n(a, 0, h, NA, dt);
+    a.nn = 1;
+  }
+  a.gamma = 0.5 * dt;
+  pcf::SolveArgs aA{};
+  if (h->mhd) {
+    const int xs[3] = {cur, 0, 0}, ns[3] = {NA, NO, 0};
+    aA = a_args_from(h, a, xs, 1, ns, a.nn, out);
+  }
+  hipError_t e = implicit_update(h, a, false, h->mhd ? &aA : nullptr);
+  if (e != hipSuccess) return fail(PCF_E_HIP, std::string("solve launch: ") + hipGetErrorString(e));
+  rotate(h);
+  h->have_old = 1.0;
+  h->previous_dt = dt;
   return PCF_OK;
 }
 
@@ -1467,6 +1509,8 @@ int32_t pcf_step(pcf_handle *h, int32_t nsteps) {
     int32_t r;
     if (ti == PCF_TI_IMEXRK222) {
       r = ars222_step(h);
+    } else if (ti == PCF_TI_CNAB2) {
+      r = cnab2_step(h);
     } else if (ti == PCF_TI_IMEXRK3) {
       // the IMEXRK3 stages of the start-up step; the history it writes is scratch
       const double hs = h->history_steps, ho = h->have_old, pd = h->previous_dt;
@@ -1479,6 +1523,25 @@ int32_t pcf_step(pcf_handle *h, int32_t nsteps) {
     }
     if (r) return r;
   }
+  return PCF_OK;
+}
+
+int32_t pcf_set_dt(pcf_handle *h, double dt) {
+  if (!h) return fail(PCF_E_INVALID, "null handle");
+  if (!(dt > 0.0) || !std::isfinite(dt)) return fail(PCF_E_INVALID, "dt must be positive and finite");
+  const int ti = h->cfg.time_integrator;
+  if (std::fabs(dt - h->cfg.dt) <= 8.0 * DBL_EPSILON * std::fabs(h->cfg.dt)) return PCF_OK;
+  if (ti == PCF_TI_SBDF3)
+    return fail(PCF_E_FIXED_DT, "SBDF3 runs a fixed dt; start a fresh solver/history for a different timestep");
+  PCF_HIP(hipSetDevice(h->device));
+  if (ti == PCF_TI_IMEXRK222) {
+    // the stage systems M - g dt L in place (storage sized by the geometry alone)
+    hipError_t e = pcf::launch_factor(h->geo, h->sm, dt * ARS_G, h->fars, false, h->stream);
+    if (e == hipSuccess && h->mhd) e = pcf::launch_factor(h->geo_eta, h->sm, dt * ARS_G, h->fA_ars, true, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) return fail(PCF_E_HIP, std::string("factorisation: ") + hipGetErrorString(e));
+  }
+  h->cfg.dt = dt;
   return PCF_OK;
 }
 

@@ -70,8 +70,8 @@ class PlaneCouetteMHDHIP(PlaneCouetteFluctuationHIP):
             raise ValueError("time_integrator must be one of {'IMEXRK222', 'IMEXRK3', 'CNAB2', 'SBDF3'}")
         if time_integrator is None:  # the reference's default ARS stepper
             time_integrator = "IMEXRK222"
-        if time_integrator not in {"SBDF3", "IMEXRK222"}:
-            raise NotImplementedError("the MHD step is built for SBDF3 and IMEXRK222")
+        if time_integrator not in {"SBDF3", "IMEXRK222", "CNAB2"}:
+            raise NotImplementedError("the MHD step is built for SBDF3, IMEXRK222 and CNAB2")
         self.eta = float(U_wall) / self.Rm
         self.magnetic_amplitude = float(magnetic_amplitude)
         self.background_b = tuple(float(v) for v in background_b)
@@ -122,9 +122,25 @@ class PlaneCouetteMHDHIP(PlaneCouetteFluctuationHIP):
         return self._new_state(flow, A)
 
     # ------------------------------------------------------------ device I/O
+    def _device_mhd_history(self, state: MHDState) -> MHDState:
+        """``_ensure_mhd_history`` (``pcf_mhd_jax.py:217-248``): SBDF3 restarts
+        flow and A histories together unless every level is present; CNAB2
+        starts a missing ``nonlinear_A_old`` at zero (the flow rule of
+        ``_device_history`` applies to ``flow``).  Levels the step does not
+        read travel as zeros."""
+        flow = state.flow
+        if self.time_integrator == "SBDF3":
+            flow_ok = all(getattr(flow, k) is not None
+                          for k in ("nonlinear_old", "nonlinear_older", "solution_old", "solution_older"))
+            if not flow_ok or any(getattr(state, name) is None for name, _ in _A_GROUPS[1:]):
+                return self._new_state(flow, state.A)
+            return state
+        z = self._zero_A()
+        return replace(state, flow=self._device_history(flow),
+                       **{name: z for name, _ in _A_GROUPS[1:] if getattr(state, name) is None})
+
     def load(self, state: MHDState) -> None:
-        if any(getattr(state, name) is None for name, _ in _A_GROUPS[1:]) or state.flow.nonlinear_old is None:
-            state = self._new_state(state.flow, state.A)
+        state = self._device_mhd_history(state)
         super().load(state.flow)
         for name, tag in _A_GROUPS:
             for comp, arr in zip("XYZ", getattr(state, name)):
@@ -135,10 +151,16 @@ class PlaneCouetteMHDHIP(PlaneCouetteFluctuationHIP):
         """The device state after steps from ``state``: SBDF3 with its own
         rings; IMEXRK222 with every history field of ``state`` passed through
         (``_step_imex_mhd`` returns ``replace(state, ...)``,
-        ``examples/pcf_mhd_jax.py:482-486``)."""
+        ``examples/pcf_mhd_jax.py:482-486``); CNAB2 with the step's nonlinear
+        rows of flow and A as the only levels (``:383-387``)."""
         st = self.fetch()
         if self.time_integrator == "IMEXRK222":
             st = replace(state, flow=replace(state.flow, u=st.flow.u, g=st.flow.g), A=st.A)
+        elif self.time_integrator == "CNAB2":
+            f = st.flow
+            st = MHDState(flow=KMMState(u=f.u, g=f.g, nonlinear_old=f.nonlinear_old, have_old=f.have_old,
+                                        previous_dt=f.previous_dt),
+                          A=st.A, nonlinear_A_old=st.nonlinear_A_old)
         self._loaded = st
         return st
 

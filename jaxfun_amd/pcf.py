@@ -52,8 +52,8 @@ class PlaneCouetteFluctuationHIP:
 
     Constructor follows ``PlaneCouetteFluctuationJax.__init__``
     (``examples/pcf_fluctuations_jax.py:35-75``).  Built options:
-    Chebyshev family; SBDF3 with IMEXRK3 startup, IMEXRK3 alone or IMEXRK222
-    (``time_integrator=None`` selects IMEXRK222 as in the reference); rotational
+    Chebyshev family; SBDF3 with IMEXRK3 startup, IMEXRK3 alone, IMEXRK222 or
+    CNAB2 (``time_integrator=None`` selects IMEXRK222 as in the reference); rotational
     nonlinear form; optimized coefficient path.  Other valid reference values raise
     ``NotImplementedError``; invalid values raise ``ValueError`` as in
     ``examples/channelflow_kmm.py:140-187``.
@@ -95,8 +95,6 @@ class PlaneCouetteFluctuationHIP:
             raise NotImplementedError("only the Chebyshev family is built")
         if time_integrator is None:  # the reference's default ARS stepper (channelflow_kmm.py:160-172)
             time_integrator = "IMEXRK222"
-        if time_integrator == "CNAB2":
-            raise NotImplementedError("CNAB2 is not built (SBDF3, IMEXRK3 and IMEXRK222 are)")
         if nonlinear_form != "rotational":
             raise NotImplementedError("only the rotational nonlinear form is built")
         if coefficient_path != "optimized":
@@ -272,10 +270,25 @@ class PlaneCouetteFluctuationHIP:
                                           out.ctypes.data_as(C.POINTER(C.c_double))))
         return out
 
+    def _device_history(self, state: KMMState) -> KMMState:
+        """The history the step reads, as ``_ensure_flow_history``
+        (``channelflow_kmm.py:612-635``) leaves it: SBDF3 restarts unless all
+        four levels are present; CNAB2 without ``nonlinear_old`` starts with
+        zero rows, ``have_old = 0``, ``previous_dt = 0``.  Levels the step does
+        not read travel as zeros."""
+        groups = ("nonlinear_old", "nonlinear_older", "solution_old", "solution_older")
+        if self.time_integrator == "SBDF3":
+            if any(getattr(state, k) is None for k in groups):
+                return self._with_zero_history(state.u, state.g)
+            return state
+        z = self._with_zero_history(state.u, state.g)
+        if self.time_integrator == "CNAB2" and state.nonlinear_old is None:
+            state = replace(state, have_old=0.0, previous_dt=0.0)
+        return replace(state, **{k: getattr(z, k) for k in groups if getattr(state, k) is None})
+
     def load(self, state: KMMState) -> None:
         """Copy a state (with its history) to the device."""
-        if state.nonlinear_old is None or state.solution_old is None:
-            state = self._with_zero_history(state.u, state.g)
+        state = self._device_history(state)
         self._set("U0", state.u[0])
         self._set("U1", state.u[1])
         self._set("U2", state.u[2])
@@ -329,11 +342,18 @@ class PlaneCouetteFluctuationHIP:
 
     # ------------------------------------------------------------ reference API
     def step(self, state: KMMState, dt=None, factor_args=None) -> KMMState:
-        """One step (``channelflow_kmm.py:1123-1137``); state in, new state out."""
+        """One step (``channelflow_kmm.py:1123-1137``); state in, new state out.
+
+        A ``dt`` other than the solver's runs this one step at ``dt`` with its
+        implicit systems factorised at ``dt`` (the reference pairs a runtime dt
+        with ``factor_args`` built for it); SBDF3 refuses it."""
         if dt is not None and not math.isclose(float(dt), self.dt, rel_tol=8 * np.finfo(float).eps):
-            raise NotImplementedError(
-                "this build runs a fixed dt per solver; start a fresh solver/history "
-                "for a different timestep")
+            keep = self.dt
+            self.set_dt(float(dt))
+            try:
+                return self.step(state)
+            finally:
+                self.set_dt(keep)
         self._ensure_loaded(state)
         self.advance(1)
         return self._result(state)
@@ -349,22 +369,28 @@ class PlaneCouetteFluctuationHIP:
         each integrator leaves: SBDF3 its own rings; IMEXRK3 those of ``state``
         unchanged (``replace``, ``channelflow_kmm.py:935``); IMEXRK222 only
         ``nonlinear_old``, ``have_old`` and ``previous_dt`` of ``state``
-        (``:1194-1200``), the device's history slots being scratch there."""
+        (``:1194-1200``), the device's history slots being scratch there;
+        CNAB2 the step's nonlinear rows as ``nonlinear_old`` with ``have_old``
+        and ``previous_dt`` (``:1112-1117``)."""
         st = self.fetch()
         if self.time_integrator == "IMEXRK3":
             st = replace(state, u=st.u, g=st.g)
         elif self.time_integrator == "IMEXRK222":
             st = KMMState(u=st.u, g=st.g, nonlinear_old=state.nonlinear_old, have_old=state.have_old,
                           previous_dt=state.previous_dt)
+        elif self.time_integrator == "CNAB2":
+            st = KMMState(u=st.u, g=st.g, nonlinear_old=st.nonlinear_old, have_old=st.have_old,
+                          previous_dt=st.previous_dt)
         self._loaded = st
         return st
 
     def set_dt(self, dt: float) -> None:
-        """``KMM.set_dt`` (``channelflow_kmm.py:1207-1222``): SBDF3 is fixed-step."""
-        if not math.isclose(float(dt), self.dt, rel_tol=8.0 * np.finfo(float).eps, abs_tol=0.0):
-            raise NotImplementedError(
-                "this build runs a fixed dt per solver; start a fresh solver/history "
-                "for a different timestep")
+        """``KMM.set_dt`` (``channelflow_kmm.py:1207-1222``): SBDF3 is fixed-step
+        (``NotImplementedError``); the other integrators take the new dt for
+        the following steps."""
+        nat.check(self._lib.pcf_set_dt(self._h, float(dt)))
+        if self.time_integrator != "SBDF3":
+            self.dt = float(dt)
 
     def diagnostics(self, state: KMMState | None = None) -> dict:
         """``PlaneCouetteFluctuationJax.diagnostics`` (``pcf_fluctuations_jax.py:131-161``)."""

@@ -218,7 +218,7 @@ SBDF3_GAMMA = 6.0 / 11.0  # sbdf3.py:11-12 IMPLICIT_SCALE
 # a[2][1] = 1 - g, b[1][0] = g, b[2][0] = d, b[2][1] = 1 - d
 ARS222_G = (2.0 - 2.0**0.5) / 2.0
 ARS222_D = 1.0 - 1.0 / (2.0 * ARS222_G)
-INTEGRATORS = ("SBDF3", "IMEXRK3", "IMEXRK222")
+INTEGRATORS = ("SBDF3", "IMEXRK3", "IMEXRK222", "CNAB2")
 
 
 @dataclass
@@ -294,6 +294,16 @@ def _lu_solve(fac: tuple[np.ndarray, int], b: np.ndarray) -> np.ndarray:
     r = bb.shape[-1]
     out = y[..., :r] + 1j * y[..., r:]
     return out[..., 0] if squeeze else out
+
+
+def variable_ab2_extrapolate(current, previous, have_old, dt, previous_dt):
+    """``variable_ab2_extrapolate`` (``src/jaxfun/integrators/cnab2.py:211-239``):
+    (1 + r/2) current - (r/2) previous with r = dt / previous_dt when a previous
+    level exists (``have_old != 0``), else current."""
+    if previous is None or float(have_old) == 0.0:
+        return tuple(current)
+    r = float(dt) / float(previous_dt)
+    return tuple((1.0 + 0.5 * r) * c - 0.5 * r * q for c, q in zip(current, previous))
 
 
 class PCFOracle:
@@ -661,14 +671,42 @@ class PCFOracle:
         return OracleState(u=stage.u, g=stage.g, nonlinear_old=s.nonlinear_old, have_old=s.have_old,
                            previous_dt=s.previous_dt)
 
+    def set_dt(self, dt: float) -> None:
+        """``KMM.set_dt`` (``channelflow_kmm.py:1207-1222``): SBDF3 keeps its dt;
+        the other integrators take the new one (factors at the new gammas)."""
+        if self.time_integrator == "SBDF3" and dt != self.dt:
+            raise NotImplementedError("SBDF3 runs a fixed dt")
+        self.dt = float(dt)
+        self.startup = tuple(self._fac((IMEXRK3_A[rk] + IMEXRK3_B[rk]) * self.dt / 2.0) for rk in range(3))
+        if self.time_integrator == "IMEXRK222":
+            self.ars = self._fac(self.dt * ARS222_G)
+
+    def step_cnab2(self, s: OracleState) -> OracleState:
+        """``_step_cnab2`` / ``_cnab2_flow_update`` (``channelflow_kmm.py:1071-1121``):
+        Crank-Nicolson for L, variable-step Adams-Bashforth-2
+        (``variable_ab2_extrapolate``, ``src/jaxfun/integrators/cnab2.py:211-239``)
+        for N; the new state keeps only ``nonlinear_old``, ``have_old``,
+        ``previous_dt``."""
+        dt = self.dt
+        p = self.primary(s)
+        cur = self.nonlinear_rhs(self.convection(s))
+        ext = variable_ab2_extrapolate(cur, s.nonlinear_old, s.have_old, dt, s.previous_dt)
+        m, ell = self.mass_rows(p), self.linear_rows(p)
+        rhs = [a + 0.5 * dt * b + dt * e for a, b, e in zip(m, ell, ext)]
+        u0n, gn, vn, wn = self._solve(self._fac(0.5 * dt), *rhs)
+        return OracleState(u=self.reconstruct(u0n, gn, vn, wn), g=gn, nonlinear_old=cur, have_old=1.0,
+                           previous_dt=dt)
+
     def step(self, s: OracleState) -> OracleState:
         """One step of the configured integrator: ``_step_sbdf3``
         (``channelflow_kmm.py:937-1064``, startup then steady), ``_step_imexrk3``
-        (``:872-935``, history passed through) or the IMEXRK222 ARS step."""
+        (``:872-935``, history passed through), the IMEXRK222 ARS step or CNAB2."""
         if self.time_integrator == "IMEXRK3":
             return self.step_imexrk3(s)
         if self.time_integrator == "IMEXRK222":
             return self.step_ars222(s)
+        if self.time_integrator == "CNAB2":
+            return self.step_cnab2(s)
         cur_p = self.primary(s)
         cur_n = self.nonlinear_rhs(self.convection(s))
         if s.history_steps < 2:

@@ -62,8 +62,8 @@ class MHDOracle(PCFOracle):
                  padding_factor=(1.0, 1.5, 1.5), perturbation_amplitude: float = 0.1,
                  magnetic_amplitude: float = 0.05, background_b=(0.0, 0.0, 0.0),
                  time_integrator: str = "SBDF3") -> None:
-        if time_integrator not in ("SBDF3", "IMEXRK222"):
-            raise ValueError("the MHD oracle steps SBDF3 and IMEXRK222")
+        if time_integrator not in ("SBDF3", "IMEXRK222", "CNAB2"):
+            raise ValueError("the MHD oracle steps SBDF3, IMEXRK222 and CNAB2")
         self.Rm = float(Re if Rm is None else Rm)
         # uniform background field added to B in physical space
         # (``_total_B_physical``, examples/pcf_mhd_mri_shearpy_jax.py:149-151)
@@ -262,11 +262,37 @@ class MHDOracle(PCFOracle):
                                    A=self.A_solve_gamma(dt * ARS222_G, rhsA))
         return replace(s, flow=replace(s.flow, u=stage.flow.u, g=stage.flow.g), A=stage.A)
 
+    def step_cnab2_mhd(self, s: MHDOracleState) -> MHDOracleState:
+        """``_step_cnab2_mhd`` (``pcf_mhd_jax.py:359-387``): the flow CNAB2 update
+        and the same Crank-Nicolson / variable AB2 update of A (its nonlinear
+        level extrapolated with the flow's ``have_old`` and ``previous_dt``);
+        the new state keeps the step's nonlinear rows of flow and A only."""
+        from .kmm import variable_ab2_extrapolate
+
+        dt = self.dt
+        f = s.flow
+        H, HA = self.mhd_convection(s)
+        cur, curA = self.nonlinear_rhs(H), self.nonlinear_A(HA)
+        oldA = s.nonlinear_A_old if s.nonlinear_A_old is not None else self.zero_A()
+        ext = variable_ab2_extrapolate(cur, f.nonlinear_old, f.have_old, dt, f.previous_dt)
+        extA = variable_ab2_extrapolate(curA, oldA if f.nonlinear_old is not None else None, f.have_old, dt,
+                                        f.previous_dt)
+        p = self.primary(f)
+        rhs = [a + 0.5 * dt * b + dt * e for a, b, e in zip(self.mass_rows(p), self.linear_rows(p), ext)]
+        u0n, gn, vn, wn = self._solve(self._fac(0.5 * dt), *rhs)
+        mA, lA = self.A_mass(s.A), self.A_lap(s.A)
+        rhsA = [mA[i] + 0.5 * dt * lA[i] + dt * extA[i] for i in range(3)]
+        flow = OracleState(u=self.reconstruct(u0n, gn, vn, wn), g=gn, nonlinear_old=cur, have_old=1.0,
+                           previous_dt=dt)
+        return MHDOracleState(flow=flow, A=self.A_solve_gamma(0.5 * dt, rhsA), nonlinear_A_old=curA)
+
     def step(self, s: MHDOracleState) -> MHDOracleState:
         """``_step_sbdf3_mhd`` / ``_step_sbdf3_steady`` (``pcf_mhd_jax.py:563-715``),
-        or the IMEXRK222 step."""
+        or the IMEXRK222 / CNAB2 step."""
         if self.time_integrator == "IMEXRK222":
             return self.step_ars222_mhd(s)
+        if self.time_integrator == "CNAB2":
+            return self.step_cnab2_mhd(s)
         f = s.flow
         cur_p = self.primary(f)
         H, HA = self.mhd_convection(s)

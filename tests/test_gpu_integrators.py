@@ -84,8 +84,9 @@ def test_imexrk222_mixed_close_to_fp64():
         assert rel < 2e-6
 
 
-def test_sbdf3_only_entry_points_reject_other_integrators():
-    solver = make((17, 16, 16), "IMEXRK222")
+@pytest.mark.parametrize("integ", ["IMEXRK222", "IMEXRK3", "CNAB2"])
+def test_sbdf3_only_entry_points_reject_other_integrators(integ):
+    solver = make((17, 16, 16), integ)
     s = solver.initial_state()
     with pytest.raises(NotImplementedError):
         solver.value_and_grad(s, 2)
@@ -93,4 +94,4 @@ def test_sbdf3_only_entry_points_reject_other_integrators():
     with pytest.raises(NotImplementedError):
         solver.advance_steady(12, graph_block=12)
     with pytest.raises(NotImplementedError):
-        make((17, 16, 16), "CNAB2")
+        solver.advance_steady(3)

@@ -189,6 +189,49 @@ def test_A_decay_follows_ars222_recurrence():
         assert err < 1e-11, (k, err, y)
 
 
+def test_A_decay_follows_crank_nicolson_recurrence_with_variable_dt():
+    """The same resistive eigenmode under CNAB2 with dt changed between steps:
+    the A nonlinear rows vanish, so each step is y' = (1 + h l / 2) y / (1 - h l / 2)
+    at that step's h (pins the dt/2 weight of the explicit eta-Laplacian and of
+    the implicit systems)."""
+    dt, Rm = 0.01, 50.0
+    o = MHDOracle(N=(17, 8, 8), Re=400.0, Rm=Rm, dt=dt, perturbation_amplitude=0.0, magnetic_amplitude=0.0,
+                  time_integrator="CNAB2")
+    kz = o.K2[0, 0, 1]
+    lam, vec = sla.eig(o.eta * (o.D_C2 - kz**2 * o.D_C0), o.D_C0)
+    i = int(np.argmax(lam.real))
+    lam, f = float(lam[i].real), np.real(vec[:, i])
+    eps = 1e-7
+    Ax = np.zeros((o.ND, o.N[1], o.N[2]), complex)
+    Ax[:, 0, 1] = Ax[:, 0, -1] = 0.5 * eps * f
+    s0 = o.initial_state()
+    s = MHDOracleState(flow=s0.flow, A=(Ax, 0 * Ax, 0 * Ax))
+    y = 1.0
+    for k, h in enumerate((dt, dt, 0.5 * dt, 2.0 * dt, 0.7 * dt)):
+        o.set_dt(h)
+        y = (1 + 0.5 * h * lam) * y / (1 - 0.5 * h * lam)
+        s = o.step(s)
+        err = np.max(np.abs(s.A[0][:, 0, 1] - y * 0.5 * eps * f)) / np.max(np.abs(0.5 * eps * f))
+        assert err < 1e-11, (k, err, y)
+        assert s.flow.previous_dt == h and s.flow.have_old == 1.0
+
+
+def test_zero_field_cnab2_reproduces_hydro_oracle():
+    kw = dict(N=(17, 16, 16), Re=400.0, dt=2e-3, perturbation_amplitude=0.05, time_integrator="CNAB2")
+    m = MHDOracle(magnetic_amplitude=0.0, **kw)
+    h = PCFOracle(**kw)
+    sm, sh = m.initial_state(), h.initial_state()
+    sm = MHDOracleState(flow=sh, A=sm.A)  # no history: the CNAB2 bootstrap
+    sm, sh = m.solve(sm, 4), h.solve(sh, 4)
+    for a, b in zip(list(sm.flow.u) + [sm.flow.g] + list(sm.flow.nonlinear_old),
+                    list(sh.u) + [sh.g] + list(sh.nonlinear_old)):
+        assert np.array_equal(np.asarray(a), np.asarray(b))
+    assert all(np.max(np.abs(a)) == 0 for a in sm.A + sm.nonlinear_A_old)
+    # the CNAB2 MHD state keeps the step's nonlinear rows only (pcf_mhd_jax.py:383-387)
+    assert sm.flow.solution_old is None and sm.A_old is None and sm.nonlinear_A_older is None
+    assert sm.flow.have_old == 1.0 and sm.flow.previous_dt == 2e-3
+
+
 def test_zero_field_ars222_reproduces_hydro_oracle():
     kw = dict(N=(17, 16, 16), Re=400.0, dt=2e-3, perturbation_amplitude=0.05, time_integrator="IMEXRK222")
     m = MHDOracle(magnetic_amplitude=0.0, **kw)

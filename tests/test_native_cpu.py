@@ -189,7 +189,7 @@ def test_python_boundary_errors_match_reference():
     with pytest.raises(NotImplementedError):
         S(nonlinear_form="gradient")
     with pytest.raises(NotImplementedError):
-        S(time_integrator="CNAB2")
+        S(coefficient_path="transform")
 
 
 class _HostSolver:

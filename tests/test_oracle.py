@@ -227,6 +227,36 @@ def test_imexrk3_alone_is_second_order():
     assert all(1.8 <= q <= 2.2 for q in orders), (errs, orders)
 
 
+def test_cnab2_is_second_order():
+    errs, orders = _self_convergence_orders("CNAB2")
+    assert all(1.8 <= q <= 2.2 for q in orders), (errs, orders)
+
+
+def test_cnab2_variable_dt_stays_second_order():
+    """dt alternating between h and h/2 (ratio 1/2 and 2 in the variable-step
+    AB2): the error against a fine SBDF3 run falls by about 4 per halving of h."""
+    T = 0.03
+    ref_o = PCFOracle(N=(9, 4, 4), Re=200.0, dt=T / 300, padding_factor=(1.0, 1.0, 1.0),
+                      perturbation_amplitude=0.01)
+    ref = ref_o.solve(_consistent(ref_o, ref_o.initial_state()), 300)
+    vref = np.concatenate([x.ravel() for x in list(ref.u) + [ref.g]])
+    errs = []
+    for n in (10, 20, 40):  # n pairs (h, h/2) with 1.5 h n = T
+        h = T / (1.5 * n)
+        o = PCFOracle(N=(9, 4, 4), Re=200.0, dt=h, padding_factor=(1.0, 1.0, 1.0), perturbation_amplitude=0.01,
+                      time_integrator="CNAB2")
+        s = _consistent(o, o.initial_state())
+        for _ in range(n):
+            o.set_dt(h)
+            s = o.step(s)
+            o.set_dt(h / 2)
+            s = o.step(s)
+        v = np.concatenate([x.ravel() for x in list(s.u) + [s.g]])
+        errs.append(float(np.linalg.norm(v - vref)))
+    orders = [math.log2(errs[i] / errs[i + 1]) for i in range(2)]
+    assert all(1.8 <= q <= 2.3 for q in orders), (errs, orders)
+
+
 def test_imexrk222_converges_to_sbdf3_solution():
     """Two independent integrators agree as dt -> 0: ARS(2,2,2) at dt = T/640
     against SBDF3 at dt = T/80 (errors ~ dt^2 and dt^3 respectively)."""
