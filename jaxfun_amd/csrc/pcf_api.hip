@@ -41,6 +41,8 @@ int32_t fail(int32_t code, const std::string &msg) {
 const double IMEX_A[3] = {8.0 / 15.0, 5.0 / 12.0, 3.0 / 4.0};   // imex_rk.py:103
 const double IMEX_B[3] = {0.0, -17.0 / 60.0, -5.0 / 12.0};      // imex_rk.py:104
 const double ARS_G = (2.0 - std::sqrt(2.0)) / 2.0;                // imex_rk.py:176
+// gamma of IMEXRK3 stage rk: the implicit weight (a + b) dt / 2 (channelflow_kmm.py:872-935)
+double stage_gamma(int rk, double dt) { return (IMEX_A[rk] + IMEX_B[rk]) * dt / 2.0; }
 
 std::string validate(const pcf_config *c) {
   if (!c) return "config is NULL";
@@ -126,6 +128,8 @@ struct pcf_handle {
   pcf::Geo geo_eta{};         // geo with nu = eta: the A systems are the g systems at eta
   pcf::FactorSet fA{};        // steady LU factors of the A systems (Dirichlet blocks)
   pcf::FactorSet fA_ars{};    // ARS(2,2,2) LU factors of the A systems
+  pcf::FactorSet fA_stage[3]{};  // IMEXRK3 stage factors of the A systems
+  pcf::FactorSet fA_cn{};     // CNAB2 factors of the A systems
   double *ddiagm = nullptr;
   int so[3] = {0, 1, 2};     // solution ring: cur, old, older
   int no[4] = {0, 1, 2, 3};  // nonlinear ring: free0, old, older, free1
@@ -136,9 +140,10 @@ struct pcf_handle {
   pcf::StructMats sm{};
   pcf::FactorSet fsteady{};   // LU factors of the steady SBDF3 systems
   pcf::FactorSet fars{};      // LU factors of M - g dt L for the ARS(2,2,2) stages
+  pcf::FactorSet fstage[3]{}; // LU factors at the three IMEXRK3 stage gammas
+  pcf::FactorSet fcn{};       // LU factors of M - (dt/2) L (CNAB2)
   double *Rt = nullptr;       // parity-compressed reconstruction rows, [nd][ldr]
   int ldr = 0;
-  double *scratch = nullptr;  // per-row data of the structured solves
   double *dVB = nullptr, *dVD = nullptr, *dVB1 = nullptr, *dVD1 = nullptr, *dfejer = nullptr,
          *dxn = nullptr, *ddiag = nullptr;
   std::map<int, hipGraphExec_t> graphs;
@@ -302,7 +307,6 @@ int32_t build_struct_mats(pcf_handle *h) {
     for (int k = 0; j0 + 2 * k < o.nb; ++k) Rc[(size_t)i * h->ldr + k] = o.R(i, j0 + 2 * k);
   }
   if ((r = h->upload(&h->Rt, Rc))) return r;
-  if ((r = h->alloc(&h->scratch, pcf::solve_scratch_doubles(h->geo)))) return r;
   return PCF_OK;
 }
 
@@ -393,54 +397,40 @@ int32_t nonlinear_into(pcf_handle *h, const double2 *u0, int slot, void *ytape =
   return PCF_OK;
 }
 
-// Structured solve + reconstruction of one implicit update; under MHD also
-// the vector-potential solve described by aA (g side, component 0).
-// factored: the stored factors fs (flow) and fsA (A), default the steady SBDF3 ones;
-// otherwise factorised on the fly at a.gamma.
-hipError_t implicit_update(pcf_handle *h, const pcf::SolveArgs &a, bool factored,
-                           const pcf::SolveArgs *aA = nullptr, const pcf::FactorSet *fs = nullptr,
-                           const pcf::FactorSet *fsA = nullptr) {
+// Factored solve + reconstruction of one implicit update on the factor sets
+// fs (flow) and fsA (A), by default the steady SBDF3 ones; under MHD also the
+// vector-potential solve described by aA (g side, component 0).  aBord: the
+// update of the bordered A_y/A_z solve (insulating walls, factorised per
+// launch) when it differs from aA.
+hipError_t implicit_update(pcf_handle *h, const pcf::SolveArgs &a, const pcf::SolveArgs *aA = nullptr,
+                           const pcf::FactorSet *fs = nullptr, const pcf::FactorSet *fsA = nullptr,
+                           const pcf::SolveArgs *aBord = nullptr) {
   hipError_t e;
-  const bool steady = factored;
   if (!fs) fs = &h->fsteady;
   if (!fsA) fsA = &h->fA;
   if ((e = mark(h, 5, 0))) return e;
-  if ((e = steady ? pcf::launch_solve_factored(h->geo, h->sm, *fs, a, 0, 0, h->stream)
-                  : pcf::launch_solve(h->geo, h->sm, a, h->scratch, false, h->stream)))
-    return e;
+  if ((e = pcf::launch_solve_factored(h->geo, h->sm, *fs, a, 0, 0, h->stream))) return e;
   if (aA) {
     // components solved with the Dirichlet (TD) systems: all three for
     // conducting walls, Ax alone for insulating walls
     const int ntd = h->insul ? 1 : 3;
-    if (steady) {
-      if ((e = pcf::launch_solve_factored(h->geo_eta, h->sm, *fsA, *aA, ntd, h->cstride, h->stream))) return e;
-    } else {
-      for (int c = 0; c < ntd; ++c) {
-        pcf::SolveArgs ac = *aA;
-        const size_t off = c * h->cstride;
-        for (int l = 0; l < 3; ++l) {
-          if (ac.g[l]) ac.g[l] += off;
-          if (ac.Ng[l]) ac.Ng[l] += off;
-        }
-        ac.g_out += off;
-        if ((e = pcf::launch_solve(h->geo_eta, h->sm, ac, h->scratch, true, h->stream))) return e;
-      }
-    }
+    if ((e = pcf::launch_solve_factored(h->geo_eta, h->sm, *fsA, *aA, ntd, h->cstride, h->stream))) return e;
     if (h->insul) {
+      const pcf::SolveArgs &ab = aBord ? *aBord : *aA;
       pcf::BorderedArgs b{};
-      for (int l = 0; l < aA->nx; ++l) {
-        b.x[l] = aA->g[l];
-        b.c[l] = aA->c[l];
-        b.d[l] = aA->d[l];
+      for (int l = 0; l < ab.nx; ++l) {
+        b.x[l] = ab.g[l];
+        b.c[l] = ab.c[l];
+        b.d[l] = ab.d[l];
       }
-      b.nx = aA->nx;
-      for (int l = 0; l < aA->nn; ++l) {
-        b.n[l] = aA->Ng[l];
-        b.e[l] = aA->e[l];
+      b.nx = ab.nx;
+      for (int l = 0; l < ab.nn; ++l) {
+        b.n[l] = ab.Ng[l];
+        b.e[l] = ab.e[l];
       }
-      b.nn = aA->nn;
-      b.gamma = aA->gamma;
-      b.out = aA->g_out;
+      b.nn = ab.nn;
+      b.gamma = ab.gamma;
+      b.out = ab.g_out;
       b.cstride = h->cstride;
       if ((e = pcf::launch_solve_bordered(h->geo_eta, b, h->stream))) return e;
     }
@@ -449,6 +439,27 @@ hipError_t implicit_update(pcf_handle *h, const pcf::SolveArgs &a, bool factored
   if ((e = mark(h, 6, 0))) return e;
   if ((e = pcf::launch_reconstruct(h->geo, h->Rt, h->ldr, a, h->stream))) return e;
   return mark(h, 6, 1);
+}
+
+// A Crank-Nicolson-form update, rhs = M x + gamma L x + sum e N solved with
+// M - gamma L at the same gamma (the IMEXRK3 stages, CNAB2), on the stored
+// factors fs / fsA at gamma: (M - gamma L)(x_new + x) = 2 M x + sum e N, so the
+// factored kernel takes z = 2 x and subtracts x from its output.  The
+// bordered A_y/A_z solve of insulating walls keeps the general form.
+hipError_t cn_update(pcf_handle *h, const pcf::SolveArgs &a, const pcf::SolveArgs *aA, const pcf::FactorSet &fs,
+                     const pcf::FactorSet &fsA) {
+  if (a.nx != 1 || a.c[0] != 1.0 || a.d[0] != a.gamma) return hipErrorInvalidValue;
+  pcf::SolveArgs af = a, aAf{};
+  af.c[0] = 2.0;
+  af.d[0] = 0.0;
+  af.sub = 1.0;
+  if (aA) {
+    aAf = *aA;
+    aAf.c[0] = 2.0;
+    aAf.d[0] = 0.0;
+    aAf.sub = 1.0;
+  }
+  return implicit_update(h, af, aA ? &aAf : nullptr, &fs, &fsA, aA);
 }
 
 pcf::SolveArgs base_args(pcf_handle *h, int out_slot) {
@@ -534,7 +545,7 @@ int32_t steady_step(pcf_handle *h, void *const *tape = nullptr) {
     const int xs[3] = {h->so[0], h->so[1], h->so[2]}, ns[3] = {A, h->no[1], h->no[2]};
     aA = a_args_from(h, a, xs, 3, ns, 3, h->so[2]);
   }
-  hipError_t e = implicit_update(h, a, true, h->mhd ? &aA : nullptr);
+  hipError_t e = implicit_update(h, a, h->mhd ? &aA : nullptr);
   if (e != hipSuccess) return fail(PCF_E_HIP, std::string("solve launch: ") + hipGetErrorString(e));
   rotate(h);
   h->history_steps = 2.0;
@@ -555,7 +566,7 @@ int32_t startup_step(pcf_handle *h, void *const *tape = nullptr) {
   if (r) return r;
   const int nslot[3] = {A, B, C};
   for (int rk = 0; rk < 3; ++rk) {
-    const double gamma = (IMEX_A[rk] + IMEX_B[rk]) * dt / 2.0;
+    const double gamma = stage_gamma(rk, dt);
     const int xs = (rk == 0) ? cur : out;
     if (rk > 0) {
       r = nonlinear_into(h, h->u0[out], nslot[rk], tape ? tape[rk] : nullptr, h->A[out]);
@@ -576,7 +587,7 @@ int32_t startup_step(pcf_handle *h, void *const *tape = nullptr) {
       const int xl[3] = {xs, 0, 0}, ns[3] = {nslot[rk], rk > 0 ? nslot[rk - 1] : 0, 0};
       aA = a_args_from(h, a, xl, 1, ns, a.nn, out);
     }
-    hipError_t e = implicit_update(h, a, false, h->mhd ? &aA : nullptr);
+    hipError_t e = cn_update(h, a, h->mhd ? &aA : nullptr, h->fstage[rk], h->fA_stage[rk]);
     if (e != hipSuccess) return fail(PCF_E_HIP, std::string("solve launch: ") + hipGetErrorString(e));
   }
   rotate(h);
@@ -619,7 +630,7 @@ int32_t ars222_step(pcf_handle *h) {
       const int xs[3] = {cur, 0, 0}, ns[3] = {NA, 0, 0};
       aA = a_args_from(h, a, xs, 1, ns, 1, x1);
     }
-    if ((e = implicit_update(h, a, true, h->mhd ? &aA : nullptr, &h->fars, &h->fA_ars)))
+    if ((e = implicit_update(h, a, h->mhd ? &aA : nullptr, &h->fars, &h->fA_ars)))
       return fail(PCF_E_HIP, std::string("solve launch: ") + hipGetErrorString(e));
   }
   if ((r = nonlinear_into(h, h->u0[x1], NB, nullptr, h->A[x1]))) return r;
@@ -637,7 +648,7 @@ int32_t ars222_step(pcf_handle *h) {
       const int xs[3] = {cur, x1, 0}, ns[3] = {NA, NB, 0};
       aA = a_args_from(h, a, xs, 2, ns, 2, x2);
     }
-    if ((e = implicit_update(h, a, true, h->mhd ? &aA : nullptr, &h->fars, &h->fA_ars)))
+    if ((e = implicit_update(h, a, h->mhd ? &aA : nullptr, &h->fars, &h->fA_ars)))
       return fail(PCF_E_HIP, std::string("solve launch: ") + hipGetErrorString(e));
   }
   rotate(h);  // x2 becomes the current slot
@@ -648,9 +659,9 @@ int32_t ars222_step(pcf_handle *h) {
 // vector-potential update, examples/pcf_mhd_jax.py:359-387):
 //   rhs = M x + (dt/2) L x + dt E,  (M - (dt/2) L) x_new = rhs,
 // E = (1 + r/2) N - (r/2) N_old with r = dt / previous_dt when have_old != 0
-// (src/jaxfun/integrators/cnab2.py:211-239), else E = N.  The on-the-fly
-// factorisation of the general solve makes any dt valid.  N lands in the
-// current nonlinear slot, which the ring shift makes NL_OLD.
+// (src/jaxfun/integrators/cnab2.py:211-239), else E = N, on the factors at
+// dt/2 (fcn, fA_cn; pcf_set_dt refactors them).  N lands in the current
+// nonlinear slot, which the ring shift makes NL_OLD.
 int32_t cnab2_step(pcf_handle *h) {
   const double dt = h->cfg.dt;
   const int cur = h->so[0], out = h->so[2], NA = h->no[0], NO = h->no[1];
@@ -677,7 +688,7 @@ int32_t cnab2_step(pcf_handle *h) {
     const int xs[3] = {cur, 0, 0}, ns[3] = {NA, NO, 0};
     aA = a_args_from(h, a, xs, 1, ns, a.nn, out);
   }
-  hipError_t e = implicit_update(h, a, false, h->mhd ? &aA : nullptr);
+  hipError_t e = cn_update(h, a, h->mhd ? &aA : nullptr, h->fcn, h->fA_cn);
   if (e != hipSuccess) return fail(PCF_E_HIP, std::string("solve launch: ") + hipGetErrorString(e));
   rotate(h);
   h->have_old = 1.0;
@@ -1358,15 +1369,21 @@ int32_t pcf_create(const pcf_config *cfg, pcf_handle **out) {
     if (e != hipSuccess)
       return bail(fail(PCF_E_UNSUPPORTED, std::string("kernel LDS limits: ") + hipGetErrorString(e)));
   }
+  const int ti = cfg->time_integrator;
+  const bool stages = (ti == PCF_TI_SBDF3 || ti == PCF_TI_IMEXRK3);  // IMEXRK3 stages run
   if ((r = build_factors(h, h->sm, cfg->dt * 6.0 / 11.0, h->fsteady))) return bail(r);
-  if (cfg->time_integrator == PCF_TI_IMEXRK222 && (r = build_factors(h, h->sm, cfg->dt * ARS_G, h->fars)))
-    return bail(r);
+  if (ti == PCF_TI_IMEXRK222 && (r = build_factors(h, h->sm, cfg->dt * ARS_G, h->fars))) return bail(r);
+  for (int rk = 0; stages && rk < 3; ++rk)
+    if ((r = build_factors(h, h->sm, stage_gamma(rk, cfg->dt), h->fstage[rk]))) return bail(r);
+  if (ti == PCF_TI_CNAB2 && (r = build_factors(h, h->sm, 0.5 * cfg->dt, h->fcn))) return bail(r);
   if (h->mhd) {
     h->geo_eta = h->geo;
     h->geo_eta.nu = cfg->eta;
     if ((r = build_factors_A(h, cfg->dt * 6.0 / 11.0, h->fA))) return bail(r);
-    if (cfg->time_integrator == PCF_TI_IMEXRK222 && (r = build_factors_A(h, cfg->dt * ARS_G, h->fA_ars)))
-      return bail(r);
+    if (ti == PCF_TI_IMEXRK222 && (r = build_factors_A(h, cfg->dt * ARS_G, h->fA_ars))) return bail(r);
+    for (int rk = 0; stages && rk < 3; ++rk)
+      if ((r = build_factors_A(h, stage_gamma(rk, cfg->dt), h->fA_stage[rk]))) return bail(r);
+    if (ti == PCF_TI_CNAB2 && (r = build_factors_A(h, 0.5 * cfg->dt, h->fA_cn))) return bail(r);
     if ((r = h->alloc(&h->ddiagm, 5))) return bail(r);
     if (h->insul) {
       if ((r = h->upload(&h->dVC, h->ops.VC.a))) return bail(r);
@@ -1534,13 +1551,18 @@ int32_t pcf_set_dt(pcf_handle *h, double dt) {
   if (ti == PCF_TI_SBDF3)
     return fail(PCF_E_FIXED_DT, "SBDF3 runs a fixed dt; start a fresh solver/history for a different timestep");
   PCF_HIP(hipSetDevice(h->device));
-  if (ti == PCF_TI_IMEXRK222) {
-    // the stage systems M - g dt L in place (storage sized by the geometry alone)
-    hipError_t e = pcf::launch_factor(h->geo, h->sm, dt * ARS_G, h->fars, false, h->stream);
-    if (e == hipSuccess && h->mhd) e = pcf::launch_factor(h->geo_eta, h->sm, dt * ARS_G, h->fA_ars, true, h->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
-    if (e != hipSuccess) return fail(PCF_E_HIP, std::string("factorisation: ") + hipGetErrorString(e));
-  }
+  // the integrator's factor sets in place (storage sized by the geometry alone)
+  hipError_t e = hipSuccess;
+  auto refactor = [&](double gamma, pcf::FactorSet &fs, pcf::FactorSet &fsA) {
+    if (e == hipSuccess) e = pcf::launch_factor(h->geo, h->sm, gamma, fs, false, h->stream);
+    if (e == hipSuccess && h->mhd) e = pcf::launch_factor(h->geo_eta, h->sm, gamma, fsA, true, h->stream);
+  };
+  if (ti == PCF_TI_IMEXRK222) refactor(dt * ARS_G, h->fars, h->fA_ars);
+  if (ti == PCF_TI_IMEXRK3)
+    for (int rk = 0; rk < 3; ++rk) refactor(stage_gamma(rk, dt), h->fstage[rk], h->fA_stage[rk]);
+  if (ti == PCF_TI_CNAB2) refactor(0.5 * dt, h->fcn, h->fA_cn);
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  if (e != hipSuccess) return fail(PCF_E_HIP, std::string("factorisation: ") + hipGetErrorString(e));
   h->cfg.dt = dt;
   return PCF_OK;
 }

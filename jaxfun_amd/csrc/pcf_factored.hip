@@ -386,17 +386,37 @@ __global__ void __launch_bounds__(256) k_solve_factored(Geo g, StructMats sm, Fa
     ZN(i, 3, lane) = y.y;
   };
   const double *F = fl + lane;
+  const double sub = a.sub;  // output x - sub x_0 (level 0 of the state)
   if (sys == 0) {
-    substitute<2, 2, 2>(ss, n, F, (size_t)NF_B * SLP, (size_t)SLP, load, keep, load,
-                        [&](int i, double2 x) { a.u0_out[(size_t)(2 * i + par) * M + m] = x; });
+    substitute<2, 2, 2>(ss, n, F, (size_t)NF_B * SLP, (size_t)SLP, load, keep, load, [&](int i, double2 x) {
+      const size_t o = (size_t)(2 * i + par) * M + m;
+      if (sub != 0.0) {
+        const double2 p = a.u0[0][o];
+        x.x -= sub * p.x;
+        x.y -= sub * p.y;
+      }
+      a.u0_out[o] = x;
+    });
   } else if (mean) {
     substitute<1, 1, 1>(ss, n, F, (size_t)NF_D * SLP, (size_t)SLP, load, keep, load, [&](int i, double2 x) {
-      a.v_out[2 * i + par] = x.x;
-      a.w_out[2 * i + par] = x.y;
+      const int J = 2 * i + par;
+      if (sub != 0.0) {
+        x.x -= sub * a.v[0][J];
+        x.y -= sub * a.w[0][J];
+      }
+      a.v_out[J] = x.x;
+      a.w_out[J] = x.y;
     });
   } else {
-    substitute<1, 1, 1>(ss, n, F, (size_t)NF_D * SLP, (size_t)SLP, load, keep, load,
-                        [&](int i, double2 x) { a.g_out[(size_t)(2 * i + par) * M + m] = x; });
+    substitute<1, 1, 1>(ss, n, F, (size_t)NF_D * SLP, (size_t)SLP, load, keep, load, [&](int i, double2 x) {
+      const size_t o = (size_t)(2 * i + par) * M + m;
+      if (sub != 0.0) {
+        const double2 p = a.g[0][o];
+        x.x -= sub * p.x;
+        x.y -= sub * p.y;
+      }
+      a.g_out[o] = x;
+    });
   }
 }
 

@@ -16,10 +16,11 @@
 //   k_rhs_mfma         Dirichlet projection of H fused with the coefficient-space
 //                      KMM rows Nu, Ng, Nv00, Nw00 (channelflow_kmm.py:810-829);
 //                      matrix cores, pcf_mfma.hip
-//   k_solve*           SBDF3/IMEXRK3 right-hand side, FP64 per-mode Helmholtz and
-//                      biharmonic solves, mean-mode solve, velocity
-//                      reconstruction (sbdf3.py:44-62, tpmatrix.py:686-799,
-//                      channelflow_kmm.py:855-870)
+//   k_solve_factored   (pcf_factored.hip) right-hand side and FP64 per-mode
+//                      Helmholtz / biharmonic / mean-mode solves on stored
+//                      factors (sbdf3.py:44-62, tpmatrix.py:686-799)
+//   k_reconstruct      velocity reconstruction (channelflow_kmm.py:855-870)
+//   k_solve_bordered   insulating-wall A_y / A_z update
 #include <algorithm>
 
 #include "pcf_adjoint.hpp"
@@ -307,307 +308,6 @@ __global__ void __launch_bounds__(256) k_rotational_mhd(Geo g, NodalMats<T> nm, 
     V *dst = Hz + f * fstride + ((size_t)x * MY + 2 * y2) * KZ;
     dst[k] = ha;
     dst[KZ + k] = hb;
-  }
-}
-
-// ------------------------------------------------------------ k_solve ----
-// Per-mode implicit update with O(n) structured solves.
-//
-// Parity-split (rows of one parity of the Shen index), each per-mode matrix
-//   A = s0 C0 + s2 C2 + s4 C4
-// is banded (lower bandwidth P, near-upper width Q) plus a far-upper part of
-// rank R for columns j >= i + Q + 1:
-//   A[i][j] = sf * sum_r U_r(i) V_r(j).
-// From the Chebyshev closed forms (T_a, T_b'') = (pi/2) b (b^2 - a^2) and the
-// fourth-derivative analogue, with the boundary sums of the Shen stencils
-// (sum S = sum S a^2 = 0 for the clamped basis, sum S = 0 for the Dirichlet
-// basis): the clamped system has P = Q = 2 and R = 2 (from C4 only), the
-// Dirichlet system P = Q = 1 and R = 1 (C2_D[i][j] = -4 pi (I+1) beyond the
-// band).  One thread solves one (mode, parity) system in three sweeps:
-//   A: bottom-up right-hand side rhs = C0 delta + C2 alpha + C4 beta + N, the
-//      far part through running suffix sums;
-//   B: top-down LU without pivoting (the reference's no-pivot banded LU,
-//      diamatrix.py:1937-1973), tracking each row's far coefficients;
-//   C: bottom-up back substitution with running suffix sums.
-// Row data between sweeps lives in a global scratch array laid out
-// [system][row][value][mode], so consecutive lanes (consecutive modes) touch
-// consecutive addresses.
-template <int P, int Q, int R, bool HAS4>
-__device__ void structured_solve(const StructSys &ss, int n, int par, int m, int M, double s0,
-                                 double s2, double s4, double sf, double a0z, double a0l,
-                                 double a2z, double a2l, double a4l, const double2 *const *xl,
-                                 const double *cl, const double *dl, int nx,
-                                 const double2 *const *nl, const double *el, int nn,
-                                 const double *vx[3], const double *wx[3],
-                                 const double *vn[3], const double *wn[3], bool mean,
-                                 double *scr, double2 *xout, double *vout, double *wout) {
-  constexpr int F = Q + 1;
-  constexpr int BW = P + Q + 1;
-  constexpr int NV = 1 + Q + R + 2;  // dinv, e[1..Q], w[0..R-1], y (re, im)
-  const size_t rowstride = (size_t)NV * M;
-  auto scr_at = [&](int i, int v) -> double & { return scr[(size_t)i * rowstride + (size_t)v * M + m]; };
-
-  // z, zL and the nonlinear combination of parity row j (full index 2j+par)
-  auto load_row = [&](int j, double2 &z, double2 &zl, double2 &nc) {
-    z = double2{0, 0};
-    zl = double2{0, 0};
-    nc = double2{0, 0};
-    if (j < 0 || j >= n) return;
-    const int J = 2 * j + par;
-    if (!mean) {
-      const size_t o = (size_t)J * M + m;
-      for (int l = 0; l < nx; ++l) {
-        double2 v = xl[l][o];
-        z.x += cl[l] * v.x;
-        z.y += cl[l] * v.y;
-        zl.x += dl[l] * v.x;
-        zl.y += dl[l] * v.y;
-      }
-      for (int l = 0; l < nn; ++l) {
-        double2 v = nl[l][o];
-        nc.x += el[l] * v.x;
-        nc.y += el[l] * v.y;
-      }
-    } else {
-      for (int l = 0; l < nx; ++l) {
-        z.x += cl[l] * vx[l][J];
-        z.y += cl[l] * wx[l][J];
-        zl.x += dl[l] * vx[l][J];
-        zl.y += dl[l] * wx[l][J];
-      }
-      for (int l = 0; l < nn; ++l) {
-        nc.x += el[l] * vn[l][J];
-        nc.y += el[l] * wn[l][J];
-      }
-    }
-  };
-
-  // ---- sweep A: right-hand side, bottom-up
-  {
-    constexpr int WN = P + F + 1;  // window rows i-P .. i+F
-    double2 dl_[WN], al_[WN], bl_[WN], nc_[WN];
-    auto fill = [&](int k, int j) {
-      double2 z, zl;
-      load_row(j, z, zl, nc_[k]);
-      dl_[k] = double2{a0z * z.x + a0l * zl.x, a0z * z.y + a0l * zl.y};
-      al_[k] = double2{a2z * z.x + a2l * zl.x, a2z * z.y + a2l * zl.y};
-      bl_[k] = double2{a4l * zl.x, a4l * zl.y};
-    };
-#pragma unroll
-    for (int k = 0; k < WN; ++k) fill(k, n - 1 - P + k);
-    double2 sig[R];
-#pragma unroll
-    for (int r = 0; r < R; ++r) sig[r] = double2{0, 0};
-    for (int i = n - 1; i >= 0; --i) {
-      if (i + F < n) {
-        const double2 ph = HAS4 ? bl_[P + F] : al_[P + F];
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-          const double v = ss.V[(size_t)(i + F) * R + r];
-          sig[r].x += v * ph.x;
-          sig[r].y += v * ph.y;
-        }
-      }
-      double2 rh = nc_[P];
-      const double *b0 = ss.b0 + (size_t)i * BW, *b2 = ss.b2 + (size_t)i * BW;
-#pragma unroll
-      for (int o = 0; o < BW; ++o) {
-        const double c0 = b0[o], c2 = b2[o];
-        rh.x += c0 * dl_[o].x + c2 * al_[o].x;
-        rh.y += c0 * dl_[o].y + c2 * al_[o].y;
-        if (HAS4) {
-          const double c4 = ss.b4[(size_t)i * BW + o];
-          rh.x += c4 * bl_[o].x;
-          rh.y += c4 * bl_[o].y;
-        }
-      }
-#pragma unroll
-      for (int r = 0; r < R; ++r) {
-        const double u = ss.U[(size_t)i * R + r];
-        rh.x += u * sig[r].x;
-        rh.y += u * sig[r].y;
-      }
-      scr_at(i, NV - 2) = rh.x;
-      scr_at(i, NV - 1) = rh.y;
-#pragma unroll
-      for (int k = WN - 1; k > 0; --k) {
-        dl_[k] = dl_[k - 1];
-        al_[k] = al_[k - 1];
-        bl_[k] = bl_[k - 1];
-        nc_[k] = nc_[k - 1];
-      }
-      fill(0, i - 1 - P);
-    }
-  }
-
-  // ---- sweep B: LU without pivoting, top-down
-  {
-    double pd[P + 1], pe[P + 1][Q + 1], pw[P + 1][R];
-    double2 py[P + 1];
-#pragma unroll
-    for (int t = 0; t <= P; ++t) {
-      pd[t] = 0;
-      py[t] = double2{0, 0};
-#pragma unroll
-      for (int q = 0; q <= Q; ++q) pe[t][q] = 0;
-#pragma unroll
-      for (int r = 0; r < R; ++r) pw[t][r] = 0;
-    }
-    for (int i = 0; i < n; ++i) {
-      double a[BW];
-      const double *b0 = ss.b0 + (size_t)i * BW, *b2 = ss.b2 + (size_t)i * BW;
-#pragma unroll
-      for (int o = 0; o < BW; ++o) {
-        double v = s0 * b0[o] + s2 * b2[o];
-        if (HAS4) v += s4 * ss.b4[(size_t)i * BW + o];
-        a[o] = v;
-      }
-      double wi[R];
-#pragma unroll
-      for (int r = 0; r < R; ++r) wi[r] = sf * ss.U[(size_t)i * R + r];
-      double2 yi{scr_at(i, NV - 2), scr_at(i, NV - 1)};
-      // eliminate columns i-P .. i-1 with the stored rows t = P .. 1 above
-#pragma unroll
-      for (int t = P; t >= 1; --t) {
-        if (i - t < 0) continue;
-        const double l = a[P - t] * pd[t];  // pd holds 1/diag
-#pragma unroll
-        for (int o = -t + 1; o <= Q; ++o) {
-          if (i + o >= n) continue;
-          const int rel = o + t;  // column offset within row i-t
-          double u;
-          if (rel <= Q) {
-            u = pe[t][rel];
-          } else {
-            u = 0.0;
-#pragma unroll
-            for (int r = 0; r < R; ++r) u += pw[t][r] * ss.V[(size_t)(i + o) * R + r];
-          }
-          a[P + o] -= l * u;
-        }
-#pragma unroll
-        for (int r = 0; r < R; ++r) wi[r] -= l * pw[t][r];
-        yi.x -= l * py[t].x;
-        yi.y -= l * py[t].y;
-      }
-      const double dinv = 1.0 / a[P];
-      scr_at(i, 0) = dinv;
-#pragma unroll
-      for (int q = 1; q <= Q; ++q) scr_at(i, q) = a[P + q];
-#pragma unroll
-      for (int r = 0; r < R; ++r) scr_at(i, 1 + Q + r) = wi[r];
-      scr_at(i, NV - 2) = yi.x;
-      scr_at(i, NV - 1) = yi.y;
-#pragma unroll
-      for (int t = P; t >= 2; --t) {
-        pd[t] = pd[t - 1];
-        py[t] = py[t - 1];
-#pragma unroll
-        for (int q = 0; q <= Q; ++q) pe[t][q] = pe[t - 1][q];
-#pragma unroll
-        for (int r = 0; r < R; ++r) pw[t][r] = pw[t - 1][r];
-      }
-      pd[1] = dinv;
-      py[1] = yi;
-#pragma unroll
-      for (int q = 1; q <= Q; ++q) pe[1][q] = a[P + q];
-#pragma unroll
-      for (int r = 0; r < R; ++r) pw[1][r] = wi[r];
-    }
-  }
-
-  // ---- sweep C: back substitution, bottom-up
-  {
-    double2 xw[F + 1];
-#pragma unroll
-    for (int k = 0; k <= F; ++k) xw[k] = double2{0, 0};
-    double2 tau[R];
-#pragma unroll
-    for (int r = 0; r < R; ++r) tau[r] = double2{0, 0};
-    for (int i = n - 1; i >= 0; --i) {
-      if (i + F < n) {
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-          const double v = ss.V[(size_t)(i + F) * R + r];
-          tau[r].x += v * xw[F].x;
-          tau[r].y += v * xw[F].y;
-        }
-      }
-      double2 acc{scr_at(i, NV - 2), scr_at(i, NV - 1)};
-#pragma unroll
-      for (int q = 1; q <= Q; ++q) {
-        const double e = scr_at(i, q);
-        acc.x -= e * xw[q].x;
-        acc.y -= e * xw[q].y;
-      }
-#pragma unroll
-      for (int r = 0; r < R; ++r) {
-        const double w = scr_at(i, 1 + Q + r);
-        acc.x -= w * tau[r].x;
-        acc.y -= w * tau[r].y;
-      }
-      const double dinv = scr_at(i, 0);
-      const double2 xi{acc.x * dinv, acc.y * dinv};
-      const int J = 2 * i + par;
-      if (mean) {
-        vout[J] = xi.x;
-        wout[J] = xi.y;
-      } else {
-        xout[(size_t)J * M + m] = xi;
-      }
-#pragma unroll
-      for (int k = F; k >= 2; --k) xw[k] = xw[k - 1];
-      xw[1] = xi;
-    }
-  }
-}
-
-// Threads [0, 4 Mp): (system, parity) = (t / Mp) with system 0 = clamped u0,
-// 1 = Dirichlet g; lanes over modes.  Threads [4 Mp, 4 Mp + 2): the 1-D mean
-// system S00 = C0_D - gamma nu C2_D for (v00 + i w00), parity = lane.
-// dir_only: threads [0, 2 Mp) take the Dirichlet systems alone (a vector-
-// potential component), without the mean system.
-__global__ void __launch_bounds__(256) k_solve(Geo g, StructMats sm, SolveArgs a, double *scratch, bool dir_only) {
-  const int M = g.m, Mp = (M + 63) / 64 * 64;
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (dir_only) {
-    t += 2 * Mp;
-    if (t >= 4 * Mp) return;
-  }
-  int sp = t / Mp, m = t - sp * Mp;
-  bool mean = false;
-  if (sp >= 4) {
-    if (sp > 4 || m >= 2) return;
-    mean = true;
-    sp = 2 + m;
-    m = 0;
-  } else if (m >= M) {
-    return;
-  }
-  const int sys = sp >> 1, par = sp & 1;
-  const StructSys &ss = sm.sys[sys][par];
-  const int n = ss.n;
-  double kyd, kzd, kyo;
-  mode_k(g, m, kyd, kzd, kyo);
-  const double k2 = mean ? 0.0 : kyd * kyd + kzd * kzd;
-  const double W = mean ? 1.0 : g.W, nu = g.nu, gm = a.gamma, wn = W * nu;
-  // scratch rows: the four mode systems, then the two mean systems
-  double *scr = mean ? scratch + (size_t)4 * g.npmax * 7 * M + (size_t)(sp - 2) * g.npmax * 7
-                     : scratch + (size_t)sp * g.npmax * 7 * M;
-  const int Ms = mean ? 1 : M;
-  if (sys == 0) {
-    // Su = W[(1 + 2 gm nu k2) C2 - gm nu C4 - (k2 + gm nu k2^2) C0]
-    // M z + L zL: delta = -k2 W z + k2^2 W nu zL, alpha = W z - 2 k2 W nu zL, beta = W nu zL
-    const double s2 = W * (1 + 2 * gm * nu * k2), s4 = -W * gm * nu, s0 = -W * (k2 + gm * nu * k2 * k2);
-    structured_solve<2, 2, 2, true>(ss, n, par, m, Ms, s0, s2, s4, s4, -k2 * W, k2 * k2 * wn, W,
-                                    -2 * k2 * wn, wn, a.u0, a.c, a.d, a.nx, a.Nu, a.e, a.nn, a.v,
-                                    a.w, a.Nv, a.Nw, false, scr, a.u0_out, nullptr, nullptr);
-  } else {
-    // Sg = W[(1 + gm nu k2) C0 - gm nu C2]; M = W C0, L = W nu (C2 - k2 C0)
-    const double s0 = W * (1 + gm * nu * k2), s2 = -W * gm * nu;
-    structured_solve<1, 1, 1, false>(ss, n, par, mean ? 0 : m, Ms, s0, s2, 0.0, s2, W, -k2 * wn,
-                                     0.0, wn, 0.0, a.g, a.c, a.d, a.nx, a.Ng, a.e, a.nn, a.v, a.w,
-                                     a.Nv, a.Nw, mean, scr, a.g_out, a.v_out, a.w_out);
   }
 }
 
@@ -1107,14 +807,6 @@ hipError_t launch_rotational(const Geo &g, const NodalMats<T> &nm, Workspace &ws
 
 size_t solve_scratch_doubles(const Geo &g) {
   return (size_t)4 * g.npmax * 7 * g.m + (size_t)2 * g.npmax * 7;
-}
-
-hipError_t launch_solve(const Geo &g, const StructMats &sm, const SolveArgs &a, double *scratch, bool dir_only,
-                        hipStream_t s) {
-  const int Mp = (g.m + 63) / 64 * 64;
-  const int threads = dir_only ? 2 * Mp : 4 * Mp + 64;
-  hipLaunchKernelGGL(k_solve, dim3((threads + 255) / 256), dim3(256), 0, s, g, sm, a, scratch, dir_only);
-  return hipGetLastError();
 }
 
 hipError_t launch_solve_bordered(const Geo &g, const BorderedArgs &a, hipStream_t s) {

@@ -91,8 +91,12 @@ struct StructMats {
   StructSys sys[2][2];  // [basis: 0 clamped, 1 Dirichlet][parity]
 };
 
-// One implicit update: rhs = M z + L zL + sum_l e_l N_l with z = sum c_l x_l,
-// zL = sum d_l x_l; solve (M - gamma L) x = rhs, then rebuild u1/u2.
+// One implicit update: rhs = M z + sum_l e_l N_l with z = sum c_l x_l; the
+// factored solve of (M - gamma L) y = rhs writes x = y - sub x_0, then u1/u2
+// are rebuilt from x.  A Crank-Nicolson-form update rhs = M x_0 + gamma L x_0
+// + sum e N at the same gamma is (M - gamma L)(x + x_0) = 2 M x_0 + sum e N,
+// i.e. z = 2 x_0, sub = 1.  d (zL = sum d_l x_l, an explicit L zL term) is read
+// only where the update feeds the bordered solve.
 struct SolveArgs {
   const double2 *u0[3];
   const double2 *g[3];
@@ -107,6 +111,7 @@ struct SolveArgs {
   double e[3];
   int nn;
   double gamma;
+  double sub;
   double2 *u0_out, *g_out;
   double *v_out, *w_out;
   double2 *u1, *u2;
@@ -160,12 +165,9 @@ struct BorderedArgs {
   size_t cstride;
 };
 hipError_t launch_solve_bordered(const Geo &g, const BorderedArgs &a, hipStream_t s);
-// General implicit update (IMEXRK3 stages): the factorisation runs with the solve.
-// dir_only: the Dirichlet systems alone (g side of a), for a vector-potential component.
-hipError_t launch_solve(const Geo &g, const StructMats &sm, const SolveArgs &a, double *scratch, bool dir_only,
-                        hipStream_t s);
 hipError_t launch_reconstruct(const Geo &g, const double *Rc, int ldk, const SolveArgs &a,
                               hipStream_t s);
+// Per-row data of the reverse pass's on-the-fly transposed solves.
 size_t solve_scratch_doubles(const Geo &g);
 // Factored steady solves: factor storage (basis -1: the mean system), the
 // one-time factorisation at gamma, and the per-step solve (SBDF3 update).
