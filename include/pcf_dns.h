@@ -216,6 +216,16 @@ int32_t pcf_diagnostics(pcf_handle *h, double *out8);
  * and the corresponding entries of examples/pcf_mhd_mri_shearpy_jax.py:431-513,802-836. */
 int32_t pcf_mhd_diagnostics(pcf_handle *h, double *out4);
 
+/* Largest magnitudes on the unpadded physical grid (synchronous): out[6] =
+ * {max|u_x|, max|u_y + U(x)|, max|u_z|, max|b_x + B0_x|, max|b_y + B0_y|,
+ *  max|b_z + B0_z|} of the current state, total velocity (base flow U = u_wall x)
+ * and, for MHD handles, total magnetic field b = curl A + B0 (zeros for
+ * hydrodynamic handles).  The device reductions behind the CFL numbers of
+ * production/health.py:120-193 (u_max, b_max of _cfl_scalars from
+ * total_velocity_physical and _total_B_physical(B, padded=False)).
+ * PCF_E_UNSUPPORTED when ny or nz has no 2/3/4 FFT plan. */
+int32_t pcf_field_maxima(pcf_handle *h, double *out6);
+
 /* Per-stage timing.  One steady step launches pcf_kernels_per_step() kernels,
  * one per stage, in the order pcf_stage_name(0..pcf_stage_count()-1):
  * nodal_backward, fft_y_inverse, rotational, fft_y_forward, forward_rhs, solve,

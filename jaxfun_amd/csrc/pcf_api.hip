@@ -92,6 +92,7 @@ struct DevMats {
   T *VBt = nullptr, *VDt = nullptr, *VD1t = nullptr, *A1t = nullptr, *A0t = nullptr,
     *AGt = nullptr, *xnode = nullptr, *twy = nullptr, *twz = nullptr;
   T *VQ1t = nullptr, *VQ2t = nullptr, *VCt = nullptr, *VC1t = nullptr, *VC2t = nullptr, *WQt = nullptr;
+  T *twy0 = nullptr, *twz0 = nullptr;  // twiddles of the unpadded grid (ny, nz)
   pcf::NodalMats<T> view{};
   pcf::MhdMats<T> mview{};  // MHD contractions (conducting or insulating walls)
 };
@@ -103,6 +104,10 @@ struct pcf_handle {
   pcf::Geo geo{};
   pcf::Operators ops;
   pcf::FFTPlan py{}, pz{};
+  pcf::Geo geo0{};            // the unpadded grid (my = ny, mz = nz) of the field maxima
+  pcf::FFTPlan py0{}, pz0{};
+  bool maxima_ok = false;     // the unpadded sizes have plans that fit the kernels
+  double *dblockmax = nullptr;
   hipStream_t stream = nullptr;
   bool own_stream = false;
   int device = 0;
@@ -260,6 +265,10 @@ int32_t build_nodal(pcf_handle *h, DevMats<T> &dm, int xc, int ic) {
   };
   if ((r = h->upload(&dm.twy, twiddles(h->geo.my)))) return r;
   if ((r = h->upload(&dm.twz, twiddles(h->geo.mz)))) return r;
+  if (h->maxima_ok) {
+    if ((r = h->upload(&dm.twy0, twiddles(h->geo.ny)))) return r;
+    if ((r = h->upload(&dm.twz0, twiddles(h->geo.nz)))) return r;
+  }
   dm.view = pcf::NodalMats<T>{dm.VBt, dm.VDt, dm.VD1t, ldx, dm.A1t, dm.A0t, dm.AGt, ldi,
                               dm.xnode, dm.twy, dm.twz};
   if (h->mhd) {
@@ -694,6 +703,27 @@ int32_t cnab2_step(pcf_handle *h) {
   h->have_old = 1.0;
   h->previous_dt = dt;
   return PCF_OK;
+}
+
+// Nodal fields of the current state, their y-lines on the unpadded grid and
+// the per-block maxima (velocity; under MHD also b = curl A).
+template <typename T>
+hipError_t field_maxima_stages(pcf_handle *h, const DevMats<T> &dm) {
+  const pcf::Geo &g = h->geo, &g0 = h->geo0;
+  pcf::NodalMats<T> nm0 = dm.view;
+  nm0.twy = dm.twy0;
+  nm0.twz = dm.twz0;
+  hipError_t e;
+  if ((e = pcf::launch_nodal_mfma<T>(g, dm.view, h->u0[h->so[0]], h->u1, h->u2, h->ws, h->stream))) return e;
+  if (h->mhd && (e = pcf::launch_nodal_mhd_mfma<T>(g, dm.mview, h->A[h->so[0]], h->cstride, h->ws, h->stream)))
+    return e;
+  T *X = reinterpret_cast<T *>(h->ws.X), *Y = reinterpret_cast<T *>(h->ws.Y);  // complex: 2 T each
+  if ((e = pcf::launch_fft_y_fields<T>(g0, nm0, h->py0, true, 3, X, Y, h->stream))) return e;
+  // b (nodal fields 6..8) after the three velocity fields of the unpadded lines
+  if (h->mhd && (e = pcf::launch_fft_y_fields<T>(g0, nm0, h->py0, true, 3, X + (size_t)2 * 6 * g.n0 * g.m,
+                                                 Y + (size_t)2 * 3 * g.n0 * g0.my * g0.kzs, h->stream)))
+    return e;
+  return pcf::launch_absmax<T>(g0, dm.xnode, dm.twz0, h->pz0, h->mhd ? 6 : 3, Y, h->dblockmax, h->stream);
 }
 
 // Field id -> (device pointer, is_complex, rows).
@@ -1300,6 +1330,15 @@ int32_t pcf_create(const pcf_config *cfg, pcf_handle **out) {
     return bail(fail(PCF_E_INVALID, "padded grid sizes must factor into 2, 3, 4"));
   if (!pcf::fft_fits(g, h->py, h->pz, cfg->precision == PCF_PREC_FP64 ? 16 : 8, h->mhd))
     return bail(fail(PCF_E_INVALID, "FFT sizes exceed the kernels' register budget"));
+  {
+    const int eb = cfg->precision == PCF_PREC_FP64 ? 16 : 8;
+    h->geo0 = g;
+    h->geo0.my = g.ny;
+    h->geo0.mz = g.nz;
+    h->maxima_ok = make_plan(g.ny, h->py0) && make_plan(g.nz, h->pz0) &&
+                   pcf::fft_fits(h->geo0, h->py0, h->pz0, eb, h->mhd) &&
+                   pcf::absmax_fits(h->geo0, h->pz0, h->mhd ? 6 : 3, eb);
+  }
 
   h->device = cfg->device;
   if (hipSetDevice(cfg->device) != hipSuccess) return bail(fail(PCF_E_HIP, "hipSetDevice failed"));
@@ -1398,6 +1437,10 @@ int32_t pcf_create(const pcf_config *cfg, pcf_handle **out) {
   if ((r = h->upload(&h->dfejer, h->ops.fejer))) return bail(r);
   if ((r = h->upload(&h->dxn, h->ops.x))) return bail(r);
   if ((r = h->alloc(&h->ddiag, 4))) return bail(r);
+  if (h->maxima_ok) {
+    const int nf = h->mhd ? 6 : 3, eb = cfg->precision == PCF_PREC_FP64 ? 16 : 8;
+    if ((r = h->alloc(&h->dblockmax, (size_t)pcf::absmax_blocks(h->geo0, nf, eb) * nf))) return bail(r);
+  }
   *out = h;
   return PCF_OK;
 }
@@ -1672,6 +1715,24 @@ int32_t pcf_mhd_diagnostics(pcf_handle *h, double *out) {
   double r = 0.0;
   for (double v : bm) r = std::max(r, v);
   out[3] = r;
+  return PCF_OK;
+}
+
+int32_t pcf_field_maxima(pcf_handle *h, double *out) {
+  if (!h || !out) return fail(PCF_E_INVALID, "null argument");
+  if (!h->maxima_ok)
+    return fail(PCF_E_UNSUPPORTED, "field maxima need ny, nz that factor into 2, 3, 4 within the FFT kernels");
+  PCF_HIP(hipSetDevice(h->device));
+  const bool fp64 = h->cfg.precision == PCF_PREC_FP64;
+  const int nf = h->mhd ? 6 : 3;
+  hipError_t e = fp64 ? field_maxima_stages<double>(h, h->md) : field_maxima_stages<float>(h, h->mf);
+  if (e != hipSuccess) return fail(PCF_E_HIP, std::string("field maxima launch: ") + hipGetErrorString(e));
+  std::vector<double> bm((size_t)pcf::absmax_blocks(h->geo0, nf, fp64 ? 16 : 8) * nf);
+  PCF_HIP(hipMemcpyAsync(bm.data(), h->dblockmax, bm.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  PCF_HIP(hipStreamSynchronize(h->stream));
+  for (int f = 0; f < 6; ++f) out[f] = 0.0;
+  for (size_t b = 0; b < bm.size() / nf; ++b)
+    for (int f = 0; f < nf; ++f) out[f] = std::max(out[f], bm[b * nf + f]);
   return PCF_OK;
 }
 

@@ -192,6 +192,83 @@ __global__ void __launch_bounds__(256) k_rotational(Geo g, NodalMats<T> nm, FFTP
   }
 }
 
+// ------------------------------------------------------- k_absmax_z ----
+// Largest |value| of nf physical fields on the unpadded grid (g is the
+// unpadded geometry, MY = ny, MZ = nz), for the CFL numbers of
+// production/health.py:120-155: the spanwise c2r FFT of y-line pairs as in
+// k_rotational, plus the base flow U x on field 1 (streamwise velocity) and
+// B0 on fields 3..5 (magnetic field), then a block maximum per field into
+// blockmax[block][nf].  Units: the line pairs (x, 2 y2), (x, 2 y2 + 1).
+template <typename T, int MAXB>
+__global__ void __launch_bounds__(256) k_absmax_z(Geo g, const typename VT<T>::V *__restrict__ twz, FFTPlan plan,
+                                                  int UPB, int nf, const T *__restrict__ xnode,
+                                                  const typename VT<T>::V *__restrict__ Y,
+                                                  double *__restrict__ blockmax) {
+  using V = typename VT<T>::V;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int MZ = g.mz, MY = g.my, KZ = g.kzs;
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  const int nunits = g.n0 * (MY / 2);
+  const int ubase = blockIdx.x * UPB;
+  const size_t fstride = (size_t)g.n0 * MY * KZ;
+  V *buf = reinterpret_cast<V *>(smem);            // [UPB][nf][MZ]
+  V *tw = buf + (size_t)UPB * nf * MZ;             // [MZ]
+  double *red = reinterpret_cast<double *>(tw + MZ);  // [waves][6]
+  for (int i = tid; i < MZ; i += nthr) tw[i] = twz[i];
+  gather_batched<4>(
+      UPB * nf * MZ,
+      [&](int idx) {
+        const int lu = idx / (nf * MZ), rem = idx - lu * nf * MZ;
+        const int f = rem / MZ, k = rem - f * MZ;
+        const int u = ubase + lu;
+        V z{0, 0};
+        if (u < nunits) {
+          const int x = u / (MY / 2), y2 = u - x * (MY / 2);
+          const V *ra = Y + f * fstride + ((size_t)x * MY + 2 * y2) * KZ;
+          const V *rb = ra + KZ;
+          if (k == 0) {
+            z = V{ra[0].x, rb[0].x};
+          } else if (k < KZ) {
+            const V a = ra[k], b = rb[k];
+            z = V{a.x - b.y, a.y + b.x};
+          } else if (k > MZ - KZ) {
+            const V a = ra[MZ - k], b = rb[MZ - k];
+            z = V{a.x + b.y, b.x - a.y};
+          }
+        }
+        return z;
+      },
+      [&](int idx, const V &z) { buf[idx] = z; });
+  __syncthreads();
+  fft_lds<V, T, true, false, MAXB>(buf, tw, plan, UPB * nf, MZ, 1);
+  double mx[6] = {0, 0, 0, 0, 0, 0};
+  for (int idx = tid; idx < UPB * nf * MZ; idx += nthr) {
+    const int lu = idx / (nf * MZ), f = (idx - lu * nf * MZ) / MZ;
+    const int u = ubase + lu;
+    if (u >= nunits) continue;
+    const int x = u / (MY / 2);
+    const double add = (f == 1) ? g.uwall * (double)xnode[x] : (f >= 3) ? g.b0[f - 3] : 0.0;
+    const V v = buf[idx];
+    const double m = fmax(fabs((double)v.x + add), fabs((double)v.y + add));
+#pragma unroll
+    for (int q = 0; q < 6; ++q)
+      if (q == f) mx[q] = fmax(mx[q], m);
+  }
+  const int wave = tid >> 6, lane = tid & 63;
+#pragma unroll
+  for (int q = 0; q < 6; ++q) {
+    double v = mx[q];
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off));
+    if (lane == 0) red[wave * 6 + q] = v;
+  }
+  __syncthreads();
+  if (tid < nf) {
+    double m = 0.0;
+    for (int w = 0; w < (nthr >> 6); ++w) m = fmax(m, red[w * 6 + tid]);
+    blockmax[(size_t)blockIdx.x * nf + tid] = m;
+  }
+}
+
 // MHD counterpart of k_rotational: twelve input fields (u, w, b, j) per
 // y-line and six outputs, H = -(u_tot x w_tot) + (-2 Omega u1, 2 Omega u0, 0)
 // - j x b_tot and E = u_tot x b_tot with b_tot = b + B0
@@ -805,6 +882,45 @@ hipError_t launch_rotational(const Geo &g, const NodalMats<T> &nm, Workspace &ws
   return hipGetLastError();
 }
 
+// Line pairs per block of k_absmax_z and its launch checks.
+static int absmax_units(const Geo &g, int nf, int elem_bytes) {
+  int upb = 4;
+  while (upb > 1 && ((size_t)upb * nf * g.mz + g.mz) * elem_bytes + 4 * 6 * sizeof(double) > 65536) upb >>= 1;
+  return upb;
+}
+
+int absmax_blocks(const Geo &g, int nf, int elem_bytes) {
+  const int upb = absmax_units(g, nf, elem_bytes);
+  return (g.n0 * (g.my / 2) + upb - 1) / upb;
+}
+
+bool absmax_fits(const Geo &g, const FFTPlan &pz, int nf, int elem_bytes) {
+  const int upb = absmax_units(g, nf, elem_bytes);
+  if (g.my % 2 || g.mz < 2 * g.kzs || nf < 1 || nf > 6) return false;
+  if (((size_t)upb * nf * g.mz + g.mz) * elem_bytes + 4 * 6 * sizeof(double) > 65536) return false;
+  return max_butterflies(pz, upb * nf) <= 12;
+}
+
+template <typename T>
+hipError_t launch_absmax(const Geo &g, const T *xnode, const T *twz, const FFTPlan &pz, int nf, const void *Y,
+                         double *blockmax, hipStream_t s) {
+  using V = typename VT<T>::V;
+  if (!absmax_fits(g, pz, nf, sizeof(V))) return hipErrorInvalidValue;
+  const int upb = absmax_units(g, nf, sizeof(V));
+  const size_t lds = ((size_t)upb * nf * g.mz + g.mz) * sizeof(V) + 4 * 6 * sizeof(double);
+  const int mb = max_butterflies(pz, upb * nf);
+  const dim3 grid(absmax_blocks(g, nf, sizeof(V)));
+  const V *tw = reinterpret_cast<const V *>(twz);
+  const V *y = reinterpret_cast<const V *>(Y);
+#define PCF_ABSMAX(MB) \
+  hipLaunchKernelGGL((k_absmax_z<T, MB>), grid, dim3(256), lds, s, g, tw, pz, upb, nf, xnode, y, blockmax)
+  if (mb <= 3) PCF_ABSMAX(3);
+  else if (mb <= 6) PCF_ABSMAX(6);
+  else PCF_ABSMAX(12);
+#undef PCF_ABSMAX
+  return hipGetLastError();
+}
+
 size_t solve_scratch_doubles(const Geo &g) {
   return (size_t)4 * g.npmax * 7 * g.m + (size_t)2 * g.npmax * 7;
 }
@@ -862,6 +978,8 @@ bool fft_fits(const Geo &g, const FFTPlan &py, const FFTPlan &pz, int elem_bytes
 }
 
 #define PCF_INSTANTIATE(T)                                                                     \
+  template hipError_t launch_absmax<T>(const Geo &, const T *, const T *, const FFTPlan &, int,   \
+                                       const void *, double *, hipStream_t);                     \
   template hipError_t launch_fft_y<T>(const Geo &, const NodalMats<T> &, Workspace &,            \
                                       const FFTPlan &, bool, bool, hipStream_t);                  \
   template hipError_t launch_fft_y_fields<T>(const Geo &, const NodalMats<T> &, const FFTPlan &,   \

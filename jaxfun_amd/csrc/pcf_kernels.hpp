@@ -198,6 +198,16 @@ hipError_t launch_diag_mhd(const Geo &g, const double *VDX, int kx, const double
 int insul_residual_blocks(const Geo &g);
 hipError_t launch_insul_residual(const Geo &g, const double2 *A, size_t cstride, double *blockmax, hipStream_t s);
 
+// Largest |value| of nf physical fields on the unpadded grid (g: the
+// unpadded geometry; Y holds their y-transformed lines, [nf][n0][ny][kzs]):
+// per-block maxima in blockmax[absmax_blocks][nf].  Fields 0..2 are the
+// velocity (field 1 gets the base flow U x), 3..5 the magnetic field (+ B0).
+template <typename T>
+hipError_t launch_absmax(const Geo &g, const T *xnode, const T *twz, const FFTPlan &pz, int nf, const void *Y,
+                         double *blockmax, hipStream_t s);
+int absmax_blocks(const Geo &g, int nf, int elem_bytes);
+bool absmax_fits(const Geo &g, const FFTPlan &pz, int nf, int elem_bytes);
+
 // Largest butterfly count per thread any FFT launch needs (host-side check
 // against the kernels' register budgets).
 bool fft_fits(const Geo &g, const FFTPlan &py, const FFTPlan &pz, int elem_bytes, bool mhd);

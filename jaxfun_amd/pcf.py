@@ -402,6 +402,39 @@ class PlaneCouetteFluctuationHIP:
                 "mean_shear")
         return {k: float(v) for k, v in zip(keys, out)}
 
+    def field_maxima(self, state: KMMState | None = None):
+        """Largest |u_i + U_i| (total velocity, ``total_velocity_physical``,
+        ``pcf_fluctuations_jax.py:114-116``) and |b_i + B0_i| (MHD; zeros
+        otherwise) on the unpadded physical grid, reduced on the device."""
+        if state is not None:
+            self._ensure_loaded(state)
+        out = np.zeros(6)
+        nat.check(self._lib.pcf_field_maxima(self._h, out.ctypes.data_as(C.POINTER(C.c_double))))
+        return tuple(float(v) for v in out[:3]), tuple(float(v) for v in out[3:])
+
+    def cfl_scalars(self, state: KMMState | None = None) -> dict:
+        """The CFL numbers of the reference's health scalars (``_cfl_scalars``,
+        ``production/health.py:120-153``, with the spacings and rates of
+        ``curl_health_scalars``, ``:156-180``): per-axis advective and Alfven
+        numbers, the diffusive number and the rotation / shear-source rates;
+        ``cfl_total`` is what ``production/adaptive.py:115-175`` reads."""
+        u_max, b_max = self.field_maxima(state)
+        n0 = self.N[0]
+        x = np.sort(np.cos((2 * np.arange(n0) + 1) * np.pi / (2 * n0)))
+        spacings = (float(np.diff(x).min()), self.Ly / self.N[1], self.Lz / self.N[2])
+        dt = self.dt
+        out = {}
+        worst = 0.0
+        for axis, label in enumerate("xyz"):
+            out["cfl_advective_" + label] = u_max[axis] * dt / spacings[axis]
+            out["cfl_alfven_" + label] = b_max[axis] * dt / spacings[axis]
+            worst = max(worst, out["cfl_advective_" + label] + out["cfl_alfven_" + label])
+        out["cfl_diffusive"] = max(self.nu, getattr(self, "eta", 0.0)) * dt / min(spacings) ** 2
+        out["cfl_rotation"] = 2.0 * abs(getattr(self, "omega", 0.0)) * dt
+        out["cfl_shear_source"] = abs(getattr(self, "shear_rate", 0.0)) * dt
+        out["cfl_total"] = max(worst, out["cfl_rotation"], out["cfl_shear_source"])
+        return out
+
     # ------------------------------------------------------------ reverse mode
     def vjp(self, state: KMMState, steps: int, cotangent=None, checkpoint_every: int = 20):
         """Run ``steps`` steps from ``state`` and pull a cotangent back to it.

@@ -296,6 +296,26 @@ def _lu_solve(fac: tuple[np.ndarray, int], b: np.ndarray) -> np.ndarray:
     return out[..., 0] if squeeze else out
 
 
+def cfl_scalars(u_max, b_max, spacings, dt, nu, eta, rotation_rate, shear_rate) -> dict:
+    """``_cfl_scalars`` (``production/health.py:120-153``): advective and Alfven
+    CFL per axis, their largest directional sum, the diffusive number
+    max(nu, eta) dt / min(h)^2, the rotation and shear-source rates, and
+    ``cfl_total`` over the explicit terms."""
+    out = {}
+    worst = 0.0
+    for axis, label in enumerate("xyz"):
+        adv = u_max[axis] * dt / spacings[axis]
+        alf = b_max[axis] * dt / spacings[axis]
+        out["cfl_advective_" + label] = adv
+        out["cfl_alfven_" + label] = alf
+        worst = max(worst, adv + alf)
+    out["cfl_diffusive"] = max(nu, eta) * dt / min(spacings) ** 2
+    out["cfl_rotation"] = 2.0 * abs(rotation_rate) * dt
+    out["cfl_shear_source"] = abs(shear_rate) * dt
+    out["cfl_total"] = max(worst, out["cfl_rotation"], out["cfl_shear_source"])
+    return out
+
+
 def variable_ab2_extrapolate(current, previous, have_old, dt, previous_dt):
     """``variable_ab2_extrapolate`` (``src/jaxfun/integrators/cnab2.py:211-239``):
     (1 + r/2) current - (r/2) previous with r = dt / previous_dt when a previous
@@ -532,6 +552,26 @@ class PCFOracle:
         b = (w0, w1, w2 + self.dUb)
         cross = (a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0])
         return tuple(-c for c in cross), up
+
+    def grid_spacings(self):
+        """(dx, Ly/ny, Lz/nz) of ``curl_health_scalars`` (``production/health.py:165-172``):
+        dx the smallest gap between Chebyshev points."""
+        x1d = np.sort(np.unique(self.x))
+        return (float(np.diff(x1d).min()), self.L1 / self.N[1], self.L2 / self.N[2])
+
+    def field_maxima(self, s: OracleState):
+        """max |.| of the total velocity (``total_velocity_physical``,
+        ``pcf_fluctuations_jax.py:114-116``) on the unpadded grid per component,
+        and of the magnetic field (none here)."""
+        u0, u1, u2 = s.u
+        up = (self.tp_backward(u0, self.SB, False), self.tp_backward(u1, self.SD, False),
+              self.tp_backward(u2, self.SD, False))
+        ut = (up[0], up[1] + self.dUb * self.xb, up[2])
+        return tuple(float(np.max(np.abs(c))) for c in ut), (0.0, 0.0, 0.0)
+
+    def cfl(self, s: OracleState, eta: float = 0.0, rotation_rate: float = 0.0, shear_rate: float = 0.0) -> dict:
+        u_max, b_max = self.field_maxima(s)
+        return cfl_scalars(u_max, b_max, self.grid_spacings(), self.dt, self.nu, eta, rotation_rate, shear_rate)
 
     def convection(self, s: OracleState):
         """Rotational convection H = -(u_tot x w_tot) (``channelflow_kmm.py:772-782``)."""

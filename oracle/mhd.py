@@ -198,6 +198,21 @@ class MHDOracle(PCFOracle):
         emf = (ut[1] * bp[2] - ut[2] * bp[1], ut[2] * bp[0] - ut[0] * bp[2], ut[0] * bp[1] - ut[1] * bp[0])
         return H, self.forward_emf(emf)
 
+    def field_maxima(self, s):
+        """Total velocity of the flow and total magnetic field b + B0
+        (``_total_B_physical(B, padded=False)``, ``pcf_mhd_mri_shearpy_jax.py:149-151``)
+        on the unpadded grid, max |.| per component."""
+        u_max, _ = super().field_maxima(s.flow)
+        bp = self.backward_B(self.B_from_A(s.A), padded=False)
+        return u_max, tuple(float(np.max(np.abs(b + b0))) for b, b0 in zip(bp, self.background_b))
+
+    def cfl(self, s, eta=None, rotation_rate=None, shear_rate=None) -> dict:
+        from .kmm import cfl_scalars
+
+        u_max, b_max = self.field_maxima(s)
+        return cfl_scalars(u_max, b_max, self.grid_spacings(), self.dt, self.nu, self.eta,
+                           getattr(self, "omega", 0.0), getattr(self, "shear_rate", 0.0))
+
     def nonlinear_A(self, HA):
         """``_A_mass_rhs`` of HA (``pcf_mhd_jax.py:157-159``)."""
         return self.A_mass(HA)

@@ -317,3 +317,50 @@ def test_oracle_reproduces_published_golden(case):
     cs = o.checksum(s)
     assert cs["element_count"] == case["checksum"]["element_count"]
     assert math.isclose(cs["l2_norm"], case["checksum"]["l2_norm"], rel_tol=1e-11)
+
+
+# ---------------------------------------------------------------- CFL numbers
+def test_cfl_scalars_known_answer():
+    """_cfl_scalars (production/health.py:120-153) on hand-computed numbers:
+    per-axis u dt / h and b dt / h, the largest directional sum, max(nu, eta)
+    dt / min(h)^2, 2|Omega| dt, |S| dt, and the total over the explicit terms."""
+    from oracle.kmm import cfl_scalars
+
+    c = cfl_scalars((1.0, 2.0, 3.0), (0.5, 0.0, 1.0), (0.1, 0.2, 0.3), 0.01, 1e-3, 2e-3, 1.5, -2.0)
+    want = {"cfl_advective_x": 0.1, "cfl_alfven_x": 0.05, "cfl_advective_y": 0.1, "cfl_alfven_y": 0.0,
+            "cfl_advective_z": 0.1, "cfl_alfven_z": 0.01 / 0.3, "cfl_diffusive": 2e-3 * 0.01 / 0.01,
+            "cfl_rotation": 0.03, "cfl_shear_source": 0.02, "cfl_total": 0.15}
+    assert set(c) == set(want)
+    for k, v in want.items():
+        assert math.isclose(c[k], v, rel_tol=1e-14, abs_tol=1e-16), (k, c[k], v)
+    # rotation-limited: 2 |Omega| dt above every directional sum
+    assert math.isclose(cfl_scalars((1.0, 2.0, 3.0), (0.5, 0.0, 1.0), (0.1, 0.2, 0.3), 0.01, 1e-3, 0.0, 10.0,
+                                    0.0)["cfl_total"], 0.2, rel_tol=1e-14)
+
+
+def test_field_maxima_match_direct_series_sum():
+    """The unpadded physical maxima against a direct evaluation of the
+    Chebyshev x Fourier series at every grid point, base flow included."""
+    o = PCFOracle(N=(9, 4, 6), Re=200.0, dt=1e-3, perturbation_amplitude=0.3)
+    s = o.initial_state()
+    n0, n1, n2 = o.N
+    x = cheb_nodes(n0)
+    y = np.arange(n1) * o.L1 / n1
+    z = np.arange(n2) * o.L2 / n2
+    ky = np.fft.fftfreq(n1, 1.0 / n1) * 2 * np.pi / o.L1
+    kz = np.fft.fftfreq(n2, 1.0 / n2) * 2 * np.pi / o.L2
+    Ey = np.exp(1j * np.outer(y, ky))  # [j, p]
+    Ez = np.exp(1j * np.outer(z, kz))  # [k, q]
+    V = npcheb.chebvander(x, n0 - 1)   # [i, b]
+    want = []
+    for c, S in zip(s.u, (o.SB, o.SD, o.SD)):
+        phi = V @ S.T  # [i, a]: the Shen basis at the points
+        u = np.einsum("ia,apq,jp,kq->ijk", phi, c, Ey, Ez)
+        want.append(u)
+    want[1] = want[1] + o.dUb * x[:, None, None]
+    got, bmax = o.field_maxima(s)
+    for g, w in zip(got, want):
+        assert math.isclose(g, float(np.max(np.abs(w))), rel_tol=1e-12), (g, float(np.max(np.abs(w))))
+    assert bmax == (0.0, 0.0, 0.0)
+    dx = float(np.min(np.diff(np.sort(x))))
+    assert o.grid_spacings() == (dx, o.L1 / n1, o.L2 / n2)
