@@ -98,21 +98,37 @@ __device__ void factor_block(const StructSys &ss, int n, double s0, double s2, d
 // read into registers first (independent loads, one wait), then the serial
 // chain runs on registers only, so a row costs its few dependent FMAs rather
 // than a load latency.
-template <int P, int Q, int R, class Load, class Keep, class Kept, class Store>
+// The value type V is double2 (one thread per complex right-hand side) or
+// double (one thread per real or imaginary part: the factors are real, so the
+// two parts never mix and half the instructions per thread carry the chain).
+__device__ __forceinline__ void fms(double &acc, double a, double b) { acc -= a * b; }
+__device__ __forceinline__ void fms(double2 &acc, double a, double2 b) {
+  acc.x -= a * b.x;
+  acc.y -= a * b.y;
+}
+__device__ __forceinline__ void fma_to(double &acc, double a, double b) { acc += a * b; }
+__device__ __forceinline__ void fma_to(double2 &acc, double a, double2 b) {
+  acc.x += a * b.x;
+  acc.y += a * b.y;
+}
+__device__ __forceinline__ double scaled(double v, double f) { return v * f; }
+__device__ __forceinline__ double2 scaled(double2 v, double f) { return double2{v.x * f, v.y * f}; }
+
+template <int P, int Q, int R, class V, class Load, class Keep, class Kept, class Store>
 __device__ __forceinline__ void substitute(const StructSys &ss, int n, const double *F, size_t rs, size_t vs,
                                            Load load, Keep keep, Kept kept, Store store) {
   constexpr int F1 = Q + 1, SC = 8;
   {
-    double2 yw[P + 1];
+    V yw[P + 1];
 #pragma unroll
-    for (int t = 0; t <= P; ++t) yw[t] = double2{0, 0};
+    for (int t = 0; t <= P; ++t) yw[t] = V{};
     for (int i0 = 0; i0 < n; i0 += SC) {
-      double2 r[SC];
+      V r[SC];
       double l[SC][P + 1];
 #pragma unroll
       for (int c = 0; c < SC; ++c) {
         const int i = i0 + c;
-        r[c] = double2{0, 0};
+        r[c] = V{};
 #pragma unroll
         for (int t = 1; t <= P; ++t) l[c][t] = 0.0;
         if (i < n) {
@@ -125,12 +141,10 @@ __device__ __forceinline__ void substitute(const StructSys &ss, int n, const dou
       for (int c = 0; c < SC; ++c) {
         const int i = i0 + c;
         if (i < n) {
-          double2 y = r[c];
+          // the newest row (t = 1) last: one dependent FMA per row on the chain
+          V y = r[c];
 #pragma unroll
-          for (int t = 1; t <= P; ++t) {
-            y.x -= l[c][t] * yw[t].x;
-            y.y -= l[c][t] * yw[t].y;
-          }
+          for (int t = P; t >= 1; --t) fms(y, l[c][t], yw[t]);
           keep(i, y);
 #pragma unroll
           for (int t = P; t >= 2; --t) yw[t] = yw[t - 1];
@@ -140,19 +154,19 @@ __device__ __forceinline__ void substitute(const StructSys &ss, int n, const dou
     }
   }
   {
-    double2 xw[F1 + 1];
+    V xw[F1 + 1];
 #pragma unroll
-    for (int k = 0; k <= F1; ++k) xw[k] = double2{0, 0};
-    double2 tau[R];
+    for (int k = 0; k <= F1; ++k) xw[k] = V{};
+    V tau[R];
 #pragma unroll
-    for (int r = 0; r < R; ++r) tau[r] = double2{0, 0};
+    for (int r = 0; r < R; ++r) tau[r] = V{};
     for (int i1 = n - 1; i1 >= 0; i1 -= SC) {
-      double2 y[SC];
+      V y[SC];
       double f[SC][1 + Q + R], v[SC][R];
 #pragma unroll
       for (int c = 0; c < SC; ++c) {
         const int i = i1 - c;
-        y[c] = double2{0, 0};
+        y[c] = V{};
 #pragma unroll
         for (int w = 0; w < 1 + Q + R; ++w) f[c][w] = 0.0;
 #pragma unroll
@@ -173,22 +187,15 @@ __device__ __forceinline__ void substitute(const StructSys &ss, int n, const dou
         if (i >= 0) {
           // tau_r = sum_{j > i + Q} V[j][r] x_j; x_{i+F1} joins it at row i
 #pragma unroll
-          for (int r = 0; r < R; ++r) {
-            tau[r].x += v[c][r] * xw[F1].x;
-            tau[r].y += v[c][r] * xw[F1].y;
-          }
-          double2 acc = y[c];
+          for (int r = 0; r < R; ++r) fma_to(tau[r], v[c][r], xw[F1]);
+          // far part and older rows first, the newest row (q = 1) last: the
+          // chain through x_{i+1} is one FMA and the pivot product per row
+          V acc = y[c];
 #pragma unroll
-          for (int q = 1; q <= Q; ++q) {
-            acc.x -= f[c][q] * xw[q].x;
-            acc.y -= f[c][q] * xw[q].y;
-          }
+          for (int r = 0; r < R; ++r) fms(acc, f[c][1 + Q + r], tau[r]);
 #pragma unroll
-          for (int r = 0; r < R; ++r) {
-            acc.x -= f[c][1 + Q + r] * tau[r].x;
-            acc.y -= f[c][1 + Q + r] * tau[r].y;
-          }
-          const double2 xi{acc.x * f[c][0], acc.y * f[c][0]};
+          for (int q = Q; q >= 1; --q) fms(acc, f[c][q], xw[q]);
+          const V xi = scaled(acc, f[c][0]);
           store(i, xi);
 #pragma unroll
           for (int k = F1; k >= 2; --k) xw[k] = xw[k - 1];
@@ -263,6 +270,7 @@ __global__ void __launch_bounds__(256) k_factor(Geo g, StructMats sm, double gam
 template <int SL>
 __global__ void __launch_bounds__(256) k_solve_factored(Geo g, StructMats sm, FactorSet fs, SolveArgs a,
                                                         int spb0, size_t cstride) {
+  static_assert(2 * SL <= 64, "the substitution threads of the modes fill at most the first wave");
   extern __shared__ __align__(16) unsigned char smem[];
   constexpr int SLP = SL + 1;
   const int M = g.m;
@@ -367,55 +375,46 @@ __global__ void __launch_bounds__(256) k_solve_factored(Geo g, StructMats sm, Fa
     ZN(j, 3, lane) = ry;
   }
   __syncthreads();
-  // phase 2: substitutions, one thread per lane (thread 64, a second wave, takes the mean)
-  int lane;
+  // phase 2: substitutions, one thread per (lane, real/imaginary part):
+  // threads [0, 2 SL) of the first wave; threads 64, 65 (a second wave) take
+  // the mean system (parts v, w)
+  int lane, part;
   bool mean = false;
-  if (threadIdx.x < SL) {
-    lane = threadIdx.x;
-  } else if (threadIdx.x == 64 && has_mean) {
+  if (threadIdx.x < 2 * SL) {
+    lane = threadIdx.x % SL;
+    part = threadIdx.x / SL;
+  } else if (threadIdx.x >= 64 && threadIdx.x < 66 && has_mean) {
     lane = SL;
+    part = threadIdx.x - 64;
     mean = true;
   } else {
     return;
   }
   const int m = mean ? 0 : m0 + lane;
   if (!mean && m >= M) return;
-  auto load = [&](int i) { return double2{ZN(i, 2, lane), ZN(i, 3, lane)}; };
-  auto keep = [&](int i, double2 y) {
-    ZN(i, 2, lane) = y.x;
-    ZN(i, 3, lane) = y.y;
-  };
+  auto load = [&](int i) { return ZN(i, 2 + part, lane); };
+  auto keep = [&](int i, double y) { ZN(i, 2 + part, lane) = y; };
   const double *F = fl + lane;
   const double sub = a.sub;  // output x - sub x_0 (level 0 of the state)
   if (sys == 0) {
-    substitute<2, 2, 2>(ss, n, F, (size_t)NF_B * SLP, (size_t)SLP, load, keep, load, [&](int i, double2 x) {
-      const size_t o = (size_t)(2 * i + par) * M + m;
-      if (sub != 0.0) {
-        const double2 p = a.u0[0][o];
-        x.x -= sub * p.x;
-        x.y -= sub * p.y;
-      }
-      a.u0_out[o] = x;
+    substitute<2, 2, 2, double>(ss, n, F, (size_t)NF_B * SLP, (size_t)SLP, load, keep, load, [&](int i, double x) {
+      const size_t o = 2 * ((size_t)(2 * i + par) * M + m) + part;
+      if (sub != 0.0) x -= sub * reinterpret_cast<const double *>(a.u0[0])[o];
+      reinterpret_cast<double *>(a.u0_out)[o] = x;
     });
   } else if (mean) {
-    substitute<1, 1, 1>(ss, n, F, (size_t)NF_D * SLP, (size_t)SLP, load, keep, load, [&](int i, double2 x) {
+    const double *x0 = part ? a.w[0] : a.v[0];
+    double *out = part ? a.w_out : a.v_out;
+    substitute<1, 1, 1, double>(ss, n, F, (size_t)NF_D * SLP, (size_t)SLP, load, keep, load, [&](int i, double x) {
       const int J = 2 * i + par;
-      if (sub != 0.0) {
-        x.x -= sub * a.v[0][J];
-        x.y -= sub * a.w[0][J];
-      }
-      a.v_out[J] = x.x;
-      a.w_out[J] = x.y;
+      if (sub != 0.0) x -= sub * x0[J];
+      out[J] = x;
     });
   } else {
-    substitute<1, 1, 1>(ss, n, F, (size_t)NF_D * SLP, (size_t)SLP, load, keep, load, [&](int i, double2 x) {
-      const size_t o = (size_t)(2 * i + par) * M + m;
-      if (sub != 0.0) {
-        const double2 p = a.g[0][o];
-        x.x -= sub * p.x;
-        x.y -= sub * p.y;
-      }
-      a.g_out[o] = x;
+    substitute<1, 1, 1, double>(ss, n, F, (size_t)NF_D * SLP, (size_t)SLP, load, keep, load, [&](int i, double x) {
+      const size_t o = 2 * ((size_t)(2 * i + par) * M + m) + part;
+      if (sub != 0.0) x -= sub * reinterpret_cast<const double *>(a.g[0])[o];
+      reinterpret_cast<double *>(a.g_out)[o] = x;
     });
   }
 }
@@ -484,9 +483,9 @@ __global__ void __launch_bounds__(256) k_adj_solve_factored(Geo g, StructMats sm
     };
     const double *F = fl + lane;
     if (sys == 0)
-      substitute<2, 2, 2>(ss, n, F, (size_t)NF_B * SLP, (size_t)SLP, rw, wr, rw, wr);
+      substitute<2, 2, 2, double2>(ss, n, F, (size_t)NF_B * SLP, (size_t)SLP, rw, wr, rw, wr);
     else
-      substitute<1, 1, 1>(ss, n, F, (size_t)NF_D * SLP, (size_t)SLP, rw, wr, rw, wr);
+      substitute<1, 1, 1, double2>(ss, n, F, (size_t)NF_D * SLP, (size_t)SLP, rw, wr, rw, wr);
   }
   __syncthreads();
   for (int idx = threadIdx.x; idx < n * SLP; idx += blockDim.x) {
