@@ -72,6 +72,7 @@ bool make_plan(int n, pcf::FFTPlan &p) {
   p.n = n;
   p.nst = 0;
   int m = n;
+  while (m % 8 == 0) p.radix[p.nst++] = 8, m /= 8;
   while (m % 4 == 0) p.radix[p.nst++] = 4, m /= 4;
   while (m % 2 == 0) p.radix[p.nst++] = 2, m /= 2;
   while (m % 3 == 0) p.radix[p.nst++] = 3, m /= 3;

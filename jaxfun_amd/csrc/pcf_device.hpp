@@ -61,6 +61,24 @@ __device__ __forceinline__ void dft_small(V *a) {
     a[2] = csub(s02, s13);
     a[1] = cadd(d02, jd13);
     a[3] = csub(d02, jd13);
+  } else if constexpr (R == 8) {
+    // two 4-point DFTs of the even and odd samples joined by w^k, w = exp(-+2 pi i / 8)
+    V e[4] = {a[0], a[2], a[4], a[6]}, o[4] = {a[1], a[3], a[5], a[7]};
+    dft_small<V, T, 4, INV>(e);
+    dft_small<V, T, 4, INV>(o);
+    const T h = (T)0.70710678118654752440084436210485;  // 1 / sqrt(2)
+    const V w1 = INV ? V{(o[1].x - o[1].y) * h, (o[1].x + o[1].y) * h} : V{(o[1].x + o[1].y) * h, (o[1].y - o[1].x) * h};
+    const V w2 = INV ? V{-o[2].y, o[2].x} : V{o[2].y, -o[2].x};
+    const V w3 = INV ? V{-(o[3].x + o[3].y) * h, (o[3].x - o[3].y) * h}
+                     : V{(o[3].y - o[3].x) * h, -(o[3].x + o[3].y) * h};
+    a[0] = cadd(e[0], o[0]);
+    a[4] = csub(e[0], o[0]);
+    a[1] = cadd(e[1], w1);
+    a[5] = csub(e[1], w1);
+    a[2] = cadd(e[2], w2);
+    a[6] = csub(e[2], w2);
+    a[3] = cadd(e[3], w3);
+    a[7] = csub(e[3], w3);
   } else if constexpr (R == 3) {
     const T c = (T)-0.5;
     const T s = INV ? (T)0.86602540378443864676372317075294 : (T)-0.86602540378443864676372317075294;
@@ -123,7 +141,9 @@ __device__ __forceinline__ void fft_lds(V *buf, const V *tw, const FFTPlan &plan
   int p = 1;
   for (int s = 0; s < plan.nst; ++s) {
     const int r = plan.radix[s];
-    if (r == 4)
+    if (r == 8)
+      fft_pass<V, T, 8, INV, BATCH_FAST, MAXB>(buf, tw, plan.n, p, nbatch, bstride, estride, tid, nthr);
+    else if (r == 4)
       fft_pass<V, T, 4, INV, BATCH_FAST, MAXB>(buf, tw, plan.n, p, nbatch, bstride, estride, tid, nthr);
     else if (r == 2)
       fft_pass<V, T, 2, INV, BATCH_FAST, MAXB>(buf, tw, plan.n, p, nbatch, bstride, estride, tid, nthr);
@@ -140,9 +160,11 @@ struct CPlan {
   int nst;
   int r[12];
 };
-__host__ __device__ constexpr CPlan cplan(int n) {
+// R8: radix-8 passes first (fewer LDS round trips, more registers per thread).
+__host__ __device__ constexpr CPlan cplan(int n, bool r8 = true) {
   CPlan p{0, {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0}};
   int m = n;
+  while (r8 && m % 8 == 0) p.r[p.nst++] = 8, m /= 8;
   while (m % 4 == 0) p.r[p.nst++] = 4, m /= 4;
   while (m % 2 == 0) p.r[p.nst++] = 2, m /= 2;
   while (m % 3 == 0) p.r[p.nst++] = 3, m /= 3;
@@ -189,21 +211,22 @@ __device__ __forceinline__ void fft_pass_c(V *buf, const V *tw, int nbatch, int 
   __syncthreads();
 }
 
-template <typename V, typename T, int N, int S, int P, bool INV, bool BATCH_FAST, int MAXB>
+template <typename V, typename T, int N, int S, int P, bool INV, bool BATCH_FAST, int MAXB, bool R8>
 __device__ __forceinline__ void fft_stages_c(V *buf, const V *tw, int nbatch, int bstride, int estride,
                                              int tid, int nthr) {
-  constexpr CPlan cp = cplan(N);
+  constexpr CPlan cp = cplan(N, R8);
   if constexpr (S < cp.nst) {
     constexpr int R = cp.r[S];
     fft_pass_c<V, T, N, R, P, INV, BATCH_FAST, MAXB>(buf, tw, nbatch, bstride, estride, tid, nthr);
-    fft_stages_c<V, T, N, S + 1, P * R, INV, BATCH_FAST, MAXB>(buf, tw, nbatch, bstride, estride, tid, nthr);
+    fft_stages_c<V, T, N, S + 1, P * R, INV, BATCH_FAST, MAXB, R8>(buf, tw, nbatch, bstride, estride, tid,
+                                                                   nthr);
   }
 }
 
-template <typename V, typename T, int N, bool INV, bool BATCH_FAST, int MAXB>
+template <typename V, typename T, int N, bool INV, bool BATCH_FAST, int MAXB, bool R8 = true>
 __device__ __forceinline__ void fft_lds_c(V *buf, const V *tw, int nbatch, int bstride, int estride) {
-  fft_stages_c<V, T, N, 0, 1, INV, BATCH_FAST, MAXB>(buf, tw, nbatch, bstride, estride, threadIdx.x,
-                                                      blockDim.x);
+  fft_stages_c<V, T, N, 0, 1, INV, BATCH_FAST, MAXB, R8>(buf, tw, nbatch, bstride, estride, threadIdx.x,
+                                                          blockDim.x);
 }
 
 // Block-cooperative gather: element idx < total of the block's staging set is

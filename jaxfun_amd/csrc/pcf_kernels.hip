@@ -138,8 +138,10 @@ __global__ void __launch_bounds__(256) k_rotational(Geo g, NodalMats<T> nm, FFTP
       },
       [&](int idx, const V &z) { buf[idx] = z; });
   __syncthreads();
+  // radix-8 passes in FP32 only: in FP64 their registers cost more occupancy
+  // than the saved LDS pass (compile-time lengths)
   if constexpr (NZC > 0)
-    fft_lds_c<V, T, NZC, true, false, MAXB>(buf, tw, UPB * 6, MZ, 1);
+    fft_lds_c<V, T, NZC, true, false, MAXB, sizeof(T) == 4>(buf, tw, UPB * 6, MZ, 1);
   else
     fft_lds<V, T, true, false, MAXB>(buf, tw, plan, UPB * 6, MZ, 1);
 
@@ -171,7 +173,7 @@ __global__ void __launch_bounds__(256) k_rotational(Geo g, NodalMats<T> nm, FFTP
   }
   __syncthreads();
   if constexpr (NZC > 0)
-    fft_lds_c<V, T, NZC, false, false, MAXB>(out, tw, UPB * 3, MZ, 1);
+    fft_lds_c<V, T, NZC, false, false, MAXB, sizeof(T) == 4>(out, tw, UPB * 3, MZ, 1);
   else
     fft_lds<V, T, false, false, MAXB>(out, tw, plan, UPB * 3, MZ, 1);
 
@@ -790,6 +792,22 @@ int max_butterflies(const FFTPlan &plan, int nbatch) {
   return mb;
 }
 
+// The same length without radix-8 passes (the FP64 k_rotational's
+// compile-time plans); launch bounds cover both plans.
+FFTPlan radix4_plan(const FFTPlan &p) {
+  FFTPlan q{};
+  q.n = p.n;
+  int m = p.n;
+  while (m % 4 == 0) q.radix[q.nst++] = 4, m /= 4;
+  while (m % 2 == 0) q.radix[q.nst++] = 2, m /= 2;
+  while (m % 3 == 0) q.radix[q.nst++] = 3, m /= 3;
+  return q;
+}
+
+int rot_butterflies(const FFTPlan &pz, int nbatch) {
+  return std::max(max_butterflies(pz, nbatch), max_butterflies(radix4_plan(pz), nbatch));
+}
+
 template <typename T>
 hipError_t launch_fft_y_fields(const Geo &g, const NodalMats<T> &nm, const FFTPlan &py, bool inverse,
                                int nf, const void *src_, void *dst_, hipStream_t s) {
@@ -856,7 +874,7 @@ hipError_t launch_rotational(const Geo &g, const NodalMats<T> &nm, Workspace &ws
   const int upb = rot_units<T>(g, mhd);
   const int nunits = g.n0 * (g.my / 2);
   const size_t lds = ((size_t)upb * (mhd ? 18 : 9) * g.mz + g.mz) * sizeof(V);
-  const int mb = max_butterflies(pz, upb * (mhd ? 12 : 6));
+  const int mb = rot_butterflies(pz, upb * (mhd ? 12 : 6));
   const dim3 grid((nunits + upb - 1) / upb);
   const V *Y = reinterpret_cast<const V *>(ws.Y);
   V *Hz = reinterpret_cast<V *>(ws.Hz);
@@ -974,7 +992,7 @@ bool fft_fits(const Geo &g, const FFTPlan &py, const FFTPlan &pz, int elem_bytes
   if (max_butterflies(py, kc) > 8) return false;
   const int upb = (elem_bytes == 16) ? rot_units<double>(g, mhd) : rot_units<float>(g, mhd);
   if (((size_t)upb * (mhd ? 18 : 9) * g.mz + g.mz) * elem_bytes > 65536) return false;
-  return max_butterflies(pz, upb * (mhd ? 12 : 6)) <= 12;
+  return rot_butterflies(pz, upb * (mhd ? 12 : 6)) <= 12;
 }
 
 #define PCF_INSTANTIATE(T)                                                                     \
