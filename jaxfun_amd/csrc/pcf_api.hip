@@ -148,7 +148,7 @@ struct pcf_handle {
   pcf::FactorSet fars{};      // LU factors of M - g dt L for the ARS(2,2,2) stages
   pcf::FactorSet fstage[3]{}; // LU factors at the three IMEXRK3 stage gammas
   pcf::FactorSet fcn{};       // LU factors of M - (dt/2) L (CNAB2)
-  double *Rt = nullptr;       // parity-compressed reconstruction rows, [nd][ldr]
+  double *Rt = nullptr;       // R^T of the reconstruction, [round4(nb)][ldr]
   int ldr = 0;
   double *dVB = nullptr, *dVD = nullptr, *dVB1 = nullptr, *dVD1 = nullptr, *dfejer = nullptr,
          *dxn = nullptr, *ddiag = nullptr;
@@ -309,14 +309,13 @@ int32_t build_struct_mats(pcf_handle *h) {
       ss.V = d;
     }
   h->geo.npmax = npmax;
-  // parity-compressed reconstruction rows: Rc[i][k] = R[i][2k + 1 - (i & 1)]
-  h->ldr = (o.nb + 2) / 2;
-  std::vector<double> Rc((size_t)o.nd * h->ldr, 0.0);
-  for (int i = 0; i < o.nd; ++i) {
-    const int j0 = 1 - (i & 1);
-    for (int k = 0; j0 + 2 * k < o.nb; ++k) Rc[(size_t)i * h->ldr + k] = o.R(i, j0 + 2 * k);
-  }
-  if ((r = h->upload(&h->Rt, Rc))) return r;
+  // R^T for the matrix-core reconstruction: [K][ldr], K = nb rows padded with
+  // zeros past round4(nb), ldr = nd rounded up to a multiple of 16
+  h->ldr = (o.nd + 15) / 16 * 16;
+  std::vector<double> Rt((size_t)((o.nb + 3) / 4 * 4) * h->ldr, 0.0);
+  for (int i = 0; i < o.nd; ++i)
+    for (int j = 0; j < o.nb; ++j) Rt[(size_t)j * h->ldr + i] = o.R(i, j);
+  if ((r = h->upload(&h->Rt, Rt))) return r;
   return PCF_OK;
 }
 

@@ -19,7 +19,7 @@
 //   k_solve_factored   (pcf_factored.hip) right-hand side and FP64 per-mode
 //                      Helmholtz / biharmonic / mean-mode solves on stored
 //                      factors (sbdf3.py:44-62, tpmatrix.py:686-799)
-//   k_reconstruct      velocity reconstruction (channelflow_kmm.py:855-870)
+//   k_recon_mfma       (pcf_mfma.hip) velocity reconstruction (channelflow_kmm.py:855-870)
 //   k_solve_bordered   insulating-wall A_y / A_z update
 #include <algorithm>
 
@@ -539,50 +539,6 @@ __global__ void __launch_bounds__(128) k_solve_bordered(Geo g, BorderedArgs a) {
   }
 }
 
-// Velocity reconstruction (channelflow_kmm.py:855-870): f = R u0 with
-// R = P_D d/dx on the clamped basis, u1 = i (ky f + kz g) / k^2,
-// u2 = i (kz f - ky g) / k^2, mean mode (0,0) <- (v00, w00).  R couples
-// only opposite parities (d/dx flips parity), so row i reads u0 rows
-// j = 1 - (i & 1), 3 - (i & 1), ...; Rc holds those entries, [i][k] for
-// j = 2k + 1 - (i & 1).  One thread per (row, mode), lanes over modes.
-__global__ void __launch_bounds__(256) k_reconstruct(Geo g, const double *__restrict__ Rc, int ldk,
-                                                     const double2 *__restrict__ u0,
-                                                     const double2 *__restrict__ gg,
-                                                     const double *__restrict__ v,
-                                                     const double *__restrict__ w,
-                                                     double2 *__restrict__ u1,
-                                                     double2 *__restrict__ u2) {
-  const int m = blockIdx.x * blockDim.x + threadIdx.x;
-  const int i = blockIdx.y;
-  if (m >= g.m) return;
-  const size_t o = (size_t)i * g.m + m;
-  if (m == 0) {
-    u1[o] = double2{v[i], 0.0};
-    u2[o] = double2{w[i], 0.0};
-    return;
-  }
-  const int j0 = 1 - (i & 1);
-  const int nk = (g.nb - j0 + 1) / 2;
-  const double *r = Rc + (size_t)i * ldk;
-  double2 f{0, 0};
-#pragma unroll 8
-  for (int k = 0; k < nk; ++k) {
-    const double2 c = u0[(size_t)(j0 + 2 * k) * g.m + m];
-    f.x += r[k] * c.x;
-    f.y += r[k] * c.y;
-  }
-  double kyd, kzd, kyo;
-  mode_k(g, m, kyd, kzd, kyo);
-  const double k2 = kyd * kyd + kzd * kzd;
-  const double den = (k2 == 0.0) ? 1.0 : k2;
-  const double ky = kyd / den, kz = kzd / den;
-  const double2 gi = gg[o];
-  const double2 s1{ky * f.x + kz * gi.x, ky * f.y + kz * gi.y};
-  const double2 s2{kz * f.x - ky * gi.x, kz * f.y - ky * gi.y};
-  u1[o] = double2{-s1.y, s1.x};
-  u2[o] = double2{-s2.y, s2.x};
-}
-
 // -------------------------------------------------------------- k_diag ----
 __global__ void __launch_bounds__(256) k_diag(Geo g, const double *__restrict__ VB,
                                               const double *__restrict__ VD,
@@ -952,14 +908,6 @@ hipError_t launch_solve_bordered(const Geo &g, const BorderedArgs &a, hipStream_
     hipLaunchKernelGGL((k_solve_bordered<4>), dim3((g.m + 3) / 4), dim3(128), 4 * per_mode, s, g, a);
   else
     return hipErrorInvalidValue;
-  return hipGetLastError();
-}
-
-hipError_t launch_reconstruct(const Geo &g, const double *Rc, int ldk, const SolveArgs &a,
-                              hipStream_t s) {
-  dim3 grid((g.m + 255) / 256, g.nd);
-  hipLaunchKernelGGL(k_reconstruct, grid, dim3(256), 0, s, g, Rc, ldk, a.u0_out, a.g_out, a.v_out,
-                     a.w_out, a.u1, a.u2);
   return hipGetLastError();
 }
 

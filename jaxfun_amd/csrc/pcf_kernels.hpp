@@ -165,8 +165,10 @@ struct BorderedArgs {
   size_t cstride;
 };
 hipError_t launch_solve_bordered(const Geo &g, const BorderedArgs &a, hipStream_t s);
-hipError_t launch_reconstruct(const Geo &g, const double *Rc, int ldk, const SolveArgs &a,
-                              hipStream_t s);
+// u1, u2 of the solved state (u0_out, g_out, v_out, w_out of a) into a.u1, a.u2;
+// Rt = R^T [K][ldr] (K: nb rows zero padded past round4(nb); ldr >= nd, a
+// multiple of 16).
+hipError_t launch_reconstruct(const Geo &g, const double *Rt, int ldr, const SolveArgs &a, hipStream_t s);
 // Per-row data of the reverse pass's on-the-fly transposed solves.
 size_t solve_scratch_doubles(const Geo &g);
 // Factored steady solves: factor storage (basis -1: the mean system), the

@@ -76,7 +76,7 @@ __device__ __forceinline__ bool nyq_mode(const Geo &g, int m) { return m / g.kzs
 // MFMA loop only waits on LDS and on the small (L2-resident) A matrices.
 // Staging loads per thread of each kernel; the launchers check that
 // blockDim * STAGE covers the kernel's panel.
-constexpr int STAGE_NODAL = 6, STAGE_RHS = 3, STAGE_NODAL_ADJ = 3, STAGE_RHS_ADJ = 4;
+constexpr int STAGE_NODAL = 6, STAGE_RHS = 3, STAGE_NODAL_ADJ = 3, STAGE_RHS_ADJ = 4, STAGE_RECON = 4;
 
 // K loop of a wave's tile: nit steps of 4 K rows.  The A operands (NA values
 // per step, from small L2-resident matrices) are read D steps ahead into a
@@ -848,6 +848,93 @@ __global__ void __launch_bounds__(512) k_rhs_adj_mfma(Geo g, AdjMats<T> am, Prim
 
 }  // namespace
 
+// ------------------------------------------------------ reconstruction ----
+// Velocity of a solved state (channelflow_kmm.py:855-870): f = R u0 with
+// R = P_D d/dx (nd x nb) on the matrix cores, then per mode
+// u1 = i (ky f + kz g) / k^2, u2 = i (kz f - ky g) / k^2 with the derivative
+// wavenumbers (Nyquist row eliminated); the mean mode takes u1 = v00,
+// u2 = w00.  FP64; u0 is read once per block instead of once per output row.
+__global__ void __launch_bounds__(512) k_recon_mfma(Geo g, const double *__restrict__ Rt, int ldr,
+                                                    const double2 *__restrict__ u0,
+                                                    const double2 *__restrict__ gg,
+                                                    const double *__restrict__ v,
+                                                    const double *__restrict__ w, double2 *__restrict__ u1,
+                                                    double2 *__restrict__ u2) {
+  using MM = Mfma<double>;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int ct = blockIdx.x;
+  const int M = g.m, kp = (g.nb + 3) / 4 * 4;
+  const Panel<double> B{reinterpret_cast<double *>(smem), kp};
+  {
+    const int total = kp * 8;
+    double2 z[STAGE_RECON];
+#pragma unroll
+    for (int t = 0; t < STAGE_RECON; ++t) {
+      const int idx = threadIdx.x + t * blockDim.x;
+      const int k = idx >> 3, m = 8 * ct + (idx & 7);
+      z[t] = double2{0, 0};
+      if (idx < total && m < M && k < g.nb) z[t] = u0[(size_t)k * M + m];
+    }
+#pragma unroll
+    for (int t = 0; t < STAGE_RECON; ++t) {
+      const int idx = threadIdx.x + t * blockDim.x;
+      if (idx >= total) continue;
+      const int k = idx >> 3, j = idx & 7;
+      B.at(0, k, 2 * j) = z[t].x;
+      B.at(0, k, 2 * j + 1) = z[t].y;
+    }
+  }
+  __syncthreads();
+  const Lane L = lane_of(g, ct);
+  double kyd, kzd, kyo;
+  mode_k(g, L.valid ? L.mcol : 0, kyd, kzd, kyo);
+  const double k2 = kyd * kyd + kzd * kzd, den = (k2 == 0.0) ? 1.0 : k2;
+  const double ky = kyd / den, kz = kzd / den;
+  const double s = L.comp ? 1.0 : -1.0;  // (i z) of this lane's component = s * z of the partner's
+  const int kq = L.l >> 4;
+  for (int x0 = (threadIdx.x >> 6) * 16; x0 < g.nd; x0 += (blockDim.x >> 6) * 16) {
+    const int xa = x0 + (L.l & 15);
+    // the epilogue's g values (mean mode: v00, w00) in flight during the K loop
+    double gp[4], vm[4], wm[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int x = x0 + MM::row(L.l, r);
+      gp[r] = vm[r] = wm[r] = 0.0;
+      if (L.valid && x < g.nd) {
+        if (L.mcol == 0) {
+          vm[r] = v[x];
+          wm[r] = w[x];
+        } else {
+          const double2 gv = gg[(size_t)x * M + L.mcol];
+          gp[r] = L.comp ? gv.x : gv.y;
+        }
+      }
+    }
+    MM::acc F{};
+    k_loop<double, 1, 8>(
+        kp / 4, [&](int it, double(&a)[1]) { a[0] = mat_at(Rt, ldr, 4 * it + kq, xa); },
+        [&](int it, const double(&a)[1]) { F = MM::mma(a[0], B.at(0, 4 * it + kq, L.c), F); });
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const double fp = partner(F[r]);
+      const int x = x0 + MM::row(L.l, r);
+      if (L.valid && x < g.nd) {
+        const size_t o = (size_t)x * M + L.mcol;
+        double a1, a2;
+        if (L.mcol == 0) {
+          a1 = L.comp ? 0.0 : vm[r];
+          a2 = L.comp ? 0.0 : wm[r];
+        } else {
+          a1 = s * (ky * fp + kz * gp[r]);
+          a2 = s * (kz * fp - ky * gp[r]);
+        }
+        reinterpret_cast<double *>(u1)[2 * o + L.comp] = a1;
+        reinterpret_cast<double *>(u2)[2 * o + L.comp] = a2;
+      }
+    }
+  }
+}
+
 // One block per tile of 8 modes, one wave per 16 output rows; LDS holds the
 // block's B panels: nf fields x roundup(K, 4) rows x 16 columns of T.
 static dim3 mfma_grid(const Geo &g) { return dim3((g.m + 7) / 8); }
@@ -866,6 +953,14 @@ hipError_t launch_nodal_mfma(const Geo &g, const NodalMats<T> &nm, const double2
   if (!stage_covers(g.n0, STAGE_NODAL, 3 * ((g.nd + 3) / 4 * 4) * 8)) return hipErrorInvalidValue;
   hipLaunchKernelGGL((k_nodal_mfma<T>), mfma_grid(g), mfma_block(g.n0), panel_bytes<T>(3, g.nd), s, g, nm,
                      u0, u1, u2, reinterpret_cast<V *>(ws.X));
+  return hipGetLastError();
+}
+
+hipError_t launch_reconstruct(const Geo &g, const double *Rt, int ldr, const SolveArgs &a, hipStream_t s) {
+  if (ldr < g.nd || ldr % 16 || !stage_covers(g.nd, STAGE_RECON, ((g.nb + 3) / 4 * 4) * 8))
+    return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_recon_mfma, mfma_grid(g), mfma_block(g.nd), panel_bytes<double>(1, g.nb), s, g, Rt, ldr,
+                     a.u0_out, a.g_out, a.v_out, a.w_out, a.u1, a.u2);
   return hipGetLastError();
 }
 
