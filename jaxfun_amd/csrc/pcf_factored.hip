@@ -10,6 +10,8 @@
 // forward and back substitutions, whose factor loads do not depend on the
 // substitution chain.  The reference likewise factors once and substitutes per
 // step (src/jaxfun/la/tpmatrix.py:1034-1150, 686-799).
+#include <type_traits>
+
 #include "pcf_adjoint.hpp"
 #include "pcf_device.hpp"
 
@@ -293,17 +295,26 @@ __global__ void __launch_bounds__(256) k_solve_factored(Geo g, StructMats sm, Fa
   const int n = ss.n;
   const int NF = (sys == 0) ? NF_B : NF_D;
   auto ZN = [&](int j, int v, int lane) -> double & { return zn[((size_t)j * 4 + v) * SLP + lane]; };
-  // factors of every (row, value, lane), consecutive lanes on consecutive modes
-  gather_batched<8>(
-      n * NF * SLP,
-      [&](int idx) {
-        const int j = idx / (NF * SLP), rem = idx - j * NF * SLP, v = rem / SLP, lane = rem - v * SLP;
-        const bool mean = (lane == SL);
-        const int m = m0 + lane;
-        if (mean ? !has_mean : m >= M) return 0.0;
-        return mean ? fs.mean[par][(size_t)j * NF_D + v] : fs.blk[sys][par][((size_t)j * NF + v) * M + m];
-      },
-      [&](int idx, double v) { fl[idx] = v; });
+  // factors of every (row, value, lane), consecutive lanes on consecutive modes;
+  // the value count as a compile-time constant keeps the index split to
+  // multiplications
+  auto stage_factors = [&](auto nfc) {
+    constexpr int NFC = decltype(nfc)::value;
+    gather_batched<8>(
+        n * NFC * SLP,
+        [&](int idx) {
+          const int j = idx / (NFC * SLP), rem = idx - j * NFC * SLP, v = rem / SLP, lane = rem - v * SLP;
+          const bool mean = (lane == SL);
+          const int m = m0 + lane;
+          if (mean ? !has_mean : m >= M) return 0.0;
+          return mean ? fs.mean[par][(size_t)j * NF_D + v] : fs.blk[sys][par][((size_t)j * NFC + v) * M + m];
+        },
+        [&](int idx, double v) { fl[idx] = v; });
+  };
+  if (sys == 0)
+    stage_factors(std::integral_constant<int, NF_B>{});
+  else
+    stage_factors(std::integral_constant<int, NF_D>{});
 
   // phase 0: z and N of every (row, lane)
   struct ZNv {
