@@ -812,13 +812,15 @@ hipError_t launch_fft_y(const Geo &g, const NodalMats<T> &nm, Workspace &ws, con
                  : launch_fft_y_fields<T>(g, nm, py, false, mhd ? 6 : 3, ws.Hz, ws.Hx, s);
 }
 
-// y-line pairs per block: two keep the butterfly registers small; LDS holds
-// (inputs + outputs) lines of MZ per unit plus the twiddle table
+// y-line pairs per block; LDS holds (inputs + outputs) lines of MZ per unit
+// plus the twiddle table.  FP64 and MHD: one pair (twice the blocks, half the
+// LDS each: measured 20.6 instead of 23.3 us at 65x64x64 in FP64); FP32: two
+// (14.3 instead of 15.5 us)
 template <typename T>
 int rot_units(const Geo &g, bool mhd) {
   using V = typename VT<T>::V;
   const int lines = mhd ? 18 : 9;
-  int upb = mhd ? 1 : 2;  // MHD: twelve lines per unit already fill the block
+  int upb = (mhd || sizeof(T) == 8) ? 1 : 2;
   while (upb > 1 && ((size_t)upb * lines * g.mz + g.mz) * sizeof(V) > 65536) upb >>= 1;
   return upb;
 }
