@@ -19,6 +19,7 @@
 
 #include "../../include/pcf_dns.h"
 #include "pcf_adjoint.hpp"
+#include "pcf_device.hpp"
 #include "pcf_kernels.hpp"
 #include "pcf_setup.hpp"
 
@@ -69,13 +70,12 @@ std::string validate(const pcf_config *c) {
 int padded(int n, double f) { return std::max(n, (int)std::floor(f * n)); }
 
 bool make_plan(int n, pcf::FFTPlan &p) {
+  // the device plans' radix choice (pcf::cplan): radix 8 only where it saves a pass
+  const pcf::CPlan c = pcf::cplan(n);
   p.n = n;
-  p.nst = 0;
+  p.nst = c.nst;
   int m = n;
-  while (m % 8 == 0) p.radix[p.nst++] = 8, m /= 8;
-  while (m % 4 == 0) p.radix[p.nst++] = 4, m /= 4;
-  while (m % 2 == 0) p.radix[p.nst++] = 2, m /= 2;
-  while (m % 3 == 0) p.radix[p.nst++] = 3, m /= 3;
+  for (int s = 0; s < c.nst; ++s) p.radix[s] = c.r[s], m /= c.r[s];
   return m == 1 && p.nst <= 12;
 }
 

@@ -160,15 +160,20 @@ struct CPlan {
   int nst;
   int r[12];
 };
-// R8: radix-8 passes first (fewer LDS round trips, more registers per thread).
-__host__ __device__ constexpr CPlan cplan(int n, bool r8 = true) {
+// Radix 8 first where that saves a pass (fewer LDS round trips for more
+// registers per thread: 96 = 8.4.3, 64 = 8.8), else radices 4, 2, 3
+// (144 = 4.4.3.3, 48 = 4.4.3); r8 = false keeps radix 4 throughout.
+__host__ __device__ constexpr CPlan cplan_with(int n, bool r8) {
   CPlan p{0, {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0}};
   int m = n;
-  while (r8 && m % 8 == 0) p.r[p.nst++] = 8, m /= 8;
-  while (m % 4 == 0) p.r[p.nst++] = 4, m /= 4;
-  while (m % 2 == 0) p.r[p.nst++] = 2, m /= 2;
-  while (m % 3 == 0) p.r[p.nst++] = 3, m /= 3;
+  while (r8 && m % 8 == 0 && p.nst < 12) p.r[p.nst++] = 8, m /= 8;
+  while (m % 4 == 0 && p.nst < 12) p.r[p.nst++] = 4, m /= 4;
+  while (m % 2 == 0 && p.nst < 12) p.r[p.nst++] = 2, m /= 2;
+  while (m % 3 == 0 && p.nst < 12) p.r[p.nst++] = 3, m /= 3;
   return p;
+}
+__host__ __device__ constexpr CPlan cplan(int n, bool r8 = true) {
+  return (r8 && cplan_with(n, true).nst < cplan_with(n, false).nst) ? cplan_with(n, true) : cplan_with(n, false);
 }
 
 template <typename V, typename T, int N, int R, int P, bool INV, bool BATCH_FAST, int MAXB>

@@ -751,12 +751,11 @@ int max_butterflies(const FFTPlan &plan, int nbatch) {
 // The same length without radix-8 passes (the FP64 k_rotational's
 // compile-time plans); launch bounds cover both plans.
 FFTPlan radix4_plan(const FFTPlan &p) {
+  const CPlan c = cplan(p.n, false);
   FFTPlan q{};
   q.n = p.n;
-  int m = p.n;
-  while (m % 4 == 0) q.radix[q.nst++] = 4, m /= 4;
-  while (m % 2 == 0) q.radix[q.nst++] = 2, m /= 2;
-  while (m % 3 == 0) q.radix[q.nst++] = 3, m /= 3;
+  q.nst = c.nst;
+  for (int s = 0; s < c.nst; ++s) q.radix[s] = c.r[s];
   return q;
 }
 
