@@ -320,7 +320,7 @@ __global__ void __launch_bounds__(256) k_rotational_mhd(Geo g, NodalMats<T> nm, 
       [&](int idx, const V &z) { buf[idx] = z; });
   __syncthreads();
   if constexpr (NZC > 0)
-    fft_lds_c<V, T, NZC, true, false, MAXB>(buf, tw, UPB * NIN, MZ, 1);
+    fft_lds_c<V, T, NZC, true, false, MAXB, sizeof(T) == 4>(buf, tw, UPB * NIN, MZ, 1);
   else
     fft_lds<V, T, true, false, MAXB>(buf, tw, plan, UPB * NIN, MZ, 1);
 
@@ -369,7 +369,7 @@ __global__ void __launch_bounds__(256) k_rotational_mhd(Geo g, NodalMats<T> nm, 
   }
   __syncthreads();
   if constexpr (NZC > 0)
-    fft_lds_c<V, T, NZC, false, false, MAXB>(out, tw, UPB * NOUT, MZ, 1);
+    fft_lds_c<V, T, NZC, false, false, MAXB, sizeof(T) == 4>(out, tw, UPB * NOUT, MZ, 1);
   else
     fft_lds<V, T, false, false, MAXB>(out, tw, plan, UPB * NOUT, MZ, 1);
 
