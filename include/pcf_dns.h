@@ -240,6 +240,14 @@ int32_t pcf_stage_count(void);
 const char *pcf_stage_name(int32_t i);
 int32_t pcf_profile_steady(pcf_handle *h, int32_t nsteps, double *stage_ms);
 
+/* Mean duration (milliseconds) of one launch of nonlinear stage `stage`
+ * (0 nodal_backward .. 4 forward_rhs) for the current state: the stages before
+ * it run once to fill its inputs, then `reps` back-to-back launches between two
+ * HIP events on the handle's stream (the launch gaps hide behind the queue).
+ * The stages only rewrite workspace and the current nonlinear slot, which the
+ * next step recomputes.  No reference counterpart. */
+int32_t pcf_profile_stage(pcf_handle *h, int32_t stage, int32_t reps, double *ms_per_launch);
+
 /* Reverse mode.  Replaces jax.value_and_grad / jax.vjp over the scanned,
  * rematerialised step (production/autograd_cost_benchmark.py:228-330,
  * src/jaxfun/integrators/cnab2.py:55) for the velocity/vorticity state.

@@ -726,6 +726,40 @@ hipError_t field_maxima_stages(pcf_handle *h, const DevMats<T> &dm) {
   return pcf::launch_absmax<T>(g0, dm.xnode, dm.twz0, h->pz0, h->mhd ? 6 : 3, Y, h->dblockmax, h->stream);
 }
 
+// The kernel(s) of nonlinear stage `stage` (0 nodal_backward .. 4 forward_rhs)
+// for the current state, as nonlinear_stages launches them.
+template <typename T>
+hipError_t stage_launch(pcf_handle *h, const DevMats<T> &dm, int stage) {
+  const pcf::Geo &g = h->geo;
+  const bool mhd = h->mhd;
+  const int slot = h->no[0], cur = h->so[0];
+  hipError_t e = hipSuccess;
+  switch (stage) {
+    case 0:
+      e = pcf::launch_nodal_mfma<T>(g, dm.view, h->u0[cur], h->u1, h->u2, h->ws, h->stream);
+      if (e == hipSuccess && mhd) e = pcf::launch_nodal_mhd_mfma<T>(g, dm.mview, h->A[cur], h->cstride, h->ws, h->stream);
+      return e;
+    case 1: return pcf::launch_fft_y<T>(g, dm.view, h->ws, h->py, true, mhd, h->stream);
+    case 2: return pcf::launch_rotational<T>(g, dm.view, h->ws, h->pz, mhd, h->stream);
+    case 3: return pcf::launch_fft_y<T>(g, dm.view, h->ws, h->py, false, mhd, h->stream);
+    default:
+      e = pcf::launch_rhs_mfma<T>(g, dm.view, h->ws, h->Nu[slot], h->Ng[slot], h->Nv[slot], h->Nw[slot], h->stream);
+      if (e == hipSuccess && mhd) e = pcf::launch_rhs_a_mfma<T>(g, dm.mview, h->ws, h->NA[slot], h->cstride, h->stream);
+      return e;
+  }
+}
+
+template <typename T>
+hipError_t stage_repeat(pcf_handle *h, const DevMats<T> &dm, int stage, int reps, hipEvent_t e0, hipEvent_t e1) {
+  hipError_t e;
+  for (int s = 0; s < stage; ++s)  // the stage's inputs for the current state
+    if ((e = stage_launch<T>(h, dm, s))) return e;
+  if ((e = hipEventRecord(e0, h->stream))) return e;
+  for (int r = 0; r < reps; ++r)
+    if ((e = stage_launch<T>(h, dm, stage))) return e;
+  return hipEventRecord(e1, h->stream);
+}
+
 // Field id -> (device pointer, is_complex, rows).
 struct FieldRef {
   void *ptr;
@@ -1773,6 +1807,29 @@ int32_t pcf_profile_steady(pcf_handle *h, int32_t nsteps, double *stage_ms) {
     for (int j = 0; j < 2; ++j) (void)hipEventDestroy(t.ev[i][j]);
   if (r) return r;
   for (int i = 0; i < NSTAGE; ++i) stage_ms[i] = acc[i] / nsteps;
+  return PCF_OK;
+}
+
+int32_t pcf_profile_stage(pcf_handle *h, int32_t stage, int32_t reps, double *ms_per_launch) {
+  if (!h || !ms_per_launch) return fail(PCF_E_INVALID, "null argument");
+  if (stage < 0 || stage > 4) return fail(PCF_E_INVALID, "stage must be a nonlinear stage (0..4)");
+  if (reps <= 0) return fail(PCF_E_INVALID, "reps must be positive");
+  PCF_HIP(hipSetDevice(h->device));
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  PCF_HIP(hipEventCreate(&e0));
+  if (hipEventCreate(&e1) != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    return fail(PCF_E_HIP, "event create failed");
+  }
+  hipError_t e = (h->cfg.precision == PCF_PREC_FP64) ? stage_repeat<double>(h, h->md, stage, reps, e0, e1)
+                                                     : stage_repeat<float>(h, h->mf, stage, reps, e0, e1);
+  float ms = 0.f;
+  if (e == hipSuccess) e = hipEventSynchronize(e1);
+  if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (e != hipSuccess) return fail(PCF_E_HIP, std::string("stage profile: ") + hipGetErrorString(e));
+  *ms_per_launch = ms / reps;
   return PCF_OK;
 }
 

@@ -293,7 +293,6 @@ __global__ void __launch_bounds__(256) k_solve_factored(Geo g, StructMats sm, Fa
   const int sys = spb >> 1, par = spb & 1;
   const StructSys &ss = sm.sys[sys][par];
   const int n = ss.n;
-  const int NF = (sys == 0) ? NF_B : NF_D;
   auto ZN = [&](int j, int v, int lane) -> double & { return zn[((size_t)j * 4 + v) * SLP + lane]; };
   // factors of every (row, value, lane), consecutive lanes on consecutive modes;
   // the value count as a compile-time constant keeps the index split to
@@ -451,21 +450,27 @@ __global__ void __launch_bounds__(256) k_adj_solve_factored(Geo g, StructMats sm
   const int sys = spb >> 1, par = spb & 1;
   const StructSys &ss = smt.sys[sys][par];
   const int n = ss.n;
-  const int NF = (sys == 0) ? NF_B : NF_D;
   double *zn = reinterpret_cast<double *>(smem);  // [ir][2][SLP]: rhs, then y, then x
   double *fl = zn + (size_t)g.npmax * 2 * SLP;    // [ir][NF][SLP]
   auto row = [&](int ir) { return 2 * (n - 1 - ir) + par; };
   auto ZN = [&](int ir, int v, int lane) -> double & { return zn[((size_t)ir * 2 + v) * SLP + lane]; };
   auto active = [&](int lane) { return (lane == SL) ? has_mean : (m0 + lane < M); };
-  gather_batched<8>(
-      n * NF * SLP,
-      [&](int idx) {
-        const int j = idx / (NF * SLP), rem = idx - j * NF * SLP, v = rem / SLP, lane = rem - v * SLP;
-        if (!active(lane)) return 0.0;
-        return (lane == SL) ? fs.mean[par][(size_t)j * NF_D + v]
-                            : fs.blk[sys][par][((size_t)j * NF + v) * M + m0 + lane];
-      },
-      [&](int idx, double v) { fl[idx] = v; });
+  auto stage_factors = [&](auto nfc) {
+    constexpr int NFC = decltype(nfc)::value;
+    gather_batched<8>(
+        n * NFC * SLP,
+        [&](int idx) {
+          const int j = idx / (NFC * SLP), rem = idx - j * NFC * SLP, v = rem / SLP, lane = rem - v * SLP;
+          if (!active(lane)) return 0.0;
+          return (lane == SL) ? fs.mean[par][(size_t)j * NF_D + v]
+                              : fs.blk[sys][par][((size_t)j * NFC + v) * M + m0 + lane];
+        },
+        [&](int idx, double v) { fl[idx] = v; });
+  };
+  if (sys == 0)
+    stage_factors(std::integral_constant<int, NF_B>{});
+  else
+    stage_factors(std::integral_constant<int, NF_D>{});
   gather_batched<4>(
       n * SLP,
       [&](int idx) {
