@@ -38,7 +38,7 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "DNS steps/s at 65×64×64: mixed vs FP64 on GPU; adjoint/forward cost ratio"
 WORKLOAD = dict(N=(65, 64, 64), Re=1000.0, dt=0.005, energy=1e-3, K=12, seed=0)
-PUBLISHED_STEPS_PER_S = 1000.0 / 27.552  # reference, RTX 5090 Laptop FP64
+PUBLISHED_STEPS_PER_S = 1000.0 / 27.552  # reference 65x64x64 SBDF3 FP64, RTX 5090 Laptop (benchmark-results/FINAL_LOCAL_CHECKPOINT.md:58)
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
 FP32_PEAK_TFLOPS = 157.3
