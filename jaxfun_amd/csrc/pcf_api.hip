@@ -136,6 +136,8 @@ struct pcf_handle {
   pcf::FactorSet fA_ars{};    // ARS(2,2,2) LU factors of the A systems
   pcf::FactorSet fA_stage[3]{};  // IMEXRK3 stage factors of the A systems
   pcf::FactorSet fA_cn{};     // CNAB2 factors of the A systems
+  hipStream_t side = nullptr;  // insulating MHD: the vector-potential kernels beside the flow ones
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   double *ddiagm = nullptr;
   int so[3] = {0, 1, 2};     // solution ring: cur, old, older
   int no[4] = {0, 1, 2, 3};  // nonlinear ring: free0, old, older, free1
@@ -358,9 +360,24 @@ inline hipError_t mark(pcf_handle *h, int stage, int edge) {
   return hipEventRecord(t->ev[stage][edge], h->stream);
 }
 
+// With insulating walls the vector-potential kernels run on a side stream
+// beside the flow kernels they do not depend on: fork_side makes the side stream wait for the
+// work queued so far on the handle stream, join_side makes the handle stream
+// wait for the side stream (inside a graph capture both become graph edges).
+hipError_t fork_side(pcf_handle *h) {
+  hipError_t e = hipEventRecord(h->ev_fork, h->stream);
+  return e != hipSuccess ? e : hipStreamWaitEvent(h->side, h->ev_fork, 0);
+}
+
+hipError_t join_side(pcf_handle *h) {
+  hipError_t e = hipEventRecord(h->ev_join, h->side);
+  return e != hipSuccess ? e : hipStreamWaitEvent(h->stream, h->ev_join, 0);
+}
+
 // Under MHD the same stages carry (u, w, b, j) forward and (H, E) back, and
 // the A fields of the state (A, three components) give b and j and receive
-// the induction rows in NA[slot].
+// the induction rows in NA[slot]; b, j and the induction rows are formed on
+// the side stream when there is one.
 template <typename T>
 hipError_t nonlinear_stages(pcf_handle *h, const DevMats<T> &dm, const double2 *u0, int slot,
                             const double2 *A) {
@@ -369,8 +386,13 @@ hipError_t nonlinear_stages(pcf_handle *h, const DevMats<T> &dm, const double2 *
   const bool mhd = h->mhd;
   hipError_t e;
   if ((e = mark(h, 0, 0))) return e;
+  if (mhd) {
+    if (h->side && (e = fork_side(h))) return e;
+    if ((e = pcf::launch_nodal_mhd_mfma<T>(g, dm.mview, A, h->cstride, h->ws, h->side ? h->side : h->stream)))
+      return e;
+  }
   if ((e = pcf::launch_nodal_mfma<T>(g, nm, u0, h->u1, h->u2, h->ws, h->stream))) return e;
-  if (mhd && (e = pcf::launch_nodal_mhd_mfma<T>(g, dm.mview, A, h->cstride, h->ws, h->stream))) return e;
+  if (h->side && (e = join_side(h))) return e;
   if ((e = mark(h, 0, 1))) return e;
   if ((e = mark(h, 1, 0))) return e;
   if ((e = pcf::launch_fft_y<T>(g, nm, h->ws, h->py, true, mhd, h->stream))) return e;
@@ -382,10 +404,15 @@ hipError_t nonlinear_stages(pcf_handle *h, const DevMats<T> &dm, const double2 *
   if ((e = pcf::launch_fft_y<T>(g, nm, h->ws, h->py, false, mhd, h->stream))) return e;
   if ((e = mark(h, 3, 1))) return e;
   if ((e = mark(h, 4, 0))) return e;
+  if (mhd) {
+    if (h->side && (e = fork_side(h))) return e;
+    if ((e = pcf::launch_rhs_a_mfma<T>(g, dm.mview, h->ws, h->NA[slot], h->cstride, h->side ? h->side : h->stream)))
+      return e;
+  }
   if ((e = pcf::launch_rhs_mfma<T>(g, nm, h->ws, h->Nu[slot], h->Ng[slot], h->Nv[slot], h->Nw[slot],
                               h->stream)))
     return e;
-  if (mhd && (e = pcf::launch_rhs_a_mfma<T>(g, dm.mview, h->ws, h->NA[slot], h->cstride, h->stream))) return e;
+  if (h->side && (e = join_side(h))) return e;
   return mark(h, 4, 1);
 }
 
@@ -418,12 +445,14 @@ hipError_t implicit_update(pcf_handle *h, const pcf::SolveArgs &a, const pcf::So
   if (!fs) fs = &h->fsteady;
   if (!fsA) fsA = &h->fA;
   if ((e = mark(h, 5, 0))) return e;
-  if ((e = pcf::launch_solve_factored(h->geo, h->sm, *fs, a, 0, 0, h->stream))) return e;
   if (aA) {
-    // components solved with the Dirichlet (TD) systems: all three for
-    // conducting walls, Ax alone for insulating walls
+    // the A solves on the side stream: components with the Dirichlet (TD)
+    // systems (all three for conducting walls, Ax alone for insulating walls)
+    // and the bordered A_y/A_z solve
+    hipStream_t sa = h->side ? h->side : h->stream;
+    if (h->side && (e = fork_side(h))) return e;
     const int ntd = h->insul ? 1 : 3;
-    if ((e = pcf::launch_solve_factored(h->geo_eta, h->sm, *fsA, *aA, ntd, h->cstride, h->stream))) return e;
+    if ((e = pcf::launch_solve_factored(h->geo_eta, h->sm, *fsA, *aA, ntd, h->cstride, sa))) return e;
     if (h->insul) {
       const pcf::SolveArgs &ab = aBord ? *aBord : *aA;
       pcf::BorderedArgs b{};
@@ -441,12 +470,14 @@ hipError_t implicit_update(pcf_handle *h, const pcf::SolveArgs &a, const pcf::So
       b.gamma = ab.gamma;
       b.out = ab.g_out;
       b.cstride = h->cstride;
-      if ((e = pcf::launch_solve_bordered(h->geo_eta, b, h->stream))) return e;
+      if ((e = pcf::launch_solve_bordered(h->geo_eta, b, sa))) return e;
     }
   }
+  if ((e = pcf::launch_solve_factored(h->geo, h->sm, *fs, a, 0, 0, h->stream))) return e;
   if ((e = mark(h, 5, 1))) return e;
   if ((e = mark(h, 6, 0))) return e;
   if ((e = pcf::launch_reconstruct(h->geo, h->Rt, h->ldr, a, h->stream))) return e;
+  if (aA && h->side && (e = join_side(h))) return e;
   return mark(h, 6, 1);
 }
 
@@ -1379,6 +1410,14 @@ int32_t pcf_create(const pcf_config *cfg, pcf_handle **out) {
   if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess)
     return bail(fail(PCF_E_HIP, "hipStreamCreate failed"));
   h->own_stream = true;
+  // the side stream pays with insulating walls (their bordered A_y/A_z solve
+  // runs beside the flow solve): 5 890 -> 6 220 steps/s at 65x64x64; with
+  // conducting walls the cross-stream edges cost more than the overlap gains
+  // (6 830 -> 6 480 steps/s), so those run on one stream
+  if (h->insul && (hipStreamCreateWithFlags(&h->side, hipStreamNonBlocking) != hipSuccess ||
+                 hipEventCreateWithFlags(&h->ev_fork, hipEventDisableTiming) != hipSuccess ||
+                 hipEventCreateWithFlags(&h->ev_join, hipEventDisableTiming) != hipSuccess))
+    return bail(fail(PCF_E_HIP, "side stream / event create failed"));
 
   int32_t r;
   const size_t M = g.m;
@@ -1485,6 +1524,10 @@ int32_t pcf_destroy(pcf_handle *h) {
   for (hipEvent_t ev : {h->adj.ev_seed, h->adj.ev_rc[0], h->adj.ev_rc[1], h->adj.ev_rev[0], h->adj.ev_rev[1]})
     if (ev) (void)hipEventDestroy(ev);
   if (h->adj.stream) (void)hipStreamDestroy(h->adj.stream);
+  if (h->side) (void)hipStreamSynchronize(h->side);
+  for (hipEvent_t ev : {h->ev_fork, h->ev_join})
+    if (ev) (void)hipEventDestroy(ev);
+  if (h->side) (void)hipStreamDestroy(h->side);
   for (void *p : h->adj_allocs) (void)hipFree(p);
   for (auto &kv : h->graphs) (void)hipGraphExecDestroy(kv.second);
   for (auto &kv : h->seg_graphs) (void)hipGraphExecDestroy(kv.second);
