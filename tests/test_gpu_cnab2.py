@@ -186,3 +186,26 @@ def test_cnab2_mhd_steps_match_oracle(kind):
     for key in ("Epert", "Emag"):
         assert math.isclose(d[key], do[key], rel_tol=1e-12), (key, d[key], do[key])
     assert d["divB_L2"] < 1e-12
+
+
+def test_imexrk222_mhd_set_dt_matches_oracle():
+    """IMEXRK222 under MHD after set_dt: the flow and vector-potential stage
+    systems are refactored in place at the new dt."""
+    from jaxfun_amd.ic import seeded_mhd_state
+
+    from tests.helpers import insulating_oracle_for, to_mhd_oracle
+    from tests.test_gpu_mhd import make_insulating
+
+    solver = make_insulating((17, 16, 16), time_integrator="IMEXRK222")
+    oracle = insulating_oracle_for(solver)
+    s = seeded_mhd_state(solver, seed=3, energy_u=1e-3, energy_b=1e-3, K=4)
+    so = to_mhd_oracle(s)
+    s, so = solver.solve(s, 2), oracle.solve(so, 2)
+    h = 0.6 * solver.dt
+    solver.set_dt(h)
+    oracle.set_dt(h)
+    s, so = solver.solve(s, 3), oracle.solve(so, 3)
+    e_u = _group_err(list(s.flow.u) + [s.flow.g], list(so.flow.u) + [so.flow.g])
+    e_a = _group_err(s.A, so.A)
+    print("imexrk222 mhd set_dt", e_u, e_a, flush=True)
+    assert e_u < TOL and e_a < TOL
