@@ -440,6 +440,7 @@ __global__ void __launch_bounds__(256) k_adj_solve_factored(Geo g, StructMats sm
                                                             Prim out, Prim nuo, PrimC r1, PrimC r2,
                                                             double e0, double e1, double e2, Prim sg,
                                                             double c0, double c1, double c2) {
+  static_assert(2 * SL <= 64, "the substitution threads of the modes fill at most the first wave");
   extern __shared__ __align__(16) unsigned char smem[];
   constexpr int SLP = SL + 1;
   const int M = g.m;
@@ -486,22 +487,24 @@ __global__ void __launch_bounds__(256) k_adj_solve_factored(Geo g, StructMats sm
         ZN(ir, 1, lane) = v.y;
       });
   __syncthreads();
-  int lane = -1;
-  if (threadIdx.x < SL)
-    lane = threadIdx.x;
-  else if (threadIdx.x == 64 && has_mean)
+  // one substitution thread per (lane, real/imaginary part), as in the
+  // forward solve: threads [0, 2 SL), and 64, 65 for the mean system
+  int lane = -1, part = 0;
+  if (threadIdx.x < 2 * SL) {
+    lane = threadIdx.x % SL;
+    part = threadIdx.x / SL;
+  } else if (threadIdx.x >= 64 && threadIdx.x < 66 && has_mean) {
     lane = SL;
+    part = threadIdx.x - 64;
+  }
   if (lane >= 0 && active(lane)) {
-    auto rw = [&](int i) { return double2{ZN(i, 0, lane), ZN(i, 1, lane)}; };
-    auto wr = [&](int i, double2 v) {
-      ZN(i, 0, lane) = v.x;
-      ZN(i, 1, lane) = v.y;
-    };
+    auto rw = [&](int i) { return ZN(i, part, lane); };
+    auto wr = [&](int i, double v) { ZN(i, part, lane) = v; };
     const double *F = fl + lane;
     if (sys == 0)
-      substitute<2, 2, 2, double2>(ss, n, F, (size_t)NF_B * SLP, (size_t)SLP, rw, wr, rw, wr);
+      substitute<2, 2, 2, double>(ss, n, F, (size_t)NF_B * SLP, (size_t)SLP, rw, wr, rw, wr);
     else
-      substitute<1, 1, 1, double2>(ss, n, F, (size_t)NF_D * SLP, (size_t)SLP, rw, wr, rw, wr);
+      substitute<1, 1, 1, double>(ss, n, F, (size_t)NF_D * SLP, (size_t)SLP, rw, wr, rw, wr);
   }
   __syncthreads();
   for (int idx = threadIdx.x; idx < n * SLP; idx += blockDim.x) {
