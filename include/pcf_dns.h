@@ -168,7 +168,9 @@ int32_t pcf_create(const pcf_config *cfg, pcf_handle **out);
 int32_t pcf_destroy(pcf_handle *h);
 
 /* Stream: all work of the handle is enqueued on this hipStream_t (NULL = the
- * library's own stream).  Passed as void* to keep HIP types out of the ABI. */
+ * library's own stream).  Insulating-wall MHD handles fork an internal side
+ * stream from it for the vector-potential kernels and join it back within
+ * every call.  Passed as void* to keep HIP types out of the ABI. */
 int32_t pcf_set_stream(pcf_handle *h, void *hip_stream);
 int32_t pcf_synchronize(pcf_handle *h);
 
