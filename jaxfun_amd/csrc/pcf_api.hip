@@ -154,7 +154,7 @@ struct pcf_handle {
   int ldr = 0;
   double *dVB = nullptr, *dVD = nullptr, *dVB1 = nullptr, *dVD1 = nullptr, *dfejer = nullptr,
          *dxn = nullptr, *ddiag = nullptr;
-  std::map<int, hipGraphExec_t> graphs;
+  std::map<uint64_t, hipGraphExec_t> graphs;  // (ring phase, steps) -> steady-step graph
   std::map<uint64_t, hipGraphExec_t> seg_graphs;  // steady runs of the reverse pass's forward sweeps
   StageTimer *timer = nullptr;  // non-null while pcf_profile_steady runs
 
@@ -915,10 +915,17 @@ std::vector<std::pair<void *, size_t>> state_pieces(pcf_handle *h) {
 // (ring phase, n, tape_base): the captured launches and pointers are the same
 // whenever the rings stand at the same phase, so one graph serves every
 // segment of the sweeps that starts there.
-int32_t steady_run(pcf_handle *h, int n, int tape_base) {
+// The ring phase: which slot holds each level of the solution and nonlinear
+// rings (captured launches hold absolute slot pointers).
+uint64_t ring_phase(const pcf_handle *h) {
   uint64_t key = 0;
   for (int i = 0; i < 3; ++i) key = key * 4 + (uint64_t)h->so[i];
   for (int i = 0; i < 4; ++i) key = key * 4 + (uint64_t)h->no[i];
+  return key;  // < 2^14
+}
+
+int32_t steady_run(pcf_handle *h, int n, int tape_base) {
+  uint64_t key = ring_phase(h);
   key |= (uint64_t)n << 16 | (uint64_t)(tape_base + 1) << 40;
   auto it = h->seg_graphs.find(key);
   if (it == h->seg_graphs.end()) {
@@ -1705,7 +1712,10 @@ int32_t pcf_step_steady_graph(pcf_handle *h, int32_t nsteps, int32_t replays) {
   if (h->history_steps < 2.0) return fail(PCF_E_INVALID, "history not bootstrapped (history_steps < 2)");
   if (nsteps <= 0 || nsteps % 12) return fail(PCF_E_INVALID, "graph length must be a positive multiple of 12 steps");
   PCF_HIP(hipSetDevice(h->device));
-  auto it = h->graphs.find(nsteps);
+  // one graph per ring phase: steps outside the graph (eager, start-up) can
+  // leave the rings at another phase than the one a cached graph was captured at
+  const uint64_t key = ring_phase(h) | (uint64_t)nsteps << 16;
+  auto it = h->graphs.find(key);
   if (it == h->graphs.end()) {
     hipGraph_t graph = nullptr;
     PCF_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
@@ -1718,7 +1728,7 @@ int32_t pcf_step_steady_graph(pcf_handle *h, int32_t nsteps, int32_t replays) {
     e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
     if (e != hipSuccess) return fail(PCF_E_HIP, std::string("instantiate: ") + hipGetErrorString(e));
-    it = h->graphs.emplace(nsteps, exec).first;
+    it = h->graphs.emplace(key, exec).first;
     // capture recorded the launches without running them and advanced the
     // ring nsteps times (a whole number of ring periods: back at the start)
   }
