@@ -247,3 +247,41 @@ def test_bench_refuses_multi_gpu():
                        text=True, timeout=120)
     assert r.returncode != 0
     assert "one GPU" in r.stderr
+
+
+@pytest.mark.parametrize("integ", ["SBDF3", "CNAB2", "IMEXRK222", "IMEXRK3"])
+def test_device_history_follows_reference_ensure_rules(integ):
+    """The history a state carries onto the device (_ensure_flow_history,
+    examples/channelflow_kmm.py:612-635): SBDF3 restarts unless all four
+    levels are present; CNAB2 without nonlinear_old starts at have_old = 0,
+    previous_dt = 0 and keeps a present level; the other integrators keep the
+    scalars and travel missing levels as zeros."""
+    from dataclasses import replace
+
+    from jaxfun_amd.ic import seeded_state
+    from jaxfun_amd.pcf import KMMState
+    from jaxfun_amd.pcf import PlaneCouetteFluctuationHIP as S
+
+    hs = _HostSolver((17, 16, 16))
+    hs.time_integrator = integ
+    hs._device_history = S._device_history.__get__(hs)
+    s = seeded_state(hs, seed=0, energy=1e-3, K=4)
+    nl = tuple(np.ones_like(a) for a in s.nonlinear_old)
+    partial = KMMState(u=s.u, g=s.g, nonlinear_old=nl, history_steps=2.0, have_old=1.0, previous_dt=0.5)
+    d = hs._device_history(partial)
+    assert all(getattr(d, k) is not None for k in ("nonlinear_old", "nonlinear_older", "solution_old",
+                                                   "solution_older"))
+    if integ == "SBDF3":  # incomplete: restart with zero levels and scalars
+        assert d.history_steps == 0.0 and d.have_old == 0.0 and d.previous_dt == 0.0
+        assert all(np.all(a == 0) for a in d.nonlinear_old)
+    else:  # levels kept, missing ones zero, scalars kept
+        assert d.history_steps == 2.0 and d.have_old == 1.0 and d.previous_dt == 0.5
+        assert all(np.all(a == 1) for a in d.nonlinear_old)
+        assert all(np.all(a == 0) for a in d.solution_old)
+    bare = replace(partial, nonlinear_old=None)
+    d = hs._device_history(bare)
+    if integ == "CNAB2":
+        assert d.have_old == 0.0 and d.previous_dt == 0.0 and d.history_steps == 2.0
+    elif integ != "SBDF3":
+        assert d.have_old == 1.0 and d.previous_dt == 0.5
+    assert all(np.all(a == 0) for a in d.nonlinear_old)
