@@ -285,3 +285,23 @@ def test_device_history_follows_reference_ensure_rules(integ):
     elif integ != "SBDF3":
         assert d.have_old == 1.0 and d.previous_dt == 0.5
     assert all(np.all(a == 0) for a in d.nonlinear_old)
+
+
+def test_product_cfl_arithmetic_matches_oracle():
+    """The host arithmetic of solver.cfl_scalars (spacings, rates, totals) on
+    the oracle's own maxima equals the oracle's restatement of _cfl_scalars
+    (production/health.py:120-153); the maxima themselves are a GPU test."""
+    from jaxfun_amd.pcf import PlaneCouetteFluctuationHIP as S
+    from oracle.mhd import RotatingShearMHDOracle
+
+    o = RotatingShearMHDOracle(N=(17, 16, 16), Re=300.0, Rm=200.0, omega=1.3, shear_rate=0.7, dt=3e-3,
+                               background_b=(0.0, 0.0, 0.05), magnetic_amplitude=0.0)
+    so = o.initial_state()
+    hs = _HostSolver((17, 16, 16))
+    hs.dt, hs.nu, hs.eta, hs.omega, hs.shear_rate = o.dt, o.nu, o.eta, o.omega, o.shear_rate
+    hs.field_maxima = lambda state=None: o.field_maxima(so)
+    got = S.cfl_scalars.__get__(hs)()
+    want = o.cfl(so)
+    assert set(got) == set(want)
+    for k, v in want.items():
+        assert math.isclose(got[k], v, rel_tol=1e-14, abs_tol=1e-300), (k, got[k], v)
