@@ -43,6 +43,8 @@ HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
 FP32_PEAK_TFLOPS = 157.3
 GRAPH_BLOCK = 12
+# the path every published SBDF3 number of the reference uses
+SBDF3_ROT = dict(time_integrator="SBDF3", nonlinear_form="rotational", coefficient_path="optimized")
 
 
 def stage_model(solver, precision: str) -> dict:
@@ -71,8 +73,10 @@ def stage_model(solver, precision: str) -> dict:
     fftf_f = fft_flops(my, 3 * n0 * kzs)
     rhs_b = 3 * n0 * m * V + (nb + nd) * m * c16
     rhs_f = m * n0 * 16 * nd
-    # three solution + three nonlinear levels in, new (u0, g) out
-    solve_b = 7 * (nb + nd) * m * c16
+    # three solution + three nonlinear levels in, new (u0, g) out, and the
+    # per-mode factors (7 values per row of the clamped blocks, 4 of the
+    # Dirichlet ones, FP64)
+    solve_b = 7 * (nb + nd) * m * c16 + (7 * nb + 4 * nd) * m * 8
     solve_f = m * 2 * (nb + nd) * 40  # O(n) structured sweeps, ~40 flops per row
     recon_b = (nb + nd) * m * c16 + 2 * nd * m * c16
     recon_f = m * nd * (nb // 2) * 4
@@ -98,16 +102,21 @@ def profile_stages(solver, nsteps: int) -> dict:
 
 
 def run_gpu(precision: str, steps: int, warmup: int, eager: bool = False):
+    """Exactly ``warmup`` untimed and ``steps`` timed steady steps after the two
+    IMEXRK3 start-up steps.  The timed steps replay 12-step HIP graphs and one
+    graph of the remainder, all captured (not run) before the timed region."""
     from jaxfun_amd import PlaneCouetteFluctuationHIP
     from jaxfun_amd.ic import seeded_state
 
     w = WORKLOAD
-    solver = PlaneCouetteFluctuationHIP(N=w["N"], Re=w["Re"], dt=w["dt"], precision=precision)
+    solver = PlaneCouetteFluctuationHIP(N=w["N"], Re=w["Re"], dt=w["dt"], precision=precision, **SBDF3_ROT)
     state = seeded_state(solver, seed=w["seed"], energy=w["energy"], K=w["K"])
     solver.load(state)
     solver.advance(2)  # IMEXRK3 startup, excluded
     block = 0 if eager else GRAPH_BLOCK
     solver.advance_steady(warmup, graph_block=block)
+    if block:
+        solver.prepare_steady(steps, block)
     solver.synchronize()
     t0 = time.perf_counter()
     solver.advance_steady(steps, graph_block=block)
@@ -116,6 +125,32 @@ def run_gpu(precision: str, steps: int, warmup: int, eager: bool = False):
     stages = profile_stages(solver, 48)
     diag = solver.diagnostics()
     return solver, state, (t1 - t0), stages, diag
+
+
+def block_protocol(solver, block_steps: int = 25, timed_blocks: int = 10) -> dict:
+    """The reference's timing protocol (production/benchmark.py:282-354): warm
+    ms per step as the median (and p90) over ``timed_blocks`` blocks of
+    ``block_steps`` steps, each block ended by a device synchronisation.  Each
+    block replays one cached graph of ``block_steps`` steps; a 25-step block
+    moves the rings one phase on (25 = 2 * 12 + 1), so the warm-up runs one
+    block per ring phase (12, where the reference runs 2) and every timed block
+    hits the cache."""
+    import numpy as np
+
+    for _ in range(12):
+        solver.advance_steady(block_steps, graph_block=block_steps)
+    solver.synchronize()
+    per = []
+    for _ in range(timed_blocks):
+        t0 = time.perf_counter()
+        solver.advance_steady(block_steps, graph_block=block_steps)
+        solver.synchronize()
+        per.append((time.perf_counter() - t0) / block_steps)
+    arr = np.asarray(per)
+    return {"block_steps": block_steps, "warmup_blocks": 12, "timed_blocks": timed_blocks,
+            "ms_per_step_p50": 1e3 * float(np.percentile(arr, 50)),
+            "ms_per_step_p90": 1e3 * float(np.percentile(arr, 90)),
+            "steps_per_s_p50": 1.0 / float(np.percentile(arr, 50))}
 
 
 # stage name -> kernel-name fragment (FP64 instance) in the rocprofv3 traffic summary
@@ -202,34 +237,51 @@ def copy_calibration(mb: int) -> dict:
     return {"bytes_read": n, "bytes_written": n}
 
 
-def cpu_baseline(state, budget_s: float) -> dict:
-    """FP64 oracle (numpy/scipy port of the reference algorithm) on host cores."""
+def _cpu_blocks(state, threads: int, budget_s: float) -> dict:
+    """The FP64 oracle on ``threads`` host threads (scipy.fft workers and the
+    BLAS pool): 2 start-up + 2 warm-up steps, then the median of 3 blocks of
+    steady steps, 10 per block unless the warm-up says that 3 blocks would
+    exceed ``budget_s`` (then fewer, at least 2)."""
     import numpy as np
+    from threadpoolctl import threadpool_limits
 
     from oracle.kmm import OracleState, PCFOracle, set_workers
 
+    w = WORKLOAD
+    with threadpool_limits(limits=threads):
+        set_workers(threads)
+        o = PCFOracle(N=w["N"], Re=w["Re"], dt=w["dt"])
+        s = o.reset_history(OracleState(u=tuple(np.array(a) for a in state.u), g=np.array(state.g)))
+        s = o.solve(s, 3)  # two start-up steps and one warm-up step, excluded
+        t0 = time.perf_counter()
+        s = o.step(s)  # second warm-up step, timed to size the blocks
+        t_step = time.perf_counter() - t0
+        per_block = int(max(2, min(10, budget_s / (3 * t_step))))
+        rates = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            s = o.solve(s, per_block)
+            rates.append(per_block / (time.perf_counter() - t0))
+        set_workers(None)
+    return {"value": float(np.median(rates)), "blocks": 3, "steps_per_block": per_block, "threads": threads}
+
+
+def cpu_baseline(state, budget_s: float) -> dict:
+    """FP64 oracle (numpy/scipy port of the reference algorithm) on the host
+    cores this process may use, and on one thread."""
     try:
         cores = len(os.sched_getaffinity(0))
     except AttributeError:  # pragma: no cover
         cores = os.cpu_count() or 1
     cores = min(cores, int(os.environ.get("OMP_NUM_THREADS", cores)))
-    set_workers(cores)
-    w = WORKLOAD
-    o = PCFOracle(N=w["N"], Re=w["Re"], dt=w["dt"])
-    s = OracleState(u=tuple(np.array(a) for a in state.u), g=np.array(state.g))
-    s = o.reset_history(s)
-    s = o.solve(s, 3)  # two startup steps + one warm steady step, excluded
-    n = 0
-    t0 = time.perf_counter()
-    while True:
-        s = o.step(s)
-        n += 1
-        el = time.perf_counter() - t0
-        if n >= 3 and el >= budget_s:
-            break
-    return {"value": n / el, "unit": "steps/s", "cores": cores, "kind": "port",
-            "sample": f"{n} steady SBDF3 steps of the 65x64x64 workload after 2 startup + 1 warm-up "
-                      f"step (scipy.fft workers={cores}, FP64 oracle)"}
+    many = _cpu_blocks(state, cores, budget_s)
+    one = _cpu_blocks(state, 1, budget_s)
+    return {"value": many["value"], "unit": "steps/s", "cores": cores, "kind": "port",
+            "sample": f"median of 3 blocks of {many['steps_per_block']} steady SBDF3 steps of the 65x64x64 "
+                      f"workload after 2 start-up + 2 warm-up steps (FP64 oracle, scipy.fft workers and BLAS "
+                      f"threads = {cores})",
+            "one_thread": {"value": one["value"], "unit": "steps/s", "cores": 1,
+                           "sample": f"median of 3 blocks of {one['steps_per_block']} steady steps, 1 thread"}}
 
 
 def mixed_error(state, steps: int) -> dict:
@@ -243,7 +295,7 @@ def mixed_error(state, steps: int) -> dict:
     w = WORKLOAD
     out = {}
     for prec in ("fp64", "mixed"):
-        s = PlaneCouetteFluctuationHIP(N=w["N"], Re=w["Re"], dt=w["dt"], precision=prec)
+        s = PlaneCouetteFluctuationHIP(N=w["N"], Re=w["Re"], dt=w["dt"], precision=prec, **SBDF3_ROT)
         out[prec] = s.solve(state, steps)
         s.close()
     ref, mix = out["fp64"], out["mixed"]
@@ -262,7 +314,7 @@ def pcie_inclusive(state, steps: int) -> dict:
     from jaxfun_amd import PlaneCouetteFluctuationHIP
 
     w = WORKLOAD
-    s = PlaneCouetteFluctuationHIP(N=w["N"], Re=w["Re"], dt=w["dt"], precision="fp64")
+    s = PlaneCouetteFluctuationHIP(N=w["N"], Re=w["Re"], dt=w["dt"], precision="fp64", **SBDF3_ROT)
     s.solve(state, 2)  # warm the handle (first-touch allocations)
     t0 = time.perf_counter()
     s.solve(state, steps)
@@ -270,6 +322,58 @@ def pcie_inclusive(state, steps: int) -> dict:
     s.close()
     return {"value": steps / el, "unit": "steps/s", "steps": steps,
             "note": "host state in, host state out, startup steps included"}
+
+
+# Mixed precision at the large sizes: 97x96x96 and 129x128x128, Re = 2000,
+# dt = 0.002, seeded perturbation energy 1e-2 over the lowest 12 modes, 2000 steps.
+LARGE = dict(sizes=((97, 96, 96), (129, 128, 128)), Re=2000.0, dt=0.002, energy=1e-2, K=12, seed=0, steps=2000)
+
+
+def large_mixed_leg(steps: int) -> list:
+    """FP64 and mixed precision from the same seeded state for ``steps`` steps
+    (2 IMEXRK3 start-up steps, then steady steps replayed from graphs): steady
+    throughput of each, the final energy, wall shear and divergence, and the
+    mixed run's max relative error per velocity field against the FP64 run."""
+    import numpy as np
+
+    from jaxfun_amd import PlaneCouetteFluctuationHIP
+    from jaxfun_amd.ic import seeded_state
+
+    L = LARGE
+    out = []
+    for N in L["sizes"]:
+        rec = {"N": list(N), "Re": L["Re"], "dt": L["dt"], "energy": L["energy"], "K": L["K"], "steps": steps}
+        finals = {}
+        for prec in ("fp64", "mixed"):
+            solver = PlaneCouetteFluctuationHIP(N=N, Re=L["Re"], dt=L["dt"], precision=prec, **SBDF3_ROT)
+            s0 = seeded_state(solver, seed=L["seed"], energy=L["energy"], K=L["K"])
+            solver.load(s0)
+            solver.advance(2)
+            solver.prepare_steady(steps - 2, GRAPH_BLOCK)
+            solver.synchronize()
+            t0 = time.perf_counter()
+            solver.advance_steady(steps - 2, graph_block=GRAPH_BLOCK)
+            solver.synchronize()
+            el = time.perf_counter() - t0
+            finals[prec] = solver.fetch()
+            d = solver.diagnostics()
+            rec[prec] = {"steady_steps_per_s": (steps - 2) / el, "Epert": d["Epert"], "Etot": d["Etot"],
+                         "wall_shear_top": d["wall_shear_top"], "wall_shear_bot": d["wall_shear_bot"],
+                         "divL2": d["divL2"]}
+            solver.close()
+        a, b = finals["fp64"], finals["mixed"]
+        err = {name: float(np.max(np.abs(x - y)) / np.max(np.abs(y)))
+               for name, x, y in zip(("u0", "u1", "u2", "g"), (*b.u, b.g), (*a.u, a.g))}
+        err["velocity"] = max(err["u0"], err["u1"], err["u2"])
+        err["Epert_rel"] = abs(rec["mixed"]["Epert"] - rec["fp64"]["Epert"]) / rec["fp64"]["Epert"]
+        err["wall_shear_abs"] = max(abs(rec["mixed"][k] - rec["fp64"][k]) for k in ("wall_shear_top", "wall_shear_bot"))
+        rec["mixed_error_vs_fp64"] = err
+        rec["ratio_mixed_vs_fp64"] = rec["mixed"]["steady_steps_per_s"] / rec["fp64"]["steady_steps_per_s"]
+        print(f"# large {N}: fp64 {rec['fp64']['steady_steps_per_s']:.0f}, mixed "
+              f"{rec['mixed']['steady_steps_per_s']:.0f} steps/s; velocity error {err['velocity']:.2e}",
+              file=sys.stderr, flush=True)
+        out.append(rec)
+    return out
 
 
 # Adjoint workload: gradient of the final perturbation energy after 400 SBDF3
@@ -286,7 +390,7 @@ def adjoint_leg(precision: str, reps: int = 3) -> dict:
     from jaxfun_amd.ic import seeded_state
 
     a = ADJOINT
-    solver = PlaneCouetteFluctuationHIP(N=a["N"], Re=a["Re"], dt=a["dt"], precision=precision)
+    solver = PlaneCouetteFluctuationHIP(N=a["N"], Re=a["Re"], dt=a["dt"], precision=precision, **SBDF3_ROT)
     s0 = seeded_state(solver, seed=a["seed"], energy=a["energy"], K=a["K"])
     lib, h = solver._lib, solver._h
 
@@ -323,7 +427,7 @@ def adjoint_fd_check(n_dirs: int = 8, eps: float = 1e-6) -> dict:
     from jaxfun_amd.ic import seeded_state
 
     a = ADJOINT
-    solver = PlaneCouetteFluctuationHIP(N=a["N"], Re=a["Re"], dt=a["dt"], precision="fp64")
+    solver = PlaneCouetteFluctuationHIP(N=a["N"], Re=a["Re"], dt=a["dt"], precision="fp64", **SBDF3_ROT)
     s0 = seeded_state(solver, seed=a["seed"], energy=a["energy"], K=a["K"])
     value, grad, _ = solver.value_and_grad(s0, a["steps"], checkpoint_every=a["every"])
     errs = []
@@ -338,7 +442,7 @@ def adjoint_fd_check(n_dirs: int = 8, eps: float = 1e-6) -> dict:
         an = float(sum(np.real(np.sum(x * y)) for x, y in zip((*grad.u, grad.g), (*d.u, d.g))))
         errs.append(abs(an - fd) / abs(fd))
     solver.close()
-    mixed = PlaneCouetteFluctuationHIP(N=a["N"], Re=a["Re"], dt=a["dt"], precision="mixed")
+    mixed = PlaneCouetteFluctuationHIP(N=a["N"], Re=a["Re"], dt=a["dt"], precision="mixed", **SBDF3_ROT)
     _, gm, _ = mixed.value_and_grad(s0, a["steps"], checkpoint_every=a["every"])
     mixed.close()
     gerr = max(float(np.max(np.abs(x - y)) / np.max(np.abs(y))) for x, y in zip((*gm.u, gm.g), (*grad.u, grad.g)))
@@ -365,7 +469,7 @@ def mhd_leg(steps: int, warmup: int) -> dict:
     for prec in ("fp64", "mixed"):
         solver = PlaneCouetteMRIShearpyInsulatingHIP(N=w["N"], Re=w["Re"], Rm=w["Rm"], omega=w["omega"],
                                                      shear_rate=w["shear_rate"], background_b=w["background_b"],
-                                                     dt=w["dt"], precision=prec)
+                                                     dt=w["dt"], precision=prec, **SBDF3_ROT)
         s0 = seeded_mhd_state(solver, seed=w["seed"], energy_u=w["energy_u"], energy_b=w["energy_b"], K=w["K"])
         solver.load(s0)
         solver.advance(2)
@@ -419,10 +523,11 @@ def published_leg(steps: int = 120, warmup: int = 24) -> list:
         integ = STEPWISE.get(kind, "SBDF3")
         if kind.startswith("hydro"):
             solver = PlaneCouetteFluctuationHIP(N=N, Re=400.0, dt=1e-3, perturbation_amplitude=0.02,
-                                                time_integrator=integ)
+                                                time_integrator=integ, nonlinear_form="rotational")
         else:
             solver = PlaneCouetteMHDHIP(N=N, Re=400.0, Rm=400.0, dt=1e-3, perturbation_amplitude=0.02,
-                                        magnetic_amplitude=0.005, time_integrator=integ)
+                                        magnetic_amplitude=0.005, time_integrator=integ,
+                                        nonlinear_form="rotational")
         solver.load(solver.initial_state())
         if kind in STEPWISE:
             solver.advance(warmup)
@@ -457,6 +562,8 @@ def main(argv=None) -> int:
     ap.add_argument("--error-steps", type=int, default=1000,
                     help="steps of the mixed-vs-FP64 per-field error check (0: skip)")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--large-steps", type=int, default=LARGE["steps"],
+                    help="steps of the 97^3 / 129^3 mixed-vs-FP64 runs (0: skip)")
     ap.add_argument("--no-mhd", action="store_true")
     ap.add_argument("--adjoint-only", action="store_true",
                     help="run only the adjoint timing leg (for kernel profiles)")
@@ -478,8 +585,10 @@ def main(argv=None) -> int:
     if args.adjoint_only:
         print(json.dumps({"adjoint": {p: adjoint_leg(p) for p in ("fp64", "mixed")}}), flush=True)
         return 0
-    steps = max(GRAPH_BLOCK, (args.steps // GRAPH_BLOCK) * GRAPH_BLOCK)
-    warmup = max(GRAPH_BLOCK, (args.warmup // GRAPH_BLOCK) * GRAPH_BLOCK)
+    if args.steps < 1 or args.warmup < 0:
+        print("--steps must be >= 1 and --warmup >= 0", file=sys.stderr)
+        return 2
+    steps, warmup = args.steps, args.warmup
     if args.mhd_only:
         print(json.dumps({"mhd": mhd_leg(min(steps, 1200), warmup)}), flush=True)
         return 0
@@ -505,6 +614,7 @@ def main(argv=None) -> int:
                    "padding": [1.0, 1.5, 1.5], "graph_block_steps": GRAPH_BLOCK},
         "stages_ms": stages,
         "roofline": roofline(solver, "fp64", stages),
+        "protocol": block_protocol(solver),
         "final_diagnostics": diag,
         "adjoint_forward_ratio": None,
     }
@@ -515,7 +625,10 @@ def main(argv=None) -> int:
         mval = steps / mel
         line["mixed"] = {"value": mval, "unit": "steps/s", "ms_per_step": 1e3 * mel / steps,
                          "ratio_vs_fp64": mval / value, "stages_ms": mstages,
-                         "roofline": roofline(msolver, "mixed", mstages)}
+                         "roofline": roofline(msolver, "mixed", mstages),
+                         "protocol": block_protocol(msolver)}
+        line["mixed"]["protocol_ratio_vs_fp64"] = (line["mixed"]["protocol"]["steps_per_s_p50"]
+                                                   / line["protocol"]["steps_per_s_p50"])
         print(f"# mixed: {mval:.1f} steps/s", file=sys.stderr, flush=True)
         msolver.close()
         if args.error_steps:
@@ -539,6 +652,10 @@ def main(argv=None) -> int:
               f"{line['mhd']['mixed']['value']:.1f} steps/s; mixed error "
               f"{max(v for k, v in line['mhd']['mixed_error_vs_fp64'].items() if k != 'steps'):.2e}",
               file=sys.stderr, flush=True)
+    if args.large_steps:
+        line["large_mixed"] = {"workload": "plane-Couette SBDF3 rotational, 97x96x96 and 129x128x128, Re=2000, "
+                                           "dt=0.002, E=1e-2 over the lowest 12 modes, mixed vs FP64",
+                               "runs": large_mixed_leg(args.large_steps)}
     if not args.no_published:
         line["published_configs"] = published_leg()
     if args.copy_calibration:

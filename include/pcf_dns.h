@@ -199,8 +199,13 @@ int32_t pcf_set_dt(pcf_handle *h, double dt);
  * benchmark times.  Replaces _step_sbdf3_steady (channelflow_kmm.py:1016-1064). */
 int32_t pcf_step_steady(pcf_handle *h, int32_t nsteps);
 
-/* Capture nsteps steady steps into a HIP graph owned by the handle and replay
- * it `replays` times (pcf_step_steady semantics). */
+/* nsteps * replays steady steps replayed from HIP graphs of nsteps steps owned
+ * by the handle (pcf_step_steady semantics, history scalars included).  A
+ * graph holds the ring slots of the phase it was captured at, so the handle
+ * caches one per (ring phase, nsteps) and captures on a miss; whole multiples
+ * of 12 steps return the rings to their phase and replay one graph.
+ * replays = 0 only captures the graph for the current phase (nothing runs and
+ * the state is unchanged), so a timed region can start with it cached. */
 int32_t pcf_step_steady_graph(pcf_handle *h, int32_t nsteps, int32_t replays);
 
 /* Diagnostics of the current state (synchronous).  out[8] =
@@ -275,6 +280,8 @@ int32_t pcf_profile_stage(pcf_handle *h, int32_t stage, int32_t reps, double *ms
 #define PCF_SEED_COTANGENT 1
 int32_t pcf_set_cotangent(pcf_handle *h, int32_t field, const double *host);
 int32_t pcf_vjp(pcf_handle *h, int32_t nsteps, int32_t checkpoint_every, int32_t seed);
+/* The gradient with respect to flow leaf `field` (PCF_F_U0 .. PCF_F_SOL_OLDER_W;
+ * any other field is PCF_E_INVALID) after pcf_vjp. */
 int32_t pcf_get_gradient(pcf_handle *h, int32_t field, double *host);
 
 #ifdef __cplusplus

@@ -924,31 +924,78 @@ uint64_t ring_phase(const pcf_handle *h) {
   return key;  // < 2^14
 }
 
-int32_t steady_run(pcf_handle *h, int n, int tape_base) {
-  uint64_t key = ring_phase(h);
-  key |= (uint64_t)n << 16 | (uint64_t)(tape_base + 1) << 40;
-  auto it = h->seg_graphs.find(key);
-  if (it == h->seg_graphs.end()) {
-    hipGraph_t graph = nullptr;
-    PCF_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-    int32_t r = PCF_OK;
-    for (int i = 0; i < n && !r; ++i) r = steady_step(h, tape_base >= 0 ? &h->adj.tape[tape_base + i] : nullptr);
-    hipError_t e = hipStreamEndCapture(h->stream, &graph);
-    if (r) return r;
-    if (e != hipSuccess) return fail(PCF_E_HIP, std::string("capture: ") + hipGetErrorString(e));
-    hipGraphExec_t exec = nullptr;
-    e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(graph);
-    if (e != hipSuccess) return fail(PCF_E_HIP, std::string("instantiate: ") + hipGetErrorString(e));
-    it = h->seg_graphs.emplace(key, exec).first;
-    // the capture advanced the rings and the history state n steps
-  } else {
-    for (int i = 0; i < n; ++i) rotate(h);
-    h->history_steps = 2.0;
-    h->have_old = 1.0;
-    h->previous_dt = h->cfg.dt;
+// The host bookkeeping of n steady steps: ring rotation and the history
+// scalars a steady step leaves (channelflow_kmm.py:1056-1064).
+void advance_host(pcf_handle *h, int n) {
+  for (int i = 0; i < n; ++i) rotate(h);
+  h->history_steps = 2.0;
+  h->have_old = 1.0;
+  h->previous_dt = h->cfg.dt;
+}
+
+// Capture n steady steps (tape slots tape_base + i, none when tape_base < 0)
+// at the rings' current phase into an executable graph.  Capture records the
+// launches without running them; the host ring indices and history scalars,
+// which steady_step advances as it enqueues, are put back afterwards, also
+// when the capture or the instantiation fails (the partial graph is then
+// destroyed and nothing is cached).
+int32_t capture_steady(pcf_handle *h, int n, int tape_base, hipGraphExec_t *out) {
+  int so[3], no[4];
+  std::copy(h->so, h->so + 3, so);
+  std::copy(h->no, h->no + 4, no);
+  const double hs = h->history_steps, ho = h->have_old, pd = h->previous_dt;
+  auto restore = [&]() {
+    std::copy(so, so + 3, h->so);
+    std::copy(no, no + 4, h->no);
+    h->history_steps = hs;
+    h->have_old = ho;
+    h->previous_dt = pd;
+  };
+  hipGraph_t graph = nullptr;
+  hipError_t e = hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal);
+  if (e != hipSuccess) return fail(PCF_E_HIP, std::string("begin capture: ") + hipGetErrorString(e));
+  int32_t r = PCF_OK;
+  for (int i = 0; i < n && !r; ++i) r = steady_step(h, tape_base >= 0 ? &h->adj.tape[tape_base + i] : nullptr);
+  e = hipStreamEndCapture(h->stream, &graph);
+  restore();
+  if (r || e != hipSuccess) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return r ? r : fail(PCF_E_HIP, std::string("capture: ") + hipGetErrorString(e));
   }
-  PCF_HIP(hipGraphLaunch(it->second, h->stream));
+  hipGraphExec_t exec = nullptr;
+  e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (e != hipSuccess) return fail(PCF_E_HIP, std::string("instantiate: ") + hipGetErrorString(e));
+  *out = exec;
+  return PCF_OK;
+}
+
+// The graph of n steady steps at the current ring phase from cache (captured
+// on a miss).  Captured launches hold absolute slot pointers, so one graph
+// serves every run that starts with the rings at the same phase.
+int32_t cached_steady(pcf_handle *h, std::map<uint64_t, hipGraphExec_t> &cache, uint64_t key,
+                      int n, int tape_base, hipGraphExec_t *out) {
+  auto it = cache.find(key);
+  if (it == cache.end()) {
+    hipGraphExec_t exec = nullptr;
+    int32_t r = capture_steady(h, n, tape_base, &exec);
+    if (r) return r;
+    it = cache.emplace(key, exec).first;
+  }
+  *out = it->second;
+  return PCF_OK;
+}
+
+// n steady steps, each keeping its nonlinear evaluation in tape slot
+// tape_base + i (no tape when tape_base < 0), replayed from a graph cached per
+// (ring phase, n, tape_base).
+int32_t steady_run(pcf_handle *h, int n, int tape_base) {
+  const uint64_t key = ring_phase(h) | (uint64_t)n << 16 | (uint64_t)(tape_base + 1) << 40;
+  hipGraphExec_t exec = nullptr;
+  int32_t r = cached_steady(h, h->seg_graphs, key, n, tape_base, &exec);
+  if (r) return r;
+  PCF_HIP(hipGraphLaunch(exec, h->stream));
+  advance_host(h, n);
   return PCF_OK;
 }
 
@@ -1710,29 +1757,20 @@ int32_t pcf_step_steady_graph(pcf_handle *h, int32_t nsteps, int32_t replays) {
   if (!h) return fail(PCF_E_INVALID, "null handle");
   if (h->cfg.time_integrator != PCF_TI_SBDF3) return fail(PCF_E_UNSUPPORTED, "steady steps are SBDF3 steps");
   if (h->history_steps < 2.0) return fail(PCF_E_INVALID, "history not bootstrapped (history_steps < 2)");
-  if (nsteps <= 0 || nsteps % 12) return fail(PCF_E_INVALID, "graph length must be a positive multiple of 12 steps");
+  if (nsteps <= 0 || nsteps > (1 << 20)) return fail(PCF_E_INVALID, "graph length must be in [1, 2^20] steps");
+  if (replays < 0) return fail(PCF_E_INVALID, "replays must be >= 0");
   PCF_HIP(hipSetDevice(h->device));
-  // one graph per ring phase: steps outside the graph (eager, start-up) can
-  // leave the rings at another phase than the one a cached graph was captured at
-  const uint64_t key = ring_phase(h) | (uint64_t)nsteps << 16;
-  auto it = h->graphs.find(key);
-  if (it == h->graphs.end()) {
-    hipGraph_t graph = nullptr;
-    PCF_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-    int32_t r = PCF_OK;
-    for (int s = 0; s < nsteps && !r; ++s) r = steady_step(h);
-    hipError_t e = hipStreamEndCapture(h->stream, &graph);
+  // one graph per (ring phase, length): a replay whose length is not a whole
+  // number of ring periods (12 steps) leaves the rings at another phase, and so
+  // can steps outside the graph (eager, start-up)
+  hipGraphExec_t exec = nullptr;
+  if (replays == 0) return cached_steady(h, h->graphs, ring_phase(h) | (uint64_t)nsteps << 16, nsteps, -1, &exec);
+  for (int rp = 0; rp < replays; ++rp) {
+    int32_t r = cached_steady(h, h->graphs, ring_phase(h) | (uint64_t)nsteps << 16, nsteps, -1, &exec);
     if (r) return r;
-    if (e != hipSuccess) return fail(PCF_E_HIP, std::string("capture: ") + hipGetErrorString(e));
-    hipGraphExec_t exec = nullptr;
-    e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(graph);
-    if (e != hipSuccess) return fail(PCF_E_HIP, std::string("instantiate: ") + hipGetErrorString(e));
-    it = h->graphs.emplace(key, exec).first;
-    // capture recorded the launches without running them and advanced the
-    // ring nsteps times (a whole number of ring periods: back at the start)
+    PCF_HIP(hipGraphLaunch(exec, h->stream));
+    advance_host(h, nsteps);
   }
-  for (int rp = 0; rp < replays; ++rp) PCF_HIP(hipGraphLaunch(it->second, h->stream));
   return PCF_OK;
 }
 
@@ -2038,8 +2076,9 @@ int32_t pcf_vjp(pcf_handle *h, int32_t steps, int32_t checkpoint_every, int32_t 
 }
 
 int32_t pcf_get_gradient(pcf_handle *h, int32_t field, double *host) {
+  // the reverse mode differentiates with respect to the flow leaves only
+  if (field < 0 || field >= PCF_F_AX) return fail(PCF_E_INVALID, "no gradient for this field (flow leaves only)");
   if (!h || !host) return fail(PCF_E_INVALID, "null argument");
-  if (field < 0 || field >= PCF_F_COUNT) return fail(PCF_E_INVALID, "unknown field");
   auto &A = h->adj;
   if (!A.have_grad) return fail(PCF_E_INVALID, "no gradient: call pcf_vjp first");
   const pcf::Geo &g = h->geo;

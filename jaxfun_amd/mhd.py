@@ -59,8 +59,8 @@ class PlaneCouetteMHDHIP(PlaneCouetteFluctuationHIP):
     def __init__(self, N=(17, 16, 16), domain=((-1.0, 1.0), (0.0, 4.0 * math.pi), (0.0, 2.0 * math.pi)),
                  Re: float = 400.0, Rm: float | None = None, U_wall: float = 1.0, dt: float = 0.01,
                  family: str = "C", padding_factor=(1.0, 1.5, 1.5), perturbation_amplitude: float = 0.1,
-                 magnetic_amplitude: float = 0.05, time_integrator: str | None = "SBDF3",
-                 nonlinear_form: str = "rotational", coefficient_path: str = "optimized",
+                 magnetic_amplitude: float = 0.05, time_integrator: str | None = None,
+                 nonlinear_form: str = "gradient", coefficient_path: str = "transform",
                  solve_batching: str = "batched", background_b=(0.0, 0.0, 0.0), precision: str = "fp64",
                  device: int = 0) -> None:
         self.Rm = float(Re if Rm is None else Rm)
@@ -123,18 +123,17 @@ class PlaneCouetteMHDHIP(PlaneCouetteFluctuationHIP):
 
     # ------------------------------------------------------------ device I/O
     def _device_mhd_history(self, state: MHDState) -> MHDState:
-        """``_ensure_mhd_history`` (``pcf_mhd_jax.py:217-248``): SBDF3 restarts
-        flow and A histories together unless every level is present; CNAB2
-        starts a missing ``nonlinear_A_old`` at zero (the flow rule of
-        ``_device_history`` applies to ``flow``).  Levels the step does not
-        read travel as zeros."""
+        """``_ensure_mhd_history`` (``pcf_mhd_jax.py:217-248``): under SBDF3 the
+        flow history follows ``_ensure_flow_history`` (restarted when a flow
+        level is missing) and a complete A history is kept as loaded; a
+        missing A level restarts flow and A together.  CNAB2 starts a missing
+        ``nonlinear_A_old`` at zero (the flow rule of ``_device_history``
+        applies to ``flow``).  Levels the step does not read travel as zeros."""
         flow = state.flow
         if self.time_integrator == "SBDF3":
-            flow_ok = all(getattr(flow, k) is not None
-                          for k in ("nonlinear_old", "nonlinear_older", "solution_old", "solution_older"))
-            if not flow_ok or any(getattr(state, name) is None for name, _ in _A_GROUPS[1:]):
-                return self._new_state(flow, state.A)
-            return state
+            if all(getattr(state, name) is not None for name, _ in _A_GROUPS[1:]):
+                return replace(state, flow=self._device_history(flow))
+            return self._new_state(flow, state.A)
         z = self._zero_A()
         return replace(state, flow=self._device_history(flow),
                        **{name: z for name, _ in _A_GROUPS[1:] if getattr(state, name) is None})
@@ -218,8 +217,8 @@ class PlaneCouetteMRIShearpyHIP(PlaneCouetteMHDHIP):
                  background_b=(0.0, 0.0, 0.025), dt: float = 0.01, family: str = "C",
                  padding_factor=(1.0, 1.5, 1.5), perturbation_amplitude: float = 0.05,
                  magnetic_amplitude: float = 0.0, magnetic_seed: str = "ax_yz",
-                 solenoidal_velocity_seed: bool = False, time_integrator: str | None = "SBDF3",
-                 nonlinear_form: str = "rotational", coefficient_path: str = "optimized",
+                 solenoidal_velocity_seed: bool = False, time_integrator: str | None = None,
+                 nonlinear_form: str = "gradient", coefficient_path: str = "transform",
                  solve_batching: str = "batched", precision: str = "fp64", device: int = 0) -> None:
         self.omega = float(omega)
         self.shear_rate = float(shear_rate)

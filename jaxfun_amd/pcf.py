@@ -51,12 +51,13 @@ class PlaneCouetteFluctuationHIP:
     """KMM SBDF3 plane-Couette solver whose step runs in HIP on one GPU.
 
     Constructor follows ``PlaneCouetteFluctuationJax.__init__``
-    (``examples/pcf_fluctuations_jax.py:35-75``).  Built options:
-    Chebyshev family; SBDF3 with IMEXRK3 startup, IMEXRK3 alone, IMEXRK222 or
-    CNAB2 (``time_integrator=None`` selects IMEXRK222 as in the reference); rotational
-    nonlinear form; optimized coefficient path.  Other valid reference values raise
-    ``NotImplementedError``; invalid values raise ``ValueError`` as in
-    ``examples/channelflow_kmm.py:140-187``.
+    (``examples/pcf_fluctuations_jax.py:35-75``), defaults included.  Built
+    options: Chebyshev family; SBDF3 with IMEXRK3 startup, IMEXRK3 alone,
+    IMEXRK222 or CNAB2 (``time_integrator=None`` selects IMEXRK222 as in the
+    reference); the rotational nonlinear form; both coefficient paths (they
+    form identical rows).  The reference's default ``nonlinear_form="gradient"``
+    and the Legendre family raise ``NotImplementedError``; invalid values raise
+    ``ValueError`` as in ``examples/channelflow_kmm.py:140-187``.
     """
 
     def __init__(
@@ -70,9 +71,9 @@ class PlaneCouetteFluctuationHIP:
         padding_factor=(1.0, 1.5, 1.5),
         perturbation_amplitude: float = 0.05,
         timestepper=None,
-        time_integrator: str | None = "SBDF3",
-        nonlinear_form: str = "rotational",
-        coefficient_path: str = "optimized",
+        time_integrator: str | None = None,
+        nonlinear_form: str = "gradient",
+        coefficient_path: str = "transform",
         solve_batching: str = "batched",
         precision: str = "fp64",
         device: int = 0,
@@ -96,9 +97,11 @@ class PlaneCouetteFluctuationHIP:
         if time_integrator is None:  # the reference's default ARS stepper (channelflow_kmm.py:160-172)
             time_integrator = "IMEXRK222"
         if nonlinear_form != "rotational":
-            raise NotImplementedError("only the rotational nonlinear form is built")
-        if coefficient_path != "optimized":
-            raise NotImplementedError("only the optimized coefficient path is built")
+            raise NotImplementedError("only the rotational nonlinear form is built; pass "
+                                      "nonlinear_form='rotational'")
+        # coefficient_path "transform" (channelflow_kmm.py:785-803,831-847) forms the
+        # same rows as "optimized" (:810-829,855-870): every quadrature on the
+        # unpadded grid is exact for these degrees, so both run the same kernels.
         if str(precision).lower() not in _PRECISIONS:
             raise ValueError("precision must be 'fp64' or 'mixed'")
         if tuple(float(v) for v in domain[0]) != (-1.0, 1.0):
@@ -328,14 +331,32 @@ class PlaneCouetteFluctuationHIP:
         nat.check(self._lib.pcf_step(self._h, int(steps)))
 
     def advance_steady(self, steps: int, graph_block: int = 0) -> None:
-        """Steady SBDF3 steps only; with graph_block (multiple of 12) replay a HIP graph."""
+        """Exactly ``steps`` steady SBDF3 steps.  With ``graph_block`` they replay
+        HIP graphs: whole blocks of ``graph_block`` steps, then one graph of the
+        remainder (graphs are cached per ring phase and length)."""
         self._loaded = None
+        steps = int(steps)
         if graph_block:
-            if steps % graph_block:
-                raise ValueError("steps must be a multiple of graph_block")
-            nat.check(self._lib.pcf_step_steady_graph(self._h, int(graph_block), steps // graph_block))
+            q, r = divmod(steps, int(graph_block))
+            if q:
+                nat.check(self._lib.pcf_step_steady_graph(self._h, int(graph_block), q))
+            if r:
+                nat.check(self._lib.pcf_step_steady_graph(self._h, r, 1))
         else:
-            nat.check(self._lib.pcf_step_steady(self._h, int(steps)))
+            nat.check(self._lib.pcf_step_steady(self._h, steps))
+
+    def prepare_steady(self, steps: int, graph_block: int) -> None:
+        """Capture, without running anything, the graphs that
+        ``advance_steady(steps, graph_block)`` replays from the current ring
+        phase; with ``graph_block`` a multiple of 12 the remainder graph starts
+        at the same phase, so the whole run then replays cached graphs."""
+        q, r = divmod(int(steps), int(graph_block))
+        if graph_block % 12:
+            raise ValueError("graph_block must be a multiple of 12 steps")
+        if q:
+            nat.check(self._lib.pcf_step_steady_graph(self._h, int(graph_block), 0))
+        if r:
+            nat.check(self._lib.pcf_step_steady_graph(self._h, r, 0))
 
     def synchronize(self) -> None:
         nat.check(self._lib.pcf_synchronize(self._h))

@@ -608,6 +608,42 @@ class PCFOracle:
         Nw = -(self.M00 @ np.real(H[2][:, 0, 0]))
         return (Nu, Ng, Nv, Nw)
 
+    def tp_scalar_product(self, u, S):
+        """``TensorProductSpace.scalar_product`` on the unpadded grid
+        (``tensorproductspace.py:501-526``): the Fourier scalar products
+        (forward sums scaled by 2 pi / df, ``Fourier.py:153-167``), then the
+        weighted Chebyshev scalar products (``Chebyshev.py:303-318``) mapped
+        onto the basis rows of S."""
+        z = fourier_forward(u, 2, self.N[2])
+        z = fourier_forward(z, 1, self.N[1])
+        return self._apply0(S, cheb_scalar_product(z, self.N0)) * self.W
+
+    def nonlinear_rhs_transform(self, H):
+        """``_nonlinear_rhs_physical_reference`` (``channelflow_kmm.py:785-803``),
+        the ``coefficient_path="transform"`` rows: derivatives of H taken to the
+        unpadded quadrature grid, then scalar products with the test bases."""
+        SD = self.SD
+        Hu = (self.tp_backward(H[1], SD, padded=False, kx=1, k1=1)
+              + self.tp_backward(H[2], SD, padded=False, kx=1, k2=1)
+              - self.tp_backward(H[0], SD, padded=False, k1=2)
+              - self.tp_backward(H[0], SD, padded=False, k2=2))
+        Hg = self.tp_backward(H[1], SD, padded=False, k2=1) - self.tp_backward(H[2], SD, padded=False, k1=1)
+        Nu = self.tp_scalar_product(Hu, self.SB)
+        Ng = self.tp_scalar_product(Hg, self.SD)
+        Nv = -(self.M00 @ np.real(H[1][:, 0, 0]))
+        Nw = -(self.M00 @ np.real(H[2][:, 0, 0]))
+        return (Nu, Ng, Nv, Nw)
+
+    def reconstruct_transform(self, u0, g, v00, w00):
+        """``_reconstruct_velocity_physical_reference`` (``channelflow_kmm.py:831-847``):
+        f = TD.forward of d(u0)/dx sampled on the unpadded grid."""
+        f = self.tp_forward(self.tp_backward(u0, self.SB, padded=False, kx=1), self.SD, self.MD1)
+        u1 = 1j * (self.KoK2[0] * f + self.KoK2[1] * g)
+        u2 = 1j * (self.KoK2[1] * f - self.KoK2[0] * g)
+        u1[:, 0, 0] = np.real(v00)
+        u2[:, 0, 0] = np.real(w00)
+        return (self.mask(u0), self.mask(u1), self.mask(u2))
+
     def reconstruct(self, u0, g, v00, w00):
         """``_reconstruct_velocity_coefficient`` (``channelflow_kmm.py:855-870``)."""
         a = cheb_derivative(self.to_orth(u0, self.SB), 1)

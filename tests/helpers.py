@@ -124,3 +124,34 @@ def insulating_oracle_for(solver):
                                perturbation_amplitude=solver.perturbation_amplitude,
                                magnetic_amplitude=solver.magnetic_amplitude,
                                time_integrator=solver.time_integrator)
+
+
+# ------------------------------------------------------------------ host-only geometry
+class HostSolver:
+    """The host-side geometry of PlaneCouetteFluctuationHIP without a GPU handle:
+    enough for the seeded initial conditions (``jaxfun_amd.ic``), which need
+    only the closed-form setup matrices of the library."""
+
+    def __init__(self, N, Ly=4 * math.pi, Lz=2 * math.pi):
+        from jaxfun_amd import _native as nat
+        from jaxfun_amd.pcf import PlaneCouetteFluctuationHIP as S
+
+        self.N = tuple(N)
+        self.Ly, self.Lz = Ly, Lz
+        self.domain = ((-1.0, 1.0), (0.0, Ly), (0.0, Lz))
+        c = nat.PcfConfig()
+        c.abi_version = nat.ABI_VERSION
+        c.n0, c.ny, c.nz = self.N
+        c.lx[0], c.lx[1] = -1.0, 1.0
+        c.ly, c.lz = Ly, Lz
+        c.nu, c.u_wall, c.dt = 1 / 400, 1.0, 1e-3
+        c.padding_y = c.padding_z = 1.5
+        c.precision = nat.PREC_FP64
+        c.time_integrator = nat.TI_SBDF3
+        c.nonlinear_form = nat.NL_ROTATIONAL
+        self._cfg = c
+        self.matrix = lambda name: nat.setup_matrix(self._cfg, name)
+        self.wavenumbers = S.wavenumbers.__get__(self)
+        self.mask_nyquist = S.mask_nyquist.__get__(self)
+        self._with_zero_history = S._with_zero_history.__get__(self)
+        self.nb, self.nd = self.N[0] - 4, self.N[0] - 2

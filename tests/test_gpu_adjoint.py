@@ -25,7 +25,8 @@ GRAD_TOL = 1e-12
 
 
 def make(N, Re=400.0, dt=2e-3, precision="fp64"):
-    return PlaneCouetteFluctuationHIP(N=N, Re=Re, dt=dt, precision=precision)
+    return PlaneCouetteFluctuationHIP(N=N, Re=Re, dt=dt, precision=precision, time_integrator="SBDF3",
+                                      nonlinear_form="rotational")
 
 
 def _mask(solver, a):

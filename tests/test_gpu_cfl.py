@@ -28,7 +28,8 @@ def _check(solver, cfl, cfl_o, tol):
 @pytest.mark.parametrize("N", [(17, 16, 16), (24, 16, 32), (33, 24, 48)], ids=["17", "24x16x32", "33x24x48"])
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_hydro_cfl_matches_oracle(N, precision):
-    solver = PlaneCouetteFluctuationHIP(N=N, Re=400.0, dt=2e-3, perturbation_amplitude=0.05, precision=precision)
+    solver = PlaneCouetteFluctuationHIP(N=N, Re=400.0, dt=2e-3, perturbation_amplitude=0.05, precision=precision,
+                                        time_integrator="SBDF3", nonlinear_form="rotational")
     oracle = oracle_for(solver)
     s = seeded_state(solver, seed=7, energy=5e-2, K=4)
     s = solver.solve(s, 3)

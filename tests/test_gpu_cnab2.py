@@ -27,7 +27,7 @@ HIST_TOL = 5e-11
 
 def make(N, integ="CNAB2", precision="fp64", dt=2e-3):
     return PlaneCouetteFluctuationHIP(N=N, Re=400.0, dt=dt, perturbation_amplitude=0.05, time_integrator=integ,
-                                      precision=precision)
+                                      nonlinear_form="rotational", precision=precision)
 
 
 def _group_err(xs, ys):

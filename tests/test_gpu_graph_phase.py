@@ -12,8 +12,10 @@ pytestmark = pytest.mark.gpu
 
 
 def test_graph_replay_after_phase_shift_matches_eager():
-    a = PlaneCouetteFluctuationHIP(N=(17, 16, 16), Re=400.0, dt=2e-3)
-    b = PlaneCouetteFluctuationHIP(N=(17, 16, 16), Re=400.0, dt=2e-3)
+    a = PlaneCouetteFluctuationHIP(N=(17, 16, 16), Re=400.0, dt=2e-3, time_integrator="SBDF3",
+                                   nonlinear_form="rotational")
+    b = PlaneCouetteFluctuationHIP(N=(17, 16, 16), Re=400.0, dt=2e-3, time_integrator="SBDF3",
+                                   nonlinear_form="rotational")
     s = seeded_state(a, seed=6, energy=5e-2, K=4)
     for x in (a, b):
         x.load(s)

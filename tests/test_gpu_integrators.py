@@ -25,7 +25,7 @@ TOL = 1e-12
 
 def make(N, integ, precision="fp64"):
     return PlaneCouetteFluctuationHIP(N=N, Re=400.0, dt=2e-3, perturbation_amplitude=0.05, time_integrator=integ,
-                                      precision=precision)
+                                      nonlinear_form="rotational", precision=precision)
 
 
 def _state_err(s, so):

@@ -29,7 +29,7 @@ HIST_TOL = 5e-11
 def make(N, precision="fp64", **kw):
     c = GOLDEN["config"]
     args = dict(Re=c["Re"], Rm=c["Rm"], dt=c["dt"], perturbation_amplitude=c["velocity_amplitude"],
-                magnetic_amplitude=c["magnetic_amplitude"])
+                magnetic_amplitude=c["magnetic_amplitude"], time_integrator="SBDF3", nonlinear_form="rotational")
     args.update(kw)
     return PlaneCouetteMHDHIP(N=N, precision=precision, **args)
 
@@ -118,7 +118,8 @@ def make_insulating(N, precision="fp64", **kw):
     from jaxfun_amd import PlaneCouetteMRIShearpyInsulatingHIP
 
     args = dict(Re=400.0, Rm=300.0, omega=0.4, shear_rate=1.5, background_b=(0.0, 0.02, 0.1), dt=2e-3,
-                perturbation_amplitude=0.05, magnetic_amplitude=0.3)
+                perturbation_amplitude=0.05, magnetic_amplitude=0.3, time_integrator="SBDF3",
+                nonlinear_form="rotational")
     args.update(kw)
     return PlaneCouetteMRIShearpyInsulatingHIP(N=N, precision=precision, **args)
 
@@ -187,7 +188,8 @@ def make_shear(N, precision="fp64", **kw):
     from jaxfun_amd import PlaneCouetteMRIShearpyHIP
 
     args = dict(Re=400.0, Rm=300.0, omega=0.4, shear_rate=1.5, background_b=(0.0, 0.02, 0.1), dt=2e-3,
-                perturbation_amplitude=0.05, magnetic_amplitude=0.3)
+                perturbation_amplitude=0.05, magnetic_amplitude=0.3, time_integrator="SBDF3",
+                nonlinear_form="rotational")
     args.update(kw)
     return PlaneCouetteMRIShearpyHIP(N=N, precision=precision, **args)
 

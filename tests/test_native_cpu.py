@@ -189,24 +189,15 @@ def test_python_boundary_errors_match_reference():
     with pytest.raises(NotImplementedError):
         S(nonlinear_form="gradient")
     with pytest.raises(NotImplementedError):
-        S(coefficient_path="transform")
+        S()  # the reference's defaults: gradient form, transform path, IMEXRK222
+    with pytest.raises(NotImplementedError):
+        S(nonlinear_form="rotational", family="L")
 
 
-class _HostSolver:
-    """Just the host-side geometry of PlaneCouetteFluctuationHIP (no GPU handle)."""
+def _HostSolver(N):
+    from tests.helpers import HostSolver
 
-    def __init__(self, N):
-        from jaxfun_amd.pcf import PlaneCouetteFluctuationHIP as S
-
-        self.N = N
-        self.Ly, self.Lz = 4 * math.pi, 2 * math.pi
-        self.domain = ((-1.0, 1.0), (0.0, self.Ly), (0.0, self.Lz))
-        self._cfg = cfg_for(*N)
-        self.matrix = lambda name: nat.setup_matrix(self._cfg, name)
-        self.wavenumbers = S.wavenumbers.__get__(self)
-        self.mask_nyquist = S.mask_nyquist.__get__(self)
-        self._with_zero_history = S._with_zero_history.__get__(self)
-        self.nb, self.nd = N[0] - 4, N[0] - 2
+    return HostSolver(N)
 
 
 def test_seeded_state_properties():
@@ -305,3 +296,49 @@ def test_product_cfl_arithmetic_matches_oracle():
     assert set(got) == set(want)
     for k, v in want.items():
         assert math.isclose(got[k], v, rel_tol=1e-14, abs_tol=1e-300), (k, got[k], v)
+
+
+def test_mhd_sbdf3_history_keeps_complete_A_levels():
+    """_ensure_mhd_history (examples/pcf_mhd_jax.py:217-231) under SBDF3: an
+    incomplete flow history restarts the flow alone and a complete A history is
+    kept as loaded; a missing A level restarts flow and A together."""
+    from dataclasses import replace
+
+    from jaxfun_amd.ic import seeded_state
+    from jaxfun_amd.mhd import MHDState, PlaneCouetteMHDHIP as M
+    from jaxfun_amd.pcf import KMMState
+    from jaxfun_amd.pcf import PlaneCouetteFluctuationHIP as S
+
+    hs = _HostSolver((17, 16, 16))
+    hs.time_integrator = "SBDF3"
+    for name in ("_device_history",):
+        setattr(hs, name, getattr(S, name).__get__(hs))
+    for name in ("_device_mhd_history", "_new_state", "_zero_A", "_a_rows"):
+        setattr(hs, name, getattr(M, name).__get__(hs))
+    s = seeded_state(hs, seed=1, energy=1e-3, K=4)
+    flow = KMMState(u=s.u, g=s.g, nonlinear_old=tuple(np.ones_like(a) for a in s.nonlinear_old),
+                    history_steps=2.0, have_old=1.0, previous_dt=0.5)
+    A = tuple(np.full((hs.nd, 16, 16), 2.0 + 0j) for _ in range(3))
+    lvl = {k: tuple(np.full((hs.nd, 16, 16), v + 0j) for _ in range(3))
+           for k, v in (("nonlinear_A_old", 3.0), ("nonlinear_A_older", 4.0), ("A_old", 5.0), ("A_older", 6.0))}
+    st = MHDState(flow=flow, A=A, **lvl)
+    d = hs._device_mhd_history(st)
+    assert d.flow.history_steps == 0.0 and d.flow.have_old == 0.0 and d.flow.previous_dt == 0.0
+    assert all(np.all(a == 0) for a in d.flow.nonlinear_old)
+    for k, v in (("nonlinear_A_old", 3.0), ("nonlinear_A_older", 4.0), ("A_old", 5.0), ("A_older", 6.0)):
+        assert all(np.all(a == v) for a in getattr(d, k)), k
+    d = hs._device_mhd_history(replace(st, A_older=None))
+    assert all(np.all(a == 0) for k in lvl for a in getattr(d, k))
+    assert d.flow.history_steps == 0.0
+
+
+def test_gradient_rejects_vector_potential_fields():
+    """The reverse mode has flow leaves only: pcf_get_gradient refuses the A
+    fields (and unknown ids) before it looks at the handle."""
+    lib = nat.load()
+    buf = (C.c_double * 8)()
+    for f in (nat.FIELD_ID["AX"], nat.FIELD_ID["A_OLDER_Z"], -1, 999):
+        assert lib.pcf_get_gradient(None, f, buf) == -1  # PCF_E_INVALID
+        assert b"flow leaves only" in lib.pcf_last_error()
+    assert lib.pcf_get_gradient(None, nat.FIELD_ID["U0"], buf) == -1  # PCF_E_INVALID
+    assert b"null" in lib.pcf_last_error()

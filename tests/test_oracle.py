@@ -364,3 +364,26 @@ def test_field_maxima_match_direct_series_sum():
     assert bmax == (0.0, 0.0, 0.0)
     dx = float(np.min(np.diff(np.sort(x))))
     assert o.grid_spacings() == (dx, o.L1 / n1, o.L2 / n2)
+
+
+@pytest.mark.parametrize("N", [(17, 16, 16), (24, 16, 32)])
+def test_transform_coefficient_path_equals_optimized(N):
+    """coefficient_path="transform" (channelflow_kmm.py:785-803,831-847) and the
+    optimized path (:810-829,855-870) form the same nonlinear rows and the same
+    reconstruction: every quadrature on the unpadded grid is exact for these
+    polynomial degrees, so the two agree to round-off on arbitrary fields."""
+    o = PCFOracle(N=N, Re=500.0, dt=1e-3)
+    rng = np.random.default_rng(11)
+
+    def field(rows):
+        c = rng.standard_normal((rows,) + N[1:]) + 1j * rng.standard_normal((rows,) + N[1:])
+        return o.mask(c)
+
+    H = tuple(field(o.ND) for _ in range(3))
+    a, b = o.nonlinear_rhs_transform(H), o.nonlinear_rhs(H)
+    for x, y in zip(a, b):
+        assert np.max(np.abs(x - y)) < 1e-12 * np.max(np.abs(y))
+    u0, g = field(o.NB), field(o.ND)
+    v, w = rng.standard_normal(o.ND), rng.standard_normal(o.ND)
+    for x, y in zip(o.reconstruct_transform(u0, g, v, w), o.reconstruct(u0, g, v, w)):
+        assert np.max(np.abs(x - y)) < 1e-12 * np.max(np.abs(y))
