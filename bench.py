@@ -61,8 +61,10 @@ def stage_model(solver, precision: str) -> dict:
     my = int(1.5 * ny)
     V = 16 if precision == "fp64" else 8
     c16 = 16
-    nodal_b = (nb + 2 * nd) * m * c16 + 6 * n0 * m * V
-    nodal_f = m * n0 * (4 * nb + 16 * nd)
+    # (u0, g, v00, w00) in (the reconstruction of u1, u2 is folded into the
+    # contraction: five products, three over u0 and two over g), 6 nodal fields out
+    nodal_b = (nb + nd) * m * c16 + 6 * n0 * m * V
+    nodal_f = m * n0 * (12 * nb + 8 * nd)
     ffti_b = 6 * n0 * m * V + 6 * n0 * my * kzs * V
     mz = int(1.5 * nz)
     fft_flops = lambda n, count: 5.0 * n * __import__("math").log2(n) * count  # noqa: E731
@@ -78,8 +80,6 @@ def stage_model(solver, precision: str) -> dict:
     # Dirichlet ones, FP64)
     solve_b = 7 * (nb + nd) * m * c16 + (7 * nb + 4 * nd) * m * 8
     solve_f = m * 2 * (nb + nd) * 40  # O(n) structured sweeps, ~40 flops per row
-    recon_b = (nb + nd) * m * c16 + 2 * nd * m * c16
-    recon_f = m * nd * (nb // 2) * 4
     return {
         "nodal_backward": (nodal_b, nodal_f),
         "fft_y_inverse": (ffti_b, ffti_f),
@@ -87,7 +87,6 @@ def stage_model(solver, precision: str) -> dict:
         "fft_y_forward": (fftf_b, fftf_f),
         "forward_rhs": (rhs_b, rhs_f),
         "solve": (solve_b, solve_f),
-        "reconstruct": (recon_b, recon_f),
     }
 
 
@@ -154,9 +153,9 @@ def block_protocol(solver, block_steps: int = 25, timed_blocks: int = 10) -> dic
 
 
 # stage name -> kernel-name fragment (FP64 instance) in the rocprofv3 traffic summary
-_KERNEL_OF = {"nodal_backward": "k_nodal_mfma<double>", "fft_y_inverse": "k_fft_y<double, true",
+_KERNEL_OF = {"nodal_backward": "k_nodal_prim_mfma<double>", "fft_y_inverse": "k_fft_y<double, true",
               "rotational": "k_rotational<double", "fft_y_forward": "k_fft_y<double, false",
-              "forward_rhs": "k_rhs_mfma<double>", "solve": "k_solve_factored", "reconstruct": "k_reconstruct"}
+              "forward_rhs": "k_rhs_mfma<double>", "solve": "k_solve_factored"}
 TRAFFIC_FILE = ROOT / "profiles" / "traffic_65x64x64.json"
 
 
@@ -178,7 +177,7 @@ def measured_traffic(stage: str, precision: str):
 def launch_ms(solver, stage: str, reps: int = 200) -> float | None:
     """Mean duration of one launch of a nonlinear stage: `reps` back-to-back
     launches between two HIP events on the library's stream (pcf_profile_stage);
-    None for the solve and reconstruction, whose repeats are not idempotent."""
+    None for the solve, whose repeats are not idempotent."""
     lib = solver._lib
     names = [lib.pcf_stage_name(i).decode() for i in range(lib.pcf_stage_count())]
     idx = names.index(stage)

@@ -157,6 +157,8 @@ int64_t pcf_field_size(const pcf_config *cfg, int32_t field);
  *   17 VC, 18 VC1, 19 VC2: Chebyshev values T_j and derivatives T_j', T_j''
  *      at the nodes (n0 x n0); 20 WQ: Chebyshev scalar products of nodal
  *      values (T_k, f) (n0 x n0) — the TC transforms of insulating walls
+ *   21 VDR = VD R, 22 VD1R = VD1 R: nodal values of the reconstructed
+ *      f = R u0 and of its x-derivative (n0 x n0-4)
  * Replaces the SymPy inner() assembly of examples/channelflow_kmm.py:261-409. */
 int32_t pcf_setup_matrix(const pcf_config *cfg, int32_t which, double *out,
                          int32_t *rows, int32_t *cols);

@@ -45,7 +45,7 @@ FIELD_ID = {name: i for i, name in enumerate(FIELDS)}
 MATRIX_ID = {
     "C0B": 0, "C2B": 1, "C4B": 2, "C0D": 3, "C2D": 4, "VB": 5, "VD": 6, "VD1": 7,
     "A1": 8, "A0": 9, "AG": 10, "R": 11, "VB1": 12, "FB": 13, "FD": 14, "VQ1": 15, "VQ2": 16,
-    "VC": 17, "VC1": 18, "VC2": 19, "WQ": 20,
+    "VC": 17, "VC1": 18, "VC2": 19, "WQ": 20, "VDR": 21, "VD1R": 22,
 }
 
 # Every symbol include/pcf_dns.h declares.

@@ -91,6 +91,11 @@ hipError_t launch_prim_sum(const Geo &g, const PrimSum &in, Prim out, hipStream_
 template <typename T>
 hipError_t launch_nodal_mfma(const Geo &g, const NodalMats<T> &nm, const double2 *u0, const double2 *u1,
                              const double2 *u2, Workspace &ws, hipStream_t s);
+// The nodal fields from (u0, g, v00, w00) with the velocity reconstruction
+// folded in (needs NodalMats::VDRt, VD1Rt).
+template <typename T>
+hipError_t launch_nodal_prim_mfma(const Geo &g, const NodalMats<T> &nm, const double2 *u0, const double2 *gf,
+                                  const double *v00, const double *w00, Workspace &ws, hipStream_t s);
 template <typename T>
 hipError_t launch_rhs_mfma(const Geo &g, const NodalMats<T> &nm, Workspace &ws, double2 *Nu, double2 *Ng,
                            double *Nv, double *Nw, hipStream_t s);

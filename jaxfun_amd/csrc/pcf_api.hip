@@ -85,7 +85,7 @@ double dfactor(double L) {
 }
 
 struct StageTimer {
-  hipEvent_t ev[7][2];
+  hipEvent_t ev[6][2];
 };
 
 template <typename T>
@@ -94,6 +94,7 @@ struct DevMats {
     *AGt = nullptr, *xnode = nullptr, *twy = nullptr, *twz = nullptr;
   T *VQ1t = nullptr, *VQ2t = nullptr, *VCt = nullptr, *VC1t = nullptr, *VC2t = nullptr, *WQt = nullptr;
   T *twy0 = nullptr, *twz0 = nullptr;  // twiddles of the unpadded grid (ny, nz)
+  T *VDRt = nullptr, *VD1Rt = nullptr;
   pcf::NodalMats<T> view{};
   pcf::MhdMats<T> mview{};  // MHD contractions (conducting or insulating walls)
 };
@@ -142,6 +143,19 @@ struct pcf_handle {
   int so[3] = {0, 1, 2};     // solution ring: cur, old, older
   int no[4] = {0, 1, 2, 3};  // nonlinear ring: free0, old, older, free1
   double history_steps = 0, have_old = 0, previous_dt = 0;
+  // u1/u2 of the current state.  A state loaded through pcf_set_field carries
+  // its own u1, u2 (u12_given: the next nonlinear evaluation reads them, as the
+  // reference's convection reads state.u).  A solve writes only (u0, g, v00,
+  // w00); the next evaluation then forms u1, u2 inside its nodal contraction
+  // (channelflow_kmm.py:855-870 folded into it), and the buffers are filled by
+  // the reconstruction kernel only when something reads them (u12_valid).
+  bool u12_given = true, u12_valid = true;
+  // A loaded state whose u1, u2 equal, bit for bit, what the reconstruction
+  // makes of its (u0, g, v00, w00) -- a state this library produced and
+  // handed out -- is stepped as if it had never left the device, so that a
+  // resume from it is exact; u12_checked records that the test was made.
+  bool u12_checked = true;
+  unsigned *dflag = nullptr;
   pcf::Workspace ws{};
   DevMats<double> md;
   DevMats<float> mf;
@@ -163,6 +177,7 @@ struct pcf_handle {
     char *dev = nullptr;
     int so[3], no[4];
     double hs, ho, pd;
+    bool given, valid;
   };
   struct Adj {
     bool ready = false;
@@ -240,6 +255,8 @@ int32_t build_nodal(pcf_handle *h, DevMats<T> &dm, int xc, int ic) {
   if ((r = h->upload(&dm.VBt, transpose_pad(o.VB, ldx)))) return r;
   if ((r = h->upload(&dm.VDt, transpose_pad(o.VD, ldx)))) return r;
   if ((r = h->upload(&dm.VD1t, transpose_pad(o.VD1, ldx)))) return r;
+  if ((r = h->upload(&dm.VDRt, transpose_pad(o.VDR, ldx)))) return r;
+  if ((r = h->upload(&dm.VD1Rt, transpose_pad(o.VD1R, ldx)))) return r;
   if ((r = h->upload(&dm.A1t, transpose_pad(o.A1, ldi)))) return r;
   if ((r = h->upload(&dm.A0t, transpose_pad(o.A0, ldi)))) return r;
   if ((r = h->upload(&dm.AGt, transpose_pad(o.AG, ldi)))) return r;
@@ -273,7 +290,7 @@ int32_t build_nodal(pcf_handle *h, DevMats<T> &dm, int xc, int ic) {
     if ((r = h->upload(&dm.twz0, twiddles(h->geo.nz)))) return r;
   }
   dm.view = pcf::NodalMats<T>{dm.VBt, dm.VDt, dm.VD1t, ldx, dm.A1t, dm.A0t, dm.AGt, ldi,
-                              dm.xnode, dm.twy, dm.twz};
+                              dm.xnode, dm.twy, dm.twz, dm.VDRt, dm.VD1Rt};
   if (h->mhd) {
     pcf::MhdMats<T> &mm = dm.mview;
     const double W = h->geo.W;
@@ -348,9 +365,9 @@ int32_t build_factors_A(pcf_handle *h, double gamma, pcf::FactorSet &fs) {
 }
 
 // Stage names, in launch order, for per-kernel timing.
-constexpr int NSTAGE = 7;
+constexpr int NSTAGE = 6;
 const char *const STAGE_NAMES[NSTAGE] = {"nodal_backward", "fft_y_inverse", "rotational",
-                                         "fft_y_forward", "forward_rhs", "solve", "reconstruct"};
+                                         "fft_y_forward", "forward_rhs", "solve"};
 
 // Record a timing event at a stage edge; accumulate the previous step's
 // interval when a stage starts again.
@@ -378,9 +395,17 @@ hipError_t join_side(pcf_handle *h) {
 // the A fields of the state (A, three components) give b and j and receive
 // the induction rows in NA[slot]; b, j and the induction rows are formed on
 // the side stream when there is one.
+// The nodal contraction of the state in solution slot xs: from the given
+// u1, u2 of a loaded state, else from (u0, g, v00, w00) with the
+// reconstruction folded in.
 template <typename T>
-hipError_t nonlinear_stages(pcf_handle *h, const DevMats<T> &dm, const double2 *u0, int slot,
-                            const double2 *A) {
+hipError_t launch_nodal_state(pcf_handle *h, const DevMats<T> &dm, int xs, hipStream_t st) {
+  if (h->u12_given) return pcf::launch_nodal_mfma<T>(h->geo, dm.view, h->u0[xs], h->u1, h->u2, h->ws, st);
+  return pcf::launch_nodal_prim_mfma<T>(h->geo, dm.view, h->u0[xs], h->g[xs], h->v[xs], h->w[xs], h->ws, st);
+}
+
+template <typename T>
+hipError_t nonlinear_stages(pcf_handle *h, const DevMats<T> &dm, int xs, int slot, const double2 *A) {
   const pcf::Geo &g = h->geo;
   const pcf::NodalMats<T> &nm = dm.view;
   const bool mhd = h->mhd;
@@ -391,7 +416,7 @@ hipError_t nonlinear_stages(pcf_handle *h, const DevMats<T> &dm, const double2 *
     if ((e = pcf::launch_nodal_mhd_mfma<T>(g, dm.mview, A, h->cstride, h->ws, h->side ? h->side : h->stream)))
       return e;
   }
-  if ((e = pcf::launch_nodal_mfma<T>(g, nm, u0, h->u1, h->u2, h->ws, h->stream))) return e;
+  if ((e = launch_nodal_state<T>(h, dm, xs, h->stream))) return e;
   if (h->side && (e = join_side(h))) return e;
   if ((e = mark(h, 0, 1))) return e;
   if ((e = mark(h, 1, 0))) return e;
@@ -416,18 +441,16 @@ hipError_t nonlinear_stages(pcf_handle *h, const DevMats<T> &dm, const double2 *
   return mark(h, 4, 1);
 }
 
-// Nonlinear rows of the state whose wall-normal velocity is u0 (with the
-// handle's u1/u2; under MHD with vector potential A) into nonlinear slot
-// `slot`.  With ytape set, the padded y-line fields of u and curl u are
-// written there (and kept) instead of the workspace: the reverse pass reads
-// them back.
-int32_t nonlinear_into(pcf_handle *h, const double2 *u0, int slot, void *ytape = nullptr,
-                       const double2 *A = nullptr) {
+// Nonlinear rows of the state in solution slot xs (under MHD with vector
+// potential A) into nonlinear slot `slot`.  With ytape set, the padded y-line
+// fields of u and curl u are written there (and kept) instead of the
+// workspace: the reverse pass reads them back.
+int32_t nonlinear_into(pcf_handle *h, int xs, int slot, void *ytape = nullptr, const double2 *A = nullptr) {
   void *saved = h->ws.Y;
   if (ytape) h->ws.Y = ytape;
   hipError_t e = (h->cfg.precision == PCF_PREC_FP64)
-                     ? nonlinear_stages<double>(h, h->md, u0, slot, A)
-                     : nonlinear_stages<float>(h, h->mf, u0, slot, A);
+                     ? nonlinear_stages<double>(h, h->md, xs, slot, A)
+                     : nonlinear_stages<float>(h, h->mf, xs, slot, A);
   h->ws.Y = saved;
   if (e != hipSuccess) return fail(PCF_E_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
   return PCF_OK;
@@ -474,11 +497,62 @@ hipError_t implicit_update(pcf_handle *h, const pcf::SolveArgs &a, const pcf::So
     }
   }
   if ((e = pcf::launch_solve_factored(h->geo, h->sm, *fs, a, 0, 0, h->stream))) return e;
-  if ((e = mark(h, 5, 1))) return e;
-  if ((e = mark(h, 6, 0))) return e;
-  if ((e = pcf::launch_reconstruct(h->geo, h->Rt, h->ldr, a, h->stream))) return e;
   if (aA && h->side && (e = join_side(h))) return e;
-  return mark(h, 6, 1);
+  // the new state's u1, u2 are formed by the next nodal contraction
+  h->u12_given = false;
+  h->u12_valid = false;
+  return mark(h, 5, 1);
+}
+
+// Fill the u1, u2 buffers of the current state when a solve left them unset
+// (channelflow_kmm.py:855-870), before anything reads them.
+hipError_t ensure_u12(pcf_handle *h);
+
+// Before the first step from a state loaded with given u1, u2: when they are
+// the reconstruction of the state's own unknowns (bitwise), drop the flag.
+// Synchronous; never called under graph capture.
+int32_t resolve_given(pcf_handle *h) {
+  if (!h->u12_given || h->u12_checked) return PCF_OK;
+  h->u12_checked = true;
+  const pcf::Geo &g = h->geo;
+  const size_t n = (size_t)g.nd * g.m;
+  double2 *r1 = reinterpret_cast<double2 *>(h->ws.X), *r2 = r1 + n;  // workspace as scratch
+  pcf::SolveArgs a{};
+  const int c = h->so[0];
+  a.u0_out = h->u0[c];
+  a.g_out = h->g[c];
+  a.v_out = h->v[c];
+  a.w_out = h->w[c];
+  a.u1 = r1;
+  a.u2 = r2;
+  unsigned diff = 1;
+  hipError_t e = pcf::launch_reconstruct(g, h->Rt, h->ldr, a, h->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(h->dflag, 0, sizeof(unsigned), h->stream);
+  if (e == hipSuccess) e = pcf::launch_bits_differ(h->u1, r1, n, h->dflag, h->stream);
+  if (e == hipSuccess) e = pcf::launch_bits_differ(h->u2, r2, n, h->dflag, h->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(&diff, h->dflag, sizeof(unsigned), hipMemcpyDeviceToHost, h->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  if (e != hipSuccess) return fail(PCF_E_HIP, std::string("state check: ") + hipGetErrorString(e));
+  if (!diff) {
+    h->u12_given = false;
+    h->u12_valid = true;
+  }
+  return PCF_OK;
+}
+
+hipError_t ensure_u12(pcf_handle *h) {
+  if (h->u12_valid) return hipSuccess;
+  pcf::SolveArgs a{};
+  const int c = h->so[0];
+  a.u0_out = h->u0[c];
+  a.g_out = h->g[c];
+  a.v_out = h->v[c];
+  a.w_out = h->w[c];
+  a.u1 = h->u1;
+  a.u2 = h->u2;
+  hipError_t e = pcf::launch_reconstruct(h->geo, h->Rt, h->ldr, a, h->stream);
+  if (e == hipSuccess) h->u12_valid = true;
+  return e;
 }
 
 // A Crank-Nicolson-form update, rhs = M x + gamma L x + sum e N solved with
@@ -568,7 +642,7 @@ pcf::SolveArgs a_args_from(pcf_handle *h, const pcf::SolveArgs &a, const int xs[
 int32_t steady_step(pcf_handle *h, void *const *tape = nullptr) {
   const double dt = h->cfg.dt;
   const int A = h->no[0];
-  int32_t r = nonlinear_into(h, h->u0[h->so[0]], A, tape ? tape[0] : nullptr, h->A[h->so[0]]);
+  int32_t r = nonlinear_into(h, h->so[0], A, tape ? tape[0] : nullptr, h->A[h->so[0]]);
   if (r) return r;
   pcf::SolveArgs a = base_args(h, h->so[2]);
   set_x(a, 0, h, h->so[0], 18.0 / 11.0, 0.0);
@@ -602,14 +676,14 @@ int32_t startup_step(pcf_handle *h, void *const *tape = nullptr) {
   const double dt = h->cfg.dt;
   const int A = h->no[0], B = h->no[3], C = h->no[2];
   const int cur = h->so[0], out = h->so[2];
-  int32_t r = nonlinear_into(h, h->u0[cur], A, tape ? tape[0] : nullptr, h->A[cur]);
+  int32_t r = nonlinear_into(h, cur, A, tape ? tape[0] : nullptr, h->A[cur]);
   if (r) return r;
   const int nslot[3] = {A, B, C};
   for (int rk = 0; rk < 3; ++rk) {
     const double gamma = stage_gamma(rk, dt);
     const int xs = (rk == 0) ? cur : out;
     if (rk > 0) {
-      r = nonlinear_into(h, h->u0[out], nslot[rk], tape ? tape[rk] : nullptr, h->A[out]);
+      r = nonlinear_into(h, out, nslot[rk], tape ? tape[rk] : nullptr, h->A[out]);
       if (r) return r;
     }
     pcf::SolveArgs a = base_args(h, out);
@@ -655,7 +729,7 @@ int32_t ars222_step(pcf_handle *h) {
   const double dt = h->cfg.dt, gm = ARS_G, dl = 1.0 - 1.0 / (2.0 * ARS_G), c1 = (1.0 - gm) / gm;
   const int cur = h->so[0], x1 = h->so[1], x2 = h->so[2];
   const int NA = h->no[0], NB = h->no[3];
-  int32_t r = nonlinear_into(h, h->u0[cur], NA, nullptr, h->A[cur]);
+  int32_t r = nonlinear_into(h, cur, NA, nullptr, h->A[cur]);
   if (r) return r;
   hipError_t e;
   {
@@ -673,7 +747,7 @@ int32_t ars222_step(pcf_handle *h) {
     if ((e = implicit_update(h, a, h->mhd ? &aA : nullptr, &h->fars, &h->fA_ars)))
       return fail(PCF_E_HIP, std::string("solve launch: ") + hipGetErrorString(e));
   }
-  if ((r = nonlinear_into(h, h->u0[x1], NB, nullptr, h->A[x1]))) return r;
+  if ((r = nonlinear_into(h, x1, NB, nullptr, h->A[x1]))) return r;
   {
     pcf::SolveArgs a = base_args(h, x2);
     set_x(a, 0, h, cur, 1.0 - c1, 0.0);
@@ -708,7 +782,7 @@ int32_t cnab2_step(pcf_handle *h) {
   const bool have = h->have_old != 0.0;
   if (have && !(h->previous_dt > 0.0))
     return fail(PCF_E_INVALID, "CNAB2 history: have_old set with previous_dt <= 0");
-  int32_t r = nonlinear_into(h, h->u0[cur], NA, nullptr, h->A[cur]);
+  int32_t r = nonlinear_into(h, cur, NA, nullptr, h->A[cur]);
   if (r) return r;
   pcf::SolveArgs a = base_args(h, out);
   set_x(a, 0, h, cur, 1.0, 0.5 * dt);
@@ -745,7 +819,7 @@ hipError_t field_maxima_stages(pcf_handle *h, const DevMats<T> &dm) {
   nm0.twy = dm.twy0;
   nm0.twz = dm.twz0;
   hipError_t e;
-  if ((e = pcf::launch_nodal_mfma<T>(g, dm.view, h->u0[h->so[0]], h->u1, h->u2, h->ws, h->stream))) return e;
+  if ((e = launch_nodal_state<T>(h, dm, h->so[0], h->stream))) return e;
   if (h->mhd && (e = pcf::launch_nodal_mhd_mfma<T>(g, dm.mview, h->A[h->so[0]], h->cstride, h->ws, h->stream)))
     return e;
   T *X = reinterpret_cast<T *>(h->ws.X), *Y = reinterpret_cast<T *>(h->ws.Y);  // complex: 2 T each
@@ -767,7 +841,7 @@ hipError_t stage_launch(pcf_handle *h, const DevMats<T> &dm, int stage) {
   hipError_t e = hipSuccess;
   switch (stage) {
     case 0:
-      e = pcf::launch_nodal_mfma<T>(g, dm.view, h->u0[cur], h->u1, h->u2, h->ws, h->stream);
+      e = launch_nodal_state<T>(h, dm, cur, h->stream);
       if (e == hipSuccess && mhd) e = pcf::launch_nodal_mhd_mfma<T>(g, dm.mview, h->A[cur], h->cstride, h->ws, h->stream);
       return e;
     case 1: return pcf::launch_fft_y<T>(g, dm.view, h->ws, h->py, true, mhd, h->stream);
@@ -931,6 +1005,15 @@ void advance_host(pcf_handle *h, int n) {
   h->history_steps = 2.0;
   h->have_old = 1.0;
   h->previous_dt = h->cfg.dt;
+  h->u12_given = false;
+  h->u12_valid = false;
+}
+
+// Graph cache key: ring phase (< 2^14), whether the first step reads given
+// u1/u2 (bit 14), the length and the tape base.
+uint64_t graph_key(const pcf_handle *h, int n, int tape_base) {
+  return ring_phase(h) | (uint64_t)(h->u12_given ? 1 : 0) << 14 | (uint64_t)n << 16 |
+         (uint64_t)(tape_base + 1) << 40;
 }
 
 // Capture n steady steps (tape slots tape_base + i, none when tape_base < 0)
@@ -944,12 +1027,15 @@ int32_t capture_steady(pcf_handle *h, int n, int tape_base, hipGraphExec_t *out)
   std::copy(h->so, h->so + 3, so);
   std::copy(h->no, h->no + 4, no);
   const double hs = h->history_steps, ho = h->have_old, pd = h->previous_dt;
+  const bool given = h->u12_given, valid = h->u12_valid;
   auto restore = [&]() {
     std::copy(so, so + 3, h->so);
     std::copy(no, no + 4, h->no);
     h->history_steps = hs;
     h->have_old = ho;
     h->previous_dt = pd;
+    h->u12_given = given;
+    h->u12_valid = valid;
   };
   hipGraph_t graph = nullptr;
   hipError_t e = hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal);
@@ -990,7 +1076,7 @@ int32_t cached_steady(pcf_handle *h, std::map<uint64_t, hipGraphExec_t> &cache, 
 // tape_base + i (no tape when tape_base < 0), replayed from a graph cached per
 // (ring phase, n, tape_base).
 int32_t steady_run(pcf_handle *h, int n, int tape_base) {
-  const uint64_t key = ring_phase(h) | (uint64_t)n << 16 | (uint64_t)(tape_base + 1) << 40;
+  const uint64_t key = graph_key(h, n, tape_base);
   hipGraphExec_t exec = nullptr;
   int32_t r = cached_steady(h, h->seg_graphs, key, n, tape_base, &exec);
   if (r) return r;
@@ -1008,6 +1094,8 @@ hipError_t snap_save(pcf_handle *h, pcf_handle::Snap &sn) {
   sn.hs = h->history_steps;
   sn.ho = h->have_old;
   sn.pd = h->previous_dt;
+  sn.given = h->u12_given;
+  sn.valid = h->u12_valid;
   return hipSuccess;
 }
 
@@ -1019,6 +1107,8 @@ hipError_t snap_load(pcf_handle *h, const pcf_handle::Snap &sn) {
   h->history_steps = sn.hs;
   h->have_old = sn.ho;
   h->previous_dt = sn.pd;
+  h->u12_given = sn.given;
+  h->u12_valid = sn.valid;
   return hipSuccess;
 }
 
@@ -1564,6 +1654,7 @@ int32_t pcf_create(const pcf_config *cfg, pcf_handle **out) {
   if ((r = h->upload(&h->dfejer, h->ops.fejer))) return bail(r);
   if ((r = h->upload(&h->dxn, h->ops.x))) return bail(r);
   if ((r = h->alloc(&h->ddiag, 4))) return bail(r);
+  if ((r = h->alloc(&h->dflag, 1))) return bail(r);
   if (h->maxima_ok) {
     const int nf = h->mhd ? 6 : 3, eb = cfg->precision == PCF_PREC_FP64 ? 16 : 8;
     if ((r = h->alloc(&h->dblockmax, (size_t)pcf::absmax_blocks(h->geo0, nf, eb) * nf))) return bail(r);
@@ -1637,6 +1728,12 @@ int32_t pcf_set_field(pcf_handle *h, int32_t field, const double *host) {
     for (int i = 0; i < g.nd; ++i) vec[i] = host[i * plane];
     double *dst = (field == PCF_F_U1) ? h->v[h->so[0]] : h->w[h->so[0]];
     PCF_HIP(hipMemcpy(dst, vec.data(), g.nd * sizeof(double), hipMemcpyHostToDevice));
+    h->u12_given = true;  // the next evaluation reads the state's own u1, u2
+    h->u12_valid = true;
+    h->u12_checked = false;
+  } else if (field == PCF_F_U0 || field == PCF_F_G) {
+    if (!h->u12_given) h->u12_valid = false;  // derived u1, u2 follow the new (u0, g)
+    h->u12_checked = false;
   }
   return PCF_OK;
 }
@@ -1646,6 +1743,8 @@ int32_t pcf_get_field(pcf_handle *h, int32_t field, double *host) {
   FieldRef fr;
   if (!field_ref(h, field, fr)) return fail(PCF_E_INVALID, "unknown field");
   const pcf::Geo &g = h->geo;
+  PCF_HIP(hipSetDevice(h->device));
+  if (field == PCF_F_U1 || field == PCF_F_U2) PCF_HIP(ensure_u12(h));
   PCF_HIP(hipStreamSynchronize(h->stream));
   if (!fr.complex_) {
     PCF_HIP(hipMemcpy(host, fr.ptr, (size_t)fr.rows * sizeof(double), hipMemcpyDeviceToHost));
@@ -1695,6 +1794,7 @@ int32_t pcf_step(pcf_handle *h, int32_t nsteps) {
   if (!h) return fail(PCF_E_INVALID, "null handle");
   if (nsteps < 0) return fail(PCF_E_INVALID, "nsteps must be >= 0");
   PCF_HIP(hipSetDevice(h->device));
+  if (int32_t r0 = resolve_given(h)) return r0;
   const int ti = h->cfg.time_integrator;
   for (int s = 0; s < nsteps; ++s) {
     int32_t r;
@@ -1746,6 +1846,7 @@ int32_t pcf_step_steady(pcf_handle *h, int32_t nsteps) {
   if (h->cfg.time_integrator != PCF_TI_SBDF3) return fail(PCF_E_UNSUPPORTED, "steady steps are SBDF3 steps");
   if (h->history_steps < 2.0) return fail(PCF_E_INVALID, "history not bootstrapped (history_steps < 2)");
   PCF_HIP(hipSetDevice(h->device));
+  if (int32_t r0 = resolve_given(h)) return r0;
   for (int s = 0; s < nsteps; ++s) {
     int32_t r = steady_step(h);
     if (r) return r;
@@ -1760,13 +1861,14 @@ int32_t pcf_step_steady_graph(pcf_handle *h, int32_t nsteps, int32_t replays) {
   if (nsteps <= 0 || nsteps > (1 << 20)) return fail(PCF_E_INVALID, "graph length must be in [1, 2^20] steps");
   if (replays < 0) return fail(PCF_E_INVALID, "replays must be >= 0");
   PCF_HIP(hipSetDevice(h->device));
+  if (int32_t r0 = resolve_given(h)) return r0;
   // one graph per (ring phase, length): a replay whose length is not a whole
   // number of ring periods (12 steps) leaves the rings at another phase, and so
   // can steps outside the graph (eager, start-up)
   hipGraphExec_t exec = nullptr;
-  if (replays == 0) return cached_steady(h, h->graphs, ring_phase(h) | (uint64_t)nsteps << 16, nsteps, -1, &exec);
+  if (replays == 0) return cached_steady(h, h->graphs, graph_key(h, nsteps, -1), nsteps, -1, &exec);
   for (int rp = 0; rp < replays; ++rp) {
-    int32_t r = cached_steady(h, h->graphs, ring_phase(h) | (uint64_t)nsteps << 16, nsteps, -1, &exec);
+    int32_t r = cached_steady(h, h->graphs, graph_key(h, nsteps, -1), nsteps, -1, &exec);
     if (r) return r;
     PCF_HIP(hipGraphLaunch(exec, h->stream));
     advance_host(h, nsteps);
@@ -1778,6 +1880,7 @@ int32_t pcf_diagnostics(pcf_handle *h, double *out) {
   if (!h || !out) return fail(PCF_E_INVALID, "null argument");
   const pcf::Geo &g = h->geo;
   PCF_HIP(hipSetDevice(h->device));
+  PCF_HIP(ensure_u12(h));
   PCF_HIP(hipMemsetAsync(h->ddiag, 0, 4 * sizeof(double), h->stream));
   hipError_t e = pcf::launch_diag(g, h->dVB, h->dVD, h->dVB1, h->dVD1, h->dfejer, h->dxn,
                                   h->u0[h->so[0]], h->u1, h->u2, h->ddiag, h->stream);
@@ -1848,6 +1951,7 @@ int32_t pcf_field_maxima(pcf_handle *h, double *out) {
   if (!h->maxima_ok)
     return fail(PCF_E_UNSUPPORTED, "field maxima need ny, nz that factor into 2, 3, 4 within the FFT kernels");
   PCF_HIP(hipSetDevice(h->device));
+  if (int32_t r0 = resolve_given(h)) return r0;
   const bool fp64 = h->cfg.precision == PCF_PREC_FP64;
   const int nf = h->mhd ? 6 : 3;
   hipError_t e = fp64 ? field_maxima_stages<double>(h, h->md) : field_maxima_stages<float>(h, h->mf);
@@ -1871,10 +1975,11 @@ int32_t pcf_profile_steady(pcf_handle *h, int32_t nsteps, double *stage_ms) {
   if (h->history_steps < 2.0) return fail(PCF_E_INVALID, "history not bootstrapped (history_steps < 2)");
   if (nsteps <= 0) return fail(PCF_E_INVALID, "nsteps must be positive");
   PCF_HIP(hipSetDevice(h->device));
+  if (int32_t r0 = resolve_given(h)) return r0;
   StageTimer t{};
   for (int i = 0; i < NSTAGE; ++i)
     for (int j = 0; j < 2; ++j) PCF_HIP(hipEventCreate(&t.ev[i][j]));
-  double acc[NSTAGE] = {0, 0, 0, 0, 0, 0, 0};
+  double acc[NSTAGE] = {0, 0, 0, 0, 0, 0};
   int32_t r = PCF_OK;
   h->timer = &t;
   for (int s = 0; s < nsteps && !r; ++s) {
@@ -1906,6 +2011,7 @@ int32_t pcf_profile_stage(pcf_handle *h, int32_t stage, int32_t reps, double *ms
   if (stage < 0 || stage > 4) return fail(PCF_E_INVALID, "stage must be a nonlinear stage (0..4)");
   if (reps <= 0) return fail(PCF_E_INVALID, "reps must be positive");
   PCF_HIP(hipSetDevice(h->device));
+  if (int32_t r0 = resolve_given(h)) return r0;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   PCF_HIP(hipEventCreate(&e0));
   if (hipEventCreate(&e1) != hipSuccess) {
@@ -2003,6 +2109,7 @@ int32_t pcf_vjp(pcf_handle *h, int32_t steps, int32_t checkpoint_every, int32_t 
   }
   pcf_handle::Snap &fin = A.snaps[nseg];
   if ((e = snap_save(h, fin))) return hip_fail(e, "snapshot");
+  if ((e = ensure_u12(h))) return hip_fail(e, "reconstruction");  // the energy seed reads u1, u2
   // seed (on the reverse stream, after the forward): mu of the final state's
   // (u, g) -> mu_{x_K}
   PCF_HIP(hipEventRecord(A.ev_seed, h->stream));

@@ -539,6 +539,18 @@ __global__ void __launch_bounds__(128) k_solve_bordered(Geo g, BorderedArgs a) {
   }
 }
 
+// -------------------------------------------------------- k_bits_differ ----
+// diff |= 1 when a and b differ in any bit (n complex values).
+__global__ void __launch_bounds__(256) k_bits_differ(const double2 *__restrict__ a, const double2 *__restrict__ b,
+                                                     size_t n, unsigned *__restrict__ diff) {
+  const unsigned long long *x = reinterpret_cast<const unsigned long long *>(a);
+  const unsigned long long *y = reinterpret_cast<const unsigned long long *>(b);
+  bool d = false;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < 2 * n; i += (size_t)gridDim.x * blockDim.x)
+    d |= x[i] != y[i];
+  if (d) atomicOr(diff, 1u);
+}
+
 // -------------------------------------------------------------- k_diag ----
 __global__ void __launch_bounds__(256) k_diag(Geo g, const double *__restrict__ VB,
                                               const double *__restrict__ VD,
@@ -912,6 +924,13 @@ hipError_t launch_solve_bordered(const Geo &g, const BorderedArgs &a, hipStream_
     hipLaunchKernelGGL((k_solve_bordered<4>), dim3((g.m + 3) / 4), dim3(128), 4 * per_mode, s, g, a);
   else
     return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+hipError_t launch_bits_differ(const double2 *a, const double2 *b, size_t n, unsigned *diff, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  const size_t blocks = std::min<size_t>(1024, (2 * n + 255) / 256);
+  hipLaunchKernelGGL(k_bits_differ, dim3((unsigned)blocks), dim3(256), 0, s, a, b, n, diff);
   return hipGetLastError();
 }
 

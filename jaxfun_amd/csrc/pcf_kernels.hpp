@@ -56,6 +56,8 @@ struct NodalMats {
   const T *xnode; // [n0]
   const T *twy;   // [my] exp(-2 pi i t / my) as (cos, sin) pairs
   const T *twz;   // [mz]
+  const T *VDRt;  // [nb][ldx]  (VD R)^T: nodal values of the reconstructed f = R u0
+  const T *VD1Rt; // [nb][ldx]  (VD1 R)^T
 };
 
 // Wall-normal matrices of the MHD contractions, transposed ([K][ld]) like
@@ -184,6 +186,8 @@ hipError_t launch_solve_factored(const Geo &g, const StructMats &sm, const Facto
 // Diagnostics partial sums: out[0] = sum wF |u|^2 (perturbation), out[1] = sum
 // wF |div u|^2, both over the full spectrum (Hermitian weights applied), and
 // out[2] = sum_x wF Re u1_00(x) x, out[3] = sum_x wF Re du1_00/dx(x).
+// diff |= 1 when the n complex values at a and b differ in any bit.
+hipError_t launch_bits_differ(const double2 *a, const double2 *b, size_t n, unsigned *diff, hipStream_t s);
 hipError_t launch_diag(const Geo &g, const double *VB, const double *VD, const double *VB1,
                        const double *VD1, const double *fejer, const double *xnode,
                        const double2 *u0, const double2 *u1, const double2 *u2, double *out,

@@ -76,7 +76,8 @@ __device__ __forceinline__ bool nyq_mode(const Geo &g, int m) { return m / g.kzs
 // MFMA loop only waits on LDS and on the small (L2-resident) A matrices.
 // Staging loads per thread of each kernel; the launchers check that
 // blockDim * STAGE covers the kernel's panel.
-constexpr int STAGE_NODAL = 6, STAGE_RHS = 3, STAGE_NODAL_ADJ = 3, STAGE_RHS_ADJ = 4, STAGE_RECON = 4;
+constexpr int STAGE_NODAL = 6, STAGE_NODAL_PRIM = 4, STAGE_RHS = 3, STAGE_NODAL_ADJ = 3, STAGE_RHS_ADJ = 4,
+              STAGE_RECON = 4;
 
 // K loop of a wave's tile: nit steps of 4 K rows.  The A operands (NA values
 // per step, from small L2-resident matrices) are read D steps ahead into a
@@ -203,6 +204,116 @@ __global__ void __launch_bounds__(512) k_nodal_mfma(Geo g, NodalMats<T> nm, cons
       Xr[5 * plane + o] = w2;
     }
   }
+  }
+}
+
+// The same nodal fields from the solver's own unknowns (u0, g, v00, w00), with
+// the velocity reconstruction (channelflow_kmm.py:855-870)
+//   u1 = i (ky f + kz g) / k^2,  u2 = i (kz f - ky g) / k^2,  f = R u0,
+//   u1(0, 0) = v00, u2(0, 0) = w00
+// folded into the contraction: P1 = (VD R) u0, P2 = (VD1 R) u0, P3 = VD g',
+// P4 = VD1 g', where g' is g with the mean mode's column replaced by
+// v00 + i w00 (g(0, 0) = 0).  Then U1 = i (ky P1 + kz P3) / k^2,
+// U2 = i (kz P1 - ky P3) / k^2, D1 and D2 likewise from P2, P4, and
+// w0 = i ky U2 - i kz U1 = P3 (zero at the mean mode), w1 = i kz U0 - D2,
+// w2 = D1 - i ky U0.  Mean mode: U1 = Re P3, U2 = Im P3, D1 = Re P4, D2 = Im P4
+// on the real lane, zero on the imaginary one.
+template <typename T>
+__global__ void __launch_bounds__(512) k_nodal_prim_mfma(Geo g, NodalMats<T> nm, const double2 *__restrict__ u0,
+                                                         const double2 *__restrict__ gf,
+                                                         const double *__restrict__ v00,
+                                                         const double *__restrict__ w00,
+                                                         typename VT<T>::V *__restrict__ X) {
+  using MM = Mfma<T>;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int ct = blockIdx.x;
+  const int M = g.m, kp = (g.nd + 3) / 4 * 4;
+  const Panel<T> B{reinterpret_cast<T *>(smem), kp};
+  {
+    const int total = 2 * kp * 8;
+    double2 z[STAGE_NODAL_PRIM];
+#pragma unroll
+    for (int t = 0; t < STAGE_NODAL_PRIM; ++t) {
+      const int idx = threadIdx.x + t * blockDim.x;
+      const int f = idx / (kp * 8), rem = idx - f * kp * 8, k = rem >> 3, j = rem & 7;
+      const int m = 8 * ct + j;
+      z[t] = double2{0, 0};
+      if (idx < total && m < M) {
+        if (f == 0 && k < g.nb) z[t] = u0[(size_t)k * M + m];
+        if (f == 1 && k < g.nd) z[t] = (m == 0) ? double2{v00[k], w00[k]} : gf[(size_t)k * M + m];
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < STAGE_NODAL_PRIM; ++t) {
+      const int idx = threadIdx.x + t * blockDim.x;
+      if (idx >= total) continue;
+      const int f = idx / (kp * 8), rem = idx - f * kp * 8, k = rem >> 3, j = rem & 7;
+      B.at(f, k, 2 * j) = (T)z[t].x;
+      B.at(f, k, 2 * j + 1) = (T)z[t].y;
+    }
+  }
+  __syncthreads();
+  const Lane L = lane_of(g, ct);
+  double kyd, kzd, kyo;
+  mode_k(g, L.valid ? L.mcol : 0, kyd, kzd, kyo);
+  const double k2d = kyd * kyd + kzd * kzd;
+  const bool mean = L.valid && L.mcol == 0;
+  // reconstruction factors ky/k^2, kz/k^2 (K_over_K2, tensorproductspace.py:80-99)
+  const T ay = (T)(k2d > 0 ? kyd / k2d : 0.0), az = (T)(k2d > 0 ? kzd / k2d : 0.0);
+  const T ky = (T)kyo, kz = (T)kzd;
+  const T s = L.comp ? (T)1 : (T)-1;  // re lane: i z -> -z.im ; im lane: i z -> +z.re
+  const size_t plane = (size_t)g.n0 * 2 * M;
+  T *Xr = reinterpret_cast<T *>(X);
+  for (int x0 = (threadIdx.x >> 6) * 16; x0 < g.n0; x0 += (blockDim.x >> 6) * 16) {
+    const int xa = x0 + (L.l & 15);
+    typename MM::acc U0{}, P1{}, P2{}, P3{}, P4{};
+    const int kq = L.l >> 4;
+    k_loop<T, 5, 5>(
+        kp / 4,
+        [&](int it, T(&a)[5]) {
+          const int k = 4 * it + kq;
+          a[0] = mat_at(nm.VBt, nm.ldx, k, xa);
+          a[1] = mat_at(nm.VDRt, nm.ldx, k, xa);
+          a[2] = mat_at(nm.VD1Rt, nm.ldx, k, xa);
+          a[3] = mat_at(nm.VDt, nm.ldx, k, xa);
+          a[4] = mat_at(nm.VD1t, nm.ldx, k, xa);
+        },
+        [&](int it, const T(&a)[5]) {
+          const int k = 4 * it + kq;
+          const T b0 = B.at(0, k, L.c), b1 = B.at(1, k, L.c);
+          U0 = MM::mma(a[0], b0, U0);
+          P1 = MM::mma(a[1], b0, P1);
+          P2 = MM::mma(a[2], b0, P2);
+          P3 = MM::mma(a[3], b1, P3);
+          P4 = MM::mma(a[4], b1, P4);
+        });
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const T u0v = U0[r], q1 = P1[r], q2 = P2[r], q3 = P3[r], q4 = P4[r];
+      const T p0 = partner(u0v), p1 = partner(q1), p2 = partner(q2), p3 = partner(q3), p4 = partner(q4);
+      T u1v = s * (ay * p1 + az * p3), u2v = s * (az * p1 - ay * p3);
+      T d1 = s * (ay * p2 + az * p4), d2 = s * (az * p2 - ay * p4);
+      T w0 = q3;
+      if (mean) {
+        u1v = L.comp ? (T)0 : q3;
+        u2v = L.comp ? (T)0 : p3;
+        d1 = L.comp ? (T)0 : q4;
+        d2 = L.comp ? (T)0 : p4;
+        w0 = (T)0;
+      }
+      const T w1 = s * kz * p0 - d2;
+      const T w2 = d1 - s * ky * p0;
+      const int x = x0 + MM::row(L.l, r);
+      if (L.valid && x < g.n0) {
+        const size_t o = (size_t)x * 2 * M + 2 * L.mcol + L.comp;
+        Xr[o] = u0v;
+        Xr[plane + o] = u1v;
+        Xr[2 * plane + o] = u2v;
+        Xr[3 * plane + o] = w0;
+        Xr[4 * plane + o] = w1;
+        Xr[5 * plane + o] = w2;
+      }
+    }
   }
 }
 
@@ -956,6 +1067,17 @@ hipError_t launch_nodal_mfma(const Geo &g, const NodalMats<T> &nm, const double2
   return hipGetLastError();
 }
 
+template <typename T>
+hipError_t launch_nodal_prim_mfma(const Geo &g, const NodalMats<T> &nm, const double2 *u0, const double2 *gf,
+                                  const double *v00, const double *w00, Workspace &ws, hipStream_t s) {
+  using V = typename VT<T>::V;
+  if (!nm.VDRt || !nm.VD1Rt) return hipErrorInvalidValue;
+  if (!stage_covers(g.n0, STAGE_NODAL_PRIM, 2 * ((g.nd + 3) / 4 * 4) * 8)) return hipErrorInvalidValue;
+  hipLaunchKernelGGL((k_nodal_prim_mfma<T>), mfma_grid(g), mfma_block(g.n0), panel_bytes<T>(2, g.nd), s, g, nm,
+                     u0, gf, v00, w00, reinterpret_cast<V *>(ws.X));
+  return hipGetLastError();
+}
+
 hipError_t launch_reconstruct(const Geo &g, const double *Rt, int ldr, const SolveArgs &a, hipStream_t s) {
   if (ldr < g.nd || ldr % 16 || !stage_covers(g.nd, STAGE_RECON, ((g.nb + 3) / 4 * 4) * 8))
     return hipErrorInvalidValue;
@@ -1060,6 +1182,9 @@ hipError_t mfma_init(const Geo &g, bool fp64) {
   template hipError_t launch_nodal_mfma<T>(const Geo &, const NodalMats<T> &, const double2 *,      \
                                            const double2 *, const double2 *, Workspace &,           \
                                            hipStream_t);                                            \
+  template hipError_t launch_nodal_prim_mfma<T>(const Geo &, const NodalMats<T> &, const double2 *, \
+                                                const double2 *, const double *, const double *,      \
+                                                Workspace &, hipStream_t);                           \
   template hipError_t launch_rhs_mfma<T>(const Geo &, const NodalMats<T> &, Workspace &, double2 *, \
                                          double2 *, double *, double *, hipStream_t);               \
   template hipError_t launch_nodal_adj_mfma<T>(const Geo &, const AdjMats<T> &, Workspace &,        \

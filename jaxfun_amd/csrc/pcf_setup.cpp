@@ -157,6 +157,9 @@ std::string build_operators(int n0, Operators &ops) {
   if (!solve(C0D, matmul(HSD, matmul(D2, SDt)), Q2)) return "singular Dirichlet mass matrix";
   LMat VQ1 = matmul(VD, Q1), VQ2 = matmul(VD, Q2);
   LMat VC1 = matmul(T, D), VC2 = matmul(T, D2);
+  // Nodal values of f = R u0 and of its x-derivative: the reconstruction
+  // folded into the nodal contraction of the next nonlinear evaluation.
+  LMat VDR = matmul(VD, R), VD1R = matmul(VD1, R);
 
   ops.C0B = to_double(C0B);
   ops.C2B = to_double(C2B);
@@ -179,6 +182,8 @@ std::string build_operators(int n0, Operators &ops) {
   ops.VC1 = to_double(VC1);
   ops.VC2 = to_double(VC2);
   ops.WQ = to_double(Wq);
+  ops.VDR = to_double(VDR);
+  ops.VD1R = to_double(VD1R);
 
   // Structured form of the parity blocks.  Far-upper factors from
   //   (T_a, T_b^(2)) = (pi/2)  b (b^2 - a^2)
@@ -314,6 +319,8 @@ const Mat *operator_by_index(const Operators &ops, int which) {
     case 18: return &ops.VC1;
     case 19: return &ops.VC2;
     case 20: return &ops.WQ;
+    case 21: return &ops.VDR;
+    case 22: return &ops.VD1R;
     default: return nullptr;
   }
 }

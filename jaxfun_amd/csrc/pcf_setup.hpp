@@ -47,6 +47,7 @@ struct Operators {
   Mat VQ1, VQ2;               // (n0 x nd): VD P_D D^k S_D^T, k = 1, 2 (MHD current)
   Mat VC, VC1, VC2;           // (n0 x n0): T_j, T_j', T_j'' at the nodes (TC fields)
   Mat WQ;                     // (n0 x n0): Chebyshev scalar products (T_k, f) of nodal values
+  Mat VDR, VD1R;              // (n0 x nb): VD R, VD1 R (nodal f = R u0 and its x-derivative)
   std::vector<double> x;      // Gauss nodes (ascending)
   std::vector<double> fejer;  // Fejer-I weights on the nodes
   std::vector<double> dphi_top, dphi_bot;  // phi_i'(+1), phi_i'(-1)

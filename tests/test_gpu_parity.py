@@ -98,6 +98,26 @@ def test_resume_is_exact():
     assert max(err.values()) == 0.0, err
 
 
+def test_given_velocity_leaves_enter_first_step():
+    """A loaded state's own u1, u2 feed its first nonlinear evaluation, as the
+    reference's convection reads state.u (channelflow_kmm.py:754-762), even when
+    they are not the reconstruction of (u0, g, v00, w00); every later step
+    reconstructs them.  A 1e-6 change of u1 alone moves the result, and the
+    result still matches the oracle stepped from the same leaves."""
+    from dataclasses import replace
+
+    solver = make((17, 16, 16))
+    oracle = oracle_for(solver)
+    s = solver.solve(solver.initial_state(), 3)
+    bumped = replace(s, u=(s.u[0], s.u[1] * (1.0 + 1e-6), s.u[2]))  # still a real field
+    other = make((17, 16, 16))
+    a = other.solve(bumped, 3)
+    b = make((17, 16, 16)).solve(s, 3)
+    err = max_rel_errors(a, oracle.solve(to_oracle(bumped), 3))
+    assert_parity(err, "given u1")
+    assert float(np.max(np.abs(a.u[0] - b.u[0]))) > 1e-12 * float(np.max(np.abs(b.u[0])))
+
+
 def test_graph_replay_matches_eager():
     solver = make((33, 32, 32))
     s = solver.solve(solver.initial_state(), 3)

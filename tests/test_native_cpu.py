@@ -123,6 +123,12 @@ def test_projection_and_nonlinear_rows_match_oracle():
     w = np.tensordot(o.SD, a * o.h[:, None, None], axes=(1, 0))
     f_ref = np.linalg.solve(o.MD1, w.reshape(N[0] - 2, -1)).reshape(w.shape)
     assert np.allclose(np.tensordot(R, u0, axes=(1, 0)), f_ref, atol=1e-12)
+    # the reconstruction folded into the nodal contraction: nodal values of f
+    # and of df/dx against the oracle's backward transform of f
+    VDR, VD1R = nat.setup_matrix(cfg, "VDR"), nat.setup_matrix(cfg, "VD1R")
+    f_orth = o.to_orth(f_ref, o.SD)[:, 0, 0]
+    assert np.allclose(VDR @ u0[:, 0, 0], cheb_backward(f_orth), atol=1e-12)
+    assert np.allclose(VD1R @ u0[:, 0, 0], cheb_backward(cheb_derivative(f_orth, 1)), atol=1e-11)
 
 
 def test_mhd_current_matrices_match_oracle_projection():
