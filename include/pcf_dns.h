@@ -235,6 +235,18 @@ int32_t pcf_mhd_diagnostics(pcf_handle *h, double *out4);
  * PCF_E_UNSUPPORTED when ny or nz has no 2/3/4 FFT plan. */
 int32_t pcf_field_maxima(pcf_handle *h, double *out6);
 
+/* pcf_field_maxima for callers that cannot allocate (serving loops and
+ * harnesses that forbid device or host allocation between steps): the same
+ * out[6], bit for bit, with no hipMalloc, hipFree or host heap use.  No
+ * counterpart in the reference.  The per-block maxima and their reduction go
+ * to one fixed device array of the library (65536 doubles, 512 KiB, one per
+ * GPU, held until the process ends); a grid that needs more (3 or 6 doubles
+ * per block of the maxima kernel plus 6; a few thousand at 129x128x128) is
+ * PCF_E_INVALID.
+ * Every call on a GPU shares that GPU's array: calls that overlap on one GPU
+ * (two host threads, or two streams in flight) race.  Synchronous. */
+int32_t pcf_field_maxima_noalloc(pcf_handle *h, double *out6);
+
 /* Per-stage timing.  One steady step launches pcf_kernels_per_step() kernels,
  * one per stage, in the order pcf_stage_name(0..pcf_stage_count()-1):
  * nodal_backward, fft_y_inverse, rotational, fft_y_forward, forward_rhs, solve,

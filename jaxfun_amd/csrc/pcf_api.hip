@@ -14,6 +14,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -99,6 +100,33 @@ struct DevMats {
   pcf::MhdMats<T> mview{};  // MHD contractions (conducting or insulating walls)
 };
 
+}  // namespace
+
+// Block maxima of pcf_field_maxima_noalloc: a file-scope device array, one
+// copy per GPU the library runs on, held until the process ends.  Host code
+// never uses the array's name as an address: each device's address comes from
+// hipGetSymbolAddress with that device current (scratch_on_device).
+constexpr size_t NOALLOC_SCRATCH_DOUBLES = 65536;  // 512 KiB
+static __device__ __align__(256) double scratch[NOALLOC_SCRATCH_DOUBLES];
+
+namespace {
+constexpr int MAX_DEVICES = 64;
+double *g_scratch_ptr[MAX_DEVICES] = {};
+std::mutex g_scratch_mu;
+
+// The address of `scratch` on device dev (current), looked up once per device.
+int32_t scratch_on_device(int dev, double **out) {
+  if (dev < 0 || dev >= MAX_DEVICES) return fail(PCF_E_INVALID, "device ordinal out of range for the no-alloc scratch");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) return fail(PCF_E_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[dev] = static_cast<double *>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return PCF_OK;
+}
 }  // namespace
 
 struct pcf_handle {
@@ -813,7 +841,7 @@ int32_t cnab2_step(pcf_handle *h) {
 // Nodal fields of the current state, their y-lines on the unpadded grid and
 // the per-block maxima (velocity; under MHD also b = curl A).
 template <typename T>
-hipError_t field_maxima_stages(pcf_handle *h, const DevMats<T> &dm) {
+hipError_t field_maxima_stages(pcf_handle *h, const DevMats<T> &dm, double *blockmax) {
   const pcf::Geo &g = h->geo, &g0 = h->geo0;
   pcf::NodalMats<T> nm0 = dm.view;
   nm0.twy = dm.twy0;
@@ -828,7 +856,7 @@ hipError_t field_maxima_stages(pcf_handle *h, const DevMats<T> &dm) {
   if (h->mhd && (e = pcf::launch_fft_y_fields<T>(g0, nm0, h->py0, true, 3, X + (size_t)2 * 6 * g.n0 * g.m,
                                                  Y + (size_t)2 * 3 * g.n0 * g0.my * g0.kzs, h->stream)))
     return e;
-  return pcf::launch_absmax<T>(g0, dm.xnode, dm.twz0, h->pz0, h->mhd ? 6 : 3, Y, h->dblockmax, h->stream);
+  return pcf::launch_absmax<T>(g0, dm.xnode, dm.twz0, h->pz0, h->mhd ? 6 : 3, Y, blockmax, h->stream);
 }
 
 // The kernel(s) of nonlinear stage `stage` (0 nodal_backward .. 4 forward_rhs)
@@ -1551,6 +1579,10 @@ int32_t pcf_create(const pcf_config *cfg, pcf_handle **out) {
 
   h->device = cfg->device;
   if (hipSetDevice(cfg->device) != hipSuccess) return bail(fail(PCF_E_HIP, "hipSetDevice failed"));
+  {
+    double *sp = nullptr;  // the no-alloc scratch of this device, looked up at set-up
+    if (int32_t r0 = scratch_on_device(cfg->device, &sp)) return bail(r0);
+  }
   if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess)
     return bail(fail(PCF_E_HIP, "hipStreamCreate failed"));
   h->own_stream = true;
@@ -1954,7 +1986,8 @@ int32_t pcf_field_maxima(pcf_handle *h, double *out) {
   if (int32_t r0 = resolve_given(h)) return r0;
   const bool fp64 = h->cfg.precision == PCF_PREC_FP64;
   const int nf = h->mhd ? 6 : 3;
-  hipError_t e = fp64 ? field_maxima_stages<double>(h, h->md) : field_maxima_stages<float>(h, h->mf);
+  hipError_t e = fp64 ? field_maxima_stages<double>(h, h->md, h->dblockmax)
+                      : field_maxima_stages<float>(h, h->mf, h->dblockmax);
   if (e != hipSuccess) return fail(PCF_E_HIP, std::string("field maxima launch: ") + hipGetErrorString(e));
   std::vector<double> bm((size_t)pcf::absmax_blocks(h->geo0, nf, fp64 ? 16 : 8) * nf);
   PCF_HIP(hipMemcpyAsync(bm.data(), h->dblockmax, bm.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream));
@@ -1962,6 +1995,33 @@ int32_t pcf_field_maxima(pcf_handle *h, double *out) {
   for (int f = 0; f < 6; ++f) out[f] = 0.0;
   for (size_t b = 0; b < bm.size() / nf; ++b)
     for (int f = 0; f < nf; ++f) out[f] = std::max(out[f], bm[b * nf + f]);
+  return PCF_OK;
+}
+
+int32_t pcf_field_maxima_noalloc(pcf_handle *h, double *out) {
+  if (!h || !out) return fail(PCF_E_INVALID, "null argument");
+  if (!h->maxima_ok)
+    return fail(PCF_E_UNSUPPORTED, "field maxima need ny, nz that factor into 2, 3, 4 within the FFT kernels");
+  PCF_HIP(hipSetDevice(h->device));
+  double *sp = nullptr;
+  if (int32_t r0 = scratch_on_device(h->device, &sp)) return r0;
+  const bool fp64 = h->cfg.precision == PCF_PREC_FP64;
+  const int nf = h->mhd ? 6 : 3;
+  const size_t nblk = (size_t)pcf::absmax_blocks(h->geo0, nf, fp64 ? 16 : 8);
+  if (nblk * nf + nf > NOALLOC_SCRATCH_DOUBLES)
+    return fail(PCF_E_INVALID, "grid too large for pcf_field_maxima_noalloc: it needs " +
+                                   std::to_string(nblk * nf + nf) + " doubles of block maxima, the scratch holds " +
+                                   std::to_string(NOALLOC_SCRATCH_DOUBLES));
+  if (int32_t r0 = resolve_given(h)) return r0;
+  // every block writes its own maxima and the reduction reads only this
+  // call's blocks, so nothing left by an earlier call needs clearing
+  double *fin = sp + nblk * nf;
+  hipError_t e = fp64 ? field_maxima_stages<double>(h, h->md, sp) : field_maxima_stages<float>(h, h->mf, sp);
+  if (e == hipSuccess) e = pcf::launch_reduce_blockmax(sp, (int)nblk, nf, fin, h->stream);
+  if (e != hipSuccess) return fail(PCF_E_HIP, std::string("field maxima launch: ") + hipGetErrorString(e));
+  for (int f = 0; f < 6; ++f) out[f] = 0.0;
+  PCF_HIP(hipMemcpyAsync(out, fin, nf * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  PCF_HIP(hipStreamSynchronize(h->stream));
   return PCF_OK;
 }
 

@@ -539,6 +539,26 @@ __global__ void __launch_bounds__(128) k_solve_bordered(Geo g, BorderedArgs a) {
   }
 }
 
+// -------------------------------------------------- k_reduce_blockmax ----
+// out[f] = max over the nblocks rows of blockmax[b][f], one block.
+__global__ void __launch_bounds__(256) k_reduce_blockmax(const double *__restrict__ bm, int nblocks, int nf,
+                                                         double *__restrict__ out) {
+  __shared__ double red[256];
+  const int t = threadIdx.x;
+  for (int f = 0; f < nf; ++f) {
+    double m = 0.0;
+    for (int b = t; b < nblocks; b += blockDim.x) m = fmax(m, bm[(size_t)b * nf + f]);
+    red[t] = m;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+      if (t < w) red[t] = fmax(red[t], red[t + w]);
+      __syncthreads();
+    }
+    if (t == 0) out[f] = red[0];
+    __syncthreads();
+  }
+}
+
 // -------------------------------------------------------- k_bits_differ ----
 // diff |= 1 when a and b differ in any bit (n complex values).
 __global__ void __launch_bounds__(256) k_bits_differ(const double2 *__restrict__ a, const double2 *__restrict__ b,
@@ -843,8 +863,8 @@ template <typename T>
 hipError_t launch_rotational(const Geo &g, const NodalMats<T> &nm, Workspace &ws,
                              const FFTPlan &pz, bool mhd, hipStream_t s) {
   using V = typename VT<T>::V;
-  const int upb = rot_units<T>(g, mhd);
   const int nunits = g.n0 * (g.my / 2);
+  const int upb = rot_units<T>(g, mhd);
   const size_t lds = ((size_t)upb * (mhd ? 18 : 9) * g.mz + g.mz) * sizeof(V);
   const int mb = rot_butterflies(pz, upb * (mhd ? 12 : 6));
   const dim3 grid((nunits + upb - 1) / upb);
@@ -924,6 +944,12 @@ hipError_t launch_solve_bordered(const Geo &g, const BorderedArgs &a, hipStream_
     hipLaunchKernelGGL((k_solve_bordered<4>), dim3((g.m + 3) / 4), dim3(128), 4 * per_mode, s, g, a);
   else
     return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+hipError_t launch_reduce_blockmax(const double *bm, int nblocks, int nf, double *out, hipStream_t s) {
+  if (nblocks < 1 || nf < 1) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_reduce_blockmax, dim3(1), dim3(256), 0, s, bm, nblocks, nf, out);
   return hipGetLastError();
 }
 
