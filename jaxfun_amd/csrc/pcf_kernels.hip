@@ -194,6 +194,193 @@ __global__ void __launch_bounds__(256) k_rotational(Geo g, NodalMats<T> nm, FFTP
   }
 }
 
+// ------------------------------------------------------- k_rot_wave ----
+// k_rotational with the spanwise FFTs held in registers: KZ lanes (a power of
+// two, KZ = nz / 2) own one y-line pair, and the padded length MZ = 3 KZ is
+// factored as 3 x KZ.  Lane l holds entries l, l + KZ, l + 2 KZ of each line.
+//   synthesis (exp +):  a 3-point DFT in the lane (the middle entry is the
+//     zero padding), twiddles exp(+2 pi i l na / MZ), then a KZ-point DFT
+//     across the lanes, radix 2 with decimation in frequency (__shfl_xor);
+//     lane l ends with the physical point na + 3 bitrev(l) in register na;
+//   the product H = -(u_tot x w_tot) point by point, in that order;
+//   analysis (exp -):   the KZ-point DFT across the lanes with decimation in
+//     time (bit-reversed input, natural output), twiddles exp(-2 pi i l na /
+//     MZ) and the 3-point DFT in the lane give entries l and 2 KZ + l, the two
+//     that survive the truncation to |kz| < KZ.
+// The two real lines of a pair travel as the real and imaginary parts of one
+// complex line (as in k_rotational).  No LDS and no barrier: a wave holds
+// 64 / KZ pairs.
+// The value of lane (lane ^ H), H a power of two up to 32.  FP32: without LDS
+// traffic, by DPP quad permutes (H = 1, 2), row shifts (H = 4, 8: the lower
+// lane of a pair reads H lanes up, the upper H lanes down) and the gfx950 lane
+// swaps v_permlane16_swap / v_permlane32_swap (H = 16, 32).
+template <int H>
+__device__ __forceinline__ unsigned xchg_u32(unsigned v, int lane) {
+  if constexpr (H == 1) {
+    return (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0xB1, 0xF, 0xF, false);  // quad_perm [1,0,3,2]
+  } else if constexpr (H == 2) {
+    return (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x4E, 0xF, 0xF, false);  // quad_perm [2,3,0,1]
+  } else if constexpr (H == 4 || H == 8) {
+    const unsigned up = (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x100 + H, 0xF, 0xF, false);  // row_shl
+    const unsigned dn = (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x110 + H, 0xF, 0xF, false);  // row_shr
+    return (lane & H) ? dn : up;
+  } else if constexpr (H == 16) {
+    const auto r = __builtin_amdgcn_permlane16_swap(v, v, false, false);
+    return (lane & 16) ? r[0] : r[1];
+  } else {
+    static_assert(H == 32, "lane exchange distance");
+    const auto r = __builtin_amdgcn_permlane32_swap(v, v, false, false);
+    return (lane & 32) ? r[0] : r[1];
+  }
+}
+
+template <int H>
+__device__ __forceinline__ float xchg(float v, int lane) {
+  return __builtin_bit_cast(float, xchg_u32<H>(__builtin_bit_cast(unsigned, v), lane));
+}
+// FP64 values go through ds_bpermute: in the step (between the streamwise
+// FFTs) the DPP form of the FP64 kernel ran 25.6 us against 22.6 us, while the
+// FP32 one gained (16.1 -> 13.4 us at 65x64x64).
+template <int H>
+__device__ __forceinline__ double xchg(double v, int lane) {
+  (void)lane;
+  return __shfl_xor(v, H, 64);
+}
+template <int H, typename V>
+__device__ __forceinline__ V xchg_c(V v, int lane) {
+  return V{xchg<H>(v.x, lane), xchg<H>(v.y, lane)};
+}
+
+// Radix-2 stages of the KZ-point DFT across the lanes.  Synthesis (DIF,
+// natural order in, bit-reversed out) runs H = KZ/2 .. 1 on the 6 x 3 values of
+// the lane; analysis (DIT, bit-reversed in, natural out) runs H = 1 .. KZ/2 on
+// 3 x 3.  ws[log2 H] is the lane's twiddle exp(-2 pi i (l mod H) / (2 H)).
+template <typename V, int H, int NF>
+__device__ __forceinline__ void dif_stages(V (&v)[NF][3], const V *ws, int lane) {
+  if constexpr (H >= 1) {
+    const bool up = (lane & H) != 0;
+    const V w = cconj(ws[__builtin_ctz(H)]);
+#pragma unroll
+    for (int f = 0; f < NF; ++f)
+#pragma unroll
+      for (int na = 0; na < 3; ++na) {
+        const V p = xchg_c<H>(v[f][na], lane);
+        v[f][na] = up ? cmul(csub(p, v[f][na]), w) : cadd(v[f][na], p);
+      }
+    dif_stages<V, H / 2, NF>(v, ws, lane);
+  }
+}
+template <typename V, int H, int KZ, int NF>
+__device__ __forceinline__ void dit_stages(V (&v)[NF][3], const V *ws, int lane) {
+  if constexpr (H < KZ) {
+    const bool up = (lane & H) != 0;
+    const V w = ws[__builtin_ctz(H)];
+#pragma unroll
+    for (int f = 0; f < NF; ++f)
+#pragma unroll
+      for (int na = 0; na < 3; ++na) {
+        const V p = xchg_c<H>(v[f][na], lane);
+        const V t = cmul(w, up ? v[f][na] : p);
+        v[f][na] = up ? csub(p, t) : cadd(v[f][na], t);
+      }
+    dit_stages<V, 2 * H, KZ, NF>(v, ws, lane);
+  }
+}
+
+template <typename T, int KZ>
+__global__ void __launch_bounds__(128) k_rot_wave(Geo g, NodalMats<T> nm, const typename VT<T>::V *__restrict__ Y,
+                                                  typename VT<T>::V *__restrict__ Hz) {
+  using V = typename VT<T>::V;
+  constexpr int MZ = 3 * KZ, GROUPS = 64 / KZ;
+  const int lane = threadIdx.x & 63, l = lane & (KZ - 1), grp = lane / KZ;
+  const int wave = (int)((blockIdx.x * (size_t)blockDim.x + threadIdx.x) >> 6);
+  const int MY = g.my;
+  const int nunits = g.n0 * (MY / 2);
+  const int u = wave * GROUPS + grp;
+  const bool live = u < nunits;
+  const int uu = live ? u : 0;  // idle lanes read a valid line and store nothing
+  const int x = uu / (MY / 2), y2 = uu - x * (MY / 2);
+  const size_t fstride = (size_t)g.n0 * MY * KZ;
+  const size_t line = ((size_t)x * MY + 2 * y2) * KZ;
+  const V *tw = reinterpret_cast<const V *>(nm.twz);  // tw[t] = exp(-2 pi i t / MZ)
+  const V w1 = tw[l], w2 = tw[2 * l];
+  // the lane's twiddle of each radix-2 stage, exp(-2 pi i (l mod h) / (2 h))
+  V ws[6];
+#pragma unroll
+  for (int s = 0, h = 1; h < KZ; ++s, h <<= 1) ws[s] = tw[(l & (h - 1)) * (MZ / (2 * h))];
+  const T c3 = (T)-0.5, s3 = (T)0.86602540378443864676372317075294;  // exp(+2 pi i / 3)
+
+  // every load issued before the first use: one memory round trip
+  const int lr = (KZ - l) & (KZ - 1);  // mirrored entry (lane 0 reads entry 0, unused)
+  const T xv = nm.xnode[x];
+  V la[6], lb[6], ma[6], mb[6];
+#pragma unroll
+  for (int f = 0; f < 6; ++f) {
+    const V *ra = Y + f * fstride + line, *rb = ra + KZ;
+    la[f] = ra[l];
+    lb[f] = rb[l];
+    ma[f] = ra[lr];
+    mb[f] = rb[lr];
+  }
+  V v[6][3];
+#pragma unroll
+  for (int f = 0; f < 6; ++f) {
+    const V a = la[f], b = lb[f], a2 = ma[f], b2 = mb[f];
+    const V z0 = (l == 0) ? V{a.x, b.x} : V{a.x - b.y, a.y + b.x};
+    const V z2 = (l == 0) ? V{0, 0} : V{a2.x + b2.y, b2.x - a2.y};
+    // 3-point synthesis with a zero middle entry: t1 = z0 + w^2 z2, t2 = z0 + w z2
+    const V wz2{c3 * z2.x - s3 * z2.y, c3 * z2.y + s3 * z2.x};   // exp(+2 pi i / 3) z2
+    const V w2z2{c3 * z2.x + s3 * z2.y, c3 * z2.y - s3 * z2.x};  // exp(+4 pi i / 3) z2
+    v[f][0] = cadd(z0, z2);
+    v[f][1] = cmul(cadd(z0, w2z2), cconj(w1));
+    v[f][2] = cmul(cadd(z0, wz2), cconj(w2));
+  }
+  // KZ-point synthesis across the lanes, decimation in frequency
+  dif_stages<V, KZ / 2, 6>(v, ws, l);
+  // the product at every point the lane holds
+  const T U = (T)g.uwall;
+  V hh[3][3];
+#pragma unroll
+  for (int na = 0; na < 3; ++na) {
+    const V vu0 = v[0][na], vu1 = v[1][na], vu2 = v[2][na], vw0 = v[3][na], vw1 = v[4][na], vw2 = v[5][na];
+    {
+      const T a0 = vu0.x, a1 = vu1.x + U * xv, a2 = vu2.x, o0 = vw0.x, o1 = vw1.x, o2 = vw2.x + U;
+      hh[0][na].x = -(a1 * o2 - a2 * o1);
+      hh[1][na].x = -(a2 * o0 - a0 * o2);
+      hh[2][na].x = -(a0 * o1 - a1 * o0);
+    }
+    {
+      const T a0 = vu0.y, a1 = vu1.y + U * xv, a2 = vu2.y, o0 = vw0.y, o1 = vw1.y, o2 = vw2.y + U;
+      hh[0][na].y = -(a1 * o2 - a2 * o1);
+      hh[1][na].y = -(a2 * o0 - a0 * o2);
+      hh[2][na].y = -(a0 * o1 - a1 * o0);
+    }
+  }
+  // KZ-point analysis across the lanes, decimation in time
+  dit_stages<V, 1, KZ, 3>(hh, ws, l);
+  const T scale = (T)(0.5 / ((double)MY * (double)MZ));
+  const int src = grp * KZ + ((KZ - l) & (KZ - 1));
+#pragma unroll
+  for (int f = 0; f < 3; ++f) {
+    // entries l (k = l) and 2 KZ + l of the analysed line
+    const V q0 = hh[f][0], q1 = cmul(hh[f][1], w1), q2 = cmul(hh[f][2], w2);
+    const V P0 = cadd(cadd(q0, q1), q2);
+    const V wq1{c3 * q1.x - s3 * q1.y, c3 * q1.y + s3 * q1.x};   // exp(+2 pi i / 3) q1
+    const V w2q2{c3 * q2.x + s3 * q2.y, c3 * q2.y - s3 * q2.x};  // exp(+4 pi i / 3) q2
+    const V P2 = cadd(cadd(q0, wq1), w2q2);
+    // P[MZ - k] = P2 of lane KZ - k (k >= 1); P[0] for k = 0
+    const V r{__shfl(P2.x, src, 64), __shfl(P2.y, src, 64)};
+    const V q = cconj(l == 0 ? P0 : r);
+    const V ha{(P0.x + q.x) * scale, (P0.y + q.y) * scale};
+    const V hb{(P0.y - q.y) * scale, -(P0.x - q.x) * scale};
+    if (live) {
+      V *dst = Hz + f * fstride + line;
+      dst[l] = ha;
+      dst[KZ + l] = hb;
+    }
+  }
+}
+
 // ------------------------------------------------------- k_absmax_z ----
 // Largest |value| of nf physical fields on the unpadded grid (g is the
 // unpadded geometry, MY = ny, MZ = nz), for the CFL numbers of
@@ -568,7 +755,7 @@ __global__ void __launch_bounds__(256) k_bits_differ(const double2 *__restrict__
   bool d = false;
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < 2 * n; i += (size_t)gridDim.x * blockDim.x)
     d |= x[i] != y[i];
-  if (d) atomicOr(diff, 1u);
+  if (__any(d) && (threadIdx.x & 63) == 0) atomicOr(diff, 1u);  // one atomic per wave
 }
 
 // -------------------------------------------------------------- k_diag ----
@@ -864,6 +1051,19 @@ hipError_t launch_rotational(const Geo &g, const NodalMats<T> &nm, Workspace &ws
                              const FFTPlan &pz, bool mhd, hipStream_t s) {
   using V = typename VT<T>::V;
   const int nunits = g.n0 * (g.my / 2);
+  if (!mhd && g.mz == 3 * g.kzs && g.my % 2 == 0) {
+    // spanwise FFTs in registers when nz / 2 is a power of two up to 64
+    const V *Yw = reinterpret_cast<const V *>(ws.Y);
+    V *Hw = reinterpret_cast<V *>(ws.Hz);
+    auto grid_for = [&](int kz) { return dim3((unsigned)((nunits + 2 * (64 / kz) - 1) / (2 * (64 / kz)))); };
+    switch (g.kzs) {
+      case 8: hipLaunchKernelGGL((k_rot_wave<T, 8>), grid_for(8), dim3(128), 0, s, g, nm, Yw, Hw); return hipGetLastError();
+      case 16: hipLaunchKernelGGL((k_rot_wave<T, 16>), grid_for(16), dim3(128), 0, s, g, nm, Yw, Hw); return hipGetLastError();
+      case 32: hipLaunchKernelGGL((k_rot_wave<T, 32>), grid_for(32), dim3(128), 0, s, g, nm, Yw, Hw); return hipGetLastError();
+      case 64: hipLaunchKernelGGL((k_rot_wave<T, 64>), grid_for(64), dim3(128), 0, s, g, nm, Yw, Hw); return hipGetLastError();
+      default: break;
+    }
+  }
   const int upb = rot_units<T>(g, mhd);
   const size_t lds = ((size_t)upb * (mhd ? 18 : 9) * g.mz + g.mz) * sizeof(V);
   const int mb = rot_butterflies(pz, upb * (mhd ? 12 : 6));
