@@ -247,6 +247,22 @@ int32_t pcf_field_maxima(pcf_handle *h, double *out6);
  * (two host threads, or two streams in flight) race.  Synchronous. */
 int32_t pcf_field_maxima_noalloc(pcf_handle *h, double *out6);
 
+/* Per-component perturbation energies of the current state (synchronous):
+ * out[i] = int |u_i|^2 dV, i = x, y, z, by the quadrature of
+ * pcf_diagnostics (their sum is Epert).  The integrals behind streak_rms =
+ * sqrt(out[1] / V) and roll_rms = sqrt((out[0] + out[2]) / V) of the
+ * reference runner's _pcf_fluctuation_scalars (production/oracles.py:4566-4597). */
+int32_t pcf_velocity_energies(pcf_handle *h, double *out3);
+
+/* Physical fluctuation fields of the current state on the unpadded grid
+ * (synchronous): out holds nf x (n0, ny, nz) real doubles, C order, x the
+ * Gauss-node index (ascending nodes), y and z the grid indices, with nf = 3
+ * (velocity u) or, for MHD handles, 6 (u, then b = curl A without B0).
+ * Replaces solver._backward_velocity (examples/channelflow_kmm.py:688-697)
+ * and _backward_B(B, padded=False) (examples/pcf_mhd_jax.py:313-333).
+ * PCF_E_UNSUPPORTED when ny or nz has no 2/3/4 FFT plan. */
+int32_t pcf_physical_fields(pcf_handle *h, double *out);
+
 /* Per-stage timing.  One steady step launches pcf_kernels_per_step() kernels,
  * one per stage, in the order pcf_stage_name(0..pcf_stage_count()-1):
  * nodal_backward, fft_y_inverse, rotational, fft_y_forward, forward_rhs, solve,

@@ -54,7 +54,7 @@ EXPORTS = (
     "pcf_create", "pcf_destroy", "pcf_set_stream", "pcf_synchronize",
     "pcf_set_field", "pcf_get_field", "pcf_set_history", "pcf_get_history",
     "pcf_step", "pcf_set_dt", "pcf_step_steady", "pcf_step_steady_graph", "pcf_diagnostics", "pcf_mhd_diagnostics", "pcf_field_maxima",
-    "pcf_field_maxima_noalloc",
+    "pcf_field_maxima_noalloc", "pcf_velocity_energies", "pcf_physical_fields",
     "pcf_kernels_per_step", "pcf_stage_count", "pcf_stage_name", "pcf_profile_steady", "pcf_profile_stage",
     "pcf_set_cotangent", "pcf_vjp", "pcf_get_gradient",
 )
@@ -130,6 +130,8 @@ def load() -> C.CDLL:
         "pcf_mhd_diagnostics": ([h, dp], C.c_int32),
         "pcf_field_maxima": ([h, dp], C.c_int32),
         "pcf_field_maxima_noalloc": ([h, dp], C.c_int32),
+        "pcf_velocity_energies": ([h, dp], C.c_int32),
+        "pcf_physical_fields": ([h, dp], C.c_int32),
         "pcf_kernels_per_step": ([h], C.c_int32),
         "pcf_stage_count": ([], C.c_int32),
         "pcf_stage_name": ([C.c_int32], C.c_char_p),

@@ -841,7 +841,7 @@ int32_t cnab2_step(pcf_handle *h) {
 // Nodal fields of the current state, their y-lines on the unpadded grid and
 // the per-block maxima (velocity; under MHD also b = curl A).
 template <typename T>
-hipError_t field_maxima_stages(pcf_handle *h, const DevMats<T> &dm, double *blockmax) {
+hipError_t field_maxima_stages(pcf_handle *h, const DevMats<T> &dm, double *blockmax, double *phys = nullptr) {
   const pcf::Geo &g = h->geo, &g0 = h->geo0;
   pcf::NodalMats<T> nm0 = dm.view;
   nm0.twy = dm.twy0;
@@ -856,7 +856,7 @@ hipError_t field_maxima_stages(pcf_handle *h, const DevMats<T> &dm, double *bloc
   if (h->mhd && (e = pcf::launch_fft_y_fields<T>(g0, nm0, h->py0, true, 3, X + (size_t)2 * 6 * g.n0 * g.m,
                                                  Y + (size_t)2 * 3 * g.n0 * g0.my * g0.kzs, h->stream)))
     return e;
-  return pcf::launch_absmax<T>(g0, dm.xnode, dm.twz0, h->pz0, h->mhd ? 6 : 3, Y, blockmax, h->stream);
+  return pcf::launch_absmax<T>(g0, dm.xnode, dm.twz0, h->pz0, h->mhd ? 6 : 3, Y, blockmax, h->stream, phys);
 }
 
 // The kernel(s) of nonlinear stage `stage` (0 nodal_backward .. 4 forward_rhs)
@@ -1685,7 +1685,7 @@ int32_t pcf_create(const pcf_config *cfg, pcf_handle **out) {
   if ((r = h->upload(&h->dVD1, h->ops.VD1.a))) return bail(r);
   if ((r = h->upload(&h->dfejer, h->ops.fejer))) return bail(r);
   if ((r = h->upload(&h->dxn, h->ops.x))) return bail(r);
-  if ((r = h->alloc(&h->ddiag, 4))) return bail(r);
+  if ((r = h->alloc(&h->ddiag, 6))) return bail(r);
   if ((r = h->alloc(&h->dflag, 1))) return bail(r);
   if (h->maxima_ok) {
     const int nf = h->mhd ? 6 : 3, eb = cfg->precision == PCF_PREC_FP64 ? 16 : 8;
@@ -1913,7 +1913,7 @@ int32_t pcf_diagnostics(pcf_handle *h, double *out) {
   const pcf::Geo &g = h->geo;
   PCF_HIP(hipSetDevice(h->device));
   PCF_HIP(ensure_u12(h));
-  PCF_HIP(hipMemsetAsync(h->ddiag, 0, 4 * sizeof(double), h->stream));
+  PCF_HIP(hipMemsetAsync(h->ddiag, 0, 6 * sizeof(double), h->stream));
   hipError_t e = pcf::launch_diag(g, h->dVB, h->dVD, h->dVB1, h->dVD1, h->dfejer, h->dxn,
                                   h->u0[h->so[0]], h->u1, h->u2, h->ddiag, h->stream);
   if (e != hipSuccess) return fail(PCF_E_HIP, std::string("diag launch: ") + hipGetErrorString(e));
@@ -2021,6 +2021,48 @@ int32_t pcf_field_maxima_noalloc(pcf_handle *h, double *out) {
   if (e != hipSuccess) return fail(PCF_E_HIP, std::string("field maxima launch: ") + hipGetErrorString(e));
   for (int f = 0; f < 6; ++f) out[f] = 0.0;
   PCF_HIP(hipMemcpyAsync(out, fin, nf * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  PCF_HIP(hipStreamSynchronize(h->stream));
+  return PCF_OK;
+}
+
+int32_t pcf_velocity_energies(pcf_handle *h, double *out3) {
+  if (!h || !out3) return fail(PCF_E_INVALID, "null argument");
+  const pcf::Geo &g = h->geo;
+  PCF_HIP(hipSetDevice(h->device));
+  PCF_HIP(ensure_u12(h));
+  PCF_HIP(hipMemsetAsync(h->ddiag, 0, 6 * sizeof(double), h->stream));
+  hipError_t e = pcf::launch_diag(g, h->dVB, h->dVD, h->dVB1, h->dVD1, h->dfejer, h->dxn, h->u0[h->so[0]], h->u1,
+                                  h->u2, h->ddiag, h->stream);
+  if (e != hipSuccess) return fail(PCF_E_HIP, std::string("diag launch: ") + hipGetErrorString(e));
+  double s[6];
+  PCF_HIP(hipMemcpyAsync(s, h->ddiag, sizeof(s), hipMemcpyDeviceToHost, h->stream));
+  PCF_HIP(hipStreamSynchronize(h->stream));
+  const double LL = h->cfg.ly * h->cfg.lz;
+  out3[0] = LL * s[4];
+  out3[1] = LL * s[5];
+  out3[2] = LL * (s[0] - s[4] - s[5]);
+  return PCF_OK;
+}
+
+int32_t pcf_physical_fields(pcf_handle *h, double *out) {
+  if (!h || !out) return fail(PCF_E_INVALID, "null argument");
+  if (!h->maxima_ok)
+    return fail(PCF_E_UNSUPPORTED, "physical fields need ny, nz that factor into 2, 3, 4 within the FFT kernels");
+  const pcf::Geo &g0 = h->geo0;
+  const int nf = h->mhd ? 6 : 3;
+  const size_t count = (size_t)nf * g0.n0 * g0.my * g0.mz;
+  // the output region: the workspace's forward-transform buffers (free between steps)
+  const size_t ve = (h->cfg.precision == PCF_PREC_FP64) ? sizeof(double2) : sizeof(float2);
+  const size_t fout = h->mhd ? 6 : 3;
+  const size_t room = fout * (size_t)h->geo.n0 * ((size_t)h->geo.my * h->geo.kzs + h->geo.m) * ve;
+  if (count * sizeof(double) > room) return fail(PCF_E_UNSUPPORTED, "physical fields exceed the workspace");
+  PCF_HIP(hipSetDevice(h->device));
+  double *phys = reinterpret_cast<double *>(h->ws.Hz);
+  const bool fp64 = h->cfg.precision == PCF_PREC_FP64;
+  hipError_t e = fp64 ? field_maxima_stages<double>(h, h->md, h->dblockmax, phys)
+                      : field_maxima_stages<float>(h, h->mf, h->dblockmax, phys);
+  if (e != hipSuccess) return fail(PCF_E_HIP, std::string("physical fields launch: ") + hipGetErrorString(e));
+  PCF_HIP(hipMemcpyAsync(out, phys, count * sizeof(double), hipMemcpyDeviceToHost, h->stream));
   PCF_HIP(hipStreamSynchronize(h->stream));
   return PCF_OK;
 }

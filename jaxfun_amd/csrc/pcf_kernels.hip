@@ -392,7 +392,7 @@ template <typename T, int MAXB>
 __global__ void __launch_bounds__(256) k_absmax_z(Geo g, const typename VT<T>::V *__restrict__ twz, FFTPlan plan,
                                                   int UPB, int nf, const T *__restrict__ xnode,
                                                   const typename VT<T>::V *__restrict__ Y,
-                                                  double *__restrict__ blockmax) {
+                                                  double *__restrict__ blockmax, double *__restrict__ phys) {
   using V = typename VT<T>::V;
   extern __shared__ __align__(16) unsigned char smem[];
   const int MZ = g.mz, MY = g.my, KZ = g.kzs;
@@ -430,6 +430,20 @@ __global__ void __launch_bounds__(256) k_absmax_z(Geo g, const typename VT<T>::V
       [&](int idx, const V &z) { buf[idx] = z; });
   __syncthreads();
   fft_lds<V, T, true, false, MAXB>(buf, tw, plan, UPB * nf, MZ, 1);
+  if (phys) {
+    // the fluctuation fields themselves, [field][x][y][z] (the two lines of a
+    // unit are y = 2 y2 and 2 y2 + 1)
+    for (int idx = tid; idx < UPB * nf * MZ; idx += nthr) {
+      const int lu = idx / (nf * MZ), rem = idx - lu * nf * MZ, f = rem / MZ, z = rem - f * MZ;
+      const int u = ubase + lu;
+      if (u >= nunits) continue;
+      const int x = u / (MY / 2), y2 = u - x * (MY / 2);
+      const V v = buf[idx];
+      double *o = phys + (((size_t)f * g.n0 + x) * MY + 2 * y2) * MZ + z;
+      o[0] = (double)v.x;
+      o[MZ] = (double)v.y;
+    }
+  }
   double mx[6] = {0, 0, 0, 0, 0, 0};
   for (int idx = tid; idx < UPB * nf * MZ; idx += nthr) {
     const int lu = idx / (nf * MZ), f = (idx - lu * nf * MZ) / MZ;
@@ -771,7 +785,7 @@ __global__ void __launch_bounds__(256) k_diag(Geo g, const double *__restrict__ 
                                               double *__restrict__ out) {
   const int m = blockIdx.x * blockDim.x + threadIdx.x;
   const int x = blockIdx.y;  // one node per block row: the matrix rows are uniform
-  double e = 0.0, dv = 0.0, cross = 0.0, msh = 0.0;
+  double e = 0.0, dv = 0.0, cross = 0.0, msh = 0.0, e0 = 0.0, e1 = 0.0;
   if (m < g.m) {
     double kyd, kzd, kyo;
     mode_k(g, m, kyd, kzd, kyo);
@@ -798,6 +812,8 @@ __global__ void __launch_bounds__(256) k_diag(Geo g, const double *__restrict__ 
       }
       const double w = fejer[x];
       e += w * wt * (U0.x * U0.x + U0.y * U0.y + U1.x * U1.x + U1.y * U1.y + U2.x * U2.x + U2.y * U2.y);
+      e0 += w * wt * (U0.x * U0.x + U0.y * U0.y);
+      e1 += w * wt * (U1.x * U1.x + U1.y * U1.y);
       // div = dU0/dx + i ky U1 + i kz U2
       double2 dvv{D0.x - kyo * U1.y - kzd * U2.y, D0.y + kyo * U1.x + kzd * U2.x};
       dv += w * wt * (dvv.x * dvv.x + dvv.y * dvv.y);
@@ -808,19 +824,21 @@ __global__ void __launch_bounds__(256) k_diag(Geo g, const double *__restrict__ 
     }
   }
   // block reduction
-  __shared__ double red[4][256];
+  __shared__ double red[6][256];
   red[0][threadIdx.x] = e;
   red[1][threadIdx.x] = dv;
   red[2][threadIdx.x] = cross;
   red[3][threadIdx.x] = msh;
+  red[4][threadIdx.x] = e0;
+  red[5][threadIdx.x] = e1;
   __syncthreads();
   for (int s = blockDim.x / 2; s > 0; s >>= 1) {
     if ((int)threadIdx.x < s)
-      for (int q = 0; q < 4; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + s];
+      for (int q = 0; q < 6; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + s];
     __syncthreads();
   }
   if (threadIdx.x == 0)
-    for (int q = 0; q < 4; ++q) atomicAdd(&out[q], red[q][0]);
+    for (int q = 0; q < 6; ++q) atomicAdd(&out[q], red[q][0]);
 }
 
 // Magnetic diagnostics of the vector potential A (three components cstride
@@ -1113,7 +1131,7 @@ bool absmax_fits(const Geo &g, const FFTPlan &pz, int nf, int elem_bytes) {
 
 template <typename T>
 hipError_t launch_absmax(const Geo &g, const T *xnode, const T *twz, const FFTPlan &pz, int nf, const void *Y,
-                         double *blockmax, hipStream_t s) {
+                         double *blockmax, hipStream_t s, double *phys) {
   using V = typename VT<T>::V;
   if (!absmax_fits(g, pz, nf, sizeof(V))) return hipErrorInvalidValue;
   const int upb = absmax_units(g, nf, sizeof(V));
@@ -1123,7 +1141,7 @@ hipError_t launch_absmax(const Geo &g, const T *xnode, const T *twz, const FFTPl
   const V *tw = reinterpret_cast<const V *>(twz);
   const V *y = reinterpret_cast<const V *>(Y);
 #define PCF_ABSMAX(MB) \
-  hipLaunchKernelGGL((k_absmax_z<T, MB>), grid, dim3(256), lds, s, g, tw, pz, upb, nf, xnode, y, blockmax)
+  hipLaunchKernelGGL((k_absmax_z<T, MB>), grid, dim3(256), lds, s, g, tw, pz, upb, nf, xnode, y, blockmax, phys)
   if (mb <= 3) PCF_ABSMAX(3);
   else if (mb <= 6) PCF_ABSMAX(6);
   else PCF_ABSMAX(12);
@@ -1194,7 +1212,7 @@ bool fft_fits(const Geo &g, const FFTPlan &py, const FFTPlan &pz, int elem_bytes
 
 #define PCF_INSTANTIATE(T)                                                                     \
   template hipError_t launch_absmax<T>(const Geo &, const T *, const T *, const FFTPlan &, int,   \
-                                       const void *, double *, hipStream_t);                     \
+                                       const void *, double *, hipStream_t, double *);           \
   template hipError_t launch_fft_y<T>(const Geo &, const NodalMats<T> &, Workspace &,            \
                                       const FFTPlan &, bool, bool, hipStream_t);                  \
   template hipError_t launch_fft_y_fields<T>(const Geo &, const NodalMats<T> &, const FFTPlan &,   \

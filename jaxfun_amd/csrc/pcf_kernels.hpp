@@ -210,9 +210,11 @@ hipError_t launch_insul_residual(const Geo &g, const double2 *A, size_t cstride,
 // unpadded geometry; Y holds their y-transformed lines, [nf][n0][ny][kzs]):
 // per-block maxima in blockmax[absmax_blocks][nf].  Fields 0..2 are the
 // velocity (field 1 gets the base flow U x), 3..5 the magnetic field (+ B0).
+// With phys set, the fluctuation fields themselves also go there, real FP64
+// [nf][n0][ny][nz] on the unpadded grid.
 template <typename T>
 hipError_t launch_absmax(const Geo &g, const T *xnode, const T *twz, const FFTPlan &pz, int nf, const void *Y,
-                         double *blockmax, hipStream_t s);
+                         double *blockmax, hipStream_t s, double *phys = nullptr);
 int absmax_blocks(const Geo &g, int nf, int elem_bytes);
 bool absmax_fits(const Geo &g, const FFTPlan &pz, int nf, int elem_bytes);
 

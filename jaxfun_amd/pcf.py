@@ -423,6 +423,62 @@ class PlaneCouetteFluctuationHIP:
                 "mean_shear")
         return {k: float(v) for k, v in zip(keys, out)}
 
+    def _physical(self, state=None) -> np.ndarray:
+        """Physical fluctuation fields of ``state`` (or of the resident state)
+        on the unpadded grid, (nf, n0, ny, nz) real, from the device transforms."""
+        if state is not None:
+            self._ensure_loaded(state)
+        nf = 6 if self._cfg.mhd else 3
+        out = np.empty((nf, self.N[0], self.N[1], self.N[2]), np.float64)
+        nat.check(self._lib.pcf_physical_fields(self._h, out.ctypes.data_as(C.POINTER(C.c_double))))
+        return out
+
+    def _backward_velocity(self, u, padded: bool = False):
+        """``_backward_velocity`` (``channelflow_kmm.py:688-697``): the physical
+        fluctuation velocity on the unpadded grid (real float64 (n0, ny, nz)
+        arrays; nodes ascending in x).  ``u`` is the velocity of a state this
+        solver holds (the last one loaded, stepped or returned); a foreign
+        velocity is loaded with g = i ky u2 - i kz u1 for the transform."""
+        if padded:
+            raise NotImplementedError("the padded physical velocity stays inside the step")
+        loaded = self._loaded
+        if loaded is None or not all(a is b for a, b in zip(u, loaded.u)):
+            ky, kz = self.wavenumbers()
+            g = self.mask_nyquist(1j * ky[None] * u[2] - 1j * kz[None] * u[1])
+            self.load(self._with_zero_history(tuple(u), g))
+            self._loaded = KMMState(u=tuple(u), g=g)
+        up = self._physical()
+        return up[0], up[1], up[2]
+
+    def velocity_energies(self, state: KMMState | None = None):
+        """(int |u_0|^2, int |u_1|^2, int |u_2|^2) dV of ``state``, device reductions."""
+        if state is not None:
+            self._ensure_loaded(state)
+        out = np.zeros(3)
+        nat.check(self._lib.pcf_velocity_energies(self._h, out.ctypes.data_as(C.POINTER(C.c_double))))
+        return tuple(float(v) for v in out)
+
+    def fluctuation_scalars(self, state: KMMState | None = None) -> dict:
+        """The scalars of the reference runner's ``_pcf_fluctuation_scalars``
+        (``production/oracles.py:4566-4597``), same keys: the diagnostics plus
+        streak_rms = sqrt(int |u_y|^2 / V) and roll_rms =
+        sqrt((int |u_x|^2 + int |u_z|^2) / V), all from device reductions."""
+        d = self.diagnostics(state)
+        e = self.velocity_energies()
+        volume = 2.0 * self.Ly * self.Lz
+        return {
+            "kinetic_energy": d["Epert"],
+            "total_kinetic_energy": d["Etot"],
+            "divergence_l2": d["divL2"],
+            "wall_shear_lower": d["wall_shear_bot"],
+            "wall_shear_upper": d["wall_shear_top"],
+            "wall_velocity_lower": d["u_bot"],
+            "wall_velocity_upper": d["u_top"],
+            "mean_shear": d["mean_shear"],
+            "streak_rms": math.sqrt(e[1] / volume),
+            "roll_rms": math.sqrt((e[0] + e[2]) / volume),
+        }
+
     def field_maxima(self, state: KMMState | None = None):
         """Largest |u_i + U_i| (total velocity, ``total_velocity_physical``,
         ``pcf_fluctuations_jax.py:114-116``) and |b_i + B0_i| (MHD; zeros

@@ -855,6 +855,25 @@ class PCFOracle:
             "mean_shear": float(np.real(self.integrate(dvdx + self.dUb)) / volume),
         }
 
+    def backward_velocity(self, s: OracleState):
+        """``_backward_velocity`` (``channelflow_kmm.py:688-697``) on the unpadded grid."""
+        u0, u1, u2 = s.u
+        return (self.tp_backward(u0, self.SB, padded=False), self.tp_backward(u1, self.SD, padded=False),
+                self.tp_backward(u2, self.SD, padded=False))
+
+    def fluctuation_scalars(self, s: OracleState) -> dict:
+        """``_pcf_fluctuation_scalars`` (``production/oracles.py:4566-4597``)."""
+        d = self.diagnostics(s)
+        up = self.backward_velocity(s)
+        volume = 2.0 * self.L1 * self.L2
+        streak = np.sqrt(np.real(self.integrate(up[1] * np.conj(up[1]))) / volume)
+        roll = np.sqrt((np.real(self.integrate(up[0] * np.conj(up[0])))
+                        + np.real(self.integrate(up[2] * np.conj(up[2])))) / volume)
+        return {"kinetic_energy": d["Epert"], "total_kinetic_energy": d["Etot"], "divergence_l2": d["divL2"],
+                "wall_shear_lower": d["wall_shear_bot"], "wall_shear_upper": d["wall_shear_top"],
+                "wall_velocity_lower": d["u_bot"], "wall_velocity_upper": d["u_top"],
+                "mean_shear": d["mean_shear"], "streak_rms": float(streak), "roll_rms": float(roll)}
+
     @staticmethod
     def checksum(s: OracleState) -> dict:
         """Full-state checksum (``production/benchmark.py:234-260``)."""

@@ -387,3 +387,44 @@ def test_transform_coefficient_path_equals_optimized(N):
     v, w = rng.standard_normal(o.ND), rng.standard_normal(o.ND)
     for x, y in zip(o.reconstruct_transform(u0, g, v, w), o.reconstruct(u0, g, v, w)):
         assert np.max(np.abs(x - y)) < 1e-12 * np.max(np.abs(y))
+
+
+def test_backward_velocity_matches_direct_series_sum():
+    """The unpadded physical fluctuation velocity against a direct evaluation of
+    the Chebyshev x Fourier series at every grid point."""
+    o = PCFOracle(N=(9, 4, 6), Re=200.0, dt=1e-3, perturbation_amplitude=0.3)
+    s = o.initial_state()
+    n0, n1, n2 = o.N
+    x = cheb_nodes(n0)
+    ky = np.fft.fftfreq(n1, 1.0 / n1) * 2 * np.pi / o.L1
+    kz = np.fft.fftfreq(n2, 1.0 / n2) * 2 * np.pi / o.L2
+    Ey = np.exp(1j * np.outer(np.arange(n1) * o.L1 / n1, ky))
+    Ez = np.exp(1j * np.outer(np.arange(n2) * o.L2 / n2, kz))
+    V = npcheb.chebvander(x, n0 - 1)
+    for got, c, S in zip(o.backward_velocity(s), s.u, (o.SB, o.SD, o.SD)):
+        want = np.einsum("ia,apq,jp,kq->ijk", V @ S.T, c, Ey, Ez)
+        assert np.max(np.abs(got - want)) < 1e-13 * np.max(np.abs(want))
+
+
+def test_fluctuation_scalars_split_the_perturbation_energy():
+    """streak_rms^2 + roll_rms^2 = Epert / V (the two integrals partition the
+    energy the published diagnostics hold), and each measures its own
+    components: a state with only u_y has no roll, one without u_y no streak."""
+    o = PCFOracle(N=(17, 16, 16), Re=400.0, dt=1e-3, perturbation_amplitude=0.05)
+    s = o.solve(o.initial_state(), 3)
+    V = 2.0 * o.L1 * o.L2
+    f = o.fluctuation_scalars(s)
+    d = o.diagnostics(s)
+    assert f["roll_rms"] > 0.0 and f["streak_rms"] > 0.0
+    assert math.isclose((f["streak_rms"] ** 2 + f["roll_rms"] ** 2) * V, d["Epert"], rel_tol=1e-13)
+    assert {k: f[k] for k in ("kinetic_energy", "total_kinetic_energy", "divergence_l2", "mean_shear")} == {
+        "kinetic_energy": d["Epert"], "total_kinetic_energy": d["Etot"], "divergence_l2": d["divL2"],
+        "mean_shear": d["mean_shear"]}
+    assert (f["wall_shear_lower"], f["wall_shear_upper"]) == (d["wall_shear_bot"], d["wall_shear_top"])
+    assert (f["wall_velocity_lower"], f["wall_velocity_upper"]) == (d["u_bot"], d["u_top"])
+    u0, u1, u2 = s.u
+    only_streak = replace(s, u=(0 * u0, u1, 0 * u2))
+    only_roll = replace(s, u=(u0, 0 * u1, u2))
+    fs, fr = o.fluctuation_scalars(only_streak), o.fluctuation_scalars(only_roll)
+    assert fs["roll_rms"] == 0.0 and math.isclose(fs["streak_rms"], f["streak_rms"], rel_tol=1e-14)
+    assert fr["streak_rms"] == 0.0 and math.isclose(fr["roll_rms"], f["roll_rms"], rel_tol=1e-14)

@@ -1,0 +1,207 @@
+"""Host logic of the problem-spec runner: spec parsing, scalar rules, health
+guards and the checkpoint container and payload (no GPU).
+
+Known answers come from the reference's own tests where they exist
+(``tests/production/test_run_problem.py:472-475`` for the saturation rule) and
+otherwise from values worked by hand in each test.
+"""
+
+import json
+import math
+import warnings
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from jaxfun_amd.pcf import KMMState
+from jaxfun_amd.runner import (
+    ProductionOracleNotImplementedError,
+    checkpoint_payload,
+    dedupe_time_rows,
+    growth_rate_from_energy,
+    kmm_time_integrator,
+    materialize_resolution_tier,
+    padding_factor,
+    raise_on_unhealthy,
+    read_checkpoint,
+    run_pcf_fluctuation_saturation,
+    saturation_passed,
+    selected_resolution,
+    state_from_checkpoint_payload,
+    stationarity_scalars,
+    steps_from_spec,
+    write_checkpoint,
+)
+
+SPEC = {
+    "problem_id": "pcf_fluct_re400",
+    "nondimensional_groups": {"Re": 400.0, "U_wall": 1.0},
+    "domain": {"x": [-1.0, 1.0], "y_period": 4 * math.pi, "z_period": 2 * math.pi},
+    "resolution": {"smoke": {"Nx": 9, "Ny": 4, "Nz": 4, "dealias": [1.0, 1.0, 1.0]},
+                   "start": {"Nx": 32, "Ny": 64, "Nz": 32}, "production": {"Nx": 64, "Ny": 128, "Nz": 64},
+                   "dealias": [1.0, 1.5, 1.5], "family": "C"},
+    "time": {"integrator": "IMEXRK222", "dt": 0.01, "final_time": 50.0},
+    "initial_condition": {"type": "finite_amplitude_fluctuation", "amplitude": 0.1},
+}
+
+
+def test_resolution_tiers():
+    assert selected_resolution(SPEC)["Nx"] == 64  # the production tier by default
+    start = materialize_resolution_tier(SPEC, "start")
+    assert selected_resolution(start)["Nx"] == 32 and start["resolution"]["dealias"] == [1.0, 1.5, 1.5]
+    assert SPEC["resolution"].get("Nx") is None  # the spec itself is not changed
+    smoke = materialize_resolution_tier(SPEC, "smoke")
+    assert selected_resolution(smoke)["dealias"] == [1.0, 1.0, 1.0]
+    with pytest.raises(ValueError, match="not defined"):
+        materialize_resolution_tier(SPEC, "huge")
+
+
+def test_padding_forms():
+    assert padding_factor({"dealias": {"z": 1.5, "x": 1.0, "y": 2.0}}) == (1.0, 2.0, 1.5)
+    assert padding_factor({"dealias": 1.5}) == (1.5, 1.5, 1.5)
+    assert padding_factor({}) == (1.0, 1.0, 1.0)
+    with warnings.catch_warnings(record=True) as w:
+        warnings.simplefilter("always")
+        assert padding_factor({"dealias": [1.0, 1.5, 1.5]}) == (1.0, 1.5, 1.5)
+        assert padding_factor({"dealias": [1.5, 1.5, 1.5]}) == (1.5, 1.5, 1.5)
+    assert [x.category for x in w] == [DeprecationWarning]  # only the non-uniform list
+    for bad in ({"x": 1.0, "y": 1.5}, [1.0, 1.5], 0.5, [1.0, "a", 1.5]):
+        with pytest.raises(ValueError):
+            padding_factor({"dealias": bad})
+
+
+def test_time_integrator_and_steps():
+    assert kmm_time_integrator(SPEC) == "IMEXRK222"
+    for bad in ("RK4", "Dedalus"):
+        with pytest.raises(ProductionOracleNotImplementedError, match="time-stepping integrator"):
+            kmm_time_integrator({"time": {"integrator": bad}})
+    with pytest.raises(ProductionOracleNotImplementedError, match="adaptive_cfl"):
+        kmm_time_integrator({"time": {"integrator": "SBDF3", "adaptive_cfl": {"target": 0.5}}})
+    assert steps_from_spec(SPEC) == 5000
+    assert steps_from_spec(SPEC, 7) == 7
+    with pytest.raises(ValueError):
+        steps_from_spec(SPEC, -1)
+    with pytest.raises(ValueError, match="integer multiple"):
+        steps_from_spec({"time": {"dt": 0.03, "final_time": 0.1}})
+
+
+def test_growth_rate_rule():
+    # 0.5 log(e1/e0) / (steps dt): e1/e0 = e^2 over t = 1 gives 1
+    assert math.isclose(growth_rate_from_energy(1.0, math.exp(2.0), 10, 0.1), 1.0, rel_tol=1e-15)
+    assert math.isclose(growth_rate_from_energy(2.0, 0.5, 4, 0.5), 0.5 * math.log(0.25) / 2.0, rel_tol=1e-15)
+    assert growth_rate_from_energy(1.0, 0.0, 1, 0.1) == -math.inf
+    assert math.isnan(growth_rate_from_energy(0.0, 1.0, 1, 0.1))
+    assert math.isnan(growth_rate_from_energy(1.0, math.inf, 1, 0.1))
+    with pytest.raises(ValueError, match="at least one"):
+        growth_rate_from_energy(1.0, 2.0, 0, 0.1)
+
+
+def test_saturation_rule_reference_cases():
+    assert not saturation_passed(float("inf"), threshold=2.0, final_energies=(1.0,))
+    assert not saturation_passed(3.0, threshold=2.0, final_energies=(float("inf"),))
+    assert not saturation_passed(1.5, threshold=2.0, final_energies=(1.0,))
+    assert saturation_passed(3.0, threshold=2.0, final_energies=(1.0, 0.0))
+    assert not saturation_passed(2.0, threshold=2.0, final_energies=(1.0,))  # strictly above
+    assert not saturation_passed(3.0, threshold=2.0, final_energies=(-1e-30,))
+    assert not saturation_passed("x", threshold=2.0, final_energies=())
+
+
+def test_stationarity_windows():
+    rows = [{"t": i, "E": v} for i, v in enumerate([1.0, 2.0, 3.0, 4.0])]
+    out = stationarity_scalars(rows, key="E")
+    # quarter = 1: previous mean 3, current mean 4, change 1/4
+    assert out["stationarity_previous_mean"] == 3.0 and out["stationarity_current_mean"] == 4.0
+    assert out["stationarity_relative_change"] == 0.25 and out["stationarity_check_passed"] is False
+    flat = [{"E": v} for v in [5.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.02]]
+    out = stationarity_scalars(flat, key="E")  # quarter = 2: means 1.0 and 1.01
+    assert math.isclose(out["stationarity_relative_change"], 0.01 / 1.01, rel_tol=1e-12)
+    assert out["stationarity_check_passed"] is True and out["stationarity_window_samples"] == 8
+    few = stationarity_scalars(rows[:3] + [{"t": 9, "E": math.nan}], key="E")
+    assert few["stationarity_window_samples"] == 3 and few["stationarity_check_passed"] is None
+
+
+def test_dedupe_keeps_the_last_row_of_a_time():
+    rows = [{"t": 0.0, "a": 1}, {"t": 0.1, "a": 2}, {"t": 0.1, "a": 3}, {"t": 0.2, "a": 4}]
+    assert dedupe_time_rows(rows) == [{"t": 0.0, "a": 1}, {"t": 0.1, "a": 3}, {"t": 0.2, "a": 4}]
+
+
+def test_health_guards():
+    good = {"Epert": 1e-3, "Etot": 1.0, "divL2": 1e-12}
+    raise_on_unhealthy(good, t=0.1, tstep=10)
+    for bad, match in (({"Epert": math.nan}, "nonfinite"), ({"Etot": math.inf}, "nonfinite"),
+                       ({"Etot": 2e30}, "runaway"), ({"divL2": 2e-2}, "divergence"),
+                       ({"divL2": math.nan}, "divergence")):
+        with pytest.raises(FloatingPointError, match=match):
+            raise_on_unhealthy({**good, **bad}, t=0.1, tstep=10)
+
+
+def _state(rng, with_history=True):
+    def f(n):
+        return rng.standard_normal((n, 4, 6)) + 1j * rng.standard_normal((n, 4, 6))
+
+    u = (f(5), f(7), f(7))
+    if not with_history:
+        return KMMState(u=u, g=f(7))
+    rows = (f(5), f(7), rng.standard_normal(7) + 0j, rng.standard_normal(7) + 0j)
+    return KMMState(u=u, g=f(7), nonlinear_old=rows, nonlinear_older=tuple(r * 2 for r in rows),
+                    solution_old=(f(5), f(7), rng.standard_normal(7) + 0j, rng.standard_normal(7) + 0j),
+                    solution_older=(f(5), f(7), rng.standard_normal(7) + 0j, rng.standard_normal(7) + 0j),
+                    history_steps=2.0, have_old=1.0, previous_dt=1e-3)
+
+
+def _same(a: KMMState, b: KMMState):
+    for name in ("u", "nonlinear_old", "nonlinear_older", "solution_old", "solution_older"):
+        x, y = getattr(a, name), getattr(b, name)
+        assert (x is None) == (y is None), name
+        if x is not None:
+            assert len(x) == len(y) and all(np.array_equal(p, q) and p.dtype == q.dtype for p, q in zip(x, y))
+    assert np.array_equal(a.g, b.g)
+    assert (a.history_steps, a.have_old, a.previous_dt) == (b.history_steps, b.have_old, b.previous_dt)
+
+
+def test_payload_keys_follow_the_reference():
+    rng = np.random.default_rng(3)
+    full = checkpoint_payload(_state(rng))
+    assert set(full) == {"u", "g", "nonlinear_old", "have_old", "previous_dt", "nonlinear_older", "solution_old",
+                         "solution_older", "history_steps"}
+    assert full["have_old"].dtype == np.float32
+    bare = checkpoint_payload(_state(rng, with_history=False))
+    assert set(bare) == {"u", "g", "history_steps"}
+    with pytest.raises(NotImplementedError):
+        state_from_checkpoint_payload(full, "axisymmetric_tc_hydro")
+
+
+def test_checkpoint_container_round_trip(tmp_path):
+    rng = np.random.default_rng(5)
+    path = tmp_path / "checkpoints" / "checkpoints.npz"
+    path.parent.mkdir()
+    states = {}
+    for tstep in (12, 24, 36):
+        st = _state(rng, with_history=tstep != 24)
+        states[tstep] = st
+        write_checkpoint(path, {"state": checkpoint_payload(st), "extra": {"a": [np.arange(3), (1.5,)]}},
+                         t=tstep * 1e-3, tstep=tstep, attrs={"state_kind": "pcf_fluctuation_saturation"})
+    write_checkpoint(path, {"state": checkpoint_payload(states[24])}, t=0.024, tstep=24,
+                     attrs={"state_kind": "pcf_fluctuation_saturation"})  # a rewritten step replaces its group
+    rec = read_checkpoint(path)
+    assert rec.tstep == 24 and rec.t == 0.024  # the last write is the latest step
+    assert rec.attrs == {"t": 0.024, "tstep": 24, "state_kind": "pcf_fluctuation_saturation"}
+    assert set(rec.fields) == {"state"}
+    _same(state_from_checkpoint_payload(rec.fields["state"]), states[24])
+    for tstep in (12, 36):
+        rec = read_checkpoint(path, step=tstep)
+        _same(state_from_checkpoint_payload(rec.fields["state"]), states[tstep])
+        assert isinstance(rec.fields["extra"]["a"], list) and isinstance(rec.fields["extra"]["a"][1], tuple)
+        assert np.array_equal(rec.fields["extra"]["a"][0], np.arange(3))
+    with np.load(path, allow_pickle=False) as f:  # the reference's tree layout
+        assert "checkpoints/36/fields/state/u/0" in f.files
+        assert "checkpoints/36/fields/state/nonlinear_old/3" in f.files
+        assert json.loads(str(f["__meta__"]))["latest_step"] == 24
+
+
+def test_run_validates_cadence_before_building(tmp_path):
+    with pytest.raises(ValueError, match="out_dir is required"):
+        run_pcf_fluctuation_saturation(SPEC, steps=2, checkpoint_every=1)
+    with pytest.raises(ValueError, match="must be positive"):
+        run_pcf_fluctuation_saturation(SPEC, steps=2, out_dir=tmp_path, diagnostics_every=0)
