@@ -211,18 +211,22 @@ __global__ void __launch_bounds__(256) k_rotational(Geo g, NodalMats<T> nm, FFTP
 // complex line (as in k_rotational).  No LDS and no barrier: a wave holds
 // 64 / KZ pairs.
 // The value of lane (lane ^ H), H a power of two up to 32.  FP32: without LDS
-// traffic, by DPP quad permutes (H = 1, 2), row shifts (H = 4, 8: the lower
-// lane of a pair reads H lanes up, the upper H lanes down) and the gfx950 lane
-// swaps v_permlane16_swap / v_permlane32_swap (H = 16, 32).
+// traffic, by DPP quad permutes (H = 1, 2), row shifts (H = 4: the lower lane
+// of a pair reads 4 lanes up, the upper 4 lanes down), a rotation of the
+// 16-lane row by 8 (H = 8) and the gfx950 lane swaps v_permlane16_swap /
+// v_permlane32_swap (H = 16, 32).
 template <int H>
 __device__ __forceinline__ unsigned xchg_u32(unsigned v, int lane) {
   if constexpr (H == 1) {
-    return (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0xB1, 0xF, 0xF, false);  // quad_perm [1,0,3,2]
+    return (unsigned)__builtin_amdgcn_mov_dpp((int)v, 0xB1, 0xF, 0xF, true);  // quad_perm [1,0,3,2]
   } else if constexpr (H == 2) {
-    return (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x4E, 0xF, 0xF, false);  // quad_perm [2,3,0,1]
-  } else if constexpr (H == 4 || H == 8) {
-    const unsigned up = (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x100 + H, 0xF, 0xF, false);  // row_shl
-    const unsigned dn = (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x110 + H, 0xF, 0xF, false);  // row_shr
+    return (unsigned)__builtin_amdgcn_mov_dpp((int)v, 0x4E, 0xF, 0xF, true);  // quad_perm [2,3,0,1]
+  } else if constexpr (H == 8) {
+    return (unsigned)__builtin_amdgcn_mov_dpp((int)v, 0x128, 0xF, 0xF, true);  // row_ror:8 (half a row)
+  } else if constexpr (H == 4) {
+    // lanes shifted in from outside the row read 0 (bound_ctrl) and are not selected
+    const unsigned up = (unsigned)__builtin_amdgcn_mov_dpp((int)v, 0x100 + H, 0xF, 0xF, true);  // row_shl
+    const unsigned dn = (unsigned)__builtin_amdgcn_mov_dpp((int)v, 0x110 + H, 0xF, 0xF, true);  // row_shr
     return (lane & H) ? dn : up;
   } else if constexpr (H == 16) {
     const auto r = __builtin_amdgcn_permlane16_swap(v, v, false, false);
@@ -238,13 +242,14 @@ template <int H>
 __device__ __forceinline__ float xchg(float v, int lane) {
   return __builtin_bit_cast(float, xchg_u32<H>(__builtin_bit_cast(unsigned, v), lane));
 }
-// FP64 values go through ds_bpermute: in the step (between the streamwise
-// FFTs) the DPP form of the FP64 kernel ran 25.6 us against 22.6 us, while the
-// FP32 one gained (16.1 -> 13.4 us at 65x64x64).
+// FP64 values as two 32-bit halves through the same exchanges: with the
+// branch-free butterflies below this ran 18.5 us in the 65x64x64 step against
+// 20.3 us through ds_bpermute (the LDS round trips then bound the stages).
 template <int H>
 __device__ __forceinline__ double xchg(double v, int lane) {
-  (void)lane;
-  return __shfl_xor(v, H, 64);
+  const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+  const unsigned lo = xchg_u32<H>((unsigned)b, lane), hi = xchg_u32<H>((unsigned)(b >> 32), lane);
+  return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
 }
 template <int H, typename V>
 __device__ __forceinline__ V xchg_c(V v, int lane) {
@@ -255,17 +260,30 @@ __device__ __forceinline__ V xchg_c(V v, int lane) {
 // natural order in, bit-reversed out) runs H = KZ/2 .. 1 on the 6 x 3 values of
 // the lane; analysis (DIT, bit-reversed in, natural out) runs H = 1 .. KZ/2 on
 // 3 x 3.  ws[log2 H] is the lane's twiddle exp(-2 pi i (l mod H) / (2 H)).
+// Branch-free butterflies: the lower lane of a pair (bit H clear) and the upper
+// one differ only in a sign s = +-1 and a twiddle that is 1 in the lower lane,
+// both fixed per stage, so every lane runs the same instructions and a stage
+// issues all its exchanges before the first result is needed.
+//   DIF: t = p + s v, v' = w t       (lower a + b, upper (a - b) w)
+//   DIT: t = w v exchanged as p, v' = p + s t   (lower a + w b, upper a - w b)
 template <typename V, int H, int NF>
 __device__ __forceinline__ void dif_stages(V (&v)[NF][3], const V *ws, int lane) {
   if constexpr (H >= 1) {
+    using T = decltype(v[0][0].x);
     const bool up = (lane & H) != 0;
-    const V w = cconj(ws[__builtin_ctz(H)]);
+    const T sg = up ? (T)-1 : (T)1;
+    const V w = up ? cconj(ws[__builtin_ctz(H)]) : V{(T)1, (T)0};
+    V p[NF][3];
+#pragma unroll
+    for (int f = 0; f < NF; ++f)
+#pragma unroll
+      for (int na = 0; na < 3; ++na) p[f][na] = xchg_c<H>(v[f][na], lane);
 #pragma unroll
     for (int f = 0; f < NF; ++f)
 #pragma unroll
       for (int na = 0; na < 3; ++na) {
-        const V p = xchg_c<H>(v[f][na], lane);
-        v[f][na] = up ? cmul(csub(p, v[f][na]), w) : cadd(v[f][na], p);
+        const V t{sg * v[f][na].x + p[f][na].x, sg * v[f][na].y + p[f][na].y};
+        v[f][na] = cmul(t, w);
       }
     dif_stages<V, H / 2, NF>(v, ws, lane);
   }
@@ -273,16 +291,23 @@ __device__ __forceinline__ void dif_stages(V (&v)[NF][3], const V *ws, int lane)
 template <typename V, int H, int KZ, int NF>
 __device__ __forceinline__ void dit_stages(V (&v)[NF][3], const V *ws, int lane) {
   if constexpr (H < KZ) {
+    using T = decltype(v[0][0].x);
     const bool up = (lane & H) != 0;
-    const V w = ws[__builtin_ctz(H)];
+    const T sg = up ? (T)-1 : (T)1;
+    const V w = up ? ws[__builtin_ctz(H)] : V{(T)1, (T)0};
+    V p[NF][3];
 #pragma unroll
     for (int f = 0; f < NF; ++f)
 #pragma unroll
       for (int na = 0; na < 3; ++na) {
-        const V p = xchg_c<H>(v[f][na], lane);
-        const V t = cmul(w, up ? v[f][na] : p);
-        v[f][na] = up ? csub(p, t) : cadd(v[f][na], t);
+        v[f][na] = cmul(w, v[f][na]);
+        p[f][na] = xchg_c<H>(v[f][na], lane);
       }
+#pragma unroll
+    for (int f = 0; f < NF; ++f)
+#pragma unroll
+      for (int na = 0; na < 3; ++na)
+        v[f][na] = V{sg * v[f][na].x + p[f][na].x, sg * v[f][na].y + p[f][na].y};
     dit_stages<V, 2 * H, KZ, NF>(v, ws, lane);
   }
 }
