@@ -180,6 +180,15 @@ int32_t pcf_synchronize(pcf_handle *h);
  * whole KMMState pytree; history scalars travel through pcf_set_history. */
 int32_t pcf_set_field(pcf_handle *h, int32_t field, const double *host);
 int32_t pcf_get_field(pcf_handle *h, int32_t field, double *host);
+/* The same exchange with device pointers (memory on the handle's device, same
+ * layout: complex fields (rows, ny, nz) complex128, vectors rows float64),
+ * ordered on the handle's stream: the caller orders its own producers and
+ * consumers against it (pcf_set_stream / pcf_synchronize).  The reference's
+ * state leaves live in device memory (jax arrays in KMMState,
+ * examples/channelflow_kmm.py:50-101); these keep a solve loop free of host
+ * copies. */
+int32_t pcf_set_field_device(pcf_handle *h, int32_t field, const double *dev);
+int32_t pcf_get_field_device(pcf_handle *h, int32_t field, double *dev);
 int32_t pcf_set_history(pcf_handle *h, double history_steps, double have_old,
                         double previous_dt);
 int32_t pcf_get_history(pcf_handle *h, double *history_steps, double *have_old,

@@ -52,7 +52,8 @@ MATRIX_ID = {
 EXPORTS = (
     "pcf_abi_version", "pcf_last_error", "pcf_field_size", "pcf_setup_matrix",
     "pcf_create", "pcf_destroy", "pcf_set_stream", "pcf_synchronize",
-    "pcf_set_field", "pcf_get_field", "pcf_set_history", "pcf_get_history",
+    "pcf_set_field", "pcf_get_field", "pcf_set_field_device", "pcf_get_field_device", "pcf_set_history",
+    "pcf_get_history",
     "pcf_step", "pcf_set_dt", "pcf_step_steady", "pcf_step_steady_graph", "pcf_diagnostics", "pcf_mhd_diagnostics", "pcf_field_maxima",
     "pcf_field_maxima_noalloc", "pcf_velocity_energies", "pcf_physical_fields",
     "pcf_kernels_per_step", "pcf_stage_count", "pcf_stage_name", "pcf_profile_steady", "pcf_profile_stage",
@@ -120,6 +121,8 @@ def load() -> C.CDLL:
         "pcf_synchronize": ([h], C.c_int32),
         "pcf_set_field": ([h, C.c_int32, dp], C.c_int32),
         "pcf_get_field": ([h, C.c_int32, dp], C.c_int32),
+        "pcf_set_field_device": ([h, C.c_int32, C.c_void_p], C.c_int32),
+        "pcf_get_field_device": ([h, C.c_int32, C.c_void_p], C.c_int32),
         "pcf_set_history": ([h, C.c_double, C.c_double, C.c_double], C.c_int32),
         "pcf_get_history": ([h, dp, dp, dp], C.c_int32),
         "pcf_step": ([h, C.c_int32], C.c_int32),

@@ -1806,6 +1806,53 @@ int32_t pcf_get_field(pcf_handle *h, int32_t field, double *host) {
   return PCF_OK;
 }
 
+// Device-memory state exchange in the boundary layout, ordered on the
+// handle's stream (no host staging, no synchronisation).
+int32_t pcf_set_field_device(pcf_handle *h, int32_t field, const double *dev) {
+  if (!h || !dev) return fail(PCF_E_INVALID, "null argument");
+  FieldRef fr;
+  if (!field_ref(h, field, fr)) return fail(PCF_E_INVALID, "unknown field");
+  const pcf::Geo &g = h->geo;
+  PCF_HIP(hipSetDevice(h->device));
+  if (!fr.complex_) {
+    PCF_HIP(hipMemcpyAsync(fr.ptr, dev, (size_t)fr.rows * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+    return PCF_OK;
+  }
+  PCF_HIP(hipMemcpy2DAsync(fr.ptr, (size_t)g.kzs * sizeof(double2), dev, (size_t)g.nz * sizeof(double2),
+                           (size_t)g.kzs * sizeof(double2), (size_t)fr.rows * g.ny, hipMemcpyDeviceToDevice,
+                           h->stream));
+  if (field == PCF_F_U1 || field == PCF_F_U2) {
+    // the mean-mode vector of the current level: Re u[:, 0, 0], one double per plane
+    double *dst = (field == PCF_F_U1) ? h->v[h->so[0]] : h->w[h->so[0]];
+    PCF_HIP(hipMemcpy2DAsync(dst, sizeof(double), dev, (size_t)g.ny * g.nz * sizeof(double2), sizeof(double),
+                             (size_t)g.nd, hipMemcpyDeviceToDevice, h->stream));
+    h->u12_given = true;
+    h->u12_valid = true;
+    h->u12_checked = false;
+  } else if (field == PCF_F_U0 || field == PCF_F_G) {
+    if (!h->u12_given) h->u12_valid = false;
+    h->u12_checked = false;
+  }
+  return PCF_OK;
+}
+
+int32_t pcf_get_field_device(pcf_handle *h, int32_t field, double *dev) {
+  if (!h || !dev) return fail(PCF_E_INVALID, "null argument");
+  FieldRef fr;
+  if (!field_ref(h, field, fr)) return fail(PCF_E_INVALID, "unknown field");
+  const pcf::Geo &g = h->geo;
+  PCF_HIP(hipSetDevice(h->device));
+  if (field == PCF_F_U1 || field == PCF_F_U2) PCF_HIP(ensure_u12(h));
+  if (!fr.complex_) {
+    PCF_HIP(hipMemcpyAsync(dev, fr.ptr, (size_t)fr.rows * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+    return PCF_OK;
+  }
+  hipError_t e = pcf::launch_expand_half(reinterpret_cast<const double2 *>(fr.ptr), reinterpret_cast<double2 *>(dev),
+                                         fr.rows, g.ny, g.nz, g.kzs, h->stream);
+  if (e != hipSuccess) return fail(PCF_E_HIP, std::string("expand launch: ") + hipGetErrorString(e));
+  return PCF_OK;
+}
+
 int32_t pcf_set_history(pcf_handle *h, double history_steps, double have_old, double previous_dt) {
   if (!h) return fail(PCF_E_INVALID, "null handle");
   h->history_steps = history_steps;

@@ -785,6 +785,30 @@ __global__ void __launch_bounds__(256) k_reduce_blockmax(const double *__restric
   }
 }
 
+// -------------------------------------------------------- k_expand_half ----
+// The Hermitian half spectrum [rows][ny][kzs] (kzs = nz / 2) in the boundary's
+// full layout [rows][ny][nz]: c(ky, kz) for kz < kzs, c(ky, -kz) =
+// conj(c(-ky, kz)) above the spanwise Nyquist plane, which is zero.  One
+// element per thread, grid-stride.
+__global__ void __launch_bounds__(256) k_expand_half(const double2 *__restrict__ half, double2 *__restrict__ full,
+                                                     int rows, int ny, int nz, int kzs) {
+  const size_t n = (size_t)rows * ny * nz;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    const size_t ry = i / nz;
+    const int kz = (int)(i - ry * nz);
+    const size_t r = ry / ny;
+    const int iy = (int)(ry - r * ny);
+    double2 v{0.0, 0.0};
+    if (kz < kzs) {
+      v = half[ry * kzs + kz];
+    } else if (kz > kzs) {
+      const double2 m = half[(r * ny + (ny - iy) % ny) * kzs + (nz - kz)];
+      v = double2{m.x, -m.y};
+    }
+    full[i] = v;
+  }
+}
+
 // -------------------------------------------------------- k_bits_differ ----
 // diff |= 1 when a and b differ in any bit (n complex values).
 __global__ void __launch_bounds__(256) k_bits_differ(const double2 *__restrict__ a, const double2 *__restrict__ b,
@@ -1193,6 +1217,14 @@ hipError_t launch_solve_bordered(const Geo &g, const BorderedArgs &a, hipStream_
 hipError_t launch_reduce_blockmax(const double *bm, int nblocks, int nf, double *out, hipStream_t s) {
   if (nblocks < 1 || nf < 1) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k_reduce_blockmax, dim3(1), dim3(256), 0, s, bm, nblocks, nf, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_expand_half(const double2 *half, double2 *full, int rows, int ny, int nz, int kzs, hipStream_t s) {
+  const size_t n = (size_t)rows * ny * nz;
+  if (n == 0) return hipSuccess;
+  const size_t blocks = std::min<size_t>(8192, (n + 255) / 256);
+  hipLaunchKernelGGL(k_expand_half, dim3((unsigned)blocks), dim3(256), 0, s, half, full, rows, ny, nz, kzs);
   return hipGetLastError();
 }
 

@@ -188,6 +188,9 @@ hipError_t launch_solve_factored(const Geo &g, const StructMats &sm, const Facto
 // out[2] = sum_x wF Re u1_00(x) x, out[3] = sum_x wF Re du1_00/dx(x).
 // out[f] = max_b blockmax[b][f] over nblocks rows of nf maxima (one block).
 hipError_t launch_reduce_blockmax(const double *bm, int nblocks, int nf, double *out, hipStream_t s);
+// The Hermitian half spectrum [rows][ny][kzs] written out in the full
+// [rows][ny][nz] layout (conjugate mirror above the zero Nyquist plane).
+hipError_t launch_expand_half(const double2 *half, double2 *full, int rows, int ny, int nz, int kzs, hipStream_t s);
 // diff |= 1 when the n complex values at a and b differ in any bit.
 hipError_t launch_bits_differ(const double2 *a, const double2 *b, size_t n, unsigned *diff, hipStream_t s);
 hipError_t launch_diag(const Geo &g, const double *VB, const double *VD, const double *VB1,

@@ -140,19 +140,21 @@ class PlaneCouetteMHDHIP(PlaneCouetteFluctuationHIP):
 
     def load(self, state: MHDState) -> None:
         state = self._device_mhd_history(state)
+        self._begin_load(state)
         super().load(state.flow)
         for name, tag in _A_GROUPS:
             for comp, arr in zip("XYZ", getattr(state, name)):
                 self._set(f"A{comp}" if not tag else f"{tag}{comp}", arr)
+        self._end_load()
         self._loaded = None
 
-    def _result(self, state: MHDState) -> MHDState:
+    def _result(self, state: MHDState, device: bool = False) -> MHDState:
         """The device state after steps from ``state``: SBDF3 with its own
         rings; IMEXRK222 with every history field of ``state`` passed through
         (``_step_imex_mhd`` returns ``replace(state, ...)``,
         ``examples/pcf_mhd_jax.py:482-486``); CNAB2 with the step's nonlinear
         rows of flow and A as the only levels (``:383-387``)."""
-        st = self.fetch()
+        st = self.fetch_device() if device else self.fetch()
         if self.time_integrator == "IMEXRK222":
             st = replace(state, flow=replace(state.flow, u=st.flow.u, g=st.flow.g), A=st.A)
         elif self.time_integrator == "CNAB2":

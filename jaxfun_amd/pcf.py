@@ -27,6 +27,45 @@ from . import _native as nat
 _PRECISIONS = {"fp64": nat.PREC_FP64, "float64": nat.PREC_FP64, "mixed": nat.PREC_MIXED}
 
 
+def _is_torch(x) -> bool:
+    return type(x).__module__.split(".")[0] == "torch"
+
+
+def _real(x):
+    if _is_torch(x):
+        return x.real if x.is_complex() else x
+    return np.real(x)
+
+
+def _leaves(state) -> list:
+    """Array leaves of a (possibly nested) state dataclass."""
+    out = []
+    for v in vars(state).values():
+        if isinstance(v, (tuple, list)):
+            out.extend(v)
+        elif hasattr(v, "__dataclass_fields__"):
+            out.extend(_leaves(v))
+        elif v is not None and not isinstance(v, (int, float)):
+            out.append(v)
+    return out
+
+
+def to_host(state):
+    """A copy of a state (KMMState or MHDState) whose torch tensor leaves are
+    numpy arrays on the host; numpy leaves are kept as they are."""
+
+    def conv(v):
+        if _is_torch(v):
+            return v.detach().cpu().numpy()
+        if isinstance(v, tuple):
+            return tuple(conv(x) for x in v)
+        if hasattr(v, "__dataclass_fields__"):
+            return to_host(v)
+        return v
+
+    return replace(state, **{k: conv(v) for k, v in vars(state).items()})
+
+
 @dataclass(frozen=True)
 class KMMState:
     """Coefficient-space KMM state (reference ``channelflow_kmm.py:50-101``).
@@ -144,6 +183,8 @@ class PlaneCouetteFluctuationHIP:
         nat.check(self._lib.pcf_create(C.byref(cfg), C.byref(handle)))
         self._h = handle
         self._loaded = None  # the state object whose values are resident on the device
+        self._pending = []  # device leaves whose copies the library stream has not finished
+        self._fetch_on_device = False
 
     def _configure(self, cfg) -> None:
         """Hook for subclasses to fill the rest of the library configuration."""
@@ -190,8 +231,8 @@ class PlaneCouetteFluctuationHIP:
     # ------------------------------------------------------------ states
     def _with_zero_history(self, u, g) -> KMMState:
         """``_reset_flow_history`` (``channelflow_kmm.py:637-652``)."""
-        zu = np.zeros_like(u[0])
-        zg = np.zeros_like(g)
+        zu = np.zeros(tuple(u[0].shape), complex)
+        zg = np.zeros(tuple(g.shape), complex)
         zv = np.zeros(self.nd)
         z = (zu, zg, zv, zv)
         return KMMState(u=tuple(u), g=g, nonlinear_old=z, nonlinear_older=z, solution_old=z,
@@ -256,6 +297,8 @@ class PlaneCouetteFluctuationHIP:
 
     # ------------------------------------------------------------ device I/O
     def _set(self, name, arr):
+        if _is_torch(arr):
+            return self._set_device(name, arr)
         arr = np.ascontiguousarray(arr)
         if np.iscomplexobj(arr):
             arr = arr.astype(np.complex128, copy=False)
@@ -264,7 +307,31 @@ class PlaneCouetteFluctuationHIP:
         nat.check(self._lib.pcf_set_field(self._h, nat.FIELD_ID[name],
                                           arr.ctypes.data_as(C.POINTER(C.c_double))))
 
+    def _torch_device(self):
+        import torch
+
+        return torch.device("cuda", int(self._cfg.device))
+
+    def _set_device(self, name, t):
+        """A torch tensor leaf straight from device memory (pcf_set_field_device):
+        cast to complex128 / float64 on this solver's GPU; kept alive until the
+        library stream has read it."""
+        import torch
+
+        t = t.to(device=self._torch_device(), dtype=torch.complex128 if t.is_complex() else torch.float64)
+        t = t.contiguous()
+        self._pending.append(t)
+        nat.check(self._lib.pcf_set_field_device(self._h, nat.FIELD_ID[name], C.c_void_p(t.data_ptr())))
+
     def _get(self, name, rows, vector=False):
+        if self._fetch_on_device:
+            import torch
+
+            shape = (rows,) if vector else (rows, self.N[1], self.N[2])
+            out = torch.empty(shape, dtype=torch.float64 if vector else torch.complex128,
+                              device=self._torch_device())
+            nat.check(self._lib.pcf_get_field_device(self._h, nat.FIELD_ID[name], C.c_void_p(out.data_ptr())))
+            return out
         if vector:
             out = np.empty(rows, np.float64)
         else:
@@ -290,8 +357,11 @@ class PlaneCouetteFluctuationHIP:
         return replace(state, **{k: getattr(z, k) for k in groups if getattr(state, k) is None})
 
     def load(self, state: KMMState) -> None:
-        """Copy a state (with its history) to the device."""
+        """Copy a state (with its history) to the device.  Leaves may be numpy
+        arrays (host copies) or torch tensors (device-to-device copies on the
+        library stream, after the producing torch stream has finished)."""
         state = self._device_history(state)
+        self._begin_load(state)
         self._set("U0", state.u[0])
         self._set("U1", state.u[1])
         self._set("U2", state.u[2])
@@ -299,10 +369,22 @@ class PlaneCouetteFluctuationHIP:
         for tag, grp in (("NL_OLD", state.nonlinear_old), ("NL_OLDER", state.nonlinear_older),
                          ("SOL_OLD", state.solution_old), ("SOL_OLDER", state.solution_older)):
             for comp, arr in zip("UGVW", grp):
-                self._set(f"{tag}_{comp}", np.real(arr) if comp in "VW" else arr)
+                self._set(f"{tag}_{comp}", _real(arr) if comp in "VW" else arr)
         nat.check(self._lib.pcf_set_history(self._h, float(state.history_steps),
                                             float(state.have_old), float(state.previous_dt)))
+        self._end_load()
         self._loaded = None
+
+    def _begin_load(self, state) -> None:
+        if any(_is_torch(leaf) for leaf in _leaves(state)):
+            import torch
+
+            torch.cuda.current_stream(self._torch_device()).synchronize()
+
+    def _end_load(self) -> None:
+        if self._pending:
+            nat.check(self._lib.pcf_synchronize(self._h))
+            self._pending.clear()
 
     def fetch(self) -> KMMState:
         """Copy the device state back (full-complex reference layout)."""
@@ -319,6 +401,18 @@ class PlaneCouetteFluctuationHIP:
                       solution_old=groups[2], solution_older=groups[3],
                       history_steps=hs.value, have_old=ho.value, previous_dt=pd.value)
         self._loaded = st
+        return st
+
+    def fetch_device(self):
+        """The device state as a state whose leaves are torch CUDA tensors
+        (complex128 fields, float64 mean-mode vectors) copied on the device
+        (pcf_get_field_device); ready on every torch stream on return."""
+        self._fetch_on_device = True
+        try:
+            st = self.fetch()
+        finally:
+            self._fetch_on_device = False
+        nat.check(self._lib.pcf_synchronize(self._h))
         return st
 
     def _ensure_loaded(self, state: KMMState) -> None:
@@ -385,7 +479,15 @@ class PlaneCouetteFluctuationHIP:
         self.advance(int(steps))
         return self._result(state)
 
-    def _result(self, state: KMMState) -> KMMState:
+    def solve_device(self, state: KMMState, steps: int) -> KMMState:
+        """``solve`` with the result's leaves as torch CUDA tensors: the state
+        stays in device memory (the reference's jax arrays do too), and
+        passing the result back in reuses what is resident."""
+        self._ensure_loaded(state)
+        self.advance(int(steps))
+        return self._result(state, device=True)
+
+    def _result(self, state: KMMState, device: bool = False) -> KMMState:
         """The device state after steps from ``state``, with the history fields
         each integrator leaves: SBDF3 its own rings; IMEXRK3 those of ``state``
         unchanged (``replace``, ``channelflow_kmm.py:935``); IMEXRK222 only
@@ -393,7 +495,7 @@ class PlaneCouetteFluctuationHIP:
         (``:1194-1200``), the device's history slots being scratch there;
         CNAB2 the step's nonlinear rows as ``nonlinear_old`` with ``have_old``
         and ``previous_dt`` (``:1112-1117``)."""
-        st = self.fetch()
+        st = self.fetch_device() if device else self.fetch()
         if self.time_integrator == "IMEXRK3":
             st = replace(state, u=st.u, g=st.g)
         elif self.time_integrator == "IMEXRK222":

@@ -33,7 +33,7 @@ from typing import Any
 
 import numpy as np
 
-from .pcf import KMMState, PlaneCouetteFluctuationHIP
+from .pcf import KMMState, PlaneCouetteFluctuationHIP, to_host
 
 KMM_INTEGRATORS = frozenset({"IMEXRK222", "IMEXRK3", "SBDF3", "CNAB2"})
 
@@ -349,8 +349,10 @@ def run_pcf_fluctuation_saturation(spec: dict, *, steps: int | None = None, out_
                                    resume_checkpoint: CheckpointRecord | None = None, on_row=None,
                                    precision: str = "fp64", solver: PlaneCouetteFluctuationHIP | None = None) -> dict:
     """``_run_pcf_fluctuation_saturation`` (``production/oracles.py:1442-1573``)
-    on the HIP solver.  Steps run in blocks between cadence points and stay on
-    the device; each block ends with the health guards.  With ``out_dir`` and
+    on the HIP solver.  Steps run in blocks between cadence points with the
+    state in device memory (torch tensor leaves; host copies only for
+    checkpoints and the returned final state); each block ends with the health
+    guards.  With ``out_dir`` and
     ``checkpoint_every`` the state payload goes to
     ``out_dir/checkpoints/checkpoints.npz`` at that cadence and at the end
     (attr ``state_kind``, ``:3167-3192``, ``:3321-3326``), and
@@ -388,7 +390,7 @@ def run_pcf_fluctuation_saturation(spec: dict, *, steps: int | None = None, out_
         ckpt.parent.mkdir(parents=True, exist_ok=True)
 
     def save(t, tstep, st):
-        write_checkpoint(ckpt, {"state": checkpoint_payload(st)}, t=t, tstep=tstep,
+        write_checkpoint(ckpt, {"state": checkpoint_payload(to_host(st))}, t=t, tstep=tstep,
                          attrs={"state_kind": state_kind})
 
     done = 0
@@ -396,7 +398,7 @@ def run_pcf_fluctuation_saturation(spec: dict, *, steps: int | None = None, out_
         block = n_steps - done
         for c in cadence:
             block = min(block, c - ((tstep0 + done) % c))
-        state = solver.solve(state, block)
+        state = solver.solve_device(state, block)  # the state stays in device memory between blocks
         done += block
         tstep = tstep0 + done
         t = t0 + done * dt
@@ -432,4 +434,4 @@ def run_pcf_fluctuation_saturation(spec: dict, *, steps: int | None = None, out_
             "streak_rms": final["streak_rms"], "roll_rms": final["roll_rms"]}
     series = dedupe_time_rows([first, *rows, last])
     scalars.update(stationarity_scalars(series, key="kinetic_energy"))
-    return {"scalars": scalars, "time_series": series, "final_state": state}
+    return {"scalars": scalars, "time_series": series, "final_state": to_host(state)}
