@@ -154,7 +154,7 @@ def block_protocol(solver, block_steps: int = 25, timed_blocks: int = 10) -> dic
 
 # stage name -> kernel-name fragment (FP64 instance) in the rocprofv3 traffic summary
 _KERNEL_OF = {"nodal_backward": "k_nodal_prim_mfma<double>", "fft_y_inverse": "k_fft_y<double, true",
-              "rotational": "k_rotational<double", "fft_y_forward": "k_fft_y<double, false",
+              "rotational": "k_rot_wave<double", "fft_y_forward": "k_fft_y<double, false",
               "forward_rhs": "k_rhs_mfma<double>", "solve": "k_solve_factored"}
 TRAFFIC_FILE = ROOT / "profiles" / "traffic_65x64x64.json"
 
@@ -218,9 +218,11 @@ def roofline(solver, precision: str, stages: dict) -> dict:
 
 
 def copy_calibration(mb: int) -> dict:
-    """One device-to-device copy of known size through the HIP runtime (its blit
-    copy kernel): that kernel's FETCH_SIZE / WRITE_SIZE counters give the scale
-    that turns the step kernels' counters into HBM bytes."""
+    """Kernels of known HBM traffic for the FETCH_SIZE / WRITE_SIZE scale: a
+    device-to-device copy of ``mb`` MiB through the HIP runtime (its blit copy
+    kernel), then, where torch is importable, a fill of mb / 2 MiB (write only,
+    16-byte stores) and elementwise products over mb / 2 MiB in float32 and in
+    float16: the scale is checked across four kernels."""
     hip = C.CDLL("libamdhip64.so")
     n = mb * (1 << 20)
     src, dst = C.c_void_p(), C.c_void_p()
@@ -233,7 +235,28 @@ def copy_calibration(mb: int) -> dict:
     hip.hipDeviceSynchronize()
     hip.hipFree(src)
     hip.hipFree(dst)
-    return {"bytes_read": n, "bytes_written": n}
+    out = {"copy": {"bytes_read": n, "bytes_written": n}}
+    try:
+        import torch
+    except ImportError:
+        return out
+    half = n // 2
+    x = torch.full((half // 4,), 1.5, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    torch.mul(x, 2.0, out=y)
+    torch.cuda.synchronize()
+    out["fill"] = {"bytes_read": 0, "bytes_written": half}
+    out["elementwise"] = {"bytes_read": half, "bytes_written": half}
+    del x, y
+    # the same product in float16 (a third kernel; torch vectorises it 8 wide, so
+    # also 16 bytes per lane)
+    x = torch.full((half // 2,), 1.5, dtype=torch.float16, device="cuda")
+    y = torch.empty_like(x)
+    torch.mul(x, 2.0, out=y)
+    torch.cuda.synchronize()
+    out["elementwise_8b"] = {"bytes_read": half, "bytes_written": half}
+    del x, y
+    return out
 
 
 def _cpu_blocks(state, threads: int, budget_s: float) -> dict:
@@ -321,6 +344,55 @@ def pcie_inclusive(state, steps: int) -> dict:
     s.close()
     return {"value": steps / el, "unit": "steps/s", "steps": steps,
             "note": "host state in, host state out, startup steps included"}
+
+
+def state_exchange(state, reps: int = 5) -> dict:
+    """One full state exchange (load then fetch of every leaf, history
+    included) through host memory (pcf_set_field / pcf_get_field, PCIe) and
+    through device memory (pcf_set_field_device / pcf_get_field_device, torch
+    tensor leaves), mean over ``reps``, and the state's size."""
+    from jaxfun_amd import PlaneCouetteFluctuationHIP
+
+    w = WORKLOAD
+    s = PlaneCouetteFluctuationHIP(N=w["N"], Re=w["Re"], dt=w["dt"], precision="fp64", **SBDF3_ROT)
+    s.load(state)
+    host = s.fetch()
+    nbytes = sum(int(a.nbytes) for a in list(host.u) + [host.g] + [x for grp in (
+        host.nonlinear_old, host.nonlinear_older, host.solution_old, host.solution_older) for x in grp])
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        s.load(host)
+        host = s.fetch()
+    el_host = (time.perf_counter() - t0) / reps
+    dev = s.fetch_device()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        s.load(dev)
+        dev = s.fetch_device()
+    el_dev = (time.perf_counter() - t0) / reps
+    s.close()
+    return {"state_bytes": nbytes, "host_ms": 1e3 * el_host, "device_ms": 1e3 * el_dev,
+            "note": "load + fetch of the whole state (20 leaves); the device path keeps torch CUDA tensors"}
+
+
+def stage_rooflines(solver, precision: str, stages: dict) -> dict:
+    """Every stage kernel against its bound: algorithmic bytes over the
+    stage's event time inside the step, the measured HBM traffic per launch
+    from the committed counter summary, and for the two matrix-core
+    contractions the model flops over the same time against the matrix peak."""
+    model = stage_model(solver, precision)
+    peak_f = FP64_PEAK_TFLOPS if precision == "fp64" else FP32_PEAK_TFLOPS
+    out = {}
+    for name, ms in stages.items():
+        nbytes, flops = model[name]
+        sec = ms * 1e-3
+        rec = {"algorithmic_bytes": nbytes, "ms_in_step": ms, "achieved_gbs": nbytes / sec / 1e9,
+               "frac_hbm": nbytes / sec / 1e9 / HBM_PEAK_GBS, "traffic": measured_traffic(name, precision)}
+        if name in ("nodal_backward", "forward_rhs"):
+            rec["mfma_tflops"] = flops / sec / 1e12
+            rec["frac_mfma"] = flops / sec / 1e12 / peak_f
+        out[name] = rec
+    return out
 
 
 # Mixed precision at the large sizes: 97x96x96 and 129x128x128, Re = 2000,
@@ -613,6 +685,7 @@ def main(argv=None) -> int:
                    "padding": [1.0, 1.5, 1.5], "graph_block_steps": GRAPH_BLOCK},
         "stages_ms": stages,
         "roofline": roofline(solver, "fp64", stages),
+        "stage_rooflines": stage_rooflines(solver, "fp64", stages),
         "protocol": block_protocol(solver),
         "final_diagnostics": diag,
         "adjoint_forward_ratio": None,
@@ -625,6 +698,7 @@ def main(argv=None) -> int:
         line["mixed"] = {"value": mval, "unit": "steps/s", "ms_per_step": 1e3 * mel / steps,
                          "ratio_vs_fp64": mval / value, "stages_ms": mstages,
                          "roofline": roofline(msolver, "mixed", mstages),
+                         "stage_rooflines": stage_rooflines(msolver, "mixed", mstages),
                          "protocol": block_protocol(msolver)}
         line["mixed"]["protocol_ratio_vs_fp64"] = (line["mixed"]["protocol"]["steps_per_s_p50"]
                                                    / line["protocol"]["steps_per_s_p50"])
@@ -636,6 +710,7 @@ def main(argv=None) -> int:
                   f"{line['mixed']['error_vs_fp64']['velocity']:.3e}", file=sys.stderr, flush=True)
     if args.pcie_steps:
         line["pcie_inclusive"] = pcie_inclusive(state, args.pcie_steps)
+        line["state_exchange"] = state_exchange(state)
     if not args.no_adjoint:
         adj = {"workload": f"grad of Epert after {ADJOINT['steps']} steps wrt the seeded initial field, "
                            f"65x64x64, Re=1000, dt=0.005, E=1e-4, checkpoints every {ADJOINT['every']}",

@@ -105,6 +105,14 @@ def load() -> C.CDLL:
         raise PcfError(
             f"{LIB_PATH} is not built; run `python -m jaxfun_amd.build` (hipcc, gfx950)"
         )
+    # torch ships its own HIP and HSA runtime libraries under other file names.
+    # Loaded before this library, its HSA runtime is the one both HIP runtimes
+    # bind to, and device pointers pass between torch and the library; loaded
+    # after, it brings up a second HSA runtime that finds no GPU.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = C.CDLL(str(LIB_PATH))
     P = C.POINTER
     dp = P(C.c_double)

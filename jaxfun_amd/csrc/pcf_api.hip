@@ -1749,10 +1749,14 @@ int32_t pcf_set_field(pcf_handle *h, int32_t field, const double *host) {
     PCF_HIP(hipMemcpy(fr.ptr, host, (size_t)fr.rows * sizeof(double), hipMemcpyHostToDevice));
     return PCF_OK;
   }
-  // full (rows, ny, nz) -> half [rows][ny][kzs]: keep the first nz/2 spanwise modes
-  PCF_HIP(hipMemcpy2D(fr.ptr, (size_t)g.kzs * sizeof(double2), host,
-                      (size_t)g.nz * sizeof(double2), (size_t)g.kzs * sizeof(double2),
-                      (size_t)fr.rows * g.ny, hipMemcpyHostToDevice));
+  // full (rows, ny, nz) -> half [rows][ny][kzs]: keep the first nz/2 spanwise
+  // modes, packed on the host so that one contiguous copy crosses the bus
+  {
+    const size_t lines = (size_t)fr.rows * g.ny, w = 2 * (size_t)g.kzs;
+    std::vector<double> half(lines * w);
+    for (size_t r = 0; r < lines; ++r) std::copy(host + r * 2 * g.nz, host + r * 2 * g.nz + w, half.begin() + r * w);
+    PCF_HIP(hipMemcpy(fr.ptr, half.data(), half.size() * sizeof(double), hipMemcpyHostToDevice));
+  }
   if (field == PCF_F_U1 || field == PCF_F_U2) {
     // the mean-mode vector of the current level is Re u[:, 0, 0]
     std::vector<double> vec(g.nd);
