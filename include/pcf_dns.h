@@ -219,6 +219,17 @@ int32_t pcf_step_steady(pcf_handle *h, int32_t nsteps);
  * the state is unchanged), so a timed region can start with it cached. */
 int32_t pcf_step_steady_graph(pcf_handle *h, int32_t nsteps, int32_t replays);
 
+/* Counters of the handle's graph caches (the steady graphs above and those of
+ * the reverse pass's forward sweeps), the counterpart of the reference's
+ * rollout cache introspection (ScanRolloutCacheInfo,
+ * src/jaxfun/integrators/cnab2.py:23-33; KMM.rollout_cache_info,
+ * examples/channelflow_kmm.py:1255-1257).  out[0] generation (times the caches
+ * were dropped: pcf_set_stream), out[1] live graphs, out[2] lookups served
+ * from the cache, out[3] lookups that captured, out[4] graphs dropped, out[5..n)
+ * the distinct step counts of the live graphs, ascending, zero-padded.  n >= 5;
+ * returns the number of distinct step counts. */
+int32_t pcf_graph_cache_info(pcf_handle *h, int64_t *out, int32_t n);
+
 /* Diagnostics of the current state (synchronous).  out[8] =
  * {Epert, Etot, divL2, u_top, u_bot, wall_shear_top, wall_shear_bot,
  *  mean_shear}; replaces PlaneCouetteFluctuationJax.diagnostics

@@ -54,7 +54,7 @@ EXPORTS = (
     "pcf_create", "pcf_destroy", "pcf_set_stream", "pcf_synchronize",
     "pcf_set_field", "pcf_get_field", "pcf_set_field_device", "pcf_get_field_device", "pcf_set_history",
     "pcf_get_history",
-    "pcf_step", "pcf_set_dt", "pcf_step_steady", "pcf_step_steady_graph", "pcf_diagnostics", "pcf_mhd_diagnostics", "pcf_field_maxima",
+    "pcf_step", "pcf_set_dt", "pcf_step_steady", "pcf_step_steady_graph", "pcf_graph_cache_info", "pcf_diagnostics", "pcf_mhd_diagnostics", "pcf_field_maxima",
     "pcf_field_maxima_noalloc", "pcf_velocity_energies", "pcf_physical_fields",
     "pcf_kernels_per_step", "pcf_stage_count", "pcf_stage_name", "pcf_profile_steady", "pcf_profile_stage",
     "pcf_set_cotangent", "pcf_vjp", "pcf_get_gradient",
@@ -137,6 +137,7 @@ def load() -> C.CDLL:
         "pcf_set_dt": ([h, C.c_double], C.c_int32),
         "pcf_step_steady": ([h, C.c_int32], C.c_int32),
         "pcf_step_steady_graph": ([h, C.c_int32, C.c_int32], C.c_int32),
+        "pcf_graph_cache_info": ([h, P(C.c_int64), C.c_int32], C.c_int32),
         "pcf_diagnostics": ([h, dp], C.c_int32),
         "pcf_mhd_diagnostics": ([h, dp], C.c_int32),
         "pcf_field_maxima": ([h, dp], C.c_int32),

@@ -198,6 +198,9 @@ struct pcf_handle {
          *dxn = nullptr, *ddiag = nullptr;
   std::map<uint64_t, hipGraphExec_t> graphs;  // (ring phase, steps) -> steady-step graph
   std::map<uint64_t, hipGraphExec_t> seg_graphs;  // steady runs of the reverse pass's forward sweeps
+  // graph cache counters: lookups that found / captured a graph, graphs
+  // dropped, and the number of times the caches were dropped (stream change)
+  int64_t graph_hits = 0, graph_misses = 0, graph_evictions = 0, graph_generation = 0;
   StageTimer *timer = nullptr;  // non-null while pcf_profile_steady runs
 
   // Reverse mode, set up by the first pcf_vjp.
@@ -1095,6 +1098,9 @@ int32_t cached_steady(pcf_handle *h, std::map<uint64_t, hipGraphExec_t> &cache, 
     int32_t r = capture_steady(h, n, tape_base, &exec);
     if (r) return r;
     it = cache.emplace(key, exec).first;
+    ++h->graph_misses;
+  } else {
+    ++h->graph_hits;
   }
   *out = it->second;
   return PCF_OK;
@@ -1726,11 +1732,30 @@ int32_t pcf_set_stream(pcf_handle *h, void *stream) {
     PCF_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
     h->own_stream = true;
   }
+  h->graph_evictions += (int64_t)(h->graphs.size() + h->seg_graphs.size());
+  ++h->graph_generation;
   for (auto &kv : h->graphs) (void)hipGraphExecDestroy(kv.second);
   h->graphs.clear();
   for (auto &kv : h->seg_graphs) (void)hipGraphExecDestroy(kv.second);
   h->seg_graphs.clear();
   return PCF_OK;
+}
+
+int32_t pcf_graph_cache_info(pcf_handle *h, int64_t *out, int32_t n) {
+  if (!h || !out) return fail(PCF_E_INVALID, "null argument");
+  if (n < 5) return fail(PCF_E_INVALID, "the counters need at least 5 slots");
+  std::vector<int64_t> lengths;
+  for (const auto *c : {&h->graphs, &h->seg_graphs})
+    for (const auto &kv : *c) lengths.push_back((int64_t)((kv.first >> 16) & 0xFFFFFF));
+  std::sort(lengths.begin(), lengths.end());
+  lengths.erase(std::unique(lengths.begin(), lengths.end()), lengths.end());
+  out[0] = h->graph_generation;
+  out[1] = (int64_t)(h->graphs.size() + h->seg_graphs.size());
+  out[2] = h->graph_hits;
+  out[3] = h->graph_misses;
+  out[4] = h->graph_evictions;
+  for (int32_t i = 5; i < n; ++i) out[i] = (size_t)(i - 5) < lengths.size() ? lengths[i - 5] : 0;
+  return (int32_t)lengths.size();
 }
 
 int32_t pcf_synchronize(pcf_handle *h) {

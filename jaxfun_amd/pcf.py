@@ -67,6 +67,20 @@ def to_host(state):
 
 
 @dataclass(frozen=True)
+class RolloutCacheInfo:
+    """Graph-cache counters in the fields of the reference's
+    ``ScanRolloutCacheInfo`` (``src/jaxfun/integrators/cnab2.py:23-33``)."""
+
+    generation: int
+    max_entries: int
+    live_entries: int
+    step_counts: tuple
+    hits: int
+    misses: int
+    evictions: int
+
+
+@dataclass(frozen=True)
 class KMMState:
     """Coefficient-space KMM state (reference ``channelflow_kmm.py:50-101``).
 
@@ -454,6 +468,57 @@ class PlaneCouetteFluctuationHIP:
 
     def synchronize(self) -> None:
         nat.check(self._lib.pcf_synchronize(self._h))
+
+    def rollout_cache_info(self) -> RolloutCacheInfo:
+        """``KMM.rollout_cache_info`` (``channelflow_kmm.py:1255-1257``): the
+        counters of the handle's HIP-graph caches in the fields of the
+        reference's ``ScanRolloutCacheInfo`` (``src/jaxfun/integrators/cnab2.py:23-33``).
+        The caches keep every graph until the stream changes, so
+        ``max_entries`` is -1 (unbounded) and ``evictions`` counts the graphs
+        dropped then."""
+        out = (C.c_int64 * 69)()
+        n = nat.check(self._lib.pcf_graph_cache_info(self._h, out, 69))
+        return RolloutCacheInfo(generation=int(out[0]), max_entries=-1, live_entries=int(out[1]),
+                                step_counts=tuple(int(v) for v in out[5:5 + min(n, 64)]), hits=int(out[2]),
+                                misses=int(out[3]), evictions=int(out[4]))
+
+    def solve_with_cadence(self, state, steps: int, cadence, *, block_size: int = 1, on_diagnostics=None,
+                           on_snapshot=None, on_checkpoint=None, should_stop=None, t0: float = 0.0,
+                           tstep0: int = 0):
+        """``KMM.solve_with_cadence`` (``channelflow_kmm.py:1264-1291``) with the
+        loop of ``jaxfun.io.run_with_cadence`` (``src/jaxfun/io/__init__.py:117-186``):
+        blocks of at most ``block_size`` steps that end on every step a
+        cadence of ``cadence`` (``diagnostics_every``, ``snapshot_every``,
+        ``checkpoint_every``) falls due; after each block the due callbacks
+        in that order, then ``should_stop(t, tstep, state)``, which may end
+        the run early."""
+        if steps < 0:
+            raise ValueError("steps must be non-negative")
+        if block_size < 1:
+            raise ValueError("block_size must be positive")
+        every = {k: getattr(cadence, k, None) for k in ("diagnostics_every", "snapshot_every", "checkpoint_every")}
+        periods = [int(e) for e in every.values() if e is not None and e > 0]
+
+        def due(tstep, e):
+            return e is not None and e > 0 and tstep % e == 0
+
+        out = state
+        tstep = int(tstep0)
+        final = tstep + int(steps)
+        while tstep < final:
+            n = min([block_size, final - tstep] + [max(1, (tstep // e + 1) * e - tstep) for e in periods])
+            out = self.solve(out, n)
+            tstep += n
+            t = float(t0) + (tstep - int(tstep0)) * float(self.dt)
+            if due(tstep, every["diagnostics_every"]) and on_diagnostics:
+                on_diagnostics(t, tstep, self.diagnostics(out))
+            if due(tstep, every["snapshot_every"]) and on_snapshot:
+                on_snapshot(t, tstep, out)
+            if due(tstep, every["checkpoint_every"]) and on_checkpoint:
+                on_checkpoint(t, tstep, out)
+            if should_stop is not None and bool(should_stop(t, tstep, out)):
+                break
+        return out
 
     # ------------------------------------------------------------ reference API
     def step(self, state: KMMState, dt=None, factor_args=None) -> KMMState:

@@ -203,3 +203,41 @@ def test_cnab2_checkpoint_preserves_variable_step_history_exactly():
     actual = make((17, 16, 16), time_integrator="CNAB2", dt=7.0e-4).solve(restored, 2)
     assert actual.have_old == 1.0 and actual.previous_dt == 7.0e-4
     assert_bitwise(actual, expected)
+
+
+def test_solve_with_cadence_matches_solve_and_calls_back():
+    """``solve_with_cadence`` (the loop the reference runner drives): the same
+    state as one ``solve``, callbacks at every due step in the reference's
+    order, and ``should_stop`` ending the run after the block it returns true."""
+    import types
+
+    s0 = make((17, 16, 16)).initial_state()
+    cad = types.SimpleNamespace(diagnostics_every=5, snapshot_every=None, checkpoint_every=12)
+    seen = []
+    out = make((17, 16, 16)).solve_with_cadence(
+        s0, 30, cad, block_size=8, on_diagnostics=lambda t, k, d: seen.append(("d", k, t, d["Epert"])),
+        on_checkpoint=lambda t, k, st: seen.append(("c", k, t)), should_stop=lambda t, k, st: False)
+    assert_bitwise(out, make((17, 16, 16)).solve(s0, 30))
+    assert [r[1] for r in seen if r[0] == "d"] == [5, 10, 15, 20, 25, 30]
+    assert [r[1] for r in seen if r[0] == "c"] == [12, 24]
+    assert all(math.isclose(r[2], r[1] * 1e-3, rel_tol=1e-12) for r in seen)
+    stops = []
+    early = make((17, 16, 16)).solve_with_cadence(s0, 30, cad, block_size=8,
+                                                  should_stop=lambda t, k, st: stops.append(k) or k >= 10)
+    assert stops == [5, 10]
+    assert_bitwise(early, make((17, 16, 16)).solve(s0, 10))
+
+
+def test_rollout_cache_info_counts_graph_lookups():
+    s = make((17, 16, 16))
+    s.load(s.initial_state())
+    s.advance(2)
+    info = s.rollout_cache_info()
+    assert (info.live_entries, info.hits, info.misses, info.max_entries) == (0, 0, 0, -1)
+    s.advance_steady(24, graph_block=12)  # one lookup per replay: captured, then served
+    s.advance_steady(12, graph_block=12)  # same ring phase: served from the cache
+    info = s.rollout_cache_info()
+    assert (info.misses, info.hits, info.live_entries, info.step_counts) == (1, 2, 1, (12,))
+    s.advance_steady(5, graph_block=12)
+    info = s.rollout_cache_info()
+    assert info.misses == 2 and info.step_counts == (5, 12) and info.generation == 0
