@@ -245,6 +245,31 @@ int32_t pcf_diagnostics(pcf_handle *h, double *out8);
  * and the corresponding entries of examples/pcf_mhd_mri_shearpy_jax.py:431-513,802-836. */
 int32_t pcf_mhd_diagnostics(pcf_handle *h, double *out4);
 
+/* The rotating-shear family's cadence diagnostics (synchronous), on the
+ * unpadded grid of the current state, b = curl A the fluctuation field and
+ * b + B0 the total field, u_tot = u + (0, u_wall x, 0), V = 2 Ly Lz.
+ * out[PCF_XDIAG_COUNT] =
+ *   0 reynolds_stress  <u_x u_y>          1 maxwell_stress  -<(b+B0)_x (b+B0)_y>
+ *   2 Emag_total  int |b+B0|^2           3..5 mean_b{x,y,z}  <b + B0>
+ *   6 mag_energy_fluct_total  int |b|^2
+ *   7 dissipation_kinetic  nu int |grad u|^2   8 dissipation_magnetic  eta int |grad b|^2
+ *   9, 10 channel_kz1_{velocity,magnetic}_rms  11 bmax  12 bmax_total
+ *   13..15 electric_{ideal,resistive,total}_l2 of -(u_tot x (b+B0)), eta curl b, their sum
+ *   16..18 divergence_e_l2, divergence_e_ideal_l2, divergence_e_resistive_l2
+ *          (after mask_nyquist(TC.forward)), 19 divergence_a_l2,
+ *   20 divergence_a_wall_linf, 21..23 electric_{ideal,resistive,}_wall_tangential_linf,
+ *   24, 25 faraday_mean_b{y,z}_tendency, 26..28 mean_b{x,y,z}_trace,
+ *   29..37 mean_e{x,y,z}_{ideal,resistive,total} (label-major).
+ * The derived keys (total_stress, alpha_Sh, alpha, mag_energy_mean,
+ * mag_energy_fluct, channel_kz1_total_rms, mean_b_trace_mismatch_linf) are
+ * scalar arithmetic on these.  Replaces PlaneCouetteMRIShearpyJax.diagnostics
+ * with _dissipation_parts, _kz1_rms, _a_divergence, magnetic_flux_from_a,
+ * electric_field_parts, _electric_wall_parts and _electromagnetic_diagnostics
+ * (examples/pcf_mhd_mri_shearpy_jax.py:178-513).  The first call allocates
+ * the diagnostics' device buffers (kept until pcf_destroy).  MHD handles only. */
+#define PCF_XDIAG_COUNT 38
+int32_t pcf_mhd_extended_diagnostics(pcf_handle *h, double *out);
+
 /* Largest magnitudes on the unpadded physical grid (synchronous): out[6] =
  * {max|u_x|, max|u_y + U(x)|, max|u_z|, max|b_x + B0_x|, max|b_y + B0_y|,
  *  max|b_z + B0_z|} of the current state, total velocity (base flow U = u_wall x)

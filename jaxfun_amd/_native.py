@@ -25,6 +25,22 @@ TI_SBDF3 = 3
 TI_CNAB2 = 4
 TIME_INTEGRATORS = {"IMEXRK3": TI_IMEXRK3, "IMEXRK222": TI_IMEXRK222, "SBDF3": TI_SBDF3, "CNAB2": TI_CNAB2}
 NL_ROTATIONAL = 1
+XDIAG_COUNT = 38
+# output order of pcf_mhd_extended_diagnostics
+XDIAG_KEYS = (
+    "reynolds_stress", "maxwell_stress", "Emag_total", "mean_bx", "mean_by", "mean_bz",
+    "mag_energy_fluct_total", "dissipation_kinetic", "dissipation_magnetic",
+    "channel_kz1_velocity_rms", "channel_kz1_magnetic_rms", "bmax", "bmax_total",
+    "electric_ideal_l2", "electric_resistive_l2", "electric_total_l2",
+    "divergence_e_l2", "divergence_e_ideal_l2", "divergence_e_resistive_l2", "divergence_a_l2",
+    "divergence_a_wall_linf", "electric_ideal_wall_tangential_linf",
+    "electric_resistive_wall_tangential_linf", "electric_wall_tangential_linf",
+    "faraday_mean_by_tendency", "faraday_mean_bz_tendency",
+    "mean_bx_trace", "mean_by_trace", "mean_bz_trace",
+    "mean_ex_ideal", "mean_ey_ideal", "mean_ez_ideal",
+    "mean_ex_resistive", "mean_ey_resistive", "mean_ez_resistive",
+    "mean_ex_total", "mean_ey_total", "mean_ez_total",
+)
 
 # enum pcf_field
 FIELDS = (
@@ -54,8 +70,8 @@ EXPORTS = (
     "pcf_create", "pcf_destroy", "pcf_set_stream", "pcf_synchronize",
     "pcf_set_field", "pcf_get_field", "pcf_set_field_device", "pcf_get_field_device", "pcf_set_history",
     "pcf_get_history",
-    "pcf_step", "pcf_set_dt", "pcf_step_steady", "pcf_step_steady_graph", "pcf_graph_cache_info", "pcf_diagnostics", "pcf_mhd_diagnostics", "pcf_field_maxima",
-    "pcf_field_maxima_noalloc", "pcf_velocity_energies", "pcf_physical_fields",
+    "pcf_step", "pcf_set_dt", "pcf_step_steady", "pcf_step_steady_graph", "pcf_graph_cache_info", "pcf_diagnostics", "pcf_mhd_diagnostics",
+    "pcf_mhd_extended_diagnostics", "pcf_field_maxima", "pcf_field_maxima_noalloc", "pcf_velocity_energies", "pcf_physical_fields",
     "pcf_kernels_per_step", "pcf_stage_count", "pcf_stage_name", "pcf_profile_steady", "pcf_profile_stage",
     "pcf_set_cotangent", "pcf_vjp", "pcf_get_gradient",
 )
@@ -140,6 +156,7 @@ def load() -> C.CDLL:
         "pcf_graph_cache_info": ([h, P(C.c_int64), C.c_int32], C.c_int32),
         "pcf_diagnostics": ([h, dp], C.c_int32),
         "pcf_mhd_diagnostics": ([h, dp], C.c_int32),
+        "pcf_mhd_extended_diagnostics": ([h, dp], C.c_int32),
         "pcf_field_maxima": ([h, dp], C.c_int32),
         "pcf_field_maxima_noalloc": ([h, dp], C.c_int32),
         "pcf_velocity_energies": ([h, dp], C.c_int32),

@@ -21,6 +21,7 @@
 #include "../../include/pcf_dns.h"
 #include "pcf_adjoint.hpp"
 #include "pcf_device.hpp"
+#include "pcf_diag.hpp"
 #include "pcf_kernels.hpp"
 #include "pcf_setup.hpp"
 
@@ -188,6 +189,26 @@ struct pcf_handle {
   DevMats<double> md;
   DevMats<float> mf;
   pcf::StructMats sm{};
+  // Device data of the rotating-shear cadence diagnostics
+  // (pcf_mhd_extended_diagnostics), allocated by its first call.
+  struct XDiag {
+    bool ready = false;
+    int P = 0, njobs = 0;
+    pcf::DiagOps ops;
+    double *EB0 = nullptr, *EB1 = nullptr, *ED0 = nullptr, *ED1 = nullptr, *ED2 = nullptr, *EC0 = nullptr,
+           *EC1 = nullptr, *EC2 = nullptr, *Dn = nullptr, *I = nullptr, *xp = nullptr, *wq = nullptr;
+    double2 *twy = nullptr, *twz = nullptr;
+    double2 *NS = nullptr;   // [XD_COUNT][P][m] nodal-spectral fields
+    double2 *G = nullptr;    // [10][P][ny][kzs] streamwise-inverse lines of the first ten
+    double *phys = nullptr;  // [10][P][ny][nz] physical u, b, J, div A
+    double *E = nullptr;     // [6][n0][ny][nz] ideal and resistive electric field
+    double2 *EH = nullptr;   // [6][n0][ny][kzs] their spanwise forward transforms
+    double2 *EC = nullptr;   // [6][n0][m] their TC coefficients at the nodes
+    double2 *DIV = nullptr;  // [2][n0][m] divergence of the ideal and resistive parts
+    pcf::ParsevalJob *jobs = nullptr;
+    double *part = nullptr;  // [njobs][n0]
+    double *emf = nullptr;   // [P][EMF_Q]
+  } xd;
   pcf::FactorSet fsteady{};   // LU factors of the steady SBDF3 systems
   pcf::FactorSet fars{};      // LU factors of M - g dt L for the ARS(2,2,2) stages
   pcf::FactorSet fstage[3]{}; // LU factors at the three IMEXRK3 stage gammas
@@ -2117,6 +2138,254 @@ int32_t pcf_velocity_energies(pcf_handle *h, double *out3) {
   out3[0] = LL * s[4];
   out3[1] = LL * s[5];
   out3[2] = LL * (s[0] - s[4] - s[5]);
+  return PCF_OK;
+}
+
+// Nodal-spectral arrays of the extended diagnostics, in this order in XDiag::NS
+// (the first ten are also transformed to the physical grid).
+enum XdField {
+  XD_U = 0,      // u0, u1, u2
+  XD_B = 3,      // b = curl A (fluctuation) in b_coeff_spaces
+  XD_J = 6,      // curl b, unprojected (_current_physical)
+  XD_DIVA = 9,   // div A
+  XD_DU = 10,    // x-derivatives of u
+  XD_DB = 13,    // x-derivatives of b
+  XD_A1 = 16,    // A_y, A_z values (mean-flux traces)
+  XD_A2 = 17,
+  XD_COUNT = 18,
+};
+
+static int32_t xdiag_setup(pcf_handle *h) {
+  auto &X = h->xd;
+  if (X.ready) return PCF_OK;
+  const pcf::Geo &g = h->geo;
+  std::string err = pcf::build_diag_operators(g.n0, X.ops);
+  if (!err.empty()) return fail(PCF_E_INVALID, "diagnostic operators: " + err);
+  const int P = X.ops.P, n0 = g.n0;
+  X.P = P;
+  auto up = [&](double **dst, const std::vector<double> &src) -> int32_t {
+    if (int32_t r = h->alloc(dst, src.size())) return r;
+    PCF_HIP(hipMemcpy(*dst, src.data(), src.size() * sizeof(double), hipMemcpyHostToDevice));
+    return PCF_OK;
+  };
+  const pcf::DiagOps &o = X.ops;
+  int32_t r = 0;
+  if ((r = up(&X.EB0, o.EB0.a)) || (r = up(&X.EB1, o.EB1.a)) || (r = up(&X.ED0, o.ED0.a)) ||
+      (r = up(&X.ED1, o.ED1.a)) || (r = up(&X.ED2, o.ED2.a)) || (r = up(&X.EC0, o.EC0.a)) ||
+      (r = up(&X.EC1, o.EC1.a)) || (r = up(&X.EC2, o.EC2.a)) || (r = up(&X.Dn, o.Dn.a)) || (r = up(&X.I, o.I.a)) ||
+      (r = up(&X.xp, o.xp)))
+    return r;
+  // interior quadrature weights fejer * (Ly / ny) (Lz / nz); the wall rows carry none
+  std::vector<double> wq(P, 0.0);
+  for (int j = 0; j < n0; ++j) wq[j] = h->ops.fejer[j] * (h->cfg.ly / g.ny) * (h->cfg.lz / g.nz);
+  if ((r = up(&X.wq, wq))) return r;
+  auto twiddles = [&](double2 **dst, int n) -> int32_t {
+    std::vector<double> t(2 * (size_t)n);
+    for (int k = 0; k < n; ++k) {
+      const long double a = 2.0L * 3.141592653589793238462643383279502884L * k / n;
+      t[2 * k] = (double)cosl(a);
+      t[2 * k + 1] = (double)sinl(a);
+    }
+    double *d = nullptr;
+    if (int32_t q = up(&d, t)) return q;
+    *dst = reinterpret_cast<double2 *>(d);
+    return PCF_OK;
+  };
+  if ((r = twiddles(&X.twy, g.ny)) || (r = twiddles(&X.twz, g.nz))) return r;
+  const size_t m = g.m, npl = (size_t)g.ny * g.nz;
+  if ((r = h->alloc(&X.NS, (size_t)XD_COUNT * P * m)) || (r = h->alloc(&X.G, (size_t)10 * P * m)) ||
+      (r = h->alloc(&X.phys, (size_t)10 * P * npl)) || (r = h->alloc(&X.E, (size_t)6 * n0 * npl)) ||
+      (r = h->alloc(&X.EH, (size_t)6 * n0 * m)) || (r = h->alloc(&X.EC, (size_t)6 * n0 * m)) ||
+      (r = h->alloc(&X.DIV, (size_t)2 * n0 * m)) || (r = h->alloc(&X.emf, (size_t)P * pcf::EMF_Q)))
+    return r;
+  // the Parseval jobs read only XDiag's own buffers, so they are set once
+  const double *b0 = g.b0;
+  auto ns = [&](int f) -> const double2 * { return X.NS + (size_t)f * P * m; };
+  std::vector<pcf::ParsevalJob> jobs;
+  auto job = [&](int kind, const double2 *a, const double2 *b, double aa = 0.0, double ab = 0.0) {
+    pcf::ParsevalJob j;
+    j.kind = kind;
+    j.a = a;
+    j.b = b ? b : a;
+    j.add0a = aa;
+    j.add0b = ab;
+    jobs.push_back(j);
+  };
+  job(pcf::PV_DOT, ns(XD_U), ns(XD_U + 1));                              // 0 Reynolds
+  job(pcf::PV_DOT, ns(XD_B), ns(XD_B + 1), b0[0], b0[1]);                // 1 Maxwell (total field)
+  for (int i = 0; i < 3; ++i) job(pcf::PV_DOT, ns(XD_B + i), nullptr, b0[i], b0[i]);  // 2-4 |b + B0|^2
+  for (int i = 0; i < 3; ++i) job(pcf::PV_MEAN, ns(XD_B + i), nullptr, b0[i]);        // 5-7 mean b + B0
+  for (int i = 0; i < 3; ++i) job(pcf::PV_DOT, ns(XD_B + i), nullptr);                // 8-10 |b|^2
+  for (int i = 0; i < 3; ++i) job(pcf::PV_DOT, ns(XD_DU + i), nullptr);               // 11-13 |du/dx|^2
+  for (int i = 0; i < 3; ++i) job(pcf::PV_K2, ns(XD_U + i), nullptr);                 // 14-16 k^2 |u|^2
+  for (int i = 0; i < 3; ++i) job(pcf::PV_DOT, ns(XD_DB + i), nullptr);               // 17-19 |db/dx|^2
+  for (int i = 0; i < 3; ++i) job(pcf::PV_K2, ns(XD_B + i), nullptr);                 // 20-22 k^2 |b|^2
+  for (int i = 0; i < 3; ++i) job(pcf::PV_KZ1, ns(XD_U + i), nullptr);                // 23-25
+  for (int i = 0; i < 3; ++i) job(pcf::PV_KZ1, ns(XD_B + i), nullptr);                // 26-28
+  job(pcf::PV_DOT, ns(XD_DIVA), nullptr);                                // 29 |div A|^2
+  const double2 *di = X.DIV, *dr = X.DIV + (size_t)n0 * m;
+  job(pcf::PV_DOT, di, nullptr);                                         // 30 |div E_ideal|^2
+  job(pcf::PV_DOT, dr, nullptr);                                         // 31 |div E_resistive|^2
+  job(pcf::PV_SUM2, di, dr);                                             // 32 |div E|^2
+  X.njobs = (int)jobs.size();
+  if ((r = h->alloc(&X.part, (size_t)X.njobs * n0))) return r;
+  double *dj = nullptr;
+  if ((r = h->alloc(&dj, jobs.size() * sizeof(pcf::ParsevalJob) / sizeof(double) + 1))) return r;
+  PCF_HIP(hipMemcpy(dj, jobs.data(), jobs.size() * sizeof(pcf::ParsevalJob), hipMemcpyHostToDevice));
+  X.jobs = reinterpret_cast<pcf::ParsevalJob *>(dj);
+  X.ready = true;
+  return PCF_OK;
+}
+
+int32_t pcf_mhd_extended_diagnostics(pcf_handle *h, double *out) {
+  if (!h || !out) return fail(PCF_E_INVALID, "null argument");
+  if (!h->mhd) return fail(PCF_E_INVALID, "not an MHD handle");
+  if (h->cfg.lx[0] != -1.0 || h->cfg.lx[1] != 1.0) return fail(PCF_E_INVALID, "wall-normal domain must be (-1, 1)");
+  PCF_HIP(hipSetDevice(h->device));
+  if (int32_t r = xdiag_setup(h)) return r;
+  PCF_HIP(ensure_u12(h));
+  auto &X = h->xd;
+  const pcf::Geo &g = h->geo;
+  const int P = X.P, n0 = g.n0, nd = g.nd, nb = g.nb;
+  const size_t m = g.m, fs = (size_t)P * m;
+  const bool ins = h->insul;
+  const int ka = ins ? n0 : nd;  // rows of A_y, A_z
+  const double2 *A = h->A[h->so[0]], *A0 = A, *A1 = A + h->cstride, *A2 = A + 2 * h->cstride;
+  const double2 *u[3] = {h->u0[h->so[0]], h->u1, h->u2};
+  auto T = [](const double *M, const double2 *F, int K, int kf = 0, double c = 1.0) {
+    pcf::EvalTerm t;
+    t.M = M;
+    t.F = F;
+    t.K = K;
+    t.kf = kf;
+    t.c = c;
+    return t;
+  };
+  auto out_of = [&](int f, int rows, std::initializer_list<pcf::EvalTerm> ts) {
+    pcf::EvalOut o;
+    o.dst = X.NS + (size_t)f * fs;
+    o.rows = rows;
+    o.nt = 0;
+    for (const auto &t : ts) o.t[o.nt++] = t;
+    return o;
+  };
+  // b = curl A in b_coeff_spaces (examples/pcf_mhd_jax.py:269-289; TC^3 for
+  // insulating walls, examples/pcf_mhd_mri_shearpy_jax.py:548-557) and its
+  // x-derivative at the P points, composed with the evaluation matrices.
+  const double *EA0 = ins ? X.EC0 : X.ED0, *EA1 = ins ? X.EC1 : X.ED1, *EA2 = ins ? X.EC2 : X.ED2;
+  std::vector<pcf::EvalOut> outs = {
+      out_of(XD_U + 0, P, {T(X.EB0, u[0], nb)}),
+      out_of(XD_U + 1, P, {T(X.ED0, u[1], nd)}),
+      out_of(XD_U + 2, P, {T(X.ED0, u[2], nd)}),
+      out_of(XD_DU + 0, P, {T(X.EB1, u[0], nb)}),
+      out_of(XD_DU + 1, P, {T(X.ED1, u[1], nd)}),
+      out_of(XD_DU + 2, P, {T(X.ED1, u[2], nd)}),
+      // bx = i ky A2 - i kz A1 (in TD for conducting walls: the projection of a TD field is itself)
+      out_of(XD_B + 0, P, {T(EA0, A2, ka, 1), T(EA0, A1, ka, 2, -1.0)}),
+      out_of(XD_DB + 0, P, {T(EA1, A2, ka, 1), T(EA1, A1, ka, 2, -1.0)}),
+      // by = i kz A0 - dA2/dx, bz = dA1/dx - i ky A0
+      out_of(XD_B + 1, P, {T(X.ED0, A0, nd, 2), T(EA1, A2, ka, 0, -1.0)}),
+      out_of(XD_DB + 1, P, {T(X.ED1, A0, nd, 2), T(EA2, A2, ka, 0, -1.0)}),
+      out_of(XD_B + 2, P, {T(EA1, A1, ka), T(X.ED0, A0, nd, 1, -1.0)}),
+      out_of(XD_DB + 2, P, {T(EA2, A1, ka), T(X.ED1, A0, nd, 1, -1.0)}),
+      // div A (examples/pcf_mhd_mri_shearpy_jax.py:243-249), A_y and A_z
+      out_of(XD_DIVA, P, {T(X.ED1, A0, nd), T(EA0, A1, ka, 1), T(EA0, A2, ka, 2)}),
+      out_of(XD_A1, P, {T(EA0, A1, ka)}),
+      out_of(XD_A2, P, {T(EA0, A2, ka)}),
+  };
+  hipError_t e = hipSuccess;
+  for (const auto &o : outs)
+    if (e == hipSuccess) e = pcf::launch_eval_terms(g, o, h->stream);
+  if (e == hipSuccess) e = pcf::launch_curl_b(g, P, X.NS + XD_B * fs, X.NS + XD_DB * fs, X.NS + XD_J * fs, h->stream);
+  // u, b, J, div A on the physical grid at all P rows
+  if (e == hipSuccess) e = pcf::launch_dft_y(g, true, 10 * P, X.twy, X.NS, X.G, h->stream);
+  if (e == hipSuccess) e = pcf::launch_dft_z_c2r(g, 10 * P, X.twz, X.G, X.phys, h->stream);
+  if (e == hipSuccess) e = pcf::launch_emf(g, P, h->cfg.eta, X.xp, X.wq, X.phys, X.E, X.emf, h->stream);
+  // TC projection of the ideal and resistive fields (mask_nyquist(TC.forward))
+  // and their divergence at the nodes
+  if (e == hipSuccess) e = pcf::launch_dft_z_r2c(g, 6 * n0, X.twz, X.E, X.EH, h->stream);
+  if (e == hipSuccess) e = pcf::launch_dft_y(g, false, 6 * n0, X.twy, X.EH, X.EC, h->stream);
+  for (int part = 0; part < 2 && e == hipSuccess; ++part) {
+    const double2 *Ex = X.EC + (size_t)(3 * part) * n0 * m;
+    pcf::EvalOut o;
+    o.dst = X.DIV + (size_t)part * n0 * m;
+    o.rows = n0;
+    o.nt = 3;
+    o.t[0] = T(X.Dn, Ex, n0);
+    o.t[1] = T(X.I, Ex + (size_t)n0 * m, n0, 1);
+    o.t[2] = T(X.I, Ex + (size_t)2 * n0 * m, n0, 2);
+    e = pcf::launch_eval_terms(g, o, h->stream);
+  }
+  if (e == hipSuccess) e = pcf::launch_parseval(g, X.jobs, X.njobs, X.part, h->stream);
+  if (e != hipSuccess) return fail(PCF_E_HIP, std::string("extended diagnostics launch: ") + hipGetErrorString(e));
+  std::vector<double> part((size_t)X.njobs * n0), emf((size_t)P * pcf::EMF_Q);
+  double2 wall[4];  // mode 0 of A_y, A_z at x = -1, +1
+  PCF_HIP(hipMemcpyAsync(part.data(), X.part, part.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  PCF_HIP(hipMemcpyAsync(emf.data(), X.emf, emf.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  for (int k = 0; k < 2; ++k)
+    for (int w = 0; w < 2; ++w)
+      PCF_HIP(hipMemcpyAsync(&wall[2 * k + w], X.NS + (size_t)(XD_A1 + k) * fs + (size_t)(n0 + w) * m,
+                             sizeof(double2), hipMemcpyDeviceToHost, h->stream));
+  PCF_HIP(hipStreamSynchronize(h->stream));
+  // assembly (examples/pcf_mhd_mri_shearpy_jax.py:178-215, 251-263, 355-429, 431-513)
+  const double LL = h->cfg.ly * h->cfg.lz, V = 2.0 * LL, lx = 2.0, *b0 = g.b0;
+  const std::vector<double> &fj = h->ops.fejer;
+  double fsum = 0.0;
+  for (int x = 0; x < n0; ++x) fsum += fj[x];
+  auto I = [&](int j) {  // Ly Lz sum_x fejer_x part[j][x]: the volume integral
+    double s = 0.0;
+    for (int x = 0; x < n0; ++x) s += fj[x] * part[(size_t)j * n0 + x];
+    return LL * s;
+  };
+  auto kz1 = [&](int j0) {
+    double s = 0.0;
+    for (int j = j0; j < j0 + 3; ++j)
+      for (int x = 0; x < n0; ++x) s += (fj[x] / fsum) * part[(size_t)j * n0 + x];
+    return g.nz >= 2 ? std::sqrt(2.0 * s) : 0.0;
+  };
+  auto sumrows = [&](int q) {
+    double s = 0.0;
+    for (int p = 0; p < n0; ++p) s += emf[(size_t)p * pcf::EMF_Q + q];
+    return s;
+  };
+  auto maxrows = [&](int q, int p0, int p1) {
+    double s = 0.0;
+    for (int p = p0; p < p1; ++p) s = std::max(s, emf[(size_t)p * pcf::EMF_Q + q]);
+    return s;
+  };
+  const double npl = (double)g.ny * g.nz;
+  auto wmean = [&](int q, int wrow) { return emf[(size_t)(n0 + wrow) * pcf::EMF_Q + q] / npl; };
+  out[0] = I(0) / V;                                   // reynolds_stress
+  out[1] = -I(1) / V;                                  // maxwell_stress
+  out[2] = I(2) + I(3) + I(4);                         // Emag_total
+  for (int i = 0; i < 3; ++i) out[3 + i] = I(5 + i) / V;  // mean_b
+  out[6] = I(8) + I(9) + I(10);                        // mag_energy_fluct_total
+  double dk = 0.0, dm = 0.0;
+  for (int j = 11; j < 17; ++j) dk += I(j);
+  for (int j = 17; j < 23; ++j) dm += I(j);
+  out[7] = h->cfg.nu * dk;                             // dissipation_kinetic
+  out[8] = h->cfg.eta * dm;                            // dissipation_magnetic
+  out[9] = kz1(23);                                    // channel_kz1_velocity_rms
+  out[10] = kz1(26);                                   // channel_kz1_magnetic_rms
+  out[11] = maxrows(pcf::EMF_BMAX, 0, n0);             // bmax
+  out[12] = maxrows(pcf::EMF_BTMAX, 0, n0);            // bmax_total
+  out[13] = std::sqrt(sumrows(pcf::EMF_IDEAL2));       // electric_ideal_l2
+  out[14] = std::sqrt(sumrows(pcf::EMF_RES2));         // electric_resistive_l2
+  out[15] = std::sqrt(sumrows(pcf::EMF_TOT2));         // electric_total_l2
+  out[16] = std::sqrt(I(32));                          // divergence_e_l2
+  out[17] = std::sqrt(I(30));                          // divergence_e_ideal_l2
+  out[18] = std::sqrt(I(31));                          // divergence_e_resistive_l2
+  out[19] = std::sqrt(I(29));                          // divergence_a_l2
+  out[20] = maxrows(pcf::EMF_WDIVA, n0, P);            // divergence_a_wall_linf
+  out[21] = maxrows(pcf::EMF_WIDEAL, n0, P);           // electric_ideal_wall_tangential_linf
+  out[22] = maxrows(pcf::EMF_WRES, n0, P);             // electric_resistive_wall_tangential_linf
+  out[23] = maxrows(pcf::EMF_WTOT, n0, P);             // electric_wall_tangential_linf
+  out[24] = (wmean(pcf::EMF_WSUM2, 1) - wmean(pcf::EMF_WSUM2, 0)) / lx;    // faraday_mean_by_tendency
+  out[25] = -(wmean(pcf::EMF_WSUM1, 1) - wmean(pcf::EMF_WSUM1, 0)) / lx;   // faraday_mean_bz_tendency
+  out[26] = b0[0];                                     // mean_b*_trace (magnetic_flux_from_a)
+  out[27] = -(wall[3].x - wall[2].x) / lx + b0[1];
+  out[28] = (wall[1].x - wall[0].x) / lx + b0[2];
+  for (int q = 0; q < 9; ++q) out[29 + q] = sumrows(pcf::EMF_MEAN + q) / V;  // mean_e{x,y,z}_{ideal,resistive,total}
   return PCF_OK;
 }
 

@@ -296,6 +296,76 @@ std::string build_operators(int n0, Operators &ops) {
   return "";
 }
 
+std::string build_diag_operators(int n0, DiagOps &d) {
+  if (n0 < 9) return "n0 must be >= 9";
+  const int N = n0, NB = n0 - 4, ND = n0 - 2, P = n0 + 2;
+  d.P = P;
+  d.xp.resize(P);
+  // T^(k)_n at the points: T[k](p, n), k = 0, 1, 2, from the three-term
+  // recurrence and its derivatives (exact at x = +-1, where the nodal
+  // cos(n theta) form has no derivative).
+  LMat T[3] = {LMat(P, N), LMat(P, N), LMat(P, N)};
+  for (int p = 0; p < P; ++p) {
+    ld x = (p < N) ? cosl(PI_L + (2 * p + 1) * PI_L / (2 * N)) : (p == N ? -1.0L : 1.0L);
+    d.xp[p] = (double)x;
+    T[0](p, 0) = 1;
+    if (N > 1) {
+      T[0](p, 1) = x;
+      T[1](p, 1) = 1;
+    }
+    for (int n = 2; n < N; ++n) {
+      T[0](p, n) = 2 * x * T[0](p, n - 1) - T[0](p, n - 2);
+      T[1](p, n) = 2 * T[0](p, n - 1) + 2 * x * T[1](p, n - 1) - T[1](p, n - 2);
+      T[2](p, n) = 4 * T[1](p, n - 1) + 2 * x * T[2](p, n - 1) - T[2](p, n - 2);
+    }
+  }
+  LMat SB(NB, N), SD(ND, N);
+  for (int i = 0; i < NB; ++i) {
+    ld n = i;
+    SB(i, i) = 1;
+    SB(i, i + 2) = 2 * (-n - 2) / (n + 3);
+    SB(i, i + 4) = (n + 1) / (n + 3);
+  }
+  for (int i = 0; i < ND; ++i) {
+    SD(i, i) = 1;
+    SD(i, i + 2) = -1;
+  }
+  LMat SBt = transpose(SB), SDt = transpose(SD);
+  d.EB0 = to_double(matmul(T[0], SBt));
+  d.EB1 = to_double(matmul(T[1], SBt));
+  d.ED0 = to_double(matmul(T[0], SDt));
+  d.ED1 = to_double(matmul(T[1], SDt));
+  d.ED2 = to_double(matmul(T[2], SDt));
+  d.EC0 = to_double(T[0]);
+  d.EC1 = to_double(T[1]);
+  d.EC2 = to_double(T[2]);
+  // T^{-1} = diag(1/h) Wq on the Gauss nodes (discrete orthogonality), so the
+  // nodal derivative of the interpolant is Dn = T' diag(1/h) Wq.
+  LMat Tn(N, N), T1n(N, N), Winv(N, N);
+  for (int j = 0; j < N; ++j) {
+    const ld th = PI_L + (2 * j + 1) * PI_L / (2 * N);
+    for (int n = 0; n < N; ++n) {
+      Tn(j, n) = T[0](j, n);
+      T1n(j, n) = T[1](j, n);
+      Winv(n, j) = (PI_L / N) * cosl(n * th) / (n == 0 ? PI_L : PI_L / 2);
+    }
+  }
+  d.Dn = to_double(matmul(T1n, Winv));
+  LMat I(N, N);
+  for (int j = 0; j < N; ++j) I(j, j) = 1;
+  d.I = to_double(I);
+  // check: Dn reproduces the derivative of every T_n, n < N, at the nodes
+  ld worst = 0, scale = 0;
+  LMat chk = matmul(matmul(T1n, Winv), Tn);
+  for (int j = 0; j < N; ++j)
+    for (int n = 0; n < N; ++n) {
+      worst = std::max(worst, fabsl(chk(j, n) - T1n(j, n)));
+      scale = std::max(scale, fabsl(T1n(j, n)));
+    }
+  if (worst > 1e-12L * scale) return "nodal differentiation matrix does not reproduce T'";
+  return "";
+}
+
 const Mat *operator_by_index(const Operators &ops, int which) {
   switch (which) {
     case 0: return &ops.C0B;

@@ -60,8 +60,23 @@ struct Operators {
   } st[2][2];
 };
 
+// Evaluation matrices of the cadence diagnostics on P = n0 + 2 points: the
+// n0 Gauss nodes (ascending) followed by the walls x = -1 and x = +1, so that
+// wall traces (examples/pcf_fluctuations_jax.py:118-129) come out of the same
+// contraction as the nodal values.  Ek<d> holds the d-th x-derivative of the
+// basis k at the points: EB (clamped, P x nb), ED (Dirichlet, P x nd), EC
+// (Chebyshev, P x n0).  Dn = T D T^{-1} (n0 x n0) differentiates nodal values
+// of a degree < n0 interpolant, the TC forward followed by backward_primitive
+// in x; I (n0 x n0) is the identity for terms without an x-derivative.
+struct DiagOps {
+  int P = 0;
+  std::vector<double> xp;  // the P points
+  Mat EB0, EB1, ED0, ED1, ED2, EC0, EC1, EC2, Dn, I;
+};
+
 // Returns an empty string on success, else an error message.
 std::string build_operators(int n0, Operators &ops);
+std::string build_diag_operators(int n0, DiagOps &d);
 
 // Export one matrix by index (see pcf_setup_matrix in include/pcf_dns.h).
 const Mat *operator_by_index(const Operators &ops, int which);

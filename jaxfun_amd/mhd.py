@@ -310,11 +310,70 @@ class PlaneCouetteMRIShearpyHIP(PlaneCouetteMHDHIP):
         return self._new_state(flow, A)
 
     def diagnostics(self, state: MHDState | None = None) -> dict:
-        """Flow and magnetic diagnostics plus the rotation numbers
-        (``examples/pcf_mhd_mri_shearpy_jax.py:94-97``)."""
-        out = super().diagnostics(state)
-        out["q_shear"] = self.q_shear
-        out["kappa2"] = self.kappa2
+        """``PlaneCouetteMRIShearpyJax.diagnostics``
+        (``examples/pcf_mhd_mri_shearpy_jax.py:431-513``): the flow and
+        magnetic diagnostics, the electromagnetic health set (``:355-429``),
+        Reynolds and Maxwell stresses and alpha, the mean-field split, viscous
+        and resistive dissipation (``:178-201``), the kz = 1 channel amplitudes
+        (``:203-215``) and the rotation numbers (``:94-97``), in the
+        reference's key order.  The reductions run on the device
+        (``pcf_mhd_extended_diagnostics``); the derived keys are scalar
+        arithmetic on them, as in the reference."""
+        base = super().diagnostics(state)
+        x = np.zeros(nat.XDIAG_COUNT)
+        nat.check(self._lib.pcf_mhd_extended_diagnostics(self._h, x.ctypes.data_as(C.POINTER(C.c_double))))
+        d = dict(zip(nat.XDIAG_KEYS, (float(v) for v in x)))
+        volume = 2.0 * self.Ly * self.Lz
+        mean_b = (d["mean_bx"], d["mean_by"], d["mean_bz"])
+        trace = (d["mean_bx_trace"], d["mean_by_trace"], d["mean_bz_trace"])
+        em = {k: d[k] for k in ("electric_ideal_l2", "electric_resistive_l2", "electric_total_l2",
+                                "divergence_e_l2", "divergence_e_ideal_l2", "divergence_e_resistive_l2",
+                                "divergence_a_l2", "divergence_a_wall_linf",
+                                "electric_ideal_wall_tangential_linf", "electric_resistive_wall_tangential_linf",
+                                "electric_wall_tangential_linf")}
+        em["faraday_mean_bx_tendency"] = 0.0
+        em["faraday_mean_by_tendency"] = d["faraday_mean_by_tendency"]
+        em["faraday_mean_bz_tendency"] = d["faraday_mean_bz_tendency"]
+        em.update({k: d[k] for k in ("mean_bx_trace", "mean_by_trace", "mean_bz_trace")})
+        em["mean_b_trace_mismatch_linf"] = max(abs(a - b) for a, b in zip(mean_b, trace))
+        for label in ("ideal", "resistive", "total"):
+            for axis in "xyz":
+                em[f"mean_e{axis}_{label}"] = d[f"mean_e{axis}_{label}"]
+        reynolds, maxwell = d["reynolds_stress"], d["maxwell_stress"]
+        transport = reynolds + maxwell
+        sh2 = self.shear_rate**2  # (S x_half_width)^2 with x in (-1, 1)
+        mag_energy_mean = volume * sum(v * v for v in mean_b)
+        kz1_u, kz1_b = d["channel_kz1_velocity_rms"], d["channel_kz1_magnetic_rms"]
+        out = {
+            **base,
+            **em,
+            "Emag_total": d["Emag_total"],
+            "bmax": d["bmax"],
+            "bmax_total": d["bmax_total"],
+            "reynolds_xy": reynolds,
+            "maxwell_xy": maxwell,
+            "reynolds_stress": reynolds,
+            "maxwell_stress": maxwell,
+            "transport_xy": transport,
+            "total_stress": transport,
+            "alpha_Sh": transport / sh2 if sh2 > 0.0 else math.nan,
+            "mean_bx": mean_b[0],
+            "mean_by": mean_b[1],
+            "mean_bz": mean_b[2],
+            "mag_energy_fluct_total": d["mag_energy_fluct_total"],
+            "mag_energy_mean": mag_energy_mean,
+            "mag_energy_fluct": d["Emag_total"] - mag_energy_mean,
+            "dissipation_kinetic": d["dissipation_kinetic"],
+            "dissipation_magnetic": d["dissipation_magnetic"],
+            "q_shear": self.q_shear,
+            "kappa2": self.kappa2,
+            "channel_kz1_velocity_rms": kz1_u,
+            "channel_kz1_magnetic_rms": kz1_b,
+            "channel_kz1_total_rms": math.sqrt(kz1_u**2 + kz1_b**2),
+        }
+        vA2 = self.background_b[2] ** 2
+        if vA2 > 0.0:
+            out["alpha"] = out["alpha_B0"] = transport / vA2
         return out
 
 

@@ -33,7 +33,7 @@ from dataclasses import dataclass, replace
 
 import numpy as np
 
-from .kmm import IMEXRK3_A, IMEXRK3_B, SBDF3_GAMMA, OracleState, PCFOracle, _lu_solve, _nopivot_lu
+from .kmm import IMEXRK3_A, IMEXRK3_B, SBDF3_GAMMA, OracleState, PCFOracle, _lu_solve, _nopivot_lu, fourier_backward
 
 
 @dataclass
@@ -424,3 +424,169 @@ class RotatingShearMHDOracle(MHDOracle):
         z = self.zero_A()
         A = (self.mask(self.tp_forward(ax.astype(complex), self.SD, self.MD1)), z[1], z[2])
         return self.reset_mhd_history(MHDOracleState(flow=flow, A=A))
+
+    # ------------------------------------------------ extended diagnostics
+    # The rotating-shear family's diagnostics (shearpy :178-513): stresses,
+    # alpha, mean-field split, kz = 1 channel amplitudes, dissipation and the
+    # electromagnetic health set, all on the unpadded grid.
+    def b_stencils(self):
+        """Stencils of ``b_coeff_spaces``: (TD, TC, TC) for conducting walls."""
+        return (self.SD, None, None)
+
+    def a_stencils(self):
+        """Stencils of ``a_coeff_spaces``: TD^3 for conducting walls."""
+        return (self.SD, self.SD, self.SD)
+
+    def _bt(self, c, S, kx=0, k1=0, k2=0):
+        """``backward_primitive`` on the unpadded grid."""
+        return self.tp_backward(c, S, padded=False, kx=kx, k1=k1, k2=k2)
+
+    def wall_trace(self, c, S, derivative=0):
+        """``_wall_trace`` (``examples/pcf_fluctuations_jax.py:118-129``): basis
+        values (or x-derivatives) at x = -1, +1 applied along axis 0, then the
+        2-D Fourier backward on the unpadded (y, z) grid; shape (2, N1, N2)."""
+        k = np.arange(self.N0, dtype=float)
+        if derivative == 0:
+            rows = np.stack([(-1.0) ** k, np.ones_like(k)])
+        else:  # T_n'(-1) = (-1)^(n+1) n^2, T_n'(1) = n^2
+            rows = np.stack([(-1.0) ** (k + 1) * k**2, k**2])
+        if S is not None:
+            rows = rows @ S.T
+        w = np.tensordot(rows, c, axes=(1, 0))
+        w = fourier_backward(w, 1, self.N[1])
+        return fourier_backward(w, 2, self.N[2])
+
+    def tc_forward(self, u):
+        """``TC.forward`` of an unpadded physical field: Chebyshev coefficients
+        (``Chebyshev.py:282-301``) and the two Fourier forwards."""
+        from .kmm import cheb_forward, fourier_forward
+
+        c = cheb_forward(u, self.N0)
+        c = fourier_forward(c, 2, self.N[2])
+        return fourier_forward(c, 1, self.N[1])
+
+    def _kz1_rms(self, fields):
+        """``_kz1_rms`` (shearpy :203-215)."""
+        from .kmm import fejer_weights
+
+        nz = self.N[2]
+        w = fejer_weights(self.N0)
+        w = w / np.sum(w)
+        power = 0.0
+        for f in fields:
+            profile = np.mean(np.real(f), axis=1)
+            coef = np.fft.fft(profile, axis=1)[:, 1] / nz
+            power += 2.0 * np.sum(w * np.abs(coef) ** 2)
+        return float(np.sqrt(power))
+
+    @staticmethod
+    def _cross(a, b):
+        """``physical_cross`` (``src/jaxfun/integrators/nonlinear.py:17-29``)."""
+        return (a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0])
+
+    def _l2(self, fields):
+        return float(np.sqrt(sum(np.real(self.integrate(np.conj(f) * f)) for f in fields)))
+
+    def extended_diagnostics(self, s) -> dict:
+        """``PlaneCouetteMRIShearpyJax.diagnostics`` (shearpy :431-513) with
+        ``_dissipation_parts`` (:178-201), ``_kz1_rms`` (:203-215),
+        ``_a_divergence`` (:243-249), ``magnetic_flux_from_a`` (:251-263),
+        ``electric_field_parts`` (:265-276), ``_current_physical`` (:278-294),
+        ``_current_wall_values`` / ``_electric_wall_parts`` (:296-353) and
+        ``_electromagnetic_diagnostics`` (:355-429)."""
+        out = dict(self.diagnostics(s))
+        Sb, Sa = self.b_stencils(), self.a_stencils()
+        Su = (self.SB, self.SD, self.SD)
+        V = 2.0 * self.L1 * self.L2
+        lx = 2.0
+        bt = self._bt
+        up = self.backward_velocity(s.flow)
+        B = self.B_from_A(s.A)
+        bf = tuple(bt(b, S) for b, S in zip(B, Sb))
+        bp = tuple(b + b0 for b, b0 in zip(bf, self.background_b))
+        reynolds = float(np.real(self.integrate(np.real(up[0] * np.conj(up[1]))))) / V
+        maxwell = -float(np.real(self.integrate(np.real(bp[0] * np.conj(bp[1]))))) / V
+        transport = reynolds + maxwell
+        emag_total = float(sum(np.real(self.integrate(np.conj(b) * b)) for b in bp))
+        dk = sum(np.real(self.integrate(np.conj(g) * g)) for c, S in zip(s.flow.u, Su)
+                 for g in (bt(c, S, kx=1), bt(c, S, k1=1), bt(c, S, k2=1)))
+        dm = sum(np.real(self.integrate(np.conj(g) * g)) for c, S in zip(B, Sb)
+                 for g in (bt(c, S, kx=1), bt(c, S, k1=1), bt(c, S, k2=1)))
+        mean_b = tuple(float(np.real(self.integrate(np.real(b)))) / V for b in bp)
+        emag_fluct = float(sum(np.real(self.integrate(np.conj(b) * b)) for b in bf))
+        mag_energy_mean = V * sum(m * m for m in mean_b)
+        kz1_u, kz1_b = self._kz1_rms(up), self._kz1_rms(bf)
+        # electric field parts (physical, unpadded)
+        ut = (up[0], up[1] + self.dUb * self.xb, up[2])
+        J = (bt(B[2], Sb[2], k1=1) - bt(B[1], Sb[1], k2=1), bt(B[0], Sb[0], k2=1) - bt(B[2], Sb[2], kx=1),
+             bt(B[1], Sb[1], kx=1) - bt(B[0], Sb[0], k1=1))
+        ideal = tuple(-e for e in self._cross(ut, bp))
+        resistive = tuple(self.eta * j for j in J)
+        electric = tuple(a + b for a, b in zip(ideal, resistive))
+
+        def div_coeffs(fields):
+            c = [self.mask(self.tc_forward(f)) for f in fields]
+            return bt(c[0], None, kx=1) + bt(c[1], None, k1=1) + bt(c[2], None, k2=1)
+
+        div_i, div_r = div_coeffs(ideal), div_coeffs(resistive)
+        div_e = div_i + div_r
+        div_a = bt(s.A[0], Sa[0], kx=1) + bt(s.A[1], Sa[1], k1=1) + bt(s.A[2], Sa[2], k2=1)
+        wt = self.wall_trace
+        iK1, iK2 = 1j * self.K1, 1j * self.K2
+        div_a_wall = wt(s.A[0], Sa[0], 1) + wt(iK1 * s.A[1], Sa[1]) + wt(iK2 * s.A[2], Sa[2])
+        # wall parts: velocity with the base flow at x = -1, +1, total field, curl B
+        xw = np.array([-1.0, 1.0])[:, None, None]
+        vw = tuple(wt(c, S) for c, S in zip(s.flow.u, Su))
+        vw = (vw[0], vw[1] + self.dUb * xw, vw[2])
+        mw = tuple(wt(c, S) + b0 for c, S, b0 in zip(B, Sb, self.background_b))
+        cw = (wt(iK1 * B[2], Sb[2]) - wt(iK2 * B[1], Sb[1]), wt(iK2 * B[0], Sb[0]) - wt(B[2], Sb[2], 1),
+              wt(B[1], Sb[1], 1) - wt(iK1 * B[0], Sb[0]))
+        ideal_w = tuple(-e for e in self._cross(vw, mw))
+        res_w = tuple(self.eta * c for c in cw)
+        tot_w = tuple(a + b for a, b in zip(ideal_w, res_w))
+
+        def tangential_linf(w):
+            return float(max(np.max(np.abs(w[i])) for i in (1, 2)))
+
+        faraday_by = float(np.real(np.mean(tot_w[2][1]) - np.mean(tot_w[2][0]))) / lx
+        faraday_bz = -float(np.real(np.mean(tot_w[1][1]) - np.mean(tot_w[1][0]))) / lx
+        ay_w, az_w = wt(s.A[1], Sa[1]), wt(s.A[2], Sa[2])
+        b0 = self.background_b
+        trace = (b0[0], float(np.real(-(np.mean(az_w[1]) - np.mean(az_w[0])) / lx)) + b0[1],
+                 float(np.real((np.mean(ay_w[1]) - np.mean(ay_w[0])) / lx)) + b0[2])
+        sq = lambda f: float(np.sqrt(np.real(self.integrate(np.conj(f) * f))))  # noqa: E731
+        out.update({
+            "Emag_total": emag_total,
+            "bmax": float(max(np.max(np.abs(b)) for b in bf)),
+            "bmax_total": float(max(np.max(np.abs(b)) for b in bp)),
+            "reynolds_xy": reynolds, "maxwell_xy": maxwell, "reynolds_stress": reynolds,
+            "maxwell_stress": maxwell, "transport_xy": transport, "total_stress": transport,
+            "mean_bx": mean_b[0], "mean_by": mean_b[1], "mean_bz": mean_b[2],
+            "mag_energy_fluct_total": emag_fluct, "mag_energy_mean": mag_energy_mean,
+            "mag_energy_fluct": emag_total - mag_energy_mean,
+            "dissipation_kinetic": self.nu * float(dk), "dissipation_magnetic": self.eta * float(dm),
+            "channel_kz1_velocity_rms": kz1_u, "channel_kz1_magnetic_rms": kz1_b,
+            "channel_kz1_total_rms": math.sqrt(kz1_u**2 + kz1_b**2),
+            "electric_ideal_l2": self._l2(ideal), "electric_resistive_l2": self._l2(resistive),
+            "electric_total_l2": self._l2(electric),
+            "divergence_e_l2": sq(div_e), "divergence_e_ideal_l2": sq(div_i),
+            "divergence_e_resistive_l2": sq(div_r), "divergence_a_l2": sq(div_a),
+            "divergence_a_wall_linf": float(np.max(np.abs(div_a_wall))),
+            "electric_ideal_wall_tangential_linf": tangential_linf(ideal_w),
+            "electric_resistive_wall_tangential_linf": tangential_linf(res_w),
+            "electric_wall_tangential_linf": tangential_linf(tot_w),
+            "faraday_mean_bx_tendency": 0.0, "faraday_mean_by_tendency": faraday_by,
+            "faraday_mean_bz_tendency": faraday_bz,
+            "mean_bx_trace": trace[0], "mean_by_trace": trace[1], "mean_bz_trace": trace[2],
+            "mean_b_trace_mismatch_linf": float(max(abs(mean_b[i] - trace[i]) for i in range(3))),
+        })
+        for label, fields in (("ideal", ideal), ("resistive", resistive), ("total", electric)):
+            for axis, f in zip("xyz", fields):
+                out[f"mean_e{axis}_{label}"] = float(np.real(self.integrate(f))) / V
+        out["q_shear"] = self.shear_rate / self.omega if self.omega != 0 else math.inf
+        out["kappa2"] = 2.0 * self.omega * (2.0 * self.omega - self.shear_rate)
+        sh2 = (self.shear_rate * 1.0) ** 2
+        out["alpha_Sh"] = transport / sh2 if sh2 > 0.0 else float("nan")
+        if b0[2] ** 2 > 0.0:
+            out["alpha"] = out["alpha_B0"] = transport / b0[2] ** 2
+        return out

@@ -87,6 +87,14 @@ class InsulatingMHDOracle(RotatingShearMHDOracle):
         return self.reset_mhd_history(MHDOracleState(flow=flow, A=A))
 
     # ------------------------------------------------------------ spaces
+    def b_stencils(self):
+        """``b_coeff_spaces`` = TC^3 (shearpy :548-557)."""
+        return (None, None, None)
+
+    def a_stencils(self):
+        """``a_coeff_spaces`` = (TD, TC, TC)."""
+        return (self.SD, None, None)
+
     def zero_A(self):
         shp = (self.N[1], self.N[2])
         return (np.zeros((self.ND,) + shp, complex), np.zeros((self.N0,) + shp, complex),
