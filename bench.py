@@ -488,6 +488,33 @@ def adjoint_leg(precision: str, reps: int = 3) -> dict:
     return {"precision": precision, "forward_ms": 1e3 * tf, "value_and_grad_ms": 1e3 * ta, "ratio": ta / tf}
 
 
+def reverse_leg(precision: str, nsteps: int = 40, reps: int = 3) -> dict:
+    """Time of one reverse steady step, measured by ``pcf_profile_reverse``
+    (reverse kernels alone on one stream, no capture, no overlapped
+    recompute), beside an eager forward steady step with its tape write, at
+    the adjoint workload (state resident)."""
+    from jaxfun_amd import PlaneCouetteFluctuationHIP
+    from jaxfun_amd import _native as nat
+    from jaxfun_amd.ic import seeded_state
+    import numpy as np
+
+    a = ADJOINT
+    solver = PlaneCouetteFluctuationHIP(N=a["N"], Re=a["Re"], dt=a["dt"], precision=precision, **SBDF3_ROT)
+    solver.load(seeded_state(solver, seed=a["seed"], energy=a["energy"], K=a["K"]))
+    solver.advance(2)  # the IMEXRK3 start-up: the history is bootstrapped
+    out = np.zeros(2)
+    runs = []
+    for _ in range(reps + 1):  # the first call allocates the tape
+        nat.check(solver._lib.pcf_profile_reverse(solver._h, nsteps, out.ctypes.data_as(C.POINTER(C.c_double))))
+        runs.append(out.copy())
+    solver.close()
+    rev = sorted(r[0] for r in runs[1:])[reps // 2]
+    fwd = sorted(r[1] for r in runs[1:])[reps // 2]
+    return {"precision": precision, "reverse_steady_step_ms": rev, "forward_steady_step_with_tape_ms": fwd,
+            "reverse_over_forward": rev / fwd, "steps": nsteps,
+            "method": "HIP events on the reverse stream around nsteps-1 reverse steady steps (pcf_profile_reverse)"}
+
+
 def adjoint_fd_check(n_dirs: int = 8, eps: float = 1e-6) -> dict:
     """FP64 gradient against central differences of the HIP forward along
     seeded divergence-free directions (same energy as the initial field), and
@@ -638,6 +665,8 @@ def main(argv=None) -> int:
     ap.add_argument("--no-mhd", action="store_true")
     ap.add_argument("--adjoint-only", action="store_true",
                     help="run only the adjoint timing leg (for kernel profiles)")
+    ap.add_argument("--reverse-only", action="store_true",
+                    help="run only the reverse-step timing (pcf_profile_reverse; for kernel profiles)")
     ap.add_argument("--no-published", action="store_true",
                     help="skip the FP64 runs on the reference's other published configurations")
     ap.add_argument("--mhd-only", action="store_true",
@@ -655,6 +684,9 @@ def main(argv=None) -> int:
         return 2
     if args.adjoint_only:
         print(json.dumps({"adjoint": {p: adjoint_leg(p) for p in ("fp64", "mixed")}}), flush=True)
+        return 0
+    if args.reverse_only:
+        print(json.dumps({"reverse": {p: reverse_leg(p) for p in ("fp64", "mixed")}}), flush=True)
         return 0
     if args.steps < 1 or args.warmup < 0:
         print("--steps must be >= 1 and --warmup >= 0", file=sys.stderr)
@@ -716,6 +748,7 @@ def main(argv=None) -> int:
                            f"65x64x64, Re=1000, dt=0.005, E=1e-4, checkpoints every {ADJOINT['every']}",
                "mixed": adjoint_leg("mixed"), "fp64": adjoint_leg("fp64")}
         adj["fd_check"] = adjoint_fd_check()
+        adj["reverse_step"] = {p: reverse_leg(p) for p in ("fp64", "mixed")}
         line["adjoint"] = adj
         line["adjoint_forward_ratio"] = adj["mixed"]["ratio"]
         print(f"# adjoint/forward: mixed {adj['mixed']['ratio']:.3f}, fp64 {adj['fp64']['ratio']:.3f}; "

@@ -322,6 +322,17 @@ int32_t pcf_stage_count(void);
 const char *pcf_stage_name(int32_t i);
 int32_t pcf_profile_steady(pcf_handle *h, int32_t nsteps, double *stage_ms);
 
+/* Reverse-pass timing outside pcf_vjp (synchronous): nsteps (1..256) eager
+ * steady steps keep their tape on the handle's stream, then the energy seed
+ * and nsteps - 1 reverse steady steps run alone on the reverse stream (no
+ * graph capture, no recompute beside them).  out[2] = {ms per reverse steady
+ * step, ms per forward steady step with its tape}.  The state is unchanged
+ * afterwards and no gradient is left.  Hydrodynamic SBDF3 handles with a
+ * bootstrapped history.  No reference counterpart: the reverse pass's kernel
+ * timing (production/autograd_cost_benchmark.py times only whole value_and_grad
+ * calls). */
+int32_t pcf_profile_reverse(pcf_handle *h, int32_t nsteps, double *out2);
+
 /* Mean duration (milliseconds) of one launch of nonlinear stage `stage`
  * (0 nodal_backward .. 4 forward_rhs) for the current state: the stages before
  * it run once to fill its inputs, then `reps` back-to-back launches between two

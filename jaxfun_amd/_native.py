@@ -73,6 +73,7 @@ EXPORTS = (
     "pcf_step", "pcf_set_dt", "pcf_step_steady", "pcf_step_steady_graph", "pcf_graph_cache_info", "pcf_diagnostics", "pcf_mhd_diagnostics",
     "pcf_mhd_extended_diagnostics", "pcf_field_maxima", "pcf_field_maxima_noalloc", "pcf_velocity_energies", "pcf_physical_fields",
     "pcf_kernels_per_step", "pcf_stage_count", "pcf_stage_name", "pcf_profile_steady", "pcf_profile_stage",
+    "pcf_profile_reverse",
     "pcf_set_cotangent", "pcf_vjp", "pcf_get_gradient",
 )
 SEED_ENERGY = 0
@@ -166,6 +167,7 @@ def load() -> C.CDLL:
         "pcf_stage_name": ([C.c_int32], C.c_char_p),
         "pcf_profile_steady": ([h, C.c_int32, dp], C.c_int32),
         "pcf_profile_stage": ([h, C.c_int32, C.c_int32, dp], C.c_int32),
+        "pcf_profile_reverse": ([h, C.c_int32, dp], C.c_int32),
         "pcf_set_cotangent": ([h, C.c_int32, dp], C.c_int32),
         "pcf_vjp": ([h, C.c_int32, C.c_int32, C.c_int32], C.c_int32),
         "pcf_get_gradient": ([h, C.c_int32, dp], C.c_int32),

@@ -2453,6 +2453,70 @@ int32_t pcf_profile_steady(pcf_handle *h, int32_t nsteps, double *stage_ms) {
   return PCF_OK;
 }
 
+// Timing of the reverse pass's steady steps outside pcf_vjp: nsteps eager
+// steady steps keep their tape (handle stream, no graph), then, after a host
+// synchronisation, the energy seed and nsteps reverse steady steps run on the
+// reverse stream alone, each bracketed by HIP events on that stream.  No
+// recompute overlaps the reverse, nothing is captured.  The state is put back
+// afterwards.  out[0] = ms per reverse steady step, out[1] = ms per forward
+// steady step with its tape write.
+int32_t pcf_profile_reverse(pcf_handle *h, int32_t nsteps, double *out) {
+  if (!h || !out) return fail(PCF_E_INVALID, "null argument");
+  if (h->mhd) return fail(PCF_E_UNSUPPORTED, "the reverse mode covers the hydrodynamic step only");
+  if (h->cfg.time_integrator != PCF_TI_SBDF3) return fail(PCF_E_UNSUPPORTED, "the reverse mode covers SBDF3 only");
+  if (h->history_steps < 2.0) return fail(PCF_E_INVALID, "history not bootstrapped (history_steps < 2)");
+  if (nsteps < 1 || nsteps > 256) return fail(PCF_E_INVALID, "nsteps must be in [1, 256]");
+  PCF_HIP(hipSetDevice(h->device));
+  if (int32_t r0 = resolve_given(h)) return r0;
+  int32_t r;
+  if ((r = adj_setup(h))) return r;
+  if ((r = adj_reserve(h, nsteps, nsteps))) return r;
+  auto &A = h->adj;
+  const pcf::Geo &g = h->geo;
+  hipError_t e;
+  if ((e = snap_save(h, A.snaps[0]))) return fail(PCF_E_HIP, std::string("snapshot: ") + hipGetErrorString(e));
+  hipEvent_t ev[4] = {};
+  for (auto &x : ev) PCF_HIP(hipEventCreate(&x));
+  auto cleanup = [&]() {
+    for (auto &x : ev) (void)hipEventDestroy(x);
+  };
+  PCF_HIP(hipEventRecord(ev[0], h->stream));
+  for (int i = 0; i < nsteps; ++i)
+    if ((r = steady_step(h, &A.tape[i]))) {
+      cleanup();
+      return r;
+    }
+  PCF_HIP(hipEventRecord(ev[1], h->stream));
+  if ((e = ensure_u12(h)) == hipSuccess) e = hipStreamSynchronize(h->stream);
+  const double scale = 2.0 * h->cfg.ly * h->cfg.lz;
+  if (e == hipSuccess)
+    e = pcf::launch_energy_seed(g, A.QB, A.QD, scale, h->u0[h->so[0]], h->u1, h->u2, A.mu_u[0], A.mu_u[1],
+                                A.mu_u[2], A.stream);
+  if (e == hipSuccess)
+    e = pcf::launch_recon_adj(g, A.RcT, A.ldrT, A.mu_u[0], A.mu_u[1], A.mu_u[2], nullptr, A.muX, A.stream);
+  for (int i = 0; i < 3 && e == hipSuccess; ++i) {
+    e = zero_prim(h, A.rho[i]);
+    A.rr[i] = i;
+  }
+  if (e == hipSuccess) e = hipEventRecord(ev[2], A.stream);
+  // steps 1..nsteps-1 of the run (index k > 0: the step-0 branch reads the
+  // initial state's leaves)
+  for (int k = nsteps - 1; k >= 1 && e == hipSuccess; --k) e = reverse_step(h, k, true, &A.tape[k]);
+  if (e == hipSuccess) e = hipEventRecord(ev[3], A.stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(A.stream);
+  float fwd = 0.0f, rev = 0.0f;
+  if (e == hipSuccess) e = hipEventElapsedTime(&fwd, ev[0], ev[1]);
+  if (e == hipSuccess) e = hipEventElapsedTime(&rev, ev[2], ev[3]);
+  cleanup();
+  if (e != hipSuccess) return fail(PCF_E_HIP, std::string("reverse profile: ") + hipGetErrorString(e));
+  if ((e = snap_load(h, A.snaps[0])) == hipSuccess) e = hipStreamSynchronize(h->stream);
+  if (e != hipSuccess) return fail(PCF_E_HIP, std::string("restore: ") + hipGetErrorString(e));
+  A.have_grad = false;
+  out[0] = nsteps > 1 ? (double)rev / (nsteps - 1) : 0.0;
+  out[1] = (double)fwd / nsteps;
+  return PCF_OK;
+}
+
 int32_t pcf_profile_stage(pcf_handle *h, int32_t stage, int32_t reps, double *ms_per_launch) {
   if (!h || !ms_per_launch) return fail(PCF_E_INVALID, "null argument");
   if (stage < 0 || stage > 4) return fail(PCF_E_INVALID, "stage must be a nonlinear stage (0..4)");
