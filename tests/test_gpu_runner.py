@@ -241,3 +241,81 @@ def test_rollout_cache_info_counts_graph_lookups():
     s.advance_steady(5, graph_block=12)
     info = s.rollout_cache_info()
     assert info.misses == 2 and info.step_counts == (5, 12) and info.generation == 0
+
+
+VP_SPEC = {
+    "problem_id": "exp_pcf_mri_vector_potential_smoke",
+    "geometry": "pcf",
+    "physics": "mri",
+    "nondimensional_groups": {"Re": 1000.0, "Rm": 1000.0, "Pm": 1.0, "Omega": 2.0 / 3.0, "S": 1.0,
+                              "q_shear": 1.5, "B0": 0.025, "nu": 0.001, "eta_mag": 0.001},
+    "boundary_conditions": {"velocity": {"type": "no_slip_shearbox_walls"}, "magnetic": {"type": "conducting"}},
+    "domain": {"x": [-1.0, 1.0], "y_period": 4.0, "z_period": 1.0},
+    "resolution": {"start": {"Nx": 17, "Ny": 16, "Nz": 16}, "dealias": {"x": 1.0, "y": 1.5, "z": 1.5},
+                   "family": "C"},
+    "time": {"integrator": "SBDF3", "dt": 0.005, "final_time": 0.06},
+    "initial_condition": {"type": "shearpy_channel_modes", "velocity_amplitude": 0.05, "magnetic_amplitude": 0.05},
+    "expected_oracle": {"divergence_b_guard_l2": 1e-12},
+}
+
+
+@pytest.mark.parametrize("wall", ["conducting", "insulating"])
+def test_vector_potential_run_matches_oracle_and_resumes_exactly(tmp_path, wall):
+    """``_run_pcf_vector_potential_mhd_saturation`` on the HIP rotating-shear
+    solvers: the final scalars equal the oracle's after the same steps, every
+    cadence row closes the shearing-box energy budget to the level the oracle's
+    own trajectory does, and a resume from the step-6 checkpoint is bitwise
+    the uninterrupted run."""
+    from jaxfun_amd.runner import run_pcf_vector_potential_mhd_saturation
+    from oracle.mhd import RotatingShearMHDOracle
+    from oracle.mhd_insulating import InsulatingMHDOracle
+
+    spec = materialize_resolution_tier(dict(VP_SPEC, boundary_conditions={"magnetic": {"type": wall}}), "start")
+    print("case vector-potential run", wall, flush=True)
+    rows = []
+    out = run_pcf_vector_potential_mhd_saturation(spec, out_dir=tmp_path / "a", checkpoint_every=6,
+                                                  diagnostics_every=1, on_row=rows.append)
+    assert len(rows) == 12 and rows[-1]["t"] == pytest.approx(0.06)
+    sc = out["scalars"]
+    assert sc["magnetic_bc"] == wall and sc["representation"] == "vector_potential"
+    cls = InsulatingMHDOracle if wall == "insulating" else RotatingShearMHDOracle
+    o = cls(N=(17, 16, 16), domain=((-1.0, 1.0), (0.0, 4.0), (0.0, 1.0)), Re=1000.0, Rm=1000.0, omega=2.0 / 3.0,
+            shear_rate=1.0, background_b=(0.0, 0.0, 0.025), dt=0.005, perturbation_amplitude=0.05,
+            magnetic_amplitude=0.05)
+    from jaxfun_amd.runner import curl_solver_from_spec
+
+    s0 = curl_solver_from_spec(spec).initial_state()
+    so = to_mhd_oracle(s0)
+    orows = []
+    for k in range(12):
+        so = o.step(so)
+        d = o.extended_diagnostics(so)
+        orows.append({"t": 0.005 * (k + 1), "total_energy": d["Epert"] + d["Emag"],
+                      "total_stress": d["total_stress"], "dissipation_kinetic": d["dissipation_kinetic"],
+                      "dissipation_magnetic": d["dissipation_magnetic"]})
+    want = o.extended_diagnostics(so)
+    for key, ref in (("kinetic_energy", want["Epert"]), ("magnetic_energy", want["Emag"]),
+                     ("maxwell_stress_xy", want["maxwell_stress"]), ("reynolds_stress", want["reynolds_stress"]),
+                     ("dissipation_kinetic", want["dissipation_kinetic"]),
+                     ("channel_kz1_total_rms", want["channel_kz1_total_rms"])):
+        got = sc[key] if key in sc else rows[-1][key]
+        assert math.isclose(got, ref, rel_tol=1e-10), (key, got, ref)
+    for key in ("electric_total_l2", "divergence_a_l2", "mean_bz", "mag_energy_fluct"):
+        assert math.isclose(sc[key], want[key], rel_tol=1e-10, abs_tol=1e-15), (key, sc[key], want[key])
+    from jaxfun_amd.runner import energy_budget_residual
+
+    ob = energy_budget_residual(orows, shear_rate=1.0, volume=8.0)
+    gb = energy_budget_residual(rows, shear_rate=1.0, volume=8.0)
+    print(wall, "energy budget residual", gb, "oracle", ob, flush=True)
+    assert math.isclose(gb, ob, rel_tol=1e-6) and gb < 1e-2
+    assert sc["energy_budget_residual"] == pytest.approx(gb, rel=1e-12)
+    assert sc["divergence_b_l2"] < 1e-12
+    rec = read_checkpoint(tmp_path / "a" / "checkpoints" / "checkpoints.npz", step=6)
+    assert rec.attrs["state_kind"] == "pcf_vector_potential_mhd_saturation"
+    resumed = run_pcf_vector_potential_mhd_saturation(spec, out_dir=tmp_path / "b", checkpoint_every=6,
+                                                      diagnostics_every=1, resume_checkpoint=rec)
+    a, b = out["final_state"], resumed["final_state"]
+    assert_bitwise(a.flow, b.flow)
+    for name in ("A", "nonlinear_A_old", "nonlinear_A_older", "A_old", "A_older"):
+        for p, q in zip(getattr(a, name), getattr(b, name)):
+            assert np.array_equal(np.asarray(p), np.asarray(q)), name
