@@ -536,6 +536,114 @@ __global__ void __launch_bounds__(256) k_rot_adj(Geo g, NodalMats<T> nm, FFTPlan
   }
 }
 
+// Cross-product adjoint of the MHD forcing and EMF (k_rotational_mhd):
+// fields 0-11 of the LDS are the forward u, omega, b, j of one padded y-line
+// pair, 12-14 the cotangent of H and 15-17 that of the EMF E.  With
+// a = u + (0, U x, 0), o = omega + (0, 0, U), bt = b + B0, c = -mu_H / (MY MZ)
+// and d = mu_E / (MY MZ), the forward H = -(a x o) + (-2 Omega u1, 2 Omega u0, 0)
+// - (j x bt), E = a x bt gives
+//   mu_u = o x c + bt x d + 2 Omega (-c1, c0, 0),  mu_omega = c x a,
+//   mu_b = c x j + d x a,                          mu_j = bt x c.
+template <typename T, int MAXB, int NZC>
+__global__ void __launch_bounds__(256) k_rot_adj_mhd(Geo g, NodalMats<T> nm, FFTPlan plan,
+                                                     const typename VT<T>::V *__restrict__ Yf,
+                                                     const typename VT<T>::V *__restrict__ Hz,
+                                                     typename VT<T>::V *__restrict__ Yout) {
+  using V = typename VT<T>::V;
+  constexpr int NIN = 18, NOUT = 12;
+  extern __shared__ __align__(16) unsigned char smem[];
+  V *buf = reinterpret_cast<V *>(smem);  // [NIN][MZ]
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  const int MZ = NZC ? NZC : g.mz, KZ = g.kzs, MY = g.my;
+  const int nunits = g.n0 * (MY / 2);
+  const int u = blockIdx.x;
+  const size_t fstride = (size_t)g.n0 * MY * KZ;
+  V *tw = buf + (size_t)NIN * MZ;
+  for (int i = tid; i < MZ; i += nthr) tw[i] = reinterpret_cast<const V *>(nm.twz)[i];
+  const int x = u / (MY / 2), y2 = u - x * (MY / 2);
+  gather_batched<4>(
+      NIN * MZ,
+      [&](int idx) {
+        const int f = idx / MZ, k = idx - f * MZ;
+        V z{0, 0};
+        if (u < nunits) {
+          const V *src = (f < 12) ? Yf + f * fstride : Hz + (f - 12) * fstride;
+          const V *ra = src + ((size_t)x * MY + 2 * y2) * KZ;
+          const V *rb = ra + KZ;
+          if (k == 0) {
+            z = V{ra[0].x, rb[0].x};
+          } else if (k < KZ) {
+            const V a = ra[k], b = rb[k];
+            z = V{a.x - b.y, a.y + b.x};
+          } else if (k > MZ - KZ) {
+            const V a = ra[MZ - k], b = rb[MZ - k];
+            z = V{a.x + b.y, b.x - a.y};
+          }
+        }
+        return z;
+      },
+      [&](int idx, const V &z) { buf[idx] = z; });
+  __syncthreads();
+  if constexpr (NZC > 0)
+    fft_lds_c<V, T, NZC, true, false, MAXB>(buf, tw, NIN, MZ, 1);
+  else
+    fft_lds<V, T, true, false, MAXB>(buf, tw, plan, NIN, MZ, 1);
+
+  const T U = (T)g.uwall, O2 = (T)g.omega2;
+  const T B0x = (T)g.b0[0], B0y = (T)g.b0[1], B0z = (T)g.b0[2];
+  const T sc = (T)(1.0 / ((double)MY * (double)MZ));
+  const T xv = nm.xnode[u < nunits ? x : 0];
+  for (int n = tid; n < MZ; n += nthr) {
+    V r[NIN];
+#pragma unroll
+    for (int f = 0; f < NIN; ++f) r[f] = buf[f * MZ + n];
+    V o[NOUT];
+#pragma unroll
+    for (int part = 0; part < 2; ++part) {
+      auto get = [&](const V &v) { return part ? v.y : v.x; };
+      const T a0 = get(r[0]), a1 = get(r[1]) + U * xv, a2 = get(r[2]);
+      const T w0 = get(r[3]), w1 = get(r[4]), w2 = get(r[5]) + U;
+      const T t0 = get(r[6]) + B0x, t1 = get(r[7]) + B0y, t2 = get(r[8]) + B0z;
+      const T j0 = get(r[9]), j1 = get(r[10]), j2 = get(r[11]);
+      const T c0 = -sc * get(r[12]), c1 = -sc * get(r[13]), c2 = -sc * get(r[14]);
+      const T d0 = sc * get(r[15]), d1 = sc * get(r[16]), d2 = sc * get(r[17]);
+      const T out[NOUT] = {w1 * c2 - w2 * c1 + t1 * d2 - t2 * d1 - O2 * c1,
+                           w2 * c0 - w0 * c2 + t2 * d0 - t0 * d2 + O2 * c0,
+                           w0 * c1 - w1 * c0 + t0 * d1 - t1 * d0,
+                           c1 * a2 - c2 * a1, c2 * a0 - c0 * a2, c0 * a1 - c1 * a0,
+                           c1 * j2 - c2 * j1 + d1 * a2 - d2 * a1,
+                           c2 * j0 - c0 * j2 + d2 * a0 - d0 * a2,
+                           c0 * j1 - c1 * j0 + d0 * a1 - d1 * a0,
+                           t1 * c2 - t2 * c1, t2 * c0 - t0 * c2, t0 * c1 - t1 * c0};
+#pragma unroll
+      for (int f = 0; f < NOUT; ++f) {
+        if (part)
+          o[f].y = out[f];
+        else
+          o[f].x = out[f];
+      }
+    }
+#pragma unroll
+    for (int f = 0; f < NOUT; ++f) buf[f * MZ + n] = o[f];
+  }
+  __syncthreads();
+  if constexpr (NZC > 0)
+    fft_lds_c<V, T, NZC, false, false, MAXB>(buf, tw, NOUT, MZ, 1);
+  else
+    fft_lds<V, T, false, false, MAXB>(buf, tw, plan, NOUT, MZ, 1);
+  if (u >= nunits) return;
+  for (int idx = tid; idx < NOUT * KZ; idx += nthr) {
+    const int f = idx / KZ, k = idx - f * KZ;
+    const V *P = buf + (size_t)f * MZ;
+    const V p = P[k], q = cconj(P[(MZ - k) % MZ]);
+    const V ha{(p.x + q.x) * (T)0.5, (p.y + q.y) * (T)0.5};
+    const V hb{(p.y - q.y) * (T)0.5, -(p.x - q.x) * (T)0.5};
+    V *dst = Yout + f * fstride + ((size_t)x * MY + 2 * y2) * KZ;
+    dst[k] = ha;
+    dst[KZ + k] = hb;
+  }
+}
+
 __global__ void __launch_bounds__(256) k_energy_seed(Geo g, const double *__restrict__ QB,
                                                      const double *__restrict__ QD, double scale,
                                                      const double2 *__restrict__ u0,
@@ -675,6 +783,40 @@ hipError_t launch_rot_adj(const Geo &g, const NodalMats<T> &nm, const void *Yf, 
 #undef PCF_ROTADJN
   return hipGetLastError();
 }
+
+template <typename T>
+hipError_t launch_rot_adj_mhd(const Geo &g, const NodalMats<T> &nm, const void *Yf, Workspace &ws,
+                              const FFTPlan &pz, hipStream_t s) {
+  using V = typename VT<T>::V;
+  const int nunits = g.n0 * (g.my / 2);
+  const size_t lds = ((size_t)18 * g.mz + g.mz) * sizeof(V);
+  if (lds > 65536) return hipErrorInvalidValue;
+  const int mb = max_bf(pz, 18);
+  const V *yf = reinterpret_cast<const V *>(Yf);
+  const V *hz = reinterpret_cast<const V *>(ws.Hz);
+  V *yo = reinterpret_cast<V *>(ws.Y);
+#define PCF_ROTADJM_N(MB, NZ) \
+  hipLaunchKernelGGL((k_rot_adj_mhd<T, MB, NZ>), dim3(nunits), dim3(256), lds, s, g, nm, pz, yf, hz, yo)
+#define PCF_ROTADJM(MB)                           \
+  switch (g.mz) {                                 \
+    case 24: PCF_ROTADJM_N(MB, 24); break;        \
+    case 48: PCF_ROTADJM_N(MB, 48); break;        \
+    case 96: PCF_ROTADJM_N(MB, 96); break;        \
+    case 144: PCF_ROTADJM_N(MB, 144); break;      \
+    case 192: PCF_ROTADJM_N(MB, 192); break;      \
+    default: PCF_ROTADJM_N(MB, 0);                \
+  }
+  if (mb <= 3) PCF_ROTADJM(3)
+  else if (mb <= 6) PCF_ROTADJM(6)
+  else if (mb <= 12) PCF_ROTADJM(12)
+  else if (mb <= 18) PCF_ROTADJM(18)
+  else return hipErrorInvalidValue;
+#undef PCF_ROTADJM
+#undef PCF_ROTADJM_N
+  return hipGetLastError();
+}
+template hipError_t launch_rot_adj_mhd<double>(const Geo &, const NodalMats<double> &, const void *, Workspace &,
+                                               const FFTPlan &, hipStream_t);
 
 hipError_t launch_prim_sum(const Geo &g, const PrimSum &in, Prim out, hipStream_t s) {
   dim3 grid((g.m + 255) / 256, g.nd);

@@ -76,6 +76,11 @@ hipError_t launch_recon_adj(const Geo &g, const double *RcT, int ldk, const doub
 template <typename T>
 hipError_t launch_rot_adj(const Geo &g, const NodalMats<T> &nm, const void *Yf, Workspace &ws,
                           const FFTPlan &pz, hipStream_t s);
+// MHD: forward fields Yf (12 padded y-line fields u, omega, b, j), cotangents
+// of H and E in ws.Hz (6 fields) -> cotangents of (u, omega, b, j) in ws.Y.
+template <typename T>
+hipError_t launch_rot_adj_mhd(const Geo &g, const NodalMats<T> &nm, const void *Yf, Workspace &ws,
+                              const FFTPlan &pz, hipStream_t s);
 // Streamwise FFT of nf fields between arbitrary buffers (layouts as k_fft_y).
 template <typename T>
 hipError_t launch_fft_y_fields(const Geo &g, const NodalMats<T> &nm, const FFTPlan &py, bool inverse,

@@ -227,6 +227,7 @@ struct pcf_handle {
   // Reverse mode, set up by the first pcf_vjp.
   struct Snap {
     char *dev = nullptr;
+    char *devA = nullptr;  // MHD: the A and NA slots
     int so[3], no[4];
     double hs, ho, pd;
     bool given, valid;
@@ -258,6 +259,18 @@ struct pcf_handle {
     std::vector<Snap> snaps;      // state every `every` steps, then the final state
     size_t snap_bytes = 0;
     bool have_grad = false;
+    // MHD (conducting walls, FP64): cotangents of the vector potential, three
+    // components cstride apart, run through the flow-shaped Dirichlet
+    // kernels at nu = eta with zero / sink buffers in the other slots.
+    bool mhd_ready = false;
+    pcf::FactorSet fA_t{};        // steady factors of the transposed A systems (geo_eta)
+    double2 *muA = nullptr, *nuA = nullptr, *sigA = nullptr, *cotA = nullptr;
+    double2 *rhoA[3]{}, *rsA[3]{}, *ghA[4]{};
+    bool cotA_set = false;
+    pcf::Prim zero{}, sink{};     // zero (never written) and write-sink primary sets
+    double *scratchA = nullptr;
+    double *mEWAG = nullptr;      // W AG^T (n0 x nd): cotangent of the nodal EMF from mu_NA
+    double *mVDT = nullptr, *mVD1T = nullptr, *mVQ1T = nullptr, *mVQ2T = nullptr;  // (nd x n0) transposes
   } adj;
   std::vector<void *> adj_allocs;  // freed on destroy or when the tape grows
 
@@ -1140,10 +1153,25 @@ int32_t steady_run(pcf_handle *h, int n, int tape_base) {
   return PCF_OK;
 }
 
+// MHD: the seven vector-potential slots (3 solution, 4 nonlinear) beside the
+// state block in a snapshot.
+hipError_t snap_mhd(pcf_handle *h, char *dev, bool save) {
+  const size_t bytes = 3 * h->cstride * sizeof(double2);
+  for (int s = 0; s < 7; ++s) {
+    double2 *slot = (s < 3) ? h->A[s] : h->NA[s - 3];
+    char *snap = dev + (size_t)s * bytes;
+    hipError_t e = save ? hipMemcpyAsync(snap, slot, bytes, hipMemcpyDeviceToDevice, h->stream)
+                        : hipMemcpyAsync(slot, snap, bytes, hipMemcpyDeviceToDevice, h->stream);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
 hipError_t snap_save(pcf_handle *h, pcf_handle::Snap &sn) {
   // the state block holds the pieces at the snapshot's offsets: one copy
   hipError_t e = hipMemcpyAsync(sn.dev, h->state_block, h->state_bytes, hipMemcpyDeviceToDevice, h->stream);
   if (e != hipSuccess) return e;
+  if (sn.devA && (e = snap_mhd(h, sn.devA, true))) return e;
   std::copy(h->so, h->so + 3, sn.so);
   std::copy(h->no, h->no + 4, sn.no);
   sn.hs = h->history_steps;
@@ -1157,6 +1185,7 @@ hipError_t snap_save(pcf_handle *h, pcf_handle::Snap &sn) {
 hipError_t snap_load(pcf_handle *h, const pcf_handle::Snap &sn) {
   hipError_t e = hipMemcpyAsync(h->state_block, sn.dev, h->state_bytes, hipMemcpyDeviceToDevice, h->stream);
   if (e != hipSuccess) return e;
+  if (sn.devA && (e = snap_mhd(h, sn.devA, false))) return e;
   std::copy(sn.so, sn.so + 3, h->so);
   std::copy(sn.no, sn.no + 4, h->no);
   h->history_steps = sn.hs;
@@ -1252,6 +1281,8 @@ int32_t adj_setup(pcf_handle *h) {
   int32_t r;
   if (!pcf::rot_adj_fits(g, h->pz, h->cfg.precision == PCF_PREC_FP64 ? 16 : 8))
     return fail(PCF_E_UNSUPPORTED, "padded spanwise grid too large for the cross-product adjoint");
+  if (h->mhd && (size_t)19 * g.mz * sizeof(double2) > 65536)
+    return fail(PCF_E_UNSUPPORTED, "padded spanwise grid too large for the MHD cross-product adjoint");
   // A' = J A^T J per parity block: same band offsets, b'[i][d+P] = b[n-1-i-d][d+P],
   // far factors U'[i] = V[n-1-i], V'[j] = U[n-1-j].
   for (int b = 0; b < 2; ++b)
@@ -1319,8 +1350,9 @@ int32_t adj_setup(pcf_handle *h) {
   if ((r = alloc_prim(h, A.muX)) || (r = alloc_prim(h, A.nu)) || (r = alloc_prim(h, A.sig))) return r;
   {
     const size_t ve = (h->cfg.precision == PCF_PREC_FP64) ? sizeof(double2) : sizeof(float2);
-    const size_t nX = 6 * (size_t)g.n0 * g.m, nY = 6 * (size_t)g.n0 * g.my * g.kzs;
-    const size_t nHz = 3 * (size_t)g.n0 * g.my * g.kzs, nHx = 3 * (size_t)g.n0 * g.m;
+    const size_t nf = h->mhd ? 12 : 6, nh = h->mhd ? 6 : 3;
+    const size_t nX = nf * (size_t)g.n0 * g.m, nY = nf * (size_t)g.n0 * g.my * g.kzs;
+    const size_t nHz = nh * (size_t)g.n0 * g.my * g.kzs, nHx = nh * (size_t)g.n0 * g.m;
     unsigned char *wsp = nullptr;
     if ((r = h->alloc(&wsp, (nX + nY + nHz + nHx) * ve))) return r;
     A.ws.X = wsp;
@@ -1352,7 +1384,7 @@ int32_t adj_reserve(pcf_handle *h, int steps, int every) {
   auto &A = h->adj;
   const pcf::Geo &g = h->geo;
   const size_t ve = (h->cfg.precision == PCF_PREC_FP64) ? sizeof(double2) : sizeof(float2);
-  const size_t slot_bytes = 6 * (size_t)g.n0 * g.my * g.kzs * ve;
+  const size_t slot_bytes = (h->mhd ? 12 : 6) * (size_t)g.n0 * g.my * g.kzs * ve;
   const int per_set = std::min(every, steps) + 4;  // start-up steps evaluate 3 times
   const size_t need_snap = (size_t)((steps + every - 1) / every) + 1;
   int32_t r;
@@ -1367,6 +1399,7 @@ int32_t adj_reserve(pcf_handle *h, int steps, int every) {
   while (A.snaps.size() < need_snap) {
     pcf_handle::Snap sn;
     if ((r = adj_alloc(h, &sn.dev, A.snap_bytes))) return r;
+    if (h->mhd && (r = adj_alloc(h, &sn.devA, 7 * 3 * h->cstride * sizeof(double2)))) return r;
     A.snaps.push_back(sn);
   }
   return PCF_OK;
@@ -1484,6 +1517,396 @@ hipError_t reverse_step(pcf_handle *h, int k, bool steady, void *const *tape) {
     if ((er = pcf::launch_adj_mass(g, h->sm, gamma, cprim(A.rs[s]), ga, A.muX, A.stream))) return er;
   }
   return hipSuccess;
+}
+
+// ------------------------------------------------------------- MHD reverse
+// The vector-potential side of the reverse pass, conducting walls (A in TD^3),
+// FP64.  Each A component is a Dirichlet system at nu = eta, so the flow's
+// transposed solves, mass and sum kernels run on it through the g slot of a
+// primary set whose other slots are a zero set (read) and a sink set (written).
+
+pcf::Prim a_out(const pcf_handle *h, double2 *comp) {
+  const auto &A = h->adj;
+  return pcf::Prim{A.sink.u0, comp, A.sink.v, A.sink.w};
+}
+pcf::PrimC a_in(const pcf_handle *h, const double2 *comp) {
+  const auto &A = h->adj;
+  return pcf::PrimC{A.zero.u0, comp, A.zero.v, A.zero.w};
+}
+
+int32_t adj_setup_mhd(pcf_handle *h) {
+  auto &A = h->adj;
+  if (A.mhd_ready) return PCF_OK;
+  const pcf::Geo &g = h->geo;
+  const pcf::Operators &o = h->ops;
+  const size_t n3 = 3 * h->cstride;
+  int32_t r;
+  if ((r = h->alloc(&A.muA, n3)) || (r = h->alloc(&A.nuA, n3)) || (r = h->alloc(&A.sigA, n3)) ||
+      (r = h->alloc(&A.cotA, n3)))
+    return r;
+  for (int i = 0; i < 3; ++i)
+    if ((r = h->alloc(&A.rhoA[i], n3)) || (r = h->alloc(&A.rsA[i], n3))) return r;
+  for (int i = 0; i < 4; ++i)
+    if ((r = h->alloc(&A.ghA[i], n3))) return r;
+  if ((r = alloc_prim(h, A.zero)) || (r = alloc_prim(h, A.sink))) return r;
+  if ((r = h->alloc(&A.scratchA, pcf::solve_scratch_doubles(g)))) return r;
+  // steady factors of the transposed blocks at nu = eta (the clamped and mean
+  // blocks of these sets only ever see the zero set)
+  for (int b = 0; b < 2; ++b)
+    for (int p = 0; p < 2; ++p)
+      if ((r = h->alloc(&A.fA_t.blk[b][p], pcf::factor_doubles(g, A.smt, b, p)))) return r;
+  for (int p = 0; p < 2; ++p)
+    if ((r = h->alloc(&A.fA_t.mean[p], pcf::factor_doubles(g, A.smt, -1, p)))) return r;
+  hipError_t e = pcf::launch_factor(h->geo_eta, A.smt, h->cfg.dt * 6.0 / 11.0, A.fA_t, false, h->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  if (e != hipSuccess) return fail(PCF_E_HIP, std::string("factorisation: ") + hipGetErrorString(e));
+  const int n0 = g.n0, nd = g.nd;
+  std::vector<double> ewag((size_t)n0 * nd), vdt((size_t)nd * n0), vd1t((size_t)nd * n0), vq1t((size_t)nd * n0),
+      vq2t((size_t)nd * n0);
+  for (int x = 0; x < n0; ++x)
+    for (int i = 0; i < nd; ++i) {
+      ewag[(size_t)x * nd + i] = g.W * o.AG(i, x);
+      vdt[(size_t)i * n0 + x] = o.VD(x, i);
+      vd1t[(size_t)i * n0 + x] = o.VD1(x, i);
+      vq1t[(size_t)i * n0 + x] = o.VQ1(x, i);
+      vq2t[(size_t)i * n0 + x] = o.VQ2(x, i);
+    }
+  if ((r = h->upload(&A.mEWAG, ewag)) || (r = h->upload(&A.mVDT, vdt)) || (r = h->upload(&A.mVD1T, vd1t)) ||
+      (r = h->upload(&A.mVQ1T, vq1t)) || (r = h->upload(&A.mVQ2T, vq2t)))
+    return r;
+  A.mhd_ready = true;
+  return PCF_OK;
+}
+
+pcf::EvalTerm eterm(const double *M, const double2 *F, int K, int kf, double c) {
+  pcf::EvalTerm t;
+  t.M = M;
+  t.F = F;
+  t.K = K;
+  t.kf = kf;
+  t.c = c;
+  return t;
+}
+
+// mu_A = (b, j map)^H (mu_b, mu_j): the transpose of the nodal b and j of
+// k_nodal_mhd_mfma (conducting walls), from nodal cotangents bj = [6][n0][m]
+// (bx, by, bz, jx, jy, jz) into out (three components cstride apart).
+hipError_t a_nodal_adj(pcf_handle *h, const double2 *bj, double2 *out) {
+  const auto &A = h->adj;
+  const pcf::Geo &g = h->geo;
+  const size_t pl = (size_t)g.n0 * g.m, cs = h->cstride;
+  const double2 *bx = bj, *by = bj + pl, *bz = bj + 2 * pl, *jx = bj + 3 * pl, *jy = bj + 4 * pl, *jz = bj + 5 * pl;
+  const int K = g.n0;
+  using namespace pcf;
+  EvalOut o[3];
+  const EvalTerm tx[5] = {eterm(A.mVDT, by, K, KF_IKZ, -1.0), eterm(A.mVDT, bz, K, KF_IKY, 1.0),
+                          eterm(A.mVDT, jx, K, KF_K2, 1.0), eterm(A.mVQ1T, jy, K, KF_IKY, -1.0),
+                          eterm(A.mVQ1T, jz, K, KF_IKZ, -1.0)};
+  const EvalTerm ty[6] = {eterm(A.mVDT, bx, K, KF_IKZ, 1.0), eterm(A.mVDT, jy, K, KF_KZZ, 1.0),
+                          eterm(A.mVDT, jz, K, KF_KYZ, -1.0), eterm(A.mVD1T, bz, K, KF_ONE, 1.0),
+                          eterm(A.mVD1T, jx, K, KF_IKY, -1.0), eterm(A.mVQ2T, jy, K, KF_ONE, -1.0)};
+  const EvalTerm tz[6] = {eterm(A.mVDT, bx, K, KF_IKY, -1.0), eterm(A.mVDT, jy, K, KF_KYZ, -1.0),
+                          eterm(A.mVDT, jz, K, KF_KYY, 1.0), eterm(A.mVD1T, by, K, KF_ONE, -1.0),
+                          eterm(A.mVD1T, jx, K, KF_IKZ, -1.0), eterm(A.mVQ2T, jz, K, KF_ONE, -1.0)};
+  const EvalTerm *ts[3] = {tx, ty, tz};
+  const int nt[3] = {5, 6, 6};
+  for (int c = 0; c < 3; ++c) {
+    o[c].dst = out + c * cs;
+    o[c].rows = g.nd;
+    o[c].nt = nt[c];
+    for (int t = 0; t < nt[c]; ++t) o[c].t[t] = ts[c][t];
+    hipError_t e = launch_eval_terms(g, o[c], A.stream);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+// Cotangents of u (and curl u), b and j of the coupled nonlinear rows from
+// adj.nu (flow rows) and adj.nuA (induction rows) at the evaluation whose
+// padded y-line fields (u, omega, b, j) are ytape: the flow part into
+// adj.muX (steady_x: with the reconstruction adjoint and M^T sig folded in)
+// or adj.mu_u, the vector-potential part into adj.muA (overwritten).
+hipError_t nl_adj_mhd(pcf_handle *h, const void *ytape, bool steady_x) {
+  auto &A = h->adj;
+  const pcf::Geo &g = h->geo;
+  const size_t pl = (size_t)g.n0 * g.m;
+  hipError_t e;
+  if ((e = pcf::launch_rhs_adj_mfma<double>(g, A.ad, A.nu, A.ws, A.stream))) return e;
+  double2 *Hx = reinterpret_cast<double2 *>(A.ws.Hx);
+  for (int c = 0; c < 3; ++c) {
+    pcf::EvalOut o;
+    o.dst = Hx + (3 + c) * pl;
+    o.rows = g.n0;
+    o.nt = 1;
+    o.t[0] = eterm(A.mEWAG, A.nuA + c * h->cstride, g.nd, pcf::KF_ONE, 1.0);
+    if ((e = pcf::launch_eval_terms(g, o, A.stream))) return e;
+  }
+  if ((e = pcf::launch_fft_y_fields<double>(g, h->md.view, h->py, true, 6, A.ws.Hx, A.ws.Hz, A.stream))) return e;
+  if ((e = pcf::launch_rot_adj_mhd<double>(g, h->md.view, ytape, A.ws, h->pz, A.stream))) return e;
+  if ((e = pcf::launch_fft_y_fields<double>(g, h->md.view, h->py, false, 12, A.ws.Y, A.ws.X, A.stream))) return e;
+  if (steady_x)
+    e = pcf::launch_nodal_adj_x_mfma<double>(g, A.ad, h->sm, cprim(A.sig), A.ws, A.muX, A.stream);
+  else
+    e = pcf::launch_nodal_adj_mfma<double>(g, A.ad, A.ws, A.mu_u[0], A.mu_u[1], A.mu_u[2], A.stream);
+  if (e != hipSuccess) return e;
+  return a_nodal_adj(h, reinterpret_cast<const double2 *>(A.ws.X) + 6 * pl, A.muA);
+}
+
+// Transposed A solves of the steady systems: out = S_A^{-T} in with
+// nuA = e0 out + e1 r1 + e2 r2 and sigA = C_SBDF . (out, r1, r2).
+hipError_t a_solve_factored(pcf_handle *h, const double2 *in, double2 *out, const double2 *r1, const double2 *r2,
+                            const double e[3]) {
+  auto &A = h->adj;
+  for (int c = 0; c < 3; ++c) {
+    const size_t o = c * h->cstride;
+    hipError_t er = pcf::launch_adj_solve_factored(h->geo_eta, A.smt, A.fA_t, a_in(h, in + o), a_out(h, out + o),
+                                                   a_out(h, A.nuA + o), a_in(h, r1 + o), a_in(h, r2 + o), e[0],
+                                                   e[1], e[2], A.stream, a_out(h, A.sigA + o), C_SBDF[0],
+                                                   C_SBDF[1], C_SBDF[2]);
+    if (er != hipSuccess) return er;
+  }
+  return hipSuccess;
+}
+
+// out = (M_A - gamma L_A)^{-T} in (start-up stages, factorised per launch).
+hipError_t a_solve(pcf_handle *h, double gamma, const double2 *in, double2 *out) {
+  auto &A = h->adj;
+  for (int c = 0; c < 3; ++c) {
+    const size_t o = c * h->cstride;
+    hipError_t er = pcf::launch_adj_solve(h->geo_eta, A.smt, gamma, a_in(h, in + o), a_out(h, out + o),
+                                          A.scratchA, A.stream);
+    if (er != hipSuccess) return er;
+  }
+  return hipSuccess;
+}
+
+struct ATerm {
+  const double2 *p;
+  double w;
+};
+
+// out += (M_A + gamma L_A)^T rstage (when given) + M_A^T sum w_t p_t.
+hipError_t a_mass(pcf_handle *h, double gamma, const double2 *rstage, std::initializer_list<ATerm> gather,
+                  double2 *out) {
+  auto &A = h->adj;
+  for (int c = 0; c < 3; ++c) {
+    const size_t o = c * h->cstride;
+    pcf::PrimSum s{};
+    for (const ATerm &t : gather) {
+      s.t[s.n] = a_in(h, t.p + o);
+      s.wgt[s.n] = t.w;
+      ++s.n;
+    }
+    hipError_t er = pcf::launch_adj_mass(h->geo_eta, h->sm, gamma, rstage ? a_in(h, rstage + o) : pcf::PrimC{}, s,
+                                         a_out(h, out + o), A.stream);
+    if (er != hipSuccess) return er;
+  }
+  return hipSuccess;
+}
+
+// out = sum w_t p_t.
+hipError_t a_sum(pcf_handle *h, std::initializer_list<ATerm> terms, double2 *out) {
+  auto &A = h->adj;
+  for (int c = 0; c < 3; ++c) {
+    const size_t o = c * h->cstride;
+    pcf::PrimSum s{};
+    for (const ATerm &t : terms) {
+      s.t[s.n] = a_in(h, t.p + o);
+      s.wgt[s.n] = t.w;
+      ++s.n;
+    }
+    hipError_t er = pcf::launch_prim_sum(h->geo, s, a_out(h, out + o), A.stream);
+    if (er != hipSuccess) return er;
+  }
+  return hipSuccess;
+}
+
+hipError_t a_zero(pcf_handle *h, double2 *p) {
+  return hipMemsetAsync(p, 0, 3 * h->cstride * sizeof(double2), h->adj.stream);
+}
+
+// Reverse of MHD step k (conducting walls): the flow as reverse_step, the
+// vector potential beside it (examples/pcf_mhd_jax.py:488-715 transposed).
+// mu of the step's output in adj.muX / adj.muA on entry, of its input on exit.
+hipError_t reverse_step_mhd(pcf_handle *h, int k, bool steady, void *const *tape) {
+  auto &A = h->adj;
+  const pcf::Geo &g = h->geo;
+  const double dt = h->cfg.dt;
+  const double e[3] = {dt * 18.0 / 11.0, -dt * 18.0 / 11.0, dt * 6.0 / 11.0};
+  int *rr = A.rr;
+  {
+    const int t = rr[2];
+    rr[2] = rr[1];
+    rr[1] = rr[0];
+    rr[0] = t;
+  }
+  hipError_t er;
+  if (steady) {
+    if ((er = pcf::launch_adj_solve_factored(g, A.smt, A.fsteady_t, cprim(A.muX), A.rho[rr[0]], A.nu,
+                                             cprim(A.rho[rr[1]]), cprim(A.rho[rr[2]]), e[0], e[1], e[2],
+                                             A.stream, A.sig, C_SBDF[0], C_SBDF[1], C_SBDF[2])))
+      return er;
+    if ((er = a_solve_factored(h, A.muA, A.rhoA[rr[0]], A.rhoA[rr[1]], A.rhoA[rr[2]], e))) return er;
+    if (k == 0) {  // initial state: its velocity leaves keep mu_u
+      if ((er = nl_adj_mhd(h, tape[0], false))) return er;
+      pcf::PrimSum ga{};
+      for (int j = 0; j < 3; ++j) ps_add(ga, A.rho[rr[j]], C_SBDF[j]);
+      if ((er = zero_prim(h, A.muX))) return er;
+      if ((er = pcf::launch_adj_mass(g, h->sm, 0.0, pcf::PrimC{}, ga, A.muX, A.stream))) return er;
+    } else if ((er = nl_adj_mhd(h, tape[0], true))) {
+      return er;
+    }
+    return a_mass(h, 0.0, nullptr, {{A.sigA, 1.0}}, A.muA);
+  }
+  if ((er = zero_prim(h, A.rho[rr[0]])) || (er = a_zero(h, A.rhoA[rr[0]]))) return er;
+  for (int s = 2; s >= 0; --s) {
+    const double gamma = (IMEX_A[s] + IMEX_B[s]) * dt / 2.0;
+    if ((er = pcf::launch_adj_solve(g, A.smt, gamma, cprim(A.muX), A.rs[s], A.scratch, A.stream))) return er;
+    if ((er = a_solve(h, gamma, A.muA, A.rsA[s]))) return er;
+    pcf::PrimSum nu{};
+    ps_add(nu, A.rs[s], dt * IMEX_A[s]);
+    if (s < 2) ps_add(nu, A.rs[s + 1], dt * IMEX_B[s + 1]);
+    if (s == 0) {
+      ps_add(nu, A.rho[rr[1]], e[1]);
+      ps_add(nu, A.rho[rr[2]], e[2]);
+    }
+    if ((er = pcf::launch_prim_sum(g, nu, A.nu, A.stream))) return er;
+    if (s == 2)
+      er = a_sum(h, {{A.rsA[2], dt * IMEX_A[2]}}, A.nuA);
+    else if (s == 1)
+      er = a_sum(h, {{A.rsA[1], dt * IMEX_A[1]}, {A.rsA[2], dt * IMEX_B[2]}}, A.nuA);
+    else
+      er = a_sum(h, {{A.rsA[0], dt * IMEX_A[0]}, {A.rsA[1], dt * IMEX_B[1]}, {A.rhoA[rr[1]], e[1]},
+                     {A.rhoA[rr[2]], e[2]}}, A.nuA);
+    if (er) return er;
+    if ((er = nl_adj_mhd(h, tape[s], false))) return er;
+    if ((er = recon_or_zero(h, s == 0 && k == 0))) return er;
+    pcf::PrimSum ga{};
+    if (s == 0) {
+      ps_add(ga, A.rho[rr[1]], C_SBDF[1]);
+      ps_add(ga, A.rho[rr[2]], C_SBDF[2]);
+    }
+    if ((er = pcf::launch_adj_mass(g, h->sm, gamma, cprim(A.rs[s]), ga, A.muX, A.stream))) return er;
+    er = (s == 0) ? a_mass(h, gamma, A.rsA[s], {{A.rhoA[rr[1]], C_SBDF[1]}, {A.rhoA[rr[2]], C_SBDF[2]}}, A.muA)
+                  : a_mass(h, gamma, A.rsA[s], {}, A.muA);
+    if (er) return er;
+  }
+  return hipSuccess;
+}
+
+// pcf_vjp for the conducting-wall MHD step: the same checkpointing as the
+// hydrodynamic reverse pass, run in sequence (each segment is recomputed on
+// the handle stream once the reverse of the next one has finished with the
+// tape, no graph capture), with the vector-potential cotangents beside the
+// flow's.  Seed: Epert + Emag of the final state, or the cotangents set.
+int32_t vjp_mhd(pcf_handle *h, int steps, int every, int seed) {
+  auto &A = h->adj;
+  const pcf::Geo &g = h->geo;
+  const int nseg = (steps + every - 1) / every;
+  const int h0 = (int)std::floor(std::min(2.0, std::max(0.0, h->history_steps)));
+  const double dt = h->cfg.dt;
+  const size_t pl = (size_t)g.n0 * g.m;
+  A.have_grad = false;
+  auto hip_fail = [&](hipError_t e, const char *what) {
+    return fail(PCF_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
+  };
+  hipError_t e;
+  // segment seg's steps on the handle stream, keeping the tape from slot 0;
+  // first[i] receives the tape slot of step ka + i
+  auto run_seg = [&](int seg, std::vector<int> &first) -> int32_t {
+    const int ka = seg * every, kb = std::min(steps, ka + every);
+    first.assign(kb - ka, 0);
+    int slot = 0;
+    for (int k = ka; k < kb; ++k) {
+      first[k - ka] = slot;
+      const bool st = h->history_steps >= 2.0;
+      int32_t r = st ? steady_step(h, &A.tape[slot]) : startup_step(h, &A.tape[slot]);
+      if (r) return r;
+      slot += st ? 1 : 3;
+    }
+    return PCF_OK;
+  };
+  std::vector<int> first;
+  int32_t r;
+  for (int seg = 0; seg < nseg; ++seg) {
+    if ((e = snap_save(h, A.snaps[seg]))) return hip_fail(e, "snapshot");
+    if ((r = run_seg(seg, first))) return r;
+  }
+  pcf_handle::Snap &fin = A.snaps[nseg];
+  if ((e = snap_save(h, fin))) return hip_fail(e, "snapshot");
+  if ((e = ensure_u12(h))) return hip_fail(e, "reconstruction");
+  PCF_HIP(hipEventRecord(A.ev_seed, h->stream));
+  PCF_HIP(hipStreamWaitEvent(A.stream, A.ev_seed, 0));
+  const size_t bytesA = 3 * h->cstride * sizeof(double2);
+  if (seed == PCF_SEED_ENERGY) {
+    const double scale = 2.0 * h->cfg.ly * h->cfg.lz;
+    e = pcf::launch_energy_seed(g, A.QB, A.QD, scale, h->u0[h->so[0]], h->u1, h->u2, A.mu_u[0], A.mu_u[1],
+                                A.mu_u[2], A.stream);
+    if (e == hipSuccess)
+      e = pcf::launch_recon_adj(g, A.RcT, A.ldrT, A.mu_u[0], A.mu_u[1], A.mu_u[2], nullptr, A.muX, A.stream);
+    // Emag: b and j of the final A at the nodes, mu_b = 2 Ly Lz fejer b, mu_j = 0
+    if (e == hipSuccess)
+      e = pcf::launch_nodal_mhd_mfma<double>(g, h->md.mview, h->A[h->so[0]], h->cstride, A.ws, A.stream);
+    double2 *bj = reinterpret_cast<double2 *>(A.ws.X) + 6 * pl;
+    if (e == hipSuccess) e = pcf::launch_seed_b(g, h->dfejer, scale, bj, A.stream);
+    if (e == hipSuccess) e = a_nodal_adj(h, bj, A.muA);
+  } else {
+    e = pcf::launch_recon_adj(g, A.RcT, A.ldrT, A.cot_u[0], A.cot_u[1], A.cot_u[2], A.cot_g, A.muX, A.stream);
+    if (e == hipSuccess)
+      e = A.cotA_set ? hipMemcpyAsync(A.muA, A.cotA, bytesA, hipMemcpyDeviceToDevice, A.stream) : a_zero(h, A.muA);
+  }
+  if (e != hipSuccess) return hip_fail(e, "seed");
+  for (int i = 0; i < 3; ++i) {
+    if ((e = zero_prim(h, A.rho[i])) || (e = a_zero(h, A.rhoA[i]))) return hip_fail(e, "memset");
+    A.rr[i] = i;
+  }
+  for (int seg = nseg - 1; seg >= 0; --seg) {
+    const int ka = seg * every, kb = std::min(steps, ka + every);
+    if (seg < nseg - 1) {
+      // the recompute rewrites the tape (and the state the seed read) only
+      // after the reverse pass before it is done with them
+      PCF_HIP(hipEventRecord(A.ev_rev[0], A.stream));
+      PCF_HIP(hipStreamWaitEvent(h->stream, A.ev_rev[0], 0));
+      if ((e = snap_load(h, A.snaps[seg]))) return hip_fail(e, "restore");
+      if ((r = run_seg(seg, first))) return r;
+      PCF_HIP(hipEventRecord(A.ev_rc[0], h->stream));
+      PCF_HIP(hipStreamWaitEvent(A.stream, A.ev_rc[0], 0));
+    }
+    for (int k = kb - 1; k >= ka; --k)
+      if ((e = reverse_step_mhd(h, k, h0 + k >= 2, &A.tape[first[k - ka]]))) return hip_fail(e, "reverse step");
+  }
+  // history leaves of the initial state: levels -1 and -2 (flow as pcf_vjp, then A)
+  {
+    const double e1 = -dt * 18.0 / 11.0, e2 = dt * 6.0 / 11.0;
+    const pcf::Prim &r0 = A.rho[A.rr[0]], &r1 = A.rho[A.rr[1]];
+    pcf::PrimSum s{};
+    for (int i = 0; i < 2; ++i)
+      if ((e = zero_prim(h, A.gh[i]))) return hip_fail(e, "memset");
+    ps_add(s, r0, C_SBDF[1]);
+    ps_add(s, r1, C_SBDF[2]);
+    if ((e = pcf::launch_adj_mass(g, h->sm, 0.0, pcf::PrimC{}, s, A.gh[0], A.stream))) return hip_fail(e, "history");
+    s = pcf::PrimSum{};
+    ps_add(s, r0, C_SBDF[2]);
+    if ((e = pcf::launch_adj_mass(g, h->sm, 0.0, pcf::PrimC{}, s, A.gh[1], A.stream))) return hip_fail(e, "history");
+    s = pcf::PrimSum{};
+    ps_add(s, r0, e1);
+    ps_add(s, r1, e2);
+    if ((e = pcf::launch_prim_sum(g, s, A.gh[2], A.stream))) return hip_fail(e, "history");
+    s = pcf::PrimSum{};
+    ps_add(s, r0, e2);
+    if ((e = pcf::launch_prim_sum(g, s, A.gh[3], A.stream))) return hip_fail(e, "history");
+    const double2 *a0 = A.rhoA[A.rr[0]], *a1 = A.rhoA[A.rr[1]];
+    if ((e = a_zero(h, A.ghA[0])) || (e = a_zero(h, A.ghA[1]))) return hip_fail(e, "memset");
+    if ((e = a_mass(h, 0.0, nullptr, {{a0, C_SBDF[1]}, {a1, C_SBDF[2]}}, A.ghA[0])) ||
+        (e = a_mass(h, 0.0, nullptr, {{a0, C_SBDF[2]}}, A.ghA[1])) || (e = a_sum(h, {{a0, e1}, {a1, e2}}, A.ghA[2])) ||
+        (e = a_sum(h, {{a0, e2}}, A.ghA[3])))
+      return hip_fail(e, "history");
+  }
+  PCF_HIP(hipStreamSynchronize(A.stream));
+  if ((e = snap_load(h, fin))) return hip_fail(e, "restore");
+  PCF_HIP(hipStreamSynchronize(h->stream));
+  A.have_grad = true;
+  return PCF_OK;
 }
 
 // Cotangent of a full-spectrum jax gradient g into the half layout:
@@ -2549,11 +2972,27 @@ int32_t pcf_set_cotangent(pcf_handle *h, int32_t field, const double *host) {
     case PCF_F_U1: idx = 1; break;
     case PCF_F_U2: idx = 2; break;
     case PCF_F_G: idx = 3; break;
-    default: return fail(PCF_E_INVALID, "cotangents are taken for U0, U1, U2 and G only");
+    case PCF_F_AX:
+    case PCF_F_AY:
+    case PCF_F_AZ: idx = 4 + (field - PCF_F_AX); break;
+    default: return fail(PCF_E_INVALID, "cotangents are taken for U0, U1, U2, G (and AX, AY, AZ under MHD) only");
   }
+  if (idx >= 4 && !h->mhd) return fail(PCF_E_INVALID, "no vector potential: not an MHD handle");
   PCF_HIP(hipSetDevice(h->device));
   int32_t r = adj_setup(h);
   if (r) return r;
+  if (idx >= 4) {
+    if (h->insul || h->cfg.precision != PCF_PREC_FP64)
+      return fail(PCF_E_UNSUPPORTED, "the MHD reverse mode covers conducting walls in FP64");
+    if ((r = adj_setup_mhd(h))) return r;
+    std::vector<double2> half;
+    half_from_cotangent(h->geo, h->geo.nd, host, half);
+    PCF_HIP(hipStreamSynchronize(h->stream));
+    PCF_HIP(hipMemcpy(h->adj.cotA + (idx - 4) * h->cstride, half.data(), half.size() * sizeof(double2),
+                      hipMemcpyHostToDevice));
+    h->adj.cotA_set = true;
+    return PCF_OK;
+  }
   const pcf::Geo &g = h->geo;
   const int rows = (idx == 0) ? g.nb : g.nd;
   std::vector<double2> half;
@@ -2567,18 +3006,23 @@ int32_t pcf_set_cotangent(pcf_handle *h, int32_t field, const double *host) {
 
 int32_t pcf_vjp(pcf_handle *h, int32_t steps, int32_t checkpoint_every, int32_t seed) {
   if (!h) return fail(PCF_E_INVALID, "null handle");
-  if (h->mhd) return fail(PCF_E_UNSUPPORTED, "the reverse mode covers the hydrodynamic step only");
+  if (h->mhd && h->insul)
+    return fail(PCF_E_UNSUPPORTED, "the MHD reverse mode covers conducting walls (A in TD^3) only");
+  if (h->mhd && h->cfg.precision != PCF_PREC_FP64)
+    return fail(PCF_E_UNSUPPORTED, "the MHD reverse mode runs in FP64");
   if (h->cfg.time_integrator != PCF_TI_SBDF3) return fail(PCF_E_UNSUPPORTED, "the reverse mode covers SBDF3 only");
   if (steps < 1) return fail(PCF_E_INVALID, "steps must be >= 1");
   if (checkpoint_every < 1) return fail(PCF_E_INVALID, "checkpoint_every must be >= 1");
   if (seed != PCF_SEED_ENERGY && seed != PCF_SEED_COTANGENT) return fail(PCF_E_INVALID, "unknown seed");
   if (seed == PCF_SEED_COTANGENT &&
-      !(h->adj.cot_set[0] || h->adj.cot_set[1] || h->adj.cot_set[2] || h->adj.cot_set[3]))
+      !(h->adj.cot_set[0] || h->adj.cot_set[1] || h->adj.cot_set[2] || h->adj.cot_set[3] || h->adj.cotA_set))
     return fail(PCF_E_INVALID, "no cotangent set (pcf_set_cotangent)");
   PCF_HIP(hipSetDevice(h->device));
   int32_t r;
   if ((r = adj_setup(h))) return r;
+  if (h->mhd && (r = adj_setup_mhd(h))) return r;
   if ((r = adj_reserve(h, steps, checkpoint_every))) return r;
+  if (h->mhd) return vjp_mhd(h, steps, checkpoint_every, seed);
   auto &A = h->adj;
   const pcf::Geo &g = h->geo;
   const int every = checkpoint_every;
@@ -2694,9 +3138,10 @@ int32_t pcf_vjp(pcf_handle *h, int32_t steps, int32_t checkpoint_every, int32_t 
 }
 
 int32_t pcf_get_gradient(pcf_handle *h, int32_t field, double *host) {
-  // the reverse mode differentiates with respect to the flow leaves only
-  if (field < 0 || field >= PCF_F_AX) return fail(PCF_E_INVALID, "no gradient for this field (flow leaves only)");
+  if (field < 0 || field >= PCF_F_COUNT) return fail(PCF_E_INVALID, "no such field");
   if (!h || !host) return fail(PCF_E_INVALID, "null argument");
+  // the vector-potential leaves have a gradient under MHD only
+  if (field >= PCF_F_AX && !h->mhd) return fail(PCF_E_INVALID, "no gradient for this field (flow leaves only)");
   auto &A = h->adj;
   if (!A.have_grad) return fail(PCF_E_INVALID, "no gradient: call pcf_vjp first");
   const pcf::Geo &g = h->geo;
@@ -2709,6 +3154,15 @@ int32_t pcf_get_gradient(pcf_handle *h, int32_t field, double *host) {
   };
   std::vector<double2> half, extra;
   int rows = g.nd;
+  if (field >= PCF_F_AX) {
+    // A, then the histories (NL_A_OLD, NL_A_OLDER, A_OLD, A_OLDER) of the initial state
+    const int grp = (field - PCF_F_AX) / 3, comp = (field - PCF_F_AX) % 3;
+    const double2 *src = (grp == 0) ? A.muA : (grp == 1) ? A.ghA[2] : (grp == 2) ? A.ghA[3] : (grp == 3) ? A.ghA[0]
+                                                                                                       : A.ghA[1];
+    PCF_HIP(fetch(src + comp * h->cstride, (size_t)rows * M, half));
+    full_from_half_grad(g, rows, half, host);
+    return PCF_OK;
+  }
   if (field == PCF_F_U0 || field == PCF_F_U1 || field == PCF_F_U2 || field == PCF_F_G) {
     if (field == PCF_F_U0) {
       rows = g.nb;

@@ -23,7 +23,10 @@ __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
   return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 
+// Streamwise wavenumber of row iy, zero on the Nyquist row (the step's
+// contractions use the same value for every wavenumber factor).
 __device__ __forceinline__ double ky_of(const Geo &g, int iy) {
+  if (g.ny % 2 == 0 && iy == g.ny / 2) return 0.0;
   return (double)(iy < (g.ny + 1) / 2 ? iy : iy - g.ny) * g.df1;
 }
 
@@ -58,7 +61,7 @@ __device__ double block_max256(double v, double *red) {
 }  // namespace
 
 // ------------------------------------------------------- k_eval_terms ----
-// out[p][m] = sum_t f_t(m) sum_i M_t[p][i] F_t[i][m], f_t = c, c i ky or c i kz.
+// out[p][m] = sum_t f_t(m) sum_i M_t[p][i] F_t[i][m], f_t a mode factor (EvalFactor).
 // One thread per (p, m): the matrix row is uniform across the block, the
 // field columns are read with consecutive modes on consecutive lanes.
 __global__ void __launch_bounds__(256) k_eval_terms(Geo g, EvalOut o) {
@@ -69,7 +72,7 @@ __global__ void __launch_bounds__(256) k_eval_terms(Geo g, EvalOut o) {
   const double ky = ky_of(g, iy), kzv = kz * g.df2;
   double2 acc = make_double2(0.0, 0.0);
 #pragma unroll
-  for (int t = 0; t < 3; ++t) {
+  for (int t = 0; t < EVAL_TERMS; ++t) {
     if (t >= o.nt) break;
     const EvalTerm &T = o.t[t];
     const double *row = T.M + (size_t)p * T.K;
@@ -80,13 +83,18 @@ __global__ void __launch_bounds__(256) k_eval_terms(Geo g, EvalOut o) {
       sr = fma(a, f.x, sr);
       si = fma(a, f.y, si);
     }
-    const double k = (T.kf == 1) ? ky : kzv;
-    if (T.kf == 0) {
-      acc.x += T.c * sr;
-      acc.y += T.c * si;
-    } else {  // c i k (sr + i si) = c k (-si + i sr)
+    if (T.kf == KF_IKY || T.kf == KF_IKZ) {  // c i k (sr + i si) = c k (-si + i sr)
+      const double k = (T.kf == KF_IKY) ? ky : kzv;
       acc.x -= T.c * k * si;
       acc.y += T.c * k * sr;
+    } else {
+      double f = 1.0;
+      if (T.kf == KF_KYY) f = ky * ky;
+      else if (T.kf == KF_KZZ) f = kzv * kzv;
+      else if (T.kf == KF_KYZ) f = ky * kzv;
+      else if (T.kf == KF_K2) f = ky * ky + kzv * kzv;
+      acc.x += T.c * f * sr;
+      acc.y += T.c * f * si;
     }
   }
   o.dst[(size_t)p * g.m + md] = acc;
@@ -311,9 +319,31 @@ __global__ void __launch_bounds__(256) k_parseval(Geo g, const ParsevalJob *__re
   if (threadIdx.x == 0) part[(size_t)j * g.n0 + x] = s;
 }
 
+// --------------------------------------------------------- k_seed_b ----
+// Cotangent of Emag = Ly Lz sum_x fejer_x sum_k |b(x, k)|^2 (full spectrum)
+// at the nodal fields X = (b, j) [6][n0][m]: mu_b = scale fejer_x b, mu_j = 0.
+__global__ void __launch_bounds__(256) k_seed_b(Geo g, const double *__restrict__ fejer, double scale,
+                                                double2 *__restrict__ X) {
+  const int x = blockIdx.y;
+  const int md = blockIdx.x * 256 + threadIdx.x;
+  if (md >= g.m) return;
+  const size_t pl = (size_t)g.n0 * g.m, at = (size_t)x * g.m + md;
+  const double w = scale * fejer[x];
+  for (int f = 0; f < 3; ++f) {
+    const double2 v = X[f * pl + at];
+    X[f * pl + at] = make_double2(w * v.x, w * v.y);
+    X[(3 + f) * pl + at] = make_double2(0.0, 0.0);
+  }
+}
+
 // ---------------------------------------------------------- launchers ----
+hipError_t launch_seed_b(const Geo &g, const double *fejer, double scale, double2 *X, hipStream_t s) {
+  hipLaunchKernelGGL(k_seed_b, dim3((g.m + 255) / 256, g.n0), dim3(256), 0, s, g, fejer, scale, X);
+  return hipGetLastError();
+}
+
 hipError_t launch_eval_terms(const Geo &g, const EvalOut &o, hipStream_t s) {
-  if (o.nt < 1 || o.nt > 3 || o.rows < 1) return hipErrorInvalidValue;
+  if (o.nt < 1 || o.nt > EVAL_TERMS || o.rows < 1) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k_eval_terms, dim3((g.m + 255) / 256, o.rows), dim3(256), 0, s, g, o);
   return hipGetLastError();
 }

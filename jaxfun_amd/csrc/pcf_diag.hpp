@@ -8,7 +8,9 @@
 namespace pcf {
 
 // One term of a nodal-spectral evaluation: f(m) * M F with M [rows][K]
-// row-major, F [K][m] and f = c (kf 0), c i ky (kf 1) or c i kz (kf 2).
+// row-major, F [K][m] and the mode factor f = c times 1, i ky, i kz, ky^2,
+// kz^2, ky kz or ky^2 + kz^2 (kf 0..6; the streamwise Nyquist wavenumber
+// counts as zero, as in the step's contractions).
 struct EvalTerm {
   const double *M = nullptr;
   const double2 *F = nullptr;
@@ -17,12 +19,15 @@ struct EvalTerm {
   double c = 1.0;
 };
 
-// out[p][m] = sum of up to three terms, p < rows.
+enum EvalFactor { KF_ONE = 0, KF_IKY = 1, KF_IKZ = 2, KF_KYY = 3, KF_KZZ = 4, KF_KYZ = 5, KF_K2 = 6 };
+
+// out[p][m] = sum of up to six terms, p < rows.
+constexpr int EVAL_TERMS = 6;
 struct EvalOut {
   double2 *dst = nullptr;
   int rows = 0;
   int nt = 0;
-  EvalTerm t[3];
+  EvalTerm t[EVAL_TERMS];
 };
 
 // Sums over the stored modes at one node (k_parseval); wt = 1 for kz = 0,
@@ -70,6 +75,8 @@ hipError_t launch_dft_z_r2c(const Geo &g, int planes, const double2 *tw, const d
                             hipStream_t s);
 hipError_t launch_emf(const Geo &g, int P, double eta, const double *xp, const double *wq, const double *phys,
                       double *E, double *part, hipStream_t s);
+// mu_b = scale fejer_x b (fields 0..2 of X = (b, j) [6][n0][m]), mu_j = 0.
+hipError_t launch_seed_b(const Geo &g, const double *fejer, double scale, double2 *X, hipStream_t s);
 hipError_t launch_parseval(const Geo &g, const ParsevalJob *jobs, int njobs, double *part, hipStream_t s);
 
 }  // namespace pcf

@@ -192,11 +192,55 @@ class PlaneCouetteMHDHIP(PlaneCouetteFluctuationHIP):
         out["Emag_total"] = float(m[2])
         return out
 
-    def vjp(self, *args, **kwargs):
-        raise NotImplementedError("the reverse mode covers the hydrodynamic step only")
+    def vjp(self, state: MHDState, steps: int, cotangent=None, checkpoint_every: int = 20):
+        """Run ``steps`` SBDF3 steps from ``state`` and pull a cotangent back to
+        every leaf of it (flow, A and their histories): the reverse mode of the
+        coupled flow / vector-potential step that the reference takes with
+        ``jax.value_and_grad`` (``production/autograd_cost_benchmark.py:56-61,228-330``
+        registers ``PlaneCouetteMRIShearpyJax`` with SBDF3).
 
-    def value_and_grad(self, *args, **kwargs):
-        raise NotImplementedError("the reverse mode covers the hydrodynamic step only")
+        ``cotangent`` is ``((u0, u1, u2, g), (Ax, Ay, Az))`` of the final state
+        in jax's convention (directional derivative ``Re sum(c * d)``); ``None``
+        differentiates Epert + Emag of the final state.  Conducting walls
+        (A in TD^3), FP64.  Returns ``(final_state, gradient)`` with the
+        gradient an ``MHDState``."""
+        if self.time_integrator != "SBDF3":
+            raise NotImplementedError("the reverse mode covers the SBDF3 step only")
+        if self.precision != "fp64":
+            raise NotImplementedError("the MHD reverse mode runs in FP64")
+        self._ensure_loaded(state)
+        self._loaded = None
+        seed = nat.SEED_ENERGY
+        if cotangent is not None:
+            flow_cot, a_cot = cotangent
+            for name, arr in list(zip(("U0", "U1", "U2", "G"), flow_cot)) + list(zip(("AX", "AY", "AZ"), a_cot)):
+                a = np.ascontiguousarray(np.asarray(arr, dtype=np.complex128))
+                nat.check(self._lib.pcf_set_cotangent(self._h, nat.FIELD_ID[name],
+                                                      a.ctypes.data_as(C.POINTER(C.c_double))))
+            seed = nat.SEED_COTANGENT
+        nat.check(self._lib.pcf_vjp(self._h, int(steps), int(checkpoint_every), seed))
+        grad = self._mhd_gradient()
+        return self.fetch(), grad
+
+    def value_and_grad(self, state: MHDState, steps: int, checkpoint_every: int = 20):
+        """Final Epert + Emag and its gradient with respect to the leaves of ``state``."""
+        final, grad = self.vjp(state, steps, None, checkpoint_every)
+        d = PlaneCouetteMHDHIP.diagnostics(self)
+        return d["Epert"] + d["Emag"], grad, final
+
+    def _mhd_gradient(self) -> MHDState:
+        flow = self._gradient()
+        groups = {}
+        for name, tag in _A_GROUPS:
+            out = []
+            for i, c in enumerate("XYZ"):
+                arr = np.empty((self._a_rows(i), self.N[1], self.N[2]), np.complex128)
+                field = f"A{c}" if not tag else f"{tag}{c}"
+                nat.check(self._lib.pcf_get_gradient(self._h, nat.FIELD_ID[field],
+                                                     arr.ctypes.data_as(C.POINTER(C.c_double))))
+                out.append(arr)
+            groups[name] = tuple(out)
+        return MHDState(flow=flow, **groups)
 
     @staticmethod
     def benchmark_state_fields(state: MHDState):
@@ -392,6 +436,12 @@ class PlaneCouetteMRIShearpyInsulatingHIP(PlaneCouetteMRIShearpyHIP):
 
     def _a_rows(self, comp: int) -> int:
         return self.nd if comp == 0 else self.N[0]
+
+    def vjp(self, *args, **kwargs):
+        raise NotImplementedError("the MHD reverse mode covers conducting walls (A in TD^3) only")
+
+    def value_and_grad(self, *args, **kwargs):
+        raise NotImplementedError("the MHD reverse mode covers conducting walls (A in TD^3) only")
 
     def _a_projection(self, comp: int) -> np.ndarray:
         """TD projection for A_x; TC coefficients (scalar products / h) for A_y, A_z."""
