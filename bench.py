@@ -515,6 +515,54 @@ def reverse_leg(precision: str, nsteps: int = 40, reps: int = 3) -> dict:
             "method": "HIP events on the reverse stream around nsteps-1 reverse steady steps (pcf_profile_reverse)"}
 
 
+ADJOINT_MHD = dict(N=(65, 64, 64), Re=1000.0, Rm=1000.0, omega=2.0 / 3.0, shear_rate=1.0,
+                   background_b=(0.0, 0.0, 0.025), domain=((-1.0, 1.0), (0.0, 4.0), (0.0, 1.0)), dt=0.005,
+                   energy_u=1e-3, energy_b=1e-3, K=12, seed=0, steps=100, every=20)
+
+
+def adjoint_mhd_leg(reps: int = 3) -> dict:
+    """Value + gradient of Epert + Emag over ADJOINT_MHD['steps'] steps of the
+    conducting-wall rotating-shear MHD step (the reference's ``mhd_mri_pcf``
+    autograd case: Re = Rm = 1000, Omega = 2/3, S = 1, B0 = (0, 0, 0.025),
+    4 x 1 box), FP64, checkpoints every ADJOINT_MHD['every'] steps, against the
+    same forward step sequence, state resident."""
+    from jaxfun_amd import PlaneCouetteMRIShearpyHIP
+    from jaxfun_amd import _native as nat
+    from jaxfun_amd.ic import seeded_mhd_state
+
+    a = ADJOINT_MHD
+    solver = PlaneCouetteMRIShearpyHIP(N=a["N"], domain=a["domain"], Re=a["Re"], Rm=a["Rm"], omega=a["omega"],
+                                       shear_rate=a["shear_rate"], background_b=a["background_b"], dt=a["dt"],
+                                       **SBDF3_ROT)
+    s0 = seeded_mhd_state(solver, seed=a["seed"], energy_u=a["energy_u"], energy_b=a["energy_b"], K=a["K"])
+    lib, h = solver._lib, solver._h
+
+    def fwd():
+        solver.load(s0)
+        solver.synchronize()
+        t0 = time.perf_counter()
+        solver.advance(a["steps"])
+        solver.synchronize()
+        return time.perf_counter() - t0
+
+    def adj():
+        solver.load(s0)
+        solver.synchronize()
+        t0 = time.perf_counter()
+        nat.check(lib.pcf_vjp(h, a["steps"], a["every"], nat.SEED_ENERGY))
+        return time.perf_counter() - t0
+
+    fwd()
+    adj()
+    tf = sorted(fwd() for _ in range(reps))[reps // 2]
+    ta = sorted(adj() for _ in range(reps))[reps // 2]
+    solver.close()
+    return {"workload": f"grad of Epert + Emag after {a['steps']} steps wrt the seeded initial state (flow, A), "
+                        "conducting-wall rotating-shear MHD 65x64x64, Re=Rm=1000, Omega=2/3, S=1, B0=(0,0,0.025), "
+                        f"dt=0.005, checkpoints every {a['every']}, FP64",
+            "forward_ms": 1e3 * tf, "value_and_grad_ms": 1e3 * ta, "ratio": ta / tf}
+
+
 def adjoint_fd_check(n_dirs: int = 8, eps: float = 1e-6) -> dict:
     """FP64 gradient against central differences of the HIP forward along
     seeded divergence-free directions (same energy as the initial field), and
@@ -665,6 +713,8 @@ def main(argv=None) -> int:
     ap.add_argument("--no-mhd", action="store_true")
     ap.add_argument("--adjoint-only", action="store_true",
                     help="run only the adjoint timing leg (for kernel profiles)")
+    ap.add_argument("--mhd-adjoint-only", action="store_true",
+                    help="run only the MHD value-and-gradient timing (for kernel profiles)")
     ap.add_argument("--reverse-only", action="store_true",
                     help="run only the reverse-step timing (pcf_profile_reverse; for kernel profiles)")
     ap.add_argument("--no-published", action="store_true",
@@ -687,6 +737,9 @@ def main(argv=None) -> int:
         return 0
     if args.reverse_only:
         print(json.dumps({"reverse": {p: reverse_leg(p) for p in ("fp64", "mixed")}}), flush=True)
+        return 0
+    if args.mhd_adjoint_only:
+        print(json.dumps({"adjoint_mhd": adjoint_mhd_leg()}), flush=True)
         return 0
     if args.steps < 1 or args.warmup < 0:
         print("--steps must be >= 1 and --warmup >= 0", file=sys.stderr)
@@ -749,6 +802,7 @@ def main(argv=None) -> int:
                "mixed": adjoint_leg("mixed"), "fp64": adjoint_leg("fp64")}
         adj["fd_check"] = adjoint_fd_check()
         adj["reverse_step"] = {p: reverse_leg(p) for p in ("fp64", "mixed")}
+        adj["mhd"] = adjoint_mhd_leg()
         line["adjoint"] = adj
         line["adjoint_forward_ratio"] = adj["mixed"]["ratio"]
         print(f"# adjoint/forward: mixed {adj['mixed']['ratio']:.3f}, fp64 {adj['fp64']['ratio']:.3f}; "

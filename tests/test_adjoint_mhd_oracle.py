@@ -155,8 +155,9 @@ def test_gradient_matches_central_differences(steps, history):
         assert abs(fd - an) <= 2e-9 * abs(an), (fd, an)
 
 
-def test_planted_sign_slip_in_the_emf_adjoint_fails():
-    """A copy of the adjoint with the EMF's cotangent sign flipped misses the
+def test_planted_sign_slip_in_the_induction_adjoint_fails():
+    """A copy of the adjoint with the sign of the A mass adjoint flipped (the
+    map that carries the induction row's cotangent back to A) misses the
     central differences: the check sees the induction path."""
     o = _oracle()
     rng = np.random.default_rng(7)
