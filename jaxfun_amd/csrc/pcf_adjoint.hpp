@@ -63,6 +63,12 @@ hipError_t launch_adj_solve_factored(const Geo &g, const StructMats &smt, const 
                                      Prim out, Prim nu, PrimC r1, PrimC r2, double e0, double e1, double e2,
                                      hipStream_t s, Prim sig = Prim{}, double c0 = 0, double c1 = 0,
                                      double c2 = 0);
+// The same; ncomp > 0: the Dirichlet blocks only, for ncomp fields whose g
+// pointers (in, out, nu, r1, r2, sig) advance by cstride per field.
+hipError_t launch_adj_solve_factored_dir(const Geo &g, const StructMats &smt, const FactorSet &fs, PrimC in,
+                                         Prim out, Prim nu, PrimC r1, PrimC r2, double e0, double e1, double e2,
+                                         hipStream_t s, Prim sig, double c0, double c1, double c2, int ncomp,
+                                         size_t cstride);
 // out += (M + gamma L)^T r_stage (if r_stage.u0) + M^T (sum of gather terms).
 hipError_t launch_adj_mass(const Geo &g, const StructMats &sm, double gamma, PrimC r_stage,
                            const PrimSum &gather, Prim out, hipStream_t s);

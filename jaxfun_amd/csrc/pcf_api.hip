@@ -271,6 +271,7 @@ struct pcf_handle {
     double *scratchA = nullptr;
     double *mEWAG = nullptr;      // W AG^T (n0 x nd): cotangent of the nodal EMF from mu_NA
     double *mVDT = nullptr, *mVD1T = nullptr, *mVQ1T = nullptr, *mVQ2T = nullptr;  // (nd x n0) transposes
+    double *mMAT = nullptr;       // W C0D^T (nd x nd): the A mass transposed
   } adj;
   std::vector<void *> adj_allocs;  // freed on destroy or when the tape grows
 
@@ -1571,8 +1572,11 @@ int32_t adj_setup_mhd(pcf_handle *h) {
       vq1t[(size_t)i * n0 + x] = o.VQ1(x, i);
       vq2t[(size_t)i * n0 + x] = o.VQ2(x, i);
     }
+  std::vector<double> mat((size_t)nd * nd);
+  for (int i = 0; i < nd; ++i)
+    for (int j = 0; j < nd; ++j) mat[(size_t)i * nd + j] = g.W * o.C0D(j, i);
   if ((r = h->upload(&A.mEWAG, ewag)) || (r = h->upload(&A.mVDT, vdt)) || (r = h->upload(&A.mVD1T, vd1t)) ||
-      (r = h->upload(&A.mVQ1T, vq1t)) || (r = h->upload(&A.mVQ2T, vq2t)))
+      (r = h->upload(&A.mVQ1T, vq1t)) || (r = h->upload(&A.mVQ2T, vq2t)) || (r = h->upload(&A.mMAT, mat)))
     return r;
   A.mhd_ready = true;
   return PCF_OK;
@@ -1590,8 +1594,9 @@ pcf::EvalTerm eterm(const double *M, const double2 *F, int K, int kf, double c) 
 
 // mu_A = (b, j map)^H (mu_b, mu_j): the transpose of the nodal b and j of
 // k_nodal_mhd_mfma (conducting walls), from nodal cotangents bj = [6][n0][m]
-// (bx, by, bz, jx, jy, jz) into out (three components cstride apart).
-hipError_t a_nodal_adj(pcf_handle *h, const double2 *bj, double2 *out) {
+// (bx, by, bz, jx, jy, jz) into out (three components cstride apart); with
+// sig given, plus M_A^T sig (the SBDF3 history gather of a steady step).
+hipError_t a_nodal_adj(pcf_handle *h, const double2 *bj, double2 *out, const double2 *sig = nullptr) {
   const auto &A = h->adj;
   const pcf::Geo &g = h->geo;
   const size_t pl = (size_t)g.n0 * g.m, cs = h->cstride;
@@ -1615,10 +1620,9 @@ hipError_t a_nodal_adj(pcf_handle *h, const double2 *bj, double2 *out) {
     o[c].rows = g.nd;
     o[c].nt = nt[c];
     for (int t = 0; t < nt[c]; ++t) o[c].t[t] = ts[c][t];
-    hipError_t e = launch_eval_terms(g, o[c], A.stream);
-    if (e != hipSuccess) return e;
+    if (sig) o[c].t[o[c].nt++] = eterm(A.mMAT, sig + c * cs, g.nd, KF_ONE, 1.0);
   }
-  return hipSuccess;
+  return launch_eval_terms(g, o, 3, A.stream);
 }
 
 // Cotangents of u (and curl u), b and j of the coupled nonlinear rows from
@@ -1626,21 +1630,21 @@ hipError_t a_nodal_adj(pcf_handle *h, const double2 *bj, double2 *out) {
 // padded y-line fields (u, omega, b, j) are ytape: the flow part into
 // adj.muX (steady_x: with the reconstruction adjoint and M^T sig folded in)
 // or adj.mu_u, the vector-potential part into adj.muA (overwritten).
-hipError_t nl_adj_mhd(pcf_handle *h, const void *ytape, bool steady_x) {
+hipError_t nl_adj_mhd(pcf_handle *h, const void *ytape, bool steady_x, const double2 *sigA = nullptr) {
   auto &A = h->adj;
   const pcf::Geo &g = h->geo;
   const size_t pl = (size_t)g.n0 * g.m;
   hipError_t e;
   if ((e = pcf::launch_rhs_adj_mfma<double>(g, A.ad, A.nu, A.ws, A.stream))) return e;
   double2 *Hx = reinterpret_cast<double2 *>(A.ws.Hx);
+  pcf::EvalOut o[3];
   for (int c = 0; c < 3; ++c) {
-    pcf::EvalOut o;
-    o.dst = Hx + (3 + c) * pl;
-    o.rows = g.n0;
-    o.nt = 1;
-    o.t[0] = eterm(A.mEWAG, A.nuA + c * h->cstride, g.nd, pcf::KF_ONE, 1.0);
-    if ((e = pcf::launch_eval_terms(g, o, A.stream))) return e;
+    o[c].dst = Hx + (3 + c) * pl;
+    o[c].rows = g.n0;
+    o[c].nt = 1;
+    o[c].t[0] = eterm(A.mEWAG, A.nuA + c * h->cstride, g.nd, pcf::KF_ONE, 1.0);
   }
+  if ((e = pcf::launch_eval_terms(g, o, 3, A.stream))) return e;
   if ((e = pcf::launch_fft_y_fields<double>(g, h->md.view, h->py, true, 6, A.ws.Hx, A.ws.Hz, A.stream))) return e;
   if ((e = pcf::launch_rot_adj_mhd<double>(g, h->md.view, ytape, A.ws, h->pz, A.stream))) return e;
   if ((e = pcf::launch_fft_y_fields<double>(g, h->md.view, h->py, false, 12, A.ws.Y, A.ws.X, A.stream))) return e;
@@ -1649,7 +1653,7 @@ hipError_t nl_adj_mhd(pcf_handle *h, const void *ytape, bool steady_x) {
   else
     e = pcf::launch_nodal_adj_mfma<double>(g, A.ad, A.ws, A.mu_u[0], A.mu_u[1], A.mu_u[2], A.stream);
   if (e != hipSuccess) return e;
-  return a_nodal_adj(h, reinterpret_cast<const double2 *>(A.ws.X) + 6 * pl, A.muA);
+  return a_nodal_adj(h, reinterpret_cast<const double2 *>(A.ws.X) + 6 * pl, A.muA, sigA);
 }
 
 // Transposed A solves of the steady systems: out = S_A^{-T} in with
@@ -1657,15 +1661,9 @@ hipError_t nl_adj_mhd(pcf_handle *h, const void *ytape, bool steady_x) {
 hipError_t a_solve_factored(pcf_handle *h, const double2 *in, double2 *out, const double2 *r1, const double2 *r2,
                             const double e[3]) {
   auto &A = h->adj;
-  for (int c = 0; c < 3; ++c) {
-    const size_t o = c * h->cstride;
-    hipError_t er = pcf::launch_adj_solve_factored(h->geo_eta, A.smt, A.fA_t, a_in(h, in + o), a_out(h, out + o),
-                                                   a_out(h, A.nuA + o), a_in(h, r1 + o), a_in(h, r2 + o), e[0],
-                                                   e[1], e[2], A.stream, a_out(h, A.sigA + o), C_SBDF[0],
-                                                   C_SBDF[1], C_SBDF[2]);
-    if (er != hipSuccess) return er;
-  }
-  return hipSuccess;
+  return pcf::launch_adj_solve_factored_dir(h->geo_eta, A.smt, A.fA_t, a_in(h, in), a_out(h, out), a_out(h, A.nuA),
+                                            a_in(h, r1), a_in(h, r2), e[0], e[1], e[2], A.stream,
+                                            a_out(h, A.sigA), C_SBDF[0], C_SBDF[1], C_SBDF[2], 3, h->cstride);
 }
 
 // out = (M_A - gamma L_A)^{-T} in (start-up stages, factorised per launch).
@@ -1747,16 +1745,15 @@ hipError_t reverse_step_mhd(pcf_handle *h, int k, bool steady, void *const *tape
                                              A.stream, A.sig, C_SBDF[0], C_SBDF[1], C_SBDF[2])))
       return er;
     if ((er = a_solve_factored(h, A.muA, A.rhoA[rr[0]], A.rhoA[rr[1]], A.rhoA[rr[2]], e))) return er;
+    // mu_A = (b, j map)^H mu_bj + M_A^T sig_A in one evaluation
     if (k == 0) {  // initial state: its velocity leaves keep mu_u
-      if ((er = nl_adj_mhd(h, tape[0], false))) return er;
+      if ((er = nl_adj_mhd(h, tape[0], false, A.sigA))) return er;
       pcf::PrimSum ga{};
       for (int j = 0; j < 3; ++j) ps_add(ga, A.rho[rr[j]], C_SBDF[j]);
       if ((er = zero_prim(h, A.muX))) return er;
-      if ((er = pcf::launch_adj_mass(g, h->sm, 0.0, pcf::PrimC{}, ga, A.muX, A.stream))) return er;
-    } else if ((er = nl_adj_mhd(h, tape[0], true))) {
-      return er;
+      return pcf::launch_adj_mass(g, h->sm, 0.0, pcf::PrimC{}, ga, A.muX, A.stream);
     }
-    return a_mass(h, 0.0, nullptr, {{A.sigA, 1.0}}, A.muA);
+    return nl_adj_mhd(h, tape[0], true, A.sigA);
   }
   if ((er = zero_prim(h, A.rho[rr[0]])) || (er = a_zero(h, A.rhoA[rr[0]]))) return er;
   for (int s = 2; s >= 0; --s) {
@@ -2716,9 +2713,7 @@ int32_t pcf_mhd_extended_diagnostics(pcf_handle *h, double *out) {
       out_of(XD_A1, P, {T(EA0, A1, ka)}),
       out_of(XD_A2, P, {T(EA0, A2, ka)}),
   };
-  hipError_t e = hipSuccess;
-  for (const auto &o : outs)
-    if (e == hipSuccess) e = pcf::launch_eval_terms(g, o, h->stream);
+  hipError_t e = pcf::launch_eval_terms(g, outs.data(), (int)outs.size(), h->stream);
   if (e == hipSuccess) e = pcf::launch_curl_b(g, P, X.NS + XD_B * fs, X.NS + XD_DB * fs, X.NS + XD_J * fs, h->stream);
   // u, b, J, div A on the physical grid at all P rows
   if (e == hipSuccess) e = pcf::launch_dft_y(g, true, 10 * P, X.twy, X.NS, X.G, h->stream);
@@ -2728,17 +2723,18 @@ int32_t pcf_mhd_extended_diagnostics(pcf_handle *h, double *out) {
   // and their divergence at the nodes
   if (e == hipSuccess) e = pcf::launch_dft_z_r2c(g, 6 * n0, X.twz, X.E, X.EH, h->stream);
   if (e == hipSuccess) e = pcf::launch_dft_y(g, false, 6 * n0, X.twy, X.EH, X.EC, h->stream);
-  for (int part = 0; part < 2 && e == hipSuccess; ++part) {
+  pcf::EvalOut dv[2];
+  for (int part = 0; part < 2; ++part) {
     const double2 *Ex = X.EC + (size_t)(3 * part) * n0 * m;
-    pcf::EvalOut o;
+    pcf::EvalOut &o = dv[part];
     o.dst = X.DIV + (size_t)part * n0 * m;
     o.rows = n0;
     o.nt = 3;
     o.t[0] = T(X.Dn, Ex, n0);
     o.t[1] = T(X.I, Ex + (size_t)n0 * m, n0, 1);
     o.t[2] = T(X.I, Ex + (size_t)2 * n0 * m, n0, 2);
-    e = pcf::launch_eval_terms(g, o, h->stream);
   }
+  if (e == hipSuccess) e = pcf::launch_eval_terms(g, dv, 2, h->stream);
   if (e == hipSuccess) e = pcf::launch_parseval(g, X.jobs, X.njobs, X.part, h->stream);
   if (e != hipSuccess) return fail(PCF_E_HIP, std::string("extended diagnostics launch: ") + hipGetErrorString(e));
   std::vector<double> part((size_t)X.njobs * n0), emf((size_t)P * pcf::EMF_Q);

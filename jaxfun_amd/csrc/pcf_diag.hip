@@ -15,6 +15,8 @@
 // Everything runs in FP64: the state is FP64 in every precision mode.
 #include "pcf_diag.hpp"
 
+#include <algorithm>
+
 namespace pcf {
 
 namespace {
@@ -61,43 +63,86 @@ __device__ double block_max256(double v, double *red) {
 }  // namespace
 
 // ------------------------------------------------------- k_eval_terms ----
-// out[p][m] = sum_t f_t(m) sum_i M_t[p][i] F_t[i][m], f_t a mode factor (EvalFactor).
-// One thread per (p, m): the matrix row is uniform across the block, the
-// field columns are read with consecutive modes on consecutive lanes.
-__global__ void __launch_bounds__(256) k_eval_terms(Geo g, EvalOut o) {
-  const int p = blockIdx.y;
-  const int md = blockIdx.x * 256 + threadIdx.x;
-  if (md >= g.m || p >= o.rows) return;
-  const int iy = md / g.kzs, kz = md - iy * g.kzs;
+// out[p][m] = sum_t f_t(m) sum_i M_t[p][i] F_t[i][m], f_t a mode factor
+// (EvalFactor), for blockIdx.z's output of the batch.  Register tile: a block
+// covers 64 consecutive modes (one per lane) and 4 EVAL_RW rows, each of its
+// four waves EVAL_RW consecutive rows, so every field value a lane loads
+// serves EVAL_RW rows and the matrix entries are wave-uniform loads.  The
+// field is read once per row tile instead of once per row.
+constexpr int EVAL_RW = 4;
+
+__global__ void __launch_bounds__(256) k_eval_terms(Geo g, EvalBatch b) {
+  const EvalOut &o = b.o[blockIdx.z];
+  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int p0 = (blockIdx.y * 4 + wave) * EVAL_RW;
+  if (p0 >= o.rows) return;
+  const int nr = min(EVAL_RW, o.rows - p0);
+  const int md = blockIdx.x * 64 + (int)(threadIdx.x & 63);
+  const bool live = md < g.m;
+  const int mc = live ? md : g.m - 1;  // lanes past the last mode load a valid column
+  const int iy = mc / g.kzs, kz = mc - iy * g.kzs;
   const double ky = ky_of(g, iy), kzv = kz * g.df2;
-  double2 acc = make_double2(0.0, 0.0);
+  double ar[EVAL_RW], ai[EVAL_RW];
 #pragma unroll
-  for (int t = 0; t < EVAL_TERMS; ++t) {
-    if (t >= o.nt) break;
+  for (int r = 0; r < EVAL_RW; ++r) ar[r] = ai[r] = 0.0;
+  for (int t = 0; t < o.nt; ++t) {
     const EvalTerm &T = o.t[t];
-    const double *row = T.M + (size_t)p * T.K;
-    double sr = 0.0, si = 0.0;
-    for (int i = 0; i < T.K; ++i) {
-      const double a = row[i];
-      const double2 f = T.F[(size_t)i * g.m + md];
-      sr = fma(a, f.x, sr);
-      si = fma(a, f.y, si);
+    const int K = T.K;
+    const double *Mr[EVAL_RW];
+#pragma unroll
+    for (int r = 0; r < EVAL_RW; ++r) Mr[r] = T.M + (size_t)(p0 + min(r, nr - 1)) * K;  // rows past nr: not stored
+    const double2 *F = T.F + mc;
+    double sr[EVAL_RW], si[EVAL_RW];
+#pragma unroll
+    for (int r = 0; r < EVAL_RW; ++r) sr[r] = si[r] = 0.0;
+    int i = 0;
+    for (; i + 4 <= K; i += 4) {
+      double2 f[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) f[u] = F[(size_t)(i + u) * g.m];
+#pragma unroll
+      for (int r = 0; r < EVAL_RW; ++r)
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const double a = Mr[r][i + u];
+          sr[r] = fma(a, f[u].x, sr[r]);
+          si[r] = fma(a, f[u].y, si[r]);
+        }
+    }
+    for (; i < K; ++i) {
+      const double2 f = F[(size_t)i * g.m];
+#pragma unroll
+      for (int r = 0; r < EVAL_RW; ++r) {
+        const double a = Mr[r][i];
+        sr[r] = fma(a, f.x, sr[r]);
+        si[r] = fma(a, f.y, si[r]);
+      }
     }
     if (T.kf == KF_IKY || T.kf == KF_IKZ) {  // c i k (sr + i si) = c k (-si + i sr)
-      const double k = (T.kf == KF_IKY) ? ky : kzv;
-      acc.x -= T.c * k * si;
-      acc.y += T.c * k * sr;
+      const double ck = T.c * ((T.kf == KF_IKY) ? ky : kzv);
+#pragma unroll
+      for (int r = 0; r < EVAL_RW; ++r) {
+        ar[r] -= ck * si[r];
+        ai[r] += ck * sr[r];
+      }
     } else {
       double f = 1.0;
       if (T.kf == KF_KYY) f = ky * ky;
       else if (T.kf == KF_KZZ) f = kzv * kzv;
       else if (T.kf == KF_KYZ) f = ky * kzv;
       else if (T.kf == KF_K2) f = ky * ky + kzv * kzv;
-      acc.x += T.c * f * sr;
-      acc.y += T.c * f * si;
+      const double cf = T.c * f;
+#pragma unroll
+      for (int r = 0; r < EVAL_RW; ++r) {
+        ar[r] += cf * sr[r];
+        ai[r] += cf * si[r];
+      }
     }
   }
-  o.dst[(size_t)p * g.m + md] = acc;
+  if (!live) return;
+#pragma unroll
+  for (int r = 0; r < EVAL_RW; ++r)
+    if (r < nr) o.dst[(size_t)(p0 + r) * g.m + md] = make_double2(ar[r], ai[r]);
 }
 
 // ---------------------------------------------------------- k_curl_b ----
@@ -342,10 +387,25 @@ hipError_t launch_seed_b(const Geo &g, const double *fejer, double scale, double
   return hipGetLastError();
 }
 
-hipError_t launch_eval_terms(const Geo &g, const EvalOut &o, hipStream_t s) {
-  if (o.nt < 1 || o.nt > EVAL_TERMS || o.rows < 1) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_eval_terms, dim3((g.m + 255) / 256, o.rows), dim3(256), 0, s, g, o);
-  return hipGetLastError();
+hipError_t launch_eval_terms(const Geo &g, const EvalOut *o, int n, hipStream_t s) {
+  for (int b0 = 0; b0 < n; b0 += EVAL_BATCH) {
+    EvalBatch b;
+    b.n = std::min(EVAL_BATCH, n - b0);
+    int rows = 0;
+    for (int j = 0; j < b.n; ++j) {
+      const EvalOut &e = o[b0 + j];
+      if (!e.dst || e.rows < 1 || e.nt < 1 || e.nt > EVAL_TERMS) return hipErrorInvalidValue;
+      for (int t = 0; t < e.nt; ++t)
+        if (!e.t[t].M || !e.t[t].F || e.t[t].K < 1) return hipErrorInvalidValue;
+      b.o[j] = e;
+      rows = std::max(rows, e.rows);
+    }
+    const int tile = 4 * EVAL_RW;
+    hipLaunchKernelGGL(k_eval_terms, dim3((g.m + 63) / 64, (rows + tile - 1) / tile, b.n), dim3(256), 0, s, g, b);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
 }
 
 hipError_t launch_curl_b(const Geo &g, int P, const double2 *b, const double2 *db, double2 *J, hipStream_t s) {

@@ -21,13 +21,20 @@ struct EvalTerm {
 
 enum EvalFactor { KF_ONE = 0, KF_IKY = 1, KF_IKZ = 2, KF_KYY = 3, KF_KZZ = 4, KF_KYZ = 5, KF_K2 = 6 };
 
-// out[p][m] = sum of up to six terms, p < rows.
-constexpr int EVAL_TERMS = 6;
+// out[p][m] = sum of up to eight terms, p < rows.
+constexpr int EVAL_TERMS = 8;
 struct EvalOut {
   double2 *dst = nullptr;
   int rows = 0;
   int nt = 0;
   EvalTerm t[EVAL_TERMS];
+};
+
+// Outputs evaluated by one launch (gridDim.z).
+constexpr int EVAL_BATCH = 4;
+struct EvalBatch {
+  int n = 0;
+  EvalOut o[EVAL_BATCH];
 };
 
 // Sums over the stored modes at one node (k_parseval); wt = 1 for kz = 0,
@@ -65,7 +72,8 @@ enum EmfSlot {
   EMF_Q = 20,
 };
 
-hipError_t launch_eval_terms(const Geo &g, const EvalOut &o, hipStream_t s);
+// The n outputs of o, EVAL_BATCH per launch.
+hipError_t launch_eval_terms(const Geo &g, const EvalOut *o, int n, hipStream_t s);
 hipError_t launch_curl_b(const Geo &g, int P, const double2 *b, const double2 *db, double2 *J, hipStream_t s);
 hipError_t launch_dft_y(const Geo &g, bool inverse, int planes, const double2 *tw, const double2 *in, double2 *out,
                         hipStream_t s);

@@ -435,19 +435,32 @@ __global__ void __launch_bounds__(256) k_solve_factored(Geo g, StructMats sm, Fa
 // mean system in lane SL), as k_solve_factored: phase 0 stages the factors and
 // the right-hand side rows (reversed) into LDS, phase 1 runs one substitution
 // per lane out of LDS, phase 2 writes rho and nu with coalesced stores.
+// spb0 = 2: the Dirichlet blocks only, without the mean system, for
+// gridDim.y fields whose g pointers advance by cstride per field (the three
+// vector-potential components).
 template <int SL>
 __global__ void __launch_bounds__(256) k_adj_solve_factored(Geo g, StructMats smt, FactorSet fs, PrimC in,
                                                             Prim out, Prim nuo, PrimC r1, PrimC r2,
                                                             double e0, double e1, double e2, Prim sg,
-                                                            double c0, double c1, double c2) {
+                                                            double c0, double c1, double c2, int spb0,
+                                                            size_t cstride) {
   static_assert(2 * SL <= 64, "the substitution threads of the modes fill at most the first wave");
   extern __shared__ __align__(16) unsigned char smem[];
   constexpr int SLP = SL + 1;
   const int M = g.m;
   const int bps = (M + SL - 1) / SL;
-  const int spb = blockIdx.x / bps;
-  const int m0 = (blockIdx.x - spb * bps) * SL;
-  const bool has_mean = (spb >= 2) && (m0 == 0);
+  const int spb = blockIdx.x / bps + spb0;
+  const int m0 = (blockIdx.x - (spb - spb0) * bps) * SL;
+  const bool has_mean = (spb0 == 0) && (spb >= 2) && (m0 == 0);
+  if (blockIdx.y) {
+    const size_t off = blockIdx.y * cstride;
+    in.g += off;
+    out.g += off;
+    nuo.g += off;
+    r1.g += off;
+    r2.g += off;
+    if (sg.g) sg.g += off;
+  }
   const int sys = spb >> 1, par = spb & 1;
   const StructSys &ss = smt.sys[sys][par];
   const int n = ss.n;
@@ -600,10 +613,18 @@ hipError_t launch_solve_factored(const Geo &g, const StructMats &sm, const Facto
 hipError_t launch_adj_solve_factored(const Geo &g, const StructMats &smt, const FactorSet &fs, PrimC in,
                                      Prim out, Prim nu, PrimC r1, PrimC r2, double e0, double e1, double e2,
                                      hipStream_t s, Prim sig, double c0, double c1, double c2) {
+  return launch_adj_solve_factored_dir(g, smt, fs, in, out, nu, r1, r2, e0, e1, e2, s, sig, c0, c1, c2, 0, 0);
+}
+
+hipError_t launch_adj_solve_factored_dir(const Geo &g, const StructMats &smt, const FactorSet &fs, PrimC in,
+                                         Prim out, Prim nu, PrimC r1, PrimC r2, double e0, double e1, double e2,
+                                         hipStream_t s, Prim sig, double c0, double c1, double c2, int ncomp,
+                                         size_t cstride) {
+  const int spb0 = ncomp > 0 ? 2 : 0, nf = ncomp > 0 ? ncomp : 1;
   auto lds = [&](int sl) { return (size_t)g.npmax * (2 + NF_B) * (sl + 1) * sizeof(double); };
-#define PCF_ASOLVE(SLV)                                                                                    \
-  hipLaunchKernelGGL(k_adj_solve_factored<SLV>, dim3(4 * ((g.m + SLV - 1) / SLV)), dim3(256), lds(SLV), s, \
-                     g, smt, fs, in, out, nu, r1, r2, e0, e1, e2, sig, c0, c1, c2)
+#define PCF_ASOLVE(SLV)                                                                                  \
+  hipLaunchKernelGGL(k_adj_solve_factored<SLV>, dim3((4 - spb0) * ((g.m + SLV - 1) / SLV), nf), dim3(256), \
+                     lds(SLV), s, g, smt, fs, in, out, nu, r1, r2, e0, e1, e2, sig, c0, c1, c2, spb0, cstride)
   if (lds(16) <= 65536) PCF_ASOLVE(16);
   else if (lds(8) <= 65536) PCF_ASOLVE(8);
   else return hipErrorInvalidValue;
