@@ -15,6 +15,8 @@
 #include "pcf_adjoint.hpp"
 #include "pcf_device.hpp"
 
+#include <algorithm>
+
 namespace pcf {
 namespace {
 
@@ -175,18 +177,39 @@ __device__ __forceinline__ SysLane sys_lane(const Geo &g, int t) {
 
 // rho = S^{-T} mu, S = M - gamma L, on the transposed index-reversed blocks.
 // With nu.u0 set, also nu = e0 rho + e1 r1 + e2 r2 (the cotangent of the
-// step's nonlinear rows, r1/r2 the two later rho).
+// step's nonlinear rows, r1/r2 the two later rho).  ncomp > 0: the Dirichlet
+// blocks only, for gridDim.y fields whose g pointers advance by cstride per
+// field, each with its own two parity slices of scratch.
 __global__ void __launch_bounds__(256) k_adj_solve(Geo g, StructMats smt, double gamma, PrimC in,
                                                    Prim out, double *scratch, Prim nuo, PrimC r1,
-                                                   PrimC r2, double e0, double e1, double e2) {
-  const SysLane L = sys_lane(g, blockIdx.x * blockDim.x + threadIdx.x);
+                                                   PrimC r2, double e0, double e1, double e2, int ncomp,
+                                                   size_t cstride) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  SysLane L{};
+  if (ncomp > 0) {
+    const int Mp = (g.m + 63) / 64 * 64;
+    L.par = t / Mp;
+    L.m = t - L.par * Mp;
+    L.sys = 1;
+    L.mean = false;
+    L.valid = L.par < 2 && L.m < g.m;
+    const size_t off = blockIdx.y * cstride;
+    in.g += off;
+    out.g += off;
+    if (nuo.g) nuo.g += off;
+    if (r1.g) r1.g += off;
+    if (r2.g) r2.g += off;
+  } else {
+    L = sys_lane(g, t);
+  }
   if (!L.valid) return;
   const int M = g.m, m = L.m, par = L.par;
   const StructSys &ss = smt.sys[L.sys][par];
   const int n = ss.n;
   const int sp = L.mean ? 2 + par : 2 * L.sys + par;
-  double *scr = L.mean ? scratch + (size_t)4 * g.npmax * 7 * M + (size_t)par * g.npmax * 7
-                       : scratch + (size_t)sp * g.npmax * 7 * M;
+  double *scr = L.mean      ? scratch + (size_t)4 * g.npmax * 7 * M + (size_t)par * g.npmax * 7
+                : ncomp > 0 ? scratch + ((size_t)blockIdx.y * 2 + par) * g.npmax * 7 * M
+                            : scratch + (size_t)sp * g.npmax * 7 * M;
   const int Ms = L.mean ? 1 : M;
   const int lane = L.mean ? 0 : m;
   const bool wnu = nuo.u0 != nullptr;
@@ -733,7 +756,24 @@ hipError_t launch_adj_solve(const Geo &g, const StructMats &smt, double gamma, P
                             double *scratch, hipStream_t s, Prim nu, PrimC r1, PrimC r2, double e0,
                             double e1, double e2) {
   hipLaunchKernelGGL(k_adj_solve, dim3((sys_threads(g) + 255) / 256), dim3(256), 0, s, g, smt, gamma,
-                     in, out, scratch, nu, r1, r2, e0, e1, e2);
+                     in, out, scratch, nu, r1, r2, e0, e1, e2, 0, (size_t)0);
+  return hipGetLastError();
+}
+
+size_t adj_solve_dir_scratch_doubles(const Geo &g, int ncomp) {
+  return std::max(solve_scratch_doubles(g), (size_t)ncomp * 2 * g.npmax * 7 * g.m);
+}
+
+hipError_t launch_adj_solve_dir(const Geo &g, const StructMats &smt, double gamma, const double2 *in,
+                                double2 *out, double *scratch, int ncomp, size_t cstride, hipStream_t s) {
+  if (ncomp < 1 || !in || !out || !scratch) return hipErrorInvalidValue;
+  const int threads = 2 * ((g.m + 63) / 64 * 64);
+  PrimC pin{};
+  pin.g = in;
+  Prim pout{};
+  pout.g = out;
+  hipLaunchKernelGGL(k_adj_solve, dim3((threads + 255) / 256, ncomp), dim3(256), 0, s, g, smt, gamma, pin, pout,
+                     scratch, Prim{}, PrimC{}, PrimC{}, 0.0, 0.0, 0.0, ncomp, cstride);
   return hipGetLastError();
 }
 

@@ -56,6 +56,11 @@ struct PrimSum {
 hipError_t launch_adj_solve(const Geo &g, const StructMats &smt, double gamma, PrimC in, Prim out,
                             double *scratch, hipStream_t s, Prim nu = Prim{}, PrimC r1 = PrimC{},
                             PrimC r2 = PrimC{}, double e0 = 0, double e1 = 0, double e2 = 0);
+// out = S^{-T} in on the Dirichlet blocks for ncomp fields cstride apart (g
+// slots only), one launch; scratch holds adj_solve_dir_scratch_doubles.
+hipError_t launch_adj_solve_dir(const Geo &g, const StructMats &smt, double gamma, const double2 *in,
+                                double2 *out, double *scratch, int ncomp, size_t cstride, hipStream_t s);
+size_t adj_solve_dir_scratch_doubles(const Geo &g, int ncomp);
 // Steady transposed solve with the factors of the transposed blocks (smt);
 // also nu = e0 out + e1 r1 + e2 r2 and, with sig.u0 set, sig = c0 out + c1 r1
 // + c2 r2.

@@ -271,7 +271,7 @@ struct pcf_handle {
     double *scratchA = nullptr;
     double *mEWAG = nullptr;      // W AG^T (n0 x nd): cotangent of the nodal EMF from mu_NA
     double *mVDT = nullptr, *mVD1T = nullptr, *mVQ1T = nullptr, *mVQ2T = nullptr;  // (nd x n0) transposes
-    double *mMAT = nullptr;       // W C0D^T (nd x nd): the A mass transposed
+    double *mMAT = nullptr, *mC2T = nullptr;  // W C0D^T, W C2D^T (nd x nd): A mass and stiffness transposed
   } adj;
   std::vector<void *> adj_allocs;  // freed on destroy or when the tape grows
 
@@ -1550,7 +1550,8 @@ int32_t adj_setup_mhd(pcf_handle *h) {
   for (int i = 0; i < 4; ++i)
     if ((r = h->alloc(&A.ghA[i], n3))) return r;
   if ((r = alloc_prim(h, A.zero)) || (r = alloc_prim(h, A.sink))) return r;
-  if ((r = h->alloc(&A.scratchA, pcf::solve_scratch_doubles(g)))) return r;
+  // the start-up transposed solves of the three components run in one launch
+  if ((r = h->alloc(&A.scratchA, pcf::adj_solve_dir_scratch_doubles(g, 3)))) return r;
   // steady factors of the transposed blocks at nu = eta (the clamped and mean
   // blocks of these sets only ever see the zero set)
   for (int b = 0; b < 2; ++b)
@@ -1572,11 +1573,15 @@ int32_t adj_setup_mhd(pcf_handle *h) {
       vq1t[(size_t)i * n0 + x] = o.VQ1(x, i);
       vq2t[(size_t)i * n0 + x] = o.VQ2(x, i);
     }
-  std::vector<double> mat((size_t)nd * nd);
+  std::vector<double> mat((size_t)nd * nd), c2t((size_t)nd * nd);
   for (int i = 0; i < nd; ++i)
-    for (int j = 0; j < nd; ++j) mat[(size_t)i * nd + j] = g.W * o.C0D(j, i);
+    for (int j = 0; j < nd; ++j) {
+      mat[(size_t)i * nd + j] = g.W * o.C0D(j, i);
+      c2t[(size_t)i * nd + j] = g.W * o.C2D(j, i);
+    }
   if ((r = h->upload(&A.mEWAG, ewag)) || (r = h->upload(&A.mVDT, vdt)) || (r = h->upload(&A.mVD1T, vd1t)) ||
-      (r = h->upload(&A.mVQ1T, vq1t)) || (r = h->upload(&A.mVQ2T, vq2t)) || (r = h->upload(&A.mMAT, mat)))
+      (r = h->upload(&A.mVQ1T, vq1t)) || (r = h->upload(&A.mVQ2T, vq2t)) || (r = h->upload(&A.mMAT, mat)) ||
+      (r = h->upload(&A.mC2T, c2t)))
     return r;
   A.mhd_ready = true;
   return PCF_OK;
@@ -1592,11 +1597,63 @@ pcf::EvalTerm eterm(const double *M, const double2 *F, int K, int kf, double c) 
   return t;
 }
 
+struct ATerm {
+  const double2 *p;
+  double w;
+};
+
+// Mass terms of an A cotangent: (M_A + gamma L_A)^T rstage (when rstage is
+// set) + M_A^T sum w_t p_t, with M_A = W C0D and L_A = eta W (C2D - k^2 C0D)
+// the Dirichlet blocks at nu = eta.
+struct AMass {
+  double gamma = 0.0;
+  const double2 *rstage = nullptr;
+  int n = 0;
+  ATerm t[3];
+};
+
+AMass amass(double gamma, const double2 *rstage, std::initializer_list<ATerm> gather) {
+  AMass m;
+  m.gamma = gamma;
+  m.rstage = rstage;
+  for (const ATerm &t : gather) m.t[m.n++] = t;
+  return m;
+}
+
+// Appends the mass terms of component c to an evaluation.
+void add_mass_terms(const pcf_handle *h, const AMass &ms, int c, pcf::EvalOut &o) {
+  const auto &A = h->adj;
+  const int nd = h->geo.nd;
+  const size_t off = c * h->cstride;
+  const double gn = ms.gamma * h->geo_eta.nu;
+  if (ms.rstage) {
+    o.t[o.nt++] = eterm(A.mMAT, ms.rstage + off, nd, pcf::KF_ONE, 1.0);
+    if (gn != 0.0) {
+      o.t[o.nt++] = eterm(A.mMAT, ms.rstage + off, nd, pcf::KF_K2, -gn);
+      o.t[o.nt++] = eterm(A.mC2T, ms.rstage + off, nd, pcf::KF_ONE, gn);
+    }
+  }
+  for (int t = 0; t < ms.n; ++t) o.t[o.nt++] = eterm(A.mMAT, ms.t[t].p + off, nd, pcf::KF_ONE, ms.t[t].w);
+}
+
+// out = the mass terms alone (three components cstride apart).
+hipError_t a_mass(pcf_handle *h, const AMass &ms, double2 *out) {
+  pcf::EvalOut o[3];
+  for (int c = 0; c < 3; ++c) {
+    o[c].dst = out + c * h->cstride;
+    o[c].rows = h->geo.nd;
+    o[c].nt = 0;
+    add_mass_terms(h, ms, c, o[c]);
+  }
+  return pcf::launch_eval_terms(h->geo, o, 3, h->adj.stream);
+}
+
 // mu_A = (b, j map)^H (mu_b, mu_j): the transpose of the nodal b and j of
 // k_nodal_mhd_mfma (conducting walls), from nodal cotangents bj = [6][n0][m]
-// (bx, by, bz, jx, jy, jz) into out (three components cstride apart); with
-// sig given, plus M_A^T sig (the SBDF3 history gather of a steady step).
-hipError_t a_nodal_adj(pcf_handle *h, const double2 *bj, double2 *out, const double2 *sig = nullptr) {
+// (bx, by, bz, jx, jy, jz) into out (three components cstride apart), plus
+// the mass terms ms when given (the history gather, and the stage term of a
+// start-up stage).
+hipError_t a_nodal_adj(pcf_handle *h, const double2 *bj, double2 *out, const AMass *ms = nullptr) {
   const auto &A = h->adj;
   const pcf::Geo &g = h->geo;
   const size_t pl = (size_t)g.n0 * g.m, cs = h->cstride;
@@ -1620,7 +1677,7 @@ hipError_t a_nodal_adj(pcf_handle *h, const double2 *bj, double2 *out, const dou
     o[c].rows = g.nd;
     o[c].nt = nt[c];
     for (int t = 0; t < nt[c]; ++t) o[c].t[t] = ts[c][t];
-    if (sig) o[c].t[o[c].nt++] = eterm(A.mMAT, sig + c * cs, g.nd, KF_ONE, 1.0);
+    if (ms) add_mass_terms(h, *ms, c, o[c]);
   }
   return launch_eval_terms(g, o, 3, A.stream);
 }
@@ -1630,7 +1687,7 @@ hipError_t a_nodal_adj(pcf_handle *h, const double2 *bj, double2 *out, const dou
 // padded y-line fields (u, omega, b, j) are ytape: the flow part into
 // adj.muX (steady_x: with the reconstruction adjoint and M^T sig folded in)
 // or adj.mu_u, the vector-potential part into adj.muA (overwritten).
-hipError_t nl_adj_mhd(pcf_handle *h, const void *ytape, bool steady_x, const double2 *sigA = nullptr) {
+hipError_t nl_adj_mhd(pcf_handle *h, const void *ytape, bool steady_x, const AMass *ms) {
   auto &A = h->adj;
   const pcf::Geo &g = h->geo;
   const size_t pl = (size_t)g.n0 * g.m;
@@ -1653,7 +1710,7 @@ hipError_t nl_adj_mhd(pcf_handle *h, const void *ytape, bool steady_x, const dou
   else
     e = pcf::launch_nodal_adj_mfma<double>(g, A.ad, A.ws, A.mu_u[0], A.mu_u[1], A.mu_u[2], A.stream);
   if (e != hipSuccess) return e;
-  return a_nodal_adj(h, reinterpret_cast<const double2 *>(A.ws.X) + 6 * pl, A.muA, sigA);
+  return a_nodal_adj(h, reinterpret_cast<const double2 *>(A.ws.X) + 6 * pl, A.muA, ms);
 }
 
 // Transposed A solves of the steady systems: out = S_A^{-T} in with
@@ -1666,40 +1723,11 @@ hipError_t a_solve_factored(pcf_handle *h, const double2 *in, double2 *out, cons
                                             a_out(h, A.sigA), C_SBDF[0], C_SBDF[1], C_SBDF[2], 3, h->cstride);
 }
 
-// out = (M_A - gamma L_A)^{-T} in (start-up stages, factorised per launch).
+// out = (M_A - gamma L_A)^{-T} in (start-up stages, factorised per launch),
+// the three components in one launch.
 hipError_t a_solve(pcf_handle *h, double gamma, const double2 *in, double2 *out) {
   auto &A = h->adj;
-  for (int c = 0; c < 3; ++c) {
-    const size_t o = c * h->cstride;
-    hipError_t er = pcf::launch_adj_solve(h->geo_eta, A.smt, gamma, a_in(h, in + o), a_out(h, out + o),
-                                          A.scratchA, A.stream);
-    if (er != hipSuccess) return er;
-  }
-  return hipSuccess;
-}
-
-struct ATerm {
-  const double2 *p;
-  double w;
-};
-
-// out += (M_A + gamma L_A)^T rstage (when given) + M_A^T sum w_t p_t.
-hipError_t a_mass(pcf_handle *h, double gamma, const double2 *rstage, std::initializer_list<ATerm> gather,
-                  double2 *out) {
-  auto &A = h->adj;
-  for (int c = 0; c < 3; ++c) {
-    const size_t o = c * h->cstride;
-    pcf::PrimSum s{};
-    for (const ATerm &t : gather) {
-      s.t[s.n] = a_in(h, t.p + o);
-      s.wgt[s.n] = t.w;
-      ++s.n;
-    }
-    hipError_t er = pcf::launch_adj_mass(h->geo_eta, h->sm, gamma, rstage ? a_in(h, rstage + o) : pcf::PrimC{}, s,
-                                         a_out(h, out + o), A.stream);
-    if (er != hipSuccess) return er;
-  }
-  return hipSuccess;
+  return pcf::launch_adj_solve_dir(h->geo_eta, A.smt, gamma, in, out, A.scratchA, 3, h->cstride, A.stream);
 }
 
 // out = sum w_t p_t.
@@ -1746,14 +1774,15 @@ hipError_t reverse_step_mhd(pcf_handle *h, int k, bool steady, void *const *tape
       return er;
     if ((er = a_solve_factored(h, A.muA, A.rhoA[rr[0]], A.rhoA[rr[1]], A.rhoA[rr[2]], e))) return er;
     // mu_A = (b, j map)^H mu_bj + M_A^T sig_A in one evaluation
+    const AMass ms = amass(0.0, nullptr, {{A.sigA, 1.0}});
     if (k == 0) {  // initial state: its velocity leaves keep mu_u
-      if ((er = nl_adj_mhd(h, tape[0], false, A.sigA))) return er;
+      if ((er = nl_adj_mhd(h, tape[0], false, &ms))) return er;
       pcf::PrimSum ga{};
       for (int j = 0; j < 3; ++j) ps_add(ga, A.rho[rr[j]], C_SBDF[j]);
       if ((er = zero_prim(h, A.muX))) return er;
       return pcf::launch_adj_mass(g, h->sm, 0.0, pcf::PrimC{}, ga, A.muX, A.stream);
     }
-    return nl_adj_mhd(h, tape[0], true, A.sigA);
+    return nl_adj_mhd(h, tape[0], true, &ms);
   }
   if ((er = zero_prim(h, A.rho[rr[0]])) || (er = a_zero(h, A.rhoA[rr[0]]))) return er;
   for (int s = 2; s >= 0; --s) {
@@ -1776,7 +1805,10 @@ hipError_t reverse_step_mhd(pcf_handle *h, int k, bool steady, void *const *tape
       er = a_sum(h, {{A.rsA[0], dt * IMEX_A[0]}, {A.rsA[1], dt * IMEX_B[1]}, {A.rhoA[rr[1]], e[1]},
                      {A.rhoA[rr[2]], e[2]}}, A.nuA);
     if (er) return er;
-    if ((er = nl_adj_mhd(h, tape[s], false))) return er;
+    // mu_A = (b, j map)^H mu_bj + (M_A + gamma L_A)^T rs_A (+ the history gather at s = 0)
+    const AMass ms = (s == 0) ? amass(gamma, A.rsA[s], {{A.rhoA[rr[1]], C_SBDF[1]}, {A.rhoA[rr[2]], C_SBDF[2]}})
+                              : amass(gamma, A.rsA[s], {});
+    if ((er = nl_adj_mhd(h, tape[s], false, &ms))) return er;
     if ((er = recon_or_zero(h, s == 0 && k == 0))) return er;
     pcf::PrimSum ga{};
     if (s == 0) {
@@ -1784,18 +1816,16 @@ hipError_t reverse_step_mhd(pcf_handle *h, int k, bool steady, void *const *tape
       ps_add(ga, A.rho[rr[2]], C_SBDF[2]);
     }
     if ((er = pcf::launch_adj_mass(g, h->sm, gamma, cprim(A.rs[s]), ga, A.muX, A.stream))) return er;
-    er = (s == 0) ? a_mass(h, gamma, A.rsA[s], {{A.rhoA[rr[1]], C_SBDF[1]}, {A.rhoA[rr[2]], C_SBDF[2]}}, A.muA)
-                  : a_mass(h, gamma, A.rsA[s], {}, A.muA);
-    if (er) return er;
   }
   return hipSuccess;
 }
 
 // pcf_vjp for the conducting-wall MHD step: the same checkpointing as the
-// hydrodynamic reverse pass, run in sequence (each segment is recomputed on
-// the handle stream once the reverse of the next one has finished with the
-// tape, no graph capture), with the vector-potential cotangents beside the
-// flow's.  Seed: Epert + Emag of the final state, or the cotangents set.
+// hydrodynamic reverse pass, with the vector-potential cotangents beside the
+// flow's.  The recompute of segment seg - 1 (eager steps on the handle
+// stream) overlaps the reverse of segment seg (reverse stream) through the
+// two tape sets, as in pcf_vjp.  Seed: Epert + Emag of the final state, or
+// the cotangents set.
 int32_t vjp_mhd(pcf_handle *h, int steps, int every, int seed) {
   auto &A = h->adj;
   const pcf::Geo &g = h->geo;
@@ -1808,26 +1838,27 @@ int32_t vjp_mhd(pcf_handle *h, int steps, int every, int seed) {
     return fail(PCF_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
   };
   hipError_t e;
-  // segment seg's steps on the handle stream, keeping the tape from slot 0;
-  // first[i] receives the tape slot of step ka + i
-  auto run_seg = [&](int seg, std::vector<int> &first) -> int32_t {
+  // segment seg's steps on the handle stream, keeping the tape in set seg & 1
+  // when keep is set; first[i] receives the tape slot of step ka + i
+  auto run_seg = [&](int seg, bool keep, std::vector<int> &first) -> int32_t {
     const int ka = seg * every, kb = std::min(steps, ka + every);
     first.assign(kb - ka, 0);
-    int slot = 0;
+    int slot = (seg & 1) * A.per_set;
     for (int k = ka; k < kb; ++k) {
       first[k - ka] = slot;
       const bool st = h->history_steps >= 2.0;
-      int32_t r = st ? steady_step(h, &A.tape[slot]) : startup_step(h, &A.tape[slot]);
+      void *const *tape = keep ? &A.tape[slot] : nullptr;
+      int32_t r = st ? steady_step(h, tape) : startup_step(h, tape);
       if (r) return r;
       slot += st ? 1 : 3;
     }
     return PCF_OK;
   };
-  std::vector<int> first;
+  std::vector<int> first_rev, first_next;
   int32_t r;
   for (int seg = 0; seg < nseg; ++seg) {
     if ((e = snap_save(h, A.snaps[seg]))) return hip_fail(e, "snapshot");
-    if ((r = run_seg(seg, first))) return r;
+    if ((r = run_seg(seg, seg == nseg - 1, first_rev))) return r;
   }
   pcf_handle::Snap &fin = A.snaps[nseg];
   if ((e = snap_save(h, fin))) return hip_fail(e, "snapshot");
@@ -1857,20 +1888,29 @@ int32_t vjp_mhd(pcf_handle *h, int steps, int every, int seed) {
     if ((e = zero_prim(h, A.rho[i])) || (e = a_zero(h, A.rhoA[i]))) return hip_fail(e, "memset");
     A.rr[i] = i;
   }
+  // the recompute overwrites the state the seed reads
+  PCF_HIP(hipEventRecord(A.ev_seed, A.stream));
+  PCF_HIP(hipStreamWaitEvent(h->stream, A.ev_seed, 0));
+  // tape set b is rewritten only after the reverse that read it (ev_rev[b])
+  // and read only after the recompute that wrote it (ev_rc[b])
+  bool rev_pending[2] = {false, false};
   for (int seg = nseg - 1; seg >= 0; --seg) {
     const int ka = seg * every, kb = std::min(steps, ka + every);
     if (seg < nseg - 1) {
-      // the recompute rewrites the tape (and the state the seed read) only
-      // after the reverse pass before it is done with them
-      PCF_HIP(hipEventRecord(A.ev_rev[0], A.stream));
-      PCF_HIP(hipStreamWaitEvent(h->stream, A.ev_rev[0], 0));
-      if ((e = snap_load(h, A.snaps[seg]))) return hip_fail(e, "restore");
-      if ((r = run_seg(seg, first))) return r;
-      PCF_HIP(hipEventRecord(A.ev_rc[0], h->stream));
-      PCF_HIP(hipStreamWaitEvent(A.stream, A.ev_rc[0], 0));
+      first_rev.swap(first_next);
+      PCF_HIP(hipStreamWaitEvent(A.stream, A.ev_rc[seg & 1], 0));
+    }
+    if (seg > 0) {
+      const int b = (seg - 1) & 1;
+      if (rev_pending[b]) PCF_HIP(hipStreamWaitEvent(h->stream, A.ev_rev[b], 0));
+      if ((e = snap_load(h, A.snaps[seg - 1]))) return hip_fail(e, "restore");
+      if ((r = run_seg(seg - 1, true, first_next))) return r;
+      PCF_HIP(hipEventRecord(A.ev_rc[b], h->stream));
     }
     for (int k = kb - 1; k >= ka; --k)
-      if ((e = reverse_step_mhd(h, k, h0 + k >= 2, &A.tape[first[k - ka]]))) return hip_fail(e, "reverse step");
+      if ((e = reverse_step_mhd(h, k, h0 + k >= 2, &A.tape[first_rev[k - ka]]))) return hip_fail(e, "reverse step");
+    PCF_HIP(hipEventRecord(A.ev_rev[seg & 1], A.stream));
+    rev_pending[seg & 1] = true;
   }
   // history leaves of the initial state: levels -1 and -2 (flow as pcf_vjp, then A)
   {
@@ -1893,9 +1933,9 @@ int32_t vjp_mhd(pcf_handle *h, int steps, int every, int seed) {
     ps_add(s, r0, e2);
     if ((e = pcf::launch_prim_sum(g, s, A.gh[3], A.stream))) return hip_fail(e, "history");
     const double2 *a0 = A.rhoA[A.rr[0]], *a1 = A.rhoA[A.rr[1]];
-    if ((e = a_zero(h, A.ghA[0])) || (e = a_zero(h, A.ghA[1]))) return hip_fail(e, "memset");
-    if ((e = a_mass(h, 0.0, nullptr, {{a0, C_SBDF[1]}, {a1, C_SBDF[2]}}, A.ghA[0])) ||
-        (e = a_mass(h, 0.0, nullptr, {{a0, C_SBDF[2]}}, A.ghA[1])) || (e = a_sum(h, {{a0, e1}, {a1, e2}}, A.ghA[2])) ||
+    if ((e = a_mass(h, amass(0.0, nullptr, {{a0, C_SBDF[1]}, {a1, C_SBDF[2]}}), A.ghA[0])) ||
+        (e = a_mass(h, amass(0.0, nullptr, {{a0, C_SBDF[2]}}), A.ghA[1])) ||
+        (e = a_sum(h, {{a0, e1}, {a1, e2}}, A.ghA[2])) ||
         (e = a_sum(h, {{a0, e2}}, A.ghA[3])))
       return hip_fail(e, "history");
   }

@@ -64,20 +64,24 @@ __device__ double block_max256(double v, double *red) {
 
 // ------------------------------------------------------- k_eval_terms ----
 // out[p][m] = sum_t f_t(m) sum_i M_t[p][i] F_t[i][m], f_t a mode factor
-// (EvalFactor), for blockIdx.z's output of the batch.  Register tile: a block
-// covers 64 consecutive modes (one per lane) and 4 EVAL_RW rows, each of its
-// four waves EVAL_RW consecutive rows, so every field value a lane loads
-// serves EVAL_RW rows and the matrix entries are wave-uniform loads.  The
-// field is read once per row tile instead of once per row.
+// (EvalFactor), for blockIdx.z's output of the batch.  A block covers 64
+// consecutive modes (one per lane) and EVAL_RW consecutive rows; each of its
+// four waves sums a quarter of every term's i range for all EVAL_RW rows, so a
+// field value a lane loads serves EVAL_RW rows, the matrix entries are
+// wave-uniform loads, and four times as many waves keep loads in flight as
+// with whole sums per wave.  The four partial sums meet in LDS in a fixed
+// order; wave r stores row p0 + r.
 constexpr int EVAL_RW = 4;
 
 __global__ void __launch_bounds__(256) k_eval_terms(Geo g, EvalBatch b) {
+  __shared__ double2 part[4][EVAL_RW][64];
   const EvalOut &o = b.o[blockIdx.z];
+  const int p0 = blockIdx.y * EVAL_RW;
+  if (p0 >= o.rows) return;  // uniform across the block
   const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-  const int p0 = (blockIdx.y * 4 + wave) * EVAL_RW;
-  if (p0 >= o.rows) return;
+  const int lane = threadIdx.x & 63;
   const int nr = min(EVAL_RW, o.rows - p0);
-  const int md = blockIdx.x * 64 + (int)(threadIdx.x & 63);
+  const int md = blockIdx.x * 64 + lane;
   const bool live = md < g.m;
   const int mc = live ? md : g.m - 1;  // lanes past the last mode load a valid column
   const int iy = mc / g.kzs, kz = mc - iy * g.kzs;
@@ -87,7 +91,8 @@ __global__ void __launch_bounds__(256) k_eval_terms(Geo g, EvalBatch b) {
   for (int r = 0; r < EVAL_RW; ++r) ar[r] = ai[r] = 0.0;
   for (int t = 0; t < o.nt; ++t) {
     const EvalTerm &T = o.t[t];
-    const int K = T.K;
+    const int K = T.K, kc = (K + 3) / 4;
+    const int i0 = min(K, wave * kc), i1 = min(K, i0 + kc);
     const double *Mr[EVAL_RW];
 #pragma unroll
     for (int r = 0; r < EVAL_RW; ++r) Mr[r] = T.M + (size_t)(p0 + min(r, nr - 1)) * K;  // rows past nr: not stored
@@ -95,8 +100,8 @@ __global__ void __launch_bounds__(256) k_eval_terms(Geo g, EvalBatch b) {
     double sr[EVAL_RW], si[EVAL_RW];
 #pragma unroll
     for (int r = 0; r < EVAL_RW; ++r) sr[r] = si[r] = 0.0;
-    int i = 0;
-    for (; i + 4 <= K; i += 4) {
+    int i = i0;
+    for (; i + 4 <= i1; i += 4) {
       double2 f[4];
 #pragma unroll
       for (int u = 0; u < 4; ++u) f[u] = F[(size_t)(i + u) * g.m];
@@ -109,7 +114,7 @@ __global__ void __launch_bounds__(256) k_eval_terms(Geo g, EvalBatch b) {
           si[r] = fma(a, f[u].y, si[r]);
         }
     }
-    for (; i < K; ++i) {
+    for (; i < i1; ++i) {
       const double2 f = F[(size_t)i * g.m];
 #pragma unroll
       for (int r = 0; r < EVAL_RW; ++r) {
@@ -139,10 +144,19 @@ __global__ void __launch_bounds__(256) k_eval_terms(Geo g, EvalBatch b) {
       }
     }
   }
-  if (!live) return;
 #pragma unroll
-  for (int r = 0; r < EVAL_RW; ++r)
-    if (r < nr) o.dst[(size_t)(p0 + r) * g.m + md] = make_double2(ar[r], ai[r]);
+  for (int r = 0; r < EVAL_RW; ++r) part[wave][r][lane] = make_double2(ar[r], ai[r]);
+  __syncthreads();
+  static_assert(EVAL_RW <= 4, "one storing wave per row");
+  if (wave >= nr || !live) return;
+  double2 v = part[0][wave][lane];
+#pragma unroll
+  for (int w = 1; w < 4; ++w) {
+    const double2 q = part[w][wave][lane];
+    v.x += q.x;
+    v.y += q.y;
+  }
+  o.dst[(size_t)(p0 + wave) * g.m + md] = v;
 }
 
 // ---------------------------------------------------------- k_curl_b ----
@@ -400,8 +414,8 @@ hipError_t launch_eval_terms(const Geo &g, const EvalOut *o, int n, hipStream_t 
       b.o[j] = e;
       rows = std::max(rows, e.rows);
     }
-    const int tile = 4 * EVAL_RW;
-    hipLaunchKernelGGL(k_eval_terms, dim3((g.m + 63) / 64, (rows + tile - 1) / tile, b.n), dim3(256), 0, s, g, b);
+    hipLaunchKernelGGL(k_eval_terms, dim3((g.m + 63) / 64, (rows + EVAL_RW - 1) / EVAL_RW, b.n), dim3(256), 0, s, g,
+                       b);
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   }

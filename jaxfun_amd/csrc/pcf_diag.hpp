@@ -21,8 +21,8 @@ struct EvalTerm {
 
 enum EvalFactor { KF_ONE = 0, KF_IKY = 1, KF_IKZ = 2, KF_KYY = 3, KF_KZZ = 4, KF_KYZ = 5, KF_K2 = 6 };
 
-// out[p][m] = sum of up to eight terms, p < rows.
-constexpr int EVAL_TERMS = 8;
+// out[p][m] = sum of up to twelve terms, p < rows.
+constexpr int EVAL_TERMS = 12;
 struct EvalOut {
   double2 *dst = nullptr;
   int rows = 0;
