@@ -1,0 +1,124 @@
+/*
+ * tc_dns.h — C ABI of the axisymmetric Taylor-Couette DNS step in
+ * libpcfdns.so (the second geometry of the same library).  Plain C types only;
+ * every entry point returns 0 on success and a negative TC_E_* code on
+ * failure, with a message from tc_last_error().
+ *
+ * The step is AxisymmetricTCDNSJax of the reference
+ * (examples/taylor_couette_dns_jax.py:217-1078): perturbations
+ * (u_r, u_theta, u_z, p) about the circular-Couette flow V = a r + b / r on
+ * Fourier(z) x Chebyshev(r), Dirichlet velocity modes and the P_N / P_{N-2}
+ * pressure pair, per-axial-mode coupled saddle-point blocks with the k = 0
+ * pressure constant pinned.
+ *
+ * Layout at the boundary: every field is exchanged in the reference's
+ * full-complex layout, C order (Nz, Nr - 2) of interleaved complex128 (re, im)
+ * doubles, axis 0 the axial Fourier index in FFT order.  Internally the
+ * library keeps the Hermitian half (kz index < Nz / 2); the Nyquist mode is
+ * not stored (the reference masks it in every nonlinear row).
+ */
+#ifndef TC_DNS_H
+#define TC_DNS_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TC_ABI_VERSION 1
+
+#define TC_OK 0
+#define TC_E_INVALID (-1)
+#define TC_E_HIP (-2)
+#define TC_E_UNSUPPORTED (-3)
+#define TC_E_FIXED_DT (-4)
+
+/* time integrators (examples/taylor_couette_dns_jax.py:286-292):
+ * CNAB2 (:758-795): reads NL_OLD when have_old != 0, then NL_OLD holds the
+ *   step's nonlinear rows and have_old = 1.
+ * SBDF3 (:885-946): ARS(4,4,3) start-up steps while history_steps < 2, then
+ *   steady SBDF3 steps; the history fields shift every step.
+ * IMEXRK3 (:797-850): every step an ARS(4,4,3) step; history untouched. */
+#define TC_TI_CNAB2 0
+#define TC_TI_SBDF3 1
+#define TC_TI_IMEXRK3 2
+
+/* AxisymmetricTCDNSJax(base, nu, Nr, Nz, Lz, dt, family="C", dealias,
+ * time_integrator) with base = CircularCouette(R1, R2, Omega1, Omega2)
+ * (examples/taylor_couette_linear_jax.py:37-48). */
+typedef struct tc_config {
+  int32_t abi_version;   /* = TC_ABI_VERSION                           */
+  int32_t nr, nz;        /* Chebyshev points (>= 6); Fourier modes (even) */
+  double r1, r2;         /* cylinder radii, 0 < r1 < r2                */
+  double omega1, omega2; /* angular velocities of the cylinders        */
+  double nu;             /* kinematic viscosity                        */
+  double lz;             /* axial period                               */
+  double dt;             /* time step                                  */
+  double dealias;        /* padding factor of the product grid (1 or 1.5) */
+  int32_t time_integrator; /* TC_TI_*                                  */
+  int32_t device;        /* HIP device ordinal                         */
+} tc_config;
+
+typedef struct tc_handle tc_handle;
+
+/* field ids for tc_set_field / tc_get_field: (Nz, Nr - 2) complex each */
+#define TC_F_UR 0
+#define TC_F_UT 1
+#define TC_F_UZ 2
+#define TC_F_P 3
+#define TC_F_NL_OLD 4      /* 3 components: 4, 5, 6    */
+#define TC_F_NL_OLDER 7    /* 7, 8, 9                  */
+#define TC_F_SOL_OLD 10    /* 10, 11, 12               */
+#define TC_F_SOL_OLDER 13  /* 13, 14, 15               */
+#define TC_F_COUNT 16
+
+/* diagnostics (examples/taylor_couette_dns_jax.py:1061-1069) */
+#define TC_DIAG_E 0
+#define TC_DIAG_DIV_LINF 1
+#define TC_DIAG_CONTINUITY_L2 2
+#define TC_DIAG_WALL 3
+#define TC_DIAG_ETH 4
+#define TC_DIAG_COUNT 5
+
+const char *tc_last_error(void);
+
+/* AxisymmetricTCDNSJax.__init__ (:229-284): builds the per-mode operators and
+ * their inverses on the host in long double, uploads them. */
+int32_t tc_create(const tc_config *cfg, tc_handle **out);
+int32_t tc_destroy(tc_handle *h);
+
+/* State exchange in the full-complex layout; scalars: have_old, history_steps
+ * (AxisymmetricTCState, :114-165). */
+int32_t tc_set_field(tc_handle *h, int32_t field, const double *host);
+int32_t tc_get_field(tc_handle *h, int32_t field, double *host);
+int32_t tc_set_scalars(tc_handle *h, double have_old, double history_steps);
+int32_t tc_get_scalars(tc_handle *h, double *have_old, double *history_steps);
+
+/* AxisymmetricTCDNSJax.step / solve (:948-990): nsteps steps of the
+ * configured integrator on the device state. */
+int32_t tc_step(tc_handle *h, int32_t nsteps);
+
+/* tc_step with the steady SBDF3 steps replayed from a captured HIP graph
+ * (no reference counterpart: a launch-bound small step); start-up steps run
+ * eagerly.  Same results as tc_step. */
+int32_t tc_step_graph(tc_handle *h, int32_t nsteps);
+
+/* E, div_linf, continuity_l2, wall, Eth of the device state (TC_DIAG_*). */
+int32_t tc_diagnostics(tc_handle *h, double *out);
+
+/* The linear eigenproblem pencil L q = lambda M q of
+ * TaylorCouetteLinearJax.assemble (examples/taylor_couette_linear_jax.py:209-237)
+ * on the configuration's radial space (N = cfg->nr, base flow and nu of cfg):
+ * out_L, out_M (4n x 4n) complex interleaved, n = Nr - 2.  Host only, no GPU. */
+int32_t tc_linear_pencil(const tc_config *cfg, int32_t m, double kz, double *out_L, double *out_M);
+
+/* dt change (:971-985): SBDF3 refuses a changed dt (TC_E_FIXED_DT); CNAB2 and
+ * IMEXRK3 rebuild their operators. */
+int32_t tc_set_dt(tc_handle *h, double dt);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TC_DNS_H */

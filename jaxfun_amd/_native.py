@@ -79,6 +79,36 @@ EXPORTS = (
 SEED_ENERGY = 0
 SEED_COTANGENT = 1
 
+# include/tc_dns.h (the Taylor-Couette step in the same library)
+TC_ABI_VERSION = 1
+TC_TI = {"CNAB2": 0, "SBDF3": 1, "IMEXRK3": 2}
+TC_FIELDS = ("UR", "UT", "UZ", "P", "NL_OLD_R", "NL_OLD_T", "NL_OLD_Z", "NL_OLDER_R", "NL_OLDER_T", "NL_OLDER_Z",
+             "SOL_OLD_R", "SOL_OLD_T", "SOL_OLD_Z", "SOL_OLDER_R", "SOL_OLDER_T", "SOL_OLDER_Z")
+TC_FIELD_ID = {name: i for i, name in enumerate(TC_FIELDS)}
+TC_DIAG_KEYS = ("E", "div_linf", "continuity_l2", "wall", "Eth")
+TC_EXPORTS = (
+    "tc_last_error", "tc_create", "tc_destroy", "tc_set_field", "tc_get_field", "tc_set_scalars", "tc_get_scalars",
+    "tc_step", "tc_step_graph", "tc_diagnostics", "tc_linear_pencil", "tc_set_dt",
+)
+
+
+class TcConfig(C.Structure):
+    _fields_ = [
+        ("abi_version", C.c_int32),
+        ("nr", C.c_int32),
+        ("nz", C.c_int32),
+        ("r1", C.c_double),
+        ("r2", C.c_double),
+        ("omega1", C.c_double),
+        ("omega2", C.c_double),
+        ("nu", C.c_double),
+        ("lz", C.c_double),
+        ("dt", C.c_double),
+        ("dealias", C.c_double),
+        ("time_integrator", C.c_int32),
+        ("device", C.c_int32),
+    ]
+
 
 class PcfConfig(C.Structure):
     _fields_ = [
@@ -172,6 +202,21 @@ def load() -> C.CDLL:
         "pcf_vjp": ([h, C.c_int32, C.c_int32, C.c_int32], C.c_int32),
         "pcf_get_gradient": ([h, C.c_int32, dp], C.c_int32),
     }
+    tcp = P(TcConfig)
+    sig.update({
+        "tc_last_error": ([], C.c_char_p),
+        "tc_create": ([tcp, P(C.c_void_p)], C.c_int32),
+        "tc_destroy": ([h], C.c_int32),
+        "tc_set_field": ([h, C.c_int32, dp], C.c_int32),
+        "tc_get_field": ([h, C.c_int32, dp], C.c_int32),
+        "tc_set_scalars": ([h, C.c_double, C.c_double], C.c_int32),
+        "tc_get_scalars": ([h, dp, dp], C.c_int32),
+        "tc_step": ([h, C.c_int32], C.c_int32),
+        "tc_step_graph": ([h, C.c_int32], C.c_int32),
+        "tc_diagnostics": ([h, dp], C.c_int32),
+        "tc_linear_pencil": ([tcp, C.c_int32, C.c_double, dp, dp], C.c_int32),
+        "tc_set_dt": ([h, C.c_double], C.c_int32),
+    })
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
         fn.argtypes = args
@@ -202,3 +247,15 @@ def setup_matrix(cfg: PcfConfig, name: str) -> np.ndarray:
     check(lib.pcf_setup_matrix(C.byref(cfg), MATRIX_ID[name],
                                out.ctypes.data_as(C.POINTER(C.c_double)), None, None))
     return out
+
+
+def tc_check(status: int) -> int:
+    """Status of a tc_* call: the reference's exception types, as check()."""
+    if status < 0:
+        msg = load().tc_last_error().decode(errors="replace")
+        if status in (-3, -4):
+            raise NotImplementedError(msg)
+        if status == -1:
+            raise ValueError(msg)
+        raise PcfError(f"Taylor-Couette library error {status}: {msg}")
+    return status

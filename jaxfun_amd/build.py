@@ -40,7 +40,7 @@ def build(verbose: bool = False) -> Path:
     hipcc = _hipcc()
     BUILD.mkdir(exist_ok=True)
     srcs = ("pcf_setup.cpp", "pcf_kernels.hip", "pcf_adjoint.hip", "pcf_mfma.hip", "pcf_factored.hip", "pcf_diag.hip",
-            "pcf_api.hip")
+            "pcf_api.hip", "tc_setup.cpp", "tc_kernels.hip", "tc_api.hip")
     cmds, objs = [], []
     for src in srcs:
         obj = BUILD / (Path(src).stem + ".o")

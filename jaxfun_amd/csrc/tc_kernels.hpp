@@ -1,0 +1,55 @@
+// Kernels of the axisymmetric Taylor-Couette step (tc_kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace tc {
+
+// Sizes and device tables of one handle.
+struct TcGeo {
+  int n = 0;          // radial dofs per mode (Nr - 2)
+  int kh = 0;         // stored axial modes 0..kh-1
+  int mr = 0, mz = 0; // product grid
+  int nr = 0;         // standard radial points
+  int nz = 0;         // standard axial points
+  double lz = 0.0;
+  const double *Ev = nullptr, *Ed = nullptr;  // (mr x n)
+  const double *Pr = nullptr;                 // (n x mr)
+  const double *invr = nullptr;               // (mr)
+  const double *kz = nullptr;                 // (kh) axial wavenumbers
+  const double2 *twz = nullptr;               // (mz) e^{2 pi i q / mz}
+  const double2 *twn = nullptr;               // (nz) e^{2 pi i q / nz}
+  const double *Mass = nullptr;               // (n x n)
+  const double *Lin = nullptr, *Lexp = nullptr;  // [kh][3n x 3n]
+  const double *Es = nullptr, *Esd = nullptr;    // (nr x n)
+  const double *rstd = nullptr, *fejer = nullptr;  // (nr)
+  const double *Drc = nullptr, *Dzc = nullptr;     // (n x n)
+};
+
+// Operators of a right-hand-side term.
+enum TcOp { TC_OP_ID = 0, TC_OP_MASS = 1, TC_OP_LIN = 2, TC_OP_LEXP = 3 };
+
+constexpr int TC_TERMS = 8;
+
+// rhs = sum_t c_t Op_t(X_t) + cN N, N = Pr G (the projected nonlinear rows of
+// the G written by k_tc_nonlinear; stored to nout when set); then
+// (u, p) = Ainv rhs.  Fields are [component][k][i] with components kh n apart.
+struct TcRhs {
+  int nt = 0;
+  int op[TC_TERMS];
+  const double2 *x[TC_TERMS];
+  double c[TC_TERMS];
+  const double2 *G = nullptr;
+  double cN = 0.0;
+  double2 *nout = nullptr;
+  const double2 *Ainv = nullptr;  // [kh][4n x 3n]
+  double2 *uout = nullptr;
+  double2 *pout = nullptr;
+};
+
+hipError_t launch_nonlinear(const TcGeo &g, const double2 *u, double2 *G, hipStream_t s);
+hipError_t launch_rhs_solve(const TcGeo &g, const TcRhs &a, hipStream_t s);
+// out[0..4] = E, div_linf, continuity_l2, wall, Eth of u.
+hipError_t launch_diag(const TcGeo &g, const double2 *u, double *out, hipStream_t s);
+
+}  // namespace tc

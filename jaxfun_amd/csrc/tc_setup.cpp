@@ -1,0 +1,372 @@
+// Host setup of the axisymmetric Taylor-Couette step, in long double.
+//
+// Weak forms (examples/taylor_couette_dns_jax.py:523-606): every bilinear form
+// (v, c(r) d^m u / dr^m) is the N-point Gauss-Chebyshev quadrature of the
+// mapped integral, (1/df) sum_q (pi/N) v(x_q) c(r_q) df^m u^(m)(x_q) with
+// df = 2 / (r2 - r1), times Lz (i kz)^m from the Fourier part; velocity modes
+// phi_i = T_i - T_{i+2}, pressure modes T_i, i < N - 2.
+#include "tc_setup.hpp"
+
+#include <cmath>
+
+#include "../../include/tc_dns.h"
+
+namespace tc {
+namespace {
+
+using LD = long double;
+using CL = std::complex<LD>;
+constexpr LD PI = 3.141592653589793238462643383279502884L;
+
+// Gauss-Chebyshev nodes in ascending order: -cos((2q + 1) pi / (2N)).
+std::vector<LD> nodes(int N) {
+  std::vector<LD> x(N);
+  for (int q = 0; q < N; ++q) x[q] = -cosl((2 * q + 1) * PI / (2 * N));
+  return x;
+}
+
+// T_k(x), T_k'(x), T_k''(x) for k < n by the three-term recurrences.
+void cheb_eval(LD x, int n, std::vector<LD> &t, std::vector<LD> &t1, std::vector<LD> &t2) {
+  t.assign(n, 0);
+  t1.assign(n, 0);
+  t2.assign(n, 0);
+  t[0] = 1;
+  if (n > 1) {
+    t[1] = x;
+    t1[1] = 1;
+  }
+  for (int k = 1; k + 1 < n; ++k) {
+    t[k + 1] = 2 * x * t[k] - t[k - 1];
+    t1[k + 1] = 2 * t[k] + 2 * x * t1[k] - t1[k - 1];
+    t2[k + 1] = 4 * t1[k] + 2 * x * t2[k] - t2[k - 1];
+  }
+}
+
+// Basis values at nodes x: [q][i] for the Dirichlet modes (D) and the
+// pressure modes (P), derivative order d in the reference coordinate.
+struct Basis {
+  int N = 0, n = 0;
+  std::vector<LD> D[3], P[3];
+};
+
+Basis basis_at(const std::vector<LD> &x, int N) {
+  Basis b;
+  b.N = N;
+  b.n = N - 2;
+  const int Q = (int)x.size(), n = b.n;
+  for (int d = 0; d < 3; ++d) {
+    b.D[d].assign((size_t)Q * n, 0);
+    b.P[d].assign((size_t)Q * n, 0);
+  }
+  std::vector<LD> t[3];
+  for (int q = 0; q < Q; ++q) {
+    cheb_eval(x[q], N, t[0], t[1], t[2]);
+    for (int d = 0; d < 3; ++d)
+      for (int i = 0; i < n; ++i) {
+        b.D[d][(size_t)q * n + i] = t[d][i] - t[d][i + 2];
+        b.P[d][(size_t)q * n + i] = t[d][i];
+      }
+  }
+  return b;
+}
+
+using Mat = std::vector<LD>;  // row-major n x n
+
+struct Forms {
+  int N = 0, n = 0;
+  LD df = 1, r1 = 1;
+  std::vector<LD> x, r;
+  Basis B;
+  Forms(int N_, LD r1_, LD r2_) : N(N_), n(N_ - 2), df(2 / (r2_ - r1_)), r1(r1_) {
+    x = nodes(N);
+    r.resize(N);
+    for (int q = 0; q < N; ++q) r[q] = r1 + (x[q] + 1) / df;
+    B = basis_at(x, N);
+  }
+  // (test_i, c(r) d^m trial_j / dr^m); test/trial: 'D' or 'P'
+  template <class Coef>
+  Mat form(char test, char trial, int m, Coef c) const {
+    const std::vector<LD> &vt = (test == 'D') ? B.D[0] : B.P[0];
+    const std::vector<LD> &vu = (trial == 'D') ? B.D[m] : B.P[m];
+    const LD scale = (PI / N) / df * powl(df, (LD)m);
+    Mat out((size_t)n * n, 0);
+    for (int q = 0; q < N; ++q) {
+      const LD cq = c(r[q]) * scale;
+      for (int i = 0; i < n; ++i) {
+        const LD a = vt[(size_t)q * n + i] * cq;
+        if (a == 0) continue;
+        for (int j = 0; j < n; ++j) out[(size_t)i * n + j] += a * vu[(size_t)q * n + j];
+      }
+    }
+    return out;
+  }
+};
+
+LD one(LD) { return 1; }
+
+// Gauss-Jordan inverse with partial pivoting, complex long double.
+bool invert(std::vector<CL> &A, int n) {
+  std::vector<CL> I((size_t)n * n, CL(0));
+  for (int i = 0; i < n; ++i) I[(size_t)i * n + i] = 1;
+  for (int c = 0; c < n; ++c) {
+    int piv = c;
+    LD best = std::abs(A[(size_t)c * n + c]);
+    for (int r = c + 1; r < n; ++r) {
+      const LD v = std::abs(A[(size_t)r * n + c]);
+      if (v > best) {
+        best = v;
+        piv = r;
+      }
+    }
+    if (best == 0) return false;
+    if (piv != c)
+      for (int j = 0; j < n; ++j) {
+        std::swap(A[(size_t)c * n + j], A[(size_t)piv * n + j]);
+        std::swap(I[(size_t)c * n + j], I[(size_t)piv * n + j]);
+      }
+    const CL inv = CL(1) / A[(size_t)c * n + c];
+    for (int j = 0; j < n; ++j) {
+      A[(size_t)c * n + j] *= inv;
+      I[(size_t)c * n + j] *= inv;
+    }
+    for (int r = 0; r < n; ++r) {
+      if (r == c) continue;
+      const CL f = A[(size_t)r * n + c];
+      if (f == CL(0)) continue;
+      for (int j = 0; j < n; ++j) {
+        A[(size_t)r * n + j] -= f * A[(size_t)c * n + j];
+        I[(size_t)r * n + j] -= f * I[(size_t)c * n + j];
+      }
+    }
+  }
+  A.swap(I);
+  return true;
+}
+
+struct Blocks {
+  Mat MD, D2, D1r, R2D, TwoOm, Gr, Gz0, Dr, Dz0;
+};
+
+Blocks blocks(const Forms &F, LD a, LD b) {
+  Blocks k;
+  k.MD = F.form('D', 'D', 0, one);
+  k.D2 = F.form('D', 'D', 2, one);
+  k.D1r = F.form('D', 'D', 1, [](LD r) { return 1 / r; });
+  k.R2D = F.form('D', 'D', 0, [](LD r) { return 1 / (r * r); });
+  k.TwoOm = F.form('D', 'D', 0, [&](LD r) { return 2 * (a + b / (r * r)); });
+  k.Gr = F.form('D', 'P', 1, one);
+  k.Gz0 = F.form('D', 'P', 0, one);
+  k.Dr = F.form('P', 'D', 1, one);
+  const Mat Dr1 = F.form('P', 'D', 0, [](LD r) { return 1 / r; });
+  for (size_t i = 0; i < k.Dr.size(); ++i) k.Dr[i] += Dr1[i];
+  k.Dz0 = F.form('P', 'D', 0, one);
+  return k;
+}
+
+// Implicit block of one mode at scale ls (examples/taylor_couette_dns_jax.py:539-569),
+// times Lz, with the k = 0 pressure constant pinned (:608-612).
+std::vector<CL> implicit_block(const Blocks &k, int n, LD kz, LD ls, const Params &p, bool pin) {
+  const int N4 = 4 * n;
+  std::vector<CL> A((size_t)N4 * N4, CL(0));
+  const LD nu = p.nu, dt = p.dt, lz = p.lz;
+  auto at = [&](int bi, int bj, int i, int j) -> CL & { return A[(size_t)(bi * n + i) * N4 + bj * n + j]; };
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j) {
+      const size_t o = (size_t)i * n + j;
+      const LD lap = k.D2[o] + k.D1r[o] - kz * kz * k.MD[o];
+      const LD diag_rt = k.MD[o] / dt - ls * nu * lap + ls * nu * k.R2D[o];
+      at(0, 0, i, j) = lz * diag_rt;
+      at(1, 1, i, j) = lz * diag_rt;
+      at(2, 2, i, j) = lz * (k.MD[o] / dt - ls * nu * lap);
+      at(0, 1, i, j) = lz * (-ls * k.TwoOm[o]);
+      at(1, 0, i, j) = lz * (-ls * (-2 * (LD)p.a) * k.MD[o]);
+      at(0, 3, i, j) = lz * k.Gr[o];
+      at(2, 3, i, j) = lz * CL(0, kz) * k.Gz0[o];
+      at(3, 0, i, j) = lz * k.Dr[o];
+      at(3, 2, i, j) = lz * CL(0, kz) * k.Dz0[o];
+    }
+  if (pin) {
+    const int row = 3 * n;
+    for (int j = 0; j < N4; ++j) A[(size_t)row * N4 + j] = 0;
+    A[(size_t)row * N4 + row] = 1;
+  }
+  return A;
+}
+
+// Explicit operator mass + ls L of one mode (3n x 3n, real), times Lz (:571-605);
+// with_mass = false gives L alone.
+std::vector<LD> explicit_block(const Blocks &k, int n, LD kz, LD ls, const Params &p, bool with_mass) {
+  const int N3 = 3 * n;
+  std::vector<LD> E((size_t)N3 * N3, 0);
+  const LD nu = p.nu, dt = p.dt, lz = p.lz, m = with_mass ? 1 : 0;
+  auto at = [&](int bi, int bj, int i, int j) -> LD & { return E[(size_t)(bi * n + i) * N3 + bj * n + j]; };
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j) {
+      const size_t o = (size_t)i * n + j;
+      const LD lap = k.D2[o] + k.D1r[o] - kz * kz * k.MD[o];
+      const LD rt = m * k.MD[o] / dt + ls * nu * lap - ls * nu * k.R2D[o];
+      at(0, 0, i, j) = lz * rt;
+      at(1, 1, i, j) = lz * rt;
+      at(2, 2, i, j) = lz * (m * k.MD[o] / dt + ls * nu * lap);
+      at(0, 1, i, j) = lz * ls * k.TwoOm[o];
+      at(1, 0, i, j) = lz * ls * (-2 * (LD)p.a) * k.MD[o];
+    }
+  return E;
+}
+
+bool inverse_columns(const Blocks &k, int n, LD kz, LD ls, const Params &p, bool pin, double *dst) {
+  std::vector<CL> A = implicit_block(k, n, kz, ls, p, pin);
+  const int N4 = 4 * n, N3 = 3 * n;
+  if (!invert(A, N4)) return false;
+  for (int i = 0; i < N4; ++i)
+    for (int j = 0; j < N3; ++j) {
+      const CL v = A[(size_t)i * N4 + j];
+      dst[2 * ((size_t)i * N3 + j)] = (double)v.real();
+      dst[2 * ((size_t)i * N3 + j) + 1] = (double)v.imag();
+    }
+  return true;
+}
+
+}  // namespace
+
+std::string build_ops(const Params &p, Ops &o) {
+  if (p.nr < 6) return "Nr must be at least 6";
+  if (p.nz < 2 || p.nz % 2) return "Nz must be even and at least 2";
+  if (!(p.r2 > p.r1 && p.r1 > 0)) return "require 0 < R1 < R2";
+  if (!(p.lz > 0) || !(p.dt > 0) || !(p.nu > 0)) return "Lz, dt and nu must be positive";
+  const int N = p.nr, n = N - 2;
+  o.n = n;
+  o.kh = p.nz / 2;
+  o.mr = p.dealias > 1.0 ? (int)(p.nr * p.dealias) : p.nr;
+  o.mz = p.dealias > 1.0 ? (int)(p.nz * p.dealias) : p.nz;
+  if (o.mr < N || o.mz < p.nz) return "invalid dealias factor";
+  const Forms F(N, p.r1, p.r2);
+  o.df = (double)F.df;
+  const Blocks K = blocks(F, p.a, p.b);
+  // nonlinear transforms
+  const std::vector<LD> xp = nodes(o.mr);
+  const Basis Bp = basis_at(xp, N);
+  o.r_pad.resize(o.mr);
+  o.Ev.assign((size_t)o.mr * n, 0);
+  o.Ed.assign((size_t)o.mr * n, 0);
+  o.Pr.assign((size_t)n * o.mr, 0);
+  for (int j = 0; j < o.mr; ++j) {
+    o.r_pad[j] = (double)(p.r1 + (xp[j] + 1) / F.df);
+    for (int i = 0; i < n; ++i) {
+      o.Ev[(size_t)j * n + i] = (double)Bp.D[0][(size_t)j * n + i];
+      o.Ed[(size_t)j * n + i] = (double)(F.df * Bp.D[1][(size_t)j * n + i]);
+      // the padded forward to N coefficients followed by the N-point scalar
+      // product is the mr-point Gauss quadrature of phi_i f (phi_i has modes
+      // i and i + 2 < N only)
+      o.Pr[(size_t)i * o.mr + j] = (double)((PI / o.mr) / F.df * Bp.D[0][(size_t)j * n + i]);
+    }
+  }
+  o.r_std.resize(N);
+  o.Es.assign((size_t)N * n, 0);
+  o.Esd.assign((size_t)N * n, 0);
+  for (int q = 0; q < N; ++q) {
+    o.r_std[q] = (double)F.r[q];
+    for (int i = 0; i < n; ++i) {
+      o.Es[(size_t)q * n + i] = (double)F.B.D[0][(size_t)q * n + i];
+      o.Esd[(size_t)q * n + i] = (double)(F.df * F.B.D[1][(size_t)q * n + i]);
+    }
+  }
+  // Fejer-I weights for the unweighted measure (src/jaxfun/galerkin/Chebyshev.py:183-209)
+  o.fejer.resize(N);
+  for (int q = 0; q < N; ++q) {
+    const LD th = PI + (2 * q + 1) * PI / (2 * N);
+    LD corr = 0;
+    for (int m = 1; m <= N / 2; ++m) corr += 2 * cosl(2 * th * m) / (4 * (LD)m * m - 1);
+    o.fejer[q] = (double)((2 / (LD)N) * (1 - corr) / F.df);
+  }
+  o.Mass.resize((size_t)n * n);
+  o.Drc.resize((size_t)n * n);
+  o.Dzc.resize((size_t)n * n);
+  for (size_t i = 0; i < o.Mass.size(); ++i) {
+    o.Mass[i] = (double)(p.lz * K.MD[i] / p.dt);
+    o.Drc[i] = (double)(p.lz * K.Dr[i]);
+    o.Dzc[i] = (double)(p.lz * K.Dz0[i]);
+  }
+  // per-mode operators
+  const int kh = o.kh, N3 = 3 * n, N4 = 4 * n;
+  o.kz.resize(kh);
+  o.Lin.assign((size_t)kh * N3 * N3, 0);
+  o.Lexp.assign((size_t)kh * N3 * N3, 0);
+  o.Ainv.assign((size_t)kh * N4 * N3 * 2, 0);
+  o.Ainv0.assign((size_t)kh * N4 * N3 * 2, 0);
+  const LD ls = (p.integrator == TC_TI_SBDF3) ? 6.0L / 11.0L : 0.5L;
+  for (int k = 0; k < kh; ++k) {
+    const LD kz = 2 * PI * k / p.lz;
+    o.kz[k] = (double)kz;
+    const std::vector<LD> L = explicit_block(K, n, kz, 1, p, false);
+    const std::vector<LD> E = explicit_block(K, n, kz, 0.5L, p, true);
+    for (size_t i = 0; i < L.size(); ++i) {
+      o.Lin[(size_t)k * N3 * N3 + i] = (double)L[i];
+      o.Lexp[(size_t)k * N3 * N3 + i] = (double)E[i];
+    }
+    if (!inverse_columns(K, n, kz, ls, p, k == 0, &o.Ainv[(size_t)k * N4 * N3 * 2]))
+      return "singular implicit block";
+    if (!inverse_columns(K, n, kz, 0.5L, p, k == 0, &o.Ainv0[(size_t)k * N4 * N3 * 2]))
+      return "singular start-up block";
+  }
+  return "";
+}
+
+std::string linear_pencil(const Params &p, int m, double kz_, std::vector<double> &Lout, std::vector<double> &Mout) {
+  if (p.nr < 6) return "Nr must be at least 6";
+  if (!(p.r2 > p.r1 && p.r1 > 0)) return "require 0 < R1 < R2";
+  const Forms F(p.nr, p.r1, p.r2);
+  const int n = F.n, N4 = 4 * n;
+  const LD a = p.a, b = p.b, mm = m, kz = kz_, nu = p.nu;
+  const Mat MD = F.form('D', 'D', 0, one);
+  const Mat D2 = F.form('D', 'D', 2, one);
+  const Mat D1r = F.form('D', 'D', 1, [](LD r) { return 1 / r; });
+  const Mat Kc = F.form('D', 'D', 0, [&](LD r) { return mm * mm / (r * r) + kz * kz; });
+  const Mat R2 = F.form('D', 'D', 0, [](LD r) { return 1 / (r * r); });
+  const Mat Adv = F.form('D', 'D', 0, [&](LD r) { return mm * (a + b / (r * r)); });
+  const Mat Two = F.form('D', 'D', 0, [&](LD r) { return 2 * a + 2 * b / (r * r); });
+  const Mat Sh = F.form('D', 'D', 0, [&](LD) { return 2 * a; });
+  const Mat Cp = F.form('D', 'D', 0, [&](LD r) { return 2 * mm / (r * r); });
+  const Mat Gr = F.form('D', 'P', 1, one);
+  const Mat Gt = F.form('D', 'P', 0, [&](LD r) { return mm / r; });
+  const Mat G0 = F.form('D', 'P', 0, one);
+  Mat Dr = F.form('P', 'D', 1, one);
+  const Mat Dr1 = F.form('P', 'D', 0, [](LD r) { return 1 / r; });
+  for (size_t i = 0; i < Dr.size(); ++i) Dr[i] += Dr1[i];
+  const Mat Dt = F.form('P', 'D', 0, [&](LD r) { return mm / r; });
+  const Mat D0 = F.form('P', 'D', 0, one);
+  std::vector<CL> L((size_t)N4 * N4, CL(0)), M((size_t)N4 * N4, CL(0));
+  auto put = [&](std::vector<CL> &X, int bi, int bj, int i, int j) -> CL & {
+    return X[(size_t)(bi * n + i) * N4 + bj * n + j];
+  };
+  const CL I(0, 1);
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j) {
+      const size_t o = (size_t)i * n + j;
+      const LD lap = D2[o] + D1r[o] - Kc[o];
+      const CL adv = -I * Adv[o], couple = I * Cp[o];
+      put(L, 0, 0, i, j) = adv + nu * (lap - R2[o]);
+      put(L, 0, 1, i, j) = Two[o] - nu * couple;
+      put(L, 0, 3, i, j) = -Gr[o];
+      put(L, 1, 0, i, j) = -Sh[o] + nu * couple;
+      put(L, 1, 1, i, j) = adv + nu * (lap - R2[o]);
+      put(L, 1, 3, i, j) = -I * Gt[o];
+      put(L, 2, 2, i, j) = adv + nu * lap;
+      put(L, 2, 3, i, j) = -I * kz * G0[o];
+      put(L, 3, 0, i, j) = Dr[o];
+      put(L, 3, 1, i, j) = I * Dt[o];
+      put(L, 3, 2, i, j) = I * kz * D0[o];
+      for (int c = 0; c < 3; ++c) put(M, c, c, i, j) = MD[o];
+    }
+  Lout.resize((size_t)N4 * N4 * 2);
+  Mout.resize((size_t)N4 * N4 * 2);
+  for (size_t i = 0; i < L.size(); ++i) {
+    Lout[2 * i] = (double)L[i].real();
+    Lout[2 * i + 1] = (double)L[i].imag();
+    Mout[2 * i] = (double)M[i].real();
+    Mout[2 * i + 1] = (double)M[i].imag();
+  }
+  return "";
+}
+
+}  // namespace tc

@@ -1,0 +1,49 @@
+// Host setup of the axisymmetric Taylor-Couette step (tc_setup.cpp): radial
+// Galerkin blocks, per-axial-mode saddle-point inverses and the transform
+// matrices of the nonlinear term, all formed in long double.
+#pragma once
+
+#include <complex>
+#include <string>
+#include <vector>
+
+namespace tc {
+
+struct Params {
+  int nr = 0, nz = 0;          // radial Chebyshev points, axial Fourier modes
+  double r1 = 1.0, r2 = 2.0;   // cylinder radii
+  double a = 0.0, b = 0.0;     // base flow V = a r + b / r
+  double nu = 0.0, lz = 0.0, dt = 0.0;
+  double dealias = 1.0;
+  int integrator = 0;          // TC_TI_* (tc_dns.h)
+};
+
+// Dense row-major matrices; complex ones as interleaved (re, im) doubles.
+struct Ops {
+  int n = 0;        // velocity (and pressure) dofs per mode: nr - 2
+  int kh = 0;       // stored axial modes 0..kh-1 (the Nyquist mode is not stored)
+  int mr = 0, mz = 0;  // padded radial points and axial points of the product grid
+  double df = 0.0;  // 2 / (r2 - r1)
+  std::vector<double> r_std, r_pad;       // radial nodes (ascending)
+  std::vector<double> fejer;              // Fejer-I weights / df at the standard nodes
+  std::vector<double> Ev, Ed;             // (mr x n): phi_i and d phi_i / dr at the padded nodes
+  std::vector<double> Es, Esd;            // (nr x n): the same at the standard nodes
+  std::vector<double> Pr;                 // (n x mr): dealiased scalar products of padded values
+  std::vector<double> Mass;               // (n x n): Lz (phi_i, phi_j) / dt
+  std::vector<double> Drc, Dzc;           // (n x n): continuity blocks (T_i, d phi_j/dr + phi_j/r), (T_i, phi_j), times Lz
+  std::vector<double> Lin;                // [kh][3n x 3n]: the linear operator L (ARS stages)
+  std::vector<double> Lexp;               // [kh][3n x 3n]: mass + L / 2 (CNAB2)
+  std::vector<double> Ainv;               // [kh][4n x 3n] complex: inverse of the pinned implicit block, velocity columns
+  std::vector<double> Ainv0;              // the same at the start-up scale 1/2 (SBDF3)
+  std::vector<double> kz;                 // kz of the stored modes
+};
+
+// Returns an empty string on success, else an error message.
+std::string build_ops(const Params &p, Ops &o);
+
+// The linear eigenproblem pencil of examples/taylor_couette_linear_jax.py:209-237
+// for azimuthal mode m and axial wavenumber kz: L = L0 + nu Lv and M, (4n x 4n)
+// complex, interleaved.
+std::string linear_pencil(const Params &p, int m, double kz, std::vector<double> &L, std::vector<double> &M);
+
+}  // namespace tc

@@ -1,0 +1,540 @@
+"""CPU FP64 restatement of the axisymmetric Taylor-Couette DNS step — TEST INFRASTRUCTURE ONLY.
+
+Nothing in the product imports this module: it is the checker of the HIP
+Taylor-Couette path (tests, ``__graft_entry__.smoke`` and the CPU baseline of
+``bench.py`` only).
+
+It restates ``AxisymmetricTCDNSJax`` (``examples/taylor_couette_dns_jax.py:217-1078``):
+the perturbation equations about the circular-Couette flow V(r) = a r + b / r
+in primitive variables (u_r, u_theta, u_z, p) on Fourier(z) x Chebyshev(r),
+Dirichlet velocity modes phi_k = T_k - T_{k+2} and the truncated orthogonal
+pressure space T_0..T_{N-3} (the P_N / P_{N-2} pair), with per-axial-mode
+coupled saddle-point blocks solved by pivoted LU after pinning the k = 0
+pressure constant; the SBDF3 step with the ARS(4,4,3) start-up
+(``:885-946``), IMEXRK3 alone (``:797-850``) and CNAB2 (``:758-795``).  The
+linear eigenproblem that seeds the golden run restates
+``TaylorCouetteLinearJax`` (``examples/taylor_couette_linear_jax.py:92-240``).
+
+Weak forms.  Every bilinear form (v, c(r) d^m u / dr^m) is the N-point
+Gauss-Chebyshev quadrature of the mapped integral, as jaxfun assembles a form
+with a non-polynomial coefficient at the test space's quadrature points
+(``src/jaxfun/galerkin/inner.py:306-349``; polynomial products of these
+degrees are exact at N points, so the same rule covers them):
+    (1/df) sum_q (pi/N) v(x_q) c(r_q) df^m u^(m)(x_q),  df = 2 / (R2 - R1),
+times the axial factor Lz (i kz)^m of the Fourier part.
+
+Pinned by the published golden run ``taylor_couette_hydro_dns_v1``
+(``production/goldens/taylor_couette_hydro_dns_v1``: linear growth rate, the
+seeded and final kinetic energy after 400 CNAB2 steps, the divergence) and
+by the checks of ``tests/test_tc_oracle.py``.
+"""
+
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, replace
+
+import numpy as np
+import scipy.linalg as sla
+from numpy.polynomial import chebyshev as npcheb
+
+from .kmm import (
+    cheb_backward,
+    cheb_derivative,
+    cheb_forward,
+    cheb_nodes,
+    cheb_scalar_product,
+    fejer_weights,
+    fourier_backward,
+    fourier_forward,
+    fourier_wavenumbers,
+    stencil_dirichlet,
+)
+
+# ARS(4,4,3) tableau (src/jaxfun/integrators/imex_rk.py:183-198)
+ARS443_A = np.array([
+    [0.0, 0.0, 0.0, 0.0, 0.0],
+    [0.0, 1.0 / 2.0, 0.0, 0.0, 0.0],
+    [0.0, 1.0 / 6.0, 1.0 / 2.0, 0.0, 0.0],
+    [0.0, -1.0 / 2.0, 1.0 / 2.0, 1.0 / 2.0, 0.0],
+    [0.0, 3.0 / 2.0, -3.0 / 2.0, 1.0 / 2.0, 1.0 / 2.0],
+])
+ARS443_B = np.array([
+    [0.0, 0.0, 0.0, 0.0, 0.0],
+    [1.0 / 2.0, 0.0, 0.0, 0.0, 0.0],
+    [11.0 / 18.0, 1.0 / 18.0, 0.0, 0.0, 0.0],
+    [5.0 / 6.0, -5.0 / 6.0, 1.0 / 2.0, 0.0, 0.0],
+    [1.0 / 4.0, 7.0 / 4.0, 3.0 / 4.0, -7.0 / 4.0, 0.0],
+])
+SBDF3_SCALE = 6.0 / 11.0  # src/jaxfun/integrators/sbdf3.py:11-12
+
+
+class CircularCouette:
+    """Base flow V = a r + b / r between cylinders R1 < R2 rotating at Omega1,
+    Omega2 (``examples/taylor_couette_linear_jax.py:31-76``)."""
+
+    def __init__(self, R1=1.0, R2=2.0, Omega1=1.0, Omega2=0.0):
+        if not (R2 > R1 > 0):
+            raise ValueError("require 0 < R1 < R2")
+        self.R1, self.R2 = float(R1), float(R2)
+        self.Omega1, self.Omega2 = float(Omega1), float(Omega2)
+        d2 = self.R2**2 - self.R1**2
+        self.a = (self.Omega2 * self.R2**2 - self.Omega1 * self.R1**2) / d2
+        self.b = (self.Omega1 - self.Omega2) * self.R1**2 * self.R2**2 / d2
+        self.gap = self.R2 - self.R1
+
+
+class RadialForms:
+    """Galerkin radial blocks on N Gauss-Chebyshev points of [R1, R2]."""
+
+    def __init__(self, N: int, R1: float, R2: float):
+        self.N = N
+        self.df = 2.0 / (R2 - R1)
+        self.x = cheb_nodes(N)
+        self.r = R1 + (self.x + 1.0) / self.df
+        self.V = npcheb.chebvander(self.x, N - 1)  # T_n(x_q)
+        self.SD = stencil_dirichlet(N)  # (N-2, N)
+        self.SP = np.eye(N)[: N - 2]  # T_0..T_{N-3}
+
+    def values(self, S, deriv=0):
+        """(q, j) values of the basis S (rows of orthogonal coefficients) and its
+        r-derivatives at the nodes."""
+        C = S.T.copy()
+        if deriv:
+            C = cheb_derivative(C, deriv) * self.df**deriv
+        return self.V @ C
+
+    def form(self, test, trial, deriv=0, coef=None):
+        c = np.ones(self.N) if coef is None else np.asarray(coef(self.r), dtype=float)
+        vt = self.values(test)
+        vu = self.values(trial, deriv)
+        return (math.pi / self.N) / self.df * (vt * c[:, None]).T @ vu
+
+
+@dataclass
+class TCState:
+    """``AxisymmetricTCState`` (``examples/taylor_couette_dns_jax.py:114-165``): full
+    complex coefficients, u components (Nz, Nr-2), p (Nz, Nr-2)."""
+
+    u: tuple
+    p: np.ndarray
+    nonlinear_old: tuple
+    have_old: float = 0.0
+    nonlinear_older: tuple | None = None
+    solution_old: tuple | None = None
+    solution_older: tuple | None = None
+    history_steps: float = 0.0
+
+    def leaves(self) -> list:
+        out = list(self.u) + [self.p] + list(self.nonlinear_old)
+        for grp in (self.nonlinear_older, self.solution_old, self.solution_older):
+            if grp is not None:
+                out += list(grp)
+        return out
+
+
+class TCOracle:
+    """``AxisymmetricTCDNSJax`` restated (``examples/taylor_couette_dns_jax.py:217-1078``)."""
+
+    def __init__(self, base: CircularCouette, nu=1.0e-2, Nr=32, Nz=16, Lz=None, dt=2.0e-3, dealias=1.5,
+                 time_integrator="CNAB2"):
+        self.base = base
+        self.nu = float(nu)
+        self.Nr, self.Nz = int(Nr), int(Nz)
+        self.dt = float(dt)
+        ti = str(time_integrator).upper()
+        if ti not in ("CNAB2", "SBDF3", "IMEXRK3"):
+            raise ValueError("time_integrator must be one of {'CNAB2', 'SBDF3', 'IMEXRK3'}")
+        self.time_integrator = ti
+        self.dealias = float(dealias)
+        self.Lz = float(Lz) if Lz is not None else 2.0 * math.pi / 3.13 * base.gap
+        self.rf = RadialForms(self.Nr, base.R1, base.R2)
+        self.Nd = self.Nr - 2
+        self.Np = self.Nr - 2
+        self.kint = fourier_wavenumbers(self.Nz)
+        self.kz = self.kint * (2.0 * math.pi / self.Lz)
+        self.kz_odd = fourier_wavenumbers(self.Nz, eliminate_highest=True) * (2.0 * math.pi / self.Lz)
+        if self.dealias > 1.0:
+            self.Mz = int(self.Nz * self.dealias)
+            self.Mr = int(self.Nr * self.dealias)
+        else:
+            self.Mz, self.Mr = self.Nz, self.Nr
+        self.r_pad = base.R1 + (cheb_nodes(self.Mr) + 1.0) / self.rf.df
+        self._radial_blocks()
+        self._build_time_operators()
+
+    # ------------------------------------------------------------ operators
+    def _radial_blocks(self):
+        rf, b = self.rf, self.base
+        SD, SP = rf.SD, rf.SP
+        self.MD = rf.form(SD, SD)
+        self.D2 = rf.form(SD, SD, 2)
+        self.D1r = rf.form(SD, SD, 1, lambda r: 1.0 / r)
+        self.R2D = rf.form(SD, SD, 0, lambda r: 1.0 / r**2)
+        self.TwoOm = rf.form(SD, SD, 0, lambda r: 2.0 * (b.a + b.b / r**2))
+        self.Gr = rf.form(SD, SP, 1)
+        self.Gz0 = rf.form(SD, SP, 0)
+        self.Dr = rf.form(SP, SD, 1) + rf.form(SP, SD, 0, lambda r: 1.0 / r)
+        self.Dz0 = rf.form(SP, SD, 0)
+
+    def _lap(self, kz):
+        return self.D2 + self.D1r - kz**2 * self.MD
+
+    def _build_operators(self, ls):
+        """Per-mode Limp (4n x 4n), Lexp and mass (3n x 3n), each times Lz
+        (``examples/taylor_couette_dns_jax.py:523-606``)."""
+        n, nu, dt, a = self.Nd, self.nu, self.dt, self.base.a
+        Lz = self.Lz
+        nm = self.Nz
+        limp = np.zeros((nm, 4 * n, 4 * n), dtype=complex)
+        lexp = np.zeros((nm, 3 * n, 3 * n), dtype=complex)
+        mass = np.zeros((nm, 3 * n, 3 * n), dtype=complex)
+        S = lambda i: slice(i * n, (i + 1) * n)  # noqa: E731
+        for m in range(nm):
+            kz = self.kz[m]
+            lap = self._lap(kz)
+            A = limp[m]
+            for i in range(3):
+                A[S(i), S(i)] += self.MD / dt
+            A[S(0), S(0)] += -ls * nu * lap + ls * nu * self.R2D
+            A[S(0), S(1)] += -ls * self.TwoOm
+            A[S(1), S(1)] += -ls * nu * lap + ls * nu * self.R2D
+            A[S(1), S(0)] += -ls * (-2.0 * a) * self.MD
+            A[S(2), S(2)] += -ls * nu * lap
+            A[S(0), S(3)] += self.Gr
+            A[S(2), S(3)] += 1j * kz * self.Gz0
+            A[S(3), S(0)] += self.Dr
+            A[S(3), S(2)] += 1j * kz * self.Dz0
+            E = lexp[m]
+            for i in range(3):
+                E[S(i), S(i)] += self.MD / dt
+                mass[m][S(i), S(i)] = self.MD / dt
+            E[S(0), S(0)] += ls * nu * lap - ls * nu * self.R2D
+            E[S(0), S(1)] += ls * self.TwoOm
+            E[S(1), S(1)] += ls * nu * lap - ls * nu * self.R2D
+            E[S(1), S(0)] += ls * (-2.0 * a) * self.MD
+            E[S(2), S(2)] += ls * nu * lap
+        return Lz * limp, Lz * lexp, Lz * mass
+
+    def _pin(self, limp):
+        """Pin the k = 0 pressure constant (``:608-612``)."""
+        limp = limp.copy()
+        row = 3 * self.Nd
+        limp[0, row, :] = 0.0
+        limp[0, row, row] = 1.0
+        return limp
+
+    def _factor(self, limp):
+        return [sla.lu_factor(A) for A in self._pin(limp)]
+
+    def _build_time_operators(self):
+        """``:303-338``."""
+        if self.time_integrator == "CNAB2":
+            self.Limp, self.Lexp, self.mass = self._build_operators(0.5)
+            self.Limp_lu = self._factor(self.Limp)
+            return
+        s_limp, s_lexp, mass = self._build_operators(0.5)
+        self.mass = mass
+        self.linear = (s_lexp - mass) / 0.5
+        self.startup_lu = self._factor(s_limp)
+        if self.time_integrator == "IMEXRK3":
+            self.Limp, self.Lexp, self.Limp_lu = s_limp, s_lexp, self.startup_lu
+            return
+        self.Limp, self.Lexp, _ = self._build_operators(SBDF3_SCALE)
+        self.Limp_lu = self._factor(self.Limp)
+
+    # --------------------------------------------------------------- transforms
+    def _to_orth(self, c):
+        return c @ self.rf.SD  # (Nz, Nr)
+
+    def _cheb_r(self, c, n):
+        """values along r (axis 1) of orthogonal coefficients at n Gauss points."""
+        return cheb_backward(c.T, n).T
+
+    def _phys(self, c):
+        """u, du/dr, du/dz on the (Mz, Mr) grid (``:709-714``)."""
+        orth = self._to_orth(c)
+        val = self._cheb_r(fourier_backward(orth, 0, self.Mz), self.Mr)
+        dr = self.rf.df * self._cheb_r(fourier_backward(cheb_derivative(orth.T, 1).T, 0, self.Mz), self.Mr)
+        dz = self._cheb_r(fourier_backward(1j * self.kz_odd[:, None] * orth, 0, self.Mz), self.Mr)
+        return val, dr, dz
+
+    def _dealias_to_standard(self, v):
+        """``:703-707``: padded forward to the N coefficients, back on the N grid."""
+        if self.dealias <= 1.0:
+            return v
+        c = fourier_forward(v, 0, self.Nz)
+        c = cheb_forward(c.T, self.Nr).T
+        return self._cheb_r(fourier_backward(c, 0, self.Nz), self.Nr)
+
+    def scalar_product(self, v):
+        """TD.scalar_product of values on the (Nz, Nr) grid, Nyquist masked."""
+        c = self.Lz * fourier_forward(v, 0, self.Nz)
+        s = cheb_scalar_product(c.T, self.Nr).T / self.rf.df
+        out = s @ self.rf.SD.T
+        return self.mask(out)
+
+    def mask(self, c):
+        c = np.array(c)
+        if self.Nz % 2 == 0:
+            c[self.Nz // 2] = 0.0
+        return c
+
+    def forward_D(self, v):
+        """TD.forward of (Nz, Nr) values: Fourier forward, then the Dirichlet
+        mass solve of the weighted scalar products."""
+        c = fourier_forward(v, 0, self.Nz)
+        s = cheb_scalar_product(c.T, self.Nr).T @ self.rf.SD.T
+        M = self.rf.SD @ np.diag(np.where(np.arange(self.Nr) == 0, math.pi, math.pi / 2)) @ self.rf.SD.T
+        return np.linalg.solve(M, s.T).T
+
+    def backward(self, c):
+        return self._cheb_r(fourier_backward(self._to_orth(c), 0, self.Nz), self.Nr)
+
+    # ---------------------------------------------------------------- physics
+    def nonlinear(self, u):
+        """inner(v_i, N_i) of the cylindrical advection (``:716-739``)."""
+        (ur, urr, urz), (ut, utr, utz), (uz, uzr, uzz) = (self._phys(c) for c in u)
+        invr = (1.0 / self.r_pad)[None, :]
+        n_r = ur * urr + uz * urz - ut * ut * invr
+        n_t = ur * utr + uz * utz + ur * ut * invr
+        n_z = ur * uzr + uz * uzz
+        return tuple(self.scalar_product(self._dealias_to_standard(f)) for f in (n_r, n_t, n_z))
+
+    def _modes(self, u):
+        return np.concatenate(list(u), axis=1)  # (Nz, 3n)
+
+    def _unmodes(self, x):
+        n = self.Nd
+        return tuple(x[:, i * n:(i + 1) * n] for i in range(3))
+
+    def apply(self, mats, u):
+        return self._unmodes(np.einsum("kij,kj->ki", mats, self._modes(u)))
+
+    def solve_rhs(self, rhs_u, lu):
+        """``:750-756``, ``:614-637``."""
+        rhs = np.concatenate([self._modes(rhs_u), np.zeros((self.Nz, self.Np), dtype=complex)], axis=1)
+        rhs[0, 3 * self.Nd] = 0.0
+        sol = np.stack([sla.lu_solve(lu[m], rhs[m]) for m in range(self.Nz)])
+        return self._unmodes(sol[:, :3 * self.Nd]), sol[:, 3 * self.Nd:]
+
+    # ------------------------------------------------------------------ states
+    def zero_state(self) -> TCState:
+        z = tuple(np.zeros((self.Nz, self.Nd), dtype=complex) for _ in range(3))
+        return TCState(u=z, p=np.zeros((self.Nz, self.Np), dtype=complex),
+                       nonlinear_old=tuple(np.zeros_like(a) for a in z))
+
+    def mesh(self):
+        z = np.arange(self.Nz) * self.Lz / self.Nz
+        return z[:, None], self.rf.r[None, :]
+
+    def state_from_physical(self, values) -> TCState:
+        u = tuple(self.forward_D(np.asarray(v, dtype=complex)) for v in values)
+        return TCState(u=u, p=np.zeros((self.Nz, self.Np), dtype=complex),
+                       nonlinear_old=tuple(np.zeros_like(a) for a in u))
+
+    def initial_state(self, amp=1.0e-3, kz_mode=1) -> TCState:
+        """Divergence-free streamfunction seed (``:651-667``)."""
+        R1, d = self.base.R1, self.base.gap
+        Z, R = self.mesh()
+        kz = 2.0 * math.pi * int(kz_mode) / self.Lz
+        arg = math.pi * (R - R1) / d
+        g = np.sin(arg) ** 2
+        gp = (2.0 * math.pi / d) * np.sin(arg) * np.cos(arg)
+        ur = amp * (kz / R) * g * np.sin(kz * Z)
+        uz = amp * (1.0 / R) * gp * np.cos(kz * Z)
+        ut = amp * np.sin(arg) * np.cos(kz * Z) + 0.0 * Z
+        return self.state_from_physical((ur, ut, uz))
+
+    def seed_linear_eigenmode(self, kz_mode=1, amp=1.0e-6, which=0):
+        """``:669-701``: the real part of the leading linear eigenmode."""
+        kz = 2.0 * math.pi * int(kz_mode) / self.Lz
+        lin = TCLinearOracle(self.base, nu=self.nu, N=self.Nr)
+        w, vecs = lin.eigs(0, kz, n_return=which + 1)
+        vec = positive_pivot_phase(vecs[:, which])
+        n = lin.n
+        st = self.zero_state()
+        comps = []
+        kpos, kneg = int(kz_mode) % self.Nz, (-int(kz_mode)) % self.Nz
+        for c in range(3):
+            block = vec[c * n:(c + 1) * n] * amp
+            arr = np.zeros((self.Nz, self.Nd), dtype=complex)
+            if kpos == kneg:
+                arr[kpos, :n] = np.real(block)
+            else:
+                arr[kpos, :n] = 0.5 * block
+                arr[kneg, :n] = 0.5 * np.conj(block)
+            comps.append(arr)
+        return replace(st, u=tuple(comps), nonlinear_old=tuple(np.zeros_like(a) for a in comps),
+                       have_old=0.0), complex(w[which])
+
+    # ------------------------------------------------------------- integrators
+    def step_cnab2(self, s: TCState) -> TCState:
+        """``:758-795``."""
+        N = self.nonlinear(s.u)
+        ex = self.apply(self.Lexp, s.u)
+        if s.have_old:
+            nab = tuple(1.5 * a - 0.5 * b for a, b in zip(N, s.nonlinear_old))
+        else:
+            nab = N
+        u, p = self.solve_rhs(tuple(e - b for e, b in zip(ex, nab)), self.Limp_lu)
+        return replace(s, u=u, p=p, nonlinear_old=N, have_old=1.0)
+
+    def step_imexrk3(self, s: TCState, lu) -> TCState:
+        """ARS(4,4,3) stages (``:797-850``, ``src/jaxfun/integrators/coupled.py:13-44``)."""
+        base_rhs = self.apply(self.mass, s.u)
+        fields, p = s.u, s.p
+        nh, lh = [], []
+        for rk in range(4):
+            nonl = self.nonlinear(fields)
+            nh.append(tuple(-v for v in nonl))
+            if rk > 0:
+                lh.append(self.apply(self.linear, fields))
+            rhs = list(base_rhs)
+            for j in range(rk + 1):
+                rhs = [r + ARS443_B[rk + 1, j] * v for r, v in zip(rhs, nh[j])]
+            for j in range(rk):
+                rhs = [r + ARS443_A[rk + 1, j + 1] * v for r, v in zip(rhs, lh[j])]
+            fields, p = self.solve_rhs(tuple(rhs), lu)
+        return replace(s, u=fields, p=p)
+
+    @staticmethod
+    def ensure_history(s: TCState) -> TCState:
+        """``:851-863``."""
+        z = tuple(np.zeros_like(v) for v in s.u)
+        return replace(s, nonlinear_older=z if s.nonlinear_older is None else s.nonlinear_older,
+                       solution_old=s.u if s.solution_old is None else s.solution_old,
+                       solution_older=s.u if s.solution_older is None else s.solution_older)
+
+    @staticmethod
+    def _shift(s: TCState, adv: TCState, cur) -> TCState:
+        """``:865-883``."""
+        return replace(adv, nonlinear_old=cur, nonlinear_older=s.nonlinear_old, solution_old=s.u,
+                       solution_older=s.solution_old, history_steps=min(2.0, s.history_steps + 1.0), have_old=1.0)
+
+    def step_sbdf3(self, s: TCState) -> TCState:
+        """``:885-946``."""
+        s = self.ensure_history(s)
+        cur = self.nonlinear(s.u)
+        if s.history_steps < 2:
+            return self._shift(s, self.step_imexrk3(s, self.startup_lu), cur)
+        m0, m1, m2 = (self.apply(self.mass, v) for v in (s.u, s.solution_old, s.solution_older))
+        rhs = tuple((18.0 * a - 9.0 * b + 2.0 * c) / 11.0 - (18.0 * x - 18.0 * y + 6.0 * z) / 11.0
+                    for a, b, c, x, y, z in zip(m0, m1, m2, cur, s.nonlinear_old, s.nonlinear_older))
+        u, p = self.solve_rhs(rhs, self.Limp_lu)
+        return self._shift(s, replace(s, u=u, p=p), cur)
+
+    def step(self, s: TCState) -> TCState:
+        if self.time_integrator == "CNAB2":
+            return self.step_cnab2(s)
+        if self.time_integrator == "IMEXRK3":
+            return self.step_imexrk3(s, self.Limp_lu)
+        return self.step_sbdf3(s)
+
+    def solve(self, s: TCState, steps: int) -> TCState:
+        if self.time_integrator == "SBDF3":
+            s = self.ensure_history(s)
+        for _ in range(int(steps)):
+            s = self.step(s)
+        return s
+
+    # ------------------------------------------------------------- diagnostics
+    def velocity_physical(self, s):
+        return tuple(self.backward(c) for c in s.u)
+
+    def _integrate(self, f):
+        w = (fejer_weights(self.Nr) / self.rf.df)[None, :] * (self.Lz / self.Nz)
+        return float(np.real(np.sum(f * w)))
+
+    def energy(self, s):
+        r = self.rf.r[None, :]
+        return self._integrate(0.5 * sum(np.real(np.conj(v) * v) for v in self.velocity_physical(s)) * r)
+
+    def divergence(self, s):
+        orth_r = self._to_orth(s.u[0])
+        dur = self.rf.df * self._cheb_r(fourier_backward(cheb_derivative(orth_r.T, 1).T, 0, self.Nz), self.Nr)
+        ur = self.backward(s.u[0])
+        duz = self._cheb_r(fourier_backward(1j * self.kz_odd[:, None] * self._to_orth(s.u[2]), 0, self.Nz), self.Nr)
+        return dur + ur / self.rf.r[None, :] + duz
+
+    def continuity_l2(self, s):
+        """``:1025-1034,1053-1059``."""
+        res = (self.Lz * (s.u[0] @ self.Dr.T)
+               + self.Lz * (1j * self.kz[:, None]) * (s.u[2] @ self.Dz0.T))
+        return float(np.linalg.norm(res))
+
+    def wall(self, s):
+        """Coefficient wall trace (``src/jaxfun/diagnostics.py:79-120``)."""
+        ends = npcheb.chebvander(np.array([-1.0, 1.0]), self.Nr - 1) @ self.rf.SD.T  # (2, n)
+        vals = [np.abs(fourier_backward(c @ ends.T, 0, self.Nz)) for c in s.u]
+        return float(max(np.max(v) for v in vals))
+
+    def diagnostics(self, s) -> dict:
+        """``:1061-1069``."""
+        vel = self.velocity_physical(s)
+        r = self.rf.r[None, :]
+        return {"E": self.energy(s), "div_linf": float(np.max(np.abs(self.divergence(s)))),
+                "continuity_l2": self.continuity_l2(s), "wall": self.wall(s),
+                "Eth": self._integrate(np.real(np.conj(vel[1]) * vel[1]) * r)}
+
+
+def positive_pivot_phase(vec):
+    """``examples/taylor_couette_dns_jax.py:99-111``."""
+    i = int(np.argmax(np.abs(vec)))
+    piv = vec[i]
+    pa = float(abs(piv))
+    if pa == 0.0:
+        return vec
+    if piv.real < 0.0 or (abs(piv.real) <= 1.0e-14 * pa and piv.imag < 0.0):
+        return -vec
+    return vec
+
+
+class TCLinearOracle:
+    """Axisymmetric-or-not linear eigenproblem L q = lambda M q
+    (``examples/taylor_couette_linear_jax.py:92-240``; ``src/jaxfun/la/eig.py:145-195``)."""
+
+    def __init__(self, base: CircularCouette, nu=1.0e-3, N=48):
+        self.base, self.nu, self.N = base, float(nu), int(N)
+        self.rf = RadialForms(self.N, base.R1, base.R2)
+        self.n = self.N - 2
+
+    def assemble(self, m, kz):
+        rf, b, n = self.rf, self.base, self.n
+        SD, SP = rf.SD, rf.SP
+        A = lambda deriv, coef=None: rf.form(SD, SD, deriv, coef)  # noqa: E731
+        lap = A(2) + A(1, lambda r: 1.0 / r) - A(0, lambda r: m**2 / r**2 + kz**2)
+        Mvv = A(0)
+        lv_diag = lap - A(0, lambda r: 1.0 / r**2)
+        adv = -1j * A(0, lambda r: m * (b.a + b.b / r**2)) if m else np.zeros((n, n), dtype=complex)
+        two_om = A(0, lambda r: 2 * b.a + 2 * b.b / r**2)
+        shear = A(0, lambda r: 2 * b.a + 0 * r)
+        couple = 1j * A(0, lambda r: 2 * m / r**2) if m else np.zeros((n, n), dtype=complex)
+        Gr = rf.form(SD, SP, 1)
+        Gt = 1j * rf.form(SD, SP, 0, lambda r: m / r) if m else np.zeros((n, n), dtype=complex)
+        Gz = 1j * kz * rf.form(SD, SP, 0)
+        Dr = rf.form(SP, SD, 1) + rf.form(SP, SD, 0, lambda r: 1.0 / r)
+        Dt = 1j * rf.form(SP, SD, 0, lambda r: m / r) if m else np.zeros((n, n), dtype=complex)
+        Dz = 1j * kz * rf.form(SP, SD, 0)
+        L0 = np.zeros((4 * n, 4 * n), dtype=complex)
+        Lv = np.zeros_like(L0)
+        M = np.zeros_like(L0)
+
+        def put(X, i, j, v):
+            X[i * n:(i + 1) * n, j * n:(j + 1) * n] = v
+
+        put(L0, 0, 0, adv), put(Lv, 0, 0, lv_diag), put(L0, 0, 1, two_om), put(Lv, 0, 1, -couple)
+        put(L0, 0, 3, -Gr), put(M, 0, 0, Mvv)
+        put(L0, 1, 0, -shear), put(Lv, 1, 0, couple), put(L0, 1, 1, adv), put(Lv, 1, 1, lv_diag)
+        put(L0, 1, 3, -Gt), put(M, 1, 1, Mvv)
+        put(L0, 2, 2, adv), put(Lv, 2, 2, lap), put(L0, 2, 3, -Gz), put(M, 2, 2, Mvv)
+        put(L0, 3, 0, Dr), put(L0, 3, 1, Dt), put(L0, 3, 2, Dz)
+        return L0 + self.nu * Lv, M
+
+    def eigs(self, m, kz, n_return=8, finite_cap=1.0e6):
+        L, M = self.assemble(m, kz)
+        w, V = sla.eig(L, M)
+        keep = np.isfinite(w) & (np.abs(w) < finite_cap)
+        w, V = w[keep], V[:, keep]
+        order = np.lexsort((-w.imag, -w.real))
+        return w[order][:n_return], V[:, order][:, :n_return]
