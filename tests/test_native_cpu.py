@@ -351,3 +351,37 @@ def test_gradient_rejects_unknown_fields():
     for f in (nat.FIELD_ID["U0"], nat.FIELD_ID["AX"], nat.FIELD_ID["A_OLDER_Z"]):
         assert lib.pcf_get_gradient(None, f, buf) == -1  # PCF_E_INVALID
         assert b"null" in lib.pcf_last_error()
+
+
+def test_taylor_couette_exports_and_pencil_match_oracle():
+    """include/tc_dns.h: every declared entry point is exported; the library's
+    long-double linear pencil (tc_linear_pencil, host only) equals the
+    golden-pinned oracle pencil (oracle/tc.py) for axisymmetric and m = 2
+    modes; bad configurations raise the reference's exception types."""
+    from oracle.tc import CircularCouette, TCLinearOracle
+
+    lib = nat.load()
+    header = (ROOT / "include" / "tc_dns.h").read_text()
+    declared = set(re.findall(r"^\s*(?:int32_t|int64_t|const char \*)\s*\*?\s*(tc_\w+)\s*\(", header, re.M))
+    assert declared == set(nat.TC_EXPORTS)
+    for name in declared:
+        assert hasattr(lib, name), name
+    cfg = nat.TcConfig()
+    cfg.abi_version = nat.TC_ABI_VERSION
+    cfg.nr, cfg.nz = 24, 8
+    cfg.r1, cfg.r2, cfg.omega1, cfg.omega2 = 1.0, 2.5, 1.0, -0.2
+    cfg.nu, cfg.lz, cfg.dt, cfg.dealias = 0.02, 2.0, 1e-3, 1.5
+    n4 = 4 * 22
+    dp = C.POINTER(C.c_double)
+    for m, kz in ((0, 3.13), (2, 1.7)):
+        L = np.empty((n4, n4), dtype=complex)
+        M = np.empty((n4, n4), dtype=complex)
+        nat.tc_check(lib.tc_linear_pencil(C.byref(cfg), m, kz, L.view(np.float64).ctypes.data_as(dp),
+                                          M.view(np.float64).ctypes.data_as(dp)))
+        Lo, Mo = TCLinearOracle(CircularCouette(1.0, 2.5, 1.0, -0.2), nu=0.02, N=24).assemble(m, kz)
+        assert np.max(np.abs(L - Lo)) <= 1e-13 * np.max(np.abs(Lo))
+        assert np.max(np.abs(M - Mo)) <= 1e-13 * np.max(np.abs(Mo))
+    cfg.r2 = 0.5
+    with pytest.raises(ValueError, match="R1 < R2"):
+        nat.tc_check(lib.tc_linear_pencil(C.byref(cfg), 0, 1.0, L.view(np.float64).ctypes.data_as(dp),
+                                          M.view(np.float64).ctypes.data_as(dp)))
