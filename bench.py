@@ -28,6 +28,7 @@ from __future__ import annotations
 import argparse
 import ctypes as C
 import json
+import math
 import os
 import sys
 import time
@@ -563,6 +564,59 @@ def adjoint_mhd_leg(reps: int = 3) -> dict:
             "forward_ms": 1e3 * tf, "value_and_grad_ms": 1e3 * ta, "ratio": ta / tf}
 
 
+TC_CASES = (
+    # production/examples/taylor_couette_hydro_dns_v1.json with the autograd
+    # benchmark's SBDF3 (production/autograd_cost_benchmark.py:64-71,396)
+    dict(name="tc_hydro 40x8", Nr=40, Nz=8, dealias=1.0),
+    dict(name="48x32 dealiased", Nr=48, Nz=32, dealias=1.5),
+)
+
+
+def tc_leg(steps: int = 2400, cpu_steps: int = 60) -> list:
+    """Axisymmetric Taylor-Couette SBDF3 steps/s on the GPU (graph replay of
+    12-step blocks, state resident) beside the oracle on the host cores, from
+    the golden case's seed (Re = 100, R1 = 1, R2 = 2, Omega1 = 1, Omega2 = 0,
+    Lz = 2 pi / 3.13, dt = 1e-3, amp 1e-6 leading eigenmode), and the growth
+    rate against the published linear one."""
+    from jaxfun_amd.tc import AxisymmetricTCDNSHIP, CircularCouette
+    from oracle.tc import CircularCouette as OBase
+    from oracle.tc import TCOracle, TCState
+
+    out = []
+    lz = 2.0 * math.pi / 3.13
+    for case in TC_CASES:
+        kw = dict(nu=0.01, Nr=case["Nr"], Nz=case["Nz"], Lz=lz, dt=1e-3, dealias=case["dealias"],
+                  time_integrator="SBDF3")
+        solver = AxisymmetricTCDNSHIP(CircularCouette(1.0, 2.0, 1.0, 0.0), **kw)
+        s0, eig = solver.seed_linear_eigenmode(kz_mode=1, amp=1e-6)
+        s = solver.solve(s0, 2)  # start-up steps
+        solver.load(s)
+        solver.advance(24)
+        t0 = time.perf_counter()
+        solver.advance(steps)
+        el = time.perf_counter() - t0  # advance returns after a stream synchronisation
+        fin = solver.fetch(s)
+        e0, e1 = solver.energy(s), solver.energy(fin)
+        growth = 0.5 * math.log(e1 / e0) / ((steps + 24) * solver.dt)
+        oracle = TCOracle(OBase(1.0, 2.0, 1.0, 0.0), **kw)
+        os_ = TCState(u=s.u, p=s.p, nonlinear_old=s.nonlinear_old, have_old=s.have_old,
+                      nonlinear_older=s.nonlinear_older, solution_old=s.solution_old,
+                      solution_older=s.solution_older, history_steps=s.history_steps)
+        oracle.solve(os_, 2)
+        t1 = time.perf_counter()
+        oracle.solve(os_, cpu_steps)
+        cpu = cpu_steps / (time.perf_counter() - t1)
+        out.append({"workload": f"axisymmetric Taylor-Couette SBDF3, {case['name']} (Nr={case['Nr']}, "
+                                f"Nz={case['Nz']}, dealias={case['dealias']}), Re=100, dt=1e-3, FP64",
+                    "steps_per_s": steps / el, "ms_per_step": 1e3 * el / steps, "steps": steps,
+                    "growth_rate": growth, "growth_rate_linear": eig.real,
+                    "cpu_oracle_steps_per_s": cpu, "cpu_sample": f"{cpu_steps} steady steps, numpy/scipy "
+                                                                 "oracle, one process",
+                    "gpu_vs_cpu": (steps / el) / cpu})
+        solver.close()
+    return out
+
+
 def adjoint_fd_check(n_dirs: int = 8, eps: float = 1e-6) -> dict:
     """FP64 gradient against central differences of the HIP forward along
     seeded divergence-free directions (same energy as the initial field), and
@@ -713,6 +767,9 @@ def main(argv=None) -> int:
     ap.add_argument("--no-mhd", action="store_true")
     ap.add_argument("--adjoint-only", action="store_true",
                     help="run only the adjoint timing leg (for kernel profiles)")
+    ap.add_argument("--tc-only", action="store_true",
+                    help="run only the Taylor-Couette leg (for kernel profiles)")
+    ap.add_argument("--no-tc", action="store_true")
     ap.add_argument("--mhd-adjoint-only", action="store_true",
                     help="run only the MHD value-and-gradient timing (for kernel profiles)")
     ap.add_argument("--reverse-only", action="store_true",
@@ -740,6 +797,9 @@ def main(argv=None) -> int:
         return 0
     if args.mhd_adjoint_only:
         print(json.dumps({"adjoint_mhd": adjoint_mhd_leg()}), flush=True)
+        return 0
+    if args.tc_only:
+        print(json.dumps({"taylor_couette": tc_leg()}), flush=True)
         return 0
     if args.steps < 1 or args.warmup < 0:
         print("--steps must be >= 1 and --warmup >= 0", file=sys.stderr)
@@ -817,6 +877,10 @@ def main(argv=None) -> int:
         line["large_mixed"] = {"workload": "plane-Couette SBDF3 rotational, 97x96x96 and 129x128x128, Re=2000, "
                                            "dt=0.002, E=1e-2 over the lowest 12 modes, mixed vs FP64",
                                "runs": large_mixed_leg(args.large_steps)}
+    if not args.no_tc:
+        line["taylor_couette"] = tc_leg()
+        print("# taylor-couette: " + ", ".join(f"{c['steps_per_s']:.0f} steps/s" for c in line["taylor_couette"]),
+              file=sys.stderr, flush=True)
     if not args.no_published:
         line["published_configs"] = published_leg()
     if args.copy_calibration:

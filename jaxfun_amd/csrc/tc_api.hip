@@ -53,10 +53,10 @@ struct tc_handle {
   tc::TcGeo geo{};
   hipStream_t stream = nullptr;
   std::vector<void *> allocs;
-  double *dEv = nullptr, *dEd = nullptr, *dPr = nullptr, *dinvr = nullptr, *dkz = nullptr, *dMass = nullptr,
-         *dLin = nullptr, *dLexp = nullptr, *dEs = nullptr, *dEsd = nullptr, *drstd = nullptr, *dfejer = nullptr,
+  double *dEv = nullptr, *dEd = nullptr, *dPrT = nullptr, *dinvr = nullptr, *dkz = nullptr, *dMassT = nullptr,
+         *dLinT = nullptr, *dLexpT = nullptr, *dEs = nullptr, *dEsd = nullptr, *drstd = nullptr, *dfejer = nullptr,
          *dDrc = nullptr, *dDzc = nullptr, *dout = nullptr;
-  double2 *dtwz = nullptr, *dtwn = nullptr, *dAinv = nullptr, *dAinv0 = nullptr;
+  double2 *dtwz = nullptr, *dtwn = nullptr, *dAinvT = nullptr, *dAinv0T = nullptr;
   double2 *S[4]{}, *N[3]{}, *P = nullptr, *G = nullptr, *XS[3]{}, *NS[3]{};
   int su = 0, s1 = 1, s2 = 2, sf = 3, nn1 = 0, nn2 = 1, nf = 2;
   double have_old = 0.0, history_steps = 0.0;
@@ -108,12 +108,13 @@ int32_t setup_ops(tc_handle *h) {
   if (!err.empty()) return fail(TC_E_INVALID, err);
   const tc::Ops &o = h->ops;
   int32_t r;
-  if ((r = h->upload(&h->dMass, o.Mass)) || (r = h->upload(&h->dLin, o.Lin)) || (r = h->upload(&h->dLexp, o.Lexp)) ||
-      (r = h->upload(&h->dAinv, o.Ainv)) || (r = h->upload(&h->dAinv0, o.Ainv0)))
+  if ((r = h->upload(&h->dMassT, o.MassT)) || (r = h->upload(&h->dLinT, o.LinT)) ||
+      (r = h->upload(&h->dLexpT, o.LexpT)) || (r = h->upload(&h->dAinvT, o.AinvT)) ||
+      (r = h->upload(&h->dAinv0T, o.Ainv0T)))
     return r;
-  h->geo.Mass = h->dMass;
-  h->geo.Lin = h->dLin;
-  h->geo.Lexp = h->dLexp;
+  h->geo.MassT = h->dMassT;
+  h->geo.LinT = h->dLinT;
+  h->geo.LexpT = h->dLexpT;
   return TC_OK;
 }
 
@@ -165,7 +166,7 @@ hipError_t solve(tc_handle *h, const double2 *u_nl, std::initializer_list<Term> 
   a.G = u_nl ? h->G : nullptr;
   a.cN = cN;
   a.nout = nout;
-  a.Ainv = Ainv;
+  a.AinvT = Ainv;
   a.uout = uout;
   a.pout = h->P;
   return tc::launch_rhs_solve(h->geo, a, h->stream);
@@ -212,7 +213,7 @@ hipError_t sbdf3_steady(tc_handle *h) {
   hipError_t e = solve(h, u,
                        {{TC_OP_MASS, u, 18.0 / 11.0}, {TC_OP_MASS, u1, -9.0 / 11.0}, {TC_OP_MASS, u2, 2.0 / 11.0},
                         {TC_OP_ID, h->N[h->nn1], 18.0 / 11.0}, {TC_OP_ID, h->N[h->nn2], -6.0 / 11.0}},
-                       -18.0 / 11.0, h->N[h->nf], h->dAinv, h->S[h->sf]);
+                       -18.0 / 11.0, h->N[h->nf], h->dAinvT, h->S[h->sf]);
   if (e != hipSuccess) return e;
   rotate_history(h);
   return hipSuccess;
@@ -225,10 +226,10 @@ int32_t one_step(tc_handle *h) {
     case TC_TI_CNAB2: {  // :758-795
       const double2 *u = h->S[h->su];
       if (h->have_old != 0.0)
-        e = solve(h, u, {{TC_OP_LEXP, u, 1.0}, {TC_OP_ID, h->N[h->nn1], 0.5}}, -1.5, h->N[h->nf], h->dAinv,
+        e = solve(h, u, {{TC_OP_LEXP, u, 1.0}, {TC_OP_ID, h->N[h->nn1], 0.5}}, -1.5, h->N[h->nf], h->dAinvT,
                   h->S[h->sf]);
       else
-        e = solve(h, u, {{TC_OP_LEXP, u, 1.0}}, -1.0, h->N[h->nf], h->dAinv, h->S[h->sf]);
+        e = solve(h, u, {{TC_OP_LEXP, u, 1.0}}, -1.0, h->N[h->nf], h->dAinvT, h->S[h->sf]);
       if (e != hipSuccess) break;
       std::swap(h->su, h->sf);
       std::swap(h->nn1, h->nf);
@@ -236,12 +237,12 @@ int32_t one_step(tc_handle *h) {
       break;
     }
     case TC_TI_IMEXRK3:  // :797-850: history untouched
-      e = ars_step(h, h->dAinv, h->NS[2]);
+      e = ars_step(h, h->dAinvT, h->NS[2]);
       if (e == hipSuccess) std::swap(h->su, h->sf);
       break;
     default:  // SBDF3 (:908-946)
       if (h->history_steps < 2.0) {
-        e = ars_step(h, h->dAinv0, h->N[h->nf]);
+        e = ars_step(h, h->dAinv0T, h->N[h->nf]);
         if (e == hipSuccess) rotate_history(h);
       } else {
         e = sbdf3_steady(h);
@@ -281,14 +282,19 @@ int32_t tc_create(const tc_config *cfg, tc_handle **out) {
     return r;
   }
   const tc::Ops &o = h->ops;
-  if (9 * (size_t)o.kh * sizeof(double2) + 12 * (size_t)o.mz * sizeof(double) > 65536 ||
-      256 * sizeof(double) + 4 * (size_t)o.kh * cfg->nr * sizeof(double2) > 65536) {
+  tc::TcGeo sz;
+  sz.n = o.n;
+  sz.kh = o.kh;
+  sz.mr = o.mr;
+  sz.mz = o.mz;
+  sz.nr = cfg->nr;
+  if (tc::nonlinear_lds(sz) > 65536 || tc::rhs_lds(sz) > 65536 || tc::diag_lds(sz) > 65536) {
     tc_destroy(h);
     return fail(TC_E_UNSUPPORTED, "grid too large for the per-block LDS tables");
   }
   std::vector<double> invr(o.mr), rstd(o.r_std);
   for (int j = 0; j < o.mr; ++j) invr[j] = 1.0 / o.r_pad[j];
-  if ((r = h->upload(&h->dEv, o.Ev)) || (r = h->upload(&h->dEd, o.Ed)) || (r = h->upload(&h->dPr, o.Pr)) ||
+  if ((r = h->upload(&h->dEv, o.Ev)) || (r = h->upload(&h->dEd, o.Ed)) || (r = h->upload(&h->dPrT, o.PrT)) ||
       (r = h->upload(&h->dinvr, invr)) || (r = h->upload(&h->dkz, o.kz)) || (r = h->upload(&h->dEs, o.Es)) ||
       (r = h->upload(&h->dEsd, o.Esd)) || (r = h->upload(&h->drstd, rstd)) || (r = h->upload(&h->dfejer, o.fejer)) ||
       (r = h->upload(&h->dDrc, o.Drc)) || (r = h->upload(&h->dDzc, o.Dzc)) ||
@@ -325,7 +331,7 @@ int32_t tc_create(const tc_config *cfg, tc_handle **out) {
   g.lz = cfg->lz;
   g.Ev = h->dEv;
   g.Ed = h->dEd;
-  g.Pr = h->dPr;
+  g.PrT = h->dPrT;
   g.invr = h->dinvr;
   g.kz = h->dkz;
   g.twz = h->dtwz;

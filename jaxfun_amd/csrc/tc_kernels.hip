@@ -5,18 +5,13 @@
 // (0..kh-1, the Hermitian half), i the radial Dirichlet index; the step is
 // small (Nr ~ 40, Nz ~ 8-32), so each kernel keeps one radial point (the
 // product kernel) or one axial mode (the solve kernel) per block with the
-// per-block data in LDS, and dot products run one wave per output row with
-// the lanes over the contracted index (coalesced reads of the row).
+// per-block data in LDS.
 #include "tc_kernels.hpp"
+
+#include <algorithm>
 
 namespace tc {
 namespace {
-
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
-}
 
 __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
   return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
@@ -37,16 +32,32 @@ __global__ void __launch_bounds__(256) k_tc_nonlinear(TcGeo g, const double2 *__
   double2 *val = reinterpret_cast<double2 *>(smem);        // [9][kh]: (c, v/dr/dz)
   double *phys = reinterpret_cast<double *>(val + 9 * kh);  // [9][mz]
   double *prod = phys + 9 * mz;                             // [3][mz]
+  double2 *part = reinterpret_cast<double2 *>(prod + 3 * mz);  // [nch][6 kh] partial sums
   const int tid = threadIdx.x;
-  for (int t = tid; t < 6 * kh; t += blockDim.x) {
-    const int c = t / (2 * kh), rem = t - c * 2 * kh, typ = rem / kh, k = rem - typ * kh;
+  // 6 kh dot products of length n (value and d/dr of three components at every
+  // mode), each split into nch chunks so that every thread has work
+  const int nd = 6 * kh, nch = max(1, (int)blockDim.x / nd), len = (n + nch - 1) / nch;
+  for (int t = tid; t < nd * nch; t += blockDim.x) {
+    const int d = t % nd, ch = t / nd;
+    const int c = d / (2 * kh), rem = d - c * 2 * kh, typ = rem / kh, k = rem - typ * kh;
     const double *E = (typ == 0 ? g.Ev : g.Ed) + (size_t)j * n;
     const double2 *x = u + ((size_t)c * kh + k) * n;
+    const int i0 = ch * len, i1 = min(n, i0 + len);
     double sr = 0.0, si = 0.0;
-    for (int i = 0; i < n; ++i) {
+    for (int i = i0; i < i1; ++i) {
       sr = fma(E[i], x[i].x, sr);
       si = fma(E[i], x[i].y, si);
     }
+    part[t] = make_double2(sr, si);
+  }
+  __syncthreads();
+  for (int d = tid; d < nd; d += blockDim.x) {
+    double sr = 0.0, si = 0.0;
+    for (int ch = 0; ch < nch; ++ch) {
+      sr += part[ch * nd + d].x;
+      si += part[ch * nd + d].y;
+    }
+    const int c = d / (2 * kh), rem = d - c * 2 * kh, typ = rem / kh, k = rem - typ * kh;
     val[(c * 3 + typ) * kh + k] = make_double2(sr, si);
     if (typ == 0) val[(c * 3 + 2) * kh + k] = make_double2(-g.kz[k] * si, g.kz[k] * sr);  // i kz u
   }
@@ -89,95 +100,111 @@ __global__ void __launch_bounds__(256) k_tc_nonlinear(TcGeo g, const double2 *__
 
 // --------------------------------------------------------- k_tc_rhs_solve --
 // One block per stored axial mode k: N = Pr G (the dealiased scalar products
-// of the products, :733-739), the right-hand side of the integrator as a sum
-// of operator terms, then (u, p) = Ainv_k rhs with the pinned inverse of the
-// mode's saddle-point block (:614-637; the pressure rows of rhs are zero, so
-// only the velocity columns of the inverse are stored).
-__global__ void __launch_bounds__(256) k_tc_rhs_solve(TcGeo g, TcRhs a) {
+// of the products, :733-739), the right-hand side of the integrator, then
+// (u, p) = Ainv_k rhs with the pinned inverse of the mode's saddle-point block
+// (:614-637; the pressure rows of rhs are zero, so only the velocity columns
+// of the inverse are stored).  The right-hand side's terms are first combined
+// per operator (sum c_t X_t for the mass terms, the L terms, the explicit
+// terms), so each operator is applied once.  One thread per output row; the
+// tables are stored transposed, so consecutive rows read consecutive
+// addresses, and the vector operands sit in LDS.
+constexpr int RHS_THREADS = 256;
+
+__global__ void __launch_bounds__(RHS_THREADS) k_tc_rhs_solve(TcGeo g, TcRhs a) {
   extern __shared__ __align__(16) unsigned char smem[];
-  const int k = blockIdx.x, kh = g.kh, n = g.n, n3 = 3 * n, n4 = 4 * n;
-  double2 *Ncur = reinterpret_cast<double2 *>(smem);  // [3n]
-  double2 *xs = Ncur + n3;                              // [3n]
-  double2 *rhs = xs + n3;                               // [3n]
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
-  auto at = [&](const double2 *f, int row) {  // component row / n, radial row % n of mode k
-    const int c = row / n;
-    return f[((size_t)c * kh + k) * n + (row - c * n)];
-  };
-  if (a.G) {
-    for (int row = wave; row < n3; row += nw) {
-      const int c = row / n, i = row - c * n;
-      const double *P = g.Pr + (size_t)i * g.mr;
-      const double2 *Gc = a.G + ((size_t)c * kh + k) * g.mr;
-      double sr = 0.0, si = 0.0;
-      for (int jj = lane; jj < g.mr; jj += 64) {
-        sr = fma(P[jj], Gc[jj].x, sr);
-        si = fma(P[jj], Gc[jj].y, si);
-      }
-      sr = wave_sum(sr);
-      si = wave_sum(si);
-      if (lane == 0) {
-        Ncur[row] = make_double2(sr, si);
-        if (a.nout) a.nout[((size_t)c * kh + k) * n + i] = make_double2(sr, si);
-      }
-    }
-  }
-  __syncthreads();
-  for (int row = threadIdx.x; row < n3; row += blockDim.x)
-    rhs[row] = a.G ? make_double2(a.cN * Ncur[row].x, a.cN * Ncur[row].y) : make_double2(0.0, 0.0);
+  const int k = blockIdx.x, kh = g.kh, n = g.n, n3 = 3 * n, n4 = 4 * n, mr = g.mr;
+  double2 *gs = reinterpret_cast<double2 *>(smem);  // [3 mr]: G of mode k
+  double2 *xm = gs + 3 * mr;                        // [3n]: combined mass operand
+  double2 *xl = xm + n3;                            // [3n]: combined L / explicit operand
+  double2 *rhs = xl + n3;                           // [3n]
+  const int tid = threadIdx.x;
+  bool has_mass = false, has_lin = false;
+  int lop = TC_OP_LIN;
   for (int t = 0; t < a.nt; ++t) {
-    __syncthreads();
-    for (int row = threadIdx.x; row < n3; row += blockDim.x) xs[row] = at(a.x[t], row);
-    __syncthreads();
-    const double ct = a.c[t];
-    if (a.op[t] == TC_OP_ID) {
-      for (int row = threadIdx.x; row < n3; row += blockDim.x) {
-        rhs[row].x += ct * xs[row].x;
-        rhs[row].y += ct * xs[row].y;
-      }
-      continue;
-    }
-    // dense rows: the mass block of the row's component, or the 3n x 3n
-    // operator of mode k
-    const bool mass = a.op[t] == TC_OP_MASS;
-    const double *Mk = (a.op[t] == TC_OP_LIN ? g.Lin : g.Lexp) + (size_t)k * n3 * n3;
-    for (int row = wave; row < n3; row += nw) {
-      const int c = row / n, i = row - c * n;
-      const double *M = mass ? g.Mass + (size_t)i * n : Mk + (size_t)row * n3;
-      const double2 *x = mass ? xs + c * n : xs;
-      const int len = mass ? n : n3;
-      double sr = 0.0, si = 0.0;
-      for (int jj = lane; jj < len; jj += 64) {
-        sr = fma(M[jj], x[jj].x, sr);
-        si = fma(M[jj], x[jj].y, si);
-      }
-      sr = wave_sum(sr);
-      si = wave_sum(si);
-      if (lane == 0) {
-        rhs[row].x += ct * sr;
-        rhs[row].y += ct * si;
-      }
+    has_mass |= a.op[t] == TC_OP_MASS;
+    if (a.op[t] == TC_OP_LIN || a.op[t] == TC_OP_LEXP) {
+      has_lin = true;
+      lop = a.op[t];
     }
   }
+  if (a.G)
+    for (int t = tid; t < 3 * mr; t += blockDim.x) {
+      const int c = t / mr, jj = t - c * mr;
+      gs[t] = a.G[((size_t)c * kh + k) * mr + jj];
+    }
+  for (int row = tid; row < n3; row += blockDim.x) {
+    const int c = row / n;
+    const size_t o = ((size_t)c * kh + k) * n + (row - c * n);
+    double2 m = make_double2(0.0, 0.0), l = m, d = m;
+    for (int t = 0; t < a.nt; ++t) {
+      const double2 x = a.x[t][o];
+      const double ct = a.c[t];
+      double2 &dst = (a.op[t] == TC_OP_MASS) ? m : (a.op[t] == TC_OP_ID) ? d : l;
+      dst.x = fma(ct, x.x, dst.x);
+      dst.y = fma(ct, x.y, dst.y);
+    }
+    xm[row] = m;
+    xl[row] = l;
+    rhs[row] = d;
+  }
   __syncthreads();
-  const double2 *A = a.Ainv + (size_t)k * n4 * n3;
-  for (int row = wave; row < n4; row += nw) {
-    const double2 *Ar = A + (size_t)row * n3;
+  for (int row = tid; row < n3; row += blockDim.x) {
+    const int c = row / n, i = row - c * n;
+    double2 r = rhs[row];
+    if (a.G) {
+      const double2 *Gc = gs + c * mr;
+      double sr = 0.0, si = 0.0;
+#pragma unroll 8
+      for (int jj = 0; jj < mr; ++jj) {
+        const double p = g.PrT[(size_t)jj * n + i];
+        sr = fma(p, Gc[jj].x, sr);
+        si = fma(p, Gc[jj].y, si);
+      }
+      if (a.nout) a.nout[((size_t)c * kh + k) * n + i] = make_double2(sr, si);
+      r.x = fma(a.cN, sr, r.x);
+      r.y = fma(a.cN, si, r.y);
+    }
+    if (has_mass) {  // block diagonal: the mass of the row's component
+      const double2 *x = xm + c * n;
+      double sr = 0.0, si = 0.0;
+#pragma unroll 8
+      for (int jj = 0; jj < n; ++jj) {
+        const double mm = g.MassT[(size_t)jj * n + i];
+        sr = fma(mm, x[jj].x, sr);
+        si = fma(mm, x[jj].y, si);
+      }
+      r.x += sr;
+      r.y += si;
+    }
+    if (has_lin) {
+      const double *M = (lop == TC_OP_LIN ? g.LinT : g.LexpT) + (size_t)k * n3 * n3;
+      double sr = 0.0, si = 0.0;
+#pragma unroll 8
+      for (int jj = 0; jj < n3; ++jj) {
+        const double mm = M[(size_t)jj * n3 + row];
+        sr = fma(mm, xl[jj].x, sr);
+        si = fma(mm, xl[jj].y, si);
+      }
+      r.x += sr;
+      r.y += si;
+    }
+    rhs[row] = r;
+  }
+  __syncthreads();
+  const double2 *A = a.AinvT + (size_t)k * n3 * n4;  // [j < 3n][r < 4n]
+  for (int row = tid; row < n4; row += blockDim.x) {
     double sr = 0.0, si = 0.0;
-    for (int jj = lane; jj < n3; jj += 64) {
-      const double2 m = Ar[jj], r = rhs[jj];
+#pragma unroll 8
+    for (int jj = 0; jj < n3; ++jj) {
+      const double2 m = A[(size_t)jj * n4 + row], r = rhs[jj];
       sr += m.x * r.x - m.y * r.y;
       si += m.x * r.y + m.y * r.x;
     }
-    sr = wave_sum(sr);
-    si = wave_sum(si);
-    if (lane == 0) {
-      if (row < n3) {
-        const int c = row / n, i = row - c * n;
-        a.uout[((size_t)c * kh + k) * n + i] = make_double2(sr, si);
-      } else {
-        a.pout[(size_t)k * n + (row - n3)] = make_double2(sr, si);
-      }
+    if (row < n3) {
+      const int c = row / n, i = row - c * n;
+      a.uout[((size_t)c * kh + k) * n + i] = make_double2(sr, si);
+    } else {
+      a.pout[(size_t)k * n + (row - n3)] = make_double2(sr, si);
     }
   }
 }
@@ -288,21 +315,37 @@ __global__ void __launch_bounds__(256) k_tc_diag(TcGeo g, const double2 *__restr
 
 }  // namespace
 
+size_t nonlinear_lds(const TcGeo &g) {
+  const int nd = 6 * g.kh, nch = std::max(1, 256 / nd);
+  return 9 * (size_t)g.kh * sizeof(double2) + 12 * (size_t)g.mz * sizeof(double) + (size_t)nd * nch * sizeof(double2);
+}
+
+size_t rhs_lds(const TcGeo &g) { return (3 * (size_t)g.mr + 9 * (size_t)g.n) * sizeof(double2); }
+
+size_t diag_lds(const TcGeo &g) { return 256 * sizeof(double) + 4 * (size_t)g.kh * g.nr * sizeof(double2); }
+
 hipError_t launch_nonlinear(const TcGeo &g, const double2 *u, double2 *G, hipStream_t s) {
-  const size_t lds = 9 * g.kh * sizeof(double2) + 12 * (size_t)g.mz * sizeof(double);
+  const size_t lds = nonlinear_lds(g);
   hipLaunchKernelGGL(k_tc_nonlinear, dim3(g.mr), dim3(256), lds, s, g, u, G);
   return hipGetLastError();
 }
 
 hipError_t launch_rhs_solve(const TcGeo &g, const TcRhs &a, hipStream_t s) {
-  if (a.nt < 0 || a.nt > TC_TERMS || !a.Ainv || !a.uout || !a.pout) return hipErrorInvalidValue;
-  const size_t lds = 9 * (size_t)g.n * sizeof(double2);
-  hipLaunchKernelGGL(k_tc_rhs_solve, dim3(g.kh), dim3(256), lds, s, g, a);
+  if (a.nt < 0 || a.nt > TC_TERMS || !a.AinvT || !a.uout || !a.pout) return hipErrorInvalidValue;
+  bool lin = false, lexp = false;
+  for (int t = 0; t < a.nt; ++t) {
+    if (!a.x[t] || a.op[t] < TC_OP_ID || a.op[t] > TC_OP_LEXP) return hipErrorInvalidValue;
+    lin |= a.op[t] == TC_OP_LIN;
+    lexp |= a.op[t] == TC_OP_LEXP;
+  }
+  if (lin && lexp) return hipErrorInvalidValue;  // one combined operand per call
+  const size_t lds = rhs_lds(g);
+  hipLaunchKernelGGL(k_tc_rhs_solve, dim3(g.kh), dim3(RHS_THREADS), lds, s, g, a);
   return hipGetLastError();
 }
 
 hipError_t launch_diag(const TcGeo &g, const double2 *u, double *out, hipStream_t s) {
-  const size_t lds = 256 * sizeof(double) + 4 * (size_t)g.kh * g.nr * sizeof(double2);
+  const size_t lds = diag_lds(g);
   hipLaunchKernelGGL(k_tc_diag, dim3(1), dim3(256), lds, s, g, u, out);
   return hipGetLastError();
 }

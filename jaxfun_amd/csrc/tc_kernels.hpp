@@ -14,13 +14,13 @@ struct TcGeo {
   int nz = 0;         // standard axial points
   double lz = 0.0;
   const double *Ev = nullptr, *Ed = nullptr;  // (mr x n)
-  const double *Pr = nullptr;                 // (n x mr)
+  const double *PrT = nullptr;                // (mr x n): the projection, transposed
   const double *invr = nullptr;               // (mr)
   const double *kz = nullptr;                 // (kh) axial wavenumbers
   const double2 *twz = nullptr;               // (mz) e^{2 pi i q / mz}
   const double2 *twn = nullptr;               // (nz) e^{2 pi i q / nz}
-  const double *Mass = nullptr;               // (n x n)
-  const double *Lin = nullptr, *Lexp = nullptr;  // [kh][3n x 3n]
+  const double *MassT = nullptr;              // (n x n), transposed
+  const double *LinT = nullptr, *LexpT = nullptr;  // [kh][3n x 3n], transposed
   const double *Es = nullptr, *Esd = nullptr;    // (nr x n)
   const double *rstd = nullptr, *fejer = nullptr;  // (nr)
   const double *Drc = nullptr, *Dzc = nullptr;     // (n x n)
@@ -34,6 +34,8 @@ constexpr int TC_TERMS = 8;
 // rhs = sum_t c_t Op_t(X_t) + cN N, N = Pr G (the projected nonlinear rows of
 // the G written by k_tc_nonlinear; stored to nout when set); then
 // (u, p) = Ainv rhs.  Fields are [component][k][i] with components kh n apart.
+// The terms of one call use at most one of TC_OP_LIN and TC_OP_LEXP (the
+// kernel applies each operator once to the combined operand).
 struct TcRhs {
   int nt = 0;
   int op[TC_TERMS];
@@ -42,7 +44,7 @@ struct TcRhs {
   const double2 *G = nullptr;
   double cN = 0.0;
   double2 *nout = nullptr;
-  const double2 *Ainv = nullptr;  // [kh][4n x 3n]
+  const double2 *AinvT = nullptr;  // [kh][3n x 4n]: the inverse's velocity columns, transposed
   double2 *uout = nullptr;
   double2 *pout = nullptr;
 };
@@ -51,5 +53,9 @@ hipError_t launch_nonlinear(const TcGeo &g, const double2 *u, double2 *G, hipStr
 hipError_t launch_rhs_solve(const TcGeo &g, const TcRhs &a, hipStream_t s);
 // out[0..4] = E, div_linf, continuity_l2, wall, Eth of u.
 hipError_t launch_diag(const TcGeo &g, const double2 *u, double *out, hipStream_t s);
+// Dynamic LDS bytes of the three kernels (the handle checks them at creation).
+size_t nonlinear_lds(const TcGeo &g);
+size_t rhs_lds(const TcGeo &g);
+size_t diag_lds(const TcGeo &g);
 
 }  // namespace tc

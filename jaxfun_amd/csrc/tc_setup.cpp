@@ -214,15 +214,16 @@ std::vector<LD> explicit_block(const Blocks &k, int n, LD kz, LD ls, const Param
   return E;
 }
 
+// The velocity columns of the inverse, transposed: dst[j][i] = inv(A)[i][j], j < 3n.
 bool inverse_columns(const Blocks &k, int n, LD kz, LD ls, const Params &p, bool pin, double *dst) {
   std::vector<CL> A = implicit_block(k, n, kz, ls, p, pin);
   const int N4 = 4 * n, N3 = 3 * n;
   if (!invert(A, N4)) return false;
-  for (int i = 0; i < N4; ++i)
+  for (int i = 0; i < N4; ++i)  // transposed: [j][i]
     for (int j = 0; j < N3; ++j) {
       const CL v = A[(size_t)i * N4 + j];
-      dst[2 * ((size_t)i * N3 + j)] = (double)v.real();
-      dst[2 * ((size_t)i * N3 + j) + 1] = (double)v.imag();
+      dst[2 * ((size_t)j * N4 + i)] = (double)v.real();
+      dst[2 * ((size_t)j * N4 + i) + 1] = (double)v.imag();
     }
   return true;
 }
@@ -249,7 +250,7 @@ std::string build_ops(const Params &p, Ops &o) {
   o.r_pad.resize(o.mr);
   o.Ev.assign((size_t)o.mr * n, 0);
   o.Ed.assign((size_t)o.mr * n, 0);
-  o.Pr.assign((size_t)n * o.mr, 0);
+  o.PrT.assign((size_t)o.mr * n, 0);
   for (int j = 0; j < o.mr; ++j) {
     o.r_pad[j] = (double)(p.r1 + (xp[j] + 1) / F.df);
     for (int i = 0; i < n; ++i) {
@@ -258,7 +259,7 @@ std::string build_ops(const Params &p, Ops &o) {
       // the padded forward to N coefficients followed by the N-point scalar
       // product is the mr-point Gauss quadrature of phi_i f (phi_i has modes
       // i and i + 2 < N only)
-      o.Pr[(size_t)i * o.mr + j] = (double)((PI / o.mr) / F.df * Bp.D[0][(size_t)j * n + i]);
+      o.PrT[(size_t)j * n + i] = (double)((PI / o.mr) / F.df * Bp.D[0][(size_t)j * n + i]);
     }
   }
   o.r_std.resize(N);
@@ -279,34 +280,37 @@ std::string build_ops(const Params &p, Ops &o) {
     for (int m = 1; m <= N / 2; ++m) corr += 2 * cosl(2 * th * m) / (4 * (LD)m * m - 1);
     o.fejer[q] = (double)((2 / (LD)N) * (1 - corr) / F.df);
   }
-  o.Mass.resize((size_t)n * n);
+  o.MassT.resize((size_t)n * n);
   o.Drc.resize((size_t)n * n);
   o.Dzc.resize((size_t)n * n);
-  for (size_t i = 0; i < o.Mass.size(); ++i) {
-    o.Mass[i] = (double)(p.lz * K.MD[i] / p.dt);
-    o.Drc[i] = (double)(p.lz * K.Dr[i]);
-    o.Dzc[i] = (double)(p.lz * K.Dz0[i]);
-  }
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j) {
+      const size_t ij = (size_t)i * n + j;
+      o.MassT[(size_t)j * n + i] = (double)(p.lz * K.MD[ij] / p.dt);
+      o.Drc[ij] = (double)(p.lz * K.Dr[ij]);
+      o.Dzc[ij] = (double)(p.lz * K.Dz0[ij]);
+    }
   // per-mode operators
   const int kh = o.kh, N3 = 3 * n, N4 = 4 * n;
   o.kz.resize(kh);
-  o.Lin.assign((size_t)kh * N3 * N3, 0);
-  o.Lexp.assign((size_t)kh * N3 * N3, 0);
-  o.Ainv.assign((size_t)kh * N4 * N3 * 2, 0);
-  o.Ainv0.assign((size_t)kh * N4 * N3 * 2, 0);
+  o.LinT.assign((size_t)kh * N3 * N3, 0);
+  o.LexpT.assign((size_t)kh * N3 * N3, 0);
+  o.AinvT.assign((size_t)kh * N4 * N3 * 2, 0);
+  o.Ainv0T.assign((size_t)kh * N4 * N3 * 2, 0);
   const LD ls = (p.integrator == TC_TI_SBDF3) ? 6.0L / 11.0L : 0.5L;
   for (int k = 0; k < kh; ++k) {
     const LD kz = 2 * PI * k / p.lz;
     o.kz[k] = (double)kz;
     const std::vector<LD> L = explicit_block(K, n, kz, 1, p, false);
     const std::vector<LD> E = explicit_block(K, n, kz, 0.5L, p, true);
-    for (size_t i = 0; i < L.size(); ++i) {
-      o.Lin[(size_t)k * N3 * N3 + i] = (double)L[i];
-      o.Lexp[(size_t)k * N3 * N3 + i] = (double)E[i];
-    }
-    if (!inverse_columns(K, n, kz, ls, p, k == 0, &o.Ainv[(size_t)k * N4 * N3 * 2]))
+    for (int i = 0; i < N3; ++i)
+      for (int j = 0; j < N3; ++j) {
+        o.LinT[(size_t)k * N3 * N3 + (size_t)j * N3 + i] = (double)L[(size_t)i * N3 + j];
+        o.LexpT[(size_t)k * N3 * N3 + (size_t)j * N3 + i] = (double)E[(size_t)i * N3 + j];
+      }
+    if (!inverse_columns(K, n, kz, ls, p, k == 0, &o.AinvT[(size_t)k * N4 * N3 * 2]))
       return "singular implicit block";
-    if (!inverse_columns(K, n, kz, 0.5L, p, k == 0, &o.Ainv0[(size_t)k * N4 * N3 * 2]))
+    if (!inverse_columns(K, n, kz, 0.5L, p, k == 0, &o.Ainv0T[(size_t)k * N4 * N3 * 2]))
       return "singular start-up block";
   }
   return "";

@@ -28,13 +28,15 @@ struct Ops {
   std::vector<double> fejer;              // Fejer-I weights / df at the standard nodes
   std::vector<double> Ev, Ed;             // (mr x n): phi_i and d phi_i / dr at the padded nodes
   std::vector<double> Es, Esd;            // (nr x n): the same at the standard nodes
-  std::vector<double> Pr;                 // (n x mr): dealiased scalar products of padded values
-  std::vector<double> Mass;               // (n x n): Lz (phi_i, phi_j) / dt
+  std::vector<double> PrT;                // (mr x n): dealiased scalar products of padded values, transposed
+  std::vector<double> MassT;              // (n x n): Lz (phi_i, phi_j) / dt, transposed
   std::vector<double> Drc, Dzc;           // (n x n): continuity blocks (T_i, d phi_j/dr + phi_j/r), (T_i, phi_j), times Lz
-  std::vector<double> Lin;                // [kh][3n x 3n]: the linear operator L (ARS stages)
-  std::vector<double> Lexp;               // [kh][3n x 3n]: mass + L / 2 (CNAB2)
-  std::vector<double> Ainv;               // [kh][4n x 3n] complex: inverse of the pinned implicit block, velocity columns
-  std::vector<double> Ainv0;              // the same at the start-up scale 1/2 (SBDF3)
+  // per-mode tables, each stored transposed (entry (i, j) at [j][i]) so that a
+  // thread per output row reads consecutive addresses
+  std::vector<double> LinT;               // [kh][3n x 3n]: the linear operator L (ARS stages)
+  std::vector<double> LexpT;              // [kh][3n x 3n]: mass + L / 2 (CNAB2)
+  std::vector<double> AinvT;              // [kh][3n x 4n] complex: velocity columns of the pinned implicit block's inverse
+  std::vector<double> Ainv0T;             // the same at the start-up scale 1/2 (SBDF3)
   std::vector<double> kz;                 // kz of the stored modes
 };
 
