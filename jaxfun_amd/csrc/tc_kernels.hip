@@ -105,17 +105,18 @@ __global__ void __launch_bounds__(256) k_tc_nonlinear(TcGeo g, const double2 *__
 // (:614-637; the pressure rows of rhs are zero, so only the velocity columns
 // of the inverse are stored).  The right-hand side's terms are first combined
 // per operator (sum c_t X_t for the mass terms, the L terms, the explicit
-// terms), so each operator is applied once.  One thread per output row; the
-// tables are stored transposed, so consecutive rows read consecutive
-// addresses, and the vector operands sit in LDS.
+// terms), so each operator is applied once.  One thread per right-hand-side
+// row; the tables are stored transposed, so consecutive rows read consecutive
+// addresses, and the vector operands sit in LDS.  gridDim.y blocks share a
+// mode: each forms the whole right-hand side and solves a slice of the rows.
 constexpr int RHS_THREADS = 256;
 
 __global__ void __launch_bounds__(RHS_THREADS) k_tc_rhs_solve(TcGeo g, TcRhs a) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int k = blockIdx.x, kh = g.kh, n = g.n, n3 = 3 * n, n4 = 4 * n, mr = g.mr;
   double2 *gs = reinterpret_cast<double2 *>(smem);  // [3 mr]: G of mode k
-  double2 *xm = gs + 3 * mr;                        // [3n]: combined mass operand
-  double2 *xl = xm + n3;                            // [3n]: combined L / explicit operand
+  double2 *xm = gs + 3 * mr;                        // [max(3n, threads)]: combined mass operand
+  double2 *xl = xm + max(n3, RHS_THREADS);          // [3n]: combined L / explicit operand
   double2 *rhs = xl + n3;                           // [3n]
   const int tid = threadIdx.x;
   bool has_mass = false, has_lin = false;
@@ -160,7 +161,7 @@ __global__ void __launch_bounds__(RHS_THREADS) k_tc_rhs_solve(TcGeo g, TcRhs a) 
         sr = fma(p, Gc[jj].x, sr);
         si = fma(p, Gc[jj].y, si);
       }
-      if (a.nout) a.nout[((size_t)c * kh + k) * n + i] = make_double2(sr, si);
+      if (a.nout && blockIdx.y == 0) a.nout[((size_t)c * kh + k) * n + i] = make_double2(sr, si);
       r.x = fma(a.cN, sr, r.x);
       r.y = fma(a.cN, si, r.y);
     }
@@ -191,15 +192,31 @@ __global__ void __launch_bounds__(RHS_THREADS) k_tc_rhs_solve(TcGeo g, TcRhs a) 
     rhs[row] = r;
   }
   __syncthreads();
+  // this block's slice of the 4n solution rows, the contracted index split
+  // into chunks over the threads (partial sums meet in LDS in a fixed order)
   const double2 *A = a.AinvT + (size_t)k * n3 * n4;  // [j < 3n][r < 4n]
-  for (int row = tid; row < n4; row += blockDim.x) {
+  const int per = (n4 + gridDim.y - 1) / gridDim.y, r0 = blockIdx.y * per, nrow = min(per, n4 - r0);
+  double2 *part = xm;  // the mass operand is no longer needed: [RHS_THREADS]
+  const int nch = max(1, (int)blockDim.x / max(nrow, 1)), clen = (n3 + nch - 1) / nch;
+  for (int t = tid; t < nrow * nch; t += blockDim.x) {
+    const int row = r0 + t % nrow, ch = t / nrow, j0 = ch * clen, j1 = min(n3, j0 + clen);
     double sr = 0.0, si = 0.0;
-#pragma unroll 8
-    for (int jj = 0; jj < n3; ++jj) {
+#pragma unroll 4
+    for (int jj = j0; jj < j1; ++jj) {
       const double2 m = A[(size_t)jj * n4 + row], r = rhs[jj];
       sr += m.x * r.x - m.y * r.y;
       si += m.x * r.y + m.y * r.x;
     }
+    part[t] = make_double2(sr, si);
+  }
+  __syncthreads();
+  for (int t = tid; t < nrow; t += blockDim.x) {
+    double sr = 0.0, si = 0.0;
+    for (int ch = 0; ch < nch; ++ch) {
+      sr += part[ch * nrow + t].x;
+      si += part[ch * nrow + t].y;
+    }
+    const int row = r0 + t;
     if (row < n3) {
       const int c = row / n, i = row - c * n;
       a.uout[((size_t)c * kh + k) * n + i] = make_double2(sr, si);
@@ -320,7 +337,13 @@ size_t nonlinear_lds(const TcGeo &g) {
   return 9 * (size_t)g.kh * sizeof(double2) + 12 * (size_t)g.mz * sizeof(double) + (size_t)nd * nch * sizeof(double2);
 }
 
-size_t rhs_lds(const TcGeo &g) { return (3 * (size_t)g.mr + 9 * (size_t)g.n) * sizeof(double2); }
+size_t rhs_lds(const TcGeo &g) {
+  // gs, xm (also the solve's partial sums), xl, rhs
+  return (3 * (size_t)g.mr + std::max(3 * (size_t)g.n, (size_t)RHS_THREADS) + 6 * (size_t)g.n) * sizeof(double2);
+}
+
+// Blocks per mode of the solve kernel: row slices of the 4n solution rows.
+constexpr int RHS_SLICES = 4;
 
 size_t diag_lds(const TcGeo &g) { return 256 * sizeof(double) + 4 * (size_t)g.kh * g.nr * sizeof(double2); }
 
@@ -340,7 +363,7 @@ hipError_t launch_rhs_solve(const TcGeo &g, const TcRhs &a, hipStream_t s) {
   }
   if (lin && lexp) return hipErrorInvalidValue;  // one combined operand per call
   const size_t lds = rhs_lds(g);
-  hipLaunchKernelGGL(k_tc_rhs_solve, dim3(g.kh), dim3(RHS_THREADS), lds, s, g, a);
+  hipLaunchKernelGGL(k_tc_rhs_solve, dim3(g.kh, RHS_SLICES), dim3(RHS_THREADS), lds, s, g, a);
   return hipGetLastError();
 }
 
