@@ -332,6 +332,23 @@ class AxisymmetricTCDNSHIP:
         """u on the standard (Nz, Nr) grid (``:1036-1037``), evaluated on the host."""
         return tuple(self._backward(c) for c in state.u)
 
+    def radial_velocity_linf(self, state: AxisymmetricTCState) -> float:
+        """max |u_r| on the standard grid (``production/oracles.py:4600-4604``)."""
+        return float(np.max(np.abs(self.velocity_physical(state)[0])))
+
+    def inner_torque(self, state: AxisymmetricTCState) -> float:
+        """Torque on the inner cylinder (``production/oracles.py:4620-4638``):
+        2 pi nu R1^2 |d(u_theta)/dr - u_theta / r + the base shear -2b / R1^2|
+        at r = R1, the axial mean.  The Dirichlet modes vanish at the wall and
+        T_k'(-1) = (-1)^(k+1) k^2."""
+        r1, b = self.base.R1, self.base.b
+        df = 2.0 / (self.base.R2 - self.base.R1)
+        k = np.arange(self.Nr, dtype=float)
+        dT = ((-1.0) ** (k + 1)) * k**2
+        dphi = df * (dT[: self.n] - dT[2:])
+        mean_dut = np.asarray(state.u[1])[0] @ dphi  # the k = 0 mode is the axial mean
+        return float(2.0 * math.pi * self.nu * r1**2 * abs(-2.0 * b / r1**2 + mean_dut))
+
     def growth_rate(self, state: AxisymmetricTCState, steps: int):
         """``:1071-1078``."""
         e0 = self.energy(state)

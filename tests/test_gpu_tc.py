@@ -140,3 +140,22 @@ def test_fixed_dt_and_integrator_rules():
     oracle.dt = 1e-3
     oracle._build_time_operators()
     _compare(cn, cn.solve(s0, 3), oracle.solve(_to_oracle(s0), 3))
+
+
+def test_runner_scalars_match_oracle():
+    """inner_torque and radial_velocity_linf against the oracle's physical
+    velocity and a direct evaluation of d(u_theta)/dr at the inner wall."""
+    from numpy.polynomial import chebyshev as npcheb
+
+    solver, oracle = _pair(ti="SBDF3")
+    s = solver.solve(solver.initial_state(amp=5e-2), 3)
+    ref_ur = np.max(np.abs(oracle.velocity_physical(_to_oracle(s))[0]))
+    assert abs(solver.radial_velocity_linf(s) - ref_ur) <= 1e-14 * ref_ur
+    df = 2.0
+    coef = np.zeros(solver.Nr, dtype=complex)
+    coef[: solver.n] += s.u[1][0]
+    coef[2:] -= s.u[1][0]
+    dut = df * npcheb.chebval(-1.0, npcheb.chebder(coef))
+    b = solver.base.b
+    ref = 2 * np.pi * solver.nu * abs(-2 * b + dut)
+    assert abs(solver.inner_torque(s) - ref) <= 1e-13 * ref
