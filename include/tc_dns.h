@@ -99,9 +99,10 @@ int32_t tc_get_scalars(tc_handle *h, double *have_old, double *history_steps);
  * configured integrator on the device state. */
 int32_t tc_step(tc_handle *h, int32_t nsteps);
 
-/* tc_step with the steady SBDF3 steps replayed from a captured HIP graph
- * (no reference counterpart: a launch-bound small step); start-up steps run
- * eagerly.  Same results as tc_step. */
+/* tc_step with blocks of 12 steady SBDF3 steps, or of CNAB2 steps after the
+ * first, replayed from captured HIP graphs (no reference counterpart: a
+ * launch-bound small step); start-up steps and remainders run eagerly.  Same
+ * results as tc_step. */
 int32_t tc_step_graph(tc_handle *h, int32_t nsteps);
 
 /* E, div_linf, continuity_l2, wall, Eth of the device state (TC_DIAG_*). */

@@ -219,23 +219,30 @@ hipError_t sbdf3_steady(tc_handle *h) {
   return hipSuccess;
 }
 
+// CNAB2 step (:758-795): explicit L/2 and the AB2 (first step: Euler) nonlinear rows.
+hipError_t cnab2_step(tc_handle *h) {
+  using namespace tc;
+  const double2 *u = h->S[h->su];
+  hipError_t e;
+  if (h->have_old != 0.0)
+    e = solve(h, u, {{TC_OP_LEXP, u, 1.0}, {TC_OP_ID, h->N[h->nn1], 0.5}}, -1.5, h->N[h->nf], h->dAinvT,
+              h->S[h->sf]);
+  else
+    e = solve(h, u, {{TC_OP_LEXP, u, 1.0}}, -1.0, h->N[h->nf], h->dAinvT, h->S[h->sf]);
+  if (e != hipSuccess) return e;
+  std::swap(h->su, h->sf);
+  std::swap(h->nn1, h->nf);
+  h->have_old = 1.0;
+  return hipSuccess;
+}
+
 int32_t one_step(tc_handle *h) {
   using namespace tc;
   hipError_t e = hipSuccess;
   switch (h->cfg.time_integrator) {
-    case TC_TI_CNAB2: {  // :758-795
-      const double2 *u = h->S[h->su];
-      if (h->have_old != 0.0)
-        e = solve(h, u, {{TC_OP_LEXP, u, 1.0}, {TC_OP_ID, h->N[h->nn1], 0.5}}, -1.5, h->N[h->nf], h->dAinvT,
-                  h->S[h->sf]);
-      else
-        e = solve(h, u, {{TC_OP_LEXP, u, 1.0}}, -1.0, h->N[h->nf], h->dAinvT, h->S[h->sf]);
-      if (e != hipSuccess) break;
-      std::swap(h->su, h->sf);
-      std::swap(h->nn1, h->nf);
-      h->have_old = 1.0;
+    case TC_TI_CNAB2:
+      e = cnab2_step(h);
       break;
-    }
     case TC_TI_IMEXRK3:  // :797-850: history untouched
       e = ars_step(h, h->dAinvT, h->NS[2]);
       if (e == hipSuccess) std::swap(h->su, h->sf);
@@ -422,19 +429,23 @@ int32_t tc_step_graph(tc_handle *h, int32_t nsteps) {
   constexpr int BLOCK = 12;  // the rings (4 and 3 slots) return to their roles after 12 steps
   int done = 0;
   while (done < nsteps) {
-    const bool steady = h->cfg.time_integrator == TC_TI_SBDF3 && h->history_steps >= 2.0;
+    // steady SBDF3 steps, and CNAB2 steps once the extrapolation has its old rows
+    const bool sbdf3 = h->cfg.time_integrator == TC_TI_SBDF3 && h->history_steps >= 2.0;
+    const bool cnab2 = h->cfg.time_integrator == TC_TI_CNAB2 && h->have_old != 0.0;
+    const bool steady = sbdf3 || cnab2;
     if (!steady || nsteps - done < BLOCK) {
       if (int32_t r = one_step(h)) return r;
       ++done;
       continue;
     }
-    const long key = ((((((long)h->su * 4 + h->s1) * 4 + h->s2) * 4 + h->sf) * 3 + h->nn1) * 3 + h->nn2) * 3 + h->nf;
+    const long key =
+        (((((((long)h->su * 4 + h->s1) * 4 + h->s2) * 4 + h->sf) * 3 + h->nn1) * 3 + h->nn2) * 3 + h->nf) * 2 + cnab2;
     auto it = h->graphs.find(key);
     if (it == h->graphs.end()) {
       hipGraph_t graph = nullptr;
       TC_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
       hipError_t e = hipSuccess;
-      for (int i = 0; i < BLOCK && e == hipSuccess; ++i) e = sbdf3_steady(h);
+      for (int i = 0; i < BLOCK && e == hipSuccess; ++i) e = sbdf3 ? sbdf3_steady(h) : cnab2_step(h);
       hipError_t ec = hipStreamEndCapture(h->stream, &graph);
       if (e != hipSuccess || ec != hipSuccess) {
         if (graph) (void)hipGraphDestroy(graph);
@@ -446,7 +457,14 @@ int32_t tc_step_graph(tc_handle *h, int32_t nsteps) {
       if (e != hipSuccess) return fail(TC_E_HIP, std::string("graph instantiate: ") + hipGetErrorString(e));
       it = h->graphs.emplace(key, exec).first;
     } else {
-      for (int i = 0; i < BLOCK; ++i) rotate_history(h);  // BLOCK rotations: the roles the graph leaves
+      for (int i = 0; i < BLOCK; ++i) {  // the roles the graph leaves (its steps rotate them BLOCK times)
+        if (sbdf3) {
+          rotate_history(h);
+        } else {
+          std::swap(h->su, h->sf);
+          std::swap(h->nn1, h->nf);
+        }
+      }
     }
     TC_HIP(hipGraphLaunch(it->second, h->stream));
     done += BLOCK;
@@ -488,6 +506,9 @@ int32_t tc_set_dt(tc_handle *h, double dt) {
   TC_HIP(hipStreamSynchronize(h->stream));
   h->cfg.dt = dt;
   h->prm.dt = dt;
+  // captured graphs hold the old tables' addresses
+  for (auto &kv : h->graphs) (void)hipGraphExecDestroy(kv.second);
+  h->graphs.clear();
   return setup_ops(h);
 }
 

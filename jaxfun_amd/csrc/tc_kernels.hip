@@ -259,51 +259,43 @@ __device__ double block_max(double v, double *red) {
 __global__ void __launch_bounds__(256) k_tc_diag(TcGeo g, const double2 *__restrict__ u, double *__restrict__ out) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int kh = g.kh, n = g.n, nr = g.nr, nz = g.nz;
-  double *red = reinterpret_cast<double *>(smem);               // [blockDim]
-  double2 *v = reinterpret_cast<double2 *>(red + blockDim.x);   // [4][kh][nr]: ur, ut, uz, dur/dr
+  double *red = reinterpret_cast<double *>(smem);              // [blockDim]
+  double2 *dv = reinterpret_cast<double2 *>(red + blockDim.x);  // [kh][nr]: the divergence per mode
   const int tid = threadIdx.x;
-  for (int t = tid; t < 4 * kh * nr; t += blockDim.x) {
-    const int f = t / (kh * nr), rem = t - f * kh * nr, k = rem / nr, q = rem - k * nr;
-    const double *E = (f == 3 ? g.Esd : g.Es) + (size_t)q * n;
-    const double2 *x = u + ((size_t)(f == 3 ? 0 : f) * kh + k) * n;
-    double sr = 0.0, si = 0.0;
-    for (int i = 0; i < n; ++i) {
-      sr = fma(E[i], x[i].x, sr);
-      si = fma(E[i], x[i].y, si);
-    }
-    v[t] = make_double2(sr, si);
-  }
-  __syncthreads();
   double e = 0.0, eth = 0.0;
   for (int t = tid; t < kh * nr; t += blockDim.x) {
     const int k = t / nr, q = t - k * nr;
+    const double *E = g.Es + (size_t)q * n, *Ed = g.Esd + (size_t)q * n;
+    double2 v[4];
+    for (int f = 0; f < 4; ++f) {
+      const double *M = f == 3 ? Ed : E;
+      const double2 *x = u + ((size_t)(f == 3 ? 0 : f) * kh + k) * n;
+      double sr = 0.0, si = 0.0;
+      for (int i = 0; i < n; ++i) {
+        sr = fma(M[i], x[i].x, sr);
+        si = fma(M[i], x[i].y, si);
+      }
+      v[f] = make_double2(sr, si);
+    }
     const double w = (k == 0 ? 1.0 : 2.0) * g.rstd[q] * g.fejer[q] * g.lz;
     double s = 0.0;
-    for (int c = 0; c < 3; ++c) {
-      const double2 a = v[(c * kh + k) * nr + q];
-      const double m2 = a.x * a.x + a.y * a.y;
-      s += m2;
-      if (c == 1) eth += w * m2;
-    }
+    for (int c = 0; c < 3; ++c) s += v[c].x * v[c].x + v[c].y * v[c].y;
     e += 0.5 * w * s;
+    eth += w * (v[1].x * v[1].x + v[1].y * v[1].y);
+    const double ir = 1.0 / g.rstd[q];
+    dv[t] = make_double2(v[3].x + v[0].x * ir - g.kz[k] * v[2].y, v[3].y + v[0].y * ir + g.kz[k] * v[2].x);
   }
   e = block_sum(e, red);
   eth = block_sum(eth, red);
   double dmax = 0.0;
   for (int t = tid; t < nz * nr; t += blockDim.x) {
     const int m = t / nr, q = t - m * nr;
-    double s0 = 0.0, s = 0.0;
-    for (int k = 0; k < kh; ++k) {
-      const double2 ur = v[(0 * kh + k) * nr + q], uz = v[(2 * kh + k) * nr + q], dr = v[(3 * kh + k) * nr + q];
-      const double2 d = make_double2(dr.x + ur.x / g.rstd[q] - g.kz[k] * uz.y, dr.y + ur.y / g.rstd[q] + g.kz[k] * uz.x);
-      if (k == 0) {
-        s0 = d.x;
-      } else {
-        const double2 w = g.twn[(k * m) % nz];
-        s += d.x * w.x - d.y * w.y;
-      }
+    double s = 0.0;
+    for (int k = 1; k < kh; ++k) {
+      const double2 d = dv[k * nr + q], w = g.twn[(k * m) % nz];
+      s += d.x * w.x - d.y * w.y;
     }
-    dmax = fmax(dmax, fabs(s0 + 2.0 * s));
+    dmax = fmax(dmax, fabs(dv[q].x + 2.0 * s));
   }
   dmax = block_max(dmax, red);
   double cres = 0.0;
@@ -345,7 +337,7 @@ size_t rhs_lds(const TcGeo &g) {
 // Blocks per mode of the solve kernel: row slices of the 4n solution rows.
 constexpr int RHS_SLICES = 4;
 
-size_t diag_lds(const TcGeo &g) { return 256 * sizeof(double) + 4 * (size_t)g.kh * g.nr * sizeof(double2); }
+size_t diag_lds(const TcGeo &g) { return 256 * sizeof(double) + (size_t)g.kh * g.nr * sizeof(double2); }
 
 hipError_t launch_nonlinear(const TcGeo &g, const double2 *u, double2 *G, hipStream_t s) {
   const size_t lds = nonlinear_lds(g);

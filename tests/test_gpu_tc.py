@@ -159,3 +159,47 @@ def test_runner_scalars_match_oracle():
     b = solver.base.b
     ref = 2 * np.pi * solver.nu * abs(-2 * b + dut)
     assert abs(solver.inner_torque(s) - ref) <= 1e-13 * ref
+
+
+def test_cnab2_graph_replay_equals_eager_steps():
+    solver, _ = _pair(ti="CNAB2", Nz=16)
+    s = solver.solve(solver.initial_state(amp=5e-2, kz_mode=1), 1)  # have_old = 1
+    from jaxfun_amd import _native as nat
+
+    solver.load(s)
+    nat.tc_check(solver._lib.tc_step(solver._h, 29))
+    eager = solver.fetch(s)
+    graph = solver.solve(s, 29)
+    for a, b in zip(eager.u + (eager.p,) + eager.nonlinear_old, graph.u + (graph.p,) + graph.nonlinear_old):
+        assert np.array_equal(a, b)
+
+
+def test_supercritical_saturation_against_float32_golden():
+    """production/goldens/tc_supercritical_saturation: Re = 100, Nr = 64,
+    Nz = 32, dealias 1.5, CNAB2, dt = 0.004, 20000 steps (t = 80) from the
+    amp = 1e-4 leading eigenmode into the saturated Taylor-vortex state.
+
+    That golden is a float32 GPU run of an earlier reference commit (its
+    metadata: production_run_dtype float32); its linear growth rate,
+    0.1273169, is 1.3e-5 away from the FP64 value that this build, the
+    oracle and the shenfun golden ``taylor_couette_hydro_dns_v1`` agree on to
+    1e-13, and its seeded energy is 12.6 % above the one its own seed rule
+    gives in FP64, so it pins nothing to FP64 accuracy.  What it does fix is
+    the saturated flow's torque on the inner cylinder and its largest radial
+    velocity, which this FP64 run reproduces to 2e-4 and 1e-2; the saturation
+    itself is checked by the golden's criterion (relative energy change over
+    the last 4 time units below 1e-2)."""
+    from jaxfun_amd.tc import AxisymmetricTCDNSHIP, CircularCouette
+
+    solver = AxisymmetricTCDNSHIP(CircularCouette(1.0, 2.0, 1.0, 0.0), nu=0.01, Nr=64, Nz=32, Lz=2.007407446944818,
+                                  dt=0.004, dealias=1.5, time_integrator="CNAB2")
+    s0, _ = solver.seed_linear_eigenmode(kz_mode=1, amp=1e-4)
+    s_a = solver.solve(s0, 19000)
+    s = solver.solve(s_a, 1000)
+    e_a, d = solver.energy(s_a), solver.diagnostics(s)
+    torque, urmax = solver.inner_torque(s), solver.radial_velocity_linf(s)
+    print("saturated", d["E"], torque, urmax, d["continuity_l2"], flush=True)
+    assert abs(d["E"] - e_a) <= 1e-2 * d["E"]
+    assert abs(torque / 0.21806299686431885 - 1.0) <= 2e-4
+    assert abs(urmax / 0.15969887375831604 - 1.0) <= 1e-2
+    assert d["continuity_l2"] < 1e-6 and d["div_linf"] < 1e-3
