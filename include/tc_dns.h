@@ -5,17 +5,19 @@
  * failure, with a message from tc_last_error().
  *
  * The step is AxisymmetricTCDNSJax of the reference
- * (examples/taylor_couette_dns_jax.py:217-1078): perturbations
+ * (examples/taylor_couette_dns_jax.py:217-1078; ntheta = 1) or its full-3D
+ * TaylorCouetteDNSJax (:1081-1356; ntheta > 1): perturbations
  * (u_r, u_theta, u_z, p) about the circular-Couette flow V = a r + b / r on
- * Fourier(z) x Chebyshev(r), Dirichlet velocity modes and the P_N / P_{N-2}
- * pressure pair, per-axial-mode coupled saddle-point blocks with the k = 0
- * pressure constant pinned.
+ * [Fourier(theta) x] Fourier(z) x Chebyshev(r), Dirichlet velocity modes and
+ * the P_N / P_{N-2} pressure pair, per-(m, kz) coupled saddle-point blocks
+ * with the (0, 0) pressure constant pinned.
  *
  * Layout at the boundary: every field is exchanged in the reference's
- * full-complex layout, C order (Nz, Nr - 2) of interleaved complex128 (re, im)
- * doubles, axis 0 the axial Fourier index in FFT order.  Internally the
- * library keeps the Hermitian half (kz index < Nz / 2); the Nyquist mode is
- * not stored (the reference masks it in every nonlinear row).
+ * full-complex layout, C order ([Ntheta,] Nz, Nr - 2) of interleaved
+ * complex128 (re, im) doubles, the Fourier axes in FFT order.  Internally the
+ * library keeps the Hermitian half (kz index < Nz / 2, every m); the axial
+ * Nyquist mode is not stored (the reference masks it in every nonlinear
+ * row).
  */
 #ifndef TC_DNS_H
 #define TC_DNS_H
@@ -26,7 +28,7 @@
 extern "C" {
 #endif
 
-#define TC_ABI_VERSION 1
+#define TC_ABI_VERSION 2
 
 #define TC_OK 0
 #define TC_E_INVALID (-1)
@@ -58,11 +60,12 @@ typedef struct tc_config {
   double dealias;        /* padding factor of the product grid (1 or 1.5) */
   int32_t time_integrator; /* TC_TI_*                                  */
   int32_t device;        /* HIP device ordinal                         */
+  int32_t ntheta;        /* azimuthal Fourier modes: 1 (axisymmetric) or even */
 } tc_config;
 
 typedef struct tc_handle tc_handle;
 
-/* field ids for tc_set_field / tc_get_field: (Nz, Nr - 2) complex each */
+/* field ids for tc_set_field / tc_get_field: ([Ntheta,] Nz, Nr - 2) complex each */
 #define TC_F_UR 0
 #define TC_F_UT 1
 #define TC_F_UZ 2

@@ -80,7 +80,7 @@ SEED_ENERGY = 0
 SEED_COTANGENT = 1
 
 # include/tc_dns.h (the Taylor-Couette step in the same library)
-TC_ABI_VERSION = 1
+TC_ABI_VERSION = 2
 TC_TI = {"CNAB2": 0, "SBDF3": 1, "IMEXRK3": 2}
 TC_FIELDS = ("UR", "UT", "UZ", "P", "NL_OLD_R", "NL_OLD_T", "NL_OLD_Z", "NL_OLDER_R", "NL_OLDER_T", "NL_OLDER_Z",
              "SOL_OLD_R", "SOL_OLD_T", "SOL_OLD_Z", "SOL_OLDER_R", "SOL_OLDER_T", "SOL_OLDER_Z")
@@ -107,6 +107,7 @@ class TcConfig(C.Structure):
         ("dealias", C.c_double),
         ("time_integrator", C.c_int32),
         ("device", C.c_int32),
+        ("ntheta", C.c_int32),
     ]
 
 

@@ -54,9 +54,10 @@ struct tc_handle {
   hipStream_t stream = nullptr;
   std::vector<void *> allocs;
   double *dEv = nullptr, *dEd = nullptr, *dPrT = nullptr, *dinvr = nullptr, *dkz = nullptr, *dMassT = nullptr,
-         *dLinT = nullptr, *dLexpT = nullptr, *dEs = nullptr, *dEsd = nullptr, *drstd = nullptr, *dfejer = nullptr,
+         *dmq = nullptr, *dmq_odd = nullptr, *dDtc = nullptr, *dEs = nullptr, *dEsd = nullptr, *drstd = nullptr, *dfejer = nullptr,
          *dDrc = nullptr, *dDzc = nullptr, *dout = nullptr;
-  double2 *dtwz = nullptr, *dtwn = nullptr, *dAinvT = nullptr, *dAinv0T = nullptr;
+  double2 *dtwz = nullptr, *dtwn = nullptr, *dtwt = nullptr, *dtwnt = nullptr, *dAinvT = nullptr, *dAinv0T = nullptr;
+  double2 *dLinT = nullptr, *dLexpT = nullptr, *ddv = nullptr;
   double2 *S[4]{}, *N[3]{}, *P = nullptr, *G = nullptr, *XS[3]{}, *NS[3]{};
   int su = 0, s1 = 1, s2 = 2, sf = 3, nn1 = 0, nn2 = 1, nf = 2;
   double have_old = 0.0, history_steps = 0.0;
@@ -99,6 +100,7 @@ tc::Params params_of(const tc_config &c) {
   p.dt = c.dt;
   p.dealias = c.dealias;
   p.integrator = c.time_integrator;
+  p.ntheta = c.ntheta;
   return p;
 }
 
@@ -111,7 +113,7 @@ int32_t setup_ops(tc_handle *h) {
   if ((r = h->upload(&h->dMassT, o.MassT)) || (r = h->upload(&h->dLinT, o.LinT)) ||
       (r = h->upload(&h->dLexpT, o.LexpT)) || (r = h->upload(&h->dAinvT, o.AinvT)) ||
       (r = h->upload(&h->dAinv0T, o.Ainv0T)))
-    return r;
+    return r;  // (complex tables as double2)
   h->geo.MassT = h->dMassT;
   h->geo.LinT = h->dLinT;
   h->geo.LexpT = h->dLexpT;
@@ -136,7 +138,7 @@ int32_t check_field(const tc_handle *h, int32_t field) {
 
 // Device pointer of a field id (one component, kh x n complex).
 double2 *field_ptr(tc_handle *h, int32_t f) {
-  const size_t cs = (size_t)h->ops.kh * h->ops.n;
+  const size_t cs = (size_t)h->ops.nm * h->ops.n;
   if (f <= TC_F_UZ) return h->S[h->su] + f * cs;
   if (f == TC_F_P) return h->P;
   if (f < TC_F_NL_OLDER) return h->N[h->nn1] + (f - TC_F_NL_OLD) * cs;
@@ -292,8 +294,11 @@ int32_t tc_create(const tc_config *cfg, tc_handle **out) {
   tc::TcGeo sz;
   sz.n = o.n;
   sz.kh = o.kh;
+  sz.nth = o.nth;
+  sz.nm = o.nm;
   sz.mr = o.mr;
   sz.mz = o.mz;
+  sz.mth = o.mth;
   sz.nr = cfg->nr;
   if (tc::nonlinear_lds(sz) > 65536 || tc::rhs_lds(sz) > 65536 || tc::diag_lds(sz) > 65536) {
     tc_destroy(h);
@@ -306,24 +311,26 @@ int32_t tc_create(const tc_config *cfg, tc_handle **out) {
       (r = h->upload(&h->dEsd, o.Esd)) || (r = h->upload(&h->drstd, rstd)) || (r = h->upload(&h->dfejer, o.fejer)) ||
       (r = h->upload(&h->dDrc, o.Drc)) || (r = h->upload(&h->dDzc, o.Dzc)) ||
       (r = h->upload(&h->dtwz, twiddles(o.mz))) || (r = h->upload(&h->dtwn, twiddles(cfg->nz))) ||
-      (r = h->alloc(&h->dout, 8))) {
+      (r = h->upload(&h->dtwt, twiddles(o.mth))) || (r = h->upload(&h->dtwnt, twiddles(o.nth))) ||
+      (r = h->upload(&h->dmq, o.mq)) || (r = h->upload(&h->dmq_odd, o.mq_odd)) || (r = h->upload(&h->dDtc, o.Dtc)) ||
+      (r = h->alloc(&h->ddv, (size_t)o.nm * cfg->nr)) || (r = h->alloc(&h->dout, 8))) {
     tc_destroy(h);
     return r;
   }
-  h->fsize = 3 * (size_t)o.kh * o.n;
+  h->fsize = 3 * (size_t)o.nm * o.n;
   for (int i = 0; i < 4 && !r; ++i) r = h->alloc(&h->S[i], h->fsize);
   for (int i = 0; i < 3 && !r; ++i) r = h->alloc(&h->N[i], h->fsize);
   for (int i = 0; i < 3 && !r; ++i) r = h->alloc(&h->XS[i], h->fsize);
   for (int i = 0; i < 3 && !r; ++i) r = h->alloc(&h->NS[i], h->fsize);
-  if (!r) r = h->alloc(&h->P, (size_t)o.kh * o.n);
-  if (!r) r = h->alloc(&h->G, 3 * (size_t)o.kh * o.mr);
+  if (!r) r = h->alloc(&h->P, (size_t)o.nm * o.n);
+  if (!r) r = h->alloc(&h->G, 3 * (size_t)o.nm * o.mr);
   if (r) {
     tc_destroy(h);
     return r;
   }
   for (int i = 0; i < 4; ++i) (void)hipMemset(h->S[i], 0, h->fsize * sizeof(double2));
   for (int i = 0; i < 3; ++i) (void)hipMemset(h->N[i], 0, h->fsize * sizeof(double2));
-  (void)hipMemset(h->P, 0, (size_t)o.kh * o.n * sizeof(double2));
+  (void)hipMemset(h->P, 0, (size_t)o.nm * o.n * sizeof(double2));
   if (hipDeviceSynchronize() != hipSuccess || hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
     tc_destroy(h);
     return fail(TC_E_HIP, "device setup failed");
@@ -331,8 +338,18 @@ int32_t tc_create(const tc_config *cfg, tc_handle **out) {
   tc::TcGeo &g = h->geo;
   g.n = o.n;
   g.kh = o.kh;
+  g.nth = o.nth;
+  g.nm = o.nm;
   g.mr = o.mr;
   g.mz = o.mz;
+  g.mth = o.mth;
+  g.fz = o.fz;
+  g.mq = h->dmq;
+  g.mq_odd = h->dmq_odd;
+  g.Dtc = h->dDtc;
+  g.twt = h->dtwt;
+  g.twn_t = h->dtwnt;
+  g.dv = h->ddv;
   g.nr = cfg->nr;
   g.nz = cfg->nz;
   g.lz = cfg->lz;
@@ -369,10 +386,14 @@ int32_t tc_destroy(tc_handle *h) {
 int32_t tc_set_field(tc_handle *h, int32_t field, const double *host) {
   if (int32_t r = check_field(h, field)) return r;
   if (!host) return fail(TC_E_INVALID, "null argument");
-  const int kh = h->ops.kh, n = h->ops.n;
+  const int kh = h->ops.kh, n = h->ops.n, nth = h->ops.nth, nz = h->cfg.nz;
   TC_HIP(hipSetDevice(h->cfg.device));
-  // the Hermitian half: rows k = 0..kh-1 of the full (Nz, n) array
-  TC_HIP(hipMemcpyAsync(field_ptr(h, field), host, (size_t)kh * n * sizeof(double2), hipMemcpyHostToDevice,
+  // the Hermitian half: rows k = 0..kh-1 of every azimuthal mode of the full
+  // (Ntheta, Nz, n) array, in the library's mode order q = im * kh + k
+  std::vector<double> half(2 * (size_t)h->ops.nm * n);
+  for (int im = 0; im < nth; ++im)
+    std::memcpy(&half[2 * (size_t)im * kh * n], host + 2 * (size_t)im * nz * n, 2 * (size_t)kh * n * sizeof(double));
+  TC_HIP(hipMemcpyAsync(field_ptr(h, field), half.data(), half.size() * sizeof(double), hipMemcpyHostToDevice,
                         h->stream));
   TC_HIP(hipStreamSynchronize(h->stream));
   return TC_OK;
@@ -381,19 +402,24 @@ int32_t tc_set_field(tc_handle *h, int32_t field, const double *host) {
 int32_t tc_get_field(tc_handle *h, int32_t field, double *host) {
   if (int32_t r = check_field(h, field)) return r;
   if (!host) return fail(TC_E_INVALID, "null argument");
-  const int kh = h->ops.kh, n = h->ops.n, nz = h->cfg.nz;
+  const int kh = h->ops.kh, n = h->ops.n, nth = h->ops.nth, nz = h->cfg.nz;
   TC_HIP(hipSetDevice(h->cfg.device));
-  std::vector<double> half(2 * (size_t)kh * n);
+  std::vector<double> half(2 * (size_t)h->ops.nm * n);
   TC_HIP(hipMemcpyAsync(half.data(), field_ptr(h, field), half.size() * sizeof(double), hipMemcpyDeviceToHost,
                         h->stream));
   TC_HIP(hipStreamSynchronize(h->stream));
-  std::memset(host, 0, 2 * (size_t)nz * n * sizeof(double));
-  std::memcpy(host, half.data(), half.size() * sizeof(double));
-  for (int k = 1; k < kh; ++k)  // f(-k) = conj f(k); the Nyquist row stays zero
-    for (int i = 0; i < n; ++i) {
-      double *dst = host + 2 * ((size_t)(nz - k) * n + i);
-      dst[0] = half[2 * ((size_t)k * n + i)];
-      dst[1] = -half[2 * ((size_t)k * n + i) + 1];
+  std::memset(host, 0, 2 * (size_t)nth * nz * n * sizeof(double));
+  for (int im = 0; im < nth; ++im)
+    for (int k = 0; k < kh; ++k) {
+      const double *src = &half[2 * ((size_t)im * kh + k) * n];
+      std::memcpy(host + 2 * ((size_t)im * nz + k) * n, src, 2 * (size_t)n * sizeof(double));
+      if (k == 0) continue;
+      // f(-m, -k) = conj f(m, k); the axial Nyquist row stays zero
+      double *dst = host + 2 * ((size_t)((nth - im) % nth) * nz + (nz - k)) * n;
+      for (int i = 0; i < n; ++i) {
+        dst[2 * i] = src[2 * i];
+        dst[2 * i + 1] = -src[2 * i + 1];
+      }
     }
   return TC_OK;
 }

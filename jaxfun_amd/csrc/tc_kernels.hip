@@ -113,7 +113,7 @@ constexpr int RHS_THREADS = 256;
 
 __global__ void __launch_bounds__(RHS_THREADS) k_tc_rhs_solve(TcGeo g, TcRhs a) {
   extern __shared__ __align__(16) unsigned char smem[];
-  const int k = blockIdx.x, kh = g.kh, n = g.n, n3 = 3 * n, n4 = 4 * n, mr = g.mr;
+  const int k = blockIdx.x, kh = g.nm, n = g.n, n3 = 3 * n, n4 = 4 * n, mr = g.mr;
   double2 *gs = reinterpret_cast<double2 *>(smem);  // [3 mr]: G of mode k
   double2 *xm = gs + 3 * mr;                        // [max(3n, threads)]: combined mass operand
   double2 *xl = xm + max(n3, RHS_THREADS);          // [3n]: combined L / explicit operand
@@ -177,14 +177,14 @@ __global__ void __launch_bounds__(RHS_THREADS) k_tc_rhs_solve(TcGeo g, TcRhs a) 
       r.x += sr;
       r.y += si;
     }
-    if (has_lin) {
-      const double *M = (lop == TC_OP_LIN ? g.LinT : g.LexpT) + (size_t)k * n3 * n3;
+    if (has_lin) {  // complex per-mode operator
+      const double2 *M = (lop == TC_OP_LIN ? g.LinT : g.LexpT) + (size_t)k * n3 * n3;
       double sr = 0.0, si = 0.0;
 #pragma unroll 8
       for (int jj = 0; jj < n3; ++jj) {
-        const double mm = M[(size_t)jj * n3 + row];
-        sr = fma(mm, xl[jj].x, sr);
-        si = fma(mm, xl[jj].y, si);
+        const double2 mm = M[(size_t)jj * n3 + row], x = xl[jj];
+        sr += mm.x * x.x - mm.y * x.y;
+        si += mm.x * x.y + mm.y * x.x;
       }
       r.x += sr;
       r.y += si;
@@ -258,18 +258,18 @@ __device__ double block_max(double v, double *red) {
 
 __global__ void __launch_bounds__(256) k_tc_diag(TcGeo g, const double2 *__restrict__ u, double *__restrict__ out) {
   extern __shared__ __align__(16) unsigned char smem[];
-  const int kh = g.kh, n = g.n, nr = g.nr, nz = g.nz;
-  double *red = reinterpret_cast<double *>(smem);              // [blockDim]
-  double2 *dv = reinterpret_cast<double2 *>(red + blockDim.x);  // [kh][nr]: the divergence per mode
+  const int kh = g.kh, nm = g.nm, n = g.n, nr = g.nr, nz = g.nz, nth = g.nth;
+  double *red = reinterpret_cast<double *>(smem);  // [blockDim]
+  double2 *dv = g.dv;                              // [nm][nr]: the divergence per mode (global scratch)
   const int tid = threadIdx.x;
   double e = 0.0, eth = 0.0;
-  for (int t = tid; t < kh * nr; t += blockDim.x) {
-    const int k = t / nr, q = t - k * nr;
-    const double *E = g.Es + (size_t)q * n, *Ed = g.Esd + (size_t)q * n;
+  for (int t = tid; t < nm * nr; t += blockDim.x) {
+    const int q = t / nr, r = t - q * nr, k = q % kh;
+    const double *E = g.Es + (size_t)r * n, *Ed = g.Esd + (size_t)r * n;
     double2 v[4];
     for (int f = 0; f < 4; ++f) {
       const double *M = f == 3 ? Ed : E;
-      const double2 *x = u + ((size_t)(f == 3 ? 0 : f) * kh + k) * n;
+      const double2 *x = u + ((size_t)(f == 3 ? 0 : f) * nm + q) * n;
       double sr = 0.0, si = 0.0;
       for (int i = 0; i < n; ++i) {
         sr = fma(M[i], x[i].x, sr);
@@ -277,39 +277,53 @@ __global__ void __launch_bounds__(256) k_tc_diag(TcGeo g, const double2 *__restr
       }
       v[f] = make_double2(sr, si);
     }
-    const double w = (k == 0 ? 1.0 : 2.0) * g.rstd[q] * g.fejer[q] * g.lz;
+    // Parseval over (theta, z): every stored k > 0 mode stands for its conjugate too
+    const double w = (k == 0 ? 1.0 : 2.0) * g.rstd[r] * g.fejer[r] * g.fz;
     double s = 0.0;
     for (int c = 0; c < 3; ++c) s += v[c].x * v[c].x + v[c].y * v[c].y;
     e += 0.5 * w * s;
     eth += w * (v[1].x * v[1].x + v[1].y * v[1].y);
-    const double ir = 1.0 / g.rstd[q];
-    dv[t] = make_double2(v[3].x + v[0].x * ir - g.kz[k] * v[2].y, v[3].y + v[0].y * ir + g.kz[k] * v[2].x);
+    const double ir = 1.0 / g.rstd[r], mo = g.mq_odd[q], kz = g.kz[q];
+    // d u_r / dr + u_r / r + (i m / r) u_theta + i kz u_z
+    dv[t] = make_double2(v[3].x + v[0].x * ir - mo * ir * v[1].y - kz * v[2].y,
+                         v[3].y + v[0].y * ir + mo * ir * v[1].x + kz * v[2].x);
   }
   e = block_sum(e, red);
   eth = block_sum(eth, red);
+  __threadfence_block();
+  __syncthreads();
   double dmax = 0.0;
-  for (int t = tid; t < nz * nr; t += blockDim.x) {
-    const int m = t / nr, q = t - m * nr;
+  for (int t = tid; t < nth * nz * nr; t += blockDim.x) {
+    const int a = t / (nz * nr), rem = t - a * nz * nr, b = rem / nr, r = rem - b * nr;
     double s = 0.0;
-    for (int k = 1; k < kh; ++k) {
-      const double2 d = dv[k * nr + q], w = g.twn[(k * m) % nz];
-      s += d.x * w.x - d.y * w.y;
+    for (int q = 0; q < nm; ++q) {
+      const int im = q / kh, k = q - im * kh;
+      const double2 d = dv[q * nr + r];
+      const int mi = (int)g.mq[q];
+      const double2 wt = g.twn_t[((mi * a) % nth + nth) % nth], wz = g.twn[(k * b) % nz];
+      const double2 w = make_double2(wt.x * wz.x - wt.y * wz.y, wt.x * wz.y + wt.y * wz.x);
+      s += (k == 0 ? 1.0 : 2.0) * (d.x * w.x - d.y * w.y);
     }
-    dmax = fmax(dmax, fabs(dv[q].x + 2.0 * s));
+    dmax = fmax(dmax, fabs(s));
   }
   dmax = block_max(dmax, red);
   double cres = 0.0;
-  for (int t = tid; t < kh * n; t += blockDim.x) {
-    const int k = t / n, i = t - k * n;
-    const double2 *ur = u + ((size_t)0 * kh + k) * n, *uz = u + ((size_t)2 * kh + k) * n;
-    double ar = 0.0, ai = 0.0, br = 0.0, bi = 0.0;
+  for (int t = tid; t < nm * n; t += blockDim.x) {
+    const int q = t / n, i = t - q * n, k = q % kh;
+    const double2 *ur = u + ((size_t)0 * nm + q) * n, *ut = u + ((size_t)1 * nm + q) * n,
+                  *uz = u + ((size_t)2 * nm + q) * n;
+    double ar = 0.0, ai = 0.0, br = 0.0, bi = 0.0, cr = 0.0, ci = 0.0;
     for (int jj = 0; jj < n; ++jj) {
-      ar = fma(g.Drc[(size_t)i * n + jj], ur[jj].x, ar);
-      ai = fma(g.Drc[(size_t)i * n + jj], ur[jj].y, ai);
-      br = fma(g.Dzc[(size_t)i * n + jj], uz[jj].x, br);
-      bi = fma(g.Dzc[(size_t)i * n + jj], uz[jj].y, bi);
+      const double dr = g.Drc[(size_t)i * n + jj], dz = g.Dzc[(size_t)i * n + jj], dt = g.Dtc[(size_t)i * n + jj];
+      ar = fma(dr, ur[jj].x, ar);
+      ai = fma(dr, ur[jj].y, ai);
+      br = fma(dz, uz[jj].x, br);
+      bi = fma(dz, uz[jj].y, bi);
+      cr = fma(dt, ut[jj].x, cr);
+      ci = fma(dt, ut[jj].y, ci);
     }
-    const double rr = ar - g.kz[k] * bi, ri = ai + g.kz[k] * br;
+    const double kz = g.kz[q], m = g.mq[q];
+    const double rr = ar - kz * bi - m * ci, ri = ai + kz * br + m * cr;
     cres += (k == 0 ? 1.0 : 2.0) * (rr * rr + ri * ri);
   }
   cres = block_sum(cres, red);
@@ -322,9 +336,115 @@ __global__ void __launch_bounds__(256) k_tc_diag(TcGeo g, const double2 *__restr
   }
 }
 
+// ------------------------------------------------------- k_tc3_nonlinear --
+// The 3D product kernel (:1234-1262): one block per padded radial point j.
+// u, du/dr, (1/r-free) du/dtheta and du/dz of the three components at every
+// stored (m, kz) mode, the azimuthal synthesis g_f,k(theta) = sum_m c e^{i m
+// theta} on mth points, the axial synthesis to the (mth x mz) grid (the
+// stored kz > 0 modes count twice for their conjugates), the products
+//   n_r = u_r du_r/dr + (u_t/r) du_r/dtheta + u_z du_r/dz - u_t^2 / r
+//   n_t = u_r du_t/dr + (u_t/r) du_t/dtheta + u_z du_t/dz + u_r u_t / r
+//   n_z = u_r du_z/dr + (u_t/r) du_z/dtheta + u_z du_z/dz,
+// then the axial and azimuthal analyses truncated to the stored modes, times
+// (2 pi / mth)(Lz / mz); the azimuthal Nyquist rows are masked.
+__global__ void __launch_bounds__(256) k_tc3_nonlinear(TcGeo g, const double2 *__restrict__ u,
+                                                       double2 *__restrict__ G) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int j = blockIdx.x, kh = g.kh, nm = g.nm, n = g.n, mz = g.mz, mth = g.mth, nth = g.nth;
+  double2 *val = reinterpret_cast<double2 *>(smem);  // [12][nm]; later prod [3][mth mz] and h [3][mth][kh]
+  const size_t region = max((size_t)12 * nm * 2, (size_t)3 * mth * mz + (size_t)3 * mth * kh * 2);
+  double2 *gs = reinterpret_cast<double2 *>(reinterpret_cast<double *>(smem) + region);  // [12][kh][mth]
+  double *prod = reinterpret_cast<double *>(smem);
+  double2 *h = reinterpret_cast<double2 *>(prod + 3 * mth * mz);
+  const int tid = threadIdx.x;
+  for (int d = tid; d < 6 * nm; d += blockDim.x) {
+    const int c = d / (2 * nm), rem = d - c * 2 * nm, typ = rem / nm, q = rem - typ * nm;
+    const double *E = (typ == 0 ? g.Ev : g.Ed) + (size_t)j * n;
+    const double2 *x = u + ((size_t)c * nm + q) * n;
+    double sr = 0.0, si = 0.0;
+    for (int i = 0; i < n; ++i) {
+      sr = fma(E[i], x[i].x, sr);
+      si = fma(E[i], x[i].y, si);
+    }
+    val[(c * 4 + typ) * nm + q] = make_double2(sr, si);
+    if (typ == 0) {
+      const double m = g.mq_odd[q], kz = g.kz[q];
+      val[(c * 4 + 2) * nm + q] = make_double2(-m * si, m * sr);    // i m u
+      val[(c * 4 + 3) * nm + q] = make_double2(-kz * si, kz * sr);  // i kz u
+    }
+  }
+  __syncthreads();
+  for (int t = tid; t < 12 * kh * mth; t += blockDim.x) {
+    const int f = t / (kh * mth), rem = t - f * kh * mth, k = rem / mth, a = rem - k * mth;
+    double sr = 0.0, si = 0.0;
+    for (int im = 0; im < nth; ++im) {
+      const double2 c = val[f * nm + im * kh + k];
+      const int mi = (int)g.mq[im * kh];
+      const double2 w = g.twt[((mi * a) % mth + mth) % mth];
+      sr += c.x * w.x - c.y * w.y;
+      si += c.x * w.y + c.y * w.x;
+    }
+    gs[t] = make_double2(sr, si);
+  }
+  __syncthreads();
+  const double ir = g.invr[j];
+  for (int p = tid; p < mth * mz; p += blockDim.x) {
+    const int a = p / mz, b = p - a * mz;
+    double v[12];
+#pragma unroll
+    for (int f = 0; f < 12; ++f) {
+      const double2 *gf = gs + (size_t)f * kh * mth;
+      double s = 0.0;
+      for (int k = 1; k < kh; ++k) {
+        const double2 c = gf[k * mth + a], w = g.twz[(k * b) % mz];
+        s += c.x * w.x - c.y * w.y;
+      }
+      v[f] = gf[a].x + 2.0 * s;
+    }
+    const double ur = v[0], urr = v[1], urt = v[2], urz = v[3];
+    const double ut = v[4], utr = v[5], utt = v[6], utz = v[7];
+    const double uz = v[8], uzr = v[9], uzt = v[10], uzz = v[11];
+    const double a_t = ut * ir;
+    prod[p] = ur * urr + a_t * urt + uz * urz - ut * ut * ir;
+    prod[mth * mz + p] = ur * utr + a_t * utt + uz * utz + ur * ut * ir;
+    prod[2 * mth * mz + p] = ur * uzr + a_t * uzt + uz * uzz;
+  }
+  __syncthreads();
+  for (int t = tid; t < 3 * mth * kh; t += blockDim.x) {
+    const int c = t / (mth * kh), rem = t - c * mth * kh, a = rem / kh, k = rem - a * kh;
+    const double *pr = prod + (size_t)c * mth * mz + (size_t)a * mz;
+    double sr = 0.0, si = 0.0;
+    for (int b = 0; b < mz; ++b) {
+      const double2 w = g.twz[(k * b) % mz];
+      sr = fma(pr[b], w.x, sr);
+      si = fma(-pr[b], w.y, si);
+    }
+    h[t] = make_double2(sr, si);  // [c][a][k]
+  }
+  __syncthreads();
+  const double sc = (2.0 * 3.141592653589793 / mth) * (g.lz / mz);
+  for (int t = tid; t < 3 * nm; t += blockDim.x) {
+    const int c = t / nm, q = t - c * nm, im = q / kh, k = q - im * kh;
+    double sr = 0.0, si = 0.0;
+    if (!(nth % 2 == 0 && im == nth / 2)) {  // the azimuthal Nyquist rows are masked
+      const int mi = (int)g.mq[q];
+      for (int a = 0; a < mth; ++a) {
+        const double2 hv = h[((size_t)c * mth + a) * kh + k], w = g.twt[((mi * a) % mth + mth) % mth];
+        sr += hv.x * w.x + hv.y * w.y;  // h e^{-i m theta_a}
+        si += hv.y * w.x - hv.x * w.y;
+      }
+    }
+    G[((size_t)c * nm + q) * g.mr + j] = make_double2(sc * sr, sc * si);
+  }
+}
+
 }  // namespace
 
 size_t nonlinear_lds(const TcGeo &g) {
+  if (g.nth > 1) {
+    const size_t region = std::max((size_t)12 * g.nm * 2, (size_t)3 * g.mth * g.mz + (size_t)3 * g.mth * g.kh * 2);
+    return (region + (size_t)12 * g.kh * g.mth * 2) * sizeof(double);
+  }
   const int nd = 6 * g.kh, nch = std::max(1, 256 / nd);
   return 9 * (size_t)g.kh * sizeof(double2) + 12 * (size_t)g.mz * sizeof(double) + (size_t)nd * nch * sizeof(double2);
 }
@@ -337,11 +457,14 @@ size_t rhs_lds(const TcGeo &g) {
 // Blocks per mode of the solve kernel: row slices of the 4n solution rows.
 constexpr int RHS_SLICES = 4;
 
-size_t diag_lds(const TcGeo &g) { return 256 * sizeof(double) + (size_t)g.kh * g.nr * sizeof(double2); }
+size_t diag_lds(const TcGeo &) { return 256 * sizeof(double); }
 
 hipError_t launch_nonlinear(const TcGeo &g, const double2 *u, double2 *G, hipStream_t s) {
   const size_t lds = nonlinear_lds(g);
-  hipLaunchKernelGGL(k_tc_nonlinear, dim3(g.mr), dim3(256), lds, s, g, u, G);
+  if (g.nth > 1)
+    hipLaunchKernelGGL(k_tc3_nonlinear, dim3(g.mr), dim3(256), lds, s, g, u, G);
+  else
+    hipLaunchKernelGGL(k_tc_nonlinear, dim3(g.mr), dim3(256), lds, s, g, u, G);
   return hipGetLastError();
 }
 
@@ -355,7 +478,7 @@ hipError_t launch_rhs_solve(const TcGeo &g, const TcRhs &a, hipStream_t s) {
   }
   if (lin && lexp) return hipErrorInvalidValue;  // one combined operand per call
   const size_t lds = rhs_lds(g);
-  hipLaunchKernelGGL(k_tc_rhs_solve, dim3(g.kh, RHS_SLICES), dim3(RHS_THREADS), lds, s, g, a);
+  hipLaunchKernelGGL(k_tc_rhs_solve, dim3(g.nm, RHS_SLICES), dim3(RHS_THREADS), lds, s, g, a);
   return hipGetLastError();
 }
 

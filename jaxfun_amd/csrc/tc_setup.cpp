@@ -144,7 +144,7 @@ bool invert(std::vector<CL> &A, int n) {
 }
 
 struct Blocks {
-  Mat MD, D2, D1r, R2D, TwoOm, Gr, Gz0, Dr, Dz0;
+  Mat MD, D2, D1r, R2D, TwoOm, Om, Gr, Gz0, Gt0, Dr, Dz0, Dt0;
 };
 
 Blocks blocks(const Forms &F, LD a, LD b) {
@@ -154,6 +154,9 @@ Blocks blocks(const Forms &F, LD a, LD b) {
   k.D1r = F.form('D', 'D', 1, [](LD r) { return 1 / r; });
   k.R2D = F.form('D', 'D', 0, [](LD r) { return 1 / (r * r); });
   k.TwoOm = F.form('D', 'D', 0, [&](LD r) { return 2 * (a + b / (r * r)); });
+  k.Om = F.form('D', 'D', 0, [&](LD r) { return a + b / (r * r); });
+  k.Gt0 = F.form('D', 'P', 0, [](LD r) { return 1 / r; });
+  k.Dt0 = F.form('P', 'D', 0, [](LD r) { return 1 / r; });
   k.Gr = F.form('D', 'P', 1, one);
   k.Gz0 = F.form('D', 'P', 0, one);
   k.Dr = F.form('P', 'D', 1, one);
@@ -163,27 +166,51 @@ Blocks blocks(const Forms &F, LD a, LD b) {
   return k;
 }
 
-// Implicit block of one mode at scale ls (examples/taylor_couette_dns_jax.py:539-569),
-// times Lz, with the k = 0 pressure constant pinned (:608-612).
-std::vector<CL> implicit_block(const Blocks &k, int n, LD kz, LD ls, const Params &p, bool pin) {
-  const int N4 = 4 * n;
+// The operator part of one (m, kz) mode times sgn (3n x 3n, complex): the
+// blocks of _add_avv_terms (examples/taylor_couette_dns_jax.py:1157-1190;
+// :523-569 for m = 0): nu (Laplacian - 1/r^2), the base-flow advection
+// -Omega d/dtheta, the 2 nu / r^2 azimuthal couplings, 2 Omega u_theta and
+// -2a u_r.
+std::vector<CL> operator_part(const Blocks &k, int n, LD m, LD kz, LD sgn, const Params &p) {
+  const int N3 = 3 * n;
+  std::vector<CL> X((size_t)N3 * N3, CL(0));
+  const LD nu = p.nu;
+  const CL I(0, 1);
+  auto at = [&](int bi, int bj, int i, int j) -> CL & { return X[(size_t)(bi * n + i) * N3 + bj * n + j]; };
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j) {
+      const size_t o = (size_t)i * n + j;
+      const LD lap = k.D2[o] + k.D1r[o] - kz * kz * k.MD[o] - m * m * k.R2D[o];
+      const CL adv = -I * m * k.Om[o];
+      at(0, 0, i, j) = sgn * (nu * lap - nu * k.R2D[o] + adv);
+      at(1, 1, i, j) = sgn * (nu * lap - nu * k.R2D[o] + adv);
+      at(2, 2, i, j) = sgn * (nu * lap + adv);
+      at(0, 1, i, j) = sgn * (-nu * 2 * I * m * k.R2D[o] + k.TwoOm[o]);
+      at(1, 0, i, j) = sgn * (nu * 2 * I * m * k.R2D[o] - 2 * (LD)p.a * k.MD[o]);
+    }
+  return X;
+}
+
+// Implicit block of one mode at scale ls (:539-569, :1198-1214), times fz,
+// with the (m, kz) = (0, 0) pressure constant pinned (:608-612).
+std::vector<CL> implicit_block(const Blocks &k, int n, LD m, LD kz, LD ls, LD fz, const Params &p, bool pin) {
+  const int N4 = 4 * n, N3 = 3 * n;
   std::vector<CL> A((size_t)N4 * N4, CL(0));
-  const LD nu = p.nu, dt = p.dt, lz = p.lz;
+  const std::vector<CL> X = operator_part(k, n, m, kz, -ls, p);
+  const CL I(0, 1);
+  for (int i = 0; i < N3; ++i)
+    for (int j = 0; j < N3; ++j) A[(size_t)i * N4 + j] = fz * X[(size_t)i * N3 + j];
   auto at = [&](int bi, int bj, int i, int j) -> CL & { return A[(size_t)(bi * n + i) * N4 + bj * n + j]; };
   for (int i = 0; i < n; ++i)
     for (int j = 0; j < n; ++j) {
       const size_t o = (size_t)i * n + j;
-      const LD lap = k.D2[o] + k.D1r[o] - kz * kz * k.MD[o];
-      const LD diag_rt = k.MD[o] / dt - ls * nu * lap + ls * nu * k.R2D[o];
-      at(0, 0, i, j) = lz * diag_rt;
-      at(1, 1, i, j) = lz * diag_rt;
-      at(2, 2, i, j) = lz * (k.MD[o] / dt - ls * nu * lap);
-      at(0, 1, i, j) = lz * (-ls * k.TwoOm[o]);
-      at(1, 0, i, j) = lz * (-ls * (-2 * (LD)p.a) * k.MD[o]);
-      at(0, 3, i, j) = lz * k.Gr[o];
-      at(2, 3, i, j) = lz * CL(0, kz) * k.Gz0[o];
-      at(3, 0, i, j) = lz * k.Dr[o];
-      at(3, 2, i, j) = lz * CL(0, kz) * k.Dz0[o];
+      for (int c = 0; c < 3; ++c) at(c, c, i, j) += fz * k.MD[o] / (LD)p.dt;
+      at(0, 3, i, j) = fz * k.Gr[o];
+      at(1, 3, i, j) = fz * I * m * k.Gt0[o];
+      at(2, 3, i, j) = fz * I * kz * k.Gz0[o];
+      at(3, 0, i, j) = fz * k.Dr[o];
+      at(3, 1, i, j) = fz * I * m * k.Dt0[o];
+      at(3, 2, i, j) = fz * I * kz * k.Dz0[o];
     }
   if (pin) {
     const int row = 3 * n;
@@ -193,30 +220,24 @@ std::vector<CL> implicit_block(const Blocks &k, int n, LD kz, LD ls, const Param
   return A;
 }
 
-// Explicit operator mass + ls L of one mode (3n x 3n, real), times Lz (:571-605);
-// with_mass = false gives L alone.
-std::vector<LD> explicit_block(const Blocks &k, int n, LD kz, LD ls, const Params &p, bool with_mass) {
+// Explicit operator mass + ls L of one mode, times fz (:571-605, :1216-1231);
+// with_mass = false gives L alone.  Stored transposed into dst (complex).
+void explicit_block_T(const Blocks &k, int n, LD m, LD kz, LD ls, LD fz, const Params &p, bool with_mass,
+                      double *dst) {
   const int N3 = 3 * n;
-  std::vector<LD> E((size_t)N3 * N3, 0);
-  const LD nu = p.nu, dt = p.dt, lz = p.lz, m = with_mass ? 1 : 0;
-  auto at = [&](int bi, int bj, int i, int j) -> LD & { return E[(size_t)(bi * n + i) * N3 + bj * n + j]; };
-  for (int i = 0; i < n; ++i)
-    for (int j = 0; j < n; ++j) {
-      const size_t o = (size_t)i * n + j;
-      const LD lap = k.D2[o] + k.D1r[o] - kz * kz * k.MD[o];
-      const LD rt = m * k.MD[o] / dt + ls * nu * lap - ls * nu * k.R2D[o];
-      at(0, 0, i, j) = lz * rt;
-      at(1, 1, i, j) = lz * rt;
-      at(2, 2, i, j) = lz * (m * k.MD[o] / dt + ls * nu * lap);
-      at(0, 1, i, j) = lz * ls * k.TwoOm[o];
-      at(1, 0, i, j) = lz * ls * (-2 * (LD)p.a) * k.MD[o];
+  const std::vector<CL> X = operator_part(k, n, m, kz, ls, p);
+  for (int i = 0; i < N3; ++i)
+    for (int j = 0; j < N3; ++j) {
+      CL v = fz * X[(size_t)i * N3 + j];
+      if (with_mass && i / n == j / n) v += fz * k.MD[(size_t)(i % n) * n + (j % n)] / (LD)p.dt;
+      dst[2 * ((size_t)j * N3 + i)] = (double)v.real();
+      dst[2 * ((size_t)j * N3 + i) + 1] = (double)v.imag();
     }
-  return E;
 }
 
 // The velocity columns of the inverse, transposed: dst[j][i] = inv(A)[i][j], j < 3n.
-bool inverse_columns(const Blocks &k, int n, LD kz, LD ls, const Params &p, bool pin, double *dst) {
-  std::vector<CL> A = implicit_block(k, n, kz, ls, p, pin);
+bool inverse_columns(const Blocks &k, int n, LD m, LD kz, LD ls, LD fz, const Params &p, bool pin, double *dst) {
+  std::vector<CL> A = implicit_block(k, n, m, kz, ls, fz, p, pin);
   const int N4 = 4 * n, N3 = 3 * n;
   if (!invert(A, N4)) return false;
   for (int i = 0; i < N4; ++i)  // transposed: [j][i]
@@ -233,6 +254,7 @@ bool inverse_columns(const Blocks &k, int n, LD kz, LD ls, const Params &p, bool
 std::string build_ops(const Params &p, Ops &o) {
   if (p.nr < 6) return "Nr must be at least 6";
   if (p.nz < 2 || p.nz % 2) return "Nz must be even and at least 2";
+  if (p.ntheta < 1 || (p.ntheta > 1 && p.ntheta % 2)) return "Ntheta must be 1 or even";
   if (!(p.r2 > p.r1 && p.r1 > 0)) return "require 0 < R1 < R2";
   if (!(p.lz > 0) || !(p.dt > 0) || !(p.nu > 0)) return "Lz, dt and nu must be positive";
   const int N = p.nr, n = N - 2;
@@ -240,7 +262,11 @@ std::string build_ops(const Params &p, Ops &o) {
   o.kh = p.nz / 2;
   o.mr = p.dealias > 1.0 ? (int)(p.nr * p.dealias) : p.nr;
   o.mz = p.dealias > 1.0 ? (int)(p.nz * p.dealias) : p.nz;
-  if (o.mr < N || o.mz < p.nz) return "invalid dealias factor";
+  o.nth = p.ntheta;
+  o.nm = o.nth * o.kh;
+  o.mth = (p.ntheta > 1 && p.dealias > 1.0) ? (int)(p.ntheta * p.dealias) : p.ntheta;
+  o.fz = p.ntheta > 1 ? (double)(2 * PI * (LD)p.lz) : p.lz;
+  if (o.mr < N || o.mz < p.nz || o.mth < p.ntheta) return "invalid dealias factor";
   const Forms F(N, p.r1, p.r2);
   o.df = (double)F.df;
   const Blocks K = blocks(F, p.a, p.b);
@@ -280,39 +306,45 @@ std::string build_ops(const Params &p, Ops &o) {
     for (int m = 1; m <= N / 2; ++m) corr += 2 * cosl(2 * th * m) / (4 * (LD)m * m - 1);
     o.fejer[q] = (double)((2 / (LD)N) * (1 - corr) / F.df);
   }
+  const LD fz = p.ntheta > 1 ? 2 * PI * (LD)p.lz : (LD)p.lz;
   o.MassT.resize((size_t)n * n);
   o.Drc.resize((size_t)n * n);
   o.Dzc.resize((size_t)n * n);
+  o.Dtc.resize((size_t)n * n);
   for (int i = 0; i < n; ++i)
     for (int j = 0; j < n; ++j) {
       const size_t ij = (size_t)i * n + j;
-      o.MassT[(size_t)j * n + i] = (double)(p.lz * K.MD[ij] / p.dt);
-      o.Drc[ij] = (double)(p.lz * K.Dr[ij]);
-      o.Dzc[ij] = (double)(p.lz * K.Dz0[ij]);
+      o.MassT[(size_t)j * n + i] = (double)(fz * K.MD[ij] / p.dt);
+      o.Drc[ij] = (double)(fz * K.Dr[ij]);
+      o.Dzc[ij] = (double)(fz * K.Dz0[ij]);
+      o.Dtc[ij] = (double)(fz * K.Dt0[ij]);
     }
-  // per-mode operators
-  const int kh = o.kh, N3 = 3 * n, N4 = 4 * n;
-  o.kz.resize(kh);
-  o.LinT.assign((size_t)kh * N3 * N3, 0);
-  o.LexpT.assign((size_t)kh * N3 * N3, 0);
-  o.AinvT.assign((size_t)kh * N4 * N3 * 2, 0);
-  o.Ainv0T.assign((size_t)kh * N4 * N3 * 2, 0);
+  // per-mode operators, mode q = im * kh + k (im in FFT order)
+  const int kh = o.kh, nm = o.nm, N3 = 3 * n, N4 = 4 * n;
+  o.kz.resize(nm);
+  o.mq.resize(nm);
+  o.mq_odd.resize(nm);
+  o.LinT.assign((size_t)nm * N3 * N3 * 2, 0);
+  o.LexpT.assign((size_t)nm * N3 * N3 * 2, 0);
+  o.AinvT.assign((size_t)nm * N4 * N3 * 2, 0);
+  o.Ainv0T.assign((size_t)nm * N4 * N3 * 2, 0);
   const LD ls = (p.integrator == TC_TI_SBDF3) ? 6.0L / 11.0L : 0.5L;
-  for (int k = 0; k < kh; ++k) {
-    const LD kz = 2 * PI * k / p.lz;
-    o.kz[k] = (double)kz;
-    const std::vector<LD> L = explicit_block(K, n, kz, 1, p, false);
-    const std::vector<LD> E = explicit_block(K, n, kz, 0.5L, p, true);
-    for (int i = 0; i < N3; ++i)
-      for (int j = 0; j < N3; ++j) {
-        o.LinT[(size_t)k * N3 * N3 + (size_t)j * N3 + i] = (double)L[(size_t)i * N3 + j];
-        o.LexpT[(size_t)k * N3 * N3 + (size_t)j * N3 + i] = (double)E[(size_t)i * N3 + j];
-      }
-    if (!inverse_columns(K, n, kz, ls, p, k == 0, &o.AinvT[(size_t)k * N4 * N3 * 2]))
-      return "singular implicit block";
-    if (!inverse_columns(K, n, kz, 0.5L, p, k == 0, &o.Ainv0T[(size_t)k * N4 * N3 * 2]))
-      return "singular start-up block";
-  }
+  for (int im = 0; im < o.nth; ++im)
+    for (int k = 0; k < kh; ++k) {
+      const int q = im * kh + k;
+      const int mi = (im < (o.nth + 1) / 2) ? im : im - o.nth;
+      const LD m = mi, kz = 2 * PI * k / p.lz;
+      o.kz[q] = (double)kz;
+      o.mq[q] = (double)mi;
+      o.mq_odd[q] = (o.nth % 2 == 0 && im == o.nth / 2) ? 0.0 : (double)mi;
+      explicit_block_T(K, n, m, kz, 1, fz, p, false, &o.LinT[(size_t)q * N3 * N3 * 2]);
+      explicit_block_T(K, n, m, kz, 0.5L, fz, p, true, &o.LexpT[(size_t)q * N3 * N3 * 2]);
+      const bool pin = (mi == 0 && k == 0);
+      if (!inverse_columns(K, n, m, kz, ls, fz, p, pin, &o.AinvT[(size_t)q * N4 * N3 * 2]))
+        return "singular implicit block";
+      if (!inverse_columns(K, n, m, kz, 0.5L, fz, p, pin, &o.Ainv0T[(size_t)q * N4 * N3 * 2]))
+        return "singular start-up block";
+    }
   return "";
 }
 

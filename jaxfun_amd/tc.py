@@ -1,7 +1,8 @@
-"""Axisymmetric Taylor-Couette DNS on the MI355X (C ABI include/tc_dns.h).
+"""Taylor-Couette DNS on the MI355X (C ABI include/tc_dns.h).
 
 Host-side mirror of the reference's ``AxisymmetricTCDNSJax``
-(``examples/taylor_couette_dns_jax.py:217-1078``) and of its base flow
+(``examples/taylor_couette_dns_jax.py:217-1078``), of the full 3D
+``TaylorCouetteDNSJax`` (``:1081-1356``) and of their base flow
 ``CircularCouette`` (``examples/taylor_couette_linear_jax.py:31-89``): the same
 constructor arguments, state fields (``AxisymmetricTCState``, ``:114-165``)
 and methods.  Every step runs in the HIP library; the host converts states
@@ -139,6 +140,7 @@ class AxisymmetricTCDNSHIP:
         c.dealias = self.dealias
         c.time_integrator = nat.TC_TI[self.time_integrator]
         c.device = int(device)
+        c.ntheta = 1
         return c
 
     def close(self):
@@ -153,8 +155,12 @@ class AxisymmetricTCDNSHIP:
             pass
 
     # ---------------------------------------------------------------- states
+    @property
+    def _shape(self):
+        return (self.Nz, self.n)
+
     def _zeros(self):
-        return np.zeros((self.Nz, self.n), dtype=complex)
+        return np.zeros(self._shape, dtype=complex)
 
     def zero_state(self) -> AxisymmetricTCState:
         u = tuple(self._zeros() for _ in range(3))
@@ -237,12 +243,12 @@ class AxisymmetricTCDNSHIP:
     # ------------------------------------------------------------ device I/O
     def _set(self, name: str, arr):
         a = np.ascontiguousarray(np.asarray(arr, dtype=np.complex128))
-        if a.shape != (self.Nz, self.n):
-            raise ValueError(f"{name}: expected shape {(self.Nz, self.n)}, got {a.shape}")
+        if a.shape != self._shape:
+            raise ValueError(f"{name}: expected shape {self._shape}, got {a.shape}")
         nat.tc_check(self._lib.tc_set_field(self._h, nat.TC_FIELD_ID[name], a.view(np.float64).ctypes.data_as(_dp)))
 
     def _get(self, name: str) -> np.ndarray:
-        a = np.empty((self.Nz, self.n), dtype=np.complex128)
+        a = np.empty(self._shape, dtype=np.complex128)
         nat.tc_check(self._lib.tc_get_field(self._h, nat.TC_FIELD_ID[name], a.view(np.float64).ctypes.data_as(_dp)))
         return a
 
@@ -355,3 +361,96 @@ class AxisymmetricTCDNSHIP:
         out = self.solve(state, steps)
         e1 = self.energy(out)
         return 0.5 * math.log(e1 / e0) / (int(steps) * self.dt), out
+
+
+class TaylorCouetteDNSHIP(AxisymmetricTCDNSHIP):
+    """``TaylorCouetteDNSJax`` on the GPU (``examples/taylor_couette_dns_jax.py:1081-1356``):
+    Fourier(theta) x Fourier(z) x Chebyshev(r), one radial velocity-pressure
+    block per (m, kz) mode.  State arrays are (Ntheta, Nz, Nr - 2) complex in
+    FFT order along both Fourier axes; the library stores the kz >= 0 half."""
+
+    def __init__(self, base: CircularCouette, nu: float = 1.0e-2, Nr: int = 24, Ntheta: int = 8, Nz: int = 16,
+                 Lz: float | None = None, dt: float = 2.0e-3, family: str = "C", dealias: float = 1.5,
+                 time_integrator: str = "CNAB2", device: int = 0):
+        self.Ntheta = int(Ntheta)
+        super().__init__(base, nu=nu, Nr=Nr, Nz=Nz, Lz=Lz, dt=dt, family=family, dealias=dealias,
+                         time_integrator=time_integrator, device=device)
+        th = np.arange(self.Ntheta) * 2.0 * math.pi / self.Ntheta
+        z = np.arange(self.Nz) * self.Lz / self.Nz
+        self.Theta, self.Z, self.R = np.meshgrid(th, z, self.r, indexing="ij")
+
+    def _config(self, device: int) -> nat.TcConfig:
+        c = super()._config(device)
+        c.ntheta = self.Ntheta
+        return c
+
+    @property
+    def _shape(self):
+        return (self.Ntheta, self.Nz, self.n)
+
+    def _forward(self, values) -> np.ndarray:
+        c = np.fft.fft2(np.asarray(values, dtype=complex), axes=(0, 1), norm="forward")
+        V = self._vander()
+        phi = V[:, : self.n] - V[:, 2:]
+        w = np.pi / self.Nr
+        rhs = w * (c @ phi)
+        M = w * phi.T @ phi
+        return np.linalg.solve(M, rhs.reshape(-1, self.n).T).T.reshape(self._shape)
+
+    def _backward(self, coef) -> np.ndarray:
+        V = self._vander()
+        phi = V[:, : self.n] - V[:, 2:]
+        return np.fft.ifft2(np.asarray(coef) @ phi.T, axes=(0, 1), norm="forward")
+
+    def _require_m(self, m: int) -> None:
+        """``_require_resolved_m`` (``examples/taylor_couette_dns_jax.py:76-84``)."""
+        if 2 * abs(int(m)) >= self.Ntheta:
+            raise ValueError(f"azimuthal mode |m|={abs(int(m))} is unresolved by Ntheta={self.Ntheta}; "
+                             "require 2|m| < Ntheta")
+
+    def initial_state(self, amp: float = 1.0e-3, m: int = 1, kz_mode: int = 1) -> AxisymmetricTCState:
+        """``:1264-1275``: amp sin(pi (r - R1) / d) cos(m theta) cos(kz z) in every component."""
+        self._require_m(m)
+        kz = 2.0 * math.pi * int(kz_mode) / self.Lz
+        shape = np.sin(math.pi * (self.R - self.base.R1) / self.base.gap)
+        f = amp * shape * np.cos(int(m) * self.Theta) * np.cos(kz * self.Z)
+        return self.state_from_physical((f, f, f))
+
+    def seed_linear_eigenmode(self, m: int = 1, kz_mode: int = 1, amp: float = 1.0e-6, which: int = 0):
+        """``:1277-1308``: the pivot phase is fixed only for m = 0."""
+        self._require_m(m)
+        kz = 2.0 * math.pi * int(kz_mode) / self.Lz
+        w, vecs = self.linear_eigs(int(m), kz, n_return=which + 1)
+        vec = vecs[:, which]
+        if int(m) == 0:
+            vec = _positive_pivot_phase(vec)
+        n = self.n
+        mpos, mneg = int(m) % self.Ntheta, (-int(m)) % self.Ntheta
+        kpos, kneg = int(kz_mode) % self.Nz, (-int(kz_mode)) % self.Nz
+        comps = []
+        for c in range(3):
+            block = vec[c * n:(c + 1) * n] * amp
+            arr = self._zeros()
+            if mpos == mneg and kpos == kneg:
+                arr[mpos, kpos, :n] = np.real(block)
+            else:
+                arr[mpos, kpos, :n] = 0.5 * block
+                arr[mneg, kneg, :n] = 0.5 * np.conj(block)
+            comps.append(arr)
+        st = AxisymmetricTCState(u=tuple(comps), p=self._zeros(), nonlinear_old=tuple(self._zeros() for _ in range(3)),
+                                 have_old=0.0)
+        return st, complex(w[which])
+
+    def velocity_physical(self, state: AxisymmetricTCState):
+        """u on the standard (Ntheta, Nz, Nr) grid (``:1323-1324``), evaluated on the host."""
+        return tuple(self._backward(c) for c in state.u)
+
+    def inner_torque(self, state: AxisymmetricTCState) -> float:
+        """The inner-cylinder torque of the (theta, z)-mean azimuthal velocity (the (0, 0) mode)."""
+        r1, b = self.base.R1, self.base.b
+        df = 2.0 / (self.base.R2 - self.base.R1)
+        k = np.arange(self.Nr, dtype=float)
+        dT = ((-1.0) ** (k + 1)) * k**2
+        dphi = df * (dT[: self.n] - dT[2:])
+        mean_dut = np.asarray(state.u[1])[0, 0] @ dphi
+        return float(2.0 * math.pi * self.nu * r1**2 * abs(-2.0 * b / r1**2 + mean_dut))

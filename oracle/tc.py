@@ -538,3 +538,219 @@ class TCLinearOracle:
         w, V = w[keep], V[:, keep]
         order = np.lexsort((-w.imag, -w.real))
         return w[order][:n_return], V[:, order][:, :n_return]
+
+
+class TC3DOracle(TCOracle):
+    """``TaylorCouetteDNSJax`` restated (``examples/taylor_couette_dns_jax.py:1081-1356``):
+    Fourier(theta) x Fourier(z) x Chebyshev(r), one saddle-point block per
+    (m, kz) with the base-flow advection -Omega d/dtheta, the 2/r^2 azimuthal
+    couplings and the 3D Laplacian (:1154-1232); every form carries the
+    Fourier factor 2 pi Lz (i m)^a (i kz)^b.  Arrays are (Ntheta, Nz, Nr - 2)."""
+
+    def __init__(self, base: CircularCouette, nu=1.0e-2, Nr=24, Ntheta=8, Nz=16, Lz=None, dt=2.0e-3, dealias=1.5,
+                 time_integrator="CNAB2"):
+        self.Ntheta = int(Ntheta)
+        self.mint = fourier_wavenumbers(self.Ntheta)
+        self.m_odd = fourier_wavenumbers(self.Ntheta, eliminate_highest=True)
+        super().__init__(base, nu=nu, Nr=Nr, Nz=Nz, Lz=Lz, dt=dt, dealias=dealias, time_integrator=time_integrator)
+        self.Mth = int(self.Ntheta * self.dealias) if self.dealias > 1.0 else self.Ntheta
+
+    def _radial_blocks(self):
+        super()._radial_blocks()
+        rf, b = self.rf, self.base
+        self.Om = rf.form(rf.SD, rf.SD, 0, lambda r: b.a + b.b / r**2)
+        self.Gt0 = rf.form(rf.SD, rf.SP, 0, lambda r: 1.0 / r)
+        self.Dt0 = rf.form(rf.SP, rf.SD, 0, lambda r: 1.0 / r)
+
+    def _operator_part(self, m, kz, sign):
+        """The _add_avv_terms blocks of one mode times sign (3n x 3n)."""
+        n, nu, a = self.Nd, self.nu, self.base.a
+        S = lambda i: slice(i * n, (i + 1) * n)  # noqa: E731
+        lap = self.D2 + self.D1r - kz**2 * self.MD - m**2 * self.R2D
+        X = np.zeros((3 * n, 3 * n), dtype=complex)
+        X[S(0), S(0)] = sign * (nu * lap - nu * self.R2D - 1j * m * self.Om)
+        X[S(0), S(1)] = sign * (-nu * 2j * m * self.R2D + self.TwoOm)
+        X[S(1), S(1)] = sign * (nu * lap - nu * self.R2D - 1j * m * self.Om)
+        X[S(1), S(0)] = sign * (nu * 2j * m * self.R2D - 2.0 * a * self.MD)
+        X[S(2), S(2)] = sign * (nu * lap - 1j * m * self.Om)
+        return X
+
+    def _build_operators(self, ls):
+        """``:1192-1232``, per (m, kz) mode, times 2 pi Lz."""
+        n, dt = self.Nd, self.dt
+        fz = 2.0 * math.pi * self.Lz
+        nm = self.Ntheta * self.Nz
+        limp = np.zeros((nm, 4 * n, 4 * n), dtype=complex)
+        lexp = np.zeros((nm, 3 * n, 3 * n), dtype=complex)
+        mass = np.zeros((nm, 3 * n, 3 * n), dtype=complex)
+        S = lambda i: slice(i * n, (i + 1) * n)  # noqa: E731
+        for it in range(self.Ntheta):
+            for iz in range(self.Nz):
+                q = it * self.Nz + iz
+                m, kz = self.mint[it], self.kz[iz]
+                M3 = np.kron(np.eye(3), self.MD / dt)
+                limp[q][:3 * n, :3 * n] = M3 + self._operator_part(m, kz, -ls)
+                lexp[q] = M3 + self._operator_part(m, kz, ls)
+                mass[q] = M3
+                A = limp[q]
+                A[S(0), S(3)] = self.Gr
+                A[S(1), S(3)] = 1j * m * self.Gt0
+                A[S(2), S(3)] = 1j * kz * self.Gz0
+                A[S(3), S(0)] = self.Dr
+                A[S(3), S(1)] = 1j * m * self.Dt0
+                A[S(3), S(2)] = 1j * kz * self.Dz0
+        return fz * limp, fz * lexp, fz * mass
+
+    # transforms over (theta, z)
+    def _orth(self, c):
+        return c @ self.rf.SD  # (Nth, Nz, Nr)
+
+    def _cheb_last(self, c, n):
+        return np.moveaxis(cheb_backward(np.moveaxis(c, -1, 0), n), 0, -1)
+
+    def _synth(self, c, nth, nz, nr):
+        v = fourier_backward(fourier_backward(c, 0, nth), 1, nz)
+        return self._cheb_last(v, nr)
+
+    def _phys(self, c):
+        orth = self._orth(c)
+        d = lambda x: self._synth(x, self.Mth, self.Mz, self.Mr)  # noqa: E731
+        dr = np.moveaxis(cheb_derivative(np.moveaxis(orth, -1, 0), 1), 0, -1)
+        return (d(orth), self.rf.df * d(dr), d(1j * self.m_odd[:, None, None] * orth),
+                d(1j * self.kz_odd[None, :, None] * orth))
+
+    def _dealias_to_standard(self, v):
+        if self.dealias <= 1.0:
+            return v
+        c = fourier_forward(fourier_forward(v, 0, self.Ntheta), 1, self.Nz)
+        c = np.moveaxis(cheb_forward(np.moveaxis(c, -1, 0), self.Nr), 0, -1)
+        return self._synth(c, self.Ntheta, self.Nz, self.Nr)
+
+    def scalar_product(self, v):
+        c = 2.0 * math.pi * self.Lz * fourier_forward(fourier_forward(v, 0, self.Ntheta), 1, self.Nz)
+        s = np.moveaxis(cheb_scalar_product(np.moveaxis(c, -1, 0), self.Nr), 0, -1) / self.rf.df
+        return self.mask(s @ self.rf.SD.T)
+
+    def mask(self, c):
+        c = np.array(c)
+        if self.Ntheta % 2 == 0:
+            c[self.Ntheta // 2] = 0.0
+        if self.Nz % 2 == 0:
+            c[:, self.Nz // 2] = 0.0
+        return c
+
+    def nonlinear(self, u):
+        """``:1242-1262``."""
+        (ur, urr, urt, urz), (ut, utr, utt, utz), (uz, uzr, uzt, uzz) = (self._phys(c) for c in u)
+        invr = (1.0 / self.r_pad)[None, None, :]
+        n_r = ur * urr + (ut * invr) * urt + uz * urz - ut * ut * invr
+        n_t = ur * utr + (ut * invr) * utt + uz * utz + ur * ut * invr
+        n_z = ur * uzr + (ut * invr) * uzt + uz * uzz
+        return tuple(self.scalar_product(self._dealias_to_standard(f)) for f in (n_r, n_t, n_z))
+
+    def _modes(self, u):
+        return np.concatenate([c.reshape(self.Ntheta * self.Nz, self.Nd) for c in u], axis=1)
+
+    def _unmodes(self, x):
+        n = self.Nd
+        return tuple(x[:, i * n:(i + 1) * n].reshape(self.Ntheta, self.Nz, n) for i in range(3))
+
+    def solve_rhs(self, rhs_u, lu):
+        nm = self.Ntheta * self.Nz
+        rhs = np.concatenate([self._modes(rhs_u), np.zeros((nm, self.Np), dtype=complex)], axis=1)
+        rhs[0, 3 * self.Nd] = 0.0
+        sol = np.stack([sla.lu_solve(lu[q], rhs[q]) for q in range(nm)])
+        return self._unmodes(sol[:, :3 * self.Nd]), sol[:, 3 * self.Nd:].reshape(self.Ntheta, self.Nz, self.Np)
+
+    def zero_state(self) -> TCState:
+        sh = (self.Ntheta, self.Nz, self.Nd)
+        z = tuple(np.zeros(sh, dtype=complex) for _ in range(3))
+        return TCState(u=z, p=np.zeros((self.Ntheta, self.Nz, self.Np), dtype=complex),
+                       nonlinear_old=tuple(np.zeros_like(a) for a in z))
+
+    def mesh(self):
+        th = np.arange(self.Ntheta) * 2.0 * math.pi / self.Ntheta
+        z = np.arange(self.Nz) * self.Lz / self.Nz
+        return th[:, None, None], z[None, :, None], self.rf.r[None, None, :]
+
+    def forward_D(self, v):
+        c = fourier_forward(fourier_forward(np.asarray(v, dtype=complex), 0, self.Ntheta), 1, self.Nz)
+        s = np.moveaxis(cheb_scalar_product(np.moveaxis(c, -1, 0), self.Nr), 0, -1) @ self.rf.SD.T
+        M = self.rf.SD @ np.diag(np.where(np.arange(self.Nr) == 0, math.pi, math.pi / 2)) @ self.rf.SD.T
+        return np.linalg.solve(M, s.reshape(-1, self.Nd).T).T.reshape(s.shape)
+
+    def state_from_physical(self, values) -> TCState:
+        u = tuple(self.forward_D(v) for v in values)
+        st = self.zero_state()
+        return replace(st, u=u)
+
+    def initial_state(self, amp=1.0e-3, m=1, kz_mode=1) -> TCState:
+        """``:1264-1275``."""
+        TH, Z, R = self.mesh()
+        kz = 2.0 * math.pi * int(kz_mode) / self.Lz
+        shape = np.sin(math.pi * (R - self.base.R1) / self.base.gap)
+        f = amp * shape * np.cos(int(m) * TH) * np.cos(kz * Z)
+        return self.state_from_physical((f, f, f))
+
+    def seed_linear_eigenmode(self, m=1, kz_mode=1, amp=1.0e-6, which=0):
+        """``:1277-1308``: no pivot phase for m != 0."""
+        kz = 2.0 * math.pi * int(kz_mode) / self.Lz
+        lin = TCLinearOracle(self.base, nu=self.nu, N=self.Nr)
+        w, vecs = lin.eigs(int(m), kz, n_return=which + 1)
+        vec = vecs[:, which]
+        if int(m) == 0:
+            vec = positive_pivot_phase(vec)
+        n = lin.n
+        mpos, mneg = int(m) % self.Ntheta, (-int(m)) % self.Ntheta
+        kpos, kneg = int(kz_mode) % self.Nz, (-int(kz_mode)) % self.Nz
+        comps = []
+        for c in range(3):
+            block = vec[c * n:(c + 1) * n] * amp
+            arr = np.zeros((self.Ntheta, self.Nz, self.Nd), dtype=complex)
+            if mpos == mneg and kpos == kneg:
+                arr[mpos, kpos, :n] = np.real(block)
+            else:
+                arr[mpos, kpos, :n] = 0.5 * block
+                arr[mneg, kneg, :n] = 0.5 * np.conj(block)
+            comps.append(arr)
+        st = self.zero_state()
+        return replace(st, u=tuple(comps), nonlinear_old=tuple(np.zeros_like(a) for a in comps)), complex(w[which])
+
+    def backward(self, c):
+        return self._synth(self._orth(c), self.Ntheta, self.Nz, self.Nr)
+
+    def _integrate(self, f):
+        w = (fejer_weights(self.Nr) / self.rf.df)[None, None, :] * (self.Lz / self.Nz) * (2.0 * math.pi / self.Ntheta)
+        return float(np.real(np.sum(f * w)))
+
+    def energy(self, s):
+        r = self.rf.r[None, None, :]
+        return self._integrate(0.5 * sum(np.real(np.conj(v) * v) for v in self.velocity_physical(s)) * r)
+
+    def divergence(self, s):
+        orth_r = self._orth(s.u[0])
+        dr = np.moveaxis(cheb_derivative(np.moveaxis(orth_r, -1, 0), 1), 0, -1)
+        syn = lambda x: self._synth(x, self.Ntheta, self.Nz, self.Nr)  # noqa: E731
+        r = self.rf.r[None, None, :]
+        return (self.rf.df * syn(dr) + syn(orth_r) / r
+                + syn(1j * self.m_odd[:, None, None] * self._orth(s.u[1])) / r
+                + syn(1j * self.kz_odd[None, :, None] * self._orth(s.u[2])))
+
+    def continuity_l2(self, s):
+        """``:1310-1321,1338-1346``."""
+        fz = 2.0 * math.pi * self.Lz
+        res = (fz * (s.u[0] @ self.Dr.T) + fz * (1j * self.mint[:, None, None]) * (s.u[1] @ self.Dt0.T)
+               + fz * (1j * self.kz[None, :, None]) * (s.u[2] @ self.Dz0.T))
+        return float(np.linalg.norm(res))
+
+    def wall(self, s):
+        ends = npcheb.chebvander(np.array([-1.0, 1.0]), self.Nr - 1) @ self.rf.SD.T
+        vals = [np.abs(fourier_backward(fourier_backward(c @ ends.T, 0, self.Ntheta), 1, self.Nz)) for c in s.u]
+        return float(max(np.max(v) for v in vals))
+
+    def diagnostics(self, s) -> dict:
+        vel = self.velocity_physical(s)
+        r = self.rf.r[None, None, :]
+        return {"E": self.energy(s), "div_linf": float(np.max(np.abs(self.divergence(s)))),
+                "continuity_l2": self.continuity_l2(s), "wall": self.wall(s),
+                "Eth": self._integrate(np.real(np.conj(vel[1]) * vel[1]) * r)}

@@ -371,6 +371,7 @@ def test_taylor_couette_exports_and_pencil_match_oracle():
     cfg.nr, cfg.nz = 24, 8
     cfg.r1, cfg.r2, cfg.omega1, cfg.omega2 = 1.0, 2.5, 1.0, -0.2
     cfg.nu, cfg.lz, cfg.dt, cfg.dealias = 0.02, 2.0, 1e-3, 1.5
+    cfg.ntheta = 1
     n4 = 4 * 22
     dp = C.POINTER(C.c_double)
     for m, kz in ((0, 3.13), (2, 1.7)):
