@@ -569,26 +569,37 @@ TC_CASES = (
     # benchmark's SBDF3 (production/autograd_cost_benchmark.py:64-71,396)
     dict(name="tc_hydro 40x8", Nr=40, Nz=8, dealias=1.0),
     dict(name="48x32 dealiased", Nr=48, Nz=32, dealias=1.5),
+    # the 3D class: production/examples/taylor_couette_hydro_3d_v1.json and the
+    # constructor defaults of TaylorCouetteDNSJax (Nr=24, Ntheta=8, Nz=16)
+    dict(name="tc_hydro_3d 12x4x4", Nr=12, Ntheta=4, Nz=4, dealias=1.5),
+    dict(name="3D defaults 24x8x16", Nr=24, Ntheta=8, Nz=16, dealias=1.5),
 )
 
 
 def tc_leg(steps: int = 2400, cpu_steps: int = 60) -> list:
-    """Axisymmetric Taylor-Couette SBDF3 steps/s on the GPU (graph replay of
-    12-step blocks, state resident) beside the oracle on the host cores, from
-    the golden case's seed (Re = 100, R1 = 1, R2 = 2, Omega1 = 1, Omega2 = 0,
-    Lz = 2 pi / 3.13, dt = 1e-3, amp 1e-6 leading eigenmode), and the growth
-    rate against the published linear one."""
-    from jaxfun_amd.tc import AxisymmetricTCDNSHIP, CircularCouette
+    """Taylor-Couette SBDF3 steps/s on the GPU (graph replay of 12-step
+    blocks, state resident) beside the oracle on the host cores, from the
+    golden cases' seeds (Re = 100, R1 = 1, R2 = 2, Omega1 = 1, Omega2 = 0,
+    Lz = 2 pi / 3.13, dt = 1e-3, amp 1e-6 leading eigenmode: m = 0 for the
+    axisymmetric class, m = 1 for the 3D one), and the growth rate against
+    the linear one."""
+    from jaxfun_amd.tc import AxisymmetricTCDNSHIP, CircularCouette, TaylorCouetteDNSHIP
     from oracle.tc import CircularCouette as OBase
-    from oracle.tc import TCOracle, TCState
+    from oracle.tc import TC3DOracle, TCOracle, TCState
 
     out = []
     lz = 2.0 * math.pi / 3.13
     for case in TC_CASES:
         kw = dict(nu=0.01, Nr=case["Nr"], Nz=case["Nz"], Lz=lz, dt=1e-3, dealias=case["dealias"],
                   time_integrator="SBDF3")
-        solver = AxisymmetricTCDNSHIP(CircularCouette(1.0, 2.0, 1.0, 0.0), **kw)
-        s0, eig = solver.seed_linear_eigenmode(kz_mode=1, amp=1e-6)
+        nth = case.get("Ntheta", 1)
+        if nth > 1:
+            kw["Ntheta"] = nth
+            solver = TaylorCouetteDNSHIP(CircularCouette(1.0, 2.0, 1.0, 0.0), **kw)
+            s0, eig = solver.seed_linear_eigenmode(m=1, kz_mode=1, amp=1e-6)
+        else:
+            solver = AxisymmetricTCDNSHIP(CircularCouette(1.0, 2.0, 1.0, 0.0), **kw)
+            s0, eig = solver.seed_linear_eigenmode(kz_mode=1, amp=1e-6)
         s = solver.solve(s0, 2)  # start-up steps
         solver.load(s)
         solver.advance(24)
@@ -598,7 +609,7 @@ def tc_leg(steps: int = 2400, cpu_steps: int = 60) -> list:
         fin = solver.fetch(s)
         e0, e1 = solver.energy(s), solver.energy(fin)
         growth = 0.5 * math.log(e1 / e0) / ((steps + 24) * solver.dt)
-        oracle = TCOracle(OBase(1.0, 2.0, 1.0, 0.0), **kw)
+        oracle = (TC3DOracle if nth > 1 else TCOracle)(OBase(1.0, 2.0, 1.0, 0.0), **kw)
         os_ = TCState(u=s.u, p=s.p, nonlinear_old=s.nonlinear_old, have_old=s.have_old,
                       nonlinear_older=s.nonlinear_older, solution_old=s.solution_old,
                       solution_older=s.solution_older, history_steps=s.history_steps)
@@ -606,8 +617,9 @@ def tc_leg(steps: int = 2400, cpu_steps: int = 60) -> list:
         t1 = time.perf_counter()
         oracle.solve(os_, cpu_steps)
         cpu = cpu_steps / (time.perf_counter() - t1)
-        out.append({"workload": f"axisymmetric Taylor-Couette SBDF3, {case['name']} (Nr={case['Nr']}, "
-                                f"Nz={case['Nz']}, dealias={case['dealias']}), Re=100, dt=1e-3, FP64",
+        kind = f"3D Taylor-Couette SBDF3, {case['name']} (Nr={case['Nr']}, Ntheta={nth}, " if nth > 1 else \
+            f"axisymmetric Taylor-Couette SBDF3, {case['name']} (Nr={case['Nr']}, "
+        out.append({"workload": kind + f"Nz={case['Nz']}, dealias={case['dealias']}), Re=100, dt=1e-3, FP64",
                     "steps_per_s": steps / el, "ms_per_step": 1e3 * el / steps, "steps": steps,
                     "growth_rate": growth, "growth_rate_linear": eig.real,
                     "cpu_oracle_steps_per_s": cpu, "cpu_sample": f"{cpu_steps} steady steps, numpy/scipy "

@@ -25,15 +25,19 @@ __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
 //   n_t = u_r du_t/dr + u_z du_t/dz + u_r u_t / r
 //   n_z = u_r du_z/dr + u_z du_z/dz,
 // and their axial analysis G[c][k][j] = (Lz / mz) sum_m n_c(z_m) e^{-i k z_m}.
+// The twiddles sit in LDS and every transform loop steps its table index
+// instead of reducing k m modulo mz.
 __global__ void __launch_bounds__(256) k_tc_nonlinear(TcGeo g, const double2 *__restrict__ u,
                                                       double2 *__restrict__ G) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int j = blockIdx.x, kh = g.kh, n = g.n, mz = g.mz;
-  double2 *val = reinterpret_cast<double2 *>(smem);        // [9][kh]: (c, v/dr/dz)
-  double *phys = reinterpret_cast<double *>(val + 9 * kh);  // [9][mz]
-  double *prod = phys + 9 * mz;                             // [3][mz]
+  double2 *tw = reinterpret_cast<double2 *>(smem);             // [mz]
+  double2 *val = tw + mz;                                      // [9][kh]: (c, v/dr/dz)
+  double *phys = reinterpret_cast<double *>(val + 9 * kh);     // [9][mz]
+  double *prod = phys + 9 * mz;                                // [3][mz]
   double2 *part = reinterpret_cast<double2 *>(prod + 3 * mz);  // [nch][6 kh] partial sums
   const int tid = threadIdx.x;
+  for (int t = tid; t < mz; t += blockDim.x) tw[t] = g.twz[t];
   // 6 kh dot products of length n (value and d/dr of three components at every
   // mode), each split into nch chunks so that every thread has work
   const int nd = 6 * kh, nch = max(1, (int)blockDim.x / nd), len = (n + nch - 1) / nch;
@@ -67,8 +71,11 @@ __global__ void __launch_bounds__(256) k_tc_nonlinear(TcGeo g, const double2 *__
     const int f = t / mz, m = t - f * mz;
     const double2 *c = val + f * kh;
     double s = 0.0;
+    int idx = 0;
     for (int k = 1; k < kh; ++k) {
-      const double2 w = g.twz[(k * m) % mz];
+      idx += m;
+      if (idx >= mz) idx -= mz;
+      const double2 w = tw[idx];
       s += c[k].x * w.x - c[k].y * w.y;
     }
     phys[t] = c[0].x + 2.0 * s;
@@ -89,10 +96,13 @@ __global__ void __launch_bounds__(256) k_tc_nonlinear(TcGeo g, const double2 *__
     const int c = t / kh, k = t - c * kh;
     const double *p = prod + c * mz;
     double sr = 0.0, si = 0.0;
+    int idx = 0;
     for (int m = 0; m < mz; ++m) {
-      const double2 w = g.twz[(k * m) % mz];
+      const double2 w = tw[idx];
       sr = fma(p[m], w.x, sr);
       si = fma(-p[m], w.y, si);
+      idx += k;
+      if (idx >= mz) idx -= mz;
     }
     G[((size_t)c * kh + k) * g.mr + j] = make_double2(sc * sr, sc * si);
   }
@@ -346,17 +356,28 @@ __global__ void __launch_bounds__(256) k_tc_diag(TcGeo g, const double2 *__restr
 //   n_t = u_r du_t/dr + (u_t/r) du_t/dtheta + u_z du_t/dz + u_r u_t / r
 //   n_z = u_r du_z/dr + (u_t/r) du_z/dtheta + u_z du_z/dz,
 // then the axial and azimuthal analyses truncated to the stored modes, times
-// (2 pi / mth)(Lz / mz); the azimuthal Nyquist rows are masked.
-__global__ void __launch_bounds__(256) k_tc3_nonlinear(TcGeo g, const double2 *__restrict__ u,
-                                                       double2 *__restrict__ G) {
+// (2 pi / mth)(Lz / mz); the azimuthal Nyquist rows are masked.  Both twiddle
+// tables sit in LDS; the axial synthesis runs k-outer so that one twiddle
+// serves all twelve fields of a point.
+constexpr int TC3_THREADS = 512;
+
+__device__ __forceinline__ int fft_wavenumber(int im, int nth) { return im < (nth + 1) / 2 ? im : im - nth; }
+
+__global__ void __launch_bounds__(TC3_THREADS) k_tc3_nonlinear(TcGeo g, const double2 *__restrict__ u,
+                                                               double2 *__restrict__ G) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int j = blockIdx.x, kh = g.kh, nm = g.nm, n = g.n, mz = g.mz, mth = g.mth, nth = g.nth;
-  double2 *val = reinterpret_cast<double2 *>(smem);  // [12][nm]; later prod [3][mth mz] and h [3][mth][kh]
+  double2 *tz = reinterpret_cast<double2 *>(smem);  // [mz]
+  double2 *tt = tz + mz;                             // [mth]
+  double *base = reinterpret_cast<double *>(tt + mth);
+  double2 *val = reinterpret_cast<double2 *>(base);  // [12][nm]; later prod [3][mth mz] and h [3][mth][kh]
   const size_t region = max((size_t)12 * nm * 2, (size_t)3 * mth * mz + (size_t)3 * mth * kh * 2);
-  double2 *gs = reinterpret_cast<double2 *>(reinterpret_cast<double *>(smem) + region);  // [12][kh][mth]
-  double *prod = reinterpret_cast<double *>(smem);
+  double2 *gs = reinterpret_cast<double2 *>(base + region);  // [12][kh][mth]
+  double *prod = base;
   double2 *h = reinterpret_cast<double2 *>(prod + 3 * mth * mz);
   const int tid = threadIdx.x;
+  for (int t = tid; t < mz; t += blockDim.x) tz[t] = g.twz[t];
+  for (int t = tid; t < mth; t += blockDim.x) tt[t] = g.twt[t];
   for (int d = tid; d < 6 * nm; d += blockDim.x) {
     const int c = d / (2 * nm), rem = d - c * 2 * nm, typ = rem / nm, q = rem - typ * nm;
     const double *E = (typ == 0 ? g.Ev : g.Ed) + (size_t)j * n;
@@ -379,8 +400,7 @@ __global__ void __launch_bounds__(256) k_tc3_nonlinear(TcGeo g, const double2 *_
     double sr = 0.0, si = 0.0;
     for (int im = 0; im < nth; ++im) {
       const double2 c = val[f * nm + im * kh + k];
-      const int mi = (int)g.mq[im * kh];
-      const double2 w = g.twt[((mi * a) % mth + mth) % mth];
+      const double2 w = tt[((fft_wavenumber(im, nth) * a) % mth + mth) % mth];
       sr += c.x * w.x - c.y * w.y;
       si += c.x * w.y + c.y * w.x;
     }
@@ -392,15 +412,20 @@ __global__ void __launch_bounds__(256) k_tc3_nonlinear(TcGeo g, const double2 *_
     const int a = p / mz, b = p - a * mz;
     double v[12];
 #pragma unroll
-    for (int f = 0; f < 12; ++f) {
-      const double2 *gf = gs + (size_t)f * kh * mth;
-      double s = 0.0;
-      for (int k = 1; k < kh; ++k) {
-        const double2 c = gf[k * mth + a], w = g.twz[(k * b) % mz];
-        s += c.x * w.x - c.y * w.y;
+    for (int f = 0; f < 12; ++f) v[f] = 0.0;
+    int idx = 0;
+    for (int k = 1; k < kh; ++k) {
+      idx += b;
+      if (idx >= mz) idx -= mz;
+      const double2 w = tz[idx];
+#pragma unroll
+      for (int f = 0; f < 12; ++f) {
+        const double2 c = gs[((size_t)f * kh + k) * mth + a];
+        v[f] += c.x * w.x - c.y * w.y;
       }
-      v[f] = gf[a].x + 2.0 * s;
     }
+#pragma unroll
+    for (int f = 0; f < 12; ++f) v[f] = gs[(size_t)f * kh * mth + a].x + 2.0 * v[f];
     const double ur = v[0], urr = v[1], urt = v[2], urz = v[3];
     const double ut = v[4], utr = v[5], utt = v[6], utz = v[7];
     const double uz = v[8], uzr = v[9], uzt = v[10], uzz = v[11];
@@ -414,10 +439,13 @@ __global__ void __launch_bounds__(256) k_tc3_nonlinear(TcGeo g, const double2 *_
     const int c = t / (mth * kh), rem = t - c * mth * kh, a = rem / kh, k = rem - a * kh;
     const double *pr = prod + (size_t)c * mth * mz + (size_t)a * mz;
     double sr = 0.0, si = 0.0;
+    int idx = 0;
     for (int b = 0; b < mz; ++b) {
-      const double2 w = g.twz[(k * b) % mz];
+      const double2 w = tz[idx];
       sr = fma(pr[b], w.x, sr);
       si = fma(-pr[b], w.y, si);
+      idx += k;
+      if (idx >= mz) idx -= mz;
     }
     h[t] = make_double2(sr, si);  // [c][a][k]
   }
@@ -427,11 +455,14 @@ __global__ void __launch_bounds__(256) k_tc3_nonlinear(TcGeo g, const double2 *_
     const int c = t / nm, q = t - c * nm, im = q / kh, k = q - im * kh;
     double sr = 0.0, si = 0.0;
     if (!(nth % 2 == 0 && im == nth / 2)) {  // the azimuthal Nyquist rows are masked
-      const int mi = (int)g.mq[q];
+      const int step = (fft_wavenumber(im, nth) % mth + mth) % mth;
+      int idx = 0;
       for (int a = 0; a < mth; ++a) {
-        const double2 hv = h[((size_t)c * mth + a) * kh + k], w = g.twt[((mi * a) % mth + mth) % mth];
+        const double2 hv = h[((size_t)c * mth + a) * kh + k], w = tt[idx];
         sr += hv.x * w.x + hv.y * w.y;  // h e^{-i m theta_a}
         si += hv.y * w.x - hv.x * w.y;
+        idx += step;
+        if (idx >= mth) idx -= mth;
       }
     }
     G[((size_t)c * nm + q) * g.mr + j] = make_double2(sc * sr, sc * si);
@@ -443,10 +474,11 @@ __global__ void __launch_bounds__(256) k_tc3_nonlinear(TcGeo g, const double2 *_
 size_t nonlinear_lds(const TcGeo &g) {
   if (g.nth > 1) {
     const size_t region = std::max((size_t)12 * g.nm * 2, (size_t)3 * g.mth * g.mz + (size_t)3 * g.mth * g.kh * 2);
-    return (region + (size_t)12 * g.kh * g.mth * 2) * sizeof(double);
+    return ((size_t)g.mz + g.mth) * sizeof(double2) + (region + (size_t)12 * g.kh * g.mth * 2) * sizeof(double);
   }
   const int nd = 6 * g.kh, nch = std::max(1, 256 / nd);
-  return 9 * (size_t)g.kh * sizeof(double2) + 12 * (size_t)g.mz * sizeof(double) + (size_t)nd * nch * sizeof(double2);
+  return ((size_t)g.mz + 9 * (size_t)g.kh) * sizeof(double2) + 12 * (size_t)g.mz * sizeof(double) +
+         (size_t)nd * nch * sizeof(double2);
 }
 
 size_t rhs_lds(const TcGeo &g) {
@@ -462,7 +494,7 @@ size_t diag_lds(const TcGeo &) { return 256 * sizeof(double); }
 hipError_t launch_nonlinear(const TcGeo &g, const double2 *u, double2 *G, hipStream_t s) {
   const size_t lds = nonlinear_lds(g);
   if (g.nth > 1)
-    hipLaunchKernelGGL(k_tc3_nonlinear, dim3(g.mr), dim3(256), lds, s, g, u, G);
+    hipLaunchKernelGGL(k_tc3_nonlinear, dim3(g.mr), dim3(TC3_THREADS), lds, s, g, u, G);
   else
     hipLaunchKernelGGL(k_tc_nonlinear, dim3(g.mr), dim3(256), lds, s, g, u, G);
   return hipGetLastError();
