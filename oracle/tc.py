@@ -754,3 +754,354 @@ class TC3DOracle(TCOracle):
         return {"E": self.energy(s), "div_linf": float(np.max(np.abs(self.divergence(s)))),
                 "continuity_l2": self.continuity_l2(s), "wall": self.wall(s),
                 "Eth": self._integrate(np.real(np.conj(vel[1]) * vel[1]) * r)}
+
+
+# ---------------------------------------------------------------------------
+# Axisymmetric conducting-wall Taylor-Couette MHD (AxisymmetricMRIDNSJax)
+# ---------------------------------------------------------------------------
+
+
+def stencil_two_point(N: int, left, right) -> np.ndarray:
+    """psi_i = T_i + d1 T_{i+1} + d2 T_{i+2} meeting one homogeneous condition
+    per wall (``src/jaxfun/galerkin/composite.py:797-871``): ``left(j)`` and
+    ``right(j)`` give the condition applied to T_j.  Shape (N - 2, N)."""
+    S = np.zeros((N - 2, N))
+    for i in range(N - 2):
+        A = np.array([[left(i + 1), left(i + 2)], [right(i + 1), right(i + 2)]], dtype=float)
+        d = np.linalg.solve(A, -np.array([left(i), right(i)], dtype=float))
+        S[i, i], S[i, i + 1], S[i, i + 2] = 1.0, d[0], d[1]
+    return S
+
+
+def stencil_robin(N: int, alpha_left: float, alpha_right: float) -> np.ndarray:
+    """psi + alpha psi_x = 0 at x = -1 and x = +1 (the "R" condition; T_j(+-1) =
+    (+-1)^j, T_j'(+-1) = (+-1)^(j+1) j^2)."""
+    return stencil_two_point(N, lambda j: (-1.0) ** j * (1.0 - alpha_left * j * j),
+                             lambda j: 1.0 + alpha_right * j * j)
+
+
+def stencil_neumann(N: int) -> np.ndarray:
+    """psi_x = 0 at both walls: T_i - i^2 / (i + 2)^2 T_{i+2}, the closed form
+    the reference's symbolic solve simplifies to (``composite.py:811-814``)."""
+    S = np.zeros((N - 2, N))
+    i = np.arange(N - 2)
+    S[i, i] = 1.0
+    S[i, i + 2] = -(i.astype(float) ** 2) / (i + 2.0) ** 2
+    return S
+
+
+class TCMRILinearOracle:
+    """Conducting-wall MRI eigenproblem L q = lambda M q on (u_r, u_theta, u_z,
+    p, b_r, b_theta, b_z) (``examples/taylor_couette_mri_jax.py:65-401``)."""
+
+    def __init__(self, base: CircularCouette, B0=0.1, nu=1e-3, eta_mag=1e-3, N=48):
+        self.base, self.B0, self.nu, self.eta = base, float(B0), float(nu), float(eta_mag)
+        self.N = int(N)
+        self.rf = RadialForms(self.N, base.R1, base.R2)
+        self.n = self.N - 2
+        df = self.rf.df
+        self.SBT = stencil_robin(self.N, base.R1 * df, base.R2 * df)
+        self.SBZ = stencil_neumann(self.N)
+
+    def assemble(self, m, kz):
+        """``assemble_parts`` (``:214-321``): L = L0 + nu Lnu + eta Leta."""
+        rf, b, n = self.rf, self.base, self.n
+        SD, SP, SBT, SBZ = rf.SD, rf.SP, self.SBT, self.SBZ
+        sp_ = {"ur": SD, "ut": SD, "uz": SD, "p": SP, "br": SD, "bt": SBT, "bz": SBZ}
+        idx = {"ur": 0, "ut": 1, "uz": 2, "p": 3, "br": 4, "bt": 5, "bz": 6}
+        ikB = 1j * kz * self.B0
+        om = lambda r: b.a + b.b / r**2  # noqa: E731
+
+        def blk(row, col, terms):
+            out = np.zeros((n, n), dtype=complex)
+            for coef, deriv in terms:
+                out = out + (rf.form(sp_[row], sp_[col], deriv) if coef is None else
+                             rf.form(sp_[row], sp_[col], deriv, lambda r, c=coef: np.real(c(r)))
+                             + 1j * rf.form(sp_[row], sp_[col], deriv, lambda r, c=coef: np.imag(c(r))))
+            return out
+
+        const = lambda v: (lambda r: v + 0.0 * r)  # noqa: E731
+        lap = [(None, 2), (lambda r: 1.0 / r, 1), (lambda r: -(m * m / r**2 + kz * kz), 0)]
+        lv = lap + [(lambda r: -1.0 / r**2, 0)]
+        couple = [(lambda r: 2j * m / r**2, 0)]
+        imOm = lambda r: 1j * m * om(r)  # noqa: E731
+        L0 = np.zeros((7 * n, 7 * n), dtype=complex)
+        Lnu, Leta, M = np.zeros_like(L0), np.zeros_like(L0), np.zeros_like(L0)
+
+        def put(X, row, col, v):
+            i, j = idx[row], idx[col]
+            X[i * n:(i + 1) * n, j * n:(j + 1) * n] += v
+
+        neg_imOm = [(lambda r: -imOm(r), 0)]
+        put(L0, "ur", "ur", blk("ur", "ur", neg_imOm))
+        put(Lnu, "ur", "ur", blk("ur", "ur", lv))
+        put(L0, "ur", "ut", blk("ur", "ut", [(lambda r: 2.0 * om(r), 0)]))
+        put(Lnu, "ur", "ut", -blk("ur", "ut", couple))
+        put(L0, "ur", "p", -blk("ur", "p", [(None, 1)]))
+        put(L0, "ur", "br", blk("ur", "br", [(const(ikB), 0)]))
+        put(M, "ur", "ur", blk("ur", "ur", [(None, 0)]))
+        put(L0, "ut", "ur", blk("ut", "ur", [(const(-2.0 * b.a), 0)]))
+        put(Lnu, "ut", "ur", blk("ut", "ur", couple))
+        put(L0, "ut", "ut", blk("ut", "ut", neg_imOm))
+        put(Lnu, "ut", "ut", blk("ut", "ut", lv))
+        put(L0, "ut", "p", -blk("ut", "p", [(lambda r: 1j * m / r, 0)]))
+        put(L0, "ut", "bt", blk("ut", "bt", [(const(ikB), 0)]))
+        put(M, "ut", "ut", blk("ut", "ut", [(None, 0)]))
+        put(L0, "uz", "uz", blk("uz", "uz", neg_imOm))
+        put(Lnu, "uz", "uz", blk("uz", "uz", lap))
+        put(L0, "uz", "p", -blk("uz", "p", [(const(1j * kz), 0)]))
+        put(L0, "uz", "bz", blk("uz", "bz", [(const(ikB), 0)]))
+        put(M, "uz", "uz", blk("uz", "uz", [(None, 0)]))
+        put(L0, "p", "ur", blk("p", "ur", [(None, 1), (lambda r: 1.0 / r, 0)]))
+        put(L0, "p", "ut", blk("p", "ut", [(lambda r: 1j * m / r, 0)]))
+        put(L0, "p", "uz", blk("p", "uz", [(const(1j * kz), 0)]))
+        put(L0, "br", "ur", blk("br", "ur", [(const(ikB), 0)]))
+        put(L0, "br", "br", blk("br", "br", neg_imOm))
+        put(Leta, "br", "br", blk("br", "br", lv))
+        put(Leta, "br", "bt", -blk("br", "bt", couple))
+        put(M, "br", "br", blk("br", "br", [(None, 0)]))
+        put(L0, "bt", "ut", blk("bt", "ut", [(const(ikB), 0)]))
+        put(L0, "bt", "br", blk("bt", "br", [(lambda r: -2.0 * b.b / r**2, 0)]))
+        put(Leta, "bt", "br", blk("bt", "br", couple))
+        put(L0, "bt", "bt", blk("bt", "bt", neg_imOm))
+        put(Leta, "bt", "bt", blk("bt", "bt", lv))
+        put(M, "bt", "bt", blk("bt", "bt", [(None, 0)]))
+        put(L0, "bz", "uz", blk("bz", "uz", [(const(ikB), 0)]))
+        put(L0, "bz", "bz", blk("bz", "bz", neg_imOm))
+        put(Leta, "bz", "bz", blk("bz", "bz", lap))
+        put(M, "bz", "bz", blk("bz", "bz", [(None, 0)]))
+        return L0 + self.nu * Lnu + self.eta * Leta, M
+
+    def eigs(self, m, kz, n_return=8, finite_cap=1.0e6):
+        L, M = self.assemble(m, kz)
+        w, V = sla.eig(L, M)
+        keep = np.isfinite(w) & (np.abs(w) < finite_cap)
+        w, V = w[keep], V[:, keep]
+        order = np.lexsort((-w.imag, -w.real))
+        return w[order][:n_return], V[:, order][:, :n_return]
+
+
+class TCMRIOracle(TCOracle):
+    """``AxisymmetricMRIDNSJax`` restated (``examples/taylor_couette_dns_jax.py:1359-1903``):
+    perturbations (u_r, u_theta, u_z, b_r, b_theta, b_z) and the total pressure
+    about V(r) e_theta and B0 e_z; b_r in the Dirichlet basis, b_theta in the
+    Robin basis of d(r b_theta)/dr = 0, b_z in the Neumann basis; per axial
+    mode one 7-field saddle-point block (``:1501-1571``).  State ``u`` holds the
+    six fields, each (Nz, Nr - 2)."""
+
+    NF = 6
+
+    def __init__(self, base: CircularCouette, B0=0.1, nu=1.0e-3, eta_mag=1.0e-3, Nr=24, Nz=16, Lz=None, dt=2.0e-3,
+                 dealias=1.5, time_integrator="CNAB2"):
+        self.B0, self.eta = float(B0), float(eta_mag)
+        Lz = float(Lz) if Lz is not None else 2.0 * math.pi / 3.0 * base.gap
+        df = 2.0 / (base.R2 - base.R1)
+        self.SBT = stencil_robin(int(Nr), base.R1 * df, base.R2 * df)
+        self.SBZ = stencil_neumann(int(Nr))
+        super().__init__(base, nu=nu, Nr=Nr, Nz=Nz, Lz=Lz, dt=dt, dealias=dealias, time_integrator=time_integrator)
+
+    @property
+    def bases(self):
+        SD = self.rf.SD
+        return (SD, SD, SD, SD, self.SBT, self.SBZ)
+
+    def _radial_blocks(self):
+        super()._radial_blocks()
+
+    def _lap_xy(self, X, Y, kz, hoop):
+        rf = self.rf
+        lap = rf.form(X, Y, 2) + rf.form(X, Y, 1, lambda r: 1.0 / r) - kz**2 * rf.form(X, Y)
+        return lap - rf.form(X, Y, 0, lambda r: 1.0 / r**2) if hoop else lap
+
+    def _operator_part(self, kz, sign):
+        """``_add_mhd_terms`` (``:1460-1499``) on the six fields, times sign."""
+        rf, b, n = self.rf, self.base, self.Nd
+        nu, eta, B0 = self.nu, self.eta, self.B0
+        B = self.bases
+        S = lambda i: slice(i * n, (i + 1) * n)  # noqa: E731
+        X = np.zeros((6 * n, 6 * n), dtype=complex)
+        ikB = 1j * kz * B0
+        X[S(0), S(0)] = nu * self._lap_xy(B[0], B[0], kz, True)
+        X[S(0), S(1)] = rf.form(B[0], B[1], 0, lambda r: 2.0 * (b.a + b.b / r**2))
+        X[S(0), S(3)] = ikB * rf.form(B[0], B[3])
+        X[S(1), S(1)] = nu * self._lap_xy(B[1], B[1], kz, True)
+        X[S(1), S(0)] = -2.0 * b.a * rf.form(B[1], B[0])
+        X[S(1), S(4)] = ikB * rf.form(B[1], B[4])
+        X[S(2), S(2)] = nu * self._lap_xy(B[2], B[2], kz, False)
+        X[S(2), S(5)] = ikB * rf.form(B[2], B[5])
+        X[S(3), S(3)] = eta * self._lap_xy(B[3], B[3], kz, True)
+        X[S(3), S(0)] = ikB * rf.form(B[3], B[0])
+        X[S(4), S(4)] = eta * self._lap_xy(B[4], B[4], kz, True)
+        X[S(4), S(1)] = ikB * rf.form(B[4], B[1])
+        X[S(4), S(3)] = rf.form(B[4], B[3], 0, lambda r: -2.0 * b.b / r**2)
+        X[S(5), S(5)] = eta * self._lap_xy(B[5], B[5], kz, False)
+        X[S(5), S(2)] = ikB * rf.form(B[5], B[2])
+        return sign * X
+
+    def _mass6(self):
+        n, B = self.Nd, self.bases
+        M = np.zeros((6 * n, 6 * n))
+        for i in range(6):
+            M[i * n:(i + 1) * n, i * n:(i + 1) * n] = self.rf.form(B[i], B[i]) / self.dt
+        return M
+
+    def _build_operators(self, ls):
+        """``:1501-1571``, times Lz; Limp in the (u_r, u_theta, u_z, Pi, b_r,
+        b_theta, b_z) order, Lexp and mass on the six fields."""
+        n, rf = self.Nd, self.rf
+        SD, SP = rf.SD, rf.SP
+        M6 = self._mass6()
+        limp = np.zeros((self.Nz, 7 * n, 7 * n), dtype=complex)
+        lexp = np.zeros((self.Nz, 6 * n, 6 * n), dtype=complex)
+        mass = np.zeros((self.Nz, 6 * n, 6 * n), dtype=complex)
+        perm = np.r_[0:3 * n, 4 * n:7 * n]  # field rows/columns inside the 7-field block
+        for m in range(self.Nz):
+            kz = self.kz[m]
+            A6 = M6 + self._operator_part(kz, -ls)
+            A = limp[m]
+            A[np.ix_(perm, perm)] = A6
+            P = slice(3 * n, 4 * n)
+            A[0:n, P] += rf.form(SD, SP, 1)
+            A[2 * n:3 * n, P] += 1j * kz * rf.form(SD, SP, 0)
+            A[P, 0:n] += rf.form(SP, SD, 1) + rf.form(SP, SD, 0, lambda r: 1.0 / r)
+            A[P, 2 * n:3 * n] += 1j * kz * rf.form(SP, SD, 0)
+            lexp[m] = M6 + self._operator_part(kz, ls)
+            mass[m] = M6
+        return self.Lz * limp, self.Lz * lexp, self.Lz * mass
+
+    # --------------------------------------------------------------- transforms
+    def _orth(self, c, f):
+        return c @ self.bases[f]
+
+    def _phys_f(self, c, f):
+        orth = self._orth(c, f)
+        val = self._cheb_r(fourier_backward(orth, 0, self.Mz), self.Mr)
+        dr = self.rf.df * self._cheb_r(fourier_backward(cheb_derivative(orth.T, 1).T, 0, self.Mz), self.Mr)
+        dz = self._cheb_r(fourier_backward(1j * self.kz_odd[:, None] * orth, 0, self.Mz), self.Mr)
+        return val, dr, dz
+
+    def scalar_product_basis(self, v, S):
+        c = self.Lz * fourier_forward(v, 0, self.Nz)
+        s = cheb_scalar_product(c.T, self.Nr).T / self.rf.df
+        return self.mask(s @ S.T)
+
+    def _std_deriv(self, chat, axis):
+        """T0.backward_primitive of (Nz, Nr) Chebyshev coefficients on the
+        standard grid: axis 0 is d/dz, axis 1 d/dr."""
+        if axis == 0:
+            return self._cheb_r(fourier_backward(1j * self.kz_odd[:, None] * chat, 0, self.Nz), self.Nr)
+        return self.rf.df * self._cheb_r(fourier_backward(cheb_derivative(chat.T, 1).T, 0, self.Nz), self.Nr)
+
+    def nonlinear(self, x):
+        """``:1603-1649``: advection minus Lorentz force on the velocity rows,
+        minus the curl of the EMF u x b on the magnetic rows."""
+        F = [self._phys_f(c, f) for f, c in enumerate(x)]
+        (ur, urr, urz), (ut, utr, utz), (uz, uzr, uzz) = F[0], F[1], F[2]
+        (br, brr, brz), (bt, btr, btz), (bz, bzr, bzz) = F[3], F[4], F[5]
+        invr = (1.0 / self.r_pad)[None, :]
+        au_r = ur * urr + uz * urz - ut * ut * invr
+        au_t = ur * utr + uz * utz + ur * ut * invr
+        au_z = ur * uzr + uz * uzz
+        lb_r = br * brr + bz * brz - bt * bt * invr
+        lb_t = br * btr + bz * btz + br * bt * invr
+        lb_z = br * bzr + bz * bzz
+        std = self._dealias_to_standard
+        nu_r, nu_t, nu_z = std(au_r - lb_r), std(au_t - lb_t), std(au_z - lb_z)
+        eps_r, eps_t, eps_z = std(ut * bz - uz * bt), std(uz * br - ur * bz), std(ur * bt - ut * br)
+        coef = lambda v: cheb_forward(fourier_forward(v, 0, self.Nz).T, self.Nr).T  # noqa: E731
+        er, et, ez = coef(eps_r), coef(eps_t), coef(eps_z)
+        nb_r = self._std_deriv(et, 0)
+        nb_t = -self._std_deriv(er, 0) + self._std_deriv(ez, 1)
+        nb_z = -self._std_deriv(et, 1) - eps_t / self.rf.r[None, :]
+        SD = self.rf.SD
+        return (self.scalar_product_basis(nu_r, SD), self.scalar_product_basis(nu_t, SD),
+                self.scalar_product_basis(nu_z, SD), self.scalar_product_basis(nb_r, SD),
+                self.scalar_product_basis(nb_t, self.SBT), self.scalar_product_basis(nb_z, self.SBZ))
+
+    def _modes(self, u):
+        return np.concatenate(list(u), axis=1)
+
+    def _unmodes(self, x):
+        n = self.Nd
+        return tuple(x[:, i * n:(i + 1) * n] for i in range(6))
+
+    def solve_rhs(self, rhs_x, lu):
+        """``_solve_mhd_rhs`` (``:1662-1669``): zero pressure rows, pinned mode 0."""
+        n = self.Nd
+        r6 = self._modes(rhs_x)
+        rhs = np.concatenate([r6[:, :3 * n], np.zeros((self.Nz, n), dtype=complex), r6[:, 3 * n:]], axis=1)
+        rhs[0, 3 * n] = 0.0
+        sol = np.stack([sla.lu_solve(lu[m], rhs[m]) for m in range(self.Nz)])
+        x = np.concatenate([sol[:, :3 * n], sol[:, 4 * n:]], axis=1)
+        return self._unmodes(x), sol[:, 3 * n:4 * n]
+
+    # ------------------------------------------------------------------ states
+    def zero_state(self) -> TCState:
+        z = tuple(np.zeros((self.Nz, self.Nd), dtype=complex) for _ in range(6))
+        return TCState(u=z, p=np.zeros((self.Nz, self.Np), dtype=complex),
+                       nonlinear_old=tuple(np.zeros_like(a) for a in z))
+
+    def forward_basis(self, v, S):
+        c = fourier_forward(np.asarray(v, dtype=complex), 0, self.Nz)
+        s = cheb_scalar_product(c.T, self.Nr).T @ S.T
+        M = S @ np.diag(np.where(np.arange(self.Nr) == 0, math.pi, math.pi / 2)) @ S.T
+        return np.linalg.solve(M, s.T).T
+
+    def state_from_physical(self, values) -> TCState:
+        x = tuple(self.forward_basis(v, S) for v, S in zip(values, self.bases))
+        return replace(self.zero_state(), u=x)
+
+    def seed_linear_eigenmode(self, kz_mode=1, amp=1.0e-6, which=0):
+        """``:1809-1842``: conducting MRI eigenmode, fields from blocks (0,1,2,4,5,6)."""
+        kz = 2.0 * math.pi * int(kz_mode) / self.Lz
+        lin = TCMRILinearOracle(self.base, B0=self.B0, nu=self.nu, eta_mag=self.eta, N=self.Nr)
+        w, vecs = lin.eigs(0, kz, n_return=which + 1)
+        vec = positive_pivot_phase(vecs[:, which])
+        n = lin.n
+        kpos, kneg = int(kz_mode) % self.Nz, (-int(kz_mode)) % self.Nz
+        comps = []
+        for blk in (0, 1, 2, 4, 5, 6):
+            block = vec[blk * n:(blk + 1) * n] * amp
+            arr = np.zeros((self.Nz, self.Nd), dtype=complex)
+            if kpos == kneg:
+                arr[kpos, :n] = np.real(block)
+            else:
+                arr[kpos, :n] = 0.5 * block
+                arr[kneg, :n] = 0.5 * np.conj(block)
+            comps.append(arr)
+        st = self.zero_state()
+        return replace(st, u=tuple(comps), nonlinear_old=tuple(np.zeros_like(a) for a in comps),
+                       have_old=0.0), complex(w[which])
+
+    # ------------------------------------------------------------- diagnostics
+    def backward_f(self, c, f):
+        return self._cheb_r(fourier_backward(self._orth(c, f), 0, self.Nz), self.Nr)
+
+    def fields_physical(self, s):
+        return tuple(self.backward_f(c, f) for f, c in enumerate(s.u))
+
+    def energy_parts(self, s):
+        r = self.rf.r[None, :]
+        F = self.fields_physical(s)
+        ek = self._integrate(0.5 * sum(np.real(np.conj(v) * v) for v in F[:3]) * r)
+        em = self._integrate(0.5 * sum(np.real(np.conj(v) * v) for v in F[3:]) * r)
+        return ek, em
+
+    def energy(self, s):
+        return sum(self.energy_parts(s))
+
+    def _div(self, s, fr, fz):
+        orth_r = self._orth(s.u[fr], fr)
+        d_r = self.rf.df * self._cheb_r(fourier_backward(cheb_derivative(orth_r.T, 1).T, 0, self.Nz), self.Nr)
+        v_r = self.backward_f(s.u[fr], fr)
+        d_z = self._cheb_r(fourier_backward(1j * self.kz_odd[:, None] * self._orth(s.u[fz], fz), 0, self.Nz), self.Nr)
+        return d_r + v_r / self.rf.r[None, :] + d_z
+
+    def diagnostics(self, s) -> dict:
+        """``:1876-1886``."""
+        ek, em = self.energy_parts(s)
+        ends = npcheb.chebvander(np.array([-1.0, 1.0]), self.Nr - 1) @ self.rf.SD.T
+        wall = max(float(np.max(np.abs(fourier_backward(c @ ends.T, 0, self.Nz)))) for c in s.u[:3])
+        return {"Ekin": ek, "Emag": em, "E": ek + em,
+                "divu": float(np.max(np.abs(self._div(s, 0, 2)))),
+                "divb": float(np.max(np.abs(self._div(s, 3, 5)))), "wall_u": wall}
