@@ -1,5 +1,5 @@
 /*
- * tc_dns.h — C ABI of the axisymmetric Taylor-Couette DNS step in
+ * tc_dns.h — C ABI of the Taylor-Couette DNS step in
  * libpcfdns.so (the second geometry of the same library).  Plain C types only;
  * every entry point returns 0 on success and a negative TC_E_* code on
  * failure, with a message from tc_last_error().
@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define TC_ABI_VERSION 2
+#define TC_ABI_VERSION 3
 
 #define TC_OK 0
 #define TC_E_INVALID (-1)
@@ -61,6 +61,13 @@ typedef struct tc_config {
   int32_t time_integrator; /* TC_TI_*                                  */
   int32_t device;        /* HIP device ordinal                         */
   int32_t ntheta;        /* azimuthal Fourier modes: 1 (axisymmetric) or even */
+  /* conducting-wall MHD, AxisymmetricMRIDNSJax (:1359-1903): mhd = 1 evolves
+   * (u_r, u_theta, u_z, b_r, b_theta, b_z) with the total pressure about the
+   * imposed axial field b0 (Alfven units), magnetic diffusivity eta_mag; b_r in
+   * the Dirichlet basis, b_theta in the Robin basis of d(r b_theta)/dr = 0,
+   * b_z in the Neumann basis.  Axisymmetric only (ntheta = 1). */
+  int32_t mhd;
+  double b0, eta_mag;
 } tc_config;
 
 typedef struct tc_handle tc_handle;
@@ -84,6 +91,24 @@ typedef struct tc_handle tc_handle;
 #define TC_DIAG_ETH 4
 #define TC_DIAG_COUNT 5
 
+/* groups for tc_set_component / tc_get_component (component c < 3, or < 6 with
+ * mhd; the pressure group has the one component 0) */
+#define TC_G_X 0          /* the evolved fields (u, or u then b)  */
+#define TC_G_P 1          /* pressure (total pressure with mhd)   */
+#define TC_G_NL_OLD 2
+#define TC_G_NL_OLDER 3
+#define TC_G_SOL_OLD 4
+#define TC_G_SOL_OLDER 5
+
+/* diagnostics with mhd (AxisymmetricMRIDNSJax.diagnostics, :1876-1886) */
+#define TC_MHD_DIAG_EKIN 0
+#define TC_MHD_DIAG_EMAG 1
+#define TC_MHD_DIAG_E 2
+#define TC_MHD_DIAG_DIVU 3
+#define TC_MHD_DIAG_DIVB 4
+#define TC_MHD_DIAG_WALL_U 5
+#define TC_MHD_DIAG_COUNT 6
+
 const char *tc_last_error(void);
 
 /* AxisymmetricTCDNSJax.__init__ (:229-284): builds the per-mode operators and
@@ -96,6 +121,10 @@ int32_t tc_destroy(tc_handle *h);
 int32_t tc_set_field(tc_handle *h, int32_t field, const double *host);
 int32_t tc_get_field(tc_handle *h, int32_t field, double *host);
 int32_t tc_set_scalars(tc_handle *h, double have_old, double history_steps);
+/* The same exchange by group and component (AxisymmetricMRIState, :169-191,
+ * with mhd; any configuration). */
+int32_t tc_set_component(tc_handle *h, int32_t group, int32_t comp, const double *host);
+int32_t tc_get_component(tc_handle *h, int32_t group, int32_t comp, double *host);
 int32_t tc_get_scalars(tc_handle *h, double *have_old, double *history_steps);
 
 /* AxisymmetricTCDNSJax.step / solve (:948-990): nsteps steps of the
@@ -108,7 +137,8 @@ int32_t tc_step(tc_handle *h, int32_t nsteps);
  * results as tc_step. */
 int32_t tc_step_graph(tc_handle *h, int32_t nsteps);
 
-/* E, div_linf, continuity_l2, wall, Eth of the device state (TC_DIAG_*). */
+/* E, div_linf, continuity_l2, wall, Eth of the device state (TC_DIAG_*); with
+ * mhd the TC_MHD_DIAG_COUNT values TC_MHD_DIAG_*. */
 int32_t tc_diagnostics(tc_handle *h, double *out);
 
 /* The linear eigenproblem pencil L q = lambda M q of
@@ -116,6 +146,13 @@ int32_t tc_diagnostics(tc_handle *h, double *out);
  * on the configuration's radial space (N = cfg->nr, base flow and nu of cfg):
  * out_L, out_M (4n x 4n) complex interleaved, n = Nr - 2.  Host only, no GPU. */
 int32_t tc_linear_pencil(const tc_config *cfg, int32_t m, double kz, double *out_L, double *out_M);
+
+/* The conducting-wall MRI pencil of TaylorCouetteMRIJax.assemble
+ * (examples/taylor_couette_mri_jax.py:214-321,393-395): L = L0 + nu Lnu +
+ * eta Leta and M, (7n x 7n) complex interleaved, unknowns (u_r, u_theta, u_z,
+ * p, b_r, b_theta, b_z); cfg supplies the base flow, nu, b0 and eta_mag.
+ * Host only. */
+int32_t tc_mri_pencil(const tc_config *cfg, int32_t m, double kz, double *out_L, double *out_M);
 
 /* dt change (:971-985): SBDF3 refuses a changed dt (TC_E_FIXED_DT); CNAB2 and
  * IMEXRK3 rebuild their operators. */

@@ -80,15 +80,18 @@ SEED_ENERGY = 0
 SEED_COTANGENT = 1
 
 # include/tc_dns.h (the Taylor-Couette step in the same library)
-TC_ABI_VERSION = 2
+TC_ABI_VERSION = 3
 TC_TI = {"CNAB2": 0, "SBDF3": 1, "IMEXRK3": 2}
 TC_FIELDS = ("UR", "UT", "UZ", "P", "NL_OLD_R", "NL_OLD_T", "NL_OLD_Z", "NL_OLDER_R", "NL_OLDER_T", "NL_OLDER_Z",
              "SOL_OLD_R", "SOL_OLD_T", "SOL_OLD_Z", "SOL_OLDER_R", "SOL_OLDER_T", "SOL_OLDER_Z")
 TC_FIELD_ID = {name: i for i, name in enumerate(TC_FIELDS)}
 TC_DIAG_KEYS = ("E", "div_linf", "continuity_l2", "wall", "Eth")
+TC_MHD_DIAG_KEYS = ("Ekin", "Emag", "E", "divu", "divb", "wall_u")
+TC_GROUP = {"X": 0, "P": 1, "NL_OLD": 2, "NL_OLDER": 3, "SOL_OLD": 4, "SOL_OLDER": 5}
 TC_EXPORTS = (
     "tc_last_error", "tc_create", "tc_destroy", "tc_set_field", "tc_get_field", "tc_set_scalars", "tc_get_scalars",
-    "tc_step", "tc_step_graph", "tc_diagnostics", "tc_linear_pencil", "tc_set_dt",
+    "tc_set_component", "tc_get_component", "tc_step", "tc_step_graph", "tc_diagnostics", "tc_linear_pencil",
+    "tc_mri_pencil", "tc_set_dt",
 )
 
 
@@ -108,6 +111,9 @@ class TcConfig(C.Structure):
         ("time_integrator", C.c_int32),
         ("device", C.c_int32),
         ("ntheta", C.c_int32),
+        ("mhd", C.c_int32),
+        ("b0", C.c_double),
+        ("eta_mag", C.c_double),
     ]
 
 
@@ -216,6 +222,9 @@ def load() -> C.CDLL:
         "tc_step_graph": ([h, C.c_int32], C.c_int32),
         "tc_diagnostics": ([h, dp], C.c_int32),
         "tc_linear_pencil": ([tcp, C.c_int32, C.c_double, dp, dp], C.c_int32),
+        "tc_mri_pencil": ([tcp, C.c_int32, C.c_double, dp, dp], C.c_int32),
+        "tc_set_component": ([h, C.c_int32, C.c_int32, dp], C.c_int32),
+        "tc_get_component": ([h, C.c_int32, C.c_int32, dp], C.c_int32),
         "tc_set_dt": ([h, C.c_double], C.c_int32),
     })
     for name, (args, res) in sig.items():

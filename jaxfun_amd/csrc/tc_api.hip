@@ -1,4 +1,4 @@
-// C ABI of the axisymmetric Taylor-Couette step (include/tc_dns.h).
+// C ABI of the Taylor-Couette step (include/tc_dns.h).
 //
 // Reference: AxisymmetricTCDNSJax (examples/taylor_couette_dns_jax.py:217-1078).
 // The device state keeps the Hermitian half of every field ([component][k][i],
@@ -55,7 +55,8 @@ struct tc_handle {
   std::vector<void *> allocs;
   double *dEv = nullptr, *dEd = nullptr, *dPrT = nullptr, *dinvr = nullptr, *dkz = nullptr, *dMassT = nullptr,
          *dmq = nullptr, *dmq_odd = nullptr, *dDtc = nullptr, *dEs = nullptr, *dEsd = nullptr, *drstd = nullptr, *dfejer = nullptr,
-         *dDrc = nullptr, *dDzc = nullptr, *dout = nullptr;
+         *dDrc = nullptr, *dDzc = nullptr, *dout = nullptr, *dQbr = nullptr, *dQbt = nullptr, *dQbtd = nullptr,
+         *dQbz = nullptr;
   double2 *dtwz = nullptr, *dtwn = nullptr, *dtwt = nullptr, *dtwnt = nullptr, *dAinvT = nullptr, *dAinv0T = nullptr;
   double2 *dLinT = nullptr, *dLexpT = nullptr, *ddv = nullptr;
   double2 *S[4]{}, *N[3]{}, *P = nullptr, *G = nullptr, *XS[3]{}, *NS[3]{};
@@ -101,6 +102,9 @@ tc::Params params_of(const tc_config &c) {
   p.dealias = c.dealias;
   p.integrator = c.time_integrator;
   p.ntheta = c.ntheta;
+  p.mhd = c.mhd;
+  p.b0 = c.b0;
+  p.eta = c.eta_mag;
   return p;
 }
 
@@ -134,6 +138,25 @@ int32_t check_field(const tc_handle *h, int32_t field) {
   if (!h) return fail(TC_E_INVALID, "null handle");
   if (field < 0 || field >= TC_F_COUNT) return fail(TC_E_INVALID, "no such field");
   return TC_OK;
+}
+
+// Device pointer of one component of a group (nm x n complex).
+double2 *component_ptr(tc_handle *h, int32_t group, int32_t comp) {
+  const size_t cs = (size_t)h->ops.nm * h->ops.n;
+  switch (group) {
+    case TC_G_X:
+      return h->S[h->su] + comp * cs;
+    case TC_G_P:
+      return h->P;
+    case TC_G_NL_OLD:
+      return h->N[h->nn1] + comp * cs;
+    case TC_G_NL_OLDER:
+      return h->N[h->nn2] + comp * cs;
+    case TC_G_SOL_OLD:
+      return h->S[h->s1] + comp * cs;
+    default:
+      return h->S[h->s2] + comp * cs;
+  }
 }
 
 // Device pointer of a field id (one component, kh x n complex).
@@ -265,6 +288,49 @@ int32_t one_step(tc_handle *h) {
   return TC_OK;
 }
 
+int32_t check_component(const tc_handle *h, int32_t group, int32_t comp) {
+  if (!h) return fail(TC_E_INVALID, "null handle");
+  if (group < TC_G_X || group > TC_G_SOL_OLDER) return fail(TC_E_INVALID, "no such group");
+  if (comp < 0 || comp >= (group == TC_G_P ? 1 : h->ops.nf)) return fail(TC_E_INVALID, "no such component");
+  return TC_OK;
+}
+
+// The Hermitian half of a full ([Ntheta,] Nz, n) host array: rows k < kh of
+// every azimuthal mode, in the library's mode order q = im * kh + k.
+int32_t upload_half(tc_handle *h, double2 *dst, const double *host) {
+  const int kh = h->ops.kh, n = h->ops.n, nth = h->ops.nth, nz = h->cfg.nz;
+  TC_HIP(hipSetDevice(h->cfg.device));
+  std::vector<double> half(2 * (size_t)h->ops.nm * n);
+  for (int im = 0; im < nth; ++im)
+    std::memcpy(&half[2 * (size_t)im * kh * n], host + 2 * (size_t)im * nz * n, 2 * (size_t)kh * n * sizeof(double));
+  TC_HIP(hipMemcpyAsync(dst, half.data(), half.size() * sizeof(double), hipMemcpyHostToDevice, h->stream));
+  TC_HIP(hipStreamSynchronize(h->stream));
+  return TC_OK;
+}
+
+// The full array from the stored half: f(-m, -k) = conj f(m, k); the axial
+// Nyquist row stays zero.
+int32_t download_full(tc_handle *h, const double2 *src_dev, double *host) {
+  const int kh = h->ops.kh, n = h->ops.n, nth = h->ops.nth, nz = h->cfg.nz;
+  TC_HIP(hipSetDevice(h->cfg.device));
+  std::vector<double> half(2 * (size_t)h->ops.nm * n);
+  TC_HIP(hipMemcpyAsync(half.data(), src_dev, half.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  TC_HIP(hipStreamSynchronize(h->stream));
+  std::memset(host, 0, 2 * (size_t)nth * nz * n * sizeof(double));
+  for (int im = 0; im < nth; ++im)
+    for (int k = 0; k < kh; ++k) {
+      const double *src = &half[2 * ((size_t)im * kh + k) * n];
+      std::memcpy(host + 2 * ((size_t)im * nz + k) * n, src, 2 * (size_t)n * sizeof(double));
+      if (k == 0) continue;
+      double *dst = host + 2 * ((size_t)((nth - im) % nth) * nz + (nz - k)) * n;
+      for (int i = 0; i < n; ++i) {
+        dst[2 * i] = src[2 * i];
+        dst[2 * i + 1] = -src[2 * i + 1];
+      }
+    }
+  return TC_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -292,6 +358,7 @@ int32_t tc_create(const tc_config *cfg, tc_handle **out) {
   }
   const tc::Ops &o = h->ops;
   tc::TcGeo sz;
+  sz.nf = o.nf;
   sz.n = o.n;
   sz.kh = o.kh;
   sz.nth = o.nth;
@@ -313,17 +380,19 @@ int32_t tc_create(const tc_config *cfg, tc_handle **out) {
       (r = h->upload(&h->dtwz, twiddles(o.mz))) || (r = h->upload(&h->dtwn, twiddles(cfg->nz))) ||
       (r = h->upload(&h->dtwt, twiddles(o.mth))) || (r = h->upload(&h->dtwnt, twiddles(o.nth))) ||
       (r = h->upload(&h->dmq, o.mq)) || (r = h->upload(&h->dmq_odd, o.mq_odd)) || (r = h->upload(&h->dDtc, o.Dtc)) ||
-      (r = h->alloc(&h->ddv, (size_t)o.nm * cfg->nr)) || (r = h->alloc(&h->dout, 8))) {
+      (r = h->upload(&h->dQbr, o.QbrT)) || (r = h->upload(&h->dQbt, o.QbtT)) || (r = h->upload(&h->dQbtd, o.QbtdT)) ||
+      (r = h->upload(&h->dQbz, o.QbzT)) || (r = h->alloc(&h->ddv, 2 * (size_t)o.nm * cfg->nr)) ||
+      (r = h->alloc(&h->dout, 8))) {
     tc_destroy(h);
     return r;
   }
-  h->fsize = 3 * (size_t)o.nm * o.n;
+  h->fsize = (size_t)o.nf * o.nm * o.n;
   for (int i = 0; i < 4 && !r; ++i) r = h->alloc(&h->S[i], h->fsize);
   for (int i = 0; i < 3 && !r; ++i) r = h->alloc(&h->N[i], h->fsize);
   for (int i = 0; i < 3 && !r; ++i) r = h->alloc(&h->XS[i], h->fsize);
   for (int i = 0; i < 3 && !r; ++i) r = h->alloc(&h->NS[i], h->fsize);
   if (!r) r = h->alloc(&h->P, (size_t)o.nm * o.n);
-  if (!r) r = h->alloc(&h->G, 3 * (size_t)o.nm * o.mr);
+  if (!r) r = h->alloc(&h->G, (o.nf == 6 ? 6 : 3) * (size_t)o.nm * o.mr);
   if (r) {
     tc_destroy(h);
     return r;
@@ -336,6 +405,11 @@ int32_t tc_create(const tc_config *cfg, tc_handle **out) {
     return fail(TC_E_HIP, "device setup failed");
   }
   tc::TcGeo &g = h->geo;
+  g.nf = o.nf;
+  g.QbrT = h->dQbr;
+  g.QbtT = h->dQbt;
+  g.QbtdT = h->dQbtd;
+  g.QbzT = h->dQbz;
   g.n = o.n;
   g.kh = o.kh;
   g.nth = o.nth;
@@ -386,42 +460,25 @@ int32_t tc_destroy(tc_handle *h) {
 int32_t tc_set_field(tc_handle *h, int32_t field, const double *host) {
   if (int32_t r = check_field(h, field)) return r;
   if (!host) return fail(TC_E_INVALID, "null argument");
-  const int kh = h->ops.kh, n = h->ops.n, nth = h->ops.nth, nz = h->cfg.nz;
-  TC_HIP(hipSetDevice(h->cfg.device));
-  // the Hermitian half: rows k = 0..kh-1 of every azimuthal mode of the full
-  // (Ntheta, Nz, n) array, in the library's mode order q = im * kh + k
-  std::vector<double> half(2 * (size_t)h->ops.nm * n);
-  for (int im = 0; im < nth; ++im)
-    std::memcpy(&half[2 * (size_t)im * kh * n], host + 2 * (size_t)im * nz * n, 2 * (size_t)kh * n * sizeof(double));
-  TC_HIP(hipMemcpyAsync(field_ptr(h, field), half.data(), half.size() * sizeof(double), hipMemcpyHostToDevice,
-                        h->stream));
-  TC_HIP(hipStreamSynchronize(h->stream));
-  return TC_OK;
+  return upload_half(h, field_ptr(h, field), host);
 }
 
 int32_t tc_get_field(tc_handle *h, int32_t field, double *host) {
   if (int32_t r = check_field(h, field)) return r;
   if (!host) return fail(TC_E_INVALID, "null argument");
-  const int kh = h->ops.kh, n = h->ops.n, nth = h->ops.nth, nz = h->cfg.nz;
-  TC_HIP(hipSetDevice(h->cfg.device));
-  std::vector<double> half(2 * (size_t)h->ops.nm * n);
-  TC_HIP(hipMemcpyAsync(half.data(), field_ptr(h, field), half.size() * sizeof(double), hipMemcpyDeviceToHost,
-                        h->stream));
-  TC_HIP(hipStreamSynchronize(h->stream));
-  std::memset(host, 0, 2 * (size_t)nth * nz * n * sizeof(double));
-  for (int im = 0; im < nth; ++im)
-    for (int k = 0; k < kh; ++k) {
-      const double *src = &half[2 * ((size_t)im * kh + k) * n];
-      std::memcpy(host + 2 * ((size_t)im * nz + k) * n, src, 2 * (size_t)n * sizeof(double));
-      if (k == 0) continue;
-      // f(-m, -k) = conj f(m, k); the axial Nyquist row stays zero
-      double *dst = host + 2 * ((size_t)((nth - im) % nth) * nz + (nz - k)) * n;
-      for (int i = 0; i < n; ++i) {
-        dst[2 * i] = src[2 * i];
-        dst[2 * i + 1] = -src[2 * i + 1];
-      }
-    }
-  return TC_OK;
+  return download_full(h, field_ptr(h, field), host);
+}
+
+int32_t tc_set_component(tc_handle *h, int32_t group, int32_t comp, const double *host) {
+  if (int32_t r = check_component(h, group, comp)) return r;
+  if (!host) return fail(TC_E_INVALID, "null argument");
+  return upload_half(h, component_ptr(h, group, comp), host);
+}
+
+int32_t tc_get_component(tc_handle *h, int32_t group, int32_t comp, double *host) {
+  if (int32_t r = check_component(h, group, comp)) return r;
+  if (!host) return fail(TC_E_INVALID, "null argument");
+  return download_full(h, component_ptr(h, group, comp), host);
 }
 
 int32_t tc_set_scalars(tc_handle *h, double have_old, double history_steps) {
@@ -504,7 +561,8 @@ int32_t tc_diagnostics(tc_handle *h, double *out) {
   TC_HIP(hipSetDevice(h->cfg.device));
   hipError_t e = tc::launch_diag(h->geo, h->S[h->su], h->dout, h->stream);
   if (e != hipSuccess) return fail(TC_E_HIP, std::string("diagnostics launch: ") + hipGetErrorString(e));
-  TC_HIP(hipMemcpyAsync(out, h->dout, TC_DIAG_COUNT * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  const int count = h->ops.nf == 6 ? TC_MHD_DIAG_COUNT : TC_DIAG_COUNT;
+  TC_HIP(hipMemcpyAsync(out, h->dout, count * sizeof(double), hipMemcpyDeviceToHost, h->stream));
   TC_HIP(hipStreamSynchronize(h->stream));
   return TC_OK;
 }
@@ -514,6 +572,17 @@ int32_t tc_linear_pencil(const tc_config *cfg, int32_t m, double kz, double *out
   if (cfg->abi_version != TC_ABI_VERSION) return fail(TC_E_INVALID, "ABI version mismatch");
   std::vector<double> L, M;
   std::string err = tc::linear_pencil(params_of(*cfg), m, kz, L, M);
+  if (!err.empty()) return fail(TC_E_INVALID, err);
+  std::memcpy(out_L, L.data(), L.size() * sizeof(double));
+  std::memcpy(out_M, M.data(), M.size() * sizeof(double));
+  return TC_OK;
+}
+
+int32_t tc_mri_pencil(const tc_config *cfg, int32_t m, double kz, double *out_L, double *out_M) {
+  if (!cfg || !out_L || !out_M) return fail(TC_E_INVALID, "null argument");
+  if (cfg->abi_version != TC_ABI_VERSION) return fail(TC_E_INVALID, "ABI version mismatch");
+  std::vector<double> L, M;
+  std::string err = tc::mri_pencil(params_of(*cfg), m, kz, L, M);
   if (!err.empty()) return fail(TC_E_INVALID, err);
   std::memcpy(out_L, L.data(), L.size() * sizeof(double));
   std::memcpy(out_M, M.data(), M.size() * sizeof(double));

@@ -18,33 +18,39 @@ __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
 }
 
 // ---------------------------------------------------------- k_tc_nonlinear --
-// One block per padded radial point j: u, du/dr and du/dz of the three
-// components at (r_j, every k), the axial synthesis on mz points, the
-// cylindrical advection products (:716-739)
+// One block per padded radial point j: value, d/dr and d/dz of every field at
+// (r_j, every k), the axial synthesis on mz points, the products, and their
+// axial analysis G[c][k][j] = (Lz / mz) sum_m n_c(z_m) e^{-i k z_m}.
+// Hydrodynamic (NF = 3, :716-739): the cylindrical advection
 //   n_r = u_r du_r/dr + u_z du_r/dz - u_t^2 / r
 //   n_t = u_r du_t/dr + u_z du_t/dz + u_r u_t / r
-//   n_z = u_r du_z/dr + u_z du_z/dz,
-// and their axial analysis G[c][k][j] = (Lz / mz) sum_m n_c(z_m) e^{-i k z_m}.
+//   n_z = u_r du_z/dr + u_z du_z/dz.
+// MHD (NF = 6, :1603-1649): the advection minus the same form of b (the
+// Lorentz force) in G[0..2] and the EMF u x b in G[3..5]; the rows of b are
+// formed from the latter in the solve kernel.
 // The twiddles sit in LDS and every transform loop steps its table index
 // instead of reducing k m modulo mz.
+template <int NF>
 __global__ void __launch_bounds__(256) k_tc_nonlinear(TcGeo g, const double2 *__restrict__ u,
                                                       double2 *__restrict__ G) {
   extern __shared__ __align__(16) unsigned char smem[];
-  const int j = blockIdx.x, kh = g.kh, n = g.n, mz = g.mz;
+  constexpr int NV = 3 * NF, NG = NF == 6 ? 6 : 3;
+  const int j = blockIdx.x, kh = g.kh, n = g.n, mz = g.mz, mr = g.mr;
   double2 *tw = reinterpret_cast<double2 *>(smem);             // [mz]
-  double2 *val = tw + mz;                                      // [9][kh]: (c, v/dr/dz)
-  double *phys = reinterpret_cast<double *>(val + 9 * kh);     // [9][mz]
-  double *prod = phys + 9 * mz;                                // [3][mz]
-  double2 *part = reinterpret_cast<double2 *>(prod + 3 * mz);  // [nch][6 kh] partial sums
+  double2 *val = tw + mz;                                      // [NV][kh]: (c, v/dr/dz)
+  double *phys = reinterpret_cast<double *>(val + NV * kh);    // [NV][mz]
+  double *prod = phys + NV * mz;                               // [NG][mz]
+  double2 *part = reinterpret_cast<double2 *>(prod + NG * mz);  // [nch][2 NF kh] partial sums
   const int tid = threadIdx.x;
   for (int t = tid; t < mz; t += blockDim.x) tw[t] = g.twz[t];
-  // 6 kh dot products of length n (value and d/dr of three components at every
+  // 2 NF kh dot products of length n (value and d/dr of every field at every
   // mode), each split into nch chunks so that every thread has work
-  const int nd = 6 * kh, nch = max(1, (int)blockDim.x / nd), len = (n + nch - 1) / nch;
+  const int nd = 2 * NF * kh, nch = max(1, (int)blockDim.x / nd), len = (n + nch - 1) / nch;
   for (int t = tid; t < nd * nch; t += blockDim.x) {
     const int d = t % nd, ch = t / nd;
     const int c = d / (2 * kh), rem = d - c * 2 * kh, typ = rem / kh, k = rem - typ * kh;
-    const double *E = (typ == 0 ? g.Ev : g.Ed) + (size_t)j * n;
+    const int basis = c < 4 ? 0 : c - 3;  // b_theta: Robin, b_z: Neumann
+    const double *E = (typ == 0 ? g.Ev : g.Ed) + (size_t)basis * mr * n + (size_t)j * n;
     const double2 *x = u + ((size_t)c * kh + k) * n;
     const int i0 = ch * len, i1 = min(n, i0 + len);
     double sr = 0.0, si = 0.0;
@@ -67,7 +73,7 @@ __global__ void __launch_bounds__(256) k_tc_nonlinear(TcGeo g, const double2 *__
   }
   __syncthreads();
   // Hermitian synthesis: f(z_m) = Re c_0 + 2 Re sum_{k>0} c_k e^{i k z_m}
-  for (int t = tid; t < 9 * mz; t += blockDim.x) {
+  for (int t = tid; t < NV * mz; t += blockDim.x) {
     const int f = t / mz, m = t - f * mz;
     const double2 *c = val + f * kh;
     double s = 0.0;
@@ -86,13 +92,27 @@ __global__ void __launch_bounds__(256) k_tc_nonlinear(TcGeo g, const double2 *__
     const double ur = phys[0 * mz + m], urr = phys[1 * mz + m], urz = phys[2 * mz + m];
     const double ut = phys[3 * mz + m], utr = phys[4 * mz + m], utz = phys[5 * mz + m];
     const double uz = phys[6 * mz + m], uzr = phys[7 * mz + m], uzz = phys[8 * mz + m];
-    prod[m] = ur * urr + uz * urz - ut * ut * ir;
-    prod[mz + m] = ur * utr + uz * utz + ur * ut * ir;
-    prod[2 * mz + m] = ur * uzr + uz * uzz;
+    double nr = ur * urr + uz * urz - ut * ut * ir;
+    double nt = ur * utr + uz * utz + ur * ut * ir;
+    double nz = ur * uzr + uz * uzz;
+    if constexpr (NF == 6) {
+      const double br = phys[9 * mz + m], brr = phys[10 * mz + m], brz = phys[11 * mz + m];
+      const double bt = phys[12 * mz + m], btr = phys[13 * mz + m], btz = phys[14 * mz + m];
+      const double bz = phys[15 * mz + m], bzr = phys[16 * mz + m], bzz = phys[17 * mz + m];
+      nr -= br * brr + bz * brz - bt * bt * ir;
+      nt -= br * btr + bz * btz + br * bt * ir;
+      nz -= br * bzr + bz * bzz;
+      prod[3 * mz + m] = ut * bz - uz * bt;
+      prod[4 * mz + m] = uz * br - ur * bz;
+      prod[5 * mz + m] = ur * bt - ut * br;
+    }
+    prod[m] = nr;
+    prod[mz + m] = nt;
+    prod[2 * mz + m] = nz;
   }
   __syncthreads();
   const double sc = g.lz / mz;
-  for (int t = tid; t < 3 * kh; t += blockDim.x) {
+  for (int t = tid; t < NG * kh; t += blockDim.x) {
     const int c = t / kh, k = t - c * kh;
     const double *p = prod + c * mz;
     double sr = 0.0, si = 0.0;
@@ -104,30 +124,60 @@ __global__ void __launch_bounds__(256) k_tc_nonlinear(TcGeo g, const double2 *__
       idx += k;
       if (idx >= mz) idx -= mz;
     }
-    G[((size_t)c * kh + k) * g.mr + j] = make_double2(sc * sr, sc * si);
+    G[((size_t)c * kh + k) * mr + j] = make_double2(sc * sr, sc * si);
   }
 }
 
-// --------------------------------------------------------- k_tc_rhs_solve --
-// One block per stored axial mode k: N = Pr G (the dealiased scalar products
-// of the products, :733-739), the right-hand side of the integrator, then
-// (u, p) = Ainv_k rhs with the pinned inverse of the mode's saddle-point block
-// (:614-637; the pressure rows of rhs are zero, so only the velocity columns
-// of the inverse are stored).  The right-hand side's terms are first combined
-// per operator (sum c_t X_t for the mass terms, the L terms, the explicit
-// terms), so each operator is applied once.  One thread per right-hand-side
-// row; the tables are stored transposed, so consecutive rows read consecutive
-// addresses, and the vector operands sit in LDS.  gridDim.y blocks share a
-// mode: each forms the whole right-hand side and solves a slice of the rows.
 constexpr int RHS_THREADS = 256;
 
+// The nonlinear row i of field c from the axial analyses G of the mode: the
+// dealiased projection of the products (velocity rows), or with the MHD
+// fields minus the curl of the EMF (G components 3..5 hold its r, theta, z
+// parts): b_r i kz Qbr G_t, b_theta -i kz Qbt G_r + Qbtd G_z, b_z Qbz G_t.
+__device__ __forceinline__ double2 nl_row(const TcGeo &g, const double2 *gs, int c, int i, double kz) {
+  const int n = g.n, mr = g.mr;
+  auto dot = [&](const double *T, const double2 *Gc) {
+    double sr = 0.0, si = 0.0;
+#pragma unroll 8
+    for (int jj = 0; jj < mr; ++jj) {
+      const double p = T[(size_t)jj * n + i];
+      sr = fma(p, Gc[jj].x, sr);
+      si = fma(p, Gc[jj].y, si);
+    }
+    return make_double2(sr, si);
+  };
+  if (c < 3) return dot(g.PrT, gs + c * mr);
+  if (c == 3) {
+    const double2 v = dot(g.QbrT, gs + 4 * mr);
+    return make_double2(-kz * v.y, kz * v.x);
+  }
+  if (c == 4) {
+    const double2 v = dot(g.QbtT, gs + 3 * mr), w = dot(g.QbtdT, gs + 5 * mr);
+    return make_double2(w.x + kz * v.y, w.y - kz * v.x);
+  }
+  return dot(g.QbzT, gs + 4 * mr);
+}
+
+// --------------------------------------------------------- k_tc_rhs_solve --
+// One block per stored Fourier mode: the nonlinear rows N from the product
+// kernel's analyses (:733-739; the MHD rows :1603-1649), the right-hand side
+// of the integrator, then (fields, p) = Ainv rhs with the pinned inverse of
+// the mode's saddle-point block (:614-637; the pressure rows of rhs are zero,
+// so only the field columns of the inverse are stored).  The right-hand
+// side's terms are first combined per operator (sum c_t X_t for the mass
+// terms, the L terms, the explicit terms), so each operator is applied once.
+// One thread per right-hand-side row; the tables are stored transposed, so
+// consecutive rows read consecutive addresses, and the vector operands sit
+// in LDS.  gridDim.y blocks share a mode: each forms the whole right-hand
+// side and solves a slice of the rows.
 __global__ void __launch_bounds__(RHS_THREADS) k_tc_rhs_solve(TcGeo g, TcRhs a) {
   extern __shared__ __align__(16) unsigned char smem[];
-  const int k = blockIdx.x, kh = g.nm, n = g.n, n3 = 3 * n, n4 = 4 * n, mr = g.mr;
-  double2 *gs = reinterpret_cast<double2 *>(smem);  // [3 mr]: G of mode k
-  double2 *xm = gs + 3 * mr;                        // [max(3n, threads)]: combined mass operand
-  double2 *xl = xm + max(n3, RHS_THREADS);          // [3n]: combined L / explicit operand
-  double2 *rhs = xl + n3;                           // [3n]
+  const int k = blockIdx.x, kh = g.nm, n = g.n, nfn = g.nf * n, n4 = nfn + n, mr = g.mr;
+  const int ng = g.nf == 6 ? 6 : 3;                 // G components
+  double2 *gs = reinterpret_cast<double2 *>(smem);  // [ng mr]: G of mode k
+  double2 *xm = gs + ng * mr;                       // [max(nf n, threads)]: combined mass operand
+  double2 *xl = xm + max(nfn, RHS_THREADS);         // [nf n]: combined L / explicit operand
+  double2 *rhs = xl + nfn;                          // [nf n]
   const int tid = threadIdx.x;
   bool has_mass = false, has_lin = false;
   int lop = TC_OP_LIN;
@@ -139,11 +189,11 @@ __global__ void __launch_bounds__(RHS_THREADS) k_tc_rhs_solve(TcGeo g, TcRhs a) 
     }
   }
   if (a.G)
-    for (int t = tid; t < 3 * mr; t += blockDim.x) {
+    for (int t = tid; t < ng * mr; t += blockDim.x) {
       const int c = t / mr, jj = t - c * mr;
       gs[t] = a.G[((size_t)c * kh + k) * mr + jj];
     }
-  for (int row = tid; row < n3; row += blockDim.x) {
+  for (int row = tid; row < nfn; row += blockDim.x) {
     const int c = row / n;
     const size_t o = ((size_t)c * kh + k) * n + (row - c * n);
     double2 m = make_double2(0.0, 0.0), l = m, d = m;
@@ -159,28 +209,23 @@ __global__ void __launch_bounds__(RHS_THREADS) k_tc_rhs_solve(TcGeo g, TcRhs a) 
     rhs[row] = d;
   }
   __syncthreads();
-  for (int row = tid; row < n3; row += blockDim.x) {
+  const double kz = g.kz[k];
+  for (int row = tid; row < nfn; row += blockDim.x) {
     const int c = row / n, i = row - c * n;
     double2 r = rhs[row];
     if (a.G) {
-      const double2 *Gc = gs + c * mr;
-      double sr = 0.0, si = 0.0;
-#pragma unroll 8
-      for (int jj = 0; jj < mr; ++jj) {
-        const double p = g.PrT[(size_t)jj * n + i];
-        sr = fma(p, Gc[jj].x, sr);
-        si = fma(p, Gc[jj].y, si);
-      }
-      if (a.nout && blockIdx.y == 0) a.nout[((size_t)c * kh + k) * n + i] = make_double2(sr, si);
-      r.x = fma(a.cN, sr, r.x);
-      r.y = fma(a.cN, si, r.y);
+      const double2 v = nl_row(g, gs, c, i, kz);
+      if (a.nout && blockIdx.y == 0) a.nout[((size_t)c * kh + k) * n + i] = v;
+      r.x = fma(a.cN, v.x, r.x);
+      r.y = fma(a.cN, v.y, r.y);
     }
-    if (has_mass) {  // block diagonal: the mass of the row's component
+    if (has_mass) {  // block diagonal: the mass of the row's field
       const double2 *x = xm + c * n;
+      const double *MT = g.MassT + (size_t)c * n * n;
       double sr = 0.0, si = 0.0;
 #pragma unroll 8
       for (int jj = 0; jj < n; ++jj) {
-        const double mm = g.MassT[(size_t)jj * n + i];
+        const double mm = MT[(size_t)jj * n + i];
         sr = fma(mm, x[jj].x, sr);
         si = fma(mm, x[jj].y, si);
       }
@@ -188,11 +233,11 @@ __global__ void __launch_bounds__(RHS_THREADS) k_tc_rhs_solve(TcGeo g, TcRhs a) 
       r.y += si;
     }
     if (has_lin) {  // complex per-mode operator
-      const double2 *M = (lop == TC_OP_LIN ? g.LinT : g.LexpT) + (size_t)k * n3 * n3;
+      const double2 *M = (lop == TC_OP_LIN ? g.LinT : g.LexpT) + (size_t)k * nfn * nfn;
       double sr = 0.0, si = 0.0;
 #pragma unroll 8
-      for (int jj = 0; jj < n3; ++jj) {
-        const double2 mm = M[(size_t)jj * n3 + row], x = xl[jj];
+      for (int jj = 0; jj < nfn; ++jj) {
+        const double2 mm = M[(size_t)jj * nfn + row], x = xl[jj];
         sr += mm.x * x.x - mm.y * x.y;
         si += mm.x * x.y + mm.y * x.x;
       }
@@ -202,14 +247,14 @@ __global__ void __launch_bounds__(RHS_THREADS) k_tc_rhs_solve(TcGeo g, TcRhs a) 
     rhs[row] = r;
   }
   __syncthreads();
-  // this block's slice of the 4n solution rows, the contracted index split
-  // into chunks over the threads (partial sums meet in LDS in a fixed order)
-  const double2 *A = a.AinvT + (size_t)k * n3 * n4;  // [j < 3n][r < 4n]
+  // this block's slice of the solution rows, the contracted index split into
+  // chunks over the threads (partial sums meet in LDS in a fixed order)
+  const double2 *A = a.AinvT + (size_t)k * nfn * n4;  // [j < nf n][r < (nf + 1) n]
   const int per = (n4 + gridDim.y - 1) / gridDim.y, r0 = blockIdx.y * per, nrow = min(per, n4 - r0);
   double2 *part = xm;  // the mass operand is no longer needed: [RHS_THREADS]
-  const int nch = max(1, (int)blockDim.x / max(nrow, 1)), clen = (n3 + nch - 1) / nch;
+  const int nch = max(1, (int)blockDim.x / max(nrow, 1)), clen = (nfn + nch - 1) / nch;
   for (int t = tid; t < nrow * nch; t += blockDim.x) {
-    const int row = r0 + t % nrow, ch = t / nrow, j0 = ch * clen, j1 = min(n3, j0 + clen);
+    const int row = r0 + t % nrow, ch = t / nrow, j0 = ch * clen, j1 = min(nfn, j0 + clen);
     double sr = 0.0, si = 0.0;
 #pragma unroll 4
     for (int jj = j0; jj < j1; ++jj) {
@@ -227,11 +272,11 @@ __global__ void __launch_bounds__(RHS_THREADS) k_tc_rhs_solve(TcGeo g, TcRhs a) 
       si += part[ch * nrow + t].y;
     }
     const int row = r0 + t;
-    if (row < n3) {
+    if (row < nfn) {
       const int c = row / n, i = row - c * n;
       a.uout[((size_t)c * kh + k) * n + i] = make_double2(sr, si);
     } else {
-      a.pout[(size_t)k * n + (row - n3)] = make_double2(sr, si);
+      a.pout[(size_t)k * n + (row - nfn)] = make_double2(sr, si);
     }
   }
 }
@@ -343,6 +388,83 @@ __global__ void __launch_bounds__(256) k_tc_diag(TcGeo g, const double2 *__restr
     out[2] = sqrt(cres);
     out[3] = 0.0;  // the Dirichlet modes vanish at both walls: T_i(+-1) = T_{i+2}(+-1)
     out[4] = eth;
+  }
+}
+
+// ----------------------------------------------------------- k_tc_mhd_diag --
+// One block, the MHD set (:1844-1886): Ekin and Emag by Parseval in z (Fejer-I
+// weights in r), the largest |div u| and |div b| on the standard grid
+// (d/dr + 1/r of the radial component plus i kz of the axial one, b_z in its
+// Neumann basis), and the wall trace of u (zero: Dirichlet modes).
+__global__ void __launch_bounds__(256) k_tc_mhd_diag(TcGeo g, const double2 *__restrict__ u,
+                                                     double *__restrict__ out) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int kh = g.kh, n = g.n, nr = g.nr, nz = g.nz;
+  const size_t se = (size_t)nr * n;
+  double *red = reinterpret_cast<double *>(smem);  // [blockDim]
+  double2 *dv = g.dv;                              // [2][kh][nr]: div u, div b per mode
+  const int tid = threadIdx.x;
+  double ek = 0.0, em = 0.0;
+  for (int t = tid; t < kh * nr; t += blockDim.x) {
+    const int k = t / nr, r = t - k * nr;
+    double2 v[6], dr[2];
+    for (int c = 0; c < 6; ++c) {
+      const int basis = c < 4 ? 0 : c - 3;
+      const double *E = g.Es + basis * se + (size_t)r * n;
+      const double2 *x = u + ((size_t)c * kh + k) * n;
+      double sr = 0.0, si = 0.0;
+      for (int i = 0; i < n; ++i) {
+        sr = fma(E[i], x[i].x, sr);
+        si = fma(E[i], x[i].y, si);
+      }
+      v[c] = make_double2(sr, si);
+    }
+    for (int h = 0; h < 2; ++h) {  // d/dr of u_r and b_r (Dirichlet)
+      const double *E = g.Esd + (size_t)r * n;
+      const double2 *x = u + ((size_t)(3 * h) * kh + k) * n;
+      double sr = 0.0, si = 0.0;
+      for (int i = 0; i < n; ++i) {
+        sr = fma(E[i], x[i].x, sr);
+        si = fma(E[i], x[i].y, si);
+      }
+      dr[h] = make_double2(sr, si);
+    }
+    const double w = (k == 0 ? 1.0 : 2.0) * g.rstd[r] * g.fejer[r] * g.lz;
+    double su = 0.0, sb = 0.0;
+    for (int c = 0; c < 3; ++c) {
+      su += v[c].x * v[c].x + v[c].y * v[c].y;
+      sb += v[c + 3].x * v[c + 3].x + v[c + 3].y * v[c + 3].y;
+    }
+    ek += 0.5 * w * su;
+    em += 0.5 * w * sb;
+    const double ir = 1.0 / g.rstd[r], kz = g.kz[k];
+    for (int h = 0; h < 2; ++h) {
+      const double2 a = v[3 * h], z = v[3 * h + 2];
+      dv[((size_t)h * kh + k) * nr + r] = make_double2(dr[h].x + a.x * ir - kz * z.y, dr[h].y + a.y * ir + kz * z.x);
+    }
+  }
+  ek = block_sum(ek, red);
+  em = block_sum(em, red);
+  __threadfence_block();
+  __syncthreads();
+  double dmax[2] = {0.0, 0.0};
+  for (int t = tid; t < 2 * nz * nr; t += blockDim.x) {
+    const int h = t / (nz * nr), rem = t - h * nz * nr, b = rem / nr, r = rem - b * nr;
+    double s = 0.0;
+    for (int k = 0; k < kh; ++k) {
+      const double2 d = dv[((size_t)h * kh + k) * nr + r], wz = g.twn[(k * b) % nz];
+      s += (k == 0 ? 1.0 : 2.0) * (d.x * wz.x - d.y * wz.y);
+    }
+    dmax[h] = fmax(dmax[h], fabs(s));
+  }
+  const double du = block_max(dmax[0], red), db = block_max(dmax[1], red);
+  if (tid == 0) {
+    out[0] = ek;
+    out[1] = em;
+    out[2] = ek + em;
+    out[3] = du;
+    out[4] = db;
+    out[5] = 0.0;  // the Dirichlet modes vanish at both walls
   }
 }
 
@@ -476,14 +598,16 @@ size_t nonlinear_lds(const TcGeo &g) {
     const size_t region = std::max((size_t)12 * g.nm * 2, (size_t)3 * g.mth * g.mz + (size_t)3 * g.mth * g.kh * 2);
     return ((size_t)g.mz + g.mth) * sizeof(double2) + (region + (size_t)12 * g.kh * g.mth * 2) * sizeof(double);
   }
-  const int nd = 6 * g.kh, nch = std::max(1, 256 / nd);
-  return ((size_t)g.mz + 9 * (size_t)g.kh) * sizeof(double2) + 12 * (size_t)g.mz * sizeof(double) +
+  const size_t nf = g.nf, ng = g.nf == 6 ? 6 : 3;
+  const int nd = 2 * g.nf * g.kh, nch = std::max(1, 256 / nd);
+  return ((size_t)g.mz + 3 * nf * g.kh) * sizeof(double2) + (3 * nf + ng) * (size_t)g.mz * sizeof(double) +
          (size_t)nd * nch * sizeof(double2);
 }
 
 size_t rhs_lds(const TcGeo &g) {
   // gs, xm (also the solve's partial sums), xl, rhs
-  return (3 * (size_t)g.mr + std::max(3 * (size_t)g.n, (size_t)RHS_THREADS) + 6 * (size_t)g.n) * sizeof(double2);
+  const size_t nfn = (size_t)g.nf * g.n, ng = g.nf == 6 ? 6 : 3;
+  return (ng * g.mr + std::max(nfn, (size_t)RHS_THREADS) + 2 * nfn) * sizeof(double2);
 }
 
 // Blocks per mode of the solve kernel: row slices of the 4n solution rows.
@@ -495,8 +619,10 @@ hipError_t launch_nonlinear(const TcGeo &g, const double2 *u, double2 *G, hipStr
   const size_t lds = nonlinear_lds(g);
   if (g.nth > 1)
     hipLaunchKernelGGL(k_tc3_nonlinear, dim3(g.mr), dim3(TC3_THREADS), lds, s, g, u, G);
+  else if (g.nf == 6)
+    hipLaunchKernelGGL(k_tc_nonlinear<6>, dim3(g.mr), dim3(256), lds, s, g, u, G);
   else
-    hipLaunchKernelGGL(k_tc_nonlinear, dim3(g.mr), dim3(256), lds, s, g, u, G);
+    hipLaunchKernelGGL(k_tc_nonlinear<3>, dim3(g.mr), dim3(256), lds, s, g, u, G);
   return hipGetLastError();
 }
 
@@ -516,7 +642,10 @@ hipError_t launch_rhs_solve(const TcGeo &g, const TcRhs &a, hipStream_t s) {
 
 hipError_t launch_diag(const TcGeo &g, const double2 *u, double *out, hipStream_t s) {
   const size_t lds = diag_lds(g);
-  hipLaunchKernelGGL(k_tc_diag, dim3(1), dim3(256), lds, s, g, u, out);
+  if (g.nf == 6)
+    hipLaunchKernelGGL(k_tc_mhd_diag, dim3(1), dim3(256), lds, s, g, u, out);
+  else
+    hipLaunchKernelGGL(k_tc_diag, dim3(1), dim3(256), lds, s, g, u, out);
   return hipGetLastError();
 }
 

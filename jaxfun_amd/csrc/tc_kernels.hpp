@@ -1,4 +1,4 @@
-// Kernels of the axisymmetric Taylor-Couette step (tc_kernels.hip).
+// Kernels of the Taylor-Couette step (tc_kernels.hip).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -7,6 +7,7 @@ namespace tc {
 
 // Sizes and device tables of one handle.
 struct TcGeo {
+  int nf = 3;         // evolved fields: u (3) or u, b (6, MHD)
   int n = 0;          // radial dofs per mode (Nr - 2)
   int kh = 0;         // stored axial modes 0..kh-1
   int nth = 1;        // azimuthal modes (1: axisymmetric)
@@ -22,15 +23,16 @@ struct TcGeo {
   const double2 *twt = nullptr;                   // (mth) e^{2 pi i q / mth}
   const double2 *twn_t = nullptr;                 // (nth) e^{2 pi i q / nth}
   double2 *dv = nullptr;                          // (nm x nr) diagnostics scratch
-  const double *Ev = nullptr, *Ed = nullptr;  // (mr x n)
+  const double *Ev = nullptr, *Ed = nullptr;  // [basis](mr x n): Dirichlet, then Robin b_theta and Neumann b_z (MHD)
+  const double *QbrT = nullptr, *QbtT = nullptr, *QbtdT = nullptr, *QbzT = nullptr;  // (mr x n) MHD EMF rows
   const double *PrT = nullptr;                // (mr x n): the projection, transposed
   const double *invr = nullptr;               // (mr)
   const double *kz = nullptr;                 // (nm) axial wavenumbers of the modes
   const double2 *twz = nullptr;               // (mz) e^{2 pi i q / mz}
   const double2 *twn = nullptr;               // (nz) e^{2 pi i q / nz}
-  const double *MassT = nullptr;              // (n x n), transposed
-  const double2 *LinT = nullptr, *LexpT = nullptr;  // [nm][3n x 3n] complex, transposed
-  const double *Es = nullptr, *Esd = nullptr;    // (nr x n)
+  const double *MassT = nullptr;              // [nf](n x n), transposed
+  const double2 *LinT = nullptr, *LexpT = nullptr;  // [nm][nf n x nf n] complex, transposed
+  const double *Es = nullptr, *Esd = nullptr;    // [basis](nr x n)
   const double *rstd = nullptr, *fejer = nullptr;  // (nr)
   const double *Drc = nullptr, *Dzc = nullptr;     // (n x n)
 };
@@ -53,14 +55,15 @@ struct TcRhs {
   const double2 *G = nullptr;
   double cN = 0.0;
   double2 *nout = nullptr;
-  const double2 *AinvT = nullptr;  // [nm][3n x 4n]: the inverse's velocity columns, transposed
+  const double2 *AinvT = nullptr;  // [nm][nf n x (nf + 1) n]: the inverse's field columns, transposed
   double2 *uout = nullptr;
   double2 *pout = nullptr;
 };
 
 hipError_t launch_nonlinear(const TcGeo &g, const double2 *u, double2 *G, hipStream_t s);
 hipError_t launch_rhs_solve(const TcGeo &g, const TcRhs &a, hipStream_t s);
-// out[0..4] = E, div_linf, continuity_l2, wall, Eth of u.
+// out[0..4] = E, div_linf, continuity_l2, wall, Eth of u; with nf = 6,
+// out[0..5] = Ekin, Emag, E, divu, divb, wall_u.
 hipError_t launch_diag(const TcGeo &g, const double2 *u, double *out, hipStream_t s);
 // Dynamic LDS bytes of the three kernels (the handle checks them at creation).
 size_t nonlinear_lds(const TcGeo &g);  // the axisymmetric or the 3D product kernel

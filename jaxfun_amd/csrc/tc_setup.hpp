@@ -1,4 +1,4 @@
-// Host setup of the axisymmetric Taylor-Couette step (tc_setup.cpp): radial
+// Host setup of the Taylor-Couette step (tc_setup.cpp): radial
 // Galerkin blocks, per-axial-mode saddle-point inverses and the transform
 // matrices of the nonlinear term, all formed in long double.
 #pragma once
@@ -17,10 +17,14 @@ struct Params {
   double nu = 0.0, lz = 0.0, dt = 0.0;
   double dealias = 1.0;
   int integrator = 0;          // TC_TI_* (tc_dns.h)
+  int mhd = 0;                 // 1: conducting-wall MHD (fields u, b; axisymmetric only)
+  double b0 = 0.0, eta = 0.0;  // imposed axial field, magnetic diffusivity
 };
 
 // Dense row-major matrices; complex ones as interleaved (re, im) doubles.
 struct Ops {
+  int nf = 3;       // evolved fields: u (3) or u, b (6)
+  int nb = 1;       // radial bases of the fields: Dirichlet (and Robin b_theta, Neumann b_z)
   int n = 0;        // velocity (and pressure) dofs per mode: nr - 2
   int kh = 0;       // stored axial modes 0..kh-1 (the Nyquist mode is not stored)
   int nth = 1;      // azimuthal modes (all stored, FFT order)
@@ -31,16 +35,22 @@ struct Ops {
   double df = 0.0;  // 2 / (r2 - r1)
   std::vector<double> r_std, r_pad;       // radial nodes (ascending)
   std::vector<double> fejer;              // Fejer-I weights / df at the standard nodes
-  std::vector<double> Ev, Ed;             // (mr x n): phi_i and d phi_i / dr at the padded nodes
-  std::vector<double> Es, Esd;            // (nr x n): the same at the standard nodes
+  std::vector<double> Ev, Ed;             // [nb](mr x n): psi_i and d psi_i / dr at the padded nodes
+  std::vector<double> Es, Esd;            // [nb](nr x n): the same at the standard nodes
   std::vector<double> PrT;                // (mr x n): dealiased scalar products of padded values, transposed
-  std::vector<double> MassT;              // (n x n): Lz (phi_i, phi_j) / dt, transposed
+  // MHD: the magnetic rows of minus the curl of the EMF from the axial analyses
+  // of its padded values (truncated Chebyshev coefficients, derivatives on the
+  // standard grid, scalar products with the row's test basis), transposed (mr x n)
+  std::vector<double> QbrT, QbtT, QbtdT, QbzT;
+  std::vector<double> MassT;              // [nf](n x n): fz (psi_i, psi_j) / dt of each field's basis, transposed
   std::vector<double> Drc, Dzc, Dtc;      // (n x n): continuity blocks (T_i, d phi_j/dr + phi_j/r), (T_i, phi_j), (T_i, phi_j/r), times fz
   // per-mode tables, each stored transposed (entry (i, j) at [j][i]) so that a
   // thread per output row reads consecutive addresses
-  std::vector<double> LinT;               // [nm][3n x 3n] complex: the linear operator L (ARS stages)
-  std::vector<double> LexpT;              // [nm][3n x 3n] complex: mass + L / 2 (CNAB2)
-  std::vector<double> AinvT;              // [nm][3n x 4n] complex: velocity columns of the pinned implicit block's inverse
+  std::vector<double> LinT;               // [nm][nf n x nf n] complex: the linear operator L (ARS stages)
+  std::vector<double> LexpT;              // [nm][nf n x nf n] complex: mass + L / 2 (CNAB2)
+  // [nm][nf n x (nf + 1) n] complex: the field columns of the pinned implicit
+  // block's inverse, unknowns ordered fields first, pressure last
+  std::vector<double> AinvT;
   std::vector<double> Ainv0T;             // the same at the start-up scale 1/2 (SBDF3)
   std::vector<double> kz, mq, mq_odd;     // (nm): kz, the azimuthal wavenumber m, m with the Nyquist one zero
 };
@@ -52,5 +62,10 @@ std::string build_ops(const Params &p, Ops &o);
 // for azimuthal mode m and axial wavenumber kz: L = L0 + nu Lv and M, (4n x 4n)
 // complex, interleaved.
 std::string linear_pencil(const Params &p, int m, double kz, std::vector<double> &L, std::vector<double> &M);
+
+// The conducting-wall MRI pencil of examples/taylor_couette_mri_jax.py:214-321
+// (L = L0 + nu Lnu + eta Leta and M, (7n x 7n) complex, interleaved, unknowns
+// (u_r, u_theta, u_z, p, b_r, b_theta, b_z)).
+std::string mri_pencil(const Params &p, int m, double kz, std::vector<double> &L, std::vector<double> &M);
 
 }  // namespace tc

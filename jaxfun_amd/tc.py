@@ -454,3 +454,206 @@ class TaylorCouetteDNSHIP(AxisymmetricTCDNSHIP):
         dphi = df * (dT[: self.n] - dT[2:])
         mean_dut = np.asarray(state.u[1])[0, 0] @ dphi
         return float(2.0 * math.pi * self.nu * r1**2 * abs(-2.0 * b / r1**2 + mean_dut))
+
+
+@dataclass(frozen=True)
+class AxisymmetricMRIState:
+    """Coefficient state of the axisymmetric MHD step
+    (``examples/taylor_couette_dns_jax.py:169-191``): x = (u_r, u_theta, u_z,
+    b_r, b_theta, b_z), p the total pressure and the histories, each
+    (Nz, Nr - 2) complex in the full-complex layout."""
+
+    x: tuple
+    p: np.ndarray
+    nonlinear_old: tuple
+    have_old: float = 0.0
+    nonlinear_older: tuple | None = None
+    solution_old: tuple | None = None
+    solution_older: tuple | None = None
+    history_steps: float = 0.0
+
+
+def _wall_stencils(N: int, R1: float, R2: float):
+    """Radial bases of b_theta and b_z as rows of Chebyshev coefficients
+    (``src/jaxfun/galerkin/composite.py:797-871``): the Robin basis of
+    d(r b_theta)/dr = 0 (psi + R df psi_x = 0 at each wall) and the Neumann
+    basis T_i - i^2/(i+2)^2 T_{i+2}."""
+    df = 2.0 / (R2 - R1)
+    n = N - 2
+    rob = np.zeros((n, N))
+    neu = np.zeros((n, N))
+    for i in range(n):
+        lft = lambda j: (-1.0) ** j * (1.0 - R1 * df * j * j)  # noqa: E731
+        rgt = lambda j: 1.0 + R2 * df * j * j  # noqa: E731
+        d = np.linalg.solve([[lft(i + 1), lft(i + 2)], [rgt(i + 1), rgt(i + 2)]], [-lft(i), -rgt(i)])
+        rob[i, i], rob[i, i + 1], rob[i, i + 2] = 1.0, d[0], d[1]
+        neu[i, i], neu[i, i + 2] = 1.0, -(i * i) / (i + 2.0) ** 2
+    return rob, neu
+
+
+class AxisymmetricMRIDNSHIP(AxisymmetricTCDNSHIP):
+    """``AxisymmetricMRIDNSJax`` on the GPU (``examples/taylor_couette_dns_jax.py:1359-1903``):
+    conducting walls, the 7-field total-pressure system per axial mode."""
+
+    def __init__(self, base: CircularCouette, B0: float = 0.1, nu: float = 1.0e-3, eta_mag: float = 1.0e-3,
+                 Nr: int = 24, Nz: int = 16, Lz: float | None = None, dt: float = 2.0e-3, family: str = "C",
+                 dealias: float = 1.5, time_integrator: str = "CNAB2", device: int = 0):
+        self.B0 = float(B0)
+        self.eta_mag = float(eta_mag)
+        Lz = float(Lz) if Lz is not None else 2.0 * math.pi / 3.0 * base.gap
+        super().__init__(base, nu=nu, Nr=Nr, Nz=Nz, Lz=Lz, dt=dt, family=family, dealias=dealias,
+                         time_integrator=time_integrator, device=device)
+        self.Rm = base.Omega1 * base.R1 * base.gap / self.eta_mag
+        self.Pm = self.nu / self.eta_mag
+        self.S = self.B0 * base.gap / self.eta_mag
+        self.Jm = 0.5 * (base.R2 - base.R1)
+        rob, neu = _wall_stencils(self.Nr, base.R1, base.R2)
+        dirichlet = np.zeros((self.n, self.Nr))
+        dirichlet[np.arange(self.n), np.arange(self.n)] = 1.0
+        dirichlet[np.arange(self.n), np.arange(self.n) + 2] = -1.0
+        self._stencils = (dirichlet,) * 4 + (rob, neu)
+
+    def _config(self, device: int) -> nat.TcConfig:
+        c = super()._config(device)
+        c.mhd = 1
+        c.b0 = self.B0
+        c.eta_mag = self.eta_mag
+        return c
+
+    # ---------------------------------------------------------------- states
+    def zero_state(self) -> AxisymmetricMRIState:
+        x = tuple(self._zeros() for _ in range(6))
+        return AxisymmetricMRIState(x=x, p=self._zeros(), nonlinear_old=tuple(self._zeros() for _ in range(6)))
+
+    def _forward_basis(self, values, S) -> np.ndarray:
+        c = np.fft.fft(np.asarray(values, dtype=complex), axis=0, norm="forward")
+        V = self._vander()
+        psi = V @ S.T
+        w = np.pi / self.Nr
+        return np.linalg.solve(w * psi.T @ psi, (w * (c @ psi)).T).T
+
+    def _backward_basis(self, coef, S) -> np.ndarray:
+        return np.fft.ifft(np.asarray(coef) @ (self._vander() @ S.T).T, axis=0, norm="forward")
+
+    def state_from_physical(self, values) -> AxisymmetricMRIState:
+        x = tuple(self._forward_basis(v, S) for v, S in zip(values, self._stencils))
+        return AxisymmetricMRIState(x=x, p=self._zeros(), nonlinear_old=tuple(self._zeros() for _ in range(6)))
+
+    def mri_pencil(self, m: int, kz: float):
+        """(L, M) of ``TaylorCouetteMRIJax.assemble`` (conducting) on this radial space."""
+        n7 = 7 * self.n
+        L = np.empty((n7, n7), dtype=complex)
+        M = np.empty((n7, n7), dtype=complex)
+        nat.tc_check(self._lib.tc_mri_pencil(C.byref(self._cfg), int(m), float(kz),
+                                             L.view(np.float64).ctypes.data_as(_dp),
+                                             M.view(np.float64).ctypes.data_as(_dp)))
+        return L, M
+
+    def linear_eigs(self, m: int, kz: float, n_return: int = 8, finite_cap: float = 1.0e6):
+        w, V = sla.eig(*self.mri_pencil(m, kz))
+        keep = np.isfinite(w) & (np.abs(w) < finite_cap)
+        w, V = w[keep], V[:, keep]
+        order = np.lexsort((-w.imag, -w.real))
+        return w[order][:n_return], V[:, order][:, :n_return]
+
+    def seed_linear_eigenmode(self, kz_mode: int = 1, amp: float = 1.0e-6, which: int = 0):
+        """``:1809-1842``: the conducting MRI eigenmode, fields from blocks (0, 1, 2, 4, 5, 6)."""
+        kz = 2.0 * math.pi * int(kz_mode) / self.Lz
+        w, vecs = self.linear_eigs(0, kz, n_return=which + 1)
+        vec = _positive_pivot_phase(vecs[:, which])
+        n = self.n
+        kpos, kneg = int(kz_mode) % self.Nz, (-int(kz_mode)) % self.Nz
+        comps = []
+        for blk in (0, 1, 2, 4, 5, 6):
+            block = vec[blk * n:(blk + 1) * n] * amp
+            arr = self._zeros()
+            if kpos == kneg:
+                arr[kpos, :n] = np.real(block)
+            else:
+                arr[kpos, :n] = 0.5 * block
+                arr[kneg, :n] = 0.5 * np.conj(block)
+            comps.append(arr)
+        st = AxisymmetricMRIState(x=tuple(comps), p=self._zeros(), nonlinear_old=tuple(self._zeros() for _ in range(6)))
+        return st, complex(w[which])
+
+    # ------------------------------------------------------------ device I/O
+    def _set_c(self, group: str, comp: int, arr):
+        a = np.ascontiguousarray(np.asarray(arr, dtype=np.complex128))
+        if a.shape != self._shape:
+            raise ValueError(f"{group}[{comp}]: expected shape {self._shape}, got {a.shape}")
+        nat.tc_check(self._lib.tc_set_component(self._h, nat.TC_GROUP[group], comp,
+                                                a.view(np.float64).ctypes.data_as(_dp)))
+
+    def _get_c(self, group: str, comp: int) -> np.ndarray:
+        a = np.empty(self._shape, dtype=np.complex128)
+        nat.tc_check(self._lib.tc_get_component(self._h, nat.TC_GROUP[group], comp,
+                                                a.view(np.float64).ctypes.data_as(_dp)))
+        return a
+
+    @staticmethod
+    def _ensure_history(s: AxisymmetricMRIState) -> AxisymmetricMRIState:
+        """``_ensure_mhd_history`` (``:1705-1718``)."""
+        z = tuple(np.zeros_like(v) for v in s.x)
+        return replace(s, nonlinear_older=z if s.nonlinear_older is None else s.nonlinear_older,
+                       solution_old=s.x if s.solution_old is None else s.solution_old,
+                       solution_older=s.x if s.solution_older is None else s.solution_older)
+
+    def load(self, s: AxisymmetricMRIState) -> None:
+        if self.time_integrator == "SBDF3":
+            s = self._ensure_history(s)
+        for c in range(6):
+            self._set_c("X", c, s.x[c])
+        self._set_c("P", 0, s.p)
+        for group, grp in (("NL_OLD", s.nonlinear_old), ("NL_OLDER", s.nonlinear_older),
+                           ("SOL_OLD", s.solution_old), ("SOL_OLDER", s.solution_older)):
+            if grp is not None:
+                for c in range(6):
+                    self._set_c(group, c, grp[c])
+        nat.tc_check(self._lib.tc_set_scalars(self._h, float(s.have_old), float(s.history_steps)))
+
+    def fetch(self, like: AxisymmetricMRIState) -> AxisymmetricMRIState:
+        grp = lambda g: tuple(self._get_c(g, c) for c in range(6))  # noqa: E731
+        ho, hs = C.c_double(), C.c_double()
+        nat.tc_check(self._lib.tc_get_scalars(self._h, C.byref(ho), C.byref(hs)))
+        x, p = grp("X"), self._get_c("P", 0)
+        if self.time_integrator == "SBDF3":
+            return AxisymmetricMRIState(x=x, p=p, nonlinear_old=grp("NL_OLD"), have_old=ho.value,
+                                        nonlinear_older=grp("NL_OLDER"), solution_old=grp("SOL_OLD"),
+                                        solution_older=grp("SOL_OLDER"), history_steps=hs.value)
+        if self.time_integrator == "CNAB2":
+            return replace(like, x=x, p=p, nonlinear_old=grp("NL_OLD"), have_old=ho.value)
+        return replace(like, x=x, p=p)
+
+    # ------------------------------------------------------------ diagnostics
+    def _device_diagnostics(self, state: AxisymmetricMRIState | None) -> np.ndarray:
+        if state is not None:
+            self.load(state)
+        out = np.zeros(len(nat.TC_MHD_DIAG_KEYS))
+        nat.tc_check(self._lib.tc_diagnostics(self._h, _ptr(out)))
+        return out
+
+    def diagnostics(self, state: AxisymmetricMRIState) -> dict:
+        """Ekin, Emag, E, divu, divb, wall_u (``:1876-1886``), device reductions."""
+        d = self._device_diagnostics(state)
+        return dict(zip(nat.TC_MHD_DIAG_KEYS, (float(v) for v in d)))
+
+    def fields_physical(self, state: AxisymmetricMRIState):
+        """The six fields on the standard (Nz, Nr) grid (``:1844-1848``), on the host."""
+        return tuple(self._backward_basis(c, S) for c, S in zip(state.x, self._stencils))
+
+    def energy_parts(self, state: AxisymmetricMRIState):
+        d = self.diagnostics(state)
+        return d["Ekin"], d["Emag"]
+
+    def energy(self, state: AxisymmetricMRIState) -> float:
+        return self.diagnostics(state)["E"]
+
+    def divergence_linf(self, state: AxisymmetricMRIState):
+        d = self.diagnostics(state)
+        return d["divu"], d["divb"]
+
+    def velocity_physical(self, state: AxisymmetricMRIState):
+        return self.fields_physical(state)[:3]
+
+    def inner_torque(self, state):
+        raise NotImplementedError("the inner-cylinder torque is a hydrodynamic runner scalar")

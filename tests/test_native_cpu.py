@@ -382,6 +382,22 @@ def test_taylor_couette_exports_and_pencil_match_oracle():
         Lo, Mo = TCLinearOracle(CircularCouette(1.0, 2.5, 1.0, -0.2), nu=0.02, N=24).assemble(m, kz)
         assert np.max(np.abs(L - Lo)) <= 1e-13 * np.max(np.abs(Lo))
         assert np.max(np.abs(M - Mo)) <= 1e-13 * np.max(np.abs(Mo))
+    # the conducting MRI pencil (tc_mri_pencil) against the oracle's assembly
+    from oracle.tc import TCMRILinearOracle
+
+    cfg.b0, cfg.eta_mag = 0.1, 0.015
+    n7 = 7 * 22
+    for m, kz in ((0, 6.0), (1, 2.5)):
+        L = np.empty((n7, n7), dtype=complex)
+        M = np.empty((n7, n7), dtype=complex)
+        nat.tc_check(lib.tc_mri_pencil(C.byref(cfg), m, kz, L.view(np.float64).ctypes.data_as(dp),
+                                       M.view(np.float64).ctypes.data_as(dp)))
+        Lo, Mo = TCMRILinearOracle(CircularCouette(1.0, 2.5, 1.0, -0.2), B0=0.1, nu=0.02, eta_mag=0.015,
+                                   N=24).assemble(m, kz)
+        assert np.max(np.abs(L - Lo)) <= 1e-13 * np.max(np.abs(Lo))
+        assert np.max(np.abs(M - Mo)) <= 1e-13 * np.max(np.abs(Mo))
+    L = np.empty((4 * 22, 4 * 22), dtype=complex)
+    M = np.empty((4 * 22, 4 * 22), dtype=complex)
     cfg.r2 = 0.5
     with pytest.raises(ValueError, match="R1 < R2"):
         nat.tc_check(lib.tc_linear_pencil(C.byref(cfg), 0, 1.0, L.view(np.float64).ctypes.data_as(dp),
