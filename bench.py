@@ -573,6 +573,10 @@ TC_CASES = (
     # constructor defaults of TaylorCouetteDNSJax (Nr=24, Ntheta=8, Nz=16)
     dict(name="tc_hydro_3d 12x4x4", Nr=12, Ntheta=4, Nz=4, dealias=1.5),
     dict(name="3D defaults 24x8x16", Nr=24, Ntheta=8, Nz=16, dealias=1.5),
+    # conducting-wall MHD: production/examples/taylor_couette_mhd_dns_v1.json
+    # (Re = Rm = 1000, B0 = 0.1, Keplerian Omega2, Lz = pi / 3) with SBDF3
+    dict(name="tc_mhd 40x8", Nr=40, Nz=8, dealias=1.0, mhd=True),
+    dict(name="MHD 48x32 dealiased", Nr=48, Nz=32, dealias=1.5, mhd=True),
 )
 
 
@@ -583,9 +587,9 @@ def tc_leg(steps: int = 2400, cpu_steps: int = 60) -> list:
     Lz = 2 pi / 3.13, dt = 1e-3, amp 1e-6 leading eigenmode: m = 0 for the
     axisymmetric class, m = 1 for the 3D one), and the growth rate against
     the linear one."""
-    from jaxfun_amd.tc import AxisymmetricTCDNSHIP, CircularCouette, TaylorCouetteDNSHIP
+    from jaxfun_amd.tc import AxisymmetricMRIDNSHIP, AxisymmetricTCDNSHIP, CircularCouette, TaylorCouetteDNSHIP
     from oracle.tc import CircularCouette as OBase
-    from oracle.tc import TC3DOracle, TCOracle, TCState
+    from oracle.tc import TC3DOracle, TCMRIOracle, TCOracle, TCState
 
     out = []
     lz = 2.0 * math.pi / 3.13
@@ -593,7 +597,13 @@ def tc_leg(steps: int = 2400, cpu_steps: int = 60) -> list:
         kw = dict(nu=0.01, Nr=case["Nr"], Nz=case["Nz"], Lz=lz, dt=1e-3, dealias=case["dealias"],
                   time_integrator="SBDF3")
         nth = case.get("Ntheta", 1)
-        if nth > 1:
+        mhd = case.get("mhd", False)
+        om2 = 0.5**1.5 if mhd else 0.0
+        if mhd:
+            kw.update(nu=1e-3, eta_mag=1e-3, B0=0.1, Lz=math.pi / 3.0, dt=2e-3)
+            solver = AxisymmetricMRIDNSHIP(CircularCouette(1.0, 2.0, 1.0, om2), **kw)
+            s0, eig = solver.seed_linear_eigenmode(kz_mode=1, amp=1e-7)
+        elif nth > 1:
             kw["Ntheta"] = nth
             solver = TaylorCouetteDNSHIP(CircularCouette(1.0, 2.0, 1.0, 0.0), **kw)
             s0, eig = solver.seed_linear_eigenmode(m=1, kz_mode=1, amp=1e-6)
@@ -609,17 +619,22 @@ def tc_leg(steps: int = 2400, cpu_steps: int = 60) -> list:
         fin = solver.fetch(s)
         e0, e1 = solver.energy(s), solver.energy(fin)
         growth = 0.5 * math.log(e1 / e0) / ((steps + 24) * solver.dt)
-        oracle = (TC3DOracle if nth > 1 else TCOracle)(OBase(1.0, 2.0, 1.0, 0.0), **kw)
-        os_ = TCState(u=s.u, p=s.p, nonlinear_old=s.nonlinear_old, have_old=s.have_old,
+        oracle = (TCMRIOracle if mhd else TC3DOracle if nth > 1 else TCOracle)(OBase(1.0, 2.0, 1.0, om2), **kw)
+        os_ = TCState(u=s.x if mhd else s.u, p=s.p, nonlinear_old=s.nonlinear_old, have_old=s.have_old,
                       nonlinear_older=s.nonlinear_older, solution_old=s.solution_old,
                       solution_older=s.solution_older, history_steps=s.history_steps)
         oracle.solve(os_, 2)
         t1 = time.perf_counter()
         oracle.solve(os_, cpu_steps)
         cpu = cpu_steps / (time.perf_counter() - t1)
-        kind = f"3D Taylor-Couette SBDF3, {case['name']} (Nr={case['Nr']}, Ntheta={nth}, " if nth > 1 else \
-            f"axisymmetric Taylor-Couette SBDF3, {case['name']} (Nr={case['Nr']}, "
-        out.append({"workload": kind + f"Nz={case['Nz']}, dealias={case['dealias']}), Re=100, dt=1e-3, FP64",
+        if mhd:
+            kind = f"axisymmetric conducting-wall Taylor-Couette MHD SBDF3, {case['name']} (Nr={case['Nr']}, "
+        elif nth > 1:
+            kind = f"3D Taylor-Couette SBDF3, {case['name']} (Nr={case['Nr']}, Ntheta={nth}, "
+        else:
+            kind = f"axisymmetric Taylor-Couette SBDF3, {case['name']} (Nr={case['Nr']}, "
+        phys = "Re=Rm=1000, B0=0.1, Keplerian, dt=2e-3" if mhd else "Re=100, dt=1e-3"
+        out.append({"workload": kind + f"Nz={case['Nz']}, dealias={case['dealias']}), {phys}, FP64",
                     "steps_per_s": steps / el, "ms_per_step": 1e3 * el / steps, "steps": steps,
                     "growth_rate": growth, "growth_rate_linear": eig.real,
                     "cpu_oracle_steps_per_s": cpu, "cpu_sample": f"{cpu_steps} steady steps, numpy/scipy "
