@@ -1105,3 +1105,209 @@ class TCMRIOracle(TCOracle):
         return {"Ekin": ek, "Emag": em, "E": ek + em,
                 "divu": float(np.max(np.abs(self._div(s, 0, 2)))),
                 "divb": float(np.max(np.abs(self._div(s, 3, 5)))), "wall_u": wall}
+
+
+class TC3DMRIOracle(TCMRIOracle):
+    """``TaylorCouetteMRIDNSJax`` restated (``examples/taylor_couette_dns_jax.py:1906-2245``):
+    the conducting-wall MHD step on Fourier(theta) x Fourier(z) x Chebyshev(r),
+    one 7-field block per (m, kz) with the base-flow advection -Omega d/dtheta
+    of every field and the 2/r^2 azimuthal couplings of u and of b
+    (``:2005-2059``); forms times 2 pi Lz.  Arrays are (Ntheta, Nz, Nr - 2)."""
+
+    _synth = TC3DOracle._synth
+    _cheb_last = TC3DOracle._cheb_last
+    _dealias_to_standard = TC3DOracle._dealias_to_standard
+    mask = TC3DOracle.mask
+    mesh = TC3DOracle.mesh
+    _integrate = TC3DOracle._integrate
+
+    def __init__(self, base: CircularCouette, B0=0.1, nu=1.0e-3, eta_mag=1.0e-3, Nr=24, Ntheta=8, Nz=16, Lz=None,
+                 dt=2.0e-3, dealias=1.5, time_integrator="CNAB2"):
+        self.Ntheta = int(Ntheta)
+        self.mint = fourier_wavenumbers(self.Ntheta)
+        self.m_odd = fourier_wavenumbers(self.Ntheta, eliminate_highest=True)
+        self.Mth = int(self.Ntheta * dealias) if dealias > 1.0 else self.Ntheta
+        super().__init__(base, B0=B0, nu=nu, eta_mag=eta_mag, Nr=Nr, Nz=Nz, Lz=Lz, dt=dt, dealias=dealias,
+                         time_integrator=time_integrator)
+
+    def _operator_part3(self, m, kz, sign):
+        """``_add_mhd_terms`` of the 3D class, times sign (6n x 6n)."""
+        rf, b, n = self.rf, self.base, self.Nd
+        B = self.bases
+        S = lambda i: slice(i * n, (i + 1) * n)  # noqa: E731
+        X = self._operator_part(kz, 1.0).astype(complex)
+        for f in range(6):  # -Omega d/dtheta and the m^2 / r^2 part of the Laplacian
+            nu_f = self.nu if f < 3 else self.eta
+            X[S(f), S(f)] += (-1j * m * rf.form(B[f], B[f], 0, lambda r: b.a + b.b / r**2)
+                              - nu_f * m * m * rf.form(B[f], B[f], 0, lambda r: 1.0 / r**2))
+        X[S(0), S(1)] += -self.nu * 2j * m * rf.form(B[0], B[1], 0, lambda r: 1.0 / r**2)
+        X[S(1), S(0)] += self.nu * 2j * m * rf.form(B[1], B[0], 0, lambda r: 1.0 / r**2)
+        X[S(3), S(4)] += -self.eta * 2j * m * rf.form(B[3], B[4], 0, lambda r: 1.0 / r**2)
+        X[S(4), S(3)] += self.eta * 2j * m * rf.form(B[4], B[3], 0, lambda r: 1.0 / r**2)
+        return sign * X
+
+    def _build_operators(self, ls):
+        n, rf = self.Nd, self.rf
+        SD, SP = rf.SD, rf.SP
+        M6 = self._mass6()
+        nm = self.Ntheta * self.Nz
+        limp = np.zeros((nm, 7 * n, 7 * n), dtype=complex)
+        lexp = np.zeros((nm, 6 * n, 6 * n), dtype=complex)
+        mass = np.zeros((nm, 6 * n, 6 * n), dtype=complex)
+        perm = np.r_[0:3 * n, 4 * n:7 * n]
+        P = slice(3 * n, 4 * n)
+        for it in range(self.Ntheta):
+            for iz in range(self.Nz):
+                q = it * self.Nz + iz
+                m, kz = self.mint[it], self.kz[iz]
+                A = limp[q]
+                A[np.ix_(perm, perm)] = M6 + self._operator_part3(m, kz, -ls)
+                A[0:n, P] += rf.form(SD, SP, 1)
+                A[n:2 * n, P] += 1j * m * rf.form(SD, SP, 0, lambda r: 1.0 / r)
+                A[2 * n:3 * n, P] += 1j * kz * rf.form(SD, SP, 0)
+                A[P, 0:n] += rf.form(SP, SD, 1) + rf.form(SP, SD, 0, lambda r: 1.0 / r)
+                A[P, n:2 * n] += 1j * m * rf.form(SP, SD, 0, lambda r: 1.0 / r)
+                A[P, 2 * n:3 * n] += 1j * kz * rf.form(SP, SD, 0)
+                lexp[q] = M6 + self._operator_part3(m, kz, ls)
+                mass[q] = M6
+        fz = 2.0 * math.pi * self.Lz
+        return fz * limp, fz * lexp, fz * mass
+
+    # --------------------------------------------------------------- transforms
+    def _phys_f(self, c, f):
+        orth = self._orth(c, f)
+        d = lambda x: self._synth(x, self.Mth, self.Mz, self.Mr)  # noqa: E731
+        dr = np.moveaxis(cheb_derivative(np.moveaxis(orth, -1, 0), 1), 0, -1)
+        return (d(orth), self.rf.df * d(dr), d(1j * self.m_odd[:, None, None] * orth),
+                d(1j * self.kz_odd[None, :, None] * orth))
+
+    def scalar_product_basis(self, v, S):
+        c = 2.0 * math.pi * self.Lz * fourier_forward(fourier_forward(v, 0, self.Ntheta), 1, self.Nz)
+        s = np.moveaxis(cheb_scalar_product(np.moveaxis(c, -1, 0), self.Nr), 0, -1) / self.rf.df
+        return self.mask(s @ S.T)
+
+    def _coef(self, v):
+        c = fourier_forward(fourier_forward(v, 0, self.Ntheta), 1, self.Nz)
+        return np.moveaxis(cheb_forward(np.moveaxis(c, -1, 0), self.Nr), 0, -1)
+
+    def _std(self, chat, which):
+        syn = lambda x: self._synth(x, self.Ntheta, self.Nz, self.Nr)  # noqa: E731
+        if which == "t":
+            return syn(1j * self.m_odd[:, None, None] * chat)
+        if which == "z":
+            return syn(1j * self.kz_odd[None, :, None] * chat)
+        return self.rf.df * syn(np.moveaxis(cheb_derivative(np.moveaxis(chat, -1, 0), 1), 0, -1))
+
+    def nonlinear(self, x):
+        """``:2141-2190``."""
+        F = [self._phys_f(c, f) for f, c in enumerate(x)]
+        (ur, urr, urt, urz), (ut, utr, utt, utz), (uz, uzr, uzt, uzz) = F[:3]
+        (br, brr, brt, brz), (bt, btr, btt, btz), (bz, bzr, bzt, bzz) = F[3:]
+        invr = (1.0 / self.r_pad)[None, None, :]
+        au_r = ur * urr + (ut * invr) * urt + uz * urz - ut * ut * invr
+        au_t = ur * utr + (ut * invr) * utt + uz * utz + ur * ut * invr
+        au_z = ur * uzr + (ut * invr) * uzt + uz * uzz
+        lb_r = br * brr + (bt * invr) * brt + bz * brz - bt * bt * invr
+        lb_t = br * btr + (bt * invr) * btt + bz * btz + br * bt * invr
+        lb_z = br * bzr + (bt * invr) * bzt + bz * bzz
+        std = self._dealias_to_standard
+        nu_r, nu_t, nu_z = std(au_r - lb_r), std(au_t - lb_t), std(au_z - lb_z)
+        eps_r, eps_t, eps_z = std(ut * bz - uz * bt), std(uz * br - ur * bz), std(ur * bt - ut * br)
+        er, et, ez = self._coef(eps_r), self._coef(eps_t), self._coef(eps_z)
+        inv_s = 1.0 / self.rf.r[None, None, :]
+        nb_r = -inv_s * self._std(ez, "t") + self._std(et, "z")
+        nb_t = -self._std(er, "z") + self._std(ez, "r")
+        nb_z = -self._std(et, "r") - eps_t * inv_s + inv_s * self._std(er, "t")
+        SD = self.rf.SD
+        return (self.scalar_product_basis(nu_r, SD), self.scalar_product_basis(nu_t, SD),
+                self.scalar_product_basis(nu_z, SD), self.scalar_product_basis(nb_r, SD),
+                self.scalar_product_basis(nb_t, self.SBT), self.scalar_product_basis(nb_z, self.SBZ))
+
+    def _modes(self, u):
+        nm = self.Ntheta * self.Nz
+        return np.concatenate([c.reshape(nm, self.Nd) for c in u], axis=1)
+
+    def _unmodes(self, x):
+        n = self.Nd
+        return tuple(x[:, i * n:(i + 1) * n].reshape(self.Ntheta, self.Nz, n) for i in range(6))
+
+    def solve_rhs(self, rhs_x, lu):
+        n, nm = self.Nd, self.Ntheta * self.Nz
+        r6 = self._modes(rhs_x)
+        rhs = np.concatenate([r6[:, :3 * n], np.zeros((nm, n), dtype=complex), r6[:, 3 * n:]], axis=1)
+        rhs[0, 3 * n] = 0.0
+        sol = np.stack([sla.lu_solve(lu[q], rhs[q]) for q in range(nm)])
+        x = np.concatenate([sol[:, :3 * n], sol[:, 4 * n:]], axis=1)
+        return self._unmodes(x), sol[:, 3 * n:4 * n].reshape(self.Ntheta, self.Nz, n)
+
+    def _pin(self, limp):
+        limp = limp.copy()
+        row = 3 * self.Nd
+        limp[0, row, :] = 0.0
+        limp[0, row, row] = 1.0
+        return limp
+
+    # ------------------------------------------------------------------ states
+    def zero_state(self) -> TCState:
+        sh = (self.Ntheta, self.Nz, self.Nd)
+        z = tuple(np.zeros(sh, dtype=complex) for _ in range(6))
+        return TCState(u=z, p=np.zeros(sh, dtype=complex), nonlinear_old=tuple(np.zeros_like(a) for a in z))
+
+    def forward_basis(self, v, S):
+        c = fourier_forward(fourier_forward(np.asarray(v, dtype=complex), 0, self.Ntheta), 1, self.Nz)
+        s = np.moveaxis(cheb_scalar_product(np.moveaxis(c, -1, 0), self.Nr), 0, -1) @ S.T
+        M = S @ np.diag(np.where(np.arange(self.Nr) == 0, math.pi, math.pi / 2)) @ S.T
+        return np.linalg.solve(M, s.reshape(-1, self.Nd).T).T.reshape(s.shape)
+
+    def seed_linear_eigenmode(self, m=1, kz_mode=1, amp=1.0e-6, which=0):
+        """``:2192-2230``: no pivot phase for m != 0."""
+        kz = 2.0 * math.pi * int(kz_mode) / self.Lz
+        lin = TCMRILinearOracle(self.base, B0=self.B0, nu=self.nu, eta_mag=self.eta, N=self.Nr)
+        w, vecs = lin.eigs(int(m), kz, n_return=which + 1)
+        vec = vecs[:, which]
+        if int(m) == 0:
+            vec = positive_pivot_phase(vec)
+        n = lin.n
+        mpos, mneg = int(m) % self.Ntheta, (-int(m)) % self.Ntheta
+        kpos, kneg = int(kz_mode) % self.Nz, (-int(kz_mode)) % self.Nz
+        comps = []
+        for blk in (0, 1, 2, 4, 5, 6):
+            block = vec[blk * n:(blk + 1) * n] * amp
+            arr = np.zeros((self.Ntheta, self.Nz, self.Nd), dtype=complex)
+            if mpos == mneg and kpos == kneg:
+                arr[mpos, kpos, :n] = np.real(block)
+            else:
+                arr[mpos, kpos, :n] = 0.5 * block
+                arr[mneg, kneg, :n] = 0.5 * np.conj(block)
+            comps.append(arr)
+        st = self.zero_state()
+        return replace(st, u=tuple(comps), nonlinear_old=tuple(np.zeros_like(a) for a in comps)), complex(w[which])
+
+    # ------------------------------------------------------------- diagnostics
+    def backward_f(self, c, f):
+        return self._synth(self._orth(c, f), self.Ntheta, self.Nz, self.Nr)
+
+    def energy_parts(self, s):
+        r = self.rf.r[None, None, :]
+        F = self.fields_physical(s)
+        ek = self._integrate(0.5 * sum(np.real(np.conj(v) * v) for v in F[:3]) * r)
+        em = self._integrate(0.5 * sum(np.real(np.conj(v) * v) for v in F[3:]) * r)
+        return ek, em
+
+    def _div(self, s, fr, ft, fz):
+        """``:2232-2245``: d/dr + 1/r of the radial, (1/r) d/dtheta of the
+        azimuthal and d/dz of the axial component."""
+        syn = lambda x: self._synth(x, self.Ntheta, self.Nz, self.Nr)  # noqa: E731
+        orth_r = self._orth(s.u[fr], fr)
+        d_r = self.rf.df * syn(np.moveaxis(cheb_derivative(np.moveaxis(orth_r, -1, 0), 1), 0, -1))
+        r = self.rf.r[None, None, :]
+        return (d_r + syn(orth_r) / r + syn(1j * self.m_odd[:, None, None] * self._orth(s.u[ft], ft)) / r
+                + syn(1j * self.kz_odd[None, :, None] * self._orth(s.u[fz], fz)))
+
+    def diagnostics(self, s) -> dict:
+        ek, em = self.energy_parts(s)
+        ends = npcheb.chebvander(np.array([-1.0, 1.0]), self.Nr - 1) @ self.rf.SD.T
+        wall = max(float(np.max(np.abs(fourier_backward(fourier_backward(c @ ends.T, 0, self.Ntheta), 1, self.Nz))))
+                   for c in s.u[:3])
+        return {"Ekin": ek, "Emag": em, "E": ek + em, "divu": float(np.max(np.abs(self._div(s, 0, 1, 2)))),
+                "divb": float(np.max(np.abs(self._div(s, 3, 4, 5)))), "wall_u": wall}

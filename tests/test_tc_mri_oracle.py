@@ -168,3 +168,148 @@ def test_zero_state_stays_zero():
                         time_integrator=ti)
         z = o.solve(o.zero_state(), 3)
         assert all(np.max(np.abs(c)) == 0.0 for c in z.u) and np.max(np.abs(z.p)) == 0.0
+
+
+# ---------------------------------------------------------------------------
+# the 3D class (TaylorCouetteMRIDNSJax)
+# ---------------------------------------------------------------------------
+
+
+def test_golden_3d_mri_sbdf3_run():
+    """production/goldens/taylor_couette_mhd_3d_v1 (written by the reference's
+    3D class itself: Nr = 12, Ntheta = 4, Nz = 4, dealias 3/2, SBDF3,
+    dt = 2e-3, three steps from the amp = 1e-7 m = 1 MRI eigenmode).  The
+    divergences are maxima over the 4 x 4 (theta, z) grid of a travelling
+    wave whose phase is the eigensolver's arbitrary eigenvector phase (no
+    pivot phase for m != 0), so they agree in size only."""
+    from oracle.tc import TC3DMRIOracle
+
+    o = TC3DMRIOracle(CircularCouette(1.0, 2.0, 1.0, KEPLER), B0=0.1, nu=1e-3, eta_mag=1e-3, Nr=12, Ntheta=4, Nz=4,
+                      Lz=GOLDEN_LZ, dt=2e-3, dealias=1.5, time_integrator="SBDF3")
+    s0, eig = o.seed_linear_eigenmode(m=1, kz_mode=1, amp=1e-7)
+    assert abs(eig.real - 0.2912937448361517) <= 1e-10
+    ek0, em0 = o.energy_parts(s0)
+    assert abs(ek0 / 7.85609816898076e-15 - 1.0) <= 1e-9 and abs(em0 / 1.1298011285580755e-14 - 1.0) <= 1e-9
+    s1 = o.solve(s0, 3)
+    ek1, em1 = o.energy_parts(s1)
+    assert abs(ek1 / 7.883607407650011e-15 - 1.0) <= 1e-9 and abs(em1 / 1.1337572869748595e-14 - 1.0) <= 1e-9
+    growth = 0.5 * math.log((ek1 + em1) / (ek0 + em0)) / 6e-3
+    assert abs(growth - 0.29129374483210074) <= 1e-7
+    d = o.diagnostics(s1)
+    assert abs(d["divu"] / 1.486388530128091e-08 - 1.0) < 0.05
+    assert abs(d["divb"] / 1.0541502761126154e-08 - 1.0) < 0.05
+
+
+def _direct_rows_3d(o, x, drop_azimuthal=False):
+    """Both row groups of the 3D class by explicit sums (no FFT or DCT)."""
+    Mth, Mz, Mr, Nth, Nz, Nr = o.Mth, o.Mz, o.Mr, o.Ntheta, o.Nz, o.Nr
+    df = o.rf.df
+
+    def nodes(M):
+        return np.cos(np.pi + (2 * np.arange(M) + 1) * np.pi / (2 * M))
+
+    xp, xs = nodes(Mr), nodes(Nr)
+    rp = o.base.R1 + (xp + 1.0) / df
+    rs = o.base.R1 + (xs + 1.0) / df
+    mint = np.fft.fftfreq(Nth, 1.0 / Nth)
+    kz = 2 * np.pi * np.fft.fftfreq(Nz, 1.0 / Nz) / o.Lz
+    m_odd, kz_odd = mint.copy(), kz.copy()
+    m_odd[Nth // 2] = 0.0
+    kz_odd[Nz // 2] = 0.0
+    Etp = np.exp(1j * np.outer(np.arange(Mth) * 2 * np.pi / Mth, mint))
+    Ezp = np.exp(1j * np.outer(np.arange(Mz) * o.Lz / Mz, kz))
+    Ets = np.exp(1j * np.outer(np.arange(Nth) * 2 * np.pi / Nth, mint))
+    Ezs = np.exp(1j * np.outer(np.arange(Nz) * o.Lz / Nz, kz))
+    fields = []
+    for f, c in enumerate(x):
+        S = o.bases[f]
+        val = np.stack([npcheb.chebval(xp, s) for s in S], axis=1)
+        der = df * np.stack([npcheb.chebval(xp, npcheb.chebder(s)) for s in S], axis=1)
+        syn = lambda cc, R: np.einsum("am,bk,mki,ri->abr", Etp, Ezp, cc, R)  # noqa: E731
+        fields.append((syn(c, val), syn(c, der), syn(1j * m_odd[:, None, None] * c, val),
+                       syn(1j * kz_odd[None, :, None] * c, val)))
+    (ur, urr, urt, urz), (ut, utr, utt, utz), (uz, uzr, uzt, uzz) = fields[:3]
+    (br, brr, brt, brz), (bt, btr, btt, btz), (bz, bzr, bzt, bzz) = fields[3:]
+    inv = 1.0 / rp[None, None, :]
+    au = 0.0 if drop_azimuthal else ut * inv
+    ab = 0.0 if drop_azimuthal else bt * inv
+    mom = (ur * urr + au * urt + uz * urz - ut * ut * inv - (br * brr + ab * brt + bz * brz - bt * bt * inv),
+           ur * utr + au * utt + uz * utz + ur * ut * inv - (br * btr + ab * btt + bz * btz + br * bt * inv),
+           ur * uzr + au * uzt + uz * uzz - (br * bzr + ab * bzt + bz * bzz))
+    emf = (ut * bz - uz * bt, uz * br - ur * bz, ur * bt - ut * br)
+    Tp = npcheb.chebvander(xp, Nr - 1)
+    Ts = npcheb.chebvander(xs, Nr - 1)
+
+    def coeffs(f):
+        c = np.einsum("am,bk,abr->mkr", np.conj(Etp), np.conj(Ezp), f) / (Mth * Mz)
+        c = (2.0 / Mr) * c @ Tp
+        c[..., 0] *= 0.5
+        return c
+
+    def std(c, kind=None):
+        cc = c.copy()
+        if kind == "r":
+            cc = df * np.apply_along_axis(lambda row: np.append(npcheb.chebder(row), 0.0), -1, cc)
+        elif kind == "t":
+            cc = 1j * m_odd[:, None, None] * cc
+        elif kind == "z":
+            cc = 1j * kz_odd[None, :, None] * cc
+        return np.einsum("am,bk,mkn,qn->abq", Ets, Ezs, cc, Ts)
+
+    def proj(vals, S):
+        psi = np.stack([npcheb.chebval(xs, s) for s in S], axis=1)
+        fk = 2 * np.pi * o.Lz * np.einsum("am,bk,abr->mkr", np.conj(Ets), np.conj(Ezs), vals) / (Nth * Nz)
+        rows = (np.pi / Nr) / df * fk @ psi
+        rows[Nth // 2] = 0.0
+        rows[:, Nz // 2] = 0.0
+        return rows
+
+    SD = o.rf.SD
+    out = [proj(std(coeffs(f)), SD) for f in mom]
+    er, et, ez = (coeffs(f) for f in emf)
+    inv_s = 1.0 / rs[None, None, :]
+    nb_r = -inv_s * std(ez, "t") + std(et, "z")
+    nb_t = -std(er, "z") + std(ez, "r")
+    nb_z = -std(et, "r") - std(et) * inv_s + inv_s * std(er, "t")
+    out += [proj(nb_r, SD), proj(nb_t, o.SBT), proj(nb_z, o.SBZ)]
+    return out
+
+
+def test_3d_nonlinear_rows_match_direct_series():
+    from oracle.tc import TC3DMRIOracle
+
+    o = TC3DMRIOracle(CircularCouette(1.0, 2.0, 1.0, KEPLER), B0=0.1, nu=1e-3, eta_mag=1e-3, Nr=10, Ntheta=6, Nz=4,
+                      dt=1e-3, dealias=1.5)
+    rng = np.random.default_rng(17)
+    x = []
+    for _ in range(6):
+        c = np.zeros((o.Ntheta, o.Nz, o.Nd), dtype=complex)
+        c[0, 0, :4] = rng.standard_normal(4)
+        for m, k in ((1, 0), (0, 1), (1, 1), (-2, 1)):
+            blk = rng.standard_normal(4) + 1j * rng.standard_normal(4)
+            c[m % o.Ntheta, k % o.Nz, :4] = blk
+            c[-m % o.Ntheta, -k % o.Nz, :4] = np.conj(blk)
+        x.append(c)
+    got = o.nonlinear(tuple(x))
+    ref = _direct_rows_3d(o, x)
+    scale = max(np.max(np.abs(r)) for r in ref)
+    for g, r in zip(got, ref):
+        assert np.max(np.abs(g - r)) <= 1e-12 * scale
+    bad = _direct_rows_3d(o, x, drop_azimuthal=True)
+    assert max(np.max(np.abs(g - b)) for g, b in zip(got, bad)) > 1e-3 * scale
+
+
+def test_3d_axisymmetric_state_steps_as_the_axisymmetric_oracle():
+    from oracle.tc import TC3DMRIOracle
+
+    base = CircularCouette(1.0, 2.0, 1.0, KEPLER)
+    kw = dict(B0=0.1, nu=2e-3, eta_mag=3e-3, Nr=12, Nz=8, dt=2e-3, dealias=1.5, time_integrator="SBDF3")
+    o2, o3 = TCMRIOracle(base, **kw), TC3DMRIOracle(base, Ntheta=4, **kw)
+    s2, _ = o2.seed_linear_eigenmode(kz_mode=1, amp=3e-2)
+    s3 = o3.zero_state()
+    s3 = type(s3)(u=tuple(np.stack([c] + [np.zeros_like(c)] * 3) for c in s2.u), p=s3.p,
+                  nonlinear_old=s3.nonlinear_old)
+    a, b = o2.solve(s2, 4), o3.solve(s3, 4)
+    for c2, c3 in zip(a.u, b.u):
+        assert np.max(np.abs(c3[0] - c2)) <= 1e-12 * np.max(np.abs(c2))
+        assert np.max(np.abs(c3[1:])) <= 1e-14 * np.max(np.abs(c2))
