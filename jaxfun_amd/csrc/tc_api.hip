@@ -56,7 +56,7 @@ struct tc_handle {
   double *dEv = nullptr, *dEd = nullptr, *dPrT = nullptr, *dinvr = nullptr, *dkz = nullptr, *dMassT = nullptr,
          *dmq = nullptr, *dmq_odd = nullptr, *dDtc = nullptr, *dEs = nullptr, *dEsd = nullptr, *drstd = nullptr, *dfejer = nullptr,
          *dDrc = nullptr, *dDzc = nullptr, *dout = nullptr, *dQbr = nullptr, *dQbt = nullptr, *dQbtd = nullptr,
-         *dQbz = nullptr;
+         *dQbz = nullptr, *dQbrr = nullptr, *dQbzr = nullptr;
   double2 *dtwz = nullptr, *dtwn = nullptr, *dtwt = nullptr, *dtwnt = nullptr, *dAinvT = nullptr, *dAinv0T = nullptr;
   double2 *dLinT = nullptr, *dLexpT = nullptr, *ddv = nullptr;
   double2 *S[4]{}, *N[3]{}, *P = nullptr, *G = nullptr, *XS[3]{}, *NS[3]{};
@@ -381,7 +381,8 @@ int32_t tc_create(const tc_config *cfg, tc_handle **out) {
       (r = h->upload(&h->dtwt, twiddles(o.mth))) || (r = h->upload(&h->dtwnt, twiddles(o.nth))) ||
       (r = h->upload(&h->dmq, o.mq)) || (r = h->upload(&h->dmq_odd, o.mq_odd)) || (r = h->upload(&h->dDtc, o.Dtc)) ||
       (r = h->upload(&h->dQbr, o.QbrT)) || (r = h->upload(&h->dQbt, o.QbtT)) || (r = h->upload(&h->dQbtd, o.QbtdT)) ||
-      (r = h->upload(&h->dQbz, o.QbzT)) || (r = h->alloc(&h->ddv, 2 * (size_t)o.nm * cfg->nr)) ||
+      (r = h->upload(&h->dQbz, o.QbzT)) || (r = h->upload(&h->dQbrr, o.QbrrT)) || (r = h->upload(&h->dQbzr, o.QbzrT)) ||
+      (r = h->alloc(&h->ddv, 2 * (size_t)o.nm * cfg->nr)) ||
       (r = h->alloc(&h->dout, 8))) {
     tc_destroy(h);
     return r;
@@ -410,6 +411,8 @@ int32_t tc_create(const tc_config *cfg, tc_handle **out) {
   g.QbtT = h->dQbt;
   g.QbtdT = h->dQbtd;
   g.QbzT = h->dQbz;
+  g.QbrrT = h->dQbrr;
+  g.QbzrT = h->dQbzr;
   g.n = o.n;
   g.kh = o.kh;
   g.nth = o.nth;

@@ -134,7 +134,7 @@ constexpr int RHS_THREADS = 256;
 // dealiased projection of the products (velocity rows), or with the MHD
 // fields minus the curl of the EMF (G components 3..5 hold its r, theta, z
 // parts): b_r i kz Qbr G_t, b_theta -i kz Qbt G_r + Qbtd G_z, b_z Qbz G_t.
-__device__ __forceinline__ double2 nl_row(const TcGeo &g, const double2 *gs, int c, int i, double kz) {
+__device__ __forceinline__ double2 nl_row(const TcGeo &g, const double2 *gs, int c, int i, double kz, double m) {
   const int n = g.n, mr = g.mr;
   auto dot = [&](const double *T, const double2 *Gc) {
     double sr = 0.0, si = 0.0;
@@ -147,15 +147,27 @@ __device__ __forceinline__ double2 nl_row(const TcGeo &g, const double2 *gs, int
     return make_double2(sr, si);
   };
   if (c < 3) return dot(g.PrT, gs + c * mr);
-  if (c == 3) {
+  if (c == 3) {  // i kz Qbr G_t - i m Qbrr G_z
     const double2 v = dot(g.QbrT, gs + 4 * mr);
-    return make_double2(-kz * v.y, kz * v.x);
+    double2 o = make_double2(-kz * v.y, kz * v.x);
+    if (m != 0.0) {
+      const double2 w = dot(g.QbrrT, gs + 5 * mr);
+      o.x += m * w.y;
+      o.y -= m * w.x;
+    }
+    return o;
   }
   if (c == 4) {
     const double2 v = dot(g.QbtT, gs + 3 * mr), w = dot(g.QbtdT, gs + 5 * mr);
     return make_double2(w.x + kz * v.y, w.y - kz * v.x);
   }
-  return dot(g.QbzT, gs + 4 * mr);
+  double2 o = dot(g.QbzT, gs + 4 * mr);  // Qbz G_t + i m Qbzr G_r
+  if (m != 0.0) {
+    const double2 w = dot(g.QbzrT, gs + 3 * mr);
+    o.x -= m * w.y;
+    o.y += m * w.x;
+  }
+  return o;
 }
 
 // --------------------------------------------------------- k_tc_rhs_solve --
@@ -209,12 +221,12 @@ __global__ void __launch_bounds__(RHS_THREADS) k_tc_rhs_solve(TcGeo g, TcRhs a) 
     rhs[row] = d;
   }
   __syncthreads();
-  const double kz = g.kz[k];
+  const double kz = g.kz[k], mo = g.mq_odd[k];
   for (int row = tid; row < nfn; row += blockDim.x) {
     const int c = row / n, i = row - c * n;
     double2 r = rhs[row];
     if (a.G) {
-      const double2 v = nl_row(g, gs, c, i, kz);
+      const double2 v = nl_row(g, gs, c, i, kz, mo);
       if (a.nout && blockIdx.y == 0) a.nout[((size_t)c * kh + k) * n + i] = v;
       r.x = fma(a.cN, v.x, r.x);
       r.y = fma(a.cN, v.y, r.y);
@@ -392,26 +404,27 @@ __global__ void __launch_bounds__(256) k_tc_diag(TcGeo g, const double2 *__restr
 }
 
 // ----------------------------------------------------------- k_tc_mhd_diag --
-// One block, the MHD set (:1844-1886): Ekin and Emag by Parseval in z (Fejer-I
-// weights in r), the largest |div u| and |div b| on the standard grid
-// (d/dr + 1/r of the radial component plus i kz of the axial one, b_z in its
-// Neumann basis), and the wall trace of u (zero: Dirichlet modes).
+// One block, the MHD set (:1844-1886; 3D :2232-2245): Ekin and Emag by
+// Parseval over the Fourier axes (Fejer-I weights in r), the largest |div u|
+// and |div b| on the standard grid (d/dr + 1/r of the radial component,
+// (1/r) d/dtheta of the azimuthal one and d/dz of the axial one, b_theta and
+// b_z in their wall bases), and the wall trace of u (zero: Dirichlet modes).
 __global__ void __launch_bounds__(256) k_tc_mhd_diag(TcGeo g, const double2 *__restrict__ u,
                                                      double *__restrict__ out) {
   extern __shared__ __align__(16) unsigned char smem[];
-  const int kh = g.kh, n = g.n, nr = g.nr, nz = g.nz;
+  const int kh = g.kh, nm = g.nm, n = g.n, nr = g.nr, nz = g.nz, nth = g.nth;
   const size_t se = (size_t)nr * n;
   double *red = reinterpret_cast<double *>(smem);  // [blockDim]
-  double2 *dv = g.dv;                              // [2][kh][nr]: div u, div b per mode
+  double2 *dv = g.dv;                              // [2][nm][nr]: div u, div b per mode
   const int tid = threadIdx.x;
   double ek = 0.0, em = 0.0;
-  for (int t = tid; t < kh * nr; t += blockDim.x) {
-    const int k = t / nr, r = t - k * nr;
+  for (int t = tid; t < nm * nr; t += blockDim.x) {
+    const int q = t / nr, r = t - q * nr, k = q % kh;
     double2 v[6], dr[2];
     for (int c = 0; c < 6; ++c) {
       const int basis = c < 4 ? 0 : c - 3;
       const double *E = g.Es + basis * se + (size_t)r * n;
-      const double2 *x = u + ((size_t)c * kh + k) * n;
+      const double2 *x = u + ((size_t)c * nm + q) * n;
       double sr = 0.0, si = 0.0;
       for (int i = 0; i < n; ++i) {
         sr = fma(E[i], x[i].x, sr);
@@ -419,17 +432,17 @@ __global__ void __launch_bounds__(256) k_tc_mhd_diag(TcGeo g, const double2 *__r
       }
       v[c] = make_double2(sr, si);
     }
-    for (int h = 0; h < 2; ++h) {  // d/dr of u_r and b_r (Dirichlet)
+    for (int hh = 0; hh < 2; ++hh) {  // d/dr of u_r and b_r (Dirichlet)
       const double *E = g.Esd + (size_t)r * n;
-      const double2 *x = u + ((size_t)(3 * h) * kh + k) * n;
+      const double2 *x = u + ((size_t)(3 * hh) * nm + q) * n;
       double sr = 0.0, si = 0.0;
       for (int i = 0; i < n; ++i) {
         sr = fma(E[i], x[i].x, sr);
         si = fma(E[i], x[i].y, si);
       }
-      dr[h] = make_double2(sr, si);
+      dr[hh] = make_double2(sr, si);
     }
-    const double w = (k == 0 ? 1.0 : 2.0) * g.rstd[r] * g.fejer[r] * g.lz;
+    const double w = (k == 0 ? 1.0 : 2.0) * g.rstd[r] * g.fejer[r] * g.fz;
     double su = 0.0, sb = 0.0;
     for (int c = 0; c < 3; ++c) {
       su += v[c].x * v[c].x + v[c].y * v[c].y;
@@ -437,10 +450,11 @@ __global__ void __launch_bounds__(256) k_tc_mhd_diag(TcGeo g, const double2 *__r
     }
     ek += 0.5 * w * su;
     em += 0.5 * w * sb;
-    const double ir = 1.0 / g.rstd[r], kz = g.kz[k];
-    for (int h = 0; h < 2; ++h) {
-      const double2 a = v[3 * h], z = v[3 * h + 2];
-      dv[((size_t)h * kh + k) * nr + r] = make_double2(dr[h].x + a.x * ir - kz * z.y, dr[h].y + a.y * ir + kz * z.x);
+    const double ir = 1.0 / g.rstd[r], kz = g.kz[q], mo = g.mq_odd[q];
+    for (int hh = 0; hh < 2; ++hh) {
+      const double2 a = v[3 * hh], th = v[3 * hh + 1], z = v[3 * hh + 2];
+      dv[((size_t)hh * nm + q) * nr + r] = make_double2(dr[hh].x + a.x * ir - mo * ir * th.y - kz * z.y,
+                                                        dr[hh].y + a.y * ir + mo * ir * th.x + kz * z.x);
     }
   }
   ek = block_sum(ek, red);
@@ -448,14 +462,18 @@ __global__ void __launch_bounds__(256) k_tc_mhd_diag(TcGeo g, const double2 *__r
   __threadfence_block();
   __syncthreads();
   double dmax[2] = {0.0, 0.0};
-  for (int t = tid; t < 2 * nz * nr; t += blockDim.x) {
-    const int h = t / (nz * nr), rem = t - h * nz * nr, b = rem / nr, r = rem - b * nr;
+  for (int t = tid; t < 2 * nth * nz * nr; t += blockDim.x) {
+    const int hh = t / (nth * nz * nr), rem0 = t - hh * nth * nz * nr;
+    const int a = rem0 / (nz * nr), rem = rem0 - a * nz * nr, b = rem / nr, r = rem - b * nr;
     double s = 0.0;
-    for (int k = 0; k < kh; ++k) {
-      const double2 d = dv[((size_t)h * kh + k) * nr + r], wz = g.twn[(k * b) % nz];
-      s += (k == 0 ? 1.0 : 2.0) * (d.x * wz.x - d.y * wz.y);
+    for (int q = 0; q < nm; ++q) {
+      const int k = q % kh, mi = (int)g.mq[q];
+      const double2 d = dv[((size_t)hh * nm + q) * nr + r];
+      const double2 wt = g.twn_t[((mi * a) % nth + nth) % nth], wz = g.twn[(k * b) % nz];
+      const double2 wv = make_double2(wt.x * wz.x - wt.y * wz.y, wt.x * wz.y + wt.y * wz.x);
+      s += (k == 0 ? 1.0 : 2.0) * (d.x * wv.x - d.y * wv.y);
     }
-    dmax[h] = fmax(dmax[h], fabs(s));
+    dmax[hh] = fmax(dmax[hh], fabs(s));
   }
   const double du = block_max(dmax[0], red), db = block_max(dmax[1], red);
   if (tid == 0) {
@@ -478,31 +496,36 @@ __global__ void __launch_bounds__(256) k_tc_mhd_diag(TcGeo g, const double2 *__r
 //   n_t = u_r du_t/dr + (u_t/r) du_t/dtheta + u_z du_t/dz + u_r u_t / r
 //   n_z = u_r du_z/dr + (u_t/r) du_z/dtheta + u_z du_z/dz,
 // then the axial and azimuthal analyses truncated to the stored modes, times
-// (2 pi / mth)(Lz / mz); the azimuthal Nyquist rows are masked.  Both twiddle
-// tables sit in LDS; the axial synthesis runs k-outer so that one twiddle
-// serves all twelve fields of a point.
+// (2 pi / mth)(Lz / mz); the azimuthal Nyquist rows are masked.  With the MHD
+// fields (NF = 6, :2141-2190) the products are the advection minus the
+// Lorentz force and the EMF u x b, as in the axisymmetric kernel.  Both
+// twiddle tables sit in LDS; the axial synthesis runs k-outer so that one
+// twiddle serves all fields of a point.
 constexpr int TC3_THREADS = 512;
 
 __device__ __forceinline__ int fft_wavenumber(int im, int nth) { return im < (nth + 1) / 2 ? im : im - nth; }
 
+template <int NF>
 __global__ void __launch_bounds__(TC3_THREADS) k_tc3_nonlinear(TcGeo g, const double2 *__restrict__ u,
                                                                double2 *__restrict__ G) {
   extern __shared__ __align__(16) unsigned char smem[];
-  const int j = blockIdx.x, kh = g.kh, nm = g.nm, n = g.n, mz = g.mz, mth = g.mth, nth = g.nth;
+  constexpr int NV = 4 * NF, NG = NF == 6 ? 6 : 3;
+  const int j = blockIdx.x, kh = g.kh, nm = g.nm, n = g.n, mz = g.mz, mth = g.mth, nth = g.nth, mr = g.mr;
   double2 *tz = reinterpret_cast<double2 *>(smem);  // [mz]
   double2 *tt = tz + mz;                             // [mth]
   double *base = reinterpret_cast<double *>(tt + mth);
-  double2 *val = reinterpret_cast<double2 *>(base);  // [12][nm]; later prod [3][mth mz] and h [3][mth][kh]
-  const size_t region = max((size_t)12 * nm * 2, (size_t)3 * mth * mz + (size_t)3 * mth * kh * 2);
-  double2 *gs = reinterpret_cast<double2 *>(base + region);  // [12][kh][mth]
+  double2 *val = reinterpret_cast<double2 *>(base);  // [NV][nm]; later prod [NG][mth mz] and h [NG][mth][kh]
+  const size_t region = max((size_t)NV * nm * 2, (size_t)NG * mth * mz + (size_t)NG * mth * kh * 2);
+  double2 *gs = reinterpret_cast<double2 *>(base + region);  // [NV][kh][mth]
   double *prod = base;
-  double2 *h = reinterpret_cast<double2 *>(prod + 3 * mth * mz);
+  double2 *h = reinterpret_cast<double2 *>(prod + NG * mth * mz);
   const int tid = threadIdx.x;
   for (int t = tid; t < mz; t += blockDim.x) tz[t] = g.twz[t];
   for (int t = tid; t < mth; t += blockDim.x) tt[t] = g.twt[t];
-  for (int d = tid; d < 6 * nm; d += blockDim.x) {
+  for (int d = tid; d < 2 * NF * nm; d += blockDim.x) {
     const int c = d / (2 * nm), rem = d - c * 2 * nm, typ = rem / nm, q = rem - typ * nm;
-    const double *E = (typ == 0 ? g.Ev : g.Ed) + (size_t)j * n;
+    const int basis = c < 4 ? 0 : c - 3;  // b_theta: Robin, b_z: Neumann
+    const double *E = (typ == 0 ? g.Ev : g.Ed) + (size_t)basis * mr * n + (size_t)j * n;
     const double2 *x = u + ((size_t)c * nm + q) * n;
     double sr = 0.0, si = 0.0;
     for (int i = 0; i < n; ++i) {
@@ -517,7 +540,7 @@ __global__ void __launch_bounds__(TC3_THREADS) k_tc3_nonlinear(TcGeo g, const do
     }
   }
   __syncthreads();
-  for (int t = tid; t < 12 * kh * mth; t += blockDim.x) {
+  for (int t = tid; t < NV * kh * mth; t += blockDim.x) {
     const int f = t / (kh * mth), rem = t - f * kh * mth, k = rem / mth, a = rem - k * mth;
     double sr = 0.0, si = 0.0;
     for (int im = 0; im < nth; ++im) {
@@ -532,32 +555,47 @@ __global__ void __launch_bounds__(TC3_THREADS) k_tc3_nonlinear(TcGeo g, const do
   const double ir = g.invr[j];
   for (int p = tid; p < mth * mz; p += blockDim.x) {
     const int a = p / mz, b = p - a * mz;
-    double v[12];
+    double v[NV];
 #pragma unroll
-    for (int f = 0; f < 12; ++f) v[f] = 0.0;
+    for (int f = 0; f < NV; ++f) v[f] = 0.0;
     int idx = 0;
     for (int k = 1; k < kh; ++k) {
       idx += b;
       if (idx >= mz) idx -= mz;
       const double2 w = tz[idx];
 #pragma unroll
-      for (int f = 0; f < 12; ++f) {
+      for (int f = 0; f < NV; ++f) {
         const double2 c = gs[((size_t)f * kh + k) * mth + a];
         v[f] += c.x * w.x - c.y * w.y;
       }
     }
 #pragma unroll
-    for (int f = 0; f < 12; ++f) v[f] = gs[(size_t)f * kh * mth + a].x + 2.0 * v[f];
+    for (int f = 0; f < NV; ++f) v[f] = gs[(size_t)f * kh * mth + a].x + 2.0 * v[f];
     const double ur = v[0], urr = v[1], urt = v[2], urz = v[3];
     const double ut = v[4], utr = v[5], utt = v[6], utz = v[7];
     const double uz = v[8], uzr = v[9], uzt = v[10], uzz = v[11];
     const double a_t = ut * ir;
-    prod[p] = ur * urr + a_t * urt + uz * urz - ut * ut * ir;
-    prod[mth * mz + p] = ur * utr + a_t * utt + uz * utz + ur * ut * ir;
-    prod[2 * mth * mz + p] = ur * uzr + a_t * uzt + uz * uzz;
+    double nr = ur * urr + a_t * urt + uz * urz - ut * ut * ir;
+    double nt = ur * utr + a_t * utt + uz * utz + ur * ut * ir;
+    double nz = ur * uzr + a_t * uzt + uz * uzz;
+    if constexpr (NF == 6) {
+      const double br = v[12], brr = v[13], brt = v[14], brz = v[15];
+      const double bt = v[16], btr = v[17], btt = v[18], btz = v[19];
+      const double bz = v[20], bzr = v[21], bzt = v[22], bzz = v[23];
+      const double b_t = bt * ir;
+      nr -= br * brr + b_t * brt + bz * brz - bt * bt * ir;
+      nt -= br * btr + b_t * btt + bz * btz + br * bt * ir;
+      nz -= br * bzr + b_t * bzt + bz * bzz;
+      prod[3 * mth * mz + p] = ut * bz - uz * bt;
+      prod[4 * mth * mz + p] = uz * br - ur * bz;
+      prod[5 * mth * mz + p] = ur * bt - ut * br;
+    }
+    prod[p] = nr;
+    prod[mth * mz + p] = nt;
+    prod[2 * mth * mz + p] = nz;
   }
   __syncthreads();
-  for (int t = tid; t < 3 * mth * kh; t += blockDim.x) {
+  for (int t = tid; t < NG * mth * kh; t += blockDim.x) {
     const int c = t / (mth * kh), rem = t - c * mth * kh, a = rem / kh, k = rem - a * kh;
     const double *pr = prod + (size_t)c * mth * mz + (size_t)a * mz;
     double sr = 0.0, si = 0.0;
@@ -573,7 +611,7 @@ __global__ void __launch_bounds__(TC3_THREADS) k_tc3_nonlinear(TcGeo g, const do
   }
   __syncthreads();
   const double sc = (2.0 * 3.141592653589793 / mth) * (g.lz / mz);
-  for (int t = tid; t < 3 * nm; t += blockDim.x) {
+  for (int t = tid; t < NG * nm; t += blockDim.x) {
     const int c = t / nm, q = t - c * nm, im = q / kh, k = q - im * kh;
     double sr = 0.0, si = 0.0;
     if (!(nth % 2 == 0 && im == nth / 2)) {  // the azimuthal Nyquist rows are masked
@@ -587,7 +625,7 @@ __global__ void __launch_bounds__(TC3_THREADS) k_tc3_nonlinear(TcGeo g, const do
         if (idx >= mth) idx -= mth;
       }
     }
-    G[((size_t)c * nm + q) * g.mr + j] = make_double2(sc * sr, sc * si);
+    G[((size_t)c * nm + q) * mr + j] = make_double2(sc * sr, sc * si);
   }
 }
 
@@ -595,8 +633,9 @@ __global__ void __launch_bounds__(TC3_THREADS) k_tc3_nonlinear(TcGeo g, const do
 
 size_t nonlinear_lds(const TcGeo &g) {
   if (g.nth > 1) {
-    const size_t region = std::max((size_t)12 * g.nm * 2, (size_t)3 * g.mth * g.mz + (size_t)3 * g.mth * g.kh * 2);
-    return ((size_t)g.mz + g.mth) * sizeof(double2) + (region + (size_t)12 * g.kh * g.mth * 2) * sizeof(double);
+    const size_t nv = 4 * (size_t)g.nf, ng = g.nf == 6 ? 6 : 3;
+    const size_t region = std::max(nv * g.nm * 2, ng * g.mth * g.mz + ng * g.mth * g.kh * 2);
+    return ((size_t)g.mz + g.mth) * sizeof(double2) + (region + nv * g.kh * g.mth * 2) * sizeof(double);
   }
   const size_t nf = g.nf, ng = g.nf == 6 ? 6 : 3;
   const int nd = 2 * g.nf * g.kh, nch = std::max(1, 256 / nd);
@@ -617,8 +656,10 @@ size_t diag_lds(const TcGeo &) { return 256 * sizeof(double); }
 
 hipError_t launch_nonlinear(const TcGeo &g, const double2 *u, double2 *G, hipStream_t s) {
   const size_t lds = nonlinear_lds(g);
-  if (g.nth > 1)
-    hipLaunchKernelGGL(k_tc3_nonlinear, dim3(g.mr), dim3(TC3_THREADS), lds, s, g, u, G);
+  if (g.nth > 1 && g.nf == 6)
+    hipLaunchKernelGGL(k_tc3_nonlinear<6>, dim3(g.mr), dim3(TC3_THREADS), lds, s, g, u, G);
+  else if (g.nth > 1)
+    hipLaunchKernelGGL(k_tc3_nonlinear<3>, dim3(g.mr), dim3(TC3_THREADS), lds, s, g, u, G);
   else if (g.nf == 6)
     hipLaunchKernelGGL(k_tc_nonlinear<6>, dim3(g.mr), dim3(256), lds, s, g, u, G);
   else

@@ -25,6 +25,7 @@ struct TcGeo {
   double2 *dv = nullptr;                          // (nm x nr) diagnostics scratch
   const double *Ev = nullptr, *Ed = nullptr;  // [basis](mr x n): Dirichlet, then Robin b_theta and Neumann b_z (MHD)
   const double *QbrT = nullptr, *QbtT = nullptr, *QbtdT = nullptr, *QbzT = nullptr;  // (mr x n) MHD EMF rows
+  const double *QbrrT = nullptr, *QbzrT = nullptr;  // (mr x n) their (1/r) d/dtheta parts (3D MHD)
   const double *PrT = nullptr;                // (mr x n): the projection, transposed
   const double *invr = nullptr;               // (mr)
   const double *kz = nullptr;                 // (nm) axial wavenumbers of the modes

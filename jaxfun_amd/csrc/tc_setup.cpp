@@ -290,8 +290,8 @@ int field_basis(int c) { return c < 4 ? BD : (c == 4 ? BR : BN); }
 char basis_char(int k) { return k == BD ? 'D' : k == BR ? 'R' : 'N'; }
 
 struct MhdBlocks {
-  Mat M[3], D2[3], D1r[3], R2[3];  // per basis: mass and Laplacian pieces
-  Mat TwoOm, M_DR, M_DN, M_RD, M_ND, RomD, Gr, Gz0, Dr, Dz0;
+  Mat M[3], D2[3], D1r[3], R2[3], Om[3];  // per basis: mass, Laplacian pieces, Omega
+  Mat TwoOm, M_DR, M_DN, M_RD, M_ND, RomD, R2_DR, R2_RD, Gr, Gz0, Gt0, Dr, Dz0, Dt0;
 };
 
 MhdBlocks mhd_blocks(const Forms &F, LD a, LD b) {
@@ -302,7 +302,12 @@ MhdBlocks mhd_blocks(const Forms &F, LD a, LD b) {
     k.D2[t] = F.form(c, c, 2, one);
     k.D1r[t] = F.form(c, c, 1, [](LD r) { return 1 / r; });
     k.R2[t] = F.form(c, c, 0, [](LD r) { return 1 / (r * r); });
+    k.Om[t] = F.form(c, c, 0, [&](LD r) { return a + b / (r * r); });
   }
+  k.R2_DR = F.form('D', 'R', 0, [](LD r) { return 1 / (r * r); });
+  k.R2_RD = F.form('R', 'D', 0, [](LD r) { return 1 / (r * r); });
+  k.Gt0 = F.form('D', 'P', 0, [](LD r) { return 1 / r; });
+  k.Dt0 = F.form('P', 'D', 0, [](LD r) { return 1 / r; });
   k.TwoOm = F.form('D', 'D', 0, [&](LD r) { return 2 * (a + b / (r * r)); });
   k.M_DR = F.form('D', 'R', 0, one);
   k.M_DN = F.form('D', 'N', 0, one);
@@ -318,18 +323,21 @@ MhdBlocks mhd_blocks(const Forms &F, LD a, LD b) {
   return k;
 }
 
-// _add_mhd_terms (:1460-1499) of one axial mode times sgn (6n x 6n): viscous
-// and resistive Laplacians (with -1/r^2 on the r and theta components), the
-// base-flow couplings 2 Omega u_theta, -2a u_r and r Omega' b_r, and the
-// i kz B0 line-tension and induction couplings.
-std::vector<CL> mhd_operator_part(const MhdBlocks &k, int n, LD kz, LD sgn, const Params &p) {
+// _add_mhd_terms (:1460-1499; 3D :2005-2059) of one (m, kz) mode times sgn
+// (6n x 6n): viscous and resistive Laplacians (with -1/r^2 on the r and
+// theta components), the base-flow advection -i m Omega of every field, the
+// +-2 i m / r^2 azimuthal couplings of u and of b, the base-flow couplings
+// 2 Omega u_theta, -2a u_r and r Omega' b_r, and the i kz B0 line-tension and
+// induction couplings.
+std::vector<CL> mhd_operator_part(const MhdBlocks &k, int n, LD m, LD kz, LD sgn, const Params &p) {
   const int N6 = 6 * n;
   std::vector<CL> X((size_t)N6 * N6, CL(0));
   const LD nu = p.nu, eta = p.eta;
   const CL ikB(0, kz * (LD)p.b0);
   auto at = [&](int bi, int bj, int i, int j) -> CL & { return X[(size_t)(bi * n + i) * N6 + bj * n + j]; };
+  const CL I(0, 1);
   auto lap = [&](int t, size_t o, bool hoop) {
-    return k.D2[t][o] + k.D1r[t][o] - kz * kz * k.M[t][o] - (hoop ? k.R2[t][o] : 0);
+    return k.D2[t][o] + k.D1r[t][o] - kz * kz * k.M[t][o] - m * m * k.R2[t][o] - (hoop ? k.R2[t][o] : 0);
   };
   for (int i = 0; i < n; ++i)
     for (int j = 0; j < n; ++j) {
@@ -349,14 +357,22 @@ std::vector<CL> mhd_operator_part(const MhdBlocks &k, int n, LD kz, LD sgn, cons
       at(4, 3, i, j) = sgn * k.RomD[o];
       at(5, 5, i, j) = sgn * eta * lap(BN, o, false);
       at(5, 2, i, j) = sgn * ikB * k.M_ND[o];
+      if (m != 0) {
+        for (int f = 0; f < 6; ++f) at(f, f, i, j) += sgn * (-I * m) * k.Om[field_basis(f)][o];
+        at(0, 1, i, j) += sgn * (-nu * 2 * I * m) * k.R2[BD][o];
+        at(1, 0, i, j) += sgn * (nu * 2 * I * m) * k.R2[BD][o];
+        at(3, 4, i, j) += sgn * (-eta * 2 * I * m) * k.R2_DR[o];
+        at(4, 3, i, j) += sgn * (eta * 2 * I * m) * k.R2_RD[o];
+      }
     }
   return X;
 }
 
 // mass + ls L (with_mass) or L alone, times fz, stored transposed (complex).
-void mhd_explicit_T(const MhdBlocks &k, int n, LD kz, LD ls, LD fz, const Params &p, bool with_mass, double *dst) {
+void mhd_explicit_T(const MhdBlocks &k, int n, LD m, LD kz, LD ls, LD fz, const Params &p, bool with_mass,
+                    double *dst) {
   const int N6 = 6 * n;
-  const std::vector<CL> X = mhd_operator_part(k, n, kz, ls, p);
+  const std::vector<CL> X = mhd_operator_part(k, n, m, kz, ls, p);
   for (int i = 0; i < N6; ++i)
     for (int j = 0; j < N6; ++j) {
       CL v = fz * X[(size_t)i * N6 + j];
@@ -369,10 +385,11 @@ void mhd_explicit_T(const MhdBlocks &k, int n, LD kz, LD ls, LD fz, const Params
 // The implicit 7-field block (:1501-1549) in the order (u_r, u_theta, u_z,
 // b_r, b_theta, b_z, Pi), times fz, kz = 0 pinned; the field columns of its
 // inverse, transposed.
-bool mhd_inverse_columns(const MhdBlocks &k, int n, LD kz, LD ls, LD fz, const Params &p, bool pin, double *dst) {
+bool mhd_inverse_columns(const MhdBlocks &k, int n, LD m, LD kz, LD ls, LD fz, const Params &p, bool pin,
+                         double *dst) {
   const int N6 = 6 * n, N7 = 7 * n;
   std::vector<CL> A((size_t)N7 * N7, CL(0));
-  const std::vector<CL> X = mhd_operator_part(k, n, kz, -ls, p);
+  const std::vector<CL> X = mhd_operator_part(k, n, m, kz, -ls, p);
   for (int i = 0; i < N6; ++i)
     for (int j = 0; j < N6; ++j) {
       CL v = fz * X[(size_t)i * N6 + j];
@@ -385,8 +402,10 @@ bool mhd_inverse_columns(const MhdBlocks &k, int n, LD kz, LD ls, LD fz, const P
     for (int j = 0; j < n; ++j) {
       const size_t o = (size_t)i * n + j;
       at(0, 6, i, j) = fz * k.Gr[o];
+      at(1, 6, i, j) = fz * I * m * k.Gt0[o];
       at(2, 6, i, j) = fz * I * kz * k.Gz0[o];
       at(6, 0, i, j) = fz * k.Dr[o];
+      at(6, 1, i, j) = fz * I * m * k.Dt0[o];
       at(6, 2, i, j) = fz * I * kz * k.Dz0[o];
     }
   if (pin) {
@@ -412,7 +431,6 @@ std::string build_ops(const Params &p, Ops &o) {
   if (p.ntheta < 1 || (p.ntheta > 1 && p.ntheta % 2)) return "Ntheta must be 1 or even";
   if (!(p.r2 > p.r1 && p.r1 > 0)) return "require 0 < R1 < R2";
   if (!(p.lz > 0) || !(p.dt > 0) || !(p.nu > 0)) return "Lz, dt and nu must be positive";
-  if (p.mhd && p.ntheta != 1) return "the MHD step is built for the axisymmetric class (ntheta = 1)";
   if (p.mhd && !(p.eta > 0)) return "eta_mag must be positive";
   const int N = p.nr, n = N - 2;
   o.n = n;
@@ -487,17 +505,23 @@ std::string build_ops(const Params &p, Ops &o) {
     o.QbtT.assign(pe, 0);
     o.QbtdT.assign(pe, 0);
     o.QbzT.assign(pe, 0);
+    o.QbrrT.assign(pe, 0);
+    o.QbzrT.assign(pe, 0);
     const LD w = (PI / N) / F.df;
     for (int j = 0; j < o.mr; ++j)
       for (int i = 0; i < n; ++i) {
-        LD br = 0, bt = 0, btd = 0, bz = 0;
+        LD br = 0, bt = 0, btd = 0, bz = 0, brr = 0, bzr = 0;
         for (int q = 0; q < N; ++q) {
           const LD v = Wv[(size_t)q * o.mr + j], d = Wd[(size_t)q * o.mr + j];
           br += F.B.V[BD][0][(size_t)q * n + i] * v;
           bt += F.B.V[BR][0][(size_t)q * n + i] * v;
           btd += F.B.V[BR][0][(size_t)q * n + i] * d;
           bz -= F.B.V[BN][0][(size_t)q * n + i] * (d + v / F.r[q]);
+          brr += F.B.V[BD][0][(size_t)q * n + i] * v / F.r[q];
+          bzr += F.B.V[BN][0][(size_t)q * n + i] * v / F.r[q];
         }
+        o.QbrrT[(size_t)j * n + i] = (double)(w * brr);
+        o.QbzrT[(size_t)j * n + i] = (double)(w * bzr);
         o.QbrT[(size_t)j * n + i] = (double)(w * br);
         o.QbtT[(size_t)j * n + i] = (double)(w * bt);
         o.QbtdT[(size_t)j * n + i] = (double)(w * btd);
@@ -549,11 +573,11 @@ std::string build_ops(const Params &p, Ops &o) {
       o.mq_odd[q] = (o.nth % 2 == 0 && im == o.nth / 2) ? 0.0 : (double)mi;
       const bool pin = (mi == 0 && k == 0);
       if (p.mhd) {
-        mhd_explicit_T(KM, n, kz, 1, fz, p, false, &o.LinT[(size_t)q * N3 * N3 * 2]);
-        mhd_explicit_T(KM, n, kz, 0.5L, fz, p, true, &o.LexpT[(size_t)q * N3 * N3 * 2]);
-        if (!mhd_inverse_columns(KM, n, kz, ls, fz, p, pin, &o.AinvT[(size_t)q * N4 * N3 * 2]))
+        mhd_explicit_T(KM, n, m, kz, 1, fz, p, false, &o.LinT[(size_t)q * N3 * N3 * 2]);
+        mhd_explicit_T(KM, n, m, kz, 0.5L, fz, p, true, &o.LexpT[(size_t)q * N3 * N3 * 2]);
+        if (!mhd_inverse_columns(KM, n, m, kz, ls, fz, p, pin, &o.AinvT[(size_t)q * N4 * N3 * 2]))
           return "singular implicit block";
-        if (!mhd_inverse_columns(KM, n, kz, 0.5L, fz, p, pin, &o.Ainv0T[(size_t)q * N4 * N3 * 2]))
+        if (!mhd_inverse_columns(KM, n, m, kz, 0.5L, fz, p, pin, &o.Ainv0T[(size_t)q * N4 * N3 * 2]))
           return "singular start-up block";
         continue;
       }

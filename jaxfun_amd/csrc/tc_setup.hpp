@@ -42,6 +42,7 @@ struct Ops {
   // of its padded values (truncated Chebyshev coefficients, derivatives on the
   // standard grid, scalar products with the row's test basis), transposed (mr x n)
   std::vector<double> QbrT, QbtT, QbtdT, QbzT;
+  std::vector<double> QbrrT, QbzrT;       // 3D: the (1/r) d/dtheta parts of the b_r and b_z rows
   std::vector<double> MassT;              // [nf](n x n): fz (psi_i, psi_j) / dt of each field's basis, transposed
   std::vector<double> Drc, Dzc, Dtc;      // (n x n): continuity blocks (T_i, d phi_j/dr + phi_j/r), (T_i, phi_j), (T_i, phi_j/r), times fz
   // per-mode tables, each stored transposed (entry (i, j) at [j][i]) so that a

@@ -657,3 +657,64 @@ class AxisymmetricMRIDNSHIP(AxisymmetricTCDNSHIP):
 
     def inner_torque(self, state):
         raise NotImplementedError("the inner-cylinder torque is a hydrodynamic runner scalar")
+
+
+class TaylorCouetteMRIDNSHIP(AxisymmetricMRIDNSHIP):
+    """``TaylorCouetteMRIDNSJax`` on the GPU (``examples/taylor_couette_dns_jax.py:1906-2245``):
+    the conducting-wall MHD step on Fourier(theta) x Fourier(z) x Chebyshev(r),
+    one 7-field block per (m, kz).  State arrays are (Ntheta, Nz, Nr - 2)."""
+
+    def __init__(self, base: CircularCouette, B0: float = 0.1, nu: float = 1.0e-3, eta_mag: float = 1.0e-3,
+                 Nr: int = 24, Ntheta: int = 8, Nz: int = 16, Lz: float | None = None, dt: float = 2.0e-3,
+                 family: str = "C", dealias: float = 1.5, time_integrator: str = "CNAB2", device: int = 0):
+        self.Ntheta = int(Ntheta)
+        super().__init__(base, B0=B0, nu=nu, eta_mag=eta_mag, Nr=Nr, Nz=Nz, Lz=Lz, dt=dt, family=family,
+                         dealias=dealias, time_integrator=time_integrator, device=device)
+        th = np.arange(self.Ntheta) * 2.0 * math.pi / self.Ntheta
+        z = np.arange(self.Nz) * self.Lz / self.Nz
+        self.Theta, self.Z, self.R = np.meshgrid(th, z, self.r, indexing="ij")
+
+    def _config(self, device: int) -> nat.TcConfig:
+        c = super()._config(device)
+        c.ntheta = self.Ntheta
+        return c
+
+    @property
+    def _shape(self):
+        return (self.Ntheta, self.Nz, self.n)
+
+    def _forward_basis(self, values, S) -> np.ndarray:
+        c = np.fft.fft2(np.asarray(values, dtype=complex), axes=(0, 1), norm="forward")
+        psi = self._vander() @ S.T
+        w = np.pi / self.Nr
+        rhs = w * (c @ psi)
+        return np.linalg.solve(w * psi.T @ psi, rhs.reshape(-1, self.n).T).T.reshape(self._shape)
+
+    def _backward_basis(self, coef, S) -> np.ndarray:
+        return np.fft.ifft2(np.asarray(coef) @ (self._vander() @ S.T).T, axes=(0, 1), norm="forward")
+
+    def seed_linear_eigenmode(self, m: int = 1, kz_mode: int = 1, amp: float = 1.0e-6, which: int = 0):
+        """``:2192-2230``: the pivot phase is fixed only for m = 0."""
+        if 2 * abs(int(m)) >= self.Ntheta:
+            raise ValueError(f"azimuthal mode |m|={abs(int(m))} is unresolved by Ntheta={self.Ntheta}; "
+                             "require 2|m| < Ntheta")
+        kz = 2.0 * math.pi * int(kz_mode) / self.Lz
+        w, vecs = self.linear_eigs(int(m), kz, n_return=which + 1)
+        vec = vecs[:, which]
+        if int(m) == 0:
+            vec = _positive_pivot_phase(vec)
+        n = self.n
+        mpos, mneg = int(m) % self.Ntheta, (-int(m)) % self.Ntheta
+        kpos, kneg = int(kz_mode) % self.Nz, (-int(kz_mode)) % self.Nz
+        comps = []
+        for blk in (0, 1, 2, 4, 5, 6):
+            block = vec[blk * n:(blk + 1) * n] * amp
+            arr = self._zeros()
+            if mpos == mneg and kpos == kneg:
+                arr[mpos, kpos, :n] = np.real(block)
+            else:
+                arr[mpos, kpos, :n] = 0.5 * block
+                arr[mneg, kneg, :n] = 0.5 * np.conj(block)
+            comps.append(arr)
+        st = AxisymmetricMRIState(x=tuple(comps), p=self._zeros(), nonlinear_old=tuple(self._zeros() for _ in range(6)))
+        return st, complex(w[which])
