@@ -577,6 +577,10 @@ TC_CASES = (
     # (Re = Rm = 1000, B0 = 0.1, Keplerian Omega2, Lz = pi / 3) with SBDF3
     dict(name="tc_mhd 40x8", Nr=40, Nz=8, dealias=1.0, mhd=True),
     dict(name="MHD 48x32 dealiased", Nr=48, Nz=32, dealias=1.5, mhd=True),
+    # the 3D MHD class: production/examples/taylor_couette_mhd_3d_v1.json, and
+    # 24x8x16 at the same physics
+    dict(name="tc_mhd_3d 12x4x4", Nr=12, Ntheta=4, Nz=4, dealias=1.5, mhd=True),
+    dict(name="3D MHD 24x8x16", Nr=24, Ntheta=8, Nz=16, dealias=1.5, mhd=True),
 )
 
 
@@ -587,9 +591,10 @@ def tc_leg(steps: int = 2400, cpu_steps: int = 60) -> list:
     Lz = 2 pi / 3.13, dt = 1e-3, amp 1e-6 leading eigenmode: m = 0 for the
     axisymmetric class, m = 1 for the 3D one), and the growth rate against
     the linear one."""
-    from jaxfun_amd.tc import AxisymmetricMRIDNSHIP, AxisymmetricTCDNSHIP, CircularCouette, TaylorCouetteDNSHIP
+    from jaxfun_amd.tc import (AxisymmetricMRIDNSHIP, AxisymmetricTCDNSHIP, CircularCouette, TaylorCouetteDNSHIP,
+                               TaylorCouetteMRIDNSHIP)
     from oracle.tc import CircularCouette as OBase
-    from oracle.tc import TC3DOracle, TCMRIOracle, TCOracle, TCState
+    from oracle.tc import TC3DMRIOracle, TC3DOracle, TCMRIOracle, TCOracle, TCState
 
     out = []
     lz = 2.0 * math.pi / 3.13
@@ -599,7 +604,11 @@ def tc_leg(steps: int = 2400, cpu_steps: int = 60) -> list:
         nth = case.get("Ntheta", 1)
         mhd = case.get("mhd", False)
         om2 = 0.5**1.5 if mhd else 0.0
-        if mhd:
+        if mhd and nth > 1:
+            kw.update(nu=1e-3, eta_mag=1e-3, B0=0.1, Lz=math.pi / 3.0, dt=2e-3, Ntheta=nth)
+            solver = TaylorCouetteMRIDNSHIP(CircularCouette(1.0, 2.0, 1.0, om2), **kw)
+            s0, eig = solver.seed_linear_eigenmode(m=1, kz_mode=1, amp=1e-7)
+        elif mhd:
             kw.update(nu=1e-3, eta_mag=1e-3, B0=0.1, Lz=math.pi / 3.0, dt=2e-3)
             solver = AxisymmetricMRIDNSHIP(CircularCouette(1.0, 2.0, 1.0, om2), **kw)
             s0, eig = solver.seed_linear_eigenmode(kz_mode=1, amp=1e-7)
@@ -619,7 +628,8 @@ def tc_leg(steps: int = 2400, cpu_steps: int = 60) -> list:
         fin = solver.fetch(s)
         e0, e1 = solver.energy(s), solver.energy(fin)
         growth = 0.5 * math.log(e1 / e0) / ((steps + 24) * solver.dt)
-        oracle = (TCMRIOracle if mhd else TC3DOracle if nth > 1 else TCOracle)(OBase(1.0, 2.0, 1.0, om2), **kw)
+        ocls = (TC3DMRIOracle if nth > 1 else TCMRIOracle) if mhd else (TC3DOracle if nth > 1 else TCOracle)
+        oracle = ocls(OBase(1.0, 2.0, 1.0, om2), **kw)
         os_ = TCState(u=s.x if mhd else s.u, p=s.p, nonlinear_old=s.nonlinear_old, have_old=s.have_old,
                       nonlinear_older=s.nonlinear_older, solution_old=s.solution_old,
                       solution_older=s.solution_older, history_steps=s.history_steps)
@@ -627,7 +637,9 @@ def tc_leg(steps: int = 2400, cpu_steps: int = 60) -> list:
         t1 = time.perf_counter()
         oracle.solve(os_, cpu_steps)
         cpu = cpu_steps / (time.perf_counter() - t1)
-        if mhd:
+        if mhd and nth > 1:
+            kind = f"3D conducting-wall Taylor-Couette MHD SBDF3, {case['name']} (Nr={case['Nr']}, Ntheta={nth}, "
+        elif mhd:
             kind = f"axisymmetric conducting-wall Taylor-Couette MHD SBDF3, {case['name']} (Nr={case['Nr']}, "
         elif nth > 1:
             kind = f"3D Taylor-Couette SBDF3, {case['name']} (Nr={case['Nr']}, Ntheta={nth}, "
