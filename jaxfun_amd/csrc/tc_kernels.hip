@@ -519,19 +519,38 @@ __global__ void __launch_bounds__(TC3_THREADS) k_tc3_nonlinear(TcGeo g, const do
   double2 *gs = reinterpret_cast<double2 *>(base + region);  // [NV][kh][mth]
   double *prod = base;
   double2 *h = reinterpret_cast<double2 *>(prod + NG * mth * mz);
+  constexpr int NB = NF == 6 ? 3 : 1;
+  double *er = reinterpret_cast<double *>(gs + (size_t)NV * kh * mth);  // [NB][2][n]: this point's basis rows
   const int tid = threadIdx.x;
   for (int t = tid; t < mz; t += blockDim.x) tz[t] = g.twz[t];
   for (int t = tid; t < mth; t += blockDim.x) tt[t] = g.twt[t];
+  for (int t = tid; t < NB * 2 * n; t += blockDim.x) {
+    const int bt = t / n, i = t - bt * n, basis = bt >> 1;
+    er[t] = ((bt & 1) ? g.Ed : g.Ev)[(size_t)basis * mr * n + (size_t)j * n + i];
+  }
+  __syncthreads();
   for (int d = tid; d < 2 * NF * nm; d += blockDim.x) {
     const int c = d / (2 * nm), rem = d - c * 2 * nm, typ = rem / nm, q = rem - typ * nm;
     const int basis = c < 4 ? 0 : c - 3;  // b_theta: Robin, b_z: Neumann
-    const double *E = (typ == 0 ? g.Ev : g.Ed) + (size_t)basis * mr * n + (size_t)j * n;
+    const double *E = er + (size_t)(basis * 2 + typ) * n;
     const double2 *x = u + ((size_t)c * nm + q) * n;
-    double sr = 0.0, si = 0.0;
-    for (int i = 0; i < n; ++i) {
+    // two partial sums and unrolled loads: the row's global loads in flight together
+    double sr = 0.0, si = 0.0, sr2 = 0.0, si2 = 0.0;
+    int i = 0;
+#pragma unroll 4
+    for (; i + 1 < n; i += 2) {
+      const double2 x0 = x[i], x1 = x[i + 1];
+      sr = fma(E[i], x0.x, sr);
+      si = fma(E[i], x0.y, si);
+      sr2 = fma(E[i + 1], x1.x, sr2);
+      si2 = fma(E[i + 1], x1.y, si2);
+    }
+    if (i < n) {
       sr = fma(E[i], x[i].x, sr);
       si = fma(E[i], x[i].y, si);
     }
+    sr += sr2;
+    si += si2;
     val[(c * 4 + typ) * nm + q] = make_double2(sr, si);
     if (typ == 0) {
       const double m = g.mq_odd[q], kz = g.kz[q];
@@ -635,7 +654,8 @@ size_t nonlinear_lds(const TcGeo &g) {
   if (g.nth > 1) {
     const size_t nv = 4 * (size_t)g.nf, ng = g.nf == 6 ? 6 : 3;
     const size_t region = std::max(nv * g.nm * 2, ng * g.mth * g.mz + ng * g.mth * g.kh * 2);
-    return ((size_t)g.mz + g.mth) * sizeof(double2) + (region + nv * g.kh * g.mth * 2) * sizeof(double);
+    const size_t rows = (g.nf == 6 ? 3 : 1) * 2 * (size_t)g.n;
+    return ((size_t)g.mz + g.mth) * sizeof(double2) + (region + nv * g.kh * g.mth * 2 + rows) * sizeof(double);
   }
   const size_t nf = g.nf, ng = g.nf == 6 ? 6 : 3;
   const int nd = 2 * g.nf * g.kh, nch = std::max(1, 256 / nd);
