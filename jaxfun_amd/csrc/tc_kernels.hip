@@ -669,7 +669,7 @@ size_t rhs_lds(const TcGeo &g) {
   return (ng * g.mr + std::max(nfn, (size_t)RHS_THREADS) + 2 * nfn) * sizeof(double2);
 }
 
-// Blocks per mode of the solve kernel: row slices of the 4n solution rows.
+// Least blocks per mode of the solve kernel: row slices of the solution rows.
 constexpr int RHS_SLICES = 4;
 
 size_t diag_lds(const TcGeo &) { return 256 * sizeof(double); }
@@ -697,7 +697,9 @@ hipError_t launch_rhs_solve(const TcGeo &g, const TcRhs &a, hipStream_t s) {
   }
   if (lin && lexp) return hipErrorInvalidValue;  // one combined operand per call
   const size_t lds = rhs_lds(g);
-  hipLaunchKernelGGL(k_tc_rhs_solve, dim3(g.nm, RHS_SLICES), dim3(RHS_THREADS), lds, s, g, a);
+  // at least RHS_SLICES row slices per mode, one more per 24 solution rows
+  const int n4 = (g.nf + 1) * g.n, slices = std::max(RHS_SLICES, (n4 + 23) / 24);
+  hipLaunchKernelGGL(k_tc_rhs_solve, dim3(g.nm, slices), dim3(RHS_THREADS), lds, s, g, a);
   return hipGetLastError();
 }
 
