@@ -936,3 +936,94 @@ def run_pcf_vector_potential_mhd_saturation(spec: dict, *, steps: int | None = N
     if budget is not None:
         scalars["energy_budget_residual"] = float(budget)
     return {"scalars": scalars, "time_series": series, "final_state": to_host(state)}
+
+
+# ---------------------------------------------------------------------------
+# Taylor-Couette DNS problems (production/oracles.py:1237-1330, 2942-3039)
+# ---------------------------------------------------------------------------
+
+
+def tc_kz_mode(spec: dict, Lz: float) -> int:
+    """``_kz_mode_from_spec`` (``production/oracles.py:4656-4675``)."""
+    mode = spec.get("mode", {})
+    m = int(mode.get("azimuthal_wavenumber", 0))
+    if m != 0 and selected_resolution(spec).get("Ntheta") is None:
+        raise NotImplementedError("non-axisymmetric Taylor-Couette modes require resolution.Ntheta; "
+                                  f"got azimuthal_wavenumber={m} for an axisymmetric solver")
+    kz_mode = int(round(float(mode["axial_wavenumber"]) * float(Lz) / (2.0 * math.pi)))
+    if kz_mode < 1:
+        raise ValueError("axial_wavenumber does not map to a positive Fourier mode")
+    return kz_mode
+
+
+def _tc_physics(spec: dict) -> dict:
+    """nu, eta and B0 of a Taylor-Couette spec: the explicit groups, else from
+    Re / Rm on the inner-cylinder velocity scale Omega1 R1 (R2 - R1)."""
+    g = spec["nondimensional_groups"]
+    scale = float(g["Omega1"]) * float(g["R1"]) * (float(g["R2"]) - float(g["R1"]))
+    nu = float(g["nu"]) if "nu" in g else scale / float(g["Re"])
+    eta = float(g["eta_mag"]) if "eta_mag" in g else (scale / float(g["Rm"]) if "Rm" in g else nu)
+    b0 = float(g.get("B0", spec.get("forcing", {}).get("B0", 0.0)))
+    return {"nu": nu, "eta": eta, "B0": b0}
+
+
+def run_taylor_couette_dns(spec: dict, *, steps: int | None = None, device: int = 0) -> dict:
+    """The Taylor-Couette DNS golden problems on the device: the hydrodynamic
+    ``_run_taylor_couette_hydro_dns`` (``production/oracles.py:1237-1330``) and,
+    for ``physics == "mhd"`` with conducting walls, ``_run_taylor_couette_mhd_dns``
+    (``:2942-3039``): the solver class from the resolution (``Ntheta`` selects
+    the 3D class), the linear-eigenmode seed, the run and the scalars and time
+    series the reference reports.  Checkpoints and snapshots of the reference's
+    runner are not written here."""
+    from .tc import (AxisymmetricMRIDNSHIP, AxisymmetricTCDNSHIP, CircularCouette, TaylorCouetteDNSHIP,
+                     TaylorCouetteMRIDNSHIP)
+
+    resolution = selected_resolution(spec)
+    g = spec["nondimensional_groups"]
+    phys = _tc_physics(spec)
+    base = CircularCouette(float(g["R1"]), float(g["R2"]), float(g["Omega1"]), float(g["Omega2"]))
+    mhd = spec.get("physics") == "mhd"
+    if mhd and magnetic_bc(spec) != "conducting":
+        raise NotImplementedError("TC MHD DNS is built for conducting walls only, "
+                                  f"got {magnetic_bc(spec)!r}")
+    ntheta = resolution.get("Ntheta")
+    kw = dict(nu=phys["nu"], Nr=int(resolution.get("Nr", resolution.get("N", 40))), Nz=int(resolution.get("Nz", 8)),
+              Lz=float(spec["domain"]["z_period"]), dt=float(spec["time"]["dt"]),
+              family=resolution.get("family", "C"), dealias=float(resolution.get("dealias", 1.0)),
+              time_integrator=str(spec["time"].get("integrator", "CNAB2")), device=device)
+    if mhd:
+        kw.update(B0=phys["B0"], eta_mag=phys["eta"])
+    if ntheta is not None:
+        kw["Ntheta"] = int(ntheta)
+        cls = TaylorCouetteMRIDNSHIP if mhd else TaylorCouetteDNSHIP
+    else:
+        cls = AxisymmetricMRIDNSHIP if mhd else AxisymmetricTCDNSHIP
+    solver = cls(base, **kw)
+    try:
+        seed = {"kz_mode": tc_kz_mode(spec, solver.Lz),
+                "amp": float(spec["initial_condition"].get("amplitude", 1.0e-7 if mhd else 1.0e-6))}
+        if ntheta is not None:
+            seed["m"] = int(spec.get("mode", {}).get("azimuthal_wavenumber", 0))
+        state, eig = solver.seed_linear_eigenmode(**seed)
+        initial = solver.diagnostics(state)
+        n = steps_from_spec(spec, steps)
+        out = solver.solve(state, n)
+        final = solver.diagnostics(out)
+    finally:
+        solver.close()
+    growth = growth_rate_from_energy(initial["E"], final["E"], n, solver.dt)
+    elapsed = n * float(spec["time"]["dt"])
+    if mhd:
+        scalars = {"kinetic_energy": final["Ekin"], "magnetic_energy": final["Emag"], "growth_rate": growth,
+                   "growth_rate_linear": eig.real, "divergence_u": final["divu"], "divergence_b": final["divb"],
+                   "magnetic_bc": "conducting"}
+        series = [{"t": 0.0, "kinetic_energy": initial["Ekin"], "magnetic_energy": initial["Emag"],
+                   "growth_rate_linear": eig.real},
+                  {"t": elapsed, "kinetic_energy": final["Ekin"], "magnetic_energy": final["Emag"],
+                   "growth_rate": growth}]
+    else:
+        scalars = {"kinetic_energy": final["E"], "growth_rate": growth, "growth_rate_linear": eig.real,
+                   "divergence_linf": final["div_linf"], "rayleigh_stable": bool(base.rayleigh_stable())}
+        series = [{"t": 0.0, "kinetic_energy": initial["E"], "growth_rate_linear": eig.real},
+                  {"t": elapsed, "kinetic_energy": final["E"], "growth_rate": growth}]
+    return {"scalars": scalars, "time_series": series}

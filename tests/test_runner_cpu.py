@@ -354,3 +354,20 @@ def test_mhd_payload_round_trip(tmp_path):
     for name in ("A", "nonlinear_A_old", "nonlinear_A_older", "A_old", "A_older"):
         for x, y in zip(getattr(back, name), getattr(st, name)):
             assert np.array_equal(x, y)
+
+
+def test_tc_spec_helpers():
+    """Host logic of the Taylor-Couette runner: kz_mode and the physics groups."""
+    from jaxfun_amd.runner import _tc_physics, tc_kz_mode
+
+    spec = {"mode": {"axial_wavenumber": 3.13, "azimuthal_wavenumber": 0}, "resolution": {"Nr": 40, "Nz": 8},
+            "nondimensional_groups": {"Omega1": 1.0, "R1": 1.0, "R2": 2.0, "Re": 100.0}}
+    assert tc_kz_mode(spec, 2.0074074463832545) == 1
+    assert _tc_physics(spec) == {"nu": 0.01, "eta": 0.01, "B0": 0.0}
+    spec["mode"]["azimuthal_wavenumber"] = 1
+    with pytest.raises(NotImplementedError):
+        tc_kz_mode(spec, 2.0)
+    spec["resolution"]["Ntheta"] = 4
+    assert tc_kz_mode(spec, 2.0074074463832545) == 1
+    spec["nondimensional_groups"].update(Rm=500.0, B0=0.1)
+    assert _tc_physics(spec) == pytest.approx({"nu": 0.01, "eta": 0.002, "B0": 0.1})
