@@ -6,6 +6,7 @@
 // buffers whose roles rotate on the host after each step (no copies).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <map>
@@ -367,9 +368,17 @@ int32_t tc_create(const tc_config *cfg, tc_handle **out) {
   sz.mz = o.mz;
   sz.mth = o.mth;
   sz.nr = cfg->nr;
-  if (tc::nonlinear_lds(sz) > 65536 || tc::rhs_lds(sz) > 65536 || tc::diag_lds(sz) > 65536) {
+  int lds_max = 65536;
+  if (hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, cfg->device) != hipSuccess)
+    lds_max = 65536;
+  const size_t need = std::max({tc::nonlinear_lds(sz), tc::rhs_lds(sz), tc::diag_lds(sz)});
+  if (need > (size_t)lds_max) {
     tc_destroy(h);
     return fail(TC_E_UNSUPPORTED, "grid too large for the per-block LDS tables");
+  }
+  if (need > 65536 && tc::allow_lds(need) != hipSuccess) {
+    tc_destroy(h);
+    return fail(TC_E_HIP, "could not raise the kernels' LDS limit");
   }
   std::vector<double> invr(o.mr), rstd(o.r_std);
   for (int j = 0; j < o.mr; ++j) invr[j] = 1.0 / o.r_pad[j];

@@ -674,6 +674,24 @@ constexpr int RHS_SLICES = 4;
 
 size_t diag_lds(const TcGeo &) { return 256 * sizeof(double); }
 
+hipError_t allow_lds(size_t bytes) {
+  // gfx950 gives a block up to 160 KB of LDS; raise every kernel's dynamic
+  // limit to what the handle needs
+  const int v = (int)bytes;
+  const void *fns[] = {reinterpret_cast<const void *>(&k_tc_nonlinear<3>),
+                       reinterpret_cast<const void *>(&k_tc_nonlinear<6>),
+                       reinterpret_cast<const void *>(&k_tc3_nonlinear<3>),
+                       reinterpret_cast<const void *>(&k_tc3_nonlinear<6>),
+                       reinterpret_cast<const void *>(&k_tc_rhs_solve),
+                       reinterpret_cast<const void *>(&k_tc_diag),
+                       reinterpret_cast<const void *>(&k_tc_mhd_diag)};
+  for (const void *f : fns) {
+    hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, v);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
 hipError_t launch_nonlinear(const TcGeo &g, const double2 *u, double2 *G, hipStream_t s) {
   const size_t lds = nonlinear_lds(g);
   if (g.nth > 1 && g.nf == 6)

@@ -70,5 +70,8 @@ hipError_t launch_diag(const TcGeo &g, const double2 *u, double *out, hipStream_
 size_t nonlinear_lds(const TcGeo &g);  // the axisymmetric or the 3D product kernel
 size_t rhs_lds(const TcGeo &g);
 size_t diag_lds(const TcGeo &g);
+// Raises the dynamic LDS limit of every kernel to bytes (at most the device's
+// per-block LDS).
+hipError_t allow_lds(size_t bytes);
 
 }  // namespace tc
