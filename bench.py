@@ -577,10 +577,11 @@ TC_CASES = (
     # (Re = Rm = 1000, B0 = 0.1, Keplerian Omega2, Lz = pi / 3) with SBDF3
     dict(name="tc_mhd 40x8", Nr=40, Nz=8, dealias=1.0, mhd=True),
     dict(name="MHD 48x32 dealiased", Nr=48, Nz=32, dealias=1.5, mhd=True),
-    # the 3D MHD class: production/examples/taylor_couette_mhd_3d_v1.json, and
-    # 24x8x16 at the same physics
+    # the 3D MHD class: production/examples/taylor_couette_mhd_3d_v1.json, its
+    # production resolution tier, and 24x8x16 at the same physics
     dict(name="tc_mhd_3d 12x4x4", Nr=12, Ntheta=4, Nz=4, dealias=1.5, mhd=True),
     dict(name="3D MHD 24x8x16", Nr=24, Ntheta=8, Nz=16, dealias=1.5, mhd=True),
+    dict(name="tc_mhd_3d production tier 24x16x16", Nr=24, Ntheta=16, Nz=16, dealias=1.5, mhd=True),
 )
 
 
