@@ -562,11 +562,18 @@ __global__ void __launch_bounds__(TC3_THREADS) k_tc3_nonlinear(TcGeo g, const do
   for (int t = tid; t < NV * kh * mth; t += blockDim.x) {
     const int f = t / (kh * mth), rem = t - f * kh * mth, k = rem / mth, a = rem - k * mth;
     double sr = 0.0, si = 0.0;
+    // table index (m a) mod mth stepped by a; it restarts once at the first
+    // negative wavenumber
+    const int hpos = (nth + 1) / 2;
+    int idx = 0;
     for (int im = 0; im < nth; ++im) {
+      if (im == hpos) idx = (((hpos - nth) * a) % mth + mth) % mth;
       const double2 c = val[f * nm + im * kh + k];
-      const double2 w = tt[((fft_wavenumber(im, nth) * a) % mth + mth) % mth];
+      const double2 w = tt[idx];
       sr += c.x * w.x - c.y * w.y;
       si += c.x * w.y + c.y * w.x;
+      idx += a;
+      if (idx >= mth) idx -= mth;
     }
     gs[t] = make_double2(sr, si);
   }
